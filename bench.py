@@ -1,0 +1,136 @@
+"""Benchmark: grid-point updates per second of the FP64 HEVI short step on one
+MI355X, with the HBM roofline fraction of the HEVI stage and the CPU oracle
+timed beside it.
+
+A step is one full dynamical short step on the device-resident state:
+numerical filter (num_diff) + four HEVI stages (RK4) + the RK4 combination,
+ghost levels and periodic halos.  Workload: 512x512x128 (DX = 1 km, DZ 50..500 m
+stretched, linear shear 0..20 m/s over 10 km, theta noise 0.1 K, UD3, dt = 2 s),
+synthetic and seeded; inputs are resident in HBM before timing starts.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size IxJxK]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+# algorithmic bytes per interior point (FP64, flat terrain, divdmp_coef = 0):
+# one HEVI stage reads 5 state0 + 5 state + 5 tendencies + DPRES0, RT2P,
+# REF_dens, REF_rhot + 15 num_diff (34 arrays, MOMZ0 counted in state0) and
+# writes 5 stage outputs + 3 mflx + 3 tflx: 45 arrays x 8 B.
+STAGE_BYTES_PER_POINT = 45 * 8
+# short step = numfilter (7 read + 15 written) + 4 stages + RK4 combination (15 + 18 read)
+STEP_BYTES_PER_POINT = 22 * 8 + 4 * STAGE_BYTES_PER_POINT + 33 * 8
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--size", default="512x512x128", help="IMAXxJMAXxKMAX")
+    p.add_argument("--cpu-sample", default="128x128x128", help="sub-domain timed on the host with the oracle")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(ni, nj, nk):
+    """The oracle (C, OpenMP) on a bounded sub-domain of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from scale_amd import inputs
+    threads = min(os.cpu_count() or 1, 16)
+    po.set_threads(threads)
+    case = inputs.shear_512x512x128(imax=ni, jmax=nj, kmax=nk, ntracer=0)
+    m = po.Model(case)
+    m.step(1)  # warm-up
+    n = 1
+    t0 = time.perf_counter()
+    m.step(n)
+    dt = time.perf_counter() - t0
+    # tracers off: one large step = one short step (numfilter + RK4) plus the cheap preparation
+    pts = ni * nj * nk
+    return dict(value=pts * n / dt, unit="grid-pt short-step updates/s", cores=threads, kind="port",
+                sample=f"oracle (C, OpenMP {threads} threads) on a {ni}x{nj}x{nk} sub-domain of the workload, "
+                       f"{n} short step(s) incl. the large-step preparation, no tracers")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see the design notes)",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    ni, nj, nk = (int(x) for x in a.size.split("x"))
+    from scale_amd import hip, inputs
+
+    case = inputs.shear_512x512x128(imax=ni, jmax=nj, kmax=nk)
+    g = case.grid
+    ctx = hip.model_from_case(case)
+    ctx.model_prepare()
+    ctx.model_short_step(a.warmup)
+    ctx.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    ctx.model_short_step(a.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kern = {}
+    for name in ("hevi_explicit", "hevi_solve", "numfilter", "halo_fill", "rk4_combine"):
+        cnt, ms = ctx.profile_query(name)
+        kern[name] = dict(count=cnt, avg_ms=(ms / cnt if cnt else None))
+    ctx.profile(False)
+    # large step incl. 3 tracers with FCT (informational)
+    ctx.synchronize()
+    t2 = time.perf_counter()
+    nls = 2
+    ctx.model_large_step(nls)
+    ctx.synchronize()
+    large_ms = (time.perf_counter() - t2) * 1e3 / nls
+    ctx.close()
+
+    pts = ni * nj * nk
+    value = pts * a.steps / elapsed
+    ms_step = elapsed * 1e3 / a.steps
+    st_ms = kern["hevi_explicit"]["avg_ms"] + kern["hevi_solve"]["avg_ms"]
+    achieved = STAGE_BYTES_PER_POINT * pts / (st_ms * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=None,
+                kernel="HEVI stage = hevi_explicit + hevi_solve", bytes_per_point=STAGE_BYTES_PER_POINT,
+                stage_ms=st_ms, kernels_avg_ms={k: v["avg_ms"] for k, v in kern.items()})
+    out = {
+        "metric": "grid-pt updates/s short step (%HBM roofline); FP32 vs FP64 speedup; adjoint/fwd",
+        "value": value,
+        "unit": "grid-pt short-step updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded shear flow + theta noise, hydrostatic base state)",
+        "config": {"workload": f"HEVI short step, FP64, {ni}x{nj}x{nk}, DX=1km, DZ 50-500 m stretched, "
+                               f"linear shear 0-20 m/s, UD3, RK4, numfilter 1e-4, dt=2 s",
+                   "grid": [ni, nj, nk], "short_step_bytes_per_point": STEP_BYTES_PER_POINT,
+                   "short_step_hbm_frac": value * STEP_BYTES_PER_POINT / 1e9 / HBM_PEAK_GBS,
+                   "large_step_with_3_fct_tracers_ms": large_ms},
+        "roofline": roof,
+    }
+    if not a.no_cpu_baseline:
+        cn, cj, ck = (int(x) for x in a.cpu_sample.split("x"))
+        out["cpu_baseline"] = cpu_baseline(cn, cj, ck)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * scale_hip.h -- C ABI of the MI355X (gfx950) HEVI short step and FCT tracer
+ * advection for SCALE-RM.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative code on failure;
+ *    scale_hip_last_error() returns a message for the calling thread.
+ *  - Array arguments of the drop-in entry points are host pointers to
+ *    contiguous Fortran arrays with the reference's shapes, e.g. DENS(KA,IA,JA)
+ *    (k fastest), num_diff(KA,IA,JA,5,3), GSQRT(KA,IA,JA,7), MAPF(IA,JA,2,4).
+ *  - Scalars are passed by reference, as a Fortran caller passes them, so the
+ *    iso_c_binding interface needs no VALUE attributes.  LOGICAL arguments are
+ *    int32_t (0 = .false., nonzero = .true.).
+ *  - Grid extents are the module variables of
+ *    scalelib/src/atmosphere/grid/scale_atmos_grid_cartesC_index.F90:256-265,
+ *    given once to scale_hip_init with the reference's 1-based values.
+ *  - The library keeps its device buffers in the context; calls are synchronous
+ *    at return except the device-resident model calls, which are ordered on
+ *    the context's stream (scale_hip_synchronize waits for them).
+ *  - Lateral boundaries must be periodic (BND_* = .false.); other values are
+ *    rejected with an error.
+ */
+#ifndef SCALE_HIP_H
+#define SCALE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct scale_hip_ctx scale_hip_ctx;
+
+/* flux scheme ids (ATMOS_DYN_FVM_FLUX_TYPE / _TRACER_TYPE) */
+#define SCALE_HIP_UD3 3
+#define SCALE_HIP_CD4 4
+
+typedef struct {
+  int32_t KA, IA, JA;             /* array extents                           */
+  int32_t KS, KE, IS, IE, JS, JE; /* interior bounds, Fortran 1-based         */
+  int32_t KHALO, IHALO, JHALO;    /* must be 2 (filter_setup, COMMON.F90:137) */
+  int32_t flux_scheme;            /* dynamics: SCALE_HIP_UD3 or SCALE_HIP_CD4 */
+  int32_t flux_scheme_tracer;     /* tracers:  SCALE_HIP_UD3 or SCALE_HIP_CD4 */
+  int32_t device;                 /* HIP device ordinal                       */
+} scale_hip_grid;
+
+/* Creates a context on grid->device (the setup role of ATMOS_DYN_setup,
+ * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn.F90:81-255). */
+int scale_hip_init(const scale_hip_grid *grid, scale_hip_ctx **ctx);
+int scale_hip_finalize(scale_hip_ctx *ctx);
+const char *scale_hip_last_error(void);
+const char *scale_hip_version(void);
+int scale_hip_synchronize(scale_hip_ctx *ctx);
+
+/* One RK stage of the HEVI short step.  Same dummies, order and meaning as
+ * the abstract interface `short` of ATMOS_DYN_Tstep_short,
+ * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_short.F90:38-128,
+ * implemented by ATMOS_DYN_Tstep_short_fvm_hevi,
+ * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_short_fvm_hevi.F90:128-1089.
+ * PROG_RK/PROG0/PROG are unused (VA = 0 for HEVI) and may be NULL; the FCT
+ * flags are unused by HEVI (HEVI.F90:224-226). */
+int scale_hip_tstep_short_hevi(scale_hip_ctx *ctx,
+    double *DENS_RK, double *MOMZ_RK, double *MOMX_RK, double *MOMY_RK, double *RHOT_RK,
+    double *PROG_RK, double *mflx_hi, double *tflx_hi,
+    const double *DENS0, const double *MOMZ0, const double *MOMX0, const double *MOMY0, const double *RHOT0,
+    const double *DENS, const double *MOMZ, const double *MOMX, const double *MOMY, const double *RHOT,
+    const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
+    const double *PROG0, const double *PROG,
+    const double *DPRES0, const double *RT2P, const double *CORIOLI,
+    const double *num_diff, const double *wdamp_coef, const double *divdmp_coef, const double *DDIV,
+    const int32_t *FLAG_FCT_MOMENTUM, const int32_t *FLAG_FCT_T, const int32_t *FLAG_FCT_ALONG_STREAM,
+    const double *CDZ, const double *FDZ, const double *FDX, const double *FDY,
+    const double *RCDZ, const double *RCDX, const double *RCDY, const double *RFDZ, const double *RFDX, const double *RFDY,
+    const double *PHI, const double *GSQRT, const double *J13G, const double *J23G, const double *J33G, const double *MAPF,
+    const double *REF_dens, const double *REF_rhot,
+    const int32_t *BND_W, const int32_t *BND_E, const int32_t *BND_S, const int32_t *BND_N,
+    const double *dtrk, const int32_t *last);
+
+/* One RK stage of flux-form tracer advection with optional FCT.  Same dummies
+ * as the abstract interface `step` of ATMOS_DYN_Tstep_tracer,
+ * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_tracer.F90:34-64,
+ * implemented by ATMOS_DYN_Tstep_tracer_fvm_heve,
+ * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_tracer_fvm_heve.F90:72-242
+ * (limiter ATMOS_DYN_fct, scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:1922-2734).
+ * MAPF is the (IA,JA,2) I_XY slice; QTRCo may alias QTRC0. */
+int scale_hip_tstep_tracer_heve(scale_hip_ctx *ctx,
+    double *QTRCo, const double *QTRC, const double *QTRC0, const double *RHOQ_t,
+    const double *DENS0, const double *DENS, const double *mflx_hi, const double *num_diff,
+    const double *GSQRT, const double *MAPF, const double *CDZ,
+    const double *RCDZ, const double *RCDX, const double *RCDY,
+    const double *dtl, const int32_t *FLAG_FCT_TRACER, const int32_t *FLAG_FCT_ALONG_STREAM);
+
+/* Hyperdiffusion fluxes num_diff(KA,IA,JA,5,3) including their halo exchange:
+ * ATMOS_DYN_numfilter_coef, scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:385-985
+ * (tables of ATMOS_DYN_filter_setup, :92-319).  ND_ORDER = 1 and
+ * ND_USE_RS = .true. are supported; other values are rejected. */
+int scale_hip_numfilter_coef(scale_hip_ctx *ctx, double *num_diff,
+    const double *DENS, const double *MOMZ, const double *MOMX, const double *MOMY, const double *RHOT,
+    const double *CDZ, const double *CDX, const double *CDY, const double *FDZ, const double *FDX, const double *FDY,
+    const double *DT, const double *REF_dens, const double *REF_pott,
+    const double *ND_COEF, const int32_t *ND_ORDER, const double *ND_SFC_FACT, const int32_t *ND_USE_RS);
+
+/* ---- device-resident integration --------------------------------------------
+ * The state stays in HBM between calls; host arrays cross PCIe only in
+ * scale_hip_model_set / _get.  One large step reproduces the on-path subset of
+ * ATMOS_DYN_Tstep_large_fvm_heve,
+ * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_large_fvm_heve.F90:377-1127:
+ * RT2P/DPRES0/REF_rhot, tendencies (DAMP_alpha = 0), numerical filter, RK4
+ * short steps (tinteg_short_rk4.F90:260-660), ghost levels, halos, the mass
+ * flux average and the RK3 tracer loop (tinteg_tracer_rk3.F90:92-208). */
+typedef struct {
+  double dtl;          /* TIME_DT_ATMOS_DYN * large/short ratio: large step [s] */
+  double dts;          /* short step [s]; nstep = ceiling((dtl-eps)/dts)       */
+  double nd_coef;      /* ATMOS_DYN_NUMERICAL_DIFF_COEF (1e-4 by default)      */
+  double nd_sfc_fact;  /* ATMOS_DYN_NUMERICAL_DIFF_sfc_fact (1.0)              */
+  double divdmp_coef;  /* ATMOS_DYN_divdmp_coef (0; only 0 is supported here)  */
+  int32_t nd_order;    /* ATMOS_DYN_NUMERICAL_DIFF_order (1)                   */
+  int32_t fct_tracer;  /* ATMOS_DYN_FLAG_FCT_TRACER                            */
+  int32_t fct_along_stream; /* ATMOS_DYN_FLAG_FCT_along_stream (1)             */
+} scale_hip_dyn_config;
+
+/* field ids for scale_hip_model_set/_get/_mass */
+enum {
+  SCALE_HIP_DENS = 0, SCALE_HIP_MOMZ = 1, SCALE_HIP_MOMX = 2, SCALE_HIP_MOMY = 3, SCALE_HIP_RHOT = 4,
+  SCALE_HIP_REF_DENS = 10, SCALE_HIP_REF_POTT = 11, SCALE_HIP_REF_PRES = 12,
+  SCALE_HIP_CORIOLI = 13,    /* (1,IA,JA) */
+  SCALE_HIP_WDAMP_COEF = 14, /* (KA)      */
+  SCALE_HIP_MFLX_HI = 20, SCALE_HIP_TFLX_HI = 21, /* (KA,IA,JA,3), get only */
+  SCALE_HIP_DENS_TP = 30, SCALE_HIP_MOMZ_TP = 31, SCALE_HIP_MOMX_TP = 32, SCALE_HIP_MOMY_TP = 33, SCALE_HIP_RHOT_TP = 34,
+  SCALE_HIP_QTRC = 100,      /* + tracer index */
+  SCALE_HIP_RHOQ_TP = 200    /* + tracer index */
+};
+
+/* Grid and metric arrays of scale_atmos_grid_cartesC(_metric); shapes as in
+ * the Tstep_short interface plus CDX(IA), CDY(JA). */
+int scale_hip_set_metrics(scale_hip_ctx *ctx,
+    const double *CDZ, const double *FDZ, const double *RCDZ, const double *RFDZ,
+    const double *CDX, const double *FDX, const double *RCDX, const double *RFDX,
+    const double *CDY, const double *FDY, const double *RCDY, const double *RFDY,
+    const double *PHI, const double *GSQRT, const double *J13G, const double *J23G,
+    const double *J33G, const double *MAPF);
+int scale_hip_model_create(scale_hip_ctx *ctx, const scale_hip_dyn_config *cfg, int32_t QA);
+int scale_hip_model_set(scale_hip_ctx *ctx, int32_t field, const double *host);
+int scale_hip_model_get(scale_hip_ctx *ctx, int32_t field, double *host);
+/* n large steps (dynamics + tracers) */
+int scale_hip_model_large_step(scale_hip_ctx *ctx, int32_t n);
+/* large-step preparation only (RT2P, DPRES0, REF_rhot, tendencies) */
+int scale_hip_model_prepare(scale_hip_ctx *ctx);
+/* n short steps: numerical filter + 4 HEVI stages + RK4 combination + halos,
+ * using the preparation of the last scale_hip_model_prepare */
+int scale_hip_model_short_step(scale_hip_ctx *ctx, int32_t n);
+/* FP64 sum over the interior of field(k,i,j) [* DENS if weight_dens] * CDZ*CDX*CDY*GSQRT/(MAPF1*MAPF2) */
+int scale_hip_model_mass(scale_hip_ctx *ctx, int32_t field, int32_t weight_dens, double *out);
+/* device pointer of a resident field (device layout: i fastest, then j, then k) */
+int scale_hip_model_device_ptr(scale_hip_ctx *ctx, int32_t field, double **dptr);
+/* the HIP stream every model call is ordered on */
+int scale_hip_stream(scale_hip_ctx *ctx, void **stream);
+
+/* Named kernel timers: hipEvents recorded around every launch on the context's
+ * stream (the counterpart of PROF_rapstart/PROF_rapend,
+ * scalelib/src/common/scale_prof.F90:156-253).  Names: hevi_explicit,
+ * hevi_solve, numfilter, halo_fill, rk4_combine, tracer_nofct, fct_philo,
+ * fct_ratio, fct_update.  Enabling resets the accumulators. */
+int scale_hip_profile_enable(scale_hip_ctx *ctx, int32_t on);
+int scale_hip_profile_query(scale_hip_ctx *ctx, const char *name, int64_t *count, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,215 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the scale_amd package.
+
+Arrays are NumPy float64 in Fortran order with the reference's shapes
+((KA,IA,JA), (KA,IA,JA,3), (KA,IA,JA,5,3), (IA,JA,2,4) ...).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+UD1, UD3, CD4 = 1, 3, 4
+
+_dp = C.POINTER(C.c_double)
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("KA", "IA", "JA", "KS", "KE", "IS", "IE", "JS", "JE")]
+
+
+class OrcMetric(C.Structure):
+    _fields_ = [(n, _dp) for n in ("CDZ", "FDZ", "RCDZ", "RFDZ", "CDX", "FDX", "RCDX", "RFDX",
+                                   "CDY", "FDY", "RCDY", "RFDY", "PHI", "GSQRT", "J13G", "J23G", "MAPF")] + [("J33G", C.c_double)]
+
+
+class OrcState(C.Structure):
+    _fields_ = [("DENS", _dp), ("MOMZ", _dp), ("MOMX", _dp), ("MOMY", _dp), ("RHOT", _dp),
+                ("QTRC", _dp), ("QA", C.c_int),
+                ("REF_dens", _dp), ("REF_pott", _dp), ("REF_pres", _dp),
+                ("DENS_tp", _dp), ("MOMZ_tp", _dp), ("MOMX_tp", _dp), ("MOMY_tp", _dp), ("RHOT_tp", _dp), ("RHOQ_tp", _dp),
+                ("CORIOLI", _dp), ("wdamp_coef", _dp), ("mflx_hi", _dp), ("tflx_hi", _dp)]
+
+
+class OrcConfig(C.Structure):
+    _fields_ = [("scheme", C.c_int), ("scheme_tracer", C.c_int), ("tinteg_short", C.c_int),
+                ("dtl", C.c_double), ("dts", C.c_double), ("nd_coef", C.c_double), ("nd_sfc_fact", C.c_double),
+                ("nd_coef_q", C.c_double), ("nd_order", C.c_int), ("nd_use_rs", C.c_int),
+                ("divdmp_coef", C.c_double), ("flag_fct_tracer", C.c_int), ("flag_fct_along_stream", C.c_int)]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "-s"], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = C.CDLL(LIB_PATH)
+        _lib.orc_mass_sum.restype = C.c_double
+        _lib.orc_get_threads.restype = C.c_int
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64
+    assert a.flags.f_contiguous or a.ndim == 1, "oracle arrays must be Fortran-contiguous"
+    return a.ctypes.data_as(_dp)
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))
+
+
+def grid_struct(g) -> OrcGrid:
+    return OrcGrid(g.KA, g.IA, g.JA, g.KS, g.KE, g.IS, g.IE, g.JS, g.JE)
+
+
+def metric_struct(m: dict) -> OrcMetric:
+    s = OrcMetric()
+    for n in ("CDZ", "FDZ", "RCDZ", "RFDZ", "CDX", "FDX", "RCDX", "RFDX", "CDY", "FDY", "RCDY", "RFDY",
+              "PHI", "GSQRT", "J13G", "J23G", "MAPF"):
+        setattr(s, n, ptr(m[n]))
+    s.J33G = float(m["J33G"])
+    return s
+
+
+def nan_like_outputs(g, n=None):
+    shape = (g.KA, g.IA, g.JA) if n is None else (g.KA, g.IA, g.JA, n)
+    return np.full(shape, np.nan, order="F")
+
+
+def tstep_short_hevi(case, inp: dict, dtrk: float, scheme: int | None = None, out: dict | None = None) -> dict:
+    """One HEVI stage.  inp holds DENS0..RHOT0, DENS..RHOT, DENS_t..RHOT_t, DPRES0, RT2P,
+    num_diff (KA,IA,JA,5,3), DDIV, REF_dens, REF_rhot; optional CORIOLI, wdamp_coef, divdmp_coef."""
+    g = case.grid
+    L = lib()
+    if out is None:
+        out = {k: nan_like_outputs(g) for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK")}
+        out["mflx_hi"] = nan_like_outputs(g, 3)
+        out["tflx_hi"] = nan_like_outputs(g, 3)
+    gs, ms = grid_struct(g), metric_struct(case.metric)
+    cor = inp.get("CORIOLI", case.corioli)
+    wd = inp.get("wdamp_coef", case.wdamp_coef)
+    rc = L.orc_tstep_short_hevi(
+        C.byref(gs), C.c_int(scheme if scheme is not None else case.scheme),
+        *[ptr(out[k]) for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK", "mflx_hi", "tflx_hi")],
+        *[ptr(inp[k]) for k in ("DENS0", "MOMZ0", "MOMX0", "MOMY0", "RHOT0", "DENS", "MOMZ", "MOMX", "MOMY", "RHOT",
+                                "DENS_t", "MOMZ_t", "MOMX_t", "MOMY_t", "RHOT_t", "DPRES0", "RT2P")],
+        ptr(cor), ptr(inp["num_diff"]), ptr(wd), C.c_double(inp.get("divdmp_coef", 0.0)), ptr(inp["DDIV"]),
+        C.byref(ms), ptr(inp["REF_dens"]), ptr(inp["REF_rhot"]), C.c_double(dtrk))
+    if rc != 0:
+        raise RuntimeError(f"orc_tstep_short_hevi failed: {rc}")
+    return out
+
+
+def numfilter_coef(case, state: dict, dt: float, nd_coef=1.0e-4, nd_sfc_fact=1.0) -> np.ndarray:
+    g = case.grid
+    nd = np.full((g.KA, g.IA, g.JA, 5, 3), np.nan, order="F")
+    gs, ms = grid_struct(g), metric_struct(case.metric)
+    rc = lib().orc_numfilter_coef(C.byref(gs), ptr(nd), *[ptr(state[k]) for k in ("DENS", "MOMZ", "MOMX", "MOMY", "RHOT")],
+                                  C.byref(ms), C.c_double(dt), ptr(case.ref["REF_dens"]), ptr(case.ref["REF_pott"]),
+                                  C.c_double(nd_coef), C.c_int(1), C.c_double(nd_sfc_fact), C.c_int(1))
+    if rc != 0:
+        raise RuntimeError("orc_numfilter_coef failed")
+    return nd
+
+
+def tstep_tracer(case, QTRCo, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff_q, dtl, fct: bool,
+                 along_stream: bool = True, scheme: int | None = None):
+    g = case.grid
+    gs = grid_struct(g)
+    m = case.metric
+    mapf_xy = np.asfortranarray(m["MAPF"][:, :, :, 0])
+    rc = lib().orc_tstep_tracer_heve(C.byref(gs), C.c_int(scheme if scheme is not None else case.scheme_tracer),
+                                     ptr(QTRCo), ptr(QTRC), ptr(QTRC0), ptr(RHOQ_t), ptr(DENS0), ptr(DENS),
+                                     ptr(mflx_hi), ptr(num_diff_q), ptr(m["GSQRT"]), ptr(mapf_xy), ptr(m["CDZ"]),
+                                     ptr(m["RCDZ"]), ptr(m["RCDX"]), ptr(m["RCDY"]), C.c_double(dtl),
+                                     C.c_int(int(fct)), C.c_int(int(along_stream)))
+    if rc != 0:
+        raise RuntimeError("orc_tstep_tracer_heve failed")
+    return QTRCo
+
+
+def get_fact_fct(rw, ru, rv) -> np.ndarray:
+    f = np.zeros(18)
+    lib().orc_get_fact_fct(ptr(f), C.c_double(rw), C.c_double(ru), C.c_double(rv))
+    return f.reshape((2, 3, 3), order="F")  # fact[a, b+1, c+1]
+
+
+def solve_direct(g, Cvec, F1, F2, F3):
+    Cvec = np.ascontiguousarray(Cvec, dtype=np.float64).copy()
+    gs = grid_struct(g)
+    lib().orc_solve_direct(C.byref(gs), ptr(Cvec), ptr(np.ascontiguousarray(F1)), ptr(np.ascontiguousarray(F2)),
+                           ptr(np.ascontiguousarray(F3)))
+    return Cvec
+
+
+def halo_periodic(g, a):
+    gs = grid_struct(g)
+    lib().orc_halo_periodic(C.byref(gs), ptr(a))
+    return a
+
+
+class Model:
+    """Whole-model integrator (LARGE.F90 on-path subset) over a Case."""
+
+    def __init__(self, case, nd_coef=1.0e-4, divdmp_coef=0.0, fct=None, along_stream=True):
+        g = case.grid
+        self.case = case
+        self.state = {k: np.array(v, order="F", copy=True) for k, v in case.state.items()}
+        qa = case.qtrc.shape[-1]
+        self.qa = qa
+        self.qtrc = np.array(case.qtrc if qa > 0 else np.zeros(g.shape + (1,)), order="F", copy=True)
+        self.mflx_hi = np.full(g.shape + (3,), np.nan, order="F")
+        self.tflx_hi = np.full(g.shape + (3,), np.nan, order="F")
+        self.rhoq_tp = np.zeros(g.shape + (max(qa, 1),), order="F")
+        self.cfg = OrcConfig(case.scheme, case.scheme_tracer, 4, case.dt, case.dt, nd_coef, 1.0, 0.0, 1, 1,
+                             divdmp_coef, int(case.fct if fct is None else fct), int(along_stream))
+        self._keep = []
+
+    def _st(self):
+        c = self.case
+        s = OrcState()
+        for k in ("DENS", "MOMZ", "MOMX", "MOMY", "RHOT"):
+            setattr(s, k, ptr(self.state[k]))
+        s.QTRC = ptr(self.qtrc)
+        s.QA = self.qa
+        s.REF_dens, s.REF_pott, s.REF_pres = (ptr(c.ref[k]) for k in ("REF_dens", "REF_pott", "REF_pres"))
+        for k in ("DENS_tp", "MOMZ_tp", "MOMX_tp", "MOMY_tp", "RHOT_tp"):
+            setattr(s, k, ptr(c.zero_tend[k]))
+        s.RHOQ_tp = ptr(self.rhoq_tp)
+        s.CORIOLI = ptr(c.corioli)
+        s.wdamp_coef = ptr(c.wdamp_coef)
+        s.mflx_hi = ptr(self.mflx_hi)
+        s.tflx_hi = ptr(self.tflx_hi)
+        return s
+
+    def step(self, n=1):
+        g = self.case.grid
+        gs, ms = grid_struct(g), metric_struct(self.case.metric)
+        st = self._st()
+        for _ in range(n):
+            rc = lib().orc_large_step(C.byref(gs), C.byref(ms), C.byref(self.cfg), C.byref(st))
+            if rc != 0:
+                raise RuntimeError("orc_large_step failed")
+
+    def mass(self, a):
+        g = self.case.grid
+        return lib().orc_mass_sum(C.byref(grid_struct(g)), C.byref(metric_struct(self.case.metric)), ptr(a))

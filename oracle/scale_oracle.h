@@ -1,0 +1,177 @@
+/*
+ * scale_oracle.h -- CPU restatement of SCALE-RM's HEVI short step, FVM flux
+ * stencils, numerical filter and FCT tracer advection.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product path (scale_amd, libscalehip.so)
+ * never links, loads or calls it.
+ *
+ * Layout: every 3-D array is the reference's Fortran array (KA,IA,JA) with k
+ * fastest, addressed here with 0-based indices: element (k,i,j) lives at
+ * k + KA*(i + IA*j).  Fortran index kF = kC + 1, so KS = KHALO (=2),
+ * KE = KS+KMAX-1, and likewise for i and j.  1-D arrays (CDZ(KA), FDZ(KA-1),
+ * ...) shift the same way: CDZ(kF) = cdz[kC].  Multi-component arrays keep
+ * the Fortran component order: GSQRT(KA,IA,JA,7) component c (0-based) at
+ * offset c*KA*IA*JA; MAPF(IA,JA,2,4) entry (i,j,n,c) at
+ * i + IA*(j + JA*(n + 2*c)); num_diff(KA,IA,JA,5,3) entry (k,i,j,v,d) at
+ * IX3 + N3*(v + 5*d).
+ *
+ * Arithmetic follows the reference expression by expression (same grouping,
+ * left-to-right evaluation, sign() as copysign) and is compiled with
+ * -ffp-contract=off so that no fused multiply-add changes a rounding.
+ *
+ * Parity status of each function is stated in the header comment of its
+ * definition in scale_oracle.c and in the repository's design notes.
+ */
+#ifndef SCALE_ORACLE_H
+#define SCALE_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* flux scheme identifiers (the order of the upwind/central scheme) */
+#define ORC_UD1 1
+#define ORC_UD3 3
+#define ORC_CD4 4
+
+typedef struct {
+  int KA, IA, JA;
+  int KS, KE, IS, IE, JS, JE; /* 0-based, inclusive */
+} orc_grid;
+
+typedef struct {
+  const double *CDZ, *FDZ, *RCDZ, *RFDZ; /* (KA), (KA-1), (KA), (KA-1) */
+  const double *CDX, *FDX, *RCDX, *RFDX; /* (IA), (IA-1), (IA), (IA-1) */
+  const double *CDY, *FDY, *RCDY, *RFDY; /* (JA), (JA-1), (JA), (JA-1) */
+  const double *PHI;                     /* (KA,IA,JA)   geopotential   */
+  const double *GSQRT, *J13G, *J23G;     /* (KA,IA,JA,7)                */
+  const double *MAPF;                    /* (IA,JA,2,4)                 */
+  double J33G;
+} orc_metric;
+
+/* ATMOS_DYN_Tstep_short_fvm_hevi (HEVI.F90:128-1089), periodic lateral
+ * boundaries only (BND_* false).  Returns 0, or -1 for unsupported options. */
+int orc_tstep_short_hevi(const orc_grid *g, int scheme,
+    double *DENS_RK, double *MOMZ_RK, double *MOMX_RK, double *MOMY_RK, double *RHOT_RK,
+    double *mflx_hi, double *tflx_hi,
+    const double *DENS0, const double *MOMZ0, const double *MOMX0, const double *MOMY0, const double *RHOT0,
+    const double *DENS, const double *MOMZ, const double *MOMX, const double *MOMY, const double *RHOT,
+    const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
+    const double *DPRES0, const double *RT2P, const double *CORIOLI,
+    const double *num_diff, const double *wdamp_coef, double divdmp_coef, const double *DDIV,
+    const orc_metric *m, const double *REF_dens, const double *REF_rhot,
+    double dtrk);
+
+/* ATMOS_DYN_numfilter_coef (COMMON.F90:385-985), ND_ORDER = 1 only,
+ * followed by the periodic halo fill of the 15 num_diff fields. */
+int orc_numfilter_coef(const orc_grid *g, double *num_diff,
+    const double *DENS, const double *MOMZ, const double *MOMX, const double *MOMY, const double *RHOT,
+    const orc_metric *m, double dt, const double *REF_dens, const double *REF_pott,
+    double nd_coef, int nd_order, double nd_sfc_fact, int nd_use_rs);
+
+/* ATMOS_DYN_divergence (COMMON.F90:1437-1515) */
+void orc_divergence(const orc_grid *g, double *DDIV,
+    const double *MOMZ, const double *MOMX, const double *MOMY, const orc_metric *m);
+
+/* ATMOS_DYN_wdamp_setup (COMMON.F90:323-381); FZ has KA+1 entries FZ(0:KA) */
+void orc_wdamp_setup(const orc_grid *g, double *wdamp_coef, double wdamp_tau,
+    double wdamp_height, const double *FZ);
+
+/* ATMOS_DYN_Tstep_tracer_fvm_heve (tracer_fvm_heve.F90:72-242);
+ * MAPF is the I_XY slice (IA,JA,2).  QTRCo may alias QTRC0. */
+int orc_tstep_tracer_heve(const orc_grid *g, int scheme,
+    double *QTRCo, const double *QTRC, const double *QTRC0, const double *RHOQ_t,
+    const double *DENS0, const double *DENS, const double *mflx_hi, const double *num_diff,
+    const double *GSQRT, const double *MAPF_XY, const double *CDZ,
+    const double *RCDZ, const double *RCDX, const double *RCDY,
+    double dtl, int flag_fct, int flag_fct_along_stream);
+
+/* ATMOS_DYN_fct (COMMON.F90:1922-2671) incl. its R+/R- halo exchange. */
+void orc_fct(const orc_grid *g, double *qflx_anti,
+    const double *phi_in, const double *DENS0, const double *DENS,
+    const double *qflx_hi, const double *qflx_lo, const double *mflx_hi,
+    const double *RDZ, const double *RDX, const double *RDY,
+    const double *GSQRT_XYZ, const double *MAPF_XY, double dt, int flag_vect);
+
+/* get_fact_fct (COMMON.F90:2676-2734): fact(0:1,-1:1,-1:1) returned as
+ * fact[a + 2*((b+1) + 3*(c+1))] for fact(a,b,c). */
+void orc_get_fact_fct(double *fact, double rw, double ru, double rv);
+
+/* periodic COMM_vars8 on one rank (COMM.F90:345-397,1600-2043): all KA
+ * levels of the IHALO/JHALO-wide lateral halo, faces and corners. */
+void orc_halo_periodic(const orc_grid *g, double *a);
+
+/* Thomas algorithm of HEVI.F90:1092-1127 on KMAX-1 unknowns.  F1/F2/F3 are
+ * indexed like the reference (KA entries, rows KS..KE-1); C has KMAX-1. */
+void orc_solve_direct(const orc_grid *g, double *C, const double *F1, const double *F2, const double *F3);
+
+/* ---- whole-model state for multi-step integrations ---------------------- */
+
+typedef struct {
+  /* prognostics (KA,IA,JA) with valid halos */
+  double *DENS, *MOMZ, *MOMX, *MOMY, *RHOT;
+  double *QTRC;        /* (KA,IA,JA,QA) */
+  int QA;
+  /* reference state (KA,IA,JA) incl. ghost levels and halos */
+  const double *REF_dens, *REF_pott, *REF_pres;
+  /* physics tendencies (zero in every workload here but kept as inputs) */
+  const double *DENS_tp, *MOMZ_tp, *MOMX_tp, *MOMY_tp, *RHOT_tp, *RHOQ_tp;
+  const double *CORIOLI;    /* (1,IA,JA) */
+  const double *wdamp_coef; /* (KA) */
+  /* outputs of the last large step */
+  double *mflx_hi, *tflx_hi; /* (KA,IA,JA,3) */
+} orc_state;
+
+typedef struct {
+  int scheme;          /* dynamics flux scheme (ORC_UD3 or ORC_CD4) */
+  int scheme_tracer;   /* tracer flux scheme */
+  int tinteg_short;    /* 4 = RK4 (only option implemented) */
+  double dtl;          /* large step */
+  double dts;          /* short step (dtl/dts rounded up as in LARGE.F90:379-382) */
+  double nd_coef, nd_sfc_fact, nd_coef_q;
+  int nd_order, nd_use_rs;
+  double divdmp_coef;
+  int flag_fct_tracer, flag_fct_along_stream;
+} orc_config;
+
+/* ATMOS_DYN_Tstep_large_fvm_heve on-path subset (LARGE.F90:377-1127) for a
+ * dry run with QA passive tracers (moist constants with QV=0 reduce to the
+ * dry ones): prep of RT2P/DPRES0/REF_rhot, tendencies (DAMP_alpha=0),
+ * numfilter, divergence, RK4 short steps, ghost-level fill, halo fill,
+ * mflx average and the tracer RK3 loop.  Advances st by one large step. */
+int orc_large_step(const orc_grid *g, const orc_metric *m, const orc_config *c, orc_state *st);
+
+/* One full RK4 short step (tinteg_short_rk4.F90:260-660) on caller buffers. */
+int orc_tinteg_short_rk4(const orc_grid *g, int scheme,
+    double *DENS, double *MOMZ, double *MOMX, double *MOMY, double *RHOT,
+    double *mflx_hi, double *tflx_hi,
+    const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
+    const double *DPRES0, const double *RT2P, const double *CORIOLI,
+    const double *num_diff, const double *wdamp_coef, double divdmp_coef, const double *DDIV,
+    const orc_metric *m, const double *REF_dens, const double *REF_rhot, double dt);
+
+/* tinteg_tracer_rk3.F90:92-208 (RK3WS2002) */
+int orc_tinteg_tracer_rk3(const orc_grid *g, int scheme,
+    double *QTRC, const double *QTRC0, const double *RHOQ_t,
+    const double *DENS0, const double *DENS, const double *mflx_hi, const double *num_diff,
+    const double *GSQRT, const double *MAPF_XY, const double *CDZ,
+    const double *RCDZ, const double *RCDX, const double *RCDY,
+    double dtl, int flag_fct, int flag_fct_along_stream);
+
+/* FP64 budgets over the interior: sum(a(k,i,j) * CDZ*CDX*CDY * GSQRT_XYZ / (MAPF1*MAPF2)),
+ * summed k-fastest then i then j in one fixed order (no threading). */
+double orc_mass_sum(const orc_grid *g, const orc_metric *m, const double *a);
+
+/* threads used by the OpenMP loops (1 = serial) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,677 @@
+// capi.hip -- extern "C" entry points of libscalehip.so (declared in
+// include/scale_hip.h): context setup, the drop-in Tstep_short / Tstep_tracer /
+// numfilter calls on host Fortran arrays, and the device-resident model.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include "../../include/scale_hip.h"
+#include "common.h"
+#include "model.h"
+#include "shp_internal.h"
+
+struct scale_hip_ctx {
+  shp::Ctx c;
+  // drop-in device buffers (allocated on first use)
+  std::vector<double*> dbuf;
+  bool dropin_ready = false;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg, int code = -1) {
+  g_err = msg;
+  return code;
+}
+int hipfail(hipError_t e, const char* where) {
+  return fail(std::string(where) + ": " + hipGetErrorString(e), -2);
+}
+#define CK(x)                                      \
+  do {                                             \
+    hipError_t e_ = (x);                           \
+    if (e_ != hipSuccess) return hipfail(e_, #x);  \
+  } while (0)
+
+using shp::Ctx;
+using shp::Dims;
+
+double* dalloc(Ctx& c, size_t n, std::vector<void*>* track) {
+  void* p = nullptr;
+  if (hipMalloc(&p, n * sizeof(double) + 256) != hipSuccess) return nullptr;
+  if (track) track->push_back(p);
+  return (double*)p;
+}
+
+int ensure_stage(Ctx& c, size_t elems) {
+  if (c.stage_elems >= elems) return 0;
+  if (c.stage) hipFree(c.stage);
+  c.stage = nullptr;
+  c.stage_elems = 0;
+  if (hipMalloc((void**)&c.stage, elems * sizeof(double)) != hipSuccess) return fail("hipMalloc of the staging buffer failed");
+  c.stage_elems = elems;
+  return 0;
+}
+
+// host Fortran array (ncomp x KA*IA*JA) -> device layout
+int upload3(Ctx& c, double* dst, const double* host, int ncomp) {
+  if (!host) return fail("null array argument");
+  const size_t n = c.d.n3 * (size_t)ncomp;
+  if (int r = ensure_stage(c, n)) return r;
+  CK(hipMemcpyAsync(c.stage, host, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CK(shp::to_device_layout(c, dst, c.stage, ncomp, c.stream));
+  return 0;
+}
+int download3(Ctx& c, double* host, const double* src, int ncomp) {
+  const size_t n = c.d.n3 * (size_t)ncomp;
+  if (int r = ensure_stage(c, n)) return r;
+  CK(shp::to_fortran_layout(c, c.stage, src, ncomp, c.stream));
+  CK(hipMemcpyAsync(host, c.stage, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  CK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+int upload_raw(Ctx& c, double* dst, const double* host, size_t n) {
+  if (!host) return fail("null array argument");
+  CK(hipMemcpyAsync(dst, host, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  return 0;
+}
+
+bool all_equal(const double* a, size_t n, double v) {
+  for (size_t q = 0; q < n; ++q)
+    if (a[q] != v) return false;
+  return true;
+}
+
+// 1-D grid arrays: CDZ FDZ RCDZ RFDZ (KA) | CDX FDX RCDX RFDX (IA) | CDY FDY RCDY RFDY (JA)
+int set_grid1d(Ctx& c, const double* CDZ, const double* FDZ, const double* RCDZ, const double* RFDZ,
+               const double* CDX, const double* FDX, const double* RCDX, const double* RFDX,
+               const double* CDY, const double* FDY, const double* RCDY, const double* RFDY) {
+  const Dims& d = c.d;
+  std::vector<double> h(4 * (size_t)(d.KA + d.IA + d.JA), 0.0);
+  double* p = h.data();
+  auto put = [&](const double* s, int n, int len) {
+    if (s) memcpy(p, s, sizeof(double) * (size_t)len);
+    p += n;
+  };
+  put(CDZ, d.KA, d.KA); put(FDZ, d.KA, d.KA - 1); put(RCDZ, d.KA, d.KA); put(RFDZ, d.KA, d.KA - 1);
+  put(CDX, d.IA, d.IA); put(FDX, d.IA, d.IA - 1); put(RCDX, d.IA, d.IA); put(RFDX, d.IA, d.IA - 1);
+  put(CDY, d.JA, d.JA); put(FDY, d.JA, d.JA - 1); put(RCDY, d.JA, d.JA); put(RFDY, d.JA, d.JA - 1);
+  CK(hipMemcpyAsync(c.g1buf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CK(hipStreamSynchronize(c.stream));
+  if (CDZ) c.hCDZ.assign(CDZ, CDZ + d.KA);
+  if (FDZ) c.hFDZ.assign(FDZ, FDZ + d.KA - 1);
+  if (CDX) c.hCDX.assign(CDX, CDX + d.IA);
+  if (FDX) c.hFDX.assign(FDX, FDX + d.IA - 1);
+  if (CDY) c.hCDY.assign(CDY, CDY + d.JA);
+  if (FDY) c.hFDY.assign(FDY, FDY + d.JA - 1);
+  return 0;
+}
+
+// metric arrays; decides the flat (TERRAIN=false) specialisation
+int set_metric3d(Ctx& c, const double* PHI, const double* GSQRT, const double* J13G, const double* J23G, double J33G, const double* MAPF) {
+  const Dims& d = c.d;
+  if (PHI) { if (int r = upload3(c, c.PHI, PHI, 1)) return r; }
+  const bool flat = GSQRT && J13G && J23G && MAPF && all_equal(GSQRT, 7 * d.n3, 1.0) && all_equal(J13G, 7 * d.n3, 0.0) &&
+                    all_equal(J23G, 7 * d.n3, 0.0) && all_equal(MAPF, 8 * d.pl, 1.0);
+  if (!flat) {
+    if (!GSQRT || !J13G || !J23G || !MAPF) return fail("metric arrays are required");
+    if (!c.GSQRT) {
+      c.GSQRT = dalloc(c, 7 * d.n3, nullptr);
+      c.J13G = dalloc(c, 7 * d.n3, nullptr);
+      c.J23G = dalloc(c, 7 * d.n3, nullptr);
+      if (!c.GSQRT || !c.J13G || !c.J23G) return fail("hipMalloc of metric arrays failed");
+    }
+    if (int r = upload3(c, c.GSQRT, GSQRT, 7)) return r;
+    if (int r = upload3(c, c.J13G, J13G, 7)) return r;
+    if (int r = upload3(c, c.J23G, J23G, 7)) return r;
+  }
+  if (MAPF) { if (int r = upload_raw(c, c.MAPF, MAPF, 8 * d.pl)) return r; }
+  c.terrain = !flat;
+  c.J33G = J33G;
+  c.metrics_set = true;
+  CK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+// numerical-filter tables (ATMOS_DYN_filter_setup, COMMON.F90:92-319) and
+// nd_coef_* (COMMON.F90:494-511) for a given DT and ND_COEF
+int set_numfilter_tables(Ctx& c, double DT, double ND_COEF, double sfc) {
+  if (c.nd_coef == ND_COEF && c.nd_dt == DT && c.nd_sfc_fact == sfc && c.CNZ3) return 0;
+  const Dims& d = c.d;
+  const int KA = d.KA, IA = d.IA, JA = d.JA, KS = d.KS, KE = d.KE, IS = d.IS, IE = d.IE, JS = d.JS, JE = d.JE;
+  if ((int)c.hCDZ.size() != KA || (int)c.hCDX.size() != IA || (int)c.hCDY.size() != JA || (int)c.hFDZ.size() != KA - 1)
+    return fail("numfilter: CDZ/CDX/CDY/FDZ/FDX/FDY must be set");
+  const double* CDZ = c.hCDZ.data(); const double* FDZ = c.hFDZ.data();
+  const double* CDX = c.hCDX.data(); const double* FDX = c.hFDX.data();
+  const double* CDY = c.hCDY.data(); const double* FDY = c.hFDY.data();
+  std::vector<double> z(3 * KA * 2, 0.0), x(3 * IA * 2, 0.0), y(3 * JA * 2, 0.0);
+#define CNZ3(a, k, b) z[(a)-1 + 3 * ((size_t)(k) + (size_t)KA * ((b)-1))]
+#define CNX3(a, i, b) x[(a)-1 + 3 * ((size_t)(i) + (size_t)IA * ((b)-1))]
+#define CNY3(a, j, b) y[(a)-1 + 3 * ((size_t)(j) + (size_t)JA * ((b)-1))]
+  for (int k = KS + 1; k <= KE - 1; ++k) {
+    CNZ3(1, k, 1) = 1.0 / (FDZ[k] * CDZ[k] * FDZ[k - 1]);
+    CNZ3(2, k, 1) = 1.0 / (FDZ[k] * CDZ[k] * FDZ[k - 1]) + 1.0 / (FDZ[k - 1] * CDZ[k] * FDZ[k - 1]) + 1.0 / (FDZ[k - 1] * CDZ[k - 1] * FDZ[k - 1]);
+  }
+  for (int k = KS + 2; k <= KE; ++k)
+    CNZ3(3, k, 1) = 1.0 / (FDZ[k - 1] * CDZ[k] * FDZ[k - 1]) + 1.0 / (FDZ[k - 1] * CDZ[k - 1] * FDZ[k - 1]) + 1.0 / (FDZ[k - 1] * CDZ[k - 1] * FDZ[k - 2]);
+  CNZ3(1, KS - 1, 1) = 1.0 / (FDZ[KS] * CDZ[KS + 1] * FDZ[KS]);
+  CNZ3(1, KS, 1) = 1.0 / (FDZ[KS] * CDZ[KS] * FDZ[KS]);
+  CNZ3(2, KS, 1) = 1.0 / (FDZ[KS] * CDZ[KS] * FDZ[KS]) + 1.0 / (FDZ[KS] * CDZ[KS] * FDZ[KS]) + 1.0 / (FDZ[KS] * CDZ[KS + 1] * FDZ[KS]);
+  CNZ3(3, KS, 1) = 1.0 / (FDZ[KS] * CDZ[KS] * FDZ[KS]) + 1.0 / (FDZ[KS] * CDZ[KS + 1] * FDZ[KS]) + 1.0 / (FDZ[KS] * CDZ[KS + 1] * FDZ[KS + 1]);
+  CNZ3(3, KS + 1, 1) = 1.0 / (FDZ[KS] * CDZ[KS + 1] * FDZ[KS]) + 1.0 / (FDZ[KS] * CDZ[KS] * FDZ[KS]) + 1.0 / (FDZ[KS] * CDZ[KS] * FDZ[KS]);
+  CNZ3(1, KE, 1) = 1.0 / (FDZ[KE - 1] * CDZ[KE] * FDZ[KE - 1]);
+  CNZ3(2, KE, 1) = 1.0 / (FDZ[KE - 1] * CDZ[KE] * FDZ[KE - 1]) + 1.0 / (FDZ[KE - 1] * CDZ[KE] * FDZ[KE - 1]) + 1.0 / (FDZ[KE - 1] * CDZ[KE - 1] * FDZ[KE - 1]);
+  CNZ3(1, KE + 1, 1) = 1.0 / (FDZ[KE - 2] * CDZ[KE - 1] * FDZ[KE - 1]);
+  CNZ3(2, KE + 1, 1) = 1.0 / (FDZ[KE - 2] * CDZ[KE - 1] * FDZ[KE - 1]) + 1.0 / (FDZ[KE - 1] * CDZ[KE - 1] * FDZ[KE - 1]) + 1.0 / (FDZ[KE - 1] * CDZ[KE] * FDZ[KE - 1]);
+  CNZ3(3, KE + 1, 1) = 1.0 / (FDZ[KE - 1] * CDZ[KE + 1] * FDZ[KE - 1]) + 1.0 / (FDZ[KE - 1] * CDZ[KE] * FDZ[KE - 1]) + 1.0 / (FDZ[KE - 1] * CDZ[KE] * FDZ[KE - 1]);
+  for (int k = KS + 1; k <= KE - 1; ++k) {
+    CNZ3(1, k, 2) = 1.0 / (CDZ[k + 1] * FDZ[k] * CDZ[k]);
+    CNZ3(2, k, 2) = 1.0 / (CDZ[k + 1] * FDZ[k] * CDZ[k]) + 1.0 / (CDZ[k] * FDZ[k] * CDZ[k]) + 1.0 / (CDZ[k] * FDZ[k - 1] * CDZ[k]);
+    CNZ3(3, k, 2) = 1.0 / (CDZ[k] * FDZ[k] * CDZ[k]) + 1.0 / (CDZ[k] * FDZ[k - 1] * CDZ[k]) + 1.0 / (CDZ[k] * FDZ[k - 1] * CDZ[k - 1]);
+  }
+  CNZ3(1, KS - 1, 2) = 1.0 / (CDZ[KS] * FDZ[KS] * CDZ[KS]);
+  CNZ3(1, KS, 2) = 1.0 / (CDZ[KS + 1] * FDZ[KS] * CDZ[KS]);
+  CNZ3(2, KS, 2) = 1.0 / (CDZ[KS + 1] * FDZ[KS] * CDZ[KS]) + 1.0 / (CDZ[KS] * FDZ[KS] * CDZ[KS]) + 1.0 / (CDZ[KS] * FDZ[KS] * CDZ[KS]);
+  CNZ3(3, KS, 2) = 1.0 / (CDZ[KS] * FDZ[KS] * CDZ[KS]) + 1.0 / (CDZ[KS] * FDZ[KS] * CDZ[KS]) + 1.0 / (CDZ[KS] * FDZ[KS] * CDZ[KS + 1]);
+  CNZ3(1, KE, 2) = 1.0 / (CDZ[KE - 1] * FDZ[KE - 1] * CDZ[KE]);
+  CNZ3(2, KE, 2) = 1.0 / (CDZ[KE - 1] * FDZ[KE - 1] * CDZ[KE]) + 1.0 / (CDZ[KE] * FDZ[KE - 1] * CDZ[KE]) + 1.0 / (CDZ[KE] * FDZ[KE - 1] * CDZ[KE]);
+  CNZ3(3, KE, 2) = 1.0 / (CDZ[KE] * FDZ[KE - 1] * CDZ[KE]) + 1.0 / (CDZ[KE] * FDZ[KE - 1] * CDZ[KE]) + 1.0 / (CDZ[KE] * FDZ[KE - 1] * CDZ[KE - 1]);
+  CNZ3(1, KE + 1, 2) = 1.0 / (CDZ[KE - 2] * FDZ[KE - 2] * CDZ[KE - 1]);
+  CNZ3(2, KE + 1, 2) = 1.0 / (CDZ[KE - 2] * FDZ[KE - 2] * CDZ[KE - 1]) + 1.0 / (CDZ[KE - 1] * FDZ[KE - 2] * CDZ[KE - 1]) + 1.0 / (CDZ[KE - 1] * FDZ[KE - 1] * CDZ[KE - 1]);
+  CNZ3(3, KE + 1, 2) = 1.0 / (CDZ[KE - 1] * FDZ[KE - 2] * CDZ[KE - 1]) + 1.0 / (CDZ[KE - 1] * FDZ[KE - 1] * CDZ[KE - 1]) + 1.0 / (CDZ[KE - 1] * FDZ[KE - 1] * CDZ[KE]);
+  CNX3(1, IS - 1, 1) = 1.0 / (FDX[IS - 1] * CDX[IS - 1] * FDX[IS - 2]);
+  for (int i = IS; i <= IE + 1; ++i) {
+    CNX3(1, i, 1) = 1.0 / (FDX[i] * CDX[i] * FDX[i - 1]);
+    CNX3(2, i, 1) = 1.0 / (FDX[i] * CDX[i] * FDX[i - 1]) + 1.0 / (FDX[i - 1] * CDX[i] * FDX[i - 1]) + 1.0 / (FDX[i - 1] * CDX[i - 1] * FDX[i - 1]);
+    CNX3(3, i, 1) = 1.0 / (FDX[i - 1] * CDX[i] * FDX[i - 1]) + 1.0 / (FDX[i - 1] * CDX[i - 1] * FDX[i - 1]) + 1.0 / (FDX[i - 1] * CDX[i - 1] * FDX[i - 2]);
+  }
+  for (int i = IS - 1; i <= IE + 1; ++i) {
+    CNX3(1, i, 2) = 1.0 / (CDX[i + 1] * FDX[i] * CDX[i]);
+    CNX3(2, i, 2) = 1.0 / (CDX[i + 1] * FDX[i] * CDX[i]) + 1.0 / (CDX[i] * FDX[i] * CDX[i]) + 1.0 / (CDX[i] * FDX[i - 1] * CDX[i]);
+    CNX3(3, i, 2) = 1.0 / (CDX[i] * FDX[i] * CDX[i]) + 1.0 / (CDX[i] * FDX[i - 1] * CDX[i]) + 1.0 / (CDX[i] * FDX[i - 1] * CDX[i - 1]);
+  }
+  CNY3(1, JS - 1, 1) = 1.0 / (FDY[JS - 1] * CDY[JS - 1] * FDY[JS - 2]);
+  for (int j = JS; j <= JE + 1; ++j) {
+    CNY3(1, j, 1) = 1.0 / (FDY[j] * CDY[j] * FDY[j - 1]);
+    CNY3(2, j, 1) = 1.0 / (FDY[j] * CDY[j] * FDY[j - 1]) + 1.0 / (FDY[j - 1] * CDY[j] * FDY[j - 1]) + 1.0 / (FDY[j - 1] * CDY[j - 1] * FDY[j - 1]);
+    CNY3(3, j, 1) = 1.0 / (FDY[j - 1] * CDY[j] * FDY[j - 1]) + 1.0 / (FDY[j - 1] * CDY[j - 1] * FDY[j - 1]) + 1.0 / (FDY[j - 1] * CDY[j - 1] * FDY[j - 2]);
+  }
+  for (int j = JS - 1; j <= JE + 1; ++j) {
+    CNY3(1, j, 2) = 1.0 / (CDY[j + 1] * FDY[j] * CDY[j]);
+    CNY3(2, j, 2) = 1.0 / (CDY[j + 1] * FDY[j] * CDY[j]) + 1.0 / (CDY[j] * FDY[j] * CDY[j]) + 1.0 / (CDY[j] * FDY[j - 1] * CDY[j]);
+    CNY3(3, j, 2) = 1.0 / (CDY[j] * FDY[j] * CDY[j]) + 1.0 / (CDY[j] * FDY[j - 1] * CDY[j]) + 1.0 / (CDY[j] * FDY[j - 1] * CDY[j - 1]);
+  }
+#undef CNZ3
+#undef CNX3
+#undef CNY3
+  // nd_coef_* with x**4 evaluated as (x*x)*(x*x), as a real**integer power is
+  const double DIFF4 = ND_COEF / ((double)(1 << 4) * DT);
+  auto p4 = [](double v) { double v2 = v * v; return v2 * v2; };
+  std::vector<double> ndc(2 * (size_t)KA + 2 * (size_t)IA + 2 * (size_t)JA, 0.0);
+  double* nd_cdz = ndc.data(); double* nd_fdz = nd_cdz + KA; double* nd_cdx = nd_fdz + KA;
+  double* nd_fdx = nd_cdx + IA; double* nd_cdy = nd_fdx + IA; double* nd_fdy = nd_cdy + JA;
+  for (int k = KS - 1; k <= KE; ++k) nd_cdz[k] = DIFF4 * p4(CDZ[k]);
+  for (int k = KS + 1; k <= KE - 1; ++k) nd_fdz[k] = DIFF4 * p4(FDZ[k]);
+  for (int i = IS; i <= IE; ++i) { nd_cdx[i] = DIFF4 * p4(CDX[i]); nd_fdx[i] = DIFF4 * p4(FDX[i]); }
+  for (int j = JS; j <= JE; ++j) { nd_cdy[j] = DIFF4 * p4(CDY[j]); nd_fdy[j] = DIFF4 * p4(FDY[j]); }
+  if (!c.CNZ3) {
+    c.CNZ3 = dalloc(c, z.size(), nullptr); c.CNX3 = dalloc(c, x.size(), nullptr);
+    c.CNY3 = dalloc(c, y.size(), nullptr); c.ndc = dalloc(c, ndc.size(), nullptr);
+    if (!c.CNZ3 || !c.CNX3 || !c.CNY3 || !c.ndc) return fail("hipMalloc of filter tables failed");
+  }
+  CK(hipMemcpyAsync(c.CNZ3, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CK(hipMemcpyAsync(c.CNX3, x.data(), x.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CK(hipMemcpyAsync(c.CNY3, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CK(hipMemcpyAsync(c.ndc, ndc.data(), ndc.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CK(hipStreamSynchronize(c.stream));
+  c.nd_coef = ND_COEF; c.nd_dt = DT; c.nd_sfc_fact = sfc;
+  return 0;
+}
+
+// drop-in buffers: 0..4 s0, 5..9 s, 10..14 t, 15 DPRES0, 16 RT2P, 17 REF_dens, 18 REF_rhot, 19 DDIV,
+// 20 num_diff(15), 21..25 outputs, 26 mflx(3), 27 tflx(3), 28 CORIOLI(IA*JA), 29 wdamp(KA)
+enum { B_S0 = 0, B_S = 5, B_T = 10, B_DPRES0 = 15, B_RT2P, B_REFD, B_REFR, B_DDIV, B_ND, B_OUT, B_MF = 26, B_TF, B_COR, B_WD, B_N };
+
+int ensure_dropin(scale_hip_ctx* h) {
+  if (h->dropin_ready) return 0;
+  Ctx& c = h->c;
+  const size_t n3 = c.d.n3;
+  h->dbuf.assign(B_N, nullptr);
+  for (int q = 0; q < B_N; ++q) {
+    size_t n = n3;
+    if (q == B_ND) n = 15 * n3;
+    else if (q == B_MF || q == B_TF) n = 3 * n3;
+    else if (q == B_COR) n = c.d.pl;
+    else if (q == B_WD) n = (size_t)c.d.KA;
+    else if (q > B_OUT + 4 && q < B_MF) continue;
+    h->dbuf[q] = dalloc(c, n, nullptr);
+    if (!h->dbuf[q]) return fail("hipMalloc of drop-in buffers failed");
+  }
+  h->dropin_ready = true;
+  return 0;
+}
+
+int check_ctx(scale_hip_ctx* h) {
+  if (!h) return fail("null context");
+  CK(hipSetDevice(h->c.device));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* scale_hip_last_error(void) { return g_err.c_str(); }
+const char* scale_hip_version(void) { return "scale-hip 0.1 (gfx950)"; }
+
+int scale_hip_init(const scale_hip_grid* g, scale_hip_ctx** out) {
+  if (!g || !out) return fail("null argument");
+  *out = nullptr;
+  if (g->KHALO != 2 || g->IHALO != 2 || g->JHALO != 2) return fail("KHALO, IHALO and JHALO must be 2");
+  const int KS = g->KS - 1, KE = g->KE - 1, IS = g->IS - 1, IE = g->IE - 1, JS = g->JS - 1, JE = g->JE - 1;
+  if (KS != 2 || IS != 2 || JS != 2 || KE + 3 != g->KA || IE + 3 != g->IA || JE + 3 != g->JA)
+    return fail("grid bounds must be KS=IS=JS=3 and KA=KE+2, IA=IE+2, JA=JE+2 (halo 2)");
+  if (KE - KS + 1 < 4) return fail("KMAX must be >= 4");
+  if (IE - IS + 1 < 2 || JE - JS + 1 < 2) return fail("IMAX and JMAX must be >= 2");
+  if ((g->flux_scheme != SCALE_HIP_UD3 && g->flux_scheme != SCALE_HIP_CD4) ||
+      (g->flux_scheme_tracer != SCALE_HIP_UD3 && g->flux_scheme_tracer != SCALE_HIP_CD4))
+    return fail("flux schemes must be UD3 (3) or CD4 (4)");
+  if (hipSetDevice(g->device) != hipSuccess) return fail("hipSetDevice failed");
+  scale_hip_ctx* h = new scale_hip_ctx();
+  Ctx& c = h->c;
+  c.device = g->device;
+  c.d.KA = g->KA; c.d.IA = g->IA; c.d.JA = g->JA;
+  c.d.KS = KS; c.d.KE = KE; c.d.IS = IS; c.d.IE = IE; c.d.JS = JS; c.d.JE = JE;
+  c.d.pl = (size_t)g->IA * g->JA;
+  c.d.n3 = c.d.pl * g->KA;
+  c.scheme = g->flux_scheme;
+  c.scheme_tracer = g->flux_scheme_tracer;
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("hipStreamCreate failed"); }
+  const size_t n1 = 4 * (size_t)(c.d.KA + c.d.IA + c.d.JA);
+  c.g1buf = dalloc(c, n1, nullptr);
+  c.MAPF = dalloc(c, 8 * c.d.pl, nullptr);
+  c.PHI = dalloc(c, c.d.n3, nullptr);
+  bool ok = c.g1buf && c.MAPF && c.PHI;
+  for (int q = 0; q < shp::NSCR && ok; ++q) ok = (c.scr[q] = dalloc(c, c.d.n3, nullptr)) != nullptr;
+  if (!ok) { scale_hip_finalize(h); return fail("hipMalloc failed in scale_hip_init"); }
+  double* p = c.g1buf;
+  const int KA = c.d.KA, IA = c.d.IA, JA = c.d.JA;
+  c.g1.CDZ = p; c.g1.FDZ = p + KA; c.g1.RCDZ = p + 2 * KA; c.g1.RFDZ = p + 3 * KA; p += 4 * KA;
+  c.g1.CDX = p; c.g1.FDX = p + IA; c.g1.RCDX = p + 2 * IA; c.g1.RFDX = p + 3 * IA; p += 4 * IA;
+  c.g1.CDY = p; c.g1.FDY = p + JA; c.g1.RCDY = p + 2 * JA; c.g1.RFDY = p + 3 * JA;
+  // flat map factor until metrics are given
+  std::vector<double> ones(8 * c.d.pl, 1.0);
+  hipMemcpy(c.MAPF, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice);
+  *out = h;
+  return 0;
+}
+
+int scale_hip_finalize(scale_hip_ctx* h) {
+  if (!h) return 0;
+  Ctx& c = h->c;
+  hipSetDevice(c.device);
+  if (c.stream) hipStreamSynchronize(c.stream);
+  auto F = [](void* p) { if (p) hipFree(p); };
+  F(c.g1buf); F(c.GSQRT); F(c.J13G); F(c.J23G); F(c.MAPF); F(c.PHI);
+  F(c.CNZ3); F(c.CNX3); F(c.CNY3); F(c.ndc); F(c.stage);
+  for (int q = 0; q < shp::NSCR; ++q) F(c.scr[q]);
+  for (double* b : h->dbuf) F(b);
+  if (c.model) {
+    for (void* q : c.model->allocs) F(q);
+    delete c.model;
+  }
+  shp::prof_destroy(c.prof);
+  if (c.stream) hipStreamDestroy(c.stream);
+  delete h;
+  return 0;
+}
+
+int scale_hip_synchronize(scale_hip_ctx* h) {
+  if (int r = check_ctx(h)) return r;
+  CK(hipStreamSynchronize(h->c.stream));
+  return 0;
+}
+
+int scale_hip_stream(scale_hip_ctx* h, void** stream) {
+  if (int r = check_ctx(h)) return r;
+  *stream = (void*)h->c.stream;
+  return 0;
+}
+
+int scale_hip_tstep_short_hevi(scale_hip_ctx* h,
+    double* DENS_RK, double* MOMZ_RK, double* MOMX_RK, double* MOMY_RK, double* RHOT_RK,
+    double* PROG_RK, double* mflx_hi, double* tflx_hi,
+    const double* DENS0, const double* MOMZ0, const double* MOMX0, const double* MOMY0, const double* RHOT0,
+    const double* DENS, const double* MOMZ, const double* MOMX, const double* MOMY, const double* RHOT,
+    const double* DENS_t, const double* MOMZ_t, const double* MOMX_t, const double* MOMY_t, const double* RHOT_t,
+    const double* PROG0, const double* PROG,
+    const double* DPRES0, const double* RT2P, const double* CORIOLI,
+    const double* num_diff, const double* wdamp_coef, const double* divdmp_coef, const double* DDIV,
+    const int32_t* FLAG_FCT_MOMENTUM, const int32_t* FLAG_FCT_T, const int32_t* FLAG_FCT_ALONG_STREAM,
+    const double* CDZ, const double* FDZ, const double* FDX, const double* FDY,
+    const double* RCDZ, const double* RCDX, const double* RCDY, const double* RFDZ, const double* RFDX, const double* RFDY,
+    const double* PHI, const double* GSQRT, const double* J13G, const double* J23G, const double* J33G, const double* MAPF,
+    const double* REF_dens, const double* REF_rhot,
+    const int32_t* BND_W, const int32_t* BND_E, const int32_t* BND_S, const int32_t* BND_N,
+    const double* dtrk, const int32_t* last) {
+  (void)PROG_RK; (void)PROG0; (void)PROG; (void)FLAG_FCT_MOMENTUM; (void)FLAG_FCT_T; (void)FLAG_FCT_ALONG_STREAM; (void)last;
+  if (int r = check_ctx(h)) return r;
+  if (!dtrk || !divdmp_coef || !J33G) return fail("null scalar argument");
+  if ((BND_W && *BND_W) || (BND_E && *BND_E) || (BND_S && *BND_S) || (BND_N && *BND_N))
+    return fail("only periodic lateral boundaries (BND_* = .false.) are supported");
+  Ctx& c = h->c;
+  if (int r = ensure_dropin(h)) return r;
+  if (int r = set_grid1d(c, CDZ, FDZ, RCDZ, RFDZ, nullptr, FDX, RCDX, RFDX, nullptr, FDY, RCDY, RFDY)) return r;
+  if (int r = set_metric3d(c, PHI, GSQRT, J13G, J23G, *J33G, MAPF)) return r;
+  auto& B = h->dbuf;
+  const double* s0[5] = {DENS0, MOMZ0, MOMX0, MOMY0, RHOT0};
+  const double* s[5] = {DENS, MOMZ, MOMX, MOMY, RHOT};
+  const double* t[5] = {DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t};
+  double* o[5] = {DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK};
+  for (int v = 0; v < 5; ++v) {
+    if (int r = upload3(c, B[B_S0 + v], s0[v], 1)) return r;
+    if (int r = upload3(c, B[B_S + v], s[v], 1)) return r;
+    if (int r = upload3(c, B[B_T + v], t[v], 1)) return r;
+    if (int r = upload3(c, B[B_OUT + v], o[v], 1)) return r;  // elements HEVI leaves alone keep their values
+  }
+  if (int r = upload3(c, B[B_DPRES0], DPRES0, 1)) return r;
+  if (int r = upload3(c, B[B_RT2P], RT2P, 1)) return r;
+  if (int r = upload3(c, B[B_REFD], REF_dens, 1)) return r;
+  if (int r = upload3(c, B[B_REFR], REF_rhot, 1)) return r;
+  if (int r = upload3(c, B[B_DDIV], DDIV, 1)) return r;
+  if (int r = upload3(c, B[B_ND], num_diff, 15)) return r;
+  if (int r = upload3(c, B[B_MF], mflx_hi, 3)) return r;
+  if (int r = upload3(c, B[B_TF], tflx_hi, 3)) return r;
+  if (int r = upload_raw(c, B[B_COR], CORIOLI, c.d.pl)) return r;
+  if (int r = upload_raw(c, B[B_WD], wdamp_coef, (size_t)c.d.KA)) return r;
+  const bool has_cor = !all_equal(CORIOLI, c.d.pl, 0.0);
+  shp::StageIO io;
+  io.ctx = &c;
+  for (int v = 0; v < 5; ++v) { io.o[v] = B[B_OUT + v]; io.s0[v] = B[B_S0 + v]; io.s[v] = B[B_S + v]; io.t[v] = B[B_T + v]; }
+  io.mflx_hi = B[B_MF]; io.tflx_hi = B[B_TF];
+  io.DPRES0 = B[B_DPRES0]; io.RT2P = B[B_RT2P]; io.CORIOLI = B[B_COR]; io.num_diff = B[B_ND]; io.wdamp_coef = B[B_WD];
+  io.DDIV = B[B_DDIV]; io.REF_dens = B[B_REFD]; io.REF_rhot = B[B_REFR];
+  io.divdmp_coef = *divdmp_coef; io.dtrk = *dtrk; io.has_coriolis = has_cor;
+  CK(shp::hevi_stage(io, c.stream));
+  for (int v = 0; v < 5; ++v)
+    if (int r = download3(c, o[v], B[B_OUT + v], 1)) return r;
+  if (int r = download3(c, mflx_hi, B[B_MF], 3)) return r;
+  if (int r = download3(c, tflx_hi, B[B_TF], 3)) return r;
+  return 0;
+}
+
+int scale_hip_tstep_tracer_heve(scale_hip_ctx* h,
+    double* QTRCo, const double* QTRC, const double* QTRC0, const double* RHOQ_t,
+    const double* DENS0, const double* DENS, const double* mflx_hi, const double* num_diff,
+    const double* GSQRT, const double* MAPF, const double* CDZ,
+    const double* RCDZ, const double* RCDX, const double* RCDY,
+    const double* dtl, const int32_t* FLAG_FCT_TRACER, const int32_t* FLAG_FCT_ALONG_STREAM) {
+  if (int r = check_ctx(h)) return r;
+  if (!dtl || !FLAG_FCT_TRACER || !FLAG_FCT_ALONG_STREAM) return fail("null scalar argument");
+  Ctx& c = h->c;
+  const Dims& d = c.d;
+  if (int r = ensure_dropin(h)) return r;
+  if (int r = set_grid1d(c, CDZ, nullptr, RCDZ, nullptr, nullptr, nullptr, RCDX, nullptr, nullptr, nullptr, RCDY, nullptr)) return r;
+  // metrics: GSQRT(KA,IA,JA,7) and the (IA,JA,2) map factor
+  const bool flat = all_equal(GSQRT, 7 * d.n3, 1.0) && all_equal(MAPF, 2 * d.pl, 1.0);
+  if (!flat) {
+    std::vector<double> m4(8 * d.pl, 1.0);
+    memcpy(m4.data(), MAPF, 2 * d.pl * sizeof(double));
+    std::vector<double> zero(7 * d.n3, 0.0);
+    if (int r = set_metric3d(c, nullptr, GSQRT, zero.data(), zero.data(), 1.0, m4.data())) return r;
+  } else {
+    c.terrain = false;
+  }
+  auto& B = h->dbuf;
+  // reuse drop-in buffers: QTRCo, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx(3), num_diff(3)
+  double* dQo = B[B_OUT]; double* dQ = B[B_S]; double* dQ0 = B[B_S + 1]; double* dRQ = B[B_S + 2];
+  double* dD0 = B[B_S + 3]; double* dD = B[B_S + 4]; double* dMF = B[B_MF]; double* dND = B[B_ND];
+  if (int r = upload3(c, dQo, QTRCo, 1)) return r;
+  if (int r = upload3(c, dQ, QTRC, 1)) return r;
+  if (int r = upload3(c, dQ0, QTRC0, 1)) return r;
+  if (int r = upload3(c, dRQ, RHOQ_t, 1)) return r;
+  if (int r = upload3(c, dD0, DENS0, 1)) return r;
+  if (int r = upload3(c, dD, DENS, 1)) return r;
+  if (int r = upload3(c, dMF, mflx_hi, 3)) return r;
+  if (int r = upload3(c, dND, num_diff, 3)) return r;
+  shp::TracerIO io;
+  io.QTRCo = dQo; io.QTRC = dQ; io.QTRC0 = dQ0; io.RHOQ_t = dRQ; io.DENS0 = dD0; io.DENS = dD;
+  io.mflx_hi = dMF; io.num_diff = dND; io.dtl = *dtl; io.flag_fct = *FLAG_FCT_TRACER; io.flag_along = *FLAG_FCT_ALONG_STREAM;
+  CK(shp::tracer_stage(c, io, c.stream));
+  return download3(c, QTRCo, dQo, 1);
+}
+
+int scale_hip_numfilter_coef(scale_hip_ctx* h, double* num_diff,
+    const double* DENS, const double* MOMZ, const double* MOMX, const double* MOMY, const double* RHOT,
+    const double* CDZ, const double* CDX, const double* CDY, const double* FDZ, const double* FDX, const double* FDY,
+    const double* DT, const double* REF_dens, const double* REF_pott,
+    const double* ND_COEF, const int32_t* ND_ORDER, const double* ND_SFC_FACT, const int32_t* ND_USE_RS) {
+  if (int r = check_ctx(h)) return r;
+  if (!DT || !ND_COEF || !ND_ORDER || !ND_SFC_FACT || !ND_USE_RS) return fail("null scalar argument");
+  if (*ND_ORDER != 1) return fail("ND_ORDER other than 1 is not supported");
+  if (!*ND_USE_RS) return fail("ND_USE_RS = .false. is not supported");
+  Ctx& c = h->c;
+  if (int r = ensure_dropin(h)) return r;
+  if (int r = set_grid1d(c, CDZ, FDZ, nullptr, nullptr, CDX, FDX, nullptr, nullptr, CDY, FDY, nullptr, nullptr)) return r;
+  if (int r = set_numfilter_tables(c, *DT, *ND_COEF, *ND_SFC_FACT)) return r;
+  auto& B = h->dbuf;
+  const double* s[5] = {DENS, MOMZ, MOMX, MOMY, RHOT};
+  for (int v = 0; v < 5; ++v)
+    if (int r = upload3(c, B[B_S + v], s[v], 1)) return r;
+  if (int r = upload3(c, B[B_REFD], REF_dens, 1)) return r;
+  if (int r = upload3(c, B[B_REFR], REF_pott, 1)) return r;
+  const double* cs[5] = {B[B_S], B[B_S + 1], B[B_S + 2], B[B_S + 3], B[B_S + 4]};
+  CK(shp::numfilter(c, B[B_ND], cs, B[B_REFD], B[B_REFR], c.stream));
+  return download3(c, num_diff, B[B_ND], 15);
+}
+
+// ---- device-resident model ---------------------------------------------------------
+int scale_hip_set_metrics(scale_hip_ctx* h,
+    const double* CDZ, const double* FDZ, const double* RCDZ, const double* RFDZ,
+    const double* CDX, const double* FDX, const double* RCDX, const double* RFDX,
+    const double* CDY, const double* FDY, const double* RCDY, const double* RFDY,
+    const double* PHI, const double* GSQRT, const double* J13G, const double* J23G,
+    const double* J33G, const double* MAPF) {
+  if (int r = check_ctx(h)) return r;
+  if (!J33G) return fail("null J33G");
+  if (!CDZ || !FDZ || !RCDZ || !RFDZ || !CDX || !FDX || !RCDX || !RFDX || !CDY || !FDY || !RCDY || !RFDY)
+    return fail("every 1-D grid array is required");
+  if (int r = set_grid1d(h->c, CDZ, FDZ, RCDZ, RFDZ, CDX, FDX, RCDX, RFDX, CDY, FDY, RCDY, RFDY)) return r;
+  return set_metric3d(h->c, PHI, GSQRT, J13G, J23G, *J33G, MAPF);
+}
+
+int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, int32_t QA) {
+  if (int r = check_ctx(h)) return r;
+  if (!cfg) return fail("null config");
+  Ctx& c = h->c;
+  if (!c.metrics_set) return fail("scale_hip_set_metrics must be called first");
+  if (c.model) return fail("model already created");
+  if (cfg->nd_order != 1) return fail("nd_order other than 1 is not supported");
+  if (cfg->divdmp_coef != 0.0) return fail("divdmp_coef > 0 is not supported by the resident model");
+  if (!(cfg->dtl > 0.0) || !(cfg->dts > 0.0)) return fail("dtl and dts must be positive");
+  if (QA < 0) return fail("QA must be >= 0");
+  shp::Model* M = new shp::Model();
+  c.model = M;
+  const size_t n3 = c.d.n3;
+  auto A = [&](size_t n) { return dalloc(c, n, &M->allocs); };
+  M->QA = QA;
+  M->DENS = A(n3); M->MOMZ = A(n3); M->MOMX = A(n3); M->MOMY = A(n3); M->RHOT = A(n3);
+  M->QTRC = QA > 0 ? A(n3 * (size_t)QA) : nullptr;
+  M->REF_dens = A(n3); M->REF_pott = A(n3); M->REF_pres = A(n3);
+  M->CORIOLI = A(c.d.pl); M->wdamp_coef = A((size_t)c.d.KA);
+  M->mflx_hi = A(3 * n3); M->tflx_hi = A(3 * n3);
+  M->RT2P = A(n3); M->DPRES0 = A(n3); M->REF_rhot = A(n3); M->DENS00 = A(n3); M->mflx_av = A(3 * n3);
+  M->num_diff = A(15 * n3); M->DDIV = A(n3);
+  for (int v = 0; v < 5; ++v) { M->t[v] = A(n3); M->S0[v] = A(n3); }
+  for (int r = 0; r < 4; ++r) {
+    for (int v = 0; v < 5; ++v) M->R[r][v] = A(n3);
+    M->mfR[r] = A(3 * n3); M->tfR[r] = A(3 * n3);
+  }
+  M->ndq = A(3 * n3); M->DENS_RK = A(n3); M->QRK1 = A(n3); M->QRK2 = A(n3); M->zeros = A(n3);
+  for (void* p : M->allocs)
+    if (!p) return fail("hipMalloc failed in scale_hip_model_create (out of device memory?)");
+  CK(hipMemsetAsync(M->mflx_hi, 0, 3 * n3 * sizeof(double), c.stream));
+  CK(hipMemsetAsync(M->tflx_hi, 0, 3 * n3 * sizeof(double), c.stream));
+  CK(hipMemsetAsync(M->ndq, 0, 3 * n3 * sizeof(double), c.stream));
+  CK(hipMemsetAsync(M->zeros, 0, n3 * sizeof(double), c.stream));
+  CK(hipMemsetAsync(M->DDIV, 0, n3 * sizeof(double), c.stream));
+  CK(hipMemsetAsync(M->CORIOLI, 0, c.d.pl * sizeof(double), c.stream));
+  CK(hipMemsetAsync(M->wdamp_coef, 0, (size_t)c.d.KA * sizeof(double), c.stream));
+  M->dtl = cfg->dtl; M->dts = cfg->dts; M->nd_coef = cfg->nd_coef; M->nd_sfc_fact = cfg->nd_sfc_fact;
+  M->divdmp_coef = cfg->divdmp_coef; M->fct = cfg->fct_tracer ? 1 : 0; M->fct_along = cfg->fct_along_stream ? 1 : 0;
+  const int nstep = (int)ceil((cfg->dtl - shp::EPS_RP) / cfg->dts);
+  if (int r = set_numfilter_tables(c, cfg->dtl / nstep, cfg->nd_coef, cfg->nd_sfc_fact)) return r;
+  CK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+static double* model_field(shp::Model* M, int32_t f, int* ncomp) {
+  *ncomp = 1;
+  switch (f) {
+    case SCALE_HIP_DENS: return M->DENS;
+    case SCALE_HIP_MOMZ: return M->MOMZ;
+    case SCALE_HIP_MOMX: return M->MOMX;
+    case SCALE_HIP_MOMY: return M->MOMY;
+    case SCALE_HIP_RHOT: return M->RHOT;
+    case SCALE_HIP_REF_DENS: return M->REF_dens;
+    case SCALE_HIP_REF_POTT: return M->REF_pott;
+    case SCALE_HIP_REF_PRES: return M->REF_pres;
+    case SCALE_HIP_MFLX_HI: *ncomp = 3; return M->mflx_hi;
+    case SCALE_HIP_TFLX_HI: *ncomp = 3; return M->tflx_hi;
+    default: break;
+  }
+  return nullptr;
+}
+
+int scale_hip_model_set(scale_hip_ctx* h, int32_t f, const double* host) {
+  if (int r = check_ctx(h)) return r;
+  Ctx& c = h->c;
+  shp::Model* M = c.model;
+  if (!M) return fail("model not created");
+  if (!host) return fail("null array");
+  const size_t n3 = c.d.n3;
+  if (f == SCALE_HIP_CORIOLI) {
+    if (int r = upload_raw(c, M->CORIOLI, host, c.d.pl)) return r;
+    M->has_coriolis = !all_equal(host, c.d.pl, 0.0);
+    CK(hipStreamSynchronize(c.stream));
+    return 0;
+  }
+  if (f == SCALE_HIP_WDAMP_COEF) {
+    if (int r = upload_raw(c, M->wdamp_coef, host, (size_t)c.d.KA)) return r;
+    CK(hipStreamSynchronize(c.stream));
+    return 0;
+  }
+  if (f >= SCALE_HIP_DENS_TP && f <= SCALE_HIP_RHOT_TP) {
+    const int v = f - SCALE_HIP_DENS_TP;
+    if (!M->tp[v]) { M->tp[v] = dalloc(c, n3, &M->allocs); if (!M->tp[v]) return fail("hipMalloc failed"); }
+    return upload3(c, M->tp[v], host, 1);
+  }
+  if (f >= SCALE_HIP_RHOQ_TP && f < SCALE_HIP_RHOQ_TP + M->QA) {
+    if (!M->RHOQ_tp) {
+      M->RHOQ_tp = dalloc(c, n3 * (size_t)M->QA, &M->allocs);
+      if (!M->RHOQ_tp) return fail("hipMalloc failed");
+      CK(hipMemsetAsync(M->RHOQ_tp, 0, n3 * (size_t)M->QA * sizeof(double), c.stream));
+    }
+    return upload3(c, M->RHOQ_tp + n3 * (size_t)(f - SCALE_HIP_RHOQ_TP), host, 1);
+  }
+  if (f >= SCALE_HIP_QTRC && f < SCALE_HIP_QTRC + M->QA) return upload3(c, M->QTRC + n3 * (size_t)(f - SCALE_HIP_QTRC), host, 1);
+  if (f == SCALE_HIP_MFLX_HI || f == SCALE_HIP_TFLX_HI) return fail("mflx_hi/tflx_hi are outputs");
+  int nc;
+  double* dst = model_field(M, f, &nc);
+  if (!dst) return fail("unknown field id");
+  int r = upload3(c, dst, host, nc);
+  if (r == 0) CK(hipStreamSynchronize(c.stream));
+  return r;
+}
+
+int scale_hip_model_get(scale_hip_ctx* h, int32_t f, double* host) {
+  if (int r = check_ctx(h)) return r;
+  Ctx& c = h->c;
+  shp::Model* M = c.model;
+  if (!M) return fail("model not created");
+  if (!host) return fail("null array");
+  if (f >= SCALE_HIP_QTRC && f < SCALE_HIP_QTRC + M->QA) return download3(c, host, M->QTRC + c.d.n3 * (size_t)(f - SCALE_HIP_QTRC), 1);
+  int nc;
+  double* src = model_field(M, f, &nc);
+  if (!src) return fail("unknown field id");
+  return download3(c, host, src, nc);
+}
+
+int scale_hip_model_device_ptr(scale_hip_ctx* h, int32_t f, double** dptr) {
+  if (int r = check_ctx(h)) return r;
+  shp::Model* M = h->c.model;
+  if (!M || !dptr) return fail("model not created");
+  if (f >= SCALE_HIP_QTRC && f < SCALE_HIP_QTRC + M->QA) { *dptr = M->QTRC + h->c.d.n3 * (size_t)(f - SCALE_HIP_QTRC); return 0; }
+  int nc;
+  *dptr = model_field(M, f, &nc);
+  return *dptr ? 0 : fail("unknown field id");
+}
+
+int scale_hip_model_large_step(scale_hip_ctx* h, int32_t n) {
+  if (int r = check_ctx(h)) return r;
+  if (!h->c.model) return fail("model not created");
+  for (int s = 0; s < n; ++s) CK(shp::model_large_step(h->c));
+  return 0;
+}
+
+int scale_hip_model_prepare(scale_hip_ctx* h) {
+  if (int r = check_ctx(h)) return r;
+  if (!h->c.model) return fail("model not created");
+  CK(shp::model_prepare(h->c));
+  return 0;
+}
+
+int scale_hip_model_short_step(scale_hip_ctx* h, int32_t n) {
+  if (int r = check_ctx(h)) return r;
+  if (!h->c.model) return fail("model not created");
+  shp::Model& M = *h->c.model;
+  const int nstep = (int)ceil((M.dtl - shp::EPS_RP) / M.dts);
+  const double dts = M.dtl / nstep;
+  for (int s = 0; s < n; ++s) CK(shp::model_short_step_rk4(h->c, dts));
+  return 0;
+}
+
+int scale_hip_model_mass(scale_hip_ctx* h, int32_t f, int32_t weight_dens, double* out) {
+  if (int r = check_ctx(h)) return r;
+  shp::Model* M = h->c.model;
+  if (!M || !out) return fail("model not created");
+  const double* src;
+  if (f >= SCALE_HIP_QTRC && f < SCALE_HIP_QTRC + M->QA) src = M->QTRC + h->c.d.n3 * (size_t)(f - SCALE_HIP_QTRC);
+  else { int nc; src = model_field(M, f, &nc); }
+  if (!src) return fail("unknown field id");
+  CK(shp::mass_sum(h->c, src, weight_dens ? M->DENS : nullptr, out));
+  return 0;
+}
+
+int scale_hip_profile_enable(scale_hip_ctx* h, int32_t on) {
+  if (int r = check_ctx(h)) return r;
+  Ctx& c = h->c;
+  CK(hipStreamSynchronize(c.stream));
+  if (on) {
+    if (!c.prof) c.prof = shp::prof_create();
+    shp::prof_collect(c);
+    shp::prof_reset(c.prof);
+  } else if (c.prof) {
+    shp::prof_collect(c);
+    shp::prof_destroy(c.prof);
+    c.prof = nullptr;
+  }
+  return 0;
+}
+
+int scale_hip_profile_query(scale_hip_ctx* h, const char* name, int64_t* count, double* total_ms) {
+  if (int r = check_ctx(h)) return r;
+  Ctx& c = h->c;
+  if (!c.prof) return fail("profiling is not enabled");
+  if (!name || !count || !total_ms) return fail("null argument");
+  CK(hipStreamSynchronize(c.stream));
+  shp::prof_collect(c);
+  long n = 0;
+  shp::prof_query(c.prof, name, &n, total_ms);
+  *count = n;
+  return 0;
+}
+
+}  // extern "C"

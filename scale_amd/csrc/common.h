@@ -1,0 +1,135 @@
+// common.h -- device layout, constants and the UD3/CD4/UD1 face-value
+// stencils shared by every HIP kernel of the HEVI short step and the FCT
+// tracer step (gfx950).
+//
+// Device layout: a 3-D field (KA,IA,JA) of the reference is stored i-fastest,
+// element (k,i,j) at i + IA*(j + JA*k) (0-based), so a wavefront of 64 lanes
+// over consecutive i reads 512 contiguous bytes per load.  Multi-component
+// fields append the component as the slowest index (GSQRT component c at
+// offset c*KA*IA*JA; num_diff(v,d) at offset (v + 5*d)*KA*IA*JA), MAPF(i,j,n,c)
+// at i + IA*(j + JA*(n + 2*c)).
+//
+// Arithmetic mirrors the reference expression by expression; the library is
+// compiled with -ffp-contract=off so no multiply-add is fused.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace shp {
+
+// staggering ids (scale_atmos_grid_cartesC_index.F90:90-101), 0-based
+enum { I_XYZ = 0, I_XYW, I_UYW, I_XVW, I_UYZ, I_XVZ, I_UVZ };
+enum { I_XY = 0, I_UY, I_XV, I_UV };
+enum { ZDIR = 0, XDIR = 1, YDIR = 2 };
+enum { I_DENS = 0, I_MOMZ, I_MOMX, I_MOMY, I_RHOT };
+
+// scale_const.F90:46,55-56,88,162
+constexpr double GRAV = 9.80665;
+constexpr double RDRY = 287.04;
+constexpr double CPDRY = 1004.64;
+constexpr double P00 = 100000.0;
+constexpr double EPS_RP = 2.220446049250313080847263336181640625e-16;
+
+// flux coefficients (scale_atmos_dyn_fvm_flux_ud3.F90:80-89, _cd4.F90:84-88, _ud1.F90:84-86)
+constexpr double F1 = 0.5;
+constexpr double F2 = 0.5;
+constexpr double F31 = -1.0 / 12.0;
+constexpr double F32 = 7.0 / 12.0;
+constexpr double F33 = 3.0 / 12.0;
+constexpr double F41 = 7.0 / 12.0;
+constexpr double F42 = -1.0 / 12.0;
+
+constexpr int UD1 = 1, UD3 = 3, CD4 = 4;
+
+struct Dims {
+  int KA, IA, JA;
+  int KS, KE, IS, IE, JS, JE;  // 0-based, inclusive
+  size_t n3;                   // KA*IA*JA
+  size_t pl;                   // IA*JA
+};
+
+__host__ __device__ inline size_t IX(const Dims& d, int k, int i, int j) {
+  return (size_t)i + (size_t)d.IA * (size_t)j + d.pl * (size_t)k;
+}
+
+// Fortran sign(a,b)
+__device__ __forceinline__ double fsign(double a, double b) { return __builtin_copysign(__builtin_fabs(a), b); }
+__device__ __forceinline__ double fmax2(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double fmin2(double a, double b) { return a < b ? a : b; }
+
+// interior face between q0 = val(idx) and q1 = val(idx+1) (ERB flux(), order S)
+template <int S>
+__device__ __forceinline__ double fv_int(double qm1, double q0, double q1, double q2, double vel) {
+  if constexpr (S == UD3)
+    return (F31 * (q2 + qm1) + F32 * (q1 + q0)) - (F31 * (q2 - qm1) + F33 * (q1 - q0)) * fsign(1.0, vel);
+  else if constexpr (S == CD4)
+    return F41 * (q1 + q0) + F42 * (q2 + qm1);
+  else
+    return F1 * (q1 + q0) - fsign(F1, vel) * (q1 - q0);
+}
+// lowest face (val(KS), val(KS+1), val(KS+2)) -- scale_atmos_dyn_fvm_flux_ud3.F90:225-229
+template <int S>
+__device__ __forceinline__ double fv_lo(double q0, double q1, double q2, double vel) {
+  if constexpr (S == UD3)
+    return F2 * (q1 + q0) * (0.5 + fsign(0.5, vel)) + (2.0 * q0 + 5.0 * q1 - q2) / 6.0 * (0.5 - fsign(0.5, vel));
+  else
+    return F2 * (q1 + q0);
+}
+// highest face (val(KE-2), val(KE-1), val(KE)) -- scale_atmos_dyn_fvm_flux_ud3.F90:231-235
+template <int S>
+__device__ __forceinline__ double fv_hi(double qm1, double q0, double q1, double vel) {
+  if constexpr (S == UD3)
+    return (2.0 * q1 + 5.0 * q0 - qm1) / 6.0 * (0.5 + fsign(0.5, vel)) + F2 * (q1 + q0) * (0.5 - fsign(0.5, vel));
+  else
+    return F2 * (q1 + q0);
+}
+
+// Metric accessors.  TERRAIN=false is the flat Cartesian grid (GSQRT=1,
+// J13G=J23G=0, MAPF=1): loads are skipped and the constants fold, which
+// reproduces the general expressions exactly (up to the sign of exact zeros).
+template <bool TERRAIN>
+struct Met {
+  const double* G;
+  const double* J13;
+  const double* J23;
+  const double* MAPF;
+  size_t n3;
+  size_t pl;
+  int IA;
+  __device__ __forceinline__ double gs(int c, size_t p) const {
+    if constexpr (TERRAIN) return G[p + n3 * (size_t)c]; else return 1.0;
+  }
+  __device__ __forceinline__ double j13(int c, size_t p) const {
+    if constexpr (TERRAIN) return J13[p + n3 * (size_t)c]; else return 0.0;
+  }
+  __device__ __forceinline__ double j23(int c, size_t p) const {
+    if constexpr (TERRAIN) return J23[p + n3 * (size_t)c]; else return 0.0;
+  }
+  // MAPF(i,j,n,c) with n = 0,1 (Fortran 1,2)
+  __device__ __forceinline__ double mapf(int i, int j, int n, int c) const {
+    if constexpr (TERRAIN) return MAPF[(size_t)i + (size_t)IA * (size_t)j + pl * (size_t)(n + 2 * c)];
+    else return 1.0;
+  }
+};
+
+// one-dimensional grid arrays, all indexed with the 0-based level/column
+struct Grid1D {
+  const double *CDZ, *FDZ, *RCDZ, *RFDZ;
+  const double *CDX, *FDX, *RCDX, *RFDX;
+  const double *CDY, *FDY, *RCDY, *RFDY;
+};
+
+// XCD-aware block index: hardware hands consecutive workgroup ids to the 8
+// XCDs round-robin; remap so that each XCD walks one contiguous range of the
+// logical tile order (neighbouring tiles then share that XCD's L2).
+__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {
+  constexpr unsigned NXCD = 8;
+  const unsigned per = (nblocks + NXCD - 1) / NXCD;
+  const unsigned xcd = bid % NXCD, loc = bid / NXCD;
+  const unsigned full = nblocks % NXCD == 0 ? NXCD : nblocks % NXCD;  // XCDs that own 'per' blocks
+  unsigned base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+  return base + loc;
+}
+
+}  // namespace shp

@@ -1,0 +1,403 @@
+// driver.hip -- the kernels around the HEVI stage and the tracer step:
+// periodic halo fill (one-rank COMM_vars8), layout transposes, the RK4 short
+// step, the large-step preparation and ghost-level fills, the tracer RK3
+// sequence and FP64 budgets.  The device-resident model (struct Model) keeps
+// every field in HBM across steps.
+//
+// References (scalelib/src/...):
+//   communication/scale_comm_cartesC.F90:345-397,1600-2043  (halo semantics)
+//   atmos-rm/dynamics/scale_atmos_dyn_tinteg_short_rk4.F90:260-660
+//   atmos-rm/dynamics/scale_atmos_dyn_tinteg_tracer_rk3.F90:92-208
+//   atmos-rm/dynamics/scale_atmos_dyn_tstep_large_fvm_heve.F90:377-1127
+#include <math.h>
+#include <string.h>
+#include "common.h"
+#include "shp_internal.h"
+#include "model.h"
+
+namespace shp {
+
+// ---- periodic halo fill ----------------------------------------------------
+struct HaloArgs {
+  Dims d;
+  double* a[16];
+  int n;
+  int nW;  // halo points per level: west/east strips then south/north rows
+};
+
+__global__ void halo_kernel(HaloArgs h) {
+  const Dims& d = h.d;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  const int f = blockIdx.z;
+  if (t >= h.nW || k >= d.KA || f >= h.n) return;
+  const int IMAX = d.IE - d.IS + 1, JMAX = d.JE - d.JS + 1;
+  const int hw = d.IS;  // halo width in x
+  const int hs = d.JS;  // halo width in y
+  int i, j;
+  const int nstrip = 2 * hw * JMAX;  // west + east columns of interior rows
+  if (t < nstrip) {
+    const int r = t / (2 * hw), c = t % (2 * hw);
+    j = d.JS + r;
+    i = c < hw ? c : d.IE + 1 + (c - hw);
+  } else {
+    const int u = t - nstrip;
+    const int r = u / d.IA, c = u % d.IA;
+    i = c;
+    j = r < hs ? r : d.JE + 1 + (r - hs);
+  }
+  const int si = i < d.IS ? i + IMAX : (i > d.IE ? i - IMAX : i);
+  const int sj = j < d.JS ? j + JMAX : (j > d.JE ? j - JMAX : j);
+  double* a = h.a[f];
+  a[IX(d, k, i, j)] = a[IX(d, k, si, sj)];
+}
+
+hipError_t halo_fill(const Ctx& c, double* const* arrs, int n, hipStream_t s) {
+  const Dims& d = c.d;
+  const int JMAX = d.JE - d.JS + 1;
+  const int nW = 2 * d.IS * JMAX + 2 * d.JS * d.IA;
+  for (int base = 0; base < n; base += 16) {
+    HaloArgs h;
+    h.d = d;
+    h.n = n - base < 16 ? n - base : 16;
+    for (int q = 0; q < h.n; ++q) h.a[q] = arrs[base + q];
+    h.nW = nW;
+    prof_start(c, "halo_fill", s);
+    hipLaunchKernelGGL(halo_kernel, dim3((nW + 255) / 256, d.KA, h.n), dim3(256), 0, s, h);
+    prof_stop(c, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// ---- layout transposes: Fortran (k,i,j) <-> device (i,j,k) -------------------
+// 32x32 tiles over (k, i) for each (j, component), staged through LDS so both
+// sides are read and written along their contiguous dimension.
+__global__ void f2d_kernel(Dims d, double* dst, const double* src) {
+  __shared__ double tile[32][33];
+  const int kb = blockIdx.x * 32, ib = blockIdx.y * 32;
+  const int jc = blockIdx.z;  // j + JA*comp
+  const size_t base = (size_t)jc * d.KA * d.IA;  // Fortran (k,i) plane of this (j,comp)
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int i = ib + r, k = kb + threadIdx.x;
+    if (i < d.IA && k < d.KA) tile[r][threadIdx.x] = src[base + (size_t)k + (size_t)d.KA * i];
+  }
+  __syncthreads();
+  const int j = jc % d.JA, comp = jc / d.JA;
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int k = kb + r, i = ib + threadIdx.x;
+    if (i < d.IA && k < d.KA) dst[(size_t)comp * d.n3 + IX(d, k, i, j)] = tile[threadIdx.x][r];
+  }
+}
+__global__ void d2f_kernel(Dims d, double* dst, const double* src) {
+  __shared__ double tile[32][33];
+  const int kb = blockIdx.x * 32, ib = blockIdx.y * 32;
+  const int jc = blockIdx.z;
+  const int j = jc % d.JA, comp = jc / d.JA;
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int k = kb + r, i = ib + threadIdx.x;
+    if (i < d.IA && k < d.KA) tile[threadIdx.x][r] = src[(size_t)comp * d.n3 + IX(d, k, i, j)];
+  }
+  __syncthreads();
+  const size_t base = (size_t)jc * d.KA * d.IA;
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int i = ib + r, k = kb + threadIdx.x;
+    if (i < d.IA && k < d.KA) dst[base + (size_t)k + (size_t)d.KA * i] = tile[r][threadIdx.x];
+  }
+}
+
+hipError_t to_device_layout(const Ctx& c, double* dst, const double* src_f, int ncomp, hipStream_t s) {
+  const Dims& d = c.d;
+  dim3 grid((d.KA + 31) / 32, (d.IA + 31) / 32, (unsigned)(d.JA * ncomp));
+  hipLaunchKernelGGL(f2d_kernel, grid, dim3(32, 8), 0, s, d, dst, src_f);
+  return hipGetLastError();
+}
+hipError_t to_fortran_layout(const Ctx& c, double* dst_f, const double* src, int ncomp, hipStream_t s) {
+  const Dims& d = c.d;
+  dim3 grid((d.KA + 31) / 32, (d.IA + 31) / 32, (unsigned)(d.JA * ncomp));
+  hipLaunchKernelGGL(d2f_kernel, grid, dim3(32, 8), 0, s, d, dst_f, src);
+  return hipGetLastError();
+}
+
+// ---- RK4 combination + ghost levels (RK4.F90:562-660, LARGE.F90:908-927) -------
+struct CombArgs {
+  Dims d;
+  double* st[5];
+  const double *r1[5], *r2[5], *r3[5], *r4[5], *s0[5];
+  double *mflx, *tflx;
+  const double *m1, *m2, *m3, *m4, *t1, *t2, *t3, *t4;
+};
+
+__device__ __forceinline__ double comb(const CombArgs& a, int v, size_t p) {
+  return (a.r1[v][p] * 2.0 + a.r2[v][p] * 4.0 + a.r3[v][p] * 2.0 + a.r4[v][p] - a.s0[v][p] * 3.0) / 6.0;
+}
+
+__global__ void __launch_bounds__(256) rk4_combine(CombArgs a) {
+  const Dims& d = a.d;
+  const int i = d.IS + blockIdx.x * 64 + threadIdx.x;
+  const int k = blockIdx.y * 4 + threadIdx.y;
+  const int j = d.JS + blockIdx.z;
+  if (i > d.IE || k >= d.KA || j > d.JE) return;
+  const size_t p = IX(d, k, i, j);
+  const int KS = d.KS, KE = d.KE;
+  // DENS, MOMX, MOMY, RHOT: combine on KS..KE; ghosts copy the combined KS / KE value
+  const int vs[4] = {0, 2, 3, 4};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int v = vs[q];
+    const int kk = k < KS ? KS : (k > KE ? KE : k);
+    a.st[v][p] = comb(a, v, IX(d, kk, i, j));
+  }
+  // MOMZ: combine on KS..KE-1, MOMZ(KE) keeps the last stage's value (0), ghosts 0
+  if (k >= KS && k <= KE - 1) a.st[1][p] = comb(a, 1, p);
+  else if (k == KE) a.st[1][p] = a.r4[1][p];
+  else a.st[1][p] = 0.0;
+  if (k >= KS && k <= KE) {
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const size_t q = p + d.n3 * (size_t)n;
+      a.mflx[q] = (a.m1[q] + a.m2[q] * 2.0 + a.m3[q] * 2.0 + a.m4[q]) / 6.0;
+      a.tflx[q] = (a.t1[q] + a.t2[q] * 2.0 + a.t3[q] * 2.0 + a.t4[q]) / 6.0;
+    }
+  }
+}
+
+// ---- large-step preparation (LARGE.F90:443-471), dry constants ------------------
+__global__ void large_prep(Dims d, const double* RHOT, const double* REF_pres, double* RT2P, double* DPRES0, double* REF_rhot) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int k = blockIdx.y * 4 + threadIdx.y;
+  const int j = blockIdx.z;
+  if (i >= d.IA || k >= d.KA || j >= d.JA) return;
+  const int KS = d.KS, KE = d.KE;
+  const double CVdry = CPDRY - RDRY;
+  const double CPovCV = CPDRY / CVdry;
+  const size_t p = IX(d, k, i, j);
+  if (k >= KS && k <= KE) {
+    const double PRES = P00 * pow(RDRY * RHOT[p] / P00, CPovCV);
+    RT2P[p] = CPovCV * PRES / RHOT[p];
+    DPRES0[p] = PRES - REF_pres[p];
+    REF_rhot[p] = RHOT[p];
+  } else if (k == KS - 1 || k == KE + 1) {
+    // ghost levels from the adjacent-but-one level; recompute that PRES here
+    const int kn = k == KS - 1 ? KS + 1 : KE - 1;
+    const size_t q = IX(d, kn, i, j);
+    const double PRES = P00 * pow(RDRY * RHOT[q] / P00, CPovCV);
+    const double dp = PRES - REF_pres[q];
+    DPRES0[p] = k == KS - 1 ? dp + (REF_pres[q] - REF_pres[p]) : dp + (REF_pres[q] - REF_pres[p]);
+    RT2P[p] = 0.0;
+    REF_rhot[p] = 0.0;
+  } else {
+    RT2P[p] = 0.0; DPRES0[p] = 0.0; REF_rhot[p] = 0.0;
+  }
+}
+
+// tendencies with DAMP_alpha = 0: X_t = X_tp on the active levels, 0 elsewhere (LARGE.F90:614-839)
+__global__ void tend_prep(Dims d, double* t, const double* tp, int kmax_off) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int k = blockIdx.y * 4 + threadIdx.y;
+  const int j = blockIdx.z;
+  if (i >= d.IA || k >= d.KA || j >= d.JA) return;
+  const size_t p = IX(d, k, i, j);
+  const bool act = k >= d.KS && k <= d.KE - kmax_off;
+  t[p] = act && tp ? tp[p] : 0.0;
+}
+
+__global__ void axpy_interior(Dims d, double* y, const double* x, double scale_div, int mode) {
+  // mode 0: y += x (interior, KS..KE); mode 1: y = x / scale_div (interior, KS..KE)
+  const int i = d.IS + blockIdx.x * 64 + threadIdx.x;
+  const int k = d.KS + blockIdx.y * 4 + threadIdx.y;
+  const int j = d.JS + blockIdx.z;
+  if (i > d.IE || k > d.KE || j > d.JE) return;
+  const size_t p = IX(d, k, i, j);
+  if (mode == 0) y[p] = y[p] + x[p];
+  else y[p] = x[p] / scale_div;
+}
+
+// DENS_RK for the tracer RK3 stages (tinteg_tracer_rk3.F90:140-172)
+__global__ void dens_rk(Dims d, double* out, const double* D0, const double* D, int third) {
+  const int i = d.IS - 1 + blockIdx.x * 64 + threadIdx.x;
+  const int k = d.KS + blockIdx.y * 4 + threadIdx.y;
+  const int j = d.JS - 1 + blockIdx.z;
+  if (i > d.IE + 1 || k > d.KE || j > d.JE + 1) return;
+  const size_t p = IX(d, k, i, j);
+  out[p] = third ? D0[p] + (D[p] - D0[p]) / 3.0 : D0[p] + (D[p] - D0[p]) * 0.5;
+}
+
+// FP64 budget: per-block partial sums of a*CDZ*CDX*CDY*GSQRT/(MAPF1*MAPF2), fixed order
+__global__ void mass_partial(Dims d, const double* a, const double* b, const double* CDZ, const double* CDX, const double* CDY,
+                             const double* GSQRT, const double* MAPF, int terrain, double* part) {
+  __shared__ double sh[256];
+  const int j = d.JS + blockIdx.x;
+  double s = 0.0;
+  const int imax = d.IE - d.IS + 1, kmax = d.KE - d.KS + 1;
+  for (int t = threadIdx.x; t < imax * kmax; t += blockDim.x) {
+    const int i = d.IS + t % imax, k = d.KS + t / imax;
+    const size_t p = IX(d, k, i, j);
+    double v = a[p];
+    if (b) v = v * b[p];
+    double w = CDZ[k] * CDX[i] * CDY[j];
+    if (terrain) w = w * GSQRT[p] / (MAPF[(size_t)i + (size_t)d.IA * j] * MAPF[(size_t)i + (size_t)d.IA * j + d.pl]);
+    s += v * w;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void mass_final(const double* part, int n, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int q = 0; q < n; ++q) s += part[q];
+    out[0] = s;
+  }
+}
+
+hipError_t mass_sum(const Ctx& c, const double* a, const double* b, double* out_host) {
+  const Dims& d = c.d;
+  const int jmax = d.JE - d.JS + 1;
+  double* part = c.scr[NSCR - 1];
+  hipLaunchKernelGGL(mass_partial, dim3(jmax), dim3(256), 0, c.stream, d, a, b, c.g1.CDZ, c.g1.CDX, c.g1.CDY, c.GSQRT, c.MAPF,
+                     (int)c.terrain, part);
+  hipLaunchKernelGGL(mass_final, dim3(1), dim3(64), 0, c.stream, part, jmax, part + jmax);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(out_host, part + jmax, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(c.stream);
+}
+
+// ---- launch helpers ------------------------------------------------------------
+static inline dim3 grid_full(const Dims& d) { return dim3((d.IA + 63) / 64, (d.KA + 3) / 4, d.JA); }
+static inline dim3 grid_int(const Dims& d) {
+  return dim3((d.IE - d.IS + 1 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 1);
+}
+static inline dim3 grid_intk(const Dims& d) {  // interior columns, all levels
+  return dim3((d.IE - d.IS + 1 + 63) / 64, (d.KA + 3) / 4, d.JE - d.JS + 1);
+}
+
+#define HIPCHK(x)                          \
+  do {                                     \
+    hipError_t e_ = (x);                   \
+    if (e_ != hipSuccess) return e_;       \
+  } while (0)
+
+// ---- device-resident model ------------------------------------------------------
+hipError_t model_short_step_rk4(Ctx& c, double dts) {
+  Model& M = *c.model;
+  const Dims& d = c.d;
+  hipStream_t s = c.stream;
+  const size_t bytes = d.n3 * sizeof(double);
+  double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
+  // numerical filter on the state at the start of the short step (LARGE.F90:845-858)
+  if (M.nd_coef == 0.0) {
+    HIPCHK(hipMemsetAsync(M.num_diff, 0, 15 * bytes, s));
+  } else {
+    const double* cst[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
+    HIPCHK(numfilter(c, M.num_diff, cst, M.REF_dens, M.REF_pott, s));
+  }
+  // DENS0 = DENS, ... (RK4.F90:326-336)
+  for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(M.S0[v], st[v], bytes, hipMemcpyDeviceToDevice, s));
+  const double dtrk[4] = {dts / 2.0, dts / 2.0, dts, dts};
+  for (int r = 0; r < 4; ++r) {
+    StageIO io;
+    io.ctx = &c;
+    for (int v = 0; v < 5; ++v) {
+      io.o[v] = M.R[r][v];
+      io.s0[v] = M.S0[v];
+      io.s[v] = r == 0 ? M.S0[v] : M.R[r - 1][v];
+      io.t[v] = M.t[v];
+    }
+    io.mflx_hi = M.mfR[r];
+    io.tflx_hi = M.tfR[r];
+    io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = M.num_diff;
+    io.wdamp_coef = M.wdamp_coef; io.DDIV = M.DDIV; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
+    io.divdmp_coef = M.divdmp_coef; io.dtrk = dtrk[r]; io.has_coriolis = M.has_coriolis;
+    HIPCHK(hevi_stage(io, s));
+    if (r < 3) HIPCHK(halo_fill(c, M.R[r], 5, s));
+  }
+  CombArgs a;
+  a.d = d;
+  for (int v = 0; v < 5; ++v) {
+    a.st[v] = st[v]; a.r1[v] = M.R[0][v]; a.r2[v] = M.R[1][v]; a.r3[v] = M.R[2][v]; a.r4[v] = M.R[3][v]; a.s0[v] = M.S0[v];
+  }
+  a.mflx = M.mflx_hi; a.tflx = M.tflx_hi;
+  a.m1 = M.mfR[0]; a.m2 = M.mfR[1]; a.m3 = M.mfR[2]; a.m4 = M.mfR[3];
+  a.t1 = M.tfR[0]; a.t2 = M.tfR[1]; a.t3 = M.tfR[2]; a.t4 = M.tfR[3];
+  prof_start(c, "rk4_combine", s);
+  hipLaunchKernelGGL(rk4_combine, grid_intk(d), dim3(64, 4), 0, s, a);
+  prof_stop(c, s);
+  HIPCHK(hipGetLastError());
+  return halo_fill(c, st, 5, s);
+}
+
+// RT2P, DPRES0, REF_rhot, DENS00, tendencies (LARGE.F90:443-839)
+hipError_t model_prepare(Ctx& c) {
+  Model& M = *c.model;
+  const Dims& d = c.d;
+  hipStream_t s = c.stream;
+  const size_t bytes = d.n3 * sizeof(double);
+  hipLaunchKernelGGL(large_prep, grid_full(d), dim3(64, 4), 0, s, d, M.RHOT, M.REF_pres, M.RT2P, M.DPRES0, M.REF_rhot);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(M.DENS00, M.DENS, bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemsetAsync(M.mflx_av, 0, 3 * bytes, s));
+  for (int v = 0; v < 5; ++v) {
+    hipLaunchKernelGGL(tend_prep, grid_full(d), dim3(64, 4), 0, s, d, M.t[v], (const double*)M.tp[v], v == 1 ? 1 : 0);
+    HIPCHK(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+hipError_t model_large_step(Ctx& c) {
+  Model& M = *c.model;
+  const Dims& d = c.d;
+  hipStream_t s = c.stream;
+  const size_t bytes = d.n3 * sizeof(double);
+  const double dtl = M.dtl;
+  const int nstep = (int)ceil((dtl - EPS_RP) / M.dts);
+  const double dts = dtl / nstep;
+  HIPCHK(model_prepare(c));
+  for (int step = 0; step < nstep; ++step) {
+    HIPCHK(model_short_step_rk4(c, dts));
+    for (int n = 0; n < 3; ++n) {
+      hipLaunchKernelGGL(axpy_interior, grid_int(d), dim3(64, 4), 0, s, d, M.mflx_av + d.n3 * n, M.mflx_hi + d.n3 * n, 1.0, 0);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  if (M.QA > 0) {
+    for (int n = 0; n < 3; ++n)
+      hipLaunchKernelGGL(axpy_interior, grid_int(d), dim3(64, 4), 0, s, d, M.mflx_hi + d.n3 * n, M.mflx_av + d.n3 * n, (double)nstep, 1);
+    HIPCHK(hipGetLastError());
+    double* mf[3] = {M.mflx_hi, M.mflx_hi + d.n3, M.mflx_hi + 2 * d.n3};
+    HIPCHK(halo_fill(c, mf, 3, s));
+    for (int iq = 0; iq < M.QA; ++iq) {
+      double* q = M.QTRC + d.n3 * (size_t)iq;
+      const double* rq = M.RHOQ_tp ? M.RHOQ_tp + d.n3 * (size_t)iq : M.zeros;
+      TracerIO io;
+      io.RHOQ_t = rq; io.DENS0 = M.DENS00; io.mflx_hi = M.mflx_hi; io.num_diff = M.ndq; io.flag_along = M.fct_along;
+      // stage 1: dt/3, no FCT
+      hipLaunchKernelGGL(dens_rk, dim3((d.IE - d.IS + 3 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 3), dim3(64, 4), 0, s,
+                         d, M.DENS_RK, M.DENS00, M.DENS, 1);
+      io.QTRCo = M.QRK1; io.QTRC = q; io.QTRC0 = q; io.DENS = M.DENS_RK; io.dtl = dtl / 3.0; io.flag_fct = 0;
+      HIPCHK(tracer_stage(c, io, s));
+      HIPCHK(halo_fill(c, &M.QRK1, 1, s));
+      // stage 2: dt/2, no FCT
+      hipLaunchKernelGGL(dens_rk, dim3((d.IE - d.IS + 3 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 3), dim3(64, 4), 0, s,
+                         d, M.DENS_RK, M.DENS00, M.DENS, 0);
+      io.QTRCo = M.QRK2; io.QTRC = M.QRK1; io.QTRC0 = q; io.DENS = M.DENS_RK; io.dtl = dtl / 2.0; io.flag_fct = 0;
+      HIPCHK(tracer_stage(c, io, s));
+      HIPCHK(halo_fill(c, &M.QRK2, 1, s));
+      // stage 3: dt, FCT as configured; written to QRK1 then copied back (QTRC0 is read at neighbours)
+      io.QTRCo = M.QRK1; io.QTRC = M.QRK2; io.QTRC0 = q; io.DENS = M.DENS; io.dtl = dtl; io.flag_fct = M.fct;
+      HIPCHK(tracer_stage(c, io, s));
+      HIPCHK(hipMemcpyAsync(q, M.QRK1, bytes, hipMemcpyDeviceToDevice, s));
+      HIPCHK(halo_fill(c, &q, 1, s));
+    }
+  }
+  return hipSuccess;
+}
+
+}  // namespace shp

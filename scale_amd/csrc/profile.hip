@@ -1,0 +1,87 @@
+// profile.hip -- named per-kernel timers with hipEvents recorded on the
+// stream the kernels run on (the counterpart of PROF_rapstart/PROF_rapend,
+// scalelib/src/common/scale_prof.F90:156-253).
+#include <string.h>
+#include <map>
+#include <string>
+#include <vector>
+#include "shp_internal.h"
+
+namespace shp {
+
+struct Prof {
+  struct Rec { std::string name; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<long, double>> acc;  // name -> (count, total ms)
+  std::string open_name;
+  hipEvent_t open_ev = nullptr;
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e; hipEventCreate(&e); return e;
+  }
+};
+
+void prof_start(const Ctx& c, const char* name, hipStream_t s) {
+  if (!c.prof) return;
+  Prof& p = *c.prof;
+  p.open_name = name;
+  p.open_ev = p.get();
+  hipEventRecord(p.open_ev, s);
+}
+
+void prof_stop(const Ctx& c, hipStream_t s) {
+  if (!c.prof || !c.prof->open_ev) return;
+  Prof& p = *c.prof;
+  hipEvent_t b = p.get();
+  hipEventRecord(b, s);
+  p.pending.push_back({p.open_name, p.open_ev, b});
+  p.open_ev = nullptr;
+  if (p.pending.size() > 4096) prof_collect(c);
+}
+
+void prof_collect(const Ctx& c) {
+  if (!c.prof) return;
+  Prof& p = *c.prof;
+  for (auto& r : p.pending) {
+    hipEventSynchronize(r.b);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, r.a, r.b);
+    auto& e = p.acc[r.name];
+    e.first += 1;
+    e.second += ms;
+    p.pool.push_back(r.a);
+    p.pool.push_back(r.b);
+  }
+  p.pending.clear();
+}
+
+Prof* prof_create() { return new Prof(); }
+void prof_destroy(Prof* p) {
+  if (!p) return;
+  for (auto& r : p->pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+  for (auto e : p->pool) hipEventDestroy(e);
+  delete p;
+}
+void prof_reset(Prof* p) {
+  if (!p) return;
+  p->acc.clear();
+}
+int prof_query(Prof* p, const char* name, long* count, double* total_ms) {
+  if (!p) return -1;
+  auto it = p->acc.find(name);
+  if (it == p->acc.end()) { *count = 0; *total_ms = 0.0; return 0; }
+  *count = it->second.first;
+  *total_ms = it->second.second;
+  return 0;
+}
+int prof_names(Prof* p, char* buf, int len) {
+  if (!p) return -1;
+  std::string s;
+  for (auto& kv : p->acc) { s += kv.first; s += "\n"; }
+  int n = (int)s.size() < len - 1 ? (int)s.size() : len - 1;
+  if (len > 0) { memcpy(buf, s.data(), (size_t)n); buf[n] = 0; }
+  return (int)s.size();
+}
+
+}  // namespace shp

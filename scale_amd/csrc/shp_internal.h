@@ -1,0 +1,83 @@
+// shp_internal.h -- host-side context and launch entry points shared by the
+// HIP translation units of libscalehip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+#include "common.h"
+
+namespace shp {
+
+constexpr int NSCR = 8;  // scratch fields of KA*IA*JA doubles owned by the context
+
+struct Model;  // device-resident model state (driver.hip)
+struct Prof;   // named kernel timers (profile.hip)
+
+struct Ctx {
+  Dims d;
+  int device = 0;
+  int scheme = UD3, scheme_tracer = UD3;
+  bool terrain = false;          // general metrics (GSQRT/J13G/J23G/MAPF) in use
+  bool metrics_set = false;
+  hipStream_t stream = nullptr;
+  // grid and metrics (device layout)
+  double* g1buf = nullptr;       // 1-D arrays, see Grid1D
+  Grid1D g1{};
+  double *GSQRT = nullptr, *J13G = nullptr, *J23G = nullptr, *MAPF = nullptr, *PHI = nullptr;
+  double J33G = 1.0;
+  // numerical filter tables
+  double *CNZ3 = nullptr, *CNX3 = nullptr, *CNY3 = nullptr, *ndc = nullptr;
+  double nd_coef = 0.0, nd_dt = 0.0, nd_sfc_fact = 1.0;
+  // scratch
+  double* scr[NSCR] = {};
+  // staging buffer for host <-> device layout transposes
+  double* stage = nullptr;
+  size_t stage_elems = 0;
+  // device-resident model
+  Model* model = nullptr;
+  // named hipEvent timers, null when profiling is off
+  Prof* prof = nullptr;
+  // host copies of 1-D grid arrays (for the numerical-filter tables)
+  std::vector<double> hCDZ, hFDZ, hCDX, hFDX, hCDY, hFDY;
+};
+
+// one HEVI stage, all pointers in device layout
+struct StageIO {
+  const Ctx* ctx;
+  double* o[5];             // DENS_RK..RHOT_RK
+  double *mflx_hi, *tflx_hi;  // (..,3)
+  const double* s0[5];      // DENS0..RHOT0
+  const double* s[5];       // DENS..RHOT
+  const double* t[5];       // DENS_t..RHOT_t
+  const double *DPRES0, *RT2P, *CORIOLI, *num_diff, *wdamp_coef, *DDIV, *REF_dens, *REF_rhot;
+  double divdmp_coef, dtrk;
+  int has_coriolis;
+};
+
+struct TracerIO {
+  double* QTRCo;
+  const double *QTRC, *QTRC0, *RHOQ_t, *DENS0, *DENS, *mflx_hi, *num_diff;
+  double dtl;
+  int flag_fct, flag_along;
+};
+
+hipError_t hevi_stage(const StageIO& io, hipStream_t s);
+// named timers around launches (no-ops unless c.prof is set): the counterpart
+// of the reference's PROF_rapstart / PROF_rapend (scale_prof.F90:156-253)
+void prof_start(const Ctx& c, const char* name, hipStream_t s);
+void prof_stop(const Ctx& c, hipStream_t s);
+void prof_collect(const Ctx& c);
+Prof* prof_create();
+void prof_destroy(Prof* p);
+void prof_reset(Prof* p);
+int prof_query(Prof* p, const char* name, long* count, double* total_ms);
+int prof_names(Prof* p, char* buf, int len);
+hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], const double* REF_dens,
+                     const double* REF_pott, hipStream_t s);
+hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s);
+hipError_t halo_fill(const Ctx& c, double* const* arrs, int n, hipStream_t s);
+// Fortran (k fastest) <-> device (i fastest) layout; ncomp fields of KA*IA*JA
+hipError_t to_device_layout(const Ctx& c, double* dst, const double* src_f, int ncomp, hipStream_t s);
+hipError_t to_fortran_layout(const Ctx& c, double* dst_f, const double* src, int ncomp, hipStream_t s);
+// 2-D (IA,JA,n) arrays are identical in both layouts
+
+}  // namespace shp

@@ -1,0 +1,362 @@
+// tracer.hip -- flux-form tracer advection with the FCT limiter on gfx950.
+//
+// Reference: ATMOS_DYN_Tstep_tracer_fvm_heve,
+//   scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_tracer_fvm_heve.F90:72-242,
+// and ATMOS_DYN_fct / get_fact_fct,
+//   scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:1922-2734.
+//
+// Without FCT a stage is one kernel: each thread evaluates the high-order
+// fluxes on the six faces of its cell and writes the updated mixing ratio.
+// With FCT a stage is three kernels around one halo exchange:
+//   fct_philo   low-order (UD1) provisional field phi_lo on the interior plus
+//               a one-cell ring (COMMON.F90:2033-2045)
+//   fct_ratio   anti-diffusive fluxes, P+/P-, the along-stream (or 7-point)
+//               bounds Q+/Q- and the ratios R+/R- (COMMON.F90:1988-2547)
+//   [periodic fill of R+/R-]                       (COMMON.F90:2558-2561)
+//   fct_update  limited anti-diffusive fluxes and the flux-form update
+//               (COMMON.F90:2570-2659, tracer_fvm_heve.F90:197-205)
+// The high- and low-order face fluxes are recomputed where needed instead of
+// being stored; only phi_lo and R+/R- go through memory.
+#include "common.h"
+#include "shp_internal.h"
+
+namespace shp {
+
+struct TrArgs {
+  Dims d;
+  const double *QTRC, *QTRC0, *RHOQ_t, *DENS0, *DENS, *mflx, *nd;
+  const double *GSQRT, *MAPF_XY;  // GSQRT(KA,IA,JA,7); MAPF(IA,JA,2) I_XY slice
+  const double *RCDZ, *RCDX, *RCDY;
+  double dtl;
+  double* QTRCo;
+  double *phi_lo, *rjpls, *rjmns;  // scratch
+};
+
+template <int S, bool TERRAIN>
+struct TrOps {
+  const TrArgs& a;
+  const Dims& d;
+  __device__ TrOps(const TrArgs& a_) : a(a_), d(a_.d) {}
+  __device__ __forceinline__ size_t ix(int k, int i, int j) const { return IX(d, k, i, j); }
+  __device__ __forceinline__ double gs(int c, size_t p) const {
+    if constexpr (TERRAIN) return a.GSQRT[p + d.n3 * (size_t)c]; else return 1.0;
+  }
+  __device__ __forceinline__ double m1(int i, int j) const {
+    if constexpr (TERRAIN) return a.MAPF_XY[(size_t)i + (size_t)d.IA * j]; else return 1.0;
+  }
+  __device__ __forceinline__ double m2(int i, int j) const {
+    if constexpr (TERRAIN) return a.MAPF_XY[(size_t)i + (size_t)d.IA * j + d.pl]; else return 1.0;
+  }
+  __device__ __forceinline__ double mZ(size_t p) const { return a.mflx[p]; }
+  __device__ __forceinline__ double mX(size_t p) const { return a.mflx[p + d.n3]; }
+  __device__ __forceinline__ double mY(size_t p) const { return a.mflx[p + 2 * d.n3]; }
+
+  // high-order fluxes on QTRC (fluxZ/X/Y_XYZ of the tracer scheme)
+  __device__ double hZ(int kk, int i, int j) const {
+    if (kk == d.KS - 1 || kk == d.KE) return 0.0;
+    const double* Q = a.QTRC;
+    size_t p = ix(kk, i, j);
+    double vel = mZ(p), fv;
+    if (kk == d.KS) fv = fv_lo<S>(Q[p], Q[ix(kk + 1, i, j)], Q[ix(kk + 2, i, j)], vel);
+    else if (kk == d.KE - 1) fv = fv_hi<S>(Q[ix(kk - 1, i, j)], Q[p], Q[ix(kk + 1, i, j)], vel);
+    else fv = fv_int<S>(Q[ix(kk - 1, i, j)], Q[p], Q[ix(kk + 1, i, j)], Q[ix(kk + 2, i, j)], vel);
+    return vel * fv + gs(I_XYW, p) * a.nd[p];
+  }
+  __device__ double hX(int k, int ii, int j) const {
+    const double* Q = a.QTRC;
+    size_t p = ix(k, ii, j);
+    double vel = mX(p);
+    return vel * fv_int<S>(Q[ix(k, ii - 1, j)], Q[p], Q[ix(k, ii + 1, j)], Q[ix(k, ii + 2, j)], vel) + gs(I_UYZ, p) * a.nd[p + d.n3];
+  }
+  __device__ double hY(int k, int i, int jj) const {
+    const double* Q = a.QTRC;
+    size_t p = ix(k, i, jj);
+    double vel = mY(p);
+    return vel * fv_int<S>(Q[ix(k, i, jj - 1)], Q[p], Q[ix(k, i, jj + 1)], Q[ix(k, i, jj + 2)], vel) + gs(I_XVZ, p) * a.nd[p + 2 * d.n3];
+  }
+  // low-order UD1 fluxes on QTRC0 (scale_atmos_dyn_fvm_flux_ud1.F90:135-249)
+  __device__ double lZ(int kk, int i, int j) const {
+    if (kk == d.KS - 1 || kk == d.KE) return 0.0;
+    const double* Q = a.QTRC0;
+    size_t p = ix(kk, i, j);
+    double vel = mZ(p);
+    return vel * fv_int<UD1>(0.0, Q[p], Q[ix(kk + 1, i, j)], 0.0, vel);
+  }
+  __device__ double lX(int k, int ii, int j) const {
+    const double* Q = a.QTRC0;
+    size_t p = ix(k, ii, j);
+    double vel = mX(p);
+    return vel * fv_int<UD1>(0.0, Q[p], Q[ix(k, ii + 1, j)], 0.0, vel);
+  }
+  __device__ double lY(int k, int i, int jj) const {
+    const double* Q = a.QTRC0;
+    size_t p = ix(k, i, jj);
+    double vel = mY(p);
+    return vel * fv_int<UD1>(0.0, Q[p], Q[ix(k, i, jj + 1)], 0.0, vel);
+  }
+  // anti-diffusive fluxes qflx_hi - qflx_lo (COMMON.F90:1988-2024)
+  __device__ double aZ(int kk, int i, int j) const {
+    if (kk == d.KS - 1 || kk == d.KE) return 0.0;
+    return hZ(kk, i, j) - lZ(kk, i, j);
+  }
+  __device__ double aX(int k, int ii, int j) const { return hX(k, ii, j) - lX(k, ii, j); }
+  __device__ double aY(int k, int i, int jj) const { return hY(k, i, jj) - lY(k, i, jj); }
+};
+
+// ---- no FCT: one kernel (tracer_fvm_heve.F90:226-231) ----------------------
+template <int S, bool TERRAIN>
+__global__ void __launch_bounds__(256) tracer_nofct(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+  const Dims& d = a.d;
+  TrOps<S, TERRAIN> o(a);
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
+  const int i = d.IS + (int)(it * 64 + threadIdx.x);
+  const int k = d.KS + (int)(kt * 4 + threadIdx.y);
+  const int j = d.JS + (int)jt;
+  if (i > d.IE || k > d.KE || j > d.JE) return;
+  const size_t p = IX(d, k, i, j);
+  a.QTRCo[p] = (a.QTRC0[p] * a.DENS0[p]
+                + a.dtl * (-((o.hZ(k, i, j) - o.hZ(k - 1, i, j)) * a.RCDZ[k]
+                             + (o.hX(k, i, j) - o.hX(k, i - 1, j)) * a.RCDX[i]
+                             + (o.hY(k, i, j) - o.hY(k, i, j - 1)) * a.RCDY[j])
+                           * o.m1(i, j) * o.m2(i, j) / o.gs(I_XYZ, p)
+                           + a.RHOQ_t[p]))
+             / a.DENS[p];
+}
+
+// ---- FCT pass 1: phi_lo on the interior plus a one-cell ring ----------------
+template <bool TERRAIN>
+__global__ void __launch_bounds__(256) fct_philo(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+  const Dims& d = a.d;
+  TrOps<UD1, TERRAIN> o(a);
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
+  const int i = d.IS - 1 + (int)(it * 64 + threadIdx.x);
+  const int k = d.KS + (int)(kt * 4 + threadIdx.y);
+  const int j = d.JS - 1 + (int)jt;
+  if (i > d.IE + 1 || k > d.KE || j > d.JE + 1) return;
+  const size_t p = IX(d, k, i, j);
+  a.phi_lo[p] = (a.QTRC0[p] * a.DENS0[p]
+                 + a.dtl * (-((o.lZ(k, i, j) - o.lZ(k - 1, i, j)) * a.RCDZ[k]
+                              + (o.lX(k, i, j) - o.lX(k, i - 1, j)) * a.RCDX[i]
+                              + (o.lY(k, i, j) - o.lY(k, i, j - 1)) * a.RCDY[j])
+                            * o.m1(i, j) * o.m2(i, j) / o.gs(I_XYZ, p)))
+              / a.DENS[p];
+}
+
+// get_fact_fct (COMMON.F90:2676-2734); fact(a,b,c) -> f[a + 2*((b+1) + 3*(c+1))]
+__device__ __forceinline__ void get_fact(double* f, double rw, double ru, double rv) {
+  const double ugev = fsign(0.5, __builtin_fabs(ru) - __builtin_fabs(rv)) + 0.5;
+  const double ugew = fsign(0.5, __builtin_fabs(ru) - __builtin_fabs(rw)) + 0.5;
+  const double vgew = fsign(0.5, __builtin_fabs(rv) - __builtin_fabs(rw)) + 0.5;
+  const double uzero = fsign(0.5, __builtin_fabs(ru) - EPS_RP) - 0.5;
+  const double vzero = fsign(0.5, __builtin_fabs(rv) - EPS_RP) - 0.5;
+  const double wzero = fsign(0.5, __builtin_fabs(rw) - EPS_RP) - 0.5;
+  const double sign_uv = fsign(0.5, ru * rv) + 0.5;
+  const double sign_uw = fsign(0.5, ru * rw) + 0.5;
+  const double sign_vw = fsign(0.5, rv * rw) + 0.5;
+  const double wu = __builtin_fabs(rw / (ru + uzero) * (1.0 + uzero));
+  const double vu = __builtin_fabs(rv / (ru + uzero) * (1.0 + uzero));
+  const double uv = __builtin_fabs(ru / (rv + vzero) * (1.0 + vzero));
+  const double wv = __builtin_fabs(rw / (rv + vzero) * (1.0 + vzero));
+  const double uw = __builtin_fabs(ru / (rw + wzero) * (1.0 + wzero));
+  const double vw = __builtin_fabs(rv / (rw + wzero) * (1.0 + wzero));
+  const double umax = ugev * ugew * (1.0 + uzero);
+  const double vmax = (1.0 - ugev) * vgew;
+  const double wmax = 1.0 - ugev * ugew - vmax;
+#define FCT_(a_, b_, c_) f[(a_) + 2 * (((b_) + 1) + 3 * ((c_) + 1))]
+  FCT_(0, 0, 0) = -ugev * ugew * uzero;
+  FCT_(1, 0, 0) = wmax * (1.0 - uw) * (1.0 - vw);
+  FCT_(0, 1, 0) = umax * (1.0 - vu) * (1.0 - wu);
+  FCT_(0, 0, 1) = vmax * (1.0 - uv) * (1.0 - wv);
+  const double c3 = (umax * vu * wu + vmax * uv * wv + wmax * uw * vw);
+  FCT_(1, 1, 1) = sign_uv * sign_uw * c3;
+  FCT_(1, -1, 1) = (1.0 - sign_uv) * (1.0 - sign_uw) * c3;
+  FCT_(1, 1, -1) = (1.0 - sign_uv) * sign_uw * c3;
+  FCT_(1, -1, -1) = sign_uv * (1.0 - sign_uw) * c3;
+  const double cuw = (ugew * wu * (1.0 - vu) + (1.0 - ugew) * uw * (1.0 - vw));
+  FCT_(1, 1, 0) = sign_uw * (1.0 - vmax) * cuw;
+  FCT_(1, -1, 0) = (1.0 - sign_uw) * (1.0 - vmax) * cuw;
+  const double cvw = (vgew * wv * (1.0 - uv) + (1.0 - vgew) * vw * (1.0 - uw));
+  FCT_(1, 0, 1) = sign_vw * (1.0 - umax) * cvw;
+  FCT_(1, 0, -1) = (1.0 - sign_vw) * (1.0 - umax) * cvw;
+  const double cuv = (ugev * vu * (1.0 - wu) + (1.0 - ugev) * uv * (1.0 - wv));
+  FCT_(0, 1, 1) = sign_uv * (1.0 - wmax) * cuv;
+  FCT_(0, 1, -1) = (1.0 - sign_uv) * (1.0 - wmax) * cuv;
+#undef FCT_
+}
+
+// upstream (qa) / downstream (qb) 14-point combinations (COMMON.F90:2096-2155)
+__device__ __forceinline__ double fct_qa(const double* f, const double* P, const Dims& d, int k1, int k0, int i, int j) {
+  return f[17] * P[IX(d, k1, i + 1, j + 1)] + f[16] * P[IX(d, k0, i + 1, j + 1)] + f[15] * P[IX(d, k1, i, j + 1)]
+       + f[14] * P[IX(d, k0, i, j + 1)] + f[13] * P[IX(d, k1, i - 1, j + 1)] + f[11] * P[IX(d, k1, i + 1, j)]
+       + f[10] * P[IX(d, k0, i + 1, j)] + f[9] * P[IX(d, k1, i, j)] + f[7] * P[IX(d, k1, i - 1, j)]
+       + f[5] * P[IX(d, k1, i + 1, j - 1)] + f[4] * P[IX(d, k0, i + 1, j - 1)] + f[3] * P[IX(d, k1, i, j - 1)]
+       + f[1] * P[IX(d, k1, i - 1, j - 1)] + f[8] * P[IX(d, k0, i, j)];
+}
+__device__ __forceinline__ double fct_qb(const double* f, const double* P, const Dims& d, int k1, int k0, int i, int j) {
+  return f[17] * P[IX(d, k1, i - 1, j - 1)] + f[16] * P[IX(d, k0, i - 1, j - 1)] + f[15] * P[IX(d, k1, i, j - 1)]
+       + f[14] * P[IX(d, k0, i, j - 1)] + f[13] * P[IX(d, k1, i + 1, j - 1)] + f[11] * P[IX(d, k1, i - 1, j)]
+       + f[10] * P[IX(d, k0, i - 1, j)] + f[9] * P[IX(d, k1, i, j)] + f[7] * P[IX(d, k1, i + 1, j)]
+       + f[5] * P[IX(d, k1, i - 1, j + 1)] + f[4] * P[IX(d, k0, i - 1, j - 1)] + f[3] * P[IX(d, k1, i, j - 1)]
+       + f[1] * P[IX(d, k1, i + 1, j + 1)] + f[8] * P[IX(d, k0, i, j)];
+}
+
+// ---- FCT pass 2: R+ and R- ----------------------------------------------------
+template <int S, bool TERRAIN, bool VECT>
+__global__ void __launch_bounds__(256) fct_ratio(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+  const Dims& d = a.d;
+  TrOps<S, TERRAIN> o(a);
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
+  const int i = d.IS + (int)(it * 64 + threadIdx.x);
+  const int k = d.KS + (int)(kt * 4 + threadIdx.y);
+  const int j = d.JS + (int)jt;
+  if (i > d.IE || k > d.KE || j > d.JE) return;
+  const int KS = d.KS, KE = d.KE;
+  const size_t p = IX(d, k, i, j);
+  const double dt = a.dtl;
+  const double az = o.aZ(k, i, j), azm = o.aZ(k - 1, i, j);
+  const double ax = o.aX(k, i, j), axm = o.aX(k, i - 1, j);
+  const double ay = o.aY(k, i, j), aym = o.aY(k, i, j - 1);
+  const double mm = o.m1(i, j) * o.m2(i, j);
+  const double g = o.gs(I_XYZ, p);
+  const double pjpls = dt * ((fmax2(0.0, azm) - fmin2(0.0, az)) * a.RCDZ[k]
+                           + (fmax2(0.0, axm) - fmin2(0.0, ax)) * a.RCDX[i]
+                           + (fmax2(0.0, aym) - fmin2(0.0, ay)) * a.RCDY[j])
+                     * o.m1(i, j) * o.m2(i, j) / g;
+  const double pjmns = dt * ((fmax2(0.0, az) - fmin2(0.0, azm)) * a.RCDZ[k]
+                           + (fmax2(0.0, ax) - fmin2(0.0, axm)) * a.RCDX[i]
+                           + (fmax2(0.0, ay) - fmin2(0.0, aym)) * a.RCDY[j])
+                     * o.m1(i, j) * o.m2(i, j) / g;
+  (void)mm;
+  const double* PI = a.QTRC0;
+  const double* PL = a.phi_lo;
+  const double pin = PI[p], plo = PL[p];
+  double qmax, qmin;
+  if constexpr (VECT) {
+    double rw;
+    int ka, kb;
+    if (k == KS) { rw = (o.mZ(p)) * a.RCDZ[KS]; ka = KS + 1; kb = KS; }
+    else if (k == KE) { rw = (o.mZ(IX(d, KE - 1, i, j))) * a.RCDZ[KE]; ka = KE; kb = KE - 1; }
+    else { rw = (o.mZ(p) + o.mZ(IX(d, k - 1, i, j))) * a.RCDZ[k]; ka = k + 1; kb = k - 1; }
+    const double ru = (o.mX(p) + o.mX(IX(d, k, i - 1, j))) * a.RCDX[i];
+    const double rv = (o.mY(p) + o.mY(IX(d, k, i, j - 1))) * a.RCDY[j];
+    double f[18];
+    get_fact(f, rw, ru, rv);
+    const double qa_in = fct_qa(f, PI, d, ka, k, i, j);
+    const double qb_in = fct_qb(f, PI, d, kb, k, i, j);
+    const double qa_lo = fct_qa(f, PL, d, ka, k, i, j);
+    const double qb_lo = fct_qb(f, PL, d, kb, k, i, j);
+    qmax = fmax2(fmax2(fmax2(fmax2(fmax2(pin, qa_in), qb_in), plo), qa_lo), qb_lo);
+    qmin = fmin2(fmin2(fmin2(fmin2(fmin2(pin, qa_in), qb_in), plo), qa_lo), qb_lo);
+  } else {  // 7-point bounds (COMMON.F90:2312-2515)
+    const int kp = k == KE ? k : k + 1, km = k == KS ? k : k - 1;
+    qmax = pin; qmin = pin;
+    const double* arrs[2] = {PI, PL};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const double* P = arrs[s];
+      const double v[7] = {P[p], P[IX(d, kp, i, j)], P[IX(d, km, i, j)], P[IX(d, k, i + 1, j)], P[IX(d, k, i - 1, j)],
+                           P[IX(d, k, i, j + 1)], P[IX(d, k, i, j - 1)]};
+#pragma unroll
+      for (int q = 0; q < 7; ++q) { qmax = fmax2(qmax, v[q]); qmin = fmin2(qmin, v[q]); }
+    }
+  }
+  const double qjpls = (qmax - plo) * a.DENS[p];
+  const double qjmns = (plo - qmin) * a.DENS[p];
+  double zerosw = 0.5 - fsign(0.5, pjpls - EPS_RP);
+  a.rjpls[p] = fmin2(1.0, qjpls * (1.0 - zerosw) / (pjpls - zerosw));
+  zerosw = 0.5 - fsign(0.5, pjmns - EPS_RP);
+  a.rjmns[p] = fmin2(1.0, qjmns * (1.0 - zerosw) / (pjmns - zerosw));
+}
+
+// ---- FCT pass 3: limited anti-diffusive fluxes and the update ------------------
+template <int S, bool TERRAIN>
+__global__ void __launch_bounds__(256) fct_update(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+  const Dims& d = a.d;
+  TrOps<S, TERRAIN> o(a);
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
+  const int i = d.IS + (int)(it * 64 + threadIdx.x);
+  const int k = d.KS + (int)(kt * 4 + threadIdx.y);
+  const int j = d.JS + (int)jt;
+  if (i > d.IE || k > d.KE || j > d.JE) return;
+  const size_t p = IX(d, k, i, j);
+  const double* RP = a.rjpls;
+  const double* RM = a.rjmns;
+  auto limZ = [&](int kk) -> double {  // face kk between levels kk and kk+1
+    if (kk == d.KS - 1 || kk == d.KE) return 0.0;
+    const double an = o.aZ(kk, i, j);
+    const double dirsw = 0.5 + fsign(0.5, an);
+    return an * (1.0 - fmin2(RP[IX(d, kk + 1, i, j)], RM[IX(d, kk, i, j)]) * (dirsw)
+                     - fmin2(RP[IX(d, kk, i, j)], RM[IX(d, kk + 1, i, j)]) * (1.0 - dirsw));
+  };
+  auto limX = [&](int ii) -> double {
+    const double an = o.aX(k, ii, j);
+    const double dirsw = 0.5 + fsign(0.5, an);
+    return an * (1.0 - fmin2(RP[IX(d, k, ii + 1, j)], RM[IX(d, k, ii, j)]) * (dirsw)
+                     - fmin2(RP[IX(d, k, ii, j)], RM[IX(d, k, ii + 1, j)]) * (1.0 - dirsw));
+  };
+  auto limY = [&](int jj) -> double {
+    const double an = o.aY(k, i, jj);
+    const double dirsw = 0.5 + fsign(0.5, an);
+    return an * (1.0 - fmin2(RP[IX(d, k, i, jj + 1)], RM[IX(d, k, i, jj)]) * (dirsw)
+                     - fmin2(RP[IX(d, k, i, jj)], RM[IX(d, k, i, jj + 1)]) * (1.0 - dirsw));
+  };
+  a.QTRCo[p] = (a.QTRC0[p] * a.DENS0[p]
+                + a.dtl * (-((o.hZ(k, i, j) - limZ(k) - o.hZ(k - 1, i, j) + limZ(k - 1)) * a.RCDZ[k]
+                             + (o.hX(k, i, j) - limX(i) - o.hX(k, i - 1, j) + limX(i - 1)) * a.RCDX[i]
+                             + (o.hY(k, i, j) - limY(j) - o.hY(k, i, j - 1) + limY(j - 1)) * a.RCDY[j])
+                           * o.m1(i, j) * o.m2(i, j) / o.gs(I_XYZ, p)
+                           + a.RHOQ_t[p]))
+             / a.DENS[p];
+}
+
+// ---------------------------------------------------------------------------
+template <int S, bool TERRAIN>
+static hipError_t tracer_t(const Ctx& c, const TrArgs& a, bool fct, bool along, hipStream_t s) {
+  const Dims& d = c.d;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
+  const unsigned ni = (imax + 63) / 64, nk = (kmax + 3) / 4;
+  const unsigned nb = ni * nk * (unsigned)jmax;
+  if (!fct) {
+    prof_start(c, "tracer_nofct", s);
+    hipLaunchKernelGGL((tracer_nofct<S, TERRAIN>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
+    prof_stop(c, s);
+    return hipGetLastError();
+  }
+  const unsigned ni2 = (imax + 2 + 63) / 64;
+  const unsigned nb2 = ni2 * nk * (unsigned)(jmax + 2);
+  prof_start(c, "fct_philo", s);
+  hipLaunchKernelGGL((fct_philo<TERRAIN>), dim3(nb2), dim3(64, 4), 0, s, a, ni2, nk, nb2);
+  prof_stop(c, s);
+  prof_start(c, "fct_ratio", s);
+  if (along) hipLaunchKernelGGL((fct_ratio<S, TERRAIN, true>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
+  else hipLaunchKernelGGL((fct_ratio<S, TERRAIN, false>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
+  prof_stop(c, s);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  double* r[2] = {a.rjpls, a.rjmns};
+  e = halo_fill(c, r, 2, s);
+  if (e != hipSuccess) return e;
+  prof_start(c, "fct_update", s);
+  hipLaunchKernelGGL((fct_update<S, TERRAIN>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
+  prof_stop(c, s);
+  return hipGetLastError();
+}
+
+hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s) {
+  TrArgs a;
+  a.d = c.d;
+  a.QTRC = io.QTRC; a.QTRC0 = io.QTRC0; a.RHOQ_t = io.RHOQ_t; a.DENS0 = io.DENS0; a.DENS = io.DENS;
+  a.mflx = io.mflx_hi; a.nd = io.num_diff; a.GSQRT = c.GSQRT; a.MAPF_XY = c.MAPF;
+  a.RCDZ = c.g1.RCDZ; a.RCDX = c.g1.RCDX; a.RCDY = c.g1.RCDY;
+  a.dtl = io.dtl; a.QTRCo = io.QTRCo;
+  a.phi_lo = c.scr[0]; a.rjpls = c.scr[1]; a.rjmns = c.scr[2];
+  const bool fct = io.flag_fct != 0, along = io.flag_along != 0;
+  if (c.scheme_tracer == UD3) return c.terrain ? tracer_t<UD3, true>(c, a, fct, along, s) : tracer_t<UD3, false>(c, a, fct, along, s);
+  return c.terrain ? tracer_t<CD4, true>(c, a, fct, along, s) : tracer_t<CD4, false>(c, a, fct, along, s);
+}
+
+}  // namespace shp

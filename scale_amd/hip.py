@@ -1,0 +1,231 @@
+"""Python front end of libscalehip.so, mirroring the reference's plug-in
+interfaces for the short-step dynamical core:
+
+* ``ATMOS_DYN_Tstep_short_fvm_hevi`` -- abstract interface ``short`` of
+  scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_short.F90:38-128;
+* ``ATMOS_DYN_Tstep_tracer_fvm_heve`` -- interface ``step`` of
+  scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_tracer.F90:34-64;
+* ``ATMOS_DYN_numfilter_coef`` --
+  scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:385-985;
+* ``Model`` -- the device-resident large step of
+  scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_large_fvm_heve.F90:171-1130.
+
+Arguments keep the reference's names, order and shapes (NumPy float64 arrays in
+Fortran order).  Errors raise ``ScaleHipError`` with the library's message, the
+counterpart of the reference's LOG_ERROR + PRC_abort.  There is no CPU
+fallback: without a loadable libscalehip.so every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libscalehip.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+UD3, CD4 = 3, 4
+FIELD = dict(DENS=0, MOMZ=1, MOMX=2, MOMY=3, RHOT=4, REF_dens=10, REF_pott=11, REF_pres=12, CORIOLI=13,
+             wdamp_coef=14, mflx_hi=20, tflx_hi=21, DENS_tp=30, MOMZ_tp=31, MOMX_tp=32, MOMY_tp=33, RHOT_tp=34)
+QTRC_BASE, RHOQ_TP_BASE = 100, 200
+
+
+class ScaleHipError(RuntimeError):
+    pass
+
+
+class HipGrid(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("KA", "IA", "JA", "KS", "KE", "IS", "IE", "JS", "JE",
+                                          "KHALO", "IHALO", "JHALO", "flux_scheme", "flux_scheme_tracer", "device")]
+
+
+class DynConfig(C.Structure):
+    _fields_ = [("dtl", C.c_double), ("dts", C.c_double), ("nd_coef", C.c_double), ("nd_sfc_fact", C.c_double),
+                ("divdmp_coef", C.c_double), ("nd_order", C.c_int32), ("fct_tracer", C.c_int32),
+                ("fct_along_stream", C.c_int32)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libscalehip.so; raises if it is missing (no silent fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise ScaleHipError(f"{path} not built; run __graft_entry__.build() or python scale_amd/build.py")
+        _lib = C.CDLL(path)
+        _lib.scale_hip_last_error.restype = C.c_char_p
+        _lib.scale_hip_version.restype = C.c_char_p
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = load_library().scale_hip_last_error().decode()
+        raise ScaleHipError(f"libscalehip error {rc}: {msg}")
+
+
+def _p(a):
+    if a is None:
+        return None
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64:
+        raise TypeError("arrays must be float64 NumPy arrays")
+    if not (a.flags.f_contiguous or a.ndim <= 1):
+        raise ValueError("arrays must be Fortran-contiguous (the reference's layout)")
+    return a.ctypes.data_as(_dp)
+
+
+def _d(x):
+    return C.byref(C.c_double(float(x)))
+
+
+def _l(x):
+    return C.byref(C.c_int32(1 if x else 0))
+
+
+class Context:
+    """One device context (scale_hip_init / scale_hip_finalize)."""
+
+    def __init__(self, grid, scheme=UD3, scheme_tracer=UD3, device=0):
+        L = load_library()
+        g = HipGrid(grid.KA, grid.IA, grid.JA, grid.KS + 1, grid.KE + 1, grid.IS + 1, grid.IE + 1, grid.JS + 1,
+                    grid.JE + 1, 2, 2, 2, scheme, scheme_tracer, device)
+        h = C.c_void_p()
+        _check(L.scale_hip_init(C.byref(g), C.byref(h)))
+        self._h = h
+        self.grid = grid
+        self.has_model = False
+
+    def close(self):
+        if self._h:
+            load_library().scale_hip_finalize(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- drop-in calls (host Fortran arrays) ----
+    def ATMOS_DYN_Tstep_short_fvm_hevi(self, DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, PROG_RK, mflx_hi, tflx_hi,
+                                       DENS0, MOMZ0, MOMX0, MOMY0, RHOT0, DENS, MOMZ, MOMX, MOMY, RHOT,
+                                       DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, PROG0, PROG,
+                                       DPRES0, RT2P, CORIOLI, num_diff, wdamp_coef, divdmp_coef, DDIV,
+                                       FLAG_FCT_MOMENTUM, FLAG_FCT_T, FLAG_FCT_ALONG_STREAM,
+                                       CDZ, FDZ, FDX, FDY, RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY,
+                                       PHI, GSQRT, J13G, J23G, J33G, MAPF, REF_dens, REF_rhot,
+                                       BND_W, BND_E, BND_S, BND_N, dtrk, last):
+        L = load_library()
+        _check(L.scale_hip_tstep_short_hevi(
+            self._h, _p(DENS_RK), _p(MOMZ_RK), _p(MOMX_RK), _p(MOMY_RK), _p(RHOT_RK), _p(PROG_RK), _p(mflx_hi), _p(tflx_hi),
+            _p(DENS0), _p(MOMZ0), _p(MOMX0), _p(MOMY0), _p(RHOT0), _p(DENS), _p(MOMZ), _p(MOMX), _p(MOMY), _p(RHOT),
+            _p(DENS_t), _p(MOMZ_t), _p(MOMX_t), _p(MOMY_t), _p(RHOT_t), _p(PROG0), _p(PROG),
+            _p(DPRES0), _p(RT2P), _p(CORIOLI), _p(num_diff), _p(wdamp_coef), _d(divdmp_coef), _p(DDIV),
+            _l(FLAG_FCT_MOMENTUM), _l(FLAG_FCT_T), _l(FLAG_FCT_ALONG_STREAM),
+            _p(CDZ), _p(FDZ), _p(FDX), _p(FDY), _p(RCDZ), _p(RCDX), _p(RCDY), _p(RFDZ), _p(RFDX), _p(RFDY),
+            _p(PHI), _p(GSQRT), _p(J13G), _p(J23G), _d(J33G), _p(MAPF), _p(REF_dens), _p(REF_rhot),
+            _l(BND_W), _l(BND_E), _l(BND_S), _l(BND_N), _d(dtrk), _l(last)))
+
+    def ATMOS_DYN_Tstep_tracer_fvm_heve(self, QTRCo, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff,
+                                        GSQRT, MAPF, CDZ, RCDZ, RCDX, RCDY, dtl, FLAG_FCT_TRACER, FLAG_FCT_ALONG_STREAM):
+        L = load_library()
+        _check(L.scale_hip_tstep_tracer_heve(
+            self._h, _p(QTRCo), _p(QTRC), _p(QTRC0), _p(RHOQ_t), _p(DENS0), _p(DENS), _p(mflx_hi), _p(num_diff),
+            _p(GSQRT), _p(MAPF), _p(CDZ), _p(RCDZ), _p(RCDX), _p(RCDY), _d(dtl), _l(FLAG_FCT_TRACER),
+            _l(FLAG_FCT_ALONG_STREAM)))
+
+    def ATMOS_DYN_numfilter_coef(self, num_diff, DENS, MOMZ, MOMX, MOMY, RHOT, CDZ, CDX, CDY, FDZ, FDX, FDY, DT,
+                                 REF_dens, REF_pott, ND_COEF, ND_ORDER, ND_SFC_FACT, ND_USE_RS):
+        L = load_library()
+        _check(L.scale_hip_numfilter_coef(
+            self._h, _p(num_diff), _p(DENS), _p(MOMZ), _p(MOMX), _p(MOMY), _p(RHOT), _p(CDZ), _p(CDX), _p(CDY),
+            _p(FDZ), _p(FDX), _p(FDY), _d(DT), _p(REF_dens), _p(REF_pott), _d(ND_COEF),
+            C.byref(C.c_int32(int(ND_ORDER))), _d(ND_SFC_FACT), _l(ND_USE_RS)))
+
+    # ---- device-resident model ----
+    def set_metrics(self, m: dict):
+        _check(load_library().scale_hip_set_metrics(
+            self._h, *[_p(m[k]) for k in ("CDZ", "FDZ", "RCDZ", "RFDZ", "CDX", "FDX", "RCDX", "RFDX",
+                                          "CDY", "FDY", "RCDY", "RFDY", "PHI", "GSQRT", "J13G", "J23G")],
+            _d(m["J33G"]), _p(m["MAPF"])))
+
+    def model_create(self, dtl, dts, QA, nd_coef=1.0e-4, nd_sfc_fact=1.0, fct_tracer=False, fct_along_stream=True):
+        cfg = DynConfig(dtl, dts, nd_coef, nd_sfc_fact, 0.0, 1, int(fct_tracer), int(fct_along_stream))
+        _check(load_library().scale_hip_model_create(self._h, C.byref(cfg), C.c_int32(QA)))
+        self.has_model = True
+        self.QA = QA
+
+    def model_set(self, name, arr):
+        fid = name if isinstance(name, int) else FIELD[name]
+        _check(load_library().scale_hip_model_set(self._h, C.c_int32(fid), _p(np.asfortranarray(arr, dtype=np.float64))))
+
+    def model_get(self, name, out=None):
+        g = self.grid
+        fid = name if isinstance(name, int) else FIELD[name]
+        ncomp = 3 if fid in (20, 21) else 1
+        if out is None:
+            out = np.zeros((g.KA, g.IA, g.JA) + ((3,) if ncomp == 3 else ()), order="F")
+        _check(load_library().scale_hip_model_get(self._h, C.c_int32(fid), _p(out)))
+        return out
+
+    def model_large_step(self, n=1):
+        _check(load_library().scale_hip_model_large_step(self._h, C.c_int32(n)))
+
+    def model_prepare(self):
+        _check(load_library().scale_hip_model_prepare(self._h))
+
+    def model_short_step(self, n=1):
+        _check(load_library().scale_hip_model_short_step(self._h, C.c_int32(n)))
+
+    def model_mass(self, name, weight_dens=False):
+        fid = name if isinstance(name, int) else FIELD[name]
+        out = C.c_double()
+        _check(load_library().scale_hip_model_mass(self._h, C.c_int32(fid), C.c_int32(int(weight_dens)), C.byref(out)))
+        return out.value
+
+    def profile(self, on=True):
+        _check(load_library().scale_hip_profile_enable(self._h, C.c_int32(int(on))))
+
+    def profile_query(self, name):
+        cnt = C.c_int64()
+        ms = C.c_double()
+        _check(load_library().scale_hip_profile_query(self._h, name.encode(), C.byref(cnt), C.byref(ms)))
+        return cnt.value, ms.value
+
+    def synchronize(self):
+        _check(load_library().scale_hip_synchronize(self._h))
+
+    def stream_ptr(self) -> int:
+        s = C.c_void_p()
+        _check(load_library().scale_hip_stream(self._h, C.byref(s)))
+        return s.value or 0
+
+
+def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, device=0) -> Context:
+    """Create a context holding the device-resident model of an inputs.Case."""
+    if ctx is None:
+        ctx = Context(case.grid, case.scheme, case.scheme_tracer, device)
+    ctx.set_metrics(case.metric)
+    qa = case.qtrc.shape[-1]
+    ctx.model_create(case.dt, case.dt, qa, nd_coef=nd_coef, fct_tracer=case.fct if fct is None else fct)
+    for k, v in case.state.items():
+        ctx.model_set(k, v)
+    for k, v in case.ref.items():
+        ctx.model_set(k, v)
+    ctx.model_set("CORIOLI", case.corioli)
+    ctx.model_set("wdamp_coef", case.wdamp_coef)
+    for iq in range(qa):
+        ctx.model_set(QTRC_BASE + iq, np.asfortranarray(case.qtrc[..., iq]))
+    return ctx

@@ -1,0 +1,187 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same seeded inputs.
+
+Single calls (one HEVI stage, numfilter, one tracer stage) perform the same
+IEEE-754 operations in the same order on both sides, so they are required to
+be bit-identical on every element the reference defines.  Multi-step
+integrations additionally go through pow() in the large-step preparation
+(RT2P, DPRES0), whose GPU and glibc implementations may differ in the last
+bit; they are held to the FP64 tolerance of the workload specification
+(1e-12 relative after N steps).
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+import pyoracle as po
+from scale_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+OUTS = ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK", "mflx_hi", "tflx_hi")
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from scale_amd import hip as shp
+    shp.load_library()
+    return shp
+
+
+def run_stage_gpu(hip, case, inp, dtrk, metric=None, scheme=None):
+    g = case.grid
+    m = metric or case.metric
+    out = {k: po.nan_like_outputs(g) for k in OUTS[:5]}
+    out["mflx_hi"] = po.nan_like_outputs(g, 3)
+    out["tflx_hi"] = po.nan_like_outputs(g, 3)
+    with hip.Context(g, scheme or case.scheme, case.scheme_tracer) as ctx:
+        ctx.ATMOS_DYN_Tstep_short_fvm_hevi(
+            out["DENS_RK"], out["MOMZ_RK"], out["MOMX_RK"], out["MOMY_RK"], out["RHOT_RK"], None, out["mflx_hi"], out["tflx_hi"],
+            inp["DENS0"], inp["MOMZ0"], inp["MOMX0"], inp["MOMY0"], inp["RHOT0"],
+            inp["DENS"], inp["MOMZ"], inp["MOMX"], inp["MOMY"], inp["RHOT"],
+            inp["DENS_t"], inp["MOMZ_t"], inp["MOMX_t"], inp["MOMY_t"], inp["RHOT_t"], None, None,
+            inp["DPRES0"], inp["RT2P"], inp.get("CORIOLI", case.corioli), inp["num_diff"], inp.get("wdamp_coef", case.wdamp_coef),
+            inp.get("divdmp_coef", 0.0), inp["DDIV"], False, False, True,
+            m["CDZ"], m["FDZ"], m["FDX"], m["FDY"], m["RCDZ"], m["RCDX"], m["RCDY"], m["RFDZ"], m["RFDX"], m["RFDY"],
+            m["PHI"], m["GSQRT"], m["J13G"], m["J23G"], m["J33G"], m["MAPF"], inp["REF_dens"], inp["REF_rhot"],
+            False, False, False, False, dtrk, False)
+    return out
+
+
+def oracle_stage(case, inp, dtrk, metric=None, scheme=None):
+    if metric is not None:
+        case = inputs.Case(**{**case.__dict__, "metric": metric})
+    return po.tstep_short_hevi(case, inp, dtrk, scheme=scheme)
+
+
+@pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+def test_hevi_stage_bitwise(hip, scheme, terrain):
+    print(f"[hevi stage] scheme={scheme} terrain={terrain}", flush=True)
+    case = H.small_case(seed=1000 + scheme)
+    inp = H.stage_inputs(case, seed=21, coriolis=terrain)
+    metric = H.terrain_metrics(case) if terrain else None
+    if terrain:
+        inp["wdamp_coef"] = np.linspace(0.0, 0.01, case.grid.KA)
+    ref = oracle_stage(case, inp, 0.5, metric, scheme)
+    got = run_stage_gpu(hip, case, inp, 0.5, metric, scheme)
+    for k in OUTS:
+        H.compare(ref[k], got[k], k)
+
+
+def test_hevi_stage_divdmp(hip):
+    print("[hevi stage] divergence damping", flush=True)
+    case = H.small_case(seed=77)
+    inp = H.stage_inputs(case, seed=5)
+    g = case.grid
+    inp["DDIV"] = np.asfortranarray(np.random.Generator(np.random.PCG64(3)).standard_normal(g.shape) * 1e-3)
+    inp["divdmp_coef"] = 0.1
+    ref = oracle_stage(case, inp, 1.0)
+    got = run_stage_gpu(hip, case, inp, 1.0)
+    for k in OUTS:
+        H.compare(ref[k], got[k], k)
+
+
+def test_numfilter_bitwise(hip):
+    print("[numfilter]", flush=True)
+    case = H.small_case(seed=99, stretched=(50.0, 400.0))
+    g = case.grid
+    st = H.perturbed(case, 4)
+    ref = po.numfilter_coef(case, st, 1.5)
+    got = np.full(ref.shape, np.nan, order="F")
+    m = case.metric
+    with hip.Context(g) as ctx:
+        ctx.ATMOS_DYN_numfilter_coef(got, st["DENS"], st["MOMZ"], st["MOMX"], st["MOMY"], st["RHOT"],
+                                     m["CDZ"], m["CDX"], m["CDY"], m["FDZ"], m["FDX"], m["FDY"], 1.5,
+                                     case.ref["REF_dens"], case.ref["REF_pott"], 1.0e-4, 1, 1.0, True)
+    H.compare(ref, got, "num_diff")
+
+
+@pytest.mark.parametrize("fct,along", [(False, True), (True, True), (True, False)], ids=["nofct", "fct_along", "fct_7pt"])
+@pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
+def test_tracer_bitwise(hip, fct, along, scheme):
+    print(f"[tracer] fct={fct} along={along} scheme={scheme}", flush=True)
+    case = H.small_case(seed=55)
+    g = case.grid
+    prep = inputs.large_step_prep(case)
+    del prep
+    st = H.perturbed(case, 8)
+    rng = np.random.Generator(np.random.PCG64(9))
+    mflx = np.asfortranarray(rng.standard_normal(g.shape + (3,)) * np.array([0.3, 8.0, 5.0]))
+    for n in range(3):
+        inputs.halo_fill(g, mflx[..., n])
+    q = np.asfortranarray(case.qtrc[..., 1])
+    q0 = np.asfortranarray(case.qtrc[..., 1] * (1.0 + 0.01 * rng.standard_normal(g.shape)))
+    inputs.halo_fill(g, q0)
+    nd = np.zeros(g.shape + (3,), order="F")
+    rq = np.zeros(g.shape, order="F")
+    mapf_xy = np.asfortranarray(case.metric["MAPF"][:, :, :, 0])
+    ref = np.full(g.shape, np.nan, order="F")
+    po.tstep_tracer(case, ref, q, q0, rq, case.state["DENS"], st["DENS"], mflx, nd, 2.0, fct, along, scheme=scheme)
+    got = np.full(g.shape, np.nan, order="F")
+    with hip.Context(g, case.scheme, scheme) as ctx:
+        m = case.metric
+        ctx.ATMOS_DYN_Tstep_tracer_fvm_heve(got, q, q0, rq, case.state["DENS"], st["DENS"], mflx, nd, m["GSQRT"], mapf_xy,
+                                            m["CDZ"], m["RCDZ"], m["RCDX"], m["RCDY"], 2.0, fct, along)
+    H.compare(ref, got, "QTRCo")
+
+
+def test_model_parity_warm_bubble(hip):
+    """Device-resident large steps (dynamics RK4 + RK3 tracers with FCT) against
+    the oracle integrator, 10 steps of a reduced warm bubble."""
+    print("[model] warm bubble 24x20x16, 10 steps", flush=True)
+    case = inputs.make_case(kmax=24, imax=20, jmax=16, noise=0.01, ntracer=2, tracer_kind=["blob", "tophat"], fct=True,
+                            bubble_r=2000.0, bubble_zc=2000.0)
+    m = po.Model(case)
+    m.step(10)
+    ctx = hip.model_from_case(case)
+    try:
+        ctx.model_large_step(10)
+        ctx.synchronize()
+        g = case.grid
+        sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+        for k in H.VARS:
+            got = ctx.model_get(k)
+            ref = m.state[k]
+            scale = np.abs(ref[sl]).max()
+            d = np.abs(got[sl] - ref[sl]).max()
+            print(f"  {k}: max|diff| {d:.3e} (scale {scale:.3e})", flush=True)
+            assert d <= 1.0e-12 * scale, k
+        for iq in range(2):
+            got = ctx.model_get(hip.QTRC_BASE + iq)
+            ref = m.qtrc[..., iq]
+            d = np.abs(got[sl] - ref[sl]).max()
+            print(f"  QTRC{iq}: max|diff| {d:.3e}", flush=True)
+            assert d <= 1.0e-12 * max(np.abs(ref[sl]).max(), 1e-30)
+    finally:
+        ctx.close()
+
+
+def test_model_mass_conservation(hip):
+    print("[model] mass conservation", flush=True)
+    case = inputs.make_case(kmax=32, imax=24, jmax=24, noise=0.1, ntracer=2, tracer_kind=["blob", "tophat"], fct=True)
+    ctx = hip.model_from_case(case)
+    try:
+        m0 = ctx.model_mass("DENS")
+        q0 = [ctx.model_mass(hip.QTRC_BASE + iq, weight_dens=True) for iq in range(2)]
+        ctx.model_large_step(20)
+        m1 = ctx.model_mass("DENS")
+        q1 = [ctx.model_mass(hip.QTRC_BASE + iq, weight_dens=True) for iq in range(2)]
+        assert abs(m1 - m0) <= 1.0e-13 * abs(m0)
+        for a, b in zip(q0, q1):
+            assert abs(b - a) <= 1.0e-13 * abs(a)
+    finally:
+        ctx.close()
+
+
+def test_model_deterministic(hip):
+    print("[model] determinism", flush=True)
+    case = H.small_case(seed=5)
+    outs = []
+    for _ in range(2):
+        ctx = hip.model_from_case(case)
+        ctx.model_large_step(3)
+        outs.append({k: ctx.model_get(k) for k in H.VARS})
+        ctx.close()
+    for k in H.VARS:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
