@@ -61,6 +61,8 @@ def lib():
         build()
         _lib = C.CDLL(LIB_PATH)
         _lib.orc_mass_sum.restype = C.c_double
+        _lib.orc_face_value.restype = C.c_double
+        _lib.orc_face_value.argtypes = [C.c_int, C.c_int] + [C.c_double] * 5
         _lib.orc_get_threads.restype = C.c_int
     return _lib
 
@@ -145,6 +147,10 @@ def tstep_tracer(case, QTRCo, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_dif
     if rc != 0:
         raise RuntimeError("orc_tstep_tracer_heve failed")
     return QTRCo
+
+
+def face_value(scheme, kind, qm1, q0, q1, q2, vel) -> float:
+    return lib().orc_face_value(scheme, kind, qm1, q0, q1, q2, vel)
 
 
 def get_fact_fct(rw, ru, rv) -> np.ndarray:

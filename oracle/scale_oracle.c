@@ -102,6 +102,13 @@ static inline double fv_hi(int s, double qm1, double q0, double q1, double vel) 
   return F2 * (q1 + q0);
 }
 
+/* exported for the closed-form tests: kind 0 interior, 1 lowest face, 2 highest face */
+double orc_face_value(int s, int kind, double qm1, double q0, double q1, double q2, double vel) {
+  if (kind == 1) return fv_lo(s, q0, q1, q2, vel);
+  if (kind == 2) return fv_hi(s, qm1, q0, q1, vel);
+  return fv_int(s, qm1, q0, q1, q2, vel);
+}
+
 /* F2H with the flux routine's own GSQRT argument (UD3.F90:75) */
 #define F2H(G, k, p, i, j) (CDZ[(k) + (p)-1] * (G)[I3((k) + (p)-1, i, j)] / (CDZ[k] * (G)[I3(k, i, j)] + CDZ[(k) + 1] * (G)[I3((k) + 1, i, j)]))
 

@@ -167,6 +167,10 @@ int orc_tinteg_tracer_rk3(const orc_grid *g, int scheme,
  * summed k-fastest then i then j in one fixed order (no threading). */
 double orc_mass_sum(const orc_grid *g, const orc_metric *m, const double *a);
 
+/* face value of the flux families: kind 0 interior (val(idx-1..idx+2)),
+ * 1 lowest face (val(KS..KS+2) in q0..q2), 2 highest face (val(KE-2..KE) in qm1..q1) */
+double orc_face_value(int scheme, int kind, double qm1, double q0, double q1, double q2, double vel);
+
 /* threads used by the OpenMP loops (1 = serial) */
 void orc_set_threads(int n);
 int orc_get_threads(void);

@@ -11,10 +11,3 @@ scale_amd/build.py); ``scale_amd.hip`` is its Python front end and
 from . import inputs  # noqa: F401
 
 __all__ = ["inputs", "hip"]
-
-
-def __getattr__(name):
-    if name == "hip":
-        from . import hip
-        return hip
-    raise AttributeError(name)

@@ -1,0 +1,499 @@
+"""Pins of the CPU oracle (oracle/liboracle.so) against facts that do not come
+from re-typing its own formulas: textbook closed forms of the stencils, an
+independent linear solver, the discrete implicit equations of the HEVI column
+(as the reference's own check_equation states them, HEVI.F90:1131-1271),
+exact conservation, symmetry of the discretisation under x<->y, translation
+invariance, and the reference unit tests' invariants
+(scalelib/test/unit/test_atmos_dyn.f90: test_const :382-454, test_conserve
+:456-592, test_cwc :594-670, test_fctminmax :672-763, test_undef :307-380) and
+its halo known-answer test (scalelib/test/unit/test_comm.f90:254-330).
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+import pyoracle as po
+from scale_amd import inputs
+
+UD1, UD3, CD4 = 1, 3, 4
+
+
+# ---------------------------------------------------------------------------
+# face-value stencils: finite-volume reconstructions from cell averages
+def cell_averages(coef, xs, h):
+    """Exact cell averages of the polynomial sum c_n x^n over [x-h/2, x+h/2]."""
+    out = []
+    for x in xs:
+        a, b = x - h / 2, x + h / 2
+        out.append(sum(c * (b ** (n + 1) - a ** (n + 1)) / (n + 1) for n, c in enumerate(coef)) / h)
+    return out
+
+
+def poly(coef, x):
+    return sum(c * x ** n for n, c in enumerate(coef))
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_cd4_exact_for_cubics(seed):
+    rng = np.random.default_rng(seed)
+    coef = rng.normal(size=4)
+    h = 0.7
+    qm1, q0, q1, q2 = cell_averages(coef, [-h, 0.0, h, 2 * h], h)
+    face = poly(coef, h / 2)
+    for vel in (1.0, -1.0):
+        v = po.face_value(CD4, 0, qm1, q0, q1, q2, vel)
+        assert v == pytest.approx(face, rel=1e-13, abs=1e-13)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_ud3_exact_for_quadratics_and_upwind_biased(seed):
+    rng = np.random.default_rng(seed)
+    coef = rng.normal(size=3)
+    h = 0.5
+    qm1, q0, q1, q2 = cell_averages(coef, [-h, 0.0, h, 2 * h], h)
+    face = poly(coef, h / 2)
+    assert po.face_value(UD3, 0, qm1, q0, q1, q2, 1.0) == pytest.approx(face, rel=1e-13, abs=1e-13)
+    assert po.face_value(UD3, 0, qm1, q0, q1, q2, -1.0) == pytest.approx(face, rel=1e-13, abs=1e-13)
+    # for a cubic the two upwind choices differ and each uses its upwind stencil only
+    coef4 = rng.normal(size=4)
+    qm1, q0, q1, q2 = cell_averages(coef4, [-h, 0.0, h, 2 * h], h)
+    up = po.face_value(UD3, 0, qm1, q0, q1, q2, 2.5)
+    assert up == pytest.approx((-qm1 + 5 * q0 + 2 * q1) / 6, rel=1e-14)
+    dn = po.face_value(UD3, 0, qm1, q0, q1, q2, -2.5)
+    assert dn == pytest.approx((2 * q0 + 5 * q1 - q2) / 6, rel=1e-14)
+    # the upwind value does not depend on the far downwind cell and vice versa
+    # (mathematically; the reference's grouping cancels it to round-off)
+    assert po.face_value(UD3, 0, qm1, q0, q1, q2 + 10.0, 1.0) == pytest.approx(up, rel=1e-13)
+    assert po.face_value(UD3, 0, qm1 + 10.0, q0, q1, q2, -1.0) == pytest.approx(dn, rel=1e-13)
+
+
+def test_ud1_is_donor_cell():
+    assert po.face_value(UD1, 0, 9.0, 1.25, 3.5, 9.0, 2.0) == 1.25
+    assert po.face_value(UD1, 0, 9.0, 1.25, 3.5, 9.0, -2.0) == 3.5
+    # sign(1,-0.0) = -1: a negative zero velocity picks the downwind cell as the reference does
+    assert po.face_value(UD1, 0, 9.0, 1.25, 3.5, 9.0, -0.0) == 3.5
+
+
+def test_boundary_faces():
+    rng = np.random.default_rng(3)
+    coef = rng.normal(size=3)
+    h = 1.0
+    q0, q1, q2 = cell_averages(coef, [0.0, h, 2 * h], h)
+    face = poly(coef, h / 2)
+    # UD3 near the bottom: second order from above (vel > 0), third order from below (vel < 0)
+    assert po.face_value(UD3, 1, 0, q0, q1, q2, 1.0) == pytest.approx(0.5 * (q0 + q1), rel=1e-15)
+    assert po.face_value(UD3, 1, 0, q0, q1, q2, -1.0) == pytest.approx(face, rel=1e-13)
+    assert po.face_value(CD4, 1, 0, q0, q1, q2, -1.0) == pytest.approx(0.5 * (q0 + q1), rel=1e-15)
+    qm1, q0, q1 = cell_averages(coef, [-h, 0.0, h], h)
+    assert po.face_value(UD3, 2, qm1, q0, q1, 0, 1.0) == pytest.approx(face, rel=1e-13)
+    assert po.face_value(UD3, 2, qm1, q0, q1, 0, -1.0) == pytest.approx(0.5 * (q0 + q1), rel=1e-15)
+
+
+# ---------------------------------------------------------------------------
+def test_thomas_matches_numpy_solve():
+    case = H.small_case(kmax=30, imax=4, jmax=4)
+    g = case.grid
+    rng = np.random.default_rng(0)
+    n = g.KE - g.KS  # KMAX-1 unknowns
+    F1 = np.zeros(g.KA); F2 = np.zeros(g.KA); F3 = np.zeros(g.KA)
+    rows = range(g.KS, g.KE)
+    for k in rows:
+        F1[k] = rng.uniform(-1, 1); F3[k] = rng.uniform(-1, 1); F2[k] = 3.0 + rng.uniform(0, 1)
+    A = np.zeros((n, n))
+    for r, k in enumerate(rows):
+        A[r, r] = F2[k]
+        if r + 1 < n:
+            A[r, r + 1] = F1[k]
+        if r > 0:
+            A[r, r - 1] = F3[k]
+    rhs = rng.normal(size=n)
+    x = po.solve_direct(g, rhs, F1, F2, F3)
+    assert np.allclose(x, np.linalg.solve(A, rhs), rtol=1e-13, atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+def _uniform_column_case(stretched=None):
+    """Horizontally uniform state with a theta perturbation profile and w = 0;
+    num_diff = 0 so the explicit tendencies vanish and the stage reduces to
+    the vertical implicit solve."""
+    case = inputs.make_case(kmax=24, imax=6, jmax=6, noise=0.0, bubble=False, u0=0.0, v0=0.0, ntracer=0,
+                            stretched=stretched)
+    g = case.grid
+    z = np.linspace(0, 1, g.KA)
+    rhot = case.state["RHOT"] * (1.0 + 0.002 * np.sin(7 * z))[:, None, None]
+    case.state["RHOT"] = np.asfortranarray(rhot)
+    return case
+
+
+@pytest.mark.parametrize("stretched", [None, (60.0, 300.0)], ids=["uniform_dz", "stretched_dz"])
+def test_hevi_column_satisfies_implicit_equations(stretched):
+    """The solved column satisfies the discrete implicit equations for rho,
+    rho*theta and rho*w (the system check_equation verifies, HEVI.F90:1131-1271),
+    written here from the equations, not from the solver's bands."""
+    case = _uniform_column_case(stretched)
+    g = case.grid
+    prep = inputs.large_step_prep(case)
+    # RT2P/DPRES0 from a different state so the pressure perturbation is nonzero
+    rhot_ref = np.asfortranarray(case.state["RHOT"] * 0.999)
+    prep["REF_rhot"] = rhot_ref
+    inp = {}
+    for v in H.VARS:
+        inp[v] = case.state[v]; inp[v + "0"] = case.state[v]; inp[v + "_t"] = g.zeros()
+    inp.update(prep)
+    inp["num_diff"] = np.zeros(g.shape + (5, 3), order="F")
+    inp["DDIV"] = g.zeros()
+    inp["REF_dens"] = np.asfortranarray(case.ref["REF_dens"] * 0.998)
+    dt = 3.0
+    out = po.tstep_short_hevi(case, inp, dt)
+    m = case.metric
+    KS, KE = g.KS, g.KE
+    i, j = g.IS + 1, g.JS + 2
+    C = out["MOMZ_RK"][:, i, j]  # MOMZ0 = MOMZ = 0 so MOMZ_RK = C
+    Dn = out["DENS_RK"][:, i, j]
+    Rn = out["RHOT_RK"][:, i, j]
+    D = case.state["DENS"][:, i, j]
+    R = case.state["RHOT"][:, i, j]
+    P = R / D
+    cdz, rcdz, rfdz = m["CDZ"], m["RCDZ"], m["RFDZ"]
+    # theta at w-levels: UD3 with sign(1, w=0) = +1 (UD3.F90:98-153)
+    PT = np.zeros(g.KA)
+    for k in range(KS + 1, KE - 1):
+        PT[k] = (-P[k - 1] + 5 * P[k] + 2 * P[k + 1]) / 6
+    PT[KS] = 0.5 * (P[KS] + P[KS + 1])
+    PT[KE - 1] = (2 * P[KE] + 5 * P[KE - 1] - P[KE - 2]) / 6
+    Cf = np.zeros(g.KA)
+    Cf[KS:KE] = C[KS:KE]
+    for k in range(KS, KE + 1):
+        lhs = (Dn[k] - D[k]) / dt
+        rhs = -(Cf[k] - Cf[k - 1]) * rcdz[k]
+        assert lhs == pytest.approx(rhs, rel=1e-9, abs=1e-15)
+        lhs = (Rn[k] - R[k]) / dt
+        rhs = -(Cf[k] * PT[k] - Cf[k - 1] * PT[k - 1]) * rcdz[k]
+        assert lhs == pytest.approx(rhs, rel=1e-9, abs=1e-13)
+    rt2p, dp0, rr = prep["RT2P"][:, i, j], prep["DPRES0"][:, i, j], rhot_ref[:, i, j]
+    refd = inp["REF_dens"][:, i, j]
+    dpres_n = dp0 + rt2p * (Rn - rr)
+    for k in range(KS, KE):
+        # explicit density anomaly at the w-level with the F2H weights (HEVI.F90:62,716-717) ...
+        f1 = cdz[k] / (cdz[k] + cdz[k + 1])
+        f2 = cdz[k + 1] / (cdz[k] + cdz[k + 1])
+        # ... and the implicit density increment as the mass-weighted mean of the
+        # two layers, which is what B = g dt^2 / (CDZ(k+1)+CDZ(k)) encodes
+        # (HEVI.F90:698-709, comment at :711); equal to F2H on a uniform grid
+        w1 = cdz[k + 1] / (cdz[k] + cdz[k + 1])
+        w2 = cdz[k] / (cdz[k] + cdz[k + 1])
+        lhs = Cf[k] / dt
+        rhs = (-(dpres_n[k + 1] - dpres_n[k]) * rfdz[k]
+               - 9.80665 * (f1 * (D[k + 1] - refd[k + 1]) + f2 * (D[k] - refd[k]))
+               - 9.80665 * (w1 * (Dn[k + 1] - D[k + 1]) + w2 * (Dn[k] - D[k])))
+        assert lhs == pytest.approx(rhs, rel=1e-8, abs=1e-10 * max(1.0, abs(rhs)))
+
+
+# ---------------------------------------------------------------------------
+def _stage(case, inp, dt=0.7, scheme=None):
+    return po.tstep_short_hevi(case, inp, dt, scheme=scheme)
+
+
+@pytest.mark.parametrize("scheme", [UD3, CD4])
+def test_stage_conserves_mass_and_rhot(scheme):
+    case = H.small_case(seed=31)
+    inp = H.stage_inputs(case, seed=8, tend=False)
+    out = _stage(case, inp, scheme=scheme)
+    g = case.grid
+    m = po.Model(case)
+    for v in ("DENS", "RHOT"):
+        a0 = m.mass(np.asfortranarray(inp[v + "0"]))
+        a1 = m.mass(np.asfortranarray(np.where(np.isnan(out[v + "_RK"]), 0.0, out[v + "_RK"])))
+        assert abs(a1 - a0) <= 2e-15 * abs(a0) * g.KA, v
+
+
+def _transpose_case_inputs(case, inp):
+    """x<->y mirror: swap i and j, MOMX and MOMY, and the x/y components."""
+    T = lambda a: np.asfortranarray(np.swapaxes(a, 1, 2))
+    ti = {}
+    for k, v in inp.items():
+        if not isinstance(v, np.ndarray) or v.ndim < 3:
+            ti[k] = v
+            continue
+        ti[k] = T(v)
+    for s in ("", "0", "_t"):
+        ti["MOMX" + s], ti["MOMY" + s] = T(inp["MOMY" + s]), T(inp["MOMX" + s])
+    nd = T(inp["num_diff"])
+    nd2 = nd.copy(order="F")
+    # swap the XDIR/YDIR components and the MOMX/MOMY variables (index 2 <-> 3)
+    for v in range(5):
+        vv = {2: 3, 3: 2}.get(v, v)
+        nd2[..., vv, 0] = nd[..., v, 0]
+        nd2[..., vv, 1] = nd[..., v, 2]
+        nd2[..., vv, 2] = nd[..., v, 1]
+    ti["num_diff"] = np.asfortranarray(nd2)
+    return ti
+
+
+def test_stage_xy_mirror_symmetry():
+    case = H.small_case(kmax=16, imax=10, jmax=10, seed=17)
+    inp = H.stage_inputs(case, seed=4)
+    out = _stage(case, inp)
+    tcase = inputs.Case(**{**case.__dict__})
+    tinp = _transpose_case_inputs(case, inp)
+    tout = _stage(tcase, tinp)
+    T = lambda a: np.swapaxes(a, 1, 2)
+    g = case.grid
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    pairs = [("DENS_RK", "DENS_RK"), ("RHOT_RK", "RHOT_RK"), ("MOMZ_RK", "MOMZ_RK"), ("MOMX_RK", "MOMY_RK"), ("MOMY_RK", "MOMX_RK")]
+    for a, b in pairs:
+        x, y = out[a][sl], T(tout[b])[sl]
+        assert np.allclose(x, y, rtol=1e-13, atol=1e-13 * np.abs(x).max()), (a, b, np.abs(x - y).max())
+    mx, my = out["mflx_hi"], T(tout["mflx_hi"])
+    assert np.allclose(mx[..., 0][sl], my[..., 0][sl], rtol=1e-13, atol=1e-16)
+    assert np.allclose(mx[..., 1][sl], my[..., 2][sl], rtol=1e-13, atol=1e-16)
+    assert np.allclose(mx[..., 2][sl], my[..., 1][sl], rtol=1e-13, atol=1e-16)
+
+
+def test_stage_translation_invariance():
+    case = H.small_case(kmax=12, imax=10, jmax=8, seed=2)
+    inp = H.stage_inputs(case, seed=6)
+    out = _stage(case, inp)
+    g = case.grid
+    si, sj = 3, 5
+
+    def shift(a):
+        a = np.array(a, order="F", copy=True)
+        core = a[:, g.IS:g.IE + 1, g.JS:g.JE + 1]
+        a[:, g.IS:g.IE + 1, g.JS:g.JE + 1] = np.roll(np.roll(core, si, axis=1), sj, axis=2)
+        inputs.halo_fill(g, a) if a.ndim == 3 else None
+        return a
+
+    sinp = {}
+    for k, v in inp.items():
+        if isinstance(v, np.ndarray) and v.ndim == 3 and v.shape == g.shape:
+            sinp[k] = shift(v)
+        elif k == "num_diff":
+            nd = np.array(v, order="F", copy=True)
+            for a_ in range(5):
+                for d_ in range(3):
+                    nd[..., a_, d_] = shift(v[..., a_, d_])
+            sinp[k] = nd
+        else:
+            sinp[k] = v
+    sout = _stage(case, sinp)
+    for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK"):
+        a = out[k][g.KS:g.KE + 1, g.IS:g.IE + 1, g.JS:g.JE + 1]
+        b = sout[k][g.KS:g.KE + 1, g.IS:g.IE + 1, g.JS:g.JE + 1]
+        assert np.array_equal(np.roll(np.roll(a, si, axis=1), sj, axis=2), b), k
+
+
+def test_stage_rest_state_stays_at_rest():
+    """Reference state, no motion, no perturbation: every tendency vanishes."""
+    case = inputs.make_case(kmax=20, imax=8, jmax=8, noise=0.0, bubble=False, u0=0.0, ntracer=0)
+    g = case.grid
+    inp = {}
+    for v in H.VARS:
+        inp[v] = case.state[v]; inp[v + "0"] = case.state[v]; inp[v + "_t"] = g.zeros()
+    prep = inputs.large_step_prep(case)
+    inp.update(prep)
+    inp["num_diff"] = po.numfilter_coef(case, case.state, 1.0)
+    inp["DDIV"] = g.zeros()
+    inp["REF_dens"] = case.ref["REF_dens"]
+    # reference pressure consistent with RHOT so DPRES0 = 0
+    inp["DPRES0"] = g.zeros()
+    out = _stage(case, inp, dt=2.0)
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    assert np.abs(out["MOMZ_RK"][sl]).max() < 1e-10
+    assert np.abs(out["MOMX_RK"][sl]).max() < 1e-10
+    assert np.abs(out["MOMY_RK"][sl]).max() < 1e-10
+    assert np.allclose(out["DENS_RK"][sl], case.state["DENS"][sl], rtol=1e-14, atol=0)
+
+
+# ---------------------------------------------------------------------------
+def test_numfilter_two_grid_wave_closed_form():
+    """For q = A(-1)^i in x the 3rd-difference flux is 8A(-1)^i/dx^3 and
+    num_diff = ND_COEF*dx/(2 DT) * A(-1)^i (density, x direction)."""
+    case = inputs.make_case(kmax=10, imax=8, jmax=6, noise=0.0, bubble=False, u0=0.0, ntracer=0)
+    g = case.grid
+    A = 1.0e-3
+    st = {k: np.array(v, order="F", copy=True) for k, v in case.state.items()}
+    sign = (-1.0) ** np.arange(g.IA)
+    st["DENS"] = np.asfortranarray(case.ref["REF_dens"] + A * sign[None, :, None])
+    dt, coef = 2.0, 1.0e-4
+    nd = po.numfilter_coef(case, st, dt, nd_coef=coef)
+    want = coef * g.dx / (2 * dt) * A * sign
+    for k in range(g.KS + 2, g.KE + 1):
+        assert np.allclose(nd[k, g.IS:g.IE + 1, g.JS:g.JE + 1, 0, 1], want[g.IS:g.IE + 1, None], rtol=1e-12)
+    # a field that varies only in x has no y or z flux
+    assert np.abs(nd[g.KS:g.KE + 1, g.IS:g.IE + 1, g.JS:g.JE + 1, 0, 2]).max() < 1e-18
+
+
+def test_numfilter_annihilates_quadratics():
+    case = inputs.make_case(kmax=10, imax=12, jmax=6, noise=0.0, bubble=False, u0=0.0, ntracer=0)
+    g = case.grid
+    st = {k: np.array(v, order="F", copy=True) for k, v in case.state.items()}
+    i = np.arange(g.IA, dtype=float)
+    st["DENS"] = np.asfortranarray(case.ref["REF_dens"] + (1e-3 + 2e-4 * i - 3e-5 * i * i)[None, :, None])
+    nd = po.numfilter_coef(case, st, 1.0)
+    inner = nd[g.KS:g.KE + 1, g.IS + 1:g.IE - 1, g.JS:g.JE + 1, 0, 1]
+    assert np.abs(inner).max() < 1e-15
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("seed", range(6))
+def test_fct_weights_partition_unity(seed):
+    rng = np.random.default_rng(seed)
+    rw, ru, rv = rng.normal(size=3) * rng.choice([1e-3, 1.0, 1e3], size=3)
+    f = po.get_fact_fct(rw, ru, rv)
+    assert f.sum() == pytest.approx(1.0, abs=1e-14)
+    assert (f >= -1e-15).all()
+
+
+def test_fct_weights_axis_aligned_and_zero_flow():
+    f = po.get_fact_fct(0.0, 2.0, 0.0)  # pure +x flow: all weight on fact(0,1,0)
+    assert f[0, 2, 1] == 1.0 and np.count_nonzero(f) == 1
+    f = po.get_fact_fct(0.0, 0.0, 0.0)  # no flow: fact(0,0,0) = 1
+    assert f[0, 1, 1] == 1.0 and np.count_nonzero(f) == 1
+    f = po.get_fact_fct(-3.0, 0.0, 0.0)  # vertical flow: fact(1,0,0)
+    assert f[1, 1, 1] == 1.0 and np.count_nonzero(f) == 1
+
+
+def _consistent_tracer_setup(seed=0, dt=4.0):
+    """DENS = DENS0 - dt*div(mflx) so the continuity equation holds discretely."""
+    case = inputs.make_case(kmax=16, imax=12, jmax=10, noise=0.0, bubble=False, ntracer=0)
+    g = case.grid
+    rng = np.random.default_rng(seed)
+    mflx = np.zeros(g.shape + (3,), order="F")
+    mflx[g.KS:g.KE, :, :, 0] = 0.5 * rng.normal(size=(g.KE - g.KS, g.IA, g.JA))
+    mflx[g.KS:g.KE + 1, :, :, 1] = 10 * rng.normal(size=(g.KE - g.KS + 1, g.IA, g.JA))
+    mflx[g.KS:g.KE + 1, :, :, 2] = 10 * rng.normal(size=(g.KE - g.KS + 1, g.IA, g.JA))
+    for n in range(3):
+        inputs.halo_fill(g, mflx[..., n])
+    m = case.metric
+    D0 = case.state["DENS"]
+    D = D0.copy(order="F")
+    KS, KE, IS, IE, JS, JE = g.KS, g.KE, g.IS, g.IE, g.JS, g.JE
+    for k in range(KS, KE + 1):
+        div = ((mflx[k, IS:IE + 1, JS:JE + 1, 0] - mflx[k - 1, IS:IE + 1, JS:JE + 1, 0]) * m["RCDZ"][k]
+               + (mflx[k, IS:IE + 1, JS:JE + 1, 1] - mflx[k, IS - 1:IE, JS:JE + 1, 1]) * m["RCDX"][IS:IE + 1, None]
+               + (mflx[k, IS:IE + 1, JS:JE + 1, 2] - mflx[k, IS:IE + 1, JS - 1:JE, 2]) * m["RCDY"][None, JS:JE + 1])
+        D[k, IS:IE + 1, JS:JE + 1] = D0[k, IS:IE + 1, JS:JE + 1] - dt * div
+    inputs.halo_fill(g, D)
+    return case, mflx, D0, D, dt
+
+
+@pytest.mark.parametrize("fct", [False, True])
+def test_tracer_constant_stays_constant(fct):
+    """test_cwc: a uniform tracer is preserved when the mass fluxes satisfy continuity."""
+    case, mflx, D0, D, dt = _consistent_tracer_setup(1)
+    g = case.grid
+    q = np.full(g.shape, 0.01, order="F")
+    out = np.full(g.shape, np.nan, order="F")
+    po.tstep_tracer(case, out, q, q, g.zeros(), D0, D, mflx, np.zeros(g.shape + (3,), order="F"), dt, fct)
+    inner = out[g.KS:g.KE + 1, g.IS:g.IE + 1, g.JS:g.JE + 1]
+    assert np.allclose(inner, 0.01, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("along", [True, False])
+def test_fct_keeps_two_level_field_in_bounds(along):
+    """test_fctminmax: q in {0, 0.1} stays within [0, 0.1] up to 1e-14 relative."""
+    case, mflx, D0, D, dt = _consistent_tracer_setup(2)
+    g = case.grid
+    rng = np.random.default_rng(5)
+    q = np.asfortranarray(np.where(rng.uniform(size=g.shape) < 0.5, 0.0, 0.1))
+    inputs.halo_fill(g, q)
+    out = np.full(g.shape, np.nan, order="F")
+    nd = np.zeros(g.shape + (3,), order="F")
+    po.tstep_tracer(case, out, q, q, g.zeros(), D0, D, mflx, nd, dt, True, along)
+    inner = out[g.KS:g.KE + 1, g.IS:g.IE + 1, g.JS:g.JE + 1]
+    eps = 1.0e-14
+    assert inner.min() >= 0.0 - eps and inner.max() <= 0.1 * (1 + eps)
+    # without the limiter the high-order scheme overshoots (the test is not vacuous)
+    out2 = np.full(g.shape, np.nan, order="F")
+    po.tstep_tracer(case, out2, q, q, g.zeros(), D0, D, mflx, nd, dt, False)
+    inner2 = out2[g.KS:g.KE + 1, g.IS:g.IE + 1, g.JS:g.JE + 1]
+    assert inner2.min() < -1e-6 or inner2.max() > 0.1 + 1e-6
+
+
+@pytest.mark.parametrize("fct", [False, True])
+def test_tracer_mass_conserved(fct):
+    case, mflx, D0, D, dt = _consistent_tracer_setup(3)
+    g = case.grid
+    rng = np.random.default_rng(9)
+    q = np.asfortranarray(np.abs(rng.normal(size=g.shape)) * 1e-3)
+    inputs.halo_fill(g, q)
+    out = np.full(g.shape, np.nan, order="F")
+    po.tstep_tracer(case, out, q, q, g.zeros(), D0, D, mflx, np.zeros(g.shape + (3,), order="F"), dt, fct)
+    m = po.Model(case)
+    before = m.mass(np.asfortranarray(q * D0))
+    after = m.mass(np.asfortranarray(np.nan_to_num(out) * D))
+    assert abs(after - before) <= 1e-14 * abs(before)
+
+
+# ---------------------------------------------------------------------------
+def test_halo_known_answer():
+    """test_comm.f90:254-330 re-expressed for one periodic domain."""
+    case = inputs.make_case(kmax=6, imax=7, jmax=5, ntracer=0)
+    g = case.grid
+    a = np.full(g.shape, np.nan, order="F")
+    kk, ii, jj = np.meshgrid(np.arange(g.KA), np.arange(g.IA), np.arange(g.JA), indexing="ij")
+    val = kk * 1e6 + ii * 1e3 + jj
+    core = (slice(None), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    a[core] = val[core]
+    po.halo_periodic(g, a)
+    imax, jmax = g.IE - g.IS + 1, g.JE - g.JS + 1
+    wrap_i = lambda i: i + imax if i < g.IS else (i - imax if i > g.IE else i)
+    wrap_j = lambda j: j + jmax if j < g.JS else (j - jmax if j > g.JE else j)
+    for i in range(g.IA):
+        for j in range(g.JA):
+            assert np.array_equal(a[:, i, j], val[:, wrap_i(i), wrap_j(j)])
+
+
+# ---------------------------------------------------------------------------
+def test_model_invariants():
+    """test_conserve / test_undef over whole large steps (dynamics + FCT tracers)."""
+    case = inputs.make_case(kmax=20, imax=12, jmax=12, noise=0.05, ntracer=2, tracer_kind=["blob", "tophat"], fct=True,
+                            bubble_r=1500.0, bubble_zc=1500.0)
+    m = po.Model(case)
+    m0 = m.mass(m.state["DENS"])
+    r0 = m.mass(m.state["RHOT"])
+    q0 = [m.mass(np.asfortranarray(m.qtrc[..., iq] * m.state["DENS"])) for iq in range(2)]
+    m.step(6)
+    g = case.grid
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    for v in H.VARS:
+        assert not np.isnan(m.state[v][sl]).any()
+    assert abs(m.mass(m.state["DENS"]) - m0) <= 1e-13 * m0
+    assert abs(m.mass(m.state["RHOT"]) - r0) <= 1e-13 * r0
+    for iq in range(2):
+        q1 = m.mass(np.asfortranarray(m.qtrc[..., iq] * m.state["DENS"]))
+        assert abs(q1 - q0[iq]) <= 1e-13 * q0[iq]
+    # FCT keeps the top-hat within its initial range
+    qt = m.qtrc[..., 1][sl]
+    assert qt.min() >= -1e-18 and qt.max() <= 1e-3 * (1 + 1e-13)
+
+
+def test_model_horizontally_uniform_stays_uniform():
+    """test_const: a horizontally uniform state stays horizontally uniform."""
+    case = inputs.make_case(kmax=16, imax=8, jmax=8, noise=0.0, bubble=False, u0=7.0, v0=-2.0, ntracer=1,
+                            tracer_kind="uniform", fct=True)
+    m = po.Model(case)
+    m.step(3)
+    g = case.grid
+    for v in H.VARS:
+        a = m.state[v][g.KS:g.KE + 1, g.IS:g.IE + 1, g.JS:g.JE + 1]
+        assert np.array_equal(a, np.broadcast_to(a[:, :1, :1], a.shape)), v
+
+
+def test_model_constant_theta_preserved():
+    """test_cwc: theta = const and q = const survive random momentum."""
+    case = inputs.make_case(kmax=16, imax=10, jmax=8, noise=0.0, bubble=False, bv=0.0, u0=5.0, ntracer=1,
+                            tracer_kind="uniform", fct=True, momz_noise=0.3)
+    g = case.grid
+    rng = np.random.default_rng(1)
+    for v in ("MOMX", "MOMY"):
+        a = case.state[v]
+        a[:, g.IS:g.IE + 1, g.JS:g.JE + 1] += rng.normal(size=a[:, g.IS:g.IE + 1, g.JS:g.JE + 1].shape)
+        inputs.halo_fill(g, a)
+    m = po.Model(case)
+    m.step(3)
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    pott = m.state["RHOT"][sl] / m.state["DENS"][sl]
+    assert np.allclose(pott, 300.0, rtol=1e-13, atol=0)
+    assert np.allclose(m.qtrc[..., 0][sl], 1e-3, rtol=1e-13, atol=0)
