@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--size", default="512x512x128", help="IMAXxJMAXxKMAX")
     p.add_argument("--cpu-sample", default="128x128x128", help="sub-domain timed on the host with the oracle")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-large-step", action="store_true", help="skip the informational large step with tracers")
+    p.add_argument("--no-copy-probe", action="store_true", help="skip the HBM copy probe")
     return p.parse_args()
 
 
@@ -90,12 +92,20 @@ def main():
         kern[name] = dict(count=cnt, avg_ms=(ms / cnt if cnt else None))
     ctx.profile(False)
     # large step incl. 3 tracers with FCT (informational)
-    ctx.synchronize()
-    t2 = time.perf_counter()
-    nls = 2
-    ctx.model_large_step(nls)
-    ctx.synchronize()
-    large_ms = (time.perf_counter() - t2) * 1e3 / nls
+    large_ms = None
+    if not a.no_large_step:
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        nls = 2
+        ctx.model_large_step(nls)
+        ctx.synchronize()
+        large_ms = (time.perf_counter() - t2) * 1e3 / nls
+    # achievable HBM bandwidth: streaming copy of 2 x 2 GiB
+    copy_gbs = None
+    if not a.no_copy_probe:
+        n_copy = 1 << 28
+        copy_ms = ctx.hbm_copy_probe(n_copy, 10)
+        copy_gbs = 16.0 * n_copy / (copy_ms * 1e-3) / 1e9
     ctx.close()
 
     pts = ni * nj * nk
@@ -105,7 +115,9 @@ def main():
     achieved = STAGE_BYTES_PER_POINT * pts / (st_ms * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=None,
                 kernel="HEVI stage = hevi_explicit + hevi_solve", bytes_per_point=STAGE_BYTES_PER_POINT,
-                stage_ms=st_ms, kernels_avg_ms={k: v["avg_ms"] for k, v in kern.items()})
+                stage_ms=st_ms, kernels_avg_ms={k: v["avg_ms"] for k, v in kern.items()},
+                copy_probe_GBs=copy_gbs,
+                frac_of_copy=(achieved / copy_gbs if copy_gbs else None))
     out = {
         "metric": "grid-pt updates/s short step (%HBM roofline); FP32 vs FP64 speedup; adjoint/fwd",
         "value": value,

@@ -164,6 +164,13 @@ int scale_hip_stream(scale_hip_ctx *ctx, void **stream);
 int scale_hip_profile_enable(scale_hip_ctx *ctx, int32_t on);
 int scale_hip_profile_query(scale_hip_ctx *ctx, const char *name, int64_t *count, double *total_ms);
 
+/* HBM probe with no counterpart in the reference: `reps` timed streaming
+ * copies of n doubles between two scratch buffers (allocated and freed by the
+ * call); *avg_ms is the mean time of one copy, which moves 16*n bytes.  Gives
+ * the achievable copy bandwidth and a kernel of known traffic to calibrate
+ * the FETCH_SIZE / WRITE_SIZE counters of the stencil kernels against. */
+int scale_hip_hbm_copy_probe(scale_hip_ctx *ctx, int64_t n, int32_t reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -674,4 +674,33 @@ int scale_hip_profile_query(scale_hip_ctx* h, const char* name, int64_t* count, 
   return 0;
 }
 
+int scale_hip_hbm_copy_probe(scale_hip_ctx* h, int64_t n, int32_t reps, double* avg_ms) {
+  if (int r = check_ctx(h)) return r;
+  if (n <= 0 || reps <= 0 || !avg_ms) return fail("hbm_copy_probe: n > 0, reps > 0 and avg_ms are required");
+  Ctx& c = h->c;
+  double *src = nullptr, *dst = nullptr;
+  CK(hipMalloc((void**)&src, (size_t)n * sizeof(double)));
+  hipError_t e = hipMalloc((void**)&dst, (size_t)n * sizeof(double));
+  if (e != hipSuccess) { hipFree(src); return hipfail(e, "hipMalloc"); }
+  hipEvent_t a = nullptr, b = nullptr;
+  float ms = 0.f;
+  if ((e = hipMemsetAsync(src, 0, (size_t)n * sizeof(double), c.stream)) == hipSuccess &&
+      (e = shp::hbm_copy(c, dst, src, (size_t)n, c.stream)) == hipSuccess &&  // warm-up
+      (e = hipEventCreate(&a)) == hipSuccess && (e = hipEventCreate(&b)) == hipSuccess &&
+      (e = hipEventRecord(a, c.stream)) == hipSuccess) {
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = shp::hbm_copy(c, dst, src, (size_t)n, c.stream);
+    if (e == hipSuccess) e = hipEventRecord(b, c.stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  }
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  hipStreamSynchronize(c.stream);
+  hipFree(src);
+  hipFree(dst);
+  if (e != hipSuccess) return hipfail(e, "hbm_copy_probe");
+  *avg_ms = (double)ms / reps;
+  return 0;
+}
+
 }  // extern "C"

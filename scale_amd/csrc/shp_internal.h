@@ -71,6 +71,8 @@ void prof_destroy(Prof* p);
 void prof_reset(Prof* p);
 int prof_query(Prof* p, const char* name, long* count, double* total_ms);
 int prof_names(Prof* p, char* buf, int len);
+// streaming copy of n doubles (timed as "hbm_copy")
+hipError_t hbm_copy(const Ctx& c, double* dst, const double* src, size_t n, hipStream_t s);
 hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s);
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s);

@@ -207,6 +207,12 @@ class Context:
     def synchronize(self):
         _check(load_library().scale_hip_synchronize(self._h))
 
+    def hbm_copy_probe(self, n, reps=10):
+        """Mean time [ms] of one streaming copy of n doubles (16*n bytes of traffic)."""
+        ms = C.c_double()
+        _check(load_library().scale_hip_hbm_copy_probe(self._h, C.c_int64(int(n)), C.c_int32(int(reps)), C.byref(ms)))
+        return ms.value
+
     def stream_ptr(self) -> int:
         s = C.c_void_p()
         _check(load_library().scale_hip_stream(self._h, C.byref(s)))
