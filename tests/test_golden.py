@@ -1,0 +1,82 @@
+"""The oracle against the committed golden fixtures (tests/golden/), and the
+HIP path against the same fixtures over the full 60-step warm-bubble run."""
+import os
+
+import numpy as np
+import pytest
+
+import helpers as H
+import pyoracle as po
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIELDS = ("DENS", "MOMZ", "MOMX", "MOMY", "RHOT")
+
+
+def stats(a):
+    a = a[~np.isnan(a)]
+    return np.array([a.sum(), np.sqrt((a * a).sum()), np.abs(a).max()])
+
+
+def interior(g, a):
+    return a[g.KS:g.KE + 1, g.IS:g.IE + 1, g.JS:g.JE + 1]
+
+
+@pytest.fixture(scope="module")
+def wb():
+    return np.load(os.path.join(GOLDEN, "warm_bubble_40x40x80.npz"))
+
+
+def test_oracle_stage_matches_golden():
+    gold = np.load(os.path.join(GOLDEN, "stage_16x16x24.npz"))
+    for scheme in (3, 4):
+        case = H.small_case(kmax=24, imax=16, jmax=16, seed=2400 + scheme)
+        inp = H.stage_inputs(case, seed=17)
+        ref = po.tstep_short_hevi(case, inp, 0.5, scheme=scheme)
+        for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK", "mflx_hi", "tflx_hi"):
+            assert np.array_equal(stats(ref[k]), gold[f"{k}_{scheme}"]), (scheme, k)
+
+
+def test_oracle_warm_bubble_matches_golden(wb):
+    from scale_amd import inputs
+    case = inputs.warm_bubble_40x40x80()
+    g = case.grid
+    m = po.Model(case)
+    m.step(1)
+    for f in FIELDS:
+        assert np.array_equal(stats(interior(g, m.state[f])), wb[f"{f}_1"]), f
+    m.step(9)
+    for f in FIELDS:
+        assert np.array_equal(stats(interior(g, m.state[f])), wb[f"{f}_10"]), f
+    assert np.array_equal(stats(interior(g, m.qtrc[..., 0])), wb["QTRC_10"])
+
+
+@pytest.mark.gpu
+def test_gpu_warm_bubble_60_steps_matches_golden(wb):
+    """60 large steps of the warm bubble on the GPU within 1e-12 relative of
+    the oracle's run: the L2 norm and max |x| of each field within 1e-12 of
+    themselves, the interior sum within 1e-12 * sqrt(N) * L2 (a bound on
+    1e-12 * sum |x|), and every element of the centre x-z slice within
+    1e-12 of the field's max |x|."""
+    from scale_amd import hip, inputs
+    print("[golden] warm bubble 40x40x80, 60 steps", flush=True)
+    case = inputs.warm_bubble_40x40x80()
+    g = case.grid
+    ctx = hip.model_from_case(case)
+    try:
+        ctx.model_large_step(60)
+        st = {f: ctx.model_get(f) for f in FIELDS}
+        q = ctx.model_get(hip.QTRC_BASE + 0)
+    finally:
+        ctx.close()
+    jc = (g.JS + g.JE) // 2
+    for f in FIELDS:
+        got = stats(interior(g, st[f]))
+        ref = wb[f"{f}_60"]
+        scale = ref[2]
+        n = (g.KE - g.KS + 1) * (g.IE - g.IS + 1) * (g.JE - g.JS + 1)
+        tol = 1e-12 * np.array([np.sqrt(n) * ref[1], ref[1], ref[2]])
+        assert np.all(np.abs(got - ref) <= tol), (f, got, ref)
+        sl = st[f][g.KS:g.KE + 1, g.IS:g.IE + 1, jc]
+        d = np.abs(sl - wb[f"{f}_slice_60"]).max()
+        assert d <= 1e-12 * scale, (f, d, scale)
+    assert np.array_equal(stats(interior(g, q)), wb["QTRC_60"])
