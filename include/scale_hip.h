@@ -158,9 +158,9 @@ int scale_hip_stream(scale_hip_ctx *ctx, void **stream);
 
 /* Named kernel timers: hipEvents recorded around every launch on the context's
  * stream (the counterpart of PROF_rapstart/PROF_rapend,
- * scalelib/src/common/scale_prof.F90:156-253).  Names: hevi_explicit,
- * hevi_solve, numfilter, halo_fill, rk4_combine, tracer_nofct, fct_philo,
- * fct_ratio, fct_update.  Enabling resets the accumulators. */
+ * scalelib/src/common/scale_prof.F90:156-253).  Names: hevi_stage,
+ * numfilter, halo_fill, rk4_combine, tracer_nofct, fct_philo, fct_ratio,
+ * fct_update, hbm_copy.  Enabling resets the accumulators. */
 int scale_hip_profile_enable(scale_hip_ctx *ctx, int32_t on);
 int scale_hip_profile_query(scale_hip_ctx *ctx, const char *name, int64_t *count, double *total_ms);
 

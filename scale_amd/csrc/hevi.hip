@@ -4,18 +4,14 @@
 //   scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_short_fvm_hevi.F90:128-1089
 // with the flux stencils of scale_atmos_dyn_fvm_flux_{ud3,cd4}.F90.
 //
-// Two kernels per stage:
-//   hevi_explicit  one thread per interior point (i fastest, 64-lane rows):
-//                  mass fluxes X/Y, theta fluxes X/Y, the explicit part of the
-//                  Z fluxes, the tendencies Sr/St/Sw, the vertical-solve
-//                  face value PT, and the complete MOMX/MOMY updates
-//                  (HEVI.F90:346-657, 816-1047).
-//   hevi_solve     one thread per column marching k: tridiagonal bands,
-//                  right-hand side, Thomas forward sweep and back
-//                  substitution into MOMZ/DENS/RHOT and the implicit parts of
-//                  the Z fluxes (HEVI.F90:688-804, 1092-1127).
-// Fluxes needed on both faces of a cell are evaluated by the two threads that
-// own them with the same expression, so both see identical values.
+// One kernel per stage, hevi_march: one thread per column (i,j), blocks of
+// 64 (i) x 4 (j) columns.  Marching k upward it evaluates the explicit part
+// of each point (mass and theta fluxes, the explicit z fluxes, the tendencies
+// Sr/St/Sw, the face value PT, the complete MOMX/MOMY updates;
+// HEVI.F90:346-657, 816-1047) and, one level behind, the tridiagonal rows and
+// the Thomas forward sweep (HEVI.F90:688-727, 1092-1115); marching back down
+// it substitutes into MOMZ/DENS/RHOT and adds the implicit parts of the z
+// fluxes (HEVI.F90:728-804, 1116-1127).
 #include "common.h"
 #include "shp_internal.h"
 
@@ -37,7 +33,7 @@ struct HeviArgs {
   // outputs
   double *DENS_RK, *MOMZ_RK, *MOMX_RK, *MOMY_RK, *RHOT_RK, *mflx_hi, *tflx_hi;
   // scratch (n3 each)
-  double *Sr, *St, *Sw, *PT, *e, *f;
+  double *PT, *e, *f;
 };
 
 template <int S, bool TERRAIN, bool DIVDMP>
@@ -104,6 +100,22 @@ struct Ops {
     else if (k == d.KE - 1) fv = fv_hi<S>(POTT(k - 1, i, j), POTT(k, i, j), POTT(k + 1, i, j), vel);
     else fv = fv_int<S>(POTT(k - 1, i, j), POTT(k, i, j), POTT(k + 1, i, j), POTT(k + 2, i, j), vel);
     return vel * fv + m.gs(I_XYW, p) * ND(I_RHOT, ZDIR, p);
+  }
+  // the same with the column's theta window w = POTT(k-1), POTT(k), POTT(k+1), POTT(k+2)
+  __device__ double tflxZw(int k, int i, int j, double vel, const double* w) const {
+    if (k == d.KS - 1 || k == d.KE) return 0.0;
+    size_t p = ix(k, i, j);
+    double fv;
+    if (k == d.KS) fv = fv_lo<S>(w[1], w[2], w[3], vel);
+    else if (k == d.KE - 1) fv = fv_hi<S>(w[0], w[1], w[2], vel);
+    else fv = fv_int<S>(w[0], w[1], w[2], w[3], vel);
+    return vel * fv + m.gs(I_XYW, p) * ND(I_RHOT, ZDIR, p);
+  }
+  __device__ double PTfw(int k, int i, int j, const double* w) const {
+    const double vel = a.MOMZ[ix(k, i, j)];
+    if (k == d.KS) return fv_lo<S>(w[1], w[2], w[3], vel);
+    if (k == d.KE - 1) return fv_hi<S>(w[0], w[1], w[2], vel);
+    return fv_int<S>(w[0], w[1], w[2], w[3], vel);
   }
   __device__ double tflxX(int k, int i, int j, double vel) const {
     size_t p = ix(k, i, j);
@@ -258,16 +270,29 @@ struct Ops {
 };
 
 // ---------------------------------------------------------------------------
+// Explicit part of one point (k,i,j): mass and theta fluxes in x and y, the
+// explicit part of the z fluxes, the tendencies Sr/St/Sw, the face value PT
+// of the vertical solve, and the complete MOMX/MOMY updates
+// (HEVI.F90:346-657, 816-1047).  Fluxes needed on both faces of a cell are
+// evaluated with the same expression by every point that uses them, so
+// neighbours see identical values.
+struct PointTend { double Sr, St, Sw, PT; };
+
+// Fluxes through the lower face of the current level, carried from the level
+// below (where they were the upper face), and the column's theta window
+// POTT(k-1..k+2).  Every face flux at KS-1 is zero.
+struct ZCarry {
+  double mZ = 0.0, tZ = 0.0, qW = 0.0, qWJ13 = 0.0, qWJ23 = 0.0;
+  double qU = 0.0, qUJ13 = 0.0, qUJ23 = 0.0, qV = 0.0, qVJ13 = 0.0, qVJ23 = 0.0;
+  double pw[4];
+};
+
 template <int S, bool TERRAIN, bool DIVDMP>
-__global__ void __launch_bounds__(256) hevi_explicit(HeviArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+__device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP>& o, const HeviArgs& a, int k, int i, int j,
+                                                    ZCarry& zc) {
   const Dims& d = a.d;
-  Ops<S, TERRAIN, DIVDMP> o(a);
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
-  const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
-  const int i = d.IS + (int)(it * 64 + threadIdx.x);
-  const int k = d.KS + (int)(kt * 4 + threadIdx.y);
-  const int j = d.JS + (int)jt;
-  if (i > d.IE || k > d.KE || j > d.JE) return;
+  PointTend r;
+  r.Sw = 0.0; r.PT = 0.0;
   const size_t p = IX(d, k, i, j);
   const double* RCDX = a.g.RCDX; const double* RCDY = a.g.RCDY; const double* RCDZ = a.g.RCDZ;
   const double* RFDX = a.g.RFDX; const double* RFDY = a.g.RFDY; const double* RFDZ = a.g.RFDZ;
@@ -282,7 +307,8 @@ __global__ void __launch_bounds__(256) hevi_explicit(HeviArgs a, unsigned ni_til
   mfY[p] = mY;
   if (i == d.IS) mfX[IX(d, k, i - 1, j)] = mXm;
   if (j == d.JS) mfY[IX(d, k, i, j - 1)] = mYm;
-  const double mZ = o.mflxZf(k, i, j), mZm = o.mflxZf(k - 1, i, j);
+  const double mZ = o.mflxZf(k, i, j), mZm = zc.mZ;
+  zc.mZ = mZ;
   if (k <= d.KE - 1) mfZ[p] = mZ;
   if (k == d.KS) mfZ[IX(d, k - 1, i, j)] = 0.0;
   if (k == d.KE) mfZ[p] = 0.0;
@@ -291,18 +317,18 @@ __global__ void __launch_bounds__(256) hevi_explicit(HeviArgs a, unsigned ni_til
     else mfZ[IX(d, k, i - 1, j)] = 0.0;
     if (k == d.KS) mfZ[IX(d, k - 1, i - 1, j)] = 0.0;
   }
-  const double M12 = m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY);
   {
     double advcv = -(mZ - mZm) * RCDZ[k];
     double advch = -((mX - mXm) * RCDX[i] + (mY - mYm) * RCDY[j]);
-    a.Sr[p] = (advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p) + a.DENS_t[p];
+    r.Sr = (advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p) + a.DENS_t[p];
   }
 
   // ---- thermodynamics (HEVI.F90:588-657) ----
   {
     const double tX = o.tflxX(k, i, j, mX), tXm = o.tflxX(k, i - 1, j, mXm);
     const double tY = o.tflxY(k, i, j, mY), tYm = o.tflxY(k, i, j - 1, mYm);
-    const double tZ = o.tflxZ(k, i, j), tZm = o.tflxZ(k - 1, i, j);
+    const double tZ = o.tflxZw(k, i, j, mZ, zc.pw), tZm = zc.tZ;
+    zc.tZ = tZ;
     tfX[p] = tX;
     tfY[p] = tY;
     if (i == d.IS) tfX[IX(d, k, i - 1, j)] = tXm;
@@ -312,14 +338,18 @@ __global__ void __launch_bounds__(256) hevi_explicit(HeviArgs a, unsigned ni_til
     if (k == d.KE) tfZ[p] = 0.0;
     double advcv = -(tZ - tZm) * RCDZ[k];
     double advch = -((tX - tXm) * RCDX[i] + (tY - tYm) * RCDY[j]);
-    a.St[p] = (advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p) + a.RHOT_t[p];
+    r.St = (advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p) + a.RHOT_t[p];
   }
 
   // ---- momentum z, explicit part (HEVI.F90:486-577) and PT (HEVI.F90:691) ----
   if (k <= d.KE - 1) {
-    double sv = o.qZ_XYW(k, i, j) - o.qZ_XYW(k - 1, i, j);
+    const double qw = o.qZ_XYW(k, i, j);
+    double sv = qw - zc.qW;
+    zc.qW = qw;
     if constexpr (TERRAIN) {
-      sv = sv + o.qJ_XYW(0, k, i, j) - o.qJ_XYW(0, k - 1, i, j) + o.qJ_XYW(1, k, i, j) - o.qJ_XYW(1, k - 1, i, j);
+      const double q13 = o.qJ_XYW(0, k, i, j), q23 = o.qJ_XYW(1, k, i, j);
+      sv = sv + q13 - zc.qWJ13 + q23 - zc.qWJ23;
+      zc.qWJ13 = q13; zc.qWJ23 = q23;
     }
     double advcv = -(sv) * RFDZ[k];
     double advch = -((o.qH_XYW(0, k, i, j) - o.qH_XYW(0, k, i - 1, j)) * RCDX[i]
@@ -328,16 +358,19 @@ __global__ void __launch_bounds__(256) hevi_explicit(HeviArgs a, unsigned ni_til
     double wdmp = -a.wdamp_coef[k] * a.MOMZ0[p];
     double sw = (advcv + advch) / m.gs(I_XYW, p) + wdmp;
     if constexpr (DIVDMP) sw = sw + a.divdmp_coef / a.dtrk * (a.DDIV[IX(d, k + 1, i, j)] - a.DDIV[p]) * a.g.FDZ[k];
-    a.Sw[p] = sw + a.MOMZ_t[p];
-    a.PT[p] = o.PTf(k, i, j);
+    r.Sw = sw + a.MOMZ_t[p];
+    r.PT = o.PTfw(k, i, j, zc.pw);
   }
-  (void)M12;
 
   // ---- momentum x (HEVI.F90:816-927), iee = min(IIE,IEH) = IE for periodic x ----
   {
-    double sv = o.qZ_UV(0, k, i, j) - o.qZ_UV(0, k - 1, i, j);
+    const double qu = o.qZ_UV(0, k, i, j);
+    double sv = qu - zc.qU;
+    zc.qU = qu;
     if constexpr (TERRAIN) {
-      sv = sv + o.qJ_UV(0, 0, k, i, j) - o.qJ_UV(0, 0, k - 1, i, j) + o.qJ_UV(0, 1, k, i, j) - o.qJ_UV(0, 1, k - 1, i, j);
+      const double q13 = o.qJ_UV(0, 0, k, i, j), q23 = o.qJ_UV(0, 1, k, i, j);
+      sv = sv + q13 - zc.qUJ13 + q23 - zc.qUJ23;
+      zc.qUJ13 = q13; zc.qUJ23 = q23;
     }
     double advcv = -(sv) * RCDZ[k];
     double advch = -((o.qX_UYZ(k, i, j) - o.qX_UYZ(k, i - 1, j)) * RFDX[i]
@@ -377,9 +410,13 @@ __global__ void __launch_bounds__(256) hevi_explicit(HeviArgs a, unsigned ni_til
 
   // ---- momentum y (HEVI.F90:936-1047) ----
   {
-    double sv = o.qZ_UV(1, k, i, j) - o.qZ_UV(1, k - 1, i, j);
+    const double qv = o.qZ_UV(1, k, i, j);
+    double sv = qv - zc.qV;
+    zc.qV = qv;
     if constexpr (TERRAIN) {
-      sv = sv + o.qJ_UV(1, 0, k, i, j) - o.qJ_UV(1, 0, k - 1, i, j) + o.qJ_UV(1, 1, k, i, j) - o.qJ_UV(1, 1, k - 1, i, j);
+      const double q13 = o.qJ_UV(1, 0, k, i, j), q23 = o.qJ_UV(1, 1, k, i, j);
+      sv = sv + q13 - zc.qVJ13 + q23 - zc.qVJ23;
+      zc.qVJ13 = q13; zc.qVJ23 = q23;
     }
     double advcv = -(sv) * RCDZ[k];
     double advch = -((o.qX_XVZ(k, i, j) - o.qX_XVZ(k, i - 1, j)) * RCDX[i]
@@ -416,19 +453,27 @@ __global__ void __launch_bounds__(256) hevi_explicit(HeviArgs a, unsigned ni_til
     }
     a.MOMY_RK[p] = a.MOMY0[p] + a.dtrk * (t + a.MOMY_t[p]);
   }
+  return r;
 }
 
 // ---------------------------------------------------------------------------
-// vertical implicit solve, one thread per column (HEVI.F90:688-804, 1092-1127)
-template <bool TERRAIN>
-__global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles, unsigned nblocks) {
+// One thread per column (i,j), marching k upward through the explicit part
+// and the forward sweep of the Thomas algorithm (HEVI.F90:688-727,
+// 1092-1115), then downward through the back substitution and the implicit
+// updates (HEVI.F90:728-804, 1116-1127).  Row r of the tridiagonal system
+// needs the tendencies of level r+1, so the sweep runs one level behind the
+// explicit part.  Between the two passes the column keeps e, f and PT in
+// scratch and Sr/St in the DENS_RK/RHOT_RK outputs; the explicit z fluxes
+// wait in mflx_hi/tflx_hi for their implicit parts.
+template <int S, bool TERRAIN, bool DIVDMP>
+__global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles, unsigned nblocks) {
   const Dims& d = a.d;
+  Ops<S, TERRAIN, DIVDMP> o(a);
   const unsigned lb = xcd_block(blockIdx.x, nblocks);
-  const int i = d.IS + (int)((lb % ni_tiles) * 256 + threadIdx.x);
-  const int j = d.JS + (int)(lb / ni_tiles);
+  const int i = d.IS + (int)((lb % ni_tiles) * 64 + threadIdx.x);
+  const int j = d.JS + (int)((lb / ni_tiles) * blockDim.y + threadIdx.y);
   if (i > d.IE || j > d.JE) return;
-  Met<TERRAIN> m;
-  m.G = a.GSQRT; m.J13 = a.J13G; m.J23 = a.J23G; m.MAPF = a.MAPF; m.n3 = d.n3; m.pl = d.pl; m.IA = d.IA;
+  const auto& m = o.m;
   const double* CDZ = a.g.CDZ; const double* RCDZ = a.g.RCDZ; const double* RFDZ = a.g.RFDZ;
   const double dtrk = a.dtrk, J33G = a.J33G;
   const int KS = d.KS, KE = d.KE;
@@ -438,40 +483,55 @@ __global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles,
   auto f2h = [&](int k, int pp) {
     return CDZ[k + pp - 1] * m.gs(I_XYZ, px(k + pp - 1)) / (CDZ[k] * m.gs(I_XYZ, px(k)) + CDZ[k + 1] * m.gs(I_XYZ, px(k + 1)));
   };
-  // level k values
-  double rt0 = a.RT2P[px(KS)], dp0 = dpres(KS), st0 = a.St[px(KS)], sr0 = a.Sr[px(KS)];
-  double d0 = a.DENS[px(KS)], rd0 = a.REF_dens[px(KS)], A0 = Af(KS, rt0), pt0 = a.PT[px(KS)], ptm = 0.0;
+  double* Sr_ = a.DENS_RK;  // Sr/St wait in the outputs until the back pass
+  double* St_ = a.RHOT_RK;
+
+  // ---- upward: explicit part at level k, sweep row k-1 ----
+  double rt0 = 0.0, dp0 = 0.0, st0 = 0.0, sr0 = 0.0, d0 = 0.0, rd0 = 0.0, A0 = 0.0, pt0 = 0.0, ptm = 0.0, sw0 = 0.0;
   double e_prev = 0.0, f_prev = 0.0, C_last = 0.0;
-  for (int k = KS; k <= KE - 1; ++k) {
-    const size_t q = px(k), q1 = px(k + 1);
-    const double rt1 = a.RT2P[q1], dp1 = dpres(k + 1), st1 = a.St[q1], sr1 = a.Sr[q1];
-    const double d1 = a.DENS[q1], rd1 = a.REF_dens[q1], A1 = Af(k + 1, rt1);
-    const double pt1 = (k + 1 <= KE - 1) ? a.PT[q1] : 0.0;
-    const double gw = m.gs(I_XYW, q);
-    const double B = GRAV * (dtrk * dtrk) * J33G / (CDZ[k + 1] + CDZ[k]);
-    const double F1v = -(pt1 * RFDZ[k] * A1 + B) / gw;
-    const double F2v = 1.0 + (pt0 * RFDZ[k] * (A1 + A0)) / gw;
-    const double F3v = -(ptm * RFDZ[k] * A0 - B) / gw;
-    const double pg = -(dp1 + rt1 * dtrk * st1 - dp0 - rt0 * dtrk * st0) * RFDZ[k] * J33G / gw
-                    - GRAV * (f2h(k, 1) * (d1 - rd1 + sr1 * dtrk) + f2h(k, 2) * (d0 - rd0 + sr0 * dtrk));
-    const double Ck = a.MOMZ[q] + dtrk * (pg + a.Sw[q]);
-    if (k == KS) {
-      const double rdenom = 1.0 / F2v;
-      e_prev = -F1v * rdenom;
-      f_prev = Ck * rdenom;
-      a.e[q] = e_prev; a.f[q] = f_prev;
-    } else if (k < KE - 1) {
-      const double rdenom = 1.0 / (F2v + F3v * e_prev);
-      const double ek = -F1v * rdenom;
-      const double fk = (Ck - F3v * f_prev) * rdenom;
-      e_prev = ek; f_prev = fk;
-      a.e[q] = ek; a.f[q] = fk;
-    } else {
-      C_last = (Ck - F3v * f_prev) / (F2v + F3v * e_prev);
+  ZCarry zc;
+  zc.pw[1] = o.POTT(KS - 1, i, j); zc.pw[2] = o.POTT(KS, i, j); zc.pw[3] = o.POTT(KS + 1, i, j);
+  for (int k = KS; k <= KE; ++k) {
+    zc.pw[0] = zc.pw[1]; zc.pw[1] = zc.pw[2]; zc.pw[2] = zc.pw[3];
+    zc.pw[3] = o.POTT(k + 2, i, j);
+    const PointTend t = explicit_point<S, TERRAIN, DIVDMP>(o, a, k, i, j, zc);
+    const size_t q1 = px(k);
+    Sr_[q1] = t.Sr;
+    St_[q1] = t.St;
+    if (k <= KE - 1) a.PT[q1] = t.PT;
+    const double rt1 = a.RT2P[q1], dp1 = dpres(k), d1 = a.DENS[q1], rd1 = a.REF_dens[q1], A1 = Af(k, rt1);
+    const double pt1 = (k <= KE - 1) ? t.PT : 0.0;
+    if (k > KS) {
+      const int r = k - 1;
+      const size_t q = px(r);
+      const double gw = m.gs(I_XYW, q);
+      const double B = GRAV * (dtrk * dtrk) * J33G / (CDZ[r + 1] + CDZ[r]);
+      const double F1v = -(pt1 * RFDZ[r] * A1 + B) / gw;
+      const double F2v = 1.0 + (pt0 * RFDZ[r] * (A1 + A0)) / gw;
+      const double F3v = -(ptm * RFDZ[r] * A0 - B) / gw;
+      const double pg = -(dp1 + rt1 * dtrk * t.St - dp0 - rt0 * dtrk * st0) * RFDZ[r] * J33G / gw
+                      - GRAV * (f2h(r, 1) * (d1 - rd1 + t.Sr * dtrk) + f2h(r, 2) * (d0 - rd0 + sr0 * dtrk));
+      const double Ck = a.MOMZ[q] + dtrk * (pg + sw0);
+      if (r == KS) {
+        const double rdenom = 1.0 / F2v;
+        e_prev = -F1v * rdenom;
+        f_prev = Ck * rdenom;
+        a.e[q] = e_prev; a.f[q] = f_prev;
+      } else if (r < KE - 1) {
+        const double rdenom = 1.0 / (F2v + F3v * e_prev);
+        const double ek = -F1v * rdenom;
+        const double fk = (Ck - F3v * f_prev) * rdenom;
+        e_prev = ek; f_prev = fk;
+        a.e[q] = ek; a.f[q] = fk;
+      } else {
+        C_last = (Ck - F3v * f_prev) / (F2v + F3v * e_prev);
+      }
+      ptm = pt0;
     }
-    rt0 = rt1; dp0 = dp1; st0 = st1; sr0 = sr1; d0 = d1; rd0 = rd1; A0 = A1; ptm = pt0; pt0 = pt1;
+    rt0 = rt1; dp0 = dp1; st0 = t.St; sr0 = t.Sr; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = t.Sw;
   }
-  // back substitution and updates (HEVI.F90:728-790)
+
+  // ---- downward: back substitution and updates (HEVI.F90:728-790) ----
   double* mfZ = a.mflx_hi; double* tfZ = a.tflx_hi;
   const double M12 = m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY);
   double Cn = C_last;                         // C(k+1)
@@ -479,9 +539,9 @@ __global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles,
   {
     const size_t q = px(KE);
     double advcv = Cn * J33G * RCDZ[KE] / m.gs(I_XYZ, q);
-    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + a.Sr[q]);
+    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + Sr_[q]);
     advcv = Cn * ptn * J33G * RCDZ[KE] / m.gs(I_XYZ, q);
-    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + a.St[q]);
+    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + St_[q]);
     a.MOMZ_RK[q] = 0.0;
   }
   {
@@ -499,17 +559,17 @@ __global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles,
     a.MOMZ_RK[q] = a.MOMZ0[q] + (Ck - a.MOMZ[q]);
     // level k+1 (KS+1..KE-1) now has both C(k+1) and C(k)
     double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
-    a.DENS_RK[q1] = a.DENS0[q1] + dtrk * (advcv + a.Sr[q1]);
+    a.DENS_RK[q1] = a.DENS0[q1] + dtrk * (advcv + Sr_[q1]);
     advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
-    a.RHOT_RK[q1] = a.RHOT0[q1] + dtrk * (advcv + a.St[q1]);
+    a.RHOT_RK[q1] = a.RHOT0[q1] + dtrk * (advcv + St_[q1]);
     Cn = Ck; ptn = ptk;
   }
   {
     const size_t q = px(KS);
     double advcv = -Cn * J33G * RCDZ[KS] / m.gs(I_XYZ, q);
-    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + a.Sr[q]);
+    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + Sr_[q]);
     advcv = -Cn * ptn * J33G * RCDZ[KS] / m.gs(I_XYZ, q);
-    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + a.St[q]);
+    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + St_[q]);
     a.MOMZ_RK[px(KS - 1)] = 0.0;
   }
 }
@@ -518,19 +578,13 @@ __global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles,
 template <int S, bool TERRAIN, bool DIVDMP>
 static hipError_t launch_t(const HeviArgs& a, hipStream_t s) {
   const Dims& d = a.d;
-  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
-  const unsigned ni = (imax + 63) / 64, nk = (kmax + 3) / 4;
-  const unsigned nb = ni * nk * (unsigned)jmax;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
+  constexpr unsigned TJ = 4;
+  const unsigned ni = (imax + 63) / 64, nj = (jmax + TJ - 1) / TJ;
+  const unsigned nb = ni * nj;
   const Ctx& c = *a.ctx;
-  prof_start(c, "hevi_explicit", s);
-  hipLaunchKernelGGL((hevi_explicit<S, TERRAIN, DIVDMP>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
-  prof_stop(c, s);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  const unsigned ns = (imax + 255) / 256;
-  const unsigned nbs = ns * (unsigned)jmax;
-  prof_start(c, "hevi_solve", s);
-  hipLaunchKernelGGL((hevi_solve<TERRAIN>), dim3(nbs), dim3(256), 0, s, a, ns, nbs);
+  prof_start(c, "hevi_stage", s);
+  hipLaunchKernelGGL((hevi_march<S, TERRAIN, DIVDMP>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   prof_stop(c, s);
   return hipGetLastError();
 }
@@ -550,7 +604,7 @@ hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
   a.divdmp_coef = io.divdmp_coef; a.dtrk = io.dtrk; a.has_coriolis = io.has_coriolis;
   a.DENS_RK = io.o[0]; a.MOMZ_RK = io.o[1]; a.MOMX_RK = io.o[2]; a.MOMY_RK = io.o[3]; a.RHOT_RK = io.o[4];
   a.mflx_hi = io.mflx_hi; a.tflx_hi = io.tflx_hi;
-  a.Sr = c.scr[0]; a.St = c.scr[1]; a.Sw = c.scr[2]; a.PT = c.scr[3]; a.e = c.scr[4]; a.f = c.scr[5];
+  a.PT = c.scr[3]; a.e = c.scr[4]; a.f = c.scr[5];
   const bool div = io.divdmp_coef > 0.0;
   const bool ter = c.terrain;
   if (c.scheme == UD3) {
