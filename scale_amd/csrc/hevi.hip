@@ -12,6 +12,8 @@
 // the Thomas forward sweep (HEVI.F90:688-727, 1092-1115); marching back down
 // it substitutes into MOMZ/DENS/RHOT and adds the implicit parts of the z
 // fluxes (HEVI.F90:728-804, 1116-1127).
+#include <stdlib.h>
+#include <string.h>
 #include "common.h"
 #include "shp_internal.h"
 
@@ -33,7 +35,7 @@ struct HeviArgs {
   // outputs
   double *DENS_RK, *MOMZ_RK, *MOMX_RK, *MOMY_RK, *RHOT_RK, *mflx_hi, *tflx_hi;
   // scratch (n3 each)
-  double *PT, *e, *f;
+  double *Sr, *St, *Sw, *PT, *e, *f;
 };
 
 template <int S, bool TERRAIN, bool DIVDMP>
@@ -276,7 +278,11 @@ struct Ops {
 // (HEVI.F90:346-657, 816-1047).  Fluxes needed on both faces of a cell are
 // evaluated with the same expression by every point that uses them, so
 // neighbours see identical values.
-struct PointTend { double Sr, St, Sw, PT; };
+struct PointTend {
+  double Sr, St, Sw, PT;                                  // tendencies and PT for the solve
+  double mX, mXm, mY, mYm, mZ, mZw, tX, tXm, tY, tYm, tZ;  // fluxes to store
+  double MX, MY;                                          // MOMX_RK, MOMY_RK
+};
 
 // Fluxes through the lower face of the current level, carried from the level
 // below (where they were the upper face), and the column's theta window
@@ -297,26 +303,15 @@ __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP
   const double* RCDX = a.g.RCDX; const double* RCDY = a.g.RCDY; const double* RCDZ = a.g.RCDZ;
   const double* RFDX = a.g.RFDX; const double* RFDY = a.g.RFDY; const double* RFDZ = a.g.RFDZ;
   const auto& m = o.m;
-  double* mfZ = a.mflx_hi; double* mfX = a.mflx_hi + d.n3; double* mfY = a.mflx_hi + 2 * d.n3;
-  double* tfZ = a.tflx_hi; double* tfX = a.tflx_hi + d.n3; double* tfY = a.tflx_hi + 2 * d.n3;
 
   // ---- continuity (HEVI.F90:369-475) ----
   const double mX = o.mflxX(k, i, j), mXm = o.mflxX(k, i - 1, j);
   const double mY = o.mflxY(k, i, j), mYm = o.mflxY(k, i, j - 1);
-  mfX[p] = mX;
-  mfY[p] = mY;
-  if (i == d.IS) mfX[IX(d, k, i - 1, j)] = mXm;
-  if (j == d.JS) mfY[IX(d, k, i, j - 1)] = mYm;
+  r.mX = mX; r.mXm = mXm; r.mY = mY; r.mYm = mYm;
   const double mZ = o.mflxZf(k, i, j), mZm = zc.mZ;
   zc.mZ = mZ;
-  if (k <= d.KE - 1) mfZ[p] = mZ;
-  if (k == d.KS) mfZ[IX(d, k - 1, i, j)] = 0.0;
-  if (k == d.KE) mfZ[p] = 0.0;
-  if (i == d.IS) {  // mflx_hi(Z) is also defined on the i = IS-1 column (HEVI.F90:370)
-    if (k <= d.KE - 1) mfZ[IX(d, k, i - 1, j)] = o.mflxZ(k, i - 1, j);
-    else mfZ[IX(d, k, i - 1, j)] = 0.0;
-    if (k == d.KS) mfZ[IX(d, k - 1, i - 1, j)] = 0.0;
-  }
+  r.mZ = mZ;
+  r.mZw = (i == d.IS && k <= d.KE - 1) ? o.mflxZ(k, i - 1, j) : 0.0;
   {
     double advcv = -(mZ - mZm) * RCDZ[k];
     double advch = -((mX - mXm) * RCDX[i] + (mY - mYm) * RCDY[j]);
@@ -329,13 +324,7 @@ __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP
     const double tY = o.tflxY(k, i, j, mY), tYm = o.tflxY(k, i, j - 1, mYm);
     const double tZ = o.tflxZw(k, i, j, mZ, zc.pw), tZm = zc.tZ;
     zc.tZ = tZ;
-    tfX[p] = tX;
-    tfY[p] = tY;
-    if (i == d.IS) tfX[IX(d, k, i - 1, j)] = tXm;
-    if (j == d.JS) tfY[IX(d, k, i, j - 1)] = tYm;
-    if (k <= d.KE - 1) tfZ[p] = tZ;
-    if (k == d.KS) tfZ[IX(d, k - 1, i, j)] = 0.0;
-    if (k == d.KE) tfZ[p] = 0.0;
+    r.tX = tX; r.tXm = tXm; r.tY = tY; r.tYm = tYm; r.tZ = tZ;
     double advcv = -(tZ - tZm) * RCDZ[k];
     double advch = -((tX - tXm) * RCDX[i] + (tY - tYm) * RCDY[j]);
     r.St = (advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p) + a.RHOT_t[p];
@@ -405,7 +394,7 @@ __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP
       t = t + a.divdmp_coef / a.dtrk * (a.DDIV[IX(d, k, i + 1, j)] / m.mapf(i + 1, j, 1, I_XY) - a.DDIV[p] / m.mapf(i, j, 0, I_XY))
                 * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY) * a.g.FDX[i];
     }
-    a.MOMX_RK[p] = a.MOMX0[p] + a.dtrk * (t + a.MOMX_t[p]);
+    r.MX = a.MOMX0[p] + a.dtrk * (t + a.MOMX_t[p]);
   }
 
   // ---- momentum y (HEVI.F90:936-1047) ----
@@ -451,9 +440,34 @@ __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP
       t = t + a.divdmp_coef / a.dtrk * (a.DDIV[IX(d, k, i, j + 1)] / m.mapf(i, j + 1, 0, I_XY) - a.DDIV[p] / m.mapf(i, j, 0, I_XY))
                 * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV) * a.g.FDY[j];
     }
-    a.MOMY_RK[p] = a.MOMY0[p] + a.dtrk * (t + a.MOMY_t[p]);
+    r.MY = a.MOMY0[p] + a.dtrk * (t + a.MOMY_t[p]);
   }
   return r;
+}
+
+// The stores of one point, issued after all of its loads: the mass and
+// theta fluxes (with the extra faces i = IS-1, j = JS-1 and the zero z faces
+// KS-1 and KE, HEVI.F90:370-438, 596-622) and MOMX_RK/MOMY_RK.
+__device__ __forceinline__ void store_point(const HeviArgs& a, int k, int i, int j, const PointTend& r) {
+  const Dims& d = a.d;
+  const size_t p = IX(d, k, i, j);
+  double* mfZ = a.mflx_hi; double* mfX = a.mflx_hi + d.n3; double* mfY = a.mflx_hi + 2 * d.n3;
+  double* tfZ = a.tflx_hi; double* tfX = a.tflx_hi + d.n3; double* tfY = a.tflx_hi + 2 * d.n3;
+  mfX[p] = r.mX;
+  mfY[p] = r.mY;
+  tfX[p] = r.tX;
+  tfY[p] = r.tY;
+  if (i == d.IS) { mfX[p - 1] = r.mXm; tfX[p - 1] = r.tXm; }
+  if (j == d.JS) { mfY[p - d.IA] = r.mYm; tfY[p - d.IA] = r.tYm; }
+  if (k <= d.KE - 1) { mfZ[p] = r.mZ; tfZ[p] = r.tZ; }
+  else { mfZ[p] = 0.0; tfZ[p] = 0.0; }
+  if (k == d.KS) { mfZ[p - d.pl] = 0.0; tfZ[p - d.pl] = 0.0; }
+  if (i == d.IS) {  // mflx_hi(Z) is also defined on the i = IS-1 column (HEVI.F90:370)
+    mfZ[p - 1] = r.mZw;
+    if (k == d.KS) mfZ[p - 1 - d.pl] = 0.0;
+  }
+  a.MOMX_RK[p] = r.MX;
+  a.MOMY_RK[p] = r.MY;
 }
 
 // ---------------------------------------------------------------------------
@@ -496,11 +510,9 @@ __global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles,
     zc.pw[3] = o.POTT(k + 2, i, j);
     const PointTend t = explicit_point<S, TERRAIN, DIVDMP>(o, a, k, i, j, zc);
     const size_t q1 = px(k);
-    Sr_[q1] = t.Sr;
-    St_[q1] = t.St;
-    if (k <= KE - 1) a.PT[q1] = t.PT;
     const double rt1 = a.RT2P[q1], dp1 = dpres(k), d1 = a.DENS[q1], rd1 = a.REF_dens[q1], A1 = Af(k, rt1);
     const double pt1 = (k <= KE - 1) ? t.PT : 0.0;
+    double ek = 0.0, fk = 0.0;
     if (k > KS) {
       const int r = k - 1;
       const size_t q = px(r);
@@ -514,20 +526,24 @@ __global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles,
       const double Ck = a.MOMZ[q] + dtrk * (pg + sw0);
       if (r == KS) {
         const double rdenom = 1.0 / F2v;
-        e_prev = -F1v * rdenom;
-        f_prev = Ck * rdenom;
-        a.e[q] = e_prev; a.f[q] = f_prev;
+        ek = -F1v * rdenom;
+        fk = Ck * rdenom;
       } else if (r < KE - 1) {
         const double rdenom = 1.0 / (F2v + F3v * e_prev);
-        const double ek = -F1v * rdenom;
-        const double fk = (Ck - F3v * f_prev) * rdenom;
-        e_prev = ek; f_prev = fk;
-        a.e[q] = ek; a.f[q] = fk;
+        ek = -F1v * rdenom;
+        fk = (Ck - F3v * f_prev) * rdenom;
       } else {
         C_last = (Ck - F3v * f_prev) / (F2v + F3v * e_prev);
       }
+      e_prev = ek; f_prev = fk;
       ptm = pt0;
     }
+    // stores of this level, after all of its loads
+    store_point(a, k, i, j, t);
+    Sr_[q1] = t.Sr;
+    St_[q1] = t.St;
+    if (k <= KE - 1) a.PT[q1] = t.PT;
+    if (k > KS && k - 1 < KE - 1) { a.e[px(k - 1)] = ek; a.f[px(k - 1)] = fk; }
     rt0 = rt1; dp0 = dp1; st0 = t.St; sr0 = t.Sr; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = t.Sw;
   }
 
@@ -575,14 +591,528 @@ __global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles,
 }
 
 // ---------------------------------------------------------------------------
+// Flat-terrain stage with LDS tiles: blocks of 64 (i) x TJ (j) columns march
+// k together.  At each level the block
+//   (a) stages DENS/MOMX/MOMY (levels k and k+1, a two-level ring), MOMZ,
+//       POTT = RHOT/DENS and DPRES at level k in LDS, with a 2-cell halo;
+//   (b) evaluates every x face and every y face of the tile once (mass and
+//       theta fluxes, the x/y fluxes of MOMZ, MOMX and MOMY) into LDS;
+//   (c) lets each column difference its faces, carry its z fluxes from the
+//       level below, update MOMX/MOMY and run the Thomas forward sweep one
+//       level behind, as hevi_march does; the back substitution follows.
+// With GSQRT = 1, J13G = J23G = 0 and MAPF = 1 every metric factor of the
+// general expressions is a multiplication or division by 1.0 (or an addition
+// of 0.0), so the expressions below, written without them, give the same
+// IEEE results (HEVI.F90:346-1047; UD3.F90/CD4.F90 flux routines).
+template <int S, bool DIVDMP, int TJ>
+__global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_tiles, unsigned nblocks) {
+  constexpr int TI = 64, CW = TI + 4, RW = TJ + 4, NT = TI * TJ;
+  __shared__ double sD[2][RW][CW], sMX[2][RW][CW], sMY[2][RW][CW];
+  __shared__ double sPO[RW][CW], sW[RW][CW], sDP[TJ + 1][TI + 1];
+  // x faces: position x = 0..TI is the face east of column i0-1+x
+  __shared__ double xM[TJ][TI + 1], xT[TJ][TI + 1], xW[TJ][TI + 1], xU[TJ][TI + 1], xV[TJ][TI + 1];
+  // y faces: position y = 0..TJ is the face north of row j0-1+y
+  __shared__ double yM[TJ + 1][TI], yT[TJ + 1][TI], yW[TJ + 1][TI], yU[TJ + 1][TI], yV[TJ + 1][TI];
+
+  const Dims& d = a.d;
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const int i0 = d.IS + (int)(lb % ni_tiles) * TI;
+  const int j0 = d.JS + (int)(lb / ni_tiles) * TJ;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = (i <= d.IE) && (j <= d.JE);
+  const int KS = d.KS, KE = d.KE;
+  const size_t n3 = d.n3;
+  const double* CDZ = a.g.CDZ; const double* RCDZ = a.g.RCDZ; const double* RFDZ = a.g.RFDZ;
+  const double* RCDX = a.g.RCDX; const double* RCDY = a.g.RCDY; const double* RFDX = a.g.RFDX; const double* RFDY = a.g.RFDY;
+  const double dtrk = a.dtrk, J33G = a.J33G;
+  const double* ND = a.num_diff;  // component (v, dir) at ND + n3*(v + 5*dir)
+  auto nd = [&](int v, int dir, size_t q) { return ND[q + n3 * (size_t)(v + 5 * dir)]; };
+  auto gix = [&](int k, int ii, int jj) { return IX(d, k, ii, jj); };
+
+  // (a) tile staging helpers: tile cell (r, c) is column i0-2+c, row j0-2+r
+  auto fill = [&](double (*dst)[CW], const double* src, int k) {
+    for (int q = tid; q < RW * CW; q += NT) {
+      const int r = q / CW, c = q - r * CW;
+      const int ii = i0 - 2 + c, jj = j0 - 2 + r;
+      dst[r][c] = (ii < d.IA && jj < d.JA) ? src[gix(k, ii, jj)] : 0.0;
+    }
+  };
+  int s0 = 0;  // ring slot of level k; slot s0^1 holds level k+1
+  fill(sD[0], a.DENS, KS); fill(sMX[0], a.MOMX, KS); fill(sMY[0], a.MOMY, KS);
+  __syncthreads();
+
+  // own-column windows at k-1..k+2 and carried z-face fluxes
+  const size_t p0 = own ? gix(0, i, j) : 0;
+  auto at = [&](const double* f, int k) { return f[p0 + d.pl * (size_t)k]; };
+  double pw[4] = {0.0, 0.0, 0.0, 0.0}, ww[4] = {0.0, 0.0, 0.0, 0.0}, xw[4] = {0.0, 0.0, 0.0, 0.0}, yw[4] = {0.0, 0.0, 0.0, 0.0};
+  if (own) {
+    for (int q = 1; q < 4; ++q) {
+      const int k = KS - 2 + q;
+      pw[q] = at(a.RHOT, k) / at(a.DENS, k);
+      ww[q] = at(a.MOMZ, k); xw[q] = at(a.MOMX, k); yw[q] = at(a.MOMY, k);
+    }
+  }
+  double cmZ = 0.0, ctZ = 0.0, cqW = 0.0, cqU = 0.0, cqV = 0.0;  // fluxes through face k-1 (zero at KS-1)
+  double rt0 = 0.0, dp0 = 0.0, st0 = 0.0, sr0 = 0.0, d0 = 0.0, rd0 = 0.0, A0 = 0.0, pt0 = 0.0, ptm = 0.0, sw0 = 0.0;
+  double e_prev = 0.0, f_prev = 0.0, C_last = 0.0;
+
+  for (int k = KS; k <= KE; ++k) {
+    const int s1 = s0 ^ 1;
+    // ---- (a) staging ----
+    fill(sD[s1], a.DENS, k + 1); fill(sMX[s1], a.MOMX, k + 1); fill(sMY[s1], a.MOMY, k + 1);
+    fill(sW, a.MOMZ, k);
+    for (int q = tid; q < RW * CW; q += NT) {
+      const int r = q / CW, c = q - r * CW;
+      const int ii = i0 - 2 + c, jj = j0 - 2 + r;
+      sPO[r][c] = (ii < d.IA && jj < d.JA) ? a.RHOT[gix(k, ii, jj)] / sD[s0][r][c] : 0.0;
+    }
+    for (int q = tid; q < (TJ + 1) * (TI + 1); q += NT) {
+      const int r = q / (TI + 1), c = q - r * (TI + 1);
+      const int ii = i0 + c, jj = j0 + r;
+      double v = 0.0;
+      if (ii < d.IA && jj < d.JA) {
+        const size_t g = gix(k, ii, jj);
+        v = a.DPRES0[g] + a.RT2P[g] * (a.RHOT[g] - a.REF_rhot[g]);
+      }
+      sDP[r][c] = v;
+    }
+    // own-column windows move up one level
+    if (own) {
+      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = at(a.RHOT, k + 2) / at(a.DENS, k + 2);
+      ww[0] = ww[1]; ww[1] = ww[2]; ww[2] = ww[3]; ww[3] = at(a.MOMZ, k + 2);
+      xw[0] = xw[1]; xw[1] = xw[2]; xw[2] = xw[3]; xw[3] = at(a.MOMX, k + 2);
+      yw[0] = yw[1]; yw[1] = yw[2]; yw[2] = yw[3]; yw[3] = at(a.MOMY, k + 2);
+    }
+    __syncthreads();
+
+    // ---- (b) faces ----
+    const double f1 = CDZ[k] / (CDZ[k] + CDZ[k + 1]);      // F2H(k,1)
+    const double f2 = CDZ[k + 1] / (CDZ[k] + CDZ[k + 1]);  // F2H(k,2)
+    auto D0 = [&](int r, int c) { return sD[s0][r][c]; };
+    auto D1 = [&](int r, int c) { return sD[s1][r][c]; };
+    for (int q = tid; q < TJ * (TI + 1); q += NT) {
+      const int row = q / (TI + 1), x = q - row * (TI + 1);
+      const int ii = i0 - 1 + x, jj = j0 + row;
+      if (ii > d.IE || jj > d.JE) continue;
+      const int r = row + 2, c = x + 1;  // tile cell of column ii
+      const size_t g = gix(k, ii, jj);
+      const double vm = sMX[s0][r][c] + nd(I_DENS, XDIR, g);
+      xM[row][x] = vm;
+      xT[row][x] = vm * fv_int<S>(sPO[r][c - 1], sPO[r][c], sPO[r][c + 1], sPO[r][c + 2], vm) + nd(I_RHOT, XDIR, g);
+      double qw = 0.0;
+      if (k < KE) {
+        const double vel = (f1 * sMX[s1][r][c] + f2 * sMX[s0][r][c])
+                         / (f1 * 0.5 * (D1(r, c) + D1(r, c + 1)) + f2 * 0.5 * (D0(r, c) + D0(r, c + 1)));
+        qw = vel * fv_int<S>(sW[r][c - 1], sW[r][c], sW[r][c + 1], sW[r][c + 2], vel) + nd(I_MOMZ, XDIR, g);
+      }
+      xW[row][x] = qw;
+      {  // fluxX_UYZ stored at ii (loop index ii+1)
+        const double vel = (0.5 * (sMX[s0][r][c + 1] + sMX[s0][r][c])) / D0(r, c + 1);
+        xU[row][x] = vel * fv_int<S>(sMX[s0][r][c - 1], sMX[s0][r][c], sMX[s0][r][c + 1], sMX[s0][r][c + 2], vel)
+                   + nd(I_MOMX, XDIR, g + 1);
+      }
+      {  // fluxX_XVZ at (ii, jj)
+        const double vel = (0.5 * (sMX[s0][r][c] + sMX[s0][r + 1][c]))
+                         / (0.25 * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1)));
+        xV[row][x] = vel * fv_int<S>(sMY[s0][r][c - 1], sMY[s0][r][c], sMY[s0][r][c + 1], sMY[s0][r][c + 2], vel)
+                   + nd(I_MOMY, XDIR, g);
+      }
+      if (x > 0 || i0 == d.IS) {  // faces IS-1 and those owned by this tile
+        a.mflx_hi[g + n3] = vm;
+        a.tflx_hi[g + n3] = xT[row][x];
+      }
+    }
+    for (int q = tid; q < (TJ + 1) * TI; q += NT) {
+      const int y = q / TI, col = q - y * TI;
+      const int ii = i0 + col, jj = j0 - 1 + y;
+      if (ii > d.IE || jj > d.JE) continue;
+      const int r = y + 1, c = col + 2;  // tile cell of row jj
+      const size_t g = gix(k, ii, jj);
+      const double vm = sMY[s0][r][c] + nd(I_DENS, YDIR, g);
+      yM[y][col] = vm;
+      yT[y][col] = vm * fv_int<S>(sPO[r - 1][c], sPO[r][c], sPO[r + 1][c], sPO[r + 2][c], vm) + nd(I_RHOT, YDIR, g);
+      double qw = 0.0;
+      if (k < KE) {
+        const double vel = (f1 * sMY[s1][r][c] + f2 * sMY[s0][r][c])
+                         / (f1 * 0.5 * (D1(r, c) + D1(r + 1, c)) + f2 * 0.5 * (D0(r, c) + D0(r + 1, c)));
+        qw = vel * fv_int<S>(sW[r - 1][c], sW[r][c], sW[r + 1][c], sW[r + 2][c], vel) + nd(I_MOMZ, YDIR, g);
+      }
+      yW[y][col] = qw;
+      {  // fluxY_UYZ at (ii, jj)
+        const double vel = (0.5 * (sMY[s0][r][c] + sMY[s0][r][c + 1]))
+                         / (0.25 * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1)));
+        yU[y][col] = vel * fv_int<S>(sMX[s0][r - 1][c], sMX[s0][r][c], sMX[s0][r + 1][c], sMX[s0][r + 2][c], vel)
+                   + nd(I_MOMX, YDIR, g);
+      }
+      {  // fluxY_XVZ stored at jj (loop index jj+1)
+        const double vel = (0.5 * (sMY[s0][r + 1][c] + sMY[s0][r][c])) / D0(r + 1, c);
+        yV[y][col] = vel * fv_int<S>(sMY[s0][r - 1][c], sMY[s0][r][c], sMY[s0][r + 1][c], sMY[s0][r + 2][c], vel)
+                   + nd(I_MOMY, YDIR, g + d.IA);
+      }
+      if (y > 0 || j0 == d.JS) {
+        a.mflx_hi[g + 2 * n3] = vm;
+        a.tflx_hi[g + 2 * n3] = yT[y][col];
+      }
+    }
+    __syncthreads();
+
+    // ---- (c) the column ----
+    if (own) {
+      const int r = ty + 2, c = tx + 2;  // tile cell of the own column
+      const size_t g = gix(k, i, j);
+      // continuity and theta (HEVI.F90:369-475, 588-657)
+      const double mZ = (k <= KE - 1) ? nd(I_DENS, ZDIR, g) : 0.0;
+      const double Sr = (-(mZ - cmZ) * RCDZ[k] + -((xM[ty][tx + 1] - xM[ty][tx]) * RCDX[i] + (yM[ty + 1][tx] - yM[ty][tx]) * RCDY[j]))
+                      + a.DENS_t[g];
+      double tZ = 0.0;
+      if (k <= KE - 1) {
+        double fv;
+        if (k == KS) fv = fv_lo<S>(pw[1], pw[2], pw[3], mZ);
+        else if (k == KE - 1) fv = fv_hi<S>(pw[0], pw[1], pw[2], mZ);
+        else fv = fv_int<S>(pw[0], pw[1], pw[2], pw[3], mZ);
+        tZ = mZ * fv + nd(I_RHOT, ZDIR, g);
+      }
+      const double St = (-(tZ - ctZ) * RCDZ[k] + -((xT[ty][tx + 1] - xT[ty][tx]) * RCDX[i] + (yT[ty + 1][tx] - yT[ty][tx]) * RCDY[j]))
+                      + a.RHOT_t[g];
+      // momentum z, explicit part (HEVI.F90:486-577) and PT
+      double Sw = 0.0, PT = 0.0;
+      if (k <= KE - 1) {
+        double qw = 0.0;
+        if (k < KE - 1) {
+          const double vel = (0.5 * (ww[1] + ww[2])) / D1(r, c);
+          const double fv = (k == KS) ? fv_lo<S>(ww[1], ww[2], ww[3], vel) : fv_int<S>(ww[0], ww[1], ww[2], ww[3], vel);
+          qw = J33G * vel * fv + nd(I_MOMZ, ZDIR, g + d.pl);
+        }
+        const double advcv = -(qw - cqW) * RFDZ[k];
+        cqW = qw;
+        const double advch = -((xW[ty][tx + 1] - xW[ty][tx]) * RCDX[i] + (yW[ty + 1][tx] - yW[ty][tx]) * RCDY[j]);
+        const double wdc = a.wdamp_coef[k];
+        const double wdmp = (wdc != 0.0) ? -wdc * a.MOMZ0[g] : -wdc * 0.0;
+        double sw = (advcv + advch) + wdmp;
+        if constexpr (DIVDMP) sw = sw + a.divdmp_coef / dtrk * (a.DDIV[g + d.pl] - a.DDIV[g]) * a.g.FDZ[k];
+        Sw = sw + a.MOMZ_t[g];
+        const double vel = ww[1];
+        if (k == KS) PT = fv_lo<S>(pw[1], pw[2], pw[3], vel);
+        else if (k == KE - 1) PT = fv_hi<S>(pw[0], pw[1], pw[2], vel);
+        else PT = fv_int<S>(pw[0], pw[1], pw[2], pw[3], vel);
+      }
+      // momentum x (HEVI.F90:816-927)
+      double MXn;
+      {
+        double qu = 0.0;
+        if (k < KE) {
+          const double vel = (0.5 * (ww[1] + sW[r][c + 1]))
+                           / (f1 * 0.5 * (D1(r, c) + D1(r, c + 1)) + f2 * 0.5 * (D0(r, c) + D0(r, c + 1)));
+          double fv;
+          if (k == KS) fv = fv_lo<S>(xw[1], xw[2], xw[3], vel);
+          else if (k == KE - 1) fv = fv_hi<S>(xw[0], xw[1], xw[2], vel);
+          else fv = fv_int<S>(xw[0], xw[1], xw[2], xw[3], vel);
+          qu = J33G * vel * fv + nd(I_MOMX, ZDIR, g);
+        }
+        const double advcv = -(qu - cqU) * RCDZ[k];
+        cqU = qu;
+        const double advch = -((xU[ty][tx + 1] - xU[ty][tx]) * RFDX[i] + (yU[ty + 1][tx] - yU[ty][tx]) * RCDY[j]);
+        const double pg = (sDP[ty][tx + 1] - sDP[ty][tx]) * RFDX[i];
+        double t = advcv + advch - pg;
+        if (a.has_coriolis) {
+          const double cf = 0.125 * (a.CORIOLI[(size_t)(i + 1) + (size_t)d.IA * j] + a.CORIOLI[(size_t)i + (size_t)d.IA * j])
+                          * (sMY[s0][r][c + 1] + sMY[s0][r][c] + sMY[s0][r - 1][c + 1] + sMY[s0][r - 1][c]);
+          t = t + cf;
+        }
+        if constexpr (DIVDMP) t = t + a.divdmp_coef / dtrk * (a.DDIV[g + 1] - a.DDIV[g]) * a.g.FDX[i];
+        MXn = a.MOMX0[g] + dtrk * (t + a.MOMX_t[g]);
+      }
+      // momentum y (HEVI.F90:936-1047)
+      double MYn;
+      {
+        double qv = 0.0;
+        if (k < KE) {
+          const double vel = (0.5 * (ww[1] + sW[r + 1][c]))
+                           / (f1 * 0.5 * (D1(r, c) + D1(r + 1, c)) + f2 * 0.5 * (D0(r, c) + D0(r + 1, c)));
+          double fv;
+          if (k == KS) fv = fv_lo<S>(yw[1], yw[2], yw[3], vel);
+          else if (k == KE - 1) fv = fv_hi<S>(yw[0], yw[1], yw[2], vel);
+          else fv = fv_int<S>(yw[0], yw[1], yw[2], yw[3], vel);
+          qv = J33G * vel * fv + nd(I_MOMY, ZDIR, g);
+        }
+        const double advcv = -(qv - cqV) * RCDZ[k];
+        cqV = qv;
+        const double advch = -((xV[ty][tx + 1] - xV[ty][tx]) * RCDX[i] + (yV[ty + 1][tx] - yV[ty][tx]) * RFDY[j]);
+        const double pg = (sDP[ty + 1][tx] - sDP[ty][tx]) * RFDY[j];
+        double t = advcv + advch - pg;
+        if (a.has_coriolis) {
+          const double cf = -0.125 * (a.CORIOLI[(size_t)i + (size_t)d.IA * (j + 1)] + a.CORIOLI[(size_t)i + (size_t)d.IA * j])
+                          * (sMX[s0][r + 1][c] + sMX[s0][r][c] + sMX[s0][r + 1][c - 1] + sMX[s0][r][c - 1]);
+          t = t + cf;
+        }
+        if constexpr (DIVDMP) t = t + a.divdmp_coef / dtrk * (a.DDIV[g + d.IA] - a.DDIV[g]) * a.g.FDY[j];
+        MYn = a.MOMY0[g] + dtrk * (t + a.MOMY_t[g]);
+      }
+      // tridiagonal row k-1 and forward sweep (HEVI.F90:688-727, 1092-1115)
+      const double rt1 = a.RT2P[g], dp1 = sDP[ty][tx], d1 = D0(r, c), rd1 = a.REF_dens[g];
+      const double A1 = dtrk * dtrk * J33G * RCDZ[k] * rt1 * J33G;
+      const double pt1 = (k <= KE - 1) ? PT : 0.0;
+      double ek = 0.0, fk = 0.0;
+      if (k > KS) {
+        const int rr = k - 1;
+        const double B = GRAV * (dtrk * dtrk) * J33G / (CDZ[rr + 1] + CDZ[rr]);
+        const double F1v = -(pt1 * RFDZ[rr] * A1 + B);
+        const double F2v = 1.0 + (pt0 * RFDZ[rr] * (A1 + A0));
+        const double F3v = -(ptm * RFDZ[rr] * A0 - B);
+        const double h1 = CDZ[rr] / (CDZ[rr] + CDZ[rr + 1]), h2 = CDZ[rr + 1] / (CDZ[rr] + CDZ[rr + 1]);
+        const double pg = -(dp1 + rt1 * dtrk * St - dp0 - rt0 * dtrk * st0) * RFDZ[rr] * J33G
+                        - GRAV * (h1 * (d1 - rd1 + Sr * dtrk) + h2 * (d0 - rd0 + sr0 * dtrk));
+        const double Ck = ww[0] + dtrk * (pg + sw0);
+        if (rr == KS) {
+          const double rdenom = 1.0 / F2v;
+          ek = -F1v * rdenom;
+          fk = Ck * rdenom;
+        } else if (rr < KE - 1) {
+          const double rdenom = 1.0 / (F2v + F3v * e_prev);
+          ek = -F1v * rdenom;
+          fk = (Ck - F3v * f_prev) * rdenom;
+        } else {
+          C_last = (Ck - F3v * f_prev) / (F2v + F3v * e_prev);
+        }
+        e_prev = ek; f_prev = fk;
+        ptm = pt0;
+      }
+      // stores of this level
+      a.MOMX_RK[g] = MXn;
+      a.MOMY_RK[g] = MYn;
+      a.DENS_RK[g] = Sr;  // Sr/St wait in the outputs until the back pass
+      a.RHOT_RK[g] = St;
+      if (k <= KE - 1) { a.mflx_hi[g] = mZ; a.tflx_hi[g] = tZ; a.PT[g] = PT; }
+      else { a.mflx_hi[g] = 0.0; a.tflx_hi[g] = 0.0; }
+      if (k == KS) { a.mflx_hi[g - d.pl] = 0.0; a.tflx_hi[g - d.pl] = 0.0; }
+      if (i == d.IS) {  // mflx_hi(Z) on the i = IS-1 column (HEVI.F90:370)
+        a.mflx_hi[g - 1] = (k <= KE - 1) ? nd(I_DENS, ZDIR, g - 1) : 0.0;
+        if (k == KS) a.mflx_hi[g - 1 - d.pl] = 0.0;
+      }
+      if (k > KS && k - 1 < KE - 1) { a.e[g - d.pl] = ek; a.f[g - d.pl] = fk; }
+      cmZ = mZ; ctZ = tZ;
+      rt0 = rt1; dp0 = dp1; st0 = St; sr0 = Sr; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = Sw;
+    }
+    __syncthreads();
+    s0 = s1;
+  }
+  if (!own) return;
+
+  // ---- downward: back substitution and updates (HEVI.F90:728-790) ----
+  double* mfZ = a.mflx_hi; double* tfZ = a.tflx_hi;
+  auto px = [&](int k) { return p0 + d.pl * (size_t)k; };
+  double Cn = C_last;
+  double ptn = a.PT[px(KE - 1)];
+  {
+    const size_t q = px(KE);
+    double advcv = Cn * J33G * RCDZ[KE];
+    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + a.DENS_RK[q]);
+    advcv = Cn * ptn * J33G * RCDZ[KE];
+    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + a.RHOT_RK[q]);
+    a.MOMZ_RK[q] = 0.0;
+  }
+  {
+    const size_t q = px(KE - 1);
+    mfZ[q] = mfZ[q] + J33G * Cn;
+    tfZ[q] = tfZ[q] + J33G * Cn * ptn;
+    a.MOMZ_RK[q] = a.MOMZ0[q] + (Cn - a.MOMZ[q]);
+  }
+  for (int k = KE - 2; k >= KS; --k) {
+    const size_t q = px(k), q1 = px(k + 1);
+    const double Ck = a.e[q] * Cn + a.f[q];
+    const double ptk = a.PT[q];
+    mfZ[q] = mfZ[q] + J33G * Ck;
+    tfZ[q] = tfZ[q] + J33G * Ck * ptk;
+    a.MOMZ_RK[q] = a.MOMZ0[q] + (Ck - a.MOMZ[q]);
+    double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1];
+    a.DENS_RK[q1] = a.DENS0[q1] + dtrk * (advcv + a.DENS_RK[q1]);
+    advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1];
+    a.RHOT_RK[q1] = a.RHOT0[q1] + dtrk * (advcv + a.RHOT_RK[q1]);
+    Cn = Ck; ptn = ptk;
+  }
+  {
+    const size_t q = px(KS);
+    double advcv = -Cn * J33G * RCDZ[KS];
+    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + a.DENS_RK[q]);
+    advcv = -Cn * ptn * J33G * RCDZ[KS];
+    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + a.RHOT_RK[q]);
+    a.MOMZ_RK[px(KS - 1)] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Alternative organisation: one thread per point for the explicit part, the
+// tendencies and PT through scratch, then one thread per column for the
+// vertical solve (the same arithmetic as hevi_march).
+template <int S, bool TERRAIN, bool DIVDMP>
+__global__ void __launch_bounds__(256) hevi_point(HeviArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+  const Dims& d = a.d;
+  Ops<S, TERRAIN, DIVDMP> o(a);
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
+  const int i = d.IS + (int)(it * 64 + threadIdx.x);
+  const int k = d.KS + (int)(kt * 4 + threadIdx.y);
+  const int j = d.JS + (int)jt;
+  if (i > d.IE || k > d.KE || j > d.JE) return;
+  ZCarry zc;
+  zc.mZ = o.mflxZf(k - 1, i, j);
+  zc.pw[0] = o.POTT(k - 1, i, j); zc.pw[1] = o.POTT(k, i, j); zc.pw[2] = o.POTT(k + 1, i, j); zc.pw[3] = o.POTT(k + 2, i, j);
+  zc.tZ = (k == d.KS) ? 0.0 : o.tflxZ(k - 1, i, j);
+  zc.qW = o.qZ_XYW(k - 1, i, j);
+  zc.qU = o.qZ_UV(0, k - 1, i, j);
+  zc.qV = o.qZ_UV(1, k - 1, i, j);
+  if constexpr (TERRAIN) {
+    zc.qWJ13 = o.qJ_XYW(0, k - 1, i, j); zc.qWJ23 = o.qJ_XYW(1, k - 1, i, j);
+    zc.qUJ13 = o.qJ_UV(0, 0, k - 1, i, j); zc.qUJ23 = o.qJ_UV(0, 1, k - 1, i, j);
+    zc.qVJ13 = o.qJ_UV(1, 0, k - 1, i, j); zc.qVJ23 = o.qJ_UV(1, 1, k - 1, i, j);
+  }
+  const PointTend t = explicit_point<S, TERRAIN, DIVDMP>(o, a, k, i, j, zc);
+  const size_t p = IX(d, k, i, j);
+  store_point(a, k, i, j, t);
+  a.Sr[p] = t.Sr;
+  a.St[p] = t.St;
+  if (k <= d.KE - 1) { a.Sw[p] = t.Sw; a.PT[p] = t.PT; }
+}
+
+// ---------------------------------------------------------------------------
+// vertical implicit solve, one thread per column (HEVI.F90:688-804, 1092-1127)
+template <bool TERRAIN>
+__global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles, unsigned nblocks) {
+  const Dims& d = a.d;
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const int i = d.IS + (int)((lb % ni_tiles) * 256 + threadIdx.x);
+  const int j = d.JS + (int)(lb / ni_tiles);
+  if (i > d.IE || j > d.JE) return;
+  Met<TERRAIN> m;
+  m.G = a.GSQRT; m.J13 = a.J13G; m.J23 = a.J23G; m.MAPF = a.MAPF; m.n3 = d.n3; m.pl = d.pl; m.IA = d.IA;
+  const double* CDZ = a.g.CDZ; const double* RCDZ = a.g.RCDZ; const double* RFDZ = a.g.RFDZ;
+  const double dtrk = a.dtrk, J33G = a.J33G;
+  const int KS = d.KS, KE = d.KE;
+  auto px = [&](int k) { return IX(d, k, i, j); };
+  auto dpres = [&](int k) { size_t q = px(k); return a.DPRES0[q] + a.RT2P[q] * (a.RHOT[q] - a.REF_rhot[q]); };
+  auto Af = [&](int k, double rt) { return dtrk * dtrk * J33G * RCDZ[k] * rt * J33G / m.gs(I_XYZ, px(k)); };
+  auto f2h = [&](int k, int pp) {
+    return CDZ[k + pp - 1] * m.gs(I_XYZ, px(k + pp - 1)) / (CDZ[k] * m.gs(I_XYZ, px(k)) + CDZ[k + 1] * m.gs(I_XYZ, px(k + 1)));
+  };
+  // level k values
+  double rt0 = a.RT2P[px(KS)], dp0 = dpres(KS), st0 = a.St[px(KS)], sr0 = a.Sr[px(KS)];
+  double d0 = a.DENS[px(KS)], rd0 = a.REF_dens[px(KS)], A0 = Af(KS, rt0), pt0 = a.PT[px(KS)], ptm = 0.0;
+  double e_prev = 0.0, f_prev = 0.0, C_last = 0.0;
+  for (int k = KS; k <= KE - 1; ++k) {
+    const size_t q = px(k), q1 = px(k + 1);
+    const double rt1 = a.RT2P[q1], dp1 = dpres(k + 1), st1 = a.St[q1], sr1 = a.Sr[q1];
+    const double d1 = a.DENS[q1], rd1 = a.REF_dens[q1], A1 = Af(k + 1, rt1);
+    const double pt1 = (k + 1 <= KE - 1) ? a.PT[q1] : 0.0;
+    const double gw = m.gs(I_XYW, q);
+    const double B = GRAV * (dtrk * dtrk) * J33G / (CDZ[k + 1] + CDZ[k]);
+    const double F1v = -(pt1 * RFDZ[k] * A1 + B) / gw;
+    const double F2v = 1.0 + (pt0 * RFDZ[k] * (A1 + A0)) / gw;
+    const double F3v = -(ptm * RFDZ[k] * A0 - B) / gw;
+    const double pg = -(dp1 + rt1 * dtrk * st1 - dp0 - rt0 * dtrk * st0) * RFDZ[k] * J33G / gw
+                    - GRAV * (f2h(k, 1) * (d1 - rd1 + sr1 * dtrk) + f2h(k, 2) * (d0 - rd0 + sr0 * dtrk));
+    const double Ck = a.MOMZ[q] + dtrk * (pg + a.Sw[q]);
+    if (k == KS) {
+      const double rdenom = 1.0 / F2v;
+      e_prev = -F1v * rdenom;
+      f_prev = Ck * rdenom;
+      a.e[q] = e_prev; a.f[q] = f_prev;
+    } else if (k < KE - 1) {
+      const double rdenom = 1.0 / (F2v + F3v * e_prev);
+      const double ek = -F1v * rdenom;
+      const double fk = (Ck - F3v * f_prev) * rdenom;
+      e_prev = ek; f_prev = fk;
+      a.e[q] = ek; a.f[q] = fk;
+    } else {
+      C_last = (Ck - F3v * f_prev) / (F2v + F3v * e_prev);
+    }
+    rt0 = rt1; dp0 = dp1; st0 = st1; sr0 = sr1; d0 = d1; rd0 = rd1; A0 = A1; ptm = pt0; pt0 = pt1;
+  }
+  // back substitution and updates (HEVI.F90:728-790)
+  double* mfZ = a.mflx_hi; double* tfZ = a.tflx_hi;
+  const double M12 = m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY);
+  double Cn = C_last;                         // C(k+1)
+  double ptn = a.PT[px(KE - 1)];              // PT(k+1)
+  {
+    const size_t q = px(KE);
+    double advcv = Cn * J33G * RCDZ[KE] / m.gs(I_XYZ, q);
+    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + a.Sr[q]);
+    advcv = Cn * ptn * J33G * RCDZ[KE] / m.gs(I_XYZ, q);
+    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + a.St[q]);
+    a.MOMZ_RK[q] = 0.0;
+  }
+  {
+    const size_t q = px(KE - 1);
+    mfZ[q] = mfZ[q] + J33G * Cn / (M12);
+    tfZ[q] = tfZ[q] + J33G * Cn * ptn / (M12);
+    a.MOMZ_RK[q] = a.MOMZ0[q] + (Cn - a.MOMZ[q]);
+  }
+  for (int k = KE - 2; k >= KS; --k) {
+    const size_t q = px(k), q1 = px(k + 1);
+    const double Ck = a.e[q] * Cn + a.f[q];
+    const double ptk = a.PT[q];
+    mfZ[q] = mfZ[q] + J33G * Ck / (M12);
+    tfZ[q] = tfZ[q] + J33G * Ck * ptk / (M12);
+    a.MOMZ_RK[q] = a.MOMZ0[q] + (Ck - a.MOMZ[q]);
+    // level k+1 (KS+1..KE-1) now has both C(k+1) and C(k)
+    double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
+    a.DENS_RK[q1] = a.DENS0[q1] + dtrk * (advcv + a.Sr[q1]);
+    advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
+    a.RHOT_RK[q1] = a.RHOT0[q1] + dtrk * (advcv + a.St[q1]);
+    Cn = Ck; ptn = ptk;
+  }
+  {
+    const size_t q = px(KS);
+    double advcv = -Cn * J33G * RCDZ[KS] / m.gs(I_XYZ, q);
+    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + a.Sr[q]);
+    advcv = -Cn * ptn * J33G * RCDZ[KS] / m.gs(I_XYZ, q);
+    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + a.St[q]);
+    a.MOMZ_RK[px(KS - 1)] = 0.0;
+  }
+}
+
+
+// ---------------------------------------------------------------------------
 template <int S, bool TERRAIN, bool DIVDMP>
 static hipError_t launch_t(const HeviArgs& a, hipStream_t s) {
   const Dims& d = a.d;
-  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
+  const Ctx& c = *a.ctx;
+  const char* v = getenv("SCALE_HIP_HEVI_VARIANT");
+  if constexpr (!TERRAIN) {
+    if (!v || strcmp(v, "tile") == 0) {
+      const char* tjs = getenv("SCALE_HIP_TILE_TJ");
+      const int tj = tjs ? atoi(tjs) : 4;
+      const unsigned ni = (imax + 63) / 64;
+      prof_start(c, "hevi_stage", s);
+      if (tj == 8) {
+        const unsigned nb = ni * ((jmax + 7) / 8);
+        hipLaunchKernelGGL((hevi_tile<S, DIVDMP, 8>), dim3(nb), dim3(64, 8), 0, s, a, ni, nb);
+      } else if (tj == 2) {
+        const unsigned nb = ni * ((jmax + 1) / 2);
+        hipLaunchKernelGGL((hevi_tile<S, DIVDMP, 2>), dim3(nb), dim3(64, 2), 0, s, a, ni, nb);
+      } else {
+        const unsigned nb = ni * ((jmax + 3) / 4);
+        hipLaunchKernelGGL((hevi_tile<S, DIVDMP, 4>), dim3(nb), dim3(64, 4), 0, s, a, ni, nb);
+      }
+      prof_stop(c, s);
+      return hipGetLastError();
+    }
+  }
+  if (v && strcmp(v, "point") == 0) {
+    const unsigned ni = (imax + 63) / 64, nk = (kmax + 3) / 4;
+    const unsigned nb = ni * nk * (unsigned)jmax;
+    prof_start(c, "hevi_stage", s);
+    hipLaunchKernelGGL((hevi_point<S, TERRAIN, DIVDMP>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
+    const unsigned ns = (imax + 255) / 256;
+    const unsigned nbs = ns * (unsigned)jmax;
+    hipLaunchKernelGGL((hevi_solve<TERRAIN>), dim3(nbs), dim3(256), 0, s, a, ns, nbs);
+    prof_stop(c, s);
+    return hipGetLastError();
+  }
   constexpr unsigned TJ = 4;
   const unsigned ni = (imax + 63) / 64, nj = (jmax + TJ - 1) / TJ;
   const unsigned nb = ni * nj;
-  const Ctx& c = *a.ctx;
   prof_start(c, "hevi_stage", s);
   hipLaunchKernelGGL((hevi_march<S, TERRAIN, DIVDMP>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   prof_stop(c, s);
@@ -604,7 +1134,7 @@ hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
   a.divdmp_coef = io.divdmp_coef; a.dtrk = io.dtrk; a.has_coriolis = io.has_coriolis;
   a.DENS_RK = io.o[0]; a.MOMZ_RK = io.o[1]; a.MOMX_RK = io.o[2]; a.MOMY_RK = io.o[3]; a.RHOT_RK = io.o[4];
   a.mflx_hi = io.mflx_hi; a.tflx_hi = io.tflx_hi;
-  a.PT = c.scr[3]; a.e = c.scr[4]; a.f = c.scr[5];
+  a.Sr = c.scr[0]; a.St = c.scr[1]; a.Sw = c.scr[2]; a.PT = c.scr[3]; a.e = c.scr[4]; a.f = c.scr[5];
   const bool div = io.divdmp_coef > 0.0;
   const bool ter = c.terrain;
   if (c.scheme == UD3) {

@@ -163,6 +163,175 @@ __global__ void __launch_bounds__(256) numfilter_kernel(NfArgs a, unsigned ni_ti
   out(I_RHOT, YDIR) = kin ? n.sfc(k, n.dy(4, 0, k, i, j) * nd_cdy[j] * 0.5 * (n.D(k, i, j + 1) + n.D(k, i, j))) : 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// The same fluxes with blocks of 64 (i) x TJ (j) columns marching k.  At each
+// level the block evaluates the five differenced fields once per tile cell
+// (2-cell halo) into LDS for the x and y differences; each column keeps the
+// fields at k-2..k+2 in registers for the z differences.  The ghost level
+// KS-1 of the DENS and RHOT z fluxes is the negated KS value (calc_diff3's
+// mirror image) and is written together with level KS.
+template <int TJ>
+__global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgs a, unsigned ni_tiles, unsigned nblocks) {
+  constexpr int TI = 64, CW = TI + 4, RW = TJ + 4, NT = TI * TJ;
+  __shared__ double sD[2][RW + 1][CW + 1];  // DENS at k and k+1, one more cell east and north
+  __shared__ double sP[5][RW][CW];          // the differenced fields at level k
+  const Dims& d = a.d;
+  Nf n(a);
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const int i0 = d.IS + (int)(lb % ni_tiles) * TI;
+  const int j0 = d.JS + (int)(lb / ni_tiles) * TJ;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = (i <= d.IE) && (j <= d.JE);
+  const int KS = d.KS, KE = d.KE;
+  const size_t n3 = d.n3;
+  const double* nd_cdz = a.ndc;
+  const double* nd_fdz = a.ndc + d.KA;
+  const double* nd_cdx = a.ndc + 2 * d.KA;
+  const double* nd_fdx = a.ndc + 2 * d.KA + d.IA;
+  const double* nd_cdy = a.ndc + 2 * d.KA + 2 * d.IA;
+  const double* nd_fdy = a.ndc + 2 * d.KA + 2 * d.IA + d.JA;
+  const size_t p0 = own ? IX(d, 0, i, j) : 0;
+  auto out = [&](int v, int dir, int k) -> double& { return a.num_diff[p0 + d.pl * (size_t)k + n3 * (size_t)(v + 5 * dir)]; };
+  auto fillD = [&](double (*dst)[CW + 1], int k) {
+    for (int q = tid; q < (RW + 1) * (CW + 1); q += NT) {
+      const int r = q / (CW + 1), c = q - r * (CW + 1);
+      const int ii = i0 - 2 + c, jj = j0 - 2 + r;
+      dst[r][c] = (ii < d.IA && jj < d.JA) ? a.DENS[IX(d, k, ii, jj)] : 1.0;
+    }
+  };
+  // the five fields at (k, ii, jj) for the own column (COMMON.F90:545, 631, 715, 798, 882)
+  auto phis = [&](int k, double* ph) {
+    const size_t q = p0 + d.pl * (size_t)k;
+    const double D = a.DENS[q];
+    ph[0] = D - a.REF_dens[q];
+    ph[1] = 2.0 * a.MOMZ[q] / (a.DENS[q + d.pl] + D);
+    ph[2] = 2.0 * a.MOMX[q] / (a.DENS[q + 1] + D);
+    ph[3] = 2.0 * a.MOMY[q] / (a.DENS[q + d.IA] + D);
+    ph[4] = a.RHOT[q] / D - a.REF_pott[q];
+  };
+
+  // levels outside KS..KE are zero (KS-1 gets its DENS/RHOT z fluxes below)
+  if (own) {
+    for (int k = 0; k < d.KA; ++k) {
+      if (k >= KS && k <= KE) continue;
+      for (int q = 0; q < 15; ++q) out(q % 5, q / 5, k) = 0.0;
+    }
+  }
+  // z windows: w[f][m] = field f at level k-2+m
+  double w[5][5];
+  for (int f = 0; f < 5; ++f) for (int m = 0; m < 5; ++m) w[f][m] = 0.0;
+  if (own) {
+    double ph[5];
+    for (int m = 3; m < 5; ++m) {  // shifted down once at the first level
+      phis(KS - 3 + m, ph);
+      for (int f = 0; f < 5; ++f) w[f][m] = ph[f];
+    }
+  }
+  int s0 = 0;
+  fillD(sD[0], KS);
+  for (int k = KS; k <= KE; ++k) {
+    const int s1 = s0 ^ 1;
+    fillD(sD[s1], k + 1);
+    if (own) {
+      double ph[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (k + 2 <= KE) phis(k + 2, ph);
+      for (int f = 0; f < 5; ++f) { w[f][0] = w[f][1]; w[f][1] = w[f][2]; w[f][2] = w[f][3]; w[f][3] = w[f][4]; w[f][4] = ph[f]; }
+    }
+    __syncthreads();
+    for (int q = tid; q < RW * CW; q += NT) {
+      const int r = q / CW, c = q - r * CW;
+      const int ii = i0 - 2 + c, jj = j0 - 2 + r;
+      if (ii >= d.IA || jj >= d.JA) { for (int f = 0; f < 5; ++f) sP[f][r][c] = 0.0; continue; }
+      const size_t g = IX(d, k, ii, jj);
+      const double D = sD[s0][r][c];
+      sP[0][r][c] = D - a.REF_dens[g];
+      sP[1][r][c] = 2.0 * a.MOMZ[g] / (sD[s1][r][c] + D);
+      sP[2][r][c] = 2.0 * a.MOMX[g] / (sD[s0][r][c + 1] + D);
+      sP[3][r][c] = 2.0 * a.MOMY[g] / (sD[s0][r + 1][c] + D);
+      sP[4][r][c] = a.RHOT[g] / D - a.REF_pott[g];
+    }
+    __syncthreads();
+    if (own) {
+      const int r = ty + 2, c = tx + 2;
+      auto Dk = [&](int di, int dj) { return sD[s0][r + dj][c + di]; };
+      auto Dk1 = [&](int di, int dj) { return sD[s1][r + dj][c + di]; };
+      auto cz = [&](int nn, int kk, int b) { return n.cz(nn, kk, b); };
+      // calc_diff3 ZDIR KO = 0 at level k in KS..KE (COMMON.F90:1668-1697);
+      // w[f][m] holds level k-2+m
+      auto dz0 = [&](int f) {
+        const double* P = w[f];
+        if (k == KS)
+          return (+cz(1, KS + 1, 1) * P[4] - cz(2, KS + 1, 1) * P[3]
+                  + cz(3, KS + 1, 1) * P[2] - cz(1, KS, 1) * P[3]);
+        if (k == KE - 1)
+          return (+cz(1, KE, 1) * P[2] - cz(2, KE, 1) * P[3]
+                  + cz(3, KE, 1) * P[2] - cz(1, KE - 1, 1) * P[1]);
+        if (k == KE)
+          return -(+cz(1, KE, 1) * P[1] - cz(2, KE, 1) * P[2]
+                   + cz(3, KE, 1) * P[1] - cz(1, KE - 1, 1) * P[0]);
+        return (+cz(1, k + 1, 1) * P[4] - cz(2, k + 1, 1) * P[3]
+                + cz(3, k + 1, 1) * P[2] - cz(1, k, 1) * P[1]);
+      };
+      // calc_diff3 ZDIR KO = 1 at level k in KS+1..KE-1 (COMMON.F90:1699-1731)
+      auto dz1 = [&](int f) {
+        const double* P = w[f];
+        if (k == KS + 1)
+          return (+cz(1, KS + 1, 2) * P[3] - cz(2, KS + 1, 2) * P[2]
+                  + cz(3, KS + 1, 2) * P[1] - cz(1, KS, 2) * P[2]);
+        if (k == KE - 1)
+          return (-cz(2, KE - 1, 2) * P[2] + cz(3, KE - 1, 2) * P[1]
+                  - cz(1, KE - 2, 2) * P[0]);
+        return (+cz(1, k, 2) * P[3] - cz(2, k, 2) * P[2]
+                + cz(3, k, 2) * P[1] - cz(1, k - 1, 2) * P[0]);
+      };
+      // calc_diff3 XDIR / YDIR (COMMON.F90:1733-1800)
+      auto dx = [&](int f, int io) {
+        if (io == 0)
+          return (+n.cx(1, i + 1, 1) * sP[f][r][c + 2] - n.cx(2, i + 1, 1) * sP[f][r][c + 1]
+                  + n.cx(3, i + 1, 1) * sP[f][r][c] - n.cx(1, i, 1) * sP[f][r][c - 1]);
+        return (+n.cx(1, i, 2) * sP[f][r][c + 1] - n.cx(2, i, 2) * sP[f][r][c]
+                + n.cx(3, i, 2) * sP[f][r][c - 1] - n.cx(1, i - 1, 2) * sP[f][r][c - 2]);
+      };
+      auto dy = [&](int f, int jo) {
+        if (jo == 0)
+          return (+n.cy(1, j + 1, 1) * sP[f][r + 2][c] - n.cy(2, j + 1, 1) * sP[f][r + 1][c]
+                  + n.cy(3, j + 1, 1) * sP[f][r][c] - n.cy(1, j, 1) * sP[f][r - 1][c]);
+        return (+n.cy(1, j, 2) * sP[f][r + 1][c] - n.cy(2, j, 2) * sP[f][r][c]
+                + n.cy(3, j, 2) * sP[f][r - 1][c] - n.cy(1, j - 1, 2) * sP[f][r - 2][c]);
+      };
+      const bool kinm = k <= KE - 1;
+      // density (COMMON.F90:552-621)
+      const double dz0d = dz0(0);
+      out(I_DENS, ZDIR, k) = dz0d * nd_cdz[k];
+      if (k == KS) out(I_DENS, ZDIR, KS - 1) = (-dz0d) * nd_cdz[KS - 1];
+      out(I_DENS, XDIR, k) = n.sfc(k, dx(0, 0) * nd_cdx[i]);
+      out(I_DENS, YDIR, k) = n.sfc(k, dy(0, 0) * nd_cdy[j]);
+      // z-momentum (COMMON.F90:637-707)
+      out(I_MOMZ, ZDIR, k) = (k >= KS + 1 && k <= KE - 1) ? dz1(1) * nd_fdz[k] * Dk(0, 0) : 0.0;
+      out(I_MOMZ, XDIR, k) = kinm ? n.sfc(k, dx(1, 0) * nd_cdx[i] * 0.25 * (Dk1(1, 0) + Dk1(0, 0) + Dk(1, 0) + Dk(0, 0))) : 0.0;
+      out(I_MOMZ, YDIR, k) = kinm ? n.sfc(k, dy(1, 0) * nd_cdy[j] * 0.25 * (Dk1(0, 1) + Dk1(0, 0) + Dk(0, 1) + Dk(0, 0))) : 0.0;
+      // x-momentum (COMMON.F90:721-790)
+      out(I_MOMX, ZDIR, k) = kinm ? dz0(2) * nd_cdz[k] * 0.25 * (Dk1(1, 0) + Dk1(0, 0) + Dk(1, 0) + Dk(0, 0)) : 0.0;
+      out(I_MOMX, XDIR, k) = n.sfc(k, dx(2, 1) * nd_fdx[i] * Dk(0, 0));
+      out(I_MOMX, YDIR, k) = n.sfc(k, dy(2, 0) * nd_cdy[j] * 0.25 * (Dk(1, 1) + Dk(1, 0) + Dk(0, 1) + Dk(0, 0)));
+      // y-momentum (COMMON.F90:804-873)
+      out(I_MOMY, ZDIR, k) = kinm ? dz0(3) * nd_cdz[k] * 0.25 * (Dk1(0, 1) + Dk1(0, 0) + Dk(0, 1) + Dk(0, 0)) : 0.0;
+      out(I_MOMY, XDIR, k) = n.sfc(k, dx(3, 0) * nd_cdx[i] * 0.25 * (Dk(1, 1) + Dk(0, 1) + Dk(1, 0) + Dk(0, 0)));
+      out(I_MOMY, YDIR, k) = n.sfc(k, dy(3, 1) * nd_fdy[j] * Dk(0, 0));
+      // rho*theta (COMMON.F90:888-962)
+      const double dz04 = dz0(4);
+      if (kinm) out(I_RHOT, ZDIR, k) = dz04 * nd_cdz[k] * 0.5 * (Dk1(0, 0) + Dk(0, 0));
+      else out(I_RHOT, ZDIR, k) = dz04 * nd_cdz[KE] * Dk(0, 0);
+      if (k == KS) out(I_RHOT, ZDIR, KS - 1) = (-dz04) * nd_cdz[KS - 1] * Dk(0, 0);
+      out(I_RHOT, XDIR, k) = n.sfc(k, dx(4, 0) * nd_cdx[i] * 0.5 * (Dk(1, 0) + Dk(0, 0)));
+      out(I_RHOT, YDIR, k) = n.sfc(k, dy(4, 0) * nd_cdy[j] * 0.5 * (Dk(0, 1) + Dk(0, 0)));
+    }
+    __syncthreads();
+    s0 = s1;
+  }
+}
+
 hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s) {
   NfArgs a;
@@ -173,10 +342,11 @@ hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], 
   a.num_diff = num_diff;
   const Dims& d = c.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
-  const unsigned ni = (imax + 63) / 64, nk = (d.KA + 3) / 4;
-  const unsigned nb = ni * nk * (unsigned)jmax;
+  constexpr int TJ = 4;
+  const unsigned ni = (imax + 63) / 64, nj = (jmax + TJ - 1) / TJ;
+  const unsigned nb = ni * nj;
   prof_start(c, "numfilter", s);
-  hipLaunchKernelGGL(numfilter_kernel, dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
+  hipLaunchKernelGGL(numfilter_tile<TJ>, dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   prof_stop(c, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -76,14 +76,16 @@ def test_fortran_dropin_bitwise(tmp_path, scheme):
     out["mflx_hi"] = np.asfortranarray(np.zeros(g.shape + (3,)))
     out["tflx_hi"] = po.nan_like_outputs(g, 3)
     ref = po.tstep_short_hevi(case, inp, dtrk, scheme=scheme, out=out)
-    for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK", "mflx_hi", "tflx_hi"):
+    for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK", "tflx_hi"):
         H.compare(ref[k], got[k], k)
 
+    # the driver writes mflx_hi after its periodic halo exchange
     mflx = np.array(ref["mflx_hi"], order="F", copy=True)
     for c in range(3):
         comp = np.asfortranarray(mflx[..., c])
         po.halo_periodic(g, comp)
         mflx[..., c] = comp
+    assert np.array_equal(mflx, got["mflx_hi"])
     q_ref = np.array(qtrco, order="F", copy=True)
     po.tstep_tracer(case, q_ref, qtrc, qtrc, rhoq_t, inp["DENS0"], inp["DENS"], mflx, nd_q, dtl,
                     fct=True, along_stream=True, scheme=scheme)
