@@ -134,32 +134,40 @@ __device__ __forceinline__ double comb(const CombArgs& a, int v, size_t p) {
 }
 
 __global__ void __launch_bounds__(256) rk4_combine(CombArgs a) {
+  // one thread per interior point; the threads of levels KS and KE also fill
+  // the ghost levels below/above, so the state may be combined in place
+  // (st aliasing s0: every thread reads its own point before writing it)
   const Dims& d = a.d;
   const int i = d.IS + blockIdx.x * 64 + threadIdx.x;
-  const int k = blockIdx.y * 4 + threadIdx.y;
+  const int k = d.KS + blockIdx.y * 4 + threadIdx.y;
   const int j = d.JS + blockIdx.z;
-  if (i > d.IE || k >= d.KA || j > d.JE) return;
+  if (i > d.IE || k > d.KE || j > d.JE) return;
   const size_t p = IX(d, k, i, j);
   const int KS = d.KS, KE = d.KE;
-  // DENS, MOMX, MOMY, RHOT: combine on KS..KE; ghosts copy the combined KS / KE value
+  double v4[4];
   const int vs[4] = {0, 2, 3, 4};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int v = vs[q];
-    const int kk = k < KS ? KS : (k > KE ? KE : k);
-    a.st[v][p] = comb(a, v, IX(d, kk, i, j));
-  }
-  // MOMZ: combine on KS..KE-1, MOMZ(KE) keeps the last stage's value (0), ghosts 0
-  if (k >= KS && k <= KE - 1) a.st[1][p] = comb(a, 1, p);
-  else if (k == KE) a.st[1][p] = a.r4[1][p];
-  else a.st[1][p] = 0.0;
-  if (k >= KS && k <= KE) {
+  for (int q = 0; q < 4; ++q) v4[q] = comb(a, vs[q], p);
+  // MOMZ: combine on KS..KE-1, MOMZ(KE) keeps the last stage's value (0)
+  const double w = (k <= KE - 1) ? comb(a, 1, p) : a.r4[1][p];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) {
-      const size_t q = p + d.n3 * (size_t)n;
-      a.mflx[q] = (a.m1[q] + a.m2[q] * 2.0 + a.m3[q] * 2.0 + a.m4[q]) / 6.0;
-      a.tflx[q] = (a.t1[q] + a.t2[q] * 2.0 + a.t3[q] * 2.0 + a.t4[q]) / 6.0;
+  for (int q = 0; q < 4; ++q) a.st[vs[q]][p] = v4[q];
+  a.st[1][p] = w;
+  // ghost levels: DENS, MOMX, MOMY, RHOT copy the combined KS / KE value; MOMZ is 0
+  if (k == KS || k == KE) {
+    const int lo = (k == KS) ? 0 : KE + 1, hi = (k == KS) ? KS - 1 : d.KA - 1;
+    for (int kk = lo; kk <= hi; ++kk) {
+      const size_t g = IX(d, kk, i, j);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a.st[vs[q]][g] = v4[q];
+      a.st[1][g] = 0.0;
     }
+  }
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const size_t q = p + d.n3 * (size_t)n;
+    a.mflx[q] = (a.m1[q] + a.m2[q] * 2.0 + a.m3[q] * 2.0 + a.m4[q]) / 6.0;
+    a.tflx[q] = (a.t1[q] + a.t2[q] * 2.0 + a.t3[q] * 2.0 + a.t4[q]) / 6.0;
   }
 }
 
@@ -299,16 +307,16 @@ hipError_t model_short_step_rk4(Ctx& c, double dts) {
     const double* cst[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
     HIPCHK(numfilter(c, M.num_diff, cst, M.REF_dens, M.REF_pott, s));
   }
-  // DENS0 = DENS, ... (RK4.F90:326-336)
-  for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(M.S0[v], st[v], bytes, hipMemcpyDeviceToDevice, s));
+  // DENS0 = DENS, ... (RK4.F90:326-336): the stages only read the state, so
+  // it serves as the start value directly and is overwritten by the combination
   const double dtrk[4] = {dts / 2.0, dts / 2.0, dts, dts};
   for (int r = 0; r < 4; ++r) {
     StageIO io;
     io.ctx = &c;
     for (int v = 0; v < 5; ++v) {
       io.o[v] = M.R[r][v];
-      io.s0[v] = M.S0[v];
-      io.s[v] = r == 0 ? M.S0[v] : M.R[r - 1][v];
+      io.s0[v] = st[v];
+      io.s[v] = r == 0 ? st[v] : M.R[r - 1][v];
       io.t[v] = M.t[v];
     }
     io.mflx_hi = M.mfR[r];
@@ -322,13 +330,13 @@ hipError_t model_short_step_rk4(Ctx& c, double dts) {
   CombArgs a;
   a.d = d;
   for (int v = 0; v < 5; ++v) {
-    a.st[v] = st[v]; a.r1[v] = M.R[0][v]; a.r2[v] = M.R[1][v]; a.r3[v] = M.R[2][v]; a.r4[v] = M.R[3][v]; a.s0[v] = M.S0[v];
+    a.st[v] = st[v]; a.r1[v] = M.R[0][v]; a.r2[v] = M.R[1][v]; a.r3[v] = M.R[2][v]; a.r4[v] = M.R[3][v]; a.s0[v] = st[v];
   }
   a.mflx = M.mflx_hi; a.tflx = M.tflx_hi;
   a.m1 = M.mfR[0]; a.m2 = M.mfR[1]; a.m3 = M.mfR[2]; a.m4 = M.mfR[3];
   a.t1 = M.tfR[0]; a.t2 = M.tfR[1]; a.t3 = M.tfR[2]; a.t4 = M.tfR[3];
   prof_start(c, "rk4_combine", s);
-  hipLaunchKernelGGL(rk4_combine, grid_intk(d), dim3(64, 4), 0, s, a);
+  hipLaunchKernelGGL(rk4_combine, grid_int(d), dim3(64, 4), 0, s, a);
   prof_stop(c, s);
   HIPCHK(hipGetLastError());
   return halo_fill(c, st, 5, s);

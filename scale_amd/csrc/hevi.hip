@@ -659,6 +659,46 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
 
   for (int k = KS; k <= KE; ++k) {
     const int s1 = s0 ^ 1;
+    // ---- loads of this level that do not go through LDS, issued first so
+    // that their latency overlaps the staging and the face evaluation ----
+    const size_t gk = own ? gix(k, i, j) : 0;
+    double o_nZD = 0.0, o_nZT = 0.0, o_nZW = 0.0, o_nZX = 0.0, o_nZY = 0.0;
+    double o_Dt = 0.0, o_Rt = 0.0, o_Wt = 0.0, o_Xt = 0.0, o_Yt = 0.0, o_X0 = 0.0, o_Y0 = 0.0, o_RT = 0.0, o_RD = 0.0, o_Z0 = 0.0;
+    double o_R2 = 1.0, o_D2 = 1.0, o_W2 = 0.0, o_X2 = 0.0, o_Y2 = 0.0;
+    const double wdc = a.wdamp_coef[k];
+    if (own) {
+      o_nZD = nd(I_DENS, ZDIR, gk); o_nZT = nd(I_RHOT, ZDIR, gk); o_nZW = nd(I_MOMZ, ZDIR, gk + d.pl);
+      o_nZX = nd(I_MOMX, ZDIR, gk); o_nZY = nd(I_MOMY, ZDIR, gk);
+      o_Dt = a.DENS_t[gk]; o_Rt = a.RHOT_t[gk]; o_Wt = a.MOMZ_t[gk]; o_Xt = a.MOMX_t[gk]; o_Yt = a.MOMY_t[gk];
+      o_X0 = a.MOMX0[gk]; o_Y0 = a.MOMY0[gk]; o_RT = a.RT2P[gk]; o_RD = a.REF_dens[gk];
+      if (wdc != 0.0) o_Z0 = a.MOMZ0[gk];
+      const size_t g2 = gk + 2 * d.pl;
+      o_R2 = a.RHOT[g2]; o_D2 = a.DENS[g2]; o_W2 = a.MOMZ[g2]; o_X2 = a.MOMX[g2]; o_Y2 = a.MOMY[g2];
+    }
+    // num_diff of the faces: parked in the face arrays, which the face
+    // evaluation reads and then overwrites position by position
+    for (int q = tid; q < TJ * (TI + 1); q += NT) {
+      const int row = q / (TI + 1), x = q - row * (TI + 1);
+      const int ii = i0 - 1 + x, jj = j0 + row;
+      if (ii > d.IE || jj > d.JE) continue;
+      const size_t g = gix(k, ii, jj);
+      xM[row][x] = nd(I_DENS, XDIR, g);
+      xT[row][x] = nd(I_RHOT, XDIR, g);
+      xW[row][x] = (k < KE) ? nd(I_MOMZ, XDIR, g) : 0.0;
+      xU[row][x] = nd(I_MOMX, XDIR, g + 1);
+      xV[row][x] = nd(I_MOMY, XDIR, g);
+    }
+    for (int q = tid; q < (TJ + 1) * TI; q += NT) {
+      const int y = q / TI, col = q - y * TI;
+      const int ii = i0 + col, jj = j0 - 1 + y;
+      if (ii > d.IE || jj > d.JE) continue;
+      const size_t g = gix(k, ii, jj);
+      yM[y][col] = nd(I_DENS, YDIR, g);
+      yT[y][col] = nd(I_RHOT, YDIR, g);
+      yW[y][col] = (k < KE) ? nd(I_MOMZ, YDIR, g) : 0.0;
+      yU[y][col] = nd(I_MOMX, YDIR, g);
+      yV[y][col] = nd(I_MOMY, YDIR, g + d.IA);
+    }
     // ---- (a) staging ----
     fill(sD[s1], a.DENS, k + 1); fill(sMX[s1], a.MOMX, k + 1); fill(sMY[s1], a.MOMY, k + 1);
     fill(sW, a.MOMZ, k);
@@ -679,10 +719,10 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
     }
     // own-column windows move up one level
     if (own) {
-      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = at(a.RHOT, k + 2) / at(a.DENS, k + 2);
-      ww[0] = ww[1]; ww[1] = ww[2]; ww[2] = ww[3]; ww[3] = at(a.MOMZ, k + 2);
-      xw[0] = xw[1]; xw[1] = xw[2]; xw[2] = xw[3]; xw[3] = at(a.MOMX, k + 2);
-      yw[0] = yw[1]; yw[1] = yw[2]; yw[2] = yw[3]; yw[3] = at(a.MOMY, k + 2);
+      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = o_R2 / o_D2;
+      ww[0] = ww[1]; ww[1] = ww[2]; ww[2] = ww[3]; ww[3] = o_W2;
+      xw[0] = xw[1]; xw[1] = xw[2]; xw[2] = xw[3]; xw[3] = o_X2;
+      yw[0] = yw[1]; yw[1] = yw[2]; yw[2] = yw[3]; yw[3] = o_Y2;
     }
     __syncthreads();
 
@@ -695,28 +735,29 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
       const int row = q / (TI + 1), x = q - row * (TI + 1);
       const int ii = i0 - 1 + x, jj = j0 + row;
       if (ii > d.IE || jj > d.JE) continue;
+      const double ndq[5] = {xM[row][x], xT[row][x], xW[row][x], xU[row][x], xV[row][x]};
       const int r = row + 2, c = x + 1;  // tile cell of column ii
       const size_t g = gix(k, ii, jj);
-      const double vm = sMX[s0][r][c] + nd(I_DENS, XDIR, g);
+      const double vm = sMX[s0][r][c] + ndq[0];
       xM[row][x] = vm;
-      xT[row][x] = vm * fv_int<S>(sPO[r][c - 1], sPO[r][c], sPO[r][c + 1], sPO[r][c + 2], vm) + nd(I_RHOT, XDIR, g);
+      xT[row][x] = vm * fv_int<S>(sPO[r][c - 1], sPO[r][c], sPO[r][c + 1], sPO[r][c + 2], vm) + ndq[1];
       double qw = 0.0;
       if (k < KE) {
         const double vel = (f1 * sMX[s1][r][c] + f2 * sMX[s0][r][c])
                          / (f1 * 0.5 * (D1(r, c) + D1(r, c + 1)) + f2 * 0.5 * (D0(r, c) + D0(r, c + 1)));
-        qw = vel * fv_int<S>(sW[r][c - 1], sW[r][c], sW[r][c + 1], sW[r][c + 2], vel) + nd(I_MOMZ, XDIR, g);
+        qw = vel * fv_int<S>(sW[r][c - 1], sW[r][c], sW[r][c + 1], sW[r][c + 2], vel) + ndq[2];
       }
       xW[row][x] = qw;
       {  // fluxX_UYZ stored at ii (loop index ii+1)
         const double vel = (0.5 * (sMX[s0][r][c + 1] + sMX[s0][r][c])) / D0(r, c + 1);
         xU[row][x] = vel * fv_int<S>(sMX[s0][r][c - 1], sMX[s0][r][c], sMX[s0][r][c + 1], sMX[s0][r][c + 2], vel)
-                   + nd(I_MOMX, XDIR, g + 1);
+                   + ndq[3];
       }
       {  // fluxX_XVZ at (ii, jj)
         const double vel = (0.5 * (sMX[s0][r][c] + sMX[s0][r + 1][c]))
                          / (0.25 * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1)));
         xV[row][x] = vel * fv_int<S>(sMY[s0][r][c - 1], sMY[s0][r][c], sMY[s0][r][c + 1], sMY[s0][r][c + 2], vel)
-                   + nd(I_MOMY, XDIR, g);
+                   + ndq[4];
       }
       if (x > 0 || i0 == d.IS) {  // faces IS-1 and those owned by this tile
         a.mflx_hi[g + n3] = vm;
@@ -727,28 +768,29 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
       const int y = q / TI, col = q - y * TI;
       const int ii = i0 + col, jj = j0 - 1 + y;
       if (ii > d.IE || jj > d.JE) continue;
+      const double ndq[5] = {yM[y][col], yT[y][col], yW[y][col], yU[y][col], yV[y][col]};
       const int r = y + 1, c = col + 2;  // tile cell of row jj
       const size_t g = gix(k, ii, jj);
-      const double vm = sMY[s0][r][c] + nd(I_DENS, YDIR, g);
+      const double vm = sMY[s0][r][c] + ndq[0];
       yM[y][col] = vm;
-      yT[y][col] = vm * fv_int<S>(sPO[r - 1][c], sPO[r][c], sPO[r + 1][c], sPO[r + 2][c], vm) + nd(I_RHOT, YDIR, g);
+      yT[y][col] = vm * fv_int<S>(sPO[r - 1][c], sPO[r][c], sPO[r + 1][c], sPO[r + 2][c], vm) + ndq[1];
       double qw = 0.0;
       if (k < KE) {
         const double vel = (f1 * sMY[s1][r][c] + f2 * sMY[s0][r][c])
                          / (f1 * 0.5 * (D1(r, c) + D1(r + 1, c)) + f2 * 0.5 * (D0(r, c) + D0(r + 1, c)));
-        qw = vel * fv_int<S>(sW[r - 1][c], sW[r][c], sW[r + 1][c], sW[r + 2][c], vel) + nd(I_MOMZ, YDIR, g);
+        qw = vel * fv_int<S>(sW[r - 1][c], sW[r][c], sW[r + 1][c], sW[r + 2][c], vel) + ndq[2];
       }
       yW[y][col] = qw;
       {  // fluxY_UYZ at (ii, jj)
         const double vel = (0.5 * (sMY[s0][r][c] + sMY[s0][r][c + 1]))
                          / (0.25 * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1)));
         yU[y][col] = vel * fv_int<S>(sMX[s0][r - 1][c], sMX[s0][r][c], sMX[s0][r + 1][c], sMX[s0][r + 2][c], vel)
-                   + nd(I_MOMX, YDIR, g);
+                   + ndq[3];
       }
       {  // fluxY_XVZ stored at jj (loop index jj+1)
         const double vel = (0.5 * (sMY[s0][r + 1][c] + sMY[s0][r][c])) / D0(r + 1, c);
         yV[y][col] = vel * fv_int<S>(sMY[s0][r - 1][c], sMY[s0][r][c], sMY[s0][r + 1][c], sMY[s0][r + 2][c], vel)
-                   + nd(I_MOMY, YDIR, g + d.IA);
+                   + ndq[4];
       }
       if (y > 0 || j0 == d.JS) {
         a.mflx_hi[g + 2 * n3] = vm;
@@ -762,19 +804,19 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
       const int r = ty + 2, c = tx + 2;  // tile cell of the own column
       const size_t g = gix(k, i, j);
       // continuity and theta (HEVI.F90:369-475, 588-657)
-      const double mZ = (k <= KE - 1) ? nd(I_DENS, ZDIR, g) : 0.0;
+      const double mZ = (k <= KE - 1) ? o_nZD : 0.0;
       const double Sr = (-(mZ - cmZ) * RCDZ[k] + -((xM[ty][tx + 1] - xM[ty][tx]) * RCDX[i] + (yM[ty + 1][tx] - yM[ty][tx]) * RCDY[j]))
-                      + a.DENS_t[g];
+                      + o_Dt;
       double tZ = 0.0;
       if (k <= KE - 1) {
         double fv;
         if (k == KS) fv = fv_lo<S>(pw[1], pw[2], pw[3], mZ);
         else if (k == KE - 1) fv = fv_hi<S>(pw[0], pw[1], pw[2], mZ);
         else fv = fv_int<S>(pw[0], pw[1], pw[2], pw[3], mZ);
-        tZ = mZ * fv + nd(I_RHOT, ZDIR, g);
+        tZ = mZ * fv + o_nZT;
       }
       const double St = (-(tZ - ctZ) * RCDZ[k] + -((xT[ty][tx + 1] - xT[ty][tx]) * RCDX[i] + (yT[ty + 1][tx] - yT[ty][tx]) * RCDY[j]))
-                      + a.RHOT_t[g];
+                      + o_Rt;
       // momentum z, explicit part (HEVI.F90:486-577) and PT
       double Sw = 0.0, PT = 0.0;
       if (k <= KE - 1) {
@@ -782,16 +824,15 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
         if (k < KE - 1) {
           const double vel = (0.5 * (ww[1] + ww[2])) / D1(r, c);
           const double fv = (k == KS) ? fv_lo<S>(ww[1], ww[2], ww[3], vel) : fv_int<S>(ww[0], ww[1], ww[2], ww[3], vel);
-          qw = J33G * vel * fv + nd(I_MOMZ, ZDIR, g + d.pl);
+          qw = J33G * vel * fv + o_nZW;
         }
         const double advcv = -(qw - cqW) * RFDZ[k];
         cqW = qw;
         const double advch = -((xW[ty][tx + 1] - xW[ty][tx]) * RCDX[i] + (yW[ty + 1][tx] - yW[ty][tx]) * RCDY[j]);
-        const double wdc = a.wdamp_coef[k];
-        const double wdmp = (wdc != 0.0) ? -wdc * a.MOMZ0[g] : -wdc * 0.0;
+        const double wdmp = -wdc * o_Z0;
         double sw = (advcv + advch) + wdmp;
         if constexpr (DIVDMP) sw = sw + a.divdmp_coef / dtrk * (a.DDIV[g + d.pl] - a.DDIV[g]) * a.g.FDZ[k];
-        Sw = sw + a.MOMZ_t[g];
+        Sw = sw + o_Wt;
         const double vel = ww[1];
         if (k == KS) PT = fv_lo<S>(pw[1], pw[2], pw[3], vel);
         else if (k == KE - 1) PT = fv_hi<S>(pw[0], pw[1], pw[2], vel);
@@ -808,7 +849,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
           if (k == KS) fv = fv_lo<S>(xw[1], xw[2], xw[3], vel);
           else if (k == KE - 1) fv = fv_hi<S>(xw[0], xw[1], xw[2], vel);
           else fv = fv_int<S>(xw[0], xw[1], xw[2], xw[3], vel);
-          qu = J33G * vel * fv + nd(I_MOMX, ZDIR, g);
+          qu = J33G * vel * fv + o_nZX;
         }
         const double advcv = -(qu - cqU) * RCDZ[k];
         cqU = qu;
@@ -821,7 +862,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
           t = t + cf;
         }
         if constexpr (DIVDMP) t = t + a.divdmp_coef / dtrk * (a.DDIV[g + 1] - a.DDIV[g]) * a.g.FDX[i];
-        MXn = a.MOMX0[g] + dtrk * (t + a.MOMX_t[g]);
+        MXn = o_X0 + dtrk * (t + o_Xt);
       }
       // momentum y (HEVI.F90:936-1047)
       double MYn;
@@ -834,7 +875,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
           if (k == KS) fv = fv_lo<S>(yw[1], yw[2], yw[3], vel);
           else if (k == KE - 1) fv = fv_hi<S>(yw[0], yw[1], yw[2], vel);
           else fv = fv_int<S>(yw[0], yw[1], yw[2], yw[3], vel);
-          qv = J33G * vel * fv + nd(I_MOMY, ZDIR, g);
+          qv = J33G * vel * fv + o_nZY;
         }
         const double advcv = -(qv - cqV) * RCDZ[k];
         cqV = qv;
@@ -847,10 +888,10 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
           t = t + cf;
         }
         if constexpr (DIVDMP) t = t + a.divdmp_coef / dtrk * (a.DDIV[g + d.IA] - a.DDIV[g]) * a.g.FDY[j];
-        MYn = a.MOMY0[g] + dtrk * (t + a.MOMY_t[g]);
+        MYn = o_Y0 + dtrk * (t + o_Yt);
       }
       // tridiagonal row k-1 and forward sweep (HEVI.F90:688-727, 1092-1115)
-      const double rt1 = a.RT2P[g], dp1 = sDP[ty][tx], d1 = D0(r, c), rd1 = a.REF_dens[g];
+      const double rt1 = o_RT, dp1 = sDP[ty][tx], d1 = D0(r, c), rd1 = o_RD;
       const double A1 = dtrk * dtrk * J33G * RCDZ[k] * rt1 * J33G;
       const double pt1 = (k <= KE - 1) ? PT : 0.0;
       double ek = 0.0, fk = 0.0;

@@ -230,8 +230,22 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgs a, unsigned ni_
   }
   int s0 = 0;
   fillD(sD[0], KS);
+  constexpr int NC = (RW * CW + NT - 1) / NT;  // tile cells per thread
   for (int k = KS; k <= KE; ++k) {
     const int s1 = s0 ^ 1;
+    // every global load of the level first: the raw fields of this thread's
+    // tile cells, DENS at k+1, and the column's fields at k+2
+    double rZ[NC], rX[NC], rY[NC], rR[NC], rRD[NC], rRP[NC];
+#pragma unroll
+    for (int h = 0; h < NC; ++h) {
+      const int q = tid + h * NT;
+      const int r = q / CW, c = q - r * CW;
+      const int ii = i0 - 2 + c, jj = j0 - 2 + r;
+      const bool ok = q < RW * CW && ii < d.IA && jj < d.JA;
+      const size_t g = ok ? IX(d, k, ii, jj) : 0;
+      rZ[h] = ok ? a.MOMZ[g] : 0.0; rX[h] = ok ? a.MOMX[g] : 0.0; rY[h] = ok ? a.MOMY[g] : 0.0;
+      rR[h] = ok ? a.RHOT[g] : 0.0; rRD[h] = ok ? a.REF_dens[g] : 0.0; rRP[h] = ok ? a.REF_pott[g] : 0.0;
+    }
     fillD(sD[s1], k + 1);
     if (own) {
       double ph[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -239,17 +253,19 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgs a, unsigned ni_
       for (int f = 0; f < 5; ++f) { w[f][0] = w[f][1]; w[f][1] = w[f][2]; w[f][2] = w[f][3]; w[f][3] = w[f][4]; w[f][4] = ph[f]; }
     }
     __syncthreads();
-    for (int q = tid; q < RW * CW; q += NT) {
+#pragma unroll
+    for (int h = 0; h < NC; ++h) {
+      const int q = tid + h * NT;
+      if (q >= RW * CW) continue;
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
       if (ii >= d.IA || jj >= d.JA) { for (int f = 0; f < 5; ++f) sP[f][r][c] = 0.0; continue; }
-      const size_t g = IX(d, k, ii, jj);
       const double D = sD[s0][r][c];
-      sP[0][r][c] = D - a.REF_dens[g];
-      sP[1][r][c] = 2.0 * a.MOMZ[g] / (sD[s1][r][c] + D);
-      sP[2][r][c] = 2.0 * a.MOMX[g] / (sD[s0][r][c + 1] + D);
-      sP[3][r][c] = 2.0 * a.MOMY[g] / (sD[s0][r + 1][c] + D);
-      sP[4][r][c] = a.RHOT[g] / D - a.REF_pott[g];
+      sP[0][r][c] = D - rRD[h];
+      sP[1][r][c] = 2.0 * rZ[h] / (sD[s1][r][c] + D);
+      sP[2][r][c] = 2.0 * rX[h] / (sD[s0][r][c + 1] + D);
+      sP[3][r][c] = 2.0 * rY[h] / (sD[s0][r + 1][c] + D);
+      sP[4][r][c] = rR[h] / D - rRP[h];
     }
     __syncthreads();
     if (own) {

@@ -85,25 +85,29 @@ int prof_names(Prof* p, char* buf, int len) {
 }
 
 // ---- HBM copy probe ----------------------------------------------------------
-// A plain streaming copy of n doubles (16 B per element-pair of traffic: one
-// read, one write), grid-stride with 4 independent loads in flight per lane.
+// A plain streaming copy of n doubles (one read and one write of 8 B each),
+// 16-byte accesses, grid-stride, four independent loads in flight per lane.
 // Its bytes are known exactly, so its time gives the achievable copy
 // bandwidth and its FETCH_SIZE / WRITE_SIZE counters calibrate those of the
 // stencil kernels.
 __global__ void __launch_bounds__(256) hbm_copy_kernel(double* __restrict__ dst, const double* __restrict__ src, size_t n) {
+  const size_t n2 = n / 2;
+  const double2* s2 = reinterpret_cast<const double2*>(src);
+  double2* d2 = reinterpret_cast<double2*>(dst);
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const double a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-    dst[i] = a; dst[i + stride] = b; dst[i + 2 * stride] = c; dst[i + 3 * stride] = d;
+  for (; i + 3 * stride < n2; i += 4 * stride) {
+    const double2 a = s2[i], b = s2[i + stride], c = s2[i + 2 * stride], e = s2[i + 3 * stride];
+    d2[i] = a; d2[i + stride] = b; d2[i + 2 * stride] = c; d2[i + 3 * stride] = e;
   }
-  for (; i < n; i += stride) dst[i] = src[i];
+  for (; i < n2; i += stride) d2[i] = s2[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) dst[n - 1] = src[n - 1];
 }
 
 hipError_t hbm_copy(const Ctx& c, double* dst, const double* src, size_t n, hipStream_t s) {
   int ncu = 256;
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
-  const unsigned nb = (unsigned)ncu * 16u;
+  const unsigned nb = (unsigned)ncu * 8u;
   prof_start(c, "hbm_copy", s);
   hipLaunchKernelGGL(hbm_copy_kernel, dim3(nb), dim3(256), 0, s, dst, src, n);
   prof_stop(c, s);
