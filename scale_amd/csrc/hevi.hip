@@ -4,16 +4,19 @@
 //   scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_short_fvm_hevi.F90:128-1089
 // with the flux stencils of scale_atmos_dyn_fvm_flux_{ud3,cd4}.F90.
 //
-// One kernel per stage, hevi_march: one thread per column (i,j), blocks of
-// 64 (i) x 4 (j) columns.  Marching k upward it evaluates the explicit part
-// of each point (mass and theta fluxes, the explicit z fluxes, the tendencies
-// Sr/St/Sw, the face value PT, the complete MOMX/MOMY updates;
-// HEVI.F90:346-657, 816-1047) and, one level behind, the tridiagonal rows and
-// the Thomas forward sweep (HEVI.F90:688-727, 1092-1115); marching back down
-// it substitutes into MOMZ/DENS/RHOT and adds the implicit parts of the z
-// fluxes (HEVI.F90:728-804, 1116-1127).
-#include <stdlib.h>
-#include <string.h>
+// One kernel per stage.  Both kernels give each thread one column (i,j) and
+// march k upward through the explicit part of every point (pressure, mass and
+// theta fluxes, the explicit z fluxes, the tendencies Sr/St/Sw, the face value
+// PT, the complete MOMX/MOMY updates; HEVI.F90:346-657, 816-1047) and, one
+// level behind, the tridiagonal rows and the Thomas forward sweep
+// (HEVI.F90:688-727, 1092-1115); marching back down they substitute into
+// MOMZ/DENS/RHOT and add the implicit parts of the z fluxes
+// (HEVI.F90:728-804, 1116-1127).
+//   hevi_tile   flat terrain (GSQRT = 1, J13G = J23G = 0, MAPF = 1): blocks
+//               of 64 x 8 columns stage each level in LDS and evaluate every
+//               x and y face once;
+//   hevi_march  general metrics: neighbours come through the caches and each
+//               point evaluates the faces it needs.
 #include "common.h"
 #include "shp_internal.h"
 
@@ -35,7 +38,7 @@ struct HeviArgs {
   // outputs
   double *DENS_RK, *MOMZ_RK, *MOMX_RK, *MOMY_RK, *RHOT_RK, *mflx_hi, *tflx_hi;
   // scratch (n3 each)
-  double *Sr, *St, *Sw, *PT, *e, *f;
+  double *PT, *e, *f;
 };
 
 template <int S, bool TERRAIN, bool DIVDMP>
@@ -983,179 +986,23 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
 }
 
 // ---------------------------------------------------------------------------
-// Alternative organisation: one thread per point for the explicit part, the
-// tendencies and PT through scratch, then one thread per column for the
-// vertical solve (the same arithmetic as hevi_march).
-template <int S, bool TERRAIN, bool DIVDMP>
-__global__ void __launch_bounds__(256) hevi_point(HeviArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
-  const Dims& d = a.d;
-  Ops<S, TERRAIN, DIVDMP> o(a);
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
-  const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
-  const int i = d.IS + (int)(it * 64 + threadIdx.x);
-  const int k = d.KS + (int)(kt * 4 + threadIdx.y);
-  const int j = d.JS + (int)jt;
-  if (i > d.IE || k > d.KE || j > d.JE) return;
-  ZCarry zc;
-  zc.mZ = o.mflxZf(k - 1, i, j);
-  zc.pw[0] = o.POTT(k - 1, i, j); zc.pw[1] = o.POTT(k, i, j); zc.pw[2] = o.POTT(k + 1, i, j); zc.pw[3] = o.POTT(k + 2, i, j);
-  zc.tZ = (k == d.KS) ? 0.0 : o.tflxZ(k - 1, i, j);
-  zc.qW = o.qZ_XYW(k - 1, i, j);
-  zc.qU = o.qZ_UV(0, k - 1, i, j);
-  zc.qV = o.qZ_UV(1, k - 1, i, j);
-  if constexpr (TERRAIN) {
-    zc.qWJ13 = o.qJ_XYW(0, k - 1, i, j); zc.qWJ23 = o.qJ_XYW(1, k - 1, i, j);
-    zc.qUJ13 = o.qJ_UV(0, 0, k - 1, i, j); zc.qUJ23 = o.qJ_UV(0, 1, k - 1, i, j);
-    zc.qVJ13 = o.qJ_UV(1, 0, k - 1, i, j); zc.qVJ23 = o.qJ_UV(1, 1, k - 1, i, j);
-  }
-  const PointTend t = explicit_point<S, TERRAIN, DIVDMP>(o, a, k, i, j, zc);
-  const size_t p = IX(d, k, i, j);
-  store_point(a, k, i, j, t);
-  a.Sr[p] = t.Sr;
-  a.St[p] = t.St;
-  if (k <= d.KE - 1) { a.Sw[p] = t.Sw; a.PT[p] = t.PT; }
-}
-
-// ---------------------------------------------------------------------------
-// vertical implicit solve, one thread per column (HEVI.F90:688-804, 1092-1127)
-template <bool TERRAIN>
-__global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles, unsigned nblocks) {
-  const Dims& d = a.d;
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
-  const int i = d.IS + (int)((lb % ni_tiles) * 256 + threadIdx.x);
-  const int j = d.JS + (int)(lb / ni_tiles);
-  if (i > d.IE || j > d.JE) return;
-  Met<TERRAIN> m;
-  m.G = a.GSQRT; m.J13 = a.J13G; m.J23 = a.J23G; m.MAPF = a.MAPF; m.n3 = d.n3; m.pl = d.pl; m.IA = d.IA;
-  const double* CDZ = a.g.CDZ; const double* RCDZ = a.g.RCDZ; const double* RFDZ = a.g.RFDZ;
-  const double dtrk = a.dtrk, J33G = a.J33G;
-  const int KS = d.KS, KE = d.KE;
-  auto px = [&](int k) { return IX(d, k, i, j); };
-  auto dpres = [&](int k) { size_t q = px(k); return a.DPRES0[q] + a.RT2P[q] * (a.RHOT[q] - a.REF_rhot[q]); };
-  auto Af = [&](int k, double rt) { return dtrk * dtrk * J33G * RCDZ[k] * rt * J33G / m.gs(I_XYZ, px(k)); };
-  auto f2h = [&](int k, int pp) {
-    return CDZ[k + pp - 1] * m.gs(I_XYZ, px(k + pp - 1)) / (CDZ[k] * m.gs(I_XYZ, px(k)) + CDZ[k + 1] * m.gs(I_XYZ, px(k + 1)));
-  };
-  // level k values
-  double rt0 = a.RT2P[px(KS)], dp0 = dpres(KS), st0 = a.St[px(KS)], sr0 = a.Sr[px(KS)];
-  double d0 = a.DENS[px(KS)], rd0 = a.REF_dens[px(KS)], A0 = Af(KS, rt0), pt0 = a.PT[px(KS)], ptm = 0.0;
-  double e_prev = 0.0, f_prev = 0.0, C_last = 0.0;
-  for (int k = KS; k <= KE - 1; ++k) {
-    const size_t q = px(k), q1 = px(k + 1);
-    const double rt1 = a.RT2P[q1], dp1 = dpres(k + 1), st1 = a.St[q1], sr1 = a.Sr[q1];
-    const double d1 = a.DENS[q1], rd1 = a.REF_dens[q1], A1 = Af(k + 1, rt1);
-    const double pt1 = (k + 1 <= KE - 1) ? a.PT[q1] : 0.0;
-    const double gw = m.gs(I_XYW, q);
-    const double B = GRAV * (dtrk * dtrk) * J33G / (CDZ[k + 1] + CDZ[k]);
-    const double F1v = -(pt1 * RFDZ[k] * A1 + B) / gw;
-    const double F2v = 1.0 + (pt0 * RFDZ[k] * (A1 + A0)) / gw;
-    const double F3v = -(ptm * RFDZ[k] * A0 - B) / gw;
-    const double pg = -(dp1 + rt1 * dtrk * st1 - dp0 - rt0 * dtrk * st0) * RFDZ[k] * J33G / gw
-                    - GRAV * (f2h(k, 1) * (d1 - rd1 + sr1 * dtrk) + f2h(k, 2) * (d0 - rd0 + sr0 * dtrk));
-    const double Ck = a.MOMZ[q] + dtrk * (pg + a.Sw[q]);
-    if (k == KS) {
-      const double rdenom = 1.0 / F2v;
-      e_prev = -F1v * rdenom;
-      f_prev = Ck * rdenom;
-      a.e[q] = e_prev; a.f[q] = f_prev;
-    } else if (k < KE - 1) {
-      const double rdenom = 1.0 / (F2v + F3v * e_prev);
-      const double ek = -F1v * rdenom;
-      const double fk = (Ck - F3v * f_prev) * rdenom;
-      e_prev = ek; f_prev = fk;
-      a.e[q] = ek; a.f[q] = fk;
-    } else {
-      C_last = (Ck - F3v * f_prev) / (F2v + F3v * e_prev);
-    }
-    rt0 = rt1; dp0 = dp1; st0 = st1; sr0 = sr1; d0 = d1; rd0 = rd1; A0 = A1; ptm = pt0; pt0 = pt1;
-  }
-  // back substitution and updates (HEVI.F90:728-790)
-  double* mfZ = a.mflx_hi; double* tfZ = a.tflx_hi;
-  const double M12 = m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY);
-  double Cn = C_last;                         // C(k+1)
-  double ptn = a.PT[px(KE - 1)];              // PT(k+1)
-  {
-    const size_t q = px(KE);
-    double advcv = Cn * J33G * RCDZ[KE] / m.gs(I_XYZ, q);
-    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + a.Sr[q]);
-    advcv = Cn * ptn * J33G * RCDZ[KE] / m.gs(I_XYZ, q);
-    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + a.St[q]);
-    a.MOMZ_RK[q] = 0.0;
-  }
-  {
-    const size_t q = px(KE - 1);
-    mfZ[q] = mfZ[q] + J33G * Cn / (M12);
-    tfZ[q] = tfZ[q] + J33G * Cn * ptn / (M12);
-    a.MOMZ_RK[q] = a.MOMZ0[q] + (Cn - a.MOMZ[q]);
-  }
-  for (int k = KE - 2; k >= KS; --k) {
-    const size_t q = px(k), q1 = px(k + 1);
-    const double Ck = a.e[q] * Cn + a.f[q];
-    const double ptk = a.PT[q];
-    mfZ[q] = mfZ[q] + J33G * Ck / (M12);
-    tfZ[q] = tfZ[q] + J33G * Ck * ptk / (M12);
-    a.MOMZ_RK[q] = a.MOMZ0[q] + (Ck - a.MOMZ[q]);
-    // level k+1 (KS+1..KE-1) now has both C(k+1) and C(k)
-    double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
-    a.DENS_RK[q1] = a.DENS0[q1] + dtrk * (advcv + a.Sr[q1]);
-    advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
-    a.RHOT_RK[q1] = a.RHOT0[q1] + dtrk * (advcv + a.St[q1]);
-    Cn = Ck; ptn = ptk;
-  }
-  {
-    const size_t q = px(KS);
-    double advcv = -Cn * J33G * RCDZ[KS] / m.gs(I_XYZ, q);
-    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + a.Sr[q]);
-    advcv = -Cn * ptn * J33G * RCDZ[KS] / m.gs(I_XYZ, q);
-    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + a.St[q]);
-    a.MOMZ_RK[px(KS - 1)] = 0.0;
-  }
-}
-
-
-// ---------------------------------------------------------------------------
 template <int S, bool TERRAIN, bool DIVDMP>
 static hipError_t launch_t(const HeviArgs& a, hipStream_t s) {
   const Dims& d = a.d;
-  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
   const Ctx& c = *a.ctx;
-  const char* v = getenv("SCALE_HIP_HEVI_VARIANT");
-  if constexpr (!TERRAIN) {
-    if (!v || strcmp(v, "tile") == 0) {
-      const char* tjs = getenv("SCALE_HIP_TILE_TJ");
-      const int tj = tjs ? atoi(tjs) : 4;
-      const unsigned ni = (imax + 63) / 64;
-      prof_start(c, "hevi_stage", s);
-      if (tj == 8) {
-        const unsigned nb = ni * ((jmax + 7) / 8);
-        hipLaunchKernelGGL((hevi_tile<S, DIVDMP, 8>), dim3(nb), dim3(64, 8), 0, s, a, ni, nb);
-      } else if (tj == 2) {
-        const unsigned nb = ni * ((jmax + 1) / 2);
-        hipLaunchKernelGGL((hevi_tile<S, DIVDMP, 2>), dim3(nb), dim3(64, 2), 0, s, a, ni, nb);
-      } else {
-        const unsigned nb = ni * ((jmax + 3) / 4);
-        hipLaunchKernelGGL((hevi_tile<S, DIVDMP, 4>), dim3(nb), dim3(64, 4), 0, s, a, ni, nb);
-      }
-      prof_stop(c, s);
-      return hipGetLastError();
-    }
-  }
-  if (v && strcmp(v, "point") == 0) {
-    const unsigned ni = (imax + 63) / 64, nk = (kmax + 3) / 4;
-    const unsigned nb = ni * nk * (unsigned)jmax;
-    prof_start(c, "hevi_stage", s);
-    hipLaunchKernelGGL((hevi_point<S, TERRAIN, DIVDMP>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
-    const unsigned ns = (imax + 255) / 256;
-    const unsigned nbs = ns * (unsigned)jmax;
-    hipLaunchKernelGGL((hevi_solve<TERRAIN>), dim3(nbs), dim3(256), 0, s, a, ns, nbs);
-    prof_stop(c, s);
-    return hipGetLastError();
-  }
-  constexpr unsigned TJ = 4;
-  const unsigned ni = (imax + 63) / 64, nj = (jmax + TJ - 1) / TJ;
-  const unsigned nb = ni * nj;
+  const unsigned ni = (imax + 63) / 64;
   prof_start(c, "hevi_stage", s);
-  hipLaunchKernelGGL((hevi_march<S, TERRAIN, DIVDMP>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  if constexpr (!TERRAIN) {
+    // flat terrain: LDS tiles of 64 x 8 columns
+    constexpr int TJ = 8;
+    const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
+    hipLaunchKernelGGL((hevi_tile<S, DIVDMP, TJ>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  } else {
+    constexpr unsigned TJ = 4;
+    const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
+    hipLaunchKernelGGL((hevi_march<S, TERRAIN, DIVDMP>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  }
   prof_stop(c, s);
   return hipGetLastError();
 }
@@ -1175,7 +1022,7 @@ hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
   a.divdmp_coef = io.divdmp_coef; a.dtrk = io.dtrk; a.has_coriolis = io.has_coriolis;
   a.DENS_RK = io.o[0]; a.MOMZ_RK = io.o[1]; a.MOMX_RK = io.o[2]; a.MOMY_RK = io.o[3]; a.RHOT_RK = io.o[4];
   a.mflx_hi = io.mflx_hi; a.tflx_hi = io.tflx_hi;
-  a.Sr = c.scr[0]; a.St = c.scr[1]; a.Sw = c.scr[2]; a.PT = c.scr[3]; a.e = c.scr[4]; a.f = c.scr[5];
+  a.PT = c.scr[3]; a.e = c.scr[4]; a.f = c.scr[5];
   const bool div = io.divdmp_coef > 0.0;
   const bool ter = c.terrain;
   if (c.scheme == UD3) {
