@@ -29,6 +29,28 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 STAGE_BYTES_PER_POINT = 45 * 8
 # short step = numfilter (7 read + 15 written) + 4 stages + RK4 combination (15 + 18 read)
 STEP_BYTES_PER_POINT = 22 * 8 + 4 * STAGE_BYTES_PER_POINT + 33 * 8
+# mixed precision: DPRES0, RT2P, REF_dens, REF_rhot stay FP64, the other 41 arrays are FP32
+MIXED_STAGE_BYTES_PER_POINT = 41 * 4 + 4 * 8
+
+
+def theta_check(case, nsteps):
+    """Potential temperature of the mixed-precision run against the FP64 run
+    after nsteps short steps from the same state."""
+    from scale_amd import hip
+    import numpy as np
+    g = case.grid
+    res = {}
+    for mixed in (False, True):
+        ctx = hip.model_from_case(case, mixed_precision=mixed)
+        ctx.model_prepare()
+        ctx.model_short_step(nsteps)
+        ctx.synchronize()
+        d, rt = ctx.model_get("DENS"), ctx.model_get("RHOT")
+        ctx.close()
+        sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+        res[mixed] = rt[sl] / d[sl]
+    diff = np.abs(res[True] - res[False])
+    return {"steps": nsteps, "max_abs_K": float(diff.max()), "rms_K": float(np.sqrt((diff * diff).mean()))}
 
 
 def parse():
@@ -41,6 +63,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-large-step", action="store_true", help="skip the informational large step with tracers")
     p.add_argument("--no-copy-probe", action="store_true", help="skip the HBM copy probe")
+    p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP32 + FP64 solve) timing")
+    p.add_argument("--theta-check", type=int, default=0, metavar="N",
+                   help="also run N short steps in FP64 and in mixed precision and report the potential-temperature difference")
     return p.parse_args()
 
 
@@ -108,6 +133,29 @@ def main():
         copy_gbs = 16.0 * n_copy / (copy_ms * 1e-3) / 1e9
     ctx.close()
 
+    # mixed precision: the same short steps with FP32 prognostics, fluxes and
+    # tendencies and the FP64 vertical solve / reference state
+    mixed = None
+    if not a.no_mixed:
+        mctx = hip.model_from_case(case, mixed_precision=True)
+        mctx.model_prepare()
+        mctx.model_short_step(a.warmup)
+        mctx.synchronize()
+        mctx.profile(True)
+        t0 = time.perf_counter()
+        mctx.model_short_step(a.steps)
+        mctx.synchronize()
+        m_el = time.perf_counter() - t0
+        m_stage = mctx.profile_query("hevi_stage")
+        mctx.profile(False)
+        mctx.close()
+        mixed = dict(value=ni * nj * nk * a.steps / m_el, ms_per_step=m_el * 1e3 / a.steps,
+                     stage_ms=m_stage[1] / max(m_stage[0], 1), speedup_vs_fp64=elapsed / m_el,
+                     stage_bytes_per_point=MIXED_STAGE_BYTES_PER_POINT)
+    theta = None
+    if a.theta_check > 0:
+        theta = theta_check(case, a.theta_check)
+
     pts = ni * nj * nk
     value = pts * a.steps / elapsed
     ms_step = elapsed * 1e3 / a.steps
@@ -137,7 +185,10 @@ def main():
                    "short_step_hbm_frac": value * STEP_BYTES_PER_POINT / 1e9 / HBM_PEAK_GBS,
                    "large_step_with_3_fct_tracers_ms": large_ms},
         "roofline": roof,
+        "fp32_vs_fp64": mixed,
     }
+    if theta is not None:
+        out["theta_check"] = theta
     if not a.no_cpu_baseline:
         cn, cj, ck = (int(x) for x in a.cpu_sample.split("x"))
         out["cpu_baseline"] = cpu_baseline(cn, cj, ck)

@@ -117,6 +117,11 @@ typedef struct {
   int32_t nd_order;    /* ATMOS_DYN_NUMERICAL_DIFF_order (1)                   */
   int32_t fct_tracer;  /* ATMOS_DYN_FLAG_FCT_TRACER                            */
   int32_t fct_along_stream; /* ATMOS_DYN_FLAG_FCT_along_stream (1)             */
+  int32_t mixed_precision;  /* 0: FP64 throughout. 1: prognostics, fluxes and
+                               tendencies in FP32 (the reference's SINGLE build),
+                               with the reference state, the pressure terms, the
+                               vertical solve and the budgets in FP64; flat
+                               terrain and QA = 0 only                         */
 } scale_hip_dyn_config;
 
 /* field ids for scale_hip_model_set/_get/_mass */

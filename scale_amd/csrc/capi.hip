@@ -492,6 +492,8 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   if (cfg->divdmp_coef != 0.0) return fail("divdmp_coef > 0 is not supported by the resident model");
   if (!(cfg->dtl > 0.0) || !(cfg->dts > 0.0)) return fail("dtl and dts must be positive");
   if (QA < 0) return fail("QA must be >= 0");
+  if (cfg->mixed_precision && c.terrain) return fail("mixed precision needs flat terrain (GSQRT = 1, J13G = J23G = 0, MAPF = 1)");
+  if (cfg->mixed_precision && QA > 0) return fail("mixed precision integrates the dynamics only (QA = 0)");
   shp::Model* M = new shp::Model();
   c.model = M;
   const size_t n3 = c.d.n3;
@@ -510,8 +512,23 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
     M->mfR[r] = A(3 * n3); M->tfR[r] = A(3 * n3);
   }
   M->ndq = A(3 * n3); M->DENS_RK = A(n3); M->QRK1 = A(n3); M->QRK2 = A(n3); M->zeros = A(n3);
+  if (cfg->mixed_precision) {
+    M->mixed = 1;
+    auto F = [&](size_t n) { return (float*)dalloc(c, (n + 1) / 2, &M->allocs); };
+    for (int v = 0; v < 5; ++v) { M->fS[v] = F(n3); M->ft[v] = F(n3); }
+    for (int r = 0; r < 4; ++r) {
+      for (int v = 0; v < 5; ++v) M->fR[r][v] = F(n3);
+      M->fmfR[r] = F(3 * n3); M->ftfR[r] = F(3 * n3);
+    }
+    M->fnum_diff = F(15 * n3); M->fDDIV = F(n3); M->fmflx = F(3 * n3); M->ftflx = F(3 * n3);
+  }
   for (void* p : M->allocs)
     if (!p) return fail("hipMalloc failed in scale_hip_model_create (out of device memory?)");
+  if (M->mixed) {
+    CK(hipMemsetAsync(M->fmflx, 0, 3 * n3 * sizeof(float), c.stream));
+    CK(hipMemsetAsync(M->ftflx, 0, 3 * n3 * sizeof(float), c.stream));
+    CK(hipMemsetAsync(M->fDDIV, 0, n3 * sizeof(float), c.stream));
+  }
   CK(hipMemsetAsync(M->mflx_hi, 0, 3 * n3 * sizeof(double), c.stream));
   CK(hipMemsetAsync(M->tflx_hi, 0, 3 * n3 * sizeof(double), c.stream));
   CK(hipMemsetAsync(M->ndq, 0, 3 * n3 * sizeof(double), c.stream));
@@ -582,6 +599,7 @@ int scale_hip_model_set(scale_hip_ctx* h, int32_t f, const double* host) {
   double* dst = model_field(M, f, &nc);
   if (!dst) return fail("unknown field id");
   int r = upload3(c, dst, host, nc);
+  if (r == 0 && f >= SCALE_HIP_DENS && f <= SCALE_HIP_RHOT) CK(shp::model_state_to_f32(c));
   if (r == 0) CK(hipStreamSynchronize(c.stream));
   return r;
 }
@@ -612,13 +630,18 @@ int scale_hip_model_device_ptr(scale_hip_ctx* h, int32_t f, double** dptr) {
 int scale_hip_model_large_step(scale_hip_ctx* h, int32_t n) {
   if (int r = check_ctx(h)) return r;
   if (!h->c.model) return fail("model not created");
-  for (int s = 0; s < n; ++s) CK(shp::model_large_step(h->c));
+  for (int s = 0; s < n; ++s) {
+    CK(shp::model_state_to_f64(h->c));  // the preparation reads the FP64 state
+    CK(shp::model_large_step(h->c));
+  }
+  CK(shp::model_state_to_f64(h->c));
   return 0;
 }
 
 int scale_hip_model_prepare(scale_hip_ctx* h) {
   if (int r = check_ctx(h)) return r;
   if (!h->c.model) return fail("model not created");
+  CK(shp::model_state_to_f64(h->c));
   CK(shp::model_prepare(h->c));
   return 0;
 }
@@ -630,6 +653,7 @@ int scale_hip_model_short_step(scale_hip_ctx* h, int32_t n) {
   const int nstep = (int)ceil((M.dtl - shp::EPS_RP) / M.dts);
   const double dts = M.dtl / nstep;
   for (int s = 0; s < n; ++s) CK(shp::model_short_step_rk4(h->c, dts));
+  CK(shp::model_state_to_f64(h->c));
   return 0;
 }
 

@@ -55,34 +55,39 @@ __host__ __device__ inline size_t IX(const Dims& d, int k, int i, int j) {
 
 // Fortran sign(a,b)
 __device__ __forceinline__ double fsign(double a, double b) { return __builtin_copysign(__builtin_fabs(a), b); }
+__device__ __forceinline__ float fsign(float a, float b) { return __builtin_copysignf(__builtin_fabsf(a), b); }
 __device__ __forceinline__ double fmax2(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double fmin2(double a, double b) { return a < b ? a : b; }
 
+// The face values below are written for a real type T: double reproduces the
+// reference's FP64 build; float is the FP32 arithmetic of the mixed-precision
+// path (the reference's SINGLE build, scale_precision.F90:37-43), with the
+// coefficients rounded to float once.
 // interior face between q0 = val(idx) and q1 = val(idx+1) (ERB flux(), order S)
-template <int S>
-__device__ __forceinline__ double fv_int(double qm1, double q0, double q1, double q2, double vel) {
+template <int S, typename T>
+__device__ __forceinline__ T fv_int(T qm1, T q0, T q1, T q2, T vel) {
   if constexpr (S == UD3)
-    return (F31 * (q2 + qm1) + F32 * (q1 + q0)) - (F31 * (q2 - qm1) + F33 * (q1 - q0)) * fsign(1.0, vel);
+    return (T(F31) * (q2 + qm1) + T(F32) * (q1 + q0)) - (T(F31) * (q2 - qm1) + T(F33) * (q1 - q0)) * fsign(T(1), vel);
   else if constexpr (S == CD4)
-    return F41 * (q1 + q0) + F42 * (q2 + qm1);
+    return T(F41) * (q1 + q0) + T(F42) * (q2 + qm1);
   else
-    return F1 * (q1 + q0) - fsign(F1, vel) * (q1 - q0);
+    return T(F1) * (q1 + q0) - fsign(T(F1), vel) * (q1 - q0);
 }
 // lowest face (val(KS), val(KS+1), val(KS+2)) -- scale_atmos_dyn_fvm_flux_ud3.F90:225-229
-template <int S>
-__device__ __forceinline__ double fv_lo(double q0, double q1, double q2, double vel) {
+template <int S, typename T>
+__device__ __forceinline__ T fv_lo(T q0, T q1, T q2, T vel) {
   if constexpr (S == UD3)
-    return F2 * (q1 + q0) * (0.5 + fsign(0.5, vel)) + (2.0 * q0 + 5.0 * q1 - q2) / 6.0 * (0.5 - fsign(0.5, vel));
+    return T(F2) * (q1 + q0) * (T(0.5) + fsign(T(0.5), vel)) + (T(2) * q0 + T(5) * q1 - q2) / T(6) * (T(0.5) - fsign(T(0.5), vel));
   else
-    return F2 * (q1 + q0);
+    return T(F2) * (q1 + q0);
 }
 // highest face (val(KE-2), val(KE-1), val(KE)) -- scale_atmos_dyn_fvm_flux_ud3.F90:231-235
-template <int S>
-__device__ __forceinline__ double fv_hi(double qm1, double q0, double q1, double vel) {
+template <int S, typename T>
+__device__ __forceinline__ T fv_hi(T qm1, T q0, T q1, T vel) {
   if constexpr (S == UD3)
-    return (2.0 * q1 + 5.0 * q0 - qm1) / 6.0 * (0.5 + fsign(0.5, vel)) + F2 * (q1 + q0) * (0.5 - fsign(0.5, vel));
+    return (T(2) * q1 + T(5) * q0 - qm1) / T(6) * (T(0.5) + fsign(T(0.5), vel)) + T(F2) * (q1 + q0) * (T(0.5) - fsign(T(0.5), vel));
   else
-    return F2 * (q1 + q0);
+    return T(F2) * (q1 + q0);
 }
 
 // Metric accessors.  TERRAIN=false is the flat Cartesian grid (GSQRT=1,

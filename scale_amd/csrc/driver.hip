@@ -18,14 +18,16 @@
 namespace shp {
 
 // ---- periodic halo fill ----------------------------------------------------
+template <typename T>
 struct HaloArgs {
   Dims d;
-  double* a[16];
+  T* a[16];
   int n;
   int nW;  // halo points per level: west/east strips then south/north rows
 };
 
-__global__ void halo_kernel(HaloArgs h) {
+template <typename T>
+__global__ void halo_kernel(HaloArgs<T> h) {
   const Dims& d = h.d;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
@@ -48,28 +50,31 @@ __global__ void halo_kernel(HaloArgs h) {
   }
   const int si = i < d.IS ? i + IMAX : (i > d.IE ? i - IMAX : i);
   const int sj = j < d.JS ? j + JMAX : (j > d.JE ? j - JMAX : j);
-  double* a = h.a[f];
+  T* a = h.a[f];
   a[IX(d, k, i, j)] = a[IX(d, k, si, sj)];
 }
 
-hipError_t halo_fill(const Ctx& c, double* const* arrs, int n, hipStream_t s) {
+template <typename T>
+static hipError_t halo_fill_t(const Ctx& c, T* const* arrs, int n, hipStream_t s) {
   const Dims& d = c.d;
   const int JMAX = d.JE - d.JS + 1;
   const int nW = 2 * d.IS * JMAX + 2 * d.JS * d.IA;
   for (int base = 0; base < n; base += 16) {
-    HaloArgs h;
+    HaloArgs<T> h;
     h.d = d;
     h.n = n - base < 16 ? n - base : 16;
     for (int q = 0; q < h.n; ++q) h.a[q] = arrs[base + q];
     h.nW = nW;
     prof_start(c, "halo_fill", s);
-    hipLaunchKernelGGL(halo_kernel, dim3((nW + 255) / 256, d.KA, h.n), dim3(256), 0, s, h);
+    hipLaunchKernelGGL(halo_kernel<T>, dim3((nW + 255) / 256, d.KA, h.n), dim3(256), 0, s, h);
     prof_stop(c, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
+hipError_t halo_fill(const Ctx& c, double* const* arrs, int n, hipStream_t s) { return halo_fill_t(c, arrs, n, s); }
+hipError_t halo_fill(const Ctx& c, float* const* arrs, int n, hipStream_t s) { return halo_fill_t(c, arrs, n, s); }
 
 // ---- layout transposes: Fortran (k,i,j) <-> device (i,j,k) -------------------
 // 32x32 tiles over (k, i) for each (j, component), staged through LDS so both
@@ -121,19 +126,23 @@ hipError_t to_fortran_layout(const Ctx& c, double* dst_f, const double* src, int
 }
 
 // ---- RK4 combination + ghost levels (RK4.F90:562-660, LARGE.F90:908-927) -------
+template <typename R>
 struct CombArgs {
   Dims d;
-  double* st[5];
-  const double *r1[5], *r2[5], *r3[5], *r4[5], *s0[5];
-  double *mflx, *tflx;
-  const double *m1, *m2, *m3, *m4, *t1, *t2, *t3, *t4;
+  R* st[5];
+  const R *r1[5], *r2[5], *r3[5], *r4[5], *s0[5];
+  R *mflx, *tflx;
+  const R *m1, *m2, *m3, *m4, *t1, *t2, *t3, *t4;
 };
 
-__device__ __forceinline__ double comb(const CombArgs& a, int v, size_t p) {
-  return (a.r1[v][p] * 2.0 + a.r2[v][p] * 4.0 + a.r3[v][p] * 2.0 + a.r4[v][p] - a.s0[v][p] * 3.0) / 6.0;
+template <typename R>
+__device__ __forceinline__ R comb(const CombArgs<R>& a, int v, size_t p) {
+  return (a.r1[v][p] * R(2) + a.r2[v][p] * R(4) + a.r3[v][p] * R(2) + a.r4[v][p] - a.s0[v][p] * R(3)) / R(6);
 }
 
-__global__ void __launch_bounds__(256) rk4_combine(CombArgs a) {
+// RK4 combination (tinteg_short_rk4.F90:560-660)
+template <typename R>
+__global__ void __launch_bounds__(256) rk4_combine(CombArgs<R> a) {
   // one thread per interior point; the threads of levels KS and KE also fill
   // the ghost levels below/above, so the state may be combined in place
   // (st aliasing s0: every thread reads its own point before writing it)
@@ -144,12 +153,12 @@ __global__ void __launch_bounds__(256) rk4_combine(CombArgs a) {
   if (i > d.IE || k > d.KE || j > d.JE) return;
   const size_t p = IX(d, k, i, j);
   const int KS = d.KS, KE = d.KE;
-  double v4[4];
+  R v4[4];
   const int vs[4] = {0, 2, 3, 4};
 #pragma unroll
   for (int q = 0; q < 4; ++q) v4[q] = comb(a, vs[q], p);
   // MOMZ: combine on KS..KE-1, MOMZ(KE) keeps the last stage's value (0)
-  const double w = (k <= KE - 1) ? comb(a, 1, p) : a.r4[1][p];
+  const R w = (k <= KE - 1) ? comb(a, 1, p) : a.r4[1][p];
 #pragma unroll
   for (int q = 0; q < 4; ++q) a.st[vs[q]][p] = v4[q];
   a.st[1][p] = w;
@@ -160,15 +169,27 @@ __global__ void __launch_bounds__(256) rk4_combine(CombArgs a) {
       const size_t g = IX(d, kk, i, j);
 #pragma unroll
       for (int q = 0; q < 4; ++q) a.st[vs[q]][g] = v4[q];
-      a.st[1][g] = 0.0;
+      a.st[1][g] = R(0);
     }
   }
 #pragma unroll
   for (int n = 0; n < 3; ++n) {
     const size_t q = p + d.n3 * (size_t)n;
-    a.mflx[q] = (a.m1[q] + a.m2[q] * 2.0 + a.m3[q] * 2.0 + a.m4[q]) / 6.0;
-    a.tflx[q] = (a.t1[q] + a.t2[q] * 2.0 + a.t3[q] * 2.0 + a.t4[q]) / 6.0;
+    a.mflx[q] = (a.m1[q] + a.m2[q] * R(2) + a.m3[q] * R(2) + a.m4[q]) / R(6);
+    a.tflx[q] = (a.t1[q] + a.t2[q] * R(2) + a.t3[q] * R(2) + a.t4[q]) / R(6);
   }
+}
+
+// element-wise conversion between the FP64 and FP32 copies of a field
+template <typename To, typename From>
+__global__ void convert_kernel(To* dst, const From* src, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = To(src[q]);
+}
+template <typename To, typename From>
+static hipError_t convert(To* dst, const From* src, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL((convert_kernel<To, From>), dim3(2048), dim3(256), 0, s, dst, src, n);
+  return hipGetLastError();
 }
 
 // ---- large-step preparation (LARGE.F90:443-471), dry constants ------------------
@@ -294,52 +315,96 @@ static inline dim3 grid_intk(const Dims& d) {  // interior columns, all levels
   } while (0)
 
 // ---- device-resident model ------------------------------------------------------
-hipError_t model_short_step_rk4(Ctx& c, double dts) {
+// the arrays one RK4 short step works on, in storage type R
+template <typename R>
+struct ShortBufs {
+  R* st[5];
+  R* (*Rk)[5];
+  R** t;
+  R* num_diff;
+  R** mfR;
+  R** tfR;
+  R *mflx, *tflx;
+  R* DDIV;
+};
+
+template <typename R>
+static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B) {
   Model& M = *c.model;
   const Dims& d = c.d;
   hipStream_t s = c.stream;
-  const size_t bytes = d.n3 * sizeof(double);
-  double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
   // numerical filter on the state at the start of the short step (LARGE.F90:845-858)
   if (M.nd_coef == 0.0) {
-    HIPCHK(hipMemsetAsync(M.num_diff, 0, 15 * bytes, s));
+    HIPCHK(hipMemsetAsync(B.num_diff, 0, 15 * d.n3 * sizeof(R), s));
   } else {
-    const double* cst[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
-    HIPCHK(numfilter(c, M.num_diff, cst, M.REF_dens, M.REF_pott, s));
+    const R* cst[5] = {B.st[0], B.st[1], B.st[2], B.st[3], B.st[4]};
+    HIPCHK(numfilter(c, B.num_diff, cst, M.REF_dens, M.REF_pott, s));
   }
   // DENS0 = DENS, ... (RK4.F90:326-336): the stages only read the state, so
   // it serves as the start value directly and is overwritten by the combination
   const double dtrk[4] = {dts / 2.0, dts / 2.0, dts, dts};
   for (int r = 0; r < 4; ++r) {
-    StageIO io;
+    StageIOT<R> io;
     io.ctx = &c;
     for (int v = 0; v < 5; ++v) {
-      io.o[v] = M.R[r][v];
-      io.s0[v] = st[v];
-      io.s[v] = r == 0 ? st[v] : M.R[r - 1][v];
-      io.t[v] = M.t[v];
+      io.o[v] = B.Rk[r][v];
+      io.s0[v] = B.st[v];
+      io.s[v] = r == 0 ? B.st[v] : B.Rk[r - 1][v];
+      io.t[v] = B.t[v];
     }
-    io.mflx_hi = M.mfR[r];
-    io.tflx_hi = M.tfR[r];
-    io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = M.num_diff;
-    io.wdamp_coef = M.wdamp_coef; io.DDIV = M.DDIV; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
+    io.mflx_hi = B.mfR[r];
+    io.tflx_hi = B.tfR[r];
+    io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = B.num_diff;
+    io.wdamp_coef = M.wdamp_coef; io.DDIV = B.DDIV; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
     io.divdmp_coef = M.divdmp_coef; io.dtrk = dtrk[r]; io.has_coriolis = M.has_coriolis;
     HIPCHK(hevi_stage(io, s));
-    if (r < 3) HIPCHK(halo_fill(c, M.R[r], 5, s));
+    if (r < 3) HIPCHK(halo_fill(c, B.Rk[r], 5, s));
   }
-  CombArgs a;
+  CombArgs<R> a;
   a.d = d;
   for (int v = 0; v < 5; ++v) {
-    a.st[v] = st[v]; a.r1[v] = M.R[0][v]; a.r2[v] = M.R[1][v]; a.r3[v] = M.R[2][v]; a.r4[v] = M.R[3][v]; a.s0[v] = st[v];
+    a.st[v] = B.st[v]; a.r1[v] = B.Rk[0][v]; a.r2[v] = B.Rk[1][v]; a.r3[v] = B.Rk[2][v]; a.r4[v] = B.Rk[3][v];
+    a.s0[v] = B.st[v];
   }
-  a.mflx = M.mflx_hi; a.tflx = M.tflx_hi;
-  a.m1 = M.mfR[0]; a.m2 = M.mfR[1]; a.m3 = M.mfR[2]; a.m4 = M.mfR[3];
-  a.t1 = M.tfR[0]; a.t2 = M.tfR[1]; a.t3 = M.tfR[2]; a.t4 = M.tfR[3];
+  a.mflx = B.mflx; a.tflx = B.tflx;
+  a.m1 = B.mfR[0]; a.m2 = B.mfR[1]; a.m3 = B.mfR[2]; a.m4 = B.mfR[3];
+  a.t1 = B.tfR[0]; a.t2 = B.tfR[1]; a.t3 = B.tfR[2]; a.t4 = B.tfR[3];
   prof_start(c, "rk4_combine", s);
-  hipLaunchKernelGGL(rk4_combine, grid_int(d), dim3(64, 4), 0, s, a);
+  hipLaunchKernelGGL(rk4_combine<R>, grid_int(d), dim3(64, 4), 0, s, a);
   prof_stop(c, s);
   HIPCHK(hipGetLastError());
-  return halo_fill(c, st, 5, s);
+  return halo_fill(c, B.st, 5, s);
+}
+
+hipError_t model_short_step_rk4(Ctx& c, double dts) {
+  Model& M = *c.model;
+  if (M.mixed) {
+    ShortBufs<float> B{{M.fS[0], M.fS[1], M.fS[2], M.fS[3], M.fS[4]}, M.fR, M.ft, M.fnum_diff, M.fmfR, M.ftfR, M.fmflx,
+                       M.ftflx, M.fDDIV};
+    return short_step_t(c, dts, B);
+  }
+  ShortBufs<double> B{{M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT}, M.R, M.t, M.num_diff, M.mfR, M.tfR, M.mflx_hi,
+                      M.tflx_hi, M.DDIV};
+  return short_step_t(c, dts, B);
+}
+
+// mixed precision: the FP32 state is the integrated one; these keep the FP64
+// copies (read by model_get, the budgets and the large-step preparation) and
+// the FP32 copies in step
+hipError_t model_state_to_f64(Ctx& c) {
+  Model& M = *c.model;
+  if (!M.mixed) return hipSuccess;
+  double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
+  for (int v = 0; v < 5; ++v) HIPCHK(convert(st[v], (const float*)M.fS[v], c.d.n3, c.stream));
+  HIPCHK(convert(M.mflx_hi, (const float*)M.fmflx, 3 * c.d.n3, c.stream));
+  return convert(M.tflx_hi, (const float*)M.ftflx, 3 * c.d.n3, c.stream);
+}
+hipError_t model_state_to_f32(Ctx& c) {
+  Model& M = *c.model;
+  if (!M.mixed) return hipSuccess;
+  const double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
+  for (int v = 0; v < 5; ++v) HIPCHK(convert(M.fS[v], st[v], c.d.n3, c.stream));
+  return hipSuccess;
 }
 
 // RT2P, DPRES0, REF_rhot, DENS00, tendencies (LARGE.F90:443-839)
@@ -355,6 +420,7 @@ hipError_t model_prepare(Ctx& c) {
   for (int v = 0; v < 5; ++v) {
     hipLaunchKernelGGL(tend_prep, grid_full(d), dim3(64, 4), 0, s, d, M.t[v], (const double*)M.tp[v], v == 1 ? 1 : 0);
     HIPCHK(hipGetLastError());
+    if (M.mixed) HIPCHK(convert(M.ft[v], (const double*)M.t[v], d.n3, s));
   }
   return hipSuccess;
 }

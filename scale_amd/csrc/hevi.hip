@@ -22,24 +22,34 @@
 
 namespace shp {
 
-struct HeviArgs {
+// Arguments of a stage.  R is the type of the prognostic, tendency, flux and
+// num_diff arrays: double, or float for the mixed-precision path, which keeps
+// the reference state, the pressure terms (DPRES0, RT2P, REF_*) and the
+// vertical solve (PT, e, f) in double.
+template <typename R>
+struct HeviArgsT {
   const Ctx* ctx;  // host side only
   Dims d;
   Grid1D g;
   const double *GSQRT, *J13G, *J23G, *MAPF, *PHI;
   double J33G;
   // inputs (device layout)
-  const double *DENS0, *MOMZ0, *MOMX0, *MOMY0, *RHOT0;
-  const double *DENS, *MOMZ, *MOMX, *MOMY, *RHOT;
-  const double *DENS_t, *MOMZ_t, *MOMX_t, *MOMY_t, *RHOT_t;
-  const double *DPRES0, *RT2P, *CORIOLI, *num_diff, *wdamp_coef, *DDIV, *REF_dens, *REF_rhot;
+  const R *DENS0, *MOMZ0, *MOMX0, *MOMY0, *RHOT0;
+  const R *DENS, *MOMZ, *MOMX, *MOMY, *RHOT;
+  const R *DENS_t, *MOMZ_t, *MOMX_t, *MOMY_t, *RHOT_t;
+  const double *DPRES0, *RT2P, *CORIOLI;
+  const R* num_diff;
+  const double* wdamp_coef;
+  const R* DDIV;
+  const double *REF_dens, *REF_rhot;
   double divdmp_coef, dtrk;
   int has_coriolis;
   // outputs
-  double *DENS_RK, *MOMZ_RK, *MOMX_RK, *MOMY_RK, *RHOT_RK, *mflx_hi, *tflx_hi;
+  R *DENS_RK, *MOMZ_RK, *MOMX_RK, *MOMY_RK, *RHOT_RK, *mflx_hi, *tflx_hi;
   // scratch (n3 each)
   double *PT, *e, *f;
 };
+using HeviArgs = HeviArgsT<double>;
 
 template <int S, bool TERRAIN, bool DIVDMP>
 struct Ops {
@@ -607,15 +617,16 @@ __global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles,
 // general expressions is a multiplication or division by 1.0 (or an addition
 // of 0.0), so the expressions below, written without them, give the same
 // IEEE results (HEVI.F90:346-1047; UD3.F90/CD4.F90 flux routines).
-template <int S, bool DIVDMP, int TJ>
-__global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_tiles, unsigned nblocks) {
+template <int S, bool DIVDMP, int TJ, typename R>
+__global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni_tiles, unsigned nblocks) {
   constexpr int TI = 64, CW = TI + 4, RW = TJ + 4, NT = TI * TJ;
-  __shared__ double sD[2][RW][CW], sMX[2][RW][CW], sMY[2][RW][CW];
-  __shared__ double sPO[RW][CW], sW[RW][CW], sDP[TJ + 1][TI + 1];
+  __shared__ R sD[2][RW][CW], sMX[2][RW][CW], sMY[2][RW][CW];
+  __shared__ R sPO[RW][CW], sW[RW][CW];
+  __shared__ double sDP[TJ + 1][TI + 1];
   // x faces: position x = 0..TI is the face east of column i0-1+x
-  __shared__ double xM[TJ][TI + 1], xT[TJ][TI + 1], xW[TJ][TI + 1], xU[TJ][TI + 1], xV[TJ][TI + 1];
+  __shared__ R xM[TJ][TI + 1], xT[TJ][TI + 1], xW[TJ][TI + 1], xU[TJ][TI + 1], xV[TJ][TI + 1];
   // y faces: position y = 0..TJ is the face north of row j0-1+y
-  __shared__ double yM[TJ + 1][TI], yT[TJ + 1][TI], yW[TJ + 1][TI], yU[TJ + 1][TI], yV[TJ + 1][TI];
+  __shared__ R yM[TJ + 1][TI], yT[TJ + 1][TI], yW[TJ + 1][TI], yU[TJ + 1][TI], yV[TJ + 1][TI];
 
   const Dims& d = a.d;
   const unsigned lb = xcd_block(blockIdx.x, nblocks);
@@ -629,16 +640,16 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
   const double* CDZ = a.g.CDZ; const double* RCDZ = a.g.RCDZ; const double* RFDZ = a.g.RFDZ;
   const double* RCDX = a.g.RCDX; const double* RCDY = a.g.RCDY; const double* RFDX = a.g.RFDX; const double* RFDY = a.g.RFDY;
   const double dtrk = a.dtrk, J33G = a.J33G;
-  const double* ND = a.num_diff;  // component (v, dir) at ND + n3*(v + 5*dir)
+  const R* ND = a.num_diff;  // component (v, dir) at ND + n3*(v + 5*dir)
   auto nd = [&](int v, int dir, size_t q) { return ND[q + n3 * (size_t)(v + 5 * dir)]; };
   auto gix = [&](int k, int ii, int jj) { return IX(d, k, ii, jj); };
 
   // (a) tile staging helpers: tile cell (r, c) is column i0-2+c, row j0-2+r
-  auto fill = [&](double (*dst)[CW], const double* src, int k) {
+  auto fill = [&](R (*dst)[CW], const R* src, int k) {
     for (int q = tid; q < RW * CW; q += NT) {
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      dst[r][c] = (ii < d.IA && jj < d.JA) ? src[gix(k, ii, jj)] : 0.0;
+      dst[r][c] = (ii < d.IA && jj < d.JA) ? src[gix(k, ii, jj)] : R(0);
     }
   };
   int s0 = 0;  // ring slot of level k; slot s0^1 holds level k+1
@@ -647,8 +658,8 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
 
   // own-column windows at k-1..k+2 and carried z-face fluxes
   const size_t p0 = own ? gix(0, i, j) : 0;
-  auto at = [&](const double* f, int k) { return f[p0 + d.pl * (size_t)k]; };
-  double pw[4] = {0.0, 0.0, 0.0, 0.0}, ww[4] = {0.0, 0.0, 0.0, 0.0}, xw[4] = {0.0, 0.0, 0.0, 0.0}, yw[4] = {0.0, 0.0, 0.0, 0.0};
+  auto at = [&](const R* f, int k) { return f[p0 + d.pl * (size_t)k]; };
+  R pw[4] = {0, 0, 0, 0}, ww[4] = {0, 0, 0, 0}, xw[4] = {0, 0, 0, 0}, yw[4] = {0, 0, 0, 0};
   if (own) {
     for (int q = 1; q < 4; ++q) {
       const int k = KS - 2 + q;
@@ -656,7 +667,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
       ww[q] = at(a.MOMZ, k); xw[q] = at(a.MOMX, k); yw[q] = at(a.MOMY, k);
     }
   }
-  double cmZ = 0.0, ctZ = 0.0, cqW = 0.0, cqU = 0.0, cqV = 0.0;  // fluxes through face k-1 (zero at KS-1)
+  R cmZ = 0, ctZ = 0, cqW = 0, cqU = 0, cqV = 0;  // fluxes through face k-1 (zero at KS-1)
   double rt0 = 0.0, dp0 = 0.0, st0 = 0.0, sr0 = 0.0, d0 = 0.0, rd0 = 0.0, A0 = 0.0, pt0 = 0.0, ptm = 0.0, sw0 = 0.0;
   double e_prev = 0.0, f_prev = 0.0, C_last = 0.0;
 
@@ -665,9 +676,10 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
     // ---- loads of this level that do not go through LDS, issued first so
     // that their latency overlaps the staging and the face evaluation ----
     const size_t gk = own ? gix(k, i, j) : 0;
-    double o_nZD = 0.0, o_nZT = 0.0, o_nZW = 0.0, o_nZX = 0.0, o_nZY = 0.0;
-    double o_Dt = 0.0, o_Rt = 0.0, o_Wt = 0.0, o_Xt = 0.0, o_Yt = 0.0, o_X0 = 0.0, o_Y0 = 0.0, o_RT = 0.0, o_RD = 0.0, o_Z0 = 0.0;
-    double o_R2 = 1.0, o_D2 = 1.0, o_W2 = 0.0, o_X2 = 0.0, o_Y2 = 0.0;
+    R o_nZD = 0, o_nZT = 0, o_nZW = 0, o_nZX = 0, o_nZY = 0;
+    R o_Dt = 0, o_Rt = 0, o_Wt = 0, o_Xt = 0, o_Yt = 0, o_X0 = 0, o_Y0 = 0, o_Z0 = 0;
+    double o_RT = 0.0, o_RD = 0.0;
+    R o_R2 = 1, o_D2 = 1, o_W2 = 0, o_X2 = 0, o_Y2 = 0;
     const double wdc = a.wdamp_coef[k];
     if (own) {
       o_nZD = nd(I_DENS, ZDIR, gk); o_nZT = nd(I_RHOT, ZDIR, gk); o_nZW = nd(I_MOMZ, ZDIR, gk + d.pl);
@@ -687,7 +699,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
       const size_t g = gix(k, ii, jj);
       xM[row][x] = nd(I_DENS, XDIR, g);
       xT[row][x] = nd(I_RHOT, XDIR, g);
-      xW[row][x] = (k < KE) ? nd(I_MOMZ, XDIR, g) : 0.0;
+      xW[row][x] = (k < KE) ? nd(I_MOMZ, XDIR, g) : R(0);
       xU[row][x] = nd(I_MOMX, XDIR, g + 1);
       xV[row][x] = nd(I_MOMY, XDIR, g);
     }
@@ -698,7 +710,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
       const size_t g = gix(k, ii, jj);
       yM[y][col] = nd(I_DENS, YDIR, g);
       yT[y][col] = nd(I_RHOT, YDIR, g);
-      yW[y][col] = (k < KE) ? nd(I_MOMZ, YDIR, g) : 0.0;
+      yW[y][col] = (k < KE) ? nd(I_MOMZ, YDIR, g) : R(0);
       yU[y][col] = nd(I_MOMX, YDIR, g);
       yV[y][col] = nd(I_MOMY, YDIR, g + d.IA);
     }
@@ -708,7 +720,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
     for (int q = tid; q < RW * CW; q += NT) {
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      sPO[r][c] = (ii < d.IA && jj < d.JA) ? a.RHOT[gix(k, ii, jj)] / sD[s0][r][c] : 0.0;
+      sPO[r][c] = (ii < d.IA && jj < d.JA) ? a.RHOT[gix(k, ii, jj)] / sD[s0][r][c] : R(0);
     }
     for (int q = tid; q < (TJ + 1) * (TI + 1); q += NT) {
       const int r = q / (TI + 1), c = q - r * (TI + 1);
@@ -716,7 +728,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
       double v = 0.0;
       if (ii < d.IA && jj < d.JA) {
         const size_t g = gix(k, ii, jj);
-        v = a.DPRES0[g] + a.RT2P[g] * (a.RHOT[g] - a.REF_rhot[g]);
+        v = a.DPRES0[g] + a.RT2P[g] * ((double)a.RHOT[g] - a.REF_rhot[g]);
       }
       sDP[r][c] = v;
     }
@@ -730,35 +742,35 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
     __syncthreads();
 
     // ---- (b) faces ----
-    const double f1 = CDZ[k] / (CDZ[k] + CDZ[k + 1]);      // F2H(k,1)
-    const double f2 = CDZ[k + 1] / (CDZ[k] + CDZ[k + 1]);  // F2H(k,2)
+    const R f1 = R(CDZ[k] / (CDZ[k] + CDZ[k + 1]));      // F2H(k,1)
+    const R f2 = R(CDZ[k + 1] / (CDZ[k] + CDZ[k + 1]));  // F2H(k,2)
     auto D0 = [&](int r, int c) { return sD[s0][r][c]; };
     auto D1 = [&](int r, int c) { return sD[s1][r][c]; };
     for (int q = tid; q < TJ * (TI + 1); q += NT) {
       const int row = q / (TI + 1), x = q - row * (TI + 1);
       const int ii = i0 - 1 + x, jj = j0 + row;
       if (ii > d.IE || jj > d.JE) continue;
-      const double ndq[5] = {xM[row][x], xT[row][x], xW[row][x], xU[row][x], xV[row][x]};
+      const R ndq[5] = {xM[row][x], xT[row][x], xW[row][x], xU[row][x], xV[row][x]};
       const int r = row + 2, c = x + 1;  // tile cell of column ii
       const size_t g = gix(k, ii, jj);
-      const double vm = sMX[s0][r][c] + ndq[0];
+      const R vm = sMX[s0][r][c] + ndq[0];
       xM[row][x] = vm;
       xT[row][x] = vm * fv_int<S>(sPO[r][c - 1], sPO[r][c], sPO[r][c + 1], sPO[r][c + 2], vm) + ndq[1];
-      double qw = 0.0;
+      R qw = 0;
       if (k < KE) {
-        const double vel = (f1 * sMX[s1][r][c] + f2 * sMX[s0][r][c])
-                         / (f1 * 0.5 * (D1(r, c) + D1(r, c + 1)) + f2 * 0.5 * (D0(r, c) + D0(r, c + 1)));
+        const R vel = (f1 * sMX[s1][r][c] + f2 * sMX[s0][r][c])
+                         / (f1 * R(0.5) * (D1(r, c) + D1(r, c + 1)) + f2 * R(0.5) * (D0(r, c) + D0(r, c + 1)));
         qw = vel * fv_int<S>(sW[r][c - 1], sW[r][c], sW[r][c + 1], sW[r][c + 2], vel) + ndq[2];
       }
       xW[row][x] = qw;
       {  // fluxX_UYZ stored at ii (loop index ii+1)
-        const double vel = (0.5 * (sMX[s0][r][c + 1] + sMX[s0][r][c])) / D0(r, c + 1);
+        const R vel = (R(0.5) * (sMX[s0][r][c + 1] + sMX[s0][r][c])) / D0(r, c + 1);
         xU[row][x] = vel * fv_int<S>(sMX[s0][r][c - 1], sMX[s0][r][c], sMX[s0][r][c + 1], sMX[s0][r][c + 2], vel)
                    + ndq[3];
       }
       {  // fluxX_XVZ at (ii, jj)
-        const double vel = (0.5 * (sMX[s0][r][c] + sMX[s0][r + 1][c]))
-                         / (0.25 * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1)));
+        const R vel = (R(0.5) * (sMX[s0][r][c] + sMX[s0][r + 1][c]))
+                    / (R(0.25) * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1)));
         xV[row][x] = vel * fv_int<S>(sMY[s0][r][c - 1], sMY[s0][r][c], sMY[s0][r][c + 1], sMY[s0][r][c + 2], vel)
                    + ndq[4];
       }
@@ -771,27 +783,27 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
       const int y = q / TI, col = q - y * TI;
       const int ii = i0 + col, jj = j0 - 1 + y;
       if (ii > d.IE || jj > d.JE) continue;
-      const double ndq[5] = {yM[y][col], yT[y][col], yW[y][col], yU[y][col], yV[y][col]};
+      const R ndq[5] = {yM[y][col], yT[y][col], yW[y][col], yU[y][col], yV[y][col]};
       const int r = y + 1, c = col + 2;  // tile cell of row jj
       const size_t g = gix(k, ii, jj);
-      const double vm = sMY[s0][r][c] + ndq[0];
+      const R vm = sMY[s0][r][c] + ndq[0];
       yM[y][col] = vm;
       yT[y][col] = vm * fv_int<S>(sPO[r - 1][c], sPO[r][c], sPO[r + 1][c], sPO[r + 2][c], vm) + ndq[1];
-      double qw = 0.0;
+      R qw = 0;
       if (k < KE) {
-        const double vel = (f1 * sMY[s1][r][c] + f2 * sMY[s0][r][c])
-                         / (f1 * 0.5 * (D1(r, c) + D1(r + 1, c)) + f2 * 0.5 * (D0(r, c) + D0(r + 1, c)));
+        const R vel = (f1 * sMY[s1][r][c] + f2 * sMY[s0][r][c])
+                         / (f1 * R(0.5) * (D1(r, c) + D1(r + 1, c)) + f2 * R(0.5) * (D0(r, c) + D0(r + 1, c)));
         qw = vel * fv_int<S>(sW[r - 1][c], sW[r][c], sW[r + 1][c], sW[r + 2][c], vel) + ndq[2];
       }
       yW[y][col] = qw;
       {  // fluxY_UYZ at (ii, jj)
-        const double vel = (0.5 * (sMY[s0][r][c] + sMY[s0][r][c + 1]))
-                         / (0.25 * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1)));
+        const R vel = (R(0.5) * (sMY[s0][r][c] + sMY[s0][r][c + 1]))
+                    / (R(0.25) * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1)));
         yU[y][col] = vel * fv_int<S>(sMX[s0][r - 1][c], sMX[s0][r][c], sMX[s0][r + 1][c], sMX[s0][r + 2][c], vel)
                    + ndq[3];
       }
       {  // fluxY_XVZ stored at jj (loop index jj+1)
-        const double vel = (0.5 * (sMY[s0][r + 1][c] + sMY[s0][r][c])) / D0(r + 1, c);
+        const R vel = (R(0.5) * (sMY[s0][r + 1][c] + sMY[s0][r][c])) / D0(r + 1, c);
         yV[y][col] = vel * fv_int<S>(sMY[s0][r - 1][c], sMY[s0][r][c], sMY[s0][r + 1][c], sMY[s0][r + 2][c], vel)
                    + ndq[4];
       }
@@ -806,97 +818,99 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
     if (own) {
       const int r = ty + 2, c = tx + 2;  // tile cell of the own column
       const size_t g = gix(k, i, j);
+      const R rcdz = R(RCDZ[k]), rcdx = R(RCDX[i]), rcdy = R(RCDY[j]), rfdx = R(RFDX[i]), rfdy = R(RFDY[j]);
       // continuity and theta (HEVI.F90:369-475, 588-657)
-      const double mZ = (k <= KE - 1) ? o_nZD : 0.0;
-      const double Sr = (-(mZ - cmZ) * RCDZ[k] + -((xM[ty][tx + 1] - xM[ty][tx]) * RCDX[i] + (yM[ty + 1][tx] - yM[ty][tx]) * RCDY[j]))
-                      + o_Dt;
-      double tZ = 0.0;
+      const R mZ = (k <= KE - 1) ? o_nZD : R(0);
+      const R Sr = (-(mZ - cmZ) * rcdz + -((xM[ty][tx + 1] - xM[ty][tx]) * rcdx + (yM[ty + 1][tx] - yM[ty][tx]) * rcdy))
+                 + o_Dt;
+      R tZ = 0;
       if (k <= KE - 1) {
-        double fv;
+        R fv;
         if (k == KS) fv = fv_lo<S>(pw[1], pw[2], pw[3], mZ);
         else if (k == KE - 1) fv = fv_hi<S>(pw[0], pw[1], pw[2], mZ);
         else fv = fv_int<S>(pw[0], pw[1], pw[2], pw[3], mZ);
         tZ = mZ * fv + o_nZT;
       }
-      const double St = (-(tZ - ctZ) * RCDZ[k] + -((xT[ty][tx + 1] - xT[ty][tx]) * RCDX[i] + (yT[ty + 1][tx] - yT[ty][tx]) * RCDY[j]))
-                      + o_Rt;
+      const R St = (-(tZ - ctZ) * rcdz + -((xT[ty][tx + 1] - xT[ty][tx]) * rcdx + (yT[ty + 1][tx] - yT[ty][tx]) * rcdy))
+                 + o_Rt;
       // momentum z, explicit part (HEVI.F90:486-577) and PT
-      double Sw = 0.0, PT = 0.0;
+      R Sw = 0, PT = 0;
       if (k <= KE - 1) {
-        double qw = 0.0;
+        R qw = 0;
         if (k < KE - 1) {
-          const double vel = (0.5 * (ww[1] + ww[2])) / D1(r, c);
-          const double fv = (k == KS) ? fv_lo<S>(ww[1], ww[2], ww[3], vel) : fv_int<S>(ww[0], ww[1], ww[2], ww[3], vel);
-          qw = J33G * vel * fv + o_nZW;
+          const R vel = (R(0.5) * (ww[1] + ww[2])) / D1(r, c);
+          const R fv = (k == KS) ? fv_lo<S>(ww[1], ww[2], ww[3], vel) : fv_int<S>(ww[0], ww[1], ww[2], ww[3], vel);
+          qw = R(J33G) * vel * fv + o_nZW;
         }
-        const double advcv = -(qw - cqW) * RFDZ[k];
+        const R advcv = -(qw - cqW) * R(RFDZ[k]);
         cqW = qw;
-        const double advch = -((xW[ty][tx + 1] - xW[ty][tx]) * RCDX[i] + (yW[ty + 1][tx] - yW[ty][tx]) * RCDY[j]);
-        const double wdmp = -wdc * o_Z0;
-        double sw = (advcv + advch) + wdmp;
-        if constexpr (DIVDMP) sw = sw + a.divdmp_coef / dtrk * (a.DDIV[g + d.pl] - a.DDIV[g]) * a.g.FDZ[k];
+        const R advch = -((xW[ty][tx + 1] - xW[ty][tx]) * rcdx + (yW[ty + 1][tx] - yW[ty][tx]) * rcdy);
+        const R wdmp = -R(wdc) * o_Z0;
+        R sw = (advcv + advch) + wdmp;
+        if constexpr (DIVDMP) sw = sw + R(a.divdmp_coef / dtrk) * (a.DDIV[g + d.pl] - a.DDIV[g]) * R(a.g.FDZ[k]);
         Sw = sw + o_Wt;
-        const double vel = ww[1];
+        const R vel = ww[1];
         if (k == KS) PT = fv_lo<S>(pw[1], pw[2], pw[3], vel);
         else if (k == KE - 1) PT = fv_hi<S>(pw[0], pw[1], pw[2], vel);
         else PT = fv_int<S>(pw[0], pw[1], pw[2], pw[3], vel);
       }
-      // momentum x (HEVI.F90:816-927)
-      double MXn;
+      // momentum x (HEVI.F90:816-927); the pressure gradient in double
+      R MXn;
       {
-        double qu = 0.0;
+        R qu = 0;
         if (k < KE) {
-          const double vel = (0.5 * (ww[1] + sW[r][c + 1]))
-                           / (f1 * 0.5 * (D1(r, c) + D1(r, c + 1)) + f2 * 0.5 * (D0(r, c) + D0(r, c + 1)));
-          double fv;
+          const R vel = (R(0.5) * (ww[1] + sW[r][c + 1]))
+                      / (f1 * R(0.5) * (D1(r, c) + D1(r, c + 1)) + f2 * R(0.5) * (D0(r, c) + D0(r, c + 1)));
+          R fv;
           if (k == KS) fv = fv_lo<S>(xw[1], xw[2], xw[3], vel);
           else if (k == KE - 1) fv = fv_hi<S>(xw[0], xw[1], xw[2], vel);
           else fv = fv_int<S>(xw[0], xw[1], xw[2], xw[3], vel);
-          qu = J33G * vel * fv + o_nZX;
+          qu = R(J33G) * vel * fv + o_nZX;
         }
-        const double advcv = -(qu - cqU) * RCDZ[k];
+        const R advcv = -(qu - cqU) * rcdz;
         cqU = qu;
-        const double advch = -((xU[ty][tx + 1] - xU[ty][tx]) * RFDX[i] + (yU[ty + 1][tx] - yU[ty][tx]) * RCDY[j]);
-        const double pg = (sDP[ty][tx + 1] - sDP[ty][tx]) * RFDX[i];
-        double t = advcv + advch - pg;
+        const R advch = -((xU[ty][tx + 1] - xU[ty][tx]) * rfdx + (yU[ty + 1][tx] - yU[ty][tx]) * rcdy);
+        const R pg = R((sDP[ty][tx + 1] - sDP[ty][tx]) * RFDX[i]);
+        R t = advcv + advch - pg;
         if (a.has_coriolis) {
-          const double cf = 0.125 * (a.CORIOLI[(size_t)(i + 1) + (size_t)d.IA * j] + a.CORIOLI[(size_t)i + (size_t)d.IA * j])
-                          * (sMY[s0][r][c + 1] + sMY[s0][r][c] + sMY[s0][r - 1][c + 1] + sMY[s0][r - 1][c]);
+          const R cf = R(0.125 * (a.CORIOLI[(size_t)(i + 1) + (size_t)d.IA * j] + a.CORIOLI[(size_t)i + (size_t)d.IA * j]))
+                     * (sMY[s0][r][c + 1] + sMY[s0][r][c] + sMY[s0][r - 1][c + 1] + sMY[s0][r - 1][c]);
           t = t + cf;
         }
-        if constexpr (DIVDMP) t = t + a.divdmp_coef / dtrk * (a.DDIV[g + 1] - a.DDIV[g]) * a.g.FDX[i];
-        MXn = o_X0 + dtrk * (t + o_Xt);
+        if constexpr (DIVDMP) t = t + R(a.divdmp_coef / dtrk) * (a.DDIV[g + 1] - a.DDIV[g]) * R(a.g.FDX[i]);
+        MXn = o_X0 + R(dtrk) * (t + o_Xt);
       }
       // momentum y (HEVI.F90:936-1047)
-      double MYn;
+      R MYn;
       {
-        double qv = 0.0;
+        R qv = 0;
         if (k < KE) {
-          const double vel = (0.5 * (ww[1] + sW[r + 1][c]))
-                           / (f1 * 0.5 * (D1(r, c) + D1(r + 1, c)) + f2 * 0.5 * (D0(r, c) + D0(r + 1, c)));
-          double fv;
+          const R vel = (R(0.5) * (ww[1] + sW[r + 1][c]))
+                      / (f1 * R(0.5) * (D1(r, c) + D1(r + 1, c)) + f2 * R(0.5) * (D0(r, c) + D0(r + 1, c)));
+          R fv;
           if (k == KS) fv = fv_lo<S>(yw[1], yw[2], yw[3], vel);
           else if (k == KE - 1) fv = fv_hi<S>(yw[0], yw[1], yw[2], vel);
           else fv = fv_int<S>(yw[0], yw[1], yw[2], yw[3], vel);
-          qv = J33G * vel * fv + o_nZY;
+          qv = R(J33G) * vel * fv + o_nZY;
         }
-        const double advcv = -(qv - cqV) * RCDZ[k];
+        const R advcv = -(qv - cqV) * rcdz;
         cqV = qv;
-        const double advch = -((xV[ty][tx + 1] - xV[ty][tx]) * RCDX[i] + (yV[ty + 1][tx] - yV[ty][tx]) * RFDY[j]);
-        const double pg = (sDP[ty + 1][tx] - sDP[ty][tx]) * RFDY[j];
-        double t = advcv + advch - pg;
+        const R advch = -((xV[ty][tx + 1] - xV[ty][tx]) * rcdx + (yV[ty + 1][tx] - yV[ty][tx]) * rfdy);
+        const R pg = R((sDP[ty + 1][tx] - sDP[ty][tx]) * RFDY[j]);
+        R t = advcv + advch - pg;
         if (a.has_coriolis) {
-          const double cf = -0.125 * (a.CORIOLI[(size_t)i + (size_t)d.IA * (j + 1)] + a.CORIOLI[(size_t)i + (size_t)d.IA * j])
-                          * (sMX[s0][r + 1][c] + sMX[s0][r][c] + sMX[s0][r + 1][c - 1] + sMX[s0][r][c - 1]);
+          const R cf = R(-0.125 * (a.CORIOLI[(size_t)i + (size_t)d.IA * (j + 1)] + a.CORIOLI[(size_t)i + (size_t)d.IA * j]))
+                     * (sMX[s0][r + 1][c] + sMX[s0][r][c] + sMX[s0][r + 1][c - 1] + sMX[s0][r][c - 1]);
           t = t + cf;
         }
-        if constexpr (DIVDMP) t = t + a.divdmp_coef / dtrk * (a.DDIV[g + d.IA] - a.DDIV[g]) * a.g.FDY[j];
-        MYn = o_Y0 + dtrk * (t + o_Yt);
+        if constexpr (DIVDMP) t = t + R(a.divdmp_coef / dtrk) * (a.DDIV[g + d.IA] - a.DDIV[g]) * R(a.g.FDY[j]);
+        MYn = o_Y0 + R(dtrk) * (t + o_Yt);
       }
-      // tridiagonal row k-1 and forward sweep (HEVI.F90:688-727, 1092-1115)
-      const double rt1 = o_RT, dp1 = sDP[ty][tx], d1 = D0(r, c), rd1 = o_RD;
+      // tridiagonal row k-1 and forward sweep in double (HEVI.F90:688-727, 1092-1115)
+      const double rt1 = o_RT, dp1 = sDP[ty][tx], d1 = (double)D0(r, c), rd1 = o_RD;
+      const double St1 = (double)St, Sr1 = (double)Sr;
       const double A1 = dtrk * dtrk * J33G * RCDZ[k] * rt1 * J33G;
-      const double pt1 = (k <= KE - 1) ? PT : 0.0;
+      const double pt1 = (k <= KE - 1) ? (double)PT : 0.0;
       double ek = 0.0, fk = 0.0;
       if (k > KS) {
         const int rr = k - 1;
@@ -905,9 +919,9 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
         const double F2v = 1.0 + (pt0 * RFDZ[rr] * (A1 + A0));
         const double F3v = -(ptm * RFDZ[rr] * A0 - B);
         const double h1 = CDZ[rr] / (CDZ[rr] + CDZ[rr + 1]), h2 = CDZ[rr + 1] / (CDZ[rr] + CDZ[rr + 1]);
-        const double pg = -(dp1 + rt1 * dtrk * St - dp0 - rt0 * dtrk * st0) * RFDZ[rr] * J33G
-                        - GRAV * (h1 * (d1 - rd1 + Sr * dtrk) + h2 * (d0 - rd0 + sr0 * dtrk));
-        const double Ck = ww[0] + dtrk * (pg + sw0);
+        const double pg = -(dp1 + rt1 * dtrk * St1 - dp0 - rt0 * dtrk * st0) * RFDZ[rr] * J33G
+                        - GRAV * (h1 * (d1 - rd1 + Sr1 * dtrk) + h2 * (d0 - rd0 + sr0 * dtrk));
+        const double Ck = (double)ww[0] + dtrk * (pg + sw0);
         if (rr == KS) {
           const double rdenom = 1.0 / F2v;
           ek = -F1v * rdenom;
@@ -927,61 +941,61 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgs a, unsigned ni_til
       a.MOMY_RK[g] = MYn;
       a.DENS_RK[g] = Sr;  // Sr/St wait in the outputs until the back pass
       a.RHOT_RK[g] = St;
-      if (k <= KE - 1) { a.mflx_hi[g] = mZ; a.tflx_hi[g] = tZ; a.PT[g] = PT; }
-      else { a.mflx_hi[g] = 0.0; a.tflx_hi[g] = 0.0; }
-      if (k == KS) { a.mflx_hi[g - d.pl] = 0.0; a.tflx_hi[g - d.pl] = 0.0; }
+      if (k <= KE - 1) { a.mflx_hi[g] = mZ; a.tflx_hi[g] = tZ; a.PT[g] = (double)PT; }
+      else { a.mflx_hi[g] = R(0); a.tflx_hi[g] = R(0); }
+      if (k == KS) { a.mflx_hi[g - d.pl] = R(0); a.tflx_hi[g - d.pl] = R(0); }
       if (i == d.IS) {  // mflx_hi(Z) on the i = IS-1 column (HEVI.F90:370)
-        a.mflx_hi[g - 1] = (k <= KE - 1) ? nd(I_DENS, ZDIR, g - 1) : 0.0;
-        if (k == KS) a.mflx_hi[g - 1 - d.pl] = 0.0;
+        a.mflx_hi[g - 1] = (k <= KE - 1) ? nd(I_DENS, ZDIR, g - 1) : R(0);
+        if (k == KS) a.mflx_hi[g - 1 - d.pl] = R(0);
       }
       if (k > KS && k - 1 < KE - 1) { a.e[g - d.pl] = ek; a.f[g - d.pl] = fk; }
       cmZ = mZ; ctZ = tZ;
-      rt0 = rt1; dp0 = dp1; st0 = St; sr0 = Sr; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = Sw;
+      rt0 = rt1; dp0 = dp1; st0 = St1; sr0 = Sr1; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = (double)Sw;
     }
     __syncthreads();
     s0 = s1;
   }
   if (!own) return;
 
-  // ---- downward: back substitution and updates (HEVI.F90:728-790) ----
-  double* mfZ = a.mflx_hi; double* tfZ = a.tflx_hi;
+  // ---- downward: back substitution and updates in double (HEVI.F90:728-790) ----
+  R* mfZ = a.mflx_hi; R* tfZ = a.tflx_hi;
   auto px = [&](int k) { return p0 + d.pl * (size_t)k; };
   double Cn = C_last;
   double ptn = a.PT[px(KE - 1)];
   {
     const size_t q = px(KE);
     double advcv = Cn * J33G * RCDZ[KE];
-    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + a.DENS_RK[q]);
+    a.DENS_RK[q] = R((double)a.DENS0[q] + dtrk * (advcv + (double)a.DENS_RK[q]));
     advcv = Cn * ptn * J33G * RCDZ[KE];
-    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + a.RHOT_RK[q]);
-    a.MOMZ_RK[q] = 0.0;
+    a.RHOT_RK[q] = R((double)a.RHOT0[q] + dtrk * (advcv + (double)a.RHOT_RK[q]));
+    a.MOMZ_RK[q] = R(0);
   }
   {
     const size_t q = px(KE - 1);
-    mfZ[q] = mfZ[q] + J33G * Cn;
-    tfZ[q] = tfZ[q] + J33G * Cn * ptn;
-    a.MOMZ_RK[q] = a.MOMZ0[q] + (Cn - a.MOMZ[q]);
+    mfZ[q] = R((double)mfZ[q] + J33G * Cn);
+    tfZ[q] = R((double)tfZ[q] + J33G * Cn * ptn);
+    a.MOMZ_RK[q] = R((double)a.MOMZ0[q] + (Cn - (double)a.MOMZ[q]));
   }
   for (int k = KE - 2; k >= KS; --k) {
     const size_t q = px(k), q1 = px(k + 1);
     const double Ck = a.e[q] * Cn + a.f[q];
     const double ptk = a.PT[q];
-    mfZ[q] = mfZ[q] + J33G * Ck;
-    tfZ[q] = tfZ[q] + J33G * Ck * ptk;
-    a.MOMZ_RK[q] = a.MOMZ0[q] + (Ck - a.MOMZ[q]);
+    mfZ[q] = R((double)mfZ[q] + J33G * Ck);
+    tfZ[q] = R((double)tfZ[q] + J33G * Ck * ptk);
+    a.MOMZ_RK[q] = R((double)a.MOMZ0[q] + (Ck - (double)a.MOMZ[q]));
     double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1];
-    a.DENS_RK[q1] = a.DENS0[q1] + dtrk * (advcv + a.DENS_RK[q1]);
+    a.DENS_RK[q1] = R((double)a.DENS0[q1] + dtrk * (advcv + (double)a.DENS_RK[q1]));
     advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1];
-    a.RHOT_RK[q1] = a.RHOT0[q1] + dtrk * (advcv + a.RHOT_RK[q1]);
+    a.RHOT_RK[q1] = R((double)a.RHOT0[q1] + dtrk * (advcv + (double)a.RHOT_RK[q1]));
     Cn = Ck; ptn = ptk;
   }
   {
     const size_t q = px(KS);
     double advcv = -Cn * J33G * RCDZ[KS];
-    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + a.DENS_RK[q]);
+    a.DENS_RK[q] = R((double)a.DENS0[q] + dtrk * (advcv + (double)a.DENS_RK[q]));
     advcv = -Cn * ptn * J33G * RCDZ[KS];
-    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + a.RHOT_RK[q]);
-    a.MOMZ_RK[px(KS - 1)] = 0.0;
+    a.RHOT_RK[q] = R((double)a.RHOT0[q] + dtrk * (advcv + (double)a.RHOT_RK[q]));
+    a.MOMZ_RK[px(KS - 1)] = R(0);
   }
 }
 
@@ -997,7 +1011,7 @@ static hipError_t launch_t(const HeviArgs& a, hipStream_t s) {
     // flat terrain: LDS tiles of 64 x 8 columns
     constexpr int TJ = 8;
     const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
-    hipLaunchKernelGGL((hevi_tile<S, DIVDMP, TJ>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+    hipLaunchKernelGGL((hevi_tile<S, DIVDMP, TJ, double>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   } else {
     constexpr unsigned TJ = 4;
     const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
@@ -1007,8 +1021,8 @@ static hipError_t launch_t(const HeviArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
-  HeviArgs a;
+template <typename R>
+static void fill_args(HeviArgsT<R>& a, const StageIOT<R>& io) {
   const Ctx& c = *io.ctx;
   a.ctx = &c;
   a.d = c.d;
@@ -1023,6 +1037,12 @@ hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
   a.DENS_RK = io.o[0]; a.MOMZ_RK = io.o[1]; a.MOMX_RK = io.o[2]; a.MOMY_RK = io.o[3]; a.RHOT_RK = io.o[4];
   a.mflx_hi = io.mflx_hi; a.tflx_hi = io.tflx_hi;
   a.PT = c.scr[3]; a.e = c.scr[4]; a.f = c.scr[5];
+}
+
+hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
+  HeviArgs a;
+  fill_args(a, io);
+  const Ctx& c = *io.ctx;
   const bool div = io.divdmp_coef > 0.0;
   const bool ter = c.terrain;
   if (c.scheme == UD3) {
@@ -1031,6 +1051,22 @@ hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
   }
   if (ter) return div ? launch_t<CD4, true, true>(a, s) : launch_t<CD4, true, false>(a, s);
   return div ? launch_t<CD4, false, true>(a, s) : launch_t<CD4, false, false>(a, s);
+}
+
+hipError_t hevi_stage(const StageIOT<float>& io, hipStream_t s) {
+  HeviArgsT<float> a;
+  fill_args(a, io);
+  const Ctx& c = *io.ctx;
+  if (c.terrain || io.divdmp_coef > 0.0) return hipErrorNotSupported;
+  const Dims& d = c.d;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
+  constexpr int TJ = 8;
+  const unsigned ni = (imax + 63) / 64, nb = ni * ((jmax + TJ - 1) / TJ);
+  prof_start(c, "hevi_stage", s);
+  if (c.scheme == UD3) hipLaunchKernelGGL((hevi_tile<UD3, false, TJ, float>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  else hipLaunchKernelGGL((hevi_tile<CD4, false, TJ, float>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  prof_stop(c, s);
+  return hipGetLastError();
 }
 
 }  // namespace shp

@@ -26,6 +26,15 @@ struct Model {
   double* R[4][5] = {};
   double *mfR[4] = {}, *tfR[4] = {};
   double *ndq = nullptr, *DENS_RK = nullptr, *QRK1 = nullptr, *QRK2 = nullptr, *zeros = nullptr;
+  // mixed precision: FP32 state, stage, tendency, filter and flux arrays
+  // (the FP64 arrays above keep the reference state, the pressure terms and
+  // a copy of the state for the large-step preparation and the budgets)
+  int mixed = 0;
+  float* fS[5] = {};
+  float* fR[4][5] = {};
+  float* ft[5] = {};
+  float *fnum_diff = nullptr, *fDDIV = nullptr, *fmflx = nullptr, *ftflx = nullptr;
+  float *fmfR[4] = {}, *ftfR[4] = {};
   // configuration (PARAM_ATMOS_DYN subset)
   double dtl = 1.0, dts = 1.0, nd_coef = 1.0e-4, nd_sfc_fact = 1.0, divdmp_coef = 0.0;
   int fct = 0, fct_along = 1;
@@ -37,5 +46,7 @@ hipError_t model_short_step_rk4(Ctx& c, double dts);
 hipError_t model_prepare(Ctx& c);
 hipError_t model_large_step(Ctx& c);
 hipError_t mass_sum(const Ctx& c, const double* a, const double* b, double* out_host);
+hipError_t model_state_to_f64(Ctx& c);
+hipError_t model_state_to_f32(Ctx& c);
 
 }  // namespace shp

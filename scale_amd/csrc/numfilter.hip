@@ -3,180 +3,55 @@
 // Reference: ATMOS_DYN_numfilter_coef with ND_ORDER = 1 and ND_USE_RS,
 //   scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:385-985,
 //   calc_diff3 :1652-1800, coefficient tables filter_setup :92-319.
-// One thread per (k,i,j) over every level of the interior columns writes all
-// 15 outputs; the differenced fields (DENS-REF_dens, w, u, v, theta-REF_pott)
-// are recomputed from the prognostics at each stencil point instead of being
-// materialised.  The periodic halo fill follows as a separate launch.
+// Blocks of 64 (i) x TJ (j) columns march k.  At each level the block
+// evaluates the five differenced fields (DENS-REF_dens, w, u, v,
+// theta-REF_pott) once per tile cell (2-cell halo) into LDS for the x and y
+// differences; each column keeps the fields at k-2..k+2 in registers for the
+// z differences.  The periodic halo fill of the 15 outputs follows as a
+// separate launch.  R is the storage type of the prognostics and of num_diff
+// (double, or float for the mixed-precision path, where the differences from
+// the reference state are taken in double).
 #include "common.h"
 #include "shp_internal.h"
 
 namespace shp {
 
-struct NfArgs {
+template <typename R>
+struct NfArgsT {
   Dims d;
-  const double *DENS, *MOMZ, *MOMX, *MOMY, *RHOT, *REF_dens, *REF_pott;
+  const R *DENS, *MOMZ, *MOMX, *MOMY, *RHOT;
+  const double *REF_dens, *REF_pott;
   const double *CNZ3, *CNX3, *CNY3;  // (3,KA,2), (3,IA,2), (3,JA,2)
   const double *ndc;                 // nd_cdz(KA) nd_fdz(KA) nd_cdx(IA) nd_fdx(IA) nd_cdy(JA) nd_fdy(JA)
   double sfc;
-  double* num_diff;
+  R* num_diff;
 };
 
+// coefficient tables of calc_diff3 (CNZ3/CNX3/CNY3 of filter_setup)
+template <typename R>
 struct Nf {
-  const NfArgs& a;
+  const NfArgsT<R>& a;
   const Dims& d;
-  __device__ Nf(const NfArgs& a_) : a(a_), d(a_.d) {}
-  __device__ __forceinline__ size_t ix(int k, int i, int j) const { return IX(d, k, i, j); }
+  __device__ Nf(const NfArgsT<R>& a_) : a(a_), d(a_.d) {}
   __device__ __forceinline__ double cz(int n, int k, int b) const { return a.CNZ3[(n - 1) + 3 * ((size_t)k + (size_t)d.KA * (b - 1))]; }
   __device__ __forceinline__ double cx(int n, int i, int b) const { return a.CNX3[(n - 1) + 3 * ((size_t)i + (size_t)d.IA * (b - 1))]; }
   __device__ __forceinline__ double cy(int n, int j, int b) const { return a.CNY3[(n - 1) + 3 * ((size_t)j + (size_t)d.JA * (b - 1))]; }
-  __device__ __forceinline__ double D(int k, int i, int j) const { return a.DENS[ix(k, i, j)]; }
-
-  // the five differenced fields (COMMON.F90:545, 631, 715, 798, 882)
-  __device__ __forceinline__ double phi(int f, int k, int i, int j) const {
-    size_t p = ix(k, i, j);
-    switch (f) {
-      case 0: return a.DENS[p] - a.REF_dens[p];
-      case 1: return 2.0 * a.MOMZ[p] / (a.DENS[ix(k + 1, i, j)] + a.DENS[p]);
-      case 2: return 2.0 * a.MOMX[p] / (a.DENS[ix(k, i + 1, j)] + a.DENS[p]);
-      case 3: return 2.0 * a.MOMY[p] / (a.DENS[ix(k, i, j + 1)] + a.DENS[p]);
-      default: return a.RHOT[p] / a.DENS[p] - a.REF_pott[p];
-    }
-  }
-  // calc_diff3 ZDIR, KO = 0 (COMMON.F90:1668-1697); ghost levels are the
-  // negated mirror images of interior ones
-  __device__ double dz0(int f, int k, int i, int j) const {
-    const int KS = d.KS, KE = d.KE;
-    bool neg = false;
-    if (k == KS - 1) { k = KS; neg = true; }
-    else if (k == KS - 2) { k = KS + 1; neg = true; }
-    else if (k == KE) { k = KE - 1; neg = true; }
-    else if (k == KE + 1) { k = KE - 2; neg = true; }
-    else if (k == KE + 2) return 0.0;
-    double v;
-    if (k == KS)
-      v = (+cz(1, KS + 1, 1) * phi(f, KS + 2, i, j) - cz(2, KS + 1, 1) * phi(f, KS + 1, i, j)
-           + cz(3, KS + 1, 1) * phi(f, KS, i, j) - cz(1, KS, 1) * phi(f, KS + 1, i, j));
-    else if (k == KE - 1)
-      v = (+cz(1, KE, 1) * phi(f, KE - 1, i, j) - cz(2, KE, 1) * phi(f, KE, i, j)
-           + cz(3, KE, 1) * phi(f, KE - 1, i, j) - cz(1, KE - 1, 1) * phi(f, KE - 2, i, j));
-    else
-      v = (+cz(1, k + 1, 1) * phi(f, k + 2, i, j) - cz(2, k + 1, 1) * phi(f, k + 1, i, j)
-           + cz(3, k + 1, 1) * phi(f, k, i, j) - cz(1, k, 1) * phi(f, k - 1, i, j));
-    return neg ? -v : v;
-  }
-  // calc_diff3 ZDIR, KO = 1 (COMMON.F90:1699-1731)
-  __device__ double dz1(int f, int k, int i, int j) const {
-    const int KS = d.KS, KE = d.KE;
-    bool neg = false;
-    if (k == KS) { k = KS + 1; neg = true; }
-    else if (k == KS - 1) { k = KS + 2; neg = true; }
-    else if (k == KS - 2) { k = KS + 3; neg = true; }
-    else if (k == KE + 1) { k = KE; neg = true; }
-    else if (k == KE + 2) { k = KE - 1; neg = true; }
-    double v;
-    if (k == KS + 1)
-      v = (+cz(1, KS + 1, 2) * phi(f, KS + 2, i, j) - cz(2, KS + 1, 2) * phi(f, KS + 1, i, j)
-           + cz(3, KS + 1, 2) * phi(f, KS, i, j) - cz(1, KS, 2) * phi(f, KS + 1, i, j));
-    else if (k == KE - 1)
-      v = (-cz(2, KE - 1, 2) * phi(f, KE - 1, i, j) + cz(3, KE - 1, 2) * phi(f, KE - 2, i, j)
-           - cz(1, KE - 2, 2) * phi(f, KE - 3, i, j));
-    else if (k == KE)
-      v = (+cz(1, KE, 2) * phi(f, KE - 1, i, j) + cz(3, KE, 2) * phi(f, KE - 1, i, j)
-           - cz(1, KE - 1, 2) * phi(f, KE - 2, i, j));
-    else
-      v = (+cz(1, k, 2) * phi(f, k + 1, i, j) - cz(2, k, 2) * phi(f, k, i, j)
-           + cz(3, k, 2) * phi(f, k - 1, i, j) - cz(1, k - 1, 2) * phi(f, k - 2, i, j));
-    return neg ? -v : v;
-  }
-  // calc_diff3 XDIR (COMMON.F90:1733-1763)
-  __device__ double dx(int f, int io, int k, int i, int j) const {
-    if (io == 0)
-      return (+cx(1, i + 1, 1) * phi(f, k, i + 2, j) - cx(2, i + 1, 1) * phi(f, k, i + 1, j)
-              + cx(3, i + 1, 1) * phi(f, k, i, j) - cx(1, i, 1) * phi(f, k, i - 1, j));
-    return (+cx(1, i, 2) * phi(f, k, i + 1, j) - cx(2, i, 2) * phi(f, k, i, j)
-            + cx(3, i, 2) * phi(f, k, i - 1, j) - cx(1, i - 1, 2) * phi(f, k, i - 2, j));
-  }
-  // calc_diff3 YDIR (COMMON.F90:1770-1800)
-  __device__ double dy(int f, int jo, int k, int i, int j) const {
-    if (jo == 0)
-      return (+cy(1, j + 1, 1) * phi(f, k, i, j + 2) - cy(2, j + 1, 1) * phi(f, k, i, j + 1)
-              + cy(3, j + 1, 1) * phi(f, k, i, j) - cy(1, j, 1) * phi(f, k, i, j - 1));
-    return (+cy(1, j, 2) * phi(f, k, i, j + 1) - cy(2, j, 2) * phi(f, k, i, j)
-            + cy(3, j, 2) * phi(f, k, i, j - 1) - cy(1, j - 1, 2) * phi(f, k, i, j - 2));
-  }
-  __device__ __forceinline__ double sfc(int k, double v) const {
-    if (k == d.KS) return v * a.sfc;
-    if (k == d.KS + 1) return v * (1.0 + a.sfc) * 0.5;
-    return v;
-  }
 };
 
-__global__ void __launch_bounds__(256) numfilter_kernel(NfArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
-  const Dims& d = a.d;
-  Nf n(a);
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
-  const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
-  const int i = d.IS + (int)(it * 64 + threadIdx.x);
-  const int k = (int)(kt * 4 + threadIdx.y);
-  const int j = d.JS + (int)jt;
-  if (i > d.IE || k >= d.KA || j > d.JE) return;
-  const int KS = d.KS, KE = d.KE;
-  const size_t p = IX(d, k, i, j);
-  const double* nd_cdz = a.ndc;
-  const double* nd_fdz = a.ndc + d.KA;
-  const double* nd_cdx = a.ndc + 2 * d.KA;
-  const double* nd_fdx = a.ndc + 2 * d.KA + d.IA;
-  const double* nd_cdy = a.ndc + 2 * d.KA + 2 * d.IA;
-  const double* nd_fdy = a.ndc + 2 * d.KA + 2 * d.IA + d.JA;
-  auto out = [&](int v, int dir) -> double& { return a.num_diff[p + d.n3 * (size_t)(v + 5 * dir)]; };
-  const bool kin = k >= KS && k <= KE;
-  const bool kinm = k >= KS && k <= KE - 1;
-  // density (COMMON.F90:552-621)
-  out(I_DENS, ZDIR) = (k >= KS - 1 && k <= KE) ? n.dz0(0, k, i, j) * nd_cdz[k] : 0.0;
-  out(I_DENS, XDIR) = kin ? n.sfc(k, n.dx(0, 0, k, i, j) * nd_cdx[i]) : 0.0;
-  out(I_DENS, YDIR) = kin ? n.sfc(k, n.dy(0, 0, k, i, j) * nd_cdy[j]) : 0.0;
-  // z-momentum (COMMON.F90:637-707)
-  out(I_MOMZ, ZDIR) = (k >= KS + 1 && k <= KE - 1) ? n.dz1(1, k, i, j) * nd_fdz[k] * n.D(k, i, j) : 0.0;
-  out(I_MOMZ, XDIR) = kinm ? n.sfc(k, n.dx(1, 0, k, i, j) * nd_cdx[i] * 0.25
-                                         * (n.D(k + 1, i + 1, j) + n.D(k + 1, i, j) + n.D(k, i + 1, j) + n.D(k, i, j))) : 0.0;
-  out(I_MOMZ, YDIR) = kinm ? n.sfc(k, n.dy(1, 0, k, i, j) * nd_cdy[j] * 0.25
-                                         * (n.D(k + 1, i, j + 1) + n.D(k + 1, i, j) + n.D(k, i, j + 1) + n.D(k, i, j))) : 0.0;
-  // x-momentum (COMMON.F90:721-790)
-  out(I_MOMX, ZDIR) = kinm ? n.dz0(2, k, i, j) * nd_cdz[k] * 0.25
-                               * (n.D(k + 1, i + 1, j) + n.D(k + 1, i, j) + n.D(k, i + 1, j) + n.D(k, i, j)) : 0.0;
-  out(I_MOMX, XDIR) = kin ? n.sfc(k, n.dx(2, 1, k, i, j) * nd_fdx[i] * n.D(k, i, j)) : 0.0;
-  out(I_MOMX, YDIR) = kin ? n.sfc(k, n.dy(2, 0, k, i, j) * nd_cdy[j] * 0.25
-                                        * (n.D(k, i + 1, j + 1) + n.D(k, i + 1, j) + n.D(k, i, j + 1) + n.D(k, i, j))) : 0.0;
-  // y-momentum (COMMON.F90:804-873)
-  out(I_MOMY, ZDIR) = kinm ? n.dz0(3, k, i, j) * nd_cdz[k] * 0.25
-                               * (n.D(k + 1, i, j + 1) + n.D(k + 1, i, j) + n.D(k, i, j + 1) + n.D(k, i, j)) : 0.0;
-  out(I_MOMY, XDIR) = kin ? n.sfc(k, n.dx(3, 0, k, i, j) * nd_cdx[i] * 0.25
-                                        * (n.D(k, i + 1, j + 1) + n.D(k, i, j + 1) + n.D(k, i + 1, j) + n.D(k, i, j))) : 0.0;
-  out(I_MOMY, YDIR) = kin ? n.sfc(k, n.dy(3, 1, k, i, j) * nd_fdy[j] * n.D(k, i, j)) : 0.0;
-  // rho*theta (COMMON.F90:888-962)
-  double rz = 0.0;
-  if (kinm) rz = n.dz0(4, k, i, j) * nd_cdz[k] * 0.5 * (n.D(k + 1, i, j) + n.D(k, i, j));
-  else if (k == KS - 1) rz = n.dz0(4, KS - 1, i, j) * nd_cdz[KS - 1] * n.D(KS, i, j);
-  else if (k == KE) rz = n.dz0(4, KE, i, j) * nd_cdz[KE] * n.D(KE, i, j);
-  out(I_RHOT, ZDIR) = rz;
-  out(I_RHOT, XDIR) = kin ? n.sfc(k, n.dx(4, 0, k, i, j) * nd_cdx[i] * 0.5 * (n.D(k, i + 1, j) + n.D(k, i, j))) : 0.0;
-  out(I_RHOT, YDIR) = kin ? n.sfc(k, n.dy(4, 0, k, i, j) * nd_cdy[j] * 0.5 * (n.D(k, i, j + 1) + n.D(k, i, j))) : 0.0;
+// a field minus its reference-state value (COMMON.F90:545, 882), in double
+template <typename R>
+__device__ __forceinline__ R phi_ref(R v, double ref) {
+  if constexpr (sizeof(R) == sizeof(double)) return v - ref;
+  else return R((double)v - ref);
 }
 
-// ---------------------------------------------------------------------------
-// The same fluxes with blocks of 64 (i) x TJ (j) columns marching k.  At each
-// level the block evaluates the five differenced fields once per tile cell
-// (2-cell halo) into LDS for the x and y differences; each column keeps the
-// fields at k-2..k+2 in registers for the z differences.  The ghost level
-// KS-1 of the DENS and RHOT z fluxes is the negated KS value (calc_diff3's
-// mirror image) and is written together with level KS.
-template <int TJ>
-__global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgs a, unsigned ni_tiles, unsigned nblocks) {
+template <int TJ, typename R>
+__global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned ni_tiles, unsigned nblocks) {
   constexpr int TI = 64, CW = TI + 4, RW = TJ + 4, NT = TI * TJ;
-  __shared__ double sD[2][RW + 1][CW + 1];  // DENS at k and k+1, one more cell east and north
-  __shared__ double sP[5][RW][CW];          // the differenced fields at level k
+  __shared__ R sD[2][RW + 1][CW + 1];  // DENS at k and k+1, one more cell east and north
+  __shared__ R sP[5][RW][CW];          // the differenced fields at level k
   const Dims& d = a.d;
-  Nf n(a);
+  const Nf<R> n(a);
   const unsigned lb = xcd_block(blockIdx.x, nblocks);
   const int i0 = d.IS + (int)(lb % ni_tiles) * TI;
   const int j0 = d.JS + (int)(lb / ni_tiles) * TJ;
@@ -192,37 +67,37 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgs a, unsigned ni_
   const double* nd_cdy = a.ndc + 2 * d.KA + 2 * d.IA;
   const double* nd_fdy = a.ndc + 2 * d.KA + 2 * d.IA + d.JA;
   const size_t p0 = own ? IX(d, 0, i, j) : 0;
-  auto out = [&](int v, int dir, int k) -> double& { return a.num_diff[p0 + d.pl * (size_t)k + n3 * (size_t)(v + 5 * dir)]; };
-  auto fillD = [&](double (*dst)[CW + 1], int k) {
+  auto out = [&](int v, int dir, int k) -> R& { return a.num_diff[p0 + d.pl * (size_t)k + n3 * (size_t)(v + 5 * dir)]; };
+  auto fillD = [&](R (*dst)[CW + 1], int k) {
     for (int q = tid; q < (RW + 1) * (CW + 1); q += NT) {
       const int r = q / (CW + 1), c = q - r * (CW + 1);
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      dst[r][c] = (ii < d.IA && jj < d.JA) ? a.DENS[IX(d, k, ii, jj)] : 1.0;
+      dst[r][c] = (ii < d.IA && jj < d.JA) ? a.DENS[IX(d, k, ii, jj)] : R(1);
     }
   };
   // the five fields at (k, ii, jj) for the own column (COMMON.F90:545, 631, 715, 798, 882)
-  auto phis = [&](int k, double* ph) {
+  auto phis = [&](int k, R* ph) {
     const size_t q = p0 + d.pl * (size_t)k;
-    const double D = a.DENS[q];
-    ph[0] = D - a.REF_dens[q];
-    ph[1] = 2.0 * a.MOMZ[q] / (a.DENS[q + d.pl] + D);
-    ph[2] = 2.0 * a.MOMX[q] / (a.DENS[q + 1] + D);
-    ph[3] = 2.0 * a.MOMY[q] / (a.DENS[q + d.IA] + D);
-    ph[4] = a.RHOT[q] / D - a.REF_pott[q];
+    const R D = a.DENS[q];
+    ph[0] = phi_ref<R>(D, a.REF_dens[q]);
+    ph[1] = R(2) * a.MOMZ[q] / (a.DENS[q + d.pl] + D);
+    ph[2] = R(2) * a.MOMX[q] / (a.DENS[q + 1] + D);
+    ph[3] = R(2) * a.MOMY[q] / (a.DENS[q + d.IA] + D);
+    ph[4] = phi_ref<R>(a.RHOT[q] / D, a.REF_pott[q]);
   };
 
   // levels outside KS..KE are zero (KS-1 gets its DENS/RHOT z fluxes below)
   if (own) {
     for (int k = 0; k < d.KA; ++k) {
       if (k >= KS && k <= KE) continue;
-      for (int q = 0; q < 15; ++q) out(q % 5, q / 5, k) = 0.0;
+      for (int q = 0; q < 15; ++q) out(q % 5, q / 5, k) = R(0);
     }
   }
   // z windows: w[f][m] = field f at level k-2+m
-  double w[5][5];
-  for (int f = 0; f < 5; ++f) for (int m = 0; m < 5; ++m) w[f][m] = 0.0;
+  R w[5][5];
+  for (int f = 0; f < 5; ++f) for (int m = 0; m < 5; ++m) w[f][m] = R(0);
   if (own) {
-    double ph[5];
+    R ph[5];
     for (int m = 3; m < 5; ++m) {  // shifted down once at the first level
       phis(KS - 3 + m, ph);
       for (int f = 0; f < 5; ++f) w[f][m] = ph[f];
@@ -235,7 +110,8 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgs a, unsigned ni_
     const int s1 = s0 ^ 1;
     // every global load of the level first: the raw fields of this thread's
     // tile cells, DENS at k+1, and the column's fields at k+2
-    double rZ[NC], rX[NC], rY[NC], rR[NC], rRD[NC], rRP[NC];
+    R rZ[NC], rX[NC], rY[NC], rR[NC];
+    double rRD[NC], rRP[NC];
 #pragma unroll
     for (int h = 0; h < NC; ++h) {
       const int q = tid + h * NT;
@@ -243,12 +119,12 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgs a, unsigned ni_
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
       const bool ok = q < RW * CW && ii < d.IA && jj < d.JA;
       const size_t g = ok ? IX(d, k, ii, jj) : 0;
-      rZ[h] = ok ? a.MOMZ[g] : 0.0; rX[h] = ok ? a.MOMX[g] : 0.0; rY[h] = ok ? a.MOMY[g] : 0.0;
-      rR[h] = ok ? a.RHOT[g] : 0.0; rRD[h] = ok ? a.REF_dens[g] : 0.0; rRP[h] = ok ? a.REF_pott[g] : 0.0;
+      rZ[h] = ok ? a.MOMZ[g] : R(0); rX[h] = ok ? a.MOMX[g] : R(0); rY[h] = ok ? a.MOMY[g] : R(0);
+      rR[h] = ok ? a.RHOT[g] : R(0); rRD[h] = ok ? a.REF_dens[g] : 0.0; rRP[h] = ok ? a.REF_pott[g] : R(0);
     }
     fillD(sD[s1], k + 1);
     if (own) {
-      double ph[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      R ph[5] = {0, 0, 0, 0, 0};
       if (k + 2 <= KE) phis(k + 2, ph);
       for (int f = 0; f < 5; ++f) { w[f][0] = w[f][1]; w[f][1] = w[f][2]; w[f][2] = w[f][3]; w[f][3] = w[f][4]; w[f][4] = ph[f]; }
     }
@@ -259,24 +135,27 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgs a, unsigned ni_
       if (q >= RW * CW) continue;
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      if (ii >= d.IA || jj >= d.JA) { for (int f = 0; f < 5; ++f) sP[f][r][c] = 0.0; continue; }
-      const double D = sD[s0][r][c];
-      sP[0][r][c] = D - rRD[h];
-      sP[1][r][c] = 2.0 * rZ[h] / (sD[s1][r][c] + D);
-      sP[2][r][c] = 2.0 * rX[h] / (sD[s0][r][c + 1] + D);
-      sP[3][r][c] = 2.0 * rY[h] / (sD[s0][r + 1][c] + D);
-      sP[4][r][c] = rR[h] / D - rRP[h];
+      if (ii >= d.IA || jj >= d.JA) { for (int f = 0; f < 5; ++f) sP[f][r][c] = R(0); continue; }
+      const R D = sD[s0][r][c];
+      sP[0][r][c] = phi_ref<R>(D, rRD[h]);
+      sP[1][r][c] = R(2) * rZ[h] / (sD[s1][r][c] + D);
+      sP[2][r][c] = R(2) * rX[h] / (sD[s0][r][c + 1] + D);
+      sP[3][r][c] = R(2) * rY[h] / (sD[s0][r + 1][c] + D);
+      sP[4][r][c] = phi_ref<R>(rR[h] / D, rRP[h]);
     }
     __syncthreads();
     if (own) {
       const int r = ty + 2, c = tx + 2;
+      auto NDC = [&](const double* t, int q) { return R(t[q]); };
       auto Dk = [&](int di, int dj) { return sD[s0][r + dj][c + di]; };
       auto Dk1 = [&](int di, int dj) { return sD[s1][r + dj][c + di]; };
-      auto cz = [&](int nn, int kk, int b) { return n.cz(nn, kk, b); };
+      auto cz = [&](int nn, int kk, int b) { return R(n.cz(nn, kk, b)); };
+      auto cx = [&](int nn, int ii, int b) { return R(n.cx(nn, ii, b)); };
+      auto cy = [&](int nn, int jj, int b) { return R(n.cy(nn, jj, b)); };
       // calc_diff3 ZDIR KO = 0 at level k in KS..KE (COMMON.F90:1668-1697);
       // w[f][m] holds level k-2+m
       auto dz0 = [&](int f) {
-        const double* P = w[f];
+        const R* P = w[f];
         if (k == KS)
           return (+cz(1, KS + 1, 1) * P[4] - cz(2, KS + 1, 1) * P[3]
                   + cz(3, KS + 1, 1) * P[2] - cz(1, KS, 1) * P[3]);
@@ -291,7 +170,7 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgs a, unsigned ni_
       };
       // calc_diff3 ZDIR KO = 1 at level k in KS+1..KE-1 (COMMON.F90:1699-1731)
       auto dz1 = [&](int f) {
-        const double* P = w[f];
+        const R* P = w[f];
         if (k == KS + 1)
           return (+cz(1, KS + 1, 2) * P[3] - cz(2, KS + 1, 2) * P[2]
                   + cz(3, KS + 1, 2) * P[1] - cz(1, KS, 2) * P[2]);
@@ -304,53 +183,59 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgs a, unsigned ni_
       // calc_diff3 XDIR / YDIR (COMMON.F90:1733-1800)
       auto dx = [&](int f, int io) {
         if (io == 0)
-          return (+n.cx(1, i + 1, 1) * sP[f][r][c + 2] - n.cx(2, i + 1, 1) * sP[f][r][c + 1]
-                  + n.cx(3, i + 1, 1) * sP[f][r][c] - n.cx(1, i, 1) * sP[f][r][c - 1]);
-        return (+n.cx(1, i, 2) * sP[f][r][c + 1] - n.cx(2, i, 2) * sP[f][r][c]
-                + n.cx(3, i, 2) * sP[f][r][c - 1] - n.cx(1, i - 1, 2) * sP[f][r][c - 2]);
+          return (+cx(1, i + 1, 1) * sP[f][r][c + 2] - cx(2, i + 1, 1) * sP[f][r][c + 1]
+                  + cx(3, i + 1, 1) * sP[f][r][c] - cx(1, i, 1) * sP[f][r][c - 1]);
+        return (+cx(1, i, 2) * sP[f][r][c + 1] - cx(2, i, 2) * sP[f][r][c]
+                + cx(3, i, 2) * sP[f][r][c - 1] - cx(1, i - 1, 2) * sP[f][r][c - 2]);
       };
       auto dy = [&](int f, int jo) {
         if (jo == 0)
-          return (+n.cy(1, j + 1, 1) * sP[f][r + 2][c] - n.cy(2, j + 1, 1) * sP[f][r + 1][c]
-                  + n.cy(3, j + 1, 1) * sP[f][r][c] - n.cy(1, j, 1) * sP[f][r - 1][c]);
-        return (+n.cy(1, j, 2) * sP[f][r + 1][c] - n.cy(2, j, 2) * sP[f][r][c]
-                + n.cy(3, j, 2) * sP[f][r - 1][c] - n.cy(1, j - 1, 2) * sP[f][r - 2][c]);
+          return (+cy(1, j + 1, 1) * sP[f][r + 2][c] - cy(2, j + 1, 1) * sP[f][r + 1][c]
+                  + cy(3, j + 1, 1) * sP[f][r][c] - cy(1, j, 1) * sP[f][r - 1][c]);
+        return (+cy(1, j, 2) * sP[f][r + 1][c] - cy(2, j, 2) * sP[f][r][c]
+                + cy(3, j, 2) * sP[f][r - 1][c] - cy(1, j - 1, 2) * sP[f][r - 2][c]);
+      };
+      auto sfc = [&](int kk, R v) {  // surface factor rows (COMMON.F90:593-594)
+        if (kk == KS) return v * R(a.sfc);
+        if (kk == KS + 1) return v * (R(1) + R(a.sfc)) * R(0.5);
+        return v;
       };
       const bool kinm = k <= KE - 1;
       // density (COMMON.F90:552-621)
-      const double dz0d = dz0(0);
-      out(I_DENS, ZDIR, k) = dz0d * nd_cdz[k];
-      if (k == KS) out(I_DENS, ZDIR, KS - 1) = (-dz0d) * nd_cdz[KS - 1];
-      out(I_DENS, XDIR, k) = n.sfc(k, dx(0, 0) * nd_cdx[i]);
-      out(I_DENS, YDIR, k) = n.sfc(k, dy(0, 0) * nd_cdy[j]);
+      const R dz0d = dz0(0);
+      out(I_DENS, ZDIR, k) = dz0d * NDC(nd_cdz, k);
+      if (k == KS) out(I_DENS, ZDIR, KS - 1) = (-dz0d) * NDC(nd_cdz, KS - 1);
+      out(I_DENS, XDIR, k) = sfc(k, dx(0, 0) * NDC(nd_cdx, i));
+      out(I_DENS, YDIR, k) = sfc(k, dy(0, 0) * NDC(nd_cdy, j));
       // z-momentum (COMMON.F90:637-707)
-      out(I_MOMZ, ZDIR, k) = (k >= KS + 1 && k <= KE - 1) ? dz1(1) * nd_fdz[k] * Dk(0, 0) : 0.0;
-      out(I_MOMZ, XDIR, k) = kinm ? n.sfc(k, dx(1, 0) * nd_cdx[i] * 0.25 * (Dk1(1, 0) + Dk1(0, 0) + Dk(1, 0) + Dk(0, 0))) : 0.0;
-      out(I_MOMZ, YDIR, k) = kinm ? n.sfc(k, dy(1, 0) * nd_cdy[j] * 0.25 * (Dk1(0, 1) + Dk1(0, 0) + Dk(0, 1) + Dk(0, 0))) : 0.0;
+      out(I_MOMZ, ZDIR, k) = (k >= KS + 1 && k <= KE - 1) ? dz1(1) * NDC(nd_fdz, k) * Dk(0, 0) : R(0);
+      out(I_MOMZ, XDIR, k) = kinm ? sfc(k, dx(1, 0) * NDC(nd_cdx, i) * R(0.25) * (Dk1(1, 0) + Dk1(0, 0) + Dk(1, 0) + Dk(0, 0))) : R(0);
+      out(I_MOMZ, YDIR, k) = kinm ? sfc(k, dy(1, 0) * NDC(nd_cdy, j) * R(0.25) * (Dk1(0, 1) + Dk1(0, 0) + Dk(0, 1) + Dk(0, 0))) : R(0);
       // x-momentum (COMMON.F90:721-790)
-      out(I_MOMX, ZDIR, k) = kinm ? dz0(2) * nd_cdz[k] * 0.25 * (Dk1(1, 0) + Dk1(0, 0) + Dk(1, 0) + Dk(0, 0)) : 0.0;
-      out(I_MOMX, XDIR, k) = n.sfc(k, dx(2, 1) * nd_fdx[i] * Dk(0, 0));
-      out(I_MOMX, YDIR, k) = n.sfc(k, dy(2, 0) * nd_cdy[j] * 0.25 * (Dk(1, 1) + Dk(1, 0) + Dk(0, 1) + Dk(0, 0)));
+      out(I_MOMX, ZDIR, k) = kinm ? dz0(2) * NDC(nd_cdz, k) * R(0.25) * (Dk1(1, 0) + Dk1(0, 0) + Dk(1, 0) + Dk(0, 0)) : R(0);
+      out(I_MOMX, XDIR, k) = sfc(k, dx(2, 1) * NDC(nd_fdx, i) * Dk(0, 0));
+      out(I_MOMX, YDIR, k) = sfc(k, dy(2, 0) * NDC(nd_cdy, j) * R(0.25) * (Dk(1, 1) + Dk(1, 0) + Dk(0, 1) + Dk(0, 0)));
       // y-momentum (COMMON.F90:804-873)
-      out(I_MOMY, ZDIR, k) = kinm ? dz0(3) * nd_cdz[k] * 0.25 * (Dk1(0, 1) + Dk1(0, 0) + Dk(0, 1) + Dk(0, 0)) : 0.0;
-      out(I_MOMY, XDIR, k) = n.sfc(k, dx(3, 0) * nd_cdx[i] * 0.25 * (Dk(1, 1) + Dk(0, 1) + Dk(1, 0) + Dk(0, 0)));
-      out(I_MOMY, YDIR, k) = n.sfc(k, dy(3, 1) * nd_fdy[j] * Dk(0, 0));
+      out(I_MOMY, ZDIR, k) = kinm ? dz0(3) * NDC(nd_cdz, k) * R(0.25) * (Dk1(0, 1) + Dk1(0, 0) + Dk(0, 1) + Dk(0, 0)) : R(0);
+      out(I_MOMY, XDIR, k) = sfc(k, dx(3, 0) * NDC(nd_cdx, i) * R(0.25) * (Dk(1, 1) + Dk(0, 1) + Dk(1, 0) + Dk(0, 0)));
+      out(I_MOMY, YDIR, k) = sfc(k, dy(3, 1) * NDC(nd_fdy, j) * Dk(0, 0));
       // rho*theta (COMMON.F90:888-962)
-      const double dz04 = dz0(4);
-      if (kinm) out(I_RHOT, ZDIR, k) = dz04 * nd_cdz[k] * 0.5 * (Dk1(0, 0) + Dk(0, 0));
-      else out(I_RHOT, ZDIR, k) = dz04 * nd_cdz[KE] * Dk(0, 0);
-      if (k == KS) out(I_RHOT, ZDIR, KS - 1) = (-dz04) * nd_cdz[KS - 1] * Dk(0, 0);
-      out(I_RHOT, XDIR, k) = n.sfc(k, dx(4, 0) * nd_cdx[i] * 0.5 * (Dk(1, 0) + Dk(0, 0)));
-      out(I_RHOT, YDIR, k) = n.sfc(k, dy(4, 0) * nd_cdy[j] * 0.5 * (Dk(0, 1) + Dk(0, 0)));
+      const R dz04 = dz0(4);
+      if (kinm) out(I_RHOT, ZDIR, k) = dz04 * NDC(nd_cdz, k) * R(0.5) * (Dk1(0, 0) + Dk(0, 0));
+      else out(I_RHOT, ZDIR, k) = dz04 * NDC(nd_cdz, KE) * Dk(0, 0);
+      if (k == KS) out(I_RHOT, ZDIR, KS - 1) = (-dz04) * NDC(nd_cdz, KS - 1) * Dk(0, 0);
+      out(I_RHOT, XDIR, k) = sfc(k, dx(4, 0) * NDC(nd_cdx, i) * R(0.5) * (Dk(1, 0) + Dk(0, 0)));
+      out(I_RHOT, YDIR, k) = sfc(k, dy(4, 0) * NDC(nd_cdy, j) * R(0.5) * (Dk(0, 1) + Dk(0, 0)));
     }
     __syncthreads();
     s0 = s1;
   }
 }
 
-hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], const double* REF_dens,
-                     const double* REF_pott, hipStream_t s) {
-  NfArgs a;
+template <typename R>
+static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], const double* REF_dens,
+                              const double* REF_pott, hipStream_t s) {
+  NfArgsT<R> a;
   a.d = c.d;
   a.DENS = st[0]; a.MOMZ = st[1]; a.MOMX = st[2]; a.MOMY = st[3]; a.RHOT = st[4];
   a.REF_dens = REF_dens; a.REF_pott = REF_pott;
@@ -362,13 +247,22 @@ hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], 
   const unsigned ni = (imax + 63) / 64, nj = (jmax + TJ - 1) / TJ;
   const unsigned nb = ni * nj;
   prof_start(c, "numfilter", s);
-  hipLaunchKernelGGL(numfilter_tile<TJ>, dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  hipLaunchKernelGGL((numfilter_tile<TJ, R>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   prof_stop(c, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  double* arrs[15];
+  R* arrs[15];
   for (int q = 0; q < 15; ++q) arrs[q] = num_diff + d.n3 * (size_t)q;
   return halo_fill(c, arrs, 15, s);
+}
+
+hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], const double* REF_dens,
+                     const double* REF_pott, hipStream_t s) {
+  return numfilter_t<double>(c, num_diff, st, REF_dens, REF_pott, s);
+}
+hipError_t numfilter(const Ctx& c, float* num_diff, const float* const st[5], const double* REF_dens,
+                     const double* REF_pott, hipStream_t s) {
+  return numfilter_t<float>(c, num_diff, st, REF_dens, REF_pott, s);
 }
 
 }  // namespace shp

@@ -40,18 +40,25 @@ struct Ctx {
   std::vector<double> hCDZ, hFDZ, hCDX, hFDX, hCDY, hFDY;
 };
 
-// one HEVI stage, all pointers in device layout
-struct StageIO {
+// one HEVI stage, all pointers in device layout; R is the storage type of the
+// prognostic, tendency, flux and num_diff arrays (float: mixed precision)
+template <typename R>
+struct StageIOT {
   const Ctx* ctx;
-  double* o[5];             // DENS_RK..RHOT_RK
-  double *mflx_hi, *tflx_hi;  // (..,3)
-  const double* s0[5];      // DENS0..RHOT0
-  const double* s[5];       // DENS..RHOT
-  const double* t[5];       // DENS_t..RHOT_t
-  const double *DPRES0, *RT2P, *CORIOLI, *num_diff, *wdamp_coef, *DDIV, *REF_dens, *REF_rhot;
+  R* o[5];                  // DENS_RK..RHOT_RK
+  R *mflx_hi, *tflx_hi;     // (..,3)
+  const R* s0[5];           // DENS0..RHOT0
+  const R* s[5];            // DENS..RHOT
+  const R* t[5];            // DENS_t..RHOT_t
+  const double *DPRES0, *RT2P, *CORIOLI;
+  const R* num_diff;
+  const double* wdamp_coef;
+  const R* DDIV;
+  const double *REF_dens, *REF_rhot;
   double divdmp_coef, dtrk;
   int has_coriolis;
 };
+using StageIO = StageIOT<double>;
 
 struct TracerIO {
   double* QTRCo;
@@ -61,6 +68,8 @@ struct TracerIO {
 };
 
 hipError_t hevi_stage(const StageIO& io, hipStream_t s);
+// mixed precision: flat terrain and divdmp_coef = 0 only
+hipError_t hevi_stage(const StageIOT<float>& io, hipStream_t s);
 // named timers around launches (no-ops unless c.prof is set): the counterpart
 // of the reference's PROF_rapstart / PROF_rapend (scale_prof.F90:156-253)
 void prof_start(const Ctx& c, const char* name, hipStream_t s);
@@ -75,8 +84,11 @@ int prof_names(Prof* p, char* buf, int len);
 hipError_t hbm_copy(const Ctx& c, double* dst, const double* src, size_t n, hipStream_t s);
 hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s);
+hipError_t numfilter(const Ctx& c, float* num_diff, const float* const st[5], const double* REF_dens,
+                     const double* REF_pott, hipStream_t s);
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s);
 hipError_t halo_fill(const Ctx& c, double* const* arrs, int n, hipStream_t s);
+hipError_t halo_fill(const Ctx& c, float* const* arrs, int n, hipStream_t s);
 // Fortran (k fastest) <-> device (i fastest) layout; ncomp fields of KA*IA*JA
 hipError_t to_device_layout(const Ctx& c, double* dst, const double* src_f, int ncomp, hipStream_t s);
 hipError_t to_fortran_layout(const Ctx& c, double* dst_f, const double* src, int ncomp, hipStream_t s);

@@ -46,7 +46,7 @@ class HipGrid(C.Structure):
 class DynConfig(C.Structure):
     _fields_ = [("dtl", C.c_double), ("dts", C.c_double), ("nd_coef", C.c_double), ("nd_sfc_fact", C.c_double),
                 ("divdmp_coef", C.c_double), ("nd_order", C.c_int32), ("fct_tracer", C.c_int32),
-                ("fct_along_stream", C.c_int32)]
+                ("fct_along_stream", C.c_int32), ("mixed_precision", C.c_int32)]
 
 
 _lib = None
@@ -161,8 +161,9 @@ class Context:
                                           "CDY", "FDY", "RCDY", "RFDY", "PHI", "GSQRT", "J13G", "J23G")],
             _d(m["J33G"]), _p(m["MAPF"])))
 
-    def model_create(self, dtl, dts, QA, nd_coef=1.0e-4, nd_sfc_fact=1.0, fct_tracer=False, fct_along_stream=True):
-        cfg = DynConfig(dtl, dts, nd_coef, nd_sfc_fact, 0.0, 1, int(fct_tracer), int(fct_along_stream))
+    def model_create(self, dtl, dts, QA, nd_coef=1.0e-4, nd_sfc_fact=1.0, fct_tracer=False, fct_along_stream=True,
+                     mixed_precision=False):
+        cfg = DynConfig(dtl, dts, nd_coef, nd_sfc_fact, 0.0, 1, int(fct_tracer), int(fct_along_stream), int(mixed_precision))
         _check(load_library().scale_hip_model_create(self._h, C.byref(cfg), C.c_int32(QA)))
         self.has_model = True
         self.QA = QA
@@ -219,13 +220,16 @@ class Context:
         return s.value or 0
 
 
-def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, device=0) -> Context:
-    """Create a context holding the device-resident model of an inputs.Case."""
+def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, device=0, mixed_precision=False) -> Context:
+    """Create a context holding the device-resident model of an inputs.Case
+    (mixed_precision: FP32 dynamics with FP64 solve and reference state; the
+    tracers are left out)."""
     if ctx is None:
         ctx = Context(case.grid, case.scheme, case.scheme_tracer, device)
     ctx.set_metrics(case.metric)
-    qa = case.qtrc.shape[-1]
-    ctx.model_create(case.dt, case.dt, qa, nd_coef=nd_coef, fct_tracer=case.fct if fct is None else fct)
+    qa = 0 if mixed_precision else case.qtrc.shape[-1]
+    ctx.model_create(case.dt, case.dt, qa, nd_coef=nd_coef, fct_tracer=case.fct if fct is None else fct,
+                     mixed_precision=mixed_precision)
     for k, v in case.state.items():
         ctx.model_set(k, v)
     for k, v in case.ref.items():
