@@ -34,13 +34,17 @@ typedef struct scale_hip_ctx scale_hip_ctx;
 /* flux scheme ids (ATMOS_DYN_FVM_FLUX_TYPE / _TRACER_TYPE) */
 #define SCALE_HIP_UD3 3
 #define SCALE_HIP_CD4 4
+/* tracers only: UD3 with the Koren (1993) limiter, the reference's default
+ * ATMOS_DYN_FVM_FLUX_TRACER_TYPE (scale-rm/src/atmos/mod_atmos_dyn_driver.F90:52),
+ * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_fvm_flux_ud3Koren1993.F90 */
+#define SCALE_HIP_UD3KOREN1993 31
 
 typedef struct {
   int32_t KA, IA, JA;             /* array extents                           */
   int32_t KS, KE, IS, IE, JS, JE; /* interior bounds, Fortran 1-based         */
   int32_t KHALO, IHALO, JHALO;    /* must be 2 (filter_setup, COMMON.F90:137) */
   int32_t flux_scheme;            /* dynamics: SCALE_HIP_UD3 or SCALE_HIP_CD4 */
-  int32_t flux_scheme_tracer;     /* tracers:  SCALE_HIP_UD3 or SCALE_HIP_CD4 */
+  int32_t flux_scheme_tracer;     /* tracers:  SCALE_HIP_UD3, _CD4 or _UD3KOREN1993 */
   int32_t device;                 /* HIP device ordinal                       */
 } scale_hip_grid;
 
@@ -99,6 +103,13 @@ int scale_hip_numfilter_coef(scale_hip_ctx *ctx, double *num_diff,
     const double *CDZ, const double *CDX, const double *CDY, const double *FDZ, const double *FDX, const double *FDY,
     const double *DT, const double *REF_dens, const double *REF_pott,
     const double *ND_COEF, const int32_t *ND_ORDER, const double *ND_SFC_FACT, const int32_t *ND_USE_RS);
+
+/* Rayleigh damping coefficient wdamp_coef(KA) from the face heights FZ(0:KA)
+ * (KA+1 values), the damping time scale and the height above which it acts:
+ * ATMOS_DYN_wdamp_setup, scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:323-381.
+ * Host computation (no device call); grid supplies KA, KS, KE. */
+int scale_hip_wdamp_setup(const scale_hip_grid *grid, double *wdamp_coef, const double *wdamp_tau,
+                          const double *wdamp_height, const double *FZ);
 
 /* ---- device-resident integration --------------------------------------------
  * The state stays in HBM between calls; host arrays cross PCIe only in

@@ -167,6 +167,14 @@ def solve_direct(g, Cvec, F1, F2, F3):
     return Cvec
 
 
+def wdamp_setup(g, wdamp_tau, wdamp_height, FZ):
+    """ATMOS_DYN_wdamp_setup; FZ holds FZ(0:KA)."""
+    out = np.zeros(g.KA)
+    lib().orc_wdamp_setup(C.byref(grid_struct(g)), ptr(out), C.c_double(wdamp_tau), C.c_double(wdamp_height),
+                          ptr(np.ascontiguousarray(FZ, dtype=np.float64)))
+    return out
+
+
 def halo_periodic(g, a):
     gs = grid_struct(g)
     lib().orc_halo_periodic(C.byref(gs), ptr(a))

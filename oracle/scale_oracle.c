@@ -80,8 +80,18 @@ static inline double fmax2(double a, double b) { return a > b ? a : b; }
 static inline double fmin2(double a, double b) { return a < b ? a : b; }
 
 /* ---- face values (ERB:59-203) ------------------------------------------- */
+/* Koren (1993) limiter, ud3Koren1993.F90:1690-1710 */
+static inline double koren_phi(double v1, double v2, double v3) {
+  const double zerosw1 = EPS_RP - fsign(EPS_RP, fabs(v1 - v2) - EPS_RP);
+  const double zerosw2 = EPS_RP - fsign(EPS_RP, fabs(v2 - v3) - EPS_RP);
+  const double r2 = 2.0 * (v1 - v2 + zerosw1 * zerosw2) / (v2 - v3 + zerosw2);
+  return fmax2(0.0, fmin2(r2, fmin2((1.0 + r2) / 3.0, 2.0)));
+}
 /* interior face between q0 = val(idx) and q1 = val(idx+1); qm1 = val(idx-1), q2 = val(idx+2) */
 static inline double fv_int(int s, double qm1, double q0, double q1, double q2, double vel) {
+  if (s == ORC_UD3KOREN) /* ud3Koren1993.F90:201-209 (z), 299-307 (x), 356-364 (y) */
+    return (q0 + 0.5 * koren_phi(q1, q0, qm1) * (q0 - qm1)) * (0.5 + fsign(0.5, vel))
+         + (q1 + 0.5 * koren_phi(q0, q1, q2) * (q1 - q2)) * (0.5 - fsign(0.5, vel));
   if (s == ORC_UD3) /* UD3.F90:194-197 */
     return (F31 * (q2 + qm1) + F32 * (q1 + q0)) - (F31 * (q2 - qm1) + F33 * (q1 - q0)) * fsign(1.0, vel);
   if (s == ORC_CD4) /* CD4.F90:190-192 */
@@ -91,12 +101,16 @@ static inline double fv_int(int s, double qm1, double q0, double q1, double q2, 
 }
 /* lowest face: q0 = val(KS), q1 = val(KS+1), q2 = val(KS+2) (UD3.F90:225-229, CD4.F90:218-219) */
 static inline double fv_lo(int s, double q0, double q1, double q2, double vel) {
+  if (s == ORC_UD3KOREN) /* ud3Koren1993.F90:234-241 */
+    return q0 * (0.5 + fsign(0.5, vel)) + (q1 + 0.5 * koren_phi(q0, q1, q2) * (q1 - q2)) * (0.5 - fsign(0.5, vel));
   if (s == ORC_UD3)
     return F2 * (q1 + q0) * (0.5 + fsign(0.5, vel)) + (2.0 * q0 + 5.0 * q1 - q2) / 6.0 * (0.5 - fsign(0.5, vel));
   return F2 * (q1 + q0);
 }
 /* highest face: qm1 = val(KE-2), q0 = val(KE-1), q1 = val(KE) (UD3.F90:231-235, CD4.F90:221-222) */
 static inline double fv_hi(int s, double qm1, double q0, double q1, double vel) {
+  if (s == ORC_UD3KOREN) /* ud3Koren1993.F90:242-249, limiter arguments as in the reference */
+    return (q0 + 0.5 * koren_phi(qm1, q0, q1) * (q0 - q1)) * (0.5 + fsign(0.5, vel)) + q1 * (0.5 - fsign(0.5, vel));
   if (s == ORC_UD3)
     return (2.0 * q1 + 5.0 * q0 - qm1) / 6.0 * (0.5 + fsign(0.5, vel)) + F2 * (q1 + q0) * (0.5 - fsign(0.5, vel));
   return F2 * (q1 + q0);

@@ -38,6 +38,7 @@ extern "C" {
 #define ORC_UD1 1
 #define ORC_UD3 3
 #define ORC_CD4 4
+#define ORC_UD3KOREN 31 /* UD3KOREN1993: tracer scheme, UD3 with the Koren limiter */
 
 typedef struct {
   int KA, IA, JA;

@@ -275,9 +275,11 @@ int scale_hip_init(const scale_hip_grid* g, scale_hip_ctx** out) {
     return fail("grid bounds must be KS=IS=JS=3 and KA=KE+2, IA=IE+2, JA=JE+2 (halo 2)");
   if (KE - KS + 1 < 4) return fail("KMAX must be >= 4");
   if (IE - IS + 1 < 2 || JE - JS + 1 < 2) return fail("IMAX and JMAX must be >= 2");
-  if ((g->flux_scheme != SCALE_HIP_UD3 && g->flux_scheme != SCALE_HIP_CD4) ||
-      (g->flux_scheme_tracer != SCALE_HIP_UD3 && g->flux_scheme_tracer != SCALE_HIP_CD4))
-    return fail("flux schemes must be UD3 (3) or CD4 (4)");
+  if (g->flux_scheme != SCALE_HIP_UD3 && g->flux_scheme != SCALE_HIP_CD4)
+    return fail("the dynamics flux scheme must be UD3 (3) or CD4 (4)");
+  if (g->flux_scheme_tracer != SCALE_HIP_UD3 && g->flux_scheme_tracer != SCALE_HIP_CD4 &&
+      g->flux_scheme_tracer != SCALE_HIP_UD3KOREN1993)
+    return fail("the tracer flux scheme must be UD3 (3), CD4 (4) or UD3KOREN1993 (31)");
   if (hipSetDevice(g->device) != hipSuccess) return fail("hipSetDevice failed");
   scale_hip_ctx* h = new scale_hip_ctx();
   Ctx& c = h->c;
@@ -489,7 +491,8 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   if (!c.metrics_set) return fail("scale_hip_set_metrics must be called first");
   if (c.model) return fail("model already created");
   if (cfg->nd_order != 1) return fail("nd_order other than 1 is not supported");
-  if (cfg->divdmp_coef != 0.0) return fail("divdmp_coef > 0 is not supported by the resident model");
+  if (cfg->divdmp_coef < 0.0) return fail("divdmp_coef must be >= 0");
+  if (cfg->divdmp_coef > 0.0 && cfg->mixed_precision) return fail("divergence damping is not supported in mixed precision");
   if (!(cfg->dtl > 0.0) || !(cfg->dts > 0.0)) return fail("dtl and dts must be positive");
   if (QA < 0) return fail("QA must be >= 0");
   if (cfg->mixed_precision && c.terrain) return fail("mixed precision needs flat terrain (GSQRT = 1, J13G = J23G = 0, MAPF = 1)");
@@ -695,6 +698,28 @@ int scale_hip_profile_query(scale_hip_ctx* h, const char* name, int64_t* count, 
   long n = 0;
   shp::prof_query(c.prof, name, &n, total_ms);
   *count = n;
+  return 0;
+}
+
+int scale_hip_wdamp_setup(const scale_hip_grid* g, double* wdamp_coef, const double* wdamp_tau, const double* wdamp_height,
+                          const double* FZ) {
+  if (!g || !wdamp_coef || !wdamp_tau || !wdamp_height || !FZ) return fail("wdamp_setup: null argument");
+  const int KA = g->KA, KS = g->KS - 1, KE = g->KE - 1;  // 0-based levels; FZ holds FZ(0:KA)
+  if (KS < 1 || KE >= KA || KS > KE) return fail("wdamp_setup: inconsistent KS/KE/KA");
+  const double* fz = FZ + 1;  // fz[k] = FZ(k+1), the top face of level k
+  const double h = *wdamp_height;
+  if (h < 0.0 || fz[KE] - h < shp::EPS_RP) {
+    for (int k = 0; k < KA; ++k) wdamp_coef[k] = 0.0;
+    return 0;
+  }
+  const double alpha = 1.0 / *wdamp_tau;
+  const double PI = 3.14159265358979323846;
+  for (int k = KS; k <= KE; ++k) {
+    const double sw = 0.5 + copysign(0.5, fz[k] - h);
+    wdamp_coef[k] = alpha * sw * 0.5 * (1.0 - cos(PI * (fz[k] - h) / (fz[KE] - h)));
+  }
+  for (int k = 0; k < KS; ++k) wdamp_coef[k] = wdamp_coef[KS];
+  for (int k = KE + 1; k < KA; ++k) wdamp_coef[k] = wdamp_coef[KE];
   return 0;
 }
 

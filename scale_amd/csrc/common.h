@@ -41,6 +41,7 @@ constexpr double F41 = 7.0 / 12.0;
 constexpr double F42 = -1.0 / 12.0;
 
 constexpr int UD1 = 1, UD3 = 3, CD4 = 4;
+constexpr int UD3KOREN = 31;  // UD3 with the Koren (1993) limiter (tracer scheme UD3KOREN1993)
 
 struct Dims {
   int KA, IA, JA;
@@ -56,6 +57,8 @@ __host__ __device__ inline size_t IX(const Dims& d, int k, int i, int j) {
 // Fortran sign(a,b)
 __device__ __forceinline__ double fsign(double a, double b) { return __builtin_copysign(__builtin_fabs(a), b); }
 __device__ __forceinline__ float fsign(float a, float b) { return __builtin_copysignf(__builtin_fabsf(a), b); }
+__device__ __forceinline__ double fabs_t(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float fabs_t(float x) { return __builtin_fabsf(x); }
 __device__ __forceinline__ double fmax2(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double fmin2(double a, double b) { return a < b ? a : b; }
 
@@ -63,10 +66,26 @@ __device__ __forceinline__ double fmin2(double a, double b) { return a < b ? a :
 // reference's FP64 build; float is the FP32 arithmetic of the mixed-precision
 // path (the reference's SINGLE build, scale_precision.F90:37-43), with the
 // coefficients rounded to float once.
+// Koren (1993) limiter of the UD3KOREN1993 scheme
+// (scale_atmos_dyn_fvm_flux_ud3Koren1993.F90:1690-1710)
+template <typename T>
+__device__ __forceinline__ T koren_phi(T v1, T v2, T v3) {
+  const T E = T(EPS_RP);
+  const T z1 = E - fsign(E, fabs_t(v1 - v2) - E);
+  const T z2 = E - fsign(E, fabs_t(v2 - v3) - E);
+  const T r2 = T(2) * (v1 - v2 + z1 * z2) / (v2 - v3 + z2);
+  const T m = ((T(1) + r2) / T(3) < T(2)) ? (T(1) + r2) / T(3) : T(2);
+  const T m2 = (r2 < m) ? r2 : m;
+  return (T(0) > m2) ? T(0) : m2;
+}
+
 // interior face between q0 = val(idx) and q1 = val(idx+1) (ERB flux(), order S)
 template <int S, typename T>
 __device__ __forceinline__ T fv_int(T qm1, T q0, T q1, T q2, T vel) {
-  if constexpr (S == UD3)
+  if constexpr (S == UD3KOREN)
+    return (q0 + T(0.5) * koren_phi(q1, q0, qm1) * (q0 - qm1)) * (T(0.5) + fsign(T(0.5), vel))
+         + (q1 + T(0.5) * koren_phi(q0, q1, q2) * (q1 - q2)) * (T(0.5) - fsign(T(0.5), vel));
+  else if constexpr (S == UD3)
     return (T(F31) * (q2 + qm1) + T(F32) * (q1 + q0)) - (T(F31) * (q2 - qm1) + T(F33) * (q1 - q0)) * fsign(T(1), vel);
   else if constexpr (S == CD4)
     return T(F41) * (q1 + q0) + T(F42) * (q2 + qm1);
@@ -76,7 +95,10 @@ __device__ __forceinline__ T fv_int(T qm1, T q0, T q1, T q2, T vel) {
 // lowest face (val(KS), val(KS+1), val(KS+2)) -- scale_atmos_dyn_fvm_flux_ud3.F90:225-229
 template <int S, typename T>
 __device__ __forceinline__ T fv_lo(T q0, T q1, T q2, T vel) {
-  if constexpr (S == UD3)
+  if constexpr (S == UD3KOREN)  // ud3Koren1993.F90:234-241
+    return q0 * (T(0.5) + fsign(T(0.5), vel))
+         + (q1 + T(0.5) * koren_phi(q0, q1, q2) * (q1 - q2)) * (T(0.5) - fsign(T(0.5), vel));
+  else if constexpr (S == UD3)
     return T(F2) * (q1 + q0) * (T(0.5) + fsign(T(0.5), vel)) + (T(2) * q0 + T(5) * q1 - q2) / T(6) * (T(0.5) - fsign(T(0.5), vel));
   else
     return T(F2) * (q1 + q0);
@@ -84,7 +106,10 @@ __device__ __forceinline__ T fv_lo(T q0, T q1, T q2, T vel) {
 // highest face (val(KE-2), val(KE-1), val(KE)) -- scale_atmos_dyn_fvm_flux_ud3.F90:231-235
 template <int S, typename T>
 __device__ __forceinline__ T fv_hi(T qm1, T q0, T q1, T vel) {
-  if constexpr (S == UD3)
+  if constexpr (S == UD3KOREN)  // ud3Koren1993.F90:242-249 (the limiter argument order of the reference)
+    return (q0 + T(0.5) * koren_phi(qm1, q0, q1) * (q0 - q1)) * (T(0.5) + fsign(T(0.5), vel))
+         + q1 * (T(0.5) - fsign(T(0.5), vel));
+  else if constexpr (S == UD3)
     return (T(2) * q1 + T(5) * q0 - qm1) / T(6) * (T(0.5) + fsign(T(0.5), vel)) + T(F2) * (q1 + q0) * (T(0.5) - fsign(T(0.5), vel));
   else
     return T(F2) * (q1 + q0);

@@ -192,6 +192,67 @@ static hipError_t convert(To* dst, const From* src, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---- 3-D momentum divergence for divergence damping ----------------------------
+// ATMOS_DYN_divergence (COMMON.F90:1437-1515): one thread per point of
+// i = IS..IE+1, j = JS..JE+1, k = KS-1..KE+1, the levels KS and KE with their
+// one-sided vertical terms
+template <bool TERRAIN>
+__global__ void divergence_kernel(Dims d, Grid1D g, Met<TERRAIN> m, double J33G, double* DDIV, const double* MOMZ,
+                                  const double* MOMX, const double* MOMY) {
+  const int i = d.IS + blockIdx.x * 64 + threadIdx.x;
+  const int k = d.KS - 1 + blockIdx.y * 4 + threadIdx.y;
+  const int j = d.JS + blockIdx.z;
+  if (i > d.IE + 1 || k > d.KE + 1 || j > d.JE + 1) return;
+  const int KS = d.KS, KE = d.KE;
+  auto X = [&](int kk, int ii, int jj) { return MOMX[IX(d, kk, ii, jj)]; };
+  auto Y = [&](int kk, int ii, int jj) { return MOMY[IX(d, kk, ii, jj)]; };
+  auto p = [&](int kk, int ii, int jj) { return IX(d, kk, ii, jj); };
+  const double hdiv = m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY)
+                    * ((X(k, i, j) * m.gs(I_UYZ, p(k, i, j)) / m.mapf(i, j, 1, I_UY)
+                        - X(k, i - 1, j) * m.gs(I_UYZ, p(k, i - 1, j)) / m.mapf(i - 1, j, 1, I_UY)) * g.RCDX[i]
+                       + (Y(k, i, j) * m.gs(I_XVZ, p(k, i, j)) / m.mapf(i, j, 0, I_XV)
+                          - Y(k, i, j - 1) * m.gs(I_XVZ, p(k, i, j - 1)) / m.mapf(i, j - 1, 0, I_XV)) * g.RCDY[j]);
+  double v;
+  if (k == KS) {
+    v = J33G * (MOMZ[p(KS, i, j)]) * g.RCDZ[KS]
+      + ((X(KS + 1, i, j) + X(KS + 1, i - 1, j)) * m.j13(I_XYW, p(KS + 1, i, j))
+         - (X(KS - 1, i, j) + X(KS, i - 1, j)) * m.j13(I_XYW, p(KS, i, j))
+         + (Y(KS + 1, i, j) + Y(KS + 1, i, j - 1)) * m.j23(I_XYW, p(KS + 1, i, j))
+         - (Y(KS, i, j) + Y(KS, i, j - 1)) * m.j23(I_XYW, p(KS, i, j))) * g.RFDZ[KS]
+      + hdiv;
+  } else if (k == KE) {
+    v = J33G * (-MOMZ[p(KE - 1, i, j)]) * g.RCDZ[KE]
+      + ((X(KE, i, j) + X(KE, i - 1, j)) * m.j13(I_XYW, p(KE, i, j))
+         - (X(KE - 1, i, j) + X(KE - 1, i - 1, j)) * m.j13(I_XYW, p(KE - 1, i, j))
+         + (Y(KE, i, j) + Y(KE, i, j - 1)) * m.j23(I_XYW, p(KE, i, j))
+         - (Y(KE - 1, i, j) + Y(KE - 1, i, j - 1)) * m.j23(I_XYW, p(KE - 1, i, j))) * g.RFDZ[KE]
+      + hdiv;
+  } else {
+    v = J33G * (MOMZ[p(k, i, j)] - MOMZ[p(k - 1, i, j)]) * g.RCDZ[k]
+      + ((X(k + 1, i, j) + X(k + 1, i - 1, j)) * m.j13(I_XYW, p(k + 1, i, j))
+         - (X(k - 1, i, j) + X(k - 1, i - 1, j)) * m.j13(I_XYW, p(k - 1, i, j))
+         + (Y(k + 1, i, j) + Y(k + 1, i, j - 1)) * m.j23(I_XYW, p(k + 1, i, j))
+         - (Y(k - 1, i, j) + Y(k - 1, i, j - 1)) * m.j23(I_XYW, p(k - 1, i, j))) / (g.FDZ[k] + g.FDZ[k - 1])
+      + hdiv;
+  }
+  DDIV[p(k, i, j)] = v;
+}
+
+hipError_t divergence(const Ctx& c, double* DDIV, const double* MOMZ, const double* MOMX, const double* MOMY, hipStream_t s) {
+  const Dims& d = c.d;
+  const dim3 grid((d.IE - d.IS + 2 + 63) / 64, (d.KE - d.KS + 3 + 3) / 4, d.JE - d.JS + 2);
+  if (c.terrain) {
+    Met<true> m;
+    m.G = c.GSQRT; m.J13 = c.J13G; m.J23 = c.J23G; m.MAPF = c.MAPF; m.n3 = d.n3; m.pl = d.pl; m.IA = d.IA;
+    hipLaunchKernelGGL(divergence_kernel<true>, grid, dim3(64, 4), 0, s, d, c.g1, m, c.J33G, DDIV, MOMZ, MOMX, MOMY);
+  } else {
+    Met<false> m;
+    m.G = c.GSQRT; m.J13 = c.J13G; m.J23 = c.J23G; m.MAPF = c.MAPF; m.n3 = d.n3; m.pl = d.pl; m.IA = d.IA;
+    hipLaunchKernelGGL(divergence_kernel<false>, grid, dim3(64, 4), 0, s, d, c.g1, m, c.J33G, DDIV, MOMZ, MOMX, MOMY);
+  }
+  return hipGetLastError();
+}
+
 // ---- large-step preparation (LARGE.F90:443-471), dry constants ------------------
 __global__ void large_prep(Dims d, const double* RHOT, const double* REF_pres, double* RT2P, double* DPRES0, double* REF_rhot) {
   const int i = blockIdx.x * 64 + threadIdx.x;
@@ -339,6 +400,10 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B) {
   } else {
     const R* cst[5] = {B.st[0], B.st[1], B.st[2], B.st[3], B.st[4]};
     HIPCHK(numfilter(c, B.num_diff, cst, M.REF_dens, M.REF_pott, s));
+  }
+  // divergence for the divergence damping (LARGE.F90:860-872)
+  if constexpr (sizeof(R) == sizeof(double)) {
+    if (M.divdmp_coef > 0.0) HIPCHK(divergence(c, B.DDIV, B.st[1], B.st[2], B.st[3], s));
   }
   // DENS0 = DENS, ... (RK4.F90:326-336): the stages only read the state, so
   // it serves as the start value directly and is overwritten by the combination

@@ -87,6 +87,8 @@ hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], 
 hipError_t numfilter(const Ctx& c, float* num_diff, const float* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s);
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s);
+// ATMOS_DYN_divergence on i = IS..IE+1, j = JS..JE+1, k = KS-1..KE+1
+hipError_t divergence(const Ctx& c, double* DDIV, const double* MOMZ, const double* MOMX, const double* MOMY, hipStream_t s);
 hipError_t halo_fill(const Ctx& c, double* const* arrs, int n, hipStream_t s);
 hipError_t halo_fill(const Ctx& c, float* const* arrs, int n, hipStream_t s);
 // Fortran (k fastest) <-> device (i fastest) layout; ncomp fields of KA*IA*JA

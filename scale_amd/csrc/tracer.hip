@@ -356,6 +356,8 @@ hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s) {
   a.phi_lo = c.scr[0]; a.rjpls = c.scr[1]; a.rjmns = c.scr[2];
   const bool fct = io.flag_fct != 0, along = io.flag_along != 0;
   if (c.scheme_tracer == UD3) return c.terrain ? tracer_t<UD3, true>(c, a, fct, along, s) : tracer_t<UD3, false>(c, a, fct, along, s);
+  if (c.scheme_tracer == UD3KOREN)
+    return c.terrain ? tracer_t<UD3KOREN, true>(c, a, fct, along, s) : tracer_t<UD3KOREN, false>(c, a, fct, along, s);
   return c.terrain ? tracer_t<CD4, true>(c, a, fct, along, s) : tracer_t<CD4, false>(c, a, fct, along, s);
 }
 

@@ -88,6 +88,17 @@ def _l(x):
     return C.byref(C.c_int32(1 if x else 0))
 
 
+def ATMOS_DYN_wdamp_setup(grid, wdamp_tau, wdamp_height, FZ):
+    """Rayleigh damping coefficient wdamp_coef(KA) from FZ(0:KA)
+    (scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:323-381); host only."""
+    g = HipGrid(grid.KA, grid.IA, grid.JA, grid.KS + 1, grid.KE + 1, grid.IS + 1, grid.IE + 1, grid.JS + 1,
+                grid.JE + 1, 2, 2, 2, UD3, UD3, 0)
+    out = np.zeros(grid.KA)
+    fz = np.ascontiguousarray(FZ, dtype=np.float64)
+    _check(load_library().scale_hip_wdamp_setup(C.byref(g), _p(out), _d(wdamp_tau), _d(wdamp_height), _p(fz)))
+    return out
+
+
 class Context:
     """One device context (scale_hip_init / scale_hip_finalize)."""
 
@@ -162,8 +173,9 @@ class Context:
             _d(m["J33G"]), _p(m["MAPF"])))
 
     def model_create(self, dtl, dts, QA, nd_coef=1.0e-4, nd_sfc_fact=1.0, fct_tracer=False, fct_along_stream=True,
-                     mixed_precision=False):
-        cfg = DynConfig(dtl, dts, nd_coef, nd_sfc_fact, 0.0, 1, int(fct_tracer), int(fct_along_stream), int(mixed_precision))
+                     mixed_precision=False, divdmp_coef=0.0):
+        cfg = DynConfig(dtl, dts, nd_coef, nd_sfc_fact, divdmp_coef, 1, int(fct_tracer), int(fct_along_stream),
+                        int(mixed_precision))
         _check(load_library().scale_hip_model_create(self._h, C.byref(cfg), C.c_int32(QA)))
         self.has_model = True
         self.QA = QA
@@ -220,7 +232,8 @@ class Context:
         return s.value or 0
 
 
-def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, device=0, mixed_precision=False) -> Context:
+def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, device=0, mixed_precision=False,
+                    divdmp_coef=0.0) -> Context:
     """Create a context holding the device-resident model of an inputs.Case
     (mixed_precision: FP32 dynamics with FP64 solve and reference state; the
     tracers are left out)."""
@@ -229,7 +242,7 @@ def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, 
     ctx.set_metrics(case.metric)
     qa = 0 if mixed_precision else case.qtrc.shape[-1]
     ctx.model_create(case.dt, case.dt, qa, nd_coef=nd_coef, fct_tracer=case.fct if fct is None else fct,
-                     mixed_precision=mixed_precision)
+                     mixed_precision=mixed_precision, divdmp_coef=divdmp_coef)
     for k, v in case.state.items():
         ctx.model_set(k, v)
     for k, v in case.ref.items():

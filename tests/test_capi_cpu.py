@@ -71,3 +71,21 @@ def test_python_front_end_raises_without_library(tmp_path):
             hip.load_library(str(tmp_path / "missing.so"))
         finally:
             hip._lib = hip._lib_backup
+
+
+@pytest.mark.parametrize("height", [-1.0, 8000.0, 15000.0, 1.0e9])
+def test_wdamp_setup_matches_oracle(height):
+    """The host Rayleigh-coefficient setup equals the oracle's bit for bit, on a
+    stretched grid, including the 'off' cases (negative height, height above
+    the top)."""
+    import numpy as np
+    import pyoracle as po
+    from scale_amd import hip, inputs
+    case = inputs.make_case(kmax=40, imax=8, jmax=8, stretched=(50.0, 500.0), ntracer=0)
+    g = case.grid
+    fz = case.fz
+    got = hip.ATMOS_DYN_wdamp_setup(g, 300.0, height, fz)
+    ref = po.wdamp_setup(g, 300.0, height, fz)
+    assert np.array_equal(got, ref)
+    if 0.0 <= height < fz[g.KE + 1]:
+        assert got.max() > 0.0 and got[g.KS] == 0.0

@@ -98,7 +98,7 @@ def test_numfilter_bitwise(hip):
 
 
 @pytest.mark.parametrize("fct,along", [(False, True), (True, True), (True, False)], ids=["nofct", "fct_along", "fct_7pt"])
-@pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
+@pytest.mark.parametrize("scheme", [3, 4, 31], ids=["UD3", "CD4", "UD3KOREN1993"])
 def test_tracer_bitwise(hip, fct, along, scheme):
     print(f"[tracer] fct={fct} along={along} scheme={scheme}", flush=True)
     case = H.small_case(seed=55)
@@ -124,6 +124,37 @@ def test_tracer_bitwise(hip, fct, along, scheme):
         ctx.ATMOS_DYN_Tstep_tracer_fvm_heve(got, q, q0, rq, case.state["DENS"], st["DENS"], mflx, nd, m["GSQRT"], mapf_xy,
                                             m["CDZ"], m["RCDZ"], m["RCDX"], m["RCDY"], 2.0, fct, along)
     H.compare(ref, got, "QTRCo")
+
+
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+def test_model_parity_divergence_damping(hip, terrain):
+    """Large steps with divergence damping (ATMOS_DYN_divergence each short
+    step, LARGE.F90:860-872) and Rayleigh damping near the top, against the
+    oracle: 5 steps, 1e-12 relative to each field's scale."""
+    print(f"[model] divdmp 0.2 + wdamp, terrain={terrain}, 5 steps", flush=True)
+    case = inputs.make_case(kmax=20, imax=16, jmax=12, noise=0.01, ntracer=0, bubble_r=2000.0, bubble_zc=2000.0,
+                            momz_noise=0.05, u0=6.0, v0=-2.0)
+    g = case.grid
+    wd = hip.ATMOS_DYN_wdamp_setup(g, 100.0, 3000.0, case.fz)
+    assert wd.max() > 0.0
+    metric = H.terrain_metrics(case, amp=0.02) if terrain else case.metric
+    case = inputs.Case(**{**case.__dict__, "wdamp_coef": np.asfortranarray(wd), "metric": metric})
+    m = po.Model(case, divdmp_coef=0.2)
+    m.step(5)
+    ctx = hip.model_from_case(case, divdmp_coef=0.2)
+    try:
+        ctx.model_large_step(5)
+        ctx.synchronize()
+        sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+        for k in H.VARS:
+            got = ctx.model_get(k)
+            ref = m.state[k]
+            scale = np.abs(ref[sl]).max()
+            d = np.abs(got[sl] - ref[sl]).max()
+            print(f"  {k}: max|diff| {d:.3e} (scale {scale:.3e})", flush=True)
+            assert d <= 1.0e-12 * scale, k
+    finally:
+        ctx.close()
 
 
 def test_model_parity_warm_bubble(hip):

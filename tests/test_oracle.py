@@ -15,7 +15,7 @@ import helpers as H
 import pyoracle as po
 from scale_amd import inputs
 
-UD1, UD3, CD4 = 1, 3, 4
+UD1, UD3, CD4, UD3K = 1, 3, 4, 31
 
 
 # ---------------------------------------------------------------------------
@@ -31,6 +31,62 @@ def cell_averages(coef, xs, h):
 
 def poly(coef, x):
     return sum(c * x ** n for n, c in enumerate(coef))
+
+
+# UD3KOREN1993 (ud3Koren1993.F90): Koren's (1993) limited third-order upwind
+# reconstruction.  Pins: constancy, exactness for linear data (limiter value 1
+# at the smoothness ratio r = 1), the TVD bound (face value between the two
+# cells it separates), first order at extrema, and the mirror symmetry
+# between the two wind directions (which fixes the limiter's argument order).
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("vel", [3.0, -3.0])
+def test_koren_constant_field(kind, vel):
+    assert po.face_value(UD3K, kind, 1.25, 1.25, 1.25, 1.25, vel) == 1.25
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_koren_exact_for_linear_data(seed):
+    rng = np.random.default_rng(seed)
+    a, d = rng.normal(), rng.uniform(0.5, 2.0) * rng.choice([-1, 1])
+    qm1, q0, q1, q2 = (a + d * n for n in range(4))
+    # upwind from the left: q0 + d/2, upwind from the right: q1 - d/2; both the face midpoint
+    assert po.face_value(UD3K, 0, qm1, q0, q1, q2, 1.0) == pytest.approx(a + 1.5 * d, rel=1e-14, abs=1e-14)
+    assert po.face_value(UD3K, 0, qm1, q0, q1, q2, -1.0) == pytest.approx(a + 1.5 * d, rel=1e-14, abs=1e-14)
+
+
+def test_koren_tvd_bound_and_extrema():
+    rng = np.random.default_rng(11)
+    for _ in range(2000):
+        qm1, q0, q1, q2 = rng.normal(size=4)
+        for vel in (1.0, -1.0):
+            v = po.face_value(UD3K, 0, qm1, q0, q1, q2, vel)
+            lo, hi = min(q0, q1), max(q0, q1)
+            assert lo - 1e-14 <= v <= hi + 1e-14
+        # a local maximum at the upwind cell: first order (the upwind value)
+        if q0 > max(qm1, q1):
+            assert po.face_value(UD3K, 0, qm1, q0, q1, q2, 1.0) == q0
+
+
+def test_koren_wind_mirror_symmetry():
+    rng = np.random.default_rng(12)
+    for _ in range(500):
+        qm1, q0, q1, q2 = rng.normal(size=4)
+        assert po.face_value(UD3K, 0, qm1, q0, q1, q2, -2.0) == po.face_value(UD3K, 0, q2, q1, q0, qm1, 2.0)
+
+
+def test_koren_boundary_faces():
+    """Lowest face: upwind from below is first order (the KS value); highest
+    face: upwind from above is first order (the KE value).  The limited side
+    of the highest face differences toward the boundary cell as
+    ud3Koren1993.F90:242-249 is written, so on linear data it returns the
+    value half a cell below q0 rather than the face midpoint."""
+    rng = np.random.default_rng(13)
+    for _ in range(200):
+        q0, q1, q2 = rng.normal(size=3)
+        assert po.face_value(UD3K, 1, 0.0, q0, q1, q2, 1.0) == q0
+        assert po.face_value(UD3K, 2, q0, q1, q2, 0.0, -1.0) == q2
+    a, d = 0.3, 0.5
+    assert po.face_value(UD3K, 2, a, a + d, a + 2 * d, 0.0, 1.0) == pytest.approx(a + 0.5 * d, rel=1e-14)
 
 
 @pytest.mark.parametrize("seed", range(5))
