@@ -673,8 +673,101 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
 
   for (int k = KS; k <= KE; ++k) {
     const int s1 = s0 ^ 1;
-    // ---- loads of this level that do not go through LDS, issued first so
-    // that their latency overlaps the staging and the face evaluation ----
+    // every other global load of the level is issued here, into registers,
+    // before any LDS write, so that the whole level waits on memory once:
+    // num_diff of the faces this thread evaluates, the staged tiles, and the
+    // DPRES inputs
+    constexpr int NX = (TJ * (TI + 1) + NT - 1) / NT, NY = ((TJ + 1) * TI + NT - 1) / NT;
+    constexpr int NC = (RW * CW + NT - 1) / NT, NP = ((TJ + 1) * (TI + 1) + NT - 1) / NT;
+    R fx[NX][5], fy[NY][5];
+#pragma unroll
+    for (int h = 0; h < NX; ++h) {
+      const int q = tid + h * NT;
+      const int row = q / (TI + 1), x = q - row * (TI + 1);
+      const int ii = i0 - 1 + x, jj = j0 + row;
+      const bool ok = q < TJ * (TI + 1) && ii <= d.IE && jj <= d.JE;
+      const size_t g = ok ? gix(k, ii, jj) : 0;
+      fx[h][0] = ok ? nd(I_DENS, XDIR, g) : R(0);
+      fx[h][1] = ok ? nd(I_RHOT, XDIR, g) : R(0);
+      fx[h][2] = (ok && k < KE) ? nd(I_MOMZ, XDIR, g) : R(0);
+      fx[h][3] = ok ? nd(I_MOMX, XDIR, g + 1) : R(0);
+      fx[h][4] = ok ? nd(I_MOMY, XDIR, g) : R(0);
+    }
+#pragma unroll
+    for (int h = 0; h < NY; ++h) {
+      const int q = tid + h * NT;
+      const int y = q / TI, col = q - y * TI;
+      const int ii = i0 + col, jj = j0 - 1 + y;
+      const bool ok = q < (TJ + 1) * TI && ii <= d.IE && jj <= d.JE;
+      const size_t g = ok ? gix(k, ii, jj) : 0;
+      fy[h][0] = ok ? nd(I_DENS, YDIR, g) : R(0);
+      fy[h][1] = ok ? nd(I_RHOT, YDIR, g) : R(0);
+      fy[h][2] = (ok && k < KE) ? nd(I_MOMZ, YDIR, g) : R(0);
+      fy[h][3] = ok ? nd(I_MOMX, YDIR, g) : R(0);
+      fy[h][4] = ok ? nd(I_MOMY, YDIR, g + d.IA) : R(0);
+    }
+    R tD[NC], tX[NC], tY[NC], tW[NC], tR[NC];
+#pragma unroll
+    for (int h = 0; h < NC; ++h) {
+      const int q = tid + h * NT;
+      const int r = q / CW, c = q - r * CW;
+      const int ii = i0 - 2 + c, jj = j0 - 2 + r;
+      const bool ok = q < RW * CW && ii < d.IA && jj < d.JA;
+      const size_t g = ok ? gix(k, ii, jj) : 0;
+      tD[h] = ok ? a.DENS[g + d.pl] : R(0);
+      tX[h] = ok ? a.MOMX[g + d.pl] : R(0);
+      tY[h] = ok ? a.MOMY[g + d.pl] : R(0);
+      tW[h] = ok ? a.MOMZ[g] : R(0);
+      tR[h] = ok ? a.RHOT[g] : R(0);
+    }
+    double pD0[NP], pRT[NP], pRR[NP];
+    R pR[NP];
+#pragma unroll
+    for (int h = 0; h < NP; ++h) {
+      const int q = tid + h * NT;
+      const int r = q / (TI + 1), c = q - r * (TI + 1);
+      const int ii = i0 + c, jj = j0 + r;
+      const bool ok = q < (TJ + 1) * (TI + 1) && ii < d.IA && jj < d.JA;
+      const size_t g = ok ? gix(k, ii, jj) : 0;
+      pD0[h] = ok ? a.DPRES0[g] : 0.0;
+      pRT[h] = ok ? a.RT2P[g] : 0.0;
+      pRR[h] = ok ? a.REF_rhot[g] : 0.0;
+      pR[h] = ok ? a.RHOT[g] : R(0);
+    }
+    // ---- (a) staging ----
+#pragma unroll
+    for (int h = 0; h < NX; ++h) {
+      const int q = tid + h * NT;
+      if (q >= TJ * (TI + 1)) continue;
+      const int row = q / (TI + 1), x = q - row * (TI + 1);
+      xM[row][x] = fx[h][0]; xT[row][x] = fx[h][1]; xW[row][x] = fx[h][2]; xU[row][x] = fx[h][3]; xV[row][x] = fx[h][4];
+    }
+#pragma unroll
+    for (int h = 0; h < NY; ++h) {
+      const int q = tid + h * NT;
+      if (q >= (TJ + 1) * TI) continue;
+      const int y = q / TI, col = q - y * TI;
+      yM[y][col] = fy[h][0]; yT[y][col] = fy[h][1]; yW[y][col] = fy[h][2]; yU[y][col] = fy[h][3]; yV[y][col] = fy[h][4];
+    }
+#pragma unroll
+    for (int h = 0; h < NC; ++h) {
+      const int q = tid + h * NT;
+      if (q >= RW * CW) continue;
+      const int r = q / CW, c = q - r * CW;
+      const int ii = i0 - 2 + c, jj = j0 - 2 + r;
+      sD[s1][r][c] = tD[h]; sMX[s1][r][c] = tX[h]; sMY[s1][r][c] = tY[h]; sW[r][c] = tW[h];
+      sPO[r][c] = (ii < d.IA && jj < d.JA) ? tR[h] / sD[s0][r][c] : R(0);
+    }
+#pragma unroll
+    for (int h = 0; h < NP; ++h) {
+      const int q = tid + h * NT;
+      if (q >= (TJ + 1) * (TI + 1)) continue;
+      const int r = q / (TI + 1), c = q - r * (TI + 1);
+      sDP[r][c] = pD0[h] + pRT[h] * ((double)pR[h] - pRR[h]);
+    }
+    __syncthreads();
+    // ---- the column's own loads of this level, in flight during the face
+    // evaluation ----
     const size_t gk = own ? gix(k, i, j) : 0;
     R o_nZD = 0, o_nZT = 0, o_nZW = 0, o_nZX = 0, o_nZY = 0;
     R o_Dt = 0, o_Rt = 0, o_Wt = 0, o_Xt = 0, o_Yt = 0, o_X0 = 0, o_Y0 = 0, o_Z0 = 0;
@@ -690,57 +783,6 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       const size_t g2 = gk + 2 * d.pl;
       o_R2 = a.RHOT[g2]; o_D2 = a.DENS[g2]; o_W2 = a.MOMZ[g2]; o_X2 = a.MOMX[g2]; o_Y2 = a.MOMY[g2];
     }
-    // num_diff of the faces: parked in the face arrays, which the face
-    // evaluation reads and then overwrites position by position
-    for (int q = tid; q < TJ * (TI + 1); q += NT) {
-      const int row = q / (TI + 1), x = q - row * (TI + 1);
-      const int ii = i0 - 1 + x, jj = j0 + row;
-      if (ii > d.IE || jj > d.JE) continue;
-      const size_t g = gix(k, ii, jj);
-      xM[row][x] = nd(I_DENS, XDIR, g);
-      xT[row][x] = nd(I_RHOT, XDIR, g);
-      xW[row][x] = (k < KE) ? nd(I_MOMZ, XDIR, g) : R(0);
-      xU[row][x] = nd(I_MOMX, XDIR, g + 1);
-      xV[row][x] = nd(I_MOMY, XDIR, g);
-    }
-    for (int q = tid; q < (TJ + 1) * TI; q += NT) {
-      const int y = q / TI, col = q - y * TI;
-      const int ii = i0 + col, jj = j0 - 1 + y;
-      if (ii > d.IE || jj > d.JE) continue;
-      const size_t g = gix(k, ii, jj);
-      yM[y][col] = nd(I_DENS, YDIR, g);
-      yT[y][col] = nd(I_RHOT, YDIR, g);
-      yW[y][col] = (k < KE) ? nd(I_MOMZ, YDIR, g) : R(0);
-      yU[y][col] = nd(I_MOMX, YDIR, g);
-      yV[y][col] = nd(I_MOMY, YDIR, g + d.IA);
-    }
-    // ---- (a) staging ----
-    fill(sD[s1], a.DENS, k + 1); fill(sMX[s1], a.MOMX, k + 1); fill(sMY[s1], a.MOMY, k + 1);
-    fill(sW, a.MOMZ, k);
-    for (int q = tid; q < RW * CW; q += NT) {
-      const int r = q / CW, c = q - r * CW;
-      const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      sPO[r][c] = (ii < d.IA && jj < d.JA) ? a.RHOT[gix(k, ii, jj)] / sD[s0][r][c] : R(0);
-    }
-    for (int q = tid; q < (TJ + 1) * (TI + 1); q += NT) {
-      const int r = q / (TI + 1), c = q - r * (TI + 1);
-      const int ii = i0 + c, jj = j0 + r;
-      double v = 0.0;
-      if (ii < d.IA && jj < d.JA) {
-        const size_t g = gix(k, ii, jj);
-        v = a.DPRES0[g] + a.RT2P[g] * ((double)a.RHOT[g] - a.REF_rhot[g]);
-      }
-      sDP[r][c] = v;
-    }
-    // own-column windows move up one level
-    if (own) {
-      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = o_R2 / o_D2;
-      ww[0] = ww[1]; ww[1] = ww[2]; ww[2] = ww[3]; ww[3] = o_W2;
-      xw[0] = xw[1]; xw[1] = xw[2]; xw[2] = xw[3]; xw[3] = o_X2;
-      yw[0] = yw[1]; yw[1] = yw[2]; yw[2] = yw[3]; yw[3] = o_Y2;
-    }
-    __syncthreads();
-
     // ---- (b) faces ----
     const R f1 = R(CDZ[k] / (CDZ[k] + CDZ[k + 1]));      // F2H(k,1)
     const R f2 = R(CDZ[k + 1] / (CDZ[k] + CDZ[k + 1]));  // F2H(k,2)
@@ -813,6 +855,14 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       }
     }
     __syncthreads();
+
+    // own-column windows move up one level
+    if (own) {
+      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = o_R2 / o_D2;
+      ww[0] = ww[1]; ww[1] = ww[2]; ww[2] = ww[3]; ww[3] = o_W2;
+      xw[0] = xw[1]; xw[1] = xw[2]; xw[2] = xw[3]; xw[3] = o_X2;
+      yw[0] = yw[1]; yw[1] = yw[2]; yw[2] = yw[3]; yw[3] = o_Y2;
+    }
 
     // ---- (c) the column ----
     if (own) {
