@@ -48,8 +48,24 @@ struct HeviArgsT {
   R *DENS_RK, *MOMZ_RK, *MOMX_RK, *MOMY_RK, *RHOT_RK, *mflx_hi, *tflx_hi;
   // scratch (n3 each)
   double *PT, *e, *f;
+  // RK4 combination fused into the last stage (hevi_tile with CMB)
+  const R *c1[5], *c2[5], *c3[5];
+  const R *cm[3], *ct[3];
+  R* st[5];
+  R *mflx_av, *tflx_av;
 };
 using HeviArgs = HeviArgsT<double>;
+
+// RK4 combination (tinteg_short_rk4.F90:560-660): (2 r1 + 4 r2 + 2 r3 + r4 - 3 s0) / 6
+template <typename R>
+__device__ __forceinline__ R rk4c(R r1, R r2, R r3, R r4, R s0) {
+  return (r1 * R(2) + r2 * R(4) + r3 * R(2) + r4 - s0 * R(3)) / R(6);
+}
+// (f1 + 2 f2 + 2 f3 + f4) / 6
+template <typename R>
+__device__ __forceinline__ R rk4f(R f1, R f2, R f3, R f4) {
+  return (f1 + f2 * R(2) + f3 * R(2) + f4) / R(6);
+}
 
 template <int S, bool TERRAIN, bool DIVDMP>
 struct Ops {
@@ -617,7 +633,7 @@ __global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles,
 // general expressions is a multiplication or division by 1.0 (or an addition
 // of 0.0), so the expressions below, written without them, give the same
 // IEEE results (HEVI.F90:346-1047; UD3.F90/CD4.F90 flux routines).
-template <int S, bool DIVDMP, int TJ, typename R>
+template <int S, bool DIVDMP, int TJ, typename R, bool CMB>
 __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni_tiles, unsigned nblocks) {
   constexpr int TI = 64, CW = TI + 4, RW = TJ + 4, NT = TI * TJ;
   __shared__ R sD[2][RW][CW], sMX[2][RW][CW], sMY[2][RW][CW];
@@ -816,7 +832,12 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
         xV[row][x] = vel * fv_int<S>(sMY[s0][r][c - 1], sMY[s0][r][c], sMY[s0][r][c + 1], sMY[s0][r][c + 2], vel)
                    + ndq[4];
       }
-      if (x > 0 || i0 == d.IS) {  // faces IS-1 and those owned by this tile
+      if constexpr (CMB) {  // averaged over the stages at the interior faces this tile owns
+        if (x > 0) {
+          a.mflx_av[g + n3] = rk4f(a.cm[0][g + n3], a.cm[1][g + n3], a.cm[2][g + n3], vm);
+          a.tflx_av[g + n3] = rk4f(a.ct[0][g + n3], a.ct[1][g + n3], a.ct[2][g + n3], xT[row][x]);
+        }
+      } else if (x > 0 || i0 == d.IS) {  // faces IS-1 and those owned by this tile
         a.mflx_hi[g + n3] = vm;
         a.tflx_hi[g + n3] = xT[row][x];
       }
@@ -849,7 +870,12 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
         yV[y][col] = vel * fv_int<S>(sMY[s0][r - 1][c], sMY[s0][r][c], sMY[s0][r + 1][c], sMY[s0][r + 2][c], vel)
                    + ndq[4];
       }
-      if (y > 0 || j0 == d.JS) {
+      if constexpr (CMB) {
+        if (y > 0) {
+          a.mflx_av[g + 2 * n3] = rk4f(a.cm[0][g + 2 * n3], a.cm[1][g + 2 * n3], a.cm[2][g + 2 * n3], vm);
+          a.tflx_av[g + 2 * n3] = rk4f(a.ct[0][g + 2 * n3], a.ct[1][g + 2 * n3], a.ct[2][g + 2 * n3], yT[y][col]);
+        }
+      } else if (y > 0 || j0 == d.JS) {
         a.mflx_hi[g + 2 * n3] = vm;
         a.tflx_hi[g + 2 * n3] = yT[y][col];
       }
@@ -987,8 +1013,19 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
         ptm = pt0;
       }
       // stores of this level
-      a.MOMX_RK[g] = MXn;
-      a.MOMY_RK[g] = MYn;
+      if constexpr (CMB) {
+        const R mx = rk4c(a.c1[2][g], a.c2[2][g], a.c3[2][g], MXn, o_X0);
+        const R my = rk4c(a.c1[3][g], a.c2[3][g], a.c3[3][g], MYn, o_Y0);
+        a.st[2][g] = mx;
+        a.st[3][g] = my;
+        if (k == KS || k == KE) {  // ghost levels copy the combined KS / KE value
+          const int lo = (k == KS) ? 0 : KE + 1, hi = (k == KS) ? KS - 1 : d.KA - 1;
+          for (int kk = lo; kk <= hi; ++kk) { a.st[2][p0 + d.pl * (size_t)kk] = mx; a.st[3][p0 + d.pl * (size_t)kk] = my; }
+        }
+      } else {
+        a.MOMX_RK[g] = MXn;
+        a.MOMY_RK[g] = MYn;
+      }
       a.DENS_RK[g] = Sr;  // Sr/St wait in the outputs until the back pass
       a.RHOT_RK[g] = St;
       if (k <= KE - 1) { a.mflx_hi[g] = mZ; a.tflx_hi[g] = tZ; a.PT[g] = (double)PT; }
@@ -1010,48 +1047,82 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
   // ---- downward: back substitution and updates in double (HEVI.F90:728-790) ----
   R* mfZ = a.mflx_hi; R* tfZ = a.tflx_hi;
   auto px = [&](int k) { return p0 + d.pl * (size_t)k; };
+  // final value of level q: stored as the stage output, or combined with
+  // stages 1-3 into the state (plus the ghost levels at KS and KE)
+  auto put_state = [&](int v, size_t q, R val, R s0v, int k) {
+    if constexpr (CMB) {
+      const R c = rk4c(a.c1[v][q], a.c2[v][q], a.c3[v][q], val, s0v);
+      a.st[v][q] = c;
+      if (k == KS || k == KE) {
+        const int lo = (k == KS) ? 0 : KE + 1, hi = (k == KS) ? KS - 1 : d.KA - 1;
+        for (int kk = lo; kk <= hi; ++kk) a.st[v][p0 + d.pl * (size_t)kk] = (v == 1) ? R(0) : c;
+      }
+    } else {
+      R* o = v == 0 ? a.DENS_RK : (v == 1 ? a.MOMZ_RK : a.RHOT_RK);
+      o[q] = val;
+    }
+  };
+  auto put_zflux = [&](size_t q, R mv, R tv) {
+    if constexpr (CMB) {
+      a.mflx_av[q] = rk4f(a.cm[0][q], a.cm[1][q], a.cm[2][q], mv);
+      a.tflx_av[q] = rk4f(a.ct[0][q], a.ct[1][q], a.ct[2][q], tv);
+    } else {
+      mfZ[q] = mv; tfZ[q] = tv;
+    }
+  };
   double Cn = C_last;
   double ptn = a.PT[px(KE - 1)];
   {
     const size_t q = px(KE);
+    const R d0 = a.DENS0[q], r0 = a.RHOT0[q];
     double advcv = Cn * J33G * RCDZ[KE];
-    a.DENS_RK[q] = R((double)a.DENS0[q] + dtrk * (advcv + (double)a.DENS_RK[q]));
+    const R dn = R((double)d0 + dtrk * (advcv + (double)a.DENS_RK[q]));
     advcv = Cn * ptn * J33G * RCDZ[KE];
-    a.RHOT_RK[q] = R((double)a.RHOT0[q] + dtrk * (advcv + (double)a.RHOT_RK[q]));
-    a.MOMZ_RK[q] = R(0);
+    const R rn = R((double)r0 + dtrk * (advcv + (double)a.RHOT_RK[q]));
+    put_state(0, q, dn, d0, KE);
+    put_state(4, q, rn, r0, KE);
+    if constexpr (CMB) {
+      a.st[1][q] = R(0);  // MOMZ(KE) keeps the last stage's value, 0
+      for (int kk = KE + 1; kk < d.KA; ++kk) a.st[1][p0 + d.pl * (size_t)kk] = R(0);
+      put_zflux(q, mfZ[q], tfZ[q]);
+    } else {
+      a.MOMZ_RK[q] = R(0);
+    }
   }
   {
     const size_t q = px(KE - 1);
-    mfZ[q] = R((double)mfZ[q] + J33G * Cn);
-    tfZ[q] = R((double)tfZ[q] + J33G * Cn * ptn);
-    a.MOMZ_RK[q] = R((double)a.MOMZ0[q] + (Cn - (double)a.MOMZ[q]));
+    const R z0 = a.MOMZ0[q];
+    put_zflux(q, R((double)mfZ[q] + J33G * Cn), R((double)tfZ[q] + J33G * Cn * ptn));
+    put_state(1, q, R((double)z0 + (Cn - (double)a.MOMZ[q])), z0, KE - 1);
   }
   for (int k = KE - 2; k >= KS; --k) {
     const size_t q = px(k), q1 = px(k + 1);
     const double Ck = a.e[q] * Cn + a.f[q];
     const double ptk = a.PT[q];
-    mfZ[q] = R((double)mfZ[q] + J33G * Ck);
-    tfZ[q] = R((double)tfZ[q] + J33G * Ck * ptk);
-    a.MOMZ_RK[q] = R((double)a.MOMZ0[q] + (Ck - (double)a.MOMZ[q]));
+    const R z0 = a.MOMZ0[q];
+    put_zflux(q, R((double)mfZ[q] + J33G * Ck), R((double)tfZ[q] + J33G * Ck * ptk));
+    put_state(1, q, R((double)z0 + (Ck - (double)a.MOMZ[q])), z0, k);
+    const R d0 = a.DENS0[q1], r0 = a.RHOT0[q1];
     double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1];
-    a.DENS_RK[q1] = R((double)a.DENS0[q1] + dtrk * (advcv + (double)a.DENS_RK[q1]));
+    put_state(0, q1, R((double)d0 + dtrk * (advcv + (double)a.DENS_RK[q1])), d0, k + 1);
     advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1];
-    a.RHOT_RK[q1] = R((double)a.RHOT0[q1] + dtrk * (advcv + (double)a.RHOT_RK[q1]));
+    put_state(4, q1, R((double)r0 + dtrk * (advcv + (double)a.RHOT_RK[q1])), r0, k + 1);
     Cn = Ck; ptn = ptk;
   }
   {
     const size_t q = px(KS);
+    const R d0 = a.DENS0[q], r0 = a.RHOT0[q];
     double advcv = -Cn * J33G * RCDZ[KS];
-    a.DENS_RK[q] = R((double)a.DENS0[q] + dtrk * (advcv + (double)a.DENS_RK[q]));
+    put_state(0, q, R((double)d0 + dtrk * (advcv + (double)a.DENS_RK[q])), d0, KS);
     advcv = -Cn * ptn * J33G * RCDZ[KS];
-    a.RHOT_RK[q] = R((double)a.RHOT0[q] + dtrk * (advcv + (double)a.RHOT_RK[q]));
-    a.MOMZ_RK[px(KS - 1)] = R(0);
+    put_state(4, q, R((double)r0 + dtrk * (advcv + (double)a.RHOT_RK[q])), r0, KS);
+    if constexpr (!CMB) a.MOMZ_RK[px(KS - 1)] = R(0);
   }
 }
 
 // ---------------------------------------------------------------------------
 template <int S, bool TERRAIN, bool DIVDMP>
-static hipError_t launch_t(const HeviArgs& a, hipStream_t s) {
+static hipError_t launch_t(const HeviArgs& a, hipStream_t s, bool cmb) {
   const Dims& d = a.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
   const Ctx& c = *a.ctx;
@@ -1061,8 +1132,10 @@ static hipError_t launch_t(const HeviArgs& a, hipStream_t s) {
     // flat terrain: LDS tiles of 64 x 8 columns
     constexpr int TJ = 8;
     const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
-    hipLaunchKernelGGL((hevi_tile<S, DIVDMP, TJ, double>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+    if (cmb) hipLaunchKernelGGL((hevi_tile<S, DIVDMP, TJ, double, true>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+    else hipLaunchKernelGGL((hevi_tile<S, DIVDMP, TJ, double, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   } else {
+    if (cmb) { prof_stop(c, s); return hipErrorNotSupported; }
     constexpr unsigned TJ = 4;
     const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
     hipLaunchKernelGGL((hevi_march<S, TERRAIN, DIVDMP>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
@@ -1087,6 +1160,9 @@ static void fill_args(HeviArgsT<R>& a, const StageIOT<R>& io) {
   a.DENS_RK = io.o[0]; a.MOMZ_RK = io.o[1]; a.MOMX_RK = io.o[2]; a.MOMY_RK = io.o[3]; a.RHOT_RK = io.o[4];
   a.mflx_hi = io.mflx_hi; a.tflx_hi = io.tflx_hi;
   a.PT = c.scr[3]; a.e = c.scr[4]; a.f = c.scr[5];
+  for (int v = 0; v < 5; ++v) { a.c1[v] = io.c1[v]; a.c2[v] = io.c2[v]; a.c3[v] = io.c3[v]; a.st[v] = io.st[v]; }
+  for (int n = 0; n < 3; ++n) { a.cm[n] = io.cm[n]; a.ct[n] = io.ct[n]; }
+  a.mflx_av = io.mflx_av; a.tflx_av = io.tflx_av;
 }
 
 hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
@@ -1095,12 +1171,13 @@ hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
   const Ctx& c = *io.ctx;
   const bool div = io.divdmp_coef > 0.0;
   const bool ter = c.terrain;
+  const bool cmb = io.combine != 0;
   if (c.scheme == UD3) {
-    if (ter) return div ? launch_t<UD3, true, true>(a, s) : launch_t<UD3, true, false>(a, s);
-    return div ? launch_t<UD3, false, true>(a, s) : launch_t<UD3, false, false>(a, s);
+    if (ter) return div ? launch_t<UD3, true, true>(a, s, cmb) : launch_t<UD3, true, false>(a, s, cmb);
+    return div ? launch_t<UD3, false, true>(a, s, cmb) : launch_t<UD3, false, false>(a, s, cmb);
   }
-  if (ter) return div ? launch_t<CD4, true, true>(a, s) : launch_t<CD4, true, false>(a, s);
-  return div ? launch_t<CD4, false, true>(a, s) : launch_t<CD4, false, false>(a, s);
+  if (ter) return div ? launch_t<CD4, true, true>(a, s, cmb) : launch_t<CD4, true, false>(a, s, cmb);
+  return div ? launch_t<CD4, false, true>(a, s, cmb) : launch_t<CD4, false, false>(a, s, cmb);
 }
 
 hipError_t hevi_stage(const StageIOT<float>& io, hipStream_t s) {
@@ -1113,8 +1190,14 @@ hipError_t hevi_stage(const StageIOT<float>& io, hipStream_t s) {
   constexpr int TJ = 8;
   const unsigned ni = (imax + 63) / 64, nb = ni * ((jmax + TJ - 1) / TJ);
   prof_start(c, "hevi_stage", s);
-  if (c.scheme == UD3) hipLaunchKernelGGL((hevi_tile<UD3, false, TJ, float>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
-  else hipLaunchKernelGGL((hevi_tile<CD4, false, TJ, float>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  const bool cmb = io.combine != 0;
+  if (c.scheme == UD3) {
+    if (cmb) hipLaunchKernelGGL((hevi_tile<UD3, false, TJ, float, true>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+    else hipLaunchKernelGGL((hevi_tile<UD3, false, TJ, float, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  } else {
+    if (cmb) hipLaunchKernelGGL((hevi_tile<CD4, false, TJ, float, true>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+    else hipLaunchKernelGGL((hevi_tile<CD4, false, TJ, float, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  }
   prof_stop(c, s);
   return hipGetLastError();
 }

@@ -57,6 +57,15 @@ struct StageIOT {
   const double *REF_dens, *REF_rhot;
   double divdmp_coef, dtrk;
   int has_coriolis;
+  // last RK4 stage only (flat terrain): combine with the outputs of stages
+  // 1-3 on the fly, writing the new state into st (which may alias s0) and the
+  // averaged fluxes into mflx_av/tflx_av, instead of o / the Z-free parts of
+  // mflx_hi, tflx_hi (tinteg_short_rk4.F90:560-660)
+  int combine = 0;
+  const R *c1[5], *c2[5], *c3[5];
+  const R *cm[3], *ct[3];
+  R* st[5];
+  R *mflx_av, *tflx_av;
 };
 using StageIO = StageIOT<double>;
 
