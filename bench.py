@@ -53,6 +53,21 @@ def theta_check(case, nsteps):
     return {"steps": nsteps, "max_abs_K": float(diff.max()), "rms_K": float(np.sqrt((diff * diff).mean()))}
 
 
+def measured_traffic(size):
+    """HBM bytes per HEVI-stage launch from the committed counter profile
+    (profiles/hevi_stage_hbm_traffic.json: FETCH_SIZE and WRITE_SIZE averaged
+    over the stage launches of one short step, calibrated on the copy probe),
+    or None when it was measured on another workload."""
+    path = os.path.join(ROOT, "profiles", "hevi_stage_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("workload") != size:
+        return None
+    return t.get("bytes_per_launch")
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -161,8 +176,9 @@ def main():
     ms_step = elapsed * 1e3 / a.steps
     st_ms = kern["hevi_stage"]["avg_ms"]
     achieved = STAGE_BYTES_PER_POINT * pts / (st_ms * 1e-3) / 1e9
-    roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=None,
-                kernel="hevi_march (one HEVI stage)", bytes_per_point=STAGE_BYTES_PER_POINT,
+    roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                traffic=measured_traffic(a.size),
+                kernel="hevi_tile (one HEVI stage)", bytes_per_point=STAGE_BYTES_PER_POINT,
                 stage_ms=st_ms, kernels_avg_ms={k: v["avg_ms"] for k, v in kern.items()},
                 copy_probe_GBs=copy_gbs,
                 frac_of_copy=(achieved / copy_gbs if copy_gbs else None))
