@@ -181,6 +181,67 @@ def halo_periodic(g, a):
     return a
 
 
+# ---- adjoint checker (scale_oracle_adj.c) -----------------------------------
+
+STATE = ("DENS", "MOMZ", "MOMX", "MOMY", "RHOT")
+
+
+def adj_halo_periodic(g, ab):
+    lib().orc_adj_halo_periodic(C.byref(grid_struct(g)), ptr(ab))
+    return ab
+
+
+def adj_tstep_short_hevi(case, inp: dict, dtrk: float, out_b: dict, scheme: int | None = None) -> dict:
+    """Adjoint of one flat-terrain HEVI stage.  out_b holds DENS_RK..RHOT_RK adjoints;
+    returns the adjoints of DENS0..RHOT0, DENS..RHOT, num_diff, DPRES0, RT2P, REF_rhot."""
+    g = case.grid
+    z = lambda *extra: np.zeros((g.KA, g.IA, g.JA) + extra, order="F")
+    res = {k + "0": z() for k in STATE}
+    res.update({k: z() for k in STATE})
+    res.update(num_diff=z(5, 3), DPRES0=z(), RT2P=z(), REF_rhot=z())
+    gs, ms = grid_struct(g), metric_struct(case.metric)
+    rc = lib().orc_adj_tstep_short_hevi(
+        C.byref(gs), C.c_int(scheme if scheme is not None else case.scheme), C.byref(ms),
+        *[ptr(inp[k]) for k in ("DENS0", "MOMZ0", "MOMX0", "MOMY0", "RHOT0", "DENS", "MOMZ", "MOMX", "MOMY", "RHOT",
+                                "DENS_t", "MOMZ_t", "MOMX_t", "MOMY_t", "RHOT_t", "DPRES0", "RT2P")],
+        ptr(inp.get("CORIOLI", case.corioli)), ptr(inp["num_diff"]), ptr(inp.get("wdamp_coef", case.wdamp_coef)),
+        ptr(inp["REF_dens"]), ptr(inp["REF_rhot"]), C.c_double(dtrk),
+        *[ptr(out_b[k + "_RK"]) for k in STATE],
+        *[ptr(res[k + "0"]) for k in STATE], *[ptr(res[k]) for k in STATE],
+        ptr(res["num_diff"]), ptr(res["DPRES0"]), ptr(res["RT2P"]), ptr(res["REF_rhot"]))
+    if rc != 0:
+        raise RuntimeError(f"orc_adj_tstep_short_hevi failed: {rc}")
+    return res
+
+
+def adj_numfilter_coef(case, state: dict, dt: float, nd_b: np.ndarray, nd_coef=1.0e-4, nd_sfc_fact=1.0) -> dict:
+    g = case.grid
+    nd_b = np.array(nd_b, order="F", copy=True)
+    res = {k: np.zeros((g.KA, g.IA, g.JA), order="F") for k in STATE}
+    gs, ms = grid_struct(g), metric_struct(case.metric)
+    rc = lib().orc_adj_numfilter_coef(C.byref(gs), C.byref(ms), C.c_double(dt), *[ptr(state[k]) for k in STATE],
+                                      ptr(case.ref["REF_dens"]), ptr(case.ref["REF_pott"]), C.c_double(nd_coef),
+                                      C.c_double(nd_sfc_fact), ptr(nd_b), *[ptr(res[k]) for k in STATE])
+    if rc != 0:
+        raise RuntimeError("orc_adj_numfilter_coef failed")
+    return res
+
+
+def adj_large_step(model: "Model", xb: dict) -> dict:
+    """Adjoint of one Model.step() from the model's current state: xb holds the
+    adjoint of the state after the step (whole arrays); returns the adjoint of
+    the state before it.  The model state is not changed."""
+    g = model.case.grid
+    out = {k: np.array(xb[k], order="F", copy=True) for k in STATE}
+    arr = (_dp * 5)(*[ptr(out[k]) for k in STATE])
+    gs, ms = grid_struct(g), metric_struct(model.case.metric)
+    st = model._st()
+    rc = lib().orc_adj_large_step(C.byref(gs), C.byref(ms), C.byref(model.cfg), C.byref(st), arr)
+    if rc != 0:
+        raise RuntimeError("orc_adj_large_step failed (flat terrain, one short step, divdmp 0 only)")
+    return out
+
+
 class Model:
     """Whole-model integrator (LARGE.F90 on-path subset) over a Case."""
 

@@ -64,6 +64,9 @@ static const double F42 = -1.0 / 12.0;
 #define MAP(mp, i, j, n) (mp)[(size_t)(i) + (size_t)IA * ((size_t)(j) + (size_t)JA * (size_t)(n))]
 
 static int g_threads = 1;
+/* optional record of the HEVI stage's intermediates, used by the adjoint checker */
+static orc_hevi_trace *g_trace = NULL;
+void orc_set_hevi_trace(orc_hevi_trace *t) { g_trace = t; }
 #define OMP_FOR _Pragma("omp parallel for num_threads(g_threads) schedule(static)")
 void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 int orc_get_threads(void) { return g_threads; }
@@ -546,6 +549,7 @@ int orc_tstep_short_hevi(const orc_grid *g, int s,
   for (int j = JJS - 2; j <= JJE + 2; ++j)
     for (int i = IIS - 2; i <= IIE + 2; ++i)
       for (int k = KS; k <= KE; ++k) POTT[I3(k, i, j)] = RHOT[I3(k, i, j)] / DENS[I3(k, i, j)];
+  if (g_trace) memcpy(g_trace->POTT, POTT, n3 * sizeof(double));
 
   /* theta fluxes (HEVI.F90:604-622) */
   fluxZ_XYZ(g, s, tfZ, mfZ, POTT, GS + n3 * I_XYW, num_diff + n3 * (I_RHOT + 5 * ZDIR), IIS, IIE, JJS, JJE);
@@ -562,6 +566,8 @@ int orc_tstep_short_hevi(const orc_grid *g, int s,
                        + (tfY[I3(k, i, j)] - tfY[I3(k, i, j - 1)]) * RCDY[j]);
         St[I3(k, i, j)] = (advcv + advch) * MAPF(i, j, 0, I_XY) * MAPF(i, j, 1, I_XY) / GSQRT(k, i, j, I_XYZ) + RHOT_t[I3(k, i, j)];
       }
+
+  if (g_trace) memcpy(g_trace->St, St, n3 * sizeof(double));
 
   /* hevi_solver (HEVI.F90:688-804) */
 #pragma omp parallel for collapse(2) num_threads(g_threads) schedule(static)
@@ -595,6 +601,15 @@ int orc_tstep_short_hevi(const orc_grid *g, int s,
         C[k - KS] = MOMZ[I3(k, i, j)] + dtrk * (pg + Sw[I3(k, i, j)]);
       }
       orc_solve_direct(g, C, F1v, F2v, F3v);
+      if (g_trace)
+        for (int k = KS; k <= KE - 1; ++k) {
+          const size_t p = I3(k, i, j);
+          g_trace->PT[p] = PT[k];
+          g_trace->F2[p] = F2v[k];
+          g_trace->F1[p] = k <= KE - 2 ? F1v[k] : 0.0;
+          g_trace->F3[p] = k >= KS + 1 ? F3v[k] : 0.0;
+          g_trace->Cp[p] = C[k - KS];
+        }
       /* C(k-KS+1) in Fortran is C[k-KS] here */
       for (int k = KS; k <= KE - 1; ++k) {
         const double ck = C[k - KS];
@@ -880,6 +895,23 @@ static void calc_diff3(const orc_grid *g, const cn_tables *t, double *diff, cons
 #undef CNZ3
 #undef CNX3
 #undef CNY3
+}
+
+void orc_filter_tables(const orc_grid *g, const orc_metric *m, double *cnz3, double *cnx3, double *cny3) {
+  GRID_LOCALS;
+  cn_tables t;
+  filter_setup(g, m, &t);
+  memcpy(cnz3, t.CNZ3, (size_t)3 * KA * 2 * sizeof(double));
+  memcpy(cnx3, t.CNX3, (size_t)3 * IA * 2 * sizeof(double));
+  memcpy(cny3, t.CNY3, (size_t)3 * JA * 2 * sizeof(double));
+  free(t.CNZ3); free(t.CNX3); free(t.CNY3);
+}
+
+void orc_calc_diff3(const orc_grid *g, const orc_metric *m, double *diff, const double *phi, int KO, int IO, int JO) {
+  cn_tables t;
+  filter_setup(g, m, &t);
+  calc_diff3(g, &t, diff, phi, KO, IO, JO);
+  free(t.CNZ3); free(t.CNX3); free(t.CNY3);
 }
 
 static inline double pow4(double x) { double x2 = x * x; return x2 * x2; }

@@ -172,6 +172,58 @@ double orc_mass_sum(const orc_grid *g, const orc_metric *m, const double *a);
  * 1 lowest face (val(KS..KS+2) in q0..q2), 2 highest face (val(KE-2..KE) in qm1..q1) */
 double orc_face_value(int scheme, int kind, double qm1, double q0, double q1, double q2, double vel);
 
+/* ---- adjoint checker (scale_oracle_adj.c) -------------------------------- */
+
+/* Intermediates of one orc_tstep_short_hevi call, recorded when a trace is
+ * set: POTT and St (3-D), and per interior column the face potential
+ * temperature PT(KS..KE-1), the bands F1/F2/F3 and the solved vertical
+ * momentum C (rows KS..KE-1, stored at k). */
+typedef struct {
+  double *POTT, *St, *PT, *F1, *F2, *F3, *Cp;
+} orc_hevi_trace;
+void orc_set_hevi_trace(orc_hevi_trace *t);
+
+/* filter_setup tables CNZ3(3,KA,2), CNX3(3,IA,2), CNY3(3,JA,2) (COMMON.F90:92-319) */
+void orc_filter_tables(const orc_grid *g, const orc_metric *m, double *cnz3, double *cnx3, double *cny3);
+/* calc_diff3 (COMMON.F90:1652-1800) on one field; diff has 3 components */
+void orc_calc_diff3(const orc_grid *g, const orc_metric *m, double *diff, const double *phi, int KO, int IO, int JO);
+
+/* Adjoint of orc_tstep_short_hevi for flat terrain, UD3/CD4 and
+ * divdmp_coef = 0.  Given the stage's inputs and the adjoints of its five
+ * prognostic outputs (read only, at the points the stage writes), adds the
+ * adjoints of its inputs into the *_b arrays (state0, state, num_diff(15),
+ * DPRES0, RT2P, REF_rhot).  Returns 0, or -1 for unsupported options. */
+int orc_adj_tstep_short_hevi(const orc_grid *g, int scheme, const orc_metric *m,
+    const double *DENS0, const double *MOMZ0, const double *MOMX0, const double *MOMY0, const double *RHOT0,
+    const double *DENS, const double *MOMZ, const double *MOMX, const double *MOMY, const double *RHOT,
+    const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
+    const double *DPRES0, const double *RT2P, const double *CORIOLI,
+    const double *num_diff, const double *wdamp_coef,
+    const double *REF_dens, const double *REF_rhot, double dtrk,
+    const double *DENS_RK_b, const double *MOMZ_RK_b, const double *MOMX_RK_b, const double *MOMY_RK_b, const double *RHOT_RK_b,
+    double *DENS0_b, double *MOMZ0_b, double *MOMX0_b, double *MOMY0_b, double *RHOT0_b,
+    double *DENS_b, double *MOMZ_b, double *MOMX_b, double *MOMY_b, double *RHOT_b,
+    double *num_diff_b, double *DPRES0_b, double *RT2P_b, double *REF_rhot_b);
+
+/* Adjoint of orc_numfilter_coef (ND_ORDER = 1, reference state, with its
+ * halo fill): given num_diff_b (15 fields, modified: consumed), adds the
+ * adjoints of DENS..RHOT. */
+int orc_adj_numfilter_coef(const orc_grid *g, const orc_metric *m, double dt,
+    const double *DENS, const double *MOMZ, const double *MOMX, const double *MOMY, const double *RHOT,
+    const double *REF_dens, const double *REF_pott, double nd_coef, double nd_sfc_fact,
+    double *num_diff_b, double *DENS_b, double *MOMZ_b, double *MOMX_b, double *MOMY_b, double *RHOT_b);
+
+/* Adjoint of one orc_large_step without tracers (one short step per large
+ * step, flat terrain, divdmp_coef = 0).  st holds the state at the start of
+ * the step; xb[0..4] (DENS..RHOT, whole arrays) hold the adjoint of the
+ * state after the step on entry and the adjoint of the state before it on
+ * return. */
+int orc_adj_large_step(const orc_grid *g, const orc_metric *m, const orc_config *c, const orc_state *st,
+    double *const xb[5]);
+
+/* adjoint of orc_halo_periodic: halo adjoints are added to their sources and cleared */
+void orc_adj_halo_periodic(const orc_grid *g, double *ab);
+
 /* threads used by the OpenMP loops (1 = serial) */
 void orc_set_threads(int n);
 int orc_get_threads(void);

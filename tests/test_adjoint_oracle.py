@@ -1,0 +1,202 @@
+"""CPU checks of the oracle's adjoint (oracle/scale_oracle_adj.c), the checker
+of the HIP adjoint.
+
+The reference has no adjoint, so nothing here compares with reference
+vectors: the adjoint is pinned by central finite differences of the oracle's
+own forward functions (which are pinned against the reference elsewhere) and
+by the exact transpose identity of the halo fill.
+
+States are pushed off the points where the forward is not differentiable:
+the UD3 upwind switch sign(vel) makes every flux a kink in its face velocity
+at vel = 0 (and the face potential temperature of the vertical solve a jump
+in MOMZ at MOMZ = 0), and the synthetic initial states have MOMY = MOMZ = 0
+and DENS = REF_dens exactly.  Tolerance: 1e-6 relative for every check.
+Directions are standard normal times each field's largest magnitude, with
+eps = 1e-7 for the large step, 1e-6 for the filter and 1e-5 for the single
+stage (1e-2 for num_diff, in which the stage is piecewise linear), so that
+truncation O(eps^2) and rounding O(1e-16/eps) both stay below it.
+"""
+import numpy as np
+import pytest
+
+from scale_amd import inputs
+
+O = pytest.importorskip("pyoracle")
+
+STATE = O.STATE
+
+
+def perturbed_case(scheme=3, seed=3):
+    case = inputs.make_case(kmax=10, imax=8, jmax=8, v0=2.0, momz_noise=0.05, noise=0.1, ntracer=0, scheme=scheme)
+    g = case.grid
+    rng = np.random.default_rng(seed)
+    for k, a in case.state.items():
+        a[g.KS:g.KE + 1] *= 1.0 + 1e-3 * rng.standard_normal(a[g.KS:g.KE + 1].shape)
+        fill(g, a, k)
+    return case
+
+
+def fill(g, a, k):
+    """ghost levels and halo as the model fills them (LARGE.F90:908-927, COMM_vars8)"""
+    if k == "MOMZ":
+        a[:g.KS] = 0.0
+        a[g.KE:] = 0.0
+    else:
+        a[:g.KS] = a[g.KS]
+        a[g.KE + 1:] = a[g.KE]
+    return O.halo_periodic(g, a)
+
+
+def stage_inputs(case, seed=5):
+    g = case.grid
+    rng = np.random.default_rng(seed)
+    s0 = case.state
+    s = {}
+    for k, v in s0.items():
+        a = np.array(v, order="F", copy=True)
+        a[g.KS:g.KE + 1] *= 1.0 + 1e-3 * rng.standard_normal(a[g.KS:g.KE + 1].shape)
+        s[k] = fill(g, a, k)
+    P = inputs.large_step_prep(case)
+    inp = {k + "0": s0[k] for k in STATE}
+    inp.update(s)
+    inp.update({k + "_t": g.zeros() for k in STATE})
+    inp.update(DPRES0=P["DPRES0"], RT2P=P["RT2P"], REF_rhot=P["REF_rhot"], REF_dens=case.ref["REF_dens"],
+               num_diff=O.numfilter_coef(case, s0, case.dt), DDIV=g.zeros())
+    inp["wdamp_coef"] = np.linspace(0.0, 0.1, g.KA)
+    inp["CORIOLI"] = np.full((1, g.IA, g.JA), 1.0e-4, order="F")
+    return inp
+
+
+def written(g, k):
+    ke = g.KE - 1 if k == "MOMZ" else g.KE
+    return (slice(g.KS, ke + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+
+
+@pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
+def test_stage_adjoint_matches_finite_differences(scheme):
+    case = perturbed_case(scheme)
+    g = case.grid
+    inp = stage_inputs(case)
+    rng = np.random.default_rng(7)
+    y = {k: np.asfortranarray(rng.standard_normal(g.shape)) for k in STATE}
+    dt = 0.5
+
+    def L(x):
+        out = O.tstep_short_hevi(case, x, dt)
+        return sum(float(np.sum(out[k + "_RK"][written(g, k)] * y[k][written(g, k)])) for k in STATE)
+
+    res = O.adj_tstep_short_hevi(case, inp, dt, {k + "_RK": y[k] for k in STATE})
+    for key in [k + "0" for k in STATE] + list(STATE) + ["num_diff", "DPRES0", "RT2P", "REF_rhot"]:
+        base = inp[key]
+        if key == "num_diff":
+            # The stage is linear in num_diff except through the sign of the
+            # vertical mass flux, which is num_diff(DENS, z) alone before the
+            # solve: that field is perturbed relatively (no sign change), so a
+            # large eps is exact up to rounding.
+            d = rng.standard_normal(base.shape) * np.abs(base).max(axis=(0, 1, 2))
+            d[..., 0, 0] = rng.standard_normal(base.shape[:3]) * np.abs(base[..., 0, 0])
+            d = np.asfortranarray(d)
+            eps = 1e-2
+        else:
+            d = np.asfortranarray(rng.standard_normal(base.shape)) * np.abs(base).max()
+            eps = 1e-5
+        xp, xm = dict(inp), dict(inp)
+        xp[key] = np.asfortranarray(base + eps * d)
+        xm[key] = np.asfortranarray(base - eps * d)
+        fd = (L(xp) - L(xm)) / (2 * eps)
+        ad = float(np.sum(res[key] * d))
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (key, fd, ad)
+
+
+def test_numfilter_adjoint_matches_finite_differences():
+    case = perturbed_case()
+    g = case.grid
+    st = case.state
+    rng = np.random.default_rng(11)
+    y = np.asfortranarray(rng.standard_normal((g.KA, g.IA, g.JA, 5, 3)))
+    res = O.adj_numfilter_coef(case, st, case.dt, y)
+    for key in STATE:
+        base = st[key]
+        d = np.asfortranarray(rng.standard_normal(base.shape)) * np.abs(base).max()
+        eps = 1e-6
+        sp, sm = dict(st), dict(st)
+        sp[key] = np.asfortranarray(base + eps * d)
+        sm[key] = np.asfortranarray(base - eps * d)
+        fd = float(np.sum((O.numfilter_coef(case, sp, case.dt) - O.numfilter_coef(case, sm, case.dt)) * y)) / (2 * eps)
+        ad = float(np.sum(res[key] * d))
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (key, fd, ad)
+
+
+def test_halo_adjoint_is_the_transpose():
+    case = perturbed_case()
+    g = case.grid
+    rng = np.random.default_rng(2)
+    x = np.asfortranarray(rng.standard_normal(g.shape))
+    y = np.asfortranarray(rng.standard_normal(g.shape))
+    hx = O.halo_periodic(g, x.copy(order="F"))
+    hty = O.adj_halo_periodic(g, y.copy(order="F"))
+    assert abs(float(np.sum(hx * y)) - float(np.sum(x * hty))) <= 1e-12 * abs(float(np.sum(hx * y)))
+
+
+def theta_box_objective(case, state):
+    """J = sum over a central box of (RHOT/DENS - REF_pott)^2 and its gradient."""
+    g = case.grid
+    kc, ic, jc = (g.KS + g.KE) // 2, (g.IS + g.IE) // 2, (g.JS + g.JE) // 2
+    box = (slice(kc - 2, kc + 2), slice(ic - 2, ic + 2), slice(jc - 2, jc + 2))
+    th = state["RHOT"] / state["DENS"] - case.ref["REF_pott"]
+    xb = {k: np.zeros(g.shape, order="F") for k in STATE}
+    xb["RHOT"][box] = 2.0 * th[box] / state["DENS"][box]
+    xb["DENS"][box] = -2.0 * th[box] * state["RHOT"][box] / state["DENS"][box] ** 2
+    return float(np.sum(th[box] ** 2)), xb
+
+
+def interior_gradient(g, xb):
+    """adjoint of 'perturb the interior, then fill ghost levels and halo'"""
+    grad = {}
+    for k in STATE:
+        b = O.adj_halo_periodic(g, np.array(xb[k], order="F", copy=True))
+        if k != "MOMZ":
+            b[g.KS] += b[:g.KS].sum(axis=0)
+            b[g.KE] += b[g.KE + 1:].sum(axis=0)
+        b[:g.KS] = 0.0
+        b[g.KE + 1:] = 0.0
+        if k == "MOMZ":
+            b[g.KE] = 0.0
+        grad[k] = b
+    return grad
+
+
+def test_large_step_adjoint_matches_finite_differences():
+    case = perturbed_case()
+    g = case.grid
+    N = 3
+
+    def run(state, n):
+        m = O.Model(case)
+        m.state = {k: np.array(v, order="F", copy=True) for k, v in state.items()}
+        m.step(n)
+        return m
+
+    traj = [{k: np.array(v, order="F", copy=True) for k, v in case.state.items()}]
+    for _ in range(N):
+        traj.append(run(traj[-1], 1).state)
+    _, xb = theta_box_objective(case, traj[N])
+    for n in reversed(range(N)):
+        m = O.Model(case)
+        m.state = traj[n]
+        xb = O.adj_large_step(m, xb)
+    grad = interior_gradient(g, xb)
+    rng = np.random.default_rng(13)
+    for _ in range(3):
+        d = {}
+        for k in STATE:
+            a = np.zeros(g.shape, order="F")
+            sl = written(g, k)
+            a[sl] = rng.standard_normal(a[sl].shape) * np.abs(case.state[k]).max()
+            d[k] = fill(g, a, k)
+        ad = sum(float(np.sum(grad[k] * d[k])) for k in STATE)
+        eps = 1e-7
+        Jp = theta_box_objective(case, run({k: np.asfortranarray(traj[0][k] + eps * d[k]) for k in STATE}, N).state)[0]
+        Jm = theta_box_objective(case, run({k: np.asfortranarray(traj[0][k] - eps * d[k]) for k in STATE}, N).state)[0]
+        fd = (Jp - Jm) / (2 * eps)
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
