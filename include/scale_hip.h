@@ -176,9 +176,45 @@ int scale_hip_stream(scale_hip_ctx *ctx, void **stream);
  * stream (the counterpart of PROF_rapstart/PROF_rapend,
  * scalelib/src/common/scale_prof.F90:156-253).  Names: hevi_stage,
  * numfilter, halo_fill, rk4_combine, tracer_nofct, fct_philo, fct_ratio,
- * fct_update, hbm_copy.  Enabling resets the accumulators. */
+ * fct_update, hbm_copy, model_forward_checkpointed, model_adjoint.  Enabling
+ * resets the accumulators. */
 int scale_hip_profile_enable(scale_hip_ctx *ctx, int32_t on);
 int scale_hip_profile_query(scale_hip_ctx *ctx, const char *name, int64_t *count, double *total_ms);
+
+/* ---- adjoint --------------------------------------------------------------
+ * No counterpart in the reference.  The transpose of the Jacobian of the
+ * forward above (with the UD3 upwind switch sign(vel) held fixed), for flat
+ * terrain, FP64, divdmp_coef = 0 and one short step per large step; other
+ * configurations are rejected.  Adjoint arrays have the shapes of the fields
+ * they belong to (Fortran layout, host memory).
+ *
+ * Adjoint of one scale_hip_tstep_short_hevi call: given the stage inputs
+ * (the same arrays as the forward call) and the adjoints *_RK_b of its five
+ * prognostic outputs, returns the adjoints of its inputs (overwritten):
+ * state0, state, num_diff(KA,IA,JA,5,3), DPRES0, RT2P and REF_rhot. */
+int scale_hip_tstep_short_hevi_adj(scale_hip_ctx *ctx,
+    const double *DENS_RK_b, const double *MOMZ_RK_b, const double *MOMX_RK_b, const double *MOMY_RK_b, const double *RHOT_RK_b,
+    double *DENS0_b, double *MOMZ0_b, double *MOMX0_b, double *MOMY0_b, double *RHOT0_b,
+    double *DENS_b, double *MOMZ_b, double *MOMX_b, double *MOMY_b, double *RHOT_b,
+    double *num_diff_b, double *DPRES0_b, double *RT2P_b, double *REF_rhot_b,
+    const double *DENS0, const double *MOMZ0, const double *MOMX0, const double *MOMY0, const double *RHOT0,
+    const double *DENS, const double *MOMZ, const double *MOMX, const double *MOMY, const double *RHOT,
+    const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
+    const double *DPRES0, const double *RT2P, const double *CORIOLI,
+    const double *num_diff, const double *wdamp_coef,
+    const double *CDZ, const double *FDZ, const double *FDX, const double *FDY,
+    const double *RCDZ, const double *RCDX, const double *RCDY, const double *RFDZ, const double *RFDX, const double *RFDY,
+    const double *PHI, const double *GSQRT, const double *J13G, const double *J23G, const double *J33G, const double *MAPF,
+    const double *REF_dens, const double *REF_rhot, const double *dtrk);
+
+/* Forward/adjoint pair of the device-resident model.  The forward runs n
+ * large steps (as scale_hip_model_large_step) and keeps the state every
+ * `every` steps in HBM.  The adjoint takes the adjoint of the five
+ * prognostics after that run (DENS_b .. RHOT_b, whole arrays) and returns in
+ * place their adjoint at its start, recomputing each segment of `every` steps
+ * from its checkpoint.  The model state is left at the end of the run. */
+int scale_hip_model_forward_checkpointed(scale_hip_ctx *ctx, int32_t n, int32_t every);
+int scale_hip_model_adjoint(scale_hip_ctx *ctx, double *DENS_b, double *MOMZ_b, double *MOMX_b, double *MOMY_b, double *RHOT_b);
 
 /* HBM probe with no counterpart in the reference: `reps` timed streaming
  * copies of n doubles between two scratch buffers (allocated and freed by the

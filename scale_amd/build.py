@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libscalehip.so")
-SOURCES = ["hevi.hip", "numfilter.hip", "tracer.hip", "driver.hip", "profile.hip", "capi.hip"]
+SOURCES = ["hevi.hip", "numfilter.hip", "tracer.hip", "driver.hip", "adjoint.hip", "profile.hip", "capi.hip"]
 ARCH = os.environ.get("SCALE_HIP_ARCH", "gfx950")
 
 FLAGS = [

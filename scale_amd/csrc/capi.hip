@@ -320,6 +320,7 @@ int scale_hip_finalize(scale_hip_ctx* h) {
   F(c.CNZ3); F(c.CNX3); F(c.CNY3); F(c.ndc); F(c.stage);
   for (int q = 0; q < shp::NSCR; ++q) F(c.scr[q]);
   for (double* b : h->dbuf) F(b);
+  shp::adj_free(c);
   if (c.model) {
     for (void* q : c.model->allocs) F(q);
     delete c.model;
@@ -402,6 +403,79 @@ int scale_hip_tstep_short_hevi(scale_hip_ctx* h,
   if (int r = download3(c, mflx_hi, B[B_MF], 3)) return r;
   if (int r = download3(c, tflx_hi, B[B_TF], 3)) return r;
   return 0;
+}
+
+int scale_hip_tstep_short_hevi_adj(scale_hip_ctx* h,
+    const double* DENS_RK_b, const double* MOMZ_RK_b, const double* MOMX_RK_b, const double* MOMY_RK_b, const double* RHOT_RK_b,
+    double* DENS0_b, double* MOMZ0_b, double* MOMX0_b, double* MOMY0_b, double* RHOT0_b,
+    double* DENS_b, double* MOMZ_b, double* MOMX_b, double* MOMY_b, double* RHOT_b,
+    double* num_diff_b, double* DPRES0_b, double* RT2P_b, double* REF_rhot_b,
+    const double* DENS0, const double* MOMZ0, const double* MOMX0, const double* MOMY0, const double* RHOT0,
+    const double* DENS, const double* MOMZ, const double* MOMX, const double* MOMY, const double* RHOT,
+    const double* DENS_t, const double* MOMZ_t, const double* MOMX_t, const double* MOMY_t, const double* RHOT_t,
+    const double* DPRES0, const double* RT2P, const double* CORIOLI,
+    const double* num_diff, const double* wdamp_coef,
+    const double* CDZ, const double* FDZ, const double* FDX, const double* FDY,
+    const double* RCDZ, const double* RCDX, const double* RCDY, const double* RFDZ, const double* RFDX, const double* RFDY,
+    const double* PHI, const double* GSQRT, const double* J13G, const double* J23G, const double* J33G, const double* MAPF,
+    const double* REF_dens, const double* REF_rhot, const double* dtrk) {
+  if (int r = check_ctx(h)) return r;
+  if (!dtrk || !J33G) return fail("null scalar argument");
+  Ctx& c = h->c;
+  if (c.scheme != shp::UD3 && c.scheme != shp::CD4) return fail("the adjoint supports the UD3 and CD4 dynamics schemes");
+  if (int r = ensure_dropin(h)) return r;
+  if (int r = set_grid1d(c, CDZ, FDZ, RCDZ, RFDZ, nullptr, FDX, RCDX, RFDX, nullptr, FDY, RCDY, RFDY)) return r;
+  if (int r = set_metric3d(c, PHI, GSQRT, J13G, J23G, *J33G, MAPF)) return r;
+  if (c.terrain) return fail("the adjoint supports flat terrain only (GSQRT = 1, J13G = J23G = 0, MAPF = 1)");
+  CK(shp::adj_setup(c, 0, 0));
+  auto& B = h->dbuf;
+  const double* s0[5] = {DENS0, MOMZ0, MOMX0, MOMY0, RHOT0};
+  const double* s[5] = {DENS, MOMZ, MOMX, MOMY, RHOT};
+  const double* t[5] = {DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t};
+  const double* ob[5] = {DENS_RK_b, MOMZ_RK_b, MOMX_RK_b, MOMY_RK_b, RHOT_RK_b};
+  double* s0b[5] = {DENS0_b, MOMZ0_b, MOMX0_b, MOMY0_b, RHOT0_b};
+  double* sb[5] = {DENS_b, MOMZ_b, MOMX_b, MOMY_b, RHOT_b};
+  for (int v = 0; v < 5; ++v) {
+    if (int r = upload3(c, B[B_S0 + v], s0[v], 1)) return r;
+    if (int r = upload3(c, B[B_S + v], s[v], 1)) return r;
+    if (int r = upload3(c, B[B_T + v], t[v], 1)) return r;
+    if (int r = upload3(c, B[B_OUT + v], ob[v], 1)) return r;
+  }
+  if (int r = upload3(c, B[B_DPRES0], DPRES0, 1)) return r;
+  if (int r = upload3(c, B[B_RT2P], RT2P, 1)) return r;
+  if (int r = upload3(c, B[B_REFD], REF_dens, 1)) return r;
+  if (int r = upload3(c, B[B_REFR], REF_rhot, 1)) return r;
+  if (int r = upload3(c, B[B_ND], num_diff, 15)) return r;
+  if (int r = upload_raw(c, B[B_COR], CORIOLI, c.d.pl)) return r;
+  if (int r = upload_raw(c, B[B_WD], wdamp_coef, (size_t)c.d.KA)) return r;
+  double *ds0b[5], *dsb[5], *dNDb, *dPb[3];
+  shp::adj_stage_bufs(c, ds0b, dsb, &dNDb, dPb);
+  const size_t bytes = c.d.n3 * sizeof(double);
+  for (int v = 0; v < 5; ++v) {
+    CK(hipMemsetAsync(ds0b[v], 0, bytes, c.stream));
+    CK(hipMemsetAsync(dsb[v], 0, bytes, c.stream));
+  }
+  CK(hipMemsetAsync(dNDb, 0, 15 * bytes, c.stream));
+  for (int q = 0; q < 3; ++q) CK(hipMemsetAsync(dPb[q], 0, bytes, c.stream));
+  shp::StageAdjIO io{};
+  io.ctx = &c;
+  for (int v = 0; v < 5; ++v) {
+    io.s0[v] = B[B_S0 + v]; io.s[v] = B[B_S + v]; io.t[v] = B[B_T + v]; io.ob[v] = B[B_OUT + v];
+    io.s0b[v] = ds0b[v]; io.sb[v] = dsb[v];
+  }
+  io.DPRES0 = B[B_DPRES0]; io.RT2P = B[B_RT2P]; io.CORIOLI = B[B_COR]; io.num_diff = B[B_ND]; io.wdamp_coef = B[B_WD];
+  io.REF_dens = B[B_REFD]; io.REF_rhot = B[B_REFR];
+  io.dtrk = *dtrk; io.has_coriolis = !all_equal(CORIOLI, c.d.pl, 0.0);
+  io.NDb = dNDb; io.DPRES0b = dPb[0]; io.RT2Pb = dPb[1]; io.REFrtb = dPb[2];
+  CK(shp::hevi_stage_adj(c, io, c.stream));
+  for (int v = 0; v < 5; ++v) {
+    if (int r = download3(c, s0b[v], ds0b[v], 1)) return r;
+    if (int r = download3(c, sb[v], dsb[v], 1)) return r;
+  }
+  if (int r = download3(c, num_diff_b, dNDb, 15)) return r;
+  if (int r = download3(c, DPRES0_b, dPb[0], 1)) return r;
+  if (int r = download3(c, RT2P_b, dPb[1], 1)) return r;
+  return download3(c, REF_rhot_b, dPb[2], 1);
 }
 
 int scale_hip_tstep_tracer_heve(scale_hip_ctx* h,
@@ -669,6 +743,40 @@ int scale_hip_model_mass(scale_hip_ctx* h, int32_t f, int32_t weight_dens, doubl
   else { int nc; src = model_field(M, f, &nc); }
   if (!src) return fail("unknown field id");
   CK(shp::mass_sum(h->c, src, weight_dens ? M->DENS : nullptr, out));
+  return 0;
+}
+
+int scale_hip_model_forward_checkpointed(scale_hip_ctx* h, int32_t n, int32_t every) {
+  if (int r = check_ctx(h)) return r;
+  Ctx& c = h->c;
+  if (!c.model) return fail("model not created");
+  if (n < 1 || every < 1) return fail("n and every must be positive");
+  hipError_t e = shp::model_forward_checkpointed(c, n, every);
+  if (e == hipErrorNotSupported)
+    return fail("the adjoint supports flat terrain, FP64, divdmp_coef = 0 and one short step per large step");
+  CK(e);
+  CK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int scale_hip_model_adjoint(scale_hip_ctx* h, double* DENS_b, double* MOMZ_b, double* MOMX_b, double* MOMY_b, double* RHOT_b) {
+  if (int r = check_ctx(h)) return r;
+  Ctx& c = h->c;
+  if (!c.model) return fail("model not created");
+  double* hb[5] = {DENS_b, MOMZ_b, MOMX_b, MOMY_b, RHOT_b};
+  CK(shp::adj_setup(c, 0, 0));
+  double* xb[5];
+  for (int v = 0; v < 5; ++v) {
+    xb[v] = shp::adj_io(c, v);
+    if (int r = upload3(c, xb[v], hb[v], 1)) return r;
+  }
+  hipError_t e = shp::model_adjoint(c, xb);
+  if (e == hipErrorNotReady) return fail("scale_hip_model_forward_checkpointed must run first");
+  if (e == hipErrorNotSupported)
+    return fail("the adjoint supports flat terrain, FP64, divdmp_coef = 0 and one short step per large step");
+  CK(e);
+  for (int v = 0; v < 5; ++v)
+    if (int r = download3(c, hb[v], xb[v], 1)) return r;
   return 0;
 }
 

@@ -48,6 +48,8 @@ struct HeviArgsT {
   R *DENS_RK, *MOMZ_RK, *MOMX_RK, *MOMY_RK, *RHOT_RK, *mflx_hi, *tflx_hi;
   // scratch (n3 each)
   double *PT, *e, *f;
+  // optional records for the adjoint (hevi_march only): St and the solved C
+  double *trSt = nullptr, *trCp = nullptr;
   // RK4 combination fused into the last stage (hevi_tile with CMB)
   const R *c1[5], *c2[5], *c3[5];
   const R *cm[3], *ct[3];
@@ -571,6 +573,7 @@ __global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles,
     store_point(a, k, i, j, t);
     Sr_[q1] = t.Sr;
     St_[q1] = t.St;
+    if (a.trSt) a.trSt[q1] = t.St;
     if (k <= KE - 1) a.PT[q1] = t.PT;
     if (k > KS && k - 1 < KE - 1) { a.e[px(k - 1)] = ek; a.f[px(k - 1)] = fk; }
     rt0 = rt1; dp0 = dp1; st0 = t.St; sr0 = t.Sr; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = t.Sw;
@@ -594,6 +597,7 @@ __global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles,
     mfZ[q] = mfZ[q] + J33G * Cn / (M12);
     tfZ[q] = tfZ[q] + J33G * Cn * ptn / (M12);
     a.MOMZ_RK[q] = a.MOMZ0[q] + (Cn - a.MOMZ[q]);
+    if (a.trCp) a.trCp[q] = Cn;
   }
   for (int k = KE - 2; k >= KS; --k) {
     const size_t q = px(k), q1 = px(k + 1);
@@ -602,6 +606,7 @@ __global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles,
     mfZ[q] = mfZ[q] + J33G * Ck / (M12);
     tfZ[q] = tfZ[q] + J33G * Ck * ptk / (M12);
     a.MOMZ_RK[q] = a.MOMZ0[q] + (Ck - a.MOMZ[q]);
+    if (a.trCp) a.trCp[q] = Ck;
     // level k+1 (KS+1..KE-1) now has both C(k+1) and C(k)
     double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
     a.DENS_RK[q1] = a.DENS0[q1] + dtrk * (advcv + Sr_[q1]);
@@ -1178,6 +1183,23 @@ hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
   }
   if (ter) return div ? launch_t<CD4, true, true>(a, s, cmb) : launch_t<CD4, true, false>(a, s, cmb);
   return div ? launch_t<CD4, false, true>(a, s, cmb) : launch_t<CD4, false, false>(a, s, cmb);
+}
+
+// flat-terrain FP64 stage through hevi_march, recording PT, St and the
+// solved vertical momentum C for the adjoint
+hipError_t hevi_stage_traced(const StageIO& io, double* PT, double* St, double* Cp, hipStream_t s) {
+  HeviArgs a;
+  fill_args(a, io);
+  const Ctx& c = *io.ctx;
+  if (c.terrain || io.divdmp_coef > 0.0 || io.combine) return hipErrorNotSupported;
+  a.PT = PT; a.trSt = St; a.trCp = Cp;
+  const Dims& d = c.d;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
+  constexpr unsigned TJ = 4;
+  const unsigned ni = (imax + 63) / 64, nb = ni * ((jmax + TJ - 1) / TJ);
+  if (c.scheme == UD3) hipLaunchKernelGGL((hevi_march<UD3, false, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  else hipLaunchKernelGGL((hevi_march<CD4, false, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  return hipGetLastError();
 }
 
 hipError_t hevi_stage(const StageIOT<float>& io, hipStream_t s) {

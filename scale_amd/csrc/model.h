@@ -48,5 +48,8 @@ hipError_t model_large_step(Ctx& c);
 hipError_t mass_sum(const Ctx& c, const double* a, const double* b, double* out_host);
 hipError_t model_state_to_f64(Ctx& c);
 hipError_t model_state_to_f32(Ctx& c);
+// adjoint (adjoint.hip): flat terrain, FP64, divdmp_coef = 0, one short step per large step
+hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every);
+hipError_t model_adjoint(Ctx& c, double* const xb[5]);
 
 }  // namespace shp

@@ -9,7 +9,8 @@ namespace shp {
 
 constexpr int NSCR = 8;  // scratch fields of KA*IA*JA doubles owned by the context
 
-struct Model;  // device-resident model state (driver.hip)
+struct Model;    // device-resident model state (driver.hip)
+struct AdjWork;  // adjoint work arrays and checkpoints (adjoint.hip)
 struct Prof;   // named kernel timers (profile.hip)
 
 struct Ctx {
@@ -34,6 +35,7 @@ struct Ctx {
   size_t stage_elems = 0;
   // device-resident model
   Model* model = nullptr;
+  AdjWork* adj = nullptr;
   // named hipEvent timers, null when profiling is off
   Prof* prof = nullptr;
   // host copies of 1-D grid arrays (for the numerical-filter tables)
@@ -79,6 +81,9 @@ struct TracerIO {
 hipError_t hevi_stage(const StageIO& io, hipStream_t s);
 // mixed precision: flat terrain and divdmp_coef = 0 only
 hipError_t hevi_stage(const StageIOT<float>& io, hipStream_t s);
+// flat terrain, FP64, no divergence damping: the stage with PT, St and the
+// solved C recorded (n3 each) for the adjoint
+hipError_t hevi_stage_traced(const StageIO& io, double* PT, double* St, double* Cp, hipStream_t s);
 // named timers around launches (no-ops unless c.prof is set): the counterpart
 // of the reference's PROF_rapstart / PROF_rapend (scale_prof.F90:156-253)
 void prof_start(const Ctx& c, const char* name, hipStream_t s);
@@ -96,6 +101,29 @@ hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], 
 hipError_t numfilter(const Ctx& c, float* num_diff, const float* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s);
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s);
+
+// Adjoint of one flat-terrain FP64 stage (adjoint.hip): given the stage's
+// inputs and the adjoints ob of its five prognostic outputs (read at the
+// points the stage writes), adds the adjoints of its inputs.
+struct StageAdjIO {
+  const Ctx* ctx;
+  const double *s0[5], *s[5], *t[5];
+  const double *DPRES0, *RT2P, *CORIOLI, *num_diff, *wdamp_coef, *REF_dens, *REF_rhot;
+  double dtrk;
+  int has_coriolis;
+  const double* ob[5];
+  double *s0b[5], *sb[5], *NDb, *DPRES0b, *RT2Pb, *REFrtb;
+};
+hipError_t hevi_stage_adj(Ctx& c, const StageAdjIO& io, hipStream_t s);
+// num_diff adjoint (15 fields, consumed) -> adjoints of DENS..RHOT (added)
+hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const double* REF_dens, const double* REF_pott,
+                         double* const stb[5], hipStream_t s);
+// adjoint work arrays (nsteps > 0: also checkpoints of a run of nsteps saved every 'every')
+hipError_t adj_setup(Ctx& c, int nsteps, int every);
+void adj_free(Ctx& c);
+double* adj_io(Ctx& c, int v);  // five device fields for the adjoint arguments of the C ABI
+// output buffers of a stage adjoint called through the C ABI (after adj_setup)
+void adj_stage_bufs(Ctx& c, double* s0b[5], double* sb[5], double** NDb, double* Pb[3]);
 // ATMOS_DYN_divergence on i = IS..IE+1, j = JS..JE+1, k = KS-1..KE+1
 hipError_t divergence(const Ctx& c, double* DDIV, const double* MOMZ, const double* MOMX, const double* MOMY, hipStream_t s);
 hipError_t halo_fill(const Ctx& c, double* const* arrs, int n, hipStream_t s);

@@ -32,6 +32,7 @@ UD3, CD4 = 3, 4
 FIELD = dict(DENS=0, MOMZ=1, MOMX=2, MOMY=3, RHOT=4, REF_dens=10, REF_pott=11, REF_pres=12, CORIOLI=13,
              wdamp_coef=14, mflx_hi=20, tflx_hi=21, DENS_tp=30, MOMZ_tp=31, MOMX_tp=32, MOMY_tp=33, RHOT_tp=34)
 QTRC_BASE, RHOQ_TP_BASE = 100, 200
+STATE = ("DENS", "MOMZ", "MOMX", "MOMY", "RHOT")
 
 
 class ScaleHipError(RuntimeError):
@@ -165,6 +166,30 @@ class Context:
             _p(FDZ), _p(FDX), _p(FDY), _d(DT), _p(REF_dens), _p(REF_pott), _d(ND_COEF),
             C.byref(C.c_int32(int(ND_ORDER))), _d(ND_SFC_FACT), _l(ND_USE_RS)))
 
+    def ATMOS_DYN_Tstep_short_fvm_hevi_adj(self, out_b: dict, DENS0, MOMZ0, MOMX0, MOMY0, RHOT0,
+                                           DENS, MOMZ, MOMX, MOMY, RHOT, DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t,
+                                           DPRES0, RT2P, CORIOLI, num_diff, wdamp_coef,
+                                           CDZ, FDZ, FDX, FDY, RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY,
+                                           PHI, GSQRT, J13G, J23G, J33G, MAPF, REF_dens, REF_rhot, dtrk) -> dict:
+        """Adjoint of one HEVI stage (flat terrain, FP64).  out_b holds the adjoints
+        DENS_RK..RHOT_RK of the stage outputs; returns the adjoints of the inputs
+        (DENS0..RHOT0, DENS..RHOT, num_diff, DPRES0, RT2P, REF_rhot)."""
+        g = self.grid
+        z = lambda *e: np.zeros((g.KA, g.IA, g.JA) + e, order="F")
+        res = {k + "0": z() for k in STATE}
+        res.update({k: z() for k in STATE})
+        res.update(num_diff=z(5, 3), DPRES0=z(), RT2P=z(), REF_rhot=z())
+        _check(load_library().scale_hip_tstep_short_hevi_adj(
+            self._h, *[_p(np.asfortranarray(out_b[k + "_RK"])) for k in STATE],
+            *[_p(res[k + "0"]) for k in STATE], *[_p(res[k]) for k in STATE],
+            _p(res["num_diff"]), _p(res["DPRES0"]), _p(res["RT2P"]), _p(res["REF_rhot"]),
+            _p(DENS0), _p(MOMZ0), _p(MOMX0), _p(MOMY0), _p(RHOT0), _p(DENS), _p(MOMZ), _p(MOMX), _p(MOMY), _p(RHOT),
+            _p(DENS_t), _p(MOMZ_t), _p(MOMX_t), _p(MOMY_t), _p(RHOT_t), _p(DPRES0), _p(RT2P), _p(CORIOLI),
+            _p(num_diff), _p(wdamp_coef),
+            _p(CDZ), _p(FDZ), _p(FDX), _p(FDY), _p(RCDZ), _p(RCDX), _p(RCDY), _p(RFDZ), _p(RFDX), _p(RFDY),
+            _p(PHI), _p(GSQRT), _p(J13G), _p(J23G), _d(J33G), _p(MAPF), _p(REF_dens), _p(REF_rhot), _d(dtrk)))
+        return res
+
     # ---- device-resident model ----
     def set_metrics(self, m: dict):
         _check(load_library().scale_hip_set_metrics(
@@ -201,6 +226,17 @@ class Context:
 
     def model_short_step(self, n=1):
         _check(load_library().scale_hip_model_short_step(self._h, C.c_int32(n)))
+
+    def model_forward_checkpointed(self, n, every):
+        """n large steps keeping the state every `every` steps for model_adjoint."""
+        _check(load_library().scale_hip_model_forward_checkpointed(self._h, C.c_int32(n), C.c_int32(every)))
+
+    def model_adjoint(self, xb: dict) -> dict:
+        """Adjoint of the last model_forward_checkpointed run: xb holds the adjoints
+        of DENS..RHOT after it (whole arrays); returns their adjoints at its start."""
+        out = {k: np.array(xb[k], dtype=np.float64, order="F", copy=True) for k in STATE}
+        _check(load_library().scale_hip_model_adjoint(self._h, *[_p(out[k]) for k in STATE]))
+        return out
 
     def model_mass(self, name, weight_dens=False):
         fid = name if isinstance(name, int) else FIELD[name]
