@@ -79,9 +79,107 @@ def parse():
     p.add_argument("--no-large-step", action="store_true", help="skip the informational large step with tracers")
     p.add_argument("--no-copy-probe", action="store_true", help="skip the HBM copy probe")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP32 + FP64 solve) timing")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    p.add_argument("--adjoint-dirs", type=int, default=64, help="finite-difference directions of the adjoint check")
     p.add_argument("--theta-check", type=int, default=0, metavar="N",
                    help="also run N short steps in FP64 and in mixed precision and report the potential-temperature difference")
     return p.parse_args()
+
+
+def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
+    """Adjoint against forward cost and finite differences on the warm bubble
+    at 128x128x64 (DX = 500 m, DZ = 250 m, u = 10 m/s, theta noise 0.01 K,
+    dt = 1 s, 3 tracers with FCT), started from the state after two large
+    steps (the initial MOMY = MOMZ = 0 sit on the kink of the upwind switch).
+    J = sum over the central 16x16x16 box of (theta - REF_pott)^2 after
+    nsteps large steps; its gradient with respect to the interior state comes
+    from the checkpointed forward (every `every` steps) and the adjoint sweep,
+    and is compared with central differences of the forward along ndirs
+    seeded directions d (standard normal per point times each field's largest
+    magnitude, perturbation eps*d)."""
+    import numpy as np
+    from scale_amd import hip, inputs
+    case = inputs.make_case(kmax=64, imax=128, jmax=128, dx=500.0, dy=500.0, dz=250.0, seed=seed, u0=10.0,
+                            noise=0.01, ntracer=3, tracer_kind=["blob", "tophat", "uniform"], dt=1.0, fct=True)
+    g = case.grid
+    K, I, J = g.KS + 32, g.IS + 64, g.JS + 64
+    box = (slice(K - 8, K + 8), slice(I - 8, I + 8), slice(J - 8, J + 8))
+    ref_pott = case.ref["REF_pott"]
+    names = hip.STATE
+
+    def objective(st):
+        th = st["RHOT"] / st["DENS"] - ref_pott
+        xb = {k: np.zeros(g.shape, order="F") for k in names}
+        xb["RHOT"][box] = 2.0 * th[box] / st["DENS"][box]
+        xb["DENS"][box] = -2.0 * th[box] * st["RHOT"][box] / st["DENS"][box] ** 2
+        return float(np.sum(th[box] ** 2)), xb
+
+    ctx = hip.model_from_case(case)
+    try:
+        ctx.model_large_step(2)
+        x0 = {k: ctx.model_get(k) for k in names}
+        q0 = [ctx.model_get(hip.QTRC_BASE + iq) for iq in range(3)]
+
+        def reset():
+            for k in names:
+                ctx.model_set(k, x0[k])
+            for iq in range(3):
+                ctx.model_set(hip.QTRC_BASE + iq, q0[iq])
+
+        # forward cost
+        reset()
+        ctx.model_large_step(nsteps)  # warm-up
+        reset()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.model_large_step(nsteps)
+        ctx.synchronize()
+        fwd_ms = (time.perf_counter() - t0) * 1e3
+        # adjoint: checkpointed forward + reverse sweep (device timers; the
+        # host transfers of the five adjoint fields are excluded)
+        reset()
+        ctx.profile(True)
+        ctx.model_forward_checkpointed(nsteps, every)
+        J0, xbN = objective({k: ctx.model_get(k) for k in names})
+        xb0 = ctx.model_adjoint(xbN)
+        _, fck_ms = ctx.profile_query("model_forward_checkpointed")
+        _, rev_ms = ctx.profile_query("model_adjoint")
+        ctx.profile(False)
+        # gradient w.r.t. the interior: fold the halo and ghost-level adjoints
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from test_adjoint_oracle import fill, interior_gradient
+        grad = interior_gradient(g, xb0)
+        rng = np.random.default_rng(seed)
+        scale = {k: float(np.abs(x0[k]).max()) for k in names}
+        errs = {e: [] for e in eps_list}
+        for _ in range(ndirs):
+            d = {}
+            for k in names:
+                a = np.zeros(g.shape, order="F")
+                ke = g.KE - 1 if k == "MOMZ" else g.KE
+                sl = (slice(g.KS, ke + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+                a[sl] = rng.standard_normal(a[sl].shape) * scale[k]
+                d[k] = fill(g, a, k)
+            ad = sum(float(np.sum(grad[k] * d[k])) for k in names)
+            for e in eps_list:
+                Js = []
+                for sgn in (1.0, -1.0):
+                    for k in names:
+                        ctx.model_set(k, np.asfortranarray(x0[k] + sgn * e * d[k]))
+                    ctx.model_large_step(nsteps)
+                    d1, r1 = ctx.model_get("DENS"), ctx.model_get("RHOT")
+                    Js.append(float(np.sum((r1[box] / d1[box] - ref_pott[box]) ** 2)))
+                fd = (Js[0] - Js[1]) / (2.0 * e)
+                errs[e].append(abs(fd - ad) / abs(ad))
+    finally:
+        ctx.close()
+    return dict(workload="warm bubble 128x128x64, DX=500 m, DZ=250 m, u=10 m/s, dt=1 s, 3 FCT tracers, FP64, UD3",
+                steps=nsteps, checkpoint_every=every, J=J0, fwd_ms=fwd_ms,
+                forward_checkpointed_ms=fck_ms, reverse_ms=rev_ms, adj_ms=fck_ms + rev_ms,
+                adj_over_fwd=(fck_ms + rev_ms) / fwd_ms,
+                fd={"directions": ndirs, **{f"eps_{e:g}": {"max_rel_err": max(v), "median_rel_err": float(np.median(v))}
+                                            for e, v in errs.items()}})
 
 
 def cpu_baseline(ni, nj, nk):
@@ -167,6 +265,9 @@ def main():
         mixed = dict(value=ni * nj * nk * a.steps / m_el, ms_per_step=m_el * 1e3 / a.steps,
                      stage_ms=m_stage[1] / max(m_stage[0], 1), speedup_vs_fp64=elapsed / m_el,
                      stage_bytes_per_point=MIXED_STAGE_BYTES_PER_POINT)
+    adj = None
+    if not a.no_adjoint:
+        adj = adjoint_leg(40, 8, a.adjoint_dirs, (1e-6, 1e-7))
     theta = None
     if a.theta_check > 0:
         theta = theta_check(case, a.theta_check)
@@ -202,6 +303,7 @@ def main():
                    "large_step_with_3_fct_tracers_ms": large_ms},
         "roofline": roof,
         "fp32_vs_fp64": mixed,
+        "adjoint": adj,
     }
     if theta is not None:
         out["theta_check"] = theta

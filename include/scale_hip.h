@@ -104,6 +104,18 @@ int scale_hip_numfilter_coef(scale_hip_ctx *ctx, double *num_diff,
     const double *DT, const double *REF_dens, const double *REF_pott,
     const double *ND_COEF, const int32_t *ND_ORDER, const double *ND_SFC_FACT, const int32_t *ND_USE_RS);
 
+/* Hyperdiffusion fluxes of one tracer num_diff_q(KA,IA,JA,3) including their
+ * halo exchange: ATMOS_DYN_numfilter_coef_q,
+ * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:989-1200, same
+ * dummies.  ND_ORDER = 1; QV (is_qv) needs ND_USE_RS = .true.  The filter
+ * tables come from the grid given to an earlier scale_hip_numfilter_coef call
+ * or to the model. */
+int scale_hip_numfilter_coef_q(scale_hip_ctx *ctx, double *num_diff_q,
+    const double *DENS, const double *QTRC, const int32_t *is_qv,
+    const double *CDZ, const double *CDX, const double *CDY, const double *dt,
+    const double *REF_qv, const int32_t *iq,
+    const double *ND_COEF, const int32_t *ND_ORDER, const double *ND_SFC_FACT, const int32_t *ND_USE_RS);
+
 /* Rayleigh damping coefficient wdamp_coef(KA) from the face heights FZ(0:KA)
  * (KA+1 values), the damping time scale and the height above which it acts:
  * ATMOS_DYN_wdamp_setup, scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:323-381.
@@ -133,6 +145,8 @@ typedef struct {
                                with the reference state, the pressure terms, the
                                vertical solve and the budgets in FP64; flat
                                terrain and QA = 0 only                         */
+  double nd_coef_q;    /* ATMOS_DYN_NUMERICAL_DIFF_COEF_TRACER (0 by default:
+                          num_diff_q = 0, LARGE.F90:1053-1054)                 */
 } scale_hip_dyn_config;
 
 /* field ids for scale_hip_model_set/_get/_mass */
@@ -175,7 +189,7 @@ int scale_hip_stream(scale_hip_ctx *ctx, void **stream);
 /* Named kernel timers: hipEvents recorded around every launch on the context's
  * stream (the counterpart of PROF_rapstart/PROF_rapend,
  * scalelib/src/common/scale_prof.F90:156-253).  Names: hevi_stage,
- * numfilter, halo_fill, rk4_combine, tracer_nofct, fct_philo, fct_ratio,
+ * numfilter, numfilter_q, halo_fill, rk4_combine, tracer_nofct, fct_philo, fct_ratio,
  * fct_update, hbm_copy, model_forward_checkpointed, model_adjoint.  Enabling
  * resets the accumulators. */
 int scale_hip_profile_enable(scale_hip_ctx *ctx, int32_t on);

@@ -133,6 +133,18 @@ def numfilter_coef(case, state: dict, dt: float, nd_coef=1.0e-4, nd_sfc_fact=1.0
     return nd
 
 
+def numfilter_coef_q(case, DENS, QTRC, dt: float, nd_coef=1.0e-4, nd_sfc_fact=1.0, is_qv=False, REF_qv=None) -> np.ndarray:
+    g = case.grid
+    nd = np.full((g.KA, g.IA, g.JA, 3), np.nan, order="F")
+    gs, ms = grid_struct(g), metric_struct(case.metric)
+    rc = lib().orc_numfilter_coef_q(C.byref(gs), ptr(nd), ptr(DENS), ptr(QTRC), C.c_int(int(is_qv)), C.byref(ms),
+                                    C.c_double(dt), ptr(REF_qv), C.c_double(nd_coef), C.c_int(1),
+                                    C.c_double(nd_sfc_fact), C.c_int(1))
+    if rc != 0:
+        raise RuntimeError("orc_numfilter_coef_q failed")
+    return nd
+
+
 def tstep_tracer(case, QTRCo, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff_q, dtl, fct: bool,
                  along_stream: bool = True, scheme: int | None = None):
     g = case.grid
@@ -245,7 +257,7 @@ def adj_large_step(model: "Model", xb: dict) -> dict:
 class Model:
     """Whole-model integrator (LARGE.F90 on-path subset) over a Case."""
 
-    def __init__(self, case, nd_coef=1.0e-4, divdmp_coef=0.0, fct=None, along_stream=True):
+    def __init__(self, case, nd_coef=1.0e-4, divdmp_coef=0.0, fct=None, along_stream=True, nd_coef_q=0.0):
         g = case.grid
         self.case = case
         self.state = {k: np.array(v, order="F", copy=True) for k, v in case.state.items()}
@@ -255,7 +267,7 @@ class Model:
         self.mflx_hi = np.full(g.shape + (3,), np.nan, order="F")
         self.tflx_hi = np.full(g.shape + (3,), np.nan, order="F")
         self.rhoq_tp = np.zeros(g.shape + (max(qa, 1),), order="F")
-        self.cfg = OrcConfig(case.scheme, case.scheme_tracer, 4, case.dt, case.dt, nd_coef, 1.0, 0.0, 1, 1,
+        self.cfg = OrcConfig(case.scheme, case.scheme_tracer, 4, case.dt, case.dt, nd_coef, 1.0, nd_coef_q, 1, 1,
                              divdmp_coef, int(case.fct if fct is None else fct), int(along_stream))
         self._keep = []
 

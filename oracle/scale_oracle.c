@@ -1056,6 +1056,62 @@ int orc_numfilter_coef(const orc_grid *g, double *num_diff,
   return 0;
 }
 
+/* ---- tracer filter (ATMOS_DYN_numfilter_coef_q, COMMON.F90:989-1200) ----
+ * ND_ORDER = 1.  The differenced field is QTRC itself for a tracer that is
+ * not QV (:1108-1112) and QTRC - REF_qv for QV with ND_USE_RS (:1085-1098);
+ * QV without the reference state (the 19-point smoothing, :1049-1077) is
+ * not supported (returns -1).  Density weights and surface rows as
+ * :1117-1184, then the halo exchange of the three components (:1186-1196). */
+int orc_numfilter_coef_q(const orc_grid *g, double *num_diff_q, const double *DENS, const double *QTRC, int is_qv,
+    const orc_metric *m, double DT, const double *REF_qv, double ND_COEF, int nd_order, double ND_SFC_FACT, int ND_USE_RS) {
+  GRID_LOCALS;
+  if (nd_order != 1 || (is_qv && !ND_USE_RS)) return -1;
+  const double *CDZ = m->CDZ, *CDX = m->CDX, *CDY = m->CDY;
+  cn_tables t;
+  filter_setup(g, m, &t);
+  const double DIFF4 = ND_COEF / ((double)(1 << 4) * DT);
+  double *nd_cdz = alloc_nan(KA), *nd_cdx = alloc_nan(IA), *nd_cdy = alloc_nan(JA);
+  for (int k = KS - 1; k <= KE; ++k) nd_cdz[k] = DIFF4 * pow4(CDZ[k]);
+  for (int i = IS; i <= IE; ++i) nd_cdx[i] = DIFF4 * pow4(CDX[i]);
+  for (int j = JS; j <= JE; ++j) nd_cdy[j] = DIFF4 * pow4(CDY[j]);
+  double *work = alloc_nan(3 * N3), *fld = alloc_nan(N3);
+  double *wZ = work, *wX = work + N3, *wY = work + 2 * N3;
+  const double *phi = QTRC;
+  if (is_qv) {
+    OMP_FOR
+    for (int j = JS - 1; j <= JE + 2; ++j)
+      for (int i = IS - 1; i <= IE + 2; ++i)
+        for (int k = KS; k <= KE; ++k) fld[I3(k, i, j)] = QTRC[I3(k, i, j)] - REF_qv[I3(k, i, j)];
+    phi = fld;
+  }
+  calc_diff3(g, &t, work, phi, 0, 0, 0);
+  double *qZ = num_diff_q, *qX = num_diff_q + N3, *qY = num_diff_q + 2 * N3;
+  OMP_FOR
+  for (int j = JS; j <= JE; ++j)
+    for (int i = IS; i <= IE; ++i) {
+      for (int k = KS; k <= KE - 1; ++k)
+        qZ[I3(k, i, j)] = wZ[I3(k, i, j)] * nd_cdz[k] * 0.5 * (DENS[I3(k + 1, i, j)] + DENS[I3(k, i, j)]);
+      for (int k = 0; k <= KS - 2; ++k) qZ[I3(k, i, j)] = 0.0;
+      qZ[I3(KS - 1, i, j)] = wZ[I3(KS - 1, i, j)] * nd_cdz[KS - 1] * DENS[I3(KS, i, j)];
+      qZ[I3(KE, i, j)] = wZ[I3(KE, i, j)] * nd_cdz[KE] * DENS[I3(KE, i, j)];
+      for (int k = KE + 1; k < KA; ++k) qZ[I3(k, i, j)] = 0.0;
+      for (int k = KS; k <= KE; ++k) {
+        qX[I3(k, i, j)] = wX[I3(k, i, j)] * nd_cdx[i] * 0.5 * (DENS[I3(k, i + 1, j)] + DENS[I3(k, i, j)]);
+        qY[I3(k, i, j)] = wY[I3(k, i, j)] * nd_cdy[j] * 0.5 * (DENS[I3(k, i, j + 1)] + DENS[I3(k, i, j)]);
+      }
+      for (int k = 0; k <= KS - 1; ++k) { qX[I3(k, i, j)] = 0.0; qY[I3(k, i, j)] = 0.0; }
+      qX[I3(KS, i, j)] = qX[I3(KS, i, j)] * ND_SFC_FACT;
+      qX[I3(KS + 1, i, j)] = qX[I3(KS + 1, i, j)] * (1.0 + ND_SFC_FACT) * 0.5;
+      qY[I3(KS, i, j)] = qY[I3(KS, i, j)] * ND_SFC_FACT;
+      qY[I3(KS + 1, i, j)] = qY[I3(KS + 1, i, j)] * (1.0 + ND_SFC_FACT) * 0.5;
+      for (int k = KE + 1; k < KA; ++k) { qX[I3(k, i, j)] = 0.0; qY[I3(k, i, j)] = 0.0; }
+    }
+  for (int n = 0; n < 3; ++n) orc_halo_periodic(g, num_diff_q + N3 * (size_t)n);
+  free(work); free(fld); free(nd_cdz); free(nd_cdx); free(nd_cdy);
+  free(t.CNZ3); free(t.CNX3); free(t.CNY3);
+  return 0;
+}
+
 /* ---- divergence for divergence damping (COMMON.F90:1437-1515) ----------- */
 void orc_divergence(const orc_grid *g, double *DDIV, const double *MOMZ, const double *MOMX, const double *MOMY, const orc_metric *m) {
   GRID_LOCALS;
@@ -1588,10 +1644,13 @@ int orc_large_step(const orc_grid *g, const orc_metric *m, const orc_config *c, 
       orc_halo_periodic(g, st->mflx_hi + N3 * n);
     }
     double *ndq = alloc_nan(3 * N3);
-    for (size_t q = 0; q < 3 * N3; ++q) ndq[q] = 0.0; /* ND_COEF_Q = 0 (driver default) */
+    for (size_t q = 0; q < 3 * N3; ++q) ndq[q] = 0.0; /* ND_COEF_Q = 0 (driver default): zeros (LARGE.F90:1053-1054) */
     const double *MAPF_XY = m->MAPF; /* MAPF(:,:,:,I_XY) */
     for (int iq = 0; iq < st->QA; ++iq) {
       double *q = st->QTRC + N3 * iq;
+      /* passive tracers (none is QV): LARGE.F90:1049-1061 with DENS00 */
+      if (c->nd_coef_q != 0.0)
+        rc |= orc_numfilter_coef_q(g, ndq, DENS00, q, 0, m, dtl, NULL, c->nd_coef_q, c->nd_order, c->nd_sfc_fact, c->nd_use_rs);
       rc |= orc_tinteg_tracer_rk3(g, c->scheme_tracer, q, q, st->RHOQ_tp + N3 * iq, DENS00, st->DENS, st->mflx_hi, ndq,
                                   m->GSQRT, MAPF_XY, m->CDZ, m->RCDZ, m->RCDX, m->RCDY, dtl,
                                   c->flag_fct_tracer, c->flag_fct_along_stream);

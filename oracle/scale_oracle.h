@@ -75,6 +75,12 @@ int orc_numfilter_coef(const orc_grid *g, double *num_diff,
     const orc_metric *m, double dt, const double *REF_dens, const double *REF_pott,
     double nd_coef, int nd_order, double nd_sfc_fact, int nd_use_rs);
 
+/* ATMOS_DYN_numfilter_coef_q (COMMON.F90:989-1200), ND_ORDER = 1, with the
+ * halo fill of num_diff_q(KA,IA,JA,3).  is_qv selects QTRC - REF_qv (which
+ * needs ND_USE_RS); returns -1 for unsupported options. */
+int orc_numfilter_coef_q(const orc_grid *g, double *num_diff_q, const double *DENS, const double *QTRC, int is_qv,
+    const orc_metric *m, double dt, const double *REF_qv, double nd_coef, int nd_order, double nd_sfc_fact, int nd_use_rs);
+
 /* ATMOS_DYN_divergence (COMMON.F90:1437-1515) */
 void orc_divergence(const orc_grid *g, double *DDIV,
     const double *MOMZ, const double *MOMX, const double *MOMY, const orc_metric *m);

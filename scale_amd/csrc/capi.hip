@@ -230,6 +230,24 @@ int set_numfilter_tables(Ctx& c, double DT, double ND_COEF, double sfc) {
   return 0;
 }
 
+// nd_coef_cdz/cdx/cdy of the tracer filter (COMMON.F90:1032-1043) for ND_COEF_Q and dt
+int set_ndq_table(Ctx& c, double dt, double ND_COEF, double sfc) {
+  if (c.ndqc && c.ndq_coef == ND_COEF && c.ndq_dt == dt && c.ndq_sfc == sfc) return 0;
+  const int KA = c.d.KA, IA = c.d.IA, JA = c.d.JA, KS = c.d.KS, KE = c.d.KE, IS = c.d.IS, IE = c.d.IE, JS = c.d.JS, JE = c.d.JE;
+  if (c.hCDZ.empty() || c.hCDX.empty() || c.hCDY.empty()) return fail("the grid (CDZ, CDX, CDY) has not been given");
+  const double DIFF4 = ND_COEF / ((double)(1 << 4) * dt);
+  auto p4 = [](double v) { double v2 = v * v; return v2 * v2; };
+  std::vector<double> t((size_t)KA + IA + JA, 0.0);
+  for (int k = KS - 1; k <= KE; ++k) t[k] = DIFF4 * p4(c.hCDZ[k]);
+  for (int i = IS; i <= IE; ++i) t[KA + i] = DIFF4 * p4(c.hCDX[i]);
+  for (int j = JS; j <= JE; ++j) t[KA + IA + j] = DIFF4 * p4(c.hCDY[j]);
+  if (!c.ndqc && !(c.ndqc = dalloc(c, t.size(), nullptr))) return fail("hipMalloc of the tracer filter table failed");
+  CK(hipMemcpyAsync(c.ndqc, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CK(hipStreamSynchronize(c.stream));
+  c.ndq_coef = ND_COEF; c.ndq_dt = dt; c.ndq_sfc = sfc;
+  return 0;
+}
+
 // drop-in buffers: 0..4 s0, 5..9 s, 10..14 t, 15 DPRES0, 16 RT2P, 17 REF_dens, 18 REF_rhot, 19 DDIV,
 // 20 num_diff(15), 21..25 outputs, 26 mflx(3), 27 tflx(3), 28 CORIOLI(IA*JA), 29 wdamp(KA)
 enum { B_S0 = 0, B_S = 5, B_T = 10, B_DPRES0 = 15, B_RT2P, B_REFD, B_REFR, B_DDIV, B_ND, B_OUT, B_MF = 26, B_TF, B_COR, B_WD, B_N };
@@ -317,7 +335,7 @@ int scale_hip_finalize(scale_hip_ctx* h) {
   if (c.stream) hipStreamSynchronize(c.stream);
   auto F = [](void* p) { if (p) hipFree(p); };
   F(c.g1buf); F(c.GSQRT); F(c.J13G); F(c.J23G); F(c.MAPF); F(c.PHI);
-  F(c.CNZ3); F(c.CNX3); F(c.CNY3); F(c.ndc); F(c.stage);
+  F(c.CNZ3); F(c.CNX3); F(c.CNY3); F(c.ndc); F(c.ndqc); F(c.stage);
   for (int q = 0; q < shp::NSCR; ++q) F(c.scr[q]);
   for (double* b : h->dbuf) F(b);
   shp::adj_free(c);
@@ -543,6 +561,30 @@ int scale_hip_numfilter_coef(scale_hip_ctx* h, double* num_diff,
   return download3(c, num_diff, B[B_ND], 15);
 }
 
+int scale_hip_numfilter_coef_q(scale_hip_ctx* h, double* num_diff_q, const double* DENS, const double* QTRC,
+    const int32_t* is_qv, const double* CDZ, const double* CDX, const double* CDY, const double* dt,
+    const double* REF_qv, const int32_t* iq, const double* ND_COEF, const int32_t* ND_ORDER,
+    const double* ND_SFC_FACT, const int32_t* ND_USE_RS) {
+  (void)iq;
+  if (int r = check_ctx(h)) return r;
+  if (!is_qv || !dt || !ND_COEF || !ND_ORDER || !ND_SFC_FACT || !ND_USE_RS) return fail("null scalar argument");
+  if (*ND_ORDER != 1) return fail("ND_ORDER other than 1 is not supported");
+  if (*is_qv && !*ND_USE_RS) return fail("QV without ND_USE_RS (the 19-point smoothing) is not supported");
+  Ctx& c = h->c;
+  if (!c.CNZ3) return fail("the filter tables come from the grid: call scale_hip_numfilter_coef or create the model first");
+  if (int r = ensure_dropin(h)) return r;
+  if (int r = set_grid1d(c, CDZ, nullptr, nullptr, nullptr, CDX, nullptr, nullptr, nullptr, CDY, nullptr, nullptr, nullptr)) return r;
+  if (int r = set_ndq_table(c, *dt, *ND_COEF, *ND_SFC_FACT)) return r;
+  auto& B = h->dbuf;
+  if (int r = upload3(c, B[B_S], DENS, 1)) return r;
+  if (int r = upload3(c, B[B_S + 1], QTRC, 1)) return r;
+  if (*is_qv) {
+    if (int r = upload3(c, B[B_S + 2], REF_qv, 1)) return r;
+  }
+  CK(shp::numfilter_q(c, B[B_MF], B[B_S], B[B_S + 1], *is_qv ? B[B_S + 2] : nullptr, c.stream));
+  return download3(c, num_diff_q, B[B_MF], 3);
+}
+
 // ---- device-resident model ---------------------------------------------------------
 int scale_hip_set_metrics(scale_hip_ctx* h,
     const double* CDZ, const double* FDZ, const double* RCDZ, const double* RFDZ,
@@ -571,6 +613,7 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   if (QA < 0) return fail("QA must be >= 0");
   if (cfg->mixed_precision && c.terrain) return fail("mixed precision needs flat terrain (GSQRT = 1, J13G = J23G = 0, MAPF = 1)");
   if (cfg->mixed_precision && QA > 0) return fail("mixed precision integrates the dynamics only (QA = 0)");
+  if (cfg->nd_coef_q < 0.0) return fail("nd_coef_q must be >= 0");
   shp::Model* M = new shp::Model();
   c.model = M;
   const size_t n3 = c.d.n3;
@@ -615,8 +658,11 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   CK(hipMemsetAsync(M->wdamp_coef, 0, (size_t)c.d.KA * sizeof(double), c.stream));
   M->dtl = cfg->dtl; M->dts = cfg->dts; M->nd_coef = cfg->nd_coef; M->nd_sfc_fact = cfg->nd_sfc_fact;
   M->divdmp_coef = cfg->divdmp_coef; M->fct = cfg->fct_tracer ? 1 : 0; M->fct_along = cfg->fct_along_stream ? 1 : 0;
+  M->nd_coef_q = cfg->nd_coef_q;
   const int nstep = (int)ceil((cfg->dtl - shp::EPS_RP) / cfg->dts);
   if (int r = set_numfilter_tables(c, cfg->dtl / nstep, cfg->nd_coef, cfg->nd_sfc_fact)) return r;
+  if (cfg->nd_coef_q != 0.0)
+    if (int r = set_ndq_table(c, cfg->dtl, cfg->nd_coef_q, cfg->nd_sfc_fact)) return r;
   CK(hipStreamSynchronize(c.stream));
   return 0;
 }

@@ -521,6 +521,8 @@ hipError_t model_large_step(Ctx& c) {
     for (int iq = 0; iq < M.QA; ++iq) {
       double* q = M.QTRC + d.n3 * (size_t)iq;
       const double* rq = M.RHOQ_tp ? M.RHOQ_tp + d.n3 * (size_t)iq : M.zeros;
+      // tracer filter from DENS00 (LARGE.F90:1049-1061); with ND_COEF_Q = 0 num_diff_q stays zero
+      if (M.nd_coef_q != 0.0) HIPCHK(numfilter_q(c, M.ndq, M.DENS00, q, nullptr, s));
       TracerIO io;
       io.RHOQ_t = rq; io.DENS0 = M.DENS00; io.mflx_hi = M.mflx_hi; io.num_diff = M.ndq; io.flag_along = M.fct_along;
       // stage 1: dt/3, no FCT

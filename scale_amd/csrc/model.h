@@ -36,7 +36,7 @@ struct Model {
   float *fnum_diff = nullptr, *fDDIV = nullptr, *fmflx = nullptr, *ftflx = nullptr;
   float *fmfR[4] = {}, *ftfR[4] = {};
   // configuration (PARAM_ATMOS_DYN subset)
-  double dtl = 1.0, dts = 1.0, nd_coef = 1.0e-4, nd_sfc_fact = 1.0, divdmp_coef = 0.0;
+  double dtl = 1.0, dts = 1.0, nd_coef = 1.0e-4, nd_sfc_fact = 1.0, divdmp_coef = 0.0, nd_coef_q = 0.0;
   int fct = 0, fct_along = 1;
   // every allocation, freed together
   std::vector<void*> allocs;

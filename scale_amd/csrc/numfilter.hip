@@ -260,6 +260,87 @@ hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], 
                      const double* REF_pott, hipStream_t s) {
   return numfilter_t<double>(c, num_diff, st, REF_dens, REF_pott, s);
 }
+
+// ---------------------------------------------------------------------------
+// Tracer filter num_diff_q(KA,IA,JA,3): ATMOS_DYN_numfilter_coef_q with
+// ND_ORDER = 1 (COMMON.F90:989-1200).  The third differences (calc_diff3,
+// :1652-1800) of the tracer, or of QTRC - REF_qv for QV (:1085-1098), times
+// the density at the flux point (:1117-1184).  It runs once per tracer per
+// large step, so one thread per point of the interior columns (all levels)
+// reads its stencil through the caches.
+struct NfqArgs {
+  Dims d;
+  const double *DENS, *Q, *REF;      // REF: REF_qv or nullptr
+  const double *CNZ3, *CNX3, *CNY3;
+  const double* ndq;                 // nd_coef_cdz(KA) nd_coef_cdx(IA) nd_coef_cdy(JA) for ND_COEF_Q and dt
+  double sfc;
+  double* out;                       // num_diff_q, 3 fields
+};
+
+__global__ void __launch_bounds__(256) numfilter_q_kernel(NfqArgs a) {
+  const Dims& d = a.d;
+  const int i = d.IS + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int k = (int)(blockIdx.y * 4 + threadIdx.y);
+  const int j = d.JS + (int)blockIdx.z;
+  if (i > d.IE || k >= d.KA) return;
+  const int KS = d.KS, KE = d.KE;
+  auto cz = [&](int n, int kk, int b) { return a.CNZ3[(n - 1) + 3 * ((size_t)kk + (size_t)d.KA * (b - 1))]; };
+  auto cx = [&](int n, int ii, int b) { return a.CNX3[(n - 1) + 3 * ((size_t)ii + (size_t)d.IA * (b - 1))]; };
+  auto cy = [&](int n, int jj, int b) { return a.CNY3[(n - 1) + 3 * ((size_t)jj + (size_t)d.JA * (b - 1))]; };
+  auto phi = [&](int kk, int ii, int jj) {
+    const size_t q = IX(d, kk, ii, jj);
+    return a.REF ? a.Q[q] - a.REF[q] : a.Q[q];
+  };
+  auto D = [&](int kk, int ii, int jj) { return a.DENS[IX(d, kk, ii, jj)]; };
+  auto dz = [&](int kk) -> double {
+    if (kk == KS)
+      return cz(1, KS + 1, 1) * phi(KS + 2, i, j) - cz(2, KS + 1, 1) * phi(KS + 1, i, j) + cz(3, KS + 1, 1) * phi(KS, i, j)
+           - cz(1, KS, 1) * phi(KS + 1, i, j);
+    if (kk == KE - 1)
+      return cz(1, KE, 1) * phi(KE - 1, i, j) - cz(2, KE, 1) * phi(KE, i, j) + cz(3, KE, 1) * phi(KE - 1, i, j)
+           - cz(1, KE - 1, 1) * phi(KE - 2, i, j);
+    return cz(1, kk + 1, 1) * phi(kk + 2, i, j) - cz(2, kk + 1, 1) * phi(kk + 1, i, j) + cz(3, kk + 1, 1) * phi(kk, i, j)
+         - cz(1, kk, 1) * phi(kk - 1, i, j);
+  };
+  const double* cdz = a.ndq;
+  const double* cdx = a.ndq + d.KA;
+  const double* cdy = a.ndq + d.KA + d.IA;
+  const size_t p = IX(d, k, i, j), n3 = d.n3;
+  double z = 0.0;
+  if (k >= KS && k <= KE - 1) z = dz(k) * cdz[k] * 0.5 * (D(k + 1, i, j) + D(k, i, j));
+  else if (k == KS - 1) z = (-dz(KS)) * cdz[KS - 1] * D(KS, i, j);
+  else if (k == KE) z = (-dz(KE - 1)) * cdz[KE] * D(KE, i, j);
+  double x = 0.0, y = 0.0;
+  if (k >= KS && k <= KE) {
+    const double wx = cx(1, i + 1, 1) * phi(k, i + 2, j) - cx(2, i + 1, 1) * phi(k, i + 1, j) + cx(3, i + 1, 1) * phi(k, i, j)
+                    - cx(1, i, 1) * phi(k, i - 1, j);
+    const double wy = cy(1, j + 1, 1) * phi(k, i, j + 2) - cy(2, j + 1, 1) * phi(k, i, j + 1) + cy(3, j + 1, 1) * phi(k, i, j)
+                    - cy(1, j, 1) * phi(k, i, j - 1);
+    x = wx * cdx[i] * 0.5 * (D(k, i + 1, j) + D(k, i, j));
+    y = wy * cdy[j] * 0.5 * (D(k, i, j + 1) + D(k, i, j));
+    if (k == KS) { x = x * a.sfc; y = y * a.sfc; }
+    else if (k == KS + 1) { x = x * (1.0 + a.sfc) * 0.5; y = y * (1.0 + a.sfc) * 0.5; }
+  }
+  a.out[p] = z;
+  a.out[p + n3] = x;
+  a.out[p + 2 * n3] = y;
+}
+
+hipError_t numfilter_q(const Ctx& c, double* num_diff_q, const double* DENS, const double* QTRC, const double* REF_qv,
+                       hipStream_t s) {
+  const Dims& d = c.d;
+  NfqArgs a;
+  a.d = d; a.DENS = DENS; a.Q = QTRC; a.REF = REF_qv;
+  a.CNZ3 = c.CNZ3; a.CNX3 = c.CNX3; a.CNY3 = c.CNY3; a.ndq = c.ndqc; a.sfc = c.ndq_sfc; a.out = num_diff_q;
+  const dim3 grid((d.IE - d.IS + 1 + 63) / 64, (d.KA + 3) / 4, d.JE - d.JS + 1);
+  prof_start(c, "numfilter_q", s);
+  hipLaunchKernelGGL(numfilter_q_kernel, grid, dim3(64, 4), 0, s, a);
+  prof_stop(c, s);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  double* arrs[3] = {num_diff_q, num_diff_q + d.n3, num_diff_q + 2 * d.n3};
+  return halo_fill(c, arrs, 3, s);
+}
 hipError_t numfilter(const Ctx& c, float* num_diff, const float* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s) {
   return numfilter_t<float>(c, num_diff, st, REF_dens, REF_pott, s);

@@ -28,6 +28,9 @@ struct Ctx {
   // numerical filter tables
   double *CNZ3 = nullptr, *CNX3 = nullptr, *CNY3 = nullptr, *ndc = nullptr;
   double nd_coef = 0.0, nd_dt = 0.0, nd_sfc_fact = 1.0;
+  // tracer filter coefficients nd_coef_cd{z,x,y} for ND_COEF_Q and the large step
+  double* ndqc = nullptr;
+  double ndq_coef = -1.0, ndq_dt = -1.0, ndq_sfc = 1.0;
   // scratch
   double* scr[NSCR] = {};
   // staging buffer for host <-> device layout transposes
@@ -101,6 +104,10 @@ hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], 
 hipError_t numfilter(const Ctx& c, float* num_diff, const float* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s);
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s);
+// ATMOS_DYN_numfilter_coef_q (ND_ORDER = 1) with the halo fill of its 3
+// fields; REF_qv non-null for QV.  Needs the filter tables and c.ndqc.
+hipError_t numfilter_q(const Ctx& c, double* num_diff_q, const double* DENS, const double* QTRC, const double* REF_qv,
+                       hipStream_t s);
 
 // Adjoint of one flat-terrain FP64 stage (adjoint.hip): given the stage's
 // inputs and the adjoints ob of its five prognostic outputs (read at the

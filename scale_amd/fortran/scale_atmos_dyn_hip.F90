@@ -22,6 +22,7 @@ module scale_atmos_dyn_hip
   public :: ATMOS_DYN_Tstep_short_fvm_hevi_hip
   public :: ATMOS_DYN_Tstep_tracer_fvm_heve_hip
   public :: ATMOS_DYN_numfilter_coef_hip
+  public :: ATMOS_DYN_numfilter_coef_q_hip
 
   integer, parameter :: DP = c_double
 
@@ -91,6 +92,15 @@ module scale_atmos_dyn_hip
        real(c_double), intent(in) :: ND_COEF, ND_SFC_FACT
        integer(c_int32_t), intent(in) :: ND_ORDER, ND_USE_RS
      end function c_numfilter
+     integer(c_int) function c_numfilter_q(h, num_diff_q, DENS, QTRC, is_qv, CDZ, CDX, CDY, dt, REF_qv, iq, &
+          ND_COEF, ND_ORDER, ND_SFC_FACT, ND_USE_RS) bind(C, name="scale_hip_numfilter_coef_q")
+       import :: c_ptr, c_int, c_double, c_int32_t
+       type(c_ptr), value :: h
+       real(c_double) :: num_diff_q(*)
+       real(c_double), intent(in) :: DENS(*), QTRC(*), CDZ(*), CDX(*), CDY(*), dt, REF_qv(*)
+       real(c_double), intent(in) :: ND_COEF, ND_SFC_FACT
+       integer(c_int32_t), intent(in) :: is_qv, iq, ND_ORDER, ND_USE_RS
+     end function c_numfilter_q
   end interface
 
 contains
@@ -185,6 +195,21 @@ contains
     call check(c_numfilter(ctx, num_diff, DENS, MOMZ, MOMX, MOMY, RHOT, CDZ, CDX, CDY, FDZ, FDX, FDY, DT, &
          REF_dens, REF_pott, ND_COEF, int(ND_ORDER, c_int32_t), ND_SFC_FACT, l2i(ND_USE_RS)), "ATMOS_DYN_numfilter_coef_hip")
   end subroutine ATMOS_DYN_numfilter_coef_hip
+
+  !> ATMOS_DYN_numfilter_coef_q (scale_atmos_dyn_common.F90:989-1200), ND_ORDER = 1
+  subroutine ATMOS_DYN_numfilter_coef_q_hip(num_diff_q, DENS, QTRC, is_qv, CDZ, CDX, CDY, dt, REF_qv, iq, &
+       ND_COEF, ND_ORDER, ND_SFC_FACT, ND_USE_RS)
+    real(DP), intent(out) :: num_diff_q(KA,IA,JA,3)
+    real(DP), intent(in) :: DENS(KA,IA,JA), QTRC(KA,IA,JA)
+    logical, intent(in) :: is_qv
+    real(DP), intent(in) :: CDZ(KA), CDX(IA), CDY(JA), dt, REF_qv(KA,IA,JA)
+    integer, intent(in) :: iq
+    real(DP), intent(in) :: ND_COEF, ND_SFC_FACT
+    integer, intent(in) :: ND_ORDER
+    logical, intent(in) :: ND_USE_RS
+    call check(c_numfilter_q(ctx, num_diff_q, DENS, QTRC, l2i(is_qv), CDZ, CDX, CDY, dt, REF_qv, int(iq, c_int32_t), &
+         ND_COEF, int(ND_ORDER, c_int32_t), ND_SFC_FACT, l2i(ND_USE_RS)), "ATMOS_DYN_numfilter_coef_q_hip")
+  end subroutine ATMOS_DYN_numfilter_coef_q_hip
 
   integer(c_int32_t) function l2i(l)
     logical, intent(in) :: l

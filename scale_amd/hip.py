@@ -47,7 +47,7 @@ class HipGrid(C.Structure):
 class DynConfig(C.Structure):
     _fields_ = [("dtl", C.c_double), ("dts", C.c_double), ("nd_coef", C.c_double), ("nd_sfc_fact", C.c_double),
                 ("divdmp_coef", C.c_double), ("nd_order", C.c_int32), ("fct_tracer", C.c_int32),
-                ("fct_along_stream", C.c_int32), ("mixed_precision", C.c_int32)]
+                ("fct_along_stream", C.c_int32), ("mixed_precision", C.c_int32), ("nd_coef_q", C.c_double)]
 
 
 _lib = None
@@ -166,6 +166,13 @@ class Context:
             _p(FDZ), _p(FDX), _p(FDY), _d(DT), _p(REF_dens), _p(REF_pott), _d(ND_COEF),
             C.byref(C.c_int32(int(ND_ORDER))), _d(ND_SFC_FACT), _l(ND_USE_RS)))
 
+    def ATMOS_DYN_numfilter_coef_q(self, num_diff_q, DENS, QTRC, is_qv, CDZ, CDX, CDY, dt, REF_qv, iq,
+                                   ND_COEF, ND_ORDER, ND_SFC_FACT, ND_USE_RS):
+        _check(load_library().scale_hip_numfilter_coef_q(
+            self._h, _p(num_diff_q), _p(DENS), _p(QTRC), _l(is_qv), _p(CDZ), _p(CDX), _p(CDY), _d(dt),
+            _p(REF_qv), C.byref(C.c_int32(int(iq))), _d(ND_COEF), C.byref(C.c_int32(int(ND_ORDER))),
+            _d(ND_SFC_FACT), _l(ND_USE_RS)))
+
     def ATMOS_DYN_Tstep_short_fvm_hevi_adj(self, out_b: dict, DENS0, MOMZ0, MOMX0, MOMY0, RHOT0,
                                            DENS, MOMZ, MOMX, MOMY, RHOT, DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t,
                                            DPRES0, RT2P, CORIOLI, num_diff, wdamp_coef,
@@ -198,9 +205,9 @@ class Context:
             _d(m["J33G"]), _p(m["MAPF"])))
 
     def model_create(self, dtl, dts, QA, nd_coef=1.0e-4, nd_sfc_fact=1.0, fct_tracer=False, fct_along_stream=True,
-                     mixed_precision=False, divdmp_coef=0.0):
+                     mixed_precision=False, divdmp_coef=0.0, nd_coef_q=0.0):
         cfg = DynConfig(dtl, dts, nd_coef, nd_sfc_fact, divdmp_coef, 1, int(fct_tracer), int(fct_along_stream),
-                        int(mixed_precision))
+                        int(mixed_precision), nd_coef_q)
         _check(load_library().scale_hip_model_create(self._h, C.byref(cfg), C.c_int32(QA)))
         self.has_model = True
         self.QA = QA
@@ -269,7 +276,7 @@ class Context:
 
 
 def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, device=0, mixed_precision=False,
-                    divdmp_coef=0.0) -> Context:
+                    divdmp_coef=0.0, nd_coef_q=0.0) -> Context:
     """Create a context holding the device-resident model of an inputs.Case
     (mixed_precision: FP32 dynamics with FP64 solve and reference state; the
     tracers are left out)."""
@@ -278,7 +285,7 @@ def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, 
     ctx.set_metrics(case.metric)
     qa = 0 if mixed_precision else case.qtrc.shape[-1]
     ctx.model_create(case.dt, case.dt, qa, nd_coef=nd_coef, fct_tracer=case.fct if fct is None else fct,
-                     mixed_precision=mixed_precision, divdmp_coef=divdmp_coef)
+                     mixed_precision=mixed_precision, divdmp_coef=divdmp_coef, nd_coef_q=nd_coef_q)
     for k, v in case.state.items():
         ctx.model_set(k, v)
     for k, v in case.ref.items():

@@ -97,6 +97,54 @@ def test_numfilter_bitwise(hip):
     H.compare(ref, got, "num_diff")
 
 
+@pytest.mark.parametrize("is_qv", [False, True], ids=["tracer", "qv"])
+def test_numfilter_q_bitwise(hip, is_qv):
+    """ATMOS_DYN_numfilter_coef_q (COMMON.F90:989-1200) through the drop-in
+    entry point: bit-identical to the oracle."""
+    print(f"[numfilter_q] is_qv={is_qv}", flush=True)
+    case = H.small_case(seed=98, stretched=(50.0, 400.0))
+    g = case.grid
+    st = H.perturbed(case, 5)
+    rng = np.random.default_rng(3)
+    q = np.asfortranarray(1e-3 * (1.0 + 0.2 * rng.standard_normal(g.shape)))
+    ref_qv = np.asfortranarray(1e-3 * (1.0 + 0.05 * rng.standard_normal(g.shape))) if is_qv else None
+    ref = po.numfilter_coef_q(case, st["DENS"], q, 3.0, nd_coef=2.0e-3, nd_sfc_fact=0.7, is_qv=is_qv, REF_qv=ref_qv)
+    got = np.full(ref.shape, np.nan, order="F")
+    m = case.metric
+    with hip.Context(g) as ctx:
+        nd = np.zeros((g.KA, g.IA, g.JA, 5, 3), order="F")  # builds the filter tables from the grid
+        ctx.ATMOS_DYN_numfilter_coef(nd, st["DENS"], st["MOMZ"], st["MOMX"], st["MOMY"], st["RHOT"],
+                                     m["CDZ"], m["CDX"], m["CDY"], m["FDZ"], m["FDX"], m["FDY"], 1.5,
+                                     case.ref["REF_dens"], case.ref["REF_pott"], 1.0e-4, 1, 1.0, True)
+        ctx.ATMOS_DYN_numfilter_coef_q(got, st["DENS"], q, is_qv, m["CDZ"], m["CDX"], m["CDY"], 3.0,
+                                       ref_qv if is_qv else g.zeros(), 1, 2.0e-3, 1, 0.7, True)
+    H.compare(ref, got, "num_diff_q")
+
+
+def test_model_parity_tracer_filter(hip):
+    """Large steps with the tracer filter on (ATMOS_DYN_NUMERICAL_DIFF_COEF_TRACER
+    > 0, LARGE.F90:1049-1061) against the oracle: 5 steps, 1e-12 relative."""
+    print("[model] tracer filter 1e-2, 5 steps", flush=True)
+    case = inputs.make_case(kmax=20, imax=16, jmax=12, noise=0.01, ntracer=2, tracer_kind=["blob", "tophat"], fct=True,
+                            bubble_r=2000.0, bubble_zc=2000.0, momz_noise=0.05, v0=-2.0)
+    g = case.grid
+    m = po.Model(case, nd_coef_q=1.0e-2)
+    m.step(5)
+    ctx = hip.model_from_case(case, nd_coef_q=1.0e-2)
+    try:
+        ctx.model_large_step(5)
+        ctx.synchronize()
+        sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+        for iq in range(2):
+            got = ctx.model_get(hip.QTRC_BASE + iq)
+            ref = m.qtrc[..., iq]
+            d = np.abs(got[sl] - ref[sl]).max()
+            print(f"  QTRC{iq}: max|diff| {d:.3e}", flush=True)
+            assert d <= 1.0e-12 * max(np.abs(ref[sl]).max(), 1e-30)
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("fct,along", [(False, True), (True, True), (True, False)], ids=["nofct", "fct_along", "fct_7pt"])
 @pytest.mark.parametrize("scheme", [3, 4, 31], ids=["UD3", "CD4", "UD3KOREN1993"])
 def test_tracer_bitwise(hip, fct, along, scheme):

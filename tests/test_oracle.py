@@ -391,6 +391,49 @@ def test_numfilter_annihilates_quadratics():
     assert np.abs(inner).max() < 1e-15
 
 
+def test_numfilter_q_is_the_theta_filter_on_the_tracer():
+    """num_diff_q of a tracer equals the RHOT rows of num_diff when the tracer
+    is theta and REF_pott = 0: the same third differences, density weights and
+    surface rows (COMMON.F90:880-962 and :1108-1184), bit for bit."""
+    case = inputs.make_case(kmax=12, imax=10, jmax=10, v0=2.0, momz_noise=0.05, noise=0.1, ntracer=0)
+    g = case.grid
+    st = case.state
+    case0 = inputs.Case(**{**case.__dict__, "ref": {**case.ref, "REF_pott": np.zeros(g.shape, order="F")}})
+    nd = po.numfilter_coef(case0, st, 1.5, nd_coef=3e-4, nd_sfc_fact=0.4)
+    theta = np.asfortranarray(st["RHOT"] / st["DENS"] - 0.0)
+    ndq = po.numfilter_coef_q(case, st["DENS"], theta, 1.5, nd_coef=3e-4, nd_sfc_fact=0.4)
+    for d in range(3):
+        assert np.array_equal(ndq[..., d], nd[..., 4, d]), d
+    assert np.abs(ndq).max() > 0.0
+
+
+def test_numfilter_q_qv_filters_the_anomaly():
+    """QV with ND_USE_RS differences QTRC - REF_qv (COMMON.F90:1085-1098)."""
+    case = inputs.make_case(kmax=12, imax=10, jmax=8, noise=0.1, ntracer=0)
+    g = case.grid
+    rng = np.random.default_rng(4)
+    q = np.asfortranarray(1e-3 * (1.0 + 0.1 * rng.standard_normal(g.shape)))
+    ref = np.asfortranarray(1e-3 * (1.0 + 0.1 * rng.standard_normal(g.shape)))
+    dens = case.state["DENS"]
+    a = po.numfilter_coef_q(case, dens, q, 1.0, is_qv=True, REF_qv=ref)
+    b = po.numfilter_coef_q(case, dens, np.asfortranarray(q - ref), 1.0)
+    assert np.array_equal(a, b)
+    assert np.abs(po.numfilter_coef_q(case, dens, q, 1.0, is_qv=True, REF_qv=q)).max() == 0.0
+
+
+def test_model_tracer_filter_conserves_tracer_mass():
+    """With the tracer filter on, tracer mass stays conserved (flux form)."""
+    case = inputs.make_case(kmax=12, imax=10, jmax=8, noise=0.05, ntracer=1, tracer_kind="blob", fct=True)
+    m = po.Model(case, nd_coef_q=1e-2)
+    q0 = m.mass(np.asfortranarray(m.qtrc[..., 0] * m.state["DENS"]))
+    m.step(4)
+    q1 = m.mass(np.asfortranarray(m.qtrc[..., 0] * m.state["DENS"]))
+    assert abs(q1 - q0) <= 1e-13 * abs(q0)
+    ref = po.Model(case)
+    ref.step(4)
+    assert np.abs(ref.qtrc - m.qtrc).max() > 0.0  # the filter acted
+
+
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("seed", range(6))
 def test_fct_weights_partition_unity(seed):
