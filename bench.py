@@ -91,7 +91,7 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
     at 128x128x64 (DX = 500 m, DZ = 250 m, u = 10 m/s, theta noise 0.01 K,
     dt = 1 s, 3 tracers with FCT), started from the state after two large
     steps (the initial MOMY = MOMZ = 0 sit on the kink of the upwind switch).
-    J = sum over the central 16x16x16 box of (theta - REF_pott)^2 after
+    J = sum over the 16x16x16 box around the bubble of (theta - REF_pott)^2 after
     nsteps large steps; its gradient with respect to the interior state comes
     from the checkpointed forward (every `every` steps) and the adjoint sweep,
     and is compared with central differences of the forward along ndirs
@@ -102,8 +102,9 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
     case = inputs.make_case(kmax=64, imax=128, jmax=128, dx=500.0, dy=500.0, dz=250.0, seed=seed, u0=10.0,
                             noise=0.01, ntracer=3, tracer_kind=["blob", "tophat", "uniform"], dt=1.0, fct=True)
     g = case.grid
-    K, I, J = g.KS + 32, g.IS + 64, g.JS + 64
-    box = (slice(K - 8, K + 8), slice(I - 8, I + 8), slice(J - 8, J + 8))
+    # the 16x16x16 box centred on the bubble (horizontal centre of the domain, z = 0..4 km)
+    I, J = g.IS + 64, g.JS + 64
+    box = (slice(g.KS, g.KS + 16), slice(I - 8, I + 8), slice(J - 8, J + 8))
     ref_pott = case.ref["REF_pott"]
     names = hip.STATE
 

@@ -1421,7 +1421,7 @@ int orc_tstep_tracer_heve(const orc_grid *g, int s,
     double dtl, int flag_fct, int flag_fct_along_stream) {
   GRID_LOCALS;
   (void)CDZ;
-  if (s != ORC_UD3 && s != ORC_CD4) return -1;
+  if (s != ORC_UD3 && s != ORC_CD4 && s != ORC_UD3KOREN) return -1;
   double *qflx_hi = alloc_nan(3 * N3), *qflx_lo = alloc_nan(3 * N3), *qflx_anti = NULL;
   const double *mZ = mflx_hi, *mX = mflx_hi + N3, *mY = mflx_hi + 2 * N3;
   const double *GXYW = GSQRT + N3 * I_XYW, *GUYZ = GSQRT + N3 * I_UYZ, *GXVZ = GSQRT + N3 * I_XVZ, *GXYZ = GSQRT + N3 * I_XYZ;

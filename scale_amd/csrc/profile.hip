@@ -14,8 +14,7 @@ struct Prof {
   std::vector<Rec> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<long, double>> acc;  // name -> (count, total ms)
-  std::string open_name;
-  hipEvent_t open_ev = nullptr;
+  std::vector<std::pair<std::string, hipEvent_t>> open;  // timers started and not yet stopped (nested)
   hipEvent_t get() {
     if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
     hipEvent_t e; hipEventCreate(&e); return e;
@@ -25,18 +24,18 @@ struct Prof {
 void prof_start(const Ctx& c, const char* name, hipStream_t s) {
   if (!c.prof) return;
   Prof& p = *c.prof;
-  p.open_name = name;
-  p.open_ev = p.get();
-  hipEventRecord(p.open_ev, s);
+  hipEvent_t e = p.get();
+  hipEventRecord(e, s);
+  p.open.emplace_back(name, e);
 }
 
 void prof_stop(const Ctx& c, hipStream_t s) {
-  if (!c.prof || !c.prof->open_ev) return;
+  if (!c.prof || c.prof->open.empty()) return;
   Prof& p = *c.prof;
   hipEvent_t b = p.get();
   hipEventRecord(b, s);
-  p.pending.push_back({p.open_name, p.open_ev, b});
-  p.open_ev = nullptr;
+  p.pending.push_back({p.open.back().first, p.open.back().second, b});
+  p.open.pop_back();
   if (p.pending.size() > 4096) prof_collect(c);
 }
 
@@ -60,6 +59,7 @@ Prof* prof_create() { return new Prof(); }
 void prof_destroy(Prof* p) {
   if (!p) return;
   for (auto& r : p->pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+  for (auto& o : p->open) hipEventDestroy(o.second);
   for (auto e : p->pool) hipEventDestroy(e);
   delete p;
 }
