@@ -20,8 +20,8 @@
 // point is pushed to the inputs that point read, with FP64 global atomics
 // where several points read the same input.  The vertical solve runs one
 // thread per column (the transposed Thomas sweep).  Forward values the
-// reverse needs (PT, St and the solved C of each stage) are recomputed by
-// hevi_stage_traced just before the stage is reversed.
+// reverse needs (PT, St and the solved C of each stage) are recorded by
+// hevi_stage_traced when the step's stages are recomputed.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -134,6 +134,7 @@ struct AdjWork {
   double *w[NW] = {};
   double *mf = nullptr, *tf = nullptr;  // traced stage fluxes (3 n3 each)
   double* io[5] = {};                   // adjoint arguments of the C ABI
+  double* tr[4][3] = {};                // PT, St, C of the four stages of the step being reversed
   std::vector<void*> allocs;
 };
 
@@ -174,6 +175,8 @@ hipError_t adj_setup(Ctx& c, int nsteps, int every) {
       for (int r = 0; r < 4 && ok; ++r) ok = (A.Rb[r][v] = aalloc(A, n3)) != nullptr;
     }
     for (int q = 0; q < 3 && ok; ++q) ok = (A.Pb[q] = aalloc(A, n3)) != nullptr;
+    for (int r = 0; r < 4 && ok; ++r)
+      for (int q = 0; q < 3 && ok; ++q) ok = (A.tr[r][q] = aalloc(A, n3)) != nullptr;
     ok = ok && (A.NDb = aalloc(A, 15 * n3)) && (A.mf = aalloc(A, 3 * n3)) && (A.tf = aalloc(A, 3 * n3));
     if (!ok) { adj_free(c); return hipErrorOutOfMemory; }
   }
@@ -213,8 +216,8 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   const int IA = d.IA, KS = d.KS, KE = d.KE, IS = d.IS, IE = d.IE, JS = d.JS, JE = d.JE;
   const double dt = io.dtrk;
 
-  // forward stage with PT, St and C recorded
-  {
+  // forward stage with PT, St and C recorded (unless the caller has them)
+  if (!io.trPT) {
     StageIO f{};
     f.ctx = &c;
     for (int v = 0; v < 5; ++v) { f.o[v] = w[W_O + v]; f.s0[v] = io.s0[v]; f.s[v] = io.s[v]; f.t[v] = io.t[v]; }
@@ -228,7 +231,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
 
   const double *D = io.s[0], *MZ = io.s[1], *MX = io.s[2], *MY = io.s[3], *RT = io.s[4];
   const double *ND = io.num_diff, *COR = io.CORIOLI, *RT2P = io.RT2P, *REFrt = io.REF_rhot, *wd = io.wdamp_coef;
-  const double *PT = w[W_PT], *St = w[W_ST], *Cp = w[W_CP];
+  const double *PT = io.trPT ? io.trPT : w[W_PT], *St = io.trSt ? io.trSt : w[W_ST], *Cp = io.trCp ? io.trCp : w[W_CP];
   double *POTT = w[W_POTT];
   double *POTTb = w[W_POTTB], *Stb = w[W_STB], *Srb = w[W_SRB], *Swb = w[W_SWB], *DPRESb = w[W_DPRESB];
   double *qZb = w[W_QZ], *qXb = w[W_QX], *qYb = w[W_QY];
@@ -906,10 +909,33 @@ static hipError_t step_adj(Ctx& c, double* const x[5], double* const xb[5]) {
   const size_t n3 = d.n3, bytes = n3 * sizeof(double);
   const int KS = d.KS, KE = d.KE;
   double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
-  // forward recomputation: RT2P/DPRES0/REF_rhot, num_diff and the stage outputs R1..R3
+  // forward recomputation: RT2P/DPRES0/REF_rhot, num_diff, and the four
+  // stages with PT, St and C recorded (the outputs R1..R3 with their halos)
   for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(st[v], x[v], bytes, hipMemcpyDeviceToDevice, s));
   HIPCHK(model_prepare(c));
-  HIPCHK(model_short_step_rk4(c, M.dts));
+  if (M.nd_coef == 0.0) {
+    HIPCHK(hipMemsetAsync(M.num_diff, 0, 15 * bytes, s));
+  } else {
+    const double* xs[5] = {x[0], x[1], x[2], x[3], x[4]};
+    HIPCHK(numfilter(c, M.num_diff, xs, M.REF_dens, M.REF_pott, s));
+  }
+  const double dtrk_f[4] = {M.dts / 2.0, M.dts / 2.0, M.dts, M.dts};
+  for (int r = 0; r < 4; ++r) {
+    StageIO f{};
+    f.ctx = &c;
+    for (int v = 0; v < 5; ++v) {
+      f.o[v] = r < 3 ? M.R[r][v] : A.w[W_O + v];
+      f.s0[v] = x[v];
+      f.s[v] = r == 0 ? x[v] : M.R[r - 1][v];
+      f.t[v] = M.t[v];
+    }
+    f.mflx_hi = A.mf; f.tflx_hi = A.tf;
+    f.DPRES0 = M.DPRES0; f.RT2P = M.RT2P; f.CORIOLI = M.CORIOLI; f.num_diff = M.num_diff;
+    f.wdamp_coef = M.wdamp_coef; f.DDIV = nullptr; f.REF_dens = M.REF_dens; f.REF_rhot = M.REF_rhot;
+    f.divdmp_coef = 0.0; f.dtrk = dtrk_f[r]; f.has_coriolis = M.has_coriolis; f.combine = 0;
+    HIPCHK(hevi_stage_traced(f, A.tr[r][0], A.tr[r][1], A.tr[r][2], s));
+    if (r < 3) HIPCHK(halo_fill(c, M.R[r], 5, s));
+  }
 
   // halo fill and ghost levels of the new state (LARGE.F90:908-927)
   for (int v = 0; v < 5; ++v) HIPCHK(halo_adj(c, xb[v], s));
@@ -959,6 +985,7 @@ static hipError_t step_adj(Ctx& c, double* const x[5], double* const xb[5]) {
     io.wdamp_coef = M.wdamp_coef; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
     io.dtrk = dtrk[r]; io.has_coriolis = M.has_coriolis;
     io.NDb = A.NDb; io.DPRES0b = A.Pb[0]; io.RT2Pb = A.Pb[1]; io.REFrtb = A.Pb[2];
+    io.trPT = A.tr[r][0]; io.trSt = A.tr[r][1]; io.trCp = A.tr[r][2];
     HIPCHK(hevi_stage_adj(c, io, s));
   }
   // RT2P, DPRES0, REF_rhot as functions of RHOT at the step start (LARGE.F90:443-471)

@@ -11,6 +11,8 @@
 // separate launch.  R is the storage type of the prognostics and of num_diff
 // (double, or float for the mixed-precision path, where the differences from
 // the reference state are taken in double).
+#include <stdlib.h>
+
 #include "common.h"
 #include "shp_internal.h"
 
@@ -243,11 +245,18 @@ static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], c
   a.num_diff = num_diff;
   const Dims& d = c.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
-  constexpr int TJ = 4;
+  // tile height: SCALE_HIP_NF_TJ = 4, 8 or 16 (default 4)
+  static const int TJ = [] {
+    const char* e = getenv("SCALE_HIP_NF_TJ");
+    const int v = e ? atoi(e) : 4;
+    return (v == 8 || v == 16) ? v : 4;
+  }();
   const unsigned ni = (imax + 63) / 64, nj = (jmax + TJ - 1) / TJ;
   const unsigned nb = ni * nj;
   prof_start(c, "numfilter", s);
-  hipLaunchKernelGGL((numfilter_tile<TJ, R>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  if (TJ == 16) hipLaunchKernelGGL((numfilter_tile<16, R>), dim3(nb), dim3(64, 16), 0, s, a, ni, nb);
+  else if (TJ == 8) hipLaunchKernelGGL((numfilter_tile<8, R>), dim3(nb), dim3(64, 8), 0, s, a, ni, nb);
+  else hipLaunchKernelGGL((numfilter_tile<4, R>), dim3(nb), dim3(64, 4), 0, s, a, ni, nb);
   prof_stop(c, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

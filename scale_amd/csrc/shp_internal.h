@@ -120,6 +120,9 @@ struct StageAdjIO {
   int has_coriolis;
   const double* ob[5];
   double *s0b[5], *sb[5], *NDb, *DPRES0b, *RT2Pb, *REFrtb;
+  // PT, St and the solved C of the forward stage (hevi_stage_traced); when
+  // null the stage adjoint reruns the forward stage to record them
+  const double *trPT = nullptr, *trSt = nullptr, *trCp = nullptr;
 };
 hipError_t hevi_stage_adj(Ctx& c, const StageAdjIO& io, hipStream_t s);
 // num_diff adjoint (15 fields, consumed) -> adjoints of DENS..RHOT (added)
