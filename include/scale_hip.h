@@ -80,6 +80,31 @@ int scale_hip_tstep_short_hevi(scale_hip_ctx *ctx,
     const int32_t *BND_W, const int32_t *BND_E, const int32_t *BND_S, const int32_t *BND_N,
     const double *dtrk, const int32_t *last);
 
+/* One RK stage of the fully explicit HEVE short step.  Same dummies as
+ * scale_hip_tstep_short_hevi (the interface `short`,
+ * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_short.F90:38-128),
+ * implemented by ATMOS_DYN_Tstep_short_fvm_heve,
+ * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1516.
+ * FLAG_FCT_MOMENTUM and FLAG_FCT_T must be .false.: their FCT branches
+ * (HEVE.F90:386, 716-810, 958-1057, 1207-1309, 1388-1400) are not built and
+ * the call fails with an error instead. */
+int scale_hip_tstep_short_heve(scale_hip_ctx *ctx,
+    double *DENS_RK, double *MOMZ_RK, double *MOMX_RK, double *MOMY_RK, double *RHOT_RK,
+    double *PROG_RK, double *mflx_hi, double *tflx_hi,
+    const double *DENS0, const double *MOMZ0, const double *MOMX0, const double *MOMY0, const double *RHOT0,
+    const double *DENS, const double *MOMZ, const double *MOMX, const double *MOMY, const double *RHOT,
+    const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
+    const double *PROG0, const double *PROG,
+    const double *DPRES0, const double *RT2P, const double *CORIOLI,
+    const double *num_diff, const double *wdamp_coef, const double *divdmp_coef, const double *DDIV,
+    const int32_t *FLAG_FCT_MOMENTUM, const int32_t *FLAG_FCT_T, const int32_t *FLAG_FCT_ALONG_STREAM,
+    const double *CDZ, const double *FDZ, const double *FDX, const double *FDY,
+    const double *RCDZ, const double *RCDX, const double *RCDY, const double *RFDZ, const double *RFDX, const double *RFDY,
+    const double *PHI, const double *GSQRT, const double *J13G, const double *J23G, const double *J33G, const double *MAPF,
+    const double *REF_dens, const double *REF_rhot,
+    const int32_t *BND_W, const int32_t *BND_E, const int32_t *BND_S, const int32_t *BND_N,
+    const double *dtrk, const int32_t *last);
+
 /* One RK stage of flux-form tracer advection with optional FCT.  Same dummies
  * as the abstract interface `step` of ATMOS_DYN_Tstep_tracer,
  * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_tracer.F90:34-64,

@@ -121,6 +121,28 @@ def tstep_short_hevi(case, inp: dict, dtrk: float, scheme: int | None = None, ou
     return out
 
 
+def tstep_short_heve(case, inp: dict, dtrk: float, scheme: int | None = None, out: dict | None = None,
+                     fct_momentum=False, fct_t=False) -> dict:
+    """One HEVE stage (scale_atmos_dyn_tstep_short_fvm_heve.F90), same inputs as tstep_short_hevi."""
+    g = case.grid
+    if out is None:
+        out = {k: nan_like_outputs(g) for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK")}
+        out["mflx_hi"] = nan_like_outputs(g, 3)
+        out["tflx_hi"] = nan_like_outputs(g, 3)
+    gs, ms = grid_struct(g), metric_struct(case.metric)
+    rc = lib().orc_tstep_short_heve(
+        C.byref(gs), C.c_int(scheme if scheme is not None else case.scheme),
+        *[ptr(out[k]) for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK", "mflx_hi", "tflx_hi")],
+        *[ptr(inp[k]) for k in ("DENS0", "MOMZ0", "MOMX0", "MOMY0", "RHOT0", "DENS", "MOMZ", "MOMX", "MOMY", "RHOT",
+                                "DENS_t", "MOMZ_t", "MOMX_t", "MOMY_t", "RHOT_t", "DPRES0", "RT2P")],
+        ptr(inp.get("CORIOLI", case.corioli)), ptr(inp["num_diff"]), ptr(inp.get("wdamp_coef", case.wdamp_coef)),
+        C.c_double(inp.get("divdmp_coef", 0.0)), ptr(inp["DDIV"]), C.c_int(int(fct_momentum)), C.c_int(int(fct_t)),
+        C.byref(ms), ptr(inp["REF_dens"]), ptr(inp["REF_rhot"]), C.c_double(dtrk))
+    if rc != 0:
+        raise RuntimeError(f"orc_tstep_short_heve failed: {rc}")
+    return out
+
+
 def numfilter_coef(case, state: dict, dt: float, nd_coef=1.0e-4, nd_sfc_fact=1.0) -> np.ndarray:
     g = case.grid
     nd = np.full((g.KA, g.IA, g.JA, 5, 3), np.nan, order="F")

@@ -733,6 +733,242 @@ int orc_tstep_short_hevi(const orc_grid *g, int s,
 #undef F2HH
 }
 
+/* ---- HEVE stage (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1401) -----
+ * The fully explicit short step, FLAG_FCT_MOMENTUM = FLAG_FCT_T = .false.
+ * (the FCT branches return -1).  HEVE = scalelib/src/atmos-rm/dynamics/
+ * scale_atmos_dyn_tstep_short_fvm_heve.F90.  The flux routines are the ones
+ * the HEVI stage uses; the differences are the explicit vertical mass flux,
+ * the explicit pressure-gradient and buoyancy terms of MOMZ and the
+ * divergence-damping forms. */
+int orc_tstep_short_heve(const orc_grid *g, int s,
+    double *DENS_RK, double *MOMZ_RK, double *MOMX_RK, double *MOMY_RK, double *RHOT_RK,
+    double *mflx_hi, double *tflx_hi,
+    const double *DENS0, const double *MOMZ0, const double *MOMX0, const double *MOMY0, const double *RHOT0,
+    const double *DENS, const double *MOMZ, const double *MOMX, const double *MOMY, const double *RHOT,
+    const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
+    const double *DPRES0, const double *RT2P, const double *CORIOLI,
+    const double *num_diff, const double *wdamp_coef, double divdmp_coef, const double *DDIV,
+    int flag_fct_momentum, int flag_fct_t,
+    const orc_metric *m, const double *REF_dens, const double *REF_rhot, double dtrk) {
+  GRID_LOCALS;
+  if ((s != ORC_UD3 && s != ORC_CD4) || flag_fct_momentum || flag_fct_t) return -1;
+  const double *CDZ = m->CDZ, *FDZ = m->FDZ, *FDX = m->FDX, *FDY = m->FDY;
+  const double *RCDZ = m->RCDZ, *RCDX = m->RCDX, *RCDY = m->RCDY;
+  const double *RFDZ = m->RFDZ, *RFDX = m->RFDX, *RFDY = m->RFDY;
+  const double *PHI = m->PHI;
+  const double J33G = m->J33G;
+  const double *GS = m->GSQRT, *J13 = m->J13G, *J23 = m->J23G, *MP = m->MAPF;
+#define GSQRT(k, i, j, c) GS[I3(k, i, j) + N3 * (c)]
+#define J13G(k, i, j, c) J13[I3(k, i, j) + N3 * (c)]
+#define J23G(k, i, j, c) J23[I3(k, i, j) + N3 * (c)]
+#define MAPF(i, j, n, c) MP[(size_t)(i) + (size_t)IA * ((size_t)(j) + (size_t)JA * ((size_t)(n) + 2 * (size_t)(c)))]
+#define ND(k, i, j, v, d) num_diff[I3(k, i, j) + N3 * ((size_t)(v) + 5 * (size_t)(d))]
+#define F2HH(k, p, c) (CDZ[(k) + (p)-1] * GSQRT((k) + (p)-1, i, j, c) / (CDZ[k] * GSQRT(k, i, j, c) + CDZ[(k) + 1] * GSQRT((k) + 1, i, j, c)))
+  const size_t n3 = N3;
+  double *mfZ = mflx_hi, *mfX = mflx_hi + n3, *mfY = mflx_hi + 2 * n3;
+  double *tfZ = tflx_hi, *tfX = tflx_hi + n3, *tfY = tflx_hi + 2 * n3;
+  double *POTT = alloc_nan(n3), *DPRES = alloc_nan(n3);
+  double *qflx_hi = alloc_nan(3 * n3), *qflx_J13 = alloc_nan(n3), *qflx_J23 = alloc_nan(n3);
+  double *qZ = qflx_hi, *qX = qflx_hi + n3, *qY = qflx_hi + 2 * n3;
+  const int IFS_OFF = 1, JFS_OFF = 1;
+  const int IEH = IE, JEH = JE;
+  const int IIS = IS, IIE = IE, JJS = JS, JJE = JE;
+
+  /* DPRES and POTT (HEVE.F90:343-366) */
+  OMP_FOR
+  for (int j = JJS - 2; j <= JJE + 2; ++j)
+    for (int i = IIS - 2; i <= IIE + 2; ++i) {
+      for (int k = KS; k <= KE; ++k) DPRES[I3(k, i, j)] = DPRES0[I3(k, i, j)] + RT2P[I3(k, i, j)] * (RHOT[I3(k, i, j)] - REF_rhot[I3(k, i, j)]);
+      DPRES[I3(KS - 1, i, j)] = DPRES0[I3(KS - 1, i, j)] - DENS[I3(KS, i, j)] * (PHI[I3(KS - 1, i, j)] - PHI[I3(KS + 1, i, j)]);
+      DPRES[I3(KE + 1, i, j)] = DPRES0[I3(KE + 1, i, j)] - DENS[I3(KE, i, j)] * (PHI[I3(KE + 1, i, j)] - PHI[I3(KE - 1, i, j)]);
+      for (int k = KS; k <= KE; ++k) POTT[I3(k, i, j)] = RHOT[I3(k, i, j)] / DENS[I3(k, i, j)];
+    }
+
+  /* mass fluxes (HEVE.F90:452-520) */
+  OMP_FOR
+  for (int j = JJS; j <= JJE; ++j)
+    for (int i = IIS; i <= IIE; ++i) {
+      for (int k = KS; k <= KE - 1; ++k)
+        mfZ[I3(k, i, j)] = J33G * MOMZ[I3(k, i, j)] / (MAPF(i, j, 0, I_XY) * MAPF(i, j, 1, I_XY))
+                         + J13G(k, i, j, I_XYW) * 0.25 * (MOMX[I3(k + 1, i, j)] + MOMX[I3(k + 1, i - 1, j)]
+                                                        + MOMX[I3(k, i, j)] + MOMX[I3(k, i - 1, j)])
+                           / MAPF(i, j, 1, I_XY)
+                         + J23G(k, i, j, I_XYW) * 0.25 * (MOMY[I3(k + 1, i, j)] + MOMY[I3(k + 1, i, j - 1)]
+                                                        + MOMY[I3(k, i, j)] + MOMY[I3(k, i, j - 1)])
+                           / MAPF(i, j, 0, I_XY)
+                         + GSQRT(k, i, j, I_XYW) * ND(k, i, j, I_DENS, ZDIR) / (MAPF(i, j, 0, I_XY) * MAPF(i, j, 1, I_XY));
+      mfZ[I3(KS - 1, i, j)] = 0.0;
+      mfZ[I3(KE, i, j)] = 0.0;
+    }
+  {
+    int iss = IIS - IFS_OFF, iee = IIE < IEH ? IIE : IEH;
+    OMP_FOR
+    for (int j = JJS; j <= JJE; ++j)
+      for (int i = iss; i <= iee; ++i)
+        for (int k = KS; k <= KE; ++k)
+          mfX[I3(k, i, j)] = GSQRT(k, i, j, I_UYZ) / MAPF(i, j, 1, I_UY) * (MOMX[I3(k, i, j)] + ND(k, i, j, I_DENS, XDIR));
+  }
+  {
+    int jss = JJS - JFS_OFF, jee = JJE < JEH ? JJE : JEH;
+    OMP_FOR
+    for (int j = jss; j <= jee; ++j)
+      for (int i = IIS; i <= IIE; ++i)
+        for (int k = KS; k <= KE; ++k)
+          mfY[I3(k, i, j)] = GSQRT(k, i, j, I_XVZ) / MAPF(i, j, 0, I_XV) * (MOMY[I3(k, i, j)] + ND(k, i, j, I_DENS, YDIR));
+  }
+  /* DENS_RK (HEVE.F90:547-572) */
+  OMP_FOR
+  for (int j = JJS; j <= JJE; ++j)
+    for (int i = IIS; i <= IIE; ++i)
+      for (int k = KS; k <= KE; ++k) {
+        double advcv = -(mfZ[I3(k, i, j)] - mfZ[I3(k - 1, i, j)]) * RCDZ[k];
+        double advch = -(mfX[I3(k, i, j)] - mfX[I3(k, i - 1, j)]) * RCDX[i]
+                       - (mfY[I3(k, i, j)] - mfY[I3(k, i, j - 1)]) * RCDY[j];
+        DENS_RK[I3(k, i, j)] = DENS0[I3(k, i, j)]
+                             + dtrk * ((advcv + advch) * MAPF(i, j, 0, I_XY) * MAPF(i, j, 1, I_XY) / GSQRT(k, i, j, I_XYZ)
+                                       + DENS_t[I3(k, i, j)]);
+      }
+
+  /* MOMZ (HEVE.F90:588-700) */
+  fluxZ_XYW(g, s, qZ, MOMZ, MOMZ, DENS, GS + n3 * I_XYZ, J33G, num_diff + n3 * (I_MOMZ + 5 * ZDIR), IIS, IIE, JJS, JJE);
+  fluxJ_XYW(g, s, 0, qflx_J13, MOMX, MOMZ, DENS, J13 + n3 * I_XYZ, MP + (size_t)IA * JA * 2 * I_XY, IIS, IIE, JJS, JJE);
+  fluxJ_XYW(g, s, 1, qflx_J23, MOMY, MOMZ, DENS, J23 + n3 * I_XYZ, MP + (size_t)IA * JA * 2 * I_XY, IIS, IIE, JJS, JJE);
+  fluxH_XYW(g, s, 0, qX, MOMX, MOMZ, DENS, GS + n3 * I_UYW, MP + (size_t)IA * JA * 2 * I_UY, num_diff + n3 * (I_MOMZ + 5 * XDIR), CDZ, IIS, IIE, JJS, JJE);
+  fluxH_XYW(g, s, 1, qY, MOMY, MOMZ, DENS, GS + n3 * I_XVW, MP + (size_t)IA * JA * 2 * I_XV, num_diff + n3 * (I_MOMZ + 5 * YDIR), CDZ, IIS, IIE, JJS, JJE);
+  OMP_FOR
+  for (int j = JJS; j <= JJE; ++j)
+    for (int i = IIS; i <= IIE; ++i) {
+      for (int k = KS; k <= KE - 1; ++k) {
+        const double pgf = J33G * (DPRES[I3(k + 1, i, j)] - DPRES[I3(k, i, j)]) * RFDZ[k];
+        const double buoy = GRAV * GSQRT(k, i, j, I_XYW)
+                          * (F2HH(k, 1, I_XYZ) * (DENS[I3(k + 1, i, j)] - REF_dens[I3(k + 1, i, j)])
+                           + F2HH(k, 2, I_XYZ) * (DENS[I3(k, i, j)] - REF_dens[I3(k, i, j)]));
+        double advcv = -(qZ[I3(k, i, j)] - qZ[I3(k - 1, i, j)]
+                       + qflx_J13[I3(k, i, j)] - qflx_J13[I3(k - 1, i, j)]
+                       + qflx_J23[I3(k, i, j)] - qflx_J23[I3(k - 1, i, j)]) * RFDZ[k];
+        double advch = -((qX[I3(k, i, j)] - qX[I3(k, i - 1, j)]) * RCDX[i]
+                       + (qY[I3(k, i, j)] - qY[I3(k, i, j - 1)]) * RCDY[j])
+                     * MAPF(i, j, 0, I_XY) * MAPF(i, j, 1, I_XY);
+        double wdamp = -wdamp_coef[k] * MOMZ0[I3(k, i, j)];
+        double div = divdmp_coef / dtrk * FDZ[k] * (DDIV[I3(k + 1, i, j)] - DDIV[I3(k, i, j)]);
+        MOMZ_RK[I3(k, i, j)] = MOMZ0[I3(k, i, j)]
+                             + dtrk * ((advcv + advch - pgf - buoy) / GSQRT(k, i, j, I_XYW) + wdamp + div + MOMZ_t[I3(k, i, j)]);
+      }
+      MOMZ_RK[I3(KS - 1, i, j)] = 0.0;
+      MOMZ_RK[I3(KE, i, j)] = 0.0;
+    }
+
+  /* MOMX (HEVE.F90:744-945) */
+  fluxZ_UVZ(g, s, 0, qZ, MOMZ, MOMX, DENS, GS + n3 * I_UYW, J33G, num_diff + n3 * (I_MOMX + 5 * ZDIR), CDZ, IIS, IIE, JJS, JJE);
+  fluxJ_UVZ(g, s, 0, 0, qflx_J13, MOMX, MOMX, DENS, GS + n3 * I_UYZ, J13 + n3 * I_UYW, MP + (size_t)IA * JA * 2 * I_UY, CDZ, IIS, IIE, JJS, JJE);
+  fluxJ_UVZ(g, s, 0, 1, qflx_J23, MOMY, MOMX, DENS, GS + n3 * I_UYZ, J23 + n3 * I_UYW, MP + (size_t)IA * JA * 2 * I_UY, CDZ, IIS, IIE, JJS, JJE);
+  fluxX_UYZ(g, s, qX, MOMX, MOMX, DENS, GS + n3 * I_XYZ, MP + (size_t)IA * JA * 2 * I_XY, num_diff + n3 * (I_MOMX + 5 * XDIR), IIS, IIE, JJS, JJE);
+  fluxY_UYZ(g, s, qY, MOMY, MOMX, DENS, GS + n3 * I_UVZ, MP + (size_t)IA * JA * 2 * I_UV, num_diff + n3 * (I_MOMX + 5 * YDIR), IIS, IIE, JJS, JJE);
+  {
+    int iee = IIE < IEH ? IIE : IEH;
+    OMP_FOR
+    for (int j = JJS; j <= JJE; ++j)
+      for (int i = IIS; i <= iee; ++i)
+        for (int k = KS; k <= KE; ++k) {
+          double advcv = -(qZ[I3(k, i, j)] - qZ[I3(k - 1, i, j)]
+                         + qflx_J13[I3(k, i, j)] - qflx_J13[I3(k - 1, i, j)]
+                         + qflx_J23[I3(k, i, j)] - qflx_J23[I3(k - 1, i, j)]) * RCDZ[k];
+          double advch = -((qX[I3(k, i, j)] - qX[I3(k, i - 1, j)]) * RFDX[i]
+                         + (qY[I3(k, i, j)] - qY[I3(k, i, j - 1)]) * RCDY[j])
+                       * MAPF(i, j, 0, I_UY) * MAPF(i, j, 1, I_UY);
+          double pgf = ((GSQRT(k, i + 1, j, I_XYZ) * DPRES[I3(k, i + 1, j)]
+                       - GSQRT(k, i, j, I_XYZ) * DPRES[I3(k, i, j)]) * RFDX[i]
+                      + (J13G(k, i, j, I_UYW)
+                         * 0.5 * (F2HH(k, 1, I_UYZ) * (DPRES[I3(k + 1, i + 1, j)] + DPRES[I3(k + 1, i, j)])
+                                + F2HH(k, 2, I_UYZ) * (DPRES[I3(k, i + 1, j)] + DPRES[I3(k, i, j)]))
+                       - J13G(k - 1, i, j, I_UYW)
+                         * 0.5 * (F2HH(k, 1, I_UYZ) * (DPRES[I3(k, i + 1, j)] + DPRES[I3(k, i, j)])
+                                + F2HH(k, 2, I_UYZ) * (DPRES[I3(k - 1, i + 1, j)] + DPRES[I3(k - 1, i, j)])))
+                        * RCDZ[k])
+                     * MAPF(i, j, 0, I_UY);
+          double cor = 0.125 * (CORIOLI[(size_t)(i + 1) + (size_t)IA * j] + CORIOLI[(size_t)i + (size_t)IA * j])
+                         * (MOMY[I3(k, i + 1, j)] + MOMY[I3(k, i, j)] + MOMY[I3(k, i + 1, j - 1)] + MOMY[I3(k, i, j - 1)])
+                     + 0.25 * MAPF(i, j, 0, I_UY) * MAPF(i, j, 1, I_UY)
+                         * (MOMY[I3(k, i, j)] + MOMY[I3(k, i, j - 1)] + MOMY[I3(k, i + 1, j)] + MOMY[I3(k, i + 1, j - 1)])
+                         * ((MOMY[I3(k, i, j)] + MOMY[I3(k, i, j - 1)] + MOMY[I3(k, i + 1, j)] + MOMY[I3(k, i + 1, j - 1)]) * 0.25
+                              * (1.0 / MAPF(i + 1, j, 1, I_XY) - 1.0 / MAPF(i, j, 1, I_XY)) * RFDX[i]
+                            - MOMX[I3(k, i, j)]
+                              * (1.0 / MAPF(i, j, 0, I_UV) - 1.0 / MAPF(i, j - 1, 0, I_UV)) * RCDY[j])
+                         * 2.0 / (DENS[I3(k, i + 1, j)] + DENS[I3(k, i, j)]);
+          double div = divdmp_coef / dtrk * FDX[i] * (DDIV[I3(k, i + 1, j)] - DDIV[I3(k, i, j)]);
+          MOMX_RK[I3(k, i, j)] = MOMX0[I3(k, i, j)]
+                               + dtrk * ((advcv + advch - pgf) / GSQRT(k, i, j, I_UYZ) + cor + div + MOMX_t[I3(k, i, j)]);
+        }
+  }
+
+  /* MOMY (HEVE.F90:993-1193) */
+  fluxZ_UVZ(g, s, 1, qZ, MOMZ, MOMY, DENS, GS + n3 * I_XVW, J33G, num_diff + n3 * (I_MOMY + 5 * ZDIR), CDZ, IIS, IIE, JJS, JJE);
+  fluxJ_UVZ(g, s, 1, 0, qflx_J13, MOMX, MOMY, DENS, GS + n3 * I_XVZ, J13 + n3 * I_XVW, MP + (size_t)IA * JA * 2 * I_XV, CDZ, IIS, IIE, JJS, JJE);
+  fluxJ_UVZ(g, s, 1, 1, qflx_J23, MOMY, MOMY, DENS, GS + n3 * I_XVZ, J23 + n3 * I_XVW, MP + (size_t)IA * JA * 2 * I_XV, CDZ, IIS, IIE, JJS, JJE);
+  fluxX_XVZ(g, s, qX, MOMX, MOMY, DENS, GS + n3 * I_UVZ, MP + (size_t)IA * JA * 2 * I_UV, num_diff + n3 * (I_MOMY + 5 * XDIR), IIS, IIE, JJS, JJE);
+  fluxY_XVZ(g, s, qY, MOMY, MOMY, DENS, GS + n3 * I_XYZ, MP + (size_t)IA * JA * 2 * I_XY, num_diff + n3 * (I_MOMY + 5 * YDIR), IIS, IIE, JJS, JJE);
+  {
+    int jee = JJE < JEH ? JJE : JEH;
+    OMP_FOR
+    for (int j = JJS; j <= jee; ++j)
+      for (int i = IIS; i <= IIE; ++i)
+        for (int k = KS; k <= KE; ++k) {
+          double advcv = -(qZ[I3(k, i, j)] - qZ[I3(k - 1, i, j)]
+                         + qflx_J13[I3(k, i, j)] - qflx_J13[I3(k - 1, i, j)]
+                         + qflx_J23[I3(k, i, j)] - qflx_J23[I3(k - 1, i, j)]) * RCDZ[k];
+          double advch = -((qX[I3(k, i, j)] - qX[I3(k, i - 1, j)]) * RCDX[i]
+                         + (qY[I3(k, i, j)] - qY[I3(k, i, j - 1)]) * RFDY[j])
+                       * MAPF(i, j, 0, I_XV) * MAPF(i, j, 1, I_XV);
+          double pgf = ((GSQRT(k, i, j + 1, I_XYZ) * DPRES[I3(k, i, j + 1)]
+                       - GSQRT(k, i, j, I_XYZ) * DPRES[I3(k, i, j)]) * RFDY[j]
+                      + (J23G(k, i, j, I_XVW)
+                         * 0.5 * (F2HH(k, 1, I_XVZ) * (DPRES[I3(k + 1, i, j + 1)] + DPRES[I3(k + 1, i, j)])
+                                + F2HH(k, 2, I_XVZ) * (DPRES[I3(k, i, j + 1)] + DPRES[I3(k, i, j)]))
+                       - J23G(k - 1, i, j, I_XVW)
+                         * 0.5 * (F2HH(k - 1, 1, I_XVZ) * (DPRES[I3(k, i, j + 1)] + DPRES[I3(k, i, j)])
+                                + F2HH(k - 1, 2, I_XVZ) * (DPRES[I3(k - 1, i, j + 1)] + DPRES[I3(k - 1, i, j)])))
+                        * RCDZ[k])
+                     * MAPF(i, j, 1, I_XV);
+          double cor = -0.125 * (CORIOLI[(size_t)i + (size_t)IA * (j + 1)] + CORIOLI[(size_t)i + (size_t)IA * j])
+                         * (MOMX[I3(k, i, j + 1)] + MOMX[I3(k, i, j)] + MOMX[I3(k, i - 1, j + 1)] + MOMX[I3(k, i - 1, j)])
+                     - 0.25 * MAPF(i, j, 0, I_XV) * MAPF(i, j, 1, I_XV)
+                         * (MOMX[I3(k, i, j)] + MOMX[I3(k, i - 1, j)] + MOMX[I3(k, i, j + 1)] + MOMX[I3(k, i - 1, j + 1)])
+                         * (MOMY[I3(k, i, j)]
+                              * (1.0 / MAPF(i, j, 1, I_UV) - 1.0 / MAPF(i - 1, j, 1, I_UV)) * RCDX[i]
+                            - 0.25 * (MOMX[I3(k, i, j)] + MOMX[I3(k, i - 1, j)] + MOMX[I3(k, i, j + 1)] + MOMX[I3(k, i - 1, j + 1)])
+                              * (1.0 / MAPF(i, j + 1, 0, I_XY) - 1.0 / MAPF(i, j, 0, I_XY)) * RFDY[j])
+                         * 2.0 / (DENS[I3(k, i, j)] + DENS[I3(k, i, j + 1)]);
+          double div = divdmp_coef / dtrk * FDY[j] * (DDIV[I3(k, i, j + 1)] - DDIV[I3(k, i, j)]);
+          MOMY_RK[I3(k, i, j)] = MOMY0[I3(k, i, j)]
+                               + dtrk * ((advcv + advch - pgf) / GSQRT(k, i, j, I_XVZ) + cor + div + MOMY_t[I3(k, i, j)]);
+        }
+  }
+
+  /* RHOT (HEVE.F90:1316-1375) */
+  fluxZ_XYZ(g, s, tfZ, mfZ, POTT, GS + n3 * I_XYW, num_diff + n3 * (I_RHOT + 5 * ZDIR), IIS, IIE, JJS, JJE);
+  fluxX_XYZ(g, s, tfX, mfX, POTT, GS + n3 * I_UYZ, num_diff + n3 * (I_RHOT + 5 * XDIR), IIS, IIE, JJS, JJE);
+  fluxY_XYZ(g, s, tfY, mfY, POTT, GS + n3 * I_XVZ, num_diff + n3 * (I_RHOT + 5 * YDIR), IIS, IIE, JJS, JJE);
+  OMP_FOR
+  for (int j = JJS; j <= JJE; ++j)
+    for (int i = IIS; i <= IIE; ++i)
+      for (int k = KS; k <= KE; ++k) {
+        double advcv = -(tfZ[I3(k, i, j)] - tfZ[I3(k - 1, i, j)]) * RCDZ[k];
+        double advch = -(tfX[I3(k, i, j)] - tfX[I3(k, i - 1, j)]) * RCDX[i]
+                       - (tfY[I3(k, i, j)] - tfY[I3(k, i, j - 1)]) * RCDY[j];
+        RHOT_RK[I3(k, i, j)] = RHOT0[I3(k, i, j)]
+                             + dtrk * ((advcv + advch) * MAPF(i, j, 0, I_XY) * MAPF(i, j, 1, I_XY) / GSQRT(k, i, j, I_XYZ)
+                                       + RHOT_t[I3(k, i, j)]);
+      }
+  free(POTT); free(DPRES); free(qflx_hi); free(qflx_J13); free(qflx_J23);
+  return 0;
+#undef GSQRT
+#undef J13G
+#undef J23G
+#undef MAPF
+#undef ND
+#undef F2HH
+}
+
 /* ---- periodic halo fill (COMM_vars8 on one periodic rank) ---------------- */
 void orc_halo_periodic(const orc_grid *g, double *a) {
   GRID_LOCALS;

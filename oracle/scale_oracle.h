@@ -68,6 +68,20 @@ int orc_tstep_short_hevi(const orc_grid *g, int scheme,
     const orc_metric *m, const double *REF_dens, const double *REF_rhot,
     double dtrk);
 
+/* ATMOS_DYN_Tstep_short_fvm_heve (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1401)
+ * without FCT (flag_fct_momentum = flag_fct_t = 0; otherwise -1), periodic
+ * lateral boundaries. */
+int orc_tstep_short_heve(const orc_grid *g, int scheme,
+    double *DENS_RK, double *MOMZ_RK, double *MOMX_RK, double *MOMY_RK, double *RHOT_RK,
+    double *mflx_hi, double *tflx_hi,
+    const double *DENS0, const double *MOMZ0, const double *MOMX0, const double *MOMY0, const double *RHOT0,
+    const double *DENS, const double *MOMZ, const double *MOMX, const double *MOMY, const double *RHOT,
+    const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
+    const double *DPRES0, const double *RT2P, const double *CORIOLI,
+    const double *num_diff, const double *wdamp_coef, double divdmp_coef, const double *DDIV,
+    int flag_fct_momentum, int flag_fct_t,
+    const orc_metric *m, const double *REF_dens, const double *REF_rhot, double dtrk);
+
 /* ATMOS_DYN_numfilter_coef (COMMON.F90:385-985), ND_ORDER = 1 only,
  * followed by the periodic halo fill of the 15 num_diff fields. */
 int orc_numfilter_coef(const orc_grid *g, double *num_diff,

@@ -361,7 +361,7 @@ int scale_hip_stream(scale_hip_ctx* h, void** stream) {
   return 0;
 }
 
-int scale_hip_tstep_short_hevi(scale_hip_ctx* h,
+static int short_stage(bool heve, scale_hip_ctx* h,
     double* DENS_RK, double* MOMZ_RK, double* MOMX_RK, double* MOMY_RK, double* RHOT_RK,
     double* PROG_RK, double* mflx_hi, double* tflx_hi,
     const double* DENS0, const double* MOMZ0, const double* MOMX0, const double* MOMY0, const double* RHOT0,
@@ -377,9 +377,12 @@ int scale_hip_tstep_short_hevi(scale_hip_ctx* h,
     const double* REF_dens, const double* REF_rhot,
     const int32_t* BND_W, const int32_t* BND_E, const int32_t* BND_S, const int32_t* BND_N,
     const double* dtrk, const int32_t* last) {
-  (void)PROG_RK; (void)PROG0; (void)PROG; (void)FLAG_FCT_MOMENTUM; (void)FLAG_FCT_T; (void)FLAG_FCT_ALONG_STREAM; (void)last;
+  (void)PROG_RK; (void)PROG0; (void)PROG; (void)FLAG_FCT_ALONG_STREAM; (void)last;
   if (int r = check_ctx(h)) return r;
   if (!dtrk || !divdmp_coef || !J33G) return fail("null scalar argument");
+  // HEVI ignores the FCT flags (HEVI.F90:224-226); HEVE's FCT branches are not built
+  if (heve && ((FLAG_FCT_MOMENTUM && *FLAG_FCT_MOMENTUM) || (FLAG_FCT_T && *FLAG_FCT_T)))
+    return fail("HEVE with FLAG_FCT_MOMENTUM or FLAG_FCT_T is not supported");
   if ((BND_W && *BND_W) || (BND_E && *BND_E) || (BND_S && *BND_S) || (BND_N && *BND_N))
     return fail("only periodic lateral boundaries (BND_* = .false.) are supported");
   Ctx& c = h->c;
@@ -415,12 +418,58 @@ int scale_hip_tstep_short_hevi(scale_hip_ctx* h,
   io.DPRES0 = B[B_DPRES0]; io.RT2P = B[B_RT2P]; io.CORIOLI = B[B_COR]; io.num_diff = B[B_ND]; io.wdamp_coef = B[B_WD];
   io.DDIV = B[B_DDIV]; io.REF_dens = B[B_REFD]; io.REF_rhot = B[B_REFR];
   io.divdmp_coef = *divdmp_coef; io.dtrk = *dtrk; io.has_coriolis = has_cor;
-  CK(shp::hevi_stage(io, c.stream));
+  CK(heve ? shp::heve_stage(io, c.stream) : shp::hevi_stage(io, c.stream));
   for (int v = 0; v < 5; ++v)
     if (int r = download3(c, o[v], B[B_OUT + v], 1)) return r;
   if (int r = download3(c, mflx_hi, B[B_MF], 3)) return r;
   if (int r = download3(c, tflx_hi, B[B_TF], 3)) return r;
   return 0;
+}
+
+int scale_hip_tstep_short_hevi(scale_hip_ctx* h,
+    double* DENS_RK, double* MOMZ_RK, double* MOMX_RK, double* MOMY_RK, double* RHOT_RK,
+    double* PROG_RK, double* mflx_hi, double* tflx_hi,
+    const double* DENS0, const double* MOMZ0, const double* MOMX0, const double* MOMY0, const double* RHOT0,
+    const double* DENS, const double* MOMZ, const double* MOMX, const double* MOMY, const double* RHOT,
+    const double* DENS_t, const double* MOMZ_t, const double* MOMX_t, const double* MOMY_t, const double* RHOT_t,
+    const double* PROG0, const double* PROG,
+    const double* DPRES0, const double* RT2P, const double* CORIOLI,
+    const double* num_diff, const double* wdamp_coef, const double* divdmp_coef, const double* DDIV,
+    const int32_t* FLAG_FCT_MOMENTUM, const int32_t* FLAG_FCT_T, const int32_t* FLAG_FCT_ALONG_STREAM,
+    const double* CDZ, const double* FDZ, const double* FDX, const double* FDY,
+    const double* RCDZ, const double* RCDX, const double* RCDY, const double* RFDZ, const double* RFDX, const double* RFDY,
+    const double* PHI, const double* GSQRT, const double* J13G, const double* J23G, const double* J33G, const double* MAPF,
+    const double* REF_dens, const double* REF_rhot,
+    const int32_t* BND_W, const int32_t* BND_E, const int32_t* BND_S, const int32_t* BND_N,
+    const double* dtrk, const int32_t* last) {
+  return short_stage(false, h, DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, PROG_RK, mflx_hi, tflx_hi, DENS0, MOMZ0,
+      MOMX0, MOMY0, RHOT0, DENS, MOMZ, MOMX, MOMY, RHOT, DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, PROG0, PROG, DPRES0,
+      RT2P, CORIOLI, num_diff, wdamp_coef, divdmp_coef, DDIV, FLAG_FCT_MOMENTUM, FLAG_FCT_T, FLAG_FCT_ALONG_STREAM,
+      CDZ, FDZ, FDX, FDY, RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY, PHI, GSQRT, J13G, J23G, J33G, MAPF, REF_dens, REF_rhot,
+      BND_W, BND_E, BND_S, BND_N, dtrk, last);
+}
+
+int scale_hip_tstep_short_heve(scale_hip_ctx* h,
+    double* DENS_RK, double* MOMZ_RK, double* MOMX_RK, double* MOMY_RK, double* RHOT_RK,
+    double* PROG_RK, double* mflx_hi, double* tflx_hi,
+    const double* DENS0, const double* MOMZ0, const double* MOMX0, const double* MOMY0, const double* RHOT0,
+    const double* DENS, const double* MOMZ, const double* MOMX, const double* MOMY, const double* RHOT,
+    const double* DENS_t, const double* MOMZ_t, const double* MOMX_t, const double* MOMY_t, const double* RHOT_t,
+    const double* PROG0, const double* PROG,
+    const double* DPRES0, const double* RT2P, const double* CORIOLI,
+    const double* num_diff, const double* wdamp_coef, const double* divdmp_coef, const double* DDIV,
+    const int32_t* FLAG_FCT_MOMENTUM, const int32_t* FLAG_FCT_T, const int32_t* FLAG_FCT_ALONG_STREAM,
+    const double* CDZ, const double* FDZ, const double* FDX, const double* FDY,
+    const double* RCDZ, const double* RCDX, const double* RCDY, const double* RFDZ, const double* RFDX, const double* RFDY,
+    const double* PHI, const double* GSQRT, const double* J13G, const double* J23G, const double* J33G, const double* MAPF,
+    const double* REF_dens, const double* REF_rhot,
+    const int32_t* BND_W, const int32_t* BND_E, const int32_t* BND_S, const int32_t* BND_N,
+    const double* dtrk, const int32_t* last) {
+  return short_stage(true, h, DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, PROG_RK, mflx_hi, tflx_hi, DENS0, MOMZ0,
+      MOMX0, MOMY0, RHOT0, DENS, MOMZ, MOMX, MOMY, RHOT, DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, PROG0, PROG, DPRES0,
+      RT2P, CORIOLI, num_diff, wdamp_coef, divdmp_coef, DDIV, FLAG_FCT_MOMENTUM, FLAG_FCT_T, FLAG_FCT_ALONG_STREAM,
+      CDZ, FDZ, FDX, FDY, RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY, PHI, GSQRT, J13G, J23G, J33G, MAPF, REF_dens, REF_rhot,
+      BND_W, BND_E, BND_S, BND_N, dtrk, last);
 }
 
 int scale_hip_tstep_short_hevi_adj(scale_hip_ctx* h,

@@ -324,6 +324,106 @@ struct ZCarry {
   double pw[4];
 };
 
+// MOMX and MOMY tendencies before divergence damping: advection, pressure
+// gradient (with its terrain part) and the Coriolis and metric terms, which
+// HEVI and HEVE share (HEVI.F90:816-927, 936-1047; HEVE.F90:744-945,
+// 993-1193).  Each carries its z fluxes from the level below in zc.
+template <int S, bool TERRAIN, bool DIVDMP>
+__device__ __forceinline__ double momx_core(const Ops<S, TERRAIN, DIVDMP>& o, const HeviArgs& a, int k, int i, int j,
+                                            ZCarry& zc) {
+  const Dims& d = a.d;
+  const size_t p = IX(d, k, i, j);
+  const double* RCDX = a.g.RCDX; const double* RCDY = a.g.RCDY; const double* RCDZ = a.g.RCDZ;
+  const double* RFDX = a.g.RFDX; const double* RFDY = a.g.RFDY;
+  const auto& m = o.m;
+  const double qu = o.qZ_UV(0, k, i, j);
+  double sv = qu - zc.qU;
+  zc.qU = qu;
+  if constexpr (TERRAIN) {
+    const double q13 = o.qJ_UV(0, 0, k, i, j), q23 = o.qJ_UV(0, 1, k, i, j);
+    sv = sv + q13 - zc.qUJ13 + q23 - zc.qUJ23;
+    zc.qUJ13 = q13; zc.qUJ23 = q23;
+  }
+  double advcv = -(sv) * RCDZ[k];
+  double advch = -((o.qX_UYZ(k, i, j) - o.qX_UYZ(k, i - 1, j)) * RFDX[i]
+                 + (o.qY_UYZ(k, i, j) - o.qY_UYZ(k, i, j - 1)) * RCDY[j])
+               * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY);
+  double pgh = (m.gs(I_XYZ, IX(d, k, i + 1, j)) * o.DPRES(k, i + 1, j) - m.gs(I_XYZ, p) * o.DPRES(k, i, j)) * RFDX[i];
+  if constexpr (TERRAIN) {
+    pgh = pgh + (m.j13(I_UYW, p)
+                   * 0.5 * (o.F2H(k, 1, I_UYZ, i, j) * (o.DPRES(k + 1, i + 1, j) + o.DPRES(k + 1, i, j))
+                          + o.F2H(k, 2, I_UYZ, i, j) * (o.DPRES(k, i + 1, j) + o.DPRES(k, i, j)))
+                 - m.j13(I_UYW, IX(d, k - 1, i, j))
+                   * 0.5 * (o.F2H(k, 1, I_UYZ, i, j) * (o.DPRES(k, i + 1, j) + o.DPRES(k, i, j))
+                          + o.F2H(k, 2, I_UYZ, i, j) * (o.DPRES(k - 1, i + 1, j) + o.DPRES(k - 1, i, j))))
+                  * RCDZ[k];
+  }
+  const double pg = pgh * m.mapf(i, j, 0, I_UY);
+  double t = (advcv + advch - pg) / m.gs(I_UYZ, p);
+  const double* Y = a.MOMY;
+  if (a.has_coriolis || TERRAIN) {
+    double cf = 0.125 * (a.CORIOLI[(size_t)(i + 1) + (size_t)d.IA * j] + a.CORIOLI[(size_t)i + (size_t)d.IA * j])
+              * (Y[IX(d, k, i + 1, j)] + Y[p] + Y[IX(d, k, i + 1, j - 1)] + Y[IX(d, k, i, j - 1)]);
+    if constexpr (TERRAIN) {
+      const double sy = Y[p] + Y[IX(d, k, i, j - 1)] + Y[IX(d, k, i + 1, j)] + Y[IX(d, k, i + 1, j - 1)];
+      cf = cf + 0.25 * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY) * sy
+              * (sy * 0.25 * (1.0 / m.mapf(i + 1, j, 1, I_XY) - 1.0 / m.mapf(i, j, 1, I_XY)) * RFDX[i]
+                 - a.MOMX[p] * (1.0 / m.mapf(i, j, 0, I_UV) - 1.0 / m.mapf(i, j - 1, 0, I_UV)) * RCDY[j])
+              * 2.0 / (o.D(k, i + 1, j) + o.D(k, i, j));
+    }
+    t = t + cf;
+  }
+  return t;
+}
+
+template <int S, bool TERRAIN, bool DIVDMP>
+__device__ __forceinline__ double momy_core(const Ops<S, TERRAIN, DIVDMP>& o, const HeviArgs& a, int k, int i, int j,
+                                            ZCarry& zc) {
+  const Dims& d = a.d;
+  const size_t p = IX(d, k, i, j);
+  const double* RCDX = a.g.RCDX; const double* RCDY = a.g.RCDY; const double* RCDZ = a.g.RCDZ;
+  const double* RFDX = a.g.RFDX; const double* RFDY = a.g.RFDY;
+  const auto& m = o.m;
+  const double qv = o.qZ_UV(1, k, i, j);
+  double sv = qv - zc.qV;
+  zc.qV = qv;
+  if constexpr (TERRAIN) {
+    const double q13 = o.qJ_UV(1, 0, k, i, j), q23 = o.qJ_UV(1, 1, k, i, j);
+    sv = sv + q13 - zc.qVJ13 + q23 - zc.qVJ23;
+    zc.qVJ13 = q13; zc.qVJ23 = q23;
+  }
+  double advcv = -(sv) * RCDZ[k];
+  double advch = -((o.qX_XVZ(k, i, j) - o.qX_XVZ(k, i - 1, j)) * RCDX[i]
+                 + (o.qY_XVZ(k, i, j) - o.qY_XVZ(k, i, j - 1)) * RFDY[j])
+               * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV);
+  double pgh = (m.gs(I_XYZ, IX(d, k, i, j + 1)) * o.DPRES(k, i, j + 1) - m.gs(I_XYZ, p) * o.DPRES(k, i, j)) * RFDY[j];
+  if constexpr (TERRAIN) {
+    pgh = pgh + (m.j23(I_XVW, p)
+                   * 0.5 * (o.F2H(k, 1, I_XVZ, i, j) * (o.DPRES(k + 1, i, j + 1) + o.DPRES(k + 1, i, j))
+                          + o.F2H(k, 2, I_XVZ, i, j) * (o.DPRES(k, i, j + 1) + o.DPRES(k, i, j)))
+                 - m.j23(I_XVW, IX(d, k - 1, i, j))
+                   * 0.5 * (o.F2H(k - 1, 1, I_XVZ, i, j) * (o.DPRES(k, i, j + 1) + o.DPRES(k, i, j))
+                          + o.F2H(k - 1, 2, I_XVZ, i, j) * (o.DPRES(k - 1, i, j + 1) + o.DPRES(k - 1, i, j))))
+                  * RCDZ[k];
+  }
+  const double pg = pgh * m.mapf(i, j, 1, I_XV);
+  double t = (advcv + advch - pg) / m.gs(I_XVZ, p);
+  const double* X = a.MOMX;
+  if (a.has_coriolis || TERRAIN) {
+    double cf = -0.125 * (a.CORIOLI[(size_t)i + (size_t)d.IA * (j + 1)] + a.CORIOLI[(size_t)i + (size_t)d.IA * j])
+              * (X[IX(d, k, i, j + 1)] + X[p] + X[IX(d, k, i - 1, j + 1)] + X[IX(d, k, i - 1, j)]);
+    if constexpr (TERRAIN) {
+      const double sx = X[p] + X[IX(d, k, i - 1, j)] + X[IX(d, k, i, j + 1)] + X[IX(d, k, i - 1, j + 1)];
+      cf = cf - 0.25 * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV) * sx
+              * (a.MOMY[p] * (1.0 / m.mapf(i, j, 1, I_UV) - 1.0 / m.mapf(i - 1, j, 1, I_UV)) * RCDX[i]
+                 - 0.25 * sx * (1.0 / m.mapf(i, j + 1, 0, I_XY) - 1.0 / m.mapf(i, j, 0, I_XY)) * RFDY[j])
+              * 2.0 / (o.D(k, i, j + 1) + o.D(k, i, j));
+    }
+    t = t + cf;
+  }
+  return t;
+}
+
 template <int S, bool TERRAIN, bool DIVDMP>
 __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP>& o, const HeviArgs& a, int k, int i, int j,
                                                     ZCarry& zc) {
@@ -384,43 +484,7 @@ __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP
 
   // ---- momentum x (HEVI.F90:816-927), iee = min(IIE,IEH) = IE for periodic x ----
   {
-    const double qu = o.qZ_UV(0, k, i, j);
-    double sv = qu - zc.qU;
-    zc.qU = qu;
-    if constexpr (TERRAIN) {
-      const double q13 = o.qJ_UV(0, 0, k, i, j), q23 = o.qJ_UV(0, 1, k, i, j);
-      sv = sv + q13 - zc.qUJ13 + q23 - zc.qUJ23;
-      zc.qUJ13 = q13; zc.qUJ23 = q23;
-    }
-    double advcv = -(sv) * RCDZ[k];
-    double advch = -((o.qX_UYZ(k, i, j) - o.qX_UYZ(k, i - 1, j)) * RFDX[i]
-                   + (o.qY_UYZ(k, i, j) - o.qY_UYZ(k, i, j - 1)) * RCDY[j])
-                 * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY);
-    double pgh = (m.gs(I_XYZ, IX(d, k, i + 1, j)) * o.DPRES(k, i + 1, j) - m.gs(I_XYZ, p) * o.DPRES(k, i, j)) * RFDX[i];
-    if constexpr (TERRAIN) {
-      pgh = pgh + (m.j13(I_UYW, p)
-                     * 0.5 * (o.F2H(k, 1, I_UYZ, i, j) * (o.DPRES(k + 1, i + 1, j) + o.DPRES(k + 1, i, j))
-                            + o.F2H(k, 2, I_UYZ, i, j) * (o.DPRES(k, i + 1, j) + o.DPRES(k, i, j)))
-                   - m.j13(I_UYW, IX(d, k - 1, i, j))
-                     * 0.5 * (o.F2H(k, 1, I_UYZ, i, j) * (o.DPRES(k, i + 1, j) + o.DPRES(k, i, j))
-                            + o.F2H(k, 2, I_UYZ, i, j) * (o.DPRES(k - 1, i + 1, j) + o.DPRES(k - 1, i, j))))
-                    * RCDZ[k];
-    }
-    const double pg = pgh * m.mapf(i, j, 0, I_UY);
-    double t = (advcv + advch - pg) / m.gs(I_UYZ, p);
-    const double* Y = a.MOMY;
-    if (a.has_coriolis || TERRAIN) {
-      double cf = 0.125 * (a.CORIOLI[(size_t)(i + 1) + (size_t)d.IA * j] + a.CORIOLI[(size_t)i + (size_t)d.IA * j])
-                * (Y[IX(d, k, i + 1, j)] + Y[p] + Y[IX(d, k, i + 1, j - 1)] + Y[IX(d, k, i, j - 1)]);
-      if constexpr (TERRAIN) {
-        const double sy = Y[p] + Y[IX(d, k, i, j - 1)] + Y[IX(d, k, i + 1, j)] + Y[IX(d, k, i + 1, j - 1)];
-        cf = cf + 0.25 * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY) * sy
-                * (sy * 0.25 * (1.0 / m.mapf(i + 1, j, 1, I_XY) - 1.0 / m.mapf(i, j, 1, I_XY)) * RFDX[i]
-                   - a.MOMX[p] * (1.0 / m.mapf(i, j, 0, I_UV) - 1.0 / m.mapf(i, j - 1, 0, I_UV)) * RCDY[j])
-                * 2.0 / (o.D(k, i + 1, j) + o.D(k, i, j));
-      }
-      t = t + cf;
-    }
+    double t = momx_core(o, a, k, i, j, zc);
     if constexpr (DIVDMP) {
       t = t + a.divdmp_coef / a.dtrk * (a.DDIV[IX(d, k, i + 1, j)] / m.mapf(i + 1, j, 1, I_XY) - a.DDIV[p] / m.mapf(i, j, 0, I_XY))
                 * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY) * a.g.FDX[i];
@@ -430,43 +494,7 @@ __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP
 
   // ---- momentum y (HEVI.F90:936-1047) ----
   {
-    const double qv = o.qZ_UV(1, k, i, j);
-    double sv = qv - zc.qV;
-    zc.qV = qv;
-    if constexpr (TERRAIN) {
-      const double q13 = o.qJ_UV(1, 0, k, i, j), q23 = o.qJ_UV(1, 1, k, i, j);
-      sv = sv + q13 - zc.qVJ13 + q23 - zc.qVJ23;
-      zc.qVJ13 = q13; zc.qVJ23 = q23;
-    }
-    double advcv = -(sv) * RCDZ[k];
-    double advch = -((o.qX_XVZ(k, i, j) - o.qX_XVZ(k, i - 1, j)) * RCDX[i]
-                   + (o.qY_XVZ(k, i, j) - o.qY_XVZ(k, i, j - 1)) * RFDY[j])
-                 * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV);
-    double pgh = (m.gs(I_XYZ, IX(d, k, i, j + 1)) * o.DPRES(k, i, j + 1) - m.gs(I_XYZ, p) * o.DPRES(k, i, j)) * RFDY[j];
-    if constexpr (TERRAIN) {
-      pgh = pgh + (m.j23(I_XVW, p)
-                     * 0.5 * (o.F2H(k, 1, I_XVZ, i, j) * (o.DPRES(k + 1, i, j + 1) + o.DPRES(k + 1, i, j))
-                            + o.F2H(k, 2, I_XVZ, i, j) * (o.DPRES(k, i, j + 1) + o.DPRES(k, i, j)))
-                   - m.j23(I_XVW, IX(d, k - 1, i, j))
-                     * 0.5 * (o.F2H(k - 1, 1, I_XVZ, i, j) * (o.DPRES(k, i, j + 1) + o.DPRES(k, i, j))
-                            + o.F2H(k - 1, 2, I_XVZ, i, j) * (o.DPRES(k - 1, i, j + 1) + o.DPRES(k - 1, i, j))))
-                    * RCDZ[k];
-    }
-    const double pg = pgh * m.mapf(i, j, 1, I_XV);
-    double t = (advcv + advch - pg) / m.gs(I_XVZ, p);
-    const double* X = a.MOMX;
-    if (a.has_coriolis || TERRAIN) {
-      double cf = -0.125 * (a.CORIOLI[(size_t)i + (size_t)d.IA * (j + 1)] + a.CORIOLI[(size_t)i + (size_t)d.IA * j])
-                * (X[IX(d, k, i, j + 1)] + X[p] + X[IX(d, k, i - 1, j + 1)] + X[IX(d, k, i - 1, j)]);
-      if constexpr (TERRAIN) {
-        const double sx = X[p] + X[IX(d, k, i - 1, j)] + X[IX(d, k, i, j + 1)] + X[IX(d, k, i - 1, j + 1)];
-        cf = cf - 0.25 * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV) * sx
-                * (a.MOMY[p] * (1.0 / m.mapf(i, j, 1, I_UV) - 1.0 / m.mapf(i - 1, j, 1, I_UV)) * RCDX[i]
-                   - 0.25 * sx * (1.0 / m.mapf(i, j + 1, 0, I_XY) - 1.0 / m.mapf(i, j, 0, I_XY)) * RFDY[j])
-                * 2.0 / (o.D(k, i, j + 1) + o.D(k, i, j));
-      }
-      t = t + cf;
-    }
+    double t = momy_core(o, a, k, i, j, zc);
     if constexpr (DIVDMP) {
       t = t + a.divdmp_coef / a.dtrk * (a.DDIV[IX(d, k, i, j + 1)] / m.mapf(i, j + 1, 0, I_XY) - a.DDIV[p] / m.mapf(i, j, 0, I_XY))
                 * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV) * a.g.FDY[j];
@@ -621,6 +649,124 @@ __global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles,
     advcv = -Cn * ptn * J33G * RCDZ[KS] / m.gs(I_XYZ, q);
     a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + St_[q]);
     a.MOMZ_RK[px(KS - 1)] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// HEVE: the fully explicit short step (ATMOS_DYN_Tstep_short_fvm_heve,
+// scale_atmos_dyn_tstep_short_fvm_heve.F90:110-1414).  Same column march as
+// hevi_march without the solve: the vertical mass flux carries J33G MOMZ
+// (HEVE.F90:468-478) and upwinds the theta flux with it, MOMZ takes its
+// pressure gradient and buoyancy explicitly (HEVE.F90:660-677), and the
+// horizontal divergence damping has the HEVE form FDX (DDIV(i+1) - DDIV(i))
+// (HEVE.F90:933, 1182).
+template <int S, bool TERRAIN, bool DIVDMP>
+__device__ __forceinline__ double heve_mflxZ(const Ops<S, TERRAIN, DIVDMP>& o, const HeviArgs& a, int k, int i, int j) {
+  const Dims& d = a.d;
+  const auto& m = o.m;
+  const size_t p = IX(d, k, i, j);
+  double r = a.J33G * a.MOMZ[p] / (m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY));
+  if constexpr (TERRAIN) {
+    const double* X = a.MOMX; const double* Y = a.MOMY;
+    r = r + m.j13(I_XYW, p) * 0.25
+                * (X[IX(d, k + 1, i, j)] + X[IX(d, k + 1, i - 1, j)] + X[p] + X[IX(d, k, i - 1, j)])
+                / m.mapf(i, j, 1, I_XY)
+          + m.j23(I_XYW, p) * 0.25
+                * (Y[IX(d, k + 1, i, j)] + Y[IX(d, k + 1, i, j - 1)] + Y[p] + Y[IX(d, k, i, j - 1)])
+                / m.mapf(i, j, 0, I_XY);
+  }
+  return r + m.gs(I_XYW, p) * o.ND(I_DENS, ZDIR, p) / (m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY));
+}
+
+template <int S, bool TERRAIN, bool DIVDMP>
+__global__ void __launch_bounds__(256) heve_march(HeviArgs a, unsigned ni_tiles, unsigned nblocks) {
+  const Dims& d = a.d;
+  Ops<S, TERRAIN, DIVDMP> o(a);
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const int i = d.IS + (int)((lb % ni_tiles) * 64 + threadIdx.x);
+  const int j = d.JS + (int)((lb / ni_tiles) * blockDim.y + threadIdx.y);
+  if (i > d.IE || j > d.JE) return;
+  const auto& m = o.m;
+  const double* RCDX = a.g.RCDX; const double* RCDY = a.g.RCDY; const double* RCDZ = a.g.RCDZ;
+  const double* RFDZ = a.g.RFDZ;
+  const double dtrk = a.dtrk;
+  const int KS = d.KS, KE = d.KE;
+  double* mfZ = a.mflx_hi; double* mfX = a.mflx_hi + d.n3; double* mfY = a.mflx_hi + 2 * d.n3;
+  double* tfZ = a.tflx_hi; double* tfX = a.tflx_hi + d.n3; double* tfY = a.tflx_hi + 2 * d.n3;
+  ZCarry zc;
+  zc.pw[1] = o.POTT(KS - 1, i, j); zc.pw[2] = o.POTT(KS, i, j); zc.pw[3] = o.POTT(KS + 1, i, j);
+  for (int k = KS; k <= KE; ++k) {
+    zc.pw[0] = zc.pw[1]; zc.pw[1] = zc.pw[2]; zc.pw[2] = zc.pw[3];
+    zc.pw[3] = o.POTT(k + 2, i, j);
+    const size_t p = IX(d, k, i, j);
+
+    // ---- continuity (HEVE.F90:452-572) ----
+    const double mX = o.mflxX(k, i, j), mXm = o.mflxX(k, i - 1, j);
+    const double mY = o.mflxY(k, i, j), mYm = o.mflxY(k, i, j - 1);
+    const double mZ = (k <= KE - 1) ? heve_mflxZ(o, a, k, i, j) : 0.0, mZm = zc.mZ;
+    zc.mZ = mZ;
+    double DR;
+    {
+      double advcv = -(mZ - mZm) * RCDZ[k];
+      double advch = -(mX - mXm) * RCDX[i] - (mY - mYm) * RCDY[j];
+      DR = a.DENS0[p] + dtrk * ((advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p)
+                                + a.DENS_t[p]);
+    }
+
+    // ---- thermodynamics (HEVE.F90:1316-1375) ----
+    const double tX = o.tflxX(k, i, j, mX), tXm = o.tflxX(k, i - 1, j, mXm);
+    const double tY = o.tflxY(k, i, j, mY), tYm = o.tflxY(k, i, j - 1, mYm);
+    const double tZ = o.tflxZw(k, i, j, mZ, zc.pw), tZm = zc.tZ;
+    zc.tZ = tZ;
+    double RR;
+    {
+      double advcv = -(tZ - tZm) * RCDZ[k];
+      double advch = -(tX - tXm) * RCDX[i] - (tY - tYm) * RCDY[j];
+      RR = a.RHOT0[p] + dtrk * ((advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p)
+                                + a.RHOT_t[p]);
+    }
+
+    // ---- momentum z (HEVE.F90:588-700) ----
+    double WR = 0.0;
+    if (k <= KE - 1) {
+      const double qw = o.qZ_XYW(k, i, j);
+      double sv = qw - zc.qW;
+      zc.qW = qw;
+      if constexpr (TERRAIN) {
+        const double q13 = o.qJ_XYW(0, k, i, j), q23 = o.qJ_XYW(1, k, i, j);
+        sv = sv + q13 - zc.qWJ13 + q23 - zc.qWJ23;
+        zc.qWJ13 = q13; zc.qWJ23 = q23;
+      }
+      const double pgf = a.J33G * (o.DPRES(k + 1, i, j) - o.DPRES(k, i, j)) * RFDZ[k];
+      const double buoy = GRAV * m.gs(I_XYW, p)
+                        * (o.F2H(k, 1, I_XYZ, i, j) * (o.D(k + 1, i, j) - a.REF_dens[IX(d, k + 1, i, j)])
+                         + o.F2H(k, 2, I_XYZ, i, j) * (o.D(k, i, j) - a.REF_dens[p]));
+      double advcv = -(sv) * RFDZ[k];
+      double advch = -((o.qH_XYW(0, k, i, j) - o.qH_XYW(0, k, i - 1, j)) * RCDX[i]
+                     + (o.qH_XYW(1, k, i, j) - o.qH_XYW(1, k, i, j - 1)) * RCDY[j])
+                   * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY);
+      double t = (advcv + advch - pgf - buoy) / m.gs(I_XYW, p) + (-a.wdamp_coef[k] * a.MOMZ0[p]);
+      if constexpr (DIVDMP) t = t + a.divdmp_coef / dtrk * a.g.FDZ[k] * (a.DDIV[IX(d, k + 1, i, j)] - a.DDIV[p]);
+      WR = a.MOMZ0[p] + dtrk * (t + a.MOMZ_t[p]);
+    }
+
+    // ---- momentum x and y (HEVE.F90:744-945, 993-1193) ----
+    double tx = momx_core(o, a, k, i, j, zc);
+    if constexpr (DIVDMP) tx = tx + a.divdmp_coef / dtrk * a.g.FDX[i] * (a.DDIV[IX(d, k, i + 1, j)] - a.DDIV[p]);
+    double ty = momy_core(o, a, k, i, j, zc);
+    if constexpr (DIVDMP) ty = ty + a.divdmp_coef / dtrk * a.g.FDY[j] * (a.DDIV[IX(d, k, i, j + 1)] - a.DDIV[p]);
+
+    // ---- stores, after every load of this level ----
+    mfX[p] = mX; mfY[p] = mY; tfX[p] = tX; tfY[p] = tY;
+    if (i == d.IS) { mfX[p - 1] = mXm; tfX[p - 1] = tXm; }
+    if (j == d.JS) { mfY[p - d.IA] = mYm; tfY[p - d.IA] = tYm; }
+    mfZ[p] = mZ; tfZ[p] = tZ;  // both 0 at KE
+    if (k == KS) { mfZ[p - d.pl] = 0.0; tfZ[p - d.pl] = 0.0; a.MOMZ_RK[p - d.pl] = 0.0; }
+    a.DENS_RK[p] = DR;
+    a.RHOT_RK[p] = RR;
+    a.MOMZ_RK[p] = WR;  // 0 at KE (HEVE.F90:697-698)
+    a.MOMX_RK[p] = a.MOMX0[p] + dtrk * (tx + a.MOMX_t[p]);
+    a.MOMY_RK[p] = a.MOMY0[p] + dtrk * (ty + a.MOMY_t[p]);
   }
 }
 
@@ -1174,7 +1320,7 @@ hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
   HeviArgs a;
   fill_args(a, io);
   const Ctx& c = *io.ctx;
-  const bool div = io.divdmp_coef > 0.0;
+  const bool div = io.divdmp_coef != 0.0;
   const bool ter = c.terrain;
   const bool cmb = io.combine != 0;
   if (c.scheme == UD3) {
@@ -1185,13 +1331,40 @@ hipError_t hevi_stage(const StageIO& io, hipStream_t s) {
   return div ? launch_t<CD4, false, true>(a, s, cmb) : launch_t<CD4, false, false>(a, s, cmb);
 }
 
+template <int S, bool TERRAIN, bool DIVDMP>
+static hipError_t launch_heve(const HeviArgs& a, hipStream_t s) {
+  const Dims& d = a.d;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
+  constexpr unsigned TJ = 4;
+  const unsigned ni = (imax + 63) / 64, nb = ni * ((jmax + TJ - 1) / TJ);
+  prof_start(*a.ctx, "heve_stage", s);
+  hipLaunchKernelGGL((heve_march<S, TERRAIN, DIVDMP>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  prof_stop(*a.ctx, s);
+  return hipGetLastError();
+}
+
+hipError_t heve_stage(const StageIO& io, hipStream_t s) {
+  HeviArgs a;
+  fill_args(a, io);
+  const Ctx& c = *io.ctx;
+  if (io.combine) return hipErrorNotSupported;
+  const bool div = io.divdmp_coef != 0.0;
+  const bool ter = c.terrain;
+  if (c.scheme == UD3) {
+    if (ter) return div ? launch_heve<UD3, true, true>(a, s) : launch_heve<UD3, true, false>(a, s);
+    return div ? launch_heve<UD3, false, true>(a, s) : launch_heve<UD3, false, false>(a, s);
+  }
+  if (ter) return div ? launch_heve<CD4, true, true>(a, s) : launch_heve<CD4, true, false>(a, s);
+  return div ? launch_heve<CD4, false, true>(a, s) : launch_heve<CD4, false, false>(a, s);
+}
+
 // flat-terrain FP64 stage through hevi_march, recording PT, St and the
 // solved vertical momentum C for the adjoint
 hipError_t hevi_stage_traced(const StageIO& io, double* PT, double* St, double* Cp, hipStream_t s) {
   HeviArgs a;
   fill_args(a, io);
   const Ctx& c = *io.ctx;
-  if (c.terrain || io.divdmp_coef > 0.0 || io.combine) return hipErrorNotSupported;
+  if (c.terrain || io.divdmp_coef != 0.0 || io.combine) return hipErrorNotSupported;
   a.PT = PT; a.trSt = St; a.trCp = Cp;
   const Dims& d = c.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
@@ -1206,7 +1379,7 @@ hipError_t hevi_stage(const StageIOT<float>& io, hipStream_t s) {
   HeviArgsT<float> a;
   fill_args(a, io);
   const Ctx& c = *io.ctx;
-  if (c.terrain || io.divdmp_coef > 0.0) return hipErrorNotSupported;
+  if (c.terrain || io.divdmp_coef != 0.0) return hipErrorNotSupported;
   const Dims& d = c.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
   constexpr int TJ = 8;

@@ -82,6 +82,8 @@ struct TracerIO {
 };
 
 hipError_t hevi_stage(const StageIO& io, hipStream_t s);
+// the explicit HEVE stage: same inputs and outputs, FP64, no fused combine
+hipError_t heve_stage(const StageIO& io, hipStream_t s);
 // mixed precision: flat terrain and divdmp_coef = 0 only
 hipError_t hevi_stage(const StageIOT<float>& io, hipStream_t s);
 // flat terrain, FP64, no divergence damping: the stage with PT, St and the

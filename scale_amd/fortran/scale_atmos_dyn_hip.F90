@@ -1,9 +1,9 @@
 !-------------------------------------------------------------------------------
 !> module scale_atmos_dyn_hip
 !!
-!! Fortran drop-in for the MI355X HEVI short step and FCT tracer advection.
+!! Fortran drop-in for the MI355X HEVI and HEVE short steps and FCT tracer advection.
 !!
-!! ATMOS_DYN_Tstep_short_fvm_hevi_hip has the dummy list, intents and meaning
+!! ATMOS_DYN_Tstep_short_fvm_hevi_hip and ATMOS_DYN_Tstep_short_fvm_heve_hip have the dummy list, intents and meaning
 !! of the abstract interface `short` of ATMOS_DYN_Tstep_short
 !! (scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_short.F90:38-128) and
 !! ATMOS_DYN_Tstep_tracer_fvm_heve_hip those of `step` of ATMOS_DYN_Tstep_tracer
@@ -20,6 +20,7 @@ module scale_atmos_dyn_hip
   public :: ATMOS_DYN_hip_setup
   public :: ATMOS_DYN_hip_finalize
   public :: ATMOS_DYN_Tstep_short_fvm_hevi_hip
+  public :: ATMOS_DYN_Tstep_short_fvm_heve_hip
   public :: ATMOS_DYN_Tstep_tracer_fvm_heve_hip
   public :: ATMOS_DYN_numfilter_coef_hip
   public :: ATMOS_DYN_numfilter_coef_q_hip
@@ -72,6 +73,28 @@ module scale_atmos_dyn_hip
        integer(c_int32_t), intent(in) :: BND_W, BND_E, BND_S, BND_N, last
        real(c_double), intent(in) :: dtrk
      end function c_tstep_short
+     integer(c_int) function c_tstep_short_heve(h, &
+          DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, PROG_RK, mflx_hi, tflx_hi, &
+          DENS0, MOMZ0, MOMX0, MOMY0, RHOT0, DENS, MOMZ, MOMX, MOMY, RHOT, &
+          DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, PROG0, PROG, &
+          DPRES0, RT2P, CORIOLI, num_diff, wdamp_coef, divdmp_coef, DDIV, &
+          FLAG_FCT_MOMENTUM, FLAG_FCT_T, FLAG_FCT_ALONG_STREAM, &
+          CDZ, FDZ, FDX, FDY, RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY, &
+          PHI, GSQRT, J13G, J23G, J33G, MAPF, REF_dens, REF_rhot, &
+          BND_W, BND_E, BND_S, BND_N, dtrk, last) bind(C, name="scale_hip_tstep_short_heve")
+       import :: c_ptr, c_int, c_double, c_int32_t
+       type(c_ptr), value :: h
+       real(c_double) :: DENS_RK(*), MOMZ_RK(*), MOMX_RK(*), MOMY_RK(*), RHOT_RK(*), PROG_RK(*), mflx_hi(*), tflx_hi(*)
+       real(c_double), intent(in) :: DENS0(*), MOMZ0(*), MOMX0(*), MOMY0(*), RHOT0(*)
+       real(c_double), intent(in) :: DENS(*), MOMZ(*), MOMX(*), MOMY(*), RHOT(*)
+       real(c_double), intent(in) :: DENS_t(*), MOMZ_t(*), MOMX_t(*), MOMY_t(*), RHOT_t(*), PROG0(*), PROG(*)
+       real(c_double), intent(in) :: DPRES0(*), RT2P(*), CORIOLI(*), num_diff(*), wdamp_coef(*), divdmp_coef, DDIV(*)
+       integer(c_int32_t), intent(in) :: FLAG_FCT_MOMENTUM, FLAG_FCT_T, FLAG_FCT_ALONG_STREAM
+       real(c_double), intent(in) :: CDZ(*), FDZ(*), FDX(*), FDY(*), RCDZ(*), RCDX(*), RCDY(*), RFDZ(*), RFDX(*), RFDY(*)
+       real(c_double), intent(in) :: PHI(*), GSQRT(*), J13G(*), J23G(*), J33G, MAPF(*), REF_dens(*), REF_rhot(*)
+       integer(c_int32_t), intent(in) :: BND_W, BND_E, BND_S, BND_N, last
+       real(c_double), intent(in) :: dtrk
+     end function c_tstep_short_heve
      integer(c_int) function c_tstep_tracer(h, QTRCo, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff, &
           GSQRT, MAPF, CDZ, RCDZ, RCDX, RCDY, dtl, FLAG_FCT_TRACER, FLAG_FCT_ALONG_STREAM) &
           bind(C, name="scale_hip_tstep_tracer_heve")
@@ -169,6 +192,56 @@ contains
          l2i(BND_W), l2i(BND_E), l2i(BND_S), l2i(BND_N), dtrk, l2i(last)), "ATMOS_DYN_Tstep_short_fvm_hevi_hip")
     ! HEVI has no extra prognostics (VA_FVM_HEVI = 0, HEVI.F90:72): PROG_RK is empty
   end subroutine ATMOS_DYN_Tstep_short_fvm_hevi_hip
+
+  !> explicit HEVE stage with the dummies of the abstract interface `short`
+  !! (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1516); the FCT flags must be .false.
+  subroutine ATMOS_DYN_Tstep_short_fvm_heve_hip( &
+       DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, &
+       PROG_RK,                                     &
+       mflx_hi, tflx_hi,                            &
+       DENS0,   MOMZ0,   MOMX0,   MOMY0,   RHOT0,   &
+       DENS,    MOMZ,    MOMX,    MOMY,    RHOT,    &
+       DENS_t,  MOMZ_t,  MOMX_t,  MOMY_t,  RHOT_t,  &
+       PROG0, PROG,                                 &
+       DPRES0, RT2P, CORIOLI,                       &
+       num_diff, wdamp_coef, divdmp_coef, DDIV,     &
+       FLAG_FCT_MOMENTUM, FLAG_FCT_T,               &
+       FLAG_FCT_ALONG_STREAM,                       &
+       CDZ, FDZ, FDX, FDY,                          &
+       RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY,          &
+       PHI, GSQRT, J13G, J23G, J33G, MAPF,          &
+       REF_dens, REF_rhot,                          &
+       BND_W, BND_E, BND_S, BND_N,                  &
+       dtrk, last                                   )
+    real(DP), intent(out)   :: DENS_RK(KA,IA,JA), MOMZ_RK(KA,IA,JA), MOMX_RK(KA,IA,JA), MOMY_RK(KA,IA,JA), RHOT_RK(KA,IA,JA)
+    real(DP), intent(out)   :: PROG_RK(KA,IA,JA,VA)
+    real(DP), intent(inout) :: mflx_hi(KA,IA,JA,3)
+    real(DP), intent(out)   :: tflx_hi(KA,IA,JA,3)
+    real(DP), intent(in), target :: DENS0(KA,IA,JA), MOMZ0(KA,IA,JA), MOMX0(KA,IA,JA), MOMY0(KA,IA,JA), RHOT0(KA,IA,JA)
+    real(DP), intent(in)    :: DENS(KA,IA,JA), MOMZ(KA,IA,JA), MOMX(KA,IA,JA), MOMY(KA,IA,JA), RHOT(KA,IA,JA)
+    real(DP), intent(in)    :: DENS_t(KA,IA,JA), MOMZ_t(KA,IA,JA), MOMX_t(KA,IA,JA), MOMY_t(KA,IA,JA), RHOT_t(KA,IA,JA)
+    real(DP), intent(in)    :: PROG0(KA,IA,JA,VA), PROG(KA,IA,JA,VA)
+    real(DP), intent(in)    :: DPRES0(KA,IA,JA), RT2P(KA,IA,JA), CORIOLI(1,IA,JA)
+    real(DP), intent(in)    :: num_diff(KA,IA,JA,5,3), wdamp_coef(KA), divdmp_coef, DDIV(KA,IA,JA)
+    logical,  intent(in)    :: FLAG_FCT_MOMENTUM, FLAG_FCT_T, FLAG_FCT_ALONG_STREAM
+    real(DP), intent(in)    :: CDZ(KA), FDZ(KA-1), FDX(IA-1), FDY(JA-1)
+    real(DP), intent(in)    :: RCDZ(KA), RCDX(IA), RCDY(JA), RFDZ(KA-1), RFDX(IA-1), RFDY(JA-1)
+    real(DP), intent(in)    :: PHI(KA,IA,JA), GSQRT(KA,IA,JA,7), J13G(KA,IA,JA,7), J23G(KA,IA,JA,7), J33G
+    real(DP), intent(in)    :: MAPF(IA,JA,2,4), REF_dens(KA,IA,JA), REF_rhot(KA,IA,JA)
+    logical,  intent(in)    :: BND_W, BND_E, BND_S, BND_N
+    real(DP), intent(in)    :: dtrk
+    logical,  intent(in)    :: last
+    real(DP) :: dummy(1)
+    call check(c_tstep_short_heve(ctx, DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, dummy, mflx_hi, tflx_hi, &
+         DENS0, MOMZ0, MOMX0, MOMY0, RHOT0, DENS, MOMZ, MOMX, MOMY, RHOT, &
+         DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, dummy, dummy, &
+         DPRES0, RT2P, CORIOLI, num_diff, wdamp_coef, divdmp_coef, DDIV, &
+         l2i(FLAG_FCT_MOMENTUM), l2i(FLAG_FCT_T), l2i(FLAG_FCT_ALONG_STREAM), &
+         CDZ, FDZ, FDX, FDY, RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY, &
+         PHI, GSQRT, J13G, J23G, J33G, MAPF, REF_dens, REF_rhot, &
+         l2i(BND_W), l2i(BND_E), l2i(BND_S), l2i(BND_N), dtrk, l2i(last)), "ATMOS_DYN_Tstep_short_fvm_heve_hip")
+    ! HEVE has no extra prognostics (VA_FVM_HEVE = 0, HEVE.F90:60): PROG_RK is empty
+  end subroutine ATMOS_DYN_Tstep_short_fvm_heve_hip
 
   !> tracer stage with the dummies of the interface `step`
   subroutine ATMOS_DYN_Tstep_tracer_fvm_heve_hip( &

@@ -150,6 +150,27 @@ class Context:
             _p(PHI), _p(GSQRT), _p(J13G), _p(J23G), _d(J33G), _p(MAPF), _p(REF_dens), _p(REF_rhot),
             _l(BND_W), _l(BND_E), _l(BND_S), _l(BND_N), _d(dtrk), _l(last)))
 
+    def ATMOS_DYN_Tstep_short_fvm_heve(self, DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, PROG_RK, mflx_hi, tflx_hi,
+                                       DENS0, MOMZ0, MOMX0, MOMY0, RHOT0, DENS, MOMZ, MOMX, MOMY, RHOT,
+                                       DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, PROG0, PROG,
+                                       DPRES0, RT2P, CORIOLI, num_diff, wdamp_coef, divdmp_coef, DDIV,
+                                       FLAG_FCT_MOMENTUM, FLAG_FCT_T, FLAG_FCT_ALONG_STREAM,
+                                       CDZ, FDZ, FDX, FDY, RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY,
+                                       PHI, GSQRT, J13G, J23G, J33G, MAPF, REF_dens, REF_rhot,
+                                       BND_W, BND_E, BND_S, BND_N, dtrk, last):
+        """One explicit HEVE stage (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1516); the FCT
+        flags must be false."""
+        L = load_library()
+        _check(L.scale_hip_tstep_short_heve(
+            self._h, _p(DENS_RK), _p(MOMZ_RK), _p(MOMX_RK), _p(MOMY_RK), _p(RHOT_RK), _p(PROG_RK), _p(mflx_hi), _p(tflx_hi),
+            _p(DENS0), _p(MOMZ0), _p(MOMX0), _p(MOMY0), _p(RHOT0), _p(DENS), _p(MOMZ), _p(MOMX), _p(MOMY), _p(RHOT),
+            _p(DENS_t), _p(MOMZ_t), _p(MOMX_t), _p(MOMY_t), _p(RHOT_t), _p(PROG0), _p(PROG),
+            _p(DPRES0), _p(RT2P), _p(CORIOLI), _p(num_diff), _p(wdamp_coef), _d(divdmp_coef), _p(DDIV),
+            _l(FLAG_FCT_MOMENTUM), _l(FLAG_FCT_T), _l(FLAG_FCT_ALONG_STREAM),
+            _p(CDZ), _p(FDZ), _p(FDX), _p(FDY), _p(RCDZ), _p(RCDX), _p(RCDY), _p(RFDZ), _p(RFDX), _p(RFDY),
+            _p(PHI), _p(GSQRT), _p(J13G), _p(J23G), _d(J33G), _p(MAPF), _p(REF_dens), _p(REF_rhot),
+            _l(BND_W), _l(BND_E), _l(BND_S), _l(BND_N), _d(dtrk), _l(last)))
+
     def ATMOS_DYN_Tstep_tracer_fvm_heve(self, QTRCo, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff,
                                         GSQRT, MAPF, CDZ, RCDZ, RCDX, RCDY, dtl, FLAG_FCT_TRACER, FLAG_FCT_ALONG_STREAM):
         L = load_library()

@@ -1,12 +1,14 @@
 !-------------------------------------------------------------------------------
 ! Test driver of the Fortran drop-in module scale_atmos_dyn_hip: reads one
-! HEVI stage's inputs from a stream file written by tests/test_gpu_fortran.py,
-! calls ATMOS_DYN_Tstep_short_fvm_hevi_hip exactly as ATMOS_DYN_tinteg_short
-! calls its procedure pointer (tinteg_short_rk4.F90:286-311), then one
-! ATMOS_DYN_Tstep_tracer_fvm_heve_hip stage, and writes the outputs.
+! short-step stage's inputs from a stream file written by
+! tests/test_gpu_fortran.py, calls ATMOS_DYN_Tstep_short_fvm_hevi_hip (or,
+! with short = 1, ATMOS_DYN_Tstep_short_fvm_heve_hip) exactly as
+! ATMOS_DYN_tinteg_short calls its procedure pointer
+! (tinteg_short_rk4.F90:286-311), then one ATMOS_DYN_Tstep_tracer_fvm_heve_hip
+! stage, and writes the outputs.
 !
 ! usage: dropin_driver <input.bin> <output.bin>
-! input:  int32 KA IA JA KS KE IS IE JS JE scheme scheme_tracer fct
+! input:  int32 KA IA JA KS KE IS IE JS JE scheme scheme_tracer fct short
 !         real64 arrays in the order of the READ statements below
 !-------------------------------------------------------------------------------
 program dropin_driver
@@ -14,7 +16,7 @@ program dropin_driver
   use scale_atmos_dyn_hip
   implicit none
 
-  integer(int32) :: hdr(12)
+  integer(int32) :: hdr(13)
   integer :: KS, KE, IS, IE, JS, JE
   character(len=512) :: fin, fout
   real(real64), allocatable :: DENS_RK(:,:,:), MOMZ_RK(:,:,:), MOMX_RK(:,:,:), MOMY_RK(:,:,:), RHOT_RK(:,:,:)
@@ -63,23 +65,43 @@ program dropin_driver
   call ATMOS_DYN_hip_setup(KA, IA, JA, KS, KE, IS, IE, JS, JE, hdr(10), hdr(11), 0)
 
   mflx_hi(:,:,:,:) = 0.0_real64
-  call ATMOS_DYN_Tstep_short_fvm_hevi_hip( &
-       DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, &
-       PROG_RK,                                     &
-       mflx_hi, tflx_hi,                            &
-       DENS0,   MOMZ0,   MOMX0,   MOMY0,   RHOT0,   &
-       DENS,    MOMZ,    MOMX,    MOMY,    RHOT,    &
-       DENS_t,  MOMZ_t,  MOMX_t,  MOMY_t,  RHOT_t,  &
-       PROG0, PROG,                                 &
-       DPRES0, RT2P, CORIOLI,                       &
-       num_diff, wdamp_coef, divdmp_coef, DDIV,     &
-       .false., .false., .true.,                    &
-       CDZ, FDZ, FDX, FDY,                          &
-       RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY,          &
-       PHI, GSQRT, J13G, J23G, J33G, MAPF,          &
-       REF_dens, REF_rhot,                          &
-       .false., .false., .false., .false.,          &
-       dtrk, .false.                                )
+  if (hdr(13) == 1) then
+     call ATMOS_DYN_Tstep_short_fvm_heve_hip( &
+          DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, &
+          PROG_RK,                                     &
+          mflx_hi, tflx_hi,                            &
+          DENS0,   MOMZ0,   MOMX0,   MOMY0,   RHOT0,   &
+          DENS,    MOMZ,    MOMX,    MOMY,    RHOT,    &
+          DENS_t,  MOMZ_t,  MOMX_t,  MOMY_t,  RHOT_t,  &
+          PROG0, PROG,                                 &
+          DPRES0, RT2P, CORIOLI,                       &
+          num_diff, wdamp_coef, divdmp_coef, DDIV,     &
+          .false., .false., .true.,                    &
+          CDZ, FDZ, FDX, FDY,                          &
+          RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY,          &
+          PHI, GSQRT, J13G, J23G, J33G, MAPF,          &
+          REF_dens, REF_rhot,                          &
+          .false., .false., .false., .false.,          &
+          dtrk, .false.                                )
+  else
+     call ATMOS_DYN_Tstep_short_fvm_hevi_hip( &
+          DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, &
+          PROG_RK,                                     &
+          mflx_hi, tflx_hi,                            &
+          DENS0,   MOMZ0,   MOMX0,   MOMY0,   RHOT0,   &
+          DENS,    MOMZ,    MOMX,    MOMY,    RHOT,    &
+          DENS_t,  MOMZ_t,  MOMX_t,  MOMY_t,  RHOT_t,  &
+          PROG0, PROG,                                 &
+          DPRES0, RT2P, CORIOLI,                       &
+          num_diff, wdamp_coef, divdmp_coef, DDIV,     &
+          .false., .false., .true.,                    &
+          CDZ, FDZ, FDX, FDY,                          &
+          RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY,          &
+          PHI, GSQRT, J13G, J23G, J33G, MAPF,          &
+          REF_dens, REF_rhot,                          &
+          .false., .false., .false., .false.,          &
+          dtrk, .false.                                )
+  end if
 
   ! tracer stage driven by the mass flux just computed, after the periodic
   ! halo exchange the large step applies to mflx_hi (LARGE.F90:1026-1031)

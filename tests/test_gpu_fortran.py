@@ -1,7 +1,7 @@
 """GPU test of the Fortran drop-in (scale_amd/fortran/scale_atmos_dyn_hip.F90):
 a Fortran program binds the module's procedures exactly as SCALE's
 ATMOS_DYN_tinteg_short / _tinteg_tracer bind their procedure pointers, runs
-one HEVI stage and one FCT tracer stage on the GPU through the C ABI, and its
+one HEVI (or HEVE) stage and one FCT tracer stage on the GPU through the C ABI, and its
 outputs must equal the CPU oracle's bit for bit.  This pins the Fortran side
 of the boundary: argument order, by-reference scalars, LOGICAL conversion and
 the (KA,IA,JA) layout."""
@@ -25,8 +25,9 @@ def f(a):
 
 
 @pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
-def test_fortran_dropin_bitwise(tmp_path, scheme):
-    print(f"[fortran drop-in] scheme={scheme}", flush=True)
+@pytest.mark.parametrize("short", ["hevi", "heve"])
+def test_fortran_dropin_bitwise(tmp_path, scheme, short):
+    print(f"[fortran drop-in] scheme={scheme} short={short}", flush=True)
     if not os.path.exists(DRIVER):
         pytest.fail("tests/fortran/dropin_driver is not built (run __graft_entry__.build())")
     case = H.small_case(seed=500 + scheme, scheme=scheme, scheme_tracer=scheme)
@@ -42,7 +43,7 @@ def test_fortran_dropin_bitwise(tmp_path, scheme):
     nd_q = np.asfortranarray(np.zeros(g.shape + (3,)))
 
     hdr = np.array([g.KA, g.IA, g.JA, g.KS + 1, g.KE + 1, g.IS + 1, g.IE + 1, g.JS + 1, g.JE + 1,
-                    scheme, scheme, 1], dtype=np.int32)
+                    scheme, scheme, 1, int(short == "heve")], dtype=np.int32)
     blob = [hdr.tobytes()]
     for v in H.VARS:
         blob.append(f(inp[v + "0"]))
@@ -75,7 +76,8 @@ def test_fortran_dropin_bitwise(tmp_path, scheme):
     out = {k: po.nan_like_outputs(g) for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK")}
     out["mflx_hi"] = np.asfortranarray(np.zeros(g.shape + (3,)))
     out["tflx_hi"] = po.nan_like_outputs(g, 3)
-    ref = po.tstep_short_hevi(case, inp, dtrk, scheme=scheme, out=out)
+    stage = po.tstep_short_heve if short == "heve" else po.tstep_short_hevi
+    ref = stage(case, inp, dtrk, scheme=scheme, out=out)
     for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK", "tflx_hi"):
         H.compare(ref[k], got[k], k)
 

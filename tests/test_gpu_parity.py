@@ -28,30 +28,31 @@ def hip():
     return shp
 
 
-def run_stage_gpu(hip, case, inp, dtrk, metric=None, scheme=None):
+def run_stage_gpu(hip, case, inp, dtrk, metric=None, scheme=None, heve=False, fct=(False, False)):
     g = case.grid
     m = metric or case.metric
     out = {k: po.nan_like_outputs(g) for k in OUTS[:5]}
     out["mflx_hi"] = po.nan_like_outputs(g, 3)
     out["tflx_hi"] = po.nan_like_outputs(g, 3)
     with hip.Context(g, scheme or case.scheme, case.scheme_tracer) as ctx:
-        ctx.ATMOS_DYN_Tstep_short_fvm_hevi(
+        stage = ctx.ATMOS_DYN_Tstep_short_fvm_heve if heve else ctx.ATMOS_DYN_Tstep_short_fvm_hevi
+        stage(
             out["DENS_RK"], out["MOMZ_RK"], out["MOMX_RK"], out["MOMY_RK"], out["RHOT_RK"], None, out["mflx_hi"], out["tflx_hi"],
             inp["DENS0"], inp["MOMZ0"], inp["MOMX0"], inp["MOMY0"], inp["RHOT0"],
             inp["DENS"], inp["MOMZ"], inp["MOMX"], inp["MOMY"], inp["RHOT"],
             inp["DENS_t"], inp["MOMZ_t"], inp["MOMX_t"], inp["MOMY_t"], inp["RHOT_t"], None, None,
             inp["DPRES0"], inp["RT2P"], inp.get("CORIOLI", case.corioli), inp["num_diff"], inp.get("wdamp_coef", case.wdamp_coef),
-            inp.get("divdmp_coef", 0.0), inp["DDIV"], False, False, True,
+            inp.get("divdmp_coef", 0.0), inp["DDIV"], fct[0], fct[1], True,
             m["CDZ"], m["FDZ"], m["FDX"], m["FDY"], m["RCDZ"], m["RCDX"], m["RCDY"], m["RFDZ"], m["RFDX"], m["RFDY"],
             m["PHI"], m["GSQRT"], m["J13G"], m["J23G"], m["J33G"], m["MAPF"], inp["REF_dens"], inp["REF_rhot"],
             False, False, False, False, dtrk, False)
     return out
 
 
-def oracle_stage(case, inp, dtrk, metric=None, scheme=None):
+def oracle_stage(case, inp, dtrk, metric=None, scheme=None, heve=False):
     if metric is not None:
         case = inputs.Case(**{**case.__dict__, "metric": metric})
-    return po.tstep_short_hevi(case, inp, dtrk, scheme=scheme)
+    return (po.tstep_short_heve if heve else po.tstep_short_hevi)(case, inp, dtrk, scheme=scheme)
 
 
 @pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
@@ -69,17 +70,60 @@ def test_hevi_stage_bitwise(hip, scheme, terrain):
         H.compare(ref[k], got[k], k)
 
 
-def test_hevi_stage_divdmp(hip):
-    print("[hevi stage] divergence damping", flush=True)
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+def test_hevi_stage_divdmp(hip, terrain):
+    print(f"[hevi stage] divergence damping terrain={terrain}", flush=True)
     case = H.small_case(seed=77)
     inp = H.stage_inputs(case, seed=5)
     g = case.grid
     inp["DDIV"] = np.asfortranarray(np.random.Generator(np.random.PCG64(3)).standard_normal(g.shape) * 1e-3)
     inp["divdmp_coef"] = 0.1
-    ref = oracle_stage(case, inp, 1.0)
-    got = run_stage_gpu(hip, case, inp, 1.0)
+    metric = H.terrain_metrics(case) if terrain else None
+    ref = oracle_stage(case, inp, 1.0, metric)
+    got = run_stage_gpu(hip, case, inp, 1.0, metric)
     for k in OUTS:
         H.compare(ref[k], got[k], k)
+
+
+@pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+def test_heve_stage_bitwise(hip, scheme, terrain):
+    """One explicit HEVE stage (HEVE.F90:109-1516) through the drop-in entry
+    point, bit-identical to the oracle."""
+    print(f"[heve stage] scheme={scheme} terrain={terrain}", flush=True)
+    case = H.small_case(seed=2000 + scheme)
+    inp = H.stage_inputs(case, seed=22, coriolis=terrain)
+    metric = H.terrain_metrics(case) if terrain else None
+    if terrain:
+        inp["wdamp_coef"] = np.linspace(0.0, 0.01, case.grid.KA)
+    ref = oracle_stage(case, inp, 0.5, metric, scheme, heve=True)
+    got = run_stage_gpu(hip, case, inp, 0.5, metric, scheme, heve=True)
+    for k in OUTS:
+        H.compare(ref[k], got[k], k)
+
+
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+def test_heve_stage_divdmp(hip, terrain):
+    print(f"[heve stage] divergence damping terrain={terrain}", flush=True)
+    case = H.small_case(seed=78)
+    inp = H.stage_inputs(case, seed=6)
+    g = case.grid
+    inp["DDIV"] = np.asfortranarray(np.random.Generator(np.random.PCG64(4)).standard_normal(g.shape) * 1e-3)
+    inp["divdmp_coef"] = 0.1
+    metric = H.terrain_metrics(case) if terrain else None
+    ref = oracle_stage(case, inp, 1.0, metric, heve=True)
+    got = run_stage_gpu(hip, case, inp, 1.0, metric, heve=True)
+    for k in OUTS:
+        H.compare(ref[k], got[k], k)
+
+
+def test_heve_rejects_fct_flags(hip):
+    print("[heve stage] FCT flags rejected", flush=True)
+    case = H.small_case(kmax=8, imax=6, jmax=6)
+    inp = H.stage_inputs(case)
+    for fct in ((True, False), (False, True)):
+        with pytest.raises(hip.ScaleHipError, match="FCT"):
+            run_stage_gpu(hip, case, inp, 0.5, heve=True, fct=fct)
 
 
 def test_numfilter_bitwise(hip):

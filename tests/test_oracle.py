@@ -362,6 +362,134 @@ def test_stage_rest_state_stays_at_rest():
 
 
 # ---------------------------------------------------------------------------
+# HEVE (scale_atmos_dyn_tstep_short_fvm_heve.F90): the fully explicit short
+# step.  Its tendency is the dt -> 0 limit of the HEVI stage started from
+# state = state0 (the implicit corrections are O(dt)), so HEVI -- pinned above
+# against its own implicit equations -- pins every HEVE term, vertical pressure
+# gradient, buoyancy and the MOMZ part of the vertical mass flux included.
+def _heve_inputs(case, seed=8):
+    inp = H.stage_inputs(case, seed=seed, tend=True, coriolis=True)
+    for v in H.VARS:
+        inp[v] = inp[v + "0"]
+    return inp
+
+
+@pytest.mark.parametrize("scheme", [UD3, CD4])
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+def test_heve_is_the_explicit_limit_of_hevi(scheme, terrain):
+    case = H.small_case(seed=31)
+    if terrain:
+        case = inputs.Case(**{**case.__dict__, "metric": H.terrain_metrics(case)})
+    g = case.grid
+    inp = _heve_inputs(case)
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    he = po.tstep_short_heve(case, inp, 1.0, scheme=scheme)
+    dts = (4.0e-3, 2.0e-3, 1.0e-3)
+    hi = [po.tstep_short_hevi(case, inp, dt, scheme=scheme) for dt in dts]
+    for v in H.VARS:
+        fe = he[v + "_RK"][sl] - inp[v + "0"][sl]
+        s = np.abs(fe).max()
+        f = [(o[v + "_RK"][sl] - inp[v + "0"][sl]) / dt for o, dt in zip(hi, dts)]
+        e = [np.abs(x - fe).max() / s for x in f]
+        if v in ("MOMX", "MOMY"):  # explicit in both schemes: equal up to rounding
+            assert max(e) < 1e-11, (v, e)
+            continue
+        # first order in dt ...
+        assert 1.8 < e[0] / e[1] < 2.2 and 1.8 < e[1] / e[2] < 2.2, (v, e)
+        r = [np.abs(2 * f[n + 1] - f[n] - fe).max() / s for n in range(2)]
+        assert r[1] < 0.1 * e[2], (v, r, e)
+        if scheme == CD4:
+            # ... and the Richardson extrapolates converge on HEVE at second
+            # order, so the limit is HEVE's tendency itself.  Not so for UD3:
+            # HEVI upwinds the explicit vertical theta flux by the sign of its
+            # explicit mass flux (J13/J23/filter part, HEVI.F90:604-608) and the
+            # implicit part by MOMZ (:691), HEVE by the sign of their sum, so
+            # the UD3 limits differ by that switching term.
+            assert 3.5 < r[0] / r[1] < 4.5, (v, r, e)
+
+
+@pytest.mark.parametrize("scheme", [UD3, CD4])
+def test_heve_conserves_mass_and_rhot(scheme):
+    case = H.small_case(seed=31)
+    inp = H.stage_inputs(case, seed=8, tend=False)
+    out = po.tstep_short_heve(case, inp, 0.7, scheme=scheme)
+    g = case.grid
+    m = po.Model(case)
+    for v in ("DENS", "RHOT"):
+        a0 = m.mass(np.asfortranarray(inp[v + "0"]))
+        a1 = m.mass(np.asfortranarray(np.where(np.isnan(out[v + "_RK"]), 0.0, out[v + "_RK"])))
+        assert abs(a1 - a0) <= 2e-15 * abs(a0) * g.KA, v
+
+
+def test_heve_xy_mirror_symmetry():
+    case = H.small_case(kmax=16, imax=10, jmax=10, seed=17)
+    inp = H.stage_inputs(case, seed=4)
+    out = po.tstep_short_heve(case, inp, 0.7)
+    tout = po.tstep_short_heve(inputs.Case(**{**case.__dict__}), _transpose_case_inputs(case, inp), 0.7)
+    T = lambda a: np.swapaxes(a, 1, 2)
+    g = case.grid
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    for a, b in [("DENS_RK", "DENS_RK"), ("RHOT_RK", "RHOT_RK"), ("MOMZ_RK", "MOMZ_RK"),
+                 ("MOMX_RK", "MOMY_RK"), ("MOMY_RK", "MOMX_RK")]:
+        x, y = out[a][sl], T(tout[b])[sl]
+        assert np.allclose(x, y, rtol=1e-13, atol=1e-13 * np.abs(x).max()), (a, b, np.abs(x - y).max())
+
+
+def test_heve_divergence_damping_matches_hevi():
+    """With unit map factors HEVI's horizontal damping rows (HEVI.F90:912,
+    :1032) reduce to HEVE's (HEVE.F90:933, :1182) and are explicit in both, so
+    the change that divdmp_coef makes to MOMX/MOMY is the same; the vertical
+    row enters HEVI's implicit solve and agrees in the dt -> 0 limit."""
+    case = H.small_case(seed=5)
+    g = case.grid
+    inp = _heve_inputs(case, seed=3)
+    rng = np.random.Generator(np.random.PCG64(9))
+    ddiv = g.zeros()
+    ddiv[:, g.IS:g.IE + 1, g.JS:g.JE + 1] = 1e-3 * rng.standard_normal((g.KA, g.IE - g.IS + 1, g.JE - g.JS + 1))
+    inputs.halo_fill(g, ddiv)
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    slz = (slice(g.KS, g.KE), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    dt = 1.0e-3
+    delta = {}
+    for name, fn in (("heve", po.tstep_short_heve), ("hevi", po.tstep_short_hevi)):
+        o = [fn(case, dict(inp, DDIV=ddiv, divdmp_coef=c), dt, scheme=CD4) for c in (0.0, 0.1)]
+        delta[name] = {v: o[1][v + "_RK"] - o[0][v + "_RK"] for v in ("MOMZ", "MOMX", "MOMY")}
+    for v in ("MOMX", "MOMY"):
+        x, y = delta["heve"][v][sl], delta["hevi"][v][sl]
+        assert np.abs(x).max() > 1e-8, v
+        assert np.allclose(x, y, rtol=1e-9, atol=1e-9 * np.abs(x).max()), v
+    x, y = delta["heve"]["MOMZ"][slz], delta["hevi"]["MOMZ"][slz]
+    assert np.abs(x).max() > 1e-8
+    assert np.abs(x - y).max() < 1e-2 * np.abs(x).max()
+
+
+def test_heve_rejects_fct_flags():
+    case = H.small_case(kmax=8, imax=6, jmax=6)
+    inp = H.stage_inputs(case)
+    for kw in ({"fct_momentum": True}, {"fct_t": True}):
+        with pytest.raises(RuntimeError):
+            po.tstep_short_heve(case, inp, 0.5, **kw)
+
+
+def test_heve_rest_state_stays_at_rest():
+    case = inputs.make_case(kmax=20, imax=8, jmax=8, noise=0.0, bubble=False, u0=0.0, ntracer=0)
+    g = case.grid
+    inp = {}
+    for v in H.VARS:
+        inp[v] = case.state[v]; inp[v + "0"] = case.state[v]; inp[v + "_t"] = g.zeros()
+    inp.update(inputs.large_step_prep(case))
+    inp["num_diff"] = po.numfilter_coef(case, case.state, 1.0)
+    inp["DDIV"] = g.zeros()
+    inp["REF_dens"] = case.ref["REF_dens"]
+    inp["DPRES0"] = g.zeros()
+    out = po.tstep_short_heve(case, inp, 2.0)
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    for v in ("MOMZ", "MOMX", "MOMY"):
+        assert np.abs(out[v + "_RK"][sl]).max() < 1e-10, v
+    assert np.allclose(out["DENS_RK"][sl], case.state["DENS"][sl], rtol=1e-14, atol=0)
+
+
+# ---------------------------------------------------------------------------
 def test_numfilter_two_grid_wave_closed_form():
     """For q = A(-1)^i in x the 3rd-difference flux is 8A(-1)^i/dx^3 and
     num_diff = ND_COEF*dx/(2 DT) * A(-1)^i (density, x direction)."""
