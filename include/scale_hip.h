@@ -161,7 +161,7 @@ typedef struct {
   double dts;          /* short step [s]; nstep = ceiling((dtl-eps)/dts)       */
   double nd_coef;      /* ATMOS_DYN_NUMERICAL_DIFF_COEF (1e-4 by default)      */
   double nd_sfc_fact;  /* ATMOS_DYN_NUMERICAL_DIFF_sfc_fact (1.0)              */
-  double divdmp_coef;  /* ATMOS_DYN_divdmp_coef (0; only 0 is supported here)  */
+  double divdmp_coef;  /* ATMOS_DYN_divdmp_coef (0 by default; >= 0; FP64)    */
   int32_t nd_order;    /* ATMOS_DYN_NUMERICAL_DIFF_order (1)                   */
   int32_t fct_tracer;  /* ATMOS_DYN_FLAG_FCT_TRACER                            */
   int32_t fct_along_stream; /* ATMOS_DYN_FLAG_FCT_along_stream (1)             */
@@ -172,6 +172,9 @@ typedef struct {
                                terrain and QA = 0 only                         */
   double nd_coef_q;    /* ATMOS_DYN_NUMERICAL_DIFF_COEF_TRACER (0 by default:
                           num_diff_q = 0, LARGE.F90:1053-1054)                 */
+  int32_t short_type;  /* ATMOS_DYN_TYPE: 0 = FVM-HEVI, 1 = FVM-HEVE (the
+                          explicit stage of scale_hip_tstep_short_heve; FP64
+                          only, combined by a separate RK4 kernel)             */
 } scale_hip_dyn_config;
 
 /* field ids for scale_hip_model_set/_get/_mass */

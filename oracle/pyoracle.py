@@ -43,7 +43,8 @@ class OrcConfig(C.Structure):
     _fields_ = [("scheme", C.c_int), ("scheme_tracer", C.c_int), ("tinteg_short", C.c_int),
                 ("dtl", C.c_double), ("dts", C.c_double), ("nd_coef", C.c_double), ("nd_sfc_fact", C.c_double),
                 ("nd_coef_q", C.c_double), ("nd_order", C.c_int), ("nd_use_rs", C.c_int),
-                ("divdmp_coef", C.c_double), ("flag_fct_tracer", C.c_int), ("flag_fct_along_stream", C.c_int)]
+                ("divdmp_coef", C.c_double), ("flag_fct_tracer", C.c_int), ("flag_fct_along_stream", C.c_int),
+                ("short_type", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -277,9 +278,11 @@ def adj_large_step(model: "Model", xb: dict) -> dict:
 
 
 class Model:
-    """Whole-model integrator (LARGE.F90 on-path subset) over a Case."""
+    """Whole-model integrator (LARGE.F90 on-path subset) over a Case; short =
+    "hevi" or "heve" selects the stage of the RK4 short step (ATMOS_DYN_TYPE)."""
 
-    def __init__(self, case, nd_coef=1.0e-4, divdmp_coef=0.0, fct=None, along_stream=True, nd_coef_q=0.0):
+    def __init__(self, case, nd_coef=1.0e-4, divdmp_coef=0.0, fct=None, along_stream=True, nd_coef_q=0.0,
+                 short="hevi"):
         g = case.grid
         self.case = case
         self.state = {k: np.array(v, order="F", copy=True) for k, v in case.state.items()}
@@ -290,7 +293,8 @@ class Model:
         self.tflx_hi = np.full(g.shape + (3,), np.nan, order="F")
         self.rhoq_tp = np.zeros(g.shape + (max(qa, 1),), order="F")
         self.cfg = OrcConfig(case.scheme, case.scheme_tracer, 4, case.dt, case.dt, nd_coef, 1.0, nd_coef_q, 1, 1,
-                             divdmp_coef, int(case.fct if fct is None else fct), int(along_stream))
+                             divdmp_coef, int(case.fct if fct is None else fct), int(along_stream),
+                             {"hevi": 0, "heve": 1}[short])
         self._keep = []
 
     def _st(self):

@@ -1734,7 +1734,7 @@ int orc_tinteg_tracer_rk3(const orc_grid *g, int s,
 }
 
 /* ---- RK4 short step (RK4:260-660) ----------------------------------------- */
-int orc_tinteg_short_rk4(const orc_grid *g, int s,
+int orc_tinteg_short_rk4(const orc_grid *g, int s, int heve,
     double *DENS, double *MOMZ, double *MOMX, double *MOMY, double *RHOT,
     double *mflx_hi, double *tflx_hi,
     const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
@@ -1756,10 +1756,16 @@ int orc_tinteg_short_rk4(const orc_grid *g, int s,
     double **in = st == 0 ? S0 : R[st - 1];
     double **out = st == 3 ? cur : R[st];
     double *mo = st == 3 ? mflx_hi : mf[st], *to = st == 3 ? tflx_hi : tf[st];
-    rc |= orc_tstep_short_hevi(g, s, out[0], out[1], out[2], out[3], out[4], mo, to,
-                               S0[0], S0[1], S0[2], S0[3], S0[4], in[0], in[1], in[2], in[3], in[4],
-                               DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, DPRES0, RT2P, CORIOLI,
-                               num_diff, wdamp_coef, divdmp_coef, DDIV, m, REF_dens, REF_rhot, dtrk_s[st]);
+    if (heve)
+      rc |= orc_tstep_short_heve(g, s, out[0], out[1], out[2], out[3], out[4], mo, to,
+                                 S0[0], S0[1], S0[2], S0[3], S0[4], in[0], in[1], in[2], in[3], in[4],
+                                 DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, DPRES0, RT2P, CORIOLI,
+                                 num_diff, wdamp_coef, divdmp_coef, DDIV, 0, 0, m, REF_dens, REF_rhot, dtrk_s[st]);
+    else
+      rc |= orc_tstep_short_hevi(g, s, out[0], out[1], out[2], out[3], out[4], mo, to,
+                                 S0[0], S0[1], S0[2], S0[3], S0[4], in[0], in[1], in[2], in[3], in[4],
+                                 DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, DPRES0, RT2P, CORIOLI,
+                                 num_diff, wdamp_coef, divdmp_coef, DDIV, m, REF_dens, REF_rhot, dtrk_s[st]);
     if (st < 3)
       for (int v = 0; v < 5; ++v) orc_halo_periodic(g, out[v]);
   }
@@ -1842,7 +1848,7 @@ int orc_large_step(const orc_grid *g, const orc_metric *m, const orc_config *c, 
     }
     if (c->divdmp_coef > 0.0) orc_divergence(g, DDIV, st->MOMZ, st->MOMX, st->MOMY, m);
     else for (size_t q = 0; q < N3; ++q) DDIV[q] = 0.0;
-    rc |= orc_tinteg_short_rk4(g, c->scheme, st->DENS, st->MOMZ, st->MOMX, st->MOMY, st->RHOT, st->mflx_hi, st->tflx_hi,
+    rc |= orc_tinteg_short_rk4(g, c->scheme, c->short_type == 1, st->DENS, st->MOMZ, st->MOMX, st->MOMY, st->RHOT, st->mflx_hi, st->tflx_hi,
                                Dt[0], Dt[1], Dt[2], Dt[3], Dt[4], DPRES0, RT2P, st->CORIOLI, num_diff, st->wdamp_coef,
                                c->divdmp_coef, DDIV, m, st->REF_dens, REF_rhot, dts);
     /* ghost levels (LARGE.F90:908-927) */

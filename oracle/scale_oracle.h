@@ -158,6 +158,7 @@ typedef struct {
   int nd_order, nd_use_rs;
   double divdmp_coef;
   int flag_fct_tracer, flag_fct_along_stream;
+  int short_type;      /* ATMOS_DYN_TYPE: 0 = FVM-HEVI, 1 = FVM-HEVE */
 } orc_config;
 
 /* ATMOS_DYN_Tstep_large_fvm_heve on-path subset (LARGE.F90:377-1127) for a
@@ -167,8 +168,9 @@ typedef struct {
  * mflx average and the tracer RK3 loop.  Advances st by one large step. */
 int orc_large_step(const orc_grid *g, const orc_metric *m, const orc_config *c, orc_state *st);
 
-/* One full RK4 short step (tinteg_short_rk4.F90:260-660) on caller buffers. */
-int orc_tinteg_short_rk4(const orc_grid *g, int scheme,
+/* One full RK4 short step (tinteg_short_rk4.F90:260-660) on caller buffers;
+ * heve = 1 runs orc_tstep_short_heve as the stage instead of the HEVI one. */
+int orc_tinteg_short_rk4(const orc_grid *g, int scheme, int heve,
     double *DENS, double *MOMZ, double *MOMX, double *MOMY, double *RHOT,
     double *mflx_hi, double *tflx_hi,
     const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,

@@ -789,7 +789,7 @@ int orc_adj_numfilter_coef(const orc_grid *g, const orc_metric *m, double DT,
 int orc_adj_large_step(const orc_grid *g, const orc_metric *m, const orc_config *c, const orc_state *st,
     double *const xb[5]) {
   GRID_LOCALS;
-  if (c->tinteg_short != 4 || c->divdmp_coef != 0.0 || c->nd_order != 1 || !c->nd_use_rs) return -1;
+  if (c->tinteg_short != 4 || c->short_type != 0 || c->divdmp_coef != 0.0 || c->nd_order != 1 || !c->nd_use_rs) return -1;
   if ((int)ceil((c->dtl - 2.220446049250313e-16) / c->dts) != 1) return -1;
   if (!is_flat(g, m)) return -1;
   const size_t n3 = N3;

@@ -1014,7 +1014,7 @@ static hipError_t step_adj(Ctx& c, double* const x[5], double* const xb[5]) {
 static hipError_t check_adjoint_model(const Ctx& c) {
   if (!c.model || c.terrain) return hipErrorNotSupported;
   const Model& M = *c.model;
-  if (M.mixed || M.divdmp_coef != 0.0) return hipErrorNotSupported;
+  if (M.mixed || M.heve || M.divdmp_coef != 0.0) return hipErrorNotSupported;
   if ((int)ceil((M.dtl - EPS_RP) / M.dts) != 1) return hipErrorNotSupported;
   return hipSuccess;
 }

@@ -663,6 +663,8 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   if (cfg->mixed_precision && c.terrain) return fail("mixed precision needs flat terrain (GSQRT = 1, J13G = J23G = 0, MAPF = 1)");
   if (cfg->mixed_precision && QA > 0) return fail("mixed precision integrates the dynamics only (QA = 0)");
   if (cfg->nd_coef_q < 0.0) return fail("nd_coef_q must be >= 0");
+  if (cfg->short_type != 0 && cfg->short_type != 1) return fail("short_type must be 0 (FVM-HEVI) or 1 (FVM-HEVE)");
+  if (cfg->short_type == 1 && cfg->mixed_precision) return fail("the HEVE short step runs in FP64 only");
   shp::Model* M = new shp::Model();
   c.model = M;
   const size_t n3 = c.d.n3;
@@ -708,6 +710,7 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   M->dtl = cfg->dtl; M->dts = cfg->dts; M->nd_coef = cfg->nd_coef; M->nd_sfc_fact = cfg->nd_sfc_fact;
   M->divdmp_coef = cfg->divdmp_coef; M->fct = cfg->fct_tracer ? 1 : 0; M->fct_along = cfg->fct_along_stream ? 1 : 0;
   M->nd_coef_q = cfg->nd_coef_q;
+  M->heve = cfg->short_type == 1 ? 1 : 0;
   const int nstep = (int)ceil((cfg->dtl - shp::EPS_RP) / cfg->dts);
   if (int r = set_numfilter_tables(c, cfg->dtl / nstep, cfg->nd_coef, cfg->nd_sfc_fact)) return r;
   if (cfg->nd_coef_q != 0.0)

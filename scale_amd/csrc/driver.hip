@@ -422,15 +422,19 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B) {
     io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = B.num_diff;
     io.wdamp_coef = M.wdamp_coef; io.DDIV = B.DDIV; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
     io.divdmp_coef = M.divdmp_coef; io.dtrk = dtrk[r]; io.has_coriolis = M.has_coriolis;
-    // flat terrain: the last stage combines the four stages into the state itself
-    io.combine = (r == 3 && !c.terrain) ? 1 : 0;
+    // flat-terrain HEVI: the last stage combines the four stages into the state itself
+    io.combine = (r == 3 && !c.terrain && !M.heve) ? 1 : 0;
     for (int v = 0; v < 5; ++v) { io.c1[v] = B.Rk[0][v]; io.c2[v] = B.Rk[1][v]; io.c3[v] = B.Rk[2][v]; io.st[v] = B.st[v]; }
     for (int n = 0; n < 3; ++n) { io.cm[n] = B.mfR[n]; io.ct[n] = B.tfR[n]; }
     io.mflx_av = B.mflx; io.tflx_av = B.tflx;
-    HIPCHK(hevi_stage(io, s));
+    if constexpr (sizeof(R) == sizeof(double)) {
+      HIPCHK(M.heve ? heve_stage(io, s) : hevi_stage(io, s));
+    } else {
+      HIPCHK(hevi_stage(io, s));
+    }
     if (r < 3) HIPCHK(halo_fill(c, B.Rk[r], 5, s));
   }
-  if (!c.terrain) return halo_fill(c, B.st, 5, s);
+  if (!c.terrain && !M.heve) return halo_fill(c, B.st, 5, s);
   CombArgs<R> a;
   a.d = d;
   for (int v = 0; v < 5; ++v) {

@@ -38,6 +38,7 @@ struct Model {
   // configuration (PARAM_ATMOS_DYN subset)
   double dtl = 1.0, dts = 1.0, nd_coef = 1.0e-4, nd_sfc_fact = 1.0, divdmp_coef = 0.0, nd_coef_q = 0.0;
   int fct = 0, fct_along = 1;
+  int heve = 0;  // ATMOS_DYN_TYPE = FVM-HEVE: explicit stages, separate RK4 combination
   // every allocation, freed together
   std::vector<void*> allocs;
 };

@@ -47,7 +47,8 @@ class HipGrid(C.Structure):
 class DynConfig(C.Structure):
     _fields_ = [("dtl", C.c_double), ("dts", C.c_double), ("nd_coef", C.c_double), ("nd_sfc_fact", C.c_double),
                 ("divdmp_coef", C.c_double), ("nd_order", C.c_int32), ("fct_tracer", C.c_int32),
-                ("fct_along_stream", C.c_int32), ("mixed_precision", C.c_int32), ("nd_coef_q", C.c_double)]
+                ("fct_along_stream", C.c_int32), ("mixed_precision", C.c_int32), ("nd_coef_q", C.c_double),
+                ("short_type", C.c_int32)]
 
 
 _lib = None
@@ -226,9 +227,9 @@ class Context:
             _d(m["J33G"]), _p(m["MAPF"])))
 
     def model_create(self, dtl, dts, QA, nd_coef=1.0e-4, nd_sfc_fact=1.0, fct_tracer=False, fct_along_stream=True,
-                     mixed_precision=False, divdmp_coef=0.0, nd_coef_q=0.0):
+                     mixed_precision=False, divdmp_coef=0.0, nd_coef_q=0.0, short="hevi"):
         cfg = DynConfig(dtl, dts, nd_coef, nd_sfc_fact, divdmp_coef, 1, int(fct_tracer), int(fct_along_stream),
-                        int(mixed_precision), nd_coef_q)
+                        int(mixed_precision), nd_coef_q, {"hevi": 0, "heve": 1}[short])
         _check(load_library().scale_hip_model_create(self._h, C.byref(cfg), C.c_int32(QA)))
         self.has_model = True
         self.QA = QA
@@ -297,16 +298,16 @@ class Context:
 
 
 def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, device=0, mixed_precision=False,
-                    divdmp_coef=0.0, nd_coef_q=0.0) -> Context:
+                    divdmp_coef=0.0, nd_coef_q=0.0, short="hevi") -> Context:
     """Create a context holding the device-resident model of an inputs.Case
     (mixed_precision: FP32 dynamics with FP64 solve and reference state; the
-    tracers are left out)."""
+    tracers are left out; short = "hevi" or "heve", ATMOS_DYN_TYPE)."""
     if ctx is None:
         ctx = Context(case.grid, case.scheme, case.scheme_tracer, device)
     ctx.set_metrics(case.metric)
     qa = 0 if mixed_precision else case.qtrc.shape[-1]
     ctx.model_create(case.dt, case.dt, qa, nd_coef=nd_coef, fct_tracer=case.fct if fct is None else fct,
-                     mixed_precision=mixed_precision, divdmp_coef=divdmp_coef, nd_coef_q=nd_coef_q)
+                     mixed_precision=mixed_precision, divdmp_coef=divdmp_coef, nd_coef_q=nd_coef_q, short=short)
     for k, v in case.state.items():
         ctx.model_set(k, v)
     for k, v in case.ref.items():

@@ -249,6 +249,39 @@ def test_model_parity_divergence_damping(hip, terrain):
         ctx.close()
 
 
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+def test_model_parity_heve(hip, terrain):
+    """Large steps with the HEVE short step (ATMOS_DYN_TYPE = FVM-HEVE: four
+    explicit stages and the RK4 combination) and FCT tracers against the
+    oracle: 5 steps at dt = 0.25 s (inside the explicit acoustic limit of
+    dz = 250 m), 1e-12 relative to each field's scale."""
+    print(f"[model] HEVE, terrain={terrain}, 5 steps", flush=True)
+    case = inputs.make_case(kmax=20, imax=16, jmax=12, noise=0.01, ntracer=1, tracer_kind=["blob"], fct=True,
+                            bubble_r=2000.0, bubble_zc=2000.0, momz_noise=0.05, v0=-2.0, dt=0.25)
+    g = case.grid
+    if terrain:
+        case = inputs.Case(**{**case.__dict__, "metric": H.terrain_metrics(case, amp=0.02)})
+    m = po.Model(case, short="heve")
+    m.step(5)
+    ctx = hip.model_from_case(case, short="heve")
+    try:
+        ctx.model_large_step(5)
+        ctx.synchronize()
+        sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+        for k in H.VARS:
+            got = ctx.model_get(k)
+            ref = m.state[k]
+            scale = np.abs(ref[sl]).max()
+            d = np.abs(got[sl] - ref[sl]).max()
+            print(f"  {k}: max|diff| {d:.3e} (scale {scale:.3e})", flush=True)
+            assert d <= 1.0e-12 * scale, k
+        got = ctx.model_get(hip.QTRC_BASE)
+        ref = m.qtrc[..., 0]
+        assert np.abs(got[sl] - ref[sl]).max() <= 1.0e-12 * np.abs(ref[sl]).max()
+    finally:
+        ctx.close()
+
+
 def test_model_parity_warm_bubble(hip):
     """Device-resident large steps (dynamics RK4 + RK3 tracers with FCT) against
     the oracle integrator, 10 steps of a reduced warm bubble."""

@@ -489,6 +489,28 @@ def test_heve_rest_state_stays_at_rest():
     assert np.allclose(out["DENS_RK"][sl], case.state["DENS"][sl], rtol=1e-14, atol=0)
 
 
+def test_heve_model_conserves_mass_and_converges_to_hevi():
+    """Large steps with the HEVE short step conserve mass to round-off, and
+    over the same 1 s of model time the HEVE and HEVI integrations differ
+    by O(dt): halving the step halves the difference (they differ only in
+    the implicit treatment of the vertical acoustic terms)."""
+    diffs = []
+    for dt, n in ((0.1, 10), (0.05, 20)):
+        case = inputs.make_case(kmax=20, imax=12, jmax=10, noise=0.01, ntracer=1, tracer_kind=["blob"], fct=True,
+                                bubble_r=1500.0, bubble_zc=1500.0, momz_noise=0.05, dt=dt)
+        g = case.grid
+        sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+        me, mi = po.Model(case, short="heve"), po.Model(case, short="hevi")
+        m0 = me.mass(me.state["DENS"])
+        me.step(n)
+        mi.step(n)
+        assert abs(me.mass(me.state["DENS"]) - m0) <= 1e-13 * m0
+        diffs.append({k: np.abs(me.state[k][sl] - mi.state[k][sl]).max() / np.abs(mi.state[k][sl]).max()
+                      for k in H.VARS if k != "MOMY"})
+    for k, d in diffs[0].items():
+        assert 1.6 < d / diffs[1][k] < 2.4, (k, d, diffs[1][k])
+
+
 # ---------------------------------------------------------------------------
 def test_numfilter_two_grid_wave_closed_form():
     """For q = A(-1)^i in x the 3rd-difference flux is 8A(-1)^i/dx^3 and
