@@ -281,11 +281,14 @@ def main():
         mctx.model_short_step(a.steps)
         mctx.synchronize()
         m_el = time.perf_counter() - t0
-        m_stage = mctx.profile_query("hevi_stage")
+        m_kern = {}
+        for name in ("hevi_stage", "numfilter", "halo_fill"):
+            cnt, ms = mctx.profile_query(name)
+            m_kern[name] = ms / cnt if cnt else None
         mctx.profile(False)
         mctx.close()
         mixed = dict(value=ni * nj * nk * a.steps / m_el, ms_per_step=m_el * 1e3 / a.steps,
-                     stage_ms=m_stage[1] / max(m_stage[0], 1), speedup_vs_fp64=elapsed / m_el,
+                     stage_ms=m_kern["hevi_stage"], kernels_avg_ms=m_kern, speedup_vs_fp64=elapsed / m_el,
                      stage_bytes_per_point=MIXED_STAGE_BYTES_PER_POINT)
     adj = None
     if not a.no_adjoint:

@@ -123,8 +123,10 @@ def tstep_short_hevi(case, inp: dict, dtrk: float, scheme: int | None = None, ou
 
 
 def tstep_short_heve(case, inp: dict, dtrk: float, scheme: int | None = None, out: dict | None = None,
-                     fct_momentum=False, fct_t=False) -> dict:
-    """One HEVE stage (scale_atmos_dyn_tstep_short_fvm_heve.F90), same inputs as tstep_short_hevi."""
+                     fct_momentum=False, fct_t=False, along_stream=True) -> dict:
+    """One HEVE stage (scale_atmos_dyn_tstep_short_fvm_heve.F90), same inputs as tstep_short_hevi.
+    With fct_t the outputs DENS_RK and mflx_hi also get their periodic halos (the
+    exchanges of HEVE.F90:1390-1401)."""
     g = case.grid
     if out is None:
         out = {k: nan_like_outputs(g) for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK")}
@@ -138,6 +140,7 @@ def tstep_short_heve(case, inp: dict, dtrk: float, scheme: int | None = None, ou
                                 "DENS_t", "MOMZ_t", "MOMX_t", "MOMY_t", "RHOT_t", "DPRES0", "RT2P")],
         ptr(inp.get("CORIOLI", case.corioli)), ptr(inp["num_diff"]), ptr(inp.get("wdamp_coef", case.wdamp_coef)),
         C.c_double(inp.get("divdmp_coef", 0.0)), ptr(inp["DDIV"]), C.c_int(int(fct_momentum)), C.c_int(int(fct_t)),
+        C.c_int(int(along_stream)),
         C.byref(ms), ptr(inp["REF_dens"]), ptr(inp["REF_rhot"]), C.c_double(dtrk))
     if rc != 0:
         raise RuntimeError(f"orc_tstep_short_heve failed: {rc}")

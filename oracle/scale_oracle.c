@@ -748,10 +748,10 @@ int orc_tstep_short_heve(const orc_grid *g, int s,
     const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
     const double *DPRES0, const double *RT2P, const double *CORIOLI,
     const double *num_diff, const double *wdamp_coef, double divdmp_coef, const double *DDIV,
-    int flag_fct_momentum, int flag_fct_t,
+    int flag_fct_momentum, int flag_fct_t, int flag_fct_along_stream,
     const orc_metric *m, const double *REF_dens, const double *REF_rhot, double dtrk) {
   GRID_LOCALS;
-  if ((s != ORC_UD3 && s != ORC_CD4) || flag_fct_momentum || flag_fct_t) return -1;
+  if ((s != ORC_UD3 && s != ORC_CD4) || flag_fct_momentum) return -1;
   const double *CDZ = m->CDZ, *FDZ = m->FDZ, *FDX = m->FDX, *FDY = m->FDY;
   const double *RCDZ = m->RCDZ, *RCDX = m->RCDX, *RCDY = m->RCDY;
   const double *RFDZ = m->RFDZ, *RFDX = m->RFDX, *RFDY = m->RFDY;
@@ -959,6 +959,41 @@ int orc_tstep_short_heve(const orc_grid *g, int s,
                              + dtrk * ((advcv + advch) * MAPF(i, j, 0, I_XY) * MAPF(i, j, 1, I_XY) / GSQRT(k, i, j, I_XYZ)
                                        + RHOT_t[I3(k, i, j)]);
       }
+
+  /* FCT of the theta flux (HEVE.F90:1388-1475): halo exchange of mflx_hi and
+   * DENS_RK, low-order UD1 fluxes of the start-of-stage theta, ATMOS_DYN_fct
+   * and the anti-diffusive correction of RHOT_RK.  As in the reference, the
+   * start-of-stage theta overwrites POTT on IS-1..IE+2, JS-1..JE+2 only
+   * (HEVE.F90:1403-1409), so the i = IS-2 and j = JS-2 cells the UD1 fluxes
+   * at IS-2 / JS-2 read keep the theta of the current state. */
+  if (flag_fct_t) {
+    for (int c = 0; c < 3; ++c) orc_halo_periodic(g, mflx_hi + n3 * c);
+    orc_halo_periodic(g, DENS_RK);
+    double *POTT0 = POTT, *tflx_lo = alloc_nan(3 * n3), *tflx_anti = alloc_nan(3 * n3);
+    OMP_FOR
+    for (int j = JS - 1; j <= JE + 2; ++j)
+      for (int i = IS - 1; i <= IE + 2; ++i)
+        for (int k = KS; k <= KE; ++k) POTT0[I3(k, i, j)] = RHOT0[I3(k, i, j)] / DENS0[I3(k, i, j)];
+    const double *ndR = num_diff + n3 * (I_RHOT + 5 * ZDIR);
+    fluxZ_XYZ(g, ORC_UD1, tflx_lo, mfZ, POTT0, GS + n3 * I_XYZ, ndR, IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    fluxX_XYZ(g, ORC_UD1, tflx_lo + n3, mfX, POTT0, GS + n3 * I_UYZ, num_diff + n3 * (I_RHOT + 5 * XDIR),
+              IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    fluxY_XYZ(g, ORC_UD1, tflx_lo + 2 * n3, mfY, POTT0, GS + n3 * I_XVZ, num_diff + n3 * (I_RHOT + 5 * YDIR),
+              IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    orc_fct(g, tflx_anti, POTT0, DENS0, DENS_RK, tflx_hi, tflx_lo, mflx_hi, RCDZ, RCDX, RCDY, GS + n3 * I_XYZ,
+            MP + (size_t)IA * JA * 2 * I_XY, dtrk, flag_fct_along_stream);
+    const double *aZ = tflx_anti, *aX = tflx_anti + n3, *aY = tflx_anti + 2 * n3;
+    OMP_FOR
+    for (int j = JJS; j <= JJE; ++j)
+      for (int i = IIS; i <= IIE; ++i)
+        for (int k = KS; k <= KE; ++k)
+          RHOT_RK[I3(k, i, j)] = RHOT_RK[I3(k, i, j)]
+                               + dtrk * ((aZ[I3(k, i, j)] - aZ[I3(k - 1, i, j)]) * RCDZ[k]
+                                         + (aX[I3(k, i, j)] - aX[I3(k, i - 1, j)]) * RCDX[i]
+                                         + (aY[I3(k, i, j)] - aY[I3(k, i, j - 1)]) * RCDY[j])
+                                 * MAPF(i, j, 0, I_XY) * MAPF(i, j, 1, I_XY) / GSQRT(k, i, j, I_XYZ);
+    free(tflx_lo); free(tflx_anti);
+  }
   free(POTT); free(DPRES); free(qflx_hi); free(qflx_J13); free(qflx_J23);
   return 0;
 #undef GSQRT
@@ -1760,7 +1795,7 @@ int orc_tinteg_short_rk4(const orc_grid *g, int s, int heve,
       rc |= orc_tstep_short_heve(g, s, out[0], out[1], out[2], out[3], out[4], mo, to,
                                  S0[0], S0[1], S0[2], S0[3], S0[4], in[0], in[1], in[2], in[3], in[4],
                                  DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, DPRES0, RT2P, CORIOLI,
-                                 num_diff, wdamp_coef, divdmp_coef, DDIV, 0, 0, m, REF_dens, REF_rhot, dtrk_s[st]);
+                                 num_diff, wdamp_coef, divdmp_coef, DDIV, 0, 0, 1, m, REF_dens, REF_rhot, dtrk_s[st]);
     else
       rc |= orc_tstep_short_hevi(g, s, out[0], out[1], out[2], out[3], out[4], mo, to,
                                  S0[0], S0[1], S0[2], S0[3], S0[4], in[0], in[1], in[2], in[3], in[4],

@@ -79,7 +79,7 @@ int orc_tstep_short_heve(const orc_grid *g, int scheme,
     const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
     const double *DPRES0, const double *RT2P, const double *CORIOLI,
     const double *num_diff, const double *wdamp_coef, double divdmp_coef, const double *DDIV,
-    int flag_fct_momentum, int flag_fct_t,
+    int flag_fct_momentum, int flag_fct_t, int flag_fct_along_stream,
     const orc_metric *m, const double *REF_dens, const double *REF_rhot, double dtrk);
 
 /* ATMOS_DYN_numfilter_coef (COMMON.F90:385-985), ND_ORDER = 1 only,

@@ -377,12 +377,12 @@ static int short_stage(bool heve, scale_hip_ctx* h,
     const double* REF_dens, const double* REF_rhot,
     const int32_t* BND_W, const int32_t* BND_E, const int32_t* BND_S, const int32_t* BND_N,
     const double* dtrk, const int32_t* last) {
-  (void)PROG_RK; (void)PROG0; (void)PROG; (void)FLAG_FCT_ALONG_STREAM; (void)last;
+  (void)PROG_RK; (void)PROG0; (void)PROG; (void)last;
   if (int r = check_ctx(h)) return r;
   if (!dtrk || !divdmp_coef || !J33G) return fail("null scalar argument");
-  // HEVI ignores the FCT flags (HEVI.F90:224-226); HEVE's FCT branches are not built
-  if (heve && ((FLAG_FCT_MOMENTUM && *FLAG_FCT_MOMENTUM) || (FLAG_FCT_T && *FLAG_FCT_T)))
-    return fail("HEVE with FLAG_FCT_MOMENTUM or FLAG_FCT_T is not supported");
+  // HEVI ignores the FCT flags (HEVI.F90:224-226); HEVE's momentum FCT is not built
+  if (heve && FLAG_FCT_MOMENTUM && *FLAG_FCT_MOMENTUM)
+    return fail("HEVE with FLAG_FCT_MOMENTUM is not supported");
   if ((BND_W && *BND_W) || (BND_E && *BND_E) || (BND_S && *BND_S) || (BND_N && *BND_N))
     return fail("only periodic lateral boundaries (BND_* = .false.) are supported");
   Ctx& c = h->c;
@@ -418,6 +418,8 @@ static int short_stage(bool heve, scale_hip_ctx* h,
   io.DPRES0 = B[B_DPRES0]; io.RT2P = B[B_RT2P]; io.CORIOLI = B[B_COR]; io.num_diff = B[B_ND]; io.wdamp_coef = B[B_WD];
   io.DDIV = B[B_DDIV]; io.REF_dens = B[B_REFD]; io.REF_rhot = B[B_REFR];
   io.divdmp_coef = *divdmp_coef; io.dtrk = *dtrk; io.has_coriolis = has_cor;
+  io.fct_t = heve && FLAG_FCT_T && *FLAG_FCT_T;
+  io.fct_along = !FLAG_FCT_ALONG_STREAM || *FLAG_FCT_ALONG_STREAM;
   CK(heve ? shp::heve_stage(io, c.stream) : shp::hevi_stage(io, c.stream));
   for (int v = 0; v < 5; ++v)
     if (int r = download3(c, o[v], B[B_OUT + v], 1)) return r;

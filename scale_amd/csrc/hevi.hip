@@ -1350,12 +1350,16 @@ hipError_t heve_stage(const StageIO& io, hipStream_t s) {
   if (io.combine) return hipErrorNotSupported;
   const bool div = io.divdmp_coef != 0.0;
   const bool ter = c.terrain;
+  hipError_t e;
   if (c.scheme == UD3) {
-    if (ter) return div ? launch_heve<UD3, true, true>(a, s) : launch_heve<UD3, true, false>(a, s);
-    return div ? launch_heve<UD3, false, true>(a, s) : launch_heve<UD3, false, false>(a, s);
+    if (ter) e = div ? launch_heve<UD3, true, true>(a, s) : launch_heve<UD3, true, false>(a, s);
+    else e = div ? launch_heve<UD3, false, true>(a, s) : launch_heve<UD3, false, false>(a, s);
+  } else {
+    if (ter) e = div ? launch_heve<CD4, true, true>(a, s) : launch_heve<CD4, true, false>(a, s);
+    else e = div ? launch_heve<CD4, false, true>(a, s) : launch_heve<CD4, false, false>(a, s);
   }
-  if (ter) return div ? launch_heve<CD4, true, true>(a, s) : launch_heve<CD4, true, false>(a, s);
-  return div ? launch_heve<CD4, false, true>(a, s) : launch_heve<CD4, false, false>(a, s);
+  if (e != hipSuccess || !io.fct_t) return e;
+  return heve_fct_theta(io, s);
 }
 
 // flat-terrain FP64 stage through hevi_march, recording PT, St and the

@@ -62,6 +62,8 @@ struct StageIOT {
   const double *REF_dens, *REF_rhot;
   double divdmp_coef, dtrk;
   int has_coriolis;
+  // HEVE only: FLAG_FCT_T and FLAG_FCT_ALONG_STREAM
+  int fct_t = 0, fct_along = 1;
   // last RK4 stage only (flat terrain): combine with the outputs of stages
   // 1-3 on the fly, writing the new state into st (which may alias s0) and the
   // averaged fluxes into mflx_av/tflx_av, instead of o / the Z-free parts of
@@ -84,6 +86,8 @@ struct TracerIO {
 hipError_t hevi_stage(const StageIO& io, hipStream_t s);
 // the explicit HEVE stage: same inputs and outputs, FP64, no fused combine
 hipError_t heve_stage(const StageIO& io, hipStream_t s);
+// FCT of the HEVE theta flux, run by heve_stage after the stage kernel (tracer.hip)
+hipError_t heve_fct_theta(const StageIO& io, hipStream_t s);
 // mixed precision: flat terrain and divdmp_coef = 0 only
 hipError_t hevi_stage(const StageIOT<float>& io, hipStream_t s);
 // flat terrain, FP64, no divergence damping: the stage with PT, St and the

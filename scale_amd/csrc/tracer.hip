@@ -25,6 +25,7 @@ namespace shp {
 struct TrArgs {
   Dims d;
   const double *QTRC, *QTRC0, *RHOQ_t, *DENS0, *DENS, *mflx, *nd;
+  size_t nds;                     // distance between the z, x and y components of nd
   const double *GSQRT, *MAPF_XY;  // GSQRT(KA,IA,JA,7); MAPF(IA,JA,2) I_XY slice
   const double *RCDZ, *RCDX, *RCDY;
   double dtl;
@@ -66,13 +67,13 @@ struct TrOps {
     const double* Q = a.QTRC;
     size_t p = ix(k, ii, j);
     double vel = mX(p);
-    return vel * fv_int<S>(Q[ix(k, ii - 1, j)], Q[p], Q[ix(k, ii + 1, j)], Q[ix(k, ii + 2, j)], vel) + gs(I_UYZ, p) * a.nd[p + d.n3];
+    return vel * fv_int<S>(Q[ix(k, ii - 1, j)], Q[p], Q[ix(k, ii + 1, j)], Q[ix(k, ii + 2, j)], vel) + gs(I_UYZ, p) * a.nd[p + a.nds];
   }
   __device__ double hY(int k, int i, int jj) const {
     const double* Q = a.QTRC;
     size_t p = ix(k, i, jj);
     double vel = mY(p);
-    return vel * fv_int<S>(Q[ix(k, i, jj - 1)], Q[p], Q[ix(k, i, jj + 1)], Q[ix(k, i, jj + 2)], vel) + gs(I_XVZ, p) * a.nd[p + 2 * d.n3];
+    return vel * fv_int<S>(Q[ix(k, i, jj - 1)], Q[p], Q[ix(k, i, jj + 1)], Q[ix(k, i, jj + 2)], vel) + gs(I_XVZ, p) * a.nd[p + 2 * a.nds];
   }
   // low-order UD1 fluxes on QTRC0 (scale_atmos_dyn_fvm_flux_ud1.F90:135-249)
   __device__ double lZ(int kk, int i, int j) const {
@@ -272,7 +273,9 @@ __global__ void __launch_bounds__(256) fct_ratio(TrArgs a, unsigned ni_tiles, un
 }
 
 // ---- FCT pass 3: limited anti-diffusive fluxes and the update ------------------
-template <int S, bool TERRAIN>
+// THETA: the HEVE theta correction RHOT_RK += dtrk div(anti) (HEVE.F90:1459-1474)
+// instead of the tracer update
+template <int S, bool TERRAIN, bool THETA>
 __global__ void __launch_bounds__(256) fct_update(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
   const Dims& d = a.d;
   TrOps<S, TERRAIN> o(a);
@@ -304,6 +307,14 @@ __global__ void __launch_bounds__(256) fct_update(TrArgs a, unsigned ni_tiles, u
     return an * (1.0 - fmin2(RP[IX(d, k, i, jj + 1)], RM[IX(d, k, i, jj)]) * (dirsw)
                      - fmin2(RP[IX(d, k, i, jj)], RM[IX(d, k, i, jj + 1)]) * (1.0 - dirsw));
   };
+  if constexpr (THETA) {
+    a.QTRCo[p] = a.QTRCo[p]
+               + a.dtl * ((limZ(k) - limZ(k - 1)) * a.RCDZ[k]
+                          + (limX(i) - limX(i - 1)) * a.RCDX[i]
+                          + (limY(j) - limY(j - 1)) * a.RCDY[j])
+                 * o.m1(i, j) * o.m2(i, j) / o.gs(I_XYZ, p);
+    return;
+  }
   a.QTRCo[p] = (a.QTRC0[p] * a.DENS0[p]
                 + a.dtl * (-((o.hZ(k, i, j) - limZ(k) - o.hZ(k - 1, i, j) + limZ(k - 1)) * a.RCDZ[k]
                              + (o.hX(k, i, j) - limX(i) - o.hX(k, i - 1, j) + limX(i - 1)) * a.RCDX[i]
@@ -314,7 +325,7 @@ __global__ void __launch_bounds__(256) fct_update(TrArgs a, unsigned ni_tiles, u
 }
 
 // ---------------------------------------------------------------------------
-template <int S, bool TERRAIN>
+template <int S, bool TERRAIN, bool THETA = false>
 static hipError_t tracer_t(const Ctx& c, const TrArgs& a, bool fct, bool along, hipStream_t s) {
   const Dims& d = c.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
@@ -341,7 +352,7 @@ static hipError_t tracer_t(const Ctx& c, const TrArgs& a, bool fct, bool along, 
   e = halo_fill(c, r, 2, s);
   if (e != hipSuccess) return e;
   prof_start(c, "fct_update", s);
-  hipLaunchKernelGGL((fct_update<S, TERRAIN>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
+  hipLaunchKernelGGL((fct_update<S, TERRAIN, THETA>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
   prof_stop(c, s);
   return hipGetLastError();
 }
@@ -350,7 +361,7 @@ hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s) {
   TrArgs a;
   a.d = c.d;
   a.QTRC = io.QTRC; a.QTRC0 = io.QTRC0; a.RHOQ_t = io.RHOQ_t; a.DENS0 = io.DENS0; a.DENS = io.DENS;
-  a.mflx = io.mflx_hi; a.nd = io.num_diff; a.GSQRT = c.GSQRT; a.MAPF_XY = c.MAPF;
+  a.mflx = io.mflx_hi; a.nd = io.num_diff; a.nds = c.d.n3; a.GSQRT = c.GSQRT; a.MAPF_XY = c.MAPF;
   a.RCDZ = c.g1.RCDZ; a.RCDX = c.g1.RCDX; a.RCDY = c.g1.RCDY;
   a.dtl = io.dtl; a.QTRCo = io.QTRCo;
   a.phi_lo = c.scr[0]; a.rjpls = c.scr[1]; a.rjmns = c.scr[2];
@@ -359,6 +370,58 @@ hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s) {
   if (c.scheme_tracer == UD3KOREN)
     return c.terrain ? tracer_t<UD3KOREN, true>(c, a, fct, along, s) : tracer_t<UD3KOREN, false>(c, a, fct, along, s);
   return c.terrain ? tracer_t<CD4, true>(c, a, fct, along, s) : tracer_t<CD4, false>(c, a, fct, along, s);
+}
+
+// ---------------------------------------------------------------------------
+// FCT of the HEVE theta flux (HEVE.F90:1388-1475).  The limiter is the
+// tracer's with theta as the advected field: the high-order fluxes are the
+// stage's (scheme of the dynamics on the current theta with num_diff(RHOT)),
+// the low-order ones UD1 on the start-of-stage theta, the densities DENS0 and
+// DENS_RK, and the update corrects RHOT_RK.  POTT keeps the reference's
+// composition: the current theta on the whole halo, overwritten by the
+// start-of-stage theta on IS-1..IE+2, JS-1..JE+2 (HEVE.F90:360-372, 1403-1409).
+__global__ void __launch_bounds__(256) heve_pott(Dims d, const double* RHOT, const double* DENS, const double* RHOT0,
+                                                 const double* DENS0, double* POTTc, double* POTT0) {
+  const int i = d.IS - 2 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int k = d.KS + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int j = d.JS - 2 + (int)blockIdx.z;
+  if (i > d.IE + 2 || k > d.KE || j > d.JE + 2) return;
+  const size_t p = IX(d, k, i, j);
+  const double pc = RHOT[p] / DENS[p];
+  POTTc[p] = pc;
+  POTT0[p] = (i >= d.IS - 1 && j >= d.JS - 1) ? RHOT0[p] / DENS0[p] : pc;
+}
+
+hipError_t heve_fct_theta(const StageIO& io, hipStream_t s) {
+  const Ctx& c = *io.ctx;
+  const Dims& d = c.d;
+  double* mf[3] = {io.mflx_hi, io.mflx_hi + d.n3, io.mflx_hi + 2 * d.n3};
+  hipError_t e = halo_fill(c, mf, 3, s);
+  if (e != hipSuccess) return e;
+  double* dk[1] = {io.o[0]};
+  e = halo_fill(c, dk, 1, s);
+  if (e != hipSuccess) return e;
+  TrArgs a;
+  a.d = d;
+  double* POTTc = c.scr[3];
+  double* POTT0 = c.scr[4];
+  {
+    const int ni = d.IE - d.IS + 5, nk = d.KE - d.KS + 1, nj = d.JE - d.JS + 5;
+    dim3 grid((ni + 63) / 64, (nk + 3) / 4, nj);
+    hipLaunchKernelGGL(heve_pott, grid, dim3(64, 4), 0, s, d, io.s[4], io.s[0], io.s0[4], io.s0[0], POTTc, POTT0);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  a.QTRC = POTTc; a.QTRC0 = POTT0; a.RHOQ_t = nullptr; a.DENS0 = io.s0[0]; a.DENS = io.o[0];
+  a.mflx = io.mflx_hi; a.nd = io.num_diff + d.n3 * I_RHOT; a.nds = 5 * d.n3;
+  a.GSQRT = c.GSQRT; a.MAPF_XY = c.MAPF;
+  a.RCDZ = c.g1.RCDZ; a.RCDX = c.g1.RCDX; a.RCDY = c.g1.RCDY;
+  a.dtl = io.dtrk; a.QTRCo = io.o[4];
+  a.phi_lo = c.scr[0]; a.rjpls = c.scr[1]; a.rjmns = c.scr[2];
+  const bool along = io.fct_along != 0;
+  if (c.scheme == UD3)
+    return c.terrain ? tracer_t<UD3, true, true>(c, a, true, along, s) : tracer_t<UD3, false, true>(c, a, true, along, s);
+  return c.terrain ? tracer_t<CD4, true, true>(c, a, true, along, s) : tracer_t<CD4, false, true>(c, a, true, along, s);
 }
 
 }  // namespace shp

@@ -28,7 +28,7 @@ def hip():
     return shp
 
 
-def run_stage_gpu(hip, case, inp, dtrk, metric=None, scheme=None, heve=False, fct=(False, False)):
+def run_stage_gpu(hip, case, inp, dtrk, metric=None, scheme=None, heve=False, fct=(False, False), along=True):
     g = case.grid
     m = metric or case.metric
     out = {k: po.nan_like_outputs(g) for k in OUTS[:5]}
@@ -42,17 +42,19 @@ def run_stage_gpu(hip, case, inp, dtrk, metric=None, scheme=None, heve=False, fc
             inp["DENS"], inp["MOMZ"], inp["MOMX"], inp["MOMY"], inp["RHOT"],
             inp["DENS_t"], inp["MOMZ_t"], inp["MOMX_t"], inp["MOMY_t"], inp["RHOT_t"], None, None,
             inp["DPRES0"], inp["RT2P"], inp.get("CORIOLI", case.corioli), inp["num_diff"], inp.get("wdamp_coef", case.wdamp_coef),
-            inp.get("divdmp_coef", 0.0), inp["DDIV"], fct[0], fct[1], True,
+            inp.get("divdmp_coef", 0.0), inp["DDIV"], fct[0], fct[1], along,
             m["CDZ"], m["FDZ"], m["FDX"], m["FDY"], m["RCDZ"], m["RCDX"], m["RCDY"], m["RFDZ"], m["RFDX"], m["RFDY"],
             m["PHI"], m["GSQRT"], m["J13G"], m["J23G"], m["J33G"], m["MAPF"], inp["REF_dens"], inp["REF_rhot"],
             False, False, False, False, dtrk, False)
     return out
 
 
-def oracle_stage(case, inp, dtrk, metric=None, scheme=None, heve=False):
+def oracle_stage(case, inp, dtrk, metric=None, scheme=None, heve=False, **kw):
     if metric is not None:
         case = inputs.Case(**{**case.__dict__, "metric": metric})
-    return (po.tstep_short_heve if heve else po.tstep_short_hevi)(case, inp, dtrk, scheme=scheme)
+    if heve:
+        return po.tstep_short_heve(case, inp, dtrk, scheme=scheme, **kw)
+    return po.tstep_short_hevi(case, inp, dtrk, scheme=scheme)
 
 
 @pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
@@ -117,13 +119,34 @@ def test_heve_stage_divdmp(hip, terrain):
         H.compare(ref[k], got[k], k)
 
 
-def test_heve_rejects_fct_flags(hip):
-    print("[heve stage] FCT flags rejected", flush=True)
+@pytest.mark.parametrize("along", [True, False], ids=["along_stream", "7point"])
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+def test_heve_stage_fct_theta_bitwise(hip, terrain, along):
+    """HEVE with FLAG_FCT_T (HEVE.F90:1388-1475) on a 10 % theta top-hat:
+    the limited RHOT_RK and the halo-exchanged DENS_RK and mflx_hi are
+    bit-identical to the oracle."""
+    print(f"[heve stage] FCT theta terrain={terrain} along={along}", flush=True)
+    case = H.small_case(seed=41, u0=15.0, v0=-8.0)
+    g = case.grid
+    inp = H.stage_inputs(case, seed=3, coriolis=terrain)
+    for key in ("RHOT0", "RHOT"):
+        a = np.array(inp[key], order="F", copy=True)
+        a[g.KS + 3:g.KS + 9, g.IS + 2:g.IS + 7, g.JS + 2:g.JS + 6] *= 1.1
+        inputs.halo_fill(g, a)
+        inp[key] = a
+    metric = H.terrain_metrics(case, amp=0.02) if terrain else None
+    ref = oracle_stage(case, inp, 2.0, metric, 4, heve=True, fct_t=True, along_stream=along)
+    got = run_stage_gpu(hip, case, inp, 2.0, metric, 4, heve=True, fct=(False, True), along=along)
+    for k in OUTS:
+        H.compare(ref[k], got[k], k)
+
+
+def test_heve_rejects_fct_momentum(hip):
+    print("[heve stage] FLAG_FCT_MOMENTUM rejected", flush=True)
     case = H.small_case(kmax=8, imax=6, jmax=6)
     inp = H.stage_inputs(case)
-    for fct in ((True, False), (False, True)):
-        with pytest.raises(hip.ScaleHipError, match="FCT"):
-            run_stage_gpu(hip, case, inp, 0.5, heve=True, fct=fct)
+    with pytest.raises(hip.ScaleHipError, match="FCT"):
+        run_stage_gpu(hip, case, inp, 0.5, heve=True, fct=(True, False))
 
 
 def test_numfilter_bitwise(hip):

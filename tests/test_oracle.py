@@ -463,12 +463,48 @@ def test_heve_divergence_damping_matches_hevi():
     assert np.abs(x - y).max() < 1e-2 * np.abs(x).max()
 
 
-def test_heve_rejects_fct_flags():
+def test_heve_rejects_fct_momentum():
     case = H.small_case(kmax=8, imax=6, jmax=6)
     inp = H.stage_inputs(case)
-    for kw in ({"fct_momentum": True}, {"fct_t": True}):
-        with pytest.raises(RuntimeError):
-            po.tstep_short_heve(case, inp, 0.5, **kw)
+    with pytest.raises(RuntimeError):
+        po.tstep_short_heve(case, inp, 0.5, fct_momentum=True)
+
+
+def _theta_tophat_inputs():
+    case = H.small_case(seed=41, u0=15.0, v0=-8.0)
+    g = case.grid
+    inp = H.stage_inputs(case, seed=3, tend=False)
+    for key in ("RHOT0", "RHOT"):
+        a = np.array(inp[key], order="F", copy=True)
+        a[g.KS + 3:g.KS + 9, g.IS + 2:g.IS + 7, g.JS + 2:g.JS + 6] *= 1.1
+        inputs.halo_fill(g, a)
+        inp[key] = a
+    return case, inp
+
+
+@pytest.mark.parametrize("along", [True, False], ids=["along_stream", "7point"])
+def test_heve_fct_t_bounds_theta_and_conserves(along):
+    """FLAG_FCT_T (HEVE.F90:1388-1475): with a 10 % theta top-hat the HEVE
+    stage without FCT over- and undershoots the initial theta range; with
+    it, theta = RHOT_RK/DENS_RK stays inside that range (the reference's
+    test_fctminmax property for theta), RHOT stays conserved, and every
+    other output is unchanged."""
+    case, inp = _theta_tophat_inputs()
+    g = case.grid
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    th0 = (inp["RHOT0"] / inp["DENS0"])[sl]
+    m = po.Model(case)
+    plain = po.tstep_short_heve(case, inp, 2.0, scheme=CD4)
+    fct = po.tstep_short_heve(case, inp, 2.0, scheme=CD4, fct_t=True, along_stream=along)
+    th_p = plain["RHOT_RK"][sl] / plain["DENS_RK"][sl]
+    th_f = fct["RHOT_RK"][sl] / fct["DENS_RK"][sl]
+    assert th_p.max() > th0.max() + 0.1 and th_p.min() < th0.min() - 0.01  # the control
+    assert th_f.max() <= th0.max() + 1e-12 and th_f.min() >= th0.min() - 1e-12
+    r0 = m.mass(np.asfortranarray(inp["RHOT0"]))
+    r1 = m.mass(np.asfortranarray(np.where(np.isnan(fct["RHOT_RK"]), 0.0, fct["RHOT_RK"])))
+    assert abs(r1 - r0) <= 2e-15 * r0 * g.KA
+    for k in ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "tflx_hi"):
+        assert np.array_equal(plain[k][sl], fct[k][sl]), k
 
 
 def test_heve_rest_state_stays_at_rest():
