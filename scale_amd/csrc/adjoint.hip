@@ -26,6 +26,8 @@
 #include <cmath>
 #include <vector>
 
+#include <stdlib.h>
+
 #include "common.h"
 #include "model.h"
 #include "shp_internal.h"
@@ -125,8 +127,21 @@ enum {
   NW
 };
 
+// What the reverse of one large step reads from its forward: the stage
+// outputs R1..R3, num_diff, RT2P/DPRES0/REF_rhot, and the PT, St and C of
+// the four stages.
+struct StepRec {
+  double* R[3][5] = {};
+  double* nd = nullptr;
+  double *RT2P = nullptr, *DPRES0 = nullptr, *REF_rhot = nullptr;
+  double* tr[4][3] = {};
+};
+
 struct AdjWork {
   int nsteps = 0, every = 0, recorded = 0;
+  // one record per step of a segment when the device has room for them
+  // (45 fields per step); otherwise each reversed step recomputes its own
+  std::vector<StepRec> recs;
   std::vector<double*> ckpt;  // states at steps 0, every, 2*every, ... (5 arrays each)
   std::vector<double*> seg;   // states of the segment being reversed (every x 5)
   double *xb[5] = {}, *S0b[5] = {}, *Rb[4][5] = {};
@@ -196,6 +211,22 @@ hipError_t adj_setup(Ctx& c, int nsteps, int every) {
       if (!p) return hipErrorOutOfMemory;
       A.seg.push_back(p);
     }
+    A.recs.clear();
+    std::vector<StepRec> recs(every);
+    // SCALE_HIP_ADJ_RECORDS=0 keeps the per-step recomputation
+    const char* env = getenv("SCALE_HIP_ADJ_RECORDS");
+    bool room = !env || atoi(env) != 0;
+    for (int e = 0; e < every && room; ++e) {
+      StepRec& r = recs[e];
+      for (int q = 0; q < 3 && room; ++q)
+        for (int v = 0; v < 5 && room; ++v) room = (r.R[q][v] = aalloc(A, n3)) != nullptr;
+      room = room && (r.nd = aalloc(A, 15 * n3)) && (r.RT2P = aalloc(A, n3)) && (r.DPRES0 = aalloc(A, n3)) &&
+             (r.REF_rhot = aalloc(A, n3));
+      for (int q = 0; q < 4 && room; ++q)
+        for (int t = 0; t < 3 && room; ++t) room = (r.tr[q][t] = aalloc(A, n3)) != nullptr;
+    }
+    (void)hipGetLastError();  // a failed allocation only means: no records
+    if (room) A.recs = recs;
     A.nsteps = nsteps; A.every = every;
   }
   return hipSuccess;
@@ -901,41 +932,67 @@ hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const d
 // large step (LARGE.F90:377-1127 with RK4.F90:300-660), reverse of one step
 // from the state x at its start; xb: adjoint of the state after the step on
 // entry, of x on return
-static hipError_t step_adj(Ctx& c, double* const x[5], double* const xb[5]) {
+// Forward of one large step from x with everything its reverse reads kept
+// in rec; with xnext, the new state (RK4 combination, ghost levels, halos)
+static hipError_t step_fwd(Ctx& c, double* const x[5], const StepRec& rec, double* const xnext[5]) {
+  Model& M = *c.model;
+  AdjWork& A = *c.adj;
+  const Dims d = c.d;
+  hipStream_t s = c.stream;
+  const size_t bytes = d.n3 * sizeof(double);
+  double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
+  for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(st[v], x[v], bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(model_prepare(c));
+  if (rec.RT2P != M.RT2P) {
+    HIPCHK(hipMemcpyAsync(rec.RT2P, M.RT2P, bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(rec.DPRES0, M.DPRES0, bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(rec.REF_rhot, M.REF_rhot, bytes, hipMemcpyDeviceToDevice, s));
+  }
+  if (M.nd_coef == 0.0) {
+    HIPCHK(hipMemsetAsync(rec.nd, 0, 15 * bytes, s));
+  } else {
+    const double* xs[5] = {x[0], x[1], x[2], x[3], x[4]};
+    HIPCHK(numfilter(c, rec.nd, xs, M.REF_dens, M.REF_pott, s));
+  }
+  const double dtrk_f[4] = {M.dts / 2.0, M.dts / 2.0, M.dts, M.dts};
+  double* R4[5];
+  for (int v = 0; v < 5; ++v) R4[v] = A.w[W_O + v];
+  for (int r = 0; r < 4; ++r) {
+    StageIO f{};
+    f.ctx = &c;
+    for (int v = 0; v < 5; ++v) {
+      f.o[v] = r < 3 ? rec.R[r][v] : R4[v];
+      f.s0[v] = x[v];
+      f.s[v] = r == 0 ? x[v] : rec.R[r - 1][v];
+      f.t[v] = M.t[v];
+    }
+    f.mflx_hi = A.mf; f.tflx_hi = A.tf;
+    f.DPRES0 = rec.DPRES0; f.RT2P = rec.RT2P; f.CORIOLI = M.CORIOLI; f.num_diff = rec.nd;
+    f.wdamp_coef = M.wdamp_coef; f.DDIV = nullptr; f.REF_dens = M.REF_dens; f.REF_rhot = rec.REF_rhot;
+    f.divdmp_coef = 0.0; f.dtrk = dtrk_f[r]; f.has_coriolis = M.has_coriolis; f.combine = 0;
+    HIPCHK(hevi_stage_traced(f, rec.tr[r][0], rec.tr[r][1], rec.tr[r][2], s));
+    if (r < 3) HIPCHK(halo_fill(c, rec.R[r], 5, s));
+  }
+  if (!xnext) return hipSuccess;
+  double* Rk[4][5];
+  for (int v = 0; v < 5; ++v) {
+    for (int r = 0; r < 3; ++r) Rk[r][v] = rec.R[r][v];
+    Rk[3][v] = R4[v];
+  }
+  const double* s0[5] = {x[0], x[1], x[2], x[3], x[4]};
+  return rk4_combine_state(c, xnext, s0, Rk, A.mf, A.tf, s);
+}
+
+// Reverse of one large step: xb holds the adjoint of the state after the
+// step on entry and of x on return
+static hipError_t step_rev(Ctx& c, double* const x[5], double* const xb[5], const StepRec& rec) {
   Model& M = *c.model;
   AdjWork& A = *c.adj;
   const Dims d = c.d;
   hipStream_t s = c.stream;
   const size_t n3 = d.n3, bytes = n3 * sizeof(double);
   const int KS = d.KS, KE = d.KE;
-  double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
-  // forward recomputation: RT2P/DPRES0/REF_rhot, num_diff, and the four
-  // stages with PT, St and C recorded (the outputs R1..R3 with their halos)
-  for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(st[v], x[v], bytes, hipMemcpyDeviceToDevice, s));
-  HIPCHK(model_prepare(c));
-  if (M.nd_coef == 0.0) {
-    HIPCHK(hipMemsetAsync(M.num_diff, 0, 15 * bytes, s));
-  } else {
-    const double* xs[5] = {x[0], x[1], x[2], x[3], x[4]};
-    HIPCHK(numfilter(c, M.num_diff, xs, M.REF_dens, M.REF_pott, s));
-  }
-  const double dtrk_f[4] = {M.dts / 2.0, M.dts / 2.0, M.dts, M.dts};
-  for (int r = 0; r < 4; ++r) {
-    StageIO f{};
-    f.ctx = &c;
-    for (int v = 0; v < 5; ++v) {
-      f.o[v] = r < 3 ? M.R[r][v] : A.w[W_O + v];
-      f.s0[v] = x[v];
-      f.s[v] = r == 0 ? x[v] : M.R[r - 1][v];
-      f.t[v] = M.t[v];
-    }
-    f.mflx_hi = A.mf; f.tflx_hi = A.tf;
-    f.DPRES0 = M.DPRES0; f.RT2P = M.RT2P; f.CORIOLI = M.CORIOLI; f.num_diff = M.num_diff;
-    f.wdamp_coef = M.wdamp_coef; f.DDIV = nullptr; f.REF_dens = M.REF_dens; f.REF_rhot = M.REF_rhot;
-    f.divdmp_coef = 0.0; f.dtrk = dtrk_f[r]; f.has_coriolis = M.has_coriolis; f.combine = 0;
-    HIPCHK(hevi_stage_traced(f, A.tr[r][0], A.tr[r][1], A.tr[r][2], s));
-    if (r < 3) HIPCHK(halo_fill(c, M.R[r], 5, s));
-  }
+  (void)n3;
 
   // halo fill and ghost levels of the new state (LARGE.F90:908-927)
   for (int v = 0; v < 5; ++v) HIPCHK(halo_adj(c, xb[v], s));
@@ -975,23 +1032,23 @@ static hipError_t step_adj(Ctx& c, double* const x[5], double* const xb[5]) {
     io.ctx = &c;
     for (int v = 0; v < 5; ++v) {
       io.s0[v] = x[v];
-      io.s[v] = r == 0 ? x[v] : M.R[r - 1][v];
+      io.s[v] = r == 0 ? x[v] : rec.R[r - 1][v];
       io.t[v] = M.t[v];
       io.ob[v] = A.Rb[r][v];
       io.s0b[v] = A.S0b[v];
       io.sb[v] = r == 0 ? A.S0b[v] : A.Rb[r - 1][v];
     }
-    io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = M.num_diff;
-    io.wdamp_coef = M.wdamp_coef; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
+    io.DPRES0 = rec.DPRES0; io.RT2P = rec.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = rec.nd;
+    io.wdamp_coef = M.wdamp_coef; io.REF_dens = M.REF_dens; io.REF_rhot = rec.REF_rhot;
     io.dtrk = dtrk[r]; io.has_coriolis = M.has_coriolis;
     io.NDb = A.NDb; io.DPRES0b = A.Pb[0]; io.RT2Pb = A.Pb[1]; io.REFrtb = A.Pb[2];
-    io.trPT = A.tr[r][0]; io.trSt = A.tr[r][1]; io.trCp = A.tr[r][2];
+    io.trPT = rec.tr[r][0]; io.trSt = rec.tr[r][1]; io.trCp = rec.tr[r][2];
     HIPCHK(hevi_stage_adj(c, io, s));
   }
   // RT2P, DPRES0, REF_rhot as functions of RHOT at the step start (LARGE.F90:443-471)
   {
     const double CPovCV = CPDRY / (CPDRY - RDRY);
-    const double *DP0b = A.Pb[0], *R2Pb = A.Pb[1], *RRb = A.Pb[2], *R2P = M.RT2P, *RT = x[4];
+    const double *DP0b = A.Pb[0], *R2Pb = A.Pb[1], *RRb = A.Pb[2], *R2P = rec.RT2P, *RT = x[4];
     double* RTb = A.S0b[4];
     double* DP0bw = A.Pb[0];
     HIPCHK(for3(Box{0, 0, 0, d.IA - 1, 0, d.JA - 1}, [=] __device__(int, int i, int j) {
@@ -1009,6 +1066,20 @@ static hipError_t step_adj(Ctx& c, double* const x[5], double* const xb[5]) {
   }
   for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(xb[v], A.S0b[v], bytes, hipMemcpyDeviceToDevice, s));
   return hipSuccess;
+}
+
+// the single record the reverse uses when the segment's records do not fit:
+// the model's own work arrays and the stage traces of AdjWork
+static StepRec model_rec(Ctx& c) {
+  Model& M = *c.model;
+  AdjWork& A = *c.adj;
+  StepRec r;
+  for (int q = 0; q < 3; ++q)
+    for (int v = 0; v < 5; ++v) r.R[q][v] = M.R[q][v];
+  r.nd = M.num_diff; r.RT2P = M.RT2P; r.DPRES0 = M.DPRES0; r.REF_rhot = M.REF_rhot;
+  for (int q = 0; q < 4; ++q)
+    for (int t = 0; t < 3; ++t) r.tr[q][t] = A.tr[q][t];
+  return r;
 }
 
 static hipError_t check_adjoint_model(const Ctx& c) {
@@ -1055,20 +1126,39 @@ hipError_t model_adjoint(Ctx& c, double* const xb[5]) {
   for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(keep[v], st[v], bytes, hipMemcpyDeviceToDevice, s));
   prof_start(c, "model_adjoint", s);
   const int nseg = (A.nsteps + A.every - 1) / A.every;
+  const bool recs = (int)A.recs.size() == A.every;
+  const StepRec one = model_rec(c);
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int n0 = sg * A.every, n1 = std::min(A.nsteps, n0 + A.every);
-    // recompute the states of the segment from its checkpoint
     for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[v], A.ckpt[5 * sg + v], bytes, hipMemcpyDeviceToDevice, s));
-    for (int n = n0 + 1; n < n1; ++n) {
-      for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(st[v], A.seg[5 * (n - 1 - n0) + v], bytes, hipMemcpyDeviceToDevice, s));
-      HIPCHK(model_prepare(c));
-      HIPCHK(model_short_step_rk4(c, M.dts));
-      for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[5 * (n - n0) + v], st[v], bytes, hipMemcpyDeviceToDevice, s));
-    }
-    for (int n = n1 - 1; n >= n0; --n) {
-      double* x[5];
-      for (int v = 0; v < 5; ++v) x[v] = A.seg[5 * (n - n0) + v];
-      HIPCHK(step_adj(c, x, xb));
+    auto seg = [&](int n, double** x) { for (int v = 0; v < 5; ++v) x[v] = A.seg[5 * (n - n0) + v]; };
+    if (recs) {
+      // one recording forward over the segment, then the reverse steps
+      for (int n = n0; n < n1; ++n) {
+        double *x[5], *xn[5];
+        seg(n, x);
+        if (n + 1 < n1) seg(n + 1, xn);
+        HIPCHK(step_fwd(c, x, A.recs[n - n0], n + 1 < n1 ? xn : nullptr));
+      }
+      for (int n = n1 - 1; n >= n0; --n) {
+        double* x[5];
+        seg(n, x);
+        HIPCHK(step_rev(c, x, xb, A.recs[n - n0]));
+      }
+    } else {
+      // recompute the states of the segment, then each reversed step its own record
+      for (int n = n0 + 1; n < n1; ++n) {
+        for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(st[v], A.seg[5 * (n - 1 - n0) + v], bytes, hipMemcpyDeviceToDevice, s));
+        HIPCHK(model_prepare(c));
+        HIPCHK(model_short_step_rk4(c, M.dts));
+        for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[5 * (n - n0) + v], st[v], bytes, hipMemcpyDeviceToDevice, s));
+      }
+      for (int n = n1 - 1; n >= n0; --n) {
+        double* x[5];
+        seg(n, x);
+        HIPCHK(step_fwd(c, x, one, nullptr));
+        HIPCHK(step_rev(c, x, xb, one));
+      }
     }
   }
   prof_stop(c, s);

@@ -451,6 +451,24 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B) {
   return halo_fill(c, B.st, 5, s);
 }
 
+// RK4 combination of four stage outputs into out (with the ghost levels and
+// the periodic halos), as the last stage of a short step leaves the state;
+// the flux averages go to the scratch mf/tf (3 n3 each), read as every stage's
+hipError_t rk4_combine_state(Ctx& c, double* const out[5], const double* const s0[5], double* const Rk[4][5], double* mf,
+                             double* tf, hipStream_t s) {
+  CombArgs<double> a;
+  a.d = c.d;
+  for (int v = 0; v < 5; ++v) {
+    a.st[v] = out[v]; a.r1[v] = Rk[0][v]; a.r2[v] = Rk[1][v]; a.r3[v] = Rk[2][v]; a.r4[v] = Rk[3][v]; a.s0[v] = s0[v];
+  }
+  a.mflx = mf; a.tflx = tf;
+  a.m1 = a.m2 = a.m3 = a.m4 = mf;
+  a.t1 = a.t2 = a.t3 = a.t4 = tf;
+  hipLaunchKernelGGL(rk4_combine<double>, grid_int(c.d), dim3(64, 4), 0, s, a);
+  HIPCHK(hipGetLastError());
+  return halo_fill(c, out, 5, s);
+}
+
 hipError_t model_short_step_rk4(Ctx& c, double dts) {
   Model& M = *c.model;
   if (M.mixed) {

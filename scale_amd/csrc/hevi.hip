@@ -1179,6 +1179,9 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       }
       a.DENS_RK[g] = Sr;  // Sr/St wait in the outputs until the back pass
       a.RHOT_RK[g] = St;
+      if constexpr (!CMB && sizeof(R) == sizeof(double)) {
+        if (a.trSt) a.trSt[g] = St;
+      }
       if (k <= KE - 1) { a.mflx_hi[g] = mZ; a.tflx_hi[g] = tZ; a.PT[g] = (double)PT; }
       else { a.mflx_hi[g] = R(0); a.tflx_hi[g] = R(0); }
       if (k == KS) { a.mflx_hi[g - d.pl] = R(0); a.tflx_hi[g - d.pl] = R(0); }
@@ -1245,6 +1248,9 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
     const R z0 = a.MOMZ0[q];
     put_zflux(q, R((double)mfZ[q] + J33G * Cn), R((double)tfZ[q] + J33G * Cn * ptn));
     put_state(1, q, R((double)z0 + (Cn - (double)a.MOMZ[q])), z0, KE - 1);
+    if constexpr (!CMB && sizeof(R) == sizeof(double)) {
+      if (a.trCp) a.trCp[q] = Cn;
+    }
   }
   for (int k = KE - 2; k >= KS; --k) {
     const size_t q = px(k), q1 = px(k + 1);
@@ -1253,6 +1259,9 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
     const R z0 = a.MOMZ0[q];
     put_zflux(q, R((double)mfZ[q] + J33G * Ck), R((double)tfZ[q] + J33G * Ck * ptk));
     put_state(1, q, R((double)z0 + (Ck - (double)a.MOMZ[q])), z0, k);
+    if constexpr (!CMB && sizeof(R) == sizeof(double)) {
+      if (a.trCp) a.trCp[q] = Ck;
+    }
     const R d0 = a.DENS0[q1], r0 = a.RHOT0[q1];
     double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1];
     put_state(0, q1, R((double)d0 + dtrk * (advcv + (double)a.DENS_RK[q1])), d0, k + 1);
@@ -1362,7 +1371,7 @@ hipError_t heve_stage(const StageIO& io, hipStream_t s) {
   return heve_fct_theta(io, s);
 }
 
-// flat-terrain FP64 stage through hevi_march, recording PT, St and the
+// flat-terrain FP64 stage through hevi_tile, recording PT, St and the
 // solved vertical momentum C for the adjoint
 hipError_t hevi_stage_traced(const StageIO& io, double* PT, double* St, double* Cp, hipStream_t s) {
   HeviArgs a;
@@ -1372,10 +1381,10 @@ hipError_t hevi_stage_traced(const StageIO& io, double* PT, double* St, double* 
   a.PT = PT; a.trSt = St; a.trCp = Cp;
   const Dims& d = c.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
-  constexpr unsigned TJ = 4;
+  constexpr int TJ = 8;
   const unsigned ni = (imax + 63) / 64, nb = ni * ((jmax + TJ - 1) / TJ);
-  if (c.scheme == UD3) hipLaunchKernelGGL((hevi_march<UD3, false, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
-  else hipLaunchKernelGGL((hevi_march<CD4, false, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  if (c.scheme == UD3) hipLaunchKernelGGL((hevi_tile<UD3, false, TJ, double, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+  else hipLaunchKernelGGL((hevi_tile<CD4, false, TJ, double, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   return hipGetLastError();
 }
 

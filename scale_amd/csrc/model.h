@@ -51,6 +51,8 @@ hipError_t model_state_to_f64(Ctx& c);
 hipError_t model_state_to_f32(Ctx& c);
 // adjoint (adjoint.hip): flat terrain, FP64, divdmp_coef = 0, one short step per large step
 hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every);
+hipError_t rk4_combine_state(Ctx& c, double* const out[5], const double* const s0[5], double* const Rk[4][5], double* mf,
+                             double* tf, hipStream_t s);
 hipError_t model_adjoint(Ctx& c, double* const xb[5]);
 
 }  // namespace shp

@@ -49,8 +49,13 @@ def test_stage_adjoint_matches_oracle(scheme):
         assert close(got[key], ref[key]), key
 
 
-def test_model_adjoint_matches_oracle():
-    print("[adjoint] model 8x8x10, 3 steps, checkpoint every 2", flush=True)
+@pytest.mark.parametrize("records", ["1", "0"], ids=["segment_records", "per_step_recompute"])
+def test_model_adjoint_matches_oracle(records, monkeypatch):
+    """Both reverse sweeps: one recording forward per segment (the default)
+    and, with SCALE_HIP_ADJ_RECORDS=0, the per-step recomputation used when
+    the records do not fit in device memory."""
+    monkeypatch.setenv("SCALE_HIP_ADJ_RECORDS", records)
+    print(f"[adjoint] model 8x8x10, 3 steps, checkpoint every 2, records={records}", flush=True)
     case = perturbed_case()
     N = 3
     traj = [{k: np.array(v, order="F", copy=True) for k, v in case.state.items()}]
