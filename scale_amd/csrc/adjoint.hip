@@ -393,89 +393,108 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     aadd(Db + p, denb); aadd(Db + p + 1, denb); aadd(Db + p + IA, denb); aadd(Db + p + 1 + IA, denb);
   }, st));
 
-  // ---- vertical implicit block (HEVI.F90:688-804), one thread per column ----
+  // ---- vertical implicit block (HEVI.F90:688-804) ----
+  // (a) per level: adjoints of the back substitution's updates of MOMZ, DENS
+  //     and RHOT (HEVI.F90:728-790), each level gathering what its own and
+  //     the upper level's updates send to it
+  HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
+    const size_t p = IX(d, k, i, j);
+    const double bd = obD[p], bt = obRT[p];
+    D0b[p] += bd; Srb[p] += dt * bd;
+    RT0b[p] += bt; Stb[p] += dt * bt;
+    double cpb = 0.0, ptb = 0.0;
+    if (k <= KE - 1) {
+      const double b = obMZ[p];
+      MZ0b[p] += b; MZb[p] -= b;
+      const double r = dt * g.RCDZ[k], r1 = dt * g.RCDZ[k + 1];
+      const double bd1 = obD[p + pl], bt1 = obRT[p + pl];
+      cpb = b - r * bd - r * PT[p] * bt + r1 * bd1 + r1 * PT[p] * bt1;
+      ptb = -r * Cp[p] * bt + r1 * Cp[p] * bt1;
+    }
+    Cpb[p] = cpb; PTb[p] = ptb;
+  }, st));
+  // (b) the transposed Thomas solve, one thread per column:
+  //     M(k,k-1) = F3(k), M(k,k) = F2(k), M(k,k+1) = F1(k)
   HIPCHK(for3(Box{0, 0, IS, IE, JS, JE}, [=] __device__(int, int i, int j) {
     const size_t c0 = IX(d, 0, i, j);
     auto at = [&](int k) { return c0 + pl * (size_t)k; };
-    for (int k = KS - 1; k <= KE + 1; ++k) { Cpb[at(k)] = 0.0; PTb[at(k)] = 0.0; Ab[at(k)] = 0.0; }
-    for (int k = KS; k <= KE - 1; ++k) {
-      const size_t p = at(k);
-      const double b = obMZ[p];
-      aadd(MZ0b + p, b); Cpb[p] += b; aadd(MZb + p, -b);
-    }
-    for (int k = KS; k <= KE; ++k) {
-      const size_t p = at(k);
-      const double bd = obD[p], bt = obRT[p];
-      aadd(D0b + p, bd); Srb[p] += dt * bd;
-      aadd(RT0b + p, bt); Stb[p] += dt * bt;
-      const double r = dt * g.RCDZ[k];
-      if (k <= KE - 1) {
-        Cpb[p] -= r * bd;
-        Cpb[p] -= r * PT[p] * bt; PTb[p] -= r * Cp[p] * bt;
-      }
-      if (k >= KS + 1) {
-        const size_t m = p - pl;
-        Cpb[m] += r * bd;
-        Cpb[m] += r * PT[m] * bt; PTb[m] += r * Cp[m] * bt;
-      }
-    }
     auto Af = [&](int k) { return dt * dt * g.RCDZ[k] * RT2P[at(k)]; };
     auto Bf = [&](int k) { return GRAV * (dt * dt) / (g.CDZ[k + 1] + g.CDZ[k]); };
     auto F1f = [&](int k) { return -(PT[at(k + 1)] * g.RFDZ[k] * Af(k + 1) + Bf(k)); };
     auto F2f = [&](int k) { return 1.0 + PT[at(k)] * g.RFDZ[k] * (Af(k + 1) + Af(k)); };
     auto F3f = [&](int k) { return -(PT[at(k - 1)] * g.RFDZ[k] * Af(k) - Bf(k)); };
-    // transposed Thomas: M(k,k-1) = F3(k), M(k,k) = F2(k), M(k,k+1) = F1(k)
-    {
-      double den = F2f(KS);
-      double cpk = (KS < KE - 1 ? F3f(KS + 1) : 0.0) / den;
-      double dpk = Cpb[at(KS)] / den;
-      cpv[at(KS)] = cpk; dpv[at(KS)] = dpk;
-      for (int k = KS + 1; k <= KE - 1; ++k) {
-        const double f1m = F1f(k - 1);
-        den = F2f(k) - f1m * cpk;
-        cpk = (k < KE - 1 ? F3f(k + 1) : 0.0) / den;
-        dpk = (Cpb[at(k)] - f1m * dpk) / den;
-        cpv[at(k)] = cpk; dpv[at(k)] = dpk;
-      }
-      double cn = dpk;
-      Cb[at(KE - 1)] = cn;
-      for (int k = KE - 2; k >= KS; --k) {
-        cn = dpv[at(k)] - cpv[at(k)] * cn;
-        Cb[at(k)] = cn;
-      }
+    double den = F2f(KS);
+    double cpk = (KS < KE - 1 ? F3f(KS + 1) : 0.0) / den;
+    double dpk = Cpb[at(KS)] / den;
+    cpv[at(KS)] = cpk; dpv[at(KS)] = dpk;
+    for (int k = KS + 1; k <= KE - 1; ++k) {
+      const double f1m = F1f(k - 1);
+      den = F2f(k) - f1m * cpk;
+      cpk = (k < KE - 1 ? F3f(k + 1) : 0.0) / den;
+      dpk = (Cpb[at(k)] - f1m * dpk) / den;
+      cpv[at(k)] = cpk; dpv[at(k)] = dpk;
     }
-    for (int k = KS; k <= KE - 1; ++k) {
-      const size_t p = at(k);
-      const double cb = Cb[p], A0 = Af(k), A1 = Af(k + 1);
+    double cn = dpk;
+    Cb[at(KE - 1)] = cn;
+    for (int k = KE - 2; k >= KS; --k) {
+      cn = dpv[at(k)] - cpv[at(k)] * cn;
+      Cb[at(k)] = cn;
+    }
+  }, st));
+  // (c) per level: adjoints of the band coefficients (through PT and
+  //     A = dt^2 RCDZ RT2P) and of the right-hand side C = MOMZ + dt (pg + Sw),
+  //     level L gathering rows L-1, L and L+1
+  HIPCHK(for3(inter, [=] __device__(int L, int i, int j) {
+    const size_t p = IX(d, L, i, j);
+    auto A_ = [&](int k) { return dt * dt * g.RCDZ[k] * RT2P[IX(d, k, i, j)]; };
+    auto row = [&](int k) { return k >= KS && k <= KE - 1; };
+    double ptb = 0.0, ab = 0.0;
+    if (row(L)) {
+      const double cb = Cb[p];
       const double F2b = -cb * Cp[p];
-      PTb[p] += F2b * g.RFDZ[k] * (A1 + A0);
-      Ab[p + pl] += F2b * PT[p] * g.RFDZ[k]; Ab[p] += F2b * PT[p] * g.RFDZ[k];
-      if (k <= KE - 2) {
-        const double F1b = -cb * Cp[p + pl];
-        PTb[p + pl] -= F1b * g.RFDZ[k] * A1; Ab[p + pl] -= F1b * PT[p + pl] * g.RFDZ[k];
+      ptb += F2b * g.RFDZ[L] * (A_(L + 1) + A_(L));
+      ab += F2b * PT[p] * g.RFDZ[L];
+      if (L >= KS + 1) ab -= (-cb * Cp[p - pl]) * PT[p - pl] * g.RFDZ[L];  // F3 of row L
+    }
+    if (row(L - 1)) {
+      const double cbm = Cb[p - pl];
+      ab += (-cbm * Cp[p - pl]) * PT[p - pl] * g.RFDZ[L - 1];  // F2 of row L-1 through A(L)
+      if (L - 1 <= KE - 2) {                                  // F1 of row L-1
+        const double F1b = -cbm * Cp[p];
+        ptb -= F1b * g.RFDZ[L - 1] * A_(L);
+        ab -= F1b * PT[p] * g.RFDZ[L - 1];
       }
-      if (k >= KS + 1) {
-        const double F3b = -cb * Cp[p - pl];
-        PTb[p - pl] -= F3b * g.RFDZ[k] * A0; Ab[p] -= F3b * PT[p - pl] * g.RFDZ[k];
-      }
-      // C(k) = MOMZ + dt*(pg + Sw)
-      aadd(MZb + p, cb);
+    }
+    if (row(L + 1) && L + 1 >= KS + 1) ptb -= (-Cb[p + pl] * Cp[p]) * g.RFDZ[L + 1] * A_(L + 1);  // F3 of row L+1
+    PTb[p] += ptb;
+    double rt2pb = ab * dt * dt * g.RCDZ[L];
+    // right-hand side: pg = -(DPRES(k+1) + RT2P(k+1) dt St(k+1) - DPRES(k) - RT2P(k) dt St(k)) RFDZ(k)
+    //                       - GRAV (f1 (D(k+1) - REFd + dt Sr(k+1)) + f2 (D(k) - REFd + dt Sr(k)))
+    double dpb = 0.0, stb = 0.0, db = 0.0, srb = 0.0;
+    if (row(L)) {
+      const double cb = Cb[p];
+      MZb[p] += cb;
       Swb[p] += dt * cb;
-      const double pgb = dt * cb;
-      const double a = -pgb * g.RFDZ[k];
-      aadd(DPRESb + p + pl, a); aadd(RT2Pb + p + pl, a * dt * St[p + pl]); Stb[p + pl] += a * RT2P[p + pl] * dt;
-      aadd(DPRESb + p, -a); aadd(RT2Pb + p, -a * dt * St[p]); Stb[p] -= a * RT2P[p] * dt;
-      const double bb = -GRAV * pgb;
-      const double f1 = g.CDZ[k] / (g.CDZ[k] + g.CDZ[k + 1]), f2 = g.CDZ[k + 1] / (g.CDZ[k] + g.CDZ[k + 1]);
-      aadd(Db + p + pl, bb * f1); Srb[p + pl] += bb * f1 * dt;
-      aadd(Db + p, bb * f2); Srb[p] += bb * f2 * dt;
+      const double a = -dt * cb * g.RFDZ[L];
+      dpb -= a; rt2pb -= a * dt * St[p]; stb -= a * RT2P[p] * dt;
+      const double bb = -GRAV * dt * cb;
+      const double f2 = g.CDZ[L + 1] / (g.CDZ[L] + g.CDZ[L + 1]);
+      db += bb * f2; srb += bb * f2 * dt;
     }
-    for (int k = KS; k <= KE; ++k) aadd(RT2Pb + at(k), Ab[at(k)] * dt * dt * g.RCDZ[k]);
-    // PT = valueW_Z(MOMZ, POTT) (UD3.F90:98-153)
-    for (int k = KS; k <= KE - 1; ++k) {
-      const size_t p = at(k);
-      (void)face_adj<S>(zkind(k, KS, KE), POTT, POTTb, p, pl, MZ[p], PTb[p]);
+    if (row(L - 1)) {
+      const double cbm = Cb[p - pl];
+      const double a = -dt * cbm * g.RFDZ[L - 1];
+      dpb += a; rt2pb += a * dt * St[p]; stb += a * RT2P[p] * dt;
+      const double bb = -GRAV * dt * cbm;
+      const double f1 = g.CDZ[L - 1] / (g.CDZ[L - 1] + g.CDZ[L]);
+      db += bb * f1; srb += bb * f1 * dt;
     }
+    DPRESb[p] += dpb; RT2Pb[p] += rt2pb; Stb[p] += stb; Db[p] += db; Srb[p] += srb;
+  }, st));
+  // (d) PT = valueW_Z(MOMZ, POTT) (UD3.F90:98-153), one face per thread
+  HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+    const size_t p = IX(d, k, i, j);
+    (void)face_adj<S>(zkind(k, KS, KE), POTT, POTTb, p, pl, MZ[p], PTb[p]);
   }, st));
 
   // ---- St (HEVI.F90:634-657) and the theta fluxes (UD3.F90:157-354) ----

@@ -245,15 +245,30 @@ static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], c
   a.num_diff = num_diff;
   const Dims& d = c.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
-  // tile height: SCALE_HIP_NF_TJ = 4, 8 or 16 (default 16: at 512x512x128
-  // FP64 the filter takes 2.94 ms at TJ = 16, 3.08 ms at 8 and 4.35 ms at 4,
-  // the larger tiles re-reading fewer halo rows)
-  static const int TJ = [] {
-    const char* e = getenv("SCALE_HIP_NF_TJ");
-    const int v = e ? atoi(e) : 16;
-    return (v == 4 || v == 8) ? v : 16;
+  // tile height: the tallest of 16, 8 and 4 rows that still gives every CU a
+  // block (taller tiles re-read fewer halo rows: at 512x512x128 FP64 the
+  // filter takes 2.94 ms at 16 rows, 3.08 ms at 8 and 4.35 ms at 4; at
+  // 128x128x64 the 16 blocks of 16 rows take 1.14 ms, the 64 of 4 rows
+  // 0.59 ms); SCALE_HIP_NF_TJ = 4, 8 or 16 overrides
+  static const int ncu = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      n = 256;
+    return n > 0 ? n : 256;
   }();
-  const unsigned ni = (imax + 63) / 64, nj = (jmax + TJ - 1) / TJ;
+  static const int TJenv = [] {
+    const char* e = getenv("SCALE_HIP_NF_TJ");
+    const int v = e ? atoi(e) : 0;
+    return (v == 4 || v == 8 || v == 16) ? v : 0;
+  }();
+  const unsigned ni = (imax + 63) / 64;
+  int TJ = TJenv;
+  if (!TJ) {
+    TJ = 4;
+    for (int t : {16, 8})
+      if (ni * (unsigned)((jmax + t - 1) / t) >= (unsigned)ncu) { TJ = t; break; }
+  }
+  const unsigned nj = (jmax + TJ - 1) / TJ;
   const unsigned nb = ni * nj;
   prof_start(c, "numfilter", s);
   if (TJ == 16) hipLaunchKernelGGL((numfilter_tile<16, R>), dim3(nb), dim3(64, 16), 0, s, a, ni, nb);
