@@ -694,6 +694,7 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
       M->fmfR[r] = F(3 * n3); M->ftfR[r] = F(3 * n3);
     }
     M->fnum_diff = F(15 * n3); M->fDDIV = F(n3); M->fmflx = F(3 * n3); M->ftflx = F(3 * n3);
+    M->foff = A(2 * (size_t)c.d.KA);
   }
   for (void* p : M->allocs)
     if (!p) return fail("hipMalloc failed in scale_hip_model_create (out of device memory?)");
@@ -701,6 +702,7 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
     CK(hipMemsetAsync(M->fmflx, 0, 3 * n3 * sizeof(float), c.stream));
     CK(hipMemsetAsync(M->ftflx, 0, 3 * n3 * sizeof(float), c.stream));
     CK(hipMemsetAsync(M->fDDIV, 0, n3 * sizeof(float), c.stream));
+    CK(hipMemsetAsync(M->foff, 0, 2 * (size_t)c.d.KA * sizeof(double), c.stream));
   }
   CK(hipMemsetAsync(M->mflx_hi, 0, 3 * n3 * sizeof(double), c.stream));
   CK(hipMemsetAsync(M->tflx_hi, 0, 3 * n3 * sizeof(double), c.stream));

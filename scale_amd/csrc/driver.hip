@@ -192,6 +192,25 @@ static hipError_t convert(To* dst, const From* src, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+// the same for a field stored in FP32 as the deviation from a level profile
+// off(k): FP32 = x - off(k), FP64 = FP32 + off(k)
+__global__ void to_dev32(float* dst, const double* src, const double* off, size_t n, size_t pl) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = float(src[q] - off[q / pl]);
+}
+__global__ void from_dev32(double* dst, const float* src, const double* off, size_t n, size_t pl) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = (double)src[q] + off[q / pl];
+}
+// the level profiles: DENS and RHOT of the column (IS, JS) of the FP64 state
+__global__ void dev_profiles(Dims d, const double* DENS, const double* RHOT, double* off) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.KA) return;
+  const size_t p = IX(d, k, d.IS, d.JS);
+  off[k] = DENS[p];
+  off[d.KA + k] = RHOT[p];
+}
+
 // ---- 3-D momentum divergence for divergence damping ----------------------------
 // ATMOS_DYN_divergence (COMMON.F90:1437-1515): one thread per point of
 // i = IS..IE+1, j = JS..JE+1, k = KS-1..KE+1, the levels KS and KE with their
@@ -399,7 +418,11 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B) {
     HIPCHK(hipMemsetAsync(B.num_diff, 0, 15 * d.n3 * sizeof(R), s));
   } else {
     const R* cst[5] = {B.st[0], B.st[1], B.st[2], B.st[3], B.st[4]};
-    HIPCHK(numfilter(c, B.num_diff, cst, M.REF_dens, M.REF_pott, s));
+    if constexpr (sizeof(R) == sizeof(double)) {
+      HIPCHK(numfilter(c, B.num_diff, cst, M.REF_dens, M.REF_pott, s));
+    } else {
+      HIPCHK(numfilter(c, B.num_diff, cst, M.REF_dens, M.REF_pott, s, M.foff, M.foff + d.KA));
+    }
   }
   // divergence for the divergence damping (LARGE.F90:860-872)
   if constexpr (sizeof(R) == sizeof(double)) {
@@ -427,6 +450,7 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B) {
     for (int v = 0; v < 5; ++v) { io.c1[v] = B.Rk[0][v]; io.c2[v] = B.Rk[1][v]; io.c3[v] = B.Rk[2][v]; io.st[v] = B.st[v]; }
     for (int n = 0; n < 3; ++n) { io.cm[n] = B.mfR[n]; io.ct[n] = B.tfR[n]; }
     io.mflx_av = B.mflx; io.tflx_av = B.tflx;
+    if constexpr (sizeof(R) == sizeof(float)) { io.offD = M.foff; io.offR = M.foff + d.KA; }
     if constexpr (sizeof(R) == sizeof(double)) {
       HIPCHK(M.heve ? heve_stage(io, s) : hevi_stage(io, s));
     } else {
@@ -488,7 +512,10 @@ hipError_t model_state_to_f64(Ctx& c) {
   Model& M = *c.model;
   if (!M.mixed) return hipSuccess;
   double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
-  for (int v = 0; v < 5; ++v) HIPCHK(convert(st[v], (const float*)M.fS[v], c.d.n3, c.stream));
+  for (int v = 1; v < 4; ++v) HIPCHK(convert(st[v], (const float*)M.fS[v], c.d.n3, c.stream));
+  hipLaunchKernelGGL(from_dev32, dim3(2048), dim3(256), 0, c.stream, M.DENS, M.fS[0], M.foff, c.d.n3, c.d.pl);
+  hipLaunchKernelGGL(from_dev32, dim3(2048), dim3(256), 0, c.stream, M.RHOT, M.fS[4], M.foff + c.d.KA, c.d.n3, c.d.pl);
+  HIPCHK(hipGetLastError());
   HIPCHK(convert(M.mflx_hi, (const float*)M.fmflx, 3 * c.d.n3, c.stream));
   return convert(M.tflx_hi, (const float*)M.ftflx, 3 * c.d.n3, c.stream);
 }
@@ -496,8 +523,14 @@ hipError_t model_state_to_f32(Ctx& c) {
   Model& M = *c.model;
   if (!M.mixed) return hipSuccess;
   const double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
-  for (int v = 0; v < 5; ++v) HIPCHK(convert(M.fS[v], st[v], c.d.n3, c.stream));
-  return hipSuccess;
+  // DENS and RHOT as deviations from the profiles of the current state (set
+  // anew here, with every FP32 field re-encoded from the FP64 state)
+  hipLaunchKernelGGL(dev_profiles, dim3((c.d.KA + 63) / 64), dim3(64), 0, c.stream, c.d, M.DENS, M.RHOT, M.foff);
+  HIPCHK(hipGetLastError());
+  for (int v = 1; v < 4; ++v) HIPCHK(convert(M.fS[v], st[v], c.d.n3, c.stream));
+  hipLaunchKernelGGL(to_dev32, dim3(2048), dim3(256), 0, c.stream, M.fS[0], M.DENS, M.foff, c.d.n3, c.d.pl);
+  hipLaunchKernelGGL(to_dev32, dim3(2048), dim3(256), 0, c.stream, M.fS[4], M.RHOT, M.foff + c.d.KA, c.d.n3, c.d.pl);
+  return hipGetLastError();
 }
 
 // RT2P, DPRES0, REF_rhot, DENS00, tendencies (LARGE.F90:443-839)

@@ -48,8 +48,11 @@ struct HeviArgsT {
   R *DENS_RK, *MOMZ_RK, *MOMX_RK, *MOMY_RK, *RHOT_RK, *mflx_hi, *tflx_hi;
   // scratch (n3 each)
   double *PT, *e, *f;
-  // optional records for the adjoint (hevi_march only): St and the solved C
+  // optional records for the adjoint: St and the solved C
   double *trSt = nullptr, *trCp = nullptr;
+  // mixed precision: DENS and RHOT (and their stage values) are stored as
+  // deviations from the level profiles offD(k), offR(k), kept in double
+  const double *offD = nullptr, *offR = nullptr;
   // RK4 combination fused into the last stage (hevi_tile with CMB)
   const R *c1[5], *c2[5], *c3[5];
   const R *cm[3], *ct[3];
@@ -810,17 +813,27 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
   const R* ND = a.num_diff;  // component (v, dir) at ND + n3*(v + 5*dir)
   auto nd = [&](int v, int dir, size_t q) { return ND[q + n3 * (size_t)(v + 5 * dir)]; };
   auto gix = [&](int k, int ii, int jj) { return IX(d, k, ii, jj); };
+  // full values of the stored DENS / RHOT at level k, and deviations to store
+  // (the identity in FP64, where the arrays hold the full values)
+  constexpr bool DEV = sizeof(R) == sizeof(float);
+  auto fullDd = [&](R v, int k) -> double { if constexpr (DEV) return (double)v + a.offD[k]; else return (double)v; };
+  auto fullTd = [&](R v, int k) -> double { if constexpr (DEV) return (double)v + a.offR[k]; else return (double)v; };
+  auto fullD = [&](R v, int k) -> R { if constexpr (DEV) return R(fullDd(v, k)); else return v; };
+  auto fullT = [&](R v, int k) -> R { if constexpr (DEV) return R(fullTd(v, k)); else return v; };
+  auto devD = [&](double v, int k) -> R { if constexpr (DEV) return R(v - a.offD[k]); else return R(v); };
+  auto devT = [&](double v, int k) -> R { if constexpr (DEV) return R(v - a.offR[k]); else return R(v); };
 
   // (a) tile staging helpers: tile cell (r, c) is column i0-2+c, row j0-2+r
-  auto fill = [&](R (*dst)[CW], const R* src, int k) {
+  auto fill = [&](R (*dst)[CW], const R* src, int k, bool dens) {
     for (int q = tid; q < RW * CW; q += NT) {
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      dst[r][c] = (ii < d.IA && jj < d.JA) ? src[gix(k, ii, jj)] : R(0);
+      const R v = (ii < d.IA && jj < d.JA) ? src[gix(k, ii, jj)] : R(0);
+      dst[r][c] = (dens && ii < d.IA && jj < d.JA) ? fullD(v, k) : v;
     }
   };
   int s0 = 0;  // ring slot of level k; slot s0^1 holds level k+1
-  fill(sD[0], a.DENS, KS); fill(sMX[0], a.MOMX, KS); fill(sMY[0], a.MOMY, KS);
+  fill(sD[0], a.DENS, KS, true); fill(sMX[0], a.MOMX, KS, false); fill(sMY[0], a.MOMY, KS, false);
   __syncthreads();
 
   // own-column windows at k-1..k+2 and carried z-face fluxes
@@ -830,7 +843,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
   if (own) {
     for (int q = 1; q < 4; ++q) {
       const int k = KS - 2 + q;
-      pw[q] = at(a.RHOT, k) / at(a.DENS, k);
+      pw[q] = fullT(at(a.RHOT, k), k) / fullD(at(a.DENS, k), k);
       ww[q] = at(a.MOMZ, k); xw[q] = at(a.MOMX, k); yw[q] = at(a.MOMY, k);
     }
   }
@@ -881,11 +894,11 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
       const bool ok = q < RW * CW && ii < d.IA && jj < d.JA;
       const size_t g = ok ? gix(k, ii, jj) : 0;
-      tD[h] = ok ? a.DENS[g + d.pl] : R(0);
+      tD[h] = ok ? fullD(a.DENS[g + d.pl], k + 1) : R(0);
       tX[h] = ok ? a.MOMX[g + d.pl] : R(0);
       tY[h] = ok ? a.MOMY[g + d.pl] : R(0);
       tW[h] = ok ? a.MOMZ[g] : R(0);
-      tR[h] = ok ? a.RHOT[g] : R(0);
+      tR[h] = ok ? fullT(a.RHOT[g], k) : R(0);
     }
     double pD0[NP], pRT[NP], pRR[NP];
     R pR[NP];
@@ -930,7 +943,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       const int q = tid + h * NT;
       if (q >= (TJ + 1) * (TI + 1)) continue;
       const int r = q / (TI + 1), c = q - r * (TI + 1);
-      sDP[r][c] = pD0[h] + pRT[h] * ((double)pR[h] - pRR[h]);
+      sDP[r][c] = pD0[h] + pRT[h] * (fullTd(pR[h], k) - pRR[h]);
     }
     __syncthreads();
     // ---- the column's own loads of this level, in flight during the face
@@ -939,7 +952,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
     R o_nZD = 0, o_nZT = 0, o_nZW = 0, o_nZX = 0, o_nZY = 0;
     R o_Dt = 0, o_Rt = 0, o_Wt = 0, o_Xt = 0, o_Yt = 0, o_X0 = 0, o_Y0 = 0, o_Z0 = 0;
     double o_RT = 0.0, o_RD = 0.0;
-    R o_R2 = 1, o_D2 = 1, o_W2 = 0, o_X2 = 0, o_Y2 = 0;
+    R o_R2 = 1, o_D2 = 1, o_W2 = 0, o_X2 = 0, o_Y2 = 0, o_D = 1;
     const double wdc = a.wdamp_coef[k];
     if (own) {
       o_nZD = nd(I_DENS, ZDIR, gk); o_nZT = nd(I_RHOT, ZDIR, gk); o_nZW = nd(I_MOMZ, ZDIR, gk + d.pl);
@@ -949,6 +962,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       if (wdc != 0.0) o_Z0 = a.MOMZ0[gk];
       const size_t g2 = gk + 2 * d.pl;
       o_R2 = a.RHOT[g2]; o_D2 = a.DENS[g2]; o_W2 = a.MOMZ[g2]; o_X2 = a.MOMX[g2]; o_Y2 = a.MOMY[g2];
+      if constexpr (DEV) o_D = a.DENS[gk];
     }
     // ---- (b) faces ----
     const R f1 = R(CDZ[k] / (CDZ[k] + CDZ[k + 1]));      // F2H(k,1)
@@ -1035,7 +1049,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
 
     // own-column windows move up one level
     if (own) {
-      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = o_R2 / o_D2;
+      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = fullT(o_R2, k + 2) / fullD(o_D2, k + 2);
       ww[0] = ww[1]; ww[1] = ww[2]; ww[2] = ww[3]; ww[3] = o_W2;
       xw[0] = xw[1]; xw[1] = xw[2]; xw[2] = xw[3]; xw[3] = o_X2;
       yw[0] = yw[1]; yw[1] = yw[2]; yw[2] = yw[3]; yw[3] = o_Y2;
@@ -1134,7 +1148,10 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
         MYn = o_Y0 + R(dtrk) * (t + o_Yt);
       }
       // tridiagonal row k-1 and forward sweep in double (HEVI.F90:688-727, 1092-1115)
-      const double rt1 = o_RT, dp1 = sDP[ty][tx], d1 = (double)D0(r, c), rd1 = o_RD;
+      double d1;
+      if constexpr (DEV) d1 = fullDd(o_D, k);  // the exact stored value, not the FP32 tile copy
+      else d1 = (double)D0(r, c);
+      const double rt1 = o_RT, dp1 = sDP[ty][tx], rd1 = o_RD;
       const double St1 = (double)St, Sr1 = (double)Sr;
       const double A1 = dtrk * dtrk * J33G * RCDZ[k] * rt1 * J33G;
       const double pt1 = (k <= KE - 1) ? (double)PT : 0.0;
@@ -1209,7 +1226,14 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       a.st[v][q] = c;
       if (k == KS || k == KE) {
         const int lo = (k == KS) ? 0 : KE + 1, hi = (k == KS) ? KS - 1 : d.KA - 1;
-        for (int kk = lo; kk <= hi; ++kk) a.st[v][p0 + d.pl * (size_t)kk] = (v == 1) ? R(0) : c;
+        for (int kk = lo; kk <= hi; ++kk) {
+          R gv = (v == 1) ? R(0) : c;
+          if constexpr (DEV) {  // the ghost level holds the same full value, its own deviation
+            if (v == 0) gv = devD(fullDd(c, k), kk);
+            if (v == 4) gv = devT(fullTd(c, k), kk);
+          }
+          a.st[v][p0 + d.pl * (size_t)kk] = gv;
+        }
       }
     } else {
       R* o = v == 0 ? a.DENS_RK : (v == 1 ? a.MOMZ_RK : a.RHOT_RK);
@@ -1230,9 +1254,9 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
     const size_t q = px(KE);
     const R d0 = a.DENS0[q], r0 = a.RHOT0[q];
     double advcv = Cn * J33G * RCDZ[KE];
-    const R dn = R((double)d0 + dtrk * (advcv + (double)a.DENS_RK[q]));
+    const R dn = devD(fullDd(d0, KE) + dtrk * (advcv + (double)a.DENS_RK[q]), KE);
     advcv = Cn * ptn * J33G * RCDZ[KE];
-    const R rn = R((double)r0 + dtrk * (advcv + (double)a.RHOT_RK[q]));
+    const R rn = devT(fullTd(r0, KE) + dtrk * (advcv + (double)a.RHOT_RK[q]), KE);
     put_state(0, q, dn, d0, KE);
     put_state(4, q, rn, r0, KE);
     if constexpr (CMB) {
@@ -1264,18 +1288,18 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
     }
     const R d0 = a.DENS0[q1], r0 = a.RHOT0[q1];
     double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1];
-    put_state(0, q1, R((double)d0 + dtrk * (advcv + (double)a.DENS_RK[q1])), d0, k + 1);
+    put_state(0, q1, devD(fullDd(d0, k + 1) + dtrk * (advcv + (double)a.DENS_RK[q1]), k + 1), d0, k + 1);
     advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1];
-    put_state(4, q1, R((double)r0 + dtrk * (advcv + (double)a.RHOT_RK[q1])), r0, k + 1);
+    put_state(4, q1, devT(fullTd(r0, k + 1) + dtrk * (advcv + (double)a.RHOT_RK[q1]), k + 1), r0, k + 1);
     Cn = Ck; ptn = ptk;
   }
   {
     const size_t q = px(KS);
     const R d0 = a.DENS0[q], r0 = a.RHOT0[q];
     double advcv = -Cn * J33G * RCDZ[KS];
-    put_state(0, q, R((double)d0 + dtrk * (advcv + (double)a.DENS_RK[q])), d0, KS);
+    put_state(0, q, devD(fullDd(d0, KS) + dtrk * (advcv + (double)a.DENS_RK[q]), KS), d0, KS);
     advcv = -Cn * ptn * J33G * RCDZ[KS];
-    put_state(4, q, R((double)r0 + dtrk * (advcv + (double)a.RHOT_RK[q])), r0, KS);
+    put_state(4, q, devT(fullTd(r0, KS) + dtrk * (advcv + (double)a.RHOT_RK[q]), KS), r0, KS);
     if constexpr (!CMB) a.MOMZ_RK[px(KS - 1)] = R(0);
   }
 }
@@ -1323,6 +1347,7 @@ static void fill_args(HeviArgsT<R>& a, const StageIOT<R>& io) {
   for (int v = 0; v < 5; ++v) { a.c1[v] = io.c1[v]; a.c2[v] = io.c2[v]; a.c3[v] = io.c3[v]; a.st[v] = io.st[v]; }
   for (int n = 0; n < 3; ++n) { a.cm[n] = io.cm[n]; a.ct[n] = io.ct[n]; }
   a.mflx_av = io.mflx_av; a.tflx_av = io.tflx_av;
+  a.offD = io.offD; a.offR = io.offR;
 }
 
 hipError_t hevi_stage(const StageIO& io, hipStream_t s) {

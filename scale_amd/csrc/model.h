@@ -34,6 +34,10 @@ struct Model {
   float* fR[4][5] = {};
   float* ft[5] = {};
   float *fnum_diff = nullptr, *fDDIV = nullptr, *fmflx = nullptr, *ftflx = nullptr;
+  // the FP32 DENS and RHOT (state and stage values) are deviations from the
+  // level profiles foff[0..KA) and foff[KA..2KA), so that their rounding
+  // error scales with the deviation, not with the full value
+  double* foff = nullptr;
   float *fmfR[4] = {}, *ftfR[4] = {};
   // configuration (PARAM_ATMOS_DYN subset)
   double dtl = 1.0, dts = 1.0, nd_coef = 1.0e-4, nd_sfc_fact = 1.0, divdmp_coef = 0.0, nd_coef_q = 0.0;

@@ -27,6 +27,8 @@ struct NfArgsT {
   const double *ndc;                 // nd_cdz(KA) nd_fdz(KA) nd_cdx(IA) nd_fdx(IA) nd_cdy(JA) nd_fdy(JA)
   double sfc;
   R* num_diff;
+  // FP32: DENS and RHOT stored as deviations from the level profiles offD, offR
+  const double *offD = nullptr, *offR = nullptr;
 };
 
 // coefficient tables of calc_diff3 (CNZ3/CNX3/CNY3 of filter_setup)
@@ -70,22 +72,26 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
   const double* nd_fdy = a.ndc + 2 * d.KA + 2 * d.IA + d.JA;
   const size_t p0 = own ? IX(d, 0, i, j) : 0;
   auto out = [&](int v, int dir, int k) -> R& { return a.num_diff[p0 + d.pl * (size_t)k + n3 * (size_t)(v + 5 * dir)]; };
+  // full density and rho*theta at level k (the stored values in FP64)
+  constexpr bool DEV = sizeof(R) == sizeof(float);
+  auto fD = [&](R v, int k) -> R { if constexpr (DEV) return R((double)v + a.offD[k]); else return v; };
+  auto fT = [&](R v, int k) -> R { if constexpr (DEV) return R((double)v + a.offR[k]); else return v; };
   auto fillD = [&](R (*dst)[CW + 1], int k) {
     for (int q = tid; q < (RW + 1) * (CW + 1); q += NT) {
       const int r = q / (CW + 1), c = q - r * (CW + 1);
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      dst[r][c] = (ii < d.IA && jj < d.JA) ? a.DENS[IX(d, k, ii, jj)] : R(1);
+      dst[r][c] = (ii < d.IA && jj < d.JA) ? fD(a.DENS[IX(d, k, ii, jj)], k) : R(1);
     }
   };
   // the five fields at (k, ii, jj) for the own column (COMMON.F90:545, 631, 715, 798, 882)
   auto phis = [&](int k, R* ph) {
     const size_t q = p0 + d.pl * (size_t)k;
-    const R D = a.DENS[q];
+    const R D = fD(a.DENS[q], k);
     ph[0] = phi_ref<R>(D, a.REF_dens[q]);
-    ph[1] = R(2) * a.MOMZ[q] / (a.DENS[q + d.pl] + D);
-    ph[2] = R(2) * a.MOMX[q] / (a.DENS[q + 1] + D);
-    ph[3] = R(2) * a.MOMY[q] / (a.DENS[q + d.IA] + D);
-    ph[4] = phi_ref<R>(a.RHOT[q] / D, a.REF_pott[q]);
+    ph[4] = phi_ref<R>(fT(a.RHOT[q], k) / D, a.REF_pott[q]);
+    ph[1] = R(2) * a.MOMZ[q] / (fD(a.DENS[q + d.pl], k + 1) + D);
+    ph[2] = R(2) * a.MOMX[q] / (fD(a.DENS[q + 1], k) + D);
+    ph[3] = R(2) * a.MOMY[q] / (fD(a.DENS[q + d.IA], k) + D);
   };
 
   // levels outside KS..KE are zero (KS-1 gets its DENS/RHOT z fluxes below)
@@ -122,7 +128,7 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
       const bool ok = q < RW * CW && ii < d.IA && jj < d.JA;
       const size_t g = ok ? IX(d, k, ii, jj) : 0;
       rZ[h] = ok ? a.MOMZ[g] : R(0); rX[h] = ok ? a.MOMX[g] : R(0); rY[h] = ok ? a.MOMY[g] : R(0);
-      rR[h] = ok ? a.RHOT[g] : R(0); rRD[h] = ok ? a.REF_dens[g] : 0.0; rRP[h] = ok ? a.REF_pott[g] : R(0);
+      rR[h] = ok ? fT(a.RHOT[g], k) : R(0); rRD[h] = ok ? a.REF_dens[g] : 0.0; rRP[h] = ok ? a.REF_pott[g] : R(0);
     }
     fillD(sD[s1], k + 1);
     if (own) {
@@ -236,8 +242,10 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
 
 template <typename R>
 static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], const double* REF_dens,
-                              const double* REF_pott, hipStream_t s) {
+                              const double* REF_pott, hipStream_t s, const double* offD = nullptr,
+                              const double* offR = nullptr) {
   NfArgsT<R> a;
+  a.offD = offD; a.offR = offR;
   a.d = c.d;
   a.DENS = st[0]; a.MOMZ = st[1]; a.MOMX = st[2]; a.MOMY = st[3]; a.RHOT = st[4];
   a.REF_dens = REF_dens; a.REF_pott = REF_pott;
@@ -368,8 +376,8 @@ hipError_t numfilter_q(const Ctx& c, double* num_diff_q, const double* DENS, con
   return halo_fill(c, arrs, 3, s);
 }
 hipError_t numfilter(const Ctx& c, float* num_diff, const float* const st[5], const double* REF_dens,
-                     const double* REF_pott, hipStream_t s) {
-  return numfilter_t<float>(c, num_diff, st, REF_dens, REF_pott, s);
+                     const double* REF_pott, hipStream_t s, const double* offD, const double* offR) {
+  return numfilter_t<float>(c, num_diff, st, REF_dens, REF_pott, s, offD, offR);
 }
 
 }  // namespace shp

@@ -64,6 +64,8 @@ struct StageIOT {
   int has_coriolis;
   // HEVE only: FLAG_FCT_T and FLAG_FCT_ALONG_STREAM
   int fct_t = 0, fct_along = 1;
+  // FP32 stage: DENS and RHOT stored as deviations from these level profiles
+  const double *offD = nullptr, *offR = nullptr;
   // last RK4 stage only (flat terrain): combine with the outputs of stages
   // 1-3 on the fly, writing the new state into st (which may alias s0) and the
   // averaged fluxes into mflx_av/tflx_av, instead of o / the Z-free parts of
@@ -107,8 +109,9 @@ int prof_names(Prof* p, char* buf, int len);
 hipError_t hbm_copy(const Ctx& c, double* dst, const double* src, size_t n, hipStream_t s);
 hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s);
+// FP32: DENS and RHOT of st are deviations from the level profiles offD, offR
 hipError_t numfilter(const Ctx& c, float* num_diff, const float* const st[5], const double* REF_dens,
-                     const double* REF_pott, hipStream_t s);
+                     const double* REF_pott, hipStream_t s, const double* offD, const double* offR);
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s);
 // ATMOS_DYN_numfilter_coef_q (ND_ORDER = 1) with the halo fill of its 3
 // fields; REF_qv non-null for QV.  Needs the filter tables and c.ndqc.

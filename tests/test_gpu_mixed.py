@@ -2,15 +2,14 @@
 tendencies; FP64 reference state, pressure terms, vertical solve and
 budgets), against the FP64 GPU path, which itself matches the oracle.
 
-Tolerances: potential temperature within 2e-3 K of the FP64 run after 60
-steps of the 40x40x80 warm bubble (bench.py --theta-check measures the
-512x512x128 shear workload); every prognostic within 1e-3 of its FP64
-counterpart relative to the field's range, with the error of each printed;
-total mass conserved to 1e-6 relative.  The FP32 storage of RHOT bounds the
-accuracy: its rounding (about 2e-5 at RHOT ~ 360) enters the pressure
-perturbation RT2P * (RHOT - REF_rhot), which is evaluated in FP64 but from
-the FP32 state, as in the reference's SINGLE build; over 60 s that gives
-MOMZ errors of O(1e-4) of its range."""
+DENS and RHOT are stored in FP32 as deviations from per-level profiles, so
+their rounding scales with the deviation rather than with RHOT ~ 360.
+Tolerances: potential temperature within 2e-4 K of the FP64 run after 60
+steps of the 40x40x80 warm bubble (measured 2.5e-5 K with UD3, 5.0e-5 K
+with CD4; bench.py --theta-check measures the 512x512x128 shear workload
+over 600 steps); every prognostic within 5e-5 of its FP64 counterpart
+relative to the field's range (measured at most 6e-6), with the error of
+each printed; total mass conserved to 1e-6 relative."""
 import numpy as np
 import pytest
 
@@ -49,13 +48,13 @@ def test_mixed_precision_warm_bubble(scheme):
     th_got = interior(g, got["RHOT"]) / interior(g, got["DENS"])
     dth = np.abs(th_got - th_ref).max()
     print(f"  max |theta_mixed - theta_fp64| = {dth:.3e} K", flush=True)
-    assert dth <= 2e-3, dth
+    assert dth <= 2e-4, dth
     for f in FIELDS:
         a, b = interior(g, ref[f]), interior(g, got[f])
         rng = max(a.max() - a.min(), np.abs(a).max() * 1e-3)
         err = np.abs(a - b).max() / rng
         print(f"  {f}: max |mixed - fp64| / range = {err:.3e}", flush=True)
-        assert err <= 1e-3, f
+        assert err <= 5e-5, f
     assert abs(m1 - m0) <= 1e-6 * abs(m0), (m0, m1)
 
 
