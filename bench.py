@@ -188,7 +188,7 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
     return dict(workload="warm bubble 128x128x64, DX=500 m, DZ=250 m, u=10 m/s, dt=1 s, 3 FCT tracers, FP64, UD3",
                 steps=nsteps, checkpoint_every=every, J=J0, fwd_ms=fwd_ms,
                 forward_checkpointed_ms=fck_ms, reverse_ms=rev_ms, adj_ms=fck_ms + rev_ms,
-                adj_over_fwd=(fck_ms + rev_ms) / fwd_ms,
+                adj_over_fwd=(fck_ms + rev_ms) / fwd_ms, reverse_over_fwd=rev_ms / fwd_ms,
                 fd=fd_summary(ads, fd2, fd4, eps_list) if ndirs > 0 else None)
 
 

@@ -184,12 +184,18 @@ hipError_t adj_setup(Ctx& c, int nsteps, int every) {
     AdjWork& A = *c.adj;
     const size_t n3 = c.d.n3;
     bool ok = true;
-    for (int q = 0; q < NW && ok; ++q) ok = (A.w[q] = aalloc(A, n3)) != nullptr;
+    // the work fields in one block (W order), and S0b, Rb, Pb in another,
+    // so that each group is cleared by one fill
+    double* wb = aalloc(A, NW * n3);
+    double* bb = aalloc(A, 28 * n3);
+    ok = wb && bb;
+    for (int q = 0; q < NW && ok; ++q) A.w[q] = wb + n3 * q;
     for (int v = 0; v < 5 && ok; ++v) {
-      ok = (A.xb[v] = aalloc(A, n3)) && (A.S0b[v] = aalloc(A, n3)) && (A.io[v] = aalloc(A, n3));
-      for (int r = 0; r < 4 && ok; ++r) ok = (A.Rb[r][v] = aalloc(A, n3)) != nullptr;
+      A.S0b[v] = bb + n3 * v;
+      for (int r = 0; r < 4; ++r) A.Rb[r][v] = bb + n3 * (5 + 5 * r + v);
+      ok = (A.xb[v] = aalloc(A, n3)) && (A.io[v] = aalloc(A, n3));
     }
-    for (int q = 0; q < 3 && ok; ++q) ok = (A.Pb[q] = aalloc(A, n3)) != nullptr;
+    for (int q = 0; q < 3 && ok; ++q) A.Pb[q] = bb + n3 * (25 + q);
     for (int r = 0; r < 4 && ok; ++r)
       for (int q = 0; q < 3 && ok; ++q) ok = (A.tr[r][q] = aalloc(A, n3)) != nullptr;
     ok = ok && (A.NDb = aalloc(A, 15 * n3)) && (A.mf = aalloc(A, 3 * n3)) && (A.tf = aalloc(A, 3 * n3));
@@ -258,7 +264,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     f.divdmp_coef = 0.0; f.dtrk = dt; f.has_coriolis = io.has_coriolis; f.combine = 0;
     HIPCHK(hevi_stage_traced(f, w[W_PT], w[W_ST], w[W_CP], st));
   }
-  for (int q = W_POTTB; q <= W_MY; ++q) HIPCHK(hipMemsetAsync(w[q], 0, n3 * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(w[W_POTTB], 0, (W_MY - W_POTTB + 1) * n3 * sizeof(double), st));
 
   const double *D = io.s[0], *MZ = io.s[1], *MX = io.s[2], *MY = io.s[3], *RT = io.s[4];
   const double *ND = io.num_diff, *COR = io.CORIOLI, *RT2P = io.RT2P, *REFrt = io.REF_rhot, *wd = io.wdamp_coef;
@@ -341,7 +347,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   }, st));
 
   // ---- momentum x (HEVI.F90:816-927) ----
-  for (int q = W_QZ; q <= W_QY; ++q) HIPCHK(hipMemsetAsync(w[q], 0, n3 * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(w[W_QZ], 0, (W_QY - W_QZ + 1) * n3 * sizeof(double), st));
   HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double b = obMX[p];
@@ -525,7 +531,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   }, st));
 
   // ---- Sw (HEVI.F90:538-577) and the MOMZ fluxes (HEVI.F90:486-525) ----
-  for (int q = W_QZ; q <= W_QY; ++q) HIPCHK(hipMemsetAsync(w[q], 0, n3 * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(w[W_QZ], 0, (W_QY - W_QZ + 1) * n3 * sizeof(double), st));
   HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double b = Swb[p];
@@ -608,29 +614,48 @@ hipError_t hevi_stage_adj(Ctx& c, const StageAdjIO& io, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // periodic halo fill (COMM_vars8), reverse: halo adjoints go to their sources
-static hipError_t halo_adj(const Ctx& c, double* ab, hipStream_t s) {
+// up to 16 fields of n3 doubles, for one launch over all of them
+struct Fields {
+  double* p[16];
+  int n;
+};
+
+// Transpose of the periodic halo fill for nf fields: the fill copies the x
+// faces of the interior rows first, then the whole south/north rows, so the
+// reverse folds the south/north halo rows (corners included) into their
+// images, then the x halos of the interior rows, zeroing what it folded.
+static hipError_t halo_adj_n(const Ctx& c, const Fields f, hipStream_t s) {
   const Dims d = c.d;
   const int IMAX = d.IE - d.IS + 1, JMAX = d.JE - d.JS + 1;
-  // the fill copies the x faces for interior j first, then whole south/north rows
-  HIPCHK(for3(Box{0, d.KA - 1, 0, d.IA - 1, d.JE + 1, d.JA - 1}, [=] __device__(int k, int i, int j) {
-    const size_t p = IX(d, k, i, j);
-    aadd(ab + IX(d, k, i, j - JMAX), ab[p]);
+  const int JH = d.JS, IH = d.IS;  // halo widths (the halo rows below JS, columns left of IS)
+  const int JN = d.JA - 1 - d.JE, IN = d.IA - 1 - d.IE;
+  HIPCHK(for3(Box{0, d.KA - 1, 0, d.IA - 1, 0, JH + JN - 1}, [=] __device__(int k, int i, int jj) {
+    const int j = jj < JH ? jj : d.JE + 1 + (jj - JH);
+    const int jt = jj < JH ? j + JMAX : j - JMAX;
+    for (int q = 0; q < f.n; ++q) {
+      double* ab = f.p[q];
+      const size_t p = IX(d, k, i, j);
+      aadd(ab + IX(d, k, i, jt), ab[p]);
+      ab[p] = 0.0;
+    }
   }, s));
-  HIPCHK(for3(Box{0, d.KA - 1, 0, d.IA - 1, 0, d.JS - 1}, [=] __device__(int k, int i, int j) {
-    const size_t p = IX(d, k, i, j);
-    aadd(ab + IX(d, k, i, j + JMAX), ab[p]);
-  }, s));
-  HIPCHK(for3(Box{0, d.KA - 1, 0, d.IA - 1, 0, d.JA - 1}, [=] __device__(int k, int i, int j) {
-    if (j < d.JS || j > d.JE) ab[IX(d, k, i, j)] = 0.0;
-  }, s));
-  HIPCHK(for3(Box{0, d.KA - 1, 0, d.IA - 1, d.JS, d.JE}, [=] __device__(int k, int i, int j) {
-    const size_t p = IX(d, k, i, j);
-    if (i > d.IE) aadd(ab + p - IMAX, ab[p]);
-    else if (i < d.IS) aadd(ab + p + IMAX, ab[p]);
-  }, s));
-  return for3(Box{0, d.KA - 1, 0, d.IA - 1, d.JS, d.JE}, [=] __device__(int k, int i, int j) {
-    if (i < d.IS || i > d.IE) ab[IX(d, k, i, j)] = 0.0;
+  return for3(Box{0, d.KA - 1, 0, IH + IN - 1, d.JS, d.JE}, [=] __device__(int k, int ii, int j) {
+    const int i = ii < IH ? ii : d.IE + 1 + (ii - IH);
+    const int it = ii < IH ? i + IMAX : i - IMAX;
+    for (int q = 0; q < f.n; ++q) {
+      double* ab = f.p[q];
+      const size_t p = IX(d, k, i, j);
+      aadd(ab + IX(d, k, it, j), ab[p]);
+      ab[p] = 0.0;
+    }
   }, s);
+}
+
+static hipError_t halo_adj5(const Ctx& c, double* const ab[5], hipStream_t s) {
+  Fields f{};
+  for (int v = 0; v < 5; ++v) f.p[v] = ab[v];
+  f.n = 5;
+  return halo_adj_n(c, f, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -770,7 +795,12 @@ hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const d
   const double *D = st[0], *MZ = st[1], *MX = st[2], *MY = st[3], *RT = st[4];
   double *Db = stb[0], *MZb = stb[1], *MXb = stb[2], *MYb = stb[3], *RTb = stb[4];
   auto NB = [=](int v, int dir) { return NDb + n3 * (size_t)(v + 5 * dir); };
-  for (int q = 0; q < 15; ++q) HIPCHK(halo_adj(c, NDb + n3 * (size_t)q, s));
+  {
+    Fields f{};
+    for (int q = 0; q < 15; ++q) f.p[q] = NDb + n3 * (size_t)q;
+    f.n = 15;
+    HIPCHK(halo_adj_n(c, f, s));
+  }
   // surface factor rows of the X and Y components
   HIPCHK(for3(Box{0, 9, IS, IE, JS, JE}, [=] __device__(int vd, int i, int j) {
     double* b = NDb + n3 * (size_t)((vd >> 1) + 5 * (1 + (vd & 1)));
@@ -1014,7 +1044,7 @@ static hipError_t step_rev(Ctx& c, double* const x[5], double* const xb[5], cons
   (void)n3;
 
   // halo fill and ghost levels of the new state (LARGE.F90:908-927)
-  for (int v = 0; v < 5; ++v) HIPCHK(halo_adj(c, xb[v], s));
+  HIPCHK(halo_adj5(c, xb, s));
   {
     double* b0 = xb[0]; double* b1 = xb[1]; double* b2 = xb[2]; double* b3 = xb[3]; double* b4 = xb[4];
     HIPCHK(for3(Box{0, 0, d.IS, d.IE, d.JS, d.JE}, [=] __device__(int, int i, int j) {
@@ -1027,10 +1057,7 @@ static hipError_t step_rev(Ctx& c, double* const x[5], double* const xb[5], cons
     }, s));
   }
   // combination (RK4.F90:562-660)
-  for (int v = 0; v < 5; ++v) {
-    HIPCHK(hipMemsetAsync(A.S0b[v], 0, bytes, s));
-    for (int r = 0; r < 4; ++r) HIPCHK(hipMemsetAsync(A.Rb[r][v], 0, bytes, s));
-  }
+  HIPCHK(hipMemsetAsync(A.S0b[0], 0, 25 * bytes, s));  // S0b and Rb[0..3], one block
   for (int v = 0; v < 5; ++v) {
     double *b = xb[v], *r1 = A.Rb[0][v], *r2 = A.Rb[1][v], *r3 = A.Rb[2][v], *r4 = A.Rb[3][v], *s0 = A.S0b[v];
     HIPCHK(for3(Box{KS, v == 1 ? KE - 1 : KE, d.IS, d.IE, d.JS, d.JE}, [=] __device__(int k, int i, int j) {
@@ -1041,12 +1068,12 @@ static hipError_t step_rev(Ctx& c, double* const x[5], double* const xb[5], cons
     }, s));
   }
   HIPCHK(hipMemsetAsync(A.NDb, 0, 15 * bytes, s));
-  for (int q = 0; q < 3; ++q) HIPCHK(hipMemsetAsync(A.Pb[q], 0, bytes, s));
+  HIPCHK(hipMemsetAsync(A.Pb[0], 0, 3 * bytes, s));
   // stages 4..1
   const double dtrk[4] = {M.dts / 2.0, M.dts / 2.0, M.dts, M.dts};
   for (int r = 3; r >= 0; --r) {
     if (r < 3)
-      for (int v = 0; v < 5; ++v) HIPCHK(halo_adj(c, A.Rb[r][v], s));
+      HIPCHK(halo_adj5(c, A.Rb[r], s));
     StageAdjIO io{};
     io.ctx = &c;
     for (int v = 0; v < 5; ++v) {

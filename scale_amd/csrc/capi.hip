@@ -664,6 +664,7 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   if (QA < 0) return fail("QA must be >= 0");
   if (cfg->mixed_precision && c.terrain) return fail("mixed precision needs flat terrain (GSQRT = 1, J13G = J23G = 0, MAPF = 1)");
   if (cfg->mixed_precision && QA > 0) return fail("mixed precision integrates the dynamics only (QA = 0)");
+  if (cfg->mixed_precision && c.d.KA > shp::KA_DEV_MAX) return fail("mixed precision supports at most 512 levels (KA)");
   if (cfg->nd_coef_q < 0.0) return fail("nd_coef_q must be >= 0");
   if (cfg->short_type != 0 && cfg->short_type != 1) return fail("short_type must be 0 (FVM-HEVI) or 1 (FVM-HEVE)");
   if (cfg->short_type == 1 && cfg->mixed_precision) return fail("the HEVE short step runs in FP64 only");

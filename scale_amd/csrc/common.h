@@ -144,6 +144,10 @@ struct Met {
 };
 
 // one-dimensional grid arrays, all indexed with the 0-based level/column
+// most levels (KA) of a mixed-precision grid: the kernels keep the level
+// profiles of the FP32 deviations in LDS
+constexpr int KA_DEV_MAX = 512;
+
 struct Grid1D {
   const double *CDZ, *FDZ, *RCDZ, *RFDZ;
   const double *CDX, *FDX, *RCDX, *RFDX;

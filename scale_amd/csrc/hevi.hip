@@ -816,12 +816,18 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
   // full values of the stored DENS / RHOT at level k, and deviations to store
   // (the identity in FP64, where the arrays hold the full values)
   constexpr bool DEV = sizeof(R) == sizeof(float);
-  auto fullDd = [&](R v, int k) -> double { if constexpr (DEV) return (double)v + a.offD[k]; else return (double)v; };
-  auto fullTd = [&](R v, int k) -> double { if constexpr (DEV) return (double)v + a.offR[k]; else return (double)v; };
+  // the level profiles staged in LDS (read every level by every thread)
+  __shared__ double sOff[DEV ? 2 * KA_DEV_MAX : 1];
+  if constexpr (DEV) {
+    for (int q = tid; q < d.KA; q += NT) { sOff[q] = a.offD[q]; sOff[KA_DEV_MAX + q] = a.offR[q]; }
+    __syncthreads();
+  }
+  auto fullDd = [&](R v, int k) -> double { if constexpr (DEV) return (double)v + sOff[k]; else return (double)v; };
+  auto fullTd = [&](R v, int k) -> double { if constexpr (DEV) return (double)v + sOff[KA_DEV_MAX + k]; else return (double)v; };
   auto fullD = [&](R v, int k) -> R { if constexpr (DEV) return R(fullDd(v, k)); else return v; };
   auto fullT = [&](R v, int k) -> R { if constexpr (DEV) return R(fullTd(v, k)); else return v; };
-  auto devD = [&](double v, int k) -> R { if constexpr (DEV) return R(v - a.offD[k]); else return R(v); };
-  auto devT = [&](double v, int k) -> R { if constexpr (DEV) return R(v - a.offR[k]); else return R(v); };
+  auto devD = [&](double v, int k) -> R { if constexpr (DEV) return R(v - sOff[k]); else return R(v); };
+  auto devT = [&](double v, int k) -> R { if constexpr (DEV) return R(v - sOff[KA_DEV_MAX + k]); else return R(v); };
 
   // (a) tile staging helpers: tile cell (r, c) is column i0-2+c, row j0-2+r
   auto fill = [&](R (*dst)[CW], const R* src, int k, bool dens) {
@@ -894,11 +900,11 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
       const bool ok = q < RW * CW && ii < d.IA && jj < d.JA;
       const size_t g = ok ? gix(k, ii, jj) : 0;
-      tD[h] = ok ? fullD(a.DENS[g + d.pl], k + 1) : R(0);
+      tD[h] = ok ? a.DENS[g + d.pl] : R(0);
       tX[h] = ok ? a.MOMX[g + d.pl] : R(0);
       tY[h] = ok ? a.MOMY[g + d.pl] : R(0);
       tW[h] = ok ? a.MOMZ[g] : R(0);
-      tR[h] = ok ? fullT(a.RHOT[g], k) : R(0);
+      tR[h] = ok ? a.RHOT[g] : R(0);
     }
     double pD0[NP], pRT[NP], pRR[NP];
     R pR[NP];
@@ -935,8 +941,10 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       if (q >= RW * CW) continue;
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      sD[s1][r][c] = tD[h]; sMX[s1][r][c] = tX[h]; sMY[s1][r][c] = tY[h]; sW[r][c] = tW[h];
-      sPO[r][c] = (ii < d.IA && jj < d.JA) ? tR[h] / sD[s0][r][c] : R(0);
+      const bool in = ii < d.IA && jj < d.JA;
+      sD[s1][r][c] = in ? fullD(tD[h], k + 1) : tD[h];
+      sMX[s1][r][c] = tX[h]; sMY[s1][r][c] = tY[h]; sW[r][c] = tW[h];
+      sPO[r][c] = in ? fullT(tR[h], k) / sD[s0][r][c] : R(0);
     }
 #pragma unroll
     for (int h = 0; h < NP; ++h) {

@@ -74,8 +74,13 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
   auto out = [&](int v, int dir, int k) -> R& { return a.num_diff[p0 + d.pl * (size_t)k + n3 * (size_t)(v + 5 * dir)]; };
   // full density and rho*theta at level k (the stored values in FP64)
   constexpr bool DEV = sizeof(R) == sizeof(float);
-  auto fD = [&](R v, int k) -> R { if constexpr (DEV) return R((double)v + a.offD[k]); else return v; };
-  auto fT = [&](R v, int k) -> R { if constexpr (DEV) return R((double)v + a.offR[k]); else return v; };
+  __shared__ double sOff[DEV ? 2 * KA_DEV_MAX : 1];  // the level profiles
+  if constexpr (DEV) {
+    for (int q = tid; q < d.KA; q += NT) { sOff[q] = a.offD[q]; sOff[KA_DEV_MAX + q] = a.offR[q]; }
+    __syncthreads();
+  }
+  auto fD = [&](R v, int k) -> R { if constexpr (DEV) return R((double)v + sOff[k]); else return v; };
+  auto fT = [&](R v, int k) -> R { if constexpr (DEV) return R((double)v + sOff[KA_DEV_MAX + k]); else return v; };
   auto fillD = [&](R (*dst)[CW + 1], int k) {
     for (int q = tid; q < (RW + 1) * (CW + 1); q += NT) {
       const int r = q / (CW + 1), c = q - r * (CW + 1);
