@@ -740,6 +740,107 @@ int orc_tstep_short_hevi(const orc_grid *g, int s,
  * the HEVI stage uses; the differences are the explicit vertical mass flux,
  * the explicit pressure-gradient and buoyancy terms of MOMZ and the
  * divergence-damping forms. */
+/* ---- UD1 momentum fluxes of the HEVE momentum FCT (UD1.F90:299-1288) ------
+ * F1 (q1 + q0) - sign(F1, vel) (q1 - q0), without num_diff; the face
+ * velocities as in the UD3 routines.  Loop ranges follow the reference,
+ * called with IIS-1..IIE+1, JJS-1..JJE+1. */
+static double ud1f(double q0, double q1, double vel) { return fv_int(ORC_UD1, 0.0, q0, q1, 0.0, vel); }
+
+/* fluxZ_XYW_ud1 (UD1.F90:299-375): z-index added by -1 */
+static void fluxZ_XYW_ud1(const orc_grid *g, double *flux, const double *mom, const double *val, const double *DENS,
+                          double J33G, int IIS, int IIE, int JJS, int JJE) {
+  GRID_LOCALS;
+  OMP_FOR
+  for (int j = JJS; j <= JJE; ++j)
+    for (int i = IIS; i <= IIE; ++i) {
+      for (int k = KS + 1; k <= KE - 1; ++k) {
+        const double vel = (0.5 * (mom[I3(k - 1, i, j)] + mom[I3(k, i, j)])) / DENS[I3(k, i, j)];
+        flux[I3(k - 1, i, j)] = J33G * vel * ud1f(val[I3(k - 1, i, j)], val[I3(k, i, j)], vel);
+      }
+      flux[I3(KS - 1, i, j)] = 0.0;
+      flux[I3(KE - 1, i, j)] = 0.0;
+      flux[I3(KE, i, j)] = 0.0;
+    }
+}
+
+/* fluxX_XYW_ud1 (dir 0, UD1.F90:519-587) and fluxY_XYW_ud1 (dir 1, :591-659) */
+static void fluxH_XYW_ud1(const orc_grid *g, int dir, double *flux, const double *mom, const double *val,
+                          const double *DENS, const double *G, const double *MAPF, const double *CDZ,
+                          int IIS, int IIE, int JJS, int JJE) {
+  GRID_LOCALS;
+  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+  const int mn = dir == 0 ? 1 : 0;
+  OMP_FOR
+  for (int j = JJS - dj; j <= JJE; ++j)
+    for (int i = IIS - di; i <= IIE; ++i) {
+      for (int k = KS; k <= KE - 1; ++k) {
+        const double f1 = F2H(G, k, 1, i, j), f2 = F2H(G, k, 2, i, j);
+        const double vel = (f1 * mom[I3(k + 1, i, j)] + f2 * mom[I3(k, i, j)])
+                         / (f1 * 0.5 * (DENS[I3(k + 1, i, j)] + DENS[I3(k + 1, i + di, j + dj)])
+                          + f2 * 0.5 * (DENS[I3(k, i, j)] + DENS[I3(k, i + di, j + dj)]));
+        flux[I3(k, i, j)] = G[I3(k, i, j)] / MAP(MAPF, i, j, mn) * vel
+                          * ud1f(val[I3(k, i, j)], val[I3(k, i + di, j + dj)], vel);
+      }
+      flux[I3(KE, i, j)] = 0.0;
+    }
+}
+
+/* fluxZ_UYZ_ud1 (stag 0, UD1.F90:664-737) and fluxZ_XVZ_ud1 (stag 1, :979-1052) */
+static void fluxZ_UVZ_ud1(const orc_grid *g, int stag, double *flux, const double *mom, const double *val,
+                          const double *DENS, const double *G, double J33G, const double *CDZ,
+                          int IIS, int IIE, int JJS, int JJE) {
+  GRID_LOCALS;
+  const int di = stag == 0 ? 1 : 0, dj = stag == 0 ? 0 : 1;
+  OMP_FOR
+  for (int j = JJS; j <= JJE; ++j)
+    for (int i = IIS; i <= IIE; ++i) {
+      for (int k = KS; k <= KE - 1; ++k) {
+        const double vel = (0.5 * (mom[I3(k, i, j)] + mom[I3(k, i + di, j + dj)]))
+                         / (F2H(G, k, 1, i, j) * 0.5 * (DENS[I3(k + 1, i, j)] + DENS[I3(k + 1, i + di, j + dj)])
+                          + F2H(G, k, 2, i, j) * 0.5 * (DENS[I3(k, i, j)] + DENS[I3(k, i + di, j + dj)]));
+        flux[I3(k, i, j)] = J33G * vel * ud1f(val[I3(k, i, j)], val[I3(k + 1, i, j)], vel);
+      }
+      flux[I3(KS - 1, i, j)] = 0.0;
+      flux[I3(KE, i, j)] = 0.0;
+    }
+}
+
+/* the flux along the momentum's own direction, stored one cell back:
+ * fluxX_UYZ_ud1 (dir 0, UD1.F90:869-920) and fluxY_XVZ_ud1 (dir 1, :1237-1288) */
+static void fluxS_ud1(const orc_grid *g, int dir, double *flux, const double *mom, const double *val,
+                      const double *DENS, const double *G, const double *MAPF, int IIS, int IIE, int JJS, int JJE) {
+  GRID_LOCALS;
+  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+  const int mn = dir == 0 ? 1 : 0;
+  OMP_FOR
+  for (int j = JJS; j <= JJE + dj; ++j)
+    for (int i = IIS; i <= IIE + di; ++i)
+      for (int k = KS; k <= KE; ++k) {
+        const double vel = (0.5 * (mom[I3(k, i, j)] + mom[I3(k, i - di, j - dj)])) / (DENS[I3(k, i, j)]);
+        flux[I3(k, i - di, j - dj)] = G[I3(k, i, j)] / MAP(MAPF, i, j, mn) * vel
+                                    * ud1f(val[I3(k, i - di, j - dj)], val[I3(k, i, j)], vel);
+      }
+}
+
+/* the cross flux: fluxY_UYZ_ud1 (stag 0, UD1.F90:924-973; faces j, j+1 of the
+ * u point) and fluxX_XVZ_ud1 (stag 1, :1184-1233; faces i, i+1 of the v point) */
+static void fluxC_ud1(const orc_grid *g, int stag, double *flux, const double *mom, const double *val,
+                      const double *DENS, const double *G, const double *MAPF, int IIS, int IIE, int JJS, int JJE) {
+  GRID_LOCALS;
+  const int di = stag == 0 ? 1 : 0, dj = stag == 0 ? 0 : 1;  /* the momentum's own staggering */
+  const int ci = stag == 0 ? 0 : 1, cj = stag == 0 ? 1 : 0;  /* the flux direction */
+  const int mn = stag == 0 ? 0 : 1;
+  OMP_FOR
+  for (int j = JJS - cj; j <= JJE; ++j)
+    for (int i = IIS - ci; i <= IIE; ++i)
+      for (int k = KS; k <= KE; ++k) {
+        const double vel = (0.5 * (mom[I3(k, i, j)] + mom[I3(k, i + di, j + dj)]))
+                         / (0.25 * (DENS[I3(k, i, j)] + DENS[I3(k, i + 1, j)] + DENS[I3(k, i, j + 1)] + DENS[I3(k, i + 1, j + 1)]));
+        flux[I3(k, i, j)] = G[I3(k, i, j)] / MAP(MAPF, i, j, mn) * vel
+                          * ud1f(val[I3(k, i, j)], val[I3(k, i + ci, j + cj)], vel);
+      }
+}
+
 int orc_tstep_short_heve(const orc_grid *g, int s,
     double *DENS_RK, double *MOMZ_RK, double *MOMX_RK, double *MOMY_RK, double *RHOT_RK,
     double *mflx_hi, double *tflx_hi,
@@ -751,7 +852,7 @@ int orc_tstep_short_heve(const orc_grid *g, int s,
     int flag_fct_momentum, int flag_fct_t, int flag_fct_along_stream,
     const orc_metric *m, const double *REF_dens, const double *REF_rhot, double dtrk) {
   GRID_LOCALS;
-  if ((s != ORC_UD3 && s != ORC_CD4) || flag_fct_momentum) return -1;
+  if (s != ORC_UD3 && s != ORC_CD4) return -1;
   const double *CDZ = m->CDZ, *FDZ = m->FDZ, *FDX = m->FDX, *FDY = m->FDY;
   const double *RCDZ = m->RCDZ, *RCDX = m->RCDX, *RCDY = m->RCDY;
   const double *RFDZ = m->RFDZ, *RFDX = m->RFDX, *RFDY = m->RFDY;
@@ -783,6 +884,56 @@ int orc_tstep_short_heve(const orc_grid *g, int s,
       DPRES[I3(KE + 1, i, j)] = DPRES0[I3(KE + 1, i, j)] - DENS[I3(KE, i, j)] * (PHI[I3(KE + 1, i, j)] - PHI[I3(KE - 1, i, j)]);
       for (int k = KS; k <= KE; ++k) POTT[I3(k, i, j)] = RHOT[I3(k, i, j)] / DENS[I3(k, i, j)];
     }
+
+  /* momentum FCT (HEVE.F90:386-450): the start-of-stage velocities with
+   * their periodic halos, the limiter's advected quantity */
+  double *VELZ = NULL, *VELX = NULL, *VELY = NULL, *mlo = NULL, *manti = NULL, *D0u = NULL, *D1u = NULL;
+  if (flag_fct_momentum) {
+    VELZ = alloc_nan(n3); VELX = alloc_nan(n3); VELY = alloc_nan(n3);
+    mlo = alloc_nan(3 * n3); manti = alloc_nan(3 * n3); D0u = alloc_nan(n3); D1u = alloc_nan(n3);
+    OMP_FOR
+    for (int j = JS - 1; j <= JE + 2; ++j)
+      for (int i = IS - 1; i <= IE + 2; ++i) {
+        for (int k = KS; k <= KE - 1; ++k)
+          VELZ[I3(k, i, j)] = 2.0 * MOMZ0[I3(k, i, j)] / (DENS0[I3(k + 1, i, j)] + DENS0[I3(k, i, j)]);
+        VELZ[I3(KE, i, j)] = 0.0;
+      }
+    OMP_FOR
+    for (int j = JS - 1; j <= JE + 2; ++j)
+      for (int i = IS - 2; i <= IE + 1; ++i)
+        for (int k = KS; k <= KE; ++k)
+          VELX[I3(k, i, j)] = 2.0 * MOMX0[I3(k, i, j)] / (DENS0[I3(k, i + 1, j)] + DENS0[I3(k, i, j)]);
+    OMP_FOR
+    for (int j = JS - 2; j <= JE + 1; ++j)
+      for (int i = IS - 1; i <= IE + 2; ++i)
+        for (int k = KS; k <= KE; ++k)
+          VELY[I3(k, i, j)] = 2.0 * MOMY0[I3(k, i, j)] / (DENS0[I3(k, i, j + 1)] + DENS0[I3(k, i, j)]);
+    orc_halo_periodic(g, VELZ); orc_halo_periodic(g, VELX); orc_halo_periodic(g, VELY);
+  }
+  /* densities averaged to the momentum's position (di, dj, dk: the
+   * neighbour), from DENS0 and the halo-exchanged DENS_RK (HEVE.F90:757-773,
+   * 1000-1010, 1250-1260); at KE the vertical average repeats KE-1 */
+#define MOM_UVW_DENS(di, dj, dk)                                                                          \
+  do {                                                                                                    \
+    for (int j = JS - 1; j <= JE + 1; ++j)                                                                \
+      for (int i = IS - 1; i <= IE + 1; ++i) {                                                            \
+        for (int k = KS; k <= KE - (dk); ++k) {                                                           \
+          D0u[I3(k, i, j)] = 0.5 * (DENS0[I3(k, i, j)] + DENS0[I3(k + (dk), i + (di), j + (dj))]);       \
+          D1u[I3(k, i, j)] = 0.5 * (DENS_RK[I3(k, i, j)] + DENS_RK[I3(k + (dk), i + (di), j + (dj))]);   \
+        }                                                                                                 \
+        if (dk) { D0u[I3(KE, i, j)] = D0u[I3(KE - 1, i, j)]; D1u[I3(KE, i, j)] = D1u[I3(KE - 1, i, j)]; } \
+      }                                                                                                   \
+  } while (0)
+  /* the high-order z flux as ATMOS_DYN_fct takes it: over the map factors,
+   * plus the J13/J23 fluxes (HEVE.F90:752-756, 990-996, 1241-1247) */
+#define MOM_HI_Z(c)                                                                                     \
+  do {                                                                                                  \
+    for (int j = JS; j <= JE; ++j)                                                                      \
+      for (int i = IS; i <= IE; ++i)                                                                    \
+        for (int k = KS; k <= KE; ++k)                                                                  \
+          qZ[I3(k, i, j)] = qZ[I3(k, i, j)] / (MAPF(i, j, 0, c) * MAPF(i, j, 1, c))                     \
+                          + qflx_J13[I3(k, i, j)] + qflx_J23[I3(k, i, j)];                              \
+  } while (0)
 
   /* mass fluxes (HEVE.F90:452-520) */
   OMP_FOR
@@ -829,6 +980,8 @@ int orc_tstep_short_heve(const orc_grid *g, int s,
                                        + DENS_t[I3(k, i, j)]);
       }
 
+  if (flag_fct_momentum) orc_halo_periodic(g, DENS_RK); /* HEVE.F90:746-747 */
+
   /* MOMZ (HEVE.F90:588-700) */
   fluxZ_XYW(g, s, qZ, MOMZ, MOMZ, DENS, GS + n3 * I_XYZ, J33G, num_diff + n3 * (I_MOMZ + 5 * ZDIR), IIS, IIE, JJS, JJE);
   fluxJ_XYW(g, s, 0, qflx_J13, MOMX, MOMZ, DENS, J13 + n3 * I_XYZ, MP + (size_t)IA * JA * 2 * I_XY, IIS, IIE, JJS, JJE);
@@ -857,6 +1010,30 @@ int orc_tstep_short_heve(const orc_grid *g, int s,
       MOMZ_RK[I3(KS - 1, i, j)] = 0.0;
       MOMZ_RK[I3(KE, i, j)] = 0.0;
     }
+
+  /* FCT of the MOMZ fluxes (HEVE.F90:716-810) */
+  if (flag_fct_momentum) {
+    fluxZ_XYW_ud1(g, mlo, MOMZ, MOMZ0, DENS, J33G, IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    fluxH_XYW_ud1(g, 0, mlo + n3, MOMX, MOMZ0, DENS, GS + n3 * I_UYZ, MP + (size_t)IA * JA * 2 * I_UY, CDZ,
+                  IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    fluxH_XYW_ud1(g, 1, mlo + 2 * n3, MOMY, MOMZ0, DENS, GS + n3 * I_XVZ, MP + (size_t)IA * JA * 2 * I_XV, CDZ,
+                  IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    MOM_HI_Z(I_XY);
+    MOM_UVW_DENS(0, 0, 1);
+    orc_fct(g, manti, VELZ, D0u, D1u, qflx_hi, mlo, mflx_hi, RFDZ, RCDX, RCDY, GS + n3 * I_XYW,
+            MP + (size_t)IA * JA * 2 * I_XY, dtrk, flag_fct_along_stream);
+    const double *aZ = manti, *aX = manti + n3, *aY = manti + 2 * n3;
+    OMP_FOR
+    for (int j = JJS; j <= JJE; ++j)
+      for (int i = IIS; i <= IIE; ++i)
+        for (int k = KS; k <= KE - 1; ++k)
+          MOMZ_RK[I3(k, i, j)] = MOMZ_RK[I3(k, i, j)]
+                               + dtrk * ((aZ[I3(k, i, j)] - aZ[I3(k - 1, i, j)]) * RFDZ[k]
+                                         + ((aX[I3(k, i, j)] - aX[I3(k, i - 1, j)]) * RCDX[i]
+                                            + (aY[I3(k, i, j)] - aY[I3(k, i, j - 1)]) * RCDY[j])
+                                           * MAPF(i, j, 0, I_XY) * MAPF(i, j, 1, I_XY))
+                                 / GSQRT(k, i, j, I_XYW);
+  }
 
   /* MOMX (HEVE.F90:744-945) */
   fluxZ_UVZ(g, s, 0, qZ, MOMZ, MOMX, DENS, GS + n3 * I_UYW, J33G, num_diff + n3 * (I_MOMX + 5 * ZDIR), CDZ, IIS, IIE, JJS, JJE);
@@ -901,6 +1078,29 @@ int orc_tstep_short_heve(const orc_grid *g, int s,
         }
   }
 
+  /* FCT of the MOMX fluxes (HEVE.F90:958-1057) */
+  if (flag_fct_momentum) {
+    fluxZ_UVZ_ud1(g, 0, mlo, MOMZ, MOMX0, DENS, GS + n3 * I_UYW, J33G, CDZ, IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    fluxS_ud1(g, 0, mlo + n3, MOMX, MOMX0, DENS, GS + n3 * I_XYZ, MP + (size_t)IA * JA * 2 * I_UY,
+              IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    fluxC_ud1(g, 0, mlo + 2 * n3, MOMY, MOMX0, DENS, GS + n3 * I_UVZ, MP + (size_t)IA * JA * 2 * I_XV,
+              IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    MOM_HI_Z(I_UY);
+    MOM_UVW_DENS(1, 0, 0);
+    orc_fct(g, manti, VELX, D0u, D1u, qflx_hi, mlo, mflx_hi, RCDZ, RFDX, RCDY, GS + n3 * I_UYZ,
+            MP + (size_t)IA * JA * 2 * I_UY, dtrk, flag_fct_along_stream);
+    const double *aZ = manti, *aX = manti + n3, *aY = manti + 2 * n3;
+    OMP_FOR
+    for (int j = JJS; j <= JJE; ++j)
+      for (int i = IIS; i <= (IIE < IEH ? IIE : IEH); ++i)
+        for (int k = KS; k <= KE; ++k)
+          MOMX_RK[I3(k, i, j)] = MOMX_RK[I3(k, i, j)]
+                               + dtrk * (((aZ[I3(k, i, j)] - aZ[I3(k - 1, i, j)]) * RCDZ[k]
+                                          + (aX[I3(k, i, j)] - aX[I3(k, i - 1, j)]) * RFDX[i]
+                                          + (aY[I3(k, i, j)] - aY[I3(k, i, j - 1)]) * RCDY[j]))
+                                 * MAPF(i, j, 0, I_UY) * MAPF(i, j, 1, I_UY) / GSQRT(k, i, j, I_UYZ);
+  }
+
   /* MOMY (HEVE.F90:993-1193) */
   fluxZ_UVZ(g, s, 1, qZ, MOMZ, MOMY, DENS, GS + n3 * I_XVW, J33G, num_diff + n3 * (I_MOMY + 5 * ZDIR), CDZ, IIS, IIE, JJS, JJE);
   fluxJ_UVZ(g, s, 1, 0, qflx_J13, MOMX, MOMY, DENS, GS + n3 * I_XVZ, J13 + n3 * I_XVW, MP + (size_t)IA * JA * 2 * I_XV, CDZ, IIS, IIE, JJS, JJE);
@@ -943,6 +1143,32 @@ int orc_tstep_short_heve(const orc_grid *g, int s,
                                + dtrk * ((advcv + advch - pgf) / GSQRT(k, i, j, I_XVZ) + cor + div + MOMY_t[I3(k, i, j)]);
         }
   }
+
+  /* FCT of the MOMY fluxes (HEVE.F90:1207-1309) */
+  if (flag_fct_momentum) {
+    fluxZ_UVZ_ud1(g, 1, mlo, MOMZ, MOMY0, DENS, GS + n3 * I_XVZ, J33G, CDZ, IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    fluxC_ud1(g, 1, mlo + n3, MOMX, MOMY0, DENS, GS + n3 * I_UVZ, MP + (size_t)IA * JA * 2 * I_XY,
+              IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    fluxS_ud1(g, 1, mlo + 2 * n3, MOMY, MOMY0, DENS, GS + n3 * I_XYZ, MP + (size_t)IA * JA * 2 * I_XY,
+              IIS - 1, IIE + 1, JJS - 1, JJE + 1);
+    MOM_HI_Z(I_XV);
+    MOM_UVW_DENS(0, 1, 0);
+    orc_fct(g, manti, VELY, D0u, D1u, qflx_hi, mlo, mflx_hi, RCDZ, RCDX, RFDY, GS + n3 * I_XVZ,
+            MP + (size_t)IA * JA * 2 * I_XV, dtrk, flag_fct_along_stream);
+    const double *aZ = manti, *aX = manti + n3, *aY = manti + 2 * n3;
+    OMP_FOR
+    for (int j = JJS; j <= (JJE < JEH ? JJE : JEH); ++j)
+      for (int i = IIS; i <= IIE; ++i)
+        for (int k = KS; k <= KE; ++k)
+          MOMY_RK[I3(k, i, j)] = MOMY_RK[I3(k, i, j)]
+                               + dtrk * (((aZ[I3(k, i, j)] - aZ[I3(k - 1, i, j)]) * RCDZ[k]
+                                          + (aX[I3(k, i, j)] - aX[I3(k, i - 1, j)]) * RCDX[i]
+                                          + (aY[I3(k, i, j)] - aY[I3(k, i, j - 1)]) * RFDY[j]))
+                                 * MAPF(i, j, 0, I_XV) * MAPF(i, j, 1, I_XV) / GSQRT(k, i, j, I_XVZ);
+    free(VELZ); free(VELX); free(VELY); free(mlo); free(manti); free(D0u); free(D1u);
+  }
+#undef MOM_UVW_DENS
+#undef MOM_HI_Z
 
   /* RHOT (HEVE.F90:1316-1375) */
   fluxZ_XYZ(g, s, tfZ, mfZ, POTT, GS + n3 * I_XYW, num_diff + n3 * (I_RHOT + 5 * ZDIR), IIS, IIE, JJS, JJE);

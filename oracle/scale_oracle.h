@@ -68,9 +68,10 @@ int orc_tstep_short_hevi(const orc_grid *g, int scheme,
     const orc_metric *m, const double *REF_dens, const double *REF_rhot,
     double dtrk);
 
-/* ATMOS_DYN_Tstep_short_fvm_heve (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1401)
- * without FCT (flag_fct_momentum = flag_fct_t = 0; otherwise -1), periodic
- * lateral boundaries. */
+/* ATMOS_DYN_Tstep_short_fvm_heve (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1516)
+ * with the momentum FCT (flag_fct_momentum, :386-450, 716-810, 958-1057,
+ * 1207-1309) and the theta FCT (flag_fct_t, :1388-1475), periodic lateral
+ * boundaries; -1 for a scheme other than UD3/CD4. */
 int orc_tstep_short_heve(const orc_grid *g, int scheme,
     double *DENS_RK, double *MOMZ_RK, double *MOMX_RK, double *MOMY_RK, double *RHOT_RK,
     double *mflx_hi, double *tflx_hi,

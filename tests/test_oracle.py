@@ -421,11 +421,13 @@ def test_heve_conserves_mass_and_rhot(scheme):
         assert abs(a1 - a0) <= 2e-15 * abs(a0) * g.KA, v
 
 
-def test_heve_xy_mirror_symmetry():
+@pytest.mark.parametrize("fct", [False, True], ids=["plain", "fct_momentum_t"])
+def test_heve_xy_mirror_symmetry(fct):
     case = H.small_case(kmax=16, imax=10, jmax=10, seed=17)
     inp = H.stage_inputs(case, seed=4)
-    out = po.tstep_short_heve(case, inp, 0.7)
-    tout = po.tstep_short_heve(inputs.Case(**{**case.__dict__}), _transpose_case_inputs(case, inp), 0.7)
+    kw = dict(fct_momentum=fct, fct_t=fct, along_stream=False)
+    out = po.tstep_short_heve(case, inp, 0.7, **kw)
+    tout = po.tstep_short_heve(inputs.Case(**{**case.__dict__}), _transpose_case_inputs(case, inp), 0.7, **kw)
     T = lambda a: np.swapaxes(a, 1, 2)
     g = case.grid
     sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
@@ -463,11 +465,125 @@ def test_heve_divergence_damping_matches_hevi():
     assert np.abs(x - y).max() < 1e-2 * np.abs(x).max()
 
 
-def test_heve_rejects_fct_momentum():
-    case = H.small_case(kmax=8, imax=6, jmax=6)
-    inp = H.stage_inputs(case)
-    with pytest.raises(RuntimeError):
-        po.tstep_short_heve(case, inp, 0.5, fct_momentum=True)
+def _momentum_shear_inputs(axis):
+    """A horizontally uniform atmosphere at hydrostatic rest with the
+    non-divergent flow u = 12, v = V(x) (axis 0) or v = 12, u = U(y) (axis 1),
+    V and U top-hats between -4 and 6 m/s.  The density stays unchanged, the
+    pressure gradient and Coriolis terms vanish, and the momentum equation of
+    the top-hat component is linear advection by the uniform component."""
+    case = inputs.make_case(kmax=20, imax=12, jmax=10, noise=0.0, bubble=False, u0=0.0, ntracer=0)
+    g = case.grid
+    inp = {}
+    for v in H.VARS:
+        inp[v] = np.array(case.state[v], order="F", copy=True)
+        inp[v + "_t"] = g.zeros()
+    inp.update(inputs.large_step_prep(case))
+    inp["num_diff"] = np.asfortranarray(np.zeros((g.KA, g.IA, g.JA, 5, 3)))
+    inp["DDIV"] = g.zeros()
+    inp["REF_dens"] = case.ref["REF_dens"]
+    inp["DPRES0"] = g.zeros()
+    D = inp["DENS"]
+    uni, hat = g.zeros(), g.zeros()
+    uni[:] = 12.0 * D
+    hat[:] = -4.0 * D
+    b = (slice(None), slice(g.IS + 3, g.IS + 7), slice(None)) if axis == 0 else \
+        (slice(None), slice(None), slice(g.JS + 3, g.JS + 7))
+    hat[b] = 6.0 * D[b]
+    for a in (uni, hat):
+        inputs.halo_fill(g, a)
+    inp["MOMX"], inp["MOMY"] = (uni, hat) if axis == 0 else (hat, uni)
+    inp["MOMZ"] = g.zeros()
+    for v in H.VARS:
+        inp[v + "0"] = inp[v]
+    return case, inp
+
+
+@pytest.mark.parametrize("along", [True, False], ids=["along_stream", "7point"])
+@pytest.mark.parametrize("scheme", [UD3, CD4])
+@pytest.mark.parametrize("axis", [0, 1], ids=["v_of_x", "u_of_y"])
+def test_heve_fct_momentum_bounds_horizontal_velocity(axis, scheme, along):
+    """FLAG_FCT_MOMENTUM (HEVE.F90:386-450, 716-810, 958-1057, 1207-1309):
+    for the advected top-hat the density is steady and the flow
+    non-divergent, so the UD1 solution is monotone and the limited velocity
+    stays inside the initial range to round-off; without FCT the scheme
+    over- and undershoots by more than half a m/s."""
+    case, inp = _momentum_shear_inputs(axis)
+    g = case.grid
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    name = "MOMY" if axis == 0 else "MOMX"
+    D = inp["DENS0"][sl]
+    q0 = inp[name + "0"][sl] / D
+    plain = po.tstep_short_heve(case, inp, 8.0, scheme=scheme)
+    fct = po.tstep_short_heve(case, inp, 8.0, scheme=scheme, fct_momentum=True, along_stream=along)
+    assert np.array_equal(plain["DENS_RK"][sl], inp["DENS0"][sl])
+    qp = plain[name + "_RK"][sl] / D
+    qf = fct[name + "_RK"][sl] / D
+    assert qp.max() > q0.max() + 0.5 and qp.min() < q0.min() - 0.5  # the control
+    assert qf.max() <= q0.max() + 1e-12 and qf.min() >= q0.min() - 1e-12
+    assert np.abs(qf - q0).max() > 1.0  # the top-hat moved
+    other = "MOMX" if axis == 0 else "MOMY"
+    assert np.allclose(fct[other + "_RK"][sl] / D, 12.0, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("scheme", [UD3, CD4])
+def test_heve_fct_momentum_limits_vertical_velocity(scheme):
+    """A 3 m/s box of w in the uniform flow u = 12: without FCT w over- and
+    undershoots its range by 0.2 m/s or more; with it the excursion is below
+    1 % of that (the residual comes from the box's own vertical mass
+    divergence, which changes DENS_RK under the box's edges)."""
+    case, inp = _momentum_shear_inputs(0)
+    g = case.grid
+    D = inp["DENS0"]
+    Dz = 0.5 * (D[:-1] + D[1:])
+    mz = g.zeros()
+    b = (slice(g.KS + 5, g.KS + 11), slice(g.IS + 3, g.IS + 8), slice(g.JS + 2, g.JS + 6))
+    mz[b] = 3.0 * Dz[b]
+    inputs.halo_fill(g, mz)
+    inp["MOMZ"] = inp["MOMZ0"] = mz
+    inp["MOMY"] = inp["MOMY0"] = g.zeros()
+    inp["wdamp_coef"] = np.zeros(g.KA)
+    slz = (slice(g.KS, g.KE), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    q0 = (mz[:-1] / Dz)[slz]
+    ex = []
+    for fct in (False, True):
+        o = po.tstep_short_heve(case, inp, 8.0, scheme=scheme, fct_momentum=fct)
+        Dn = np.array(o["DENS_RK"], order="F", copy=True)
+        inputs.halo_fill(g, Dn)
+        q = (o["MOMZ_RK"][:-1] / (0.5 * (Dn[:-1] + Dn[1:])))[slz]
+        ex.append((q.max() - q0.max(), q0.min() - q.min()))
+    assert min(ex[0]) > 0.15, ex
+    assert max(ex[1]) < 1e-2 * min(ex[0]), ex
+
+
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+def test_heve_fct_momentum_is_conservative(terrain):
+    """The FCT correction of each momentum component is a flux divergence:
+    weighted by each point's volume over its map factors it sums to zero
+    over the periodic domain (the limited anti-diffusive fluxes vanish at
+    the bottom and top), it is not zero, and DENS_RK, RHOT_RK, mflx_hi and
+    tflx_hi are unchanged."""
+    case = H.small_case(seed=31)
+    if terrain:
+        case = inputs.Case(**{**case.__dict__, "metric": H.terrain_metrics(case)})
+    g, m = case.grid, case.metric
+    inp = _heve_inputs(case)
+    plain = po.tstep_short_heve(case, inp, 2.0, scheme=UD3)
+    fct = po.tstep_short_heve(case, inp, 2.0, scheme=UD3, fct_momentum=True)
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    for k in ("DENS_RK", "RHOT_RK", "tflx_hi"):
+        assert np.array_equal(plain[k][sl], fct[k][sl]), k
+    assert np.array_equal(plain["mflx_hi"][sl], fct["mflx_hi"][sl])
+    GS, MP = m["GSQRT"], m["MAPF"]
+    ii, jj = slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1)
+    for v, kk, c, mc, rz, rx, ry in (("MOMZ", slice(g.KS, g.KE), 1, 0, "RFDZ", "RCDX", "RCDY"),
+                                     ("MOMX", slice(g.KS, g.KE + 1), 4, 1, "RCDZ", "RFDX", "RCDY"),
+                                     ("MOMY", slice(g.KS, g.KE + 1), 5, 2, "RCDZ", "RCDX", "RFDY")):
+        d = fct[v + "_RK"][kk, ii, jj] - plain[v + "_RK"][kk, ii, jj]
+        w = GS[kk, ii, jj, c] / (MP[ii, jj, 0, mc] * MP[ii, jj, 1, mc])[None] \
+            / (m[rz][kk][:, None, None] * m[rx][ii][None, :, None] * m[ry][jj][None, None, :])
+        scale = np.abs(d * w).sum()
+        assert np.abs(d).max() > 1e-6 * np.abs(plain[v + "_RK"][kk, ii, jj]).max(), v
+        assert abs((d * w).sum()) < 1e-12 * scale, (v, (d * w).sum(), scale)
 
 
 def _theta_tophat_inputs():
