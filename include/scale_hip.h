@@ -85,11 +85,13 @@ int scale_hip_tstep_short_hevi(scale_hip_ctx *ctx,
  * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_short.F90:38-128),
  * implemented by ATMOS_DYN_Tstep_short_fvm_heve,
  * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1516.
- * FLAG_FCT_T limits the theta flux with ATMOS_DYN_fct (HEVE.F90:1388-1475;
- * DENS_RK and mflx_hi then also get their periodic halos, as the reference's
- * exchanges give them).  FLAG_FCT_MOMENTUM must be .false.: the momentum FCT
- * (HEVE.F90:386-450, 716-810, 958-1057, 1207-1309) is not built and the call
- * fails with an error instead. */
+ * FLAG_FCT_MOMENTUM limits the momentum fluxes with ATMOS_DYN_fct on the
+ * start-of-stage velocities (HEVE.F90:386-450, 716-810, 958-1057, 1207-1309;
+ * DENS_RK then also gets its periodic halo) and FLAG_FCT_T the theta flux
+ * (HEVE.F90:1388-1475; DENS_RK and mflx_hi then get their periodic halos),
+ * as the reference's exchanges leave them.  The first call with
+ * FLAG_FCT_MOMENTUM allocates ten KA*IA*JA work arrays, kept until
+ * scale_hip_finalize. */
 int scale_hip_tstep_short_heve(scale_hip_ctx *ctx,
     double *DENS_RK, double *MOMZ_RK, double *MOMX_RK, double *MOMY_RK, double *RHOT_RK,
     double *PROG_RK, double *mflx_hi, double *tflx_hi,

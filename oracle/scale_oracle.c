@@ -733,9 +733,9 @@ int orc_tstep_short_hevi(const orc_grid *g, int s,
 #undef F2HH
 }
 
-/* ---- HEVE stage (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1401) -----
- * The fully explicit short step, FLAG_FCT_MOMENTUM = FLAG_FCT_T = .false.
- * (the FCT branches return -1).  HEVE = scalelib/src/atmos-rm/dynamics/
+/* ---- HEVE stage (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1516) -----
+ * The fully explicit short step with its optional FCT of the momentum and
+ * theta fluxes.  HEVE = scalelib/src/atmos-rm/dynamics/
  * scale_atmos_dyn_tstep_short_fvm_heve.F90.  The flux routines are the ones
  * the HEVI stage uses; the differences are the explicit vertical mass flux,
  * the explicit pressure-gradient and buoyancy terms of MOMZ and the

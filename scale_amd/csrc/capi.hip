@@ -337,6 +337,7 @@ int scale_hip_finalize(scale_hip_ctx* h) {
   F(c.g1buf); F(c.GSQRT); F(c.J13G); F(c.J23G); F(c.MAPF); F(c.PHI);
   F(c.CNZ3); F(c.CNX3); F(c.CNY3); F(c.ndc); F(c.ndqc); F(c.stage);
   for (int q = 0; q < shp::NSCR; ++q) F(c.scr[q]);
+  F(c.mfct);
   for (double* b : h->dbuf) F(b);
   shp::adj_free(c);
   if (c.model) {
@@ -380,13 +381,14 @@ static int short_stage(bool heve, scale_hip_ctx* h,
   (void)PROG_RK; (void)PROG0; (void)PROG; (void)last;
   if (int r = check_ctx(h)) return r;
   if (!dtrk || !divdmp_coef || !J33G) return fail("null scalar argument");
-  // HEVI ignores the FCT flags (HEVI.F90:224-226); HEVE's momentum FCT is not built
-  if (heve && FLAG_FCT_MOMENTUM && *FLAG_FCT_MOMENTUM)
-    return fail("HEVE with FLAG_FCT_MOMENTUM is not supported");
+  // HEVI ignores the FCT flags (HEVI.F90:224-226)
+  const bool fct_mom = heve && FLAG_FCT_MOMENTUM && *FLAG_FCT_MOMENTUM;
   if ((BND_W && *BND_W) || (BND_E && *BND_E) || (BND_S && *BND_S) || (BND_N && *BND_N))
     return fail("only periodic lateral boundaries (BND_* = .false.) are supported");
   Ctx& c = h->c;
   if (int r = ensure_dropin(h)) return r;
+  if (fct_mom && !c.mfct && !(c.mfct = dalloc(c, shp::MFCT_FIELDS * c.d.n3, nullptr)))
+    return fail("hipMalloc failed for the momentum FCT work arrays");
   if (int r = set_grid1d(c, CDZ, FDZ, RCDZ, RFDZ, nullptr, FDX, RCDX, RFDX, nullptr, FDY, RCDY, RFDY)) return r;
   if (int r = set_metric3d(c, PHI, GSQRT, J13G, J23G, *J33G, MAPF)) return r;
   auto& B = h->dbuf;
@@ -418,6 +420,7 @@ static int short_stage(bool heve, scale_hip_ctx* h,
   io.DPRES0 = B[B_DPRES0]; io.RT2P = B[B_RT2P]; io.CORIOLI = B[B_COR]; io.num_diff = B[B_ND]; io.wdamp_coef = B[B_WD];
   io.DDIV = B[B_DDIV]; io.REF_dens = B[B_REFD]; io.REF_rhot = B[B_REFR];
   io.divdmp_coef = *divdmp_coef; io.dtrk = *dtrk; io.has_coriolis = has_cor;
+  io.fct_momentum = fct_mom;
   io.fct_t = heve && FLAG_FCT_T && *FLAG_FCT_T;
   io.fct_along = !FLAG_FCT_ALONG_STREAM || *FLAG_FCT_ALONG_STREAM;
   CK(heve ? shp::heve_stage(io, c.stream) : shp::hevi_stage(io, c.stream));

@@ -1385,11 +1385,203 @@ static hipError_t launch_heve(const HeviArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// HEVE momentum FCT (FLAG_FCT_MOMENTUM; HEVE.F90:386-450, 716-810, 958-1057,
+// 1207-1309).  After the stage kernel, for each momentum component C
+// (0 MOMZ, 1 MOMX, 2 MOMY):
+//   mfct_vel    its start-of-stage velocity on the interior (HEVE.F90:386-446),
+//               followed by the periodic halo fill the exchange performs
+//   mfct_flux   the UD1 fluxes of the start-of-stage momentum carried by the
+//               current one (scale_atmos_dyn_fvm_flux_ud1.F90) on the interior
+//               and the rings the low-order field reads, and the anti-diffusive
+//               fluxes hi - lo on the interior; the high-order z flux enters
+//               divided by the map factors and joined by its J13/J23 parts
+//               (HEVE.F90:752-756, 990-996, 1241-1247)
+//   fct_momentum_limit (tracer.hip): low-order field, ratios, limited fluxes
+//               and the MOM_RK update.
+// Work arrays: Ctx::mfct, MFCT_FIELDS fields of n3.
+template <int C>
+__global__ void __launch_bounds__(256) mfct_vel(Dims d, const double* M0, const double* D0, double* vel) {
+  const int i = d.IS + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int k = d.KS + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int j = d.JS + (int)blockIdx.z;
+  if (i > d.IE || k > d.KE || j > d.JE) return;
+  const size_t p = IX(d, k, i, j);
+  if constexpr (C == 0) vel[p] = (k <= d.KE - 1) ? 2.0 * M0[p] / (D0[IX(d, k + 1, i, j)] + D0[p]) : 0.0;
+  else if constexpr (C == 1) vel[p] = 2.0 * M0[p] / (D0[IX(d, k, i + 1, j)] + D0[p]);
+  else vel[p] = 2.0 * M0[p] / (D0[IX(d, k, i, j + 1)] + D0[p]);
+}
+
+__device__ __forceinline__ double ud1v(double q0, double q1, double vel) { return fv_int<UD1>(0.0, q0, q1, 0.0, vel); }
+
+// the low-order fluxes: z at stored level kk, x at face ii, y at face jj
+template <int S, bool TERRAIN, int C>
+__device__ double mlo_z(const Ops<S, TERRAIN, false>& o, const HeviArgs& a, int kk, int i, int j) {
+  const Dims& d = a.d;
+  const double* W = a.MOMZ;
+  if constexpr (C == 0) {  // fluxZ_XYW_ud1 (UD1.F90:299-375)
+    if (kk == d.KS - 1 || kk >= d.KE - 1) return 0.0;
+    const double* V = a.MOMZ0;
+    const int k = kk + 1;
+    const double vel = (0.5 * (W[IX(d, k - 1, i, j)] + W[IX(d, k, i, j)])) / o.D(k, i, j);
+    return a.J33G * vel * ud1v(V[IX(d, kk, i, j)], V[IX(d, k, i, j)], vel);
+  } else {  // fluxZ_UYZ_ud1 / fluxZ_XVZ_ud1 (UD1.F90:664-737, 979-1052)
+    if (kk == d.KS - 1 || kk == d.KE) return 0.0;
+    const double* V = C == 1 ? a.MOMX0 : a.MOMY0;
+    const int di = C == 1 ? 1 : 0, dj = C == 1 ? 0 : 1;
+    const int c = C == 1 ? I_UYW : I_XVZ;  // the GSQRT component the reference passes
+    const double vel = (0.5 * (W[IX(d, kk, i, j)] + W[IX(d, kk, i + di, j + dj)]))
+                     / (o.F2H(kk, 1, c, i, j) * 0.5 * (o.D(kk + 1, i, j) + o.D(kk + 1, i + di, j + dj))
+                      + o.F2H(kk, 2, c, i, j) * 0.5 * (o.D(kk, i, j) + o.D(kk, i + di, j + dj)));
+    return a.J33G * vel * ud1v(V[IX(d, kk, i, j)], V[IX(d, kk + 1, i, j)], vel);
+  }
+}
+
+template <int S, bool TERRAIN, int C>
+__device__ double mlo_h(const Ops<S, TERRAIN, false>& o, const HeviArgs& a, int dir, int k, int i, int j) {
+  const Dims& d = a.d;
+  const auto& m = o.m;
+  if constexpr (C == 0) {  // fluxX_XYW_ud1 / fluxY_XYW_ud1 (UD1.F90:519-659)
+    if (k == d.KE) return 0.0;
+    const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+    const int c = dir == 0 ? I_UYZ : I_XVZ;
+    const double* M = dir == 0 ? a.MOMX : a.MOMY;
+    const double* V = a.MOMZ0;
+    const double f1 = o.F2H(k, 1, c, i, j), f2 = o.F2H(k, 2, c, i, j);
+    const double vel = (f1 * M[IX(d, k + 1, i, j)] + f2 * M[IX(d, k, i, j)])
+                     / (f1 * 0.5 * (o.D(k + 1, i, j) + o.D(k + 1, i + di, j + dj))
+                      + f2 * 0.5 * (o.D(k, i, j) + o.D(k, i + di, j + dj)));
+    const double mp = dir == 0 ? m.mapf(i, j, 1, I_UY) : m.mapf(i, j, 0, I_XV);
+    return m.gs(c, IX(d, k, i, j)) / mp * vel * ud1v(V[IX(d, k, i, j)], V[IX(d, k, i + di, j + dj)], vel);
+  } else {
+    const bool own = (C == 1) == (dir == 0);  // the flux along the component's own direction
+    const double* V = C == 1 ? a.MOMX0 : a.MOMY0;
+    if (own) {  // fluxX_UYZ_ud1 / fluxY_XVZ_ud1 at loop index (i,j)+1, stored one cell back (UD1.F90:869-920, 1237-1288)
+      const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+      const int ip = i + di, jp = j + dj;
+      const double* M = C == 1 ? a.MOMX : a.MOMY;
+      const double vel = (0.5 * (M[IX(d, k, ip, jp)] + M[IX(d, k, i, j)])) / (o.D(k, ip, jp));
+      const double mp = dir == 0 ? m.mapf(ip, jp, 1, I_UY) : m.mapf(ip, jp, 0, I_XY);
+      return m.gs(I_XYZ, IX(d, k, ip, jp)) / mp * vel * ud1v(V[IX(d, k, i, j)], V[IX(d, k, ip, jp)], vel);
+    }
+    // cross fluxes fluxY_UYZ_ud1 / fluxX_XVZ_ud1 (UD1.F90:924-973, 1184-1233)
+    const int si = C == 1 ? 1 : 0, sj = C == 1 ? 0 : 1;  // the component's staggering
+    const int ci = dir == 0 ? 1 : 0, cj = dir == 0 ? 0 : 1;
+    const double* M = dir == 0 ? a.MOMX : a.MOMY;
+    const double vel = (0.5 * (M[IX(d, k, i, j)] + M[IX(d, k, i + si, j + sj)]))
+                     / (0.25 * (o.D(k, i, j) + o.D(k, i + 1, j) + o.D(k, i, j + 1) + o.D(k, i + 1, j + 1)));
+    const double mp = C == 1 ? m.mapf(i, j, 0, I_XV) : m.mapf(i, j, 1, I_XY);
+    return m.gs(I_UVZ, IX(d, k, i, j)) / mp * vel * ud1v(V[IX(d, k, i, j)], V[IX(d, k, i + ci, j + cj)], vel);
+  }
+}
+
+// the high-order fluxes of the stage, the z flux in the form ATMOS_DYN_fct takes
+template <int S, bool TERRAIN, int C>
+__device__ double mhi_z(const Ops<S, TERRAIN, false>& o, int kk, int i, int j) {
+  const auto& m = o.m;
+  if constexpr (C == 0) {
+    double q = o.qZ_XYW(kk, i, j) / (m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY));
+    if constexpr (TERRAIN) q = q + o.qJ_XYW(0, kk, i, j) + o.qJ_XYW(1, kk, i, j);
+    return q;
+  } else {
+    constexpr int st = C == 1 ? 0 : 1;
+    const int mc = C == 1 ? I_UY : I_XV;
+    double q = o.qZ_UV(st, kk, i, j) / (m.mapf(i, j, 0, mc) * m.mapf(i, j, 1, mc));
+    if constexpr (TERRAIN) q = q + o.qJ_UV(st, 0, kk, i, j) + o.qJ_UV(st, 1, kk, i, j);
+    return q;
+  }
+}
+template <int S, bool TERRAIN, int C>
+__device__ double mhi_h(const Ops<S, TERRAIN, false>& o, int dir, int k, int i, int j) {
+  if constexpr (C == 0) return o.qH_XYW(dir, k, i, j);
+  else if constexpr (C == 1) return dir == 0 ? o.qX_UYZ(k, i, j) : o.qY_UYZ(k, i, j);
+  else return dir == 0 ? o.qX_XVZ(k, i, j) : o.qY_XVZ(k, i, j);
+}
+
+// lo: z on columns IS-1..IE+1 x JS-1..JE+1, levels KS-1..KE; x on faces
+// IS-2..IE+1 x JS-1..JE+1; y on IS-1..IE+1 x faces JS-2..JE+1 (levels KS..KE).
+// anti: z on the interior columns (0 at KS-1 and KE), x on faces IS-1..IE,
+// y on faces JS-1..JE.
+template <int S, bool TERRAIN, int C>
+__global__ void __launch_bounds__(256) mfct_flux(HeviArgs a, double* lo, double* anti) {
+  const Dims& d = a.d;
+  Ops<S, TERRAIN, false> o(a);
+  const int i = d.IS - 2 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int k = d.KS - 1 + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int j = d.JS - 2 + (int)blockIdx.z;
+  if (i > d.IE + 1 || k > d.KE || j > d.JE + 1) return;
+  const size_t p = IX(d, k, i, j), n3 = d.n3;
+  const bool in_i = i >= d.IS && i <= d.IE, in_j = j >= d.JS && j <= d.JE;
+  const bool ri = i >= d.IS - 1, rj = j >= d.JS - 1;
+  if (ri && rj) {
+    const double l = mlo_z<S, TERRAIN, C>(o, a, k, i, j);
+    lo[p] = l;
+    if (in_i && in_j) anti[p] = (k >= d.KS && k <= d.KE - 1) ? mhi_z<S, TERRAIN, C>(o, k, i, j) - l : 0.0;
+  }
+  if (k < d.KS) return;
+  if (rj) {
+    const double l = mlo_h<S, TERRAIN, C>(o, a, 0, k, i, j);
+    lo[p + n3] = l;
+    if (in_j && ri && i <= d.IE) anti[p + n3] = mhi_h<S, TERRAIN, C>(o, 0, k, i, j) - l;
+  }
+  if (ri) {
+    const double l = mlo_h<S, TERRAIN, C>(o, a, 1, k, i, j);
+    lo[p + 2 * n3] = l;
+    if (in_i && rj && j <= d.JE) anti[p + 2 * n3] = mhi_h<S, TERRAIN, C>(o, 1, k, i, j) - l;
+  }
+}
+
+template <int S, bool TERRAIN>
+static hipError_t heve_fct_momentum_t(const HeviArgs& a, const StageIO& io, hipStream_t s) {
+  const Ctx& c = *io.ctx;
+  const Dims& d = c.d;
+  const size_t n3 = d.n3;
+  double* W = c.mfct;
+  double *vel = W, *lo = W + n3, *anti = W + 4 * n3, *phi_lo = W + 7 * n3, *rp = W + 8 * n3, *rm = W + 9 * n3;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
+  const dim3 blk(64, 4);
+  const dim3 gi((imax + 63) / 64, (kmax + 3) / 4, jmax), gf((imax + 4 + 63) / 64, (kmax + 2 + 3) / 4, jmax + 4);
+  double* dk[1] = {io.o[0]};
+  hipError_t e = halo_fill(c, dk, 1, s);  // DENS_RK (HEVE.F90:746-747)
+  for (int C = 0; C < 3 && e == hipSuccess; ++C) {
+    const double* M0 = io.s0[1 + C];
+    prof_start(c, "mfct_flux", s);
+    if (C == 0) hipLaunchKernelGGL(mfct_vel<0>, gi, blk, 0, s, d, M0, io.s0[0], vel);
+    else if (C == 1) hipLaunchKernelGGL(mfct_vel<1>, gi, blk, 0, s, d, M0, io.s0[0], vel);
+    else hipLaunchKernelGGL(mfct_vel<2>, gi, blk, 0, s, d, M0, io.s0[0], vel);
+    if ((e = hipGetLastError()) != hipSuccess) break;
+    double* v[1] = {vel};
+    if ((e = halo_fill(c, v, 1, s)) != hipSuccess) break;
+    if (C == 0) hipLaunchKernelGGL((mfct_flux<S, TERRAIN, 0>), gf, blk, 0, s, a, lo, anti);
+    else if (C == 1) hipLaunchKernelGGL((mfct_flux<S, TERRAIN, 1>), gf, blk, 0, s, a, lo, anti);
+    else hipLaunchKernelGGL((mfct_flux<S, TERRAIN, 2>), gf, blk, 0, s, a, lo, anti);
+    prof_stop(c, s);
+    if ((e = hipGetLastError()) != hipSuccess) break;
+    MomFct f;
+    f.vel = vel; f.D0 = io.s0[0]; f.D1 = io.o[0];
+    f.dk = C == 0; f.di = C == 1; f.dj = C == 2;
+    f.lo = lo; f.anti = anti; f.mflx = io.mflx_hi;
+    f.RDZ = C == 0 ? c.g1.RFDZ : c.g1.RCDZ;
+    f.RDX = C == 1 ? c.g1.RFDX : c.g1.RCDX;
+    f.RDY = C == 2 ? c.g1.RFDY : c.g1.RCDY;
+    const int gc = C == 0 ? I_XYW : C == 1 ? I_UYZ : I_XVZ;
+    const int mc = C == 0 ? I_XY : C == 1 ? I_UY : I_XV;
+    f.G = c.GSQRT + n3 * (size_t)gc;
+    f.MAPF = c.MAPF + d.pl * 2 * (size_t)mc;
+    f.dt = io.dtrk; f.along = io.fct_along != 0;
+    f.phi_lo = phi_lo; f.rjpls = rp; f.rjmns = rm;
+    f.out = io.o[1 + C];
+    e = fct_momentum_limit(c, f, s);
+  }
+  return e;
+}
+
 hipError_t heve_stage(const StageIO& io, hipStream_t s) {
   HeviArgs a;
   fill_args(a, io);
   const Ctx& c = *io.ctx;
   if (io.combine) return hipErrorNotSupported;
+  if (io.fct_momentum && !c.mfct) return hipErrorInvalidValue;
   const bool div = io.divdmp_coef != 0.0;
   const bool ter = c.terrain;
   hipError_t e;
@@ -1399,6 +1591,10 @@ hipError_t heve_stage(const StageIO& io, hipStream_t s) {
   } else {
     if (ter) e = div ? launch_heve<CD4, true, true>(a, s) : launch_heve<CD4, true, false>(a, s);
     else e = div ? launch_heve<CD4, false, true>(a, s) : launch_heve<CD4, false, false>(a, s);
+  }
+  if (e == hipSuccess && io.fct_momentum) {
+    if (c.scheme == UD3) e = ter ? heve_fct_momentum_t<UD3, true>(a, io, s) : heve_fct_momentum_t<UD3, false>(a, io, s);
+    else e = ter ? heve_fct_momentum_t<CD4, true>(a, io, s) : heve_fct_momentum_t<CD4, false>(a, io, s);
   }
   if (e != hipSuccess || !io.fct_t) return e;
   return heve_fct_theta(io, s);

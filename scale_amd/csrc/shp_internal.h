@@ -8,6 +8,7 @@
 namespace shp {
 
 constexpr int NSCR = 8;  // scratch fields of KA*IA*JA doubles owned by the context
+constexpr int MFCT_FIELDS = 10;  // velocity, low-order and anti-diffusive fluxes (3 each), phi_lo, R+, R-
 
 struct Model;    // device-resident model state (driver.hip)
 struct AdjWork;  // adjoint work arrays and checkpoints (adjoint.hip)
@@ -33,6 +34,8 @@ struct Ctx {
   double ndq_coef = -1.0, ndq_dt = -1.0, ndq_sfc = 1.0;
   // scratch
   double* scr[NSCR] = {};
+  // work arrays of the HEVE momentum FCT (MFCT_FIELDS x n3), allocated on first use
+  double* mfct = nullptr;
   // staging buffer for host <-> device layout transposes
   double* stage = nullptr;
   size_t stage_elems = 0;
@@ -62,8 +65,8 @@ struct StageIOT {
   const double *REF_dens, *REF_rhot;
   double divdmp_coef, dtrk;
   int has_coriolis;
-  // HEVE only: FLAG_FCT_T and FLAG_FCT_ALONG_STREAM
-  int fct_t = 0, fct_along = 1;
+  // HEVE only: FLAG_FCT_MOMENTUM (needs Ctx::mfct), FLAG_FCT_T and FLAG_FCT_ALONG_STREAM
+  int fct_momentum = 0, fct_t = 0, fct_along = 1;
   // FP32 stage: DENS and RHOT stored as deviations from these level profiles
   const double *offD = nullptr, *offR = nullptr;
   // last RK4 stage only (flat terrain): combine with the outputs of stages
@@ -90,6 +93,19 @@ hipError_t hevi_stage(const StageIO& io, hipStream_t s);
 hipError_t heve_stage(const StageIO& io, hipStream_t s);
 // FCT of the HEVE theta flux, run by heve_stage after the stage kernel (tracer.hip)
 hipError_t heve_fct_theta(const StageIO& io, hipStream_t s);
+// ATMOS_DYN_fct of one HEVE momentum component on stored low-order and
+// anti-diffusive fluxes, then its MOM_RK update (tracer.hip)
+struct MomFct {
+  const double *vel, *D0, *D1;   // the velocity (with halos); DENS0, DENS_RK (with halos)
+  int dk, di, dj;                // the staggering: densities averaged with (k+dk, i+di, j+dj)
+  const double *lo, *anti, *mflx;  // 3 components each
+  const double *RDZ, *RDX, *RDY, *G, *MAPF;
+  double dt;
+  bool along;
+  double *phi_lo, *rjpls, *rjmns;  // scratch
+  double* out;                     // MOM_RK
+};
+hipError_t fct_momentum_limit(const Ctx& c, const MomFct& f, hipStream_t s);
 // mixed precision: flat terrain and divdmp_coef = 0 only
 hipError_t hevi_stage(const StageIOT<float>& io, hipStream_t s);
 // flat terrain, FP64, no divergence damping: the stage with PT, St and the

@@ -424,4 +424,216 @@ hipError_t heve_fct_theta(const StageIO& io, hipStream_t s) {
   return c.terrain ? tracer_t<CD4, true, true>(c, a, true, along, s) : tracer_t<CD4, false, true>(c, a, true, along, s);
 }
 
+// ---------------------------------------------------------------------------
+// ATMOS_DYN_fct on stored fluxes, for the HEVE momentum FCT
+// (HEVE.F90:776-810, 1006-1057, 1259-1309): the advected field is a velocity
+// at a staggered point, its densities the averages of DENS0 and DENS_RK with
+// the neighbour (k+dk, i+di, j+dj) (at KE the vertical average repeats KE-1,
+// HEVE.F90:757-773), and the low-order and anti-diffusive fluxes come from
+// arrays.  Same three passes as the tracer FCT.
+struct ArrFctArgs {
+  Dims d;
+  const double *phi, *D0, *D1;
+  int dk, di, dj;
+  const double *lo, *anti, *mflx;
+  const double *RDZ, *RDX, *RDY;
+  const double *G, *MAPF;  // GSQRT component and MAPF(:,:,:,c) of the staggered point
+  double dt;
+  double *phi_lo, *rjpls, *rjmns;
+  double* out;
+  int kmax_upd;  // the update covers KS..kmax_upd
+};
+
+template <bool TERRAIN>
+struct AfOps {
+  const ArrFctArgs& a;
+  const Dims& d;
+  __device__ AfOps(const ArrFctArgs& a_) : a(a_), d(a_.d) {}
+  __device__ __forceinline__ size_t ix(int k, int i, int j) const { return IX(d, k, i, j); }
+  __device__ __forceinline__ double gs(size_t p) const { if constexpr (TERRAIN) return a.G[p]; else return 1.0; }
+  __device__ __forceinline__ double m1(int i, int j) const {
+    if constexpr (TERRAIN) return a.MAPF[(size_t)i + (size_t)d.IA * j]; else return 1.0;
+  }
+  __device__ __forceinline__ double m2(int i, int j) const {
+    if constexpr (TERRAIN) return a.MAPF[(size_t)i + (size_t)d.IA * j + d.pl]; else return 1.0;
+  }
+  __device__ __forceinline__ double dav(const double* D, int k, int i, int j) const {
+    const int kk = (a.dk && k == d.KE) ? d.KE - 1 : k;
+    return 0.5 * (D[ix(kk, i, j)] + D[ix(kk + a.dk, i + a.di, j + a.dj)]);
+  }
+  __device__ __forceinline__ double lZ(size_t p) const { return a.lo[p]; }
+  __device__ __forceinline__ double lX(size_t p) const { return a.lo[p + d.n3]; }
+  __device__ __forceinline__ double lY(size_t p) const { return a.lo[p + 2 * d.n3]; }
+  __device__ __forceinline__ double aZ(size_t p) const { return a.anti[p]; }
+  __device__ __forceinline__ double aX(size_t p) const { return a.anti[p + d.n3]; }
+  __device__ __forceinline__ double aY(size_t p) const { return a.anti[p + 2 * d.n3]; }
+};
+
+template <bool TERRAIN>
+__global__ void __launch_bounds__(256) afct_philo(ArrFctArgs a) {
+  const Dims& d = a.d;
+  AfOps<TERRAIN> o(a);
+  const int i = d.IS - 1 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int k = d.KS + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int j = d.JS - 1 + (int)blockIdx.z;
+  if (i > d.IE + 1 || k > d.KE || j > d.JE + 1) return;
+  const size_t p = IX(d, k, i, j);
+  a.phi_lo[p] = (a.phi[p] * o.dav(a.D0, k, i, j)
+                 + a.dt * (-((o.lZ(p) - o.lZ(IX(d, k - 1, i, j))) * a.RDZ[k]
+                             + (o.lX(p) - o.lX(IX(d, k, i - 1, j))) * a.RDX[i]
+                             + (o.lY(p) - o.lY(IX(d, k, i, j - 1))) * a.RDY[j])
+                           * o.m1(i, j) * o.m2(i, j) / o.gs(p)))
+              / o.dav(a.D1, k, i, j);
+}
+
+template <bool TERRAIN, bool VECT>
+__global__ void __launch_bounds__(256) afct_ratio(ArrFctArgs a) {
+  const Dims& d = a.d;
+  AfOps<TERRAIN> o(a);
+  const int i = d.IS + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int k = d.KS + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int j = d.JS + (int)blockIdx.z;
+  if (i > d.IE || k > d.KE || j > d.JE) return;
+  const int KS = d.KS, KE = d.KE;
+  const size_t p = IX(d, k, i, j);
+  const double dt = a.dt;
+  const double az = o.aZ(p), azm = o.aZ(IX(d, k - 1, i, j));
+  const double ax = o.aX(p), axm = o.aX(IX(d, k, i - 1, j));
+  const double ay = o.aY(p), aym = o.aY(IX(d, k, i, j - 1));
+  const double g = o.gs(p);
+  const double pjpls = dt * ((fmax2(0.0, azm) - fmin2(0.0, az)) * a.RDZ[k]
+                           + (fmax2(0.0, axm) - fmin2(0.0, ax)) * a.RDX[i]
+                           + (fmax2(0.0, aym) - fmin2(0.0, ay)) * a.RDY[j])
+                     * o.m1(i, j) * o.m2(i, j) / g;
+  const double pjmns = dt * ((fmax2(0.0, az) - fmin2(0.0, azm)) * a.RDZ[k]
+                           + (fmax2(0.0, ax) - fmin2(0.0, axm)) * a.RDX[i]
+                           + (fmax2(0.0, ay) - fmin2(0.0, aym)) * a.RDY[j])
+                     * o.m1(i, j) * o.m2(i, j) / g;
+  const double* PI = a.phi;
+  const double* PL = a.phi_lo;
+  const double pin = PI[p], plo = PL[p];
+  double qmax, qmin;
+  if constexpr (VECT) {
+    const double* mZ = a.mflx; const double* mX = a.mflx + d.n3; const double* mY = a.mflx + 2 * d.n3;
+    double rw;
+    int ka, kb;
+    if (k == KS) { rw = (mZ[p]) * a.RDZ[KS]; ka = KS + 1; kb = KS; }
+    else if (k == KE) { rw = (mZ[IX(d, KE - 1, i, j)]) * a.RDZ[KE]; ka = KE; kb = KE - 1; }
+    else { rw = (mZ[p] + mZ[IX(d, k - 1, i, j)]) * a.RDZ[k]; ka = k + 1; kb = k - 1; }
+    const double ru = (mX[p] + mX[IX(d, k, i - 1, j)]) * a.RDX[i];
+    const double rv = (mY[p] + mY[IX(d, k, i, j - 1)]) * a.RDY[j];
+    double f[18];
+    get_fact(f, rw, ru, rv);
+    const double qa_in = fct_qa(f, PI, d, ka, k, i, j);
+    const double qb_in = fct_qb(f, PI, d, kb, k, i, j);
+    const double qa_lo = fct_qa(f, PL, d, ka, k, i, j);
+    const double qb_lo = fct_qb(f, PL, d, kb, k, i, j);
+    qmax = fmax2(fmax2(fmax2(fmax2(fmax2(pin, qa_in), qb_in), plo), qa_lo), qb_lo);
+    qmin = fmin2(fmin2(fmin2(fmin2(fmin2(pin, qa_in), qb_in), plo), qa_lo), qb_lo);
+  } else {
+    const int kp = k == KE ? k : k + 1, km = k == KS ? k : k - 1;
+    qmax = pin; qmin = pin;
+    const double* arrs[2] = {PI, PL};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const double* P = arrs[s];
+      const double v[7] = {P[p], P[IX(d, kp, i, j)], P[IX(d, km, i, j)], P[IX(d, k, i + 1, j)], P[IX(d, k, i - 1, j)],
+                           P[IX(d, k, i, j + 1)], P[IX(d, k, i, j - 1)]};
+#pragma unroll
+      for (int q = 0; q < 7; ++q) { qmax = fmax2(qmax, v[q]); qmin = fmin2(qmin, v[q]); }
+    }
+  }
+  const double D = o.dav(a.D1, k, i, j);
+  const double qjpls = (qmax - plo) * D;
+  const double qjmns = (plo - qmin) * D;
+  double zerosw = 0.5 - fsign(0.5, pjpls - EPS_RP);
+  a.rjpls[p] = fmin2(1.0, qjpls * (1.0 - zerosw) / (pjpls - zerosw));
+  zerosw = 0.5 - fsign(0.5, pjmns - EPS_RP);
+  a.rjmns[p] = fmin2(1.0, qjmns * (1.0 - zerosw) / (pjmns - zerosw));
+}
+
+// the limited anti-diffusive fluxes and the momentum update: WFORM is the
+// MOMZ form dt ((aZ) RFDZ + ((aX) RCDX + (aY) RCDY) M1 M2) / G (HEVE.F90:795-803),
+// otherwise dt ((aZ) RCDZ + (aX) RFDX + (aY) RCDY) M1 M2 / G (:1035-1042, 1284-1291)
+template <bool TERRAIN, bool WFORM>
+__global__ void __launch_bounds__(256) afct_update(ArrFctArgs a) {
+  const Dims& d = a.d;
+  AfOps<TERRAIN> o(a);
+  const int i = d.IS + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int k = d.KS + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int j = d.JS + (int)blockIdx.z;
+  if (i > d.IE || k > a.kmax_upd || j > d.JE) return;
+  const size_t p = IX(d, k, i, j);
+  const double* RP = a.rjpls;
+  const double* RM = a.rjmns;
+  auto limZ = [&](int kk) -> double {
+    if (kk == d.KS - 1 || kk == d.KE) return 0.0;
+    const double an = o.aZ(IX(d, kk, i, j));
+    const double dirsw = 0.5 + fsign(0.5, an);
+    return an * (1.0 - fmin2(RP[IX(d, kk + 1, i, j)], RM[IX(d, kk, i, j)]) * (dirsw)
+                     - fmin2(RP[IX(d, kk, i, j)], RM[IX(d, kk + 1, i, j)]) * (1.0 - dirsw));
+  };
+  auto limX = [&](int ii) -> double {
+    const double an = o.aX(IX(d, k, ii, j));
+    const double dirsw = 0.5 + fsign(0.5, an);
+    return an * (1.0 - fmin2(RP[IX(d, k, ii + 1, j)], RM[IX(d, k, ii, j)]) * (dirsw)
+                     - fmin2(RP[IX(d, k, ii, j)], RM[IX(d, k, ii + 1, j)]) * (1.0 - dirsw));
+  };
+  auto limY = [&](int jj) -> double {
+    const double an = o.aY(IX(d, k, i, jj));
+    const double dirsw = 0.5 + fsign(0.5, an);
+    return an * (1.0 - fmin2(RP[IX(d, k, i, jj + 1)], RM[IX(d, k, i, jj)]) * (dirsw)
+                     - fmin2(RP[IX(d, k, i, jj)], RM[IX(d, k, i, jj + 1)]) * (1.0 - dirsw));
+  };
+  if constexpr (WFORM) {
+    a.out[p] = a.out[p]
+             + a.dt * ((limZ(k) - limZ(k - 1)) * a.RDZ[k]
+                       + ((limX(i) - limX(i - 1)) * a.RDX[i] + (limY(j) - limY(j - 1)) * a.RDY[j])
+                         * o.m1(i, j) * o.m2(i, j))
+               / o.gs(p);
+  } else {
+    a.out[p] = a.out[p]
+             + a.dt * (((limZ(k) - limZ(k - 1)) * a.RDZ[k] + (limX(i) - limX(i - 1)) * a.RDX[i]
+                        + (limY(j) - limY(j - 1)) * a.RDY[j]))
+               * o.m1(i, j) * o.m2(i, j) / o.gs(p);
+  }
+}
+
+template <bool TERRAIN>
+static hipError_t afct_t(const Ctx& c, const ArrFctArgs& a, bool along, bool wform, hipStream_t s) {
+  const Dims& d = c.d;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
+  const dim3 blk(64, 4);
+  const dim3 g2((imax + 2 + 63) / 64, (kmax + 3) / 4, jmax + 2), g1((imax + 63) / 64, (kmax + 3) / 4, jmax);
+  prof_start(c, "mfct_limit", s);
+  hipLaunchKernelGGL(afct_philo<TERRAIN>, g2, blk, 0, s, a);
+  if (along) hipLaunchKernelGGL((afct_ratio<TERRAIN, true>), g1, blk, 0, s, a);
+  else hipLaunchKernelGGL((afct_ratio<TERRAIN, false>), g1, blk, 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  double* r[2] = {a.rjpls, a.rjmns};
+  e = halo_fill(c, r, 2, s);
+  if (e != hipSuccess) return e;
+  if (wform) hipLaunchKernelGGL((afct_update<TERRAIN, true>), g1, blk, 0, s, a);
+  else hipLaunchKernelGGL((afct_update<TERRAIN, false>), g1, blk, 0, s, a);
+  prof_stop(c, s);
+  return hipGetLastError();
+}
+
+hipError_t fct_momentum_limit(const Ctx& c, const MomFct& f, hipStream_t s) {
+  ArrFctArgs a;
+  a.d = c.d;
+  a.phi = f.vel; a.D0 = f.D0; a.D1 = f.D1;
+  a.dk = f.dk; a.di = f.di; a.dj = f.dj;
+  a.lo = f.lo; a.anti = f.anti; a.mflx = f.mflx;
+  a.RDZ = f.RDZ; a.RDX = f.RDX; a.RDY = f.RDY;
+  a.G = f.G; a.MAPF = f.MAPF;
+  a.dt = f.dt;
+  a.phi_lo = f.phi_lo; a.rjpls = f.rjpls; a.rjmns = f.rjmns;
+  a.out = f.out;
+  a.kmax_upd = f.dk ? c.d.KE - 1 : c.d.KE;
+  const bool wform = f.dk != 0;
+  return c.terrain ? afct_t<true>(c, a, f.along, wform, s) : afct_t<false>(c, a, f.along, wform, s);
+}
+
 }  // namespace shp

@@ -194,7 +194,7 @@ contains
   end subroutine ATMOS_DYN_Tstep_short_fvm_hevi_hip
 
   !> explicit HEVE stage with the dummies of the abstract interface `short`
-  !! (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1516); FLAG_FCT_MOMENTUM must be .false.
+  !! (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1516), with its momentum and theta FCT
   subroutine ATMOS_DYN_Tstep_short_fvm_heve_hip( &
        DENS_RK, MOMZ_RK, MOMX_RK, MOMY_RK, RHOT_RK, &
        PROG_RK,                                     &

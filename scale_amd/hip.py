@@ -160,7 +160,7 @@ class Context:
                                        PHI, GSQRT, J13G, J23G, J33G, MAPF, REF_dens, REF_rhot,
                                        BND_W, BND_E, BND_S, BND_N, dtrk, last):
         """One explicit HEVE stage (scale_atmos_dyn_tstep_short_fvm_heve.F90:109-1516);
-        FLAG_FCT_T is applied, FLAG_FCT_MOMENTUM must be false."""
+        FLAG_FCT_MOMENTUM and FLAG_FCT_T are applied as in the reference."""
         L = load_library()
         _check(L.scale_hip_tstep_short_heve(
             self._h, _p(DENS_RK), _p(MOMZ_RK), _p(MOMX_RK), _p(MOMY_RK), _p(RHOT_RK), _p(PROG_RK), _p(mflx_hi), _p(tflx_hi),

@@ -141,12 +141,65 @@ def test_heve_stage_fct_theta_bitwise(hip, terrain, along):
         H.compare(ref[k], got[k], k)
 
 
-def test_heve_rejects_fct_momentum(hip):
-    print("[heve stage] FLAG_FCT_MOMENTUM rejected", flush=True)
-    case = H.small_case(kmax=8, imax=6, jmax=6)
-    inp = H.stage_inputs(case)
-    with pytest.raises(hip.ScaleHipError, match="FCT"):
-        run_stage_gpu(hip, case, inp, 0.5, heve=True, fct=(True, False))
+def _momentum_tophat_inputs(case, seed, coriolis):
+    """Stage inputs with boxes of stronger wind in MOMX, MOMY and MOMZ (state
+    and state0), so that the momentum limiter is active."""
+    g = case.grid
+    inp = H.stage_inputs(case, seed=seed, coriolis=coriolis)
+    boxes = {"MOMX": (slice(g.KS + 3, g.KS + 9), slice(g.IS + 2, g.IS + 6), slice(g.JS + 2, g.JS + 6), 1.5),
+             "MOMY": (slice(g.KS + 4, g.KS + 10), slice(g.IS + 5, g.IS + 9), slice(g.JS + 3, g.JS + 7), 1.4),
+             "MOMZ": (slice(g.KS + 5, g.KS + 11), slice(g.IS + 3, g.IS + 8), slice(g.JS + 1, g.JS + 5), None)}
+    for v, (sk, si, sj, f) in boxes.items():
+        for key in (v + "0", v):
+            a = np.array(inp[key], order="F", copy=True)
+            if f is None:
+                a[sk, si, sj] += 2.0 * inp["DENS0"][sk, si, sj]
+            else:
+                a[sk, si, sj] *= f
+            inputs.halo_fill(g, a)
+            inp[key] = a
+    return inp
+
+
+@pytest.mark.parametrize("along", [True, False], ids=["along_stream", "7point"])
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+@pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
+def test_heve_stage_fct_momentum_bitwise(hip, scheme, terrain, along):
+    """HEVE with FLAG_FCT_MOMENTUM (HEVE.F90:386-450, 716-810, 958-1057,
+    1207-1309) on boxes of stronger wind: the limited MOMZ_RK, MOMX_RK and
+    MOMY_RK and every other output are bit-identical to the oracle."""
+    print(f"[heve stage] FCT momentum scheme={scheme} terrain={terrain} along={along}", flush=True)
+    case = H.small_case(seed=43, u0=15.0, v0=-8.0)
+    inp = _momentum_tophat_inputs(case, 4, terrain)
+    metric = H.terrain_metrics(case, amp=0.02) if terrain else None
+    ref = oracle_stage(case, inp, 2.0, metric, scheme, heve=True, fct_momentum=True, along_stream=along)
+    plain = oracle_stage(case, inp, 2.0, metric, scheme, heve=True)
+    g = case.grid
+    sl = (slice(g.KS, g.KE), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    for v in ("MOMZ_RK", "MOMX_RK", "MOMY_RK"):  # the limiter acts
+        assert np.abs(ref[v][sl] - plain[v][sl]).max() > 1e-6 * np.abs(plain[v][sl]).max(), v
+    got = run_stage_gpu(hip, case, inp, 2.0, metric, scheme, heve=True, fct=(True, False), along=along)
+    for k in OUTS:
+        H.compare(ref[k], got[k], k)
+
+
+@pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
+def test_heve_stage_fct_momentum_and_theta_bitwise(hip, terrain):
+    """Both limiters of the HEVE stage in one call, bit-identical to the oracle."""
+    print(f"[heve stage] FCT momentum + theta terrain={terrain}", flush=True)
+    case = H.small_case(seed=44, u0=15.0, v0=-8.0)
+    g = case.grid
+    inp = _momentum_tophat_inputs(case, 5, terrain)
+    for key in ("RHOT0", "RHOT"):
+        a = np.array(inp[key], order="F", copy=True)
+        a[g.KS + 3:g.KS + 9, g.IS + 2:g.IS + 7, g.JS + 2:g.JS + 6] *= 1.1
+        inputs.halo_fill(g, a)
+        inp[key] = a
+    metric = H.terrain_metrics(case, amp=0.02) if terrain else None
+    ref = oracle_stage(case, inp, 2.0, metric, 3, heve=True, fct_momentum=True, fct_t=True)
+    got = run_stage_gpu(hip, case, inp, 2.0, metric, 3, heve=True, fct=(True, True))
+    for k in OUTS:
+        H.compare(ref[k], got[k], k)
 
 
 def test_numfilter_bitwise(hip):
