@@ -222,8 +222,11 @@ int scale_hip_stream(scale_hip_ctx *ctx, void **stream);
  * stream (the counterpart of PROF_rapstart/PROF_rapend,
  * scalelib/src/common/scale_prof.F90:156-253).  Names: hevi_stage,
  * numfilter, numfilter_q, halo_fill, rk4_combine, tracer_nofct, fct_philo, fct_ratio,
- * fct_update, hbm_copy, model_forward_checkpointed, model_adjoint.  Enabling
- * resets the accumulators. */
+ * fct_update, mfct_flux, mfct_limit, hbm_copy, model_large_step,
+ * model_forward_checkpointed, model_adjoint.  on = 1 times every launch;
+ * on = 2 only the outermost timer of nested ones (model_large_step,
+ * model_forward_checkpointed, model_adjoint: two events per call, nothing
+ * recorded between the kernels inside).  Enabling resets the accumulators. */
 int scale_hip_profile_enable(scale_hip_ctx *ctx, int32_t on);
 int scale_hip_profile_query(scale_hip_ctx *ctx, const char *name, int64_t *count, double *total_ms);
 

@@ -127,18 +127,11 @@ enum {
   NW
 };
 
-// What the reverse of one large step reads from its forward: the stage
-// outputs R1..R3, num_diff, RT2P/DPRES0/REF_rhot, and the PT, St and C of
-// the four stages.
-struct StepRec {
-  double* R[3][5] = {};
-  double* nd = nullptr;
-  double *RT2P = nullptr, *DPRES0 = nullptr, *REF_rhot = nullptr;
-  double* tr[4][3] = {};
-};
-
 struct AdjWork {
   int nsteps = 0, every = 0, recorded = 0;
+  // the checkpointed forward ran its last segment with records: the reverse
+  // sweep starts on it without a recording forward
+  bool last_recorded = false;
   // one record per step of a segment when the device has room for them
   // (45 fields per step); otherwise each reversed step recomputes its own
   std::vector<StepRec> recs;
@@ -1136,7 +1129,9 @@ static hipError_t check_adjoint_model(const Ctx& c) {
   return hipSuccess;
 }
 
-// forward run of nsteps large steps that keeps the state every 'every' steps
+// forward run of nsteps large steps that keeps the state every 'every' steps;
+// with room for the segment records, the last segment also keeps its step
+// states and records, as the reverse sweep's recording forward would
 hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every) {
   HIPCHK(check_adjoint_model(c));
   HIPCHK(adj_setup(c, nsteps, every));
@@ -1144,15 +1139,26 @@ hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every) {
   Model& M = *c.model;
   const size_t bytes = c.d.n3 * sizeof(double);
   double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
+  const bool recs = (int)A.recs.size() == A.every;
+  const int nlast = ((nsteps - 1) / every) * every;  // first step of the last segment
+  A.recorded = 0;
+  A.last_recorded = false;
   prof_start(c, "model_forward_checkpointed", c.stream);
   for (int n = 0; n < nsteps; ++n) {
     if (n % every == 0)
       for (int v = 0; v < 5; ++v)
         HIPCHK(hipMemcpyAsync(A.ckpt[5 * (n / every) + v], st[v], bytes, hipMemcpyDeviceToDevice, c.stream));
-    HIPCHK(model_large_step(c));
+    if (recs && n >= nlast) {
+      for (int v = 0; v < 5; ++v)
+        HIPCHK(hipMemcpyAsync(A.seg[5 * (n - nlast) + v], st[v], bytes, hipMemcpyDeviceToDevice, c.stream));
+      HIPCHK(model_large_step(c, &A.recs[n - nlast]));
+    } else {
+      HIPCHK(model_large_step(c));
+    }
   }
   prof_stop(c, c.stream);
   A.recorded = 1;
+  A.last_recorded = recs;
   return hipSuccess;
 }
 
@@ -1179,8 +1185,10 @@ hipError_t model_adjoint(Ctx& c, double* const xb[5]) {
     for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[v], A.ckpt[5 * sg + v], bytes, hipMemcpyDeviceToDevice, s));
     auto seg = [&](int n, double** x) { for (int v = 0; v < 5; ++v) x[v] = A.seg[5 * (n - n0) + v]; };
     if (recs) {
-      // one recording forward over the segment, then the reverse steps
-      for (int n = n0; n < n1; ++n) {
+      // one recording forward over the segment (done by the checkpointed
+      // forward for the last one), then the reverse steps
+      const bool have = A.last_recorded && sg == nseg - 1;
+      for (int n = n0; n < n1 && !have; ++n) {
         double *x[5], *xn[5];
         seg(n, x);
         if (n + 1 < n1) seg(n + 1, xn);

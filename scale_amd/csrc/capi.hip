@@ -813,11 +813,13 @@ int scale_hip_model_device_ptr(scale_hip_ctx* h, int32_t f, double** dptr) {
 int scale_hip_model_large_step(scale_hip_ctx* h, int32_t n) {
   if (int r = check_ctx(h)) return r;
   if (!h->c.model) return fail("model not created");
+  shp::prof_start(h->c, "model_large_step", h->c.stream);
   for (int s = 0; s < n; ++s) {
     CK(shp::model_state_to_f64(h->c));  // the preparation reads the FP64 state
     CK(shp::model_large_step(h->c));
   }
   CK(shp::model_state_to_f64(h->c));
+  shp::prof_stop(h->c, h->c.stream);
   return 0;
 }
 
@@ -894,6 +896,7 @@ int scale_hip_profile_enable(scale_hip_ctx* h, int32_t on) {
     if (!c.prof) c.prof = shp::prof_create();
     shp::prof_collect(c);
     shp::prof_reset(c.prof);
+    shp::prof_outer_only(c.prof, on == 2);
   } else if (c.prof) {
     shp::prof_collect(c);
     shp::prof_destroy(c.prof);

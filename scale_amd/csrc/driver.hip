@@ -409,19 +409,23 @@ struct ShortBufs {
 };
 
 template <typename R>
-static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B) {
+static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B, const StepRec* rec = nullptr) {
   Model& M = *c.model;
   const Dims& d = c.d;
   hipStream_t s = c.stream;
+  // rec (FP64 flat HEVI): num_diff, the stage outputs R1..R3 and the stage
+  // traces go to the record itself, which the reverse sweep reads
+  R* const num_diff = rec ? (R*)rec->nd : B.num_diff;
+  auto Rk = [&](int r, int v) -> R* { return (rec && r < 3) ? (R*)rec->R[r][v] : B.Rk[r][v]; };
   // numerical filter on the state at the start of the short step (LARGE.F90:845-858)
   if (M.nd_coef == 0.0) {
-    HIPCHK(hipMemsetAsync(B.num_diff, 0, 15 * d.n3 * sizeof(R), s));
+    HIPCHK(hipMemsetAsync(num_diff, 0, 15 * d.n3 * sizeof(R), s));
   } else {
     const R* cst[5] = {B.st[0], B.st[1], B.st[2], B.st[3], B.st[4]};
     if constexpr (sizeof(R) == sizeof(double)) {
-      HIPCHK(numfilter(c, B.num_diff, cst, M.REF_dens, M.REF_pott, s));
+      HIPCHK(numfilter(c, num_diff, cst, M.REF_dens, M.REF_pott, s));
     } else {
-      HIPCHK(numfilter(c, B.num_diff, cst, M.REF_dens, M.REF_pott, s, M.foff, M.foff + d.KA));
+      HIPCHK(numfilter(c, num_diff, cst, M.REF_dens, M.REF_pott, s, M.foff, M.foff + d.KA));
     }
   }
   // divergence for the divergence damping (LARGE.F90:860-872)
@@ -435,34 +439,38 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B) {
     StageIOT<R> io;
     io.ctx = &c;
     for (int v = 0; v < 5; ++v) {
-      io.o[v] = B.Rk[r][v];
+      io.o[v] = Rk(r, v);
       io.s0[v] = B.st[v];
-      io.s[v] = r == 0 ? B.st[v] : B.Rk[r - 1][v];
+      io.s[v] = r == 0 ? B.st[v] : Rk(r - 1, v);
       io.t[v] = B.t[v];
     }
     io.mflx_hi = B.mfR[r];
     io.tflx_hi = B.tfR[r];
-    io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = B.num_diff;
+    io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = num_diff;
     io.wdamp_coef = M.wdamp_coef; io.DDIV = B.DDIV; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
     io.divdmp_coef = M.divdmp_coef; io.dtrk = dtrk[r]; io.has_coriolis = M.has_coriolis;
     // flat-terrain HEVI: the last stage combines the four stages into the state itself
     io.combine = (r == 3 && !c.terrain && !M.heve) ? 1 : 0;
-    for (int v = 0; v < 5; ++v) { io.c1[v] = B.Rk[0][v]; io.c2[v] = B.Rk[1][v]; io.c3[v] = B.Rk[2][v]; io.st[v] = B.st[v]; }
+    for (int v = 0; v < 5; ++v) { io.c1[v] = Rk(0, v); io.c2[v] = Rk(1, v); io.c3[v] = Rk(2, v); io.st[v] = B.st[v]; }
     for (int n = 0; n < 3; ++n) { io.cm[n] = B.mfR[n]; io.ct[n] = B.tfR[n]; }
     io.mflx_av = B.mflx; io.tflx_av = B.tflx;
     if constexpr (sizeof(R) == sizeof(float)) { io.offD = M.foff; io.offR = M.foff + d.KA; }
+    if (rec) { io.trPT = rec->tr[r][0]; io.trSt = rec->tr[r][1]; io.trCp = rec->tr[r][2]; }
     if constexpr (sizeof(R) == sizeof(double)) {
       HIPCHK(M.heve ? heve_stage(io, s) : hevi_stage(io, s));
     } else {
       HIPCHK(hevi_stage(io, s));
     }
-    if (r < 3) HIPCHK(halo_fill(c, B.Rk[r], 5, s));
+    if (r < 3) {
+      R* const rr[5] = {Rk(r, 0), Rk(r, 1), Rk(r, 2), Rk(r, 3), Rk(r, 4)};
+      HIPCHK(halo_fill(c, rr, 5, s));
+    }
   }
   if (!c.terrain && !M.heve) return halo_fill(c, B.st, 5, s);
   CombArgs<R> a;
   a.d = d;
   for (int v = 0; v < 5; ++v) {
-    a.st[v] = B.st[v]; a.r1[v] = B.Rk[0][v]; a.r2[v] = B.Rk[1][v]; a.r3[v] = B.Rk[2][v]; a.r4[v] = B.Rk[3][v];
+    a.st[v] = B.st[v]; a.r1[v] = Rk(0, v); a.r2[v] = Rk(1, v); a.r3[v] = Rk(2, v); a.r4[v] = B.Rk[3][v];
     a.s0[v] = B.st[v];
   }
   a.mflx = B.mflx; a.tflx = B.tflx;
@@ -493,8 +501,9 @@ hipError_t rk4_combine_state(Ctx& c, double* const out[5], const double* const s
   return halo_fill(c, out, 5, s);
 }
 
-hipError_t model_short_step_rk4(Ctx& c, double dts) {
+hipError_t model_short_step_rk4(Ctx& c, double dts, const StepRec* rec) {
   Model& M = *c.model;
+  if (rec && (M.mixed || M.heve || c.terrain || M.divdmp_coef != 0.0)) return hipErrorNotSupported;
   if (M.mixed) {
     ShortBufs<float> B{{M.fS[0], M.fS[1], M.fS[2], M.fS[3], M.fS[4]}, M.fR, M.ft, M.fnum_diff, M.fmfR, M.ftfR, M.fmflx,
                        M.ftflx, M.fDDIV};
@@ -502,7 +511,7 @@ hipError_t model_short_step_rk4(Ctx& c, double dts) {
   }
   ShortBufs<double> B{{M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT}, M.R, M.t, M.num_diff, M.mfR, M.tfR, M.mflx_hi,
                       M.tflx_hi, M.DDIV};
-  return short_step_t(c, dts, B);
+  return short_step_t(c, dts, B, rec);
 }
 
 // mixed precision: the FP32 state is the integrated one; these keep the FP64
@@ -551,7 +560,7 @@ hipError_t model_prepare(Ctx& c) {
   return hipSuccess;
 }
 
-hipError_t model_large_step(Ctx& c) {
+hipError_t model_large_step(Ctx& c, const StepRec* rec) {
   Model& M = *c.model;
   const Dims& d = c.d;
   hipStream_t s = c.stream;
@@ -559,9 +568,15 @@ hipError_t model_large_step(Ctx& c) {
   const double dtl = M.dtl;
   const int nstep = (int)ceil((dtl - EPS_RP) / M.dts);
   const double dts = dtl / nstep;
+  if (rec && nstep != 1) return hipErrorNotSupported;
   HIPCHK(model_prepare(c));
+  if (rec) {
+    HIPCHK(hipMemcpyAsync(rec->RT2P, M.RT2P, bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(rec->DPRES0, M.DPRES0, bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(rec->REF_rhot, M.REF_rhot, bytes, hipMemcpyDeviceToDevice, s));
+  }
   for (int step = 0; step < nstep; ++step) {
-    HIPCHK(model_short_step_rk4(c, dts));
+    HIPCHK(model_short_step_rk4(c, dts, rec));
     for (int n = 0; n < 3; ++n) {
       hipLaunchKernelGGL(axpy_interior, grid_int(d), dim3(64, 4), 0, s, d, M.mflx_av + d.n3 * n, M.mflx_hi + d.n3 * n, 1.0, 0);
     }

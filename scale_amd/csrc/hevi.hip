@@ -1204,7 +1204,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       }
       a.DENS_RK[g] = Sr;  // Sr/St wait in the outputs until the back pass
       a.RHOT_RK[g] = St;
-      if constexpr (!CMB && sizeof(R) == sizeof(double)) {
+      if constexpr (sizeof(R) == sizeof(double)) {
         if (a.trSt) a.trSt[g] = St;
       }
       if (k <= KE - 1) { a.mflx_hi[g] = mZ; a.tflx_hi[g] = tZ; a.PT[g] = (double)PT; }
@@ -1280,7 +1280,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
     const R z0 = a.MOMZ0[q];
     put_zflux(q, R((double)mfZ[q] + J33G * Cn), R((double)tfZ[q] + J33G * Cn * ptn));
     put_state(1, q, R((double)z0 + (Cn - (double)a.MOMZ[q])), z0, KE - 1);
-    if constexpr (!CMB && sizeof(R) == sizeof(double)) {
+    if constexpr (sizeof(R) == sizeof(double)) {
       if (a.trCp) a.trCp[q] = Cn;
     }
   }
@@ -1291,7 +1291,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
     const R z0 = a.MOMZ0[q];
     put_zflux(q, R((double)mfZ[q] + J33G * Ck), R((double)tfZ[q] + J33G * Ck * ptk));
     put_state(1, q, R((double)z0 + (Ck - (double)a.MOMZ[q])), z0, k);
-    if constexpr (!CMB && sizeof(R) == sizeof(double)) {
+    if constexpr (sizeof(R) == sizeof(double)) {
       if (a.trCp) a.trCp[q] = Ck;
     }
     const R d0 = a.DENS0[q1], r0 = a.RHOT0[q1];
@@ -1356,6 +1356,7 @@ static void fill_args(HeviArgsT<R>& a, const StageIOT<R>& io) {
   for (int n = 0; n < 3; ++n) { a.cm[n] = io.cm[n]; a.ct[n] = io.ct[n]; }
   a.mflx_av = io.mflx_av; a.tflx_av = io.tflx_av;
   a.offD = io.offD; a.offR = io.offR;
+  if (io.trSt) { a.PT = io.trPT; a.trSt = io.trSt; a.trCp = io.trCp; }
 }
 
 hipError_t hevi_stage(const StageIO& io, hipStream_t s) {

@@ -47,9 +47,11 @@ struct Model {
   std::vector<void*> allocs;
 };
 
-hipError_t model_short_step_rk4(Ctx& c, double dts);
+// rec (FP64 flat HEVI without divergence damping, one short step per large
+// step): the step also keeps in rec what the reverse sweep reads
+hipError_t model_short_step_rk4(Ctx& c, double dts, const StepRec* rec = nullptr);
 hipError_t model_prepare(Ctx& c);
-hipError_t model_large_step(Ctx& c);
+hipError_t model_large_step(Ctx& c, const StepRec* rec = nullptr);
 hipError_t mass_sum(const Ctx& c, const double* a, const double* b, double* out_host);
 hipError_t model_state_to_f64(Ctx& c);
 hipError_t model_state_to_f32(Ctx& c);

@@ -15,6 +15,10 @@ struct Prof {
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<long, double>> acc;  // name -> (count, total ms)
   std::vector<std::pair<std::string, hipEvent_t>> open;  // timers started and not yet stopped (nested)
+  // outermost only: timers nested inside a running one record nothing, so a
+  // region is timed by two events whatever it launches
+  bool outer_only = false;
+  int depth = 0;
   hipEvent_t get() {
     if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
     hipEvent_t e; hipEventCreate(&e); return e;
@@ -24,14 +28,17 @@ struct Prof {
 void prof_start(const Ctx& c, const char* name, hipStream_t s) {
   if (!c.prof) return;
   Prof& p = *c.prof;
+  if (p.outer_only && p.depth++ > 0) return;
   hipEvent_t e = p.get();
   hipEventRecord(e, s);
   p.open.emplace_back(name, e);
 }
 
 void prof_stop(const Ctx& c, hipStream_t s) {
-  if (!c.prof || c.prof->open.empty()) return;
+  if (!c.prof) return;
   Prof& p = *c.prof;
+  if (p.outer_only && (p.depth == 0 || --p.depth > 0)) return;
+  if (p.open.empty()) return;
   hipEvent_t b = p.get();
   hipEventRecord(b, s);
   p.pending.push_back({p.open.back().first, p.open.back().second, b});
@@ -66,6 +73,11 @@ void prof_destroy(Prof* p) {
 void prof_reset(Prof* p) {
   if (!p) return;
   p->acc.clear();
+}
+void prof_outer_only(Prof* p, bool on) {
+  if (!p) return;
+  p->outer_only = on;
+  p->depth = 0;
 }
 int prof_query(Prof* p, const char* name, long* count, double* total_ms) {
   if (!p) return -1;

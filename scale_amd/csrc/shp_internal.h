@@ -69,6 +69,9 @@ struct StageIOT {
   int fct_momentum = 0, fct_t = 0, fct_along = 1;
   // FP32 stage: DENS and RHOT stored as deviations from these level profiles
   const double *offD = nullptr, *offR = nullptr;
+  // FP64 flat-terrain HEVI only: PT kept in trPT (instead of the context's
+  // scratch), St and the solved C recorded in trSt, trCp, for the adjoint
+  double *trPT = nullptr, *trSt = nullptr, *trCp = nullptr;
   // last RK4 stage only (flat terrain): combine with the outputs of stages
   // 1-3 on the fly, writing the new state into st (which may alias s0) and the
   // averaged fluxes into mflx_av/tflx_av, instead of o / the Z-free parts of
@@ -80,6 +83,16 @@ struct StageIOT {
   R *mflx_av, *tflx_av;
 };
 using StageIO = StageIOT<double>;
+
+// What the reverse of one large step reads from its forward: the stage
+// outputs R1..R3, num_diff, RT2P/DPRES0/REF_rhot, and the PT, St and C of
+// the four stages.
+struct StepRec {
+  double* R[3][5] = {};
+  double* nd = nullptr;
+  double *RT2P = nullptr, *DPRES0 = nullptr, *REF_rhot = nullptr;
+  double* tr[4][3] = {};
+};
 
 struct TracerIO {
   double* QTRCo;
@@ -119,6 +132,7 @@ void prof_collect(const Ctx& c);
 Prof* prof_create();
 void prof_destroy(Prof* p);
 void prof_reset(Prof* p);
+void prof_outer_only(Prof* p, bool on);  // time only the outermost of nested timers
 int prof_query(Prof* p, const char* name, long* count, double* total_ms);
 int prof_names(Prof* p, char* buf, int len);
 // streaming copy of n doubles (timed as "hbm_copy")

@@ -274,6 +274,8 @@ class Context:
         return out.value
 
     def profile(self, on=True):
+        """on: True/1 times every launch, 2 only the outermost of nested timers
+        (whole calls such as model_large_step), False stops profiling."""
         _check(load_library().scale_hip_profile_enable(self._h, C.c_int32(int(on))))
 
     def profile_query(self, name):
