@@ -818,9 +818,30 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
   constexpr bool DEV = sizeof(R) == sizeof(float);
   // the level profiles staged in LDS (read every level by every thread)
   __shared__ double sOff[DEV ? 2 * KA_DEV_MAX : 1];
+  // FP32: the 1-D z arrays in LDS too, so that the level's arithmetic issues
+  // no global load behind the next level's prefetch (sZ rows: CDZ, RCDZ,
+  // RFDZ, wdamp_coef)
+  __shared__ double sZ[DEV ? 4 * KA_DEV_MAX : 1];
   if constexpr (DEV) {
-    for (int q = tid; q < d.KA; q += NT) { sOff[q] = a.offD[q]; sOff[KA_DEV_MAX + q] = a.offR[q]; }
+    for (int q = tid; q < d.KA; q += NT) {
+      sOff[q] = a.offD[q]; sOff[KA_DEV_MAX + q] = a.offR[q];
+      sZ[q] = CDZ[q]; sZ[KA_DEV_MAX + q] = RCDZ[q]; sZ[3 * KA_DEV_MAX + q] = a.wdamp_coef[q];
+      if (q < d.KA - 1) sZ[2 * KA_DEV_MAX + q] = RFDZ[q];
+    }
     __syncthreads();
+  }
+  auto zCDZ = [&](int k) -> double { if constexpr (DEV) return sZ[k]; else return CDZ[k]; };
+  auto zRCDZ = [&](int k) -> double { if constexpr (DEV) return sZ[KA_DEV_MAX + k]; else return RCDZ[k]; };
+  auto zRFDZ = [&](int k) -> double { if constexpr (DEV) return sZ[2 * KA_DEV_MAX + k]; else return RFDZ[k]; };
+  auto zWD = [&](int k) -> double { if constexpr (DEV) return sZ[3 * KA_DEV_MAX + k]; else return a.wdamp_coef[k]; };
+  // the own column's x/y metrics and Coriolis factors, fixed over k
+  const bool ownc = (i <= d.IE) && (j <= d.JE);
+  const double hRCDX = ownc ? RCDX[i] : 0.0, hRCDY = ownc ? RCDY[j] : 0.0;
+  const double hRFDX = ownc ? RFDX[i] : 0.0, hRFDY = ownc ? RFDY[j] : 0.0;
+  double corx = 0.0, cory = 0.0;
+  if (ownc && a.has_coriolis) {
+    corx = 0.125 * (a.CORIOLI[(size_t)(i + 1) + (size_t)d.IA * j] + a.CORIOLI[(size_t)i + (size_t)d.IA * j]);
+    cory = -0.125 * (a.CORIOLI[(size_t)i + (size_t)d.IA * (j + 1)] + a.CORIOLI[(size_t)i + (size_t)d.IA * j]);
   }
   auto fullDd = [&](R v, int k) -> double { if constexpr (DEV) return (double)v + sOff[k]; else return (double)v; };
   auto fullTd = [&](R v, int k) -> double { if constexpr (DEV) return (double)v + sOff[KA_DEV_MAX + k]; else return (double)v; };
@@ -857,15 +878,23 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
   double rt0 = 0.0, dp0 = 0.0, st0 = 0.0, sr0 = 0.0, d0 = 0.0, rd0 = 0.0, A0 = 0.0, pt0 = 0.0, ptm = 0.0, sw0 = 0.0;
   double e_prev = 0.0, f_prev = 0.0, C_last = 0.0;
 
-  for (int k = KS; k <= KE; ++k) {
-    const int s1 = s0 ^ 1;
-    // every other global load of the level is issued here, into registers,
-    // before any LDS write, so that the whole level waits on memory once:
-    // num_diff of the faces this thread evaluates, the staged tiles, and the
-    // DPRES inputs
-    constexpr int NX = (TJ * (TI + 1) + NT - 1) / NT, NY = ((TJ + 1) * TI + NT - 1) / NT;
-    constexpr int NC = (RW * CW + NT - 1) / NT, NP = ((TJ + 1) * (TI + 1) + NT - 1) / NT;
-    R fx[NX][5], fy[NY][5];
+  // Every other global load of level k goes into these registers before any
+  // LDS write of the level, so that the level waits on memory once: num_diff
+  // of the faces this thread evaluates, the staged tiles and the DPRES
+  // inputs.  PIPE (FP32): the loads of level k+1 are issued right after the
+  // staging of level k, behind the column's own loads, and stay in flight
+  // through the face and column work of level k.
+  constexpr bool PIPE = DEV;
+  constexpr int NX = (TJ * (TI + 1) + NT - 1) / NT, NY = ((TJ + 1) * TI + NT - 1) / NT;
+  constexpr int NC = (RW * CW + NT - 1) / NT, NP = ((TJ + 1) * (TI + 1) + NT - 1) / NT;
+  R fx[NX][5], fy[NY][5];
+  R tD[NC], tX[NC], tY[NC], tW[NC], tR[NC];
+  double pD0[NP], pRT[NP], pRR[NP];
+  R pR[NP];
+  // PIPE: loads at invalid positions go to element 0 and are discarded by a
+  // select, so that every load is issued unconditionally, with no branch
+  // around it that would hide from the wait counters how many are in flight.
+  auto issue = [&](int k) {
 #pragma unroll
     for (int h = 0; h < NX; ++h) {
       const int q = tid + h * NT;
@@ -873,11 +902,21 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       const int ii = i0 - 1 + x, jj = j0 + row;
       const bool ok = q < TJ * (TI + 1) && ii <= d.IE && jj <= d.JE;
       const size_t g = ok ? gix(k, ii, jj) : 0;
-      fx[h][0] = ok ? nd(I_DENS, XDIR, g) : R(0);
-      fx[h][1] = ok ? nd(I_RHOT, XDIR, g) : R(0);
-      fx[h][2] = (ok && k < KE) ? nd(I_MOMZ, XDIR, g) : R(0);
-      fx[h][3] = ok ? nd(I_MOMX, XDIR, g + 1) : R(0);
-      fx[h][4] = ok ? nd(I_MOMY, XDIR, g) : R(0);
+      if constexpr (PIPE) {
+        const R v0 = nd(I_DENS, XDIR, g), v1 = nd(I_RHOT, XDIR, g), v2 = nd(I_MOMZ, XDIR, g);
+        const R v3 = nd(I_MOMX, XDIR, g + 1), v4 = nd(I_MOMY, XDIR, g);
+        fx[h][0] = ok ? v0 : R(0);
+        fx[h][1] = ok ? v1 : R(0);
+        fx[h][2] = (ok && k < KE) ? v2 : R(0);
+        fx[h][3] = ok ? v3 : R(0);
+        fx[h][4] = ok ? v4 : R(0);
+      } else {
+        fx[h][0] = ok ? nd(I_DENS, XDIR, g) : R(0);
+        fx[h][1] = ok ? nd(I_RHOT, XDIR, g) : R(0);
+        fx[h][2] = (ok && k < KE) ? nd(I_MOMZ, XDIR, g) : R(0);
+        fx[h][3] = ok ? nd(I_MOMX, XDIR, g + 1) : R(0);
+        fx[h][4] = ok ? nd(I_MOMY, XDIR, g) : R(0);
+      }
     }
 #pragma unroll
     for (int h = 0; h < NY; ++h) {
@@ -886,13 +925,22 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       const int ii = i0 + col, jj = j0 - 1 + y;
       const bool ok = q < (TJ + 1) * TI && ii <= d.IE && jj <= d.JE;
       const size_t g = ok ? gix(k, ii, jj) : 0;
-      fy[h][0] = ok ? nd(I_DENS, YDIR, g) : R(0);
-      fy[h][1] = ok ? nd(I_RHOT, YDIR, g) : R(0);
-      fy[h][2] = (ok && k < KE) ? nd(I_MOMZ, YDIR, g) : R(0);
-      fy[h][3] = ok ? nd(I_MOMX, YDIR, g) : R(0);
-      fy[h][4] = ok ? nd(I_MOMY, YDIR, g + d.IA) : R(0);
+      if constexpr (PIPE) {
+        const R v0 = nd(I_DENS, YDIR, g), v1 = nd(I_RHOT, YDIR, g), v2 = nd(I_MOMZ, YDIR, g);
+        const R v3 = nd(I_MOMX, YDIR, g), v4 = nd(I_MOMY, YDIR, g + d.IA);
+        fy[h][0] = ok ? v0 : R(0);
+        fy[h][1] = ok ? v1 : R(0);
+        fy[h][2] = (ok && k < KE) ? v2 : R(0);
+        fy[h][3] = ok ? v3 : R(0);
+        fy[h][4] = ok ? v4 : R(0);
+      } else {
+        fy[h][0] = ok ? nd(I_DENS, YDIR, g) : R(0);
+        fy[h][1] = ok ? nd(I_RHOT, YDIR, g) : R(0);
+        fy[h][2] = (ok && k < KE) ? nd(I_MOMZ, YDIR, g) : R(0);
+        fy[h][3] = ok ? nd(I_MOMX, YDIR, g) : R(0);
+        fy[h][4] = ok ? nd(I_MOMY, YDIR, g + d.IA) : R(0);
+      }
     }
-    R tD[NC], tX[NC], tY[NC], tW[NC], tR[NC];
 #pragma unroll
     for (int h = 0; h < NC; ++h) {
       const int q = tid + h * NT;
@@ -900,14 +948,21 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
       const bool ok = q < RW * CW && ii < d.IA && jj < d.JA;
       const size_t g = ok ? gix(k, ii, jj) : 0;
-      tD[h] = ok ? a.DENS[g + d.pl] : R(0);
-      tX[h] = ok ? a.MOMX[g + d.pl] : R(0);
-      tY[h] = ok ? a.MOMY[g + d.pl] : R(0);
-      tW[h] = ok ? a.MOMZ[g] : R(0);
-      tR[h] = ok ? a.RHOT[g] : R(0);
+      if constexpr (PIPE) {
+        const R v0 = a.DENS[g + d.pl], v1 = a.MOMX[g + d.pl], v2 = a.MOMY[g + d.pl], v3 = a.MOMZ[g], v4 = a.RHOT[g];
+        tD[h] = ok ? v0 : R(0);
+        tX[h] = ok ? v1 : R(0);
+        tY[h] = ok ? v2 : R(0);
+        tW[h] = ok ? v3 : R(0);
+        tR[h] = ok ? v4 : R(0);
+      } else {
+        tD[h] = ok ? a.DENS[g + d.pl] : R(0);
+        tX[h] = ok ? a.MOMX[g + d.pl] : R(0);
+        tY[h] = ok ? a.MOMY[g + d.pl] : R(0);
+        tW[h] = ok ? a.MOMZ[g] : R(0);
+        tR[h] = ok ? a.RHOT[g] : R(0);
+      }
     }
-    double pD0[NP], pRT[NP], pRR[NP];
-    R pR[NP];
 #pragma unroll
     for (int h = 0; h < NP; ++h) {
       const int q = tid + h * NT;
@@ -915,11 +970,26 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       const int ii = i0 + c, jj = j0 + r;
       const bool ok = q < (TJ + 1) * (TI + 1) && ii < d.IA && jj < d.JA;
       const size_t g = ok ? gix(k, ii, jj) : 0;
-      pD0[h] = ok ? a.DPRES0[g] : 0.0;
-      pRT[h] = ok ? a.RT2P[g] : 0.0;
-      pRR[h] = ok ? a.REF_rhot[g] : 0.0;
-      pR[h] = ok ? a.RHOT[g] : R(0);
+      if constexpr (PIPE) {
+        const double v0 = a.DPRES0[g], v1 = a.RT2P[g], v2 = a.REF_rhot[g];
+        const R v3 = a.RHOT[g];
+        pD0[h] = ok ? v0 : 0.0;
+        pRT[h] = ok ? v1 : 0.0;
+        pRR[h] = ok ? v2 : 0.0;
+        pR[h] = ok ? v3 : R(0);
+      } else {
+        pD0[h] = ok ? a.DPRES0[g] : 0.0;
+        pRT[h] = ok ? a.RT2P[g] : 0.0;
+        pRR[h] = ok ? a.REF_rhot[g] : 0.0;
+        pR[h] = ok ? a.RHOT[g] : R(0);
+      }
     }
+  };
+  if constexpr (PIPE) issue(KS);
+
+  for (int k = KS; k <= KE; ++k) {
+    const int s1 = s0 ^ 1;
+    if constexpr (!PIPE) issue(k);
     // ---- (a) staging ----
 #pragma unroll
     for (int h = 0; h < NX; ++h) {
@@ -956,25 +1026,49 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
     __syncthreads();
     // ---- the column's own loads of this level, in flight during the face
     // evaluation ----
-    const size_t gk = own ? gix(k, i, j) : 0;
+    // (PIPE: issued by every thread, at the first interior column when the
+    // thread has none, and the next level's prefetch likewise, so that no
+    // branch hides from the wait counters how many loads are in flight)
+    const size_t gk = own ? gix(k, i, j) : (PIPE ? gix(k, d.IS, d.JS) : 0);
     R o_nZD = 0, o_nZT = 0, o_nZW = 0, o_nZX = 0, o_nZY = 0;
     R o_Dt = 0, o_Rt = 0, o_Wt = 0, o_Xt = 0, o_Yt = 0, o_X0 = 0, o_Y0 = 0, o_Z0 = 0;
     double o_RT = 0.0, o_RD = 0.0;
     R o_R2 = 1, o_D2 = 1, o_W2 = 0, o_X2 = 0, o_Y2 = 0, o_D = 1;
-    const double wdc = a.wdamp_coef[k];
-    if (own) {
+    const double wdc = zWD(k);
+    R o_nDw = 0;  // mflx_hi(Z) of the i = IS-1 column
+    if (own || PIPE) {
       o_nZD = nd(I_DENS, ZDIR, gk); o_nZT = nd(I_RHOT, ZDIR, gk); o_nZW = nd(I_MOMZ, ZDIR, gk + d.pl);
       o_nZX = nd(I_MOMX, ZDIR, gk); o_nZY = nd(I_MOMY, ZDIR, gk);
       o_Dt = a.DENS_t[gk]; o_Rt = a.RHOT_t[gk]; o_Wt = a.MOMZ_t[gk]; o_Xt = a.MOMX_t[gk]; o_Yt = a.MOMY_t[gk];
       o_X0 = a.MOMX0[gk]; o_Y0 = a.MOMY0[gk]; o_RT = a.RT2P[gk]; o_RD = a.REF_dens[gk];
-      if (wdc != 0.0) o_Z0 = a.MOMZ0[gk];
+      if constexpr (PIPE) {
+        const R z0 = a.MOMZ0[gk];
+        o_Z0 = wdc != 0.0 ? z0 : R(0);
+      } else {
+        if (wdc != 0.0) o_Z0 = a.MOMZ0[gk];
+      }
       const size_t g2 = gk + 2 * d.pl;
       o_R2 = a.RHOT[g2]; o_D2 = a.DENS[g2]; o_W2 = a.MOMZ[g2]; o_X2 = a.MOMX[g2]; o_Y2 = a.MOMY[g2];
       if constexpr (DEV) o_D = a.DENS[gk];
+      if constexpr (PIPE) {
+        const R w = nd(I_DENS, ZDIR, gk - 1);
+        o_nDw = (i == d.IS && k <= KE - 1) ? w : R(0);
+      } else {
+        if (i == d.IS && k <= KE - 1) o_nDw = nd(I_DENS, ZDIR, gk - 1);
+      }
+    }
+    if constexpr (PIPE) {
+      // the next level's loads go out after the own loads (the wait for those
+      // then leaves the prefetch in flight), at KE too (level KE+1 exists and
+      // its values go unused): no scheduling across this point
+      __builtin_amdgcn_sched_barrier(0);
+      issue(k + 1);
+      __builtin_amdgcn_sched_barrier(0);
     }
     // ---- (b) faces ----
-    const R f1 = R(CDZ[k] / (CDZ[k] + CDZ[k + 1]));      // F2H(k,1)
-    const R f2 = R(CDZ[k + 1] / (CDZ[k] + CDZ[k + 1]));  // F2H(k,2)
+    const double cz0 = zCDZ(k), cz1 = zCDZ(k + 1);
+    const R f1 = R(cz0 / (cz0 + cz1));  // F2H(k,1)
+    const R f2 = R(cz1 / (cz0 + cz1));  // F2H(k,2)
     auto D0 = [&](int r, int c) { return sD[s0][r][c]; };
     auto D1 = [&](int r, int c) { return sD[s1][r][c]; };
     for (int q = tid; q < TJ * (TI + 1); q += NT) {
@@ -1067,7 +1161,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
     if (own) {
       const int r = ty + 2, c = tx + 2;  // tile cell of the own column
       const size_t g = gix(k, i, j);
-      const R rcdz = R(RCDZ[k]), rcdx = R(RCDX[i]), rcdy = R(RCDY[j]), rfdx = R(RFDX[i]), rfdy = R(RFDY[j]);
+      const R rcdz = R(zRCDZ(k)), rcdx = R(hRCDX), rcdy = R(hRCDY), rfdx = R(hRFDX), rfdy = R(hRFDY);
       // continuity and theta (HEVI.F90:369-475, 588-657)
       const R mZ = (k <= KE - 1) ? o_nZD : R(0);
       const R Sr = (-(mZ - cmZ) * rcdz + -((xM[ty][tx + 1] - xM[ty][tx]) * rcdx + (yM[ty + 1][tx] - yM[ty][tx]) * rcdy))
@@ -1091,7 +1185,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
           const R fv = (k == KS) ? fv_lo<S>(ww[1], ww[2], ww[3], vel) : fv_int<S>(ww[0], ww[1], ww[2], ww[3], vel);
           qw = R(J33G) * vel * fv + o_nZW;
         }
-        const R advcv = -(qw - cqW) * R(RFDZ[k]);
+        const R advcv = -(qw - cqW) * R(zRFDZ(k));
         cqW = qw;
         const R advch = -((xW[ty][tx + 1] - xW[ty][tx]) * rcdx + (yW[ty + 1][tx] - yW[ty][tx]) * rcdy);
         const R wdmp = -R(wdc) * o_Z0;
@@ -1119,10 +1213,10 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
         const R advcv = -(qu - cqU) * rcdz;
         cqU = qu;
         const R advch = -((xU[ty][tx + 1] - xU[ty][tx]) * rfdx + (yU[ty + 1][tx] - yU[ty][tx]) * rcdy);
-        const R pg = R((sDP[ty][tx + 1] - sDP[ty][tx]) * RFDX[i]);
+        const R pg = R((sDP[ty][tx + 1] - sDP[ty][tx]) * hRFDX);
         R t = advcv + advch - pg;
         if (a.has_coriolis) {
-          const R cf = R(0.125 * (a.CORIOLI[(size_t)(i + 1) + (size_t)d.IA * j] + a.CORIOLI[(size_t)i + (size_t)d.IA * j]))
+          const R cf = R(corx)
                      * (sMY[s0][r][c + 1] + sMY[s0][r][c] + sMY[s0][r - 1][c + 1] + sMY[s0][r - 1][c]);
           t = t + cf;
         }
@@ -1145,10 +1239,10 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
         const R advcv = -(qv - cqV) * rcdz;
         cqV = qv;
         const R advch = -((xV[ty][tx + 1] - xV[ty][tx]) * rcdx + (yV[ty + 1][tx] - yV[ty][tx]) * rfdy);
-        const R pg = R((sDP[ty + 1][tx] - sDP[ty][tx]) * RFDY[j]);
+        const R pg = R((sDP[ty + 1][tx] - sDP[ty][tx]) * hRFDY);
         R t = advcv + advch - pg;
         if (a.has_coriolis) {
-          const R cf = R(-0.125 * (a.CORIOLI[(size_t)i + (size_t)d.IA * (j + 1)] + a.CORIOLI[(size_t)i + (size_t)d.IA * j]))
+          const R cf = R(cory)
                      * (sMX[s0][r + 1][c] + sMX[s0][r][c] + sMX[s0][r + 1][c - 1] + sMX[s0][r][c - 1]);
           t = t + cf;
         }
@@ -1161,17 +1255,18 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       else d1 = (double)D0(r, c);
       const double rt1 = o_RT, dp1 = sDP[ty][tx], rd1 = o_RD;
       const double St1 = (double)St, Sr1 = (double)Sr;
-      const double A1 = dtrk * dtrk * J33G * RCDZ[k] * rt1 * J33G;
+      const double A1 = dtrk * dtrk * J33G * zRCDZ(k) * rt1 * J33G;
       const double pt1 = (k <= KE - 1) ? (double)PT : 0.0;
       double ek = 0.0, fk = 0.0;
       if (k > KS) {
         const int rr = k - 1;
-        const double B = GRAV * (dtrk * dtrk) * J33G / (CDZ[rr + 1] + CDZ[rr]);
-        const double F1v = -(pt1 * RFDZ[rr] * A1 + B);
-        const double F2v = 1.0 + (pt0 * RFDZ[rr] * (A1 + A0));
-        const double F3v = -(ptm * RFDZ[rr] * A0 - B);
-        const double h1 = CDZ[rr] / (CDZ[rr] + CDZ[rr + 1]), h2 = CDZ[rr + 1] / (CDZ[rr] + CDZ[rr + 1]);
-        const double pg = -(dp1 + rt1 * dtrk * St1 - dp0 - rt0 * dtrk * st0) * RFDZ[rr] * J33G
+        const double c0 = zCDZ(rr), c1 = zCDZ(rr + 1), rf = zRFDZ(rr);
+        const double B = GRAV * (dtrk * dtrk) * J33G / (c1 + c0);
+        const double F1v = -(pt1 * rf * A1 + B);
+        const double F2v = 1.0 + (pt0 * rf * (A1 + A0));
+        const double F3v = -(ptm * rf * A0 - B);
+        const double h1 = c0 / (c0 + c1), h2 = c1 / (c0 + c1);
+        const double pg = -(dp1 + rt1 * dtrk * St1 - dp0 - rt0 * dtrk * st0) * rf * J33G
                         - GRAV * (h1 * (d1 - rd1 + Sr1 * dtrk) + h2 * (d0 - rd0 + sr0 * dtrk));
         const double Ck = (double)ww[0] + dtrk * (pg + sw0);
         if (rr == KS) {
@@ -1211,7 +1306,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       else { a.mflx_hi[g] = R(0); a.tflx_hi[g] = R(0); }
       if (k == KS) { a.mflx_hi[g - d.pl] = R(0); a.tflx_hi[g - d.pl] = R(0); }
       if (i == d.IS) {  // mflx_hi(Z) on the i = IS-1 column (HEVI.F90:370)
-        a.mflx_hi[g - 1] = (k <= KE - 1) ? nd(I_DENS, ZDIR, g - 1) : R(0);
+        a.mflx_hi[g - 1] = (k <= KE - 1) ? o_nDw : R(0);
         if (k == KS) a.mflx_hi[g - 1 - d.pl] = R(0);
       }
       if (k > KS && k - 1 < KE - 1) { a.e[g - d.pl] = ek; a.f[g - d.pl] = fk; }
