@@ -773,6 +773,14 @@ __global__ void __launch_bounds__(256) heve_march(HeviArgs a, unsigned ni_tiles,
   }
 }
 
+// Division in the tile kernel: exact in FP64; in FP32 the hardware
+// reciprocal (1 ulp) times the numerator, within the FP32 path's own rounding
+template <typename R>
+__device__ __forceinline__ R rdiv(R x, R y) {
+  if constexpr (sizeof(R) == sizeof(float)) return x * __builtin_amdgcn_rcpf(y);
+  else return x / y;
+}
+
 // ---------------------------------------------------------------------------
 // Flat-terrain stage with LDS tiles: blocks of 64 (i) x TJ (j) columns march
 // k together.  At each level the block
@@ -870,7 +878,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
   if (own) {
     for (int q = 1; q < 4; ++q) {
       const int k = KS - 2 + q;
-      pw[q] = fullT(at(a.RHOT, k), k) / fullD(at(a.DENS, k), k);
+      pw[q] = rdiv(fullT(at(a.RHOT, k), k), fullD(at(a.DENS, k), k));
       ww[q] = at(a.MOMZ, k); xw[q] = at(a.MOMX, k); yw[q] = at(a.MOMY, k);
     }
   }
@@ -1014,7 +1022,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       const bool in = ii < d.IA && jj < d.JA;
       sD[s1][r][c] = in ? fullD(tD[h], k + 1) : tD[h];
       sMX[s1][r][c] = tX[h]; sMY[s1][r][c] = tY[h]; sW[r][c] = tW[h];
-      sPO[r][c] = in ? fullT(tR[h], k) / sD[s0][r][c] : R(0);
+      sPO[r][c] = in ? rdiv(fullT(tR[h], k), sD[s0][r][c]) : R(0);
     }
 #pragma unroll
     for (int h = 0; h < NP; ++h) {
@@ -1083,19 +1091,19 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       xT[row][x] = vm * fv_int<S>(sPO[r][c - 1], sPO[r][c], sPO[r][c + 1], sPO[r][c + 2], vm) + ndq[1];
       R qw = 0;
       if (k < KE) {
-        const R vel = (f1 * sMX[s1][r][c] + f2 * sMX[s0][r][c])
-                         / (f1 * R(0.5) * (D1(r, c) + D1(r, c + 1)) + f2 * R(0.5) * (D0(r, c) + D0(r, c + 1)));
+        const R vel = rdiv((f1 * sMX[s1][r][c] + f2 * sMX[s0][r][c]),
+                           (f1 * R(0.5) * (D1(r, c) + D1(r, c + 1)) + f2 * R(0.5) * (D0(r, c) + D0(r, c + 1))));
         qw = vel * fv_int<S>(sW[r][c - 1], sW[r][c], sW[r][c + 1], sW[r][c + 2], vel) + ndq[2];
       }
       xW[row][x] = qw;
       {  // fluxX_UYZ stored at ii (loop index ii+1)
-        const R vel = (R(0.5) * (sMX[s0][r][c + 1] + sMX[s0][r][c])) / D0(r, c + 1);
+        const R vel = rdiv((R(0.5) * (sMX[s0][r][c + 1] + sMX[s0][r][c])), D0(r, c + 1));
         xU[row][x] = vel * fv_int<S>(sMX[s0][r][c - 1], sMX[s0][r][c], sMX[s0][r][c + 1], sMX[s0][r][c + 2], vel)
                    + ndq[3];
       }
       {  // fluxX_XVZ at (ii, jj)
-        const R vel = (R(0.5) * (sMX[s0][r][c] + sMX[s0][r + 1][c]))
-                    / (R(0.25) * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1)));
+        const R vel = rdiv((R(0.5) * (sMX[s0][r][c] + sMX[s0][r + 1][c])),
+                      (R(0.25) * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1))));
         xV[row][x] = vel * fv_int<S>(sMY[s0][r][c - 1], sMY[s0][r][c], sMY[s0][r][c + 1], sMY[s0][r][c + 2], vel)
                    + ndq[4];
       }
@@ -1121,19 +1129,19 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       yT[y][col] = vm * fv_int<S>(sPO[r - 1][c], sPO[r][c], sPO[r + 1][c], sPO[r + 2][c], vm) + ndq[1];
       R qw = 0;
       if (k < KE) {
-        const R vel = (f1 * sMY[s1][r][c] + f2 * sMY[s0][r][c])
-                         / (f1 * R(0.5) * (D1(r, c) + D1(r + 1, c)) + f2 * R(0.5) * (D0(r, c) + D0(r + 1, c)));
+        const R vel = rdiv((f1 * sMY[s1][r][c] + f2 * sMY[s0][r][c]),
+                           (f1 * R(0.5) * (D1(r, c) + D1(r + 1, c)) + f2 * R(0.5) * (D0(r, c) + D0(r + 1, c))));
         qw = vel * fv_int<S>(sW[r - 1][c], sW[r][c], sW[r + 1][c], sW[r + 2][c], vel) + ndq[2];
       }
       yW[y][col] = qw;
       {  // fluxY_UYZ at (ii, jj)
-        const R vel = (R(0.5) * (sMY[s0][r][c] + sMY[s0][r][c + 1]))
-                    / (R(0.25) * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1)));
+        const R vel = rdiv((R(0.5) * (sMY[s0][r][c] + sMY[s0][r][c + 1])),
+                      (R(0.25) * (D0(r, c) + D0(r, c + 1) + D0(r + 1, c) + D0(r + 1, c + 1))));
         yU[y][col] = vel * fv_int<S>(sMX[s0][r - 1][c], sMX[s0][r][c], sMX[s0][r + 1][c], sMX[s0][r + 2][c], vel)
                    + ndq[3];
       }
       {  // fluxY_XVZ stored at jj (loop index jj+1)
-        const R vel = (R(0.5) * (sMY[s0][r + 1][c] + sMY[s0][r][c])) / D0(r + 1, c);
+        const R vel = rdiv((R(0.5) * (sMY[s0][r + 1][c] + sMY[s0][r][c])), D0(r + 1, c));
         yV[y][col] = vel * fv_int<S>(sMY[s0][r - 1][c], sMY[s0][r][c], sMY[s0][r + 1][c], sMY[s0][r + 2][c], vel)
                    + ndq[4];
       }
@@ -1151,7 +1159,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
 
     // own-column windows move up one level
     if (own) {
-      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = fullT(o_R2, k + 2) / fullD(o_D2, k + 2);
+      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = rdiv(fullT(o_R2, k + 2), fullD(o_D2, k + 2));
       ww[0] = ww[1]; ww[1] = ww[2]; ww[2] = ww[3]; ww[3] = o_W2;
       xw[0] = xw[1]; xw[1] = xw[2]; xw[2] = xw[3]; xw[3] = o_X2;
       yw[0] = yw[1]; yw[1] = yw[2]; yw[2] = yw[3]; yw[3] = o_Y2;
@@ -1181,7 +1189,7 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       if (k <= KE - 1) {
         R qw = 0;
         if (k < KE - 1) {
-          const R vel = (R(0.5) * (ww[1] + ww[2])) / D1(r, c);
+          const R vel = rdiv((R(0.5) * (ww[1] + ww[2])), D1(r, c));
           const R fv = (k == KS) ? fv_lo<S>(ww[1], ww[2], ww[3], vel) : fv_int<S>(ww[0], ww[1], ww[2], ww[3], vel);
           qw = R(J33G) * vel * fv + o_nZW;
         }
@@ -1202,8 +1210,8 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       {
         R qu = 0;
         if (k < KE) {
-          const R vel = (R(0.5) * (ww[1] + sW[r][c + 1]))
-                      / (f1 * R(0.5) * (D1(r, c) + D1(r, c + 1)) + f2 * R(0.5) * (D0(r, c) + D0(r, c + 1)));
+          const R vel = rdiv((R(0.5) * (ww[1] + sW[r][c + 1])),
+                        (f1 * R(0.5) * (D1(r, c) + D1(r, c + 1)) + f2 * R(0.5) * (D0(r, c) + D0(r, c + 1))));
           R fv;
           if (k == KS) fv = fv_lo<S>(xw[1], xw[2], xw[3], vel);
           else if (k == KE - 1) fv = fv_hi<S>(xw[0], xw[1], xw[2], vel);
@@ -1228,8 +1236,8 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
       {
         R qv = 0;
         if (k < KE) {
-          const R vel = (R(0.5) * (ww[1] + sW[r + 1][c]))
-                      / (f1 * R(0.5) * (D1(r, c) + D1(r + 1, c)) + f2 * R(0.5) * (D0(r, c) + D0(r + 1, c)));
+          const R vel = rdiv((R(0.5) * (ww[1] + sW[r + 1][c])),
+                        (f1 * R(0.5) * (D1(r, c) + D1(r + 1, c)) + f2 * R(0.5) * (D0(r, c) + D0(r + 1, c))));
           R fv;
           if (k == KS) fv = fv_lo<S>(yw[1], yw[2], yw[3], vel);
           else if (k == KE - 1) fv = fv_hi<S>(yw[0], yw[1], yw[2], vel);

@@ -122,15 +122,21 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
   for (int k = KS; k <= KE; ++k) {
     const int s1 = s0 ^ 1;
     // every global load of the level first: the raw fields of this thread's
-    // tile cells, DENS at k+1, and the column's fields at k+2
+    // tile cells, DENS at k+1, and the column's fields at k+2.  The cells of
+    // the block's own columns are not loaded: their five fields at level k
+    // are the middle of those columns' z windows, computed from the same
+    // loads two levels earlier.
     R rZ[NC], rX[NC], rY[NC], rR[NC];
     double rRD[NC], rRP[NC];
+    auto owned = [&](int r, int c) {  // tile cell (r, c) is the own column of a thread of this block
+      return r >= 2 && r < TJ + 2 && c >= 2 && c < TI + 2 && i0 - 2 + c <= d.IE && j0 - 2 + r <= d.JE;
+    };
 #pragma unroll
     for (int h = 0; h < NC; ++h) {
       const int q = tid + h * NT;
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      const bool ok = q < RW * CW && ii < d.IA && jj < d.JA;
+      const bool ok = q < RW * CW && ii < d.IA && jj < d.JA && !owned(r, c);
       const size_t g = ok ? IX(d, k, ii, jj) : 0;
       rZ[h] = ok ? a.MOMZ[g] : R(0); rX[h] = ok ? a.MOMX[g] : R(0); rY[h] = ok ? a.MOMY[g] : R(0);
       rR[h] = ok ? fT(a.RHOT[g], k) : R(0); rRD[h] = ok ? a.REF_dens[g] : 0.0; rRP[h] = ok ? a.REF_pott[g] : R(0);
@@ -142,12 +148,15 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
       for (int f = 0; f < 5; ++f) { w[f][0] = w[f][1]; w[f][1] = w[f][2]; w[f][2] = w[f][3]; w[f][3] = w[f][4]; w[f][4] = ph[f]; }
     }
     __syncthreads();
+    if (own)
+      for (int f = 0; f < 5; ++f) sP[f][ty + 2][tx + 2] = w[f][2];
 #pragma unroll
     for (int h = 0; h < NC; ++h) {
       const int q = tid + h * NT;
       if (q >= RW * CW) continue;
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
+      if (owned(r, c)) continue;
       if (ii >= d.IA || jj >= d.JA) { for (int f = 0; f < 5; ++f) sP[f][r][c] = R(0); continue; }
       const R D = sD[s0][r][c];
       sP[0][r][c] = phi_ref<R>(D, rRD[h]);
