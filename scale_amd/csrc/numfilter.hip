@@ -99,6 +99,20 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
     ph[3] = R(2) * a.MOMY[q] / (fD(a.DENS[q + d.IA], k) + D);
   };
 
+  // the z coefficient tables in LDS (CNZ3 (3,KA,2), then nd_cdz, nd_fdz):
+  // the level's arithmetic then issues no global load
+  extern __shared__ double sZt[];
+  for (int q = tid; q < 6 * d.KA; q += NT) sZt[q] = a.CNZ3[q];
+  for (int q = tid; q < d.KA; q += NT) { sZt[6 * d.KA + q] = nd_cdz[q]; sZt[7 * d.KA + q] = nd_fdz[q]; }
+  // the own column's x and y coefficients, fixed over k
+  const int ic = own ? i : d.IS, jc = own ? j : d.JS;
+  const R cxa1 = R(n.cx(1, ic + 1, 1)), cxa2 = R(n.cx(2, ic + 1, 1)), cxa3 = R(n.cx(3, ic + 1, 1)), cxa4 = R(n.cx(1, ic, 1));
+  const R cxb1 = R(n.cx(1, ic, 2)), cxb2 = R(n.cx(2, ic, 2)), cxb3 = R(n.cx(3, ic, 2)), cxb4 = R(n.cx(1, ic - 1, 2));
+  const R cya1 = R(n.cy(1, jc + 1, 1)), cya2 = R(n.cy(2, jc + 1, 1)), cya3 = R(n.cy(3, jc + 1, 1)), cya4 = R(n.cy(1, jc, 1));
+  const R cyb1 = R(n.cy(1, jc, 2)), cyb2 = R(n.cy(2, jc, 2)), cyb3 = R(n.cy(3, jc, 2)), cyb4 = R(n.cy(1, jc - 1, 2));
+  const R ncdx = R(nd_cdx[ic]), nfdx = R(nd_fdx[ic]), ncdy = R(nd_cdy[jc]), nfdy = R(nd_fdy[jc]);
+  __syncthreads();
+
   // levels outside KS..KE are zero (KS-1 gets its DENS/RHOT z fluxes below)
   if (own) {
     for (int k = 0; k < d.KA; ++k) {
@@ -118,19 +132,23 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
   }
   int s0 = 0;
   fillD(sD[0], KS);
-  constexpr int NC = (RW * CW + NT - 1) / NT;  // tile cells per thread
-  for (int k = KS; k <= KE; ++k) {
-    const int s1 = s0 ^ 1;
-    // every global load of the level first: the raw fields of this thread's
-    // tile cells, DENS at k+1, and the column's fields at k+2.  The cells of
-    // the block's own columns are not loaded: their five fields at level k
-    // are the middle of those columns' z windows, computed from the same
-    // loads two levels earlier.
-    R rZ[NC], rX[NC], rY[NC], rR[NC];
-    double rRD[NC], rRP[NC];
-    auto owned = [&](int r, int c) {  // tile cell (r, c) is the own column of a thread of this block
-      return r >= 2 && r < TJ + 2 && c >= 2 && c < TI + 2 && i0 - 2 + c <= d.IE && j0 - 2 + r <= d.JE;
-    };
+
+  // The global loads of level k, issued during the arithmetic of level k-1:
+  // the raw fields of this thread's tile cells (the cells of the block's own
+  // columns excepted: their five fields at level k are the middle of those
+  // columns' z windows, from the same loads two levels earlier), DENS at k+1
+  // for the ring, and the raw fields of the own column at k+2.
+  constexpr int NC = (RW * CW + NT - 1) / NT;               // tile cells per thread
+  constexpr int NF = ((RW + 1) * (CW + 1) + NT - 1) / NT;   // DENS ring cells per thread
+  auto owned = [&](int r, int c) {  // tile cell (r, c) is the own column of a thread of this block
+    return r >= 2 && r < TJ + 2 && c >= 2 && c < TI + 2 && i0 - 2 + c <= d.IE && j0 - 2 + r <= d.JE;
+  };
+  R rZ[NC], rX[NC], rY[NC], rR[NC];
+  double rRD[NC], rRP[NC];
+  R rDn[NF];
+  R oD = 1, oDz = 1, oDx = 1, oDy = 1, oR = 0, oZ = 0, oX = 0, oY = 0;
+  double oRD = 0.0, oRP = 0.0;
+  auto issue = [&](int k) {
 #pragma unroll
     for (int h = 0; h < NC; ++h) {
       const int q = tid + h * NT;
@@ -141,10 +159,43 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
       rZ[h] = ok ? a.MOMZ[g] : R(0); rX[h] = ok ? a.MOMX[g] : R(0); rY[h] = ok ? a.MOMY[g] : R(0);
       rR[h] = ok ? fT(a.RHOT[g], k) : R(0); rRD[h] = ok ? a.REF_dens[g] : 0.0; rRP[h] = ok ? a.REF_pott[g] : R(0);
     }
-    fillD(sD[s1], k + 1);
+#pragma unroll
+    for (int h = 0; h < NF; ++h) {
+      const int q = tid + h * NT;
+      const int r = q / (CW + 1), c = q - r * (CW + 1);
+      const int ii = i0 - 2 + c, jj = j0 - 2 + r;
+      const bool ok = q < (RW + 1) * (CW + 1) && ii < d.IA && jj < d.JA;
+      rDn[h] = ok ? fD(a.DENS[IX(d, k + 1, ii, jj)], k + 1) : R(1);
+    }
+    if (own && k + 2 <= KE) {
+      const size_t q = p0 + d.pl * (size_t)(k + 2);
+      oD = a.DENS[q]; oDz = a.DENS[q + d.pl]; oDx = a.DENS[q + 1]; oDy = a.DENS[q + d.IA];
+      oR = a.RHOT[q]; oZ = a.MOMZ[q]; oX = a.MOMX[q]; oY = a.MOMY[q];
+      oRD = a.REF_dens[q]; oRP = a.REF_pott[q];
+    }
+  };
+  if (KS <= KE) issue(KS);
+  for (int k = KS; k <= KE; ++k) {
+    const int s1 = s0 ^ 1;
+#pragma unroll
+    for (int h = 0; h < NF; ++h) {
+      const int q = tid + h * NT;
+      if (q >= (RW + 1) * (CW + 1)) continue;
+      const int r = q / (CW + 1), c = q - r * (CW + 1);
+      sD[s1][r][c] = rDn[h];
+    }
     if (own) {
+      // the five fields at k+2 (COMMON.F90:545, 631, 715, 798, 882), as phis computes them
       R ph[5] = {0, 0, 0, 0, 0};
-      if (k + 2 <= KE) phis(k + 2, ph);
+      if (k + 2 <= KE) {
+        const int kk = k + 2;
+        const R D = fD(oD, kk);
+        ph[0] = phi_ref<R>(D, oRD);
+        ph[4] = phi_ref<R>(fT(oR, kk) / D, oRP);
+        ph[1] = R(2) * oZ / (fD(oDz, kk + 1) + D);
+        ph[2] = R(2) * oX / (fD(oDx, kk) + D);
+        ph[3] = R(2) * oY / (fD(oDy, kk) + D);
+      }
       for (int f = 0; f < 5; ++f) { w[f][0] = w[f][1]; w[f][1] = w[f][2]; w[f][2] = w[f][3]; w[f][3] = w[f][4]; w[f][4] = ph[f]; }
     }
     __syncthreads();
@@ -166,14 +217,15 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
       sP[4][r][c] = phi_ref<R>(rR[h] / D, rRP[h]);
     }
     __syncthreads();
+    // the next level's loads, in flight through this level's arithmetic
+    if (k < KE) issue(k + 1);
     if (own) {
       const int r = ty + 2, c = tx + 2;
-      auto NDC = [&](const double* t, int q) { return R(t[q]); };
       auto Dk = [&](int di, int dj) { return sD[s0][r + dj][c + di]; };
       auto Dk1 = [&](int di, int dj) { return sD[s1][r + dj][c + di]; };
-      auto cz = [&](int nn, int kk, int b) { return R(n.cz(nn, kk, b)); };
-      auto cx = [&](int nn, int ii, int b) { return R(n.cx(nn, ii, b)); };
-      auto cy = [&](int nn, int jj, int b) { return R(n.cy(nn, jj, b)); };
+      auto cz = [&](int nn, int kk, int b) { return R(sZt[(nn - 1) + 3 * (kk + d.KA * (b - 1))]); };
+      auto ncdz = [&](int kk) { return R(sZt[6 * d.KA + kk]); };
+      auto nfdz = [&](int kk) { return R(sZt[7 * d.KA + kk]); };
       // calc_diff3 ZDIR KO = 0 at level k in KS..KE (COMMON.F90:1668-1697);
       // w[f][m] holds level k-2+m
       auto dz0 = [&](int f) {
@@ -205,17 +257,17 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
       // calc_diff3 XDIR / YDIR (COMMON.F90:1733-1800)
       auto dx = [&](int f, int io) {
         if (io == 0)
-          return (+cx(1, i + 1, 1) * sP[f][r][c + 2] - cx(2, i + 1, 1) * sP[f][r][c + 1]
-                  + cx(3, i + 1, 1) * sP[f][r][c] - cx(1, i, 1) * sP[f][r][c - 1]);
-        return (+cx(1, i, 2) * sP[f][r][c + 1] - cx(2, i, 2) * sP[f][r][c]
-                + cx(3, i, 2) * sP[f][r][c - 1] - cx(1, i - 1, 2) * sP[f][r][c - 2]);
+          return (+cxa1 * sP[f][r][c + 2] - cxa2 * sP[f][r][c + 1]
+                  + cxa3 * sP[f][r][c] - cxa4 * sP[f][r][c - 1]);
+        return (+cxb1 * sP[f][r][c + 1] - cxb2 * sP[f][r][c]
+                + cxb3 * sP[f][r][c - 1] - cxb4 * sP[f][r][c - 2]);
       };
       auto dy = [&](int f, int jo) {
         if (jo == 0)
-          return (+cy(1, j + 1, 1) * sP[f][r + 2][c] - cy(2, j + 1, 1) * sP[f][r + 1][c]
-                  + cy(3, j + 1, 1) * sP[f][r][c] - cy(1, j, 1) * sP[f][r - 1][c]);
-        return (+cy(1, j, 2) * sP[f][r + 1][c] - cy(2, j, 2) * sP[f][r][c]
-                + cy(3, j, 2) * sP[f][r - 1][c] - cy(1, j - 1, 2) * sP[f][r - 2][c]);
+          return (+cya1 * sP[f][r + 2][c] - cya2 * sP[f][r + 1][c]
+                  + cya3 * sP[f][r][c] - cya4 * sP[f][r - 1][c]);
+        return (+cyb1 * sP[f][r + 1][c] - cyb2 * sP[f][r][c]
+                + cyb3 * sP[f][r - 1][c] - cyb4 * sP[f][r - 2][c]);
       };
       auto sfc = [&](int kk, R v) {  // surface factor rows (COMMON.F90:593-594)
         if (kk == KS) return v * R(a.sfc);
@@ -225,29 +277,29 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
       const bool kinm = k <= KE - 1;
       // density (COMMON.F90:552-621)
       const R dz0d = dz0(0);
-      out(I_DENS, ZDIR, k) = dz0d * NDC(nd_cdz, k);
-      if (k == KS) out(I_DENS, ZDIR, KS - 1) = (-dz0d) * NDC(nd_cdz, KS - 1);
-      out(I_DENS, XDIR, k) = sfc(k, dx(0, 0) * NDC(nd_cdx, i));
-      out(I_DENS, YDIR, k) = sfc(k, dy(0, 0) * NDC(nd_cdy, j));
+      out(I_DENS, ZDIR, k) = dz0d * ncdz(k);
+      if (k == KS) out(I_DENS, ZDIR, KS - 1) = (-dz0d) * ncdz(KS - 1);
+      out(I_DENS, XDIR, k) = sfc(k, dx(0, 0) * ncdx);
+      out(I_DENS, YDIR, k) = sfc(k, dy(0, 0) * ncdy);
       // z-momentum (COMMON.F90:637-707)
-      out(I_MOMZ, ZDIR, k) = (k >= KS + 1 && k <= KE - 1) ? dz1(1) * NDC(nd_fdz, k) * Dk(0, 0) : R(0);
-      out(I_MOMZ, XDIR, k) = kinm ? sfc(k, dx(1, 0) * NDC(nd_cdx, i) * R(0.25) * (Dk1(1, 0) + Dk1(0, 0) + Dk(1, 0) + Dk(0, 0))) : R(0);
-      out(I_MOMZ, YDIR, k) = kinm ? sfc(k, dy(1, 0) * NDC(nd_cdy, j) * R(0.25) * (Dk1(0, 1) + Dk1(0, 0) + Dk(0, 1) + Dk(0, 0))) : R(0);
+      out(I_MOMZ, ZDIR, k) = (k >= KS + 1 && k <= KE - 1) ? dz1(1) * nfdz(k) * Dk(0, 0) : R(0);
+      out(I_MOMZ, XDIR, k) = kinm ? sfc(k, dx(1, 0) * ncdx * R(0.25) * (Dk1(1, 0) + Dk1(0, 0) + Dk(1, 0) + Dk(0, 0))) : R(0);
+      out(I_MOMZ, YDIR, k) = kinm ? sfc(k, dy(1, 0) * ncdy * R(0.25) * (Dk1(0, 1) + Dk1(0, 0) + Dk(0, 1) + Dk(0, 0))) : R(0);
       // x-momentum (COMMON.F90:721-790)
-      out(I_MOMX, ZDIR, k) = kinm ? dz0(2) * NDC(nd_cdz, k) * R(0.25) * (Dk1(1, 0) + Dk1(0, 0) + Dk(1, 0) + Dk(0, 0)) : R(0);
-      out(I_MOMX, XDIR, k) = sfc(k, dx(2, 1) * NDC(nd_fdx, i) * Dk(0, 0));
-      out(I_MOMX, YDIR, k) = sfc(k, dy(2, 0) * NDC(nd_cdy, j) * R(0.25) * (Dk(1, 1) + Dk(1, 0) + Dk(0, 1) + Dk(0, 0)));
+      out(I_MOMX, ZDIR, k) = kinm ? dz0(2) * ncdz(k) * R(0.25) * (Dk1(1, 0) + Dk1(0, 0) + Dk(1, 0) + Dk(0, 0)) : R(0);
+      out(I_MOMX, XDIR, k) = sfc(k, dx(2, 1) * nfdx * Dk(0, 0));
+      out(I_MOMX, YDIR, k) = sfc(k, dy(2, 0) * ncdy * R(0.25) * (Dk(1, 1) + Dk(1, 0) + Dk(0, 1) + Dk(0, 0)));
       // y-momentum (COMMON.F90:804-873)
-      out(I_MOMY, ZDIR, k) = kinm ? dz0(3) * NDC(nd_cdz, k) * R(0.25) * (Dk1(0, 1) + Dk1(0, 0) + Dk(0, 1) + Dk(0, 0)) : R(0);
-      out(I_MOMY, XDIR, k) = sfc(k, dx(3, 0) * NDC(nd_cdx, i) * R(0.25) * (Dk(1, 1) + Dk(0, 1) + Dk(1, 0) + Dk(0, 0)));
-      out(I_MOMY, YDIR, k) = sfc(k, dy(3, 1) * NDC(nd_fdy, j) * Dk(0, 0));
+      out(I_MOMY, ZDIR, k) = kinm ? dz0(3) * ncdz(k) * R(0.25) * (Dk1(0, 1) + Dk1(0, 0) + Dk(0, 1) + Dk(0, 0)) : R(0);
+      out(I_MOMY, XDIR, k) = sfc(k, dx(3, 0) * ncdx * R(0.25) * (Dk(1, 1) + Dk(0, 1) + Dk(1, 0) + Dk(0, 0)));
+      out(I_MOMY, YDIR, k) = sfc(k, dy(3, 1) * nfdy * Dk(0, 0));
       // rho*theta (COMMON.F90:888-962)
       const R dz04 = dz0(4);
-      if (kinm) out(I_RHOT, ZDIR, k) = dz04 * NDC(nd_cdz, k) * R(0.5) * (Dk1(0, 0) + Dk(0, 0));
-      else out(I_RHOT, ZDIR, k) = dz04 * NDC(nd_cdz, KE) * Dk(0, 0);
-      if (k == KS) out(I_RHOT, ZDIR, KS - 1) = (-dz04) * NDC(nd_cdz, KS - 1) * Dk(0, 0);
-      out(I_RHOT, XDIR, k) = sfc(k, dx(4, 0) * NDC(nd_cdx, i) * R(0.5) * (Dk(1, 0) + Dk(0, 0)));
-      out(I_RHOT, YDIR, k) = sfc(k, dy(4, 0) * NDC(nd_cdy, j) * R(0.5) * (Dk(0, 1) + Dk(0, 0)));
+      if (kinm) out(I_RHOT, ZDIR, k) = dz04 * ncdz(k) * R(0.5) * (Dk1(0, 0) + Dk(0, 0));
+      else out(I_RHOT, ZDIR, k) = dz04 * ncdz(KE) * Dk(0, 0);
+      if (k == KS) out(I_RHOT, ZDIR, KS - 1) = (-dz04) * ncdz(KS - 1) * Dk(0, 0);
+      out(I_RHOT, XDIR, k) = sfc(k, dx(4, 0) * ncdx * R(0.5) * (Dk(1, 0) + Dk(0, 0)));
+      out(I_RHOT, YDIR, k) = sfc(k, dy(4, 0) * ncdy * R(0.5) * (Dk(0, 1) + Dk(0, 0)));
     }
     __syncthreads();
     s0 = s1;
@@ -267,11 +319,10 @@ static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], c
   a.num_diff = num_diff;
   const Dims& d = c.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
-  // tile height: the tallest of 16, 8 and 4 rows that still gives every CU a
-  // block (taller tiles re-read fewer halo rows: at 512x512x128 FP64 the
-  // filter takes 2.94 ms at 16 rows, 3.08 ms at 8 and 4.35 ms at 4; at
-  // 128x128x64 the 16 blocks of 16 rows take 1.14 ms, the 64 of 4 rows
-  // 0.59 ms); SCALE_HIP_NF_TJ = 4, 8 or 16 overrides
+  // tile height: 8 rows when that gives every CU a block, 4 otherwise (at
+  // 512x512x128 the filter takes 1.94 ms at 8 rows, 1.95 at 4 and 4.24 at 16
+  // in FP64, whose 1024-thread blocks are held to 128 VGPRs and spill; FP32
+  // 1.06, 1.80 and 1.09 ms); SCALE_HIP_NF_TJ = 4, 8 or 16 overrides
   static const int ncu = [] {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -287,15 +338,15 @@ static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], c
   int TJ = TJenv;
   if (!TJ) {
     TJ = 4;
-    for (int t : {16, 8})
-      if (ni * (unsigned)((jmax + t - 1) / t) >= (unsigned)ncu) { TJ = t; break; }
+    if (ni * (unsigned)((jmax + 7) / 8) >= (unsigned)ncu) TJ = 8;
   }
   const unsigned nj = (jmax + TJ - 1) / TJ;
   const unsigned nb = ni * nj;
+  const size_t lds = 8 * (size_t)d.KA * sizeof(double);  // the z coefficient tables
   prof_start(c, "numfilter", s);
-  if (TJ == 16) hipLaunchKernelGGL((numfilter_tile<16, R>), dim3(nb), dim3(64, 16), 0, s, a, ni, nb);
-  else if (TJ == 8) hipLaunchKernelGGL((numfilter_tile<8, R>), dim3(nb), dim3(64, 8), 0, s, a, ni, nb);
-  else hipLaunchKernelGGL((numfilter_tile<4, R>), dim3(nb), dim3(64, 4), 0, s, a, ni, nb);
+  if (TJ == 16) hipLaunchKernelGGL((numfilter_tile<16, R>), dim3(nb), dim3(64, 16), lds, s, a, ni, nb);
+  else if (TJ == 8) hipLaunchKernelGGL((numfilter_tile<8, R>), dim3(nb), dim3(64, 8), lds, s, a, ni, nb);
+  else hipLaunchKernelGGL((numfilter_tile<4, R>), dim3(nb), dim3(64, 4), lds, s, a, ni, nb);
   prof_stop(c, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
