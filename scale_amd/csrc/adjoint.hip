@@ -967,7 +967,7 @@ hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const d
     aadd(RTb + p, fb[p] / D[p]);
     aadd(Db + p, -fb[p] * RT[p] / (D[p] * D[p]));
   }, s));
-  return hipMemsetAsync(NDb, 0, 15 * n3 * sizeof(double), s);
+  return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------
@@ -976,20 +976,16 @@ hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const d
 // entry, of x on return
 // Forward of one large step from x with everything its reverse reads kept
 // in rec; with xnext, the new state (RK4 combination, ghost levels, halos)
+// (the tendencies M.t are the ones the large-step preparation left: they do
+// not depend on the state)
 static hipError_t step_fwd(Ctx& c, double* const x[5], const StepRec& rec, double* const xnext[5]) {
   Model& M = *c.model;
   AdjWork& A = *c.adj;
   const Dims d = c.d;
   hipStream_t s = c.stream;
   const size_t bytes = d.n3 * sizeof(double);
-  double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
-  for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(st[v], x[v], bytes, hipMemcpyDeviceToDevice, s));
-  HIPCHK(model_prepare(c));
-  if (rec.RT2P != M.RT2P) {
-    HIPCHK(hipMemcpyAsync(rec.RT2P, M.RT2P, bytes, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(rec.DPRES0, M.DPRES0, bytes, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(rec.REF_rhot, M.REF_rhot, bytes, hipMemcpyDeviceToDevice, s));
-  }
+  // RT2P, DPRES0, REF_rhot of this step, from its RHOT, straight into the record
+  HIPCHK(large_prep_into(c, x[4], rec.RT2P, rec.DPRES0, rec.REF_rhot, s));
   if (M.nd_coef == 0.0) {
     HIPCHK(hipMemsetAsync(rec.nd, 0, 15 * bytes, s));
   } else {
@@ -1050,7 +1046,8 @@ static hipError_t step_rev(Ctx& c, double* const x[5], double* const xb[5], cons
     }, s));
   }
   // combination (RK4.F90:562-660)
-  HIPCHK(hipMemsetAsync(A.S0b[0], 0, 25 * bytes, s));  // S0b and Rb[0..3], one block
+  // S0b and Rb[0..2], one block (Rb[3] is only read, at the points written below)
+  HIPCHK(hipMemsetAsync(A.S0b[0], 0, 20 * bytes, s));
   for (int v = 0; v < 5; ++v) {
     double *b = xb[v], *r1 = A.Rb[0][v], *r2 = A.Rb[1][v], *r3 = A.Rb[2][v], *r4 = A.Rb[3][v], *s0 = A.S0b[v];
     HIPCHK(for3(Box{KS, v == 1 ? KE - 1 : KE, d.IS, d.IE, d.JS, d.JE}, [=] __device__(int k, int i, int j) {
@@ -1177,6 +1174,7 @@ hipError_t model_adjoint(Ctx& c, double* const xb[5]) {
   for (int v = 0; v < 5; ++v) keep[v] = A.xb[v];  // the final state, restored at the end
   for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(keep[v], st[v], bytes, hipMemcpyDeviceToDevice, s));
   prof_start(c, "model_adjoint", s);
+  HIPCHK(model_prepare(c));  // the tendencies M.t, which every step shares
   const int nseg = (A.nsteps + A.every - 1) / A.every;
   const bool recs = (int)A.recs.size() == A.every;
   const StepRec one = model_rec(c);

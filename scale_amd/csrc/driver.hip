@@ -542,6 +542,12 @@ hipError_t model_state_to_f32(Ctx& c) {
   return hipGetLastError();
 }
 
+// RT2P, DPRES0, REF_rhot from RHOT alone (LARGE.F90:443-471), into the given arrays
+hipError_t large_prep_into(Ctx& c, const double* RHOT, double* RT2P, double* DPRES0, double* REF_rhot, hipStream_t s) {
+  hipLaunchKernelGGL(large_prep, grid_full(c.d), dim3(64, 4), 0, s, c.d, RHOT, c.model->REF_pres, RT2P, DPRES0, REF_rhot);
+  return hipGetLastError();
+}
+
 // RT2P, DPRES0, REF_rhot, DENS00, tendencies (LARGE.F90:443-839)
 hipError_t model_prepare(Ctx& c) {
   Model& M = *c.model;

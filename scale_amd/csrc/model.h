@@ -51,6 +51,7 @@ struct Model {
 // step): the step also keeps in rec what the reverse sweep reads
 hipError_t model_short_step_rk4(Ctx& c, double dts, const StepRec* rec = nullptr);
 hipError_t model_prepare(Ctx& c);
+hipError_t large_prep_into(Ctx& c, const double* RHOT, double* RT2P, double* DPRES0, double* REF_rhot, hipStream_t s);
 hipError_t model_large_step(Ctx& c, const StepRec* rec = nullptr);
 hipError_t mass_sum(const Ctx& c, const double* a, const double* b, double* out_host);
 hipError_t model_state_to_f64(Ctx& c);

@@ -77,8 +77,8 @@ struct StageIOT {
   // averaged fluxes into mflx_av/tflx_av, instead of o / the Z-free parts of
   // mflx_hi, tflx_hi (tinteg_short_rk4.F90:560-660)
   int combine = 0;
-  const R *c1[5], *c2[5], *c3[5];
-  const R *cm[3], *ct[3];
+  const R *c1[5], *c2[5], *c3[5];  // DENS..RHOT outputs of stages 1, 2, 3
+  const R *cm[3], *ct[3];          // mflx_hi, tflx_hi (3 components, 3 n3 each) of stages 1, 2, 3
   R* st[5];
   R *mflx_av, *tflx_av;
 };
@@ -164,7 +164,7 @@ struct StageAdjIO {
   const double *trPT = nullptr, *trSt = nullptr, *trCp = nullptr;
 };
 hipError_t hevi_stage_adj(Ctx& c, const StageAdjIO& io, hipStream_t s);
-// num_diff adjoint (15 fields, consumed) -> adjoints of DENS..RHOT (added)
+// num_diff adjoint (15 fields, read) -> adjoints of DENS..RHOT (added)
 hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const double* REF_dens, const double* REF_pott,
                          double* const stb[5], hipStream_t s);
 // adjoint work arrays (nsteps > 0: also checkpoints of a run of nsteps saved every 'every')
