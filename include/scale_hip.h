@@ -38,13 +38,17 @@ typedef struct scale_hip_ctx scale_hip_ctx;
  * ATMOS_DYN_FVM_FLUX_TRACER_TYPE (scale-rm/src/atmos/mod_atmos_dyn_driver.F90:52),
  * scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_fvm_flux_ud3Koren1993.F90 */
 #define SCALE_HIP_UD3KOREN1993 31
+/* tracers only: first-order upwind and second-order central
+ * (scale_atmos_dyn_fvm_flux_ud1.F90, scale_atmos_dyn_fvm_flux_cd2.F90) */
+#define SCALE_HIP_UD1 1
+#define SCALE_HIP_CD2 2
 
 typedef struct {
   int32_t KA, IA, JA;             /* array extents                           */
   int32_t KS, KE, IS, IE, JS, JE; /* interior bounds, Fortran 1-based         */
   int32_t KHALO, IHALO, JHALO;    /* must be 2 (filter_setup, COMMON.F90:137) */
   int32_t flux_scheme;            /* dynamics: SCALE_HIP_UD3 or SCALE_HIP_CD4 */
-  int32_t flux_scheme_tracer;     /* tracers:  SCALE_HIP_UD3, _CD4 or _UD3KOREN1993 */
+  int32_t flux_scheme_tracer;     /* tracers:  SCALE_HIP_UD3, _CD4, _UD3KOREN1993, _UD1 or _CD2 */
   int32_t device;                 /* HIP device ordinal                       */
 } scale_hip_grid;
 

@@ -17,7 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-UD1, UD3, CD4 = 1, 3, 4
+UD1, CD2, UD3, CD4 = 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 

@@ -99,6 +99,8 @@ static inline double fv_int(int s, double qm1, double q0, double q1, double q2, 
     return (F31 * (q2 + qm1) + F32 * (q1 + q0)) - (F31 * (q2 - qm1) + F33 * (q1 - q0)) * fsign(1.0, vel);
   if (s == ORC_CD4) /* CD4.F90:190-192 */
     return F41 * (q1 + q0) + F42 * (q2 + qm1);
+  if (s == ORC_CD2) /* cd2.F90:168-171 */
+    return F2 * (q1 + q0);
   /* UD1.F90:153-154 */
   return F1 * (q1 + q0) - fsign(F1, vel) * (q1 - q0);
 }
@@ -108,6 +110,7 @@ static inline double fv_lo(int s, double q0, double q1, double q2, double vel) {
     return q0 * (0.5 + fsign(0.5, vel)) + (q1 + 0.5 * koren_phi(q0, q1, q2) * (q1 - q2)) * (0.5 - fsign(0.5, vel));
   if (s == ORC_UD3)
     return F2 * (q1 + q0) * (0.5 + fsign(0.5, vel)) + (2.0 * q0 + 5.0 * q1 - q2) / 6.0 * (0.5 - fsign(0.5, vel));
+  if (s == ORC_UD1) return fv_int(s, 0.0, q0, q1, 0.0, vel); /* no boundary form (UD1.F90:135-171) */
   return F2 * (q1 + q0);
 }
 /* highest face: qm1 = val(KE-2), q0 = val(KE-1), q1 = val(KE) (UD3.F90:231-235, CD4.F90:221-222) */
@@ -116,6 +119,7 @@ static inline double fv_hi(int s, double qm1, double q0, double q1, double vel) 
     return (q0 + 0.5 * koren_phi(qm1, q0, q1) * (q0 - q1)) * (0.5 + fsign(0.5, vel)) + q1 * (0.5 - fsign(0.5, vel));
   if (s == ORC_UD3)
     return (2.0 * q1 + 5.0 * q0 - qm1) / 6.0 * (0.5 + fsign(0.5, vel)) + F2 * (q1 + q0) * (0.5 - fsign(0.5, vel));
+  if (s == ORC_UD1) return fv_int(s, 0.0, q0, q1, 0.0, vel);
   return F2 * (q1 + q0);
 }
 
@@ -1918,7 +1922,7 @@ int orc_tstep_tracer_heve(const orc_grid *g, int s,
     double dtl, int flag_fct, int flag_fct_along_stream) {
   GRID_LOCALS;
   (void)CDZ;
-  if (s != ORC_UD3 && s != ORC_CD4 && s != ORC_UD3KOREN) return -1;
+  if (s != ORC_UD3 && s != ORC_CD4 && s != ORC_UD3KOREN && s != ORC_UD1 && s != ORC_CD2) return -1;
   double *qflx_hi = alloc_nan(3 * N3), *qflx_lo = alloc_nan(3 * N3), *qflx_anti = NULL;
   const double *mZ = mflx_hi, *mX = mflx_hi + N3, *mY = mflx_hi + 2 * N3;
   const double *GXYW = GSQRT + N3 * I_XYW, *GUYZ = GSQRT + N3 * I_UYZ, *GXVZ = GSQRT + N3 * I_XVZ, *GXYZ = GSQRT + N3 * I_XYZ;

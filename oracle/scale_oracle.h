@@ -36,6 +36,7 @@ extern "C" {
 
 /* flux scheme identifiers (the order of the upwind/central scheme) */
 #define ORC_UD1 1
+#define ORC_CD2 2 /* tracer scheme: second-order central (cd2.F90) */
 #define ORC_UD3 3
 #define ORC_CD4 4
 #define ORC_UD3KOREN 31 /* UD3KOREN1993: tracer scheme, UD3 with the Koren limiter */

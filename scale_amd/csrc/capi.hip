@@ -296,8 +296,9 @@ int scale_hip_init(const scale_hip_grid* g, scale_hip_ctx** out) {
   if (g->flux_scheme != SCALE_HIP_UD3 && g->flux_scheme != SCALE_HIP_CD4)
     return fail("the dynamics flux scheme must be UD3 (3) or CD4 (4)");
   if (g->flux_scheme_tracer != SCALE_HIP_UD3 && g->flux_scheme_tracer != SCALE_HIP_CD4 &&
-      g->flux_scheme_tracer != SCALE_HIP_UD3KOREN1993)
-    return fail("the tracer flux scheme must be UD3 (3), CD4 (4) or UD3KOREN1993 (31)");
+      g->flux_scheme_tracer != SCALE_HIP_UD3KOREN1993 && g->flux_scheme_tracer != SCALE_HIP_UD1 &&
+      g->flux_scheme_tracer != SCALE_HIP_CD2)
+    return fail("the tracer flux scheme must be UD3 (3), CD4 (4), UD3KOREN1993 (31), UD1 (1) or CD2 (2)");
   if (hipSetDevice(g->device) != hipSuccess) return fail("hipSetDevice failed");
   scale_hip_ctx* h = new scale_hip_ctx();
   Ctx& c = h->c;

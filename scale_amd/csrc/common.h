@@ -40,7 +40,7 @@ constexpr double F33 = 3.0 / 12.0;
 constexpr double F41 = 7.0 / 12.0;
 constexpr double F42 = -1.0 / 12.0;
 
-constexpr int UD1 = 1, UD3 = 3, CD4 = 4;
+constexpr int UD1 = 1, CD2 = 2, UD3 = 3, CD4 = 4;
 constexpr int UD3KOREN = 31;  // UD3 with the Koren (1993) limiter (tracer scheme UD3KOREN1993)
 
 struct Dims {
@@ -89,6 +89,8 @@ __device__ __forceinline__ T fv_int(T qm1, T q0, T q1, T q2, T vel) {
     return (T(F31) * (q2 + qm1) + T(F32) * (q1 + q0)) - (T(F31) * (q2 - qm1) + T(F33) * (q1 - q0)) * fsign(T(1), vel);
   else if constexpr (S == CD4)
     return T(F41) * (q1 + q0) + T(F42) * (q2 + qm1);
+  else if constexpr (S == CD2)  // scale_atmos_dyn_fvm_flux_cd2.F90:168-171
+    return T(F2) * (q1 + q0);
   else
     return T(F1) * (q1 + q0) - fsign(T(F1), vel) * (q1 - q0);
 }
@@ -100,6 +102,8 @@ __device__ __forceinline__ T fv_lo(T q0, T q1, T q2, T vel) {
          + (q1 + T(0.5) * koren_phi(q0, q1, q2) * (q1 - q2)) * (T(0.5) - fsign(T(0.5), vel));
   else if constexpr (S == UD3)
     return T(F2) * (q1 + q0) * (T(0.5) + fsign(T(0.5), vel)) + (T(2) * q0 + T(5) * q1 - q2) / T(6) * (T(0.5) - fsign(T(0.5), vel));
+  else if constexpr (S == UD1)  // the UD1 face has no boundary form (UD1.F90:135-171)
+    return fv_int<UD1, T>(T(0), q0, q1, T(0), vel);
   else
     return T(F2) * (q1 + q0);
 }
@@ -111,6 +115,8 @@ __device__ __forceinline__ T fv_hi(T qm1, T q0, T q1, T vel) {
          + q1 * (T(0.5) - fsign(T(0.5), vel));
   else if constexpr (S == UD3)
     return (T(2) * q1 + T(5) * q0 - qm1) / T(6) * (T(0.5) + fsign(T(0.5), vel)) + T(F2) * (q1 + q0) * (T(0.5) - fsign(T(0.5), vel));
+  else if constexpr (S == UD1)
+    return fv_int<UD1, T>(T(0), q0, q1, T(0), vel);
   else
     return T(F2) * (q1 + q0);
 }

@@ -61,18 +61,21 @@ struct TrOps {
     if (kk == d.KS) fv = fv_lo<S>(Q[p], Q[ix(kk + 1, i, j)], Q[ix(kk + 2, i, j)], vel);
     else if (kk == d.KE - 1) fv = fv_hi<S>(Q[ix(kk - 1, i, j)], Q[p], Q[ix(kk + 1, i, j)], vel);
     else fv = fv_int<S>(Q[ix(kk - 1, i, j)], Q[p], Q[ix(kk + 1, i, j)], Q[ix(kk + 2, i, j)], vel);
+    if constexpr (S == UD1) return vel * fv;  // the UD1 routines take no num_diff (UD1.F90:135-249)
     return vel * fv + gs(I_XYW, p) * a.nd[p];
   }
   __device__ double hX(int k, int ii, int j) const {
     const double* Q = a.QTRC;
     size_t p = ix(k, ii, j);
     double vel = mX(p);
+    if constexpr (S == UD1) return vel * fv_int<UD1>(0.0, Q[p], Q[ix(k, ii + 1, j)], 0.0, vel);
     return vel * fv_int<S>(Q[ix(k, ii - 1, j)], Q[p], Q[ix(k, ii + 1, j)], Q[ix(k, ii + 2, j)], vel) + gs(I_UYZ, p) * a.nd[p + a.nds];
   }
   __device__ double hY(int k, int i, int jj) const {
     const double* Q = a.QTRC;
     size_t p = ix(k, i, jj);
     double vel = mY(p);
+    if constexpr (S == UD1) return vel * fv_int<UD1>(0.0, Q[p], Q[ix(k, i, jj + 1)], 0.0, vel);
     return vel * fv_int<S>(Q[ix(k, i, jj - 1)], Q[p], Q[ix(k, i, jj + 1)], Q[ix(k, i, jj + 2)], vel) + gs(I_XVZ, p) * a.nd[p + 2 * a.nds];
   }
   // low-order UD1 fluxes on QTRC0 (scale_atmos_dyn_fvm_flux_ud1.F90:135-249)
@@ -369,6 +372,8 @@ hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s) {
   if (c.scheme_tracer == UD3) return c.terrain ? tracer_t<UD3, true>(c, a, fct, along, s) : tracer_t<UD3, false>(c, a, fct, along, s);
   if (c.scheme_tracer == UD3KOREN)
     return c.terrain ? tracer_t<UD3KOREN, true>(c, a, fct, along, s) : tracer_t<UD3KOREN, false>(c, a, fct, along, s);
+  if (c.scheme_tracer == CD2) return c.terrain ? tracer_t<CD2, true>(c, a, fct, along, s) : tracer_t<CD2, false>(c, a, fct, along, s);
+  if (c.scheme_tracer == UD1) return c.terrain ? tracer_t<UD1, true>(c, a, fct, along, s) : tracer_t<UD1, false>(c, a, fct, along, s);
   return c.terrain ? tracer_t<CD4, true>(c, a, fct, along, s) : tracer_t<CD4, false>(c, a, fct, along, s);
 }
 

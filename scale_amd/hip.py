@@ -29,6 +29,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 UD3, CD4 = 3, 4
+UD3KOREN1993, UD1, CD2 = 31, 1, 2  # tracer schemes
 FIELD = dict(DENS=0, MOMZ=1, MOMX=2, MOMY=3, RHOT=4, REF_dens=10, REF_pott=11, REF_pres=12, CORIOLI=13,
              wdamp_coef=14, mflx_hi=20, tflx_hi=21, DENS_tp=30, MOMZ_tp=31, MOMX_tp=32, MOMY_tp=33, RHOT_tp=34)
 QTRC_BASE, RHOQ_TP_BASE = 100, 200

@@ -266,7 +266,7 @@ def test_model_parity_tracer_filter(hip):
 
 
 @pytest.mark.parametrize("fct,along", [(False, True), (True, True), (True, False)], ids=["nofct", "fct_along", "fct_7pt"])
-@pytest.mark.parametrize("scheme", [3, 4, 31], ids=["UD3", "CD4", "UD3KOREN1993"])
+@pytest.mark.parametrize("scheme", [3, 4, 31, 1, 2], ids=["UD3", "CD4", "UD3KOREN1993", "UD1", "CD2"])
 def test_tracer_bitwise(hip, fct, along, scheme):
     print(f"[tracer] fct={fct} along={along} scheme={scheme}", flush=True)
     case = H.small_case(seed=55)
@@ -281,7 +281,7 @@ def test_tracer_bitwise(hip, fct, along, scheme):
     q = np.asfortranarray(case.qtrc[..., 1])
     q0 = np.asfortranarray(case.qtrc[..., 1] * (1.0 + 0.01 * rng.standard_normal(g.shape)))
     inputs.halo_fill(g, q0)
-    nd = np.zeros(g.shape + (3,), order="F")
+    nd = np.asfortranarray(1e-5 * rng.standard_normal(g.shape + (3,)))  # num_diff_q (unused by UD1)
     rq = np.zeros(g.shape, order="F")
     mapf_xy = np.asfortranarray(case.metric["MAPF"][:, :, :, 0])
     ref = np.full(g.shape, np.nan, order="F")

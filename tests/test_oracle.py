@@ -827,6 +827,73 @@ def test_tracer_mass_conserved(fct):
     assert abs(after - before) <= 1e-14 * abs(before)
 
 
+@pytest.mark.parametrize("scheme", [po.CD2, po.UD1], ids=["CD2", "UD1"])
+@pytest.mark.parametrize("axis", [0, 1, 2], ids=["x", "y", "z"])
+def test_tracer_cd2_ud1_fourier_closed_form(scheme, axis):
+    """The CD2 and UD1 tracer schemes (cd2.F90, ud1.F90) advecting a sine wave
+    by a uniform mass flux rho*u over a uniform density: one step is
+    q - dt u/dx * (q(n+1) - q(n-1))/2 for CD2 and q - dt u/dx * (q(n) - q(n-1))
+    for UD1 (u > 0), i.e. for q = sin(2 pi n / N) the closed forms
+    sin(2 pi/N) cos(2 pi n/N) and 2 sin(pi/N) cos(2 pi (n - 1/2)/N).  In z the
+    levels two away from the top and bottom follow the same form."""
+    case = inputs.make_case(kmax=16, imax=12, jmax=10, noise=0.0, bubble=False, u0=0.0, ntracer=0)
+    g, m = case.grid, case.metric
+    D = np.full(g.shape, 1.2, order="F")
+    kk, ii, jj = np.meshgrid(np.arange(g.KA), np.arange(g.IA), np.arange(g.JA), indexing="ij")
+    n = (ii - g.IS, jj - g.JS, kk - g.KS)[axis]
+    N = (g.IE - g.IS + 1, g.JE - g.JS + 1, g.KE - g.KS + 1)[axis]
+    q = np.asfortranarray(1.0 + 1e-3 * np.sin(2 * np.pi * n / N))
+    u, dt = 3.0, 0.5
+    mflx = np.zeros(g.shape + (3,), order="F")
+    comp = (1, 2, 0)[axis]
+    mflx[..., comp] = 1.2 * u
+    if axis == 2:  # no flux through the bottom and the top
+        mflx[:g.KS, :, :, 0] = 0.0
+        mflx[g.KE:, :, :, 0] = 0.0
+    nd = np.zeros(g.shape + (3,), order="F")
+    out = np.full(g.shape, np.nan, order="F")
+    po.tstep_tracer(case, out, q, q, g.zeros(), D, D, mflx, nd, dt, False, scheme=scheme)
+    dx = (1.0 / m["RCDX"][g.IS], 1.0 / m["RCDY"][g.JS], 1.0 / m["RCDZ"][g.KS])[axis]
+    if scheme == po.CD2:
+        dq = np.sin(2 * np.pi / N) * np.cos(2 * np.pi * n / N)
+    else:
+        dq = 2 * np.sin(np.pi / N) * np.cos(2 * np.pi * (n - 0.5) / N)
+    expect = q - dt * u / dx * 1e-3 * dq
+    sl = [slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1)]
+    if axis == 2:
+        sl[0] = slice(g.KS + 2, g.KE - 1)
+    sl = tuple(sl)
+    assert np.abs(out[sl] - expect[sl]).max() < 1e-14, np.abs(out[sl] - expect[sl]).max()
+    assert np.abs(out[sl] - q[sl]).max() > 1e-6  # the wave moved
+
+
+def test_tracer_ud1_monotone_and_without_num_diff():
+    """UD1 as the tracer scheme: with continuity-consistent fluxes a two-level
+    field stays within its bounds without any limiter (the monotone scheme),
+    and the num_diff argument does not enter (the UD1 routines take none,
+    UD1.F90:135-249), while CD2 does use it."""
+    case, mflx, D0, D, dt = _consistent_tracer_setup(2)
+    g = case.grid
+    rng = np.random.default_rng(6)
+    q = np.asfortranarray(np.where(rng.uniform(size=g.shape) < 0.5, 0.0, 0.1))
+    inputs.halo_fill(g, q)
+    nd0 = np.zeros(g.shape + (3,), order="F")
+    nd1 = np.asfortranarray(1e-4 * rng.standard_normal(g.shape + (3,)))
+    inner = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    outs = {}
+    for scheme in (po.UD1, po.CD2):
+        for tag, nd in (("0", nd0), ("1", nd1)):
+            o = np.full(g.shape, np.nan, order="F")
+            po.tstep_tracer(case, o, q, q, g.zeros(), D0, D, mflx, nd, dt, False, scheme=scheme)
+            outs[scheme, tag] = o[inner]
+    u = outs[po.UD1, "0"]
+    assert u.min() >= -1e-15 and u.max() <= 0.1 * (1 + 1e-14)
+    assert np.array_equal(outs[po.UD1, "0"], outs[po.UD1, "1"])
+    assert not np.array_equal(outs[po.CD2, "0"], outs[po.CD2, "1"])
+    c = outs[po.CD2, "0"]
+    assert c.min() < -1e-6 or c.max() > 0.1 + 1e-6  # the central scheme is not monotone
+
+
 # ---------------------------------------------------------------------------
 def test_halo_known_answer():
     """test_comm.f90:254-330 re-expressed for one periodic domain."""
