@@ -42,13 +42,16 @@ typedef struct scale_hip_ctx scale_hip_ctx;
  * (scale_atmos_dyn_fvm_flux_ud1.F90, scale_atmos_dyn_fvm_flux_cd2.F90) */
 #define SCALE_HIP_UD1 1
 #define SCALE_HIP_CD2 2
+#define SCALE_HIP_UD5 5 /* tracer only; needs IHALO = JHALO = 3 and KMAX >= 5 */
+#define SCALE_HIP_CD6 6 /* tracer only; needs IHALO = JHALO = 3 and KMAX >= 5 */
 
 typedef struct {
   int32_t KA, IA, JA;             /* array extents                           */
   int32_t KS, KE, IS, IE, JS, JE; /* interior bounds, Fortran 1-based         */
-  int32_t KHALO, IHALO, JHALO;    /* must be 2 (filter_setup, COMMON.F90:137) */
+  int32_t KHALO, IHALO, JHALO;    /* KHALO = 2; IHALO = JHALO = 2, or 3 for the UD5/CD6 tracer
+                                      schemes (filter_setup needs at least 2, COMMON.F90:112) */
   int32_t flux_scheme;            /* dynamics: SCALE_HIP_UD3 or SCALE_HIP_CD4 */
-  int32_t flux_scheme_tracer;     /* tracers:  SCALE_HIP_UD3, _CD4, _UD3KOREN1993, _UD1 or _CD2 */
+  int32_t flux_scheme_tracer;     /* tracers:  SCALE_HIP_UD3, _CD4, _UD3KOREN1993, _UD1, _CD2, _UD5 or _CD6 */
   int32_t device;                 /* HIP device ordinal                       */
 } scale_hip_grid;
 

@@ -17,7 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-UD1, CD2, UD3, CD4 = 1, 2, 3, 4
+UD1, CD2, UD3, CD4, UD5, CD6 = 1, 2, 3, 4, 5, 6
 
 _dp = C.POINTER(C.c_double)
 
@@ -64,6 +64,8 @@ def lib():
         _lib.orc_mass_sum.restype = C.c_double
         _lib.orc_face_value.restype = C.c_double
         _lib.orc_face_value.argtypes = [C.c_int, C.c_int] + [C.c_double] * 5
+        _lib.orc_face_value6.restype = C.c_double
+        _lib.orc_face_value6.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.c_double]
         _lib.orc_get_threads.restype = C.c_int
     return _lib
 
@@ -189,6 +191,13 @@ def tstep_tracer(case, QTRCo, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_dif
 
 def face_value(scheme, kind, qm1, q0, q1, q2, vel) -> float:
     return lib().orc_face_value(scheme, kind, qm1, q0, q1, q2, vel)
+
+
+def face_value6(scheme, kind, q, vel) -> float:
+    """UD5/CD6 face values: kind 0 interior (q = val(k-2..k+3)), 1 face KS and
+    3 face KS+1 (q = val(KS..KS+5)), 2 face KE-1 and 4 face KE-2 (q = val(KE-5..KE))."""
+    qa = (C.c_double * 6)(*[float(x) for x in q])
+    return lib().orc_face_value6(scheme, kind, qa, vel)
 
 
 def get_fact_fct(rw, ru, rv) -> np.ndarray:

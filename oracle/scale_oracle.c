@@ -53,6 +53,15 @@ static const double F32 = 7.0 / 12.0;
 static const double F33 = 3.0 / 12.0;
 static const double F41 = 7.0 / 12.0;
 static const double F42 = -1.0 / 12.0;
+/* ud5.F90:92-96, cd6.F90:91-93 */
+static const double F51 = 1.0 / 60.0;
+static const double F52 = -8.0 / 60.0;
+static const double F53 = 37.0 / 60.0;
+static const double F54 = -5.0 / 60.0;
+static const double F55 = 10.0 / 60.0;
+static const double F61 = 37.0 / 60.0;
+static const double F62 = -8.0 / 60.0;
+static const double F63 = 1.0 / 60.0;
 
 #define GRID_LOCALS                                                        \
   const int KA = g->KA, IA = g->IA, JA = g->JA;                            \
@@ -130,6 +139,51 @@ double orc_face_value(int s, int kind, double qm1, double q0, double q1, double 
   return fv_int(s, qm1, q0, q1, q2, vel);
 }
 
+/* ---- six-point faces of UD5 and CD6 ---------------------------------------- */
+/* interior face between q0 = val(idx) and q1 = val(idx+1) (ud5.F90:236-242, cd6.F90:195-199) */
+static inline double fv6_int(int s, double qm2, double qm1, double q0, double q1, double q2, double q3, double vel) {
+  if (s == ORC_UD5)
+    return (F51 * (q3 + qm2) + F52 * (q2 + qm1) + F53 * (q1 + q0))
+         - (F51 * (q3 - qm2) + F54 * (q2 - qm1) + F55 * (q1 - q0)) * fsign(1.0, vel);
+  return F61 * (q1 + q0) + F62 * (q2 + qm1) + F63 * (q3 + qm2);
+}
+/* face KS from v = val(KS..KS+2) (ud5.F90:270-275, cd6.F90:217-220) */
+static inline double fv6_ks(int s, const double *v, double vel) {
+  if (s == ORC_UD5)
+    return F2 * (v[1] + v[0]) * (0.5 + fsign(0.5, vel)) + (2.0 * v[0] + 5.0 * v[1] - v[2]) / 6.0 * (0.5 - fsign(0.5, vel));
+  return F2 * (v[1] + v[0]);
+}
+/* face KE-1 from v = val(KE-2..KE) (ud5.F90:276-281, cd6.F90:221-224) */
+static inline double fv6_ke1(int s, const double *v, double vel) {
+  if (s == ORC_UD5)
+    return (2.0 * v[2] + 5.0 * v[1] - v[0]) / 6.0 * (0.5 + fsign(0.5, vel)) + F2 * (v[2] + v[1]) * (0.5 - fsign(0.5, vel));
+  return F2 * (v[2] + v[1]);
+}
+/* face KS+1 from v = val(KS..KS+4) (ud5.F90:283-292, cd6.F90:225-229) */
+static inline double fv6_ks1(int s, const double *v, double vel) {
+  if (s == ORC_UD5)
+    return (2.0 * v[2] + 5.0 * v[1] - v[0]) / 6.0 * (0.5 + fsign(0.5, vel))
+         + (-3.0 * v[0] + 27.0 * v[1] + 47.0 * v[2] - 13.0 * v[3] + 2.0 * v[4]) / 60.0 * (0.5 - fsign(0.5, vel));
+  return F41 * (v[2] + v[1]) + F42 * (v[3] + v[0]);
+}
+/* face KE-2 from v = val(KE-4..KE) (ud5.F90:293-302, cd6.F90:230-234) */
+static inline double fv6_ke2(int s, const double *v, double vel) {
+  if (s == ORC_UD5)
+    return (-3.0 * v[4] + 27.0 * v[3] + 47.0 * v[2] - 13.0 * v[1] + 2.0 * v[0]) / 60.0 * (0.5 + fsign(0.5, vel))
+         + (2.0 * v[2] + 5.0 * v[3] - v[4]) / 6.0 * (0.5 - fsign(0.5, vel));
+  return F41 * (v[3] + v[2]) + F42 * (v[4] + v[1]);
+}
+double orc_face_value6(int s, int kind, const double *q, double vel) {
+  switch (kind) {
+    case 1: return fv6_ks(s, q, vel);
+    case 2: return fv6_ke1(s, q + 3, vel);
+    case 3: return fv6_ks1(s, q, vel);
+    case 4: return fv6_ke2(s, q + 1, vel);
+    default: return fv6_int(s, q[0], q[1], q[2], q[3], q[4], q[5], vel);
+  }
+}
+static inline int six_point(int s) { return s == ORC_UD5 || s == ORC_CD6; }
+
 /* F2H with the flux routine's own GSQRT argument (UD3.F90:75) */
 #define F2H(G, k, p, i, j) (CDZ[(k) + (p)-1] * (G)[I3((k) + (p)-1, i, j)] / (CDZ[k] * (G)[I3(k, i, j)] + CDZ[(k) + 1] * (G)[I3((k) + 1, i, j)]))
 
@@ -157,6 +211,24 @@ static void fluxZ_XYZ(const orc_grid *g, int s, double *flux, const double *mflx
         }
         flux[I3(KS - 1, i, j)] = 0.0;
         flux[I3(KE, i, j)] = 0.0;
+      }
+    return;
+  }
+  if (six_point(s)) { /* ud5.F90:194-321, cd6.F90:170-276: interior, then KS, KE-1, KS+1, KE-2 in this order */
+    OMP_FOR
+    for (int j = JJS; j <= JJE; ++j)
+      for (int i = IIS; i <= IIE; ++i) {
+        const double *v = val + I3(0, i, j);
+        double *f = flux + I3(0, i, j);
+        const double *m = mflx + I3(0, i, j), *gg = G + I3(0, i, j), *n = nd + I3(0, i, j);
+        for (int k = KS + 2; k <= KE - 3; ++k)
+          f[k] = m[k] * fv6_int(s, v[k - 2], v[k - 1], v[k], v[k + 1], v[k + 2], v[k + 3], m[k]) + gg[k] * n[k];
+        f[KS - 1] = 0.0;
+        f[KS] = m[KS] * fv6_ks(s, v + KS, m[KS]) + gg[KS] * n[KS];
+        f[KE - 1] = m[KE - 1] * fv6_ke1(s, v + KE - 2, m[KE - 1]) + gg[KE - 1] * n[KE - 1];
+        f[KS + 1] = m[KS + 1] * fv6_ks1(s, v + KS, m[KS + 1]) + gg[KS + 1] * n[KS + 1];
+        f[KE - 2] = m[KE - 2] * fv6_ke2(s, v + KE - 4, m[KE - 2]) + gg[KE - 2] * n[KE - 2];
+        f[KE] = 0.0;
       }
     return;
   }
@@ -190,6 +262,10 @@ static void fluxX_XYZ(const orc_grid *g, int s, double *flux, const double *mflx
         double vel = mflx[I3(k, i, j)];
         if (s == ORC_UD1)
           flux[I3(k, i, j)] = vel * fv_int(s, 0.0, val[I3(k, i, j)], val[I3(k, i + 1, j)], 0.0, vel);
+        else if (six_point(s)) /* ud5.F90:348-372, cd6.F90:290-310 */
+          flux[I3(k, i, j)] = vel * fv6_int(s, val[I3(k, i - 2, j)], val[I3(k, i - 1, j)], val[I3(k, i, j)],
+                                            val[I3(k, i + 1, j)], val[I3(k, i + 2, j)], val[I3(k, i + 3, j)], vel)
+                            + G[I3(k, i, j)] * nd[I3(k, i, j)];
         else
           flux[I3(k, i, j)] = vel * fv_int(s, val[I3(k, i - 1, j)], val[I3(k, i, j)], val[I3(k, i + 1, j)], val[I3(k, i + 2, j)], vel)
                             + G[I3(k, i, j)] * nd[I3(k, i, j)];
@@ -207,6 +283,10 @@ static void fluxY_XYZ(const orc_grid *g, int s, double *flux, const double *mflx
         double vel = mflx[I3(k, i, j)];
         if (s == ORC_UD1)
           flux[I3(k, i, j)] = vel * fv_int(s, 0.0, val[I3(k, i, j)], val[I3(k, i, j + 1)], 0.0, vel);
+        else if (six_point(s)) /* ud5.F90:408-432, cd6.F90:347-367 */
+          flux[I3(k, i, j)] = vel * fv6_int(s, val[I3(k, i, j - 2)], val[I3(k, i, j - 1)], val[I3(k, i, j)],
+                                            val[I3(k, i, j + 1)], val[I3(k, i, j + 2)], val[I3(k, i, j + 3)], vel)
+                            + G[I3(k, i, j)] * nd[I3(k, i, j)];
         else
           flux[I3(k, i, j)] = vel * fv_int(s, val[I3(k, i, j - 1)], val[I3(k, i, j)], val[I3(k, i, j + 1)], val[I3(k, i, j + 2)], vel)
                             + G[I3(k, i, j)] * nd[I3(k, i, j)];
@@ -1922,7 +2002,7 @@ int orc_tstep_tracer_heve(const orc_grid *g, int s,
     double dtl, int flag_fct, int flag_fct_along_stream) {
   GRID_LOCALS;
   (void)CDZ;
-  if (s != ORC_UD3 && s != ORC_CD4 && s != ORC_UD3KOREN && s != ORC_UD1 && s != ORC_CD2) return -1;
+  if (s != ORC_UD3 && s != ORC_CD4 && s != ORC_UD3KOREN && s != ORC_UD1 && s != ORC_CD2 && s != ORC_UD5 && s != ORC_CD6) return -1;
   double *qflx_hi = alloc_nan(3 * N3), *qflx_lo = alloc_nan(3 * N3), *qflx_anti = NULL;
   const double *mZ = mflx_hi, *mX = mflx_hi + N3, *mY = mflx_hi + 2 * N3;
   const double *GXYW = GSQRT + N3 * I_XYW, *GUYZ = GSQRT + N3 * I_UYZ, *GXVZ = GSQRT + N3 * I_XVZ, *GXYZ = GSQRT + N3 * I_XYZ;

@@ -39,6 +39,8 @@ extern "C" {
 #define ORC_CD2 2 /* tracer scheme: second-order central (cd2.F90) */
 #define ORC_UD3 3
 #define ORC_CD4 4
+#define ORC_UD5 5 /* tracer scheme: fifth-order upwind (ud5.F90), needs IHALO = JHALO = 3 */
+#define ORC_CD6 6 /* tracer scheme: sixth-order central (cd6.F90), needs IHALO = JHALO = 3 */
 #define ORC_UD3KOREN 31 /* UD3KOREN1993: tracer scheme, UD3 with the Koren limiter */
 
 typedef struct {
@@ -195,6 +197,10 @@ double orc_mass_sum(const orc_grid *g, const orc_metric *m, const double *a);
 /* face value of the flux families: kind 0 interior (val(idx-1..idx+2)),
  * 1 lowest face (val(KS..KS+2) in q0..q2), 2 highest face (val(KE-2..KE) in qm1..q1) */
 double orc_face_value(int scheme, int kind, double qm1, double q0, double q1, double q2, double vel);
+/* six-point face values of UD5/CD6: kind 0 interior face between q[2] and q[3]
+ * (q = val(k-2..k+3)); kind 1 the face KS, 3 the face KS+1 (q = val(KS..KS+5));
+ * kind 2 the face KE-1, 4 the face KE-2 (q = val(KE-5..KE)). */
+double orc_face_value6(int s, int kind, const double *q, double vel);
 
 /* ---- adjoint checker (scale_oracle_adj.c) -------------------------------- */
 

@@ -287,18 +287,24 @@ const char* scale_hip_version(void) { return "scale-hip 0.1 (gfx950)"; }
 int scale_hip_init(const scale_hip_grid* g, scale_hip_ctx** out) {
   if (!g || !out) return fail("null argument");
   *out = nullptr;
-  if (g->KHALO != 2 || g->IHALO != 2 || g->JHALO != 2) return fail("KHALO, IHALO and JHALO must be 2");
+  // KHALO = 2; IHALO = JHALO = 2, or 3 (needed by the UD5/CD6 tracer schemes)
+  if (g->KHALO != 2 || g->IHALO != g->JHALO || (g->IHALO != 2 && g->IHALO != 3))
+    return fail("KHALO must be 2 and IHALO = JHALO must be 2 or 3");
+  const int H = g->IHALO;
   const int KS = g->KS - 1, KE = g->KE - 1, IS = g->IS - 1, IE = g->IE - 1, JS = g->JS - 1, JE = g->JE - 1;
-  if (KS != 2 || IS != 2 || JS != 2 || KE + 3 != g->KA || IE + 3 != g->IA || JE + 3 != g->JA)
-    return fail("grid bounds must be KS=IS=JS=3 and KA=KE+2, IA=IE+2, JA=JE+2 (halo 2)");
+  if (KS != 2 || IS != H || JS != H || KE + 3 != g->KA || IE + H + 1 != g->IA || JE + H + 1 != g->JA)
+    return fail("grid bounds must be KS=3, IS=JS=IHALO+1 and KA=KE+2, IA=IE+IHALO, JA=JE+IHALO");
   if (KE - KS + 1 < 4) return fail("KMAX must be >= 4");
-  if (IE - IS + 1 < 2 || JE - JS + 1 < 2) return fail("IMAX and JMAX must be >= 2");
+  if (IE - IS + 1 < H || JE - JS + 1 < H) return fail("IMAX and JMAX must be >= IHALO");
   if (g->flux_scheme != SCALE_HIP_UD3 && g->flux_scheme != SCALE_HIP_CD4)
     return fail("the dynamics flux scheme must be UD3 (3) or CD4 (4)");
   if (g->flux_scheme_tracer != SCALE_HIP_UD3 && g->flux_scheme_tracer != SCALE_HIP_CD4 &&
       g->flux_scheme_tracer != SCALE_HIP_UD3KOREN1993 && g->flux_scheme_tracer != SCALE_HIP_UD1 &&
-      g->flux_scheme_tracer != SCALE_HIP_CD2)
-    return fail("the tracer flux scheme must be UD3 (3), CD4 (4), UD3KOREN1993 (31), UD1 (1) or CD2 (2)");
+      g->flux_scheme_tracer != SCALE_HIP_CD2 && g->flux_scheme_tracer != SCALE_HIP_UD5 &&
+      g->flux_scheme_tracer != SCALE_HIP_CD6)
+    return fail("the tracer flux scheme must be UD3 (3), CD4 (4), UD3KOREN1993 (31), UD1 (1), CD2 (2), UD5 (5) or CD6 (6)");
+  if ((g->flux_scheme_tracer == SCALE_HIP_UD5 || g->flux_scheme_tracer == SCALE_HIP_CD6) && (H != 3 || KE - KS + 1 < 5))
+    return fail("the UD5 and CD6 tracer schemes need IHALO = JHALO = 3 and KMAX >= 5");
   if (hipSetDevice(g->device) != hipSuccess) return fail("hipSetDevice failed");
   scale_hip_ctx* h = new scale_hip_ctx();
   Ctx& c = h->c;
@@ -496,6 +502,7 @@ int scale_hip_tstep_short_hevi_adj(scale_hip_ctx* h,
   if (!dtrk || !J33G) return fail("null scalar argument");
   Ctx& c = h->c;
   if (c.scheme != shp::UD3 && c.scheme != shp::CD4) return fail("the adjoint supports the UD3 and CD4 dynamics schemes");
+  if (c.d.IS != 2) return fail("the adjoint supports IHALO = 2 only");
   if (int r = ensure_dropin(h)) return r;
   if (int r = set_grid1d(c, CDZ, FDZ, RCDZ, RFDZ, nullptr, FDX, RCDX, RFDX, nullptr, FDY, RCDY, RFDY)) return r;
   if (int r = set_metric3d(c, PHI, GSQRT, J13G, J23G, *J33G, MAPF)) return r;
@@ -666,6 +673,7 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   if (cfg->divdmp_coef > 0.0 && cfg->mixed_precision) return fail("divergence damping is not supported in mixed precision");
   if (!(cfg->dtl > 0.0) || !(cfg->dts > 0.0)) return fail("dtl and dts must be positive");
   if (QA < 0) return fail("QA must be >= 0");
+  if (cfg->mixed_precision && c.d.IS != 2) return fail("mixed precision supports IHALO = 2 only");
   if (cfg->mixed_precision && c.terrain) return fail("mixed precision needs flat terrain (GSQRT = 1, J13G = J23G = 0, MAPF = 1)");
   if (cfg->mixed_precision && QA > 0) return fail("mixed precision integrates the dynamics only (QA = 0)");
   if (cfg->mixed_precision && c.d.KA > shp::KA_DEV_MAX) return fail("mixed precision supports at most 512 levels (KA)");

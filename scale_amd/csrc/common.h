@@ -40,7 +40,16 @@ constexpr double F33 = 3.0 / 12.0;
 constexpr double F41 = 7.0 / 12.0;
 constexpr double F42 = -1.0 / 12.0;
 
-constexpr int UD1 = 1, CD2 = 2, UD3 = 3, CD4 = 4;
+// scale_atmos_dyn_fvm_flux_ud5.F90:92-96, scale_atmos_dyn_fvm_flux_cd6.F90:91-93
+constexpr double F51 = 1.0 / 60.0;
+constexpr double F52 = -8.0 / 60.0;
+constexpr double F53 = 37.0 / 60.0;
+constexpr double F54 = -5.0 / 60.0;
+constexpr double F55 = 10.0 / 60.0;
+constexpr double F61 = 37.0 / 60.0;
+constexpr double F62 = -8.0 / 60.0;
+constexpr double F63 = 1.0 / 60.0;
+constexpr int UD1 = 1, CD2 = 2, UD3 = 3, CD4 = 4, UD5 = 5, CD6 = 6;
 constexpr int UD3KOREN = 31;  // UD3 with the Koren (1993) limiter (tracer scheme UD3KOREN1993)
 
 struct Dims {
@@ -119,6 +128,50 @@ __device__ __forceinline__ T fv_hi(T qm1, T q0, T q1, T vel) {
     return fv_int<UD1, T>(T(0), q0, q1, T(0), vel);
   else
     return T(F2) * (q1 + q0);
+}
+
+// Six-point faces of the UD5 and CD6 tracer schemes (the lateral halo is 3).
+// Interior face between q0 = val(idx) and q1 = val(idx+1)
+// (scale_atmos_dyn_fvm_flux_ud5.F90:236-242, scale_atmos_dyn_fvm_flux_cd6.F90:195-199)
+template <int S>
+__device__ __forceinline__ double fv6_int(double qm2, double qm1, double q0, double q1, double q2, double q3, double vel) {
+  if constexpr (S == UD5)
+    return (F51 * (q3 + qm2) + F52 * (q2 + qm1) + F53 * (q1 + q0))
+         - (F51 * (q3 - qm2) + F54 * (q2 - qm1) + F55 * (q1 - q0)) * fsign(1.0, vel);
+  else
+    return F61 * (q1 + q0) + F62 * (q2 + qm1) + F63 * (q3 + qm2);
+}
+// the faces next to the walls; v holds val(KS..) for KS and KS+1, val(KE-4..KE)
+// for KE-2 and val(KE-2..KE) for KE-1 (ud5.F90:270-302, cd6.F90:217-234)
+template <int S>
+__device__ __forceinline__ double fv6_ks(const double* v, double vel) {
+  if constexpr (S == UD5)
+    return F2 * (v[1] + v[0]) * (0.5 + fsign(0.5, vel)) + (2.0 * v[0] + 5.0 * v[1] - v[2]) / 6.0 * (0.5 - fsign(0.5, vel));
+  else
+    return F2 * (v[1] + v[0]);
+}
+template <int S>
+__device__ __forceinline__ double fv6_ke1(const double* v, double vel) {
+  if constexpr (S == UD5)
+    return (2.0 * v[2] + 5.0 * v[1] - v[0]) / 6.0 * (0.5 + fsign(0.5, vel)) + F2 * (v[2] + v[1]) * (0.5 - fsign(0.5, vel));
+  else
+    return F2 * (v[2] + v[1]);
+}
+template <int S>
+__device__ __forceinline__ double fv6_ks1(const double* v, double vel) {
+  if constexpr (S == UD5)
+    return (2.0 * v[2] + 5.0 * v[1] - v[0]) / 6.0 * (0.5 + fsign(0.5, vel))
+         + (-3.0 * v[0] + 27.0 * v[1] + 47.0 * v[2] - 13.0 * v[3] + 2.0 * v[4]) / 60.0 * (0.5 - fsign(0.5, vel));
+  else
+    return F41 * (v[2] + v[1]) + F42 * (v[3] + v[0]);
+}
+template <int S>
+__device__ __forceinline__ double fv6_ke2(const double* v, double vel) {
+  if constexpr (S == UD5)
+    return (-3.0 * v[4] + 27.0 * v[3] + 47.0 * v[2] - 13.0 * v[1] + 2.0 * v[0]) / 60.0 * (0.5 + fsign(0.5, vel))
+         + (2.0 * v[2] + 5.0 * v[3] - v[4]) / 6.0 * (0.5 - fsign(0.5, vel));
+  else
+    return F41 * (v[3] + v[2]) + F42 * (v[4] + v[1]);
 }
 
 // Metric accessors.  TERRAIN=false is the flat Cartesian grid (GSQRT=1,

@@ -58,6 +58,19 @@ struct TrOps {
     const double* Q = a.QTRC;
     size_t p = ix(kk, i, j);
     double vel = mZ(p), fv;
+    if constexpr (S == UD5 || S == CD6) {
+      // levels are d.pl apart in this layout; v[] gathers the column values a face form reads.
+      // The reference assigns the interior, then KS, KE-1, KS+1 and KE-2: the last one wins.
+      const size_t sk = d.pl;
+      double v[5];
+      auto col = [&](int k0, int n) { for (int m = 0; m < n; ++m) v[m] = Q[ix(k0 + m, i, j)]; };
+      if (kk == d.KE - 2) { col(d.KE - 4, 5); fv = fv6_ke2<S>(v, vel); }
+      else if (kk == d.KS + 1) { col(d.KS, 5); fv = fv6_ks1<S>(v, vel); }
+      else if (kk == d.KE - 1) { col(d.KE - 2, 3); fv = fv6_ke1<S>(v, vel); }
+      else if (kk == d.KS) { col(d.KS, 3); fv = fv6_ks<S>(v, vel); }
+      else fv = fv6_int<S>(Q[p - 2 * sk], Q[p - sk], Q[p], Q[p + sk], Q[p + 2 * sk], Q[p + 3 * sk], vel);
+      return vel * fv + gs(I_XYW, p) * a.nd[p];
+    }
     if (kk == d.KS) fv = fv_lo<S>(Q[p], Q[ix(kk + 1, i, j)], Q[ix(kk + 2, i, j)], vel);
     else if (kk == d.KE - 1) fv = fv_hi<S>(Q[ix(kk - 1, i, j)], Q[p], Q[ix(kk + 1, i, j)], vel);
     else fv = fv_int<S>(Q[ix(kk - 1, i, j)], Q[p], Q[ix(kk + 1, i, j)], Q[ix(kk + 2, i, j)], vel);
@@ -69,6 +82,9 @@ struct TrOps {
     size_t p = ix(k, ii, j);
     double vel = mX(p);
     if constexpr (S == UD1) return vel * fv_int<UD1>(0.0, Q[p], Q[ix(k, ii + 1, j)], 0.0, vel);
+    if constexpr (S == UD5 || S == CD6)
+      return vel * fv6_int<S>(Q[ix(k, ii - 2, j)], Q[ix(k, ii - 1, j)], Q[p], Q[ix(k, ii + 1, j)], Q[ix(k, ii + 2, j)],
+                              Q[ix(k, ii + 3, j)], vel) + gs(I_UYZ, p) * a.nd[p + a.nds];
     return vel * fv_int<S>(Q[ix(k, ii - 1, j)], Q[p], Q[ix(k, ii + 1, j)], Q[ix(k, ii + 2, j)], vel) + gs(I_UYZ, p) * a.nd[p + a.nds];
   }
   __device__ double hY(int k, int i, int jj) const {
@@ -76,6 +92,9 @@ struct TrOps {
     size_t p = ix(k, i, jj);
     double vel = mY(p);
     if constexpr (S == UD1) return vel * fv_int<UD1>(0.0, Q[p], Q[ix(k, i, jj + 1)], 0.0, vel);
+    if constexpr (S == UD5 || S == CD6)
+      return vel * fv6_int<S>(Q[ix(k, i, jj - 2)], Q[ix(k, i, jj - 1)], Q[p], Q[ix(k, i, jj + 1)], Q[ix(k, i, jj + 2)],
+                              Q[ix(k, i, jj + 3)], vel) + gs(I_XVZ, p) * a.nd[p + 2 * a.nds];
     return vel * fv_int<S>(Q[ix(k, i, jj - 1)], Q[p], Q[ix(k, i, jj + 1)], Q[ix(k, i, jj + 2)], vel) + gs(I_XVZ, p) * a.nd[p + 2 * a.nds];
   }
   // low-order UD1 fluxes on QTRC0 (scale_atmos_dyn_fvm_flux_ud1.F90:135-249)
@@ -374,6 +393,8 @@ hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s) {
     return c.terrain ? tracer_t<UD3KOREN, true>(c, a, fct, along, s) : tracer_t<UD3KOREN, false>(c, a, fct, along, s);
   if (c.scheme_tracer == CD2) return c.terrain ? tracer_t<CD2, true>(c, a, fct, along, s) : tracer_t<CD2, false>(c, a, fct, along, s);
   if (c.scheme_tracer == UD1) return c.terrain ? tracer_t<UD1, true>(c, a, fct, along, s) : tracer_t<UD1, false>(c, a, fct, along, s);
+  if (c.scheme_tracer == UD5) return c.terrain ? tracer_t<UD5, true>(c, a, fct, along, s) : tracer_t<UD5, false>(c, a, fct, along, s);
+  if (c.scheme_tracer == CD6) return c.terrain ? tracer_t<CD6, true>(c, a, fct, along, s) : tracer_t<CD6, false>(c, a, fct, along, s);
   return c.terrain ? tracer_t<CD4, true>(c, a, fct, along, s) : tracer_t<CD4, false>(c, a, fct, along, s);
 }
 

@@ -134,7 +134,7 @@ contains
     integer, intent(in) :: flux_scheme, flux_scheme_tracer, device
     type(scale_hip_grid) :: g
     KA = KA_in; IA = IA_in; JA = JA_in
-    g = scale_hip_grid(KA, IA, JA, KS, KE, IS, IE, JS, JE, 2, 2, 2, flux_scheme, flux_scheme_tracer, device)
+    g = scale_hip_grid(KA, IA, JA, KS, KE, IS, IE, JS, JE, KS - 1, IS - 1, JS - 1, flux_scheme, flux_scheme_tracer, device)
     call check(c_init(g, ctx), "ATMOS_DYN_hip_setup")
   end subroutine ATMOS_DYN_hip_setup
 

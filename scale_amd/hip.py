@@ -29,7 +29,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 UD3, CD4 = 3, 4
-UD3KOREN1993, UD1, CD2 = 31, 1, 2  # tracer schemes
+UD3KOREN1993, UD1, CD2, UD5, CD6 = 31, 1, 2, 5, 6  # tracer schemes (UD5 and CD6 need a lateral halo of 3)
 FIELD = dict(DENS=0, MOMZ=1, MOMX=2, MOMY=3, RHOT=4, REF_dens=10, REF_pott=11, REF_pres=12, CORIOLI=13,
              wdamp_coef=14, mflx_hi=20, tflx_hi=21, DENS_tp=30, MOMZ_tp=31, MOMX_tp=32, MOMY_tp=33, RHOT_tp=34)
 QTRC_BASE, RHOQ_TP_BASE = 100, 200
@@ -95,7 +95,7 @@ def ATMOS_DYN_wdamp_setup(grid, wdamp_tau, wdamp_height, FZ):
     """Rayleigh damping coefficient wdamp_coef(KA) from FZ(0:KA)
     (scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:323-381); host only."""
     g = HipGrid(grid.KA, grid.IA, grid.JA, grid.KS + 1, grid.KE + 1, grid.IS + 1, grid.IE + 1, grid.JS + 1,
-                grid.JE + 1, 2, 2, 2, UD3, UD3, 0)
+                grid.JE + 1, grid.KS, grid.IS, grid.JS, UD3, UD3, 0)
     out = np.zeros(grid.KA)
     fz = np.ascontiguousarray(FZ, dtype=np.float64)
     _check(load_library().scale_hip_wdamp_setup(C.byref(g), _p(out), _d(wdamp_tau), _d(wdamp_height), _p(fz)))
@@ -108,7 +108,7 @@ class Context:
     def __init__(self, grid, scheme=UD3, scheme_tracer=UD3, device=0):
         L = load_library()
         g = HipGrid(grid.KA, grid.IA, grid.JA, grid.KS + 1, grid.KE + 1, grid.IS + 1, grid.IE + 1, grid.JS + 1,
-                    grid.JE + 1, 2, 2, 2, scheme, scheme_tracer, device)
+                    grid.JE + 1, grid.KS, grid.IS, grid.JS, scheme, scheme_tracer, device)
         h = C.c_void_p()
         _check(L.scale_hip_init(C.byref(g), C.byref(h)))
         self._h = h

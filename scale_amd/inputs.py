@@ -41,6 +41,7 @@ class Grid:
     dx: float
     dy: float
     dz: np.ndarray  # (kmax,) layer thicknesses, bottom to top
+    hx: int = HALO  # lateral halo width (IHALO = JHALO): 2, or 3 for the UD5/CD6 tracer schemes
 
     @property
     def KA(self):
@@ -48,11 +49,11 @@ class Grid:
 
     @property
     def IA(self):
-        return self.imax + 2 * HALO
+        return self.imax + 2 * self.hx
 
     @property
     def JA(self):
-        return self.jmax + 2 * HALO
+        return self.jmax + 2 * self.hx
 
     @property
     def KS(self):
@@ -64,19 +65,19 @@ class Grid:
 
     @property
     def IS(self):
-        return HALO
+        return self.hx
 
     @property
     def IE(self):
-        return HALO + self.imax - 1
+        return self.hx + self.imax - 1
 
     @property
     def JS(self):
-        return HALO
+        return self.hx
 
     @property
     def JE(self):
-        return HALO + self.jmax - 1
+        return self.hx + self.jmax - 1
 
     @property
     def shape(self):
@@ -88,11 +89,11 @@ class Grid:
 
 def halo_fill(g: Grid, a: np.ndarray) -> np.ndarray:
     """Periodic lateral halo (one-rank COMM_vars8) in place; all k levels."""
-    IS, IE, JS, JE = g.IS, g.IE, g.JS, g.JE
-    a[:, :IS, JS:JE + 1] = a[:, IE - HALO + 1:IE + 1, JS:JE + 1]
-    a[:, IE + 1:, JS:JE + 1] = a[:, IS:IS + HALO, JS:JE + 1]
-    a[:, :, :JS] = a[:, :, JE - HALO + 1:JE + 1]
-    a[:, :, JE + 1:] = a[:, :, JS:JS + HALO]
+    IS, IE, JS, JE, h = g.IS, g.IE, g.JS, g.JE, g.hx
+    a[:, :IS, JS:JE + 1] = a[:, IE - h + 1:IE + 1, JS:JE + 1]
+    a[:, IE + 1:, JS:JE + 1] = a[:, IS:IS + h, JS:JE + 1]
+    a[:, :, :JS] = a[:, :, JE - h + 1:JE + 1]
+    a[:, :, JE + 1:] = a[:, :, JS:JS + h]
     return a
 
 
@@ -179,7 +180,7 @@ def make_case(kmax=24, imax=16, jmax=16, dx=500.0, dy=500.0, dz=250.0, *, seed=2
               bv=0.01, pott_sfc=300.0, u_profile="uniform", u0=10.0, v0=0.0, shear_depth=10000.0,
               bubble=True, bubble_dtheta=3.0, bubble_r=2000.0, bubble_zc=2000.0,
               noise=0.01, ntracer=1, tracer_kind="zero", dt=1.0, scheme=3, scheme_tracer=3,
-              fct=True, stretched=None, momz_noise=0.0) -> Case:
+              fct=True, stretched=None, momz_noise=0.0, halo=HALO) -> Case:
     """Build one synthetic workload.
 
     ``stretched=(dz_bottom, dz_top)`` makes layer thickness grow linearly
@@ -190,7 +191,7 @@ def make_case(kmax=24, imax=16, jmax=16, dx=500.0, dy=500.0, dz=250.0, *, seed=2
         dzs = np.array([dzb + (dzt - dzb) * kk / max(kmax - 1, 1) for kk in range(kmax)])
     else:
         dzs = np.full(kmax, float(dz))
-    g = Grid(kmax, imax, jmax, float(dx), float(dy), dzs)
+    g = Grid(kmax, imax, jmax, float(dx), float(dy), dzs, hx=halo)
     KA, IA, JA = g.shape
     KS, KE, IS, IE, JS, JE = g.KS, g.KE, g.IS, g.IE, g.JS, g.JE
     fz, cz, cdz, fdz, rcdz, rfdz = vertical_grid(g)

@@ -46,6 +46,14 @@ def test_init_rejects_bad_grids():
     g = hip.HipGrid(20, 14, 14, 3, 18, 3, 12, 3, 12, 3, 3, 3, 3, 3, 0)
     assert lib.scale_hip_init(C.byref(g), C.byref(h)) != 0
     assert b"HALO" in lib.scale_hip_last_error()
+    # UD5/CD6 tracers need a lateral halo of 3 and KMAX >= 5
+    for s in (5, 6):
+        g = hip.HipGrid(20, 14, 14, 3, 18, 3, 12, 3, 12, 2, 2, 2, 3, s, 0)
+        assert lib.scale_hip_init(C.byref(g), C.byref(h)) != 0
+        assert b"UD5" in lib.scale_hip_last_error()
+        g = hip.HipGrid(8, 16, 16, 3, 6, 4, 13, 4, 13, 2, 3, 3, 3, s, 0)  # KMAX = 4
+        assert lib.scale_hip_init(C.byref(g), C.byref(h)) != 0
+        assert b"KMAX >= 5" in lib.scale_hip_last_error()
     # unsupported flux scheme
     g = hip.HipGrid(20, 14, 14, 3, 18, 3, 12, 3, 12, 2, 2, 2, 5, 3, 0)
     assert lib.scale_hip_init(C.byref(g), C.byref(h)) != 0

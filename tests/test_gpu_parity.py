@@ -266,10 +266,11 @@ def test_model_parity_tracer_filter(hip):
 
 
 @pytest.mark.parametrize("fct,along", [(False, True), (True, True), (True, False)], ids=["nofct", "fct_along", "fct_7pt"])
-@pytest.mark.parametrize("scheme", [3, 4, 31, 1, 2], ids=["UD3", "CD4", "UD3KOREN1993", "UD1", "CD2"])
-def test_tracer_bitwise(hip, fct, along, scheme):
-    print(f"[tracer] fct={fct} along={along} scheme={scheme}", flush=True)
-    case = H.small_case(seed=55)
+@pytest.mark.parametrize("scheme,halo", [(3, 2), (4, 2), (31, 2), (1, 2), (2, 2), (5, 3), (6, 3), (3, 3)],
+                         ids=["UD3", "CD4", "UD3KOREN1993", "UD1", "CD2", "UD5_halo3", "CD6_halo3", "UD3_halo3"])
+def test_tracer_bitwise(hip, fct, along, scheme, halo):
+    print(f"[tracer] fct={fct} along={along} scheme={scheme} halo={halo}", flush=True)
+    case = H.small_case(seed=55, halo=halo)
     g = case.grid
     prep = inputs.large_step_prep(case)
     del prep
@@ -292,6 +293,34 @@ def test_tracer_bitwise(hip, fct, along, scheme):
         ctx.ATMOS_DYN_Tstep_tracer_fvm_heve(got, q, q0, rq, case.state["DENS"], st["DENS"], mflx, nd, m["GSQRT"], mapf_xy,
                                             m["CDZ"], m["RCDZ"], m["RCDX"], m["RCDY"], 2.0, fct, along)
     H.compare(ref, got, "QTRCo")
+
+
+@pytest.mark.parametrize("scheme_tracer", [5, 6], ids=["UD5", "CD6"])
+def test_model_parity_halo3(hip, scheme_tracer):
+    """Large steps on a grid with a lateral halo of 3 (IHALO = JHALO = 3, as
+    the UD5/CD6 tracer schemes need) with FCT tracers and the tracer filter,
+    against the oracle: 5 steps, 1e-12 relative for the dynamics and tracers."""
+    print(f"[model] halo 3, tracer scheme {scheme_tracer}, 5 steps", flush=True)
+    case = inputs.make_case(kmax=20, imax=16, jmax=12, noise=0.01, ntracer=2, tracer_kind=["blob", "tophat"], fct=True,
+                            bubble_r=2000.0, bubble_zc=2000.0, momz_noise=0.05, v0=-2.0, scheme_tracer=scheme_tracer,
+                            halo=3)
+    g = case.grid
+    assert g.IS == 3
+    m = po.Model(case, nd_coef_q=1.0e-2)
+    m.step(5)
+    ctx = hip.model_from_case(case, nd_coef_q=1.0e-2)
+    try:
+        ctx.model_large_step(5)
+        ctx.synchronize()
+        sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+        fields = [(k, ctx.model_get(k), m.state[k]) for k in H.VARS]
+        fields += [(f"QTRC{iq}", ctx.model_get(hip.QTRC_BASE + iq), m.qtrc[..., iq]) for iq in range(2)]
+        for name, got, ref in fields:
+            d = np.abs(got[sl] - ref[sl]).max()
+            print(f"  {name}: max|diff| {d:.3e}", flush=True)
+            assert d <= 1.0e-12 * max(np.abs(ref[sl]).max(), 1e-30), name
+    finally:
+        ctx.close()
 
 
 @pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])

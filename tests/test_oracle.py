@@ -867,6 +867,88 @@ def test_tracer_cd2_ud1_fourier_closed_form(scheme, axis):
     assert np.abs(out[sl] - q[sl]).max() > 1e-6  # the wave moved
 
 
+# (scheme, kind, vel sign) -> highest polynomial degree reproduced exactly from
+# cell averages: the stencil's order of accuracy minus one
+_FACE6_DEGREE = {
+    (5, 0, 1): 4, (5, 0, -1): 4, (6, 0, 1): 5, (6, 0, -1): 5,      # 5-point upwind / 6-point central
+    (5, 1, 1): 1, (5, 1, -1): 2, (6, 1, 1): 1, (6, 1, -1): 1,      # KS: F2 / 3-point upwind
+    (5, 3, 1): 2, (5, 3, -1): 4, (6, 3, 1): 3, (6, 3, -1): 3,      # KS+1: 3- / 5-point upwind, CD4
+    (5, 2, 1): 2, (5, 2, -1): 1, (6, 2, 1): 1, (6, 2, -1): 1,      # KE-1
+    (5, 4, 1): 4, (5, 4, -1): 2, (6, 4, 1): 3, (6, 4, -1): 3,      # KE-2
+}
+
+
+@pytest.mark.parametrize("key", sorted(_FACE6_DEGREE), ids=lambda k: f"s{k[0]}_kind{k[1]}_v{k[2]:+d}")
+def test_face6_reproduces_polynomials_from_cell_averages(key):
+    """The UD5 and CD6 face values (ud5.F90:236-302, cd6.F90:195-234) are
+    finite-volume reconstructions: fed the cell averages of a polynomial on a
+    unit grid they return its value at the face exactly up to the stencil's
+    degree, and miss it one degree higher.  A wrong coefficient, a reversed
+    stencil or an upwind side taken from the wrong sign of the velocity breaks
+    one of these."""
+    s, kind, sgn = key
+    deg = _FACE6_DEGREE[key]
+    x = np.arange(6, dtype=float)                       # cell centres of q[0..5]
+    face = {0: 2.5, 1: 0.5, 3: 1.5, 2: 4.5, 4: 3.5}[kind]
+    x0 = 2.5 if kind == 0 else 0.0                      # centre the monomials on the face region
+
+    def cell_avg(p):
+        return (((x + 0.5 - x0) ** (p + 1)) - ((x - 0.5 - x0) ** (p + 1))) / (p + 1)
+
+    for p in range(deg + 2):
+        got = po.face_value6(s, kind, cell_avg(p), 1.7 * sgn)
+        exact = (face - x0) ** p
+        if p <= deg:
+            assert abs(got - exact) < 1e-12, (p, got, exact)
+        else:
+            assert abs(got - exact) > 1e-6, (p, got, exact)
+
+
+@pytest.mark.parametrize("scheme", [po.UD5, po.CD6], ids=["UD5", "CD6"])
+@pytest.mark.parametrize("fct", [False, True], ids=["nofct", "fct"])
+def test_tracer_ud5_cd6_conserve_mass_and_constants(scheme, fct):
+    """UD5/CD6 tracer steps on a grid with a lateral halo of 3: the flux form
+    conserves the tracer mass sum(DENS q) to round-off over a periodic domain
+    with closed top and bottom, and a constant field with a density that
+    follows the mass-flux divergence stays constant."""
+    case = H.small_case(kmax=12, imax=10, jmax=8, seed=7, halo=3)
+    g = case.grid
+    assert g.IS == 3 and g.IA == g.imax + 6
+    rng = np.random.Generator(np.random.PCG64(17))
+    mflx = np.asfortranarray(rng.standard_normal(g.shape + (3,)) * np.array([0.3, 8.0, 5.0]))
+    mflx[:g.KS, :, :, 0] = 0.0
+    mflx[g.KE:, :, :, 0] = 0.0
+    for n in range(3):
+        inputs.halo_fill(g, mflx[..., n])
+    q = np.asfortranarray(1.0 + 0.1 * rng.standard_normal(g.shape))
+    inputs.halo_fill(g, q)
+    nd = np.asfortranarray(1e-5 * rng.standard_normal(g.shape + (3,)))
+    for n in range(3):
+        inputs.halo_fill(g, nd[..., n])
+    D0 = case.state["DENS"]
+    dt = 2.0
+    # the density consistent with the fluxes: D = D0 - dt * div(mflx)
+    m = case.metric
+    D = D0.copy(order="F")
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    kz, ix, jy = (np.arange(g.KS, g.KE + 1)[:, None, None], np.arange(g.IS, g.IE + 1)[None, :, None],
+                  np.arange(g.JS, g.JE + 1)[None, None, :])
+    div = ((mflx[kz, ix, jy, 0] - mflx[kz - 1, ix, jy, 0]) * m["RCDZ"][kz]
+           + (mflx[kz, ix, jy, 1] - mflx[kz, ix - 1, jy, 1]) * m["RCDX"][ix]
+           + (mflx[kz, ix, jy, 2] - mflx[kz, ix, jy - 1, 2]) * m["RCDY"][jy])
+    D[sl] = D0[sl] - dt * div
+    inputs.halo_fill(g, D)
+    out = np.full(g.shape, np.nan, order="F")
+    po.tstep_tracer(case, out, q, q, g.zeros(), D0, D, mflx, nd, dt, fct, scheme=scheme)
+    vol = 1.0 / (m["RCDZ"][kz] * m["RCDX"][ix] * m["RCDY"][jy])
+    mass0 = (D0[sl] * q[sl] * vol).sum()
+    mass1 = (D[sl] * out[sl] * vol).sum()
+    assert abs(mass1 - mass0) < 1e-12 * abs(mass0)
+    one = np.asfortranarray(np.ones(g.shape))
+    po.tstep_tracer(case, out, one, one, g.zeros(), D0, D, mflx, np.zeros_like(nd), dt, fct, scheme=scheme)
+    assert np.abs(out[sl] - 1.0).max() < 1e-12
+
+
 def test_tracer_ud1_monotone_and_without_num_diff():
     """UD1 as the tracer scheme: with continuity-consistent fluxes a two-level
     field stays within its bounds without any limiter (the monotone scheme),
