@@ -85,6 +85,9 @@ def parse():
     p.add_argument("--no-copy-probe", action="store_true", help="skip the HBM copy probe")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP32 + FP64 solve) timing")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    p.add_argument("--tracer-scheme", type=int, default=None, choices=[1, 2, 3, 4, 5, 6, 31],
+                   help="also time large steps whose 3 FCT tracers use this scheme "
+                        "(UD1=1, CD2=2, UD3=3, CD4=4, UD5=5, CD6=6, UD3KOREN1993=31; 5 and 6 on a lateral halo of 3)")
     p.add_argument("--adjoint-dirs", type=int, default=64, help="finite-difference directions of the adjoint check")
     p.add_argument("--theta-check", type=int, default=0, metavar="N",
                    help="also run N short steps in FP64 and in mixed precision and report the potential-temperature difference")
@@ -271,6 +274,21 @@ def main():
         copy_ms = ctx.hbm_copy_probe(n_copy, 10)
         copy_gbs = 16.0 * n_copy / (copy_ms * 1e-3) / 1e9
     ctx.close()
+    # informational: large steps with the tracers on another flux scheme
+    tracer_leg = None
+    if a.tracer_scheme is not None:
+        ts = a.tracer_scheme
+        tcase = inputs.shear_512x512x128(imax=ni, jmax=nj, kmax=nk, scheme_tracer=ts, halo=3 if ts in (5, 6) else 2)
+        tctx = hip.model_from_case(tcase)
+        tctx.model_prepare()
+        tctx.model_large_step(1)
+        tctx.synchronize()
+        t3 = time.perf_counter()
+        tctx.model_large_step(2)
+        tctx.synchronize()
+        tracer_leg = {"scheme": ts, "halo": tcase.grid.IS, "large_step_ms": (time.perf_counter() - t3) * 1e3 / 2}
+        tctx.close()
+        print(f"tracer scheme {ts}: large step {tracer_leg['large_step_ms']:.2f} ms", file=sys.stderr, flush=True)
 
     # mixed precision: the same short steps with FP32 prognostics, fluxes and
     # tendencies and the FP64 vertical solve / reference state
@@ -330,7 +348,8 @@ def main():
                                f"linear shear 0-20 m/s, UD3, RK4, numfilter 1e-4, dt=2 s",
                    "grid": [ni, nj, nk], "short_step_bytes_per_point": STEP_BYTES_PER_POINT,
                    "short_step_hbm_frac": value * STEP_BYTES_PER_POINT / 1e9 / HBM_PEAK_GBS,
-                   "large_step_with_3_fct_tracers_ms": large_ms},
+                   "large_step_with_3_fct_tracers_ms": large_ms,
+                   **({"large_step_other_tracer_scheme": tracer_leg} if tracer_leg else {})},
         "roofline": roof,
         "fp32_vs_fp64": mixed,
         "adjoint": adj,
