@@ -186,7 +186,20 @@ typedef struct {
   int32_t short_type;  /* ATMOS_DYN_TYPE: 0 = FVM-HEVI, 1 = FVM-HEVE (the
                           explicit stage of scale_hip_tstep_short_heve; FP64
                           only, combined by a separate RK4 kernel)             */
+  int32_t tinteg_short;  /* ATMOS_DYN_TINTEG_SHORT_TYPE: SCALE_HIP_RK4 (or 0),
+                            SCALE_HIP_RK3 (Heun) or SCALE_HIP_RK3WS2002
+                            (scale_atmos_dyn_tinteg_short_rk3.F90:188-576)    */
+  int32_t tinteg_tracer; /* ATMOS_DYN_TINTEG_TRACER_TYPE: SCALE_HIP_TRACER_RK3WS2002
+                            (0) or SCALE_HIP_TRACER_EULER
+                            (scale_atmos_dyn_tinteg_tracer_euler.F90:80-133)  */
 } scale_hip_dyn_config;
+
+/* time integrator ids of scale_hip_dyn_config */
+#define SCALE_HIP_RK3 3
+#define SCALE_HIP_RK4 4
+#define SCALE_HIP_RK3WS2002 32
+#define SCALE_HIP_TRACER_RK3WS2002 0
+#define SCALE_HIP_TRACER_EULER 1
 
 /* field ids for scale_hip_model_set/_get/_mass */
 enum {
@@ -271,6 +284,17 @@ int scale_hip_tstep_short_hevi_adj(scale_hip_ctx *ctx,
  * from its checkpoint.  The model state is left at the end of the run. */
 int scale_hip_model_forward_checkpointed(scale_hip_ctx *ctx, int32_t n, int32_t every);
 int scale_hip_model_adjoint(scale_hip_ctx *ctx, double *DENS_b, double *MOMZ_b, double *MOMX_b, double *MOMY_b, double *RHOT_b);
+
+/* Launch-shape knobs, with no counterpart in the reference (whose loops take
+ * IBLOCK/JBLOCK from PARAM_ATMOS_DYN; here the tiles are chosen from the grid
+ * and the CU count unless a knob is set).  Results do not depend on them:
+ * every tile shape computes the same operations in the same order.
+ *   SCALE_HIP_TUNE_NF_TJ: rows of the numerical-filter tile, 4, 8 or 16
+ *   (0 = chosen from the grid).
+ * Unknown knobs and values outside the listed ones are rejected. */
+enum { SCALE_HIP_TUNE_NF_TJ = 1 };
+int scale_hip_tune_set(scale_hip_ctx *ctx, int32_t knob, int32_t value);
+int scale_hip_tune_get(scale_hip_ctx *ctx, int32_t knob, int32_t *value);
 
 /* HBM probe with no counterpart in the reference: `reps` timed streaming
  * copies of n doubles between two scratch buffers (allocated and freed by the

@@ -44,7 +44,11 @@ class OrcConfig(C.Structure):
                 ("dtl", C.c_double), ("dts", C.c_double), ("nd_coef", C.c_double), ("nd_sfc_fact", C.c_double),
                 ("nd_coef_q", C.c_double), ("nd_order", C.c_int), ("nd_use_rs", C.c_int),
                 ("divdmp_coef", C.c_double), ("flag_fct_tracer", C.c_int), ("flag_fct_along_stream", C.c_int),
-                ("short_type", C.c_int)]
+                ("short_type", C.c_int), ("tinteg_tracer", C.c_int)]
+
+# ATMOS_DYN_TINTEG_SHORT_TYPE and ATMOS_DYN_TINTEG_TRACER_TYPE ids of the oracle
+TINTEG_SHORT = {"RK4": 4, "RK3": 3, "RK3WS2002": 32}
+TINTEG_TRACER = {"RK3WS2002": 0, "EULER": 1}
 
 
 def build(force: bool = False) -> str:
@@ -222,6 +226,31 @@ def wdamp_setup(g, wdamp_tau, wdamp_height, FZ):
     return out
 
 
+def terrain_metrics(g, Zsfc, fz, metric: dict) -> dict:
+    """orc_terrain_metrics: GSQRT, J13G, J23G (KA,IA,JA,7) and PHI from the
+    surface height Zsfc(IA,JA) and the flat grid's FZ(0:KA); metric supplies
+    RCDZ, RFDZ, RCDX, RFDX, RCDY, RFDY."""
+    fz = np.ascontiguousarray(fz, dtype=np.float64)
+    cz = np.ascontiguousarray(0.5 * (fz[1:] + fz[:-1]))
+    out = dict(GSQRT=np.full((g.KA, g.IA, g.JA, 7), np.nan, order="F"),
+               J13G=np.full((g.KA, g.IA, g.JA, 7), np.nan, order="F"),
+               J23G=np.full((g.KA, g.IA, g.JA, 7), np.nan, order="F"),
+               PHI=np.full((g.KA, g.IA, g.JA), np.nan, order="F"))
+    lib().orc_terrain_metrics(C.byref(grid_struct(g)), ptr(np.asfortranarray(Zsfc, dtype=np.float64)), ptr(cz), ptr(fz),
+                              *[ptr(np.ascontiguousarray(metric[k])) for k in ("RCDZ", "RFDZ", "RCDX", "RFDX", "RCDY", "RFDY")],
+                              ptr(out["GSQRT"]), ptr(out["J13G"]), ptr(out["J23G"]), ptr(out["PHI"]))
+    return out
+
+
+def divergence(case, MOMZ, MOMX, MOMY, metric=None) -> np.ndarray:
+    """ATMOS_DYN_divergence (i = IS..IE+1, j = JS..JE+1, k = KS-1..KE+1); NaN elsewhere."""
+    g = case.grid
+    out = np.full(g.shape, np.nan, order="F")
+    lib().orc_divergence(C.byref(grid_struct(g)), ptr(out), ptr(MOMZ), ptr(MOMX), ptr(MOMY),
+                         C.byref(metric_struct(metric or case.metric)))
+    return out
+
+
 def halo_periodic(g, a):
     gs = grid_struct(g)
     lib().orc_halo_periodic(C.byref(gs), ptr(a))
@@ -294,7 +323,7 @@ class Model:
     "hevi" or "heve" selects the stage of the RK4 short step (ATMOS_DYN_TYPE)."""
 
     def __init__(self, case, nd_coef=1.0e-4, divdmp_coef=0.0, fct=None, along_stream=True, nd_coef_q=0.0,
-                 short="hevi"):
+                 short="hevi", dtl=None, tinteg_short="RK4", tinteg_tracer="RK3WS2002"):
         g = case.grid
         self.case = case
         self.state = {k: np.array(v, order="F", copy=True) for k, v in case.state.items()}
@@ -304,9 +333,11 @@ class Model:
         self.mflx_hi = np.full(g.shape + (3,), np.nan, order="F")
         self.tflx_hi = np.full(g.shape + (3,), np.nan, order="F")
         self.rhoq_tp = np.zeros(g.shape + (max(qa, 1),), order="F")
-        self.cfg = OrcConfig(case.scheme, case.scheme_tracer, 4, case.dt, case.dt, nd_coef, 1.0, nd_coef_q, 1, 1,
+        # dtl: the large step (case.dt, the short step, by default: nstep = 1)
+        self.cfg = OrcConfig(case.scheme, case.scheme_tracer, TINTEG_SHORT[tinteg_short],
+                             case.dt if dtl is None else dtl, case.dt, nd_coef, 1.0, nd_coef_q, 1, 1,
                              divdmp_coef, int(case.fct if fct is None else fct), int(along_stream),
-                             {"hevi": 0, "heve": 1}[short])
+                             {"hevi": 0, "heve": 1}[short], TINTEG_TRACER[tinteg_tracer])
         self._keep = []
 
     def _st(self):

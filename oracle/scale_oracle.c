@@ -15,6 +15,8 @@
  *   TRACER = scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_tracer_fvm_heve.F90
  *   RK4    = scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tinteg_short_rk4.F90
  *   TRK3   = scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tinteg_tracer_rk3.F90
+ *   RK3    = scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tinteg_short_rk3.F90
+ *   TEUL   = scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tinteg_tracer_euler.F90
  *   LARGE  = scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_tstep_large_fvm_heve.F90
  *   COMM   = scalelib/src/communication/scale_comm_cartesC.F90
  *
@@ -2078,6 +2080,93 @@ int orc_tinteg_tracer_rk3(const orc_grid *g, int s,
   return rc;
 }
 
+/* ---- tracer Euler (TEUL:80-133) ------------------------------------------- */
+/* One stage of dtl with the end-of-step density; the reference's output
+ * aliases its input QTRC (TEUL:120-121), which its stage reads only before
+ * the update loop, so the stage writes a separate field here and copies it. */
+int orc_tinteg_tracer_euler(const orc_grid *g, int s,
+    double *QTRC, const double *QTRC0, const double *RHOQ_t,
+    const double *DENS0, const double *DENS, const double *mflx_hi, const double *num_diff,
+    const double *GSQRT, const double *MAPF, const double *CDZ,
+    const double *RCDZ, const double *RCDX, const double *RCDY,
+    double dtl, int flag_fct, int flag_fct_along_stream) {
+  GRID_LOCALS;
+  double *out = alloc_nan(N3);
+  memcpy(out, QTRC, N3 * sizeof(double));
+  int rc = orc_tstep_tracer_heve(g, s, out, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff, GSQRT, MAPF, CDZ, RCDZ,
+                                 RCDX, RCDY, dtl, flag_fct, flag_fct_along_stream);
+  memcpy(QTRC, out, N3 * sizeof(double));
+  free(out);
+  return rc;
+}
+
+/* ---- RK3 short step (RK3:188-576) ----------------------------------------- */
+int orc_tinteg_short_rk3(const orc_grid *g, int s, int heve, int ws2002,
+    double *DENS, double *MOMZ, double *MOMX, double *MOMY, double *RHOT,
+    double *mflx_hi, double *tflx_hi,
+    const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
+    const double *DPRES0, const double *RT2P, const double *CORIOLI,
+    const double *num_diff, const double *wdamp_coef, double divdmp_coef, const double *DDIV,
+    const orc_metric *m, const double *REF_dens, const double *REF_rhot, double dt) {
+  GRID_LOCALS;
+  double *S0[5], *R[2][5], *mf1 = alloc_nan(3 * N3), *tf1 = alloc_nan(3 * N3);
+  double *mf2 = alloc_nan(3 * N3), *tf2 = alloc_nan(3 * N3);
+  double *cur[5] = {DENS, MOMZ, MOMX, MOMY, RHOT};
+  for (int v = 0; v < 5; ++v) {
+    S0[v] = alloc_nan(N3);
+    memcpy(S0[v], cur[v], N3 * sizeof(double)); /* DENS0 = DENS ... (RK3:306-317) */
+    for (int r = 0; r < 2; ++r) R[r][v] = alloc_nan(N3);
+  }
+  /* fact_dt1, fact_dt2 (RK3:141-166); dtrk = dt * fact (RK3:359, :412) */
+  const double fact_dt1 = 1.0 / 3.0, fact_dt2 = ws2002 ? 1.0 / 2.0 : 2.0 / 3.0;
+  const double dtrk_s[3] = {dt * fact_dt1, dt * fact_dt2, dt};
+  double *mo_s[3] = {mf1, mf2, mflx_hi}, *to_s[3] = {tf1, tf2, tflx_hi};
+  int rc = 0;
+  for (int st = 0; st < 3; ++st) {
+    double **in = st == 0 ? S0 : R[st - 1];
+    double **out = st == 2 ? cur : R[st];
+    if (heve)
+      rc |= orc_tstep_short_heve(g, s, out[0], out[1], out[2], out[3], out[4], mo_s[st], to_s[st],
+                                 S0[0], S0[1], S0[2], S0[3], S0[4], in[0], in[1], in[2], in[3], in[4],
+                                 DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, DPRES0, RT2P, CORIOLI,
+                                 num_diff, wdamp_coef, divdmp_coef, DDIV, 0, 0, 1, m, REF_dens, REF_rhot, dtrk_s[st]);
+    else
+      rc |= orc_tstep_short_hevi(g, s, out[0], out[1], out[2], out[3], out[4], mo_s[st], to_s[st],
+                                 S0[0], S0[1], S0[2], S0[3], S0[4], in[0], in[1], in[2], in[3], in[4],
+                                 DENS_t, MOMZ_t, MOMX_t, MOMY_t, RHOT_t, DPRES0, RT2P, CORIOLI,
+                                 num_diff, wdamp_coef, divdmp_coef, DDIV, m, REF_dens, REF_rhot, dtrk_s[st]);
+    if (st < 2)
+      for (int v = 0; v < 5; ++v) orc_halo_periodic(g, out[v]);
+  }
+  if (!ws2002) {
+    /* Heun's combination (RK3:505-571); MOMZ on KS..KE-1 */
+    for (int v = 0; v < 5; ++v) {
+      const int kend = v == 1 ? KE - 1 : KE;
+      OMP_FOR
+      for (int j = JS; j <= JE; ++j)
+        for (int i = IS; i <= IE; ++i)
+          for (int k = KS; k <= kend; ++k) {
+            size_t p = I3(k, i, j);
+            cur[v][p] = (R[0][v][p] * 3.0 + cur[v][p] * 3.0 - S0[v][p] * 2.0) / 4.0;
+          }
+    }
+    for (int n = 0; n < 3; ++n)
+      for (int j = JS; j <= JE; ++j)
+        for (int i = IS; i <= IE; ++i)
+          for (int k = KS; k <= KE; ++k) {
+            size_t p = I3(k, i, j) + N3 * n;
+            mflx_hi[p] = (mf1[p] + mflx_hi[p] * 3.0) / 4.0;
+            tflx_hi[p] = (tf1[p] + tflx_hi[p] * 3.0) / 4.0;
+          }
+  }
+  for (int v = 0; v < 5; ++v) {
+    free(S0[v]);
+    for (int r = 0; r < 2; ++r) free(R[r][v]);
+  }
+  free(mf1); free(tf1); free(mf2); free(tf2);
+  return rc;
+}
+
 /* ---- RK4 short step (RK4:260-660) ----------------------------------------- */
 int orc_tinteg_short_rk4(const orc_grid *g, int s, int heve,
     double *DENS, double *MOMZ, double *MOMX, double *MOMY, double *RHOT,
@@ -2144,7 +2233,9 @@ int orc_tinteg_short_rk4(const orc_grid *g, int s, int heve,
 /* ---- large step (LARGE:377-1127) ------------------------------------------ */
 int orc_large_step(const orc_grid *g, const orc_metric *m, const orc_config *c, orc_state *st) {
   GRID_LOCALS;
-  if (c->tinteg_short != 4) return -1;
+  if (c->tinteg_short != ORC_TINTEG_RK4 && c->tinteg_short != ORC_TINTEG_RK3 && c->tinteg_short != ORC_TINTEG_RK3WS2002)
+    return -1;
+  if (c->tinteg_tracer != ORC_TINTEG_TRACER_RK3WS2002 && c->tinteg_tracer != ORC_TINTEG_TRACER_EULER) return -1;
   const double dtl = c->dtl;
   const int nstep = (int)ceil((dtl - EPS_RP) / c->dts);
   const double dts = dtl / nstep;
@@ -2193,9 +2284,15 @@ int orc_large_step(const orc_grid *g, const orc_metric *m, const orc_config *c, 
     }
     if (c->divdmp_coef > 0.0) orc_divergence(g, DDIV, st->MOMZ, st->MOMX, st->MOMY, m);
     else for (size_t q = 0; q < N3; ++q) DDIV[q] = 0.0;
-    rc |= orc_tinteg_short_rk4(g, c->scheme, c->short_type == 1, st->DENS, st->MOMZ, st->MOMX, st->MOMY, st->RHOT, st->mflx_hi, st->tflx_hi,
-                               Dt[0], Dt[1], Dt[2], Dt[3], Dt[4], DPRES0, RT2P, st->CORIOLI, num_diff, st->wdamp_coef,
-                               c->divdmp_coef, DDIV, m, st->REF_dens, REF_rhot, dts);
+    if (c->tinteg_short == ORC_TINTEG_RK4)
+      rc |= orc_tinteg_short_rk4(g, c->scheme, c->short_type == 1, st->DENS, st->MOMZ, st->MOMX, st->MOMY, st->RHOT, st->mflx_hi, st->tflx_hi,
+                                 Dt[0], Dt[1], Dt[2], Dt[3], Dt[4], DPRES0, RT2P, st->CORIOLI, num_diff, st->wdamp_coef,
+                                 c->divdmp_coef, DDIV, m, st->REF_dens, REF_rhot, dts);
+    else
+      rc |= orc_tinteg_short_rk3(g, c->scheme, c->short_type == 1, c->tinteg_short == ORC_TINTEG_RK3WS2002,
+                                 st->DENS, st->MOMZ, st->MOMX, st->MOMY, st->RHOT, st->mflx_hi, st->tflx_hi,
+                                 Dt[0], Dt[1], Dt[2], Dt[3], Dt[4], DPRES0, RT2P, st->CORIOLI, num_diff, st->wdamp_coef,
+                                 c->divdmp_coef, DDIV, m, st->REF_dens, REF_rhot, dts);
     /* ghost levels (LARGE.F90:908-927) */
     OMP_FOR
     for (int j = JS; j <= JE; ++j)
@@ -2238,9 +2335,14 @@ int orc_large_step(const orc_grid *g, const orc_metric *m, const orc_config *c, 
       /* passive tracers (none is QV): LARGE.F90:1049-1061 with DENS00 */
       if (c->nd_coef_q != 0.0)
         rc |= orc_numfilter_coef_q(g, ndq, DENS00, q, 0, m, dtl, NULL, c->nd_coef_q, c->nd_order, c->nd_sfc_fact, c->nd_use_rs);
-      rc |= orc_tinteg_tracer_rk3(g, c->scheme_tracer, q, q, st->RHOQ_tp + N3 * iq, DENS00, st->DENS, st->mflx_hi, ndq,
-                                  m->GSQRT, MAPF_XY, m->CDZ, m->RCDZ, m->RCDX, m->RCDY, dtl,
-                                  c->flag_fct_tracer, c->flag_fct_along_stream);
+      if (c->tinteg_tracer == ORC_TINTEG_TRACER_EULER)
+        rc |= orc_tinteg_tracer_euler(g, c->scheme_tracer, q, q, st->RHOQ_tp + N3 * iq, DENS00, st->DENS, st->mflx_hi, ndq,
+                                      m->GSQRT, MAPF_XY, m->CDZ, m->RCDZ, m->RCDX, m->RCDY, dtl,
+                                      c->flag_fct_tracer, c->flag_fct_along_stream);
+      else
+        rc |= orc_tinteg_tracer_rk3(g, c->scheme_tracer, q, q, st->RHOQ_tp + N3 * iq, DENS00, st->DENS, st->mflx_hi, ndq,
+                                    m->GSQRT, MAPF_XY, m->CDZ, m->RCDZ, m->RCDX, m->RCDY, dtl,
+                                    c->flag_fct_tracer, c->flag_fct_along_stream);
       orc_halo_periodic(g, q);
     }
     free(ndq);
@@ -2248,6 +2350,107 @@ int orc_large_step(const orc_grid *g, const orc_metric *m, const orc_config *c, 
   for (int v = 0; v < 5; ++v) free(Dt[v]);
   free(RT2P); free(DPRES0); free(REF_rhot); free(DENS00); free(mflx_av); free(num_diff); free(DDIV);
   return rc;
+}
+
+/* ---- terrain-following metrics (REAL:391-545, METRIC:200-345) ------------- *
+ * REAL = scalelib/src/atmosphere/grid/scale_atmos_grid_cartesC_real.F90,
+ * METRIC = scalelib/src/atmosphere/grid/scale_atmos_grid_cartesC_metric.F90.
+ * Levels are 0-based; face index kf = 0..KA is the reference's FZ(kf), the
+ * top face of level kf-1. */
+void orc_terrain_metrics(const orc_grid *g, const double *Zsfc, const double *CZ, const double *FZ,
+    const double *RCDZ, const double *RFDZ, const double *RCDX, const double *RFDX,
+    const double *RCDY, const double *RFDY,
+    double *GSQRT, double *J13G, double *J23G, double *PHI) {
+  GRID_LOCALS;
+  const size_t KF = (size_t)KA + 1;
+  const double Htop = FZ[KE + 1] - FZ[KS]; /* FZ(KE) - FZ(KS-1) (REAL:410) */
+#define ZS(i, j) Zsfc[(size_t)(i) + (size_t)IA * (size_t)(j)]
+  /* surface heights at the x, u, v and uv points (REAL:413-455, 478-545) */
+  double *zx = malloc(sizeof(double) * IA * JA), *zu = malloc(sizeof(double) * IA * JA);
+  double *zv = malloc(sizeof(double) * IA * JA), *zuv = malloc(sizeof(double) * IA * JA);
+#define P2(i, j) ((size_t)(i) + (size_t)IA * (size_t)(j))
+  for (int j = 0; j < JA; ++j)
+    for (int i = 0; i < IA; ++i) {
+      zx[P2(i, j)] = ZS(i, j);
+      zu[P2(i, j)] = i < IA - 1 ? (ZS(i, j) + ZS(i + 1, j)) * 0.5 : ZS(IA - 1, j);
+      zv[P2(i, j)] = j < JA - 1 ? (ZS(i, j) + ZS(i, j + 1)) * 0.5 : ZS(i, JA - 1);
+      if (i < IA - 1 && j < JA - 1) zuv[P2(i, j)] = (ZS(i, j) + ZS(i + 1, j) + ZS(i, j + 1) + ZS(i + 1, j + 1)) * 0.25;
+      else if (j < JA - 1) zuv[P2(i, j)] = (ZS(IA - 1, j) + ZS(IA - 1, j + 1)) * 0.5;
+      else if (i < IA - 1) zuv[P2(i, j)] = (ZS(i, JA - 1) + ZS(i + 1, JA - 1)) * 0.5;
+      else zuv[P2(i, j)] = ZS(IA - 1, JA - 1);
+    }
+  /* Gal-Chen heights ( Htop - Zs ) / Htop * Z + Zs of the cell centres and faces */
+  double *cz[4], *fz[4];
+  const double *zs[4] = {zx, zu, zv, zuv};
+  for (int s = 0; s < 4; ++s) {
+    cz[s] = malloc(sizeof(double) * N3);
+    fz[s] = malloc(sizeof(double) * KF * IA * JA);
+    for (int j = 0; j < JA; ++j)
+      for (int i = 0; i < IA; ++i) {
+        const double Zs = zs[s][P2(i, j)];
+        for (int k = 0; k < KA; ++k) cz[s][I3(k, i, j)] = (Htop - Zs) / Htop * CZ[k] + Zs;
+        for (int kf = 0; kf <= KA; ++kf) fz[s][(size_t)kf + KF * P2(i, j)] = (Htop - Zs) / Htop * FZ[kf] + Zs;
+      }
+  }
+  enum { X_ = 0, U_ = 1, V_ = 2, UV_ = 3 };
+#define CZS(s, k, i, j) cz[s][I3(k, i, j)]
+#define FZS(s, kf, i, j) fz[s][(size_t)(kf) + KF * P2(i, j)]
+#define GS(k, i, j, c) GSQRT[I3(k, i, j) + N3 * (c)]
+#define J13(k, i, j, c) J13G[I3(k, i, j) + N3 * (c)]
+#define J23(k, i, j, c) J23G[I3(k, i, j) + N3 * (c)]
+  /* G^1/2 (METRIC:231-279) */
+  for (int j = JS; j <= JE; ++j)
+    for (int i = IS; i <= IE; ++i) {
+      for (int k = 0; k < KA; ++k) {
+        GS(k, i, j, I_XYZ) = (FZS(X_, k + 1, i, j) - FZS(X_, k, i, j)) * RCDZ[k];
+        GS(k, i, j, I_UYZ) = (FZS(U_, k + 1, i, j) - FZS(U_, k, i, j)) * RCDZ[k];
+        GS(k, i, j, I_XVZ) = (FZS(V_, k + 1, i, j) - FZS(V_, k, i, j)) * RCDZ[k];
+        GS(k, i, j, I_UVZ) = (FZS(UV_, k + 1, i, j) - FZS(UV_, k, i, j)) * RCDZ[k];
+      }
+      for (int k = 0; k < KA - 1; ++k) {
+        GS(k, i, j, I_XYW) = (CZS(X_, k + 1, i, j) - CZS(X_, k, i, j)) * RFDZ[k];
+        GS(k, i, j, I_UYW) = (CZS(U_, k + 1, i, j) - CZS(U_, k, i, j)) * RFDZ[k];
+        GS(k, i, j, I_XVW) = (CZS(V_, k + 1, i, j) - CZS(V_, k, i, j)) * RFDZ[k];
+      }
+      GS(KA - 1, i, j, I_XYW) = GS(KA - 2, i, j, I_XYW);
+      GS(KA - 1, i, j, I_UYW) = GS(KA - 2, i, j, I_UYW);
+      GS(KA - 1, i, j, I_XVW) = GS(KA - 2, i, j, I_XVW);
+    }
+  /* J13G, J23G (METRIC:297-323) */
+  for (int j = JS; j <= JE; ++j)
+    for (int i = IS; i <= IE; ++i)
+      for (int k = 0; k < KA; ++k) {
+        J13(k, i, j, I_XYZ) = -(CZS(U_, k, i, j) - CZS(U_, k, i - 1, j)) * RCDX[i];
+        J13(k, i, j, I_XYW) = -(FZS(U_, k + 1, i, j) - FZS(U_, k + 1, i - 1, j)) * RCDX[i];
+        J13(k, i, j, I_UYW) = -(FZS(X_, k + 1, i + 1, j) - FZS(X_, k + 1, i, j)) * RFDX[i];
+        J13(k, i, j, I_XVW) = -(FZS(UV_, k + 1, i, j) - FZS(UV_, k + 1, i - 1, j)) * RCDX[i];
+        J13(k, i, j, I_UYZ) = -(CZS(X_, k, i + 1, j) - CZS(X_, k, i, j)) * RFDX[i];
+        J13(k, i, j, I_XVZ) = -(CZS(UV_, k, i, j) - CZS(UV_, k, i - 1, j)) * RCDX[i];
+        J13(k, i, j, I_UVZ) = -(CZS(V_, k, i + 1, j) - CZS(V_, k, i, j)) * RFDX[i];
+        J23(k, i, j, I_XYZ) = -(CZS(V_, k, i, j) - CZS(V_, k, i, j - 1)) * RCDY[j];
+        J23(k, i, j, I_XYW) = -(FZS(V_, k + 1, i, j) - FZS(V_, k + 1, i, j - 1)) * RCDY[j];
+        J23(k, i, j, I_XVW) = -(FZS(X_, k + 1, i, j + 1) - FZS(X_, k + 1, i, j)) * RFDY[j];
+        J23(k, i, j, I_UYW) = -(FZS(UV_, k + 1, i, j) - FZS(UV_, k + 1, i, j - 1)) * RCDY[j];
+        J23(k, i, j, I_XVZ) = -(CZS(X_, k, i, j + 1) - CZS(X_, k, i, j)) * RFDY[j];
+        J23(k, i, j, I_UYZ) = -(CZS(UV_, k, i, j) - CZS(UV_, k, i, j - 1)) * RCDY[j];
+        J23(k, i, j, I_UVZ) = -(CZS(U_, k, i, j + 1) - CZS(U_, k, i, j)) * RFDY[j];
+      }
+  for (int c = 0; c < 7; ++c) {
+    orc_halo_periodic(g, GSQRT + N3 * c);
+    orc_halo_periodic(g, J13G + N3 * c);
+    orc_halo_periodic(g, J23G + N3 * c);
+  }
+  /* PHI = GRAV * CZ(real) (REAL:549) */
+  for (size_t q = 0; q < N3; ++q) PHI[q] = GRAV * cz[X_][q];
+  for (int s = 0; s < 4; ++s) { free(cz[s]); free(fz[s]); }
+  free(zx); free(zu); free(zv); free(zuv);
+#undef ZS
+#undef P2
+#undef CZS
+#undef FZS
+#undef GS
+#undef J13
+#undef J23
 }
 
 /* ---- FP64 budget ---------------------------------------------------------- */

@@ -155,7 +155,7 @@ typedef struct {
 typedef struct {
   int scheme;          /* dynamics flux scheme (ORC_UD3 or ORC_CD4) */
   int scheme_tracer;   /* tracer flux scheme */
-  int tinteg_short;    /* 4 = RK4 (only option implemented) */
+  int tinteg_short;    /* ATMOS_DYN_TINTEG_SHORT_TYPE: ORC_TINTEG_RK4, _RK3 (Heun) or _RK3WS2002 */
   double dtl;          /* large step */
   double dts;          /* short step (dtl/dts rounded up as in LARGE.F90:379-382) */
   double nd_coef, nd_sfc_fact, nd_coef_q;
@@ -163,7 +163,25 @@ typedef struct {
   double divdmp_coef;
   int flag_fct_tracer, flag_fct_along_stream;
   int short_type;      /* ATMOS_DYN_TYPE: 0 = FVM-HEVI, 1 = FVM-HEVE */
+  int tinteg_tracer;   /* ATMOS_DYN_TINTEG_TRACER_TYPE: ORC_TINTEG_TRACER_RK3WS2002 or _EULER */
 } orc_config;
+
+/* Terrain-following metrics from the surface height Zsfc(IA,JA) (periodic
+ * halo filled): the Gal-Chen heights of ATMOS_GRID_CARTESC_REAL_calc_Z
+ * (scalelib/src/atmosphere/grid/scale_atmos_grid_cartesC_real.F90:391-545) at
+ * the eight staggered positions, then G^1/2, J13G and J23G of
+ * ATMOS_GRID_CARTESC_METRIC_terrainfollowing
+ * (scale_atmos_grid_cartesC_metric.F90:200-345) on the interior columns
+ * with their periodic halos, and PHI = GRAV * CZ(real).  CZ(KA) and FZ(0:KA)
+ * (KA+1 values) are the flat grid's heights; RCDZ..RFDY as in orc_metric. */
+void orc_terrain_metrics(const orc_grid *g, const double *Zsfc, const double *CZ, const double *FZ,
+    const double *RCDZ, const double *RFDZ, const double *RCDX, const double *RFDX,
+    const double *RCDY, const double *RFDY,
+    double *GSQRT, double *J13G, double *J23G, double *PHI);
+
+/* time integrators (scale_atmos_dyn_tinteg_short.F90, _tinteg_tracer.F90) */
+enum { ORC_TINTEG_RK3 = 3, ORC_TINTEG_RK4 = 4, ORC_TINTEG_RK3WS2002 = 32 };
+enum { ORC_TINTEG_TRACER_RK3WS2002 = 0, ORC_TINTEG_TRACER_EULER = 1 };
 
 /* ATMOS_DYN_Tstep_large_fvm_heve on-path subset (LARGE.F90:377-1127) for a
  * dry run with QA passive tracers (moist constants with QV=0 reduce to the
@@ -181,6 +199,26 @@ int orc_tinteg_short_rk4(const orc_grid *g, int scheme, int heve,
     const double *DPRES0, const double *RT2P, const double *CORIOLI,
     const double *num_diff, const double *wdamp_coef, double divdmp_coef, const double *DDIV,
     const orc_metric *m, const double *REF_dens, const double *REF_rhot, double dt);
+
+/* One RK3 short step (tinteg_short_rk3.F90:188-576) on caller buffers:
+ * ws2002 = 0 is Heun's method (stages dt/3, 2dt/3, dt, then the combination
+ * (3 r1 + 3 r3 - 2 phi_n)/4), ws2002 = 1 Wicher and Skamarock (2002)
+ * (dt/3, dt/2, dt; phi_{n+1} = r3). */
+int orc_tinteg_short_rk3(const orc_grid *g, int scheme, int heve, int ws2002,
+    double *DENS, double *MOMZ, double *MOMX, double *MOMY, double *RHOT,
+    double *mflx_hi, double *tflx_hi,
+    const double *DENS_t, const double *MOMZ_t, const double *MOMX_t, const double *MOMY_t, const double *RHOT_t,
+    const double *DPRES0, const double *RT2P, const double *CORIOLI,
+    const double *num_diff, const double *wdamp_coef, double divdmp_coef, const double *DDIV,
+    const orc_metric *m, const double *REF_dens, const double *REF_rhot, double dt);
+
+/* tinteg_tracer_euler.F90:80-133: one tracer stage of dtl into QTRC */
+int orc_tinteg_tracer_euler(const orc_grid *g, int scheme,
+    double *QTRC, const double *QTRC0, const double *RHOQ_t,
+    const double *DENS0, const double *DENS, const double *mflx_hi, const double *num_diff,
+    const double *GSQRT, const double *MAPF_XY, const double *CDZ,
+    const double *RCDZ, const double *RCDX, const double *RCDY,
+    double dtl, int flag_fct, int flag_fct_along_stream);
 
 /* tinteg_tracer_rk3.F90:92-208 (RK3WS2002) */
 int orc_tinteg_tracer_rk3(const orc_grid *g, int scheme,

@@ -1121,7 +1121,7 @@ static StepRec model_rec(Ctx& c) {
 static hipError_t check_adjoint_model(const Ctx& c) {
   if (!c.model || c.terrain || c.d.IS != 2) return hipErrorNotSupported;
   const Model& M = *c.model;
-  if (M.mixed || M.heve || M.divdmp_coef != 0.0) return hipErrorNotSupported;
+  if (M.mixed || M.heve || M.divdmp_coef != 0.0 || M.tinteg_short != 4) return hipErrorNotSupported;
   if ((int)ceil((M.dtl - EPS_RP) / M.dts) != 1) return hipErrorNotSupported;
   return hipSuccess;
 }

@@ -680,6 +680,11 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   if (cfg->nd_coef_q < 0.0) return fail("nd_coef_q must be >= 0");
   if (cfg->short_type != 0 && cfg->short_type != 1) return fail("short_type must be 0 (FVM-HEVI) or 1 (FVM-HEVE)");
   if (cfg->short_type == 1 && cfg->mixed_precision) return fail("the HEVE short step runs in FP64 only");
+  const int ts = cfg->tinteg_short == 0 ? SCALE_HIP_RK4 : cfg->tinteg_short;
+  if (ts != SCALE_HIP_RK4 && ts != SCALE_HIP_RK3 && ts != SCALE_HIP_RK3WS2002)
+    return fail("tinteg_short must be SCALE_HIP_RK4 (4), SCALE_HIP_RK3 (3) or SCALE_HIP_RK3WS2002 (32)");
+  if (cfg->tinteg_tracer != SCALE_HIP_TRACER_RK3WS2002 && cfg->tinteg_tracer != SCALE_HIP_TRACER_EULER)
+    return fail("tinteg_tracer must be SCALE_HIP_TRACER_RK3WS2002 (0) or SCALE_HIP_TRACER_EULER (1)");
   shp::Model* M = new shp::Model();
   c.model = M;
   const size_t n3 = c.d.n3;
@@ -728,6 +733,8 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   M->divdmp_coef = cfg->divdmp_coef; M->fct = cfg->fct_tracer ? 1 : 0; M->fct_along = cfg->fct_along_stream ? 1 : 0;
   M->nd_coef_q = cfg->nd_coef_q;
   M->heve = cfg->short_type == 1 ? 1 : 0;
+  M->tinteg_short = ts;
+  M->tinteg_tracer = cfg->tinteg_tracer;
   const int nstep = (int)ceil((cfg->dtl - shp::EPS_RP) / cfg->dts);
   if (int r = set_numfilter_tables(c, cfg->dtl / nstep, cfg->nd_coef, cfg->nd_sfc_fact)) return r;
   if (cfg->nd_coef_q != 0.0)
@@ -870,7 +877,7 @@ int scale_hip_model_forward_checkpointed(scale_hip_ctx* h, int32_t n, int32_t ev
   if (n < 1 || every < 1) return fail("n and every must be positive");
   hipError_t e = shp::model_forward_checkpointed(c, n, every);
   if (e == hipErrorNotSupported)
-    return fail("the adjoint supports flat terrain, FP64, divdmp_coef = 0 and one short step per large step");
+    return fail("the adjoint supports flat terrain, FP64, RK4 short steps, divdmp_coef = 0 and one short step per large step");
   CK(e);
   CK(hipStreamSynchronize(c.stream));
   return 0;
@@ -890,7 +897,7 @@ int scale_hip_model_adjoint(scale_hip_ctx* h, double* DENS_b, double* MOMZ_b, do
   hipError_t e = shp::model_adjoint(c, xb);
   if (e == hipErrorNotReady) return fail("scale_hip_model_forward_checkpointed must run first");
   if (e == hipErrorNotSupported)
-    return fail("the adjoint supports flat terrain, FP64, divdmp_coef = 0 and one short step per large step");
+    return fail("the adjoint supports flat terrain, FP64, RK4 short steps, divdmp_coef = 0 and one short step per large step");
   CK(e);
   for (int v = 0; v < 5; ++v)
     if (int r = download3(c, hb[v], xb[v], 1)) return r;
@@ -947,6 +954,27 @@ int scale_hip_wdamp_setup(const scale_hip_grid* g, double* wdamp_coef, const dou
   for (int k = 0; k < KS; ++k) wdamp_coef[k] = wdamp_coef[KS];
   for (int k = KE + 1; k < KA; ++k) wdamp_coef[k] = wdamp_coef[KE];
   return 0;
+}
+
+int scale_hip_tune_set(scale_hip_ctx* h, int32_t knob, int32_t value) {
+  if (int r = check_ctx(h)) return r;
+  switch (knob) {
+    case SCALE_HIP_TUNE_NF_TJ:
+      if (value != 0 && value != 4 && value != 8 && value != 16) return fail("SCALE_HIP_TUNE_NF_TJ must be 0, 4, 8 or 16");
+      h->c.nf_tj = value;
+      return 0;
+    default:
+      return fail("unknown tuning knob " + std::to_string(knob));
+  }
+}
+
+int scale_hip_tune_get(scale_hip_ctx* h, int32_t knob, int32_t* value) {
+  if (int r = check_ctx(h)) return r;
+  if (!value) return fail("null argument");
+  switch (knob) {
+    case SCALE_HIP_TUNE_NF_TJ: *value = h->c.nf_tj; return 0;
+    default: return fail("unknown tuning knob " + std::to_string(knob));
+  }
 }
 
 int scale_hip_hbm_copy_probe(scale_hip_ctx* h, int64_t n, int32_t reps, double* avg_ms) {

@@ -180,6 +180,45 @@ __global__ void __launch_bounds__(256) rk4_combine(CombArgs<R> a) {
   }
 }
 
+// RK3 end of step (tinteg_short_rk3.F90:502-571): Heun's combination
+// (3 r1 + 3 r3 - 2 phi_n)/4 of the state (MOMZ on KS..KE-1) and (f1 + 3 f3)/4
+// of the fluxes, or for RK3WS2002 the third stage itself; the ghost levels
+// as rk4_combine (r2 and r4 are unused)
+template <typename R>
+__global__ void __launch_bounds__(256) rk3_combine(CombArgs<R> a, int heun) {
+  const Dims& d = a.d;
+  const int i = d.IS + blockIdx.x * 64 + threadIdx.x;
+  const int k = d.KS + blockIdx.y * 4 + threadIdx.y;
+  const int j = d.JS + blockIdx.z;
+  if (i > d.IE || k > d.KE || j > d.JE) return;
+  const size_t p = IX(d, k, i, j);
+  const int KS = d.KS, KE = d.KE;
+  auto c3 = [&](int v) { return (a.r1[v][p] * R(3) + a.r3[v][p] * R(3) - a.s0[v][p] * R(2)) / R(4); };
+  R v4[4];
+  const int vs[4] = {0, 2, 3, 4};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v4[q] = heun ? c3(vs[q]) : a.r3[vs[q]][p];
+  const R w = (heun && k <= KE - 1) ? c3(1) : a.r3[1][p];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) a.st[vs[q]][p] = v4[q];
+  a.st[1][p] = w;
+  if (k == KS || k == KE) {
+    const int lo = (k == KS) ? 0 : KE + 1, hi = (k == KS) ? KS - 1 : d.KA - 1;
+    for (int kk = lo; kk <= hi; ++kk) {
+      const size_t g = IX(d, kk, i, j);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a.st[vs[q]][g] = v4[q];
+      a.st[1][g] = R(0);
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const size_t q = p + d.n3 * (size_t)n;
+    a.mflx[q] = heun ? (a.m1[q] + a.m3[q] * R(3)) / R(4) : a.m3[q];
+    a.tflx[q] = heun ? (a.t1[q] + a.t3[q] * R(3)) / R(4) : a.t3[q];
+  }
+}
+
 // element-wise conversion between the FP64 and FP32 copies of a field
 template <typename To, typename From>
 __global__ void convert_kernel(To* dst, const From* src, size_t n) {
@@ -433,9 +472,14 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B, const 
     if (M.divdmp_coef > 0.0) HIPCHK(divergence(c, B.DDIV, B.st[1], B.st[2], B.st[3], s));
   }
   // DENS0 = DENS, ... (RK4.F90:326-336): the stages only read the state, so
-  // it serves as the start value directly and is overwritten by the combination
-  const double dtrk[4] = {dts / 2.0, dts / 2.0, dts, dts};
-  for (int r = 0; r < 4; ++r) {
+  // it serves as the start value directly and is overwritten by the combination.
+  // RK3 (tinteg_short_rk3.F90:141-166, 355-490): dtrk = dt * fact_dt1, dt * fact_dt2, dt
+  const bool rk4 = M.tinteg_short == 4;
+  const int nst = rk4 ? 4 : 3;
+  const double fact_dt1 = 1.0 / 3.0, fact_dt2 = M.tinteg_short == 32 ? 1.0 / 2.0 : 2.0 / 3.0;
+  double dtrk[4] = {dts / 2.0, dts / 2.0, dts, dts};
+  if (!rk4) { dtrk[0] = dts * fact_dt1; dtrk[1] = dts * fact_dt2; dtrk[2] = dts; }
+  for (int r = 0; r < nst; ++r) {
     StageIOT<R> io;
     io.ctx = &c;
     for (int v = 0; v < 5; ++v) {
@@ -449,8 +493,8 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B, const 
     io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = num_diff;
     io.wdamp_coef = M.wdamp_coef; io.DDIV = B.DDIV; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
     io.divdmp_coef = M.divdmp_coef; io.dtrk = dtrk[r]; io.has_coriolis = M.has_coriolis;
-    // flat-terrain HEVI: the last stage combines the four stages into the state itself
-    io.combine = (r == 3 && !c.terrain && !M.heve) ? 1 : 0;
+    // flat-terrain HEVI RK4: the last stage combines the four stages into the state itself
+    io.combine = (rk4 && r == 3 && !c.terrain && !M.heve) ? 1 : 0;
     for (int v = 0; v < 5; ++v) { io.c1[v] = Rk(0, v); io.c2[v] = Rk(1, v); io.c3[v] = Rk(2, v); io.st[v] = B.st[v]; }
     for (int n = 0; n < 3; ++n) { io.cm[n] = B.mfR[n]; io.ct[n] = B.tfR[n]; }
     io.mflx_av = B.mflx; io.tflx_av = B.tflx;
@@ -461,10 +505,26 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B, const 
     } else {
       HIPCHK(hevi_stage(io, s));
     }
-    if (r < 3) {
+    if (r < nst - 1) {
       R* const rr[5] = {Rk(r, 0), Rk(r, 1), Rk(r, 2), Rk(r, 3), Rk(r, 4)};
       HIPCHK(halo_fill(c, rr, 5, s));
     }
+  }
+  if (!rk4) {
+    CombArgs<R> a;
+    a.d = d;
+    for (int v = 0; v < 5; ++v) {
+      a.st[v] = B.st[v]; a.r1[v] = Rk(0, v); a.r2[v] = Rk(1, v); a.r3[v] = Rk(2, v); a.r4[v] = Rk(2, v);
+      a.s0[v] = B.st[v];
+    }
+    a.mflx = B.mflx; a.tflx = B.tflx;
+    a.m1 = B.mfR[0]; a.m2 = B.mfR[1]; a.m3 = B.mfR[2]; a.m4 = B.mfR[2];
+    a.t1 = B.tfR[0]; a.t2 = B.tfR[1]; a.t3 = B.tfR[2]; a.t4 = B.tfR[2];
+    prof_start(c, "rk3_combine", s);
+    hipLaunchKernelGGL(rk3_combine<R>, grid_int(d), dim3(64, 4), 0, s, a, M.tinteg_short == 3 ? 1 : 0);
+    prof_stop(c, s);
+    HIPCHK(hipGetLastError());
+    return halo_fill(c, B.st, 5, s);
   }
   if (!c.terrain && !M.heve) return halo_fill(c, B.st, 5, s);
   CombArgs<R> a;
@@ -503,7 +563,7 @@ hipError_t rk4_combine_state(Ctx& c, double* const out[5], const double* const s
 
 hipError_t model_short_step_rk4(Ctx& c, double dts, const StepRec* rec) {
   Model& M = *c.model;
-  if (rec && (M.mixed || M.heve || c.terrain || M.divdmp_coef != 0.0)) return hipErrorNotSupported;
+  if (rec && (M.mixed || M.heve || c.terrain || M.divdmp_coef != 0.0 || M.tinteg_short != 4)) return hipErrorNotSupported;
   if (M.mixed) {
     ShortBufs<float> B{{M.fS[0], M.fS[1], M.fS[2], M.fS[3], M.fS[4]}, M.fR, M.ft, M.fnum_diff, M.fmfR, M.ftfR, M.fmflx,
                        M.ftflx, M.fDDIV};
@@ -601,6 +661,16 @@ hipError_t model_large_step(Ctx& c, const StepRec* rec) {
       if (M.nd_coef_q != 0.0) HIPCHK(numfilter_q(c, M.ndq, M.DENS00, q, nullptr, s));
       TracerIO io;
       io.RHOQ_t = rq; io.DENS0 = M.DENS00; io.mflx_hi = M.mflx_hi; io.num_diff = M.ndq; io.flag_along = M.fct_along;
+      if (M.tinteg_tracer == 1) {
+        // EULER (tinteg_tracer_euler.F90:80-133): one stage of dtl with the
+        // end-of-step density, FCT as configured; written to QRK1 then copied
+        // back (the stage reads its input at neighbours)
+        io.QTRCo = M.QRK1; io.QTRC = q; io.QTRC0 = q; io.DENS = M.DENS; io.dtl = dtl; io.flag_fct = M.fct;
+        HIPCHK(tracer_stage(c, io, s));
+        HIPCHK(hipMemcpyAsync(q, M.QRK1, bytes, hipMemcpyDeviceToDevice, s));
+        HIPCHK(halo_fill(c, &q, 1, s));
+        continue;
+      }
       // stage 1: dt/3, no FCT
       hipLaunchKernelGGL(dens_rk, dim3((d.IE - d.IS + 3 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 3), dim3(64, 4), 0, s,
                          d, M.DENS_RK, M.DENS00, M.DENS, 1);

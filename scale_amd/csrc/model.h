@@ -43,6 +43,9 @@ struct Model {
   double dtl = 1.0, dts = 1.0, nd_coef = 1.0e-4, nd_sfc_fact = 1.0, divdmp_coef = 0.0, nd_coef_q = 0.0;
   int fct = 0, fct_along = 1;
   int heve = 0;  // ATMOS_DYN_TYPE = FVM-HEVE: explicit stages, separate RK4 combination
+  // ATMOS_DYN_TINTEG_SHORT_TYPE: 4 = RK4, 3 = RK3 (Heun), 32 = RK3WS2002;
+  // ATMOS_DYN_TINTEG_TRACER_TYPE: 0 = RK3WS2002, 1 = EULER
+  int tinteg_short = 4, tinteg_tracer = 0;
   // every allocation, freed together
   std::vector<void*> allocs;
 };

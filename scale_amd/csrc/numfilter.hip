@@ -322,7 +322,8 @@ static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], c
   // tile height: 8 rows when that gives every CU a block, 4 otherwise (at
   // 512x512x128 the filter takes 1.94 ms at 8 rows, 1.95 at 4 and 4.24 at 16
   // in FP64, whose 1024-thread blocks are held to 128 VGPRs and spill; FP32
-  // 1.06, 1.80 and 1.09 ms); SCALE_HIP_NF_TJ = 4, 8 or 16 overrides
+  // 1.06, 1.80 and 1.09 ms); the context's knob (scale_hip_tune_set), else
+  // SCALE_HIP_NF_TJ = 4, 8 or 16 in the environment, overrides
   static const int ncu = [] {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -335,7 +336,7 @@ static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], c
     return (v == 4 || v == 8 || v == 16) ? v : 0;
   }();
   const unsigned ni = (imax + 63) / 64;
-  int TJ = TJenv;
+  int TJ = c.nf_tj ? c.nf_tj : TJenv;
   if (!TJ) {
     TJ = 4;
     if (ni * (unsigned)((jmax + 7) / 8) >= (unsigned)ncu) TJ = 8;

@@ -44,6 +44,8 @@ struct Ctx {
   AdjWork* adj = nullptr;
   // named hipEvent timers, null when profiling is off
   Prof* prof = nullptr;
+  // launch-shape knobs (scale_hip_tune_set); 0 = chosen from the grid
+  int nf_tj = 0;  // numerical-filter tile height: 4, 8 or 16 rows
   // host copies of 1-D grid arrays (for the numerical-filter tables)
   std::vector<double> hCDZ, hFDZ, hCDX, hFDX, hCDY, hFDY;
 };

@@ -49,7 +49,13 @@ class DynConfig(C.Structure):
     _fields_ = [("dtl", C.c_double), ("dts", C.c_double), ("nd_coef", C.c_double), ("nd_sfc_fact", C.c_double),
                 ("divdmp_coef", C.c_double), ("nd_order", C.c_int32), ("fct_tracer", C.c_int32),
                 ("fct_along_stream", C.c_int32), ("mixed_precision", C.c_int32), ("nd_coef_q", C.c_double),
-                ("short_type", C.c_int32)]
+                ("short_type", C.c_int32), ("tinteg_short", C.c_int32), ("tinteg_tracer", C.c_int32)]
+
+
+# ATMOS_DYN_TINTEG_SHORT_TYPE / ATMOS_DYN_TINTEG_TRACER_TYPE ids of scale_hip_dyn_config
+TINTEG_SHORT = {"RK4": 4, "RK3": 3, "RK3WS2002": 32}
+TINTEG_TRACER = {"RK3WS2002": 0, "EULER": 1}
+TUNE_NF_TJ = 1  # scale_hip_tune_set knob: rows of the numerical-filter tile (0 = from the grid)
 
 
 _lib = None
@@ -228,9 +234,11 @@ class Context:
             _d(m["J33G"]), _p(m["MAPF"])))
 
     def model_create(self, dtl, dts, QA, nd_coef=1.0e-4, nd_sfc_fact=1.0, fct_tracer=False, fct_along_stream=True,
-                     mixed_precision=False, divdmp_coef=0.0, nd_coef_q=0.0, short="hevi"):
+                     mixed_precision=False, divdmp_coef=0.0, nd_coef_q=0.0, short="hevi", tinteg_short="RK4",
+                     tinteg_tracer="RK3WS2002"):
         cfg = DynConfig(dtl, dts, nd_coef, nd_sfc_fact, divdmp_coef, 1, int(fct_tracer), int(fct_along_stream),
-                        int(mixed_precision), nd_coef_q, {"hevi": 0, "heve": 1}[short])
+                        int(mixed_precision), nd_coef_q, {"hevi": 0, "heve": 1}[short], TINTEG_SHORT[tinteg_short],
+                        TINTEG_TRACER[tinteg_tracer])
         _check(load_library().scale_hip_model_create(self._h, C.byref(cfg), C.c_int32(QA)))
         self.has_model = True
         self.QA = QA
@@ -285,6 +293,15 @@ class Context:
         _check(load_library().scale_hip_profile_query(self._h, name.encode(), C.byref(cnt), C.byref(ms)))
         return cnt.value, ms.value
 
+    def tune_set(self, knob, value):
+        """Set a launch-shape knob (scale_hip_tune_set); results do not depend on it."""
+        _check(load_library().scale_hip_tune_set(self._h, C.c_int32(int(knob)), C.c_int32(int(value))))
+
+    def tune_get(self, knob):
+        v = C.c_int32()
+        _check(load_library().scale_hip_tune_get(self._h, C.c_int32(int(knob)), C.byref(v)))
+        return v.value
+
     def synchronize(self):
         _check(load_library().scale_hip_synchronize(self._h))
 
@@ -301,16 +318,20 @@ class Context:
 
 
 def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, device=0, mixed_precision=False,
-                    divdmp_coef=0.0, nd_coef_q=0.0, short="hevi") -> Context:
+                    divdmp_coef=0.0, nd_coef_q=0.0, short="hevi", dtl=None, tinteg_short="RK4",
+                    tinteg_tracer="RK3WS2002") -> Context:
     """Create a context holding the device-resident model of an inputs.Case
     (mixed_precision: FP32 dynamics with FP64 solve and reference state; the
-    tracers are left out; short = "hevi" or "heve", ATMOS_DYN_TYPE)."""
+    tracers are left out; short = "hevi" or "heve", ATMOS_DYN_TYPE; dtl: the
+    large step, case.dt (the short step) by default, so nstep = ceil(dtl/dt))."""
     if ctx is None:
         ctx = Context(case.grid, case.scheme, case.scheme_tracer, device)
     ctx.set_metrics(case.metric)
     qa = 0 if mixed_precision else case.qtrc.shape[-1]
-    ctx.model_create(case.dt, case.dt, qa, nd_coef=nd_coef, fct_tracer=case.fct if fct is None else fct,
-                     mixed_precision=mixed_precision, divdmp_coef=divdmp_coef, nd_coef_q=nd_coef_q, short=short)
+    ctx.model_create(case.dt if dtl is None else dtl, case.dt, qa, nd_coef=nd_coef,
+                     fct_tracer=case.fct if fct is None else fct, mixed_precision=mixed_precision,
+                     divdmp_coef=divdmp_coef, nd_coef_q=nd_coef_q, short=short, tinteg_short=tinteg_short,
+                     tinteg_tracer=tinteg_tracer)
     for k, v in case.state.items():
         ctx.model_set(k, v)
     for k, v in case.ref.items():

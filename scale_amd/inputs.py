@@ -277,6 +277,23 @@ def make_case(kmax=24, imax=16, jmax=16, dx=500.0, dy=500.0, dz=250.0, *, seed=2
                 dt=float(dt), scheme=scheme, scheme_tracer=scheme_tracer, fct=fct, fz=fz)
 
 
+def bell_topography(g: Grid, height=500.0, half_width=5000.0, cx=None, cy=None, ry=None) -> np.ndarray:
+    """Surface height Zsfc(IA,JA) of a bell-shaped mountain
+    height / (1 + ((x-cx)/a)^2 + ((y-cy)/b)^2) (scale-rm/src/preprocess/
+    mod_mktopo.F90:238-246, BELL_eachnode = .false.), centred in the domain by
+    default, evaluated on the interior and given a periodic halo (the
+    topography's COMM_vars8 fill, scale_topography.F90:107-131)."""
+    xc = (np.arange(g.IA) - g.IS + 0.5) * g.dx
+    yc = (np.arange(g.JA) - g.JS + 0.5) * g.dy
+    cx = 0.5 * g.imax * g.dx if cx is None else cx
+    cy = 0.5 * g.jmax * g.dy if cy is None else cy
+    ry = half_width if ry is None else ry
+    dist = ((xc[:, None] - cx) / half_width) ** 2 + ((yc[None, :] - cy) / ry) ** 2
+    z = np.asfortranarray(height / (1.0 + dist))
+    halo_fill(g, z[None])
+    return z
+
+
 def large_step_prep(case: Case):
     """RT2P, DPRES0, REF_rhot of LARGE.F90:443-471 (dry form) over the whole array."""
     g = case.grid

@@ -1047,3 +1047,522 @@ def test_model_constant_theta_preserved():
     pott = m.state["RHOT"][sl] / m.state["DENS"][sl]
     assert np.allclose(pott, 300.0, rtol=1e-13, atol=0)
     assert np.allclose(m.qtrc[..., 0][sl], 1e-3, rtol=1e-13, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# time integrators (scale_atmos_dyn_tinteg_short_rk3.F90:188-576,
+# scale_atmos_dyn_tinteg_tracer_euler.F90:80-133)
+def _explicit_large_step(tinteg_short, nstep, T=1.0):
+    """One large step of T seconds in nstep short steps of the explicit HEVE
+    stage (X_RK = X0 + dtrk f(X)) with the filter off: within a large step
+    RT2P/DPRES0/REF_rhot are frozen, so the short steps integrate the ODE
+    dX/dt = f(X) and the textbook order of the RK method shows."""
+    case = inputs.make_case(kmax=16, imax=10, jmax=8, noise=0.01, ntracer=0, bubble_r=1500.0, bubble_zc=1500.0,
+                            momz_noise=0.05, dt=T / nstep, scheme=CD4, v0=-2.0)
+    m = po.Model(case, nd_coef=0.0, short="heve", dtl=T, tinteg_short=tinteg_short)
+    m.step(1)
+    return case, m.state
+
+
+@pytest.mark.parametrize("tinteg,order", [("RK3", 3), ("RK3WS2002", 3), ("RK4", 4)])
+def test_short_integrators_converge_at_their_order(tinteg, order):
+    """Heun's RK3 (stages dt/3, 2dt/3, dt and (3 r1 + 3 r3 - 2 phi)/4),
+    Wicher-Skamarock RK3 (dt/3, dt/2, dt) and RK4 are third, third and fourth
+    order: against an RK4 run with 128 short steps, halving the step divides
+    the error of every prognostic by about 2^order.  A wrong stage factor or
+    combination weight drops the order to 1 or 2 (ratio 2 or 4)."""
+    case, ref = _explicit_large_step("RK4", 128)
+    g = case.grid
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    errs = []
+    for n in (4, 8, 16):
+        _, st = _explicit_large_step(tinteg, n)
+        errs.append({k: np.abs(st[k][sl] - ref[k][sl]).max() / np.abs(ref[k][sl]).max() for k in H.VARS})
+    lo = {3: 6.0, 4: 11.0}[order]
+    hi = {3: 11.0, 4: 30.0}[order]
+    for k in H.VARS:
+        r = errs[1][k] / errs[2][k]
+        assert lo < r < hi, (tinteg, k, [e[k] for e in errs])
+
+
+def test_heun_and_ws2002_differ():
+    """The two RK3 variants are different methods (same order, different
+    error constants), so the integrator switch takes effect."""
+    _, a = _explicit_large_step("RK3", 4)
+    _, b = _explicit_large_step("RK3WS2002", 4)
+    assert not np.array_equal(a["MOMX"], b["MOMX"])
+
+
+@pytest.mark.parametrize("tinteg", ["RK3", "RK3WS2002"])
+def test_rk3_model_conserves_mass(tinteg):
+    """test_conserve over large steps with the RK3 short integrators (HEVI, FCT tracers)."""
+    case = inputs.make_case(kmax=20, imax=12, jmax=12, noise=0.05, ntracer=1, tracer_kind=["tophat"], fct=True,
+                            bubble_r=1500.0, bubble_zc=1500.0)
+    m = po.Model(case, tinteg_short=tinteg)
+    m0 = m.mass(m.state["DENS"])
+    q0 = m.mass(np.asfortranarray(m.qtrc[..., 0] * m.state["DENS"]))
+    m.step(4)
+    assert abs(m.mass(m.state["DENS"]) - m0) <= 1e-13 * m0
+    assert abs(m.mass(np.asfortranarray(m.qtrc[..., 0] * m.state["DENS"])) - q0) <= 1e-13 * q0
+
+
+def _tracer_run(tinteg_tracer, n, T=8.0, fct=False):
+    case = inputs.make_case(kmax=16, imax=12, jmax=10, noise=0.01, ntracer=1, tracer_kind=["blob"], bubble_r=1500.0,
+                            bubble_zc=1500.0, momz_noise=0.05, dt=T / n, scheme=CD4, scheme_tracer=CD4, u0=12.0,
+                            v0=-5.0, fct=fct)
+    m = po.Model(case, tinteg_tracer=tinteg_tracer)
+    q0 = m.mass(np.asfortranarray(m.qtrc[..., 0] * m.state["DENS"]))
+    m.step(n)
+    return case, m, q0
+
+
+def test_tracer_euler_is_first_order():
+    """Over fixed model time the forward-Euler tracer integrator differs from
+    the third-order RK3WS2002 one by O(dt): halving the step halves the
+    difference, while tracer mass is conserved to round-off."""
+    diffs = []
+    for n in (4, 8, 16):
+        case, me, q0 = _tracer_run("EULER", n)
+        _, mr, _ = _tracer_run("RK3WS2002", n)
+        g = case.grid
+        sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+        q1 = me.mass(np.asfortranarray(me.qtrc[..., 0] * me.state["DENS"]))
+        assert abs(q1 - q0) <= 1e-13 * q0
+        diffs.append(np.abs(me.qtrc[..., 0][sl] - mr.qtrc[..., 0][sl]).max())
+    assert 1.8 < diffs[0] / diffs[1] < 2.2 and 1.8 < diffs[1] / diffs[2] < 2.2, diffs
+
+
+@pytest.mark.parametrize("fct", [False, True], ids=["nofct", "fct"])
+def test_tracer_euler_fct_keeps_bounds(fct):
+    """test_fctminmax for the Euler integrator: with FCT the top-hat tracer
+    stays within its initial range; without it the same run leaves it."""
+    case = inputs.make_case(kmax=16, imax=12, jmax=10, noise=0.01, ntracer=1, tracer_kind=["tophat"], bubble_r=1500.0,
+                            bubble_zc=1500.0, momz_noise=0.05, dt=2.0, u0=12.0, v0=-5.0, fct=fct)
+    m = po.Model(case, tinteg_tracer="EULER")
+    m.step(5)
+    g = case.grid
+    qt = m.qtrc[..., 0][g.KS:g.KE + 1, g.IS:g.IE + 1, g.JS:g.JE + 1]
+    inside = qt.min() >= -1e-18 and qt.max() <= 1e-3 * (1 + 1e-13)
+    assert inside == fct, (qt.min(), qt.max())
+
+
+# ---------------------------------------------------------------------------
+# numerical filter: the closed-form response to 2-grid waves, every row
+_ND_VARS = ("DENS", "MOMZ", "MOMX", "MOMY", "RHOT")
+# levels where num_diff(:,:,:,v,d) is defined (COMMON.F90:541-962); others are 0
+_ND_ROWS = {("DENS", 0): (-1, 0), ("MOMZ", 0): (1, -1), ("MOMX", 0): (0, -1), ("MOMY", 0): (0, -1),
+            ("RHOT", 0): (-1, 0), ("MOMZ", 1): (0, -1), ("MOMZ", 2): (0, -1)}
+
+
+@pytest.mark.parametrize("sfc", [1.0, 0.3])
+@pytest.mark.parametrize("d", [0, 1, 2], ids=["z", "x", "y"])
+@pytest.mark.parametrize("var", _ND_VARS)
+def test_numfilter_two_grid_wave_every_row(var, d, sfc):
+    """On a uniform grid a 2-grid wave A(-1)^m along direction d is an
+    eigenfunction of the filter's third difference, the ghost-level mirror
+    rows included (COMMON.F90:1600-1700): the difference is +-8A(-1)^m/D^3
+    (+ for a field centred along d, whose difference sits on the faces; - for
+    the field staggered along d), so num_diff = ND_COEF/(2^4 DT) D^4 times
+    that, times the density at the flux point for the velocities and theta
+    (uniform density rho0 here), and in x and y the surface rows KS and KS+1
+    carry ND_SFC_FACT and (1 + ND_SFC_FACT)/2 (COMMON.F90:593-594).  Every
+    other direction of the same variable is exactly 0.  A flipped sign, a
+    shifted index, a wrong density weight, mirror row or surface factor
+    changes some row by a factor other than 1."""
+    case = inputs.make_case(kmax=12, imax=8, jmax=6, noise=0.0, bubble=False, u0=0.0, ntracer=0, dz=250.0)
+    g = case.grid
+    KS, KE = g.KS, g.KE
+    rho0, th0, A = 1.1, 300.0, 1.0e-3
+    st = {k: np.zeros(g.shape, order="F") for k in _ND_VARS}
+    st["DENS"][:] = rho0
+    st["RHOT"][:] = rho0 * th0
+    ref = {"REF_dens": np.full(g.shape, rho0, order="F"), "REF_pott": np.full(g.shape, th0, order="F"),
+           "REF_pres": case.ref["REF_pres"]}
+    idx = np.indices(g.shape)[d]
+    wave = (-1.0) ** idx
+    if var == "DENS":
+        st["DENS"] = np.asfortranarray(rho0 + A * wave)
+    elif var == "RHOT":
+        st["RHOT"] = np.asfortranarray(rho0 * (th0 + A * wave))
+    else:
+        st[var] = np.asfortranarray(rho0 * A * wave)
+        if var == "MOMZ":
+            st["MOMZ"][:KS] = 0.0
+            st["MOMZ"][KE:] = 0.0
+    case = inputs.Case(**{**case.__dict__, "ref": ref})
+    dt, coef = 2.0, 1.0e-4
+    nd = po.numfilter_coef(case, st, dt, nd_coef=coef, nd_sfc_fact=sfc)
+    spacing = (250.0, g.dx, g.dy)[d]
+    v = _ND_VARS.index(var)
+    staggered = (var, d) in (("MOMZ", 0), ("MOMX", 1), ("MOMY", 2))
+    weight = 1.0 if var == "DENS" else rho0
+    want = (-1.0 if staggered else 1.0) * coef / (16.0 * dt) * spacing ** 4 * 8.0 * A / spacing ** 3 * weight * wave
+    lo, hi = _ND_ROWS.get((var, d), (0, 0))
+    ks, ke = KS + lo, KE + hi
+    if d > 0:
+        want = want.copy()
+        want[KS] *= sfc
+        want[KS + 1] *= (1.0 + sfc) * 0.5
+    if (var, d) == ("MOMZ", 0):
+        # the top row takes the wall value MOMZ(KE) = 0 in place of the wave's
+        # (COMMON.F90:1696-1698): -3A - 3A - A instead of -8A
+        want = want.copy()
+        want[KE - 1] *= 7.0 / 8.0
+    isl, jsl = slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1)
+    got = nd[:, isl, jsl, v, d]
+    assert np.allclose(got[ks:ke + 1], want[ks:ke + 1, isl, jsl], rtol=1e-10, atol=0), (var, d)
+    assert not got[:ks].any() and not got[ke + 1:].any(), (var, d)
+    for dd in range(3):
+        if dd == d:
+            continue
+        other = nd[:, isl, jsl, v, dd]
+        if var == "MOMZ" and dd == 0:
+            # a horizontal wave in w is constant in z except for the wall value
+            # MOMZ(KE) = 0, which the top row sees: -(w(KE-1))/dz^3 (COMMON.F90:1696-1698)
+            top = -coef / (16.0 * dt) * 250.0 * A * rho0 * wave[KE - 1, isl, jsl]
+            assert np.allclose(other[KE - 1], top, rtol=1e-10, atol=0), (var, d)
+            other = np.delete(other, KE - 1, axis=0)
+        assert not other.any(), (var, d, dd)
+
+
+# ---------------------------------------------------------------------------
+# ATMOS_DYN_divergence (COMMON.F90:1437-1515) and ATMOS_DYN_wdamp_setup (:323-381)
+@pytest.mark.parametrize("stretched", [None, (100.0, 400.0)], ids=["uniform_dz", "stretched_dz"])
+def test_divergence_of_a_polynomial_momentum_field(stretched):
+    """MOMX = a x, MOMY = b y and MOMZ = c z (H - z) (zero at the surface and
+    the top, as the walls require) have the divergence a + b + c (H - 2 z):
+    the flux differences over a cell are exact for these polynomials, so every
+    level KS..KE, the one-sided wall rows included, gives the closed form to
+    round-off.  A sign or index slip in any term, or a wrong wall row, is O(1)."""
+    case = inputs.make_case(kmax=12, imax=8, jmax=6, noise=0.0, bubble=False, u0=0.0, ntracer=0, dz=250.0,
+                            stretched=stretched)
+    g = case.grid
+    KS, KE = g.KS, g.KE
+    fz = case.fz  # FZ(0:KA): fz[k + 1] is the top face of level k
+    zf = fz[1:]   # top face of each level (the w point)
+    zc = 0.5 * (fz[1:] + fz[:-1])
+    H = zf[KE]
+    x_u = (np.arange(g.IA) - g.IS + 1.0) * g.dx   # u points: east face of cell i
+    y_v = (np.arange(g.JA) - g.JS + 1.0) * g.dy
+    a, b, c = 2.0e-3, -1.5e-3, 4.0e-7
+    MOMX = np.asfortranarray(np.broadcast_to(a * x_u[None, :, None], g.shape).copy())
+    MOMY = np.asfortranarray(np.broadcast_to(b * y_v[None, None, :], g.shape).copy())
+    w = c * zf * (H - zf)
+    w[:KS] = 0.0
+    w[KE:] = 0.0
+    MOMZ = np.asfortranarray(np.broadcast_to(w[:, None, None], g.shape).copy())
+    dd = po.divergence(case, MOMZ, MOMX, MOMY)
+    want = a + b + c * (H - 2.0 * zc)
+    isl, jsl = slice(g.IS, g.IE + 2), slice(g.JS, g.JE + 2)
+    for k in range(KS, KE + 1):
+        assert np.allclose(dd[k, isl, jsl], want[k], rtol=1e-10, atol=1e-15), k
+
+
+@pytest.mark.parametrize("height", [4000.0, 10000.0])
+def test_wdamp_setup_properties(height):
+    """The Rayleigh coefficient alpha/2 (1 - cos(pi (z - h)/(z_top - h)))
+    at the w points: 0 at and below h, alpha = 1/tau exactly at the top face,
+    alpha/2 where z is halfway, strictly increasing in between, ghost levels
+    copied from KS / KE; h at or above the top switches it off."""
+    case = inputs.make_case(kmax=40, imax=4, jmax=4, noise=0.0, bubble=False, ntracer=0, dz=500.0)
+    g = case.grid
+    KS, KE = g.KS, g.KE
+    tau = 250.0
+    wd = po.wdamp_setup(g, tau, height, case.fz)
+    zf = case.fz[1:]
+    above = zf[KS:KE + 1] > height
+    assert not wd[KS:KE + 1][~above].any()
+    assert wd[KE] == pytest.approx(1.0 / tau, rel=1e-15)
+    ka = np.flatnonzero(above) + KS
+    assert np.all(np.diff(wd[ka]) > 0.0)
+    mid = 0.5 * (height + zf[KE])
+    kmid = np.flatnonzero(np.isclose(zf, mid))
+    assert kmid.size == 1 and wd[kmid[0]] == pytest.approx(0.5 / tau, rel=1e-14)
+    assert np.all(wd[:KS] == wd[KS]) and np.all(wd[KE + 1:] == wd[KE])
+    assert not po.wdamp_setup(g, tau, zf[KE], case.fz).any()
+
+
+# ---------------------------------------------------------------------------
+# HEVI horizontal momentum update (HEVI.F90:816-927, 936-1047): pressure
+# gradient, Coriolis, map-factor metric term and momentum advection
+def _quiet_inputs(case):
+    """Stage inputs of a fluid at rest in the reference state: DENS = REF_dens,
+    RHOT = REF_rhot (so DPRES = DPRES0 exactly), no momentum, no tendencies,
+    no filter; the tests add one ingredient at a time."""
+    g = case.grid
+    prep = inputs.large_step_prep(case)
+    inp = {}
+    for v in H.VARS:
+        base = case.state["DENS"] if v == "DENS" else (prep["REF_rhot"] if v == "RHOT" else g.zeros())
+        inp[v + "0"] = np.array(base, order="F", copy=True)
+        inp[v] = np.array(base, order="F", copy=True)
+        inp[v + "_t"] = g.zeros()
+    inp["RHOT0"][:] = case.state["RHOT"]
+    inp["RHOT"][:] = case.state["RHOT"]
+    inp["RT2P"] = prep["RT2P"]
+    inp["REF_rhot"] = np.array(case.state["RHOT"], order="F", copy=True)
+    inp["DPRES0"] = np.array(prep["DPRES0"], order="F", copy=True)
+    inp["REF_dens"] = case.ref["REF_dens"]
+    inp["num_diff"] = g.zeros(5, 3)
+    inp["DDIV"] = g.zeros()
+    return inp
+
+
+def _calm_case(**kw):
+    args = dict(kmax=10, imax=8, jmax=6, noise=0.0, bubble=False, u0=0.0, v0=0.0, ntracer=0, dz=300.0)
+    args.update(kw)
+    return inputs.make_case(**args)
+
+
+def _coords(g):
+    xc = (np.arange(g.IA) - g.IS + 0.5) * g.dx
+    yc = (np.arange(g.JA) - g.JS + 0.5) * g.dy
+    return xc, yc, xc + 0.5 * g.dx, yc + 0.5 * g.dy
+
+
+def _tendency(out, inp, v, dtrk):
+    return (out[v + "_RK"] - inp[v + "0"]) / dtrk
+
+
+@pytest.mark.parametrize("scheme", [UD3, CD4])
+def test_hevi_horizontal_pressure_gradient_closed_form(scheme):
+    """DPRES = P(z) + q1 x + q2 x^2 + r1 y + r2 y^2 at rest: the MOMX and MOMY
+    tendencies are -(q1 + 2 q2 x_u) and -(r1 + 2 r2 y_v), the exact gradient
+    at the staggered points.  A sign error flips them; an index shift moves
+    them by 2 q2 dx."""
+    case = _calm_case()
+    g = case.grid
+    inp = _quiet_inputs(case)
+    xc, yc, xu, yv = _coords(g)
+    q1, q2, r1, r2 = 0.02, 3.0e-6, -0.015, -2.0e-6
+    add = q1 * xc[:, None] + q2 * xc[:, None] ** 2 + r1 * yc[None, :] + r2 * yc[None, :] ** 2
+    inp["DPRES0"] = np.asfortranarray(inp["DPRES0"] + add[None, :, :])
+    dt = 0.6
+    out = po.tstep_short_hevi(case, inp, dt, scheme=scheme)
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    tx = _tendency(out, inp, "MOMX", dt)[sl]
+    ty = _tendency(out, inp, "MOMY", dt)[sl]
+    wx = -(q1 + 2.0 * q2 * xu[g.IS:g.IE + 1])[None, :, None]
+    wy = -(r1 + 2.0 * r2 * yv[g.JS:g.JE + 1])[None, None, :]
+    assert np.allclose(tx, np.broadcast_to(wx, tx.shape), rtol=1e-7, atol=0)
+    assert np.allclose(ty, np.broadcast_to(wy, ty.shape), rtol=1e-7, atol=0)
+
+
+def test_hevi_coriolis_closed_form():
+    """Uniform v with f = f0 + beta x + gamma y: the MOMX tendency is
+    f(x_u, y) rho v; uniform u: the MOMY tendency is -f(x, y_v) rho u
+    (HEVI.F90:900-905, 1020-1025).  A sign error or an average over the
+    wrong neighbours (j+1 in place of i+1) fails."""
+    case = _calm_case()
+    g = case.grid
+    xc, yc, xu, yv = _coords(g)
+    f0, beta, gamma = 1.0e-4, 2.0e-9, -3.0e-9
+    cor = np.asfortranarray((f0 + beta * xc[:, None] + gamma * yc[None, :])[None])
+    rho = case.state["DENS"]
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    dt = 0.5
+    for comp, vel in (("MOMY", 7.0), ("MOMX", -5.0)):
+        inp = _quiet_inputs(case)
+        inp["CORIOLI"] = cor
+        for key in (comp, comp + "0"):
+            inp[key] = np.asfortranarray(rho * vel)
+        out = po.tstep_short_hevi(case, inp, dt)
+        if comp == "MOMY":  # MOMX tendency at (x_u, y)
+            got = _tendency(out, inp, "MOMX", dt)[sl]
+            want = (f0 + beta * xu[:, None] + gamma * yc[None, :])[None] * rho * vel
+        else:               # MOMY tendency at (x, y_v)
+            got = _tendency(out, inp, "MOMY", dt)[sl]
+            want = -(f0 + beta * xc[:, None] + gamma * yv[None, :])[None] * rho * vel
+        assert np.allclose(got, want[sl], rtol=1e-7, atol=0), comp
+
+
+def test_hevi_map_factor_metric_term_closed_form():
+    """A map factor linear in its reciprocal: m_y = 1/(1 + a x) (m_x = 1) with
+    uniform v gives the MOMX tendency m_y(x_u) rho v^2 a; m_x = 1/(1 + b y)
+    (m_y = 1) with uniform u gives the MOMY tendency m_x(y_v) rho u^2 b: the
+    metric term cf of HEVI.F90:900-906, 1020-1026 (f = 0).  Swapping the
+    factors, their indices or the sign fails."""
+    case = _calm_case()
+    g = case.grid
+    xc, yc, xu, yv = _coords(g)
+    rho = case.state["DENS"]
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    dt = 0.5
+    a, b = 2.0e-6, -3.0e-6
+    # MAPF(i,j,n,c): n = 0 x factor, 1 y factor; c = XY, UY, XV, UV
+    xs = {0: xc, 1: xu, 2: xc, 3: xu}
+    ys = {0: yc, 1: yc, 2: yv, 3: yv}
+    for comp, vel in (("MOMY", 6.0), ("MOMX", -4.0)):
+        mapf = np.ones((g.IA, g.JA, 2, 4), order="F")
+        for c in range(4):
+            if comp == "MOMY":
+                mapf[:, :, 1, c] = (1.0 / (1.0 + a * xs[c]))[:, None]
+            else:
+                mapf[:, :, 0, c] = (1.0 / (1.0 + b * ys[c]))[None, :]
+        cm = inputs.Case(**{**case.__dict__, "metric": {**case.metric, "MAPF": mapf}})
+        inp = _quiet_inputs(cm)
+        for key in (comp, comp + "0"):
+            inp[key] = np.asfortranarray(rho * vel)
+        out = po.tstep_short_hevi(cm, inp, dt)
+        if comp == "MOMY":
+            got = _tendency(out, inp, "MOMX", dt)[sl]
+            want = (1.0 / (1.0 + a * xu))[None, :, None] * rho * vel * vel * a
+        else:
+            got = _tendency(out, inp, "MOMY", dt)[sl]
+            want = (1.0 / (1.0 + b * yv))[None, None, :] * rho * vel * vel * b
+        assert np.allclose(got, np.broadcast_to(want, rho.shape)[sl], rtol=1e-9, atol=0), comp
+
+
+def _advection_error(scheme, n_per_wave, comp):
+    """Max error of the MOMX (or MOMY) tendency of u = u0 + eps sin(k x)
+    (or v in y) at constant density against the exact -d(rho u^2)/dx."""
+    nx = n_per_wave
+    L = 16000.0
+    dx = L / nx
+    if comp == "MOMX":
+        case = _calm_case(imax=nx, jmax=4, dx=dx)
+    else:
+        case = _calm_case(imax=4, jmax=nx, dy=dx)
+    g = case.grid
+    inp = _quiet_inputs(case)
+    rho = 1.15
+    for v in ("DENS", "DENS0"):
+        inp[v] = np.full(g.shape, rho, order="F")
+    xc, yc, xu, yv = _coords(g)
+    kw = 2 * np.pi / L
+    u0, eps = 8.0, 2.0
+    s = xu if comp == "MOMX" else yv
+    vel = u0 + eps * np.sin(kw * s)
+    shape = (1, -1, 1) if comp == "MOMX" else (1, 1, -1)
+    for key in (comp, comp + "0"):
+        inp[key] = np.asfortranarray(np.broadcast_to(rho * vel.reshape(shape), g.shape).copy())
+    dt = 0.5
+    out = po.tstep_short_hevi(case, inp, dt, scheme=scheme)
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    got = _tendency(out, inp, comp, dt)[sl]
+    exact = -rho * 2.0 * vel * eps * kw * np.cos(kw * s)
+    return np.abs(got - np.broadcast_to(exact.reshape(shape), g.shape)[sl]).max() / np.abs(exact).max()
+
+
+@pytest.mark.parametrize("comp", ["MOMX", "MOMY"])
+@pytest.mark.parametrize("scheme", [UD3, CD4])
+def test_hevi_momentum_advection_is_second_order(scheme, comp):
+    """u = u0 + eps sin(k x) advecting itself at constant density: the
+    flux-form tendency converges to -d(rho u^2)/dx at second order (the face
+    velocity is a two-point average; the face value is 3rd/4th order), so
+    doubling the resolution divides the error by about 4.  A shifted index or
+    a wrong upwind sign leaves a first-order or O(1) error."""
+    e = [_advection_error(scheme, n, comp) for n in (16, 32, 64)]
+    assert e[0] < 0.05, e
+    for a, b in zip(e, e[1:]):
+        assert 3.5 < a / b < 4.5, e
+
+
+# ---------------------------------------------------------------------------
+# terrain-following metrics (scale_atmos_grid_cartesC_real.F90:391-545,
+# scale_atmos_grid_cartesC_metric.F90:200-345) and the terrain pressure gradient
+def _metric_case(**kw):
+    args = dict(kmax=12, imax=12, jmax=10, noise=0.0, bubble=False, u0=0.0, ntracer=0, dz=250.0)
+    args.update(kw)
+    return inputs.make_case(**args)
+
+
+def test_terrain_metrics_flat_and_uniform_lift():
+    """Zsfc = 0 gives the flat metrics (G^1/2 = 1 to round-off, J13G = J23G = 0
+    exactly); a uniform surface height h0 compresses every column by
+    (H - h0)/H, with J13G = J23G = 0."""
+    case = _metric_case()
+    g = case.grid
+    H_ = case.fz[g.KE + 1] - case.fz[g.KS]
+    for h0 in (0.0, 600.0):
+        m = po.terrain_metrics(g, np.full((g.IA, g.JA), h0, order="F"), case.fz, case.metric)
+        assert np.allclose(m["GSQRT"], (H_ - h0) / H_, rtol=1e-13, atol=0)
+        assert not m["J13G"].any() and not m["J23G"].any()
+        cz = 0.5 * (case.fz[1:] + case.fz[:-1])
+        assert np.allclose(m["PHI"], inputs.GRAV * ((H_ - h0) / H_ * cz + h0)[:, None, None], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("axis", [0, 1], ids=["x", "y"])
+def test_terrain_metrics_linear_slope_closed_form(axis):
+    """Zsfc = s x (or s y): z(x, zeta) = (H - Zs)/H zeta + Zs, so
+    G^1/2 = (H - Zs)/H at every staggered point and J13G (J23G) = -s (1 - zeta/H)
+    at each point's own zeta (cell centre or face), exactly, away from the
+    periodic seam.  A sign, a wrong staggered position or a swapped
+    centre/face height fails."""
+    case = _metric_case()
+    g = case.grid
+    H_ = case.fz[g.KE + 1] - case.fz[g.KS]
+    s = 0.03
+    xc = (np.arange(g.IA) - g.IS + 0.5) * g.dx
+    yc = (np.arange(g.JA) - g.JS + 0.5) * g.dy
+    zs = np.asfortranarray(s * (xc[:, None] + 0 * yc[None, :]) if axis == 0 else s * (0 * xc[:, None] + yc[None, :]))
+    m = po.terrain_metrics(g, zs, case.fz, case.metric)
+    fz = case.fz
+    cz = 0.5 * (fz[1:] + fz[:-1])
+    zeta = {"c": cz, "f": fz[1:]}  # cell centre / top face of each level
+    # staggered x / y offsets (in cells) and vertical position of each component
+    comps = {0: (0.0, 0.0, "c"), 1: (0.0, 0.0, "f"), 2: (0.5, 0.0, "f"), 3: (0.0, 0.5, "f"),
+             4: (0.5, 0.0, "c"), 5: (0.0, 0.5, "c"), 6: (0.5, 0.5, "c")}
+    if axis == 0:
+        isl, jsl = slice(g.IS + 1, g.IE - 1), slice(g.JS, g.JE + 1)
+    else:
+        isl, jsl = slice(g.IS, g.IE + 1), slice(g.JS + 1, g.JE - 1)
+    kk = slice(g.KS, g.KE + 1)
+    J = m["J13G"] if axis == 0 else m["J23G"]
+    Jo = m["J23G"] if axis == 0 else m["J13G"]
+    for c, (ox, oy, pos) in comps.items():
+        z_ = zeta[pos][kk][:, None, None]
+        pos_xy = (xc[isl] + ox * g.dx)[:, None] if axis == 0 else (yc[jsl] + oy * g.dy)[None, :]
+        zs_c = s * pos_xy
+        gs_want = (H_ - zs_c)[None] / H_ + 0 * z_
+        assert np.allclose(m["GSQRT"][kk, isl, jsl, c] if c in (0, 4, 5, 6) else m["GSQRT"][kk, isl, jsl, c],
+                           np.broadcast_to(gs_want, m["GSQRT"][kk, isl, jsl, c].shape), rtol=1e-12, atol=0), c
+        j_want = -s * (1.0 - z_ / H_)
+        assert np.allclose(J[kk, isl, jsl, c], np.broadcast_to(j_want, J[kk, isl, jsl, c].shape), rtol=1e-12,
+                           atol=1e-16), c
+        assert np.abs(Jo[kk, isl, jsl, c]).max() < 1e-16, c
+
+
+def _terrain_pg_error(refine):
+    """Max |MOMX tendency| of a fluid at rest whose pressure perturbation
+    depends on the physical height only, over a bell mountain, relative to
+    the terrain-following part G^1/2 dp/dx|zeta it must cancel."""
+    n = refine
+    case = _metric_case(kmax=12 * n, imax=24 * n, jmax=4, dx=1000.0 / n, dz=500.0 / n)
+    g = case.grid
+    zs = inputs.bell_topography(g, height=800.0, half_width=4000.0, cy=0.0, ry=1e12)  # a ridge along y
+    m = po.terrain_metrics(g, zs, case.fz, case.metric)
+    cm = inputs.Case(**{**case.__dict__, "metric": {**case.metric, **m}})
+    inp = _quiet_inputs(cm)
+    zreal = m["PHI"] / inputs.GRAV
+    # hydrostatic with the lowest level's density, so that the ghost level
+    # extrapolation DPRES(KS-1) = DPRES0(KS-1) - DENS(KS) (PHI(KS-1) - PHI(KS+1))
+    # (HEVI.F90:357) continues it when DPRES0(KS-1) = DPRES0(KS+1), as the
+    # large step leaves it for a hydrostatic reference state; a curvature term
+    # e^{-z/Hs} is added on the interior levels and on their ghost copies
+    rho_s = case.state["DENS"][g.KS, g.IS, g.JS]
+    curv = 300.0 * np.exp(-zreal / 3000.0)
+    p = 500.0 - rho_s * inputs.GRAV * zreal + curv
+    p[g.KS - 1] = p[g.KS + 1] + curv[g.KS - 1] - curv[g.KS + 1]
+    p[g.KE + 1] = p[g.KE - 1] + curv[g.KE + 1] - curv[g.KE - 1]
+    inp["DPRES0"] = np.asfortranarray(p)
+    dt = 0.5
+    out = po.tstep_short_hevi(cm, inp, dt)
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    tx = _tendency(out, inp, "MOMX", dt)[sl]
+    # the terrain-following gradient alone (what a dropped J13G term would leave)
+    tf = np.abs((p[:, g.IS + 1:g.IE + 2] - p[:, g.IS:g.IE + 1]) / g.dx)[g.KS:g.KE + 1]
+    return np.abs(tx).max() / tf.max()
+
+
+def test_hevi_terrain_pressure_gradient_is_second_order():
+    """Over a bell mountain (the metrics of orc_terrain_metrics) a pressure
+    that depends on physical height only has no horizontal gradient: the
+    HEVI MOMX tendency (G^1/2 dp/dx|zeta plus the J13G term, HEVI.F90:880-895)
+    is a small fraction of either part and falls by about 4 per halving of
+    the grid spacing.  A sign or staggering error in the J13G term leaves an
+    O(1) fraction."""
+    e = [_terrain_pg_error(n) for n in (1, 2, 4)]
+    assert e[0] < 0.05, e
+    for a, b in zip(e, e[1:]):
+        assert 3.0 < a / b < 5.0, e
