@@ -161,10 +161,7 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
         _, rev_ms = ctx.profile_query("model_adjoint")
         ctx.profile(False)
         # gradient w.r.t. the interior: fold the halo and ghost-level adjoints
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        from test_adjoint_oracle import fill, interior_gradient
-        grad = interior_gradient(g, xb0)
+        grad = inputs.interior_gradient(g, xb0)
         rng = np.random.default_rng(seed)
         scale = {k: float(np.abs(x0[k]).max()) for k in names}
         ads, fd2, fd4 = [], [], []
@@ -184,7 +181,7 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
                 ke = g.KE - 1 if k == "MOMZ" else g.KE
                 sl = (slice(g.KS, ke + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
                 a[sl] = rng.standard_normal(a[sl].shape) * scale[k]
-                d[k] = fill(g, a, k)
+                d[k] = inputs.fill_ghost_and_halo(g, a, k)
             ads.append(sum(float(np.sum(grad[k] * d[k])) for k in names))
             # second-order central difference at eps = e2
             fd2.append((J_at(d, e2) - J_at(d, -e2)) / (2.0 * e2))
@@ -211,25 +208,51 @@ def fd_summary(ads, fd2, fd4, eps_list):
     return out
 
 
-def cpu_baseline(ni, nj, nk):
-    """The oracle (C, OpenMP) on a bounded sub-domain of the same workload."""
+def host_cpu():
+    """(logical CPUs the OS reports, CPU model name) of this host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+# host threads of the CPU baseline: the GPU box gives one GPU's job a share of
+# 16 host threads, whatever the OS reports, so the baseline uses at most 16
+CPU_THREADS_MAX = 16
+
+
+def cpu_baseline(ni, nj, nk, reps=5):
+    """The oracle (C, OpenMP) on a bounded sub-domain of the same workload:
+    one warm-up short step, then the median of `reps` timed short steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     from scale_amd import inputs
-    threads = min(os.cpu_count() or 1, 16)
+    nproc, model = host_cpu()
+    threads = min(nproc, CPU_THREADS_MAX)
     po.set_threads(threads)
     case = inputs.shear_512x512x128(imax=ni, jmax=nj, kmax=nk, ntracer=0)
     m = po.Model(case)
     m.step(1)  # warm-up
-    n = 1
-    t0 = time.perf_counter()
-    m.step(n)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        m.step(1)
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    dt = times[len(times) // 2]
     # tracers off: one large step = one short step (numfilter + RK4) plus the cheap preparation
     pts = ni * nj * nk
-    return dict(value=pts * n / dt, unit="grid-pt short-step updates/s", cores=threads, kind="port",
-                sample=f"oracle (C, OpenMP {threads} threads) on a {ni}x{nj}x{nk} sub-domain of the workload, "
-                       f"{n} short step(s) incl. the large-step preparation, no tracers")
+    return dict(value=pts / dt, unit="grid-pt short-step updates/s", cores=threads, kind="port",
+                host_nproc=nproc, host_cpu=model, step_s_median=dt, step_s_all=times,
+                sample=f"oracle (C, OpenMP {threads} threads of {nproc} reported; one GPU's job gets a 16-thread share) "
+                       f"on a {ni}x{nj}x{nk} sub-domain of the workload, median of {reps} short steps after 1 warm-up, "
+                       f"each incl. the large-step preparation, no tracers")
 
 
 def main():

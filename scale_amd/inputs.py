@@ -97,6 +97,52 @@ def halo_fill(g: Grid, a: np.ndarray) -> np.ndarray:
     return a
 
 
+def halo_fill_adjoint(g: Grid, ab: np.ndarray) -> np.ndarray:
+    """Transpose of halo_fill, in place: each halo value is added to the
+    point it was copied from, the y rows first (they were filled last and
+    their sources include the x halo), then the x strips; the halo ends zero."""
+    IS, IE, JS, JE, h = g.IS, g.IE, g.JS, g.JE, g.hx
+    ab[:, :, JS:JS + h] += ab[:, :, JE + 1:]
+    ab[:, :, JE + 1:] = 0.0
+    ab[:, :, JE - h + 1:JE + 1] += ab[:, :, :JS]
+    ab[:, :, :JS] = 0.0
+    ab[:, IS:IS + h, JS:JE + 1] += ab[:, IE + 1:, JS:JE + 1]
+    ab[:, IE + 1:, JS:JE + 1] = 0.0
+    ab[:, IE - h + 1:IE + 1, JS:JE + 1] += ab[:, :IS, JS:JE + 1]
+    ab[:, :IS, JS:JE + 1] = 0.0
+    return ab
+
+
+def fill_ghost_and_halo(g: Grid, a: np.ndarray, var: str) -> np.ndarray:
+    """Ghost levels and periodic halo of a prognostic as a large step leaves
+    them (LARGE.F90:908-927, then COMM_vars8), in place: MOMZ is zero below KS
+    and from KE up, the other fields repeat KS below and KE above."""
+    if var == "MOMZ":
+        a[:g.KS] = 0.0
+        a[g.KE:] = 0.0
+    else:
+        a[:g.KS] = a[g.KS]
+        a[g.KE + 1:] = a[g.KE]
+    return halo_fill(g, a)
+
+
+def interior_gradient(g: Grid, xb: dict) -> dict:
+    """Gradient with respect to the interior prognostics (MOMZ on KS..KE-1)
+    from adjoints of whole fields: the transpose of fill_ghost_and_halo."""
+    grad = {}
+    for k in ("DENS", "MOMZ", "MOMX", "MOMY", "RHOT"):
+        b = halo_fill_adjoint(g, np.array(xb[k], order="F", copy=True))
+        if k != "MOMZ":
+            b[g.KS] += b[:g.KS].sum(axis=0)
+            b[g.KE] += b[g.KE + 1:].sum(axis=0)
+        b[:g.KS] = 0.0
+        b[g.KE + 1:] = 0.0
+        if k == "MOMZ":
+            b[g.KE] = 0.0
+        grad[k] = b
+    return grad
+
+
 def vertical_grid(g: Grid):
     """FZ(0:KA), CZ, CDZ, FDZ, RCDZ, RFDZ (0-based: FZ[kF] for kF=0..KA)."""
     KA, KS, KE = g.KA, g.KS, g.KE

@@ -200,3 +200,31 @@ def test_large_step_adjoint_matches_finite_differences():
         Jm = theta_box_objective(case, run({k: np.asfortranarray(traj[0][k] - eps * d[k]) for k in STATE}, N).state)[0]
         fd = (Jp - Jm) / (2 * eps)
         assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+
+
+def test_package_gradient_helpers_match_the_oracle_transposes():
+    """scale_amd.inputs.fill_ghost_and_halo / halo_fill_adjoint /
+    interior_gradient (used by the bench's gradient check) against the
+    oracle's halo fill and its transpose: <F x, y> = <x, F^T y> and equality
+    with this module's fill / interior_gradient, on halo 2 and halo 3."""
+    for hx in (2, 3):
+        case = inputs.make_case(kmax=8, imax=7, jmax=6, dx=500.0, dy=500.0, dz=250.0, seed=3, halo=hx)
+        g = case.grid
+        rng = np.random.default_rng(17 + hx)
+        for k in STATE:
+            x = np.asfortranarray(rng.standard_normal(g.shape))
+            y = np.asfortranarray(rng.standard_normal(g.shape))
+            fx = inputs.fill_ghost_and_halo(g, np.array(x, order="F", copy=True), k)
+            assert np.array_equal(fx, fill(g, np.array(x, order="F", copy=True), k))
+            gy = inputs.interior_gradient(g, {v: y for v in STATE})[k]
+            assert np.array_equal(gy, interior_gradient(g, {v: y for v in STATE})[k])
+            # the adjoint identity on the interior part of x
+            xi = np.zeros_like(x)
+            ke = g.KE - 1 if k == "MOMZ" else g.KE
+            sl = (slice(g.KS, ke + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+            xi[sl] = x[sl]
+            lhs = float(np.sum(inputs.fill_ghost_and_halo(g, np.array(xi, order="F", copy=True), k) * y))
+            rhs = float(np.sum(xi * gy))
+            assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1.0)
+        hb = inputs.halo_fill_adjoint(g, np.array(y, order="F", copy=True))
+        assert np.array_equal(hb, O.adj_halo_periodic(g, np.array(y, order="F", copy=True)))
