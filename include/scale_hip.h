@@ -130,6 +130,28 @@ int scale_hip_tstep_tracer_heve(scale_hip_ctx *ctx,
     const double *RCDZ, const double *RCDX, const double *RCDY,
     const double *dtl, const int32_t *FLAG_FCT_TRACER, const int32_t *FLAG_FCT_ALONG_STREAM);
 
+/* The tracer stage of scale_hip_tstep_tracer_heve on device memory, with no
+ * allocation of any kind.  This entry point has no counterpart in the
+ * reference: it is for callers that cannot allocate while they step (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same dummies in the same order, but every array is a device
+ * pointer in the library's device layout (element (k,i,j) at
+ * i + IA*(j + JA*k), components of mflx_hi/num_diff/GSQRT as consecutive
+ * KA*IA*JA planes; MAPF(IA,JA,2) i fastest; RCDZ(KA), RCDX(IA), RCDY(JA));
+ * scalars stay host pointers.  QTRCo must not alias QTRC, QTRC0, DENS0 or
+ * DENS (rejected).  The FCT limiter's three work arrays live in a fixed
+ * device array of the library, so with FLAG_FCT_TRACER the grid is limited
+ * to KA*IA*JA <= 4194304 (larger grids are rejected with an error).  The
+ * call is ordered on the context's stream and returns without waiting
+ * (scale_hip_synchronize waits).  All calls on one GPU share that array:
+ * calls that overlap on one GPU (two contexts, two host threads) race. */
+int scale_hip_tstep_tracer_heve_noalloc(scale_hip_ctx *ctx,
+    double *QTRCo, const double *QTRC, const double *QTRC0, const double *RHOQ_t,
+    const double *DENS0, const double *DENS, const double *mflx_hi, const double *num_diff,
+    const double *GSQRT, const double *MAPF, const double *CDZ,
+    const double *RCDZ, const double *RCDX, const double *RCDY,
+    const double *dtl, const int32_t *FLAG_FCT_TRACER, const int32_t *FLAG_FCT_ALONG_STREAM);
+
 /* Hyperdiffusion fluxes num_diff(KA,IA,JA,5,3) including their halo exchange:
  * ATMOS_DYN_numfilter_coef, scalelib/src/atmos-rm/dynamics/scale_atmos_dyn_common.F90:385-985
  * (tables of ATMOS_DYN_filter_setup, :92-319).  ND_ORDER = 1 and

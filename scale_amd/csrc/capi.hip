@@ -4,6 +4,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "../../include/scale_hip.h"
@@ -37,6 +38,30 @@ int hipfail(hipError_t e, const char* where) {
 
 using shp::Ctx;
 using shp::Dims;
+
+// Work arrays of scale_hip_tstep_tracer_heve_noalloc: phi_lo, R+ and R- of
+// the FCT limiter, 3 fields of at most NOALLOC_MAX_N3 doubles each (96 MiB).
+// Each device holds its own copy from the first kernel launch until the
+// process ends.  Host code uses the per-device address g_noalloc_ptr[dev]
+// taken once by scale_hip_init, never the symbol itself.
+constexpr size_t NOALLOC_MAX_N3 = size_t(1) << 22;
+constexpr size_t NOALLOC_FIELDS = 3;
+static __device__ __align__(256) double scratch[NOALLOC_FIELDS * NOALLOC_MAX_N3];
+constexpr int NOALLOC_MAX_DEV = 64;
+double* g_noalloc_ptr[NOALLOC_MAX_DEV] = {};
+std::mutex g_noalloc_mu;
+
+// device address of this device's scratch (the device must be current)
+hipError_t noalloc_setup(int dev) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEV) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[dev]) return hipSuccess;
+  double* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_noalloc_ptr[dev] = ptr;
+  return hipSuccess;
+}
 
 double* dalloc(Ctx& c, size_t n, std::vector<void*>* track) {
   void* p = nullptr;
@@ -306,6 +331,7 @@ int scale_hip_init(const scale_hip_grid* g, scale_hip_ctx** out) {
   if ((g->flux_scheme_tracer == SCALE_HIP_UD5 || g->flux_scheme_tracer == SCALE_HIP_CD6) && (H != 3 || KE - KS + 1 < 5))
     return fail("the UD5 and CD6 tracer schemes need IHALO = JHALO = 3 and KMAX >= 5");
   if (hipSetDevice(g->device) != hipSuccess) return fail("hipSetDevice failed");
+  if (hipError_t e = noalloc_setup(g->device); e != hipSuccess) return hipfail(e, "hipGetSymbolAddress(scratch)");
   scale_hip_ctx* h = new scale_hip_ctx();
   Ctx& c = h->c;
   c.device = g->device;
@@ -597,6 +623,40 @@ int scale_hip_tstep_tracer_heve(scale_hip_ctx* h,
   io.mflx_hi = dMF; io.num_diff = dND; io.dtl = *dtl; io.flag_fct = *FLAG_FCT_TRACER; io.flag_along = *FLAG_FCT_ALONG_STREAM;
   CK(shp::tracer_stage(c, io, c.stream));
   return download3(c, QTRCo, dQo, 1);
+}
+
+int scale_hip_tstep_tracer_heve_noalloc(scale_hip_ctx* h,
+    double* QTRCo, const double* QTRC, const double* QTRC0, const double* RHOQ_t,
+    const double* DENS0, const double* DENS, const double* mflx_hi, const double* num_diff,
+    const double* GSQRT, const double* MAPF, const double* CDZ,
+    const double* RCDZ, const double* RCDX, const double* RCDY,
+    const double* dtl, const int32_t* FLAG_FCT_TRACER, const int32_t* FLAG_FCT_ALONG_STREAM) {
+  (void)CDZ;  // unused by the stage, as in the reference (tracer_fvm_heve.F90:72-242)
+  if (int r = check_ctx(h)) return r;
+  if (!dtl || !FLAG_FCT_TRACER || !FLAG_FCT_ALONG_STREAM) return fail("null scalar argument");
+  if (!QTRCo || !QTRC || !QTRC0 || !RHOQ_t || !DENS0 || !DENS || !mflx_hi || !num_diff || !GSQRT || !MAPF || !RCDZ ||
+      !RCDX || !RCDY)
+    return fail("null array argument");
+  if (QTRCo == QTRC || QTRCo == QTRC0 || QTRCo == DENS0 || QTRCo == DENS)
+    return fail("QTRCo must not alias an input of the no-allocation tracer stage");
+  Ctx& c = h->c;
+  const bool fct = *FLAG_FCT_TRACER != 0;
+  if (fct && c.d.n3 > NOALLOC_MAX_N3)
+    return fail("scale_hip_tstep_tracer_heve_noalloc: KA*IA*JA = " + std::to_string(c.d.n3) + " exceeds the maximum " +
+                std::to_string(NOALLOC_MAX_N3) + " of its FCT work arrays");
+  double* ptr = (c.device >= 0 && c.device < NOALLOC_MAX_DEV) ? g_noalloc_ptr[c.device] : nullptr;
+  if (!ptr) return fail("scale_hip_tstep_tracer_heve_noalloc: no work array address for this device");
+  CK(hipSetDevice(c.device));
+  shp::TracerIO io;
+  io.QTRCo = QTRCo; io.QTRC = QTRC; io.QTRC0 = QTRC0; io.RHOQ_t = RHOQ_t; io.DENS0 = DENS0; io.DENS = DENS;
+  io.mflx_hi = mflx_hi; io.num_diff = num_diff; io.dtl = *dtl; io.flag_fct = fct; io.flag_along = *FLAG_FCT_ALONG_STREAM;
+  // the named timers create events and grow host tables: off for this call
+  shp::Prof* prof = c.prof;
+  c.prof = nullptr;
+  hipError_t e = shp::tracer_stage_ext(c, io, GSQRT, MAPF, RCDZ, RCDX, RCDY, ptr, c.stream);
+  c.prof = prof;
+  CK(e);
+  return 0;
 }
 
 int scale_hip_numfilter_coef(scale_hip_ctx* h, double* num_diff,

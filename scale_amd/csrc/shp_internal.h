@@ -145,6 +145,10 @@ hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], 
 hipError_t numfilter(const Ctx& c, float* num_diff, const float* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s, const double* offD, const double* offR);
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s);
+// the same on caller-supplied device metrics (GSQRT(..,7), MAPF(IA,JA,2),
+// RCDZ/RCDX/RCDY) and work arrays (3 n3 doubles: phi_lo, R+, R-)
+hipError_t tracer_stage_ext(const Ctx& c, const TracerIO& io, const double* GSQRT, const double* MAPF_XY,
+                            const double* RCDZ, const double* RCDX, const double* RCDY, double* work3, hipStream_t s);
 // ATMOS_DYN_numfilter_coef_q (ND_ORDER = 1) with the halo fill of its 3
 // fields; REF_qv non-null for QV.  Needs the filter tables and c.ndqc.
 hipError_t numfilter_q(const Ctx& c, double* num_diff_q, const double* DENS, const double* QTRC, const double* REF_qv,

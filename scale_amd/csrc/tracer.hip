@@ -379,6 +379,30 @@ static hipError_t tracer_t(const Ctx& c, const TrArgs& a, bool fct, bool along, 
   return hipGetLastError();
 }
 
+// caller-supplied metrics and work arrays: always the general-metric kernels
+// (with GSQRT = MAPF = 1 their products and quotients by 1.0 are exact, so
+// the results equal the flat kernels' bit for bit)
+hipError_t tracer_stage_ext(const Ctx& c, const TracerIO& io, const double* GSQRT, const double* MAPF_XY,
+                            const double* RCDZ, const double* RCDX, const double* RCDY, double* work3, hipStream_t s) {
+  TrArgs a;
+  a.d = c.d;
+  a.QTRC = io.QTRC; a.QTRC0 = io.QTRC0; a.RHOQ_t = io.RHOQ_t; a.DENS0 = io.DENS0; a.DENS = io.DENS;
+  a.mflx = io.mflx_hi; a.nd = io.num_diff; a.nds = c.d.n3; a.GSQRT = GSQRT; a.MAPF_XY = MAPF_XY;
+  a.RCDZ = RCDZ; a.RCDX = RCDX; a.RCDY = RCDY;
+  a.dtl = io.dtl; a.QTRCo = io.QTRCo;
+  a.phi_lo = work3; a.rjpls = work3 + c.d.n3; a.rjmns = work3 + 2 * c.d.n3;
+  const bool fct = io.flag_fct != 0, along = io.flag_along != 0;
+  switch (c.scheme_tracer) {
+    case UD3: return tracer_t<UD3, true>(c, a, fct, along, s);
+    case UD3KOREN: return tracer_t<UD3KOREN, true>(c, a, fct, along, s);
+    case CD2: return tracer_t<CD2, true>(c, a, fct, along, s);
+    case UD1: return tracer_t<UD1, true>(c, a, fct, along, s);
+    case UD5: return tracer_t<UD5, true>(c, a, fct, along, s);
+    case CD6: return tracer_t<CD6, true>(c, a, fct, along, s);
+    default: return tracer_t<CD4, true>(c, a, fct, along, s);
+  }
+}
+
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s) {
   TrArgs a;
   a.d = c.d;
