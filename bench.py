@@ -281,8 +281,10 @@ def main():
         cnt, ms = ctx.profile_query(name)
         kern[name] = dict(count=cnt, avg_ms=(ms / cnt if cnt else None))
     ctx.profile(False)
-    # large step incl. 3 tracers with FCT (informational)
+    # large step incl. 3 tracers with FCT (informational), then the same
+    # steps with the kernel timers on for the tracer kernels' averages
     large_ms = None
+    tracer_kern = None
     if not a.no_large_step:
         ctx.synchronize()
         t2 = time.perf_counter()
@@ -290,12 +292,28 @@ def main():
         ctx.model_large_step(nls)
         ctx.synchronize()
         large_ms = (time.perf_counter() - t2) * 1e3 / nls
-    # achievable HBM bandwidth: streaming copy of 2 x 2 GiB
+        ctx.profile(True)
+        ctx.model_large_step(nls)
+        ctx.synchronize()
+        tracer_kern = {}
+        for name in ("tracer_nofct", "fct_philo", "fct_ratio", "fct_update"):
+            cnt, ms = ctx.profile_query(name)
+            if cnt:
+                tracer_kern[name] = ms / cnt
+        ctx.profile(False)
+    # achievable HBM bandwidth: streaming copy of 2 x 2 GiB, and the other
+    # probes (non-temporal copy, read-only, write-only, hipMemcpy D2D)
     copy_gbs = None
+    probes = None
     if not a.no_copy_probe:
         n_copy = 1 << 28
         copy_ms = ctx.hbm_copy_probe(n_copy, 10)
         copy_gbs = 16.0 * n_copy / (copy_ms * 1e-3) / 1e9
+        probes = {}
+        for kind in hip.Context.HBM_PROBES:
+            ms = ctx.hbm_probe(kind, n_copy, 10)
+            nbytes = (8 if kind in ("read", "write") else 16) * n_copy
+            probes[kind] = {"ms": ms, "GBs": nbytes / (ms * 1e-3) / 1e9}
     ctx.close()
     # informational: large steps with the tracers on another flux scheme
     tracer_leg = None
@@ -353,6 +371,7 @@ def main():
                 bytes_per_point=STAGE_LAUNCH_BYTES_PER_POINT,
                 stage_ms=st_ms, kernels_avg_ms={k: v["avg_ms"] for k, v in kern.items()},
                 copy_probe_GBs=copy_gbs,
+                bandwidth_probes=probes,
                 frac_of_copy=(achieved / copy_gbs if copy_gbs else None))
     out = {
         "metric": "grid-pt updates/s short step (%HBM roofline); FP32 vs FP64 speedup; adjoint/fwd",
@@ -372,6 +391,7 @@ def main():
                    "grid": [ni, nj, nk], "short_step_bytes_per_point": STEP_BYTES_PER_POINT,
                    "short_step_hbm_frac": value * STEP_BYTES_PER_POINT / 1e9 / HBM_PEAK_GBS,
                    "large_step_with_3_fct_tracers_ms": large_ms,
+                   "tracer_kernels_avg_ms": tracer_kern,
                    **({"large_step_other_tracer_scheme": tracer_leg} if tracer_leg else {})},
         "roofline": roof,
         "fp32_vs_fp64": mixed,

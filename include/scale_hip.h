@@ -313,8 +313,11 @@ int scale_hip_model_adjoint(scale_hip_ctx *ctx, double *DENS_b, double *MOMZ_b, 
  * every tile shape computes the same operations in the same order.
  *   SCALE_HIP_TUNE_NF_TJ: rows of the numerical-filter tile, 4, 8 or 16
  *   (0 = chosen from the grid).
+ *   SCALE_HIP_TUNE_TRACER_TILE: the FCT tracer stage on LDS tiles (flat
+ *   terrain; UD3, CD4, UD3KOREN1993, UD1 and CD2): 1 (default) where the grid
+ *   gives every CU two tiles, 2 always, 0 never (per-point kernels).
  * Unknown knobs and values outside the listed ones are rejected. */
-enum { SCALE_HIP_TUNE_NF_TJ = 1 };
+enum { SCALE_HIP_TUNE_NF_TJ = 1, SCALE_HIP_TUNE_TRACER_TILE = 2 };
 int scale_hip_tune_set(scale_hip_ctx *ctx, int32_t knob, int32_t value);
 int scale_hip_tune_get(scale_hip_ctx *ctx, int32_t knob, int32_t *value);
 
@@ -324,6 +327,13 @@ int scale_hip_tune_get(scale_hip_ctx *ctx, int32_t knob, int32_t *value);
  * the achievable copy bandwidth and a kernel of known traffic to calibrate
  * the FETCH_SIZE / WRITE_SIZE counters of the stencil kernels against. */
 int scale_hip_hbm_copy_probe(scale_hip_ctx *ctx, int64_t n, int32_t reps, double *avg_ms);
+
+/* Bandwidth probes on two fresh buffers of n doubles, timed by events over
+ * reps launches after one warm-up (average ms per launch): kind 0 the copy
+ * above, 1 the copy with non-temporal loads and stores, 2 read-only (a
+ * block-sum reduction), 3 write-only (a fill), 4 hipMemcpyAsync device to
+ * device.  No counterpart in the reference: it measures the device. */
+int scale_hip_hbm_probe(scale_hip_ctx *ctx, int32_t kind, int64_t n, int32_t reps, double *avg_ms);
 
 #ifdef __cplusplus
 }

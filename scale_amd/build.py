@@ -1,40 +1,62 @@
 """Builds libscalehip.so (all HIP sources for gfx950) in-tree with hipcc.
 
--ffp-contract=off keeps every multiply and add separately rounded, as in the
-reference Fortran; -Xclang -target-feature -Xclang -packed-fp32-ops keeps the
-compiler from emitting packed FP32 instructions.
+Each source compiles to an object in scale_amd/_build/ in parallel, then one
+hipcc line links the shared library.  -ffp-contract=off keeps every multiply
+and add separately rounded, as in the reference Fortran; -Xclang
+-target-feature -Xclang -packed-fp32-ops keeps the compiler from emitting
+packed FP32 instructions.
 """
 from __future__ import annotations
 
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libscalehip.so")
-SOURCES = ["hevi.hip", "numfilter.hip", "tracer.hip", "driver.hip", "adjoint.hip", "profile.hip", "capi.hip"]
+SOURCES = ["hevi.hip", "numfilter.hip", "tracer.hip", "tracer_tile.hip", "driver.hip", "adjoint.hip", "profile.hip",
+           "capi.hip"]
 ARCH = os.environ.get("SCALE_HIP_ARCH", "gfx950")
 
 FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-shared", "-std=c++17",
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
 
 
-def build(verbose: bool = False, extra=None) -> str:
+def build(verbose: bool = False, extra=None, jobs: int | None = None) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = jobs or min(len(SOURCES), os.cpu_count() or 1, 8)
+
+    def compile_one(src):
+        obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc, *FLAGS, *(extra or []), "-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        return obj, r
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(compile_one, SOURCES))
+    failed = [r for _, r in results if r.returncode != 0]
+    for r in failed:
+        sys.stderr.write(r.stdout + r.stderr)
+    if failed:
+        raise RuntimeError("hipcc failed building libscalehip.so")
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *FLAGS, *(extra or []), "-o", tmp, *srcs]
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *[o for o, _ in results]]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libscalehip.so")
+        raise RuntimeError("hipcc failed linking libscalehip.so")
     os.replace(tmp, LIB)
     return LIB
 

@@ -1023,6 +1023,10 @@ int scale_hip_tune_set(scale_hip_ctx* h, int32_t knob, int32_t value) {
       if (value != 0 && value != 4 && value != 8 && value != 16) return fail("SCALE_HIP_TUNE_NF_TJ must be 0, 4, 8 or 16");
       h->c.nf_tj = value;
       return 0;
+    case SCALE_HIP_TUNE_TRACER_TILE:
+      if (value < 0 || value > 2) return fail("SCALE_HIP_TUNE_TRACER_TILE must be 0, 1 or 2");
+      h->c.tracer_tile = value;
+      return 0;
     default:
       return fail("unknown tuning knob " + std::to_string(knob));
   }
@@ -1033,6 +1037,7 @@ int scale_hip_tune_get(scale_hip_ctx* h, int32_t knob, int32_t* value) {
   if (!value) return fail("null argument");
   switch (knob) {
     case SCALE_HIP_TUNE_NF_TJ: *value = h->c.nf_tj; return 0;
+    case SCALE_HIP_TUNE_TRACER_TILE: *value = h->c.tracer_tile; return 0;
     default: return fail("unknown tuning knob " + std::to_string(knob));
   }
 }
@@ -1062,6 +1067,37 @@ int scale_hip_hbm_copy_probe(scale_hip_ctx* h, int64_t n, int32_t reps, double* 
   hipFree(src);
   hipFree(dst);
   if (e != hipSuccess) return hipfail(e, "hbm_copy_probe");
+  *avg_ms = (double)ms / reps;
+  return 0;
+}
+
+int scale_hip_hbm_probe(scale_hip_ctx* h, int32_t kind, int64_t n, int32_t reps, double* avg_ms) {
+  if (int r = check_ctx(h)) return r;
+  if (kind < 0 || kind > 4 || n <= 1 || reps <= 0 || !avg_ms)
+    return fail("hbm_probe: kind 0..4, n > 1, reps > 0 and avg_ms are required");
+  Ctx& c = h->c;
+  double *src = nullptr, *dst = nullptr, *part = nullptr;
+  CK(hipMalloc((void**)&src, (size_t)n * sizeof(double)));
+  hipError_t e = hipMalloc((void**)&dst, (size_t)n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&part, 4096 * sizeof(double));
+  hipEvent_t a = nullptr, b = nullptr;
+  float ms = 0.f;
+  if (e == hipSuccess && (e = hipMemsetAsync(src, 0, (size_t)n * sizeof(double), c.stream)) == hipSuccess &&
+      (e = shp::hbm_probe(c, kind, dst, src, part, (size_t)n, c.stream)) == hipSuccess &&  // warm-up
+      (e = hipEventCreate(&a)) == hipSuccess && (e = hipEventCreate(&b)) == hipSuccess &&
+      (e = hipEventRecord(a, c.stream)) == hipSuccess) {
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = shp::hbm_probe(c, kind, dst, src, part, (size_t)n, c.stream);
+    if (e == hipSuccess) e = hipEventRecord(b, c.stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  }
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  hipStreamSynchronize(c.stream);
+  if (src) hipFree(src);
+  if (dst) hipFree(dst);
+  if (part) hipFree(part);
+  if (e != hipSuccess) return hipfail(e, "hbm_probe");
   *avg_ms = (double)ms / reps;
   return 0;
 }

@@ -116,6 +116,72 @@ __global__ void __launch_bounds__(256) hbm_copy_kernel(double* __restrict__ dst,
   if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) dst[n - 1] = src[n - 1];
 }
 
+// The same copy with non-temporal loads and stores (no L2 allocation for
+// streaming data).
+__global__ void __launch_bounds__(256) hbm_copy_nt_kernel(double* __restrict__ dst, const double* __restrict__ src, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const double a = __builtin_nontemporal_load(src + i), b = __builtin_nontemporal_load(src + i + stride);
+    const double c = __builtin_nontemporal_load(src + i + 2 * stride), e = __builtin_nontemporal_load(src + i + 3 * stride);
+    __builtin_nontemporal_store(a, dst + i);
+    __builtin_nontemporal_store(b, dst + i + stride);
+    __builtin_nontemporal_store(c, dst + i + 2 * stride);
+    __builtin_nontemporal_store(e, dst + i + 3 * stride);
+  }
+  for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+// Read-only stream: each block sums its share of src (16-byte loads, four in
+// flight per lane) and writes one partial sum, so the traffic is the read.
+__global__ void __launch_bounds__(256) hbm_read_kernel(double* __restrict__ part, const double* __restrict__ src, size_t n) {
+  const size_t n2 = n / 2;
+  const double2* s2 = reinterpret_cast<const double2*>(src);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  for (; i + 3 * stride < n2; i += 4 * stride) {
+    const double2 a = s2[i], b = s2[i + stride], c = s2[i + 2 * stride], e = s2[i + 3 * stride];
+    acc += (a.x + a.y) + (b.x + b.y) + (c.x + c.y) + (e.x + e.y);
+  }
+  for (; i < n2; i += stride) acc += s2[i].x + s2[i].y;
+  __shared__ double sh[256];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Write-only stream: fill dst with a value (16-byte stores).
+__global__ void __launch_bounds__(256) hbm_fill_kernel(double* __restrict__ dst, size_t n, double v) {
+  const size_t n2 = n / 2;
+  double2* d2 = reinterpret_cast<double2*>(dst);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const double2 w = make_double2(v, v);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) d2[i] = w;
+}
+
+// One launch of probe `kind` over n doubles (0 copy, 1 non-temporal copy,
+// 2 read-only, 3 write-only, 4 hipMemcpyAsync device to device); part needs
+// ncu*8 doubles for kind 2.
+hipError_t hbm_probe(const Ctx& c, int kind, double* dst, const double* src, double* part, size_t n, hipStream_t s) {
+  int ncu = 256;
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
+  const unsigned nb = (unsigned)ncu * 8u;
+  switch (kind) {
+    case 0: hipLaunchKernelGGL(hbm_copy_kernel, dim3(nb), dim3(256), 0, s, dst, src, n); break;
+    case 1: hipLaunchKernelGGL(hbm_copy_nt_kernel, dim3(nb), dim3(256), 0, s, dst, src, n); break;
+    case 2: hipLaunchKernelGGL(hbm_read_kernel, dim3(nb), dim3(256), 0, s, part, src, n); break;
+    case 3: hipLaunchKernelGGL(hbm_fill_kernel, dim3(nb), dim3(256), 0, s, dst, n, 1.0); break;
+    case 4: return hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t hbm_copy(const Ctx& c, double* dst, const double* src, size_t n, hipStream_t s) {
   int ncu = 256;
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);

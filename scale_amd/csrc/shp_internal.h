@@ -46,6 +46,7 @@ struct Ctx {
   Prof* prof = nullptr;
   // launch-shape knobs (scale_hip_tune_set); 0 = chosen from the grid
   int nf_tj = 0;  // numerical-filter tile height: 4, 8 or 16 rows
+  int tracer_tile = 1;  // tracer stage on LDS tiles where supported (0: per-point kernels)
   // host copies of 1-D grid arrays (for the numerical-filter tables)
   std::vector<double> hCDZ, hFDZ, hCDX, hFDX, hCDY, hFDY;
 };
@@ -139,12 +140,18 @@ int prof_query(Prof* p, const char* name, long* count, double* total_ms);
 int prof_names(Prof* p, char* buf, int len);
 // streaming copy of n doubles (timed as "hbm_copy")
 hipError_t hbm_copy(const Ctx& c, double* dst, const double* src, size_t n, hipStream_t s);
+// bandwidth probes: 0 copy, 1 non-temporal copy, 2 read-only, 3 write-only, 4 hipMemcpyAsync D2D
+hipError_t hbm_probe(const Ctx& c, int kind, double* dst, const double* src, double* part, size_t n, hipStream_t s);
 hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s);
 // FP32: DENS and RHOT of st are deviations from the level profiles offD, offR
 hipError_t numfilter(const Ctx& c, float* num_diff, const float* const st[5], const double* REF_dens,
                      const double* REF_pott, hipStream_t s, const double* offD, const double* offR);
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s);
+// the LDS-tiled tracer stage (flat terrain; UD3, CD4, UD3KOREN1993, UD1, CD2)
+// with R+/R- in the given fields; tracer_stage uses it where it applies
+bool tracer_tile_supported(const Ctx& c);
+hipError_t tracer_stage_tile(const Ctx& c, const TracerIO& io, double* rjpls, double* rjmns, hipStream_t s);
 // the same on caller-supplied device metrics (GSQRT(..,7), MAPF(IA,JA,2),
 // RCDZ/RCDX/RCDY) and work arrays (3 n3 doubles: phi_lo, R+, R-)
 hipError_t tracer_stage_ext(const Ctx& c, const TracerIO& io, const double* GSQRT, const double* MAPF_XY,

@@ -18,6 +18,7 @@
 // The high- and low-order face fluxes are recomputed where needed instead of
 // being stored; only phi_lo and R+/R- go through memory.
 #include "common.h"
+#include "fct.h"
 #include "shp_internal.h"
 
 namespace shp {
@@ -165,48 +166,6 @@ __global__ void __launch_bounds__(256) fct_philo(TrArgs a, unsigned ni_tiles, un
                               + (o.lY(k, i, j) - o.lY(k, i, j - 1)) * a.RCDY[j])
                             * o.m1(i, j) * o.m2(i, j) / o.gs(I_XYZ, p)))
               / a.DENS[p];
-}
-
-// get_fact_fct (COMMON.F90:2676-2734); fact(a,b,c) -> f[a + 2*((b+1) + 3*(c+1))]
-__device__ __forceinline__ void get_fact(double* f, double rw, double ru, double rv) {
-  const double ugev = fsign(0.5, __builtin_fabs(ru) - __builtin_fabs(rv)) + 0.5;
-  const double ugew = fsign(0.5, __builtin_fabs(ru) - __builtin_fabs(rw)) + 0.5;
-  const double vgew = fsign(0.5, __builtin_fabs(rv) - __builtin_fabs(rw)) + 0.5;
-  const double uzero = fsign(0.5, __builtin_fabs(ru) - EPS_RP) - 0.5;
-  const double vzero = fsign(0.5, __builtin_fabs(rv) - EPS_RP) - 0.5;
-  const double wzero = fsign(0.5, __builtin_fabs(rw) - EPS_RP) - 0.5;
-  const double sign_uv = fsign(0.5, ru * rv) + 0.5;
-  const double sign_uw = fsign(0.5, ru * rw) + 0.5;
-  const double sign_vw = fsign(0.5, rv * rw) + 0.5;
-  const double wu = __builtin_fabs(rw / (ru + uzero) * (1.0 + uzero));
-  const double vu = __builtin_fabs(rv / (ru + uzero) * (1.0 + uzero));
-  const double uv = __builtin_fabs(ru / (rv + vzero) * (1.0 + vzero));
-  const double wv = __builtin_fabs(rw / (rv + vzero) * (1.0 + vzero));
-  const double uw = __builtin_fabs(ru / (rw + wzero) * (1.0 + wzero));
-  const double vw = __builtin_fabs(rv / (rw + wzero) * (1.0 + wzero));
-  const double umax = ugev * ugew * (1.0 + uzero);
-  const double vmax = (1.0 - ugev) * vgew;
-  const double wmax = 1.0 - ugev * ugew - vmax;
-#define FCT_(a_, b_, c_) f[(a_) + 2 * (((b_) + 1) + 3 * ((c_) + 1))]
-  FCT_(0, 0, 0) = -ugev * ugew * uzero;
-  FCT_(1, 0, 0) = wmax * (1.0 - uw) * (1.0 - vw);
-  FCT_(0, 1, 0) = umax * (1.0 - vu) * (1.0 - wu);
-  FCT_(0, 0, 1) = vmax * (1.0 - uv) * (1.0 - wv);
-  const double c3 = (umax * vu * wu + vmax * uv * wv + wmax * uw * vw);
-  FCT_(1, 1, 1) = sign_uv * sign_uw * c3;
-  FCT_(1, -1, 1) = (1.0 - sign_uv) * (1.0 - sign_uw) * c3;
-  FCT_(1, 1, -1) = (1.0 - sign_uv) * sign_uw * c3;
-  FCT_(1, -1, -1) = sign_uv * (1.0 - sign_uw) * c3;
-  const double cuw = (ugew * wu * (1.0 - vu) + (1.0 - ugew) * uw * (1.0 - vw));
-  FCT_(1, 1, 0) = sign_uw * (1.0 - vmax) * cuw;
-  FCT_(1, -1, 0) = (1.0 - sign_uw) * (1.0 - vmax) * cuw;
-  const double cvw = (vgew * wv * (1.0 - uv) + (1.0 - vgew) * vw * (1.0 - uw));
-  FCT_(1, 0, 1) = sign_vw * (1.0 - umax) * cvw;
-  FCT_(1, 0, -1) = (1.0 - sign_vw) * (1.0 - umax) * cvw;
-  const double cuv = (ugev * vu * (1.0 - wu) + (1.0 - ugev) * uv * (1.0 - wv));
-  FCT_(0, 1, 1) = sign_uv * (1.0 - wmax) * cuv;
-  FCT_(0, 1, -1) = (1.0 - sign_uv) * (1.0 - wmax) * cuv;
-#undef FCT_
 }
 
 // upstream (qa) / downstream (qb) 14-point combinations (COMMON.F90:2096-2155)
@@ -404,6 +363,9 @@ hipError_t tracer_stage_ext(const Ctx& c, const TracerIO& io, const double* GSQR
 }
 
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s) {
+  // FCT on flat terrain with a two-cell scheme and a grid with blocks enough:
+  // the LDS-tiled kernels (tracer_tile.hip)
+  if (io.flag_fct && c.tracer_tile && tracer_tile_supported(c)) return tracer_stage_tile(c, io, c.scr[1], c.scr[2], s);
   TrArgs a;
   a.d = c.d;
   a.QTRC = io.QTRC; a.QTRC0 = io.QTRC0; a.RHOQ_t = io.RHOQ_t; a.DENS0 = io.DENS0; a.DENS = io.DENS;

@@ -56,6 +56,7 @@ class DynConfig(C.Structure):
 TINTEG_SHORT = {"RK4": 4, "RK3": 3, "RK3WS2002": 32}
 TINTEG_TRACER = {"RK3WS2002": 0, "EULER": 1}
 TUNE_NF_TJ = 1  # scale_hip_tune_set knob: rows of the numerical-filter tile (0 = from the grid)
+TUNE_TRACER_TILE = 2  # knob: FCT tracer stage on LDS tiles: 1 (default) on large grids, 2 always, 0 never
 
 
 _lib = None
@@ -309,6 +310,17 @@ class Context:
         """Mean time [ms] of one streaming copy of n doubles (16*n bytes of traffic)."""
         ms = C.c_double()
         _check(load_library().scale_hip_hbm_copy_probe(self._h, C.c_int64(int(n)), C.c_int32(int(reps)), C.byref(ms)))
+        return ms.value
+
+    HBM_PROBES = ("copy", "copy_nontemporal", "read", "write", "hipMemcpyDtoD")
+
+    def hbm_probe(self, kind, n, reps=10):
+        """Mean time [ms] of one launch of bandwidth probe `kind` (an index or a
+        name of HBM_PROBES) over n doubles."""
+        k = self.HBM_PROBES.index(kind) if isinstance(kind, str) else int(kind)
+        ms = C.c_double()
+        _check(load_library().scale_hip_hbm_probe(self._h, C.c_int32(k), C.c_int64(int(n)), C.c_int32(int(reps)),
+                                                  C.byref(ms)))
         return ms.value
 
     def stream_ptr(self) -> int:
