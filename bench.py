@@ -39,23 +39,84 @@ MIXED_STAGE_BYTES_PER_POINT = 41 * 4 + 4 * 8
 
 
 def theta_check(case, nsteps):
-    """Potential temperature of the mixed-precision run against the FP64 run
-    after nsteps short steps from the same state."""
+    """Mixed precision against FP64 from the same state after nsteps large
+    steps (short step and the FCT tracers): max and rms of the potential
+    temperature difference, and per prognostic and tracer the largest
+    difference, absolute and relative to the field's FP64 range."""
     from scale_amd import hip
     import numpy as np
     g = case.grid
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    names = list(hip.STATE) + [f"QTRC{iq}" for iq in range(case.qtrc.shape[-1])]
     res = {}
     for mixed in (False, True):
         ctx = hip.model_from_case(case, mixed_precision=mixed)
-        ctx.model_prepare()
-        ctx.model_short_step(nsteps)
+        ctx.model_large_step(nsteps)
         ctx.synchronize()
-        d, rt = ctx.model_get("DENS"), ctx.model_get("RHOT")
+        res[mixed] = {n: ctx.model_get(n if n in hip.STATE else hip.QTRC_BASE + int(n[4:]))[sl] for n in names}
         ctx.close()
-        sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
-        res[mixed] = rt[sl] / d[sl]
-    diff = np.abs(res[True] - res[False])
-    return {"steps": nsteps, "max_abs_K": float(diff.max()), "rms_K": float(np.sqrt((diff * diff).mean()))}
+    th = {m: res[m]["RHOT"] / res[m]["DENS"] for m in res}
+    diff = np.abs(th[True] - th[False])
+    fields = {}
+    for n in names:
+        a, b = res[False][n], res[True][n]
+        rng = float(max(a.max() - a.min(), np.abs(a).max() * 1e-3, 1e-300))
+        d = float(np.abs(a - b).max())
+        fields[n] = {"max_abs": d, "max_over_range": d / rng}
+    return {"steps": nsteps, "large_steps_with_tracers": True, "max_abs_K": float(diff.max()),
+            "rms_K": float(np.sqrt((diff * diff).mean())), "fields": fields}
+
+
+def ensemble_leg(steps=5, tune_steps=2):
+    """The 64-member ensemble of uneven domains (24 x 96x96x48, 24 x
+    128x128x64, 16 x 192x192x64; mixed precision, one FCT tracer each): large
+    steps of all members one model after the other, against the same steps
+    as one ensemble (one filter and four stage launches over every member's
+    tiles, autotuned shapes, the step replayed from a hipGraph).  Value:
+    member grid-point large-step updates per second."""
+    from scale_amd import hip, inputs
+    cases = inputs.ensemble_members()
+    pts = sum(c.grid.imax * c.grid.jmax * c.grid.kmax for c in cases)
+    t0 = time.perf_counter()
+    ctxs = [hip.model_from_case(c, mixed_precision=True) for c in cases]
+    setup_s = time.perf_counter() - t0
+    try:
+        for ctx in ctxs:  # warm-up
+            ctx.model_large_step(1)
+        for ctx in ctxs:
+            ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for ctx in ctxs:
+                ctx.model_large_step(1)
+        for ctx in ctxs:
+            ctx.synchronize()
+        seq = (time.perf_counter() - t0) / steps
+        with hip.Ensemble(ctxs) as ens:
+            best_ms, report = ens.tune(tune_steps)
+            shape = ens.get_shape()
+            ens.set_graph(True)
+            ens.large_step(1)  # capture + first replay
+            ens.synchronize()
+            t0 = time.perf_counter()
+            ens.large_step(steps)
+            ens.synchronize()
+            batched = (time.perf_counter() - t0) / steps
+            ens.set_graph(False)
+            t0 = time.perf_counter()
+            ens.large_step(steps)
+            ens.synchronize()
+            direct = (time.perf_counter() - t0) / steps
+    finally:
+        for ctx in ctxs:
+            ctx.close()
+    return {"workload": "64 members: 24 x 96x96x48, 24 x 128x128x64, 16 x 192x192x64; DX=500 m, DZ=250 m, dt=1 s; "
+                        "theta noise N(0, 0.5 K) and u ~ U(5, 15) m/s per member; 1 FCT tracer; mixed precision",
+            "members": len(cases), "points": pts, "steps": steps, "setup_s": setup_s,
+            "sequential_ms_per_large_step": seq * 1e3, "ensemble_graph_ms_per_large_step": batched * 1e3,
+            "ensemble_direct_ms_per_large_step": direct * 1e3, "tuned_best_ms": best_ms,
+            "tuned_shape": {"stage_bpc": shape[0], "nf_bpc": shape[1], "nf_tj": shape[2]}, "tune_report": report,
+            "member_updates_per_s": pts / batched, "speedup_vs_sequential": seq / batched}
 
 
 def measured_traffic(size):
@@ -85,12 +146,14 @@ def parse():
     p.add_argument("--no-copy-probe", action="store_true", help="skip the HBM copy probe")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP32 + FP64 solve) timing")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    p.add_argument("--no-ensemble", action="store_true", help="skip the 64-member ensemble leg")
     p.add_argument("--tracer-scheme", type=int, default=None, choices=[1, 2, 3, 4, 5, 6, 31],
                    help="also time large steps whose 3 FCT tracers use this scheme "
                         "(UD1=1, CD2=2, UD3=3, CD4=4, UD5=5, CD6=6, UD3KOREN1993=31; 5 and 6 on a lateral halo of 3)")
     p.add_argument("--adjoint-dirs", type=int, default=64, help="finite-difference directions of the adjoint check")
-    p.add_argument("--theta-check", type=int, default=0, metavar="N",
-                   help="also run N short steps in FP64 and in mixed precision and report the potential-temperature difference")
+    p.add_argument("--theta-check", type=int, default=600, metavar="N",
+                   help="run N large steps (with the 3 FCT tracers) in FP64 and in mixed precision and report the "
+                        "potential-temperature and per-field differences (0 skips it)")
     return p.parse_args()
 
 
@@ -356,6 +419,9 @@ def main():
     adj = None
     if not a.no_adjoint:
         adj = adjoint_leg(40, 8, a.adjoint_dirs, (1e-6, 1e-5))
+    ens = None
+    if not a.no_ensemble:
+        ens = ensemble_leg()
     theta = None
     if a.theta_check > 0:
         theta = theta_check(case, a.theta_check)
@@ -396,6 +462,7 @@ def main():
         "roofline": roof,
         "fp32_vs_fp64": mixed,
         "adjoint": adj,
+        "ensemble": ens,
     }
     if theta is not None:
         out["theta_check"] = theta

@@ -202,7 +202,9 @@ typedef struct {
                                tendencies in FP32 (the reference's SINGLE build),
                                with the reference state, the pressure terms, the
                                vertical solve and the budgets in FP64; flat
-                               terrain and QA = 0 only                         */
+                               terrain only.  Tracers advance in FP64 on the
+                               FP64 copies of the end-of-step density and of
+                               the averaged mass fluxes                       */
   double nd_coef_q;    /* ATMOS_DYN_NUMERICAL_DIFF_COEF_TRACER (0 by default:
                           num_diff_q = 0, LARGE.F90:1053-1054)                 */
   int32_t short_type;  /* ATMOS_DYN_TYPE: 0 = FVM-HEVI, 1 = FVM-HEVE (the
@@ -334,6 +336,38 @@ int scale_hip_hbm_copy_probe(scale_hip_ctx *ctx, int64_t n, int32_t reps, double
  * block-sum reduction), 3 write-only (a fill), 4 hipMemcpyAsync device to
  * device.  No counterpart in the reference: it measures the device. */
 int scale_hip_hbm_probe(scale_hip_ctx *ctx, int32_t kind, int64_t n, int32_t reps, double *avg_ms);
+
+/* Ensembles: member contexts, each with its model created and its state set,
+ * of any grid sizes on one device, stepped together.  The numerical filter
+ * and each of the four HEVI stages of a short step run as one launch over
+ * the tiles of every member (a member table on the device, blocks walking
+ * the tiles); the other launches of a large step stay per member.  The
+ * members must share the precision (FP64 or mixed), the dynamics scheme and
+ * the short steps per large step, with RK4 short steps on flat terrain and
+ * no divergence damping; each member's results are bit-identical to its own
+ * scale_hip_model_large_step.  The reference runs ensemble members as
+ * separate jobs of one program (the bulk-job communicator split,
+ * scale-rm/src/scale-rm.F90:180-184); this has no other counterpart there.
+ * The member contexts stay owned by the caller and must outlive the
+ * ensemble; their fields are read with scale_hip_model_get after
+ * scale_hip_ensemble_synchronize.
+ *   set_graph: capture one large step of the whole ensemble in a hipGraph
+ *     on the next call and replay it from then on (1), or launch directly (0).
+ *   set_shape: persistent blocks per CU of the batched stage (0..2; 0 = one
+ *     block per tile) and filter (0..4), and the filter's tile rows (4 or 8).
+ *   tune: the autotuner: times `steps` large steps of every candidate shape
+ *     after one warm-up step, keeps the fastest, and returns the members to
+ *     their state before the call; *best_ms is the fastest time per large
+ *     step, report (len bytes, may be NULL) lists every candidate. */
+typedef struct scale_hip_ensemble scale_hip_ensemble;
+int scale_hip_ensemble_create(scale_hip_ctx *const *members, int32_t n, scale_hip_ensemble **ens);
+int scale_hip_ensemble_destroy(scale_hip_ensemble *ens);
+int scale_hip_ensemble_large_step(scale_hip_ensemble *ens, int32_t n);
+int scale_hip_ensemble_synchronize(scale_hip_ensemble *ens);
+int scale_hip_ensemble_set_graph(scale_hip_ensemble *ens, int32_t on);
+int scale_hip_ensemble_set_shape(scale_hip_ensemble *ens, int32_t stage_bpc, int32_t nf_bpc, int32_t nf_tj);
+int scale_hip_ensemble_get_shape(scale_hip_ensemble *ens, int32_t *stage_bpc, int32_t *nf_bpc, int32_t *nf_tj);
+int scale_hip_ensemble_tune(scale_hip_ensemble *ens, int32_t steps, double *best_ms, char *report, int32_t len);
 
 #ifdef __cplusplus
 }

@@ -63,9 +63,13 @@ hipError_t noalloc_setup(int dev) {
   return hipSuccess;
 }
 
+// device allocation of n doubles, zeroed on the context's stream, ahead of
+// any later work of the context (elements no kernel writes, such as the
+// ghost levels of work fields, then hold zeros, not earlier contents)
 double* dalloc(Ctx& c, size_t n, std::vector<void*>* track) {
   void* p = nullptr;
   if (hipMalloc(&p, n * sizeof(double) + 256) != hipSuccess) return nullptr;
+  if (hipMemsetAsync(p, 0, n * sizeof(double) + 256, c.stream) != hipSuccess) { (void)hipFree(p); return nullptr; }
   if (track) track->push_back(p);
   return (double*)p;
 }
@@ -356,7 +360,11 @@ int scale_hip_init(const scale_hip_grid* g, scale_hip_ctx** out) {
   c.g1.CDY = p; c.g1.FDY = p + JA; c.g1.RCDY = p + 2 * JA; c.g1.RFDY = p + 3 * JA;
   // flat map factor until metrics are given
   std::vector<double> ones(8 * c.d.pl, 1.0);
-  hipMemcpy(c.MAPF, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (hipMemcpyAsync(c.MAPF, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+      hipStreamSynchronize(c.stream) != hipSuccess) {
+    scale_hip_finalize(h);
+    return fail("copying the initial map factors failed in scale_hip_init");
+  }
   *out = h;
   return 0;
 }
@@ -735,7 +743,6 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   if (QA < 0) return fail("QA must be >= 0");
   if (cfg->mixed_precision && c.d.IS != 2) return fail("mixed precision supports IHALO = 2 only");
   if (cfg->mixed_precision && c.terrain) return fail("mixed precision needs flat terrain (GSQRT = 1, J13G = J23G = 0, MAPF = 1)");
-  if (cfg->mixed_precision && QA > 0) return fail("mixed precision integrates the dynamics only (QA = 0)");
   if (cfg->mixed_precision && c.d.KA > shp::KA_DEV_MAX) return fail("mixed precision supports at most 512 levels (KA)");
   if (cfg->nd_coef_q < 0.0) return fail("nd_coef_q must be >= 0");
   if (cfg->short_type != 0 && cfg->short_type != 1) return fail("short_type must be 0 (FVM-HEVI) or 1 (FVM-HEVE)");
@@ -1099,6 +1106,184 @@ int scale_hip_hbm_probe(scale_hip_ctx* h, int32_t kind, int64_t n, int32_t reps,
   if (part) hipFree(part);
   if (e != hipSuccess) return hipfail(e, "hbm_probe");
   *avg_ms = (double)ms / reps;
+  return 0;
+}
+
+}  // extern "C"
+
+// ---- ensembles ---------------------------------------------------------------------
+struct scale_hip_ensemble {
+  shp::Ensemble E;
+};
+
+static int check_ens(scale_hip_ensemble* e) { return e ? 0 : fail("null ensemble"); }
+
+// device copies of every member's state (the prognostics, their FP32
+// encoding in mixed precision, the tracers), for the tuner to return to
+struct EnsSaved {
+  std::vector<std::pair<void*, std::pair<const void*, size_t>>> items;  // copy, (source, bytes)
+  ~EnsSaved() { for (auto& it : items) (void)hipFree(it.first); }
+  hipError_t add(const void* src, size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return e;
+    items.push_back({p, {src, bytes}});
+    return hipMemcpy(p, src, bytes, hipMemcpyDeviceToDevice);
+  }
+  hipError_t restore(hipStream_t s) {
+    for (auto& it : items) {
+      hipError_t e = hipMemcpyAsync(const_cast<void*>(it.second.first), it.first, it.second.second, hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipStreamSynchronize(s);
+  }
+};
+
+extern "C" {
+
+int scale_hip_ensemble_create(scale_hip_ctx* const* members, int32_t n, scale_hip_ensemble** out) {
+  if (!members || n <= 0 || !out) return fail("ensemble_create: members, n > 0 and out are required");
+  *out = nullptr;
+  auto* h = new scale_hip_ensemble();
+  shp::Ensemble& E = h->E;
+  for (int m = 0; m < n; ++m) {
+    if (!members[m]) { delete h; return fail("null member context"); }
+    E.members.push_back(&members[m]->c);
+  }
+  E.device = E.members[0]->device;
+  const char* why = nullptr;
+  if (shp::ensemble_check(E, &why) != hipSuccess) { delete h; return fail(why ? why : "unsupported ensemble"); }
+  if (hipSetDevice(E.device) != hipSuccess) { delete h; return fail("hipSetDevice failed"); }
+  hipDeviceGetAttribute(&E.ncu, hipDeviceAttributeMultiprocessorCount, E.device);
+  if (hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("hipStreamCreate failed"); }
+  hipError_t e = shp::ensemble_build(E);
+  if (e != hipSuccess) {
+    shp::ensemble_free(E);
+    hipStreamDestroy(E.stream);
+    delete h;
+    return hipfail(e, "ensemble_build");
+  }
+  *out = h;
+  return 0;
+}
+
+int scale_hip_ensemble_destroy(scale_hip_ensemble* h) {
+  if (!h) return 0;
+  shp::ensemble_free(h->E);
+  if (h->E.stream) hipStreamDestroy(h->E.stream);
+  delete h;
+  return 0;
+}
+
+int scale_hip_ensemble_large_step(scale_hip_ensemble* h, int32_t n) {
+  if (int r = check_ens(h)) return r;
+  CK(hipSetDevice(h->E.device));
+  for (int s = 0; s < n; ++s) CK(shp::ensemble_large_step(h->E));
+  return 0;
+}
+
+int scale_hip_ensemble_synchronize(scale_hip_ensemble* h) {
+  if (int r = check_ens(h)) return r;
+  CK(hipStreamSynchronize(h->E.stream));
+  return 0;
+}
+
+int scale_hip_ensemble_set_graph(scale_hip_ensemble* h, int32_t on) {
+  if (int r = check_ens(h)) return r;
+  shp::Ensemble& E = h->E;
+  CK(hipStreamSynchronize(E.stream));
+  if (E.exec) { (void)hipGraphExecDestroy(E.exec); E.exec = nullptr; }
+  if (E.graph) { (void)hipGraphDestroy(E.graph); E.graph = nullptr; }
+  E.use_graph = on != 0;
+  return 0;
+}
+
+static int ens_set_shape(shp::Ensemble& E, int stage_bpc, int nf_bpc, int nf_tj) {
+  if (stage_bpc < 0 || stage_bpc > 2 || nf_bpc < 0 || nf_bpc > 4 || (nf_tj != 4 && nf_tj != 8))
+    return fail("ensemble shape: stage_bpc 0..2, nf_bpc 0..4 and nf_tj 4 or 8");
+  CK(hipStreamSynchronize(E.stream));
+  if (E.exec) { (void)hipGraphExecDestroy(E.exec); E.exec = nullptr; }
+  if (E.graph) { (void)hipGraphDestroy(E.graph); E.graph = nullptr; }
+  const bool rebuild = nf_tj != E.nf_tj;
+  E.stage_bpc = stage_bpc; E.nf_bpc = nf_bpc; E.nf_tj = nf_tj;
+  if (rebuild) CK(shp::ensemble_build(E));
+  return 0;
+}
+
+int scale_hip_ensemble_set_shape(scale_hip_ensemble* h, int32_t stage_bpc, int32_t nf_bpc, int32_t nf_tj) {
+  if (int r = check_ens(h)) return r;
+  return ens_set_shape(h->E, stage_bpc, nf_bpc, nf_tj);
+}
+
+int scale_hip_ensemble_get_shape(scale_hip_ensemble* h, int32_t* stage_bpc, int32_t* nf_bpc, int32_t* nf_tj) {
+  if (int r = check_ens(h)) return r;
+  if (!stage_bpc || !nf_bpc || !nf_tj) return fail("null argument");
+  *stage_bpc = h->E.stage_bpc; *nf_bpc = h->E.nf_bpc; *nf_tj = h->E.nf_tj;
+  return 0;
+}
+
+int scale_hip_ensemble_tune(scale_hip_ensemble* h, int32_t steps, double* best_ms, char* report, int32_t len) {
+  if (int r = check_ens(h)) return r;
+  if (steps < 1 || !best_ms) return fail("ensemble_tune: steps >= 1 and best_ms are required");
+  shp::Ensemble& E = h->E;
+  CK(hipSetDevice(E.device));
+  for (shp::Ctx* cp : E.members) CK(hipStreamSynchronize(cp->stream));
+  EnsSaved saved;
+  for (shp::Ctx* cp : E.members) {
+    shp::Model& M = *cp->model;
+    const size_t b = cp->d.n3 * sizeof(double);
+    double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
+    for (int v = 0; v < 5; ++v) CK(saved.add(st[v], b));
+    if (M.QA > 0) CK(saved.add(M.QTRC, b * (size_t)M.QA));
+    if (M.mixed) {
+      for (int v = 0; v < 5; ++v) CK(saved.add(M.fS[v], cp->d.n3 * sizeof(float)));
+      CK(saved.add(M.foff, 2 * (size_t)cp->d.KA * sizeof(double)));
+    }
+  }
+  const bool graph = E.use_graph;
+  E.use_graph = false;
+  hipEvent_t a = nullptr, b = nullptr;
+  CK(hipEventCreate(&a));
+  CK(hipEventCreate(&b));
+  std::string rep;
+  double best = 1e300;
+  int bs = E.stage_bpc, bn = E.nf_bpc, bt = E.nf_tj;
+  const int stage_c[3] = {0, 1, 2}, nfb_c[2] = {0, 2}, tj_c[2] = {8, 4};
+  int err = 0;
+  for (int tj : tj_c) {
+    for (int sb : stage_c) {
+      for (int nb : nfb_c) {
+        if ((err = ens_set_shape(E, sb, nb, tj))) break;
+        hipError_t e = shp::ensemble_large_step(E);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(a, E.stream);
+        for (int q = 0; q < steps && e == hipSuccess; ++q) e = shp::ensemble_large_step(E);
+        if (e == hipSuccess) e = hipEventRecord(b, E.stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (e == hipSuccess) e = saved.restore(E.stream);
+        if (e != hipSuccess) { err = hipfail(e, "ensemble_tune"); break; }
+        const double per = ms / steps;
+        rep += "stage_bpc=" + std::to_string(sb) + " nf_bpc=" + std::to_string(nb) + " nf_tj=" + std::to_string(tj) +
+               ": " + std::to_string(per) + " ms\n";
+        if (per < best) { best = per; bs = sb; bn = nb; bt = tj; }
+      }
+      if (err) break;
+    }
+    if (err) break;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  E.use_graph = graph;
+  if (err) return err;
+  if (int r = ens_set_shape(E, bs, bn, bt)) return r;
+  *best_ms = best;
+  rep += "chosen: stage_bpc=" + std::to_string(bs) + " nf_bpc=" + std::to_string(bn) + " nf_tj=" + std::to_string(bt) + "\n";
+  if (report && len > 0) {
+    const size_t nc = rep.size() < (size_t)len - 1 ? rep.size() : (size_t)len - 1;
+    memcpy(report, rep.data(), nc);
+    report[nc] = 0;
+  }
   return 0;
 }
 

@@ -418,6 +418,19 @@ hipError_t mass_sum(const Ctx& c, const double* a, const double* b, double* out_
   return hipStreamSynchronize(c.stream);
 }
 
+// zero fill by a kernel (a step sequence may be captured in a graph, which
+// takes no memset nodes here)
+template <typename T>
+__global__ void fill_zero_kernel(T* a, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) a[q] = T(0);
+}
+template <typename T>
+static hipError_t fill_zero(T* a, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(fill_zero_kernel<T>, dim3(1024), dim3(256), 0, s, a, n);
+  return hipGetLastError();
+}
+
 // ---- launch helpers ------------------------------------------------------------
 static inline dim3 grid_full(const Dims& d) { return dim3((d.IA + 63) / 64, (d.KA + 3) / 4, d.JA); }
 static inline dim3 grid_int(const Dims& d) {
@@ -447,6 +460,40 @@ struct ShortBufs {
   R* DDIV;
 };
 
+// the inputs and outputs of stage r of a short step of length dts
+template <typename R>
+static StageIOT<R> short_stage_io(Ctx& c, double dts, const ShortBufs<R>& B, int r, const StepRec* rec = nullptr) {
+  Model& M = *c.model;
+  const Dims& d = c.d;
+  R* const num_diff = rec ? (R*)rec->nd : B.num_diff;
+  auto Rk = [&](int q, int v) -> R* { return (rec && q < 3) ? (R*)rec->R[q][v] : B.Rk[q][v]; };
+  const bool rk4 = M.tinteg_short == 4;
+  const double fact_dt1 = 1.0 / 3.0, fact_dt2 = M.tinteg_short == 32 ? 1.0 / 2.0 : 2.0 / 3.0;
+  double dtrk[4] = {dts / 2.0, dts / 2.0, dts, dts};
+  if (!rk4) { dtrk[0] = dts * fact_dt1; dtrk[1] = dts * fact_dt2; dtrk[2] = dts; }
+  StageIOT<R> io;
+  io.ctx = &c;
+  for (int v = 0; v < 5; ++v) {
+    io.o[v] = Rk(r, v);
+    io.s0[v] = B.st[v];
+    io.s[v] = r == 0 ? B.st[v] : Rk(r - 1, v);
+    io.t[v] = B.t[v];
+  }
+  io.mflx_hi = B.mfR[r];
+  io.tflx_hi = B.tfR[r];
+  io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = num_diff;
+  io.wdamp_coef = M.wdamp_coef; io.DDIV = B.DDIV; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
+  io.divdmp_coef = M.divdmp_coef; io.dtrk = dtrk[r]; io.has_coriolis = M.has_coriolis;
+  // flat-terrain HEVI RK4: the last stage combines the four stages into the state itself
+  io.combine = (rk4 && r == 3 && !c.terrain && !M.heve) ? 1 : 0;
+  for (int v = 0; v < 5; ++v) { io.c1[v] = Rk(0, v); io.c2[v] = Rk(1, v); io.c3[v] = Rk(2, v); io.st[v] = B.st[v]; }
+  for (int n = 0; n < 3; ++n) { io.cm[n] = B.mfR[n]; io.ct[n] = B.tfR[n]; }
+  io.mflx_av = B.mflx; io.tflx_av = B.tflx;
+  if constexpr (sizeof(R) == sizeof(float)) { io.offD = M.foff; io.offR = M.foff + d.KA; }
+  if (rec) { io.trPT = rec->tr[r][0]; io.trSt = rec->tr[r][1]; io.trCp = rec->tr[r][2]; }
+  return io;
+}
+
 template <typename R>
 static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B, const StepRec* rec = nullptr) {
   Model& M = *c.model;
@@ -458,7 +505,7 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B, const 
   auto Rk = [&](int r, int v) -> R* { return (rec && r < 3) ? (R*)rec->R[r][v] : B.Rk[r][v]; };
   // numerical filter on the state at the start of the short step (LARGE.F90:845-858)
   if (M.nd_coef == 0.0) {
-    HIPCHK(hipMemsetAsync(num_diff, 0, 15 * d.n3 * sizeof(R), s));
+    HIPCHK(fill_zero(num_diff, 15 * d.n3, s));
   } else {
     const R* cst[5] = {B.st[0], B.st[1], B.st[2], B.st[3], B.st[4]};
     if constexpr (sizeof(R) == sizeof(double)) {
@@ -476,30 +523,8 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B, const 
   // RK3 (tinteg_short_rk3.F90:141-166, 355-490): dtrk = dt * fact_dt1, dt * fact_dt2, dt
   const bool rk4 = M.tinteg_short == 4;
   const int nst = rk4 ? 4 : 3;
-  const double fact_dt1 = 1.0 / 3.0, fact_dt2 = M.tinteg_short == 32 ? 1.0 / 2.0 : 2.0 / 3.0;
-  double dtrk[4] = {dts / 2.0, dts / 2.0, dts, dts};
-  if (!rk4) { dtrk[0] = dts * fact_dt1; dtrk[1] = dts * fact_dt2; dtrk[2] = dts; }
   for (int r = 0; r < nst; ++r) {
-    StageIOT<R> io;
-    io.ctx = &c;
-    for (int v = 0; v < 5; ++v) {
-      io.o[v] = Rk(r, v);
-      io.s0[v] = B.st[v];
-      io.s[v] = r == 0 ? B.st[v] : Rk(r - 1, v);
-      io.t[v] = B.t[v];
-    }
-    io.mflx_hi = B.mfR[r];
-    io.tflx_hi = B.tfR[r];
-    io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = num_diff;
-    io.wdamp_coef = M.wdamp_coef; io.DDIV = B.DDIV; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
-    io.divdmp_coef = M.divdmp_coef; io.dtrk = dtrk[r]; io.has_coriolis = M.has_coriolis;
-    // flat-terrain HEVI RK4: the last stage combines the four stages into the state itself
-    io.combine = (rk4 && r == 3 && !c.terrain && !M.heve) ? 1 : 0;
-    for (int v = 0; v < 5; ++v) { io.c1[v] = Rk(0, v); io.c2[v] = Rk(1, v); io.c3[v] = Rk(2, v); io.st[v] = B.st[v]; }
-    for (int n = 0; n < 3; ++n) { io.cm[n] = B.mfR[n]; io.ct[n] = B.tfR[n]; }
-    io.mflx_av = B.mflx; io.tflx_av = B.tflx;
-    if constexpr (sizeof(R) == sizeof(float)) { io.offD = M.foff; io.offR = M.foff + d.KA; }
-    if (rec) { io.trPT = rec->tr[r][0]; io.trSt = rec->tr[r][1]; io.trCp = rec->tr[r][2]; }
+    const StageIOT<R> io = short_stage_io(c, dts, B, r, rec);
     if constexpr (sizeof(R) == sizeof(double)) {
       HIPCHK(M.heve ? heve_stage(io, s) : hevi_stage(io, s));
     } else {
@@ -617,7 +642,7 @@ hipError_t model_prepare(Ctx& c) {
   hipLaunchKernelGGL(large_prep, grid_full(d), dim3(64, 4), 0, s, d, M.RHOT, M.REF_pres, M.RT2P, M.DPRES0, M.REF_rhot);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(M.DENS00, M.DENS, bytes, hipMemcpyDeviceToDevice, s));
-  HIPCHK(hipMemsetAsync(M.mflx_av, 0, 3 * bytes, s));
+  HIPCHK(fill_zero(M.mflx_av, 3 * d.n3, s));
   for (int v = 0; v < 5; ++v) {
     hipLaunchKernelGGL(tend_prep, grid_full(d), dim3(64, 4), 0, s, d, M.t[v], (const double*)M.tp[v], v == 1 ? 1 : 0);
     HIPCHK(hipGetLastError());
@@ -626,71 +651,277 @@ hipError_t model_prepare(Ctx& c) {
   return hipSuccess;
 }
 
-hipError_t model_large_step(Ctx& c, const StepRec* rec) {
+// the number of short steps of a large step (LARGE.F90:402-404)
+static int model_nstep(const Model& M) { return (int)ceil((M.dtl - EPS_RP) / M.dts); }
+
+// preparation of a large step (and the record of its pressure terms)
+static hipError_t large_begin(Ctx& c, const StepRec* rec) {
+  Model& M = *c.model;
+  const size_t bytes = c.d.n3 * sizeof(double);
+  HIPCHK(model_prepare(c));
+  if (rec) {
+    HIPCHK(hipMemcpyAsync(rec->RT2P, M.RT2P, bytes, hipMemcpyDeviceToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(rec->DPRES0, M.DPRES0, bytes, hipMemcpyDeviceToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(rec->REF_rhot, M.REF_rhot, bytes, hipMemcpyDeviceToDevice, c.stream));
+  }
+  return hipSuccess;
+}
+
+// after each short step: the sum of the mass fluxes over the short steps (LARGE.F90:960-970)
+static hipError_t large_after_short(Ctx& c) {
   Model& M = *c.model;
   const Dims& d = c.d;
   hipStream_t s = c.stream;
-  const size_t bytes = d.n3 * sizeof(double);
-  const double dtl = M.dtl;
-  const int nstep = (int)ceil((dtl - EPS_RP) / M.dts);
-  const double dts = dtl / nstep;
-  if (rec && nstep != 1) return hipErrorNotSupported;
-  HIPCHK(model_prepare(c));
-  if (rec) {
-    HIPCHK(hipMemcpyAsync(rec->RT2P, M.RT2P, bytes, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(rec->DPRES0, M.DPRES0, bytes, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(rec->REF_rhot, M.REF_rhot, bytes, hipMemcpyDeviceToDevice, s));
+  // mixed precision: the step's averaged mass flux is FP32; the average over
+  // the short steps and the tracer step use its FP64 copy
+  if (M.mixed && M.QA > 0) HIPCHK(convert(M.mflx_hi, (const float*)M.fmflx, 3 * d.n3, s));
+  for (int n = 0; n < 3; ++n) {
+    hipLaunchKernelGGL(axpy_interior, grid_int(d), dim3(64, 4), 0, s, d, M.mflx_av + d.n3 * n, M.mflx_hi + d.n3 * n, 1.0, 0);
   }
+  return hipGetLastError();
+}
+
+// the levels KS..KE of a tracer stage's output back into the tracer (its
+// ghost levels keep their values, as the reference's QTRC does)
+static hipError_t copy_levels(const Ctx& c, double* dst, const double* src, hipStream_t s) {
+  const Dims& d = c.d;
+  const size_t off = d.pl * (size_t)d.KS, n = d.pl * (size_t)(d.KE - d.KS + 1);
+  return hipMemcpyAsync(dst + off, src + off, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+}
+
+// the tracers of the large step with the mass flux averaged over its nstep
+// short steps (LARGE.F90:1016-1127)
+static hipError_t large_tracers(Ctx& c, int nstep) {
+  Model& M = *c.model;
+  const Dims& d = c.d;
+  hipStream_t s = c.stream;
+  const double dtl = M.dtl;
+  if (M.QA <= 0) return hipSuccess;
+  // mixed precision: the tracers advance in FP64 on the FP64 copies of the
+  // FP32 state (DENS at the end of the step) and of the mass fluxes
+  if (M.mixed) HIPCHK(model_state_to_f64(c));
+  for (int n = 0; n < 3; ++n)
+    hipLaunchKernelGGL(axpy_interior, grid_int(d), dim3(64, 4), 0, s, d, M.mflx_hi + d.n3 * n, M.mflx_av + d.n3 * n, (double)nstep, 1);
+  HIPCHK(hipGetLastError());
+  double* mf[3] = {M.mflx_hi, M.mflx_hi + d.n3, M.mflx_hi + 2 * d.n3};
+  HIPCHK(halo_fill(c, mf, 3, s));
+  for (int iq = 0; iq < M.QA; ++iq) {
+    double* q = M.QTRC + d.n3 * (size_t)iq;
+    const double* rq = M.RHOQ_tp ? M.RHOQ_tp + d.n3 * (size_t)iq : M.zeros;
+    // tracer filter from DENS00 (LARGE.F90:1049-1061); with ND_COEF_Q = 0 num_diff_q stays zero
+    if (M.nd_coef_q != 0.0) HIPCHK(numfilter_q(c, M.ndq, M.DENS00, q, nullptr, s));
+    TracerIO io;
+    io.RHOQ_t = rq; io.DENS0 = M.DENS00; io.mflx_hi = M.mflx_hi; io.num_diff = M.ndq; io.flag_along = M.fct_along;
+    if (M.tinteg_tracer == 1) {
+      // EULER (tinteg_tracer_euler.F90:80-133): one stage of dtl with the
+      // end-of-step density, FCT as configured; written to QRK1 then copied
+      // back (the stage reads its input at neighbours)
+      io.QTRCo = M.QRK1; io.QTRC = q; io.QTRC0 = q; io.DENS = M.DENS; io.dtl = dtl; io.flag_fct = M.fct;
+      HIPCHK(tracer_stage(c, io, s));
+      HIPCHK(copy_levels(c, q, M.QRK1, s));
+      HIPCHK(halo_fill(c, &q, 1, s));
+      continue;
+    }
+    // stage 1: dt/3, no FCT
+    hipLaunchKernelGGL(dens_rk, dim3((d.IE - d.IS + 3 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 3), dim3(64, 4), 0, s,
+                       d, M.DENS_RK, M.DENS00, M.DENS, 1);
+    io.QTRCo = M.QRK1; io.QTRC = q; io.QTRC0 = q; io.DENS = M.DENS_RK; io.dtl = dtl / 3.0; io.flag_fct = 0;
+    HIPCHK(tracer_stage(c, io, s));
+    HIPCHK(halo_fill(c, &M.QRK1, 1, s));
+    // stage 2: dt/2, no FCT
+    hipLaunchKernelGGL(dens_rk, dim3((d.IE - d.IS + 3 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 3), dim3(64, 4), 0, s,
+                       d, M.DENS_RK, M.DENS00, M.DENS, 0);
+    io.QTRCo = M.QRK2; io.QTRC = M.QRK1; io.QTRC0 = q; io.DENS = M.DENS_RK; io.dtl = dtl / 2.0; io.flag_fct = 0;
+    HIPCHK(tracer_stage(c, io, s));
+    HIPCHK(halo_fill(c, &M.QRK2, 1, s));
+    // stage 3: dt, FCT as configured; written to QRK1 then copied back (QTRC0 is read at neighbours)
+    io.QTRCo = M.QRK1; io.QTRC = M.QRK2; io.QTRC0 = q; io.DENS = M.DENS; io.dtl = dtl; io.flag_fct = M.fct;
+    HIPCHK(tracer_stage(c, io, s));
+    HIPCHK(copy_levels(c, q, M.QRK1, s));
+    HIPCHK(halo_fill(c, &q, 1, s));
+  }
+  return hipSuccess;
+}
+
+hipError_t model_large_step(Ctx& c, const StepRec* rec) {
+  Model& M = *c.model;
+  const int nstep = model_nstep(M);
+  const double dts = M.dtl / nstep;
+  if (rec && nstep != 1) return hipErrorNotSupported;
+  HIPCHK(large_begin(c, rec));
   for (int step = 0; step < nstep; ++step) {
     HIPCHK(model_short_step_rk4(c, dts, rec));
-    for (int n = 0; n < 3; ++n) {
-      hipLaunchKernelGGL(axpy_interior, grid_int(d), dim3(64, 4), 0, s, d, M.mflx_av + d.n3 * n, M.mflx_hi + d.n3 * n, 1.0, 0);
-    }
-    HIPCHK(hipGetLastError());
+    HIPCHK(large_after_short(c));
   }
-  if (M.QA > 0) {
-    for (int n = 0; n < 3; ++n)
-      hipLaunchKernelGGL(axpy_interior, grid_int(d), dim3(64, 4), 0, s, d, M.mflx_hi + d.n3 * n, M.mflx_av + d.n3 * n, (double)nstep, 1);
-    HIPCHK(hipGetLastError());
-    double* mf[3] = {M.mflx_hi, M.mflx_hi + d.n3, M.mflx_hi + 2 * d.n3};
-    HIPCHK(halo_fill(c, mf, 3, s));
-    for (int iq = 0; iq < M.QA; ++iq) {
-      double* q = M.QTRC + d.n3 * (size_t)iq;
-      const double* rq = M.RHOQ_tp ? M.RHOQ_tp + d.n3 * (size_t)iq : M.zeros;
-      // tracer filter from DENS00 (LARGE.F90:1049-1061); with ND_COEF_Q = 0 num_diff_q stays zero
-      if (M.nd_coef_q != 0.0) HIPCHK(numfilter_q(c, M.ndq, M.DENS00, q, nullptr, s));
-      TracerIO io;
-      io.RHOQ_t = rq; io.DENS0 = M.DENS00; io.mflx_hi = M.mflx_hi; io.num_diff = M.ndq; io.flag_along = M.fct_along;
-      if (M.tinteg_tracer == 1) {
-        // EULER (tinteg_tracer_euler.F90:80-133): one stage of dtl with the
-        // end-of-step density, FCT as configured; written to QRK1 then copied
-        // back (the stage reads its input at neighbours)
-        io.QTRCo = M.QRK1; io.QTRC = q; io.QTRC0 = q; io.DENS = M.DENS; io.dtl = dtl; io.flag_fct = M.fct;
-        HIPCHK(tracer_stage(c, io, s));
-        HIPCHK(hipMemcpyAsync(q, M.QRK1, bytes, hipMemcpyDeviceToDevice, s));
-        HIPCHK(halo_fill(c, &q, 1, s));
-        continue;
-      }
-      // stage 1: dt/3, no FCT
-      hipLaunchKernelGGL(dens_rk, dim3((d.IE - d.IS + 3 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 3), dim3(64, 4), 0, s,
-                         d, M.DENS_RK, M.DENS00, M.DENS, 1);
-      io.QTRCo = M.QRK1; io.QTRC = q; io.QTRC0 = q; io.DENS = M.DENS_RK; io.dtl = dtl / 3.0; io.flag_fct = 0;
-      HIPCHK(tracer_stage(c, io, s));
-      HIPCHK(halo_fill(c, &M.QRK1, 1, s));
-      // stage 2: dt/2, no FCT
-      hipLaunchKernelGGL(dens_rk, dim3((d.IE - d.IS + 3 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 3), dim3(64, 4), 0, s,
-                         d, M.DENS_RK, M.DENS00, M.DENS, 0);
-      io.QTRCo = M.QRK2; io.QTRC = M.QRK1; io.QTRC0 = q; io.DENS = M.DENS_RK; io.dtl = dtl / 2.0; io.flag_fct = 0;
-      HIPCHK(tracer_stage(c, io, s));
-      HIPCHK(halo_fill(c, &M.QRK2, 1, s));
-      // stage 3: dt, FCT as configured; written to QRK1 then copied back (QTRC0 is read at neighbours)
-      io.QTRCo = M.QRK1; io.QTRC = M.QRK2; io.QTRC0 = q; io.DENS = M.DENS; io.dtl = dtl; io.flag_fct = M.fct;
-      HIPCHK(tracer_stage(c, io, s));
-      HIPCHK(hipMemcpyAsync(q, M.QRK1, bytes, hipMemcpyDeviceToDevice, s));
-      HIPCHK(halo_fill(c, &q, 1, s));
+  return large_tracers(c, nstep);
+}
+
+// ---- ensembles ---------------------------------------------------------------------
+// The supported configuration: every member a FP64 or every member a mixed
+// precision HEVI model with RK4 short steps on flat terrain without divergence
+// damping, one dynamics scheme and one number of short steps per large step.
+hipError_t ensemble_check(const Ensemble& E, const char** why) {
+  *why = nullptr;
+  if (E.members.empty()) { *why = "an ensemble needs at least one member"; return hipErrorInvalidValue; }
+  const Ctx& c0 = *E.members[0];
+  for (const Ctx* cp : E.members) {
+    const Ctx& c = *cp;
+    const Model* M = c.model;
+    if (!M) { *why = "every member needs a created model"; return hipErrorInvalidValue; }
+    if (c.device != E.device) { *why = "the members must be on one device"; return hipErrorInvalidValue; }
+    if (c.terrain || M->heve || M->tinteg_short != 4 || M->divdmp_coef != 0.0) {
+      *why = "ensemble members must be HEVI with RK4 short steps on flat terrain without divergence damping";
+      return hipErrorNotSupported;
+    }
+    if (M->mixed != c0.model->mixed || c.scheme != c0.scheme || model_nstep(*M) != model_nstep(*c0.model)) {
+      *why = "ensemble members must share the precision, the dynamics scheme and the short steps per large step";
+      return hipErrorNotSupported;
     }
   }
   return hipSuccess;
+}
+
+template <typename R>
+static ShortBufs<R> model_bufs(Model& M);
+template <>
+ShortBufs<double> model_bufs<double>(Model& M) {
+  return ShortBufs<double>{{M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT}, M.R, M.t, M.num_diff, M.mfR, M.tfR, M.mflx_hi,
+                           M.tflx_hi, M.DDIV};
+}
+template <>
+ShortBufs<float> model_bufs<float>(Model& M) {
+  return ShortBufs<float>{{M.fS[0], M.fS[1], M.fS[2], M.fS[3], M.fS[4]}, M.fR, M.ft, M.fnum_diff, M.fmfR, M.ftfR,
+                          M.fmflx, M.ftflx, M.fDDIV};
+}
+
+template <typename R>
+static hipError_t ensemble_build_t(Ensemble& E) {
+  const int n = (int)E.members.size();
+  for (int r = 0; r < 4; ++r) {
+    std::vector<StageIOT<R>> ios;
+    for (Ctx* cp : E.members) {
+      Model& M = *cp->model;
+      const ShortBufs<R> B = model_bufs<R>(M);
+      ios.push_back(short_stage_io(*cp, M.dtl / model_nstep(M), B, r));
+    }
+    HIPCHK(hevi_batch_build(ios.data(), n, E.stage[r]));
+  }
+  std::vector<NfIn<R>> in;
+  for (Ctx* cp : E.members) {
+    Model& M = *cp->model;
+    const ShortBufs<R> B = model_bufs<R>(M);
+    NfIn<R> f;
+    f.c = cp; f.num_diff = B.num_diff;
+    for (int v = 0; v < 5; ++v) f.st[v] = B.st[v];
+    f.REF_dens = M.REF_dens; f.REF_pott = M.REF_pott;
+    f.offD = M.mixed ? M.foff : nullptr; f.offR = M.mixed ? M.foff + cp->d.KA : nullptr;
+    in.push_back(f);
+  }
+  return numfilter_batch_build(in.data(), n, E.nf_tj, E.nf);
+}
+
+hipError_t ensemble_build(Ensemble& E) {
+  const char* why;
+  HIPCHK(ensemble_check(E, &why));
+  E.built = false;
+  hipError_t e = E.members[0]->model->mixed ? ensemble_build_t<float>(E) : ensemble_build_t<double>(E);
+  if (e == hipSuccess) E.built = true;
+  return e;
+}
+
+// one large step of every member in the phase-major order
+template <typename R>
+static hipError_t ensemble_step_t(Ensemble& E) {
+  hipStream_t s = E.stream;
+  const int nstep = model_nstep(*E.members[0]->model);
+  for (Ctx* cp : E.members) {
+    if (cp->model->mixed) HIPCHK(model_state_to_f64(*cp));
+    HIPCHK(large_begin(*cp, nullptr));
+  }
+  for (int step = 0; step < nstep; ++step) {
+    // numerical filter (LARGE.F90:845-858): one launch, then each member's halo
+    HIPCHK(numfilter_batch_launch(E.nf, E.nf_bpc, E.ncu, s));
+    for (Ctx* cp : E.members) {
+      Model& M = *cp->model;
+      const ShortBufs<R> B = model_bufs<R>(M);
+      if (M.nd_coef == 0.0) {
+        HIPCHK(fill_zero(B.num_diff, 15 * cp->d.n3, s));
+      } else {
+        R* arrs[15];
+        for (int q = 0; q < 15; ++q) arrs[q] = B.num_diff + cp->d.n3 * (size_t)q;
+        HIPCHK(halo_fill(*cp, arrs, 15, s));
+      }
+    }
+    // four stages, the last with the RK4 combination into the state
+    for (int r = 0; r < 4; ++r) {
+      HIPCHK(hevi_batch_launch(E.stage[r], E.stage_bpc, E.ncu, s));
+      for (Ctx* cp : E.members) {
+        const ShortBufs<R> B = model_bufs<R>(*cp->model);
+        if (r < 3) {
+          R* const rr[5] = {B.Rk[r][0], B.Rk[r][1], B.Rk[r][2], B.Rk[r][3], B.Rk[r][4]};
+          HIPCHK(halo_fill(*cp, rr, 5, s));
+        } else {
+          HIPCHK(halo_fill(*cp, B.st, 5, s));
+        }
+      }
+    }
+    for (Ctx* cp : E.members) HIPCHK(large_after_short(*cp));
+  }
+  for (Ctx* cp : E.members) {
+    HIPCHK(large_tracers(*cp, nstep));
+    if (cp->model->mixed) HIPCHK(model_state_to_f64(*cp));
+  }
+  return hipSuccess;
+}
+
+// the members' own streams and timers are set aside while the ensemble's
+// stream carries their work
+struct MemberStreams {
+  Ensemble& E;
+  std::vector<hipStream_t> st;
+  std::vector<Prof*> pr;
+  explicit MemberStreams(Ensemble& e) : E(e) {
+    for (Ctx* cp : E.members) {
+      st.push_back(cp->stream); pr.push_back(cp->prof);
+      cp->stream = E.stream; cp->prof = nullptr;
+    }
+  }
+  ~MemberStreams() {
+    for (size_t m = 0; m < E.members.size(); ++m) { E.members[m]->stream = st[m]; E.members[m]->prof = pr[m]; }
+  }
+};
+
+static hipError_t ensemble_step_direct(Ensemble& E) {
+  return E.members[0]->model->mixed ? ensemble_step_t<float>(E) : ensemble_step_t<double>(E);
+}
+
+hipError_t ensemble_large_step(Ensemble& E) {
+  if (!E.built) HIPCHK(ensemble_build(E));
+  // the members' pending work on their own streams comes first
+  for (Ctx* cp : E.members) HIPCHK(hipStreamSynchronize(cp->stream));
+  MemberStreams guard(E);
+  if (!E.use_graph) return ensemble_step_direct(E);
+  if (!E.exec) {
+    HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeRelaxed));
+    const hipError_t e = ensemble_step_direct(E);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(E.stream, &g);
+    if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return e; }
+    HIPCHK(e2);
+    E.graph = g;
+    HIPCHK(hipGraphInstantiate(&E.exec, E.graph, nullptr, nullptr, 0));
+  }
+  return hipGraphLaunch(E.exec, E.stream);
+}
+
+void ensemble_free(Ensemble& E) {
+  if (E.stream) (void)hipStreamSynchronize(E.stream);
+  if (E.exec) (void)hipGraphExecDestroy(E.exec);
+  if (E.graph) (void)hipGraphDestroy(E.graph);
+  E.exec = nullptr; E.graph = nullptr;
+  for (auto& b : E.stage) batch_free(b);
+  batch_free(E.nf);
+  E.built = false;
 }
 
 }  // namespace shp

@@ -17,6 +17,7 @@
 //               x and y face once;
 //   hevi_march  general metrics: neighbours come through the caches and each
 //               point evaluates the faces it needs.
+#include <vector>
 #include "common.h"
 #include "shp_internal.h"
 
@@ -795,8 +796,12 @@ __device__ __forceinline__ R rdiv(R x, R y) {
 // general expressions is a multiplication or division by 1.0 (or an addition
 // of 0.0), so the expressions below, written without them, give the same
 // IEEE results (HEVI.F90:346-1047; UD3.F90/CD4.F90 flux routines).
+// One tile (logical tile index lb of a grid ni_tiles tiles wide) of the
+// stage; the kernels below run it once per block or, for an ensemble, over a
+// table of members.  A thread that owns no column returns early from the
+// back substitution, after the last barrier.
 template <int S, bool DIVDMP, int TJ, typename R, bool CMB>
-__global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni_tiles, unsigned nblocks) {
+__device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned lb, unsigned ni_tiles) {
   constexpr int TI = 64, CW = TI + 4, RW = TJ + 4, NT = TI * TJ;
   __shared__ R sD[2][RW][CW], sMX[2][RW][CW], sMY[2][RW][CW];
   __shared__ R sPO[RW][CW], sW[RW][CW];
@@ -807,7 +812,6 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
   __shared__ R yM[TJ + 1][TI], yT[TJ + 1][TI], yW[TJ + 1][TI], yU[TJ + 1][TI], yV[TJ + 1][TI];
 
   const Dims& d = a.d;
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
   const int i0 = d.IS + (int)(lb % ni_tiles) * TI;
   const int j0 = d.JS + (int)(lb / ni_tiles) * TJ;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
@@ -1415,6 +1419,33 @@ __global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni
   }
 }
 
+template <int S, bool DIVDMP, int TJ, typename R, bool CMB>
+__global__ void __launch_bounds__(64 * TJ) hevi_tile(HeviArgsT<R> a, unsigned ni_tiles, unsigned nblocks) {
+  hevi_tile_body<S, DIVDMP, TJ, R, CMB>(a, xcd_block(blockIdx.x, nblocks), ni_tiles);
+}
+
+// The stages of many members (an ensemble of grids of any sizes) in one
+// launch: tab[m] are member m's arguments, its tiles are the logical tiles
+// first[m] .. first[m+1]-1 of total, ni[m] tiles wide.  The blocks walk the
+// tiles with a stride of the grid size (persistent when the grid is smaller
+// than total); a grid size that is a multiple of 8 keeps each XCD on one
+// contiguous range of logical tiles.
+template <int S, bool DIVDMP, int TJ, typename R, bool CMB>
+__global__ void __launch_bounds__(64 * TJ) hevi_tile_batch(const HeviArgsT<R>* __restrict__ tab,
+                                                         const unsigned* __restrict__ first,
+                                                         const unsigned* __restrict__ ni, int nmem, unsigned total) {
+  for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+    const unsigned lt = xcd_block(t, total);
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (first[mid] <= lt) lo = mid; else hi = mid - 1;
+    }
+    hevi_tile_body<S, DIVDMP, TJ, R, CMB>(tab[lo], lt - first[lo], ni[lo]);
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 template <int S, bool TERRAIN, bool DIVDMP>
 static hipError_t launch_t(const HeviArgs& a, hipStream_t s, bool cmb) {
@@ -1460,6 +1491,88 @@ static void fill_args(HeviArgsT<R>& a, const StageIOT<R>& io) {
   a.mflx_av = io.mflx_av; a.tflx_av = io.tflx_av;
   a.offD = io.offD; a.offR = io.offR;
   if (io.trSt) { a.PT = io.trPT; a.trSt = io.trSt; a.trCp = io.trCp; }
+}
+
+// ---- ensemble tables ---------------------------------------------------------
+void batch_free(BatchBuf& b) {
+  if (b.dev) (void)hipFree(b.dev);
+  b = BatchBuf{};
+}
+
+// one block of device memory: the argument structs, then first[n+1], ni[n]
+template <typename A>
+static hipError_t batch_upload(const std::vector<A>& args, const std::vector<unsigned>& first,
+                               const std::vector<unsigned>& ni, BatchBuf& b) {
+  const size_t na = args.size() * sizeof(A), nf = first.size() * sizeof(unsigned), nn = ni.size() * sizeof(unsigned);
+  const size_t off_f = (na + 255) / 256 * 256, off_n = off_f + (nf + 255) / 256 * 256;
+  batch_free(b);
+  hipError_t e = hipMalloc(&b.dev, off_n + nn + 256);
+  if (e != hipSuccess) { b.dev = nullptr; return e; }
+  char* p = (char*)b.dev;
+  if ((e = hipMemcpy(p, args.data(), na, hipMemcpyHostToDevice)) != hipSuccess) return e;
+  if ((e = hipMemcpy(p + off_f, first.data(), nf, hipMemcpyHostToDevice)) != hipSuccess) return e;
+  if ((e = hipMemcpy(p + off_n, ni.data(), nn, hipMemcpyHostToDevice)) != hipSuccess) return e;
+  b.nmem = (int)args.size();
+  b.total = first.back();
+  return hipSuccess;
+}
+template <typename A>
+static void batch_tables(const BatchBuf& b, const A*& tab, const unsigned*& first, const unsigned*& ni) {
+  const size_t na = (size_t)b.nmem * sizeof(A), nf = ((size_t)b.nmem + 1) * sizeof(unsigned);
+  const size_t off_f = (na + 255) / 256 * 256, off_n = off_f + (nf + 255) / 256 * 256;
+  tab = (const A*)b.dev;
+  first = (const unsigned*)((const char*)b.dev + off_f);
+  ni = (const unsigned*)((const char*)b.dev + off_n);
+}
+// grid of a batched launch: one block per tile, or a persistent grid of a
+// multiple of 8 blocks (each XCD then walks one range of logical tiles)
+static unsigned batch_grid(unsigned total, int blocks_per_cu, int ncu) {
+  if (blocks_per_cu <= 0) return total;
+  unsigned g = (unsigned)(blocks_per_cu * ncu);
+  g = g / 8 * 8;
+  if (g < 8) g = 8;
+  return g < total ? g : total;
+}
+
+template <typename R>
+static hipError_t hevi_batch_build_t(const StageIOT<R>* ios, int n, BatchBuf& b) {
+  if (n <= 0) return hipErrorInvalidValue;
+  std::vector<HeviArgsT<R>> args((size_t)n);
+  std::vector<unsigned> first((size_t)n + 1, 0), ni((size_t)n, 0);
+  const int scheme = ios[0].ctx->scheme, cmb = ios[0].combine;
+  for (int m = 0; m < n; ++m) {
+    const StageIOT<R>& io = ios[m];
+    const Ctx& c = *io.ctx;
+    if (c.terrain || io.divdmp_coef != 0.0 || c.scheme != scheme || io.combine != cmb) return hipErrorNotSupported;
+    fill_args(args[(size_t)m], io);
+    const Dims& d = c.d;
+    ni[(size_t)m] = (unsigned)((d.IE - d.IS + 1 + 63) / 64);
+    first[(size_t)m + 1] = first[(size_t)m] + ni[(size_t)m] * (unsigned)((d.JE - d.JS + 1 + 7) / 8);
+  }
+  hipError_t e = batch_upload(args, first, ni, b);
+  b.scheme = scheme; b.cmb = cmb; b.fp32 = sizeof(R) == sizeof(float); b.tj = 8;
+  return e;
+}
+hipError_t hevi_batch_build(const StageIOT<double>* ios, int n, BatchBuf& b) { return hevi_batch_build_t(ios, n, b); }
+hipError_t hevi_batch_build(const StageIOT<float>* ios, int n, BatchBuf& b) { return hevi_batch_build_t(ios, n, b); }
+
+template <int S, typename R, bool CMB>
+static hipError_t hevi_batch_launch_t(const BatchBuf& b, int bpc, int ncu, hipStream_t s) {
+  const HeviArgsT<R>* tab;
+  const unsigned *first, *ni;
+  batch_tables(b, tab, first, ni);
+  hipLaunchKernelGGL((hevi_tile_batch<S, false, 8, R, CMB>), dim3(batch_grid(b.total, bpc, ncu)), dim3(64, 8), 0, s,
+                     tab, first, ni, b.nmem, b.total);
+  return hipGetLastError();
+}
+hipError_t hevi_batch_launch(const BatchBuf& b, int bpc, int ncu, hipStream_t s) {
+  if (!b.dev) return hipErrorInvalidValue;
+  if (b.fp32) {
+    if (b.scheme == UD3) return b.cmb ? hevi_batch_launch_t<UD3, float, true>(b, bpc, ncu, s) : hevi_batch_launch_t<UD3, float, false>(b, bpc, ncu, s);
+    return b.cmb ? hevi_batch_launch_t<CD4, float, true>(b, bpc, ncu, s) : hevi_batch_launch_t<CD4, float, false>(b, bpc, ncu, s);
+  }
+  if (b.scheme == UD3) return b.cmb ? hevi_batch_launch_t<UD3, double, true>(b, bpc, ncu, s) : hevi_batch_launch_t<UD3, double, false>(b, bpc, ncu, s);
+  return b.cmb ? hevi_batch_launch_t<CD4, double, true>(b, bpc, ncu, s) : hevi_batch_launch_t<CD4, double, false>(b, bpc, ncu, s);
 }
 
 hipError_t hevi_stage(const StageIO& io, hipStream_t s) {

@@ -50,6 +50,32 @@ struct Model {
   std::vector<void*> allocs;
 };
 
+// An ensemble: member contexts (each with its model, of any grid size) on
+// one device, stepped together.  The filter and the four stages of a short
+// step run as one launch each over the tiles of every member; the other
+// launches stay per member, and the whole large step may be replayed from a
+// captured hipGraph.  (The reference runs ensemble members as separate jobs
+// of one MPI program: the bulk-job split of scale-rm/src/scale-rm.F90:180-184.)
+struct Ensemble {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Ctx*> members;
+  int ncu = 256;
+  // launch shapes: persistent blocks per CU of the batched stage and filter
+  // (0: one block per tile) and the filter's tile rows
+  int stage_bpc = 0, nf_bpc = 0, nf_tj = 8;
+  BatchBuf stage[4], nf;
+  bool built = false;
+  // the large step captured once and replayed
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+hipError_t ensemble_check(const Ensemble& E, const char** why);
+hipError_t ensemble_build(Ensemble& E);
+hipError_t ensemble_large_step(Ensemble& E);
+void ensemble_free(Ensemble& E);
+
 // rec (FP64 flat HEVI without divergence damping, one short step per large
 // step): the step also keeps in rec what the reverse sweep reads
 hipError_t model_short_step_rk4(Ctx& c, double dts, const StepRec* rec = nullptr);

@@ -13,6 +13,8 @@
 // the reference state are taken in double).
 #include <stdlib.h>
 
+#include <vector>
+
 #include "common.h"
 #include "shp_internal.h"
 
@@ -49,14 +51,14 @@ __device__ __forceinline__ R phi_ref(R v, double ref) {
   else return R((double)v - ref);
 }
 
+// one tile (logical index lb of a grid ni_tiles tiles wide)
 template <int TJ, typename R>
-__global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned ni_tiles, unsigned nblocks) {
+__device__ __forceinline__ void numfilter_tile_body(const NfArgsT<R>& a, unsigned lb, unsigned ni_tiles) {
   constexpr int TI = 64, CW = TI + 4, RW = TJ + 4, NT = TI * TJ;
   __shared__ R sD[2][RW + 1][CW + 1];  // DENS at k and k+1, one more cell east and north
   __shared__ R sP[5][RW][CW];          // the differenced fields at level k
   const Dims& d = a.d;
   const Nf<R> n(a);
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
   const int i0 = d.IS + (int)(lb % ni_tiles) * TI;
   const int j0 = d.JS + (int)(lb / ni_tiles) * TJ;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TI * ty;
@@ -306,6 +308,28 @@ __global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned
   }
 }
 
+template <int TJ, typename R>
+__global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned ni_tiles, unsigned nblocks) {
+  numfilter_tile_body<TJ, R>(a, xcd_block(blockIdx.x, nblocks), ni_tiles);
+}
+
+// the filters of many members in one launch (tables as hevi_tile_batch)
+template <int TJ, typename R>
+__global__ void __launch_bounds__(64 * TJ) numfilter_tile_batch(const NfArgsT<R>* __restrict__ tab,
+                                                              const unsigned* __restrict__ first,
+                                                              const unsigned* __restrict__ ni, int nmem, unsigned total) {
+  for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+    const unsigned lt = xcd_block(t, total);
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (first[mid] <= lt) lo = mid; else hi = mid - 1;
+    }
+    numfilter_tile_body<TJ, R>(tab[lo], lt - first[lo], ni[lo]);
+    __syncthreads();
+  }
+}
+
 template <typename R>
 static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], const double* REF_dens,
                               const double* REF_pott, hipStream_t s, const double* offD = nullptr,
@@ -354,6 +378,65 @@ static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], c
   R* arrs[15];
   for (int q = 0; q < 15; ++q) arrs[q] = num_diff + d.n3 * (size_t)q;
   return halo_fill(c, arrs, 15, s);
+}
+
+// ---- ensemble tables (layout as hevi.hip's batch tables) ------------------------
+template <typename R>
+static hipError_t nf_batch_build_t(const NfIn<R>* in, int n, int tj, BatchBuf& b) {
+  if (n <= 0 || (tj != 4 && tj != 8)) return hipErrorInvalidValue;
+  std::vector<NfArgsT<R>> args((size_t)n);
+  std::vector<unsigned> first((size_t)n + 1, 0), ni((size_t)n, 0);
+  int ka_max = 0;
+  for (int m = 0; m < n; ++m) {
+    const Ctx& c = *in[m].c;
+    NfArgsT<R>& a = args[(size_t)m];
+    a.offD = in[m].offD; a.offR = in[m].offR;
+    a.d = c.d;
+    a.DENS = in[m].st[0]; a.MOMZ = in[m].st[1]; a.MOMX = in[m].st[2]; a.MOMY = in[m].st[3]; a.RHOT = in[m].st[4];
+    a.REF_dens = in[m].REF_dens; a.REF_pott = in[m].REF_pott;
+    a.CNZ3 = c.CNZ3; a.CNX3 = c.CNX3; a.CNY3 = c.CNY3; a.ndc = c.ndc; a.sfc = c.nd_sfc_fact;
+    a.num_diff = in[m].num_diff;
+    const Dims& d = c.d;
+    ni[(size_t)m] = (unsigned)((d.IE - d.IS + 1 + 63) / 64);
+    first[(size_t)m + 1] = first[(size_t)m] + ni[(size_t)m] * (unsigned)((d.JE - d.JS + 1 + tj - 1) / tj);
+    ka_max = d.KA > ka_max ? d.KA : ka_max;
+  }
+  const size_t na = args.size() * sizeof(NfArgsT<R>), nf = first.size() * sizeof(unsigned), nn = ni.size() * sizeof(unsigned);
+  const size_t off_f = (na + 255) / 256 * 256, off_n = off_f + (nf + 255) / 256 * 256;
+  batch_free(b);
+  hipError_t e = hipMalloc(&b.dev, off_n + nn + 256);
+  if (e != hipSuccess) { b.dev = nullptr; return e; }
+  char* p = (char*)b.dev;
+  if ((e = hipMemcpy(p, args.data(), na, hipMemcpyHostToDevice)) != hipSuccess) return e;
+  if ((e = hipMemcpy(p + off_f, first.data(), nf, hipMemcpyHostToDevice)) != hipSuccess) return e;
+  if ((e = hipMemcpy(p + off_n, ni.data(), nn, hipMemcpyHostToDevice)) != hipSuccess) return e;
+  b.nmem = n; b.total = first.back(); b.tj = tj; b.fp32 = sizeof(R) == sizeof(float); b.ka_max = ka_max;
+  return hipSuccess;
+}
+hipError_t numfilter_batch_build(const NfIn<double>* in, int n, int tj, BatchBuf& b) { return nf_batch_build_t(in, n, tj, b); }
+hipError_t numfilter_batch_build(const NfIn<float>* in, int n, int tj, BatchBuf& b) { return nf_batch_build_t(in, n, tj, b); }
+
+template <int TJ, typename R>
+static hipError_t nf_batch_launch_t(const BatchBuf& b, int bpc, int ncu, hipStream_t s) {
+  const size_t na = (size_t)b.nmem * sizeof(NfArgsT<R>), nf = ((size_t)b.nmem + 1) * sizeof(unsigned);
+  const size_t off_f = (na + 255) / 256 * 256, off_n = off_f + (nf + 255) / 256 * 256;
+  const NfArgsT<R>* tab = (const NfArgsT<R>*)b.dev;
+  const unsigned* first = (const unsigned*)((const char*)b.dev + off_f);
+  const unsigned* ni = (const unsigned*)((const char*)b.dev + off_n);
+  unsigned g = b.total;
+  if (bpc > 0) {
+    g = (unsigned)(bpc * ncu) / 8 * 8;
+    if (g < 8) g = 8;
+    if (g > b.total) g = b.total;
+  }
+  const size_t lds = 8 * (size_t)b.ka_max * sizeof(double);
+  hipLaunchKernelGGL((numfilter_tile_batch<TJ, R>), dim3(g), dim3(64, TJ), lds, s, tab, first, ni, b.nmem, b.total);
+  return hipGetLastError();
+}
+hipError_t numfilter_batch_launch(const BatchBuf& b, int bpc, int ncu, hipStream_t s) {
+  if (!b.dev) return hipErrorInvalidValue;
+  if (b.fp32) return b.tj == 8 ? nf_batch_launch_t<8, float>(b, bpc, ncu, s) : nf_batch_launch_t<4, float>(b, bpc, ncu, s);
+  return b.tj == 8 ? nf_batch_launch_t<8, double>(b, bpc, ncu, s) : nf_batch_launch_t<4, double>(b, bpc, ncu, s);
 }
 
 hipError_t numfilter(const Ctx& c, double* num_diff, const double* const st[5], const double* REF_dens,

@@ -104,6 +104,35 @@ struct TracerIO {
   int flag_fct, flag_along;
 };
 
+// ---- ensembles: one launch over the tiles of many members' grids ----
+// Device tables of the members' argument structs with first[nmem+1] (first
+// logical tile of each member) and ni[nmem] (tiles across), built once.
+struct BatchBuf {
+  void* dev = nullptr;
+  int nmem = 0;
+  unsigned total = 0;  // tiles over all members
+  int scheme = UD3, tj = 8, fp32 = 0, cmb = 0, ka_max = 0;
+};
+void batch_free(BatchBuf& b);
+// the HEVI stage of every member: flat terrain, no divergence damping, one
+// scheme and one precision (cmb: the fused RK4 combination of the last stage)
+hipError_t hevi_batch_build(const StageIOT<double>* ios, int n, BatchBuf& b);
+hipError_t hevi_batch_build(const StageIOT<float>* ios, int n, BatchBuf& b);
+// blocks_per_cu = 0: one block per tile; otherwise a persistent grid of
+// blocks_per_cu blocks per CU walking the tiles
+hipError_t hevi_batch_launch(const BatchBuf& b, int blocks_per_cu, int ncu, hipStream_t s);
+// the numerical filter of every member (then each member's halo fill)
+template <typename R>
+struct NfIn {
+  const Ctx* c;
+  R* num_diff;
+  const R* st[5];
+  const double *REF_dens, *REF_pott, *offD, *offR;
+};
+hipError_t numfilter_batch_build(const NfIn<double>* in, int n, int tj, BatchBuf& b);
+hipError_t numfilter_batch_build(const NfIn<float>* in, int n, int tj, BatchBuf& b);
+hipError_t numfilter_batch_launch(const BatchBuf& b, int blocks_per_cu, int ncu, hipStream_t s);
+
 hipError_t hevi_stage(const StageIO& io, hipStream_t s);
 // the explicit HEVE stage: same inputs and outputs, FP64, no fused combine
 hipError_t heve_stage(const StageIO& io, hipStream_t s);

@@ -333,13 +333,14 @@ def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, 
                     divdmp_coef=0.0, nd_coef_q=0.0, short="hevi", dtl=None, tinteg_short="RK4",
                     tinteg_tracer="RK3WS2002") -> Context:
     """Create a context holding the device-resident model of an inputs.Case
-    (mixed_precision: FP32 dynamics with FP64 solve and reference state; the
-    tracers are left out; short = "hevi" or "heve", ATMOS_DYN_TYPE; dtl: the
-    large step, case.dt (the short step) by default, so nstep = ceil(dtl/dt))."""
+    (mixed_precision: FP32 dynamics with FP64 solve and reference state, the
+    tracers advanced in FP64; short = "hevi" or "heve", ATMOS_DYN_TYPE; dtl:
+    the large step, case.dt (the short step) by default, so nstep =
+    ceil(dtl/dt))."""
     if ctx is None:
         ctx = Context(case.grid, case.scheme, case.scheme_tracer, device)
     ctx.set_metrics(case.metric)
-    qa = 0 if mixed_precision else case.qtrc.shape[-1]
+    qa = case.qtrc.shape[-1]
     ctx.model_create(case.dt if dtl is None else dtl, case.dt, qa, nd_coef=nd_coef,
                      fct_tracer=case.fct if fct is None else fct, mixed_precision=mixed_precision,
                      divdmp_coef=divdmp_coef, nd_coef_q=nd_coef_q, short=short, tinteg_short=tinteg_short,
@@ -353,3 +354,53 @@ def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, 
     for iq in range(qa):
         ctx.model_set(QTRC_BASE + iq, np.asfortranarray(case.qtrc[..., iq]))
     return ctx
+
+class Ensemble:
+    """Member contexts (each with its model) stepped together
+    (scale_hip_ensemble_*): the filter and the HEVI stages of all members run
+    as one launch each; optional hipGraph replay and autotuned launch shapes.
+    The member contexts stay owned by the caller."""
+
+    def __init__(self, members):
+        L = load_library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        h = C.c_void_p()
+        _check(L.scale_hip_ensemble_create(arr, C.c_int32(len(self.members)), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            load_library().scale_hip_ensemble_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def large_step(self, n=1):
+        _check(load_library().scale_hip_ensemble_large_step(self._h, C.c_int32(int(n))))
+
+    def synchronize(self):
+        _check(load_library().scale_hip_ensemble_synchronize(self._h))
+
+    def set_graph(self, on=True):
+        _check(load_library().scale_hip_ensemble_set_graph(self._h, C.c_int32(1 if on else 0)))
+
+    def set_shape(self, stage_bpc, nf_bpc, nf_tj):
+        _check(load_library().scale_hip_ensemble_set_shape(self._h, C.c_int32(stage_bpc), C.c_int32(nf_bpc),
+                                                           C.c_int32(nf_tj)))
+
+    def get_shape(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(load_library().scale_hip_ensemble_get_shape(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def tune(self, steps=2):
+        """Autotune the launch shapes; returns (best ms per large step, report)."""
+        best = C.c_double()
+        buf = C.create_string_buffer(4096)
+        _check(load_library().scale_hip_ensemble_tune(self._h, C.c_int32(int(steps)), C.byref(best), buf, 4096))
+        return best.value, buf.value.decode()

@@ -376,3 +376,20 @@ def shear_512x512x128(**kw) -> Case:
                 scheme=3, scheme_tracer=3, fct=True)
     args.update(kw)
     return make_case(**args)
+
+
+def ensemble_members(scale=1):
+    """The 64-member ensemble of uneven domains: 24 members at 96x96x48, 24 at
+    128x128x64 and 16 at 192x192x64 (DX = DY = 500 m, DZ = 250 m, dt = 1 s),
+    each with theta noise N(0, 0.5 K) from its own seed and a uniform wind u
+    drawn from U(5, 15) m/s, one top-hat tracer with FCT.  scale > 1 divides
+    the horizontal sizes (for tests)."""
+    rng = np.random.default_rng(20261202 + 3000)
+    winds = rng.uniform(5.0, 15.0, size=64)
+    sizes = [(96, 48)] * 24 + [(128, 64)] * 24 + [(192, 64)] * 16
+    out = []
+    for m, (n, k) in enumerate(sizes):
+        out.append(make_case(kmax=k, imax=n // scale, jmax=n // scale, dx=500.0, dy=500.0, dz=250.0,
+                             seed=20261202 + 3000 + m, noise=0.5, u0=float(winds[m]), bubble=False, ntracer=1,
+                             tracer_kind=["tophat"], fct=True, dt=1.0))
+    return out
