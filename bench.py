@@ -38,6 +38,11 @@ STAGE_LAUNCH_BYTES_PER_POINT = STAGE_BYTES_PER_POINT + COMBINE_BYTES_PER_POINT /
 MIXED_STAGE_BYTES_PER_POINT = 41 * 4 + 4 * 8
 
 
+def progress(msg):
+    """One line per finished leg on stderr, so a long run shows it is alive."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def theta_check(case, nsteps):
     """Mixed precision against FP64 from the same state after nsteps large
     steps (short step and the FCT tracers): max and rms of the potential
@@ -55,6 +60,7 @@ def theta_check(case, nsteps):
         ctx.synchronize()
         res[mixed] = {n: ctx.model_get(n if n in hip.STATE else hip.QTRC_BASE + int(n[4:]))[sl] for n in names}
         ctx.close()
+        progress(f"precision check: {nsteps} large steps, {'mixed' if mixed else 'FP64'} done")
     th = {m: res[m]["RHOT"] / res[m]["DENS"] for m in res}
     diff = np.abs(th[True] - th[False])
     fields = {}
@@ -80,6 +86,7 @@ def ensemble_leg(steps=5, tune_steps=2):
     t0 = time.perf_counter()
     ctxs = [hip.model_from_case(c, mixed_precision=True) for c in cases]
     setup_s = time.perf_counter() - t0
+    progress(f"ensemble: {len(ctxs)} members set up in {setup_s:.1f} s")
     try:
         for ctx in ctxs:  # warm-up
             ctx.model_large_step(1)
@@ -92,8 +99,10 @@ def ensemble_leg(steps=5, tune_steps=2):
         for ctx in ctxs:
             ctx.synchronize()
         seq = (time.perf_counter() - t0) / steps
+        progress(f"ensemble: members one after the other {seq * 1e3:.2f} ms per large step")
         with hip.Ensemble(ctxs) as ens:
             best_ms, report = ens.tune(tune_steps)
+            progress(f"ensemble: tuned, best {best_ms:.2f} ms per large step")
             shape = ens.get_shape()
             ens.set_graph(True)
             ens.large_step(1)  # capture + first replay
@@ -227,7 +236,8 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
         grad = inputs.interior_gradient(g, xb0)
         rng = np.random.default_rng(seed)
         scale = {k: float(np.abs(x0[k]).max()) for k in names}
-        ads, fd2, fd4 = [], [], []
+        ads, fd2, fd4, dirs = [], [], [], []
+        sweep_dirs = 8
         e2, e4 = eps_list
 
         def J_at(d, t):
@@ -237,7 +247,9 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
             d1, r1 = ctx.model_get("DENS"), ctx.model_get("RHOT")
             return float(np.sum((r1[box] / d1[box] - ref_pott[box]) ** 2))
 
-        for _ in range(ndirs):
+        for idir in range(ndirs):
+            if idir % 16 == 0:
+                progress(f"adjoint: finite differences, direction {idir} of {ndirs}")
             d = {}
             for k in names:
                 a = np.zeros(g.shape, order="F")
@@ -250,13 +262,26 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
             fd2.append((J_at(d, e2) - J_at(d, -e2)) / (2.0 * e2))
             # fourth-order central difference at eps = e4
             fd4.append((8.0 * (J_at(d, e4) - J_at(d, -e4)) - (J_at(d, 2 * e4) - J_at(d, -2 * e4))) / (12.0 * e4))
+            dirs.append(d)
+        # eps sweep of the second-order difference on the first directions:
+        # the error falls with eps until no face velocity crosses zero (the
+        # upwind switch is a kink of J), then rounding takes over
+        sweep = {}
+        for e in (1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9):
+            errs = []
+            for q in range(min(sweep_dirs, len(dirs))):
+                fd = (J_at(dirs[q], e) - J_at(dirs[q], -e)) / (2.0 * e)
+                errs.append(abs(fd - ads[q]) / abs(ads[q]))
+            sweep[f"{e:.0e}"] = {"median_rel_err": float(np.median(errs)), "max_rel_err": float(np.max(errs))}
+        progress("adjoint: eps sweep done")
     finally:
         ctx.close()
     return dict(workload="warm bubble 128x128x64, DX=500 m, DZ=250 m, u=10 m/s, dt=1 s, 3 FCT tracers, FP64, UD3",
                 steps=nsteps, checkpoint_every=every, J=J0, fwd_ms=fwd_ms, fwd_host_ms=fwd_host_ms,
                 forward_checkpointed_ms=fck_ms, reverse_ms=rev_ms, adj_ms=fck_ms + rev_ms,
                 adj_over_fwd=(fck_ms + rev_ms) / fwd_ms, reverse_over_fwd=rev_ms / fwd_ms,
-                fd=fd_summary(ads, fd2, fd4, eps_list) if ndirs > 0 else None)
+                fd=fd_summary(ads, fd2, fd4, eps_list) if ndirs > 0 else None,
+                fd_eps_sweep=sweep if ndirs > 0 else None)
 
 
 def fd_summary(ads, fd2, fd4, eps_list):
@@ -378,6 +403,7 @@ def main():
             nbytes = (8 if kind in ("read", "write") else 16) * n_copy
             probes[kind] = {"ms": ms, "GBs": nbytes / (ms * 1e-3) / 1e9}
     ctx.close()
+    progress(f"FP64 short steps: {elapsed * 1e3 / a.steps:.3f} ms/step; large step {large_ms} ms")
     # informational: large steps with the tracers on another flux scheme
     tracer_leg = None
     if a.tracer_scheme is not None:
@@ -413,18 +439,23 @@ def main():
             m_kern[name] = ms / cnt if cnt else None
         mctx.profile(False)
         mctx.close()
+        progress(f"mixed short steps: {m_el * 1e3 / a.steps:.3f} ms/step")
         mixed = dict(value=ni * nj * nk * a.steps / m_el, ms_per_step=m_el * 1e3 / a.steps,
                      stage_ms=m_kern["hevi_stage"], kernels_avg_ms=m_kern, speedup_vs_fp64=elapsed / m_el,
                      stage_bytes_per_point=MIXED_STAGE_BYTES_PER_POINT)
     adj = None
     if not a.no_adjoint:
         adj = adjoint_leg(40, 8, a.adjoint_dirs, (1e-6, 1e-5))
+        progress(f"adjoint: adj/fwd {adj['adj_over_fwd']:.3f}")
     ens = None
     if not a.no_ensemble:
         ens = ensemble_leg()
+        progress(f"ensemble: {ens['ensemble_graph_ms_per_large_step']:.2f} ms/large step "
+                 f"({ens['speedup_vs_sequential']:.2f}x the sequential members)")
     theta = None
     if a.theta_check > 0:
         theta = theta_check(case, a.theta_check)
+        progress(f"precision check: max |dtheta| {theta['max_abs_K']:.3e} K after {a.theta_check} large steps")
 
     pts = ni * nj * nk
     value = pts * a.steps / elapsed
@@ -469,6 +500,7 @@ def main():
     if not a.no_cpu_baseline:
         cn, cj, ck = (int(x) for x in a.cpu_sample.split("x"))
         out["cpu_baseline"] = cpu_baseline(cn, cj, ck)
+        progress("CPU baseline done")
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,130 @@
+"""Tangent-linear model of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+liboracle_tl.so is scale_oracle.c compiled with dual numbers (dual.h): every
+real carries its derivative along one direction, so running the oracle's
+large step on (state, tangent) pairs returns the state after the step and
+L dx, the Jacobian of the step applied to dx, exactly.  The values it
+computes are the oracle's own, bit for bit.  Used by the tests to check the
+adjoints (oracle and GPU) by <L dx, y> = <dx, L^T y>.
+
+A dual array is a (2, ...) float64 array in Fortran order: element [0, ...]
+the value, [1, ...] the tangent, adjacent in memory as the C++ pair is.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+import pyoracle as po
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_TL = os.path.join(HERE, "liboracle_tl.so")
+
+_dp = C.POINTER(C.c_double)
+
+
+class DualC(C.Structure):
+    _fields_ = [("v", C.c_double), ("t", C.c_double)]
+
+
+def _dual_struct(base):
+    """The oracle struct with each double field a (value, tangent) pair."""
+    fields = [(n, DualC if t is C.c_double else t) for n, t in base._fields_]
+    return type(base.__name__ + "Dual", (C.Structure,), {"_fields_": fields})
+
+
+OrcMetricD = _dual_struct(po.OrcMetric)
+OrcConfigD = _dual_struct(po.OrcConfig)
+OrcStateD = _dual_struct(po.OrcState)  # pointers and ints only: the same layout
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_TL):
+            subprocess.run(["make", "-C", HERE, "-s"], check=True)
+        _lib = C.CDLL(LIB_TL)
+    return _lib
+
+
+def dual(v, t=None):
+    v = np.asarray(v, dtype=np.float64)
+    a = np.zeros((2,) + v.shape, order="F")
+    a[0] = v
+    if t is not None:
+        a[1] = t
+    return a
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags.f_contiguous and a.shape[0] == 2
+    return a.ctypes.data_as(_dp)
+
+
+class TangentModel:
+    """The oracle's Model (po.Model) on dual numbers: step() advances the
+    state and its tangent together."""
+
+    def __init__(self, case, state: dict, dstate: dict, dqtrc=None, **model_kw):
+        g = case.grid
+        self.case = case
+        self._keep = []
+        pm = po.Model(case, **model_kw)
+        self.cfg = OrcConfigD()
+        for n, t in po.OrcConfig._fields_:
+            val = getattr(pm.cfg, n)
+            setattr(self.cfg, n, DualC(val, 0.0) if t is C.c_double else val)
+        self.x = {k: dual(state[k], dstate.get(k)) for k in po.STATE}
+        qa = case.qtrc.shape[-1]
+        self.qa = qa
+        q0 = case.qtrc if qa > 0 else np.zeros(g.shape + (1,))
+        self.q = dual(q0, dqtrc)
+        m = case.metric
+        self.ms = OrcMetricD()
+        for n in ("CDZ", "FDZ", "RCDZ", "RFDZ", "CDX", "FDX", "RCDX", "RFDX", "CDY", "FDY", "RCDY", "RFDY",
+                  "PHI", "GSQRT", "J13G", "J23G", "MAPF"):
+            a = dual(m[n])
+            self._keep.append(a)
+            setattr(self.ms, n, dptr(a))
+        self.ms.J33G = DualC(float(m["J33G"]), 0.0)
+        st = OrcStateD()
+        for k in po.STATE:
+            setattr(st, k, dptr(self.x[k]))
+        st.QTRC = dptr(self.q)
+        st.QA = qa
+        for k in ("REF_dens", "REF_pott", "REF_pres"):
+            a = dual(case.ref[k])
+            self._keep.append(a)
+            setattr(st, k, dptr(a))
+        for k in ("DENS_tp", "MOMZ_tp", "MOMX_tp", "MOMY_tp", "RHOT_tp"):
+            a = dual(case.zero_tend[k])
+            self._keep.append(a)
+            setattr(st, k, dptr(a))
+        for k, v in (("RHOQ_tp", np.zeros(g.shape + (max(qa, 1),))), ("CORIOLI", case.corioli),
+                     ("wdamp_coef", case.wdamp_coef), ("mflx_hi", np.full(g.shape + (3,), np.nan)),
+                     ("tflx_hi", np.full(g.shape + (3,), np.nan))):
+            a = dual(v)
+            self._keep.append(a)
+            setattr(st, k, dptr(a))
+        self.st = st
+        self.gs = po.grid_struct(g)
+
+    def step(self, n=1):
+        for _ in range(n):
+            rc = lib().orc_large_step(C.byref(self.gs), C.byref(self.ms), C.byref(self.cfg), C.byref(self.st))
+            if rc != 0:
+                raise RuntimeError("orc_large_step (dual) failed")
+
+    def values(self):
+        return {k: np.asfortranarray(self.x[k][0]) for k in po.STATE}
+
+    def tangents(self):
+        return {k: np.asfortranarray(self.x[k][1]) for k in po.STATE}
+
+    def qtrc(self):
+        return np.asfortranarray(self.q[0]), np.asfortranarray(self.q[1])
