@@ -245,6 +245,24 @@ int scale_hip_set_metrics(scale_hip_ctx *ctx,
     const double *CDY, const double *FDY, const double *RCDY, const double *RFDY,
     const double *PHI, const double *GSQRT, const double *J13G, const double *J23G,
     const double *J33G, const double *MAPF);
+
+/* Terrain-following metrics computed on the device from the surface height,
+ * in place of scale_hip_set_metrics: the Gal-Chen heights
+ * z = (Htop - Zsfc)/Htop * zeta + Zsfc of the cell centres CZ(KA) and faces
+ * FZ(0:KA) (KA+1 values) at the x, u, v and uv points, Htop = FZ(KE) -
+ * FZ(KS-1), and from them GSQRT, J13G, J23G (7 components each, periodic
+ * halos), PHI = GRAV * z and J33G = 1
+ * (scalelib/src/atmosphere/grid/scale_atmos_grid_cartesC_real.F90:391-563,
+ *  scalelib/src/atmosphere/grid/scale_atmos_grid_cartesC_metric.F90:200-345).
+ * Zsfc(IA,JA) with its halo filled; MAPF(IA,JA,2,4) as given. */
+int scale_hip_set_terrain(scale_hip_ctx *ctx,
+    const double *CDZ, const double *FDZ, const double *RCDZ, const double *RFDZ,
+    const double *CDX, const double *FDX, const double *RCDX, const double *RFDX,
+    const double *CDY, const double *FDY, const double *RCDY, const double *RFDY,
+    const double *CZ, const double *FZ, const double *Zsfc, const double *MAPF);
+/* The context's metrics back in the reference's layout: GSQRT, J13G, J23G
+ * (KA,IA,JA,7) and PHI (KA,IA,JA); any may be NULL. */
+int scale_hip_get_metrics(scale_hip_ctx *ctx, double *GSQRT, double *J13G, double *J23G, double *PHI);
 int scale_hip_model_create(scale_hip_ctx *ctx, const scale_hip_dyn_config *cfg, int32_t QA);
 int scale_hip_model_set(scale_hip_ctx *ctx, int32_t field, const double *host);
 int scale_hip_model_get(scale_hip_ctx *ctx, int32_t field, double *host);

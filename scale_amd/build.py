@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libscalehip.so")
 SOURCES = ["hevi.hip", "numfilter.hip", "tracer.hip", "tracer_tile.hip", "driver.hip", "adjoint.hip", "profile.hip",
-           "capi.hip"]
+           "metrics.hip", "capi.hip"]
 ARCH = os.environ.get("SCALE_HIP_ARCH", "gfx950")
 
 FLAGS = [

@@ -730,6 +730,55 @@ int scale_hip_set_metrics(scale_hip_ctx* h,
   return set_metric3d(h->c, PHI, GSQRT, J13G, J23G, *J33G, MAPF);
 }
 
+int scale_hip_set_terrain(scale_hip_ctx* h,
+    const double* CDZ, const double* FDZ, const double* RCDZ, const double* RFDZ,
+    const double* CDX, const double* FDX, const double* RCDX, const double* RFDX,
+    const double* CDY, const double* FDY, const double* RCDY, const double* RFDY,
+    const double* CZ, const double* FZ, const double* Zsfc, const double* MAPF) {
+  if (int r = check_ctx(h)) return r;
+  if (!CDZ || !FDZ || !RCDZ || !RFDZ || !CDX || !FDX || !RCDX || !RFDX || !CDY || !FDY || !RCDY || !RFDY || !CZ || !FZ ||
+      !Zsfc || !MAPF)
+    return fail("set_terrain: every array is required");
+  Ctx& c = h->c;
+  const Dims& d = c.d;
+  if (int r = set_grid1d(c, CDZ, FDZ, RCDZ, RFDZ, CDX, FDX, RCDX, RFDX, CDY, FDY, RCDY, RFDY)) return r;
+  if (!c.GSQRT) {
+    c.GSQRT = dalloc(c, 7 * d.n3, nullptr);
+    c.J13G = dalloc(c, 7 * d.n3, nullptr);
+    c.J23G = dalloc(c, 7 * d.n3, nullptr);
+    if (!c.GSQRT || !c.J13G || !c.J23G) return fail("hipMalloc of metric arrays failed");
+  }
+  // the surface and the reference heights go through the staging buffer
+  const size_t nz = (size_t)d.pl + (size_t)d.KA + (size_t)d.KA + 1;
+  if (int r = ensure_stage(c, nz)) return r;
+  double* zs = c.stage;
+  double* cz = zs + d.pl;
+  double* fz = cz + d.KA;
+  CK(hipMemcpyAsync(zs, Zsfc, d.pl * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CK(hipMemcpyAsync(cz, CZ, (size_t)d.KA * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  CK(hipMemcpyAsync(fz, FZ, ((size_t)d.KA + 1) * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  const double Htop = FZ[d.KE + 1] - FZ[d.KS];  // FZ(KE) - FZ(KS-1) (REAL:410)
+  CK(shp::terrain_metrics(c, zs, cz, fz, Htop, c.GSQRT, c.J13G, c.J23G, c.PHI, c.stream));
+  if (int r = upload_raw(c, c.MAPF, MAPF, 8 * d.pl)) return r;
+  CK(hipStreamSynchronize(c.stream));
+  c.terrain = !(all_equal(Zsfc, d.pl, 0.0) && all_equal(MAPF, 8 * d.pl, 1.0));
+  c.J33G = 1.0;
+  c.metrics_set = true;
+  return 0;
+}
+
+int scale_hip_get_metrics(scale_hip_ctx* h, double* GSQRT, double* J13G, double* J23G, double* PHI) {
+  if (int r = check_ctx(h)) return r;
+  Ctx& c = h->c;
+  if (!c.metrics_set) return fail("no metrics set");
+  if (!c.GSQRT) return fail("the metrics are flat (GSQRT = 1, J13G = J23G = 0): nothing stored");
+  if (GSQRT) { if (int r = download3(c, GSQRT, c.GSQRT, 7)) return r; }
+  if (J13G) { if (int r = download3(c, J13G, c.J13G, 7)) return r; }
+  if (J23G) { if (int r = download3(c, J23G, c.J23G, 7)) return r; }
+  if (PHI) { if (int r = download3(c, PHI, c.PHI, 1)) return r; }
+  return 0;
+}
+
 int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, int32_t QA) {
   if (int r = check_ctx(h)) return r;
   if (!cfg) return fail("null config");
@@ -1156,7 +1205,11 @@ int scale_hip_ensemble_create(scale_hip_ctx* const* members, int32_t n, scale_hi
   if (hipSetDevice(E.device) != hipSuccess) { delete h; return fail("hipSetDevice failed"); }
   hipDeviceGetAttribute(&E.ncu, hipDeviceAttributeMultiprocessorCount, E.device);
   if (hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("hipStreamCreate failed"); }
-  hipError_t e = shp::ensemble_build(E);
+  bool ok = hipEventCreateWithFlags(&E.ev_main, hipEventDisableTiming) == hipSuccess;
+  for (int l = 0; l < E.nlanes && ok; ++l)
+    ok = hipStreamCreateWithFlags(&E.lane[l], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&E.ev_lane[l], hipEventDisableTiming) == hipSuccess;
+  hipError_t e = ok ? shp::ensemble_build(E) : hipErrorOutOfMemory;
   if (e != hipSuccess) {
     shp::ensemble_free(E);
     hipStreamDestroy(E.stream);

@@ -829,61 +829,83 @@ hipError_t ensemble_build(Ensemble& E) {
   return e;
 }
 
-// one large step of every member in the phase-major order
+// one large step of every member in the phase-major order: the batched
+// launches on the ensemble's stream, each member's own launches on its lane
 template <typename R>
 static hipError_t ensemble_step_t(Ensemble& E) {
   hipStream_t s = E.stream;
   const int nstep = model_nstep(*E.members[0]->model);
+  // the lanes wait for the ensemble stream, then the ensemble stream for them
+  auto fork = [&]() -> hipError_t {
+    HIPCHK(hipEventRecord(E.ev_main, s));
+    for (int l = 0; l < E.nlanes; ++l) HIPCHK(hipStreamWaitEvent(E.lane[l], E.ev_main, 0));
+    return hipSuccess;
+  };
+  auto join = [&]() -> hipError_t {
+    for (int l = 0; l < E.nlanes; ++l) {
+      HIPCHK(hipEventRecord(E.ev_lane[l], E.lane[l]));
+      HIPCHK(hipStreamWaitEvent(s, E.ev_lane[l], 0));
+    }
+    return hipSuccess;
+  };
+  HIPCHK(fork());
   for (Ctx* cp : E.members) {
     if (cp->model->mixed) HIPCHK(model_state_to_f64(*cp));
     HIPCHK(large_begin(*cp, nullptr));
   }
+  HIPCHK(join());
   for (int step = 0; step < nstep; ++step) {
     // numerical filter (LARGE.F90:845-858): one launch, then each member's halo
     HIPCHK(numfilter_batch_launch(E.nf, E.nf_bpc, E.ncu, s));
+    HIPCHK(fork());
     for (Ctx* cp : E.members) {
       Model& M = *cp->model;
       const ShortBufs<R> B = model_bufs<R>(M);
       if (M.nd_coef == 0.0) {
-        HIPCHK(fill_zero(B.num_diff, 15 * cp->d.n3, s));
+        HIPCHK(fill_zero(B.num_diff, 15 * cp->d.n3, cp->stream));
       } else {
         R* arrs[15];
         for (int q = 0; q < 15; ++q) arrs[q] = B.num_diff + cp->d.n3 * (size_t)q;
-        HIPCHK(halo_fill(*cp, arrs, 15, s));
+        HIPCHK(halo_fill(*cp, arrs, 15, cp->stream));
       }
     }
+    HIPCHK(join());
     // four stages, the last with the RK4 combination into the state
     for (int r = 0; r < 4; ++r) {
       HIPCHK(hevi_batch_launch(E.stage[r], E.stage_bpc, E.ncu, s));
+      HIPCHK(fork());
       for (Ctx* cp : E.members) {
         const ShortBufs<R> B = model_bufs<R>(*cp->model);
         if (r < 3) {
           R* const rr[5] = {B.Rk[r][0], B.Rk[r][1], B.Rk[r][2], B.Rk[r][3], B.Rk[r][4]};
-          HIPCHK(halo_fill(*cp, rr, 5, s));
+          HIPCHK(halo_fill(*cp, rr, 5, cp->stream));
         } else {
-          HIPCHK(halo_fill(*cp, B.st, 5, s));
+          HIPCHK(halo_fill(*cp, B.st, 5, cp->stream));
+          HIPCHK(large_after_short(*cp));
         }
       }
+      HIPCHK(join());
     }
-    for (Ctx* cp : E.members) HIPCHK(large_after_short(*cp));
   }
+  HIPCHK(fork());
   for (Ctx* cp : E.members) {
     HIPCHK(large_tracers(*cp, nstep));
     if (cp->model->mixed) HIPCHK(model_state_to_f64(*cp));
   }
-  return hipSuccess;
+  return join();
 }
 
 // the members' own streams and timers are set aside while the ensemble's
-// stream carries their work
+// lanes carry their work
 struct MemberStreams {
   Ensemble& E;
   std::vector<hipStream_t> st;
   std::vector<Prof*> pr;
   explicit MemberStreams(Ensemble& e) : E(e) {
-    for (Ctx* cp : E.members) {
+    for (size_t m = 0; m < E.members.size(); ++m) {
+      Ctx* cp = E.members[m];
       st.push_back(cp->stream); pr.push_back(cp->prof);
-      cp->stream = E.stream; cp->prof = nullptr;
+      cp->stream = E.lane[m % (size_t)E.nlanes]; cp->prof = nullptr;
     }
   }
   ~MemberStreams() {
@@ -916,6 +938,11 @@ hipError_t ensemble_large_step(Ensemble& E) {
 
 void ensemble_free(Ensemble& E) {
   if (E.stream) (void)hipStreamSynchronize(E.stream);
+  for (int l = 0; l < Ensemble::MAX_LANES; ++l) {
+    if (E.lane[l]) { (void)hipStreamSynchronize(E.lane[l]); (void)hipStreamDestroy(E.lane[l]); E.lane[l] = nullptr; }
+    if (E.ev_lane[l]) { (void)hipEventDestroy(E.ev_lane[l]); E.ev_lane[l] = nullptr; }
+  }
+  if (E.ev_main) { (void)hipEventDestroy(E.ev_main); E.ev_main = nullptr; }
   if (E.exec) (void)hipGraphExecDestroy(E.exec);
   if (E.graph) (void)hipGraphDestroy(E.graph);
   E.exec = nullptr; E.graph = nullptr;

@@ -66,6 +66,13 @@ struct Ensemble {
   int stage_bpc = 0, nf_bpc = 0, nf_tj = 8;
   BatchBuf stage[4], nf;
   bool built = false;
+  // lanes: streams the members' own launches spread over (member m on lane
+  // m % nlanes), forked from and joined back to `stream` around each batched
+  // launch, so that the small per-member kernels of different members overlap
+  static constexpr int MAX_LANES = 8;
+  int nlanes = 4;
+  hipStream_t lane[MAX_LANES] = {};
+  hipEvent_t ev_main = nullptr, ev_lane[MAX_LANES] = {};
   // the large step captured once and replayed
   bool use_graph = false;
   hipGraph_t graph = nullptr;

@@ -218,6 +218,11 @@ void adj_stage_bufs(Ctx& c, double* s0b[5], double* sb[5], double** NDb, double*
 // ATMOS_DYN_divergence on i = IS..IE+1, j = JS..JE+1, k = KS-1..KE+1
 hipError_t divergence(const Ctx& c, double* DDIV, const double* MOMZ, const double* MOMX, const double* MOMY, hipStream_t s);
 hipError_t halo_fill(const Ctx& c, double* const* arrs, int n, hipStream_t s);
+// terrain-following metrics from the surface height Zsfc(IA,JA) (device) and
+// the reference heights CZ(KA), FZ(0:KA) (device), Htop = FZ(KE) - FZ(KS-1):
+// GSQRT, J13G, J23G (7 components each, with halos) and PHI (metrics.hip)
+hipError_t terrain_metrics(const Ctx& c, const double* Zsfc, const double* CZ, const double* FZ, double Htop,
+                           double* GSQRT, double* J13G, double* J23G, double* PHI, hipStream_t s);
 hipError_t halo_fill(const Ctx& c, float* const* arrs, int n, hipStream_t s);
 // Fortran (k fastest) <-> device (i fastest) layout; ncomp fields of KA*IA*JA
 hipError_t to_device_layout(const Ctx& c, double* dst, const double* src_f, int ncomp, hipStream_t s);

@@ -234,6 +234,26 @@ class Context:
                                           "CDY", "FDY", "RCDY", "RFDY", "PHI", "GSQRT", "J13G", "J23G")],
             _d(m["J33G"]), _p(m["MAPF"])))
 
+    def set_terrain(self, m: dict, fz, zsfc):
+        """Terrain-following metrics computed on the device (scale_hip_set_terrain)
+        from the surface height zsfc(IA,JA), the reference face heights fz(0:KA)
+        (cell centres halfway) and the 1-D grid arrays and MAPF of m."""
+        fz = np.ascontiguousarray(fz, dtype=np.float64)
+        cz = np.ascontiguousarray(0.5 * (fz[1:] + fz[:-1]))
+        _check(load_library().scale_hip_set_terrain(
+            self._h, *[_p(np.ascontiguousarray(m[k])) for k in ("CDZ", "FDZ", "RCDZ", "RFDZ", "CDX", "FDX", "RCDX",
+                                                                  "RFDX", "CDY", "FDY", "RCDY", "RFDY")],
+            _p(cz), _p(fz), _p(np.asfortranarray(zsfc, dtype=np.float64)), _p(np.asfortranarray(m["MAPF"]))))
+
+    def get_metrics(self):
+        """GSQRT, J13G, J23G (KA,IA,JA,7) and PHI (KA,IA,JA) as the context holds them."""
+        g = self.grid
+        out = dict(GSQRT=np.zeros((g.KA, g.IA, g.JA, 7), order="F"), J13G=np.zeros((g.KA, g.IA, g.JA, 7), order="F"),
+                   J23G=np.zeros((g.KA, g.IA, g.JA, 7), order="F"), PHI=np.zeros((g.KA, g.IA, g.JA), order="F"))
+        _check(load_library().scale_hip_get_metrics(self._h, _p(out["GSQRT"]), _p(out["J13G"]), _p(out["J23G"]),
+                                                    _p(out["PHI"])))
+        return out
+
     def model_create(self, dtl, dts, QA, nd_coef=1.0e-4, nd_sfc_fact=1.0, fct_tracer=False, fct_along_stream=True,
                      mixed_precision=False, divdmp_coef=0.0, nd_coef_q=0.0, short="hevi", tinteg_short="RK4",
                      tinteg_tracer="RK3WS2002"):
@@ -331,15 +351,19 @@ class Context:
 
 def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, device=0, mixed_precision=False,
                     divdmp_coef=0.0, nd_coef_q=0.0, short="hevi", dtl=None, tinteg_short="RK4",
-                    tinteg_tracer="RK3WS2002") -> Context:
+                    tinteg_tracer="RK3WS2002", zsfc=None) -> Context:
     """Create a context holding the device-resident model of an inputs.Case
     (mixed_precision: FP32 dynamics with FP64 solve and reference state, the
     tracers advanced in FP64; short = "hevi" or "heve", ATMOS_DYN_TYPE; dtl:
     the large step, case.dt (the short step) by default, so nstep =
-    ceil(dtl/dt))."""
+    ceil(dtl/dt)); zsfc: a surface height (IA,JA) whose terrain-following
+    metrics the device computes, instead of case.metric's)."""
     if ctx is None:
         ctx = Context(case.grid, case.scheme, case.scheme_tracer, device)
-    ctx.set_metrics(case.metric)
+    if zsfc is None:
+        ctx.set_metrics(case.metric)
+    else:  # terrain-following metrics generated on the device from the surface height
+        ctx.set_terrain(case.metric, case.fz, zsfc)
     qa = case.qtrc.shape[-1]
     ctx.model_create(case.dt if dtl is None else dtl, case.dt, qa, nd_coef=nd_coef,
                      fct_tracer=case.fct if fct is None else fct, mixed_precision=mixed_precision,
