@@ -947,7 +947,7 @@ int scale_hip_model_large_step(scale_hip_ctx* h, int32_t n) {
   if (!h->c.model) return fail("model not created");
   shp::prof_start(h->c, "model_large_step", h->c.stream);
   for (int s = 0; s < n; ++s) {
-    CK(shp::model_state_to_f64(h->c));  // the preparation reads the FP64 state
+    CK(shp::model_state_to_f64(h->c, shp::F64_DENS | shp::F64_RHOT));  // the preparation reads them
     CK(shp::model_large_step(h->c));
   }
   CK(shp::model_state_to_f64(h->c));
@@ -1231,7 +1231,7 @@ int scale_hip_ensemble_destroy(scale_hip_ensemble* h) {
 int scale_hip_ensemble_large_step(scale_hip_ensemble* h, int32_t n) {
   if (int r = check_ens(h)) return r;
   CK(hipSetDevice(h->E.device));
-  for (int s = 0; s < n; ++s) CK(shp::ensemble_large_step(h->E));
+  CK(shp::ensemble_large_step(h->E, n));
   return 0;
 }
 
@@ -1307,9 +1307,9 @@ int scale_hip_ensemble_tune(scale_hip_ensemble* h, int32_t steps, double* best_m
     for (int sb : stage_c) {
       for (int nb : nfb_c) {
         if ((err = ens_set_shape(E, sb, nb, tj))) break;
-        hipError_t e = shp::ensemble_large_step(E);  // warm-up
+        hipError_t e = shp::ensemble_large_step(E, 1);  // warm-up
         if (e == hipSuccess) e = hipEventRecord(a, E.stream);
-        for (int q = 0; q < steps && e == hipSuccess; ++q) e = shp::ensemble_large_step(E);
+        if (e == hipSuccess) e = shp::ensemble_large_step(E, steps);
         if (e == hipSuccess) e = hipEventRecord(b, E.stream);
         if (e == hipSuccess) e = hipEventSynchronize(b);
         float ms = 0.f;

@@ -26,17 +26,12 @@ struct HaloArgs {
   int nW;  // halo points per level: west/east strips then south/north rows
 };
 
-template <typename T>
-__global__ void halo_kernel(HaloArgs<T> h) {
-  const Dims& d = h.d;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
-  const int f = blockIdx.z;
-  if (t >= h.nW || k >= d.KA || f >= h.n) return;
+// halo point t of a level (west/east strips of the interior rows, then the
+// south/north rows) and the interior point it copies
+__device__ __forceinline__ void halo_point(const Dims& d, int t, int& i, int& j, int& si, int& sj) {
   const int IMAX = d.IE - d.IS + 1, JMAX = d.JE - d.JS + 1;
   const int hw = d.IS;  // halo width in x
   const int hs = d.JS;  // halo width in y
-  int i, j;
   const int nstrip = 2 * hw * JMAX;  // west + east columns of interior rows
   if (t < nstrip) {
     const int r = t / (2 * hw), c = t % (2 * hw);
@@ -48,10 +43,41 @@ __global__ void halo_kernel(HaloArgs<T> h) {
     i = c;
     j = r < hs ? r : d.JE + 1 + (r - hs);
   }
-  const int si = i < d.IS ? i + IMAX : (i > d.IE ? i - IMAX : i);
-  const int sj = j < d.JS ? j + JMAX : (j > d.JE ? j - JMAX : j);
+  si = i < d.IS ? i + IMAX : (i > d.IE ? i - IMAX : i);
+  sj = j < d.JS ? j + JMAX : (j > d.JE ? j - JMAX : j);
+}
+
+template <typename T>
+__global__ void halo_kernel(HaloArgs<T> h) {
+  const Dims& d = h.d;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  const int f = blockIdx.z;
+  if (t >= h.nW || k >= d.KA || f >= h.n) return;
+  int i, j, si, sj;
+  halo_point(d, t, i, j, si, sj);
   T* a = h.a[f];
   a[IX(d, k, i, j)] = a[IX(d, k, si, sj)];
+}
+
+// the halos of many fields of many grids in one launch: blockIdx.z is the
+// field, each with its own grid (ensembles)
+template <typename T>
+struct HaloEnt {
+  Dims d;
+  T* a;
+  int nW;
+};
+template <typename T>
+__global__ void halo_batch_kernel(const HaloEnt<T>* __restrict__ tab) {
+  const HaloEnt<T>& e = tab[blockIdx.z];
+  const Dims& d = e.d;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (t >= e.nW || k >= d.KA) return;
+  int i, j, si, sj;
+  halo_point(d, t, i, j, si, sj);
+  e.a[IX(d, k, i, j)] = e.a[IX(d, k, si, sj)];
 }
 
 template <typename T>
@@ -362,6 +388,31 @@ __global__ void axpy_interior(Dims d, double* y, const double* x, double scale_d
   else y[p] = x[p] / scale_div;
 }
 
+// av += f on the interior, the three components of a mass flux in one launch
+// (blockIdx.z = j + JMAX * component)
+template <typename R>
+__global__ void mflx_sum(Dims d, double* av, const R* f) {
+  const int jmax = d.JE - d.JS + 1;
+  const int i = d.IS + blockIdx.x * 64 + threadIdx.x;
+  const int k = d.KS + blockIdx.y * 4 + threadIdx.y;
+  const int j = d.JS + (int)(blockIdx.z % jmax);
+  const size_t comp = blockIdx.z / jmax;
+  if (i > d.IE || k > d.KE) return;
+  const size_t p = IX(d, k, i, j) + d.n3 * comp;
+  av[p] = av[p] + (double)f[p];
+}
+// mflx = av / nstep on the interior, three components in one launch
+__global__ void mflx_mean(Dims d, double* mflx, const double* av, double nstep) {
+  const int jmax = d.JE - d.JS + 1;
+  const int i = d.IS + blockIdx.x * 64 + threadIdx.x;
+  const int k = d.KS + blockIdx.y * 4 + threadIdx.y;
+  const int j = d.JS + (int)(blockIdx.z % jmax);
+  const size_t comp = blockIdx.z / jmax;
+  if (i > d.IE || k > d.KE) return;
+  const size_t p = IX(d, k, i, j) + d.n3 * comp;
+  mflx[p] = av[p] / nstep;
+}
+
 // DENS_RK for the tracer RK3 stages (tinteg_tracer_rk3.F90:140-172)
 __global__ void dens_rk(Dims d, double* out, const double* D0, const double* D, int third) {
   const int i = d.IS - 1 + blockIdx.x * 64 + threadIdx.x;
@@ -435,6 +486,9 @@ static hipError_t fill_zero(T* a, size_t n, hipStream_t s) {
 static inline dim3 grid_full(const Dims& d) { return dim3((d.IA + 63) / 64, (d.KA + 3) / 4, d.JA); }
 static inline dim3 grid_int(const Dims& d) {
   return dim3((d.IE - d.IS + 1 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 1);
+}
+static inline dim3 grid_int3(const Dims& d) {  // interior, three components
+  return dim3((d.IE - d.IS + 1 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, 3 * (d.JE - d.JS + 1));
 }
 static inline dim3 grid_intk(const Dims& d) {  // interior columns, all levels
   return dim3((d.IE - d.IS + 1 + 63) / 64, (d.KA + 3) / 4, d.JE - d.JS + 1);
@@ -602,14 +656,18 @@ hipError_t model_short_step_rk4(Ctx& c, double dts, const StepRec* rec) {
 // mixed precision: the FP32 state is the integrated one; these keep the FP64
 // copies (read by model_get, the budgets and the large-step preparation) and
 // the FP32 copies in step
-hipError_t model_state_to_f64(Ctx& c) {
+hipError_t model_state_to_f64(Ctx& c, int parts) {
   Model& M = *c.model;
   if (!M.mixed) return hipSuccess;
   double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
-  for (int v = 1; v < 4; ++v) HIPCHK(convert(st[v], (const float*)M.fS[v], c.d.n3, c.stream));
-  hipLaunchKernelGGL(from_dev32, dim3(2048), dim3(256), 0, c.stream, M.DENS, M.fS[0], M.foff, c.d.n3, c.d.pl);
-  hipLaunchKernelGGL(from_dev32, dim3(2048), dim3(256), 0, c.stream, M.RHOT, M.fS[4], M.foff + c.d.KA, c.d.n3, c.d.pl);
+  if (parts & F64_MOM)
+    for (int v = 1; v < 4; ++v) HIPCHK(convert(st[v], (const float*)M.fS[v], c.d.n3, c.stream));
+  if (parts & F64_DENS)
+    hipLaunchKernelGGL(from_dev32, dim3(2048), dim3(256), 0, c.stream, M.DENS, M.fS[0], M.foff, c.d.n3, c.d.pl);
+  if (parts & F64_RHOT)
+    hipLaunchKernelGGL(from_dev32, dim3(2048), dim3(256), 0, c.stream, M.RHOT, M.fS[4], M.foff + c.d.KA, c.d.n3, c.d.pl);
   HIPCHK(hipGetLastError());
+  if (!(parts & F64_FLUX)) return hipSuccess;
   HIPCHK(convert(M.mflx_hi, (const float*)M.fmflx, 3 * c.d.n3, c.stream));
   return convert(M.tflx_hi, (const float*)M.ftflx, 3 * c.d.n3, c.stream);
 }
@@ -643,11 +701,15 @@ hipError_t model_prepare(Ctx& c) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(M.DENS00, M.DENS, bytes, hipMemcpyDeviceToDevice, s));
   HIPCHK(fill_zero(M.mflx_av, 3 * d.n3, s));
+  // without physics tendencies the tendencies stay zero: set them once
+  const bool none = !M.tp[0] && !M.tp[1] && !M.tp[2] && !M.tp[3] && !M.tp[4];
+  if (none && M.tend_zero) return hipSuccess;
   for (int v = 0; v < 5; ++v) {
     hipLaunchKernelGGL(tend_prep, grid_full(d), dim3(64, 4), 0, s, d, M.t[v], (const double*)M.tp[v], v == 1 ? 1 : 0);
     HIPCHK(hipGetLastError());
     if (M.mixed) HIPCHK(convert(M.ft[v], (const double*)M.t[v], d.n3, s));
   }
+  M.tend_zero = none;
   return hipSuccess;
 }
 
@@ -667,16 +729,17 @@ static hipError_t large_begin(Ctx& c, const StepRec* rec) {
   return hipSuccess;
 }
 
-// after each short step: the sum of the mass fluxes over the short steps (LARGE.F90:960-970)
+// after each short step: the sum of the mass fluxes over the short steps
+// (LARGE.F90:960-970), the three components in one launch (in mixed
+// precision from the FP32 fluxes, widened on the fly)
 static hipError_t large_after_short(Ctx& c) {
   Model& M = *c.model;
   const Dims& d = c.d;
-  hipStream_t s = c.stream;
-  // mixed precision: the step's averaged mass flux is FP32; the average over
-  // the short steps and the tracer step use its FP64 copy
-  if (M.mixed && M.QA > 0) HIPCHK(convert(M.mflx_hi, (const float*)M.fmflx, 3 * d.n3, s));
-  for (int n = 0; n < 3; ++n) {
-    hipLaunchKernelGGL(axpy_interior, grid_int(d), dim3(64, 4), 0, s, d, M.mflx_av + d.n3 * n, M.mflx_hi + d.n3 * n, 1.0, 0);
+  if (M.mixed) {
+    if (M.QA <= 0) return hipSuccess;  // only the tracers read the sum
+    hipLaunchKernelGGL(mflx_sum<float>, grid_int3(d), dim3(64, 4), 0, c.stream, d, M.mflx_av, (const float*)M.fmflx);
+  } else {
+    hipLaunchKernelGGL(mflx_sum<double>, grid_int3(d), dim3(64, 4), 0, c.stream, d, M.mflx_av, (const double*)M.mflx_hi);
   }
   return hipGetLastError();
 }
@@ -697,11 +760,10 @@ static hipError_t large_tracers(Ctx& c, int nstep) {
   hipStream_t s = c.stream;
   const double dtl = M.dtl;
   if (M.QA <= 0) return hipSuccess;
-  // mixed precision: the tracers advance in FP64 on the FP64 copies of the
-  // FP32 state (DENS at the end of the step) and of the mass fluxes
-  if (M.mixed) HIPCHK(model_state_to_f64(c));
-  for (int n = 0; n < 3; ++n)
-    hipLaunchKernelGGL(axpy_interior, grid_int(d), dim3(64, 4), 0, s, d, M.mflx_hi + d.n3 * n, M.mflx_av + d.n3 * n, (double)nstep, 1);
+  // mixed precision: the tracers advance in FP64 on the FP64 copy of the
+  // end-of-step density and the FP64 sum of the mass fluxes
+  if (M.mixed) HIPCHK(model_state_to_f64(c, F64_DENS));
+  hipLaunchKernelGGL(mflx_mean, grid_int3(d), dim3(64, 4), 0, s, d, M.mflx_hi, (const double*)M.mflx_av, (double)nstep);
   HIPCHK(hipGetLastError());
   double* mf[3] = {M.mflx_hi, M.mflx_hi + d.n3, M.mflx_hi + 2 * d.n3};
   HIPCHK(halo_fill(c, mf, 3, s));
@@ -794,9 +856,51 @@ ShortBufs<float> model_bufs<float>(Model& M) {
                           M.fmflx, M.ftflx, M.fDDIV};
 }
 
+// a device table of halo entries for the given fields of every member
+template <typename R>
+static hipError_t halo_table(Ensemble& E, int slot, const std::vector<std::pair<const Ctx*, R*>>& fields) {
+  std::vector<HaloEnt<R>> ents;
+  int wmax = 0, kmax = 0;
+  for (const auto& f : fields) {
+    const Dims& d = f.first->d;
+    HaloEnt<R> e;
+    e.d = d; e.a = f.second; e.nW = 2 * d.IS * (d.JE - d.JS + 1) + 2 * d.JS * d.IA;
+    wmax = e.nW > wmax ? e.nW : wmax;
+    kmax = d.KA > kmax ? d.KA : kmax;
+    ents.push_back(e);
+  }
+  if (E.halo_tab[slot]) { (void)hipFree(E.halo_tab[slot]); E.halo_tab[slot] = nullptr; }
+  HIPCHK(hipMalloc(&E.halo_tab[slot], ents.size() * sizeof(HaloEnt<R>)));
+  HIPCHK(hipMemcpy(E.halo_tab[slot], ents.data(), ents.size() * sizeof(HaloEnt<R>), hipMemcpyHostToDevice));
+  E.halo_n[slot] = (int)ents.size(); E.halo_w[slot] = wmax; E.halo_k[slot] = kmax;
+  return hipSuccess;
+}
+template <typename R>
+static hipError_t halo_table_launch(const Ensemble& E, int slot, hipStream_t s) {
+  hipLaunchKernelGGL(halo_batch_kernel<R>, dim3((E.halo_w[slot] + 255) / 256, E.halo_k[slot], E.halo_n[slot]), dim3(256), 0,
+                     s, (const HaloEnt<R>*)E.halo_tab[slot]);
+  return hipGetLastError();
+}
+
 template <typename R>
 static hipError_t ensemble_build_t(Ensemble& E) {
   const int n = (int)E.members.size();
+  // halo tables: slot 0 the filter's 15 fields, 1..3 the stage outputs of
+  // stages 1..3, 4 the state
+  for (int slot = 0; slot < 5; ++slot) {
+    std::vector<std::pair<const Ctx*, R*>> f;
+    for (Ctx* cp : E.members) {
+      const ShortBufs<R> B = model_bufs<R>(*cp->model);
+      if (slot == 0) {
+        for (int q = 0; q < 15; ++q) f.push_back({cp, B.num_diff + cp->d.n3 * (size_t)q});
+      } else if (slot < 4) {
+        for (int v = 0; v < 5; ++v) f.push_back({cp, B.Rk[slot - 1][v]});
+      } else {
+        for (int v = 0; v < 5; ++v) f.push_back({cp, B.st[v]});
+      }
+    }
+    HIPCHK(halo_table<R>(E, slot, f));
+  }
   for (int r = 0; r < 4; ++r) {
     std::vector<StageIOT<R>> ios;
     for (Ctx* cp : E.members) {
@@ -850,49 +954,46 @@ static hipError_t ensemble_step_t(Ensemble& E) {
   };
   HIPCHK(fork());
   for (Ctx* cp : E.members) {
-    if (cp->model->mixed) HIPCHK(model_state_to_f64(*cp));
+    // the preparation reads the FP64 DENS and RHOT
+    if (cp->model->mixed) HIPCHK(model_state_to_f64(*cp, F64_DENS | F64_RHOT));
     HIPCHK(large_begin(*cp, nullptr));
   }
   HIPCHK(join());
   for (int step = 0; step < nstep; ++step) {
-    // numerical filter (LARGE.F90:845-858): one launch, then each member's halo
+    // numerical filter (LARGE.F90:845-858) and its halos: one launch each
+    // (a member with ND_COEF = 0 gets zeros)
     HIPCHK(numfilter_batch_launch(E.nf, E.nf_bpc, E.ncu, s));
-    HIPCHK(fork());
+    HIPCHK(halo_table_launch<R>(E, 0, s));
     for (Ctx* cp : E.members) {
-      Model& M = *cp->model;
-      const ShortBufs<R> B = model_bufs<R>(M);
-      if (M.nd_coef == 0.0) {
-        HIPCHK(fill_zero(B.num_diff, 15 * cp->d.n3, cp->stream));
-      } else {
-        R* arrs[15];
-        for (int q = 0; q < 15; ++q) arrs[q] = B.num_diff + cp->d.n3 * (size_t)q;
-        HIPCHK(halo_fill(*cp, arrs, 15, cp->stream));
-      }
+      if (cp->model->nd_coef == 0.0) HIPCHK(fill_zero(model_bufs<R>(*cp->model).num_diff, 15 * cp->d.n3, s));
     }
-    HIPCHK(join());
-    // four stages, the last with the RK4 combination into the state
+    // four stages, the last with the RK4 combination into the state, each
+    // with the halos of its outputs in one launch
     for (int r = 0; r < 4; ++r) {
       HIPCHK(hevi_batch_launch(E.stage[r], E.stage_bpc, E.ncu, s));
-      HIPCHK(fork());
-      for (Ctx* cp : E.members) {
-        const ShortBufs<R> B = model_bufs<R>(*cp->model);
-        if (r < 3) {
-          R* const rr[5] = {B.Rk[r][0], B.Rk[r][1], B.Rk[r][2], B.Rk[r][3], B.Rk[r][4]};
-          HIPCHK(halo_fill(*cp, rr, 5, cp->stream));
-        } else {
-          HIPCHK(halo_fill(*cp, B.st, 5, cp->stream));
-          HIPCHK(large_after_short(*cp));
-        }
-      }
-      HIPCHK(join());
+      HIPCHK(halo_table_launch<R>(E, r + 1, s));
     }
+    HIPCHK(fork());
+    for (Ctx* cp : E.members) HIPCHK(large_after_short(*cp));
+    HIPCHK(join());
   }
   HIPCHK(fork());
-  for (Ctx* cp : E.members) {
-    HIPCHK(large_tracers(*cp, nstep));
-    if (cp->model->mixed) HIPCHK(model_state_to_f64(*cp));
-  }
+  for (Ctx* cp : E.members) HIPCHK(large_tracers(*cp, nstep));
   return join();
+}
+
+// every member's FP64 copies of its FP32 state and fluxes (mixed precision),
+// as scale_hip_model_large_step leaves them
+static hipError_t ensemble_finish(Ensemble& E) {
+  hipStream_t s = E.stream;
+  HIPCHK(hipEventRecord(E.ev_main, s));
+  for (int l = 0; l < E.nlanes; ++l) HIPCHK(hipStreamWaitEvent(E.lane[l], E.ev_main, 0));
+  for (Ctx* cp : E.members) HIPCHK(model_state_to_f64(*cp));
+  for (int l = 0; l < E.nlanes; ++l) {
+    HIPCHK(hipEventRecord(E.ev_lane[l], E.lane[l]));
+    HIPCHK(hipStreamWaitEvent(s, E.ev_lane[l], 0));
+  }
+  return hipSuccess;
 }
 
 // the members' own streams and timers are set aside while the ensemble's
@@ -917,23 +1018,29 @@ static hipError_t ensemble_step_direct(Ensemble& E) {
   return E.members[0]->model->mixed ? ensemble_step_t<float>(E) : ensemble_step_t<double>(E);
 }
 
-hipError_t ensemble_large_step(Ensemble& E) {
+hipError_t ensemble_large_step(Ensemble& E, int n) {
   if (!E.built) HIPCHK(ensemble_build(E));
   // the members' pending work on their own streams comes first
   for (Ctx* cp : E.members) HIPCHK(hipStreamSynchronize(cp->stream));
   MemberStreams guard(E);
-  if (!E.use_graph) return ensemble_step_direct(E);
-  if (!E.exec) {
-    HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeRelaxed));
-    const hipError_t e = ensemble_step_direct(E);
-    hipGraph_t g = nullptr;
-    const hipError_t e2 = hipStreamEndCapture(E.stream, &g);
-    if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return e; }
-    HIPCHK(e2);
-    E.graph = g;
-    HIPCHK(hipGraphInstantiate(&E.exec, E.graph, nullptr, nullptr, 0));
+  for (int q = 0; q < n; ++q) {
+    if (!E.use_graph) {
+      HIPCHK(ensemble_step_direct(E));
+      continue;
+    }
+    if (!E.exec) {
+      HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeRelaxed));
+      const hipError_t e = ensemble_step_direct(E);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(E.stream, &g);
+      if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return e; }
+      HIPCHK(e2);
+      E.graph = g;
+      HIPCHK(hipGraphInstantiate(&E.exec, E.graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(E.exec, E.stream));
   }
-  return hipGraphLaunch(E.exec, E.stream);
+  return E.members[0]->model->mixed ? ensemble_finish(E) : hipSuccess;
 }
 
 void ensemble_free(Ensemble& E) {
@@ -948,6 +1055,10 @@ void ensemble_free(Ensemble& E) {
   E.exec = nullptr; E.graph = nullptr;
   for (auto& b : E.stage) batch_free(b);
   batch_free(E.nf);
+  for (int q = 0; q < 5; ++q) {
+    if (E.halo_tab[q]) (void)hipFree(E.halo_tab[q]);
+    E.halo_tab[q] = nullptr;
+  }
   E.built = false;
 }
 

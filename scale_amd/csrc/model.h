@@ -46,6 +46,8 @@ struct Model {
   // ATMOS_DYN_TINTEG_SHORT_TYPE: 4 = RK4, 3 = RK3 (Heun), 32 = RK3WS2002;
   // ATMOS_DYN_TINTEG_TRACER_TYPE: 0 = RK3WS2002, 1 = EULER
   int tinteg_short = 4, tinteg_tracer = 0;
+  // the tendencies hold zeros with no physics tendencies set: not refilled
+  bool tend_zero = false;
   // every allocation, freed together
   std::vector<void*> allocs;
 };
@@ -65,6 +67,9 @@ struct Ensemble {
   // (0: one block per tile) and the filter's tile rows
   int stage_bpc = 0, nf_bpc = 0, nf_tj = 8;
   BatchBuf stage[4], nf;
+  // halo tables (device): the filter's fields, the outputs of stages 1..3, the state
+  void* halo_tab[5] = {};
+  int halo_n[5] = {}, halo_w[5] = {}, halo_k[5] = {};
   bool built = false;
   // lanes: streams the members' own launches spread over (member m on lane
   // m % nlanes), forked from and joined back to `stream` around each batched
@@ -80,7 +85,7 @@ struct Ensemble {
 };
 hipError_t ensemble_check(const Ensemble& E, const char** why);
 hipError_t ensemble_build(Ensemble& E);
-hipError_t ensemble_large_step(Ensemble& E);
+hipError_t ensemble_large_step(Ensemble& E, int n);
 void ensemble_free(Ensemble& E);
 
 // rec (FP64 flat HEVI without divergence damping, one short step per large
@@ -90,7 +95,9 @@ hipError_t model_prepare(Ctx& c);
 hipError_t large_prep_into(Ctx& c, const double* RHOT, double* RT2P, double* DPRES0, double* REF_rhot, hipStream_t s);
 hipError_t model_large_step(Ctx& c, const StepRec* rec = nullptr);
 hipError_t mass_sum(const Ctx& c, const double* a, const double* b, double* out_host);
-hipError_t model_state_to_f64(Ctx& c);
+// parts of the FP64 copies to refresh from the FP32 state (mixed precision)
+enum { F64_DENS = 1, F64_RHOT = 2, F64_MOM = 4, F64_FLUX = 8, F64_ALL = 15 };
+hipError_t model_state_to_f64(Ctx& c, int parts = F64_ALL);
 hipError_t model_state_to_f32(Ctx& c);
 // adjoint (adjoint.hip): flat terrain, FP64, divdmp_coef = 0, one short step per large step
 hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every);
