@@ -92,3 +92,53 @@ def test_fortran_dropin_bitwise(tmp_path, scheme, short):
     po.tstep_tracer(case, q_ref, qtrc, qtrc, rhoq_t, inp["DENS0"], inp["DENS"], mflx, nd_q, dtl,
                     fct=True, along_stream=True, scheme=scheme)
     assert np.array_equal(q_ref, got["QTRCo"])
+
+
+LARGE_DRIVER = os.path.join(ROOT, "tests", "fortran", "large_driver")
+
+
+@pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
+def test_fortran_large_step_dropin(tmp_path, scheme):
+    """The large-step drop-in ATMOS_DYN_Tstep_large_fvm_heve_hip, called from
+    Fortran with the dummies of the interface `large` for 3 large steps of a
+    case with two FCT tracers: the state and the tracers within 1e-12 of the
+    oracle model (pow() in the large-step preparation may round its last bit
+    differently on the GPU)."""
+    print(f"[fortran large-step drop-in] scheme={scheme}", flush=True)
+    if not os.path.exists(LARGE_DRIVER):
+        pytest.fail("tests/fortran/large_driver is not built (run __graft_entry__.build())")
+    from scale_amd import inputs
+    case = inputs.make_case(kmax=20, imax=16, jmax=12, noise=0.05, ntracer=2, tracer_kind=["blob", "tophat"], fct=True,
+                            bubble_r=2000.0, bubble_zc=2000.0, momz_noise=0.05, v0=-2.0, scheme=scheme,
+                            scheme_tracer=scheme)
+    g = case.grid
+    m = case.metric
+    nsteps = 3
+    hdr = np.array([g.KA, g.IA, g.JA, g.KS + 1, g.KE + 1, g.IS + 1, g.IE + 1, g.JS + 1, g.JE + 1,
+                    scheme, scheme, 2, nsteps, 1], dtype=np.int32)
+    blob = [hdr.tobytes()]
+    blob += [f(case.state[k]) for k in H.VARS] + [f(case.qtrc)]
+    blob += [f(case.ref["REF_dens"]), f(case.ref["REF_pott"]), f(case.ref["REF_pres"]), f(case.corioli),
+             f(case.wdamp_coef)]
+    blob += [f(m[k]) for k in ("CDZ", "CDX", "CDY", "FDZ", "FDX", "FDY", "RCDZ", "RCDX", "RCDY", "RFDZ", "RFDX", "RFDY")]
+    blob += [f(m["PHI"]), f(m["GSQRT"]), f(m["J13G"]), f(m["J23G"]), f(m["J33G"]), f(m["MAPF"])]
+    blob += [f(case.dt), f(case.dt), f(1.0e-4)]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    fin.write_bytes(b"".join(blob))
+    r = subprocess.run([LARGE_DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.frombuffer(fout.read_bytes(), dtype=np.float64)
+    n3 = g.KA * g.IA * g.JA
+    assert raw.size == 7 * n3
+    ref = po.Model(case)
+    ref.step(nsteps)
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    for q, k in enumerate(H.VARS):
+        got = raw[q * n3:(q + 1) * n3].reshape(g.shape, order="F")
+        d = np.abs(got[sl] - ref.state[k][sl]).max()
+        print(f"  {k}: max |diff| {d:.3e}", flush=True)
+        assert d <= 1e-12 * np.abs(ref.state[k][sl]).max(), k
+    for iq in range(2):
+        got = raw[(5 + iq) * n3:(6 + iq) * n3].reshape(g.shape, order="F")
+        d = np.abs(got[sl] - ref.qtrc[..., iq][sl]).max()
+        assert d <= 1e-12 * max(np.abs(ref.qtrc[..., iq][sl]).max(), 1e-30), iq
