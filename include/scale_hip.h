@@ -130,6 +130,25 @@ int scale_hip_tstep_tracer_heve(scale_hip_ctx *ctx,
     const double *RCDZ, const double *RCDX, const double *RCDY,
     const double *dtl, const int32_t *FLAG_FCT_TRACER, const int32_t *FLAG_FCT_ALONG_STREAM);
 
+/* Adjoint of one scale_hip_tstep_tracer_heve call with the FCT limiter frozen.
+ * The reference has no adjoint; this is the transpose of the stage's
+ * Jacobian in QTRC, QTRC0, RHOQ_t and num_diff (KA,IA,JA,3) with the mass
+ * fluxes and densities held fixed and the limiter's ratios R+/R-
+ * (scale_atmos_dyn_common.F90:2522-2547) held at their forward values, so the
+ * limited anti-diffusive flux of each face is a fixed multiple of hi - lo.
+ * Takes the forward's inputs (same meaning and layout; RHOQ_t is not needed)
+ * and QTRCo_b, the adjoint of QTRCo, read at the points the stage writes
+ * (KS..KE, IS..IE, JS..JE); overwrites QTRC_b, QTRC0_b, RHOQ_t_b and
+ * num_diff_b on the whole array (halo points included: the stage reads QTRC
+ * and QTRC0 there).  UD3KOREN1993 (a limiter in the face form) is rejected. */
+int scale_hip_tstep_tracer_heve_adj(scale_hip_ctx *ctx,
+    const double *QTRCo_b, double *QTRC_b, double *QTRC0_b, double *RHOQ_t_b, double *num_diff_b,
+    const double *QTRC, const double *QTRC0,
+    const double *DENS0, const double *DENS, const double *mflx_hi, const double *num_diff,
+    const double *GSQRT, const double *MAPF, const double *CDZ,
+    const double *RCDZ, const double *RCDX, const double *RCDY,
+    const double *dtl, const int32_t *FLAG_FCT_TRACER, const int32_t *FLAG_FCT_ALONG_STREAM);
+
 /* The tracer stage of scale_hip_tstep_tracer_heve on device memory, with no
  * allocation of any kind.  This entry point has no counterpart in the
  * reference: it is for callers that cannot allocate while they step (code

@@ -13,7 +13,8 @@
  * Where the forward takes a branch on a value (sign(), max/min, abs,
  * comparisons, the upwind switch, the FCT limiter's choices), the tangent
  * follows the branch the value takes: the derivative of the branch taken,
- * as the adjoint holds such switches fixed.
+ * as the adjoint holds such switches fixed.  The FCT ratios R+/R- themselves
+ * are held constant (ORC_FROZEN): the limiter frozen at its forward values.
  *
  * Memory layout: a Dual is {value, tangent}, 16 bytes, so an array of n
  * Duals is a (2, n) array of doubles with the pair index fastest.
@@ -116,6 +117,11 @@ static inline int isnan(const Dual& a) { return isnan(a.v); }
 // piecewise constant: no tangent
 static inline Dual ceil(const Dual& a) { return Dual(ceil(a.v)); }
 static inline Dual floor(const Dual& a) { return Dual(floor(a.v)); }
+
+// the FCT limiter's ratios R+/R- are held at their values (no tangent): the
+// tangent-linear model of the frozen limiter, which the tracer adjoint
+// transposes
+#define ORC_FROZEN(x) (Dual((x).v, 0.0))
 
 #define double Dual
 

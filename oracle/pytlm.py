@@ -128,3 +128,51 @@ class TangentModel:
 
     def qtrc(self):
         return np.asfortranarray(self.q[0]), np.asfortranarray(self.q[1])
+
+
+def _tracer_call(fn, case, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff_q, dtl, fct, along_stream,
+                 scheme, metric, tangents):
+    g = case.grid
+    m = metric if metric is not None else case.metric
+    t = tangents or {}
+    keep = []
+
+    def d(v, key=None):
+        a = dual(v, t.get(key))
+        keep.append(a)
+        return dptr(a)
+
+    mapf_xy = np.asfortranarray(m["MAPF"][:, :, :, 0])
+    q = dual(QTRC, t.get("QTRC"))
+    args = [d(QTRC0, "QTRC0"), d(RHOQ_t, "RHOQ_t"), d(DENS0), d(DENS), d(mflx_hi), d(num_diff_q, "num_diff"),
+            d(m["GSQRT"]), d(mapf_xy), d(m["CDZ"]), d(m["RCDZ"]), d(m["RCDX"]), d(m["RCDY"]), DualC(float(dtl), 0.0),
+            C.c_int(int(fct)), C.c_int(int(along_stream))]
+    sch = C.c_int(scheme if scheme is not None else case.scheme_tracer)
+    if fn == "stage":
+        out = dual(np.full(g.shape, np.nan))
+        rc = lib().orc_tstep_tracer_heve(C.byref(po.grid_struct(g)), sch, dptr(out), dptr(q), *args)
+    else:
+        out = q
+        rc = lib().orc_tinteg_tracer_rk3(C.byref(po.grid_struct(g)), sch, dptr(out), *args)
+    if rc != 0:
+        raise RuntimeError(f"tracer {fn} (dual) failed")
+    return np.asfortranarray(out[0]), np.asfortranarray(out[1])
+
+
+def tracer_stage_tangent(case, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff_q, dtl, fct, along_stream=True,
+                         scheme=None, metric=None, tangents=None):
+    """One tracer stage (orc_tstep_tracer_heve) on dual numbers: returns
+    (QTRCo, its tangent) along the directions tangents["QTRC" | "QTRC0" |
+    "RHOQ_t" | "num_diff"] (absent keys: zero), the mass fluxes, densities and
+    metrics held fixed and the FCT ratios R+/R- frozen at their forward
+    values."""
+    return _tracer_call("stage", case, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff_q, dtl, fct, along_stream,
+                        scheme, metric, tangents)
+
+
+def tracer_rk3_tangent(case, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff_q, dtl, fct, along_stream=True,
+                       scheme=None, metric=None, tangents=None):
+    """The three-stage tracer integration (orc_tinteg_tracer_rk3) on dual
+    numbers, as tracer_stage_tangent: returns (QTRC after, its tangent)."""
+    return _tracer_call("rk3", case, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff_q, dtl, fct, along_stream,
+                        scheme, metric, tangents)

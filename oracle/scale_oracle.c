@@ -90,6 +90,11 @@ static double *alloc_nan(size_t n) {
 
 /* Fortran sign(a,b) = |a| with the sign bit of b (gfortran honours -0.0). */
 static inline double fsign(double a, double b) { return copysign(fabs(a), b); }
+/* the FCT ratios R+/R-: plain values here; the tangent-linear build (dual.h)
+ * holds them constant, the frozen limiter of the tracer adjoint */
+#ifndef ORC_FROZEN
+#define ORC_FROZEN(x) (x)
+#endif
 static inline double fmax2(double a, double b) { return a > b ? a : b; }
 static inline double fmin2(double a, double b) { return a < b ? a : b; }
 
@@ -1842,6 +1847,12 @@ static inline double fct_qb(const double *f, const double *phi, size_t (*ix)(int
 }
 static size_t ix3f(int k, int i, int j, const orc_grid *g) { const int KA = g->KA, IA = g->IA; return I3(k, i, j); }
 
+/* Test hook of the frozen-limiter checks: when set, orc_fct copies its
+ * ratios R+ then R- (2 x KA*IA*JA, before their halo fill) out to
+ * orc_fct_save_r, and replaces them by orc_fct_fixed_r. */
+double *orc_fct_save_r = NULL;
+const double *orc_fct_fixed_r = NULL;
+
 void orc_fct(const orc_grid *g, double *qflx_anti,
     const double *phi_in, const double *DENS0, const double *DENS,
     const double *qflx_hi, const double *qflx_lo, const double *mflx_hi,
@@ -1953,10 +1964,18 @@ void orc_fct(const orc_grid *g, double *qflx_anti,
     for (int i = IIS; i <= IIE; ++i)
       for (int k = KS; k <= KE; ++k) {
         double zerosw = 0.5 - fsign(0.5, pjpls[I3(k, i, j)] - EPSILON);
-        rjpls[I3(k, i, j)] = fmin2(1.0, qjpls[I3(k, i, j)] * (1.0 - zerosw) / (pjpls[I3(k, i, j)] - zerosw));
+        rjpls[I3(k, i, j)] = ORC_FROZEN(fmin2(1.0, qjpls[I3(k, i, j)] * (1.0 - zerosw) / (pjpls[I3(k, i, j)] - zerosw)));
         zerosw = 0.5 - fsign(0.5, pjmns[I3(k, i, j)] - EPSILON);
-        rjmns[I3(k, i, j)] = fmin2(1.0, qjmns[I3(k, i, j)] * (1.0 - zerosw) / (pjmns[I3(k, i, j)] - zerosw));
+        rjmns[I3(k, i, j)] = ORC_FROZEN(fmin2(1.0, qjmns[I3(k, i, j)] * (1.0 - zerosw) / (pjmns[I3(k, i, j)] - zerosw)));
       }
+  if (orc_fct_fixed_r) {
+    memcpy(rjpls, orc_fct_fixed_r, N3 * sizeof(double));
+    memcpy(rjmns, orc_fct_fixed_r + N3, N3 * sizeof(double));
+  }
+  if (orc_fct_save_r) {
+    memcpy(orc_fct_save_r, rjpls, N3 * sizeof(double));
+    memcpy(orc_fct_save_r + N3, rjmns, N3 * sizeof(double));
+  }
   orc_halo_periodic(g, rjpls);
   orc_halo_periodic(g, rjmns);
   /* limited anti-diffusive fluxes (COMMON.F90:2570-2659) */

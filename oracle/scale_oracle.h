@@ -123,6 +123,12 @@ void orc_fct(const orc_grid *g, double *qflx_anti,
     const double *RDZ, const double *RDX, const double *RDY,
     const double *GSQRT_XYZ, const double *MAPF_XY, double dt, int flag_vect);
 
+/* test hook of the frozen-limiter checks (both NULL by default): orc_fct
+ * copies R+ then R- (2 x KA*IA*JA, before their halo fill) out to
+ * orc_fct_save_r and replaces them by orc_fct_fixed_r when these are set */
+extern double *orc_fct_save_r;
+extern const double *orc_fct_fixed_r;
+
 /* get_fact_fct (COMMON.F90:2676-2734): fact(0:1,-1:1,-1:1) returned as
  * fact[a + 2*((b+1) + 3*(c+1))] for fact(a,b,c). */
 void orc_get_fact_fct(double *fact, double rw, double ru, double rv);

@@ -633,6 +633,57 @@ int scale_hip_tstep_tracer_heve(scale_hip_ctx* h,
   return download3(c, QTRCo, dQo, 1);
 }
 
+int scale_hip_tstep_tracer_heve_adj(scale_hip_ctx* h,
+    const double* QTRCo_b, double* QTRC_b, double* QTRC0_b, double* RHOQ_t_b, double* num_diff_b,
+    const double* QTRC, const double* QTRC0,
+    const double* DENS0, const double* DENS, const double* mflx_hi, const double* num_diff,
+    const double* GSQRT, const double* MAPF, const double* CDZ,
+    const double* RCDZ, const double* RCDX, const double* RCDY,
+    const double* dtl, const int32_t* FLAG_FCT_TRACER, const int32_t* FLAG_FCT_ALONG_STREAM) {
+  if (int r = check_ctx(h)) return r;
+  if (!dtl || !FLAG_FCT_TRACER || !FLAG_FCT_ALONG_STREAM) return fail("null scalar argument");
+  if (!QTRCo_b || !QTRC_b || !QTRC0_b || !RHOQ_t_b || !num_diff_b || !QTRC || !QTRC0 || !DENS0 || !DENS || !mflx_hi ||
+      !num_diff || !GSQRT || !MAPF || !RCDZ || !RCDX || !RCDY)
+    return fail("null array argument");
+  Ctx& c = h->c;
+  const Dims& d = c.d;
+  if (c.scheme_tracer == shp::UD3KOREN)
+    return fail("the tracer adjoint needs a face form linear in the tracer; UD3KOREN1993's limiter is not");
+  if (int r = ensure_dropin(h)) return r;
+  if (int r = set_grid1d(c, CDZ, nullptr, RCDZ, nullptr, nullptr, nullptr, RCDX, nullptr, nullptr, nullptr, RCDY, nullptr)) return r;
+  const bool flat = all_equal(GSQRT, 7 * d.n3, 1.0) && all_equal(MAPF, 2 * d.pl, 1.0);
+  if (!flat) {
+    std::vector<double> m4(8 * d.pl, 1.0);
+    memcpy(m4.data(), MAPF, 2 * d.pl * sizeof(double));
+    std::vector<double> zero(7 * d.n3, 0.0);
+    if (int r = set_metric3d(c, nullptr, GSQRT, zero.data(), zero.data(), 1.0, m4.data())) return r;
+  } else {
+    c.terrain = false;
+  }
+  auto& B = h->dbuf;
+  // inputs: QTRCo_b, QTRC, QTRC0, DENS0, DENS, mflx(3), num_diff(3); outputs in
+  // the tendency buffers and num_diff's components 3..5
+  double* dOb = B[B_OUT]; double* dQ = B[B_S]; double* dQ0 = B[B_S + 1];
+  double* dD0 = B[B_S + 3]; double* dD = B[B_S + 4]; double* dMF = B[B_MF]; double* dND = B[B_ND];
+  double* dQb = B[B_T]; double* dQ0b = B[B_T + 1]; double* dRQb = B[B_T + 2]; double* dNDb = B[B_ND] + 3 * d.n3;
+  if (int r = upload3(c, dOb, QTRCo_b, 1)) return r;
+  if (int r = upload3(c, dQ, QTRC, 1)) return r;
+  if (int r = upload3(c, dQ0, QTRC0, 1)) return r;
+  if (int r = upload3(c, dD0, DENS0, 1)) return r;
+  if (int r = upload3(c, dD, DENS, 1)) return r;
+  if (int r = upload3(c, dMF, mflx_hi, 3)) return r;
+  if (int r = upload3(c, dND, num_diff, 3)) return r;
+  shp::TracerIO io{};
+  io.QTRC = dQ; io.QTRC0 = dQ0; io.DENS0 = dD0; io.DENS = dD; io.mflx_hi = dMF; io.num_diff = dND;
+  io.dtl = *dtl; io.flag_fct = *FLAG_FCT_TRACER; io.flag_along = *FLAG_FCT_ALONG_STREAM;
+  shp::TracerAdjIO adj{dOb, dQb, dQ0b, dRQb, dNDb};
+  CK(shp::tracer_stage_adj(c, io, adj, c.stream));
+  if (int r = download3(c, QTRC_b, dQb, 1)) return r;
+  if (int r = download3(c, QTRC0_b, dQ0b, 1)) return r;
+  if (int r = download3(c, RHOQ_t_b, dRQb, 1)) return r;
+  return download3(c, num_diff_b, dNDb, 3);
+}
+
 int scale_hip_tstep_tracer_heve_noalloc(scale_hip_ctx* h,
     double* QTRCo, const double* QTRC, const double* QTRC0, const double* RHOQ_t,
     const double* DENS0, const double* DENS, const double* mflx_hi, const double* num_diff,

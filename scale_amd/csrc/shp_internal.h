@@ -185,6 +185,15 @@ hipError_t tracer_stage_tile(const Ctx& c, const TracerIO& io, double* rjpls, do
 // RCDZ/RCDX/RCDY) and work arrays (3 n3 doubles: phi_lo, R+, R-)
 hipError_t tracer_stage_ext(const Ctx& c, const TracerIO& io, const double* GSQRT, const double* MAPF_XY,
                             const double* RCDZ, const double* RCDX, const double* RCDY, double* work3, hipStream_t s);
+// adjoint of one tracer stage (tracer_stage's inputs in io) with the FCT
+// limiter frozen at its forward R+/R-: from QTRCo_b (read at the interior)
+// overwrites the adjoints of QTRC, QTRC0, RHOQ_t and num_diff (3 planes) on
+// the whole array.  Uses c.scr[0..7]; UD3KOREN1993 is rejected.
+struct TracerAdjIO {
+  const double* QTRCo_b;
+  double *QTRC_b, *QTRC0_b, *RHOQ_t_b, *num_diff_b;
+};
+hipError_t tracer_stage_adj(const Ctx& c, const TracerIO& io, const TracerAdjIO& adj, hipStream_t s);
 // ATMOS_DYN_numfilter_coef_q (ND_ORDER = 1) with the halo fill of its 3
 // fields; REF_qv non-null for QV.  Needs the filter tables and c.ndqc.
 hipError_t numfilter_q(const Ctx& c, double* num_diff_q, const double* DENS, const double* QTRC, const double* REF_qv,

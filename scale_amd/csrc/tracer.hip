@@ -648,4 +648,234 @@ hipError_t fct_momentum_limit(const Ctx& c, const MomFct& f, hipStream_t s) {
   return c.terrain ? afct_t<true>(c, a, f.along, wform, s) : afct_t<false>(c, a, f.along, wform, s);
 }
 
+
+// ---------------------------------------------------------------------------
+// Adjoint of one tracer stage with the FCT limiter frozen.
+//
+// With the mass fluxes and densities held fixed, the stage is linear in
+// QTRC, QTRC0, RHOQ_t and num_diff once the limiter's ratios R+/R- (and with
+// them the factor W of each anti-diffusive flux, limited = W (hi - lo)) keep
+// their forward values: the face flux of the update is (1 - W) hi + W lo
+// (COMMON.F90:2570-2659, tracer_fvm_heve.F90:197-205).  The adjoint runs the
+// forward's first two FCT passes for R+/R-, then
+//   tradj_faces   the adjoint of each face flux, split into its high-order
+//                 part (1 - W) F_b and low-order part W F_b, and the
+//                 adjoint of num_diff (GSQRT times the high-order part)
+//   tradj_gather  for each point, the sum over the faces whose stencil reads
+//                 it of (face adjoint) x vel x (the face form's coefficient
+//                 of that point), plus the direct terms of the update
+// The coefficient of a point in a face form is the form evaluated on the
+// unit vector at that point: the forms are linear in the values for a fixed
+// velocity (UD3KOREN1993's limiter is not, and is rejected).
+struct TrAdjArgs {
+  TrArgs f;            // the forward's arguments; f.rjpls/f.rjmns hold R+/R-
+  const double* Qo_b;  // adjoint of QTRCo, read at the interior
+  double *hb[3], *lb[3];                 // face adjoints (z, x, y): high- and low-order parts
+  double *Q_b, *Q0_b, *RQ_b, *ND_b;      // adjoints of QTRC, QTRC0, RHOQ_t, num_diff (3 planes)
+};
+
+// z face kk between levels kk and kk+1: the coefficient of level k in the
+// high-order face form (the branches of TrOps::hZ)
+template <int S>
+__device__ __forceinline__ double tradj_zcoef(const Dims& d, int kk, int k, double vel) {
+  auto e = [&](int m) { return m == k ? 1.0 : 0.0; };
+  if constexpr (S == UD5 || S == CD6) {
+    double v[5];
+    auto col = [&](int k0, int n) { for (int m = 0; m < n; ++m) v[m] = e(k0 + m); };
+    if (kk == d.KE - 2) { col(d.KE - 4, 5); return fv6_ke2<S>(v, vel); }
+    if (kk == d.KS + 1) { col(d.KS, 5); return fv6_ks1<S>(v, vel); }
+    if (kk == d.KE - 1) { col(d.KE - 2, 3); return fv6_ke1<S>(v, vel); }
+    if (kk == d.KS) { col(d.KS, 3); return fv6_ks<S>(v, vel); }
+    return fv6_int<S>(e(kk - 2), e(kk - 1), e(kk), e(kk + 1), e(kk + 2), e(kk + 3), vel);
+  } else {
+    if (kk == d.KS) return fv_lo<S>(e(kk), e(kk + 1), e(kk + 2), vel);
+    if (kk == d.KE - 1) return fv_hi<S>(e(kk - 1), e(kk), e(kk + 1), vel);
+    return fv_int<S>(e(kk - 1), e(kk), e(kk + 1), e(kk + 2), vel);
+  }
+}
+// horizontal face ii between points ii and ii+1: the coefficient of point i
+template <int S>
+__device__ __forceinline__ double tradj_hcoef(int ii, int i, double vel) {
+  auto e = [&](int m) { return m == i ? 1.0 : 0.0; };
+  if constexpr (S == UD1) return fv_int<UD1>(0.0, e(ii), e(ii + 1), 0.0, vel);
+  else if constexpr (S == UD5 || S == CD6) return fv6_int<S>(e(ii - 2), e(ii - 1), e(ii), e(ii + 1), e(ii + 2), e(ii + 3), vel);
+  else return fv_int<S>(e(ii - 1), e(ii), e(ii + 1), e(ii + 2), vel);
+}
+// the UD1 (low-order) face: coefficient of point i in face ii
+__device__ __forceinline__ double tradj_lcoef(int ii, int i, double vel) {
+  return fv_int<UD1>(0.0, ii == i ? 1.0 : 0.0, ii + 1 == i ? 1.0 : 0.0, 0.0, vel);
+}
+
+template <int S, bool TERRAIN, bool FCT>
+__global__ void __launch_bounds__(256) tradj_faces(TrAdjArgs b) {
+  const TrArgs& a = b.f;
+  const Dims& d = a.d;
+  TrOps<S, TERRAIN> o(a);
+  const int i = (int)(blockIdx.x * 64 + threadIdx.x);
+  const int k = (int)(blockIdx.y * 4 + threadIdx.y);
+  const int j = (int)blockIdx.z;
+  if (i >= d.IA || k >= d.KA || j >= d.JA) return;
+  const size_t p = IX(d, k, i, j), n3 = d.n3;
+  const double* RP = a.rjpls;
+  const double* RM = a.rjmns;
+  // dt m1 m2 / (GSQRT DENS) QTRCo_b at an interior point, zero elsewhere: the
+  // adjoint of the flux difference there is -+ this times RCDZ/RCDX/RCDY
+  auto A = [&](int kk, int ii, int jj) -> double {
+    if (kk < d.KS || kk > d.KE || ii < d.IS || ii > d.IE || jj < d.JS || jj > d.JE) return 0.0;
+    const size_t q = IX(d, kk, ii, jj);
+    return a.dtl * b.Qo_b[q] * o.m1(ii, jj) * o.m2(ii, jj) / (o.gs(I_XYZ, q) * a.DENS[q]);
+  };
+  // the frozen factor W of a face's limited anti-diffusive flux W (hi - lo)
+  auto W = [&](double an, size_t pl, size_t pu) -> double {
+    const double dirsw = 0.5 + fsign(0.5, an);
+    return 1.0 - fmin2(RP[pu], RM[pl]) * dirsw - fmin2(RP[pl], RM[pu]) * (1.0 - dirsw);
+  };
+  const bool in_i = i >= d.IS && i <= d.IE, in_j = j >= d.JS && j <= d.JE, in_k = k >= d.KS && k <= d.KE;
+  double h[3] = {0.0, 0.0, 0.0}, l[3] = {0.0, 0.0, 0.0};
+  const bool v[3] = {in_i && in_j && k >= d.KS && k <= d.KE - 1, in_k && in_j && i >= d.IS - 1 && i <= d.IE,
+                     in_k && in_i && j >= d.JS - 1 && j <= d.JE};
+  // z face k (KS..KE-1)
+  if (v[0]) {
+    const double fb = -A(k, i, j) * a.RCDZ[k] + A(k + 1, i, j) * a.RCDZ[k + 1];
+    double w = 0.0;
+    if constexpr (FCT) w = W(o.aZ(k, i, j), p, IX(d, k + 1, i, j));
+    h[0] = (1.0 - w) * fb; l[0] = w * fb;
+  }
+  // x face i (IS-1..IE)
+  if (v[1]) {
+    const double fb = -A(k, i, j) * a.RCDX[i] + A(k, i + 1, j) * a.RCDX[i + 1];
+    double w = 0.0;
+    if constexpr (FCT) w = W(o.aX(k, i, j), p, IX(d, k, i + 1, j));
+    h[1] = (1.0 - w) * fb; l[1] = w * fb;
+  }
+  // y face j (JS-1..JE)
+  if (v[2]) {
+    const double fb = -A(k, i, j) * a.RCDY[j] + A(k, i, j + 1) * a.RCDY[j + 1];
+    double w = 0.0;
+    if constexpr (FCT) w = W(o.aY(k, i, j), p, IX(d, k, i, j + 1));
+    h[2] = (1.0 - w) * fb; l[2] = w * fb;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    b.hb[c][p] = h[c];
+    if constexpr (FCT) b.lb[c][p] = l[c];
+  }
+  // num_diff enters the high-order fluxes as + GSQRT num_diff (not for UD1)
+  if constexpr (S == UD1) {
+    b.ND_b[p] = 0.0; b.ND_b[p + n3] = 0.0; b.ND_b[p + 2 * n3] = 0.0;
+  } else {
+    b.ND_b[p] = v[0] ? o.gs(I_XYW, p) * h[0] : 0.0;
+    b.ND_b[p + n3] = v[1] ? o.gs(I_UYZ, p) * h[1] : 0.0;
+    b.ND_b[p + 2 * n3] = v[2] ? o.gs(I_XVZ, p) * h[2] : 0.0;
+  }
+}
+
+template <int S, bool TERRAIN, bool FCT>
+__global__ void __launch_bounds__(256) tradj_gather(TrAdjArgs b) {
+  const TrArgs& a = b.f;
+  const Dims& d = a.d;
+  TrOps<S, TERRAIN> o(a);
+  const int i = (int)(blockIdx.x * 64 + threadIdx.x);
+  const int k = (int)(blockIdx.y * 4 + threadIdx.y);
+  const int j = (int)blockIdx.z;
+  if (i >= d.IA || k >= d.KA || j >= d.JA) return;
+  const size_t p = IX(d, k, i, j);
+  const bool in_i = i >= d.IS && i <= d.IE, in_j = j >= d.JS && j <= d.JE, in_k = k >= d.KS && k <= d.KE;
+  // the faces whose high-order form can read point m: m-3 .. m+2
+  constexpr int R0 = (S == UD5 || S == CD6) ? 3 : (S == UD1 ? 1 : 2);
+  constexpr int R1 = (S == UD5 || S == CD6) ? 2 : (S == UD1 ? 0 : 1);
+  double qb = 0.0, q0b = 0.0;
+  if (in_i && in_j) {
+    for (int kk = max(k - R0, d.KS); kk <= min(k + R1, d.KE - 1); ++kk) {
+      const size_t f = IX(d, kk, i, j);
+      const double vel = o.mZ(f);
+      qb += b.hb[0][f] * vel * tradj_zcoef<S>(d, kk, k, vel);
+      if constexpr (FCT) if (kk >= k - 1) q0b += b.lb[0][f] * vel * tradj_lcoef(kk, k, vel);
+    }
+  }
+  if (in_k && in_j) {
+    for (int ii = max(i - R0, d.IS - 1); ii <= min(i + R1, d.IE); ++ii) {
+      const size_t f = IX(d, k, ii, j);
+      const double vel = o.mX(f);
+      qb += b.hb[1][f] * vel * tradj_hcoef<S>(ii, i, vel);
+      if constexpr (FCT) if (ii >= i - 1) q0b += b.lb[1][f] * vel * tradj_lcoef(ii, i, vel);
+    }
+  }
+  if (in_k && in_i) {
+    for (int jj = max(j - R0, d.JS - 1); jj <= min(j + R1, d.JE); ++jj) {
+      const size_t f = IX(d, k, i, jj);
+      const double vel = o.mY(f);
+      qb += b.hb[2][f] * vel * tradj_hcoef<S>(jj, j, vel);
+      if constexpr (FCT) if (jj >= j - 1) q0b += b.lb[2][f] * vel * tradj_lcoef(jj, j, vel);
+    }
+  }
+  double rqb = 0.0;
+  if (in_i && in_j && in_k) {
+    const double ob = b.Qo_b[p] / a.DENS[p];
+    q0b += ob * a.DENS0[p];
+    rqb = a.dtl * ob;
+  }
+  b.Q_b[p] = qb;
+  b.Q0_b[p] = q0b;
+  b.RQ_b[p] = rqb;
+}
+
+template <int S, bool TERRAIN>
+static hipError_t tracer_adj_t(const Ctx& c, TrAdjArgs& b, bool fct, bool along, hipStream_t s) {
+  const Dims& d = c.d;
+  const TrArgs& a = b.f;
+  if (fct) {  // R+/R- of the forward (its first two passes and the halo fill)
+    const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
+    const unsigned ni = (imax + 63) / 64, nk = (kmax + 3) / 4;
+    const unsigned nb = ni * nk * (unsigned)jmax;
+    const unsigned ni2 = (imax + 2 + 63) / 64;
+    const unsigned nb2 = ni2 * nk * (unsigned)(jmax + 2);
+    hipLaunchKernelGGL((fct_philo<TERRAIN>), dim3(nb2), dim3(64, 4), 0, s, a, ni2, nk, nb2);
+    if (along) hipLaunchKernelGGL((fct_ratio<S, TERRAIN, true>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
+    else hipLaunchKernelGGL((fct_ratio<S, TERRAIN, false>), dim3(nb), dim3(64, 4), 0, s, a, ni, nk, nb);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    double* r[2] = {a.rjpls, a.rjmns};
+    e = halo_fill(c, r, 2, s);
+    if (e != hipSuccess) return e;
+  }
+  const dim3 g((d.IA + 63) / 64, (d.KA + 3) / 4, d.JA), blk(64, 4);
+  prof_start(c, "tracer_adj", s);
+  if (fct) {
+    hipLaunchKernelGGL((tradj_faces<S, TERRAIN, true>), g, blk, 0, s, b);
+    hipLaunchKernelGGL((tradj_gather<S, TERRAIN, true>), g, blk, 0, s, b);
+  } else {
+    hipLaunchKernelGGL((tradj_faces<S, TERRAIN, false>), g, blk, 0, s, b);
+    hipLaunchKernelGGL((tradj_gather<S, TERRAIN, false>), g, blk, 0, s, b);
+  }
+  prof_stop(c, s);
+  return hipGetLastError();
+}
+
+hipError_t tracer_stage_adj(const Ctx& c, const TracerIO& io, const TracerAdjIO& adj, hipStream_t s) {
+  if (c.scheme_tracer == UD3KOREN) return hipErrorInvalidValue;
+  TrAdjArgs b;
+  TrArgs& a = b.f;
+  a.d = c.d;
+  a.QTRC = io.QTRC; a.QTRC0 = io.QTRC0; a.RHOQ_t = nullptr; a.DENS0 = io.DENS0; a.DENS = io.DENS;
+  a.mflx = io.mflx_hi; a.nd = io.num_diff; a.nds = c.d.n3; a.GSQRT = c.GSQRT; a.MAPF_XY = c.MAPF;
+  a.RCDZ = c.g1.RCDZ; a.RCDX = c.g1.RCDX; a.RCDY = c.g1.RCDY;
+  a.dtl = io.dtl; a.QTRCo = nullptr;
+  // work: phi_lo, R+, R- in scr[0..2]; the face adjoints in scr[3..5] and
+  // (phi_lo being dead by then) scr[0], scr[6], scr[7]
+  a.phi_lo = c.scr[0]; a.rjpls = c.scr[1]; a.rjmns = c.scr[2];
+  b.hb[0] = c.scr[3]; b.hb[1] = c.scr[4]; b.hb[2] = c.scr[5];
+  b.lb[0] = c.scr[0]; b.lb[1] = c.scr[6]; b.lb[2] = c.scr[7];
+  b.Qo_b = adj.QTRCo_b; b.Q_b = adj.QTRC_b; b.Q0_b = adj.QTRC0_b; b.RQ_b = adj.RHOQ_t_b; b.ND_b = adj.num_diff_b;
+  const bool fct = io.flag_fct != 0, along = io.flag_along != 0;
+  switch (c.scheme_tracer) {
+    case UD3: return c.terrain ? tracer_adj_t<UD3, true>(c, b, fct, along, s) : tracer_adj_t<UD3, false>(c, b, fct, along, s);
+    case CD2: return c.terrain ? tracer_adj_t<CD2, true>(c, b, fct, along, s) : tracer_adj_t<CD2, false>(c, b, fct, along, s);
+    case UD1: return c.terrain ? tracer_adj_t<UD1, true>(c, b, fct, along, s) : tracer_adj_t<UD1, false>(c, b, fct, along, s);
+    case UD5: return c.terrain ? tracer_adj_t<UD5, true>(c, b, fct, along, s) : tracer_adj_t<UD5, false>(c, b, fct, along, s);
+    case CD6: return c.terrain ? tracer_adj_t<CD6, true>(c, b, fct, along, s) : tracer_adj_t<CD6, false>(c, b, fct, along, s);
+    default: return c.terrain ? tracer_adj_t<CD4, true>(c, b, fct, along, s) : tracer_adj_t<CD4, false>(c, b, fct, along, s);
+  }
+}
+
 }  // namespace shp

@@ -22,6 +22,8 @@ import os
 
 import numpy as np
 
+from . import inputs
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libscalehip.so")
 
@@ -187,6 +189,66 @@ class Context:
             self._h, _p(QTRCo), _p(QTRC), _p(QTRC0), _p(RHOQ_t), _p(DENS0), _p(DENS), _p(mflx_hi), _p(num_diff),
             _p(GSQRT), _p(MAPF), _p(CDZ), _p(RCDZ), _p(RCDX), _p(RCDY), _d(dtl), _l(FLAG_FCT_TRACER),
             _l(FLAG_FCT_ALONG_STREAM)))
+
+    def ATMOS_DYN_Tstep_tracer_fvm_heve_adj(self, QTRCo_b, QTRC, QTRC0, DENS0, DENS, mflx_hi, num_diff,
+                                            GSQRT, MAPF, CDZ, RCDZ, RCDX, RCDY, dtl, FLAG_FCT_TRACER,
+                                            FLAG_FCT_ALONG_STREAM) -> dict:
+        """Adjoint of one tracer stage with the FCT limiter frozen at its
+        forward R+/R- (scale_hip_tstep_tracer_heve_adj).  QTRCo_b is read at
+        the interior; returns the adjoints QTRC, QTRC0, RHOQ_t and num_diff
+        (KA,IA,JA,3) of the stage's inputs on the whole array."""
+        g = self.grid
+        z = lambda *e: np.zeros((g.KA, g.IA, g.JA) + e, order="F")
+        res = dict(QTRC=z(), QTRC0=z(), RHOQ_t=z(), num_diff=z(3))
+        _check(load_library().scale_hip_tstep_tracer_heve_adj(
+            self._h, _p(np.asfortranarray(QTRCo_b)), _p(res["QTRC"]), _p(res["QTRC0"]), _p(res["RHOQ_t"]),
+            _p(res["num_diff"]), _p(QTRC), _p(QTRC0), _p(DENS0), _p(DENS), _p(mflx_hi), _p(num_diff),
+            _p(GSQRT), _p(MAPF), _p(CDZ), _p(RCDZ), _p(RCDX), _p(RCDY), _d(dtl), _l(FLAG_FCT_TRACER),
+            _l(FLAG_FCT_ALONG_STREAM)))
+        return res
+
+    def ATMOS_DYN_Tinteg_tracer_rk3_adj(self, QTRC_b, QTRC, QTRC0, RHOQ_t, DENS0, DENS, mflx_hi, num_diff,
+                                        GSQRT, MAPF, CDZ, RCDZ, RCDX, RCDY, dtl, FLAG_FCT_TRACER,
+                                        FLAG_FCT_ALONG_STREAM) -> dict:
+        """Adjoint of the three-stage tracer integration
+        ATMOS_DYN_tinteg_tracer_rk3 (scale_atmos_dyn_tinteg_tracer_rk3.F90:
+        92-208) with the last stage's FCT limiter frozen: the stages of dtl/3
+        and dtl/2 on DENS0 + (DENS - DENS0)/3 and /2 (no FCT), each followed
+        by the periodic halo fill, and the stage of dtl on DENS.  The two
+        intermediate fields are recomputed with the forward stage (the last
+        stage's limiter reads the second); the three stage adjoints run in
+        reverse with the halo fills transposed between them.  QTRC_b is read
+        at the interior; returns the adjoints of QTRC, QTRC0, RHOQ_t and
+        num_diff as ATMOS_DYN_Tstep_tracer_fvm_heve_adj does."""
+        g = self.grid
+        sl = (slice(g.KS, g.KE + 1), slice(g.IS - 1, g.IE + 2), slice(g.JS - 1, g.JE + 2))
+        dens_rk = [np.array(DENS, order="F", copy=True), np.array(DENS, order="F", copy=True)]
+        dens_rk[0][sl] = DENS0[sl] + (DENS[sl] - DENS0[sl]) / 3.0  # TRK3:143-147
+        dens_rk[1][sl] = DENS0[sl] + (DENS[sl] - DENS0[sl]) * 0.5  # TRK3:171-175
+        met = (GSQRT, MAPF, CDZ, RCDZ, RCDX, RCDY)
+        fields = [np.asfortranarray(QTRC)]
+        for n, dt in enumerate((dtl / 3.0, dtl / 2.0)):
+            out = np.zeros((g.KA, g.IA, g.JA), order="F")
+            self.ATMOS_DYN_Tstep_tracer_fvm_heve(out, fields[-1], QTRC0, RHOQ_t, DENS0, dens_rk[n], mflx_hi, num_diff,
+                                                 *met, dt, False, FLAG_FCT_ALONG_STREAM)
+            inputs.halo_fill(g, out)
+            fields.append(out)
+        stages = [(fields[2], DENS, dtl, FLAG_FCT_TRACER), (fields[1], dens_rk[1], dtl / 2.0, False),
+                  (fields[0], dens_rk[0], dtl / 3.0, False)]
+        res = None
+        ob = np.asfortranarray(QTRC_b)
+        for n, (q, d, dt, fct) in enumerate(stages):
+            r = self.ATMOS_DYN_Tstep_tracer_fvm_heve_adj(ob, q, QTRC0, DENS0, d, mflx_hi, num_diff, *met, dt, fct,
+                                                         FLAG_FCT_ALONG_STREAM)
+            if res is None:
+                res = r
+            else:
+                for k in ("QTRC0", "RHOQ_t", "num_diff"):
+                    res[k] += r[k]
+                res["QTRC"] = r["QTRC"]
+            if n < 2:
+                ob = inputs.halo_fill_adjoint(g, r["QTRC"].copy())
+        return res
 
     def ATMOS_DYN_numfilter_coef(self, num_diff, DENS, MOMZ, MOMX, MOMY, RHOT, CDZ, CDX, CDY, FDZ, FDX, FDY, DT,
                                  REF_dens, REF_pott, ND_COEF, ND_ORDER, ND_SFC_FACT, ND_USE_RS):

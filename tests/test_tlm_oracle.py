@@ -8,7 +8,13 @@ dual.h) and the dot-product test of the oracle's adjoint against it.
 * <L dx, y> = <dx, L^T y> for 3 large steps, L from the TLM and L^T from the
   oracle adjoint (scale_oracle_adj.c), to 1e-12 relative, on several seeded
   directions and weights.
+* The tracer stage on dual numbers (R+/R- of the FCT limiter frozen) computes
+  the oracle's values bit for bit, and its tangent is the derivative of the
+  stage with R+/R- fixed at the oracle's own ratios (central differences,
+  1e-7), not the derivative of the free limiter.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -117,3 +123,97 @@ def test_dot_product_oracle_adjoint(scheme):
         rhs = sum(float(np.sum(grad[k] * d[k])) for k in STATE)
         print(f"scheme {scheme} trial {trial}: <L dx, y> = {lhs:.15e}, <dx, L^T y> = {rhs:.15e}")
         assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+
+
+# ---- tracer stage with the FCT limiter frozen ----------------------------------
+
+def tracer_inputs(seed=9, scheme=3):
+    """One tracer stage's inputs on the tophat tracer (the limiter active):
+    random face mass fluxes, a perturbed QTRC0, num_diff_q and RHOQ_t."""
+    case = inputs.make_case(kmax=10, imax=8, jmax=8, ntracer=2, tracer_kind=["blob", "tophat"], fct=True,
+                            scheme_tracer=scheme)
+    g = case.grid
+    rng = np.random.default_rng(seed)
+    mflx = np.asfortranarray(rng.standard_normal(g.shape + (3,)) * np.array([0.3, 8.0, 5.0]))
+    for n in range(3):
+        inputs.halo_fill(g, mflx[..., n])
+    q = np.asfortranarray(case.qtrc[..., 1])
+    q0 = np.asfortranarray(q * (1.0 + 0.01 * rng.standard_normal(g.shape)))
+    inputs.halo_fill(g, q0)
+    nd = np.asfortranarray(1e-5 * rng.standard_normal(g.shape + (3,)))
+    rq = np.asfortranarray(1e-6 * rng.standard_normal(g.shape))
+    dens = np.asfortranarray(case.state["DENS"] * (1.0 + 1e-3 * rng.standard_normal(g.shape)))
+    inputs.halo_fill(g, dens)
+    return case, dict(QTRC=q, QTRC0=q0, RHOQ_t=rq, DENS0=case.state["DENS"], DENS=dens, mflx_hi=mflx, num_diff_q=nd)
+
+
+def tracer_directions(case, rng, x):
+    g = case.grid
+    return {"QTRC": rng.standard_normal(g.shape) * np.abs(x["QTRC"]).max(),
+            "QTRC0": rng.standard_normal(g.shape) * np.abs(x["QTRC0"]).max(),
+            "RHOQ_t": rng.standard_normal(g.shape) * 1e-6,
+            "num_diff": rng.standard_normal(g.shape + (3,)) * 1e-5}
+
+
+@pytest.mark.parametrize("fct,along", [(True, True), (True, False), (False, True)], ids=["fct_along", "fct_7pt", "nofct"])
+def test_dual_tracer_stage_values_are_the_oracle(fct, along):
+    case, x = tracer_inputs()
+    ref = np.full(case.grid.shape, np.nan, order="F")
+    O.tstep_tracer(case, ref, x["QTRC"], x["QTRC0"], x["RHOQ_t"], x["DENS0"], x["DENS"], x["mflx_hi"],
+                   x["num_diff_q"], 2.0, fct, along)
+    d = tracer_directions(case, np.random.default_rng(4), x)
+    val, _ = T.tracer_stage_tangent(case, **x, dtl=2.0, fct=fct, along_stream=along, tangents=d)
+    assert np.array_equal(val, ref, equal_nan=True)
+
+
+@pytest.mark.parametrize("along", [True, False], ids=["along", "7pt"])
+def test_frozen_limiter_tangent(along):
+    """The tracer stage's tangent with R+/R- frozen is the derivative of the
+    stage with R+/R- fixed: the oracle's own ratios at the base point are
+    captured (orc_fct_save_r) and imposed (orc_fct_fixed_r) on central
+    differences of the plain oracle, which then agree with the tangent to
+    1e-7 at eps = 1e-7 (the sign switch of each anti-diffusive flux is the
+    only kink left).  Without the imposed ratios the differences miss the
+    frozen tangent by far more where the limiter is active, so the check
+    tells a frozen limiter from a differentiated one."""
+    case, x = tracer_inputs()
+    g = case.grid
+    lib = O.lib()
+    save = np.zeros(2 * int(np.prod(g.shape)))
+    sp = C.c_void_p.in_dll(lib, "orc_fct_save_r")
+    fp = C.c_void_p.in_dll(lib, "orc_fct_fixed_r")
+    dt = 2.0
+
+    def run(eps, dd):
+        out = np.full(g.shape, np.nan, order="F")
+        xs = dict(x)
+        xs["QTRC"] = np.asfortranarray(x["QTRC"] + eps * dd["QTRC"])
+        xs["QTRC0"] = np.asfortranarray(x["QTRC0"] + eps * dd["QTRC0"])
+        xs["RHOQ_t"] = np.asfortranarray(x["RHOQ_t"] + eps * dd["RHOQ_t"])
+        xs["num_diff_q"] = np.asfortranarray(x["num_diff_q"] + eps * dd["num_diff"])
+        O.tstep_tracer(case, out, xs["QTRC"], xs["QTRC0"], xs["RHOQ_t"], x["DENS0"], x["DENS"], x["mflx_hi"],
+                       xs["num_diff_q"], dt, True, along)
+        return out
+
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    dd = tracer_directions(case, np.random.default_rng(5), x)
+    _, tan = T.tracer_stage_tangent(case, **x, dtl=dt, fct=True, along_stream=along, tangents=dd)
+    eps = 1e-7
+    sp.value = save.ctypes.data
+    try:
+        run(0.0, dd)
+    finally:
+        sp.value = None
+    limited = np.nanmin(save) < 1.0
+    assert limited, "the limiter should be active on the tophat"
+    fp.value = save.ctypes.data
+    try:
+        fd = (run(eps, dd) - run(-eps, dd)) / (2 * eps)
+    finally:
+        fp.value = None
+    err = np.abs(fd[sl] - tan[sl]).max() / np.abs(tan[sl]).max()
+    free = (run(eps, dd) - run(-eps, dd)) / (2 * eps)
+    err_free = np.abs(free[sl] - tan[sl]).max() / np.abs(tan[sl]).max()
+    print(f"along={along}: frozen FD vs tangent {err:.2e}; unfrozen FD vs tangent {err_free:.2e}")
+    assert err < 1e-7, err
+    assert err_free > 1e-4, err_free
