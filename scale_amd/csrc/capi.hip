@@ -870,6 +870,7 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
     M->mfR[r] = A(3 * n3); M->tfR[r] = A(3 * n3);
   }
   M->ndq = A(3 * n3); M->DENS_RK = A(n3); M->QRK1 = A(n3); M->QRK2 = A(n3); M->zeros = A(n3);
+  if (QA > 0 && cfg->tinteg_tracer == SCALE_HIP_TRACER_RK3WS2002) M->DENS_RK2 = A(n3);
   if (cfg->mixed_precision) {
     M->mixed = 1;
     auto F = [&](size_t n) { return (float*)dalloc(c, (n + 1) / 2, &M->allocs); };

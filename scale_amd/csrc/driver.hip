@@ -413,14 +413,17 @@ __global__ void mflx_mean(Dims d, double* mflx, const double* av, double nstep) 
   mflx[p] = av[p] / nstep;
 }
 
-// DENS_RK for the tracer RK3 stages (tinteg_tracer_rk3.F90:140-172)
-__global__ void dens_rk(Dims d, double* out, const double* D0, const double* D, int third) {
+// DENS_RK of the first and second tracer RK3 stages (tinteg_tracer_rk3.F90:
+// 140-147, 168-175), both at once: the same for every tracer of the step
+__global__ void dens_rk(Dims d, double* third, double* half, const double* D0, const double* D) {
   const int i = d.IS - 1 + blockIdx.x * 64 + threadIdx.x;
   const int k = d.KS + blockIdx.y * 4 + threadIdx.y;
   const int j = d.JS - 1 + blockIdx.z;
   if (i > d.IE + 1 || k > d.KE || j > d.JE + 1) return;
   const size_t p = IX(d, k, i, j);
-  out[p] = third ? D0[p] + (D[p] - D0[p]) / 3.0 : D0[p] + (D[p] - D0[p]) * 0.5;
+  const double d0 = D0[p], d1 = D[p];
+  third[p] = d0 + (d1 - d0) / 3.0;
+  half[p] = d0 + (d1 - d0) * 0.5;
 }
 
 // FP64 budget: per-block partial sums of a*CDZ*CDX*CDY*GSQRT/(MAPF1*MAPF2), fixed order
@@ -767,6 +770,11 @@ static hipError_t large_tracers(Ctx& c, int nstep) {
   HIPCHK(hipGetLastError());
   double* mf[3] = {M.mflx_hi, M.mflx_hi + d.n3, M.mflx_hi + 2 * d.n3};
   HIPCHK(halo_fill(c, mf, 3, s));
+  if (M.tinteg_tracer != 1) {
+    hipLaunchKernelGGL(dens_rk, dim3((d.IE - d.IS + 3 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 3), dim3(64, 4), 0, s,
+                       d, M.DENS_RK, M.DENS_RK2, (const double*)M.DENS00, (const double*)M.DENS);
+    HIPCHK(hipGetLastError());
+  }
   for (int iq = 0; iq < M.QA; ++iq) {
     double* q = M.QTRC + d.n3 * (size_t)iq;
     const double* rq = M.RHOQ_tp ? M.RHOQ_tp + d.n3 * (size_t)iq : M.zeros;
@@ -785,15 +793,11 @@ static hipError_t large_tracers(Ctx& c, int nstep) {
       continue;
     }
     // stage 1: dt/3, no FCT
-    hipLaunchKernelGGL(dens_rk, dim3((d.IE - d.IS + 3 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 3), dim3(64, 4), 0, s,
-                       d, M.DENS_RK, M.DENS00, M.DENS, 1);
     io.QTRCo = M.QRK1; io.QTRC = q; io.QTRC0 = q; io.DENS = M.DENS_RK; io.dtl = dtl / 3.0; io.flag_fct = 0;
     HIPCHK(tracer_stage(c, io, s));
     HIPCHK(halo_fill(c, &M.QRK1, 1, s));
     // stage 2: dt/2, no FCT
-    hipLaunchKernelGGL(dens_rk, dim3((d.IE - d.IS + 3 + 63) / 64, (d.KE - d.KS + 1 + 3) / 4, d.JE - d.JS + 3), dim3(64, 4), 0, s,
-                       d, M.DENS_RK, M.DENS00, M.DENS, 0);
-    io.QTRCo = M.QRK2; io.QTRC = M.QRK1; io.QTRC0 = q; io.DENS = M.DENS_RK; io.dtl = dtl / 2.0; io.flag_fct = 0;
+    io.QTRCo = M.QRK2; io.QTRC = M.QRK1; io.QTRC0 = q; io.DENS = M.DENS_RK2; io.dtl = dtl / 2.0; io.flag_fct = 0;
     HIPCHK(tracer_stage(c, io, s));
     HIPCHK(halo_fill(c, &M.QRK2, 1, s));
     // stage 3: dt, FCT as configured; written to QRK1 then copied back (QTRC0 is read at neighbours)

@@ -26,6 +26,7 @@ struct Model {
   double* R[4][5] = {};
   double *mfR[4] = {}, *tfR[4] = {};
   double *ndq = nullptr, *DENS_RK = nullptr, *QRK1 = nullptr, *QRK2 = nullptr, *zeros = nullptr;
+  double* DENS_RK2 = nullptr;  // the second RK3 tracer stage's density (DENS_RK holds the first's)
   // mixed precision: FP32 state, stage, tendency, filter and flux arrays
   // (the FP64 arrays above keep the reference state, the pressure terms and
   // a copy of the state for the large-step preparation and the budgets)
