@@ -785,8 +785,10 @@ __device__ __forceinline__ R rdiv(R x, R y) {
 // ---------------------------------------------------------------------------
 // Flat-terrain stage with LDS tiles: blocks of 64 (i) x TJ (j) columns march
 // k together.  At each level the block
-//   (a) stages DENS/MOMX/MOMY (levels k and k+1, a two-level ring), MOMZ,
-//       POTT = RHOT/DENS and DPRES at level k in LDS, with a 2-cell halo;
+//   (a) stages DENS/MOMX/MOMY/MOMZ and POTT = RHOT/DENS in LDS with a 2-cell
+//       halo, in a three-level ring (k, k+1 and k+2: the faces read k and
+//       k+1, the own column's 4-level windows take k+2 from it), and DPRES
+//       at level k;
 //   (b) evaluates every x face and every y face of the tile once (mass and
 //       theta fluxes, the x/y fluxes of MOMZ, MOMX and MOMY) into LDS;
 //   (c) lets each column difference its faces, carry its z fluxes from the
@@ -803,8 +805,8 @@ __device__ __forceinline__ R rdiv(R x, R y) {
 template <int S, bool DIVDMP, int TJ, typename R, bool CMB>
 __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned lb, unsigned ni_tiles) {
   constexpr int TI = 64, CW = TI + 4, RW = TJ + 4, NT = TI * TJ;
-  __shared__ R sD[2][RW][CW], sMX[2][RW][CW], sMY[2][RW][CW];
-  __shared__ R sPO[RW][CW], sW[RW][CW];
+  __shared__ R sD[3][RW][CW], sMX[3][RW][CW], sMY[3][RW][CW];  // ring slot = level % 3
+  __shared__ R sPO[3][RW][CW], sW[3][RW][CW];
   __shared__ double sDP[TJ + 1][TI + 1];
   // x faces: position x = 0..TI is the face east of column i0-1+x
   __shared__ R xM[TJ][TI + 1], xT[TJ][TI + 1], xW[TJ][TI + 1], xU[TJ][TI + 1], xV[TJ][TI + 1];
@@ -862,17 +864,22 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
   auto devD = [&](double v, int k) -> R { if constexpr (DEV) return R(v - sOff[k]); else return R(v); };
   auto devT = [&](double v, int k) -> R { if constexpr (DEV) return R(v - sOff[KA_DEV_MAX + k]); else return R(v); };
 
-  // (a) tile staging helpers: tile cell (r, c) is column i0-2+c, row j0-2+r
-  auto fill = [&](R (*dst)[CW], const R* src, int k, bool dens) {
+  // (a) the first two levels of the ring; tile cell (r, c) is column i0-2+c,
+  // row j0-2+r
+  for (int lv = KS; lv <= KS + 1; ++lv) {
+    const int sl = lv % 3;
     for (int q = tid; q < RW * CW; q += NT) {
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      const R v = (ii < d.IA && jj < d.JA) ? src[gix(k, ii, jj)] : R(0);
-      dst[r][c] = (dens && ii < d.IA && jj < d.JA) ? fullD(v, k) : v;
+      const bool in = ii < d.IA && jj < d.JA;
+      const size_t g = in ? gix(lv, ii, jj) : 0;
+      const R vD = in ? a.DENS[g] : R(0), vX = in ? a.MOMX[g] : R(0), vY = in ? a.MOMY[g] : R(0);
+      const R vW = in ? a.MOMZ[g] : R(0), vR = in ? a.RHOT[g] : R(0);
+      sD[sl][r][c] = in ? fullD(vD, lv) : vD;
+      sMX[sl][r][c] = vX; sMY[sl][r][c] = vY; sW[sl][r][c] = vW;
+      sPO[sl][r][c] = in ? rdiv(fullT(vR, lv), fullD(vD, lv)) : R(0);
     }
-  };
-  int s0 = 0;  // ring slot of level k; slot s0^1 holds level k+1
-  fill(sD[0], a.DENS, KS, true); fill(sMX[0], a.MOMX, KS, false); fill(sMY[0], a.MOMY, KS, false);
+  }
   __syncthreads();
 
   // own-column windows at k-1..k+2 and carried z-face fluxes
@@ -958,19 +965,19 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       const int q = tid + h * NT;
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
-      const bool ok = q < RW * CW && ii < d.IA && jj < d.JA;
-      const size_t g = ok ? gix(k, ii, jj) : 0;
+      const bool ok = q < RW * CW && ii < d.IA && jj < d.JA && k + 2 < d.KA;
+      const size_t g = ok ? gix(k + 2, ii, jj) : 0;  // the ring's new level
       if constexpr (PIPE) {
-        const R v0 = a.DENS[g + d.pl], v1 = a.MOMX[g + d.pl], v2 = a.MOMY[g + d.pl], v3 = a.MOMZ[g], v4 = a.RHOT[g];
+        const R v0 = a.DENS[g], v1 = a.MOMX[g], v2 = a.MOMY[g], v3 = a.MOMZ[g], v4 = a.RHOT[g];
         tD[h] = ok ? v0 : R(0);
         tX[h] = ok ? v1 : R(0);
         tY[h] = ok ? v2 : R(0);
         tW[h] = ok ? v3 : R(0);
         tR[h] = ok ? v4 : R(0);
       } else {
-        tD[h] = ok ? a.DENS[g + d.pl] : R(0);
-        tX[h] = ok ? a.MOMX[g + d.pl] : R(0);
-        tY[h] = ok ? a.MOMY[g + d.pl] : R(0);
+        tD[h] = ok ? a.DENS[g] : R(0);
+        tX[h] = ok ? a.MOMX[g] : R(0);
+        tY[h] = ok ? a.MOMY[g] : R(0);
         tW[h] = ok ? a.MOMZ[g] : R(0);
         tR[h] = ok ? a.RHOT[g] : R(0);
       }
@@ -1000,7 +1007,7 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
   if constexpr (PIPE) issue(KS);
 
   for (int k = KS; k <= KE; ++k) {
-    const int s1 = s0 ^ 1;
+    const int s0 = k % 3, s1 = (k + 1) % 3, s2 = (k + 2) % 3;  // ring slots of levels k, k+1, k+2
     if constexpr (!PIPE) issue(k);
     // ---- (a) staging ----
 #pragma unroll
@@ -1024,9 +1031,9 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       const int r = q / CW, c = q - r * CW;
       const int ii = i0 - 2 + c, jj = j0 - 2 + r;
       const bool in = ii < d.IA && jj < d.JA;
-      sD[s1][r][c] = in ? fullD(tD[h], k + 1) : tD[h];
-      sMX[s1][r][c] = tX[h]; sMY[s1][r][c] = tY[h]; sW[r][c] = tW[h];
-      sPO[r][c] = in ? rdiv(fullT(tR[h], k), sD[s0][r][c]) : R(0);
+      sD[s2][r][c] = in ? fullD(tD[h], k + 2) : tD[h];
+      sMX[s2][r][c] = tX[h]; sMY[s2][r][c] = tY[h]; sW[s2][r][c] = tW[h];
+      sPO[s2][r][c] = in ? rdiv(fullT(tR[h], k + 2), fullD(tD[h], k + 2)) : R(0);
     }
 #pragma unroll
     for (int h = 0; h < NP; ++h) {
@@ -1045,7 +1052,7 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
     R o_nZD = 0, o_nZT = 0, o_nZW = 0, o_nZX = 0, o_nZY = 0;
     R o_Dt = 0, o_Rt = 0, o_Wt = 0, o_Xt = 0, o_Yt = 0, o_X0 = 0, o_Y0 = 0, o_Z0 = 0;
     double o_RT = 0.0, o_RD = 0.0;
-    R o_R2 = 1, o_D2 = 1, o_W2 = 0, o_X2 = 0, o_Y2 = 0, o_D = 1;
+    R o_D = 1;
     const double wdc = zWD(k);
     R o_nDw = 0;  // mflx_hi(Z) of the i = IS-1 column
     if (own || PIPE) {
@@ -1059,8 +1066,6 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       } else {
         if (wdc != 0.0) o_Z0 = a.MOMZ0[gk];
       }
-      const size_t g2 = gk + 2 * d.pl;
-      o_R2 = a.RHOT[g2]; o_D2 = a.DENS[g2]; o_W2 = a.MOMZ[g2]; o_X2 = a.MOMX[g2]; o_Y2 = a.MOMY[g2];
       if constexpr (DEV) o_D = a.DENS[gk];
       if constexpr (PIPE) {
         const R w = nd(I_DENS, ZDIR, gk - 1);
@@ -1072,7 +1077,8 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
     if constexpr (PIPE) {
       // the next level's loads go out after the own loads (the wait for those
       // then leaves the prefetch in flight), at KE too (level KE+1 exists and
-      // its values go unused): no scheduling across this point
+      // its values go unused; the ring's level KE+3 does not and reads element
+      // 0 instead): no scheduling across this point
       __builtin_amdgcn_sched_barrier(0);
       issue(k + 1);
       __builtin_amdgcn_sched_barrier(0);
@@ -1092,12 +1098,12 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       const size_t g = gix(k, ii, jj);
       const R vm = sMX[s0][r][c] + ndq[0];
       xM[row][x] = vm;
-      xT[row][x] = vm * fv_int<S>(sPO[r][c - 1], sPO[r][c], sPO[r][c + 1], sPO[r][c + 2], vm) + ndq[1];
+      xT[row][x] = vm * fv_int<S>(sPO[s0][r][c - 1], sPO[s0][r][c], sPO[s0][r][c + 1], sPO[s0][r][c + 2], vm) + ndq[1];
       R qw = 0;
       if (k < KE) {
         const R vel = rdiv((f1 * sMX[s1][r][c] + f2 * sMX[s0][r][c]),
                            (f1 * R(0.5) * (D1(r, c) + D1(r, c + 1)) + f2 * R(0.5) * (D0(r, c) + D0(r, c + 1))));
-        qw = vel * fv_int<S>(sW[r][c - 1], sW[r][c], sW[r][c + 1], sW[r][c + 2], vel) + ndq[2];
+        qw = vel * fv_int<S>(sW[s0][r][c - 1], sW[s0][r][c], sW[s0][r][c + 1], sW[s0][r][c + 2], vel) + ndq[2];
       }
       xW[row][x] = qw;
       {  // fluxX_UYZ stored at ii (loop index ii+1)
@@ -1130,12 +1136,12 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       const size_t g = gix(k, ii, jj);
       const R vm = sMY[s0][r][c] + ndq[0];
       yM[y][col] = vm;
-      yT[y][col] = vm * fv_int<S>(sPO[r - 1][c], sPO[r][c], sPO[r + 1][c], sPO[r + 2][c], vm) + ndq[1];
+      yT[y][col] = vm * fv_int<S>(sPO[s0][r - 1][c], sPO[s0][r][c], sPO[s0][r + 1][c], sPO[s0][r + 2][c], vm) + ndq[1];
       R qw = 0;
       if (k < KE) {
         const R vel = rdiv((f1 * sMY[s1][r][c] + f2 * sMY[s0][r][c]),
                            (f1 * R(0.5) * (D1(r, c) + D1(r + 1, c)) + f2 * R(0.5) * (D0(r, c) + D0(r + 1, c))));
-        qw = vel * fv_int<S>(sW[r - 1][c], sW[r][c], sW[r + 1][c], sW[r + 2][c], vel) + ndq[2];
+        qw = vel * fv_int<S>(sW[s0][r - 1][c], sW[s0][r][c], sW[s0][r + 1][c], sW[s0][r + 2][c], vel) + ndq[2];
       }
       yW[y][col] = qw;
       {  // fluxY_UYZ at (ii, jj)
@@ -1161,12 +1167,13 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
     }
     __syncthreads();
 
-    // own-column windows move up one level
+    // own-column windows move up one level, level k+2 from the ring
     if (own) {
-      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = rdiv(fullT(o_R2, k + 2), fullD(o_D2, k + 2));
-      ww[0] = ww[1]; ww[1] = ww[2]; ww[2] = ww[3]; ww[3] = o_W2;
-      xw[0] = xw[1]; xw[1] = xw[2]; xw[2] = xw[3]; xw[3] = o_X2;
-      yw[0] = yw[1]; yw[1] = yw[2]; yw[2] = yw[3]; yw[3] = o_Y2;
+      const int r = ty + 2, c = tx + 2;
+      pw[0] = pw[1]; pw[1] = pw[2]; pw[2] = pw[3]; pw[3] = sPO[s2][r][c];
+      ww[0] = ww[1]; ww[1] = ww[2]; ww[2] = ww[3]; ww[3] = sW[s2][r][c];
+      xw[0] = xw[1]; xw[1] = xw[2]; xw[2] = xw[3]; xw[3] = sMX[s2][r][c];
+      yw[0] = yw[1]; yw[1] = yw[2]; yw[2] = yw[3]; yw[3] = sMY[s2][r][c];
     }
 
     // ---- (c) the column ----
@@ -1214,7 +1221,7 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       {
         R qu = 0;
         if (k < KE) {
-          const R vel = rdiv((R(0.5) * (ww[1] + sW[r][c + 1])),
+          const R vel = rdiv((R(0.5) * (ww[1] + sW[s0][r][c + 1])),
                         (f1 * R(0.5) * (D1(r, c) + D1(r, c + 1)) + f2 * R(0.5) * (D0(r, c) + D0(r, c + 1))));
           R fv;
           if (k == KS) fv = fv_lo<S>(xw[1], xw[2], xw[3], vel);
@@ -1240,7 +1247,7 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       {
         R qv = 0;
         if (k < KE) {
-          const R vel = rdiv((R(0.5) * (ww[1] + sW[r + 1][c])),
+          const R vel = rdiv((R(0.5) * (ww[1] + sW[s0][r + 1][c])),
                         (f1 * R(0.5) * (D1(r, c) + D1(r + 1, c)) + f2 * R(0.5) * (D0(r, c) + D0(r + 1, c))));
           R fv;
           if (k == KS) fv = fv_lo<S>(yw[1], yw[2], yw[3], vel);
@@ -1326,7 +1333,6 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       rt0 = rt1; dp0 = dp1; st0 = St1; sr0 = Sr1; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = (double)Sw;
     }
     __syncthreads();
-    s0 = s1;
   }
   if (!own) return;
 
