@@ -1321,7 +1321,9 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       if constexpr (sizeof(R) == sizeof(double)) {
         if (a.trSt) a.trSt[g] = St;
       }
-      if (k <= KE - 1) { a.mflx_hi[g] = mZ; a.tflx_hi[g] = tZ; a.PT[g] = (double)PT; }
+      // the explicit z mass flux is num_diff itself: the back pass reads it
+      // there again instead of a copy stored here
+      if (k <= KE - 1) { a.tflx_hi[g] = tZ; a.PT[g] = (double)PT; }
       else { a.mflx_hi[g] = R(0); a.tflx_hi[g] = R(0); }
       if (k == KS) { a.mflx_hi[g - d.pl] = R(0); a.tflx_hi[g - d.pl] = R(0); }
       if (i == d.IS) {  // mflx_hi(Z) on the i = IS-1 column (HEVI.F90:370)
@@ -1391,7 +1393,7 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
   {
     const size_t q = px(KE - 1);
     const R z0 = a.MOMZ0[q];
-    put_zflux(q, R((double)mfZ[q] + J33G * Cn), R((double)tfZ[q] + J33G * Cn * ptn));
+    put_zflux(q, R((double)nd(I_DENS, ZDIR, q) + J33G * Cn), R((double)tfZ[q] + J33G * Cn * ptn));
     put_state(1, q, R((double)z0 + (Cn - (double)a.MOMZ[q])), z0, KE - 1);
     if constexpr (sizeof(R) == sizeof(double)) {
       if (a.trCp) a.trCp[q] = Cn;
@@ -1402,7 +1404,7 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
     const double Ck = a.e[q] * Cn + a.f[q];
     const double ptk = a.PT[q];
     const R z0 = a.MOMZ0[q];
-    put_zflux(q, R((double)mfZ[q] + J33G * Ck), R((double)tfZ[q] + J33G * Ck * ptk));
+    put_zflux(q, R((double)nd(I_DENS, ZDIR, q) + J33G * Ck), R((double)tfZ[q] + J33G * Ck * ptk));
     put_state(1, q, R((double)z0 + (Ck - (double)a.MOMZ[q])), z0, k);
     if constexpr (sizeof(R) == sizeof(double)) {
       if (a.trCp) a.trCp[q] = Ck;
