@@ -308,14 +308,21 @@ __device__ __forceinline__ void numfilter_tile_body(const NfArgsT<R>& a, unsigne
   }
 }
 
+// FP32 with 8 rows: held to 128 VGPRs, four waves per SIMD (at 134 VGPRs,
+// three, the 512x512x128 filter takes 1.50 ms instead of 1.06)
 template <int TJ, typename R>
-__global__ void __launch_bounds__(64 * TJ) numfilter_tile(NfArgsT<R> a, unsigned ni_tiles, unsigned nblocks) {
+constexpr int nf_min_waves() { return (TJ == 8 && sizeof(R) == sizeof(float)) ? 4 : 1; }
+
+template <int TJ, typename R>
+__global__ void __launch_bounds__(64 * TJ) __attribute__((amdgpu_waves_per_eu(nf_min_waves<TJ, R>())))
+numfilter_tile(NfArgsT<R> a, unsigned ni_tiles, unsigned nblocks) {
   numfilter_tile_body<TJ, R>(a, xcd_block(blockIdx.x, nblocks), ni_tiles);
 }
 
 // the filters of many members in one launch (tables as hevi_tile_batch)
 template <int TJ, typename R>
-__global__ void __launch_bounds__(64 * TJ) numfilter_tile_batch(const NfArgsT<R>* __restrict__ tab,
+__global__ void __launch_bounds__(64 * TJ) __attribute__((amdgpu_waves_per_eu(nf_min_waves<TJ, R>())))
+numfilter_tile_batch(const NfArgsT<R>* __restrict__ tab,
                                                               const unsigned* __restrict__ first,
                                                               const unsigned* __restrict__ ni, int nmem, unsigned total) {
   for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
@@ -344,9 +351,9 @@ static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], c
   const Dims& d = c.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
   // tile height: 8 rows when that gives every CU a block, 4 otherwise (at
-  // 512x512x128 the filter takes 1.94 ms at 8 rows, 1.95 at 4 and 4.24 at 16
+  // 512x512x128 the filter takes 1.95 ms at 8 rows, 1.96 at 4 and 4.04 at 16
   // in FP64, whose 1024-thread blocks are held to 128 VGPRs and spill; FP32
-  // 1.06, 1.80 and 1.09 ms); the context's knob (scale_hip_tune_set), else
+  // 1.11, 1.79 and 1.18 ms); the context's knob (scale_hip_tune_set), else
   // SCALE_HIP_NF_TJ = 4, 8 or 16 in the environment, overrides
   static const int ncu = [] {
     int dev = 0, n = 0;
