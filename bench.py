@@ -224,14 +224,57 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
         fwd_host_ms = (time.perf_counter() - t0) * 1e3
         _, fwd_ms = ctx.profile_query("model_large_step")
         ctx.profile(False)
+        def adjoint_run(c, tape):
+            """checkpointed forward + reverse sweep with `tape` taped segments
+            (0: as many as half the free HBM holds); (fwd_ckpt_ms, rev_ms, xb0, J)"""
+            c.tune_set(hip.TUNE_ADJ_TAPE, tape)
+            c.profile(2)
+            c.model_forward_checkpointed(nsteps, every)
+            J, xbn = objective({k: c.model_get(k) for k in names})
+            xb = c.model_adjoint(xbn)
+            _, f_ms = c.profile_query("model_forward_checkpointed")
+            _, r_ms = c.profile_query("model_adjoint")
+            c.profile(False)
+            return f_ms, r_ms, xb, J
+
+        # the default: the forward keeps every segment's tape in HBM (they fit:
+        # 400 B per point and step), so the reverse sweep recomputes nothing
         reset()
-        ctx.profile(2)
-        ctx.model_forward_checkpointed(nsteps, every)
-        J0, xbN = objective({k: ctx.model_get(k) for k in names})
-        xb0 = ctx.model_adjoint(xbN)
-        _, fck_ms = ctx.profile_query("model_forward_checkpointed")
-        _, rev_ms = ctx.profile_query("model_adjoint")
-        ctx.profile(False)
+        fck_ms, rev_ms, xb0, J0 = adjoint_run(ctx, 0)
+        # the classic checkpoint schedule: only the last segment taped, every
+        # other segment recomputed once from its checkpoint
+        reset()
+        fck1_ms, rev1_ms, _, _ = adjoint_run(ctx, 1)
+        ctx.tune_set(hip.TUNE_ADJ_TAPE, 0)
+        progress(f"adjoint: forward {fwd_ms:.1f} ms, all segments taped {fck_ms + rev_ms:.1f} ms, "
+                 f"last segment taped {fck1_ms + rev1_ms:.1f} ms")
+        # like for like without tracers: the same dynamics state in a model
+        # with no tracers (the reverse sweep has no tracer work either way)
+        dyn = {}
+        case0 = inputs.make_case(kmax=64, imax=128, jmax=128, dx=500.0, dy=500.0, dz=250.0, seed=seed, u0=10.0,
+                                 noise=0.01, ntracer=0, dt=1.0, fct=True)
+        ctx0 = hip.model_from_case(case0)
+        try:
+            def reset0():
+                for k in names:
+                    ctx0.model_set(k, x0[k])
+            reset0()
+            ctx0.model_large_step(nsteps)  # warm-up
+            reset0()
+            ctx0.synchronize()
+            ctx0.profile(2)
+            ctx0.model_large_step(nsteps)
+            _, fwd0_ms = ctx0.profile_query("model_large_step")
+            ctx0.profile(False)
+            for tape, key in ((0, "all_segments_taped"), (1, "last_segment_taped")):
+                reset0()
+                f0, r0, _, _ = adjoint_run(ctx0, tape)
+                dyn[key] = {"forward_checkpointed_ms": f0, "reverse_ms": r0, "adj_over_fwd": (f0 + r0) / fwd0_ms}
+            dyn["fwd_ms"] = fwd0_ms
+        finally:
+            ctx0.close()
+        progress(f"adjoint, dynamics only: forward {fwd0_ms:.1f} ms, adj/fwd {dyn['all_segments_taped']['adj_over_fwd']:.2f} "
+                 f"(all taped), {dyn['last_segment_taped']['adj_over_fwd']:.2f} (last segment taped)")
         # gradient w.r.t. the interior: fold the halo and ghost-level adjoints
         grad = inputs.interior_gradient(g, xb0)
         rng = np.random.default_rng(seed)
@@ -278,8 +321,12 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
         ctx.close()
     return dict(workload="warm bubble 128x128x64, DX=500 m, DZ=250 m, u=10 m/s, dt=1 s, 3 FCT tracers, FP64, UD3",
                 steps=nsteps, checkpoint_every=every, J=J0, fwd_ms=fwd_ms, fwd_host_ms=fwd_host_ms,
+                tapes="every segment's tape kept in HBM by the checkpointed forward (the default where they fit)",
                 forward_checkpointed_ms=fck_ms, reverse_ms=rev_ms, adj_ms=fck_ms + rev_ms,
                 adj_over_fwd=(fck_ms + rev_ms) / fwd_ms, reverse_over_fwd=rev_ms / fwd_ms,
+                last_segment_taped={"forward_checkpointed_ms": fck1_ms, "reverse_ms": rev1_ms,
+                                    "adj_over_fwd": (fck1_ms + rev1_ms) / fwd_ms},
+                dynamics_only=dyn,
                 fd=fd_summary(ads, fd2, fd4, eps_list) if ndirs > 0 else None,
                 fd_eps_sweep=sweep if ndirs > 0 else None)
 

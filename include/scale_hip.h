@@ -342,21 +342,31 @@ int scale_hip_tstep_short_hevi_adj(scale_hip_ctx *ctx,
  * `every` steps in HBM.  The adjoint takes the adjoint of the five
  * prognostics after that run (DENS_b .. RHOT_b, whole arrays) and returns in
  * place their adjoint at its start, recomputing each segment of `every` steps
- * from its checkpoint.  The model state is left at the end of the run. */
+ * that the forward kept no tape of (SCALE_HIP_TUNE_ADJ_TAPE below) from its
+ * checkpoint.  The model state is left at the end of the run. */
 int scale_hip_model_forward_checkpointed(scale_hip_ctx *ctx, int32_t n, int32_t every);
 int scale_hip_model_adjoint(scale_hip_ctx *ctx, double *DENS_b, double *MOMZ_b, double *MOMX_b, double *MOMY_b, double *RHOT_b);
 
-/* Launch-shape knobs, with no counterpart in the reference (whose loops take
- * IBLOCK/JBLOCK from PARAM_ATMOS_DYN; here the tiles are chosen from the grid
- * and the CU count unless a knob is set).  Results do not depend on them:
- * every tile shape computes the same operations in the same order.
+/* Launch-shape and memory knobs, with no counterpart in the reference (whose
+ * loops take IBLOCK/JBLOCK from PARAM_ATMOS_DYN; here the tiles are chosen
+ * from the grid and the CU count unless a knob is set).  Results do not
+ * depend on them: every tile shape computes the same operations in the same
+ * order.
  *   SCALE_HIP_TUNE_NF_TJ: rows of the numerical-filter tile, 4, 8 or 16
  *   (0 = chosen from the grid).
  *   SCALE_HIP_TUNE_TRACER_TILE: the FCT tracer stage on LDS tiles (flat
  *   terrain; UD3, CD4, UD3KOREN1993, UD1 and CD2): 1 (default) where the grid
  *   gives every CU two tiles, 2 always, 0 never (per-point kernels).
+ *   SCALE_HIP_TUNE_ADJ_TAPE: the number of trailing segments (of `every`
+ *   steps) whose tapes (the step states and the 45 fields per step the
+ *   reverse step reads, 400 B per grid point and step) the checkpointed
+ *   forward keeps in device memory, so that the adjoint reverses them without
+ *   a recording forward: n >= 1 at most n, 0 (default) as many as half the
+ *   device memory free holds when the forward first runs with its n and
+ *   every.  The segments
+ *   without a tape are recomputed from their checkpoints as before.
  * Unknown knobs and values outside the listed ones are rejected. */
-enum { SCALE_HIP_TUNE_NF_TJ = 1, SCALE_HIP_TUNE_TRACER_TILE = 2 };
+enum { SCALE_HIP_TUNE_NF_TJ = 1, SCALE_HIP_TUNE_TRACER_TILE = 2, SCALE_HIP_TUNE_ADJ_TAPE = 3 };
 int scale_hip_tune_set(scale_hip_ctx *ctx, int32_t knob, int32_t value);
 int scale_hip_tune_get(scale_hip_ctx *ctx, int32_t knob, int32_t *value);
 

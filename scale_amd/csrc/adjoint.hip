@@ -128,15 +128,19 @@ enum {
 };
 
 struct AdjWork {
-  int nsteps = 0, every = 0, recorded = 0;
-  // the checkpointed forward ran its last segment with records: the reverse
-  // sweep starts on it without a recording forward
-  bool last_recorded = false;
-  // one record per step of a segment when the device has room for them
-  // (45 fields per step); otherwise each reversed step recomputes its own
-  std::vector<StepRec> recs;
+  int nsteps = 0, every = 0, recorded = 0, tape_limit = -1;
+  // the checkpointed forward ran its last `taped` segments with records: the
+  // reverse sweep takes them without a recording forward
+  int taped = 0;
+  // tapes: one record per step (45 fields) and the step's state (5), for the
+  // last `tseg` segments of the run, as many as the tape budget holds (at
+  // least one segment, the work area of the recording forward of the others);
+  // with no room for one segment's records each reversed step recomputes its
+  // own instead
+  int tseg = 0;
+  std::vector<StepRec> recs;  // tseg x every records, segment by segment
   std::vector<double*> ckpt;  // states at steps 0, every, 2*every, ... (5 arrays each)
-  std::vector<double*> seg;   // states of the segment being reversed (every x 5)
+  std::vector<double*> seg;   // states of the taped steps (max(tseg, 1) x every x 5)
   double *xb[5] = {}, *S0b[5] = {}, *Rb[4][5] = {};
   double *NDb = nullptr, *Pb[3] = {};
   double *w[NW] = {};
@@ -195,38 +199,61 @@ hipError_t adj_setup(Ctx& c, int nsteps, int every) {
     if (!ok) { adj_free(c); return hipErrorOutOfMemory; }
   }
   AdjWork& A = *c.adj;
-  if (nsteps > 0 && (A.nsteps != nsteps || A.every != every)) {
+  if (nsteps > 0 && (A.nsteps != nsteps || A.every != every || A.tape_limit != c.adj_tape)) {
     const size_t n3 = c.d.n3;
     const int nck = (nsteps + every - 1) / every;
     // drop the storage of a previous run (kept in allocs until adj_free) and size anew
-    A.ckpt.clear(); A.seg.clear();
+    A.ckpt.clear(); A.seg.clear(); A.recs.clear(); A.tseg = 0;
     for (int q = 0; q < 5 * nck; ++q) {
       double* p = aalloc(A, n3);
       if (!p) return hipErrorOutOfMemory;
       A.ckpt.push_back(p);
     }
-    for (int q = 0; q < 5 * every; ++q) {
-      double* p = aalloc(A, n3);
-      if (!p) return hipErrorOutOfMemory;
-      A.seg.push_back(p);
-    }
-    A.recs.clear();
-    std::vector<StepRec> recs(every);
-    // SCALE_HIP_ADJ_RECORDS=0 keeps the per-step recomputation
+    // the tape budget: c.adj_tape segments, or (0) as many as half the free
+    // device memory holds; SCALE_HIP_ADJ_RECORDS=0 keeps no records at all
+    // (each reversed step recomputes its own)
     const char* env = getenv("SCALE_HIP_ADJ_RECORDS");
-    bool room = !env || atoi(env) != 0;
-    for (int e = 0; e < every && room; ++e) {
-      StepRec& r = recs[e];
-      for (int q = 0; q < 3 && room; ++q)
-        for (int v = 0; v < 5 && room; ++v) room = (r.R[q][v] = aalloc(A, n3)) != nullptr;
-      room = room && (r.nd = aalloc(A, 15 * n3)) && (r.RT2P = aalloc(A, n3)) && (r.DPRES0 = aalloc(A, n3)) &&
-             (r.REF_rhot = aalloc(A, n3));
-      for (int q = 0; q < 4 && room; ++q)
-        for (int t = 0; t < 3 && room; ++t) room = (r.tr[q][t] = aalloc(A, n3)) != nullptr;
+    const bool records = !env || atoi(env) != 0;
+    const size_t per_seg = (size_t)every * 50 * n3 * sizeof(double);
+    int want = c.adj_tape > 0 ? std::min(c.adj_tape, nck) : nck;
+    if (c.adj_tape == 0) {
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) == hipSuccess) want = (int)std::max<size_t>(1, std::min<size_t>(nck, fr / 2 / per_seg));
     }
-    (void)hipGetLastError();  // a failed allocation only means: no records
-    if (room) A.recs = recs;
-    A.nsteps = nsteps; A.every = every;
+    if (!records) want = 0;
+    for (int sg = 0; sg < want; ++sg) {
+      std::vector<double*> sts;
+      std::vector<StepRec> recs(every);
+      bool room = true;
+      for (int q = 0; q < 5 * every && room; ++q) {
+        double* p = aalloc(A, n3);
+        room = p != nullptr;
+        if (room) sts.push_back(p);
+      }
+      for (int e = 0; e < every && room; ++e) {
+        StepRec& r = recs[e];
+        for (int q = 0; q < 3 && room; ++q)
+          for (int v = 0; v < 5 && room; ++v) room = (r.R[q][v] = aalloc(A, n3)) != nullptr;
+        room = room && (r.nd = aalloc(A, 15 * n3)) && (r.RT2P = aalloc(A, n3)) && (r.DPRES0 = aalloc(A, n3)) &&
+               (r.REF_rhot = aalloc(A, n3));
+        for (int q = 0; q < 4 && room; ++q)
+          for (int t = 0; t < 3 && room; ++t) room = (r.tr[q][t] = aalloc(A, n3)) != nullptr;
+      }
+      if (!room) break;  // a failed allocation only means: no more tapes
+      A.seg.insert(A.seg.end(), sts.begin(), sts.end());
+      A.recs.insert(A.recs.end(), recs.begin(), recs.end());
+      ++A.tseg;
+    }
+    (void)hipGetLastError();
+    if (A.tseg == 0) {  // the per-step recomputation needs one segment's states
+      A.seg.clear();
+      for (int q = 0; q < 5 * every; ++q) {
+        double* p = aalloc(A, n3);
+        if (!p) return hipErrorOutOfMemory;
+        A.seg.push_back(p);
+      }
+    }
+    A.nsteps = nsteps; A.every = every; A.tape_limit = c.adj_tape;
   }
   return hipSuccess;
 }
@@ -1127,8 +1154,8 @@ static hipError_t check_adjoint_model(const Ctx& c) {
 }
 
 // forward run of nsteps large steps that keeps the state every 'every' steps;
-// with room for the segment records, the last segment also keeps its step
-// states and records, as the reverse sweep's recording forward would
+// its last A.tseg segments also keep their step states and records (the
+// tapes), as the reverse sweep's recording forward would
 hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every) {
   HIPCHK(check_adjoint_model(c));
   HIPCHK(adj_setup(c, nsteps, every));
@@ -1136,26 +1163,26 @@ hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every) {
   Model& M = *c.model;
   const size_t bytes = c.d.n3 * sizeof(double);
   double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
-  const bool recs = (int)A.recs.size() == A.every;
-  const int nlast = ((nsteps - 1) / every) * every;  // first step of the last segment
+  const int nseg = (nsteps + every - 1) / every;
+  const int nfirst = (nseg - A.tseg) * every;  // first taped step
   A.recorded = 0;
-  A.last_recorded = false;
+  A.taped = 0;
   prof_start(c, "model_forward_checkpointed", c.stream);
   for (int n = 0; n < nsteps; ++n) {
     if (n % every == 0)
       for (int v = 0; v < 5; ++v)
         HIPCHK(hipMemcpyAsync(A.ckpt[5 * (n / every) + v], st[v], bytes, hipMemcpyDeviceToDevice, c.stream));
-    if (recs && n >= nlast) {
+    if (A.tseg > 0 && n >= nfirst) {
       for (int v = 0; v < 5; ++v)
-        HIPCHK(hipMemcpyAsync(A.seg[5 * (n - nlast) + v], st[v], bytes, hipMemcpyDeviceToDevice, c.stream));
-      HIPCHK(model_large_step(c, &A.recs[n - nlast]));
+        HIPCHK(hipMemcpyAsync(A.seg[5 * (n - nfirst) + v], st[v], bytes, hipMemcpyDeviceToDevice, c.stream));
+      HIPCHK(model_large_step(c, &A.recs[n - nfirst]));
     } else {
       HIPCHK(model_large_step(c));
     }
   }
   prof_stop(c, c.stream);
   A.recorded = 1;
-  A.last_recorded = recs;
+  A.taped = A.tseg;
   return hipSuccess;
 }
 
@@ -1176,26 +1203,30 @@ hipError_t model_adjoint(Ctx& c, double* const xb[5]) {
   prof_start(c, "model_adjoint", s);
   HIPCHK(model_prepare(c));  // the tendencies M.t, which every step shares
   const int nseg = (A.nsteps + A.every - 1) / A.every;
-  const bool recs = (int)A.recs.size() == A.every;
+  const bool recs = A.tseg > 0;
   const StepRec one = model_rec(c);
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int n0 = sg * A.every, n1 = std::min(A.nsteps, n0 + A.every);
-    for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[v], A.ckpt[5 * sg + v], bytes, hipMemcpyDeviceToDevice, s));
-    auto seg = [&](int n, double** x) { for (int v = 0; v < 5; ++v) x[v] = A.seg[5 * (n - n0) + v]; };
+    // a taped segment reverses from its own slot of the tapes; the others
+    // record into slot 0, free by then (its segment is reversed before them)
+    const bool have = recs && sg >= nseg - A.taped;
+    const int base = have ? (sg - (nseg - A.tseg)) * A.every : 0;
+    auto seg = [&](int n, double** x) { for (int v = 0; v < 5; ++v) x[v] = A.seg[5 * (base + n - n0) + v]; };
+    if (!have)
+      for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[v], A.ckpt[5 * sg + v], bytes, hipMemcpyDeviceToDevice, s));
     if (recs) {
-      // one recording forward over the segment (done by the checkpointed
-      // forward for the last one), then the reverse steps
-      const bool have = A.last_recorded && sg == nseg - 1;
+      // one recording forward over the segment unless the checkpointed
+      // forward kept its tape, then the reverse steps
       for (int n = n0; n < n1 && !have; ++n) {
         double *x[5], *xn[5];
         seg(n, x);
         if (n + 1 < n1) seg(n + 1, xn);
-        HIPCHK(step_fwd(c, x, A.recs[n - n0], n + 1 < n1 ? xn : nullptr));
+        HIPCHK(step_fwd(c, x, A.recs[base + n - n0], n + 1 < n1 ? xn : nullptr));
       }
       for (int n = n1 - 1; n >= n0; --n) {
         double* x[5];
         seg(n, x);
-        HIPCHK(step_rev(c, x, xb, A.recs[n - n0]));
+        HIPCHK(step_rev(c, x, xb, A.recs[base + n - n0]));
       }
     } else {
       // recompute the states of the segment, then each reversed step its own record

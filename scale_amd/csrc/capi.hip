@@ -1135,6 +1135,10 @@ int scale_hip_tune_set(scale_hip_ctx* h, int32_t knob, int32_t value) {
       if (value < 0 || value > 2) return fail("SCALE_HIP_TUNE_TRACER_TILE must be 0, 1 or 2");
       h->c.tracer_tile = value;
       return 0;
+    case SCALE_HIP_TUNE_ADJ_TAPE:
+      if (value < 0) return fail("SCALE_HIP_TUNE_ADJ_TAPE must be >= 0");
+      h->c.adj_tape = value;
+      return 0;
     default:
       return fail("unknown tuning knob " + std::to_string(knob));
   }
@@ -1146,6 +1150,7 @@ int scale_hip_tune_get(scale_hip_ctx* h, int32_t knob, int32_t* value) {
   switch (knob) {
     case SCALE_HIP_TUNE_NF_TJ: *value = h->c.nf_tj; return 0;
     case SCALE_HIP_TUNE_TRACER_TILE: *value = h->c.tracer_tile; return 0;
+    case SCALE_HIP_TUNE_ADJ_TAPE: *value = h->c.adj_tape; return 0;
     default: return fail("unknown tuning knob " + std::to_string(knob));
   }
 }

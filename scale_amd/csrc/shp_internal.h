@@ -42,6 +42,7 @@ struct Ctx {
   // device-resident model
   Model* model = nullptr;
   AdjWork* adj = nullptr;
+  int adj_tape = 0;  // segments whose tapes the checkpointed forward keeps (0: half the free memory's worth)
   // named hipEvent timers, null when profiling is off
   Prof* prof = nullptr;
   // launch-shape knobs (scale_hip_tune_set); 0 = chosen from the grid

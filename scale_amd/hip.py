@@ -59,6 +59,7 @@ TINTEG_SHORT = {"RK4": 4, "RK3": 3, "RK3WS2002": 32}
 TINTEG_TRACER = {"RK3WS2002": 0, "EULER": 1}
 TUNE_NF_TJ = 1  # scale_hip_tune_set knob: rows of the numerical-filter tile (0 = from the grid)
 TUNE_TRACER_TILE = 2  # knob: FCT tracer stage on LDS tiles: 1 (default) on large grids, 2 always, 0 never
+TUNE_ADJ_TAPE = 3  # knob: trailing segments whose adjoint tapes stay in HBM (0: half the free memory's worth)
 
 
 _lib = None

@@ -49,13 +49,15 @@ def test_stage_adjoint_matches_oracle(scheme):
         assert close(got[key], ref[key]), key
 
 
-@pytest.mark.parametrize("records", ["1", "0"], ids=["segment_records", "per_step_recompute"])
-def test_model_adjoint_matches_oracle(records, monkeypatch):
-    """Both reverse sweeps: one recording forward per segment (the default)
-    and, with SCALE_HIP_ADJ_RECORDS=0, the per-step recomputation used when
-    the records do not fit in device memory."""
+@pytest.mark.parametrize("records,tape", [("1", 0), ("1", 1), ("0", 0)],
+                         ids=["all_segments_taped", "last_segment_taped", "per_step_recompute"])
+def test_model_adjoint_matches_oracle(records, tape, monkeypatch):
+    """The three reverse sweeps: every segment's tape kept by the forward (the
+    default where memory holds them), only the last one's (the others recorded
+    again from their checkpoints) and, with SCALE_HIP_ADJ_RECORDS=0, the
+    per-step recomputation used when no records fit in device memory."""
     monkeypatch.setenv("SCALE_HIP_ADJ_RECORDS", records)
-    print(f"[adjoint] model 8x8x10, 3 steps, checkpoint every 2, records={records}", flush=True)
+    print(f"[adjoint] model 8x8x10, 3 steps, checkpoint every 2, records={records}, taped segments={tape}", flush=True)
     case = perturbed_case()
     N = 3
     traj = [{k: np.array(v, order="F", copy=True) for k, v in case.state.items()}]
@@ -72,6 +74,7 @@ def test_model_adjoint_matches_oracle(records, monkeypatch):
         xb = O.adj_large_step(m, xb)
     ctx = hip.model_from_case(case)
     try:
+        ctx.tune_set(hip.TUNE_ADJ_TAPE, tape)
         ctx.model_forward_checkpointed(N, 2)
         for k in STATE:  # the forward itself matches the oracle
             assert close(ctx.model_get(k), traj[N][k], 1e-12), k
