@@ -26,7 +26,8 @@ namespace shp {
 // Arguments of a stage.  R is the type of the prognostic, tendency, flux and
 // num_diff arrays: double, or float for the mixed-precision path, which keeps
 // the reference state, the pressure terms (DPRES0, RT2P, REF_*) and the
-// vertical solve (PT, e, f) in double.
+// vertical solve in double; the solve's coefficients e, f and the face
+// value PT pass from the upward to the downward sweep through scratch in R.
 template <typename R>
 struct HeviArgsT {
   const Ctx* ctx;  // host side only
@@ -47,8 +48,9 @@ struct HeviArgsT {
   int has_coriolis;
   // outputs
   R *DENS_RK, *MOMZ_RK, *MOMX_RK, *MOMY_RK, *RHOT_RK, *mflx_hi, *tflx_hi;
-  // scratch (n3 each)
-  double *PT, *e, *f;
+  // scratch (n3 each): in FP32 the Thomas coefficients are rounded to float
+  // between the sweeps (PT is a float value already), like the prognostics
+  R *PT, *e, *f;
   // optional records for the adjoint: St and the solved C
   double *trSt = nullptr, *trCp = nullptr;
   // mixed precision: DENS and RHOT (and their stage values) are stored as
@@ -1323,14 +1325,14 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       }
       // the explicit z mass flux is num_diff itself: the back pass reads it
       // there again instead of a copy stored here
-      if (k <= KE - 1) { a.tflx_hi[g] = tZ; a.PT[g] = (double)PT; }
+      if (k <= KE - 1) { a.tflx_hi[g] = tZ; a.PT[g] = PT; }
       else { a.mflx_hi[g] = R(0); a.tflx_hi[g] = R(0); }
       if (k == KS) { a.mflx_hi[g - d.pl] = R(0); a.tflx_hi[g - d.pl] = R(0); }
       if (i == d.IS) {  // mflx_hi(Z) on the i = IS-1 column (HEVI.F90:370)
         a.mflx_hi[g - 1] = (k <= KE - 1) ? o_nDw : R(0);
         if (k == KS) a.mflx_hi[g - 1 - d.pl] = R(0);
       }
-      if (k > KS && k - 1 < KE - 1) { a.e[g - d.pl] = ek; a.f[g - d.pl] = fk; }
+      if (k > KS && k - 1 < KE - 1) { a.e[g - d.pl] = R(ek); a.f[g - d.pl] = R(fk); }
       cmZ = mZ; ctZ = tZ;
       rt0 = rt1; dp0 = dp1; st0 = St1; sr0 = Sr1; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = (double)Sw;
     }
@@ -1372,7 +1374,7 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
     }
   };
   double Cn = C_last;
-  double ptn = a.PT[px(KE - 1)];
+  double ptn = (double)a.PT[px(KE - 1)];
   {
     const size_t q = px(KE);
     const R d0 = a.DENS0[q], r0 = a.RHOT0[q];
@@ -1401,8 +1403,8 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
   }
   for (int k = KE - 2; k >= KS; --k) {
     const size_t q = px(k), q1 = px(k + 1);
-    const double Ck = a.e[q] * Cn + a.f[q];
-    const double ptk = a.PT[q];
+    const double Ck = (double)a.e[q] * Cn + (double)a.f[q];
+    const double ptk = (double)a.PT[q];
     const R z0 = a.MOMZ0[q];
     put_zflux(q, R((double)nd(I_DENS, ZDIR, q) + J33G * Ck), R((double)tfZ[q] + J33G * Ck * ptk));
     put_state(1, q, R((double)z0 + (Ck - (double)a.MOMZ[q])), z0, k);
@@ -1493,12 +1495,14 @@ static void fill_args(HeviArgsT<R>& a, const StageIOT<R>& io) {
   a.divdmp_coef = io.divdmp_coef; a.dtrk = io.dtrk; a.has_coriolis = io.has_coriolis;
   a.DENS_RK = io.o[0]; a.MOMZ_RK = io.o[1]; a.MOMX_RK = io.o[2]; a.MOMY_RK = io.o[3]; a.RHOT_RK = io.o[4];
   a.mflx_hi = io.mflx_hi; a.tflx_hi = io.tflx_hi;
-  a.PT = c.scr[3]; a.e = c.scr[4]; a.f = c.scr[5];
+  a.PT = (R*)c.scr[3]; a.e = (R*)c.scr[4]; a.f = (R*)c.scr[5];
   for (int v = 0; v < 5; ++v) { a.c1[v] = io.c1[v]; a.c2[v] = io.c2[v]; a.c3[v] = io.c3[v]; a.st[v] = io.st[v]; }
   for (int n = 0; n < 3; ++n) { a.cm[n] = io.cm[n]; a.ct[n] = io.ct[n]; }
   a.mflx_av = io.mflx_av; a.tflx_av = io.tflx_av;
   a.offD = io.offD; a.offR = io.offR;
-  if (io.trSt) { a.PT = io.trPT; a.trSt = io.trSt; a.trCp = io.trCp; }
+  if constexpr (sizeof(R) == sizeof(double)) {
+    if (io.trSt) { a.PT = io.trPT; a.trSt = io.trSt; a.trCp = io.trCp; }
+  }
 }
 
 // ---- ensemble tables ---------------------------------------------------------
