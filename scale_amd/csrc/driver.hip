@@ -13,6 +13,7 @@
 #include <string.h>
 #include "common.h"
 #include "shp_internal.h"
+#include "batch.h"
 #include "model.h"
 
 namespace shp {
@@ -862,7 +863,7 @@ ShortBufs<float> model_bufs<float>(Model& M) {
 
 // a device table of halo entries for the given fields of every member
 template <typename R>
-static hipError_t halo_table(Ensemble& E, int slot, const std::vector<std::pair<const Ctx*, R*>>& fields) {
+static hipError_t halo_table(Ensemble::Tab& t, const std::vector<std::pair<const Ctx*, R*>>& fields) {
   std::vector<HaloEnt<R>> ents;
   int wmax = 0, kmax = 0;
   for (const auto& f : fields) {
@@ -873,17 +874,193 @@ static hipError_t halo_table(Ensemble& E, int slot, const std::vector<std::pair<
     kmax = d.KA > kmax ? d.KA : kmax;
     ents.push_back(e);
   }
-  if (E.halo_tab[slot]) { (void)hipFree(E.halo_tab[slot]); E.halo_tab[slot] = nullptr; }
-  HIPCHK(hipMalloc(&E.halo_tab[slot], ents.size() * sizeof(HaloEnt<R>)));
-  HIPCHK(hipMemcpy(E.halo_tab[slot], ents.data(), ents.size() * sizeof(HaloEnt<R>), hipMemcpyHostToDevice));
-  E.halo_n[slot] = (int)ents.size(); E.halo_w[slot] = wmax; E.halo_k[slot] = kmax;
+  if (t.dev) { (void)hipFree(t.dev); t.dev = nullptr; }
+  HIPCHK(hipMalloc(&t.dev, ents.size() * sizeof(HaloEnt<R>)));
+  HIPCHK(hipMemcpy(t.dev, ents.data(), ents.size() * sizeof(HaloEnt<R>), hipMemcpyHostToDevice));
+  t.n = (int)ents.size(); t.w = wmax; t.k = kmax;
   return hipSuccess;
 }
 template <typename R>
-static hipError_t halo_table_launch(const Ensemble& E, int slot, hipStream_t s) {
-  hipLaunchKernelGGL(halo_batch_kernel<R>, dim3((E.halo_w[slot] + 255) / 256, E.halo_k[slot], E.halo_n[slot]), dim3(256), 0,
-                     s, (const HaloEnt<R>*)E.halo_tab[slot]);
+static hipError_t halo_table_launch(const Ensemble::Tab& t, hipStream_t s) {
+  hipLaunchKernelGGL(halo_batch_kernel<R>, dim3((t.w + 255) / 256, t.k, t.n), dim3(256), 0, s, (const HaloEnt<R>*)t.dev);
   return hipGetLastError();
+}
+static void tab_free(Ensemble::Tab& t) {
+  if (t.dev) (void)hipFree(t.dev);
+  t = Ensemble::Tab{};
+}
+
+// ---- ensemble tracers: point tables ---------------------------------------------
+// the preparation of the tracer stages of one member: the mean mass flux
+// (LARGE.F90:1016-1027) on the interior and the two RK3 stage densities on
+// the interior and its one-cell ring, as mflx_mean and dens_rk do
+struct TrPrepEnt {
+  Dims d;
+  double* mflx;
+  const double* av;
+  double nstep;
+  double *rk1, *rk2;
+  const double *D00, *D;
+};
+__global__ void __launch_bounds__(256) tr_prep_batch(const TrPrepEnt* __restrict__ tab, const unsigned* __restrict__ first,
+                                                     int nmem) {
+  const unsigned lt = blockIdx.x;
+  const int m = batch_member(first, nmem, lt);
+  const TrPrepEnt& e = tab[m];
+  const Dims& d = e.d;
+  const int ri = d.IE - d.IS + 3, rj = d.JE - d.JS + 3;
+  const size_t t = (size_t)(lt - first[m]) * 256 + threadIdx.x;
+  const size_t np = (size_t)ri * rj * (size_t)(d.KE - d.KS + 1);
+  if (t >= np) return;
+  const int i = d.IS - 1 + (int)(t % (size_t)ri);
+  const int j = d.JS - 1 + (int)((t / (size_t)ri) % (size_t)rj);
+  const int k = d.KS + (int)(t / ((size_t)ri * rj));
+  const size_t p = IX(d, k, i, j);
+  const double d0 = e.D00[p], d1 = e.D[p];
+  e.rk1[p] = d0 + (d1 - d0) / 3.0;
+  e.rk2[p] = d0 + (d1 - d0) * 0.5;
+  if (i >= d.IS && i <= d.IE && j >= d.JS && j <= d.JE)
+    for (int c = 0; c < 3; ++c) e.mflx[p + d.n3 * c] = e.av[p + d.n3 * c] / e.nstep;
+}
+// a stage's output back into the tracer, levels KS..KE (copy_levels)
+struct CopyEnt {
+  double* dst;
+  const double* src;
+  size_t n;
+};
+__global__ void __launch_bounds__(256) copy_batch(const CopyEnt* __restrict__ tab, const unsigned* __restrict__ first, int nmem) {
+  const unsigned lt = blockIdx.x;
+  const int m = batch_member(first, nmem, lt);
+  const CopyEnt& e = tab[m];
+  const size_t t0 = (size_t)(lt - first[m]) * 1024 + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const size_t t = t0 + 256 * (size_t)r;
+    if (t < e.n) e.dst[t] = e.src[t];
+  }
+}
+// a point table: the entries, then first[n+1] (blocks of each member)
+template <typename E>
+static hipError_t point_table(Ensemble::Tab& t, const std::vector<E>& ents, const std::vector<unsigned>& blocks) {
+  std::vector<unsigned> first(ents.size() + 1, 0);
+  for (size_t m = 0; m < ents.size(); ++m) first[m + 1] = first[m] + blocks[m];
+  const size_t ne = ents.size() * sizeof(E), off = (ne + 255) / 256 * 256;
+  tab_free(t);
+  HIPCHK(hipMalloc(&t.dev, off + first.size() * sizeof(unsigned)));
+  HIPCHK(hipMemcpy(t.dev, ents.data(), ne, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((char*)t.dev + off, first.data(), first.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+  t.n = (int)ents.size(); t.total = first.back();
+  return hipSuccess;
+}
+template <typename E>
+static const unsigned* point_first(const Ensemble::Tab& t) {
+  return (const unsigned*)((const char*)t.dev + ((size_t)t.n * sizeof(E) + 255) / 256 * 256);
+}
+
+// the batched tracer tables, where every member qualifies (see Ensemble)
+static hipError_t ensemble_tracers_build(Ensemble& E) {
+  E.tr_batched = false;
+  const Model& M0 = *E.members[0]->model;
+  const int qa = M0.QA;
+  if (qa <= 0) return hipSuccess;
+  for (Ctx* cp : E.members) {
+    const Model& M = *cp->model;
+    const int st = cp->scheme_tracer;
+    if (M.QA != qa || M.tinteg_tracer == 1 || M.nd_coef_q != 0.0 || M.fct != M0.fct || M.fct_along != M0.fct_along ||
+        cp->terrain || st != E.members[0]->scheme_tracer || st == UD5 || st == CD6)
+      return hipSuccess;  // the members' own tracer steps on the lanes
+  }
+  const int n = (int)E.members.size();
+  std::vector<const Ctx*> cs;
+  for (Ctx* cp : E.members) cs.push_back(cp);
+  for (auto* v : {&E.tr_s1, &E.tr_s2, &E.tr_ph, &E.tr_s3})
+    for (auto& b : *v) batch_free(b);
+  for (auto* v : {&E.tr_q, &E.tr_copy})
+    for (auto& t : *v) tab_free(t);
+  E.tr_s1.assign((size_t)qa, BatchBuf{}); E.tr_s2.assign((size_t)qa, BatchBuf{});
+  E.tr_ph.assign((size_t)qa, BatchBuf{}); E.tr_s3.assign((size_t)qa, BatchBuf{});
+  E.tr_q.assign((size_t)qa, Ensemble::Tab{}); E.tr_copy.assign((size_t)qa, Ensemble::Tab{});
+  for (int iq = 0; iq < qa; ++iq) {
+    std::vector<TracerIO> s1, s2, s3;
+    std::vector<CopyEnt> cp_ents;
+    std::vector<unsigned> cp_blocks;
+    std::vector<std::pair<const Ctx*, double*>> qf;
+    for (Ctx* cp : E.members) {
+      Model& M = *cp->model;
+      const Dims& d = cp->d;
+      double* q = M.QTRC + d.n3 * (size_t)iq;
+      const double* rq = M.RHOQ_tp ? M.RHOQ_tp + d.n3 * (size_t)iq : M.zeros;
+      TracerIO io{};
+      io.RHOQ_t = rq; io.DENS0 = M.DENS00; io.mflx_hi = M.mflx_hi; io.num_diff = M.ndq; io.flag_along = M.fct_along;
+      io.QTRCo = M.QRK1; io.QTRC = q; io.QTRC0 = q; io.DENS = M.DENS_RK; io.dtl = M.dtl / 3.0; io.flag_fct = 0;
+      s1.push_back(io);
+      io.QTRCo = M.QRK2; io.QTRC = M.QRK1; io.DENS = M.DENS_RK2; io.dtl = M.dtl / 2.0;
+      s2.push_back(io);
+      io.QTRCo = M.QRK1; io.QTRC = M.QRK2; io.DENS = M.DENS; io.dtl = M.dtl; io.flag_fct = M.fct;
+      s3.push_back(io);
+      const size_t off = d.pl * (size_t)d.KS, nn = d.pl * (size_t)(d.KE - d.KS + 1);
+      cp_ents.push_back(CopyEnt{q + off, M.QRK1 + off, nn});
+      cp_blocks.push_back((unsigned)((nn + 1023) / 1024));
+      qf.push_back({cp, q});
+    }
+    HIPCHK(tracer_batch_build(cs.data(), s1.data(), n, false, E.tr_s1[(size_t)iq]));
+    HIPCHK(tracer_batch_build(cs.data(), s2.data(), n, false, E.tr_s2[(size_t)iq]));
+    HIPCHK(tracer_batch_build(cs.data(), s3.data(), n, true, E.tr_ph[(size_t)iq]));
+    HIPCHK(tracer_batch_build(cs.data(), s3.data(), n, false, E.tr_s3[(size_t)iq]));
+    HIPCHK(point_table(E.tr_copy[(size_t)iq], cp_ents, cp_blocks));
+    HIPCHK(halo_table<double>(E.tr_q[(size_t)iq], qf));
+  }
+  std::vector<std::pair<const Ctx*, double*>> mf, q1, q2, rr;
+  std::vector<TrPrepEnt> prep;
+  std::vector<unsigned> prep_blocks;
+  for (Ctx* cp : E.members) {
+    Model& M = *cp->model;
+    const Dims& d = cp->d;
+    for (int c = 0; c < 3; ++c) mf.push_back({cp, M.mflx_hi + d.n3 * (size_t)c});
+    q1.push_back({cp, M.QRK1});
+    q2.push_back({cp, M.QRK2});
+    rr.push_back({cp, cp->scr[1]});
+    rr.push_back({cp, cp->scr[2]});
+    prep.push_back(TrPrepEnt{d, M.mflx_hi, M.mflx_av, (double)model_nstep(M), M.DENS_RK, M.DENS_RK2, M.DENS00, M.DENS});
+    const size_t np = (size_t)(d.IE - d.IS + 3) * (size_t)(d.JE - d.JS + 3) * (size_t)(d.KE - d.KS + 1);
+    prep_blocks.push_back((unsigned)((np + 255) / 256));
+  }
+  HIPCHK(halo_table<double>(E.tr_mf, mf));
+  HIPCHK(halo_table<double>(E.tr_q1, q1));
+  HIPCHK(halo_table<double>(E.tr_q2, q2));
+  HIPCHK(halo_table<double>(E.tr_r, rr));
+  HIPCHK(point_table(E.tr_prep, prep, prep_blocks));
+  E.tr_qa = qa; E.tr_fct = M0.fct;
+  E.tr_batched = true;
+  return hipSuccess;
+}
+
+// the tracers of every member (large_tracers, batched): the preparation,
+// then per tracer the three stages, each with the halo fill of its output
+static hipError_t ensemble_tracers(Ensemble& E, hipStream_t s) {
+  hipLaunchKernelGGL(tr_prep_batch, dim3(E.tr_prep.total), dim3(256), 0, s, (const TrPrepEnt*)E.tr_prep.dev,
+                     point_first<TrPrepEnt>(E.tr_prep), E.tr_prep.n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(halo_table_launch<double>(E.tr_mf, s));
+  for (int iq = 0; iq < E.tr_qa; ++iq) {
+    HIPCHK(tracer_batch_launch(E.tr_s1[(size_t)iq], 0, s));  // stage 1: dt/3, no FCT
+    HIPCHK(halo_table_launch<double>(E.tr_q1, s));
+    HIPCHK(tracer_batch_launch(E.tr_s2[(size_t)iq], 0, s));  // stage 2: dt/2, no FCT
+    HIPCHK(halo_table_launch<double>(E.tr_q2, s));
+    if (E.tr_fct) {  // stage 3: dt, FCT
+      HIPCHK(tracer_batch_launch(E.tr_ph[(size_t)iq], 1, s));
+      HIPCHK(tracer_batch_launch(E.tr_s3[(size_t)iq], 2, s));
+      HIPCHK(halo_table_launch<double>(E.tr_r, s));
+      HIPCHK(tracer_batch_launch(E.tr_s3[(size_t)iq], 3, s));
+    } else {
+      HIPCHK(tracer_batch_launch(E.tr_s3[(size_t)iq], 0, s));
+    }
+    const Ensemble::Tab& t = E.tr_copy[(size_t)iq];
+    hipLaunchKernelGGL(copy_batch, dim3(t.total), dim3(256), 0, s, (const CopyEnt*)t.dev, point_first<CopyEnt>(t), t.n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(halo_table_launch<double>(E.tr_q[(size_t)iq], s));
+  }
+  return hipSuccess;
 }
 
 template <typename R>
@@ -903,7 +1080,7 @@ static hipError_t ensemble_build_t(Ensemble& E) {
         for (int v = 0; v < 5; ++v) f.push_back({cp, B.st[v]});
       }
     }
-    HIPCHK(halo_table<R>(E, slot, f));
+    HIPCHK(halo_table<R>(E.halo[slot], f));
   }
   for (int r = 0; r < 4; ++r) {
     std::vector<StageIOT<R>> ios;
@@ -925,7 +1102,8 @@ static hipError_t ensemble_build_t(Ensemble& E) {
     f.offD = M.mixed ? M.foff : nullptr; f.offR = M.mixed ? M.foff + cp->d.KA : nullptr;
     in.push_back(f);
   }
-  return numfilter_batch_build(in.data(), n, E.nf_tj, E.nf);
+  HIPCHK(numfilter_batch_build(in.data(), n, E.nf_tj, E.nf));
+  return ensemble_tracers_build(E);
 }
 
 hipError_t ensemble_build(Ensemble& E) {
@@ -967,7 +1145,7 @@ static hipError_t ensemble_step_t(Ensemble& E) {
     // numerical filter (LARGE.F90:845-858) and its halos: one launch each
     // (a member with ND_COEF = 0 gets zeros)
     HIPCHK(numfilter_batch_launch(E.nf, E.nf_bpc, E.ncu, s));
-    HIPCHK(halo_table_launch<R>(E, 0, s));
+    HIPCHK(halo_table_launch<R>(E.halo[0], s));
     for (Ctx* cp : E.members) {
       if (cp->model->nd_coef == 0.0) HIPCHK(fill_zero(model_bufs<R>(*cp->model).num_diff, 15 * cp->d.n3, s));
     }
@@ -975,15 +1153,23 @@ static hipError_t ensemble_step_t(Ensemble& E) {
     // with the halos of its outputs in one launch
     for (int r = 0; r < 4; ++r) {
       HIPCHK(hevi_batch_launch(E.stage[r], E.stage_bpc, E.ncu, s));
-      HIPCHK(halo_table_launch<R>(E, r + 1, s));
+      HIPCHK(halo_table_launch<R>(E.halo[r + 1], s));
     }
     HIPCHK(fork());
     for (Ctx* cp : E.members) HIPCHK(large_after_short(*cp));
     HIPCHK(join());
   }
-  HIPCHK(fork());
-  for (Ctx* cp : E.members) HIPCHK(large_tracers(*cp, nstep));
-  return join();
+  if (!E.tr_batched) {
+    HIPCHK(fork());
+    for (Ctx* cp : E.members) HIPCHK(large_tracers(*cp, nstep));
+    return join();
+  }
+  if (E.members[0]->model->mixed) {  // the tracers advance on the FP64 copy of the density
+    HIPCHK(fork());
+    for (Ctx* cp : E.members) HIPCHK(model_state_to_f64(*cp, F64_DENS));
+    HIPCHK(join());
+  }
+  return ensemble_tracers(E, s);
 }
 
 // every member's FP64 copies of its FP32 state and fluxes (mixed precision),
@@ -1059,10 +1245,17 @@ void ensemble_free(Ensemble& E) {
   E.exec = nullptr; E.graph = nullptr;
   for (auto& b : E.stage) batch_free(b);
   batch_free(E.nf);
-  for (int q = 0; q < 5; ++q) {
-    if (E.halo_tab[q]) (void)hipFree(E.halo_tab[q]);
-    E.halo_tab[q] = nullptr;
+  for (auto& t : E.halo) tab_free(t);
+  for (auto* v : {&E.tr_s1, &E.tr_s2, &E.tr_ph, &E.tr_s3}) {
+    for (auto& b : *v) batch_free(b);
+    v->clear();
   }
+  for (auto* t : {&E.tr_mf, &E.tr_q1, &E.tr_q2, &E.tr_r, &E.tr_prep}) tab_free(*t);
+  for (auto* v : {&E.tr_q, &E.tr_copy}) {
+    for (auto& t : *v) tab_free(t);
+    v->clear();
+  }
+  E.tr_batched = false;
   E.built = false;
 }
 

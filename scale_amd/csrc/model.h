@@ -68,9 +68,26 @@ struct Ensemble {
   // (0: one block per tile) and the filter's tile rows
   int stage_bpc = 0, nf_bpc = 0, nf_tj = 8;
   BatchBuf stage[4], nf;
-  // halo tables (device): the filter's fields, the outputs of stages 1..3, the state
-  void* halo_tab[5] = {};
-  int halo_n[5] = {}, halo_w[5] = {}, halo_k[5] = {};
+  // a device table of per-member entries: n entries, the widest w and
+  // deepest k of them (halo tables), or total blocks (point tables)
+  struct Tab {
+    void* dev = nullptr;
+    int n = 0, w = 0, k = 0;
+    unsigned total = 0;
+  };
+  // halo tables: the filter's fields, the outputs of stages 1..3, the state
+  Tab halo[5];
+  // the tracers of every member in batched launches, where the members have
+  // the same number of tracers, the RK3 tracer integration, no tracer filter,
+  // one FCT setting and one two-cell tracer scheme: per tracer the tables of
+  // stages 1 and 2, of the FCT phi_lo pass and of the last stage; halo tables
+  // of the mass fluxes, QRK1, QRK2, R+/R- and each tracer; the point tables
+  // of the preparation (flux mean, stage densities) and of each copy back
+  bool tr_batched = false;
+  int tr_qa = 0, tr_fct = 0;
+  std::vector<BatchBuf> tr_s1, tr_s2, tr_ph, tr_s3;
+  Tab tr_mf, tr_q1, tr_q2, tr_r, tr_prep;
+  std::vector<Tab> tr_q, tr_copy;
   bool built = false;
   // lanes: streams the members' own launches spread over (member m on lane
   // m % nlanes), forked from and joined back to `stream` around each batched

@@ -17,6 +17,7 @@
 
 #include "common.h"
 #include "shp_internal.h"
+#include "batch.h"
 
 namespace shp {
 
@@ -327,11 +328,7 @@ numfilter_tile_batch(const NfArgsT<R>* __restrict__ tab,
                                                               const unsigned* __restrict__ ni, int nmem, unsigned total) {
   for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
     const unsigned lt = xcd_block(t, total);
-    int lo = 0, hi = nmem - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) / 2;
-      if (first[mid] <= lt) lo = mid; else hi = mid - 1;
-    }
+    const int lo = batch_member(first, nmem, lt);
     numfilter_tile_body<TJ, R>(tab[lo], lt - first[lo], ni[lo]);
     __syncthreads();
   }
@@ -387,7 +384,7 @@ static hipError_t numfilter_t(const Ctx& c, R* num_diff, const R* const st[5], c
   return halo_fill(c, arrs, 15, s);
 }
 
-// ---- ensemble tables (layout as hevi.hip's batch tables) ------------------------
+// ---- ensemble tables (batch.h) ----------------------------------------------------
 template <typename R>
 static hipError_t nf_batch_build_t(const NfIn<R>* in, int n, int tj, BatchBuf& b) {
   if (n <= 0 || (tj != 4 && tj != 8)) return hipErrorInvalidValue;
@@ -408,34 +405,19 @@ static hipError_t nf_batch_build_t(const NfIn<R>* in, int n, int tj, BatchBuf& b
     first[(size_t)m + 1] = first[(size_t)m] + ni[(size_t)m] * (unsigned)((d.JE - d.JS + 1 + tj - 1) / tj);
     ka_max = d.KA > ka_max ? d.KA : ka_max;
   }
-  const size_t na = args.size() * sizeof(NfArgsT<R>), nf = first.size() * sizeof(unsigned), nn = ni.size() * sizeof(unsigned);
-  const size_t off_f = (na + 255) / 256 * 256, off_n = off_f + (nf + 255) / 256 * 256;
-  batch_free(b);
-  hipError_t e = hipMalloc(&b.dev, off_n + nn + 256);
-  if (e != hipSuccess) { b.dev = nullptr; return e; }
-  char* p = (char*)b.dev;
-  if ((e = hipMemcpy(p, args.data(), na, hipMemcpyHostToDevice)) != hipSuccess) return e;
-  if ((e = hipMemcpy(p + off_f, first.data(), nf, hipMemcpyHostToDevice)) != hipSuccess) return e;
-  if ((e = hipMemcpy(p + off_n, ni.data(), nn, hipMemcpyHostToDevice)) != hipSuccess) return e;
-  b.nmem = n; b.total = first.back(); b.tj = tj; b.fp32 = sizeof(R) == sizeof(float); b.ka_max = ka_max;
-  return hipSuccess;
+  const hipError_t e = batch_upload(args, first, ni, b);
+  b.tj = tj; b.fp32 = sizeof(R) == sizeof(float); b.ka_max = ka_max;
+  return e;
 }
 hipError_t numfilter_batch_build(const NfIn<double>* in, int n, int tj, BatchBuf& b) { return nf_batch_build_t(in, n, tj, b); }
 hipError_t numfilter_batch_build(const NfIn<float>* in, int n, int tj, BatchBuf& b) { return nf_batch_build_t(in, n, tj, b); }
 
 template <int TJ, typename R>
 static hipError_t nf_batch_launch_t(const BatchBuf& b, int bpc, int ncu, hipStream_t s) {
-  const size_t na = (size_t)b.nmem * sizeof(NfArgsT<R>), nf = ((size_t)b.nmem + 1) * sizeof(unsigned);
-  const size_t off_f = (na + 255) / 256 * 256, off_n = off_f + (nf + 255) / 256 * 256;
-  const NfArgsT<R>* tab = (const NfArgsT<R>*)b.dev;
-  const unsigned* first = (const unsigned*)((const char*)b.dev + off_f);
-  const unsigned* ni = (const unsigned*)((const char*)b.dev + off_n);
-  unsigned g = b.total;
-  if (bpc > 0) {
-    g = (unsigned)(bpc * ncu) / 8 * 8;
-    if (g < 8) g = 8;
-    if (g > b.total) g = b.total;
-  }
+  const NfArgsT<R>* tab;
+  const unsigned *first, *ni;
+  batch_tables(b, tab, first, ni);
+  const unsigned g = batch_grid(b.total, bpc, ncu);
   const size_t lds = 8 * (size_t)b.ka_max * sizeof(double);
   hipLaunchKernelGGL((numfilter_tile_batch<TJ, R>), dim3(g), dim3(64, TJ), lds, s, tab, first, ni, b.nmem, b.total);
   return hipGetLastError();

@@ -195,6 +195,14 @@ struct TracerAdjIO {
   double *QTRC_b, *QTRC0_b, *RHOQ_t_b, *num_diff_b;
 };
 hipError_t tracer_stage_adj(const Ctx& c, const TracerIO& io, const TracerAdjIO& adj, hipStream_t s);
+// one pass of the tracer stage of every member in one launch (flat terrain,
+// one tracer scheme with two-cell faces, one FCT bound kind): the table of
+// the members' stage arguments (ios: as tracer_stage takes them; the FCT
+// work arrays are each member's c.scr[0..2]), for the phi_lo pass (philo) or
+// for the others; kinds 0 the stage without FCT, 1 phi_lo, 2 R+/R-, 3 the
+// limited update (the R+/R- halo fill between 2 and 3 is the caller's)
+hipError_t tracer_batch_build(const Ctx* const* cs, const TracerIO* ios, int n, bool philo, BatchBuf& b);
+hipError_t tracer_batch_launch(const BatchBuf& b, int kind, hipStream_t s);
 // ATMOS_DYN_numfilter_coef_q (ND_ORDER = 1) with the halo fill of its 3
 // fields; REF_qv non-null for QV.  Needs the filter tables and c.ndqc.
 hipError_t numfilter_q(const Ctx& c, double* num_diff_q, const double* DENS, const double* QTRC, const double* REF_qv,

@@ -20,6 +20,7 @@
 #include "common.h"
 #include "fct.h"
 #include "shp_internal.h"
+#include "batch.h"
 
 namespace shp {
 
@@ -128,11 +129,13 @@ struct TrOps {
 };
 
 // ---- no FCT: one kernel (tracer_fvm_heve.F90:226-231) ----------------------
+// Each pass is a body over the (64 i) x (4 k) x (1 j) tile lb of a grid
+// ni_tiles x nk_tiles tiles across, run by its kernel once per block or, for
+// an ensemble, by tracer_batch over a table of members.
 template <int S, bool TERRAIN>
-__global__ void __launch_bounds__(256) tracer_nofct(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+__device__ __forceinline__ void tracer_nofct_body(const TrArgs& a, unsigned lb, unsigned ni_tiles, unsigned nk_tiles) {
   const Dims& d = a.d;
   TrOps<S, TERRAIN> o(a);
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
   const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
   const int i = d.IS + (int)(it * 64 + threadIdx.x);
   const int k = d.KS + (int)(kt * 4 + threadIdx.y);
@@ -147,13 +150,16 @@ __global__ void __launch_bounds__(256) tracer_nofct(TrArgs a, unsigned ni_tiles,
                            + a.RHOQ_t[p]))
              / a.DENS[p];
 }
+template <int S, bool TERRAIN>
+__global__ void __launch_bounds__(256) tracer_nofct(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+  tracer_nofct_body<S, TERRAIN>(a, xcd_block(blockIdx.x, nblocks), ni_tiles, nk_tiles);
+}
 
 // ---- FCT pass 1: phi_lo on the interior plus a one-cell ring ----------------
 template <bool TERRAIN>
-__global__ void __launch_bounds__(256) fct_philo(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+__device__ __forceinline__ void fct_philo_body(const TrArgs& a, unsigned lb, unsigned ni_tiles, unsigned nk_tiles) {
   const Dims& d = a.d;
   TrOps<UD1, TERRAIN> o(a);
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
   const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
   const int i = d.IS - 1 + (int)(it * 64 + threadIdx.x);
   const int k = d.KS + (int)(kt * 4 + threadIdx.y);
@@ -166,6 +172,10 @@ __global__ void __launch_bounds__(256) fct_philo(TrArgs a, unsigned ni_tiles, un
                               + (o.lY(k, i, j) - o.lY(k, i, j - 1)) * a.RCDY[j])
                             * o.m1(i, j) * o.m2(i, j) / o.gs(I_XYZ, p)))
               / a.DENS[p];
+}
+template <bool TERRAIN>
+__global__ void __launch_bounds__(256) fct_philo(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+  fct_philo_body<TERRAIN>(a, xcd_block(blockIdx.x, nblocks), ni_tiles, nk_tiles);
 }
 
 // upstream (qa) / downstream (qb) 14-point combinations (COMMON.F90:2096-2155)
@@ -186,10 +196,9 @@ __device__ __forceinline__ double fct_qb(const double* f, const double* P, const
 
 // ---- FCT pass 2: R+ and R- ----------------------------------------------------
 template <int S, bool TERRAIN, bool VECT>
-__global__ void __launch_bounds__(256) fct_ratio(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+__device__ __forceinline__ void fct_ratio_body(const TrArgs& a, unsigned lb, unsigned ni_tiles, unsigned nk_tiles) {
   const Dims& d = a.d;
   TrOps<S, TERRAIN> o(a);
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
   const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
   const int i = d.IS + (int)(it * 64 + threadIdx.x);
   const int k = d.KS + (int)(kt * 4 + threadIdx.y);
@@ -252,15 +261,18 @@ __global__ void __launch_bounds__(256) fct_ratio(TrArgs a, unsigned ni_tiles, un
   zerosw = 0.5 - fsign(0.5, pjmns - EPS_RP);
   a.rjmns[p] = fmin2(1.0, qjmns * (1.0 - zerosw) / (pjmns - zerosw));
 }
+template <int S, bool TERRAIN, bool VECT>
+__global__ void __launch_bounds__(256) fct_ratio(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+  fct_ratio_body<S, TERRAIN, VECT>(a, xcd_block(blockIdx.x, nblocks), ni_tiles, nk_tiles);
+}
 
 // ---- FCT pass 3: limited anti-diffusive fluxes and the update ------------------
 // THETA: the HEVE theta correction RHOT_RK += dtrk div(anti) (HEVE.F90:1459-1474)
 // instead of the tracer update
 template <int S, bool TERRAIN, bool THETA>
-__global__ void __launch_bounds__(256) fct_update(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+__device__ __forceinline__ void fct_update_body(const TrArgs& a, unsigned lb, unsigned ni_tiles, unsigned nk_tiles) {
   const Dims& d = a.d;
   TrOps<S, TERRAIN> o(a);
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
   const unsigned it = lb % ni_tiles, kt = (lb / ni_tiles) % nk_tiles, jt = lb / (ni_tiles * nk_tiles);
   const int i = d.IS + (int)(it * 64 + threadIdx.x);
   const int k = d.KS + (int)(kt * 4 + threadIdx.y);
@@ -303,6 +315,88 @@ __global__ void __launch_bounds__(256) fct_update(TrArgs a, unsigned ni_tiles, u
                            * o.m1(i, j) * o.m2(i, j) / o.gs(I_XYZ, p)
                            + a.RHOQ_t[p]))
              / a.DENS[p];
+}
+template <int S, bool TERRAIN, bool THETA>
+__global__ void __launch_bounds__(256) fct_update(TrArgs a, unsigned ni_tiles, unsigned nk_tiles, unsigned nblocks) {
+  fct_update_body<S, TERRAIN, THETA>(a, xcd_block(blockIdx.x, nblocks), ni_tiles, nk_tiles);
+}
+
+// ---- ensembles: one launch of a pass over the tiles of many members --------
+// tab[m] is member m's TrArgs, first[m] its first logical tile, nk[m] its
+// tile counts (ni in the low 16 bits, nk in the high ones); flat terrain
+enum { TRB_NOFCT = 0, TRB_PHILO = 1, TRB_RATIO = 2, TRB_UPDATE = 3 };
+template <int KIND, int S, bool VECT>
+__global__ void __launch_bounds__(256) tracer_batch(const TrArgs* __restrict__ tab, const unsigned* __restrict__ first,
+                                                    const unsigned* __restrict__ nk, int nmem, unsigned total) {
+  const unsigned lt = xcd_block(blockIdx.x, total);
+  const int m = batch_member(first, nmem, lt);
+  const unsigned lb = lt - first[m], ni_t = nk[m] & 0xffffu, nk_t = nk[m] >> 16;
+  if constexpr (KIND == TRB_NOFCT) tracer_nofct_body<S, false>(tab[m], lb, ni_t, nk_t);
+  else if constexpr (KIND == TRB_PHILO) fct_philo_body<false>(tab[m], lb, ni_t, nk_t);
+  else if constexpr (KIND == TRB_RATIO) fct_ratio_body<S, false, VECT>(tab[m], lb, ni_t, nk_t);
+  else fct_update_body<S, false, false>(tab[m], lb, ni_t, nk_t);
+}
+
+// kinds: the stage without FCT, or the FCT passes (phi_lo on its ring tiles,
+// then R+/R- and the update on the interior tiles, which share one table)
+hipError_t tracer_batch_build(const Ctx* const* cs, const TracerIO* ios, int n, bool philo, BatchBuf& b) {
+  if (n <= 0) return hipErrorInvalidValue;
+  std::vector<TrArgs> args((size_t)n);
+  std::vector<unsigned> first((size_t)n + 1, 0), nk((size_t)n, 0);
+  const int scheme = cs[0]->scheme_tracer, along = ios[0].flag_along;
+  for (int m = 0; m < n; ++m) {
+    const Ctx& c = *cs[m];
+    const TracerIO& io = ios[m];
+    if (c.terrain || c.scheme_tracer != scheme || io.flag_along != along || c.scheme_tracer == UD5 ||
+        c.scheme_tracer == CD6)
+      return hipErrorNotSupported;
+    TrArgs& a = args[(size_t)m];
+    a.d = c.d;
+    a.QTRC = io.QTRC; a.QTRC0 = io.QTRC0; a.RHOQ_t = io.RHOQ_t; a.DENS0 = io.DENS0; a.DENS = io.DENS;
+    a.mflx = io.mflx_hi; a.nd = io.num_diff; a.nds = c.d.n3; a.GSQRT = c.GSQRT; a.MAPF_XY = c.MAPF;
+    a.RCDZ = c.g1.RCDZ; a.RCDX = c.g1.RCDX; a.RCDY = c.g1.RCDY;
+    a.dtl = io.dtl; a.QTRCo = io.QTRCo;
+    a.phi_lo = c.scr[0]; a.rjpls = c.scr[1]; a.rjmns = c.scr[2];
+    const Dims& d = c.d;
+    const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
+    const unsigned ni_t = philo ? (imax + 2 + 63) / 64 : (imax + 63) / 64, nk_t = (kmax + 3) / 4;
+    const unsigned nj_t = philo ? jmax + 2 : jmax;
+    if (ni_t > 0xffffu || nk_t > 0xffffu) return hipErrorNotSupported;
+    nk[(size_t)m] = ni_t | (nk_t << 16);
+    first[(size_t)m + 1] = first[(size_t)m] + ni_t * nk_t * nj_t;
+  }
+  const hipError_t e = batch_upload(args, first, nk, b);
+  b.scheme = scheme; b.tj = along;
+  return e;
+}
+
+template <int KIND, int S>
+static hipError_t tracer_batch_t(const BatchBuf& b, hipStream_t s) {
+  const TrArgs* tab;
+  const unsigned *first, *nk;
+  batch_tables(b, tab, first, nk);
+  if (b.tj) hipLaunchKernelGGL((tracer_batch<KIND, S, true>), dim3(b.total), dim3(64, 4), 0, s, tab, first, nk, b.nmem, b.total);
+  else hipLaunchKernelGGL((tracer_batch<KIND, S, false>), dim3(b.total), dim3(64, 4), 0, s, tab, first, nk, b.nmem, b.total);
+  return hipGetLastError();
+}
+template <int KIND>
+static hipError_t tracer_batch_k(const BatchBuf& b, hipStream_t s) {
+  switch (b.scheme) {
+    case UD3: return tracer_batch_t<KIND, UD3>(b, s);
+    case UD3KOREN: return tracer_batch_t<KIND, UD3KOREN>(b, s);
+    case CD2: return tracer_batch_t<KIND, CD2>(b, s);
+    case UD1: return tracer_batch_t<KIND, UD1>(b, s);
+    default: return tracer_batch_t<KIND, CD4>(b, s);
+  }
+}
+hipError_t tracer_batch_launch(const BatchBuf& b, int kind, hipStream_t s) {
+  if (!b.dev) return hipErrorInvalidValue;
+  switch (kind) {
+    case TRB_NOFCT: return tracer_batch_k<TRB_NOFCT>(b, s);
+    case TRB_PHILO: return tracer_batch_k<TRB_PHILO>(b, s);
+    case TRB_RATIO: return tracer_batch_k<TRB_RATIO>(b, s);
+    default: return tracer_batch_k<TRB_UPDATE>(b, s);
+  }
 }
 
 // ---------------------------------------------------------------------------
