@@ -973,6 +973,10 @@ static hipError_t ensemble_tracers_build(Ensemble& E) {
   const int n = (int)E.members.size();
   std::vector<const Ctx*> cs;
   for (Ctx* cp : E.members) cs.push_back(cp);
+  // the FCT stage on LDS tiles for the two-cell schemes (the tiles of all
+  // members keep every CU busy even where one member's would not)
+  const int st = E.members[0]->scheme_tracer;
+  E.tr_tile = M0.fct && (st == UD3 || st == CD4 || st == UD3KOREN || st == UD1 || st == CD2);
   for (auto* v : {&E.tr_s1, &E.tr_s2, &E.tr_ph, &E.tr_s3})
     for (auto& b : *v) batch_free(b);
   for (auto* v : {&E.tr_q, &E.tr_copy})
@@ -1005,8 +1009,12 @@ static hipError_t ensemble_tracers_build(Ensemble& E) {
     }
     HIPCHK(tracer_batch_build(cs.data(), s1.data(), n, false, E.tr_s1[(size_t)iq]));
     HIPCHK(tracer_batch_build(cs.data(), s2.data(), n, false, E.tr_s2[(size_t)iq]));
-    HIPCHK(tracer_batch_build(cs.data(), s3.data(), n, true, E.tr_ph[(size_t)iq]));
-    HIPCHK(tracer_batch_build(cs.data(), s3.data(), n, false, E.tr_s3[(size_t)iq]));
+    if (E.tr_tile) {
+      HIPCHK(trt_batch_build(cs.data(), s3.data(), n, E.tr_s3[(size_t)iq]));
+    } else {
+      HIPCHK(tracer_batch_build(cs.data(), s3.data(), n, true, E.tr_ph[(size_t)iq]));
+      HIPCHK(tracer_batch_build(cs.data(), s3.data(), n, false, E.tr_s3[(size_t)iq]));
+    }
     HIPCHK(point_table(E.tr_copy[(size_t)iq], cp_ents, cp_blocks));
     HIPCHK(halo_table<double>(E.tr_q[(size_t)iq], qf));
   }
@@ -1047,7 +1055,11 @@ static hipError_t ensemble_tracers(Ensemble& E, hipStream_t s) {
     HIPCHK(halo_table_launch<double>(E.tr_q1, s));
     HIPCHK(tracer_batch_launch(E.tr_s2[(size_t)iq], 0, s));  // stage 2: dt/2, no FCT
     HIPCHK(halo_table_launch<double>(E.tr_q2, s));
-    if (E.tr_fct) {  // stage 3: dt, FCT
+    if (E.tr_fct && E.tr_tile) {  // stage 3: dt, FCT on tiles
+      HIPCHK(trt_batch_launch(E.tr_s3[(size_t)iq], 0, s));
+      HIPCHK(halo_table_launch<double>(E.tr_r, s));
+      HIPCHK(trt_batch_launch(E.tr_s3[(size_t)iq], 1, s));
+    } else if (E.tr_fct) {  // stage 3: dt, FCT
       HIPCHK(tracer_batch_launch(E.tr_ph[(size_t)iq], 1, s));
       HIPCHK(tracer_batch_launch(E.tr_s3[(size_t)iq], 2, s));
       HIPCHK(halo_table_launch<double>(E.tr_r, s));

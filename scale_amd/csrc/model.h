@@ -83,7 +83,7 @@ struct Ensemble {
   // stages 1 and 2, of the FCT phi_lo pass and of the last stage; halo tables
   // of the mass fluxes, QRK1, QRK2, R+/R- and each tracer; the point tables
   // of the preparation (flux mean, stage densities) and of each copy back
-  bool tr_batched = false;
+  bool tr_batched = false, tr_tile = false;
   int tr_qa = 0, tr_fct = 0;
   std::vector<BatchBuf> tr_s1, tr_s2, tr_ph, tr_s3;
   Tab tr_mf, tr_q1, tr_q2, tr_r, tr_prep;

@@ -203,6 +203,10 @@ hipError_t tracer_stage_adj(const Ctx& c, const TracerIO& io, const TracerAdjIO&
 // limited update (the R+/R- halo fill between 2 and 3 is the caller's)
 hipError_t tracer_batch_build(const Ctx* const* cs, const TracerIO* ios, int n, bool philo, BatchBuf& b);
 hipError_t tracer_batch_launch(const BatchBuf& b, int kind, hipStream_t s);
+// the same FCT passes on LDS tiles (tracer_tile.hip; two-cell schemes):
+// kinds 0 R+/R- (phi_lo within), 1 the update
+hipError_t trt_batch_build(const Ctx* const* cs, const TracerIO* ios, int n, BatchBuf& b);
+hipError_t trt_batch_launch(const BatchBuf& b, int kind, hipStream_t s);
 // ATMOS_DYN_numfilter_coef_q (ND_ORDER = 1) with the halo fill of its 3
 // fields; REF_qv non-null for QV.  Needs the filter tables and c.ndqc.
 hipError_t numfilter_q(const Ctx& c, double* num_diff_q, const double* DENS, const double* QTRC, const double* REF_qv,

@@ -20,6 +20,7 @@
 #include "common.h"
 #include "fct.h"
 #include "shp_internal.h"
+#include "batch.h"
 
 namespace shp {
 
@@ -70,8 +71,11 @@ __device__ __forceinline__ double tt_hz(const Dims& d, int kk, double vel, const
 }
 
 // ---- FCT pass 1: phi_lo (ring of one), anti-diffusive fluxes, R+/R- ------------
+// each pass is a body over tile lb of a grid ni_tiles tiles across, run by
+// its kernel once per block or, for an ensemble, by trt_batch over a table
+// of members
 template <int S, int TJ, bool VECT>
-__global__ void __launch_bounds__(64 * TJ) trt_ratio(TtArgs a, unsigned ni_tiles, unsigned nblocks) {
+__device__ __forceinline__ void trt_ratio_body(const TtArgs& a, unsigned lb, unsigned ni_tiles) {
   constexpr int W2 = 68, H2 = TJ + 4, W1 = 66, H1 = TJ + 2;
   __shared__ double sQ0[5][H2][W2];  // QTRC0, two-cell ring, levels k-2..k+2 (slot = level % 5)
   __shared__ double sQ[H2][W2];      // QTRC at level k, two-cell ring
@@ -81,7 +85,6 @@ __global__ void __launch_bounds__(64 * TJ) trt_ratio(TtArgs a, unsigned ni_tiles
   __shared__ double sAY[TJ + 1][64]; // anti-diffusive y fluxes through faces j0-1 .. j0+TJ-1
   const Dims& d = a.d;
   const int KS = d.KS, KE = d.KE;
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
   const int i0 = d.IS + 64 * (int)(lb % ni_tiles), j0 = d.JS + TJ * (int)(lb / ni_tiles);
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + 64 * ty, nthr = 64 * TJ;
   const int i = i0 + tx, j = j0 + ty;
@@ -241,6 +244,11 @@ __global__ void __launch_bounds__(64 * TJ) trt_ratio(TtArgs a, unsigned ni_tiles
   }
 }
 
+template <int S, int TJ, bool VECT>
+__global__ void __launch_bounds__(64 * TJ) trt_ratio(TtArgs a, unsigned ni_tiles, unsigned nblocks) {
+  trt_ratio_body<S, TJ, VECT>(a, xcd_block(blockIdx.x, nblocks), ni_tiles);
+}
+
 // ---- FCT pass 2: limited anti-diffusive fluxes and the update ------------------
 // Software-pipelined: every global load of level k+1 (R+/R- of level k+2) is
 // issued into registers right after the level-k barrier, so that it is in
@@ -252,7 +260,7 @@ __global__ void __launch_bounds__(64 * TJ) trt_ratio(TtArgs a, unsigned ni_tiles
 // prefetch at two blocks per CU by 15 %; the ratio pass, heavier in
 // arithmetic, keeps two blocks per CU without it.
 template <int S, int TJ>
-__global__ void __launch_bounds__(64 * TJ) trt_update(TtArgs a, unsigned ni_tiles, unsigned nblocks) {
+__device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, unsigned ni_tiles) {
   constexpr int W2 = 68, H2 = TJ + 4, W1 = 66, H1 = TJ + 2, NT = 64 * TJ;
   constexpr int NQ = (H2 * W2 + NT - 1) / NT, NR = (H1 * W1 + NT - 1) / NT;
   constexpr int NX = (TJ * 65 + NT - 1) / NT, NY = ((TJ + 1) * 64 + NT - 1) / NT;
@@ -264,7 +272,6 @@ __global__ void __launch_bounds__(64 * TJ) trt_update(TtArgs a, unsigned ni_tile
   __shared__ double sHY[TJ + 1][64], sLY[TJ + 1][64];
   const Dims& d = a.d;
   const int KS = d.KS, KE = d.KE;
-  const unsigned lb = xcd_block(blockIdx.x, nblocks);
   const int i0 = d.IS + 64 * (int)(lb % ni_tiles), j0 = d.JS + TJ * (int)(lb / ni_tiles);
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + 64 * ty;
   const int i = i0 + tx, j = j0 + ty;
@@ -429,6 +436,25 @@ __global__ void __launch_bounds__(64 * TJ) trt_update(TtArgs a, unsigned ni_tile
   }
 }
 
+template <int S, int TJ>
+__global__ void __launch_bounds__(64 * TJ) trt_update(TtArgs a, unsigned ni_tiles, unsigned nblocks) {
+  trt_update_body<S, TJ>(a, xcd_block(blockIdx.x, nblocks), ni_tiles);
+}
+
+// both passes over the tiles of many members (batch.h tables; ni[m] the
+// member's tiles across)
+template <int KIND, int S, int TJ, bool VECT>
+__global__ void __launch_bounds__(64 * TJ) trt_batch(const TtArgs* __restrict__ tab, const unsigned* __restrict__ first,
+                                                     const unsigned* __restrict__ ni, int nmem, unsigned total) {
+  for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+    const unsigned lt = xcd_block(t, total);
+    const int m = batch_member(first, nmem, lt);
+    if constexpr (KIND == 0) trt_ratio_body<S, TJ, VECT>(tab[m], lt - first[m], ni[m]);
+    else trt_update_body<S, TJ>(tab[m], lt - first[m], ni[m]);
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 template <int S, int TJR, int TJU>
 static hipError_t trt_launch(const Ctx& c, const TtArgs& a, bool along, hipStream_t s) {
@@ -483,6 +509,63 @@ hipError_t tracer_stage_tile(const Ctx& c, const TracerIO& io, double* rjpls, do
     case CD2: return trt_launch<CD2, TT_TJR, TT_TJ>(c, a, along, s);
     default: return hipErrorNotSupported;
   }
+}
+
+// the FCT passes of every member on tiles (the members' stage-3 arguments;
+// flat terrain, one two-cell scheme, one bound kind)
+hipError_t trt_batch_build(const Ctx* const* cs, const TracerIO* ios, int n, BatchBuf& b) {
+  if (n <= 0) return hipErrorInvalidValue;
+  std::vector<TtArgs> args((size_t)n);
+  std::vector<unsigned> first((size_t)n + 1, 0), ni((size_t)n, 0);
+  const int scheme = cs[0]->scheme_tracer, along = ios[0].flag_along;
+  for (int m = 0; m < n; ++m) {
+    const Ctx& c = *cs[m];
+    const TracerIO& io = ios[m];
+    const int st = c.scheme_tracer;
+    if (c.terrain || st != scheme || io.flag_along != along || !(st == UD3 || st == CD4 || st == UD3KOREN || st == UD1 || st == CD2))
+      return hipErrorNotSupported;
+    TtArgs& a = args[(size_t)m];
+    a.d = c.d;
+    a.QTRC = io.QTRC; a.QTRC0 = io.QTRC0; a.RHOQ_t = io.RHOQ_t; a.DENS0 = io.DENS0; a.DENS = io.DENS;
+    a.mflx = io.mflx_hi; a.nd = io.num_diff; a.nds = c.d.n3;
+    a.RCDZ = c.g1.RCDZ; a.RCDX = c.g1.RCDX; a.RCDY = c.g1.RCDY;
+    a.dtl = io.dtl; a.QTRCo = io.QTRCo;
+    a.rjpls = c.scr[1]; a.rjmns = c.scr[2];
+    const Dims& d = c.d;
+    ni[(size_t)m] = (unsigned)((d.IE - d.IS + 1 + 63) / 64);
+    first[(size_t)m + 1] = first[(size_t)m] + ni[(size_t)m] * (unsigned)((d.JE - d.JS + 1 + TT_TJ - 1) / TT_TJ);
+  }
+  const hipError_t e = batch_upload(args, first, ni, b);
+  b.scheme = scheme; b.tj = along;
+  return e;
+}
+
+template <int KIND, int S>
+static hipError_t trt_batch_t(const BatchBuf& b, hipStream_t s) {
+  const TtArgs* tab;
+  const unsigned *first, *ni;
+  batch_tables(b, tab, first, ni);
+  if (KIND == 0 && b.tj)
+    hipLaunchKernelGGL((trt_batch<0, S, TT_TJ, true>), dim3(b.total), dim3(64, TT_TJ), 0, s, tab, first, ni, b.nmem, b.total);
+  else if (KIND == 0)
+    hipLaunchKernelGGL((trt_batch<0, S, TT_TJ, false>), dim3(b.total), dim3(64, TT_TJ), 0, s, tab, first, ni, b.nmem, b.total);
+  else
+    hipLaunchKernelGGL((trt_batch<1, S, TT_TJ, false>), dim3(b.total), dim3(64, TT_TJ), 0, s, tab, first, ni, b.nmem, b.total);
+  return hipGetLastError();
+}
+template <int KIND>
+static hipError_t trt_batch_k(const BatchBuf& b, hipStream_t s) {
+  switch (b.scheme) {
+    case UD3: return trt_batch_t<KIND, UD3>(b, s);
+    case UD3KOREN: return trt_batch_t<KIND, UD3KOREN>(b, s);
+    case CD2: return trt_batch_t<KIND, CD2>(b, s);
+    case UD1: return trt_batch_t<KIND, UD1>(b, s);
+    default: return trt_batch_t<KIND, CD4>(b, s);
+  }
+}
+hipError_t trt_batch_launch(const BatchBuf& b, int kind, hipStream_t s) {
+  if (!b.dev) return hipErrorInvalidValue;
+  return kind == 0 ? trt_batch_k<0>(b, s) : trt_batch_k<1>(b, s);
 }
 
 }  // namespace shp
