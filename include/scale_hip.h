@@ -292,6 +292,18 @@ int scale_hip_model_prepare(scale_hip_ctx *ctx);
 /* n short steps: numerical filter + 4 HEVI stages + RK4 combination + halos,
  * using the preparation of the last scale_hip_model_prepare */
 int scale_hip_model_short_step(scale_hip_ctx *ctx, int32_t n);
+/* Restart (checkpoint/resume) of the device-resident model: the variables of
+ * the reference's atmosphere restart, DENS, MOMZ, MOMX, MOMY, RHOT and the
+ * QA tracers (scale-rm/src/atmos/mod_atmos_vars.F90:985-1093, 3409-3483,
+ * written through FILE_CARTESC as NetCDF there), here as one binary file:
+ * 8 bytes "SCLHIPR1", int32 KA, IA, JA, QA, then each field as a FP64
+ * (KA,IA,JA) array in Fortran order with its halo.  A run restarted from a
+ * file written after step n continues bit for bit as the run that went on
+ * (the reference's restart/500m test, runA against runB1 + runB2).  Reading
+ * checks the grid and the number of tracers; FP64 models only (a model in
+ * mixed precision is rejected). */
+int scale_hip_model_restart_write(scale_hip_ctx *ctx, const char *path);
+int scale_hip_model_restart_read(scale_hip_ctx *ctx, const char *path);
 /* FP64 sum over the interior of field(k,i,j) [* DENS if weight_dens] * CDZ*CDX*CDY*GSQRT/(MAPF1*MAPF2) */
 int scale_hip_model_mass(scale_hip_ctx *ctx, int32_t field, int32_t weight_dens, double *out);
 /* device pointer of a resident field (device layout: i fastest, then j, then k) */

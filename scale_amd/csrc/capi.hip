@@ -1026,6 +1026,72 @@ int scale_hip_model_short_step(scale_hip_ctx* h, int32_t n) {
   return 0;
 }
 
+// restart files: 8 bytes of magic, int32 KA, IA, JA, QA, then DENS, MOMZ,
+// MOMX, MOMY, RHOT and QTRC(1..QA), each a FP64 (KA,IA,JA) array in Fortran
+// order, halos included
+static const char kRestartMagic[8] = {'S', 'C', 'L', 'H', 'I', 'P', 'R', '1'};
+
+int scale_hip_model_restart_write(scale_hip_ctx* h, const char* path) {
+  if (int r = check_ctx(h)) return r;
+  Ctx& c = h->c;
+  shp::Model* M = c.model;
+  if (!M) return fail("model not created");
+  if (!path) return fail("null path");
+  if (M->mixed) return fail("restart files hold the FP64 state: a model in mixed precision cannot write one");
+  const size_t n3 = c.d.n3;
+  std::vector<double> buf(n3);
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(std::string("cannot open ") + path + " for writing");
+  const int32_t hdr[4] = {c.d.KA, c.d.IA, c.d.JA, M->QA};
+  bool ok = fwrite(kRestartMagic, 1, 8, f) == 8 && fwrite(hdr, sizeof(int32_t), 4, f) == 4;
+  double* fields[5] = {M->DENS, M->MOMZ, M->MOMX, M->MOMY, M->RHOT};
+  for (int q = 0; q < 5 + M->QA && ok; ++q) {
+    const double* src = q < 5 ? fields[q] : M->QTRC + n3 * (size_t)(q - 5);
+    if (int r = download3(c, buf.data(), src, 1)) { fclose(f); return r; }
+    ok = fwrite(buf.data(), sizeof(double), n3, f) == n3;
+  }
+  ok = (fclose(f) == 0) && ok;
+  return ok ? 0 : fail(std::string("writing ") + path + " failed");
+}
+
+int scale_hip_model_restart_read(scale_hip_ctx* h, const char* path) {
+  if (int r = check_ctx(h)) return r;
+  Ctx& c = h->c;
+  shp::Model* M = c.model;
+  if (!M) return fail("model not created");
+  if (!path) return fail("null path");
+  if (M->mixed) return fail("restart files hold the FP64 state: a model in mixed precision cannot read one");
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(std::string("cannot open ") + path);
+  char magic[8];
+  int32_t hdr[4];
+  if (fread(magic, 1, 8, f) != 8 || memcmp(magic, kRestartMagic, 8) != 0 || fread(hdr, sizeof(int32_t), 4, f) != 4) {
+    fclose(f);
+    return fail(std::string(path) + " is not a restart file of this library");
+  }
+  if (hdr[0] != c.d.KA || hdr[1] != c.d.IA || hdr[2] != c.d.JA || hdr[3] != M->QA) {
+    fclose(f);
+    return fail(std::string(path) + ": grid (KA,IA,JA) = (" + std::to_string(hdr[0]) + "," + std::to_string(hdr[1]) + "," +
+                std::to_string(hdr[2]) + ") with " + std::to_string(hdr[3]) + " tracers does not match the model's (" +
+                std::to_string(c.d.KA) + "," + std::to_string(c.d.IA) + "," + std::to_string(c.d.JA) + ") with " +
+                std::to_string(M->QA));
+  }
+  const size_t n3 = c.d.n3;
+  std::vector<double> buf(n3);
+  double* fields[5] = {M->DENS, M->MOMZ, M->MOMX, M->MOMY, M->RHOT};
+  for (int q = 0; q < 5 + M->QA; ++q) {
+    if (fread(buf.data(), sizeof(double), n3, f) != n3) {
+      fclose(f);
+      return fail(std::string(path) + " is truncated");
+    }
+    double* dst = q < 5 ? fields[q] : M->QTRC + n3 * (size_t)(q - 5);
+    if (int r = upload3(c, dst, buf.data(), 1)) { fclose(f); return r; }
+  }
+  fclose(f);
+  CK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
 int scale_hip_model_mass(scale_hip_ctx* h, int32_t f, int32_t weight_dens, double* out) {
   if (int r = check_ctx(h)) return r;
   shp::Model* M = h->c.model;

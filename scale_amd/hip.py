@@ -340,6 +340,14 @@ class Context:
         _check(load_library().scale_hip_model_get(self._h, C.c_int32(fid), _p(out)))
         return out
 
+    def model_restart_write(self, path):
+        """The model's restart file (scale_hip_model_restart_write)."""
+        _check(load_library().scale_hip_model_restart_write(self._h, os.fsencode(path)))
+
+    def model_restart_read(self, path):
+        """The model's state and tracers from a restart file (scale_hip_model_restart_read)."""
+        _check(load_library().scale_hip_model_restart_read(self._h, os.fsencode(path)))
+
     def model_large_step(self, n=1):
         _check(load_library().scale_hip_model_large_step(self._h, C.c_int32(n)))
 
