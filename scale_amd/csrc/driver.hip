@@ -339,25 +339,25 @@ hipError_t divergence(const Ctx& c, double* DDIV, const double* MOMZ, const doub
 }
 
 // ---- large-step preparation (LARGE.F90:443-471), dry constants ------------------
-__global__ void large_prep(Dims d, const double* RHOT, const double* REF_pres, double* RT2P, double* DPRES0, double* REF_rhot) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int k = blockIdx.y * 4 + threadIdx.y;
-  const int j = blockIdx.z;
-  if (i >= d.IA || k >= d.KA || j >= d.JA) return;
+// rhot(kk) is RHOT at level kk of the point's column
+template <typename F>
+__device__ __forceinline__ void large_prep_point(const Dims& d, int k, int i, int j, F rhot, const double* REF_pres,
+                                                 double* RT2P, double* DPRES0, double* REF_rhot) {
   const int KS = d.KS, KE = d.KE;
   const double CVdry = CPDRY - RDRY;
   const double CPovCV = CPDRY / CVdry;
   const size_t p = IX(d, k, i, j);
   if (k >= KS && k <= KE) {
-    const double PRES = P00 * pow(RDRY * RHOT[p] / P00, CPovCV);
-    RT2P[p] = CPovCV * PRES / RHOT[p];
+    const double R = rhot(k);
+    const double PRES = P00 * pow(RDRY * R / P00, CPovCV);
+    RT2P[p] = CPovCV * PRES / R;
     DPRES0[p] = PRES - REF_pres[p];
-    REF_rhot[p] = RHOT[p];
+    REF_rhot[p] = R;
   } else if (k == KS - 1 || k == KE + 1) {
     // ghost levels from the adjacent-but-one level; recompute that PRES here
     const int kn = k == KS - 1 ? KS + 1 : KE - 1;
     const size_t q = IX(d, kn, i, j);
-    const double PRES = P00 * pow(RDRY * RHOT[q] / P00, CPovCV);
+    const double PRES = P00 * pow(RDRY * rhot(kn) / P00, CPovCV);
     const double dp = PRES - REF_pres[q];
     DPRES0[p] = k == KS - 1 ? dp + (REF_pres[q] - REF_pres[p]) : dp + (REF_pres[q] - REF_pres[p]);
     RT2P[p] = 0.0;
@@ -365,6 +365,13 @@ __global__ void large_prep(Dims d, const double* RHOT, const double* REF_pres, d
   } else {
     RT2P[p] = 0.0; DPRES0[p] = 0.0; REF_rhot[p] = 0.0;
   }
+}
+__global__ void large_prep(Dims d, const double* RHOT, const double* REF_pres, double* RT2P, double* DPRES0, double* REF_rhot) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int k = blockIdx.y * 4 + threadIdx.y;
+  const int j = blockIdx.z;
+  if (i >= d.IA || k >= d.KA || j >= d.JA) return;
+  large_prep_point(d, k, i, j, [&](int kk) { return RHOT[IX(d, kk, i, j)]; }, REF_pres, RT2P, DPRES0, REF_rhot);
 }
 
 // tendencies with DAMP_alpha = 0: X_t = X_tp on the active levels, 0 elsewhere (LARGE.F90:614-839)
@@ -696,6 +703,26 @@ hipError_t large_prep_into(Ctx& c, const double* RHOT, double* RT2P, double* DPR
 }
 
 // RT2P, DPRES0, REF_rhot, DENS00, tendencies (LARGE.F90:443-839)
+// the tendencies (LARGE.F90:614-839): without physics tendencies they stay
+// zero and are set once
+static bool tend_needed(const Model& M) {
+  const bool none = !M.tp[0] && !M.tp[1] && !M.tp[2] && !M.tp[3] && !M.tp[4];
+  return !(none && M.tend_zero);
+}
+static hipError_t model_prepare_tend(Ctx& c) {
+  Model& M = *c.model;
+  const Dims& d = c.d;
+  hipStream_t s = c.stream;
+  const bool none = !M.tp[0] && !M.tp[1] && !M.tp[2] && !M.tp[3] && !M.tp[4];
+  if (!tend_needed(M)) return hipSuccess;
+  for (int v = 0; v < 5; ++v) {
+    hipLaunchKernelGGL(tend_prep, grid_full(d), dim3(64, 4), 0, s, d, M.t[v], (const double*)M.tp[v], v == 1 ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    if (M.mixed) HIPCHK(convert(M.ft[v], (const double*)M.t[v], d.n3, s));
+  }
+  M.tend_zero = none;
+  return hipSuccess;
+}
 hipError_t model_prepare(Ctx& c) {
   Model& M = *c.model;
   const Dims& d = c.d;
@@ -705,16 +732,7 @@ hipError_t model_prepare(Ctx& c) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(M.DENS00, M.DENS, bytes, hipMemcpyDeviceToDevice, s));
   HIPCHK(fill_zero(M.mflx_av, 3 * d.n3, s));
-  // without physics tendencies the tendencies stay zero: set them once
-  const bool none = !M.tp[0] && !M.tp[1] && !M.tp[2] && !M.tp[3] && !M.tp[4];
-  if (none && M.tend_zero) return hipSuccess;
-  for (int v = 0; v < 5; ++v) {
-    hipLaunchKernelGGL(tend_prep, grid_full(d), dim3(64, 4), 0, s, d, M.t[v], (const double*)M.tp[v], v == 1 ? 1 : 0);
-    HIPCHK(hipGetLastError());
-    if (M.mixed) HIPCHK(convert(M.ft[v], (const double*)M.t[v], d.n3, s));
-  }
-  M.tend_zero = none;
-  return hipSuccess;
+  return model_prepare_tend(c);
 }
 
 // the number of short steps of a large step (LARGE.F90:402-404)
@@ -957,6 +975,94 @@ static const unsigned* point_first(const Ensemble::Tab& t) {
   return (const unsigned*)((const char*)t.dev + ((size_t)t.n * sizeof(E) + 255) / 256 * 256);
 }
 
+// the start of the large step of every member (large_begin without records,
+// the tendencies aside): in mixed precision the FP64 DENS and RHOT from the
+// FP32 state (from_dev32), then RT2P, DPRES0, REF_rhot (large_prep), DENS00 =
+// DENS and the flux sum cleared; one thread per point of the whole array
+struct BeginEnt {
+  Dims d;
+  const float *fD, *fR;  // mixed precision (else null)
+  const double* off;     // their level profiles (DENS, then RHOT)
+  double *DENS, *RHOT, *DENS00, *mflx_av;
+  const double* REF_pres;
+  double *RT2P, *DPRES0, *REF_rhot;
+};
+__global__ void __launch_bounds__(256) begin_batch(const BeginEnt* __restrict__ tab, const unsigned* __restrict__ first, int nmem) {
+  const unsigned lt = blockIdx.x;
+  const int m = batch_member(first, nmem, lt);
+  const BeginEnt& e = tab[m];
+  const Dims& d = e.d;
+  const size_t q = (size_t)(lt - first[m]) * 256 + threadIdx.x;
+  if (q >= d.n3) return;
+  const int k = (int)(q / d.pl);
+  const size_t r = q - (size_t)k * d.pl;
+  const int j = (int)(r / (size_t)d.IA), i = (int)(r - (size_t)j * d.IA);
+  double D;
+  if (e.fD) {
+    D = (double)e.fD[q] + e.off[k];
+    e.DENS[q] = D;
+    e.RHOT[q] = (double)e.fR[q] + e.off[d.KA + k];
+  } else {
+    D = e.DENS[q];
+  }
+  auto rhot = [&](int kk) {
+    const size_t p = IX(d, kk, i, j);
+    return e.fR ? (double)e.fR[p] + e.off[d.KA + kk] : e.RHOT[p];
+  };
+  large_prep_point(d, k, i, j, rhot, e.REF_pres, e.RT2P, e.DPRES0, e.REF_rhot);
+  e.DENS00[q] = D;
+  for (int c = 0; c < 3; ++c) e.mflx_av[q + d.n3 * c] = 0.0;
+}
+// the flux sum after each short step (large_after_short) of every member
+struct SumEnt {
+  Dims d;
+  double* av;
+  const float* f32;
+  const double* f64;
+};
+__global__ void __launch_bounds__(256) sum_batch(const SumEnt* __restrict__ tab, const unsigned* __restrict__ first, int nmem) {
+  const unsigned lt = blockIdx.x;
+  const int m = batch_member(first, nmem, lt);
+  const SumEnt& e = tab[m];
+  const Dims& d = e.d;
+  const int ni = d.IE - d.IS + 1, nj = d.JE - d.JS + 1;
+  const size_t t = (size_t)(lt - first[m]) * 256 + threadIdx.x;
+  if (t >= (size_t)ni * nj * (size_t)(d.KE - d.KS + 1)) return;
+  const int i = d.IS + (int)(t % (size_t)ni);
+  const int j = d.JS + (int)((t / (size_t)ni) % (size_t)nj);
+  const int k = d.KS + (int)(t / ((size_t)ni * nj));
+  const size_t p = IX(d, k, i, j);
+  for (int c = 0; c < 3; ++c) {
+    const size_t q = p + d.n3 * c;
+    e.av[q] = e.av[q] + (e.f32 ? (double)e.f32[q] : e.f64[q]);
+  }
+}
+static hipError_t ensemble_begin_build(Ensemble& E) {
+  std::vector<BeginEnt> be;
+  std::vector<unsigned> bb;
+  std::vector<SumEnt> se;
+  std::vector<unsigned> sb;
+  for (Ctx* cp : E.members) {
+    Model& M = *cp->model;
+    const Dims& d = cp->d;
+    BeginEnt b{};
+    b.d = d;
+    if (M.mixed) { b.fD = M.fS[0]; b.fR = M.fS[4]; b.off = M.foff; }
+    b.DENS = M.DENS; b.RHOT = M.RHOT; b.DENS00 = M.DENS00; b.mflx_av = M.mflx_av;
+    b.REF_pres = M.REF_pres; b.RT2P = M.RT2P; b.DPRES0 = M.DPRES0; b.REF_rhot = M.REF_rhot;
+    be.push_back(b);
+    bb.push_back((unsigned)((d.n3 + 255) / 256));
+    if (M.mixed && M.QA <= 0) continue;  // only the tracers read the sum
+    SumEnt e{d, M.mflx_av, M.mixed ? (const float*)M.fmflx : nullptr, M.mixed ? nullptr : (const double*)M.mflx_hi};
+    se.push_back(e);
+    sb.push_back((unsigned)(((size_t)(d.IE - d.IS + 1) * (d.JE - d.JS + 1) * (d.KE - d.KS + 1) + 255) / 256));
+  }
+  HIPCHK(point_table(E.begin, be, bb));
+  if (se.empty()) tab_free(E.sum);
+  else HIPCHK(point_table(E.sum, se, sb));
+  return hipSuccess;
+}
+
 // the batched tracer tables, where every member qualifies (see Ensemble)
 static hipError_t ensemble_tracers_build(Ensemble& E) {
   E.tr_batched = false;
@@ -1115,6 +1221,7 @@ static hipError_t ensemble_build_t(Ensemble& E) {
     in.push_back(f);
   }
   HIPCHK(numfilter_batch_build(in.data(), n, E.nf_tj, E.nf));
+  HIPCHK(ensemble_begin_build(E));
   return ensemble_tracers_build(E);
 }
 
@@ -1146,13 +1253,19 @@ static hipError_t ensemble_step_t(Ensemble& E) {
     }
     return hipSuccess;
   };
-  HIPCHK(fork());
-  for (Ctx* cp : E.members) {
-    // the preparation reads the FP64 DENS and RHOT
-    if (cp->model->mixed) HIPCHK(model_state_to_f64(*cp, F64_DENS | F64_RHOT));
-    HIPCHK(large_begin(*cp, nullptr));
+  // the preparation of every member (reading the FP64 DENS and RHOT, refreshed
+  // from the FP32 state in mixed precision) in one launch; the tendencies on
+  // the lanes where they are (re)computed
+  hipLaunchKernelGGL(begin_batch, dim3(E.begin.total), dim3(256), 0, s, (const BeginEnt*)E.begin.dev,
+                     point_first<BeginEnt>(E.begin), E.begin.n);
+  HIPCHK(hipGetLastError());
+  bool tend = false;
+  for (Ctx* cp : E.members) tend = tend || tend_needed(*cp->model);
+  if (tend) {
+    HIPCHK(fork());
+    for (Ctx* cp : E.members) HIPCHK(model_prepare_tend(*cp));
+    HIPCHK(join());
   }
-  HIPCHK(join());
   for (int step = 0; step < nstep; ++step) {
     // numerical filter (LARGE.F90:845-858) and its halos: one launch each
     // (a member with ND_COEF = 0 gets zeros)
@@ -1167,9 +1280,11 @@ static hipError_t ensemble_step_t(Ensemble& E) {
       HIPCHK(hevi_batch_launch(E.stage[r], E.stage_bpc, E.ncu, s));
       HIPCHK(halo_table_launch<R>(E.halo[r + 1], s));
     }
-    HIPCHK(fork());
-    for (Ctx* cp : E.members) HIPCHK(large_after_short(*cp));
-    HIPCHK(join());
+    if (E.sum.dev) {  // the flux sums of every member
+      hipLaunchKernelGGL(sum_batch, dim3(E.sum.total), dim3(256), 0, s, (const SumEnt*)E.sum.dev, point_first<SumEnt>(E.sum),
+                         E.sum.n);
+      HIPCHK(hipGetLastError());
+    }
   }
   if (!E.tr_batched) {
     HIPCHK(fork());
@@ -1262,7 +1377,7 @@ void ensemble_free(Ensemble& E) {
     for (auto& b : *v) batch_free(b);
     v->clear();
   }
-  for (auto* t : {&E.tr_mf, &E.tr_q1, &E.tr_q2, &E.tr_r, &E.tr_prep}) tab_free(*t);
+  for (auto* t : {&E.tr_mf, &E.tr_q1, &E.tr_q2, &E.tr_r, &E.tr_prep, &E.begin, &E.sum}) tab_free(*t);
   for (auto* v : {&E.tr_q, &E.tr_copy}) {
     for (auto& t : *v) tab_free(t);
     v->clear();

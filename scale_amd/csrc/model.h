@@ -77,6 +77,9 @@ struct Ensemble {
   };
   // halo tables: the filter's fields, the outputs of stages 1..3, the state
   Tab halo[5];
+  // point tables of the large step's preparation and of the flux sum after
+  // each short step, every member in one launch
+  Tab begin, sum;
   // the tracers of every member in batched launches, where the members have
   // the same number of tracers, the RK3 tracer integration, no tracer filter,
   // one FCT setting and one two-cell tracer scheme: per tracer the tables of
