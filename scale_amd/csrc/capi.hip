@@ -869,7 +869,7 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
     for (int v = 0; v < 5; ++v) M->R[r][v] = A(n3);
     M->mfR[r] = A(3 * n3); M->tfR[r] = A(3 * n3);
   }
-  M->ndq = A(3 * n3); M->DENS_RK = A(n3); M->QRK1 = A(n3); M->QRK2 = A(n3); M->zeros = A(n3);
+  M->ndq = A(3 * n3); M->DENS_RK = A(n3); M->QRK1 = A(n3); M->QRK2 = A(n3);
   if (QA > 0 && cfg->tinteg_tracer == SCALE_HIP_TRACER_RK3WS2002) M->DENS_RK2 = A(n3);
   if (cfg->mixed_precision) {
     M->mixed = 1;
@@ -893,7 +893,6 @@ int scale_hip_model_create(scale_hip_ctx* h, const scale_hip_dyn_config* cfg, in
   CK(hipMemsetAsync(M->mflx_hi, 0, 3 * n3 * sizeof(double), c.stream));
   CK(hipMemsetAsync(M->tflx_hi, 0, 3 * n3 * sizeof(double), c.stream));
   CK(hipMemsetAsync(M->ndq, 0, 3 * n3 * sizeof(double), c.stream));
-  CK(hipMemsetAsync(M->zeros, 0, n3 * sizeof(double), c.stream));
   CK(hipMemsetAsync(M->DDIV, 0, n3 * sizeof(double), c.stream));
   CK(hipMemsetAsync(M->CORIOLI, 0, c.d.pl * sizeof(double), c.stream));
   CK(hipMemsetAsync(M->wdamp_coef, 0, (size_t)c.d.KA * sizeof(double), c.stream));

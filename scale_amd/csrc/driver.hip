@@ -796,11 +796,14 @@ static hipError_t large_tracers(Ctx& c, int nstep) {
   }
   for (int iq = 0; iq < M.QA; ++iq) {
     double* q = M.QTRC + d.n3 * (size_t)iq;
-    const double* rq = M.RHOQ_tp ? M.RHOQ_tp + d.n3 * (size_t)iq : M.zeros;
-    // tracer filter from DENS00 (LARGE.F90:1049-1061); with ND_COEF_Q = 0 num_diff_q stays zero
+    // zero sources and, with ND_COEF_Q = 0, zero num_diff_q go in as null
+    // arrays: the stages add 0.0 without loading it
+    const double* rq = M.RHOQ_tp ? M.RHOQ_tp + d.n3 * (size_t)iq : nullptr;
+    // tracer filter from DENS00 (LARGE.F90:1049-1061)
     if (M.nd_coef_q != 0.0) HIPCHK(numfilter_q(c, M.ndq, M.DENS00, q, nullptr, s));
     TracerIO io;
-    io.RHOQ_t = rq; io.DENS0 = M.DENS00; io.mflx_hi = M.mflx_hi; io.num_diff = M.ndq; io.flag_along = M.fct_along;
+    io.RHOQ_t = rq; io.DENS0 = M.DENS00; io.mflx_hi = M.mflx_hi; io.num_diff = M.nd_coef_q != 0.0 ? M.ndq : nullptr;
+    io.flag_along = M.fct_along;
     if (M.tinteg_tracer == 1) {
       // EULER (tinteg_tracer_euler.F90:80-133): one stage of dtl with the
       // end-of-step density, FCT as configured; written to QRK1 then copied
@@ -1099,9 +1102,9 @@ static hipError_t ensemble_tracers_build(Ensemble& E) {
       Model& M = *cp->model;
       const Dims& d = cp->d;
       double* q = M.QTRC + d.n3 * (size_t)iq;
-      const double* rq = M.RHOQ_tp ? M.RHOQ_tp + d.n3 * (size_t)iq : M.zeros;
-      TracerIO io{};
-      io.RHOQ_t = rq; io.DENS0 = M.DENS00; io.mflx_hi = M.mflx_hi; io.num_diff = M.ndq; io.flag_along = M.fct_along;
+      const double* rq = M.RHOQ_tp ? M.RHOQ_tp + d.n3 * (size_t)iq : nullptr;
+      TracerIO io{};  // no tracer filter in the batched case: num_diff_q is zero (null)
+      io.RHOQ_t = rq; io.DENS0 = M.DENS00; io.mflx_hi = M.mflx_hi; io.num_diff = nullptr; io.flag_along = M.fct_along;
       io.QTRCo = M.QRK1; io.QTRC = q; io.QTRC0 = q; io.DENS = M.DENS_RK; io.dtl = M.dtl / 3.0; io.flag_fct = 0;
       s1.push_back(io);
       io.QTRCo = M.QRK2; io.QTRC = M.QRK1; io.DENS = M.DENS_RK2; io.dtl = M.dtl / 2.0;

@@ -25,7 +25,7 @@ struct Model {
   double* S0[5] = {};
   double* R[4][5] = {};
   double *mfR[4] = {}, *tfR[4] = {};
-  double *ndq = nullptr, *DENS_RK = nullptr, *QRK1 = nullptr, *QRK2 = nullptr, *zeros = nullptr;
+  double *ndq = nullptr, *DENS_RK = nullptr, *QRK1 = nullptr, *QRK2 = nullptr;
   double* DENS_RK2 = nullptr;  // the second RK3 tracer stage's density (DENS_RK holds the first's)
   // mixed precision: FP32 state, stage, tendency, filter and flux arrays
   // (the FP64 arrays above keep the reference state, the pressure terms and

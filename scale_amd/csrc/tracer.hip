@@ -50,6 +50,11 @@ struct TrOps {
   __device__ __forceinline__ double m2(int i, int j) const {
     if constexpr (TERRAIN) return a.MAPF_XY[(size_t)i + (size_t)d.IA * j + d.pl]; else return 1.0;
   }
+  // num_diff and the source term: a null array stands for zeros (the model
+  // without a tracer filter or tracer tendencies), added as the literal 0.0,
+  // which gives the same sums as zeros loaded from memory
+  __device__ __forceinline__ double ndv(size_t q) const { return a.nd ? a.nd[q] : 0.0; }
+  __device__ __forceinline__ double rqv(size_t q) const { return a.RHOQ_t ? a.RHOQ_t[q] : 0.0; }
   __device__ __forceinline__ double mZ(size_t p) const { return a.mflx[p]; }
   __device__ __forceinline__ double mX(size_t p) const { return a.mflx[p + d.n3]; }
   __device__ __forceinline__ double mY(size_t p) const { return a.mflx[p + 2 * d.n3]; }
@@ -71,13 +76,13 @@ struct TrOps {
       else if (kk == d.KE - 1) { col(d.KE - 2, 3); fv = fv6_ke1<S>(v, vel); }
       else if (kk == d.KS) { col(d.KS, 3); fv = fv6_ks<S>(v, vel); }
       else fv = fv6_int<S>(Q[p - 2 * sk], Q[p - sk], Q[p], Q[p + sk], Q[p + 2 * sk], Q[p + 3 * sk], vel);
-      return vel * fv + gs(I_XYW, p) * a.nd[p];
+      return vel * fv + gs(I_XYW, p) * ndv(p);
     }
     if (kk == d.KS) fv = fv_lo<S>(Q[p], Q[ix(kk + 1, i, j)], Q[ix(kk + 2, i, j)], vel);
     else if (kk == d.KE - 1) fv = fv_hi<S>(Q[ix(kk - 1, i, j)], Q[p], Q[ix(kk + 1, i, j)], vel);
     else fv = fv_int<S>(Q[ix(kk - 1, i, j)], Q[p], Q[ix(kk + 1, i, j)], Q[ix(kk + 2, i, j)], vel);
     if constexpr (S == UD1) return vel * fv;  // the UD1 routines take no num_diff (UD1.F90:135-249)
-    return vel * fv + gs(I_XYW, p) * a.nd[p];
+    return vel * fv + gs(I_XYW, p) * ndv(p);
   }
   __device__ double hX(int k, int ii, int j) const {
     const double* Q = a.QTRC;
@@ -86,8 +91,8 @@ struct TrOps {
     if constexpr (S == UD1) return vel * fv_int<UD1>(0.0, Q[p], Q[ix(k, ii + 1, j)], 0.0, vel);
     if constexpr (S == UD5 || S == CD6)
       return vel * fv6_int<S>(Q[ix(k, ii - 2, j)], Q[ix(k, ii - 1, j)], Q[p], Q[ix(k, ii + 1, j)], Q[ix(k, ii + 2, j)],
-                              Q[ix(k, ii + 3, j)], vel) + gs(I_UYZ, p) * a.nd[p + a.nds];
-    return vel * fv_int<S>(Q[ix(k, ii - 1, j)], Q[p], Q[ix(k, ii + 1, j)], Q[ix(k, ii + 2, j)], vel) + gs(I_UYZ, p) * a.nd[p + a.nds];
+                              Q[ix(k, ii + 3, j)], vel) + gs(I_UYZ, p) * ndv(p + a.nds);
+    return vel * fv_int<S>(Q[ix(k, ii - 1, j)], Q[p], Q[ix(k, ii + 1, j)], Q[ix(k, ii + 2, j)], vel) + gs(I_UYZ, p) * ndv(p + a.nds);
   }
   __device__ double hY(int k, int i, int jj) const {
     const double* Q = a.QTRC;
@@ -96,8 +101,8 @@ struct TrOps {
     if constexpr (S == UD1) return vel * fv_int<UD1>(0.0, Q[p], Q[ix(k, i, jj + 1)], 0.0, vel);
     if constexpr (S == UD5 || S == CD6)
       return vel * fv6_int<S>(Q[ix(k, i, jj - 2)], Q[ix(k, i, jj - 1)], Q[p], Q[ix(k, i, jj + 1)], Q[ix(k, i, jj + 2)],
-                              Q[ix(k, i, jj + 3)], vel) + gs(I_XVZ, p) * a.nd[p + 2 * a.nds];
-    return vel * fv_int<S>(Q[ix(k, i, jj - 1)], Q[p], Q[ix(k, i, jj + 1)], Q[ix(k, i, jj + 2)], vel) + gs(I_XVZ, p) * a.nd[p + 2 * a.nds];
+                              Q[ix(k, i, jj + 3)], vel) + gs(I_XVZ, p) * ndv(p + 2 * a.nds);
+    return vel * fv_int<S>(Q[ix(k, i, jj - 1)], Q[p], Q[ix(k, i, jj + 1)], Q[ix(k, i, jj + 2)], vel) + gs(I_XVZ, p) * ndv(p + 2 * a.nds);
   }
   // low-order UD1 fluxes on QTRC0 (scale_atmos_dyn_fvm_flux_ud1.F90:135-249)
   __device__ double lZ(int kk, int i, int j) const {
@@ -147,7 +152,7 @@ __device__ __forceinline__ void tracer_nofct_body(const TrArgs& a, unsigned lb, 
                              + (o.hX(k, i, j) - o.hX(k, i - 1, j)) * a.RCDX[i]
                              + (o.hY(k, i, j) - o.hY(k, i, j - 1)) * a.RCDY[j])
                            * o.m1(i, j) * o.m2(i, j) / o.gs(I_XYZ, p)
-                           + a.RHOQ_t[p]))
+                           + o.rqv(p)))
              / a.DENS[p];
 }
 template <int S, bool TERRAIN>
@@ -313,7 +318,7 @@ __device__ __forceinline__ void fct_update_body(const TrArgs& a, unsigned lb, un
                              + (o.hX(k, i, j) - limX(i) - o.hX(k, i - 1, j) + limX(i - 1)) * a.RCDX[i]
                              + (o.hY(k, i, j) - limY(j) - o.hY(k, i, j - 1) + limY(j - 1)) * a.RCDY[j])
                            * o.m1(i, j) * o.m2(i, j) / o.gs(I_XYZ, p)
-                           + a.RHOQ_t[p]))
+                           + o.rqv(p)))
              / a.DENS[p];
 }
 template <int S, bool TERRAIN, bool THETA>

@@ -24,7 +24,7 @@
 
 namespace shp {
 
-struct TtArgs {
+struct TtArgs {  // nd and RHOQ_t may be null: zeros (as in TrArgs)
   Dims d;
   const double *QTRC, *QTRC0, *RHOQ_t, *DENS0, *DENS, *mflx, *nd;
   size_t nds;  // distance between the z, x and y components of nd
@@ -144,7 +144,7 @@ __device__ __forceinline__ void trt_ratio_body(const TtArgs& a, unsigned lb, uns
           const size_t p = pk + (size_t)ii + (size_t)d.IA * jj;
           const int c = f + 1, rr = r + 2;
           const double vel = mX[p];
-          sAX[r][f] = tt_hflux<S>(vel, sQ[rr][c - 1], sQ[rr][c], sQ[rr][c + 1], sQ[rr][c + 2], a.nd[p + a.nds])
+          sAX[r][f] = tt_hflux<S>(vel, sQ[rr][c - 1], sQ[rr][c], sQ[rr][c + 1], sQ[rr][c + 2], (a.nd ? a.nd[p + a.nds] : 0.0))
                     - tt_lflux(vel, Q0[rr][c], Q0[rr][c + 1]);
         }
       }
@@ -155,7 +155,7 @@ __device__ __forceinline__ void trt_ratio_body(const TtArgs& a, unsigned lb, uns
           const size_t p = pk + (size_t)ii + (size_t)d.IA * jj;
           const int c = c0 + 2, rr = r + 1;
           const double vel = mY[p];
-          sAY[r][c0] = tt_hflux<S>(vel, sQ[rr - 1][c], sQ[rr][c], sQ[rr + 1][c], sQ[rr + 2][c], a.nd[p + 2 * a.nds])
+          sAY[r][c0] = tt_hflux<S>(vel, sQ[rr - 1][c], sQ[rr][c], sQ[rr + 1][c], sQ[rr + 2][c], (a.nd ? a.nd[p + 2 * a.nds] : 0.0))
                      - tt_lflux(vel, Q0[rr][c], Q0[rr + 1][c]);
         }
       }
@@ -169,7 +169,7 @@ __device__ __forceinline__ void trt_ratio_body(const TtArgs& a, unsigned lb, uns
       double az = 0.0;
       if (k != KE) {
         const double vel = mZ[p];
-        az = tt_hz<S>(d, k, vel, w, a.nd[p]) - tt_lflux(vel, Q0k[r2][c2], sQ0[(k + 1) % 5][r2][c2]);
+        az = tt_hz<S>(d, k, vel, w, a.nd ? a.nd[p] : 0.0) - tt_lflux(vel, Q0k[r2][c2], sQ0[(k + 1) % 5][r2][c2]);
       }
       const double ax = sAX[ty][tx + 1], axm = sAX[ty][tx];
       const double ay = sAY[ty + 1][tx], aym = sAY[ty][tx];
@@ -319,7 +319,7 @@ __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, un
       const int ii = i0 - 1 + f, jj = j0 + r;
       const bool ok = t < TJ * 65 && ii <= d.IE && jj <= d.JE;
       const size_t p = ok ? pk + (size_t)ii + (size_t)d.IA * jj : 0;
-      const double vm = mX[p], vn = a.nd[p + a.nds];
+      const double vm = mX[p], vn = a.nd ? a.nd[p + a.nds] : 0.0;
       pXm[h] = ok ? vm : 0.0;
       pXn[h] = ok ? vn : 0.0;
     }
@@ -330,12 +330,12 @@ __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, un
       const int ii = i0 + c0, jj = j0 - 1 + r;
       const bool ok = t < (TJ + 1) * 64 && ii <= d.IE && jj <= d.JE;
       const size_t p = ok ? pk + (size_t)ii + (size_t)d.IA * jj : 0;
-      const double vm = mY[p], vn = a.nd[p + 2 * a.nds];
+      const double vm = mY[p], vn = a.nd ? a.nd[p + 2 * a.nds] : 0.0;
       pYm[h] = ok ? vm : 0.0;
       pYn[h] = ok ? vn : 0.0;
     }
     const size_t p = pk + pc;
-    const double vz = mZ[p], vn = a.nd[p], v0 = a.DENS0[p], vq = a.RHOQ_t[p], vd = a.DENS[p];
+    const double vz = mZ[p], vn = a.nd ? a.nd[p] : 0.0, v0 = a.DENS0[p], vq = a.RHOQ_t ? a.RHOQ_t[p] : 0.0, vd = a.DENS[p];
     pZ = (k != KE) ? vz : 0.0;
     pN = (k != KE) ? vn : 0.0;
     pD0 = v0; pRQ = vq; pD = vd;
