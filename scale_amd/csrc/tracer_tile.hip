@@ -480,7 +480,7 @@ static hipError_t trt_launch(const Ctx& c, const TtArgs& a, bool along, hipStrea
 // (two each, the LDS limit); smaller grids keep the per-point kernels, whose
 // launches spread over k as well.
 constexpr int TT_TJ = 8;   // rows of the update tile (and of the size criterion)
-constexpr int TT_TJR = 8;   // rows of the ratio tile (16 rows: 1.94 ms against 1.71 at 512x512x128)
+constexpr int TT_TJR = 8;   // rows of the ratio tile (16 rows: 1.94 ms, 4 rows: 2.39, against 1.53 at 8 rows, 512x512x128)
 bool tracer_tile_supported(const Ctx& c) {
   const int st = c.scheme_tracer;
   if (c.terrain || !(st == UD3 || st == CD4 || st == UD3KOREN || st == UD1 || st == CD2)) return false;
