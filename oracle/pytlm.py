@@ -66,6 +66,53 @@ def dptr(a):
     return a.ctypes.data_as(_dp)
 
 
+def _metric_dual(m, keep):
+    """The oracle metric struct on dual numbers (zero tangents)."""
+    ms = OrcMetricD()
+    for n in ("CDZ", "FDZ", "RCDZ", "RFDZ", "CDX", "FDX", "RCDX", "RFDX", "CDY", "FDY", "RCDY", "RFDY",
+              "PHI", "GSQRT", "J13G", "J23G", "MAPF"):
+        a = dual(m[n])
+        keep.append(a)
+        setattr(ms, n, dptr(a))
+    ms.J33G = DualC(float(m["J33G"]), 0.0)
+    return ms
+
+
+STAGE_OUT = ("DENS_RK", "MOMZ_RK", "MOMX_RK", "MOMY_RK", "RHOT_RK")
+
+
+def stage_tangent(case, inp: dict, dtrk: float, tangents=None, scheme=None):
+    """One HEVI stage (orc_tstep_short_hevi) on dual numbers.  inp as for
+    pyoracle.tstep_short_hevi; tangents maps input keys (DENS0..RHOT0,
+    DENS..RHOT, DPRES0, RT2P, num_diff, REF_rhot, ...) to directions (absent
+    keys: zero).  The metrics are case.metric, held fixed.  Returns
+    ({DENS_RK..RHOT_RK: value}, {DENS_RK..RHOT_RK: tangent})."""
+    g = case.grid
+    t = tangents or {}
+    keep = []
+
+    def d(key, v=None):
+        a = dual(inp[key] if v is None else v, t.get(key))
+        keep.append(a)
+        return dptr(a)
+
+    ms = _metric_dual(case.metric, keep)
+    outs = {k: dual(np.full(g.shape, np.nan)) for k in STAGE_OUT}
+    mf, tf = dual(np.full(g.shape + (3,), np.nan)), dual(np.full(g.shape + (3,), np.nan))
+    ins = ("DENS0", "MOMZ0", "MOMX0", "MOMY0", "RHOT0", "DENS", "MOMZ", "MOMX", "MOMY", "RHOT",
+           "DENS_t", "MOMZ_t", "MOMX_t", "MOMY_t", "RHOT_t", "DPRES0", "RT2P")
+    rc = lib().orc_tstep_short_hevi(
+        C.byref(po.grid_struct(g)), C.c_int(scheme if scheme is not None else case.scheme),
+        *[dptr(outs[k]) for k in STAGE_OUT], dptr(mf), dptr(tf), *[d(k) for k in ins],
+        d("CORIOLI", inp.get("CORIOLI", case.corioli)), d("num_diff"),
+        d("wdamp_coef", inp.get("wdamp_coef", case.wdamp_coef)), DualC(float(inp.get("divdmp_coef", 0.0)), 0.0),
+        d("DDIV"), C.byref(ms), d("REF_dens"), d("REF_rhot"), DualC(float(dtrk), 0.0))
+    if rc != 0:
+        raise RuntimeError("orc_tstep_short_hevi (dual) failed")
+    return ({k: np.asfortranarray(outs[k][0]) for k in STAGE_OUT},
+            {k: np.asfortranarray(outs[k][1]) for k in STAGE_OUT})
+
+
 class TangentModel:
     """The oracle's Model (po.Model) on dual numbers: step() advances the
     state and its tangent together."""
@@ -84,14 +131,7 @@ class TangentModel:
         self.qa = qa
         q0 = case.qtrc if qa > 0 else np.zeros(g.shape + (1,))
         self.q = dual(q0, dqtrc)
-        m = case.metric
-        self.ms = OrcMetricD()
-        for n in ("CDZ", "FDZ", "RCDZ", "RFDZ", "CDX", "FDX", "RCDX", "RFDX", "CDY", "FDY", "RCDY", "RFDY",
-                  "PHI", "GSQRT", "J13G", "J23G", "MAPF"):
-            a = dual(m[n])
-            self._keep.append(a)
-            setattr(self.ms, n, dptr(a))
-        self.ms.J33G = DualC(float(m["J33G"]), 0.0)
+        self.ms = _metric_dual(case.metric, self._keep)
         st = OrcStateD()
         for k in po.STATE:
             setattr(st, k, dptr(self.x[k]))

@@ -1,5 +1,5 @@
-// adjoint.hip -- reverse-mode derivative (adjoint) of the flat-terrain HEVI
-// stage, the numerical filter and the large step, with checkpointing.
+// adjoint.hip -- reverse-mode derivative (adjoint) of the HEVI stage, the
+// numerical filter and the large step, with checkpointing.
 //
 // The reference has no adjoint.  This is the transpose of the Jacobian of the
 // forward this library computes (ATMOS_DYN_Tstep_short_fvm_hevi,
@@ -10,7 +10,8 @@
 // derived statement by statement.  The upwind switch sign(vel) is held fixed
 // (the derivative wherever no face velocity is exactly zero).
 //
-// Scope: flat terrain (GSQRT = 1, J13G = J23G = 0, MAPF = 1, J33G = 1), UD3
+// Scope: flat or terrain-following grids (GSQRT, J13G, J23G, J33G and MAPF
+// as the context holds them; the metrics are constants of the adjoint), UD3
 // or CD4, FP64, divdmp_coef = 0, one short step per large step, periodic
 // lateral boundaries.  Tracers are passive (they never feed back into the
 // dynamics), so the adjoint of the dynamics ignores them.
@@ -121,7 +122,7 @@ enum {
   W_O = 0,                 // 5 traced stage outputs
   W_PT = 5, W_ST, W_CP, W_POTT,
   W_POTTB, W_STB, W_SRB, W_SWB, W_DPRESB,
-  W_QZ, W_QX, W_QY, W_TZ, W_TX, W_TY, W_MZ, W_MX, W_MY,
+  W_QZ, W_QX, W_QY, W_Q13, W_Q23, W_TZ, W_TX, W_TY, W_MZ, W_MX, W_MY,
   W_CPB, W_PTB, W_AB, W_CB, W_CPV, W_DPV,
   W_FLD, W_FB, W_W0, W_W1, W_W2, W_WB0, W_WB1, W_WB2,
   NW
@@ -259,11 +260,34 @@ hipError_t adj_setup(Ctx& c, int nsteps, int every) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage adjoint (reverse of HEVI.F90:128-1089 for flat terrain)
+// Stage adjoint (reverse of HEVI.F90:128-1089).  TER = false is the flat
+// grid: the metric accessors return the constants and the terrain-only loop
+// nests (the J13/J23 fluxes) drop out.
 
+namespace {
+// F2H(k, pp) of a column with the GSQRT component cG (UD3.F90:75 and the
+// F2H of HEVI.F90): CDZ(k+pp-1) G(k+pp-1) / (CDZ(k) G(k) + CDZ(k+1) G(k+1))
+template <bool TER>
+__device__ __forceinline__ double f2h(const Met<TER>& m, const Grid1D& g, const Dims& d, int cG, int k, int pp, int i, int j) {
+  const size_t p = IX(d, k, i, j);
+  return g.CDZ[k + pp - 1] * m.gs(cG, p + d.pl * (size_t)(pp - 1)) / (g.CDZ[k] * m.gs(cG, p) + g.CDZ[k + 1] * m.gs(cG, p + d.pl));
+}
 
+// the vertical mass flux before the solve at (k, i, j) (HEVI.F90:369-392)
+template <bool TER>
+__device__ __forceinline__ double mflx_z0(const Met<TER>& m, const Dims& d, const double* MX, const double* MY, const double* ND,
+                                          int k, int i, int j) {
+  const size_t p = IX(d, k, i, j), pl = d.pl, IA = d.IA;
+  const double M0 = m.mapf(i, j, 0, I_XY), M1 = m.mapf(i, j, 1, I_XY);
+  const double gz = m.gs(I_XYW, p) / (M0 * M1) * ND[p];
+  if constexpr (TER)
+    return m.j13(I_XYW, p) * 0.25 / M1 * (MX[p + pl] + MX[p + pl - 1] + MX[p] + MX[p - 1])
+         + m.j23(I_XYW, p) * 0.25 / M0 * (MY[p + pl] + MY[p + pl - IA] + MY[p] + MY[p - IA]) + gz;
+  else return gz;
+}
+}  // namespace
 
-template <int S>
+template <int S, bool TER>
 static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   const Dims d = c.d;
   const Grid1D g = c.g1;
@@ -271,7 +295,9 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   double* const* w = A.w;
   const size_t n3 = d.n3, pl = d.pl;
   const int IA = d.IA, KS = d.KS, KE = d.KE, IS = d.IS, IE = d.IE, JS = d.JS, JE = d.JE;
-  const double dt = io.dtrk;
+  const double dt = io.dtrk, J33G = c.J33G;
+  const Met<TER> m{c.GSQRT, c.J13G, c.J23G, c.MAPF, n3, pl, IA};
+  const double* PHI = c.PHI;
 
   // forward stage with PT, St and C recorded (unless the caller has them)
   if (!io.trPT) {
@@ -291,10 +317,10 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   const double *PT = io.trPT ? io.trPT : w[W_PT], *St = io.trSt ? io.trSt : w[W_ST], *Cp = io.trCp ? io.trCp : w[W_CP];
   double *POTT = w[W_POTT];
   double *POTTb = w[W_POTTB], *Stb = w[W_STB], *Srb = w[W_SRB], *Swb = w[W_SWB], *DPRESb = w[W_DPRESB];
-  double *qZb = w[W_QZ], *qXb = w[W_QX], *qYb = w[W_QY];
+  double *qZb = w[W_QZ], *qXb = w[W_QX], *qYb = w[W_QY], *q13b = w[W_Q13], *q23b = w[W_Q23];
   double *tZb = w[W_TZ], *tXb = w[W_TX], *tYb = w[W_TY];
   double *mZb = w[W_MZ], *mXb = w[W_MX], *mYb = w[W_MY];
-  double *Cpb = w[W_CPB], *PTb = w[W_PTB], *Ab = w[W_AB], *Cb = w[W_CB], *cpv = w[W_CPV], *dpv = w[W_DPV];
+  double *Cpb = w[W_CPB], *PTb = w[W_PTB], *Cb = w[W_CB], *cpv = w[W_CPV], *dpv = w[W_DPV];
   double *Db = io.sb[0], *MZb = io.sb[1], *MXb = io.sb[2], *MYb = io.sb[3], *RTb = io.sb[4];
   double *D0b = io.s0b[0], *MZ0b = io.s0b[1], *MX0b = io.s0b[2], *MY0b = io.s0b[3], *RT0b = io.s0b[4];
   double *NDb = io.NDb, *DPRES0b = io.DPRES0b, *RT2Pb = io.RT2Pb, *REFrtb = io.REFrtb;
@@ -314,43 +340,118 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     POTT[p] = RT[p] / D[p];
   }, st));
 
+  // The J13/J23 fluxes of the horizontal momenta (UD3.F90:849-1039,
+  // 1269-1459), terrain only: flux(k) = J vel / mp fv(val, J vel) with
+  // vel = (f1 mom(k+1) + f2 mom(k)) / (f1 <D>(k+1) + f2 <D>(k)); mom is the
+  // 4-point mean (quad) where it lives at the other horizontal staggering.
+  // stag 0: the UYZ family (D averaged over i, i+1), 1: XVZ (over j, j+1).
+  auto fluxj_uvz = [&](int stag, int jd, const double* val, double* valb, const double* qb) -> hipError_t {
+    const bool quad = stag != jd;
+    const double* mom = jd == 0 ? MX : MY;
+    double* momb = jd == 0 ? MXb : MYb;
+    const ptrdiff_t sd = stag == 0 ? 1 : IA;
+    const ptrdiff_t oa = stag == 0 ? 1 : -1, ob = stag == 0 ? -(ptrdiff_t)IA : (ptrdiff_t)IA;
+    const int cG = stag == 0 ? I_UYZ : I_XVZ, cJ = stag == 0 ? I_UYW : I_XVW, cM = stag == 0 ? I_UY : I_XV;
+    const int mn = jd == 0 ? 1 : 0;
+    return for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+      const size_t p = IX(d, k, i, j);
+      const double f1 = f2h(m, g, d, cG, k, 1, i, j), f2 = f2h(m, g, d, cG, k, 2, i, j);
+      auto mm = [&](size_t q) {
+        const double* r = mom + q;
+        return quad ? 0.25 * (r[0] + r[oa] + r[ob] + r[oa + ob]) : r[0];
+      };
+      const double num = f1 * mm(p + pl) + f2 * mm(p);
+      const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + sd]) + f2 * 0.5 * (D[p] + D[p + sd]);
+      const double vr = num / den;
+      const double Jm = jd == 0 ? m.j13(cJ, p) : m.j23(cJ, p);
+      const double vel = vr * Jm, mp = m.mapf(i, j, mn, cM);
+      const double fb = qb[p];
+      const double velb = fb * face_adj<S>(zkind(k, KS, KE), val, valb, p, pl, vel, fb * vel / mp) / mp;
+      const double vrb = velb * Jm;
+      const double numb = vrb / den, denb = -vrb * vr / den;
+      auto mmb = [&](size_t q, double x) {
+        double* r = momb + q;
+        if (quad) { const double y = 0.25 * x; aadd(r, y); aadd(r + oa, y); aadd(r + ob, y); aadd(r + oa + ob, y); }
+        else aadd(r, x);
+      };
+      mmb(p + pl, f1 * numb); mmb(p, f2 * numb);
+      aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + sd, 0.5 * f1 * denb);
+      aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + sd, 0.5 * f2 * denb);
+    }, st);
+  };
+
   // ---- momentum y (HEVI.F90:936-1047) ----
   HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double b = obMY[p];
     aadd(MY0b + p, b);
     const double T = dt * b;
-    aadd(qZb + p, -T * g.RCDZ[k]); aadd(qZb + p - pl, T * g.RCDZ[k]);
-    aadd(qXb + p, -T * g.RCDX[i]); aadd(qXb + p - 1, T * g.RCDX[i]);
-    aadd(qYb + p, -T * g.RFDY[j]); aadd(qYb + p - IA, T * g.RFDY[j]);
-    aadd(DPRESb + p + IA, -T * g.RFDY[j]); aadd(DPRESb + p, T * g.RFDY[j]);
+    const double u = T / m.gs(I_XVZ, p);
+    aadd(qZb + p, -u * g.RCDZ[k]); aadd(qZb + p - pl, u * g.RCDZ[k]);
+    if constexpr (TER) {
+      aadd(q13b + p, -u * g.RCDZ[k]); aadd(q13b + p - pl, u * g.RCDZ[k]);
+      aadd(q23b + p, -u * g.RCDZ[k]); aadd(q23b + p - pl, u * g.RCDZ[k]);
+    }
+    const double um = u * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV);
+    aadd(qXb + p, -um * g.RCDX[i]); aadd(qXb + p - 1, um * g.RCDX[i]);
+    aadd(qYb + p, -um * g.RFDY[j]); aadd(qYb + p - IA, um * g.RFDY[j]);
+    // pg: the bracket times MAPF(i,j,2,I_XV)
+    const double v = -u * m.mapf(i, j, 1, I_XV);
+    aadd(DPRESb + p + IA, v * g.RFDY[j] * m.gs(I_XYZ, p + IA)); aadd(DPRESb + p, -v * g.RFDY[j] * m.gs(I_XYZ, p));
+    if constexpr (TER) {
+      const double wv = v * g.RCDZ[k] * 0.5;
+      const double a1 = wv * m.j23(I_XVW, p), a2 = wv * m.j23(I_XVW, p - pl);
+      const double f1k = f2h(m, g, d, I_XVZ, k, 1, i, j), f2k = f2h(m, g, d, I_XVZ, k, 2, i, j);
+      const double f1m = f2h(m, g, d, I_XVZ, k - 1, 1, i, j), f2m = f2h(m, g, d, I_XVZ, k - 1, 2, i, j);
+      aadd(DPRESb + p + pl + IA, a1 * f1k); aadd(DPRESb + p + pl, a1 * f1k);
+      aadd(DPRESb + p + IA, a1 * f2k - a2 * f1m); aadd(DPRESb + p, a1 * f2k - a2 * f1m);
+      aadd(DPRESb + p - pl + IA, -a2 * f2m); aadd(DPRESb + p - pl, -a2 * f2m);
+    }
     if (cor) {
       const double cc = -0.125 * (COR[(size_t)i + (size_t)IA * (j + 1)] + COR[(size_t)i + (size_t)IA * j]) * T;
       aadd(MXb + p + IA, cc); aadd(MXb + p, cc); aadd(MXb + p - 1 + IA, cc); aadd(MXb + p - 1, cc);
+    }
+    if constexpr (TER) {
+      // the map-factor term of cf, -c0 Xs (MOMY a - Xs/4 b) 2/(D(j+1) + D(j))
+      const double Xs = MX[p] + MX[p - 1] + MX[p + IA] + MX[p - 1 + IA];
+      const double c0 = 0.25 * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV);
+      const double aa = (1.0 / m.mapf(i, j, 1, I_UV) - 1.0 / m.mapf(i - 1, j, 1, I_UV)) * g.RCDX[i];
+      const double bb = (1.0 / m.mapf(i, j + 1, 0, I_XY) - 1.0 / m.mapf(i, j, 0, I_XY)) * g.RFDY[j];
+      const double Dd = D[p + IA] + D[p], rD = 2.0 / Dd;
+      const double xsb = -T * c0 * (MY[p] * aa - 0.5 * Xs * bb) * rD;
+      aadd(MXb + p, xsb); aadd(MXb + p - 1, xsb); aadd(MXb + p + IA, xsb); aadd(MXb + p - 1 + IA, xsb);
+      aadd(MYb + p, -T * c0 * Xs * aa * rD);
+      const double cf2 = -c0 * Xs * (MY[p] * aa - 0.25 * Xs * bb) * rD;
+      aadd(Db + p + IA, -T * cf2 / Dd); aadd(Db + p, -T * cf2 / Dd);
     }
   }, st));
   // fluxZ_XVZ (UD3.F90:1173-1265)
   HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double f1 = g.CDZ[k] / (g.CDZ[k] + g.CDZ[k + 1]), f2 = g.CDZ[k + 1] / (g.CDZ[k] + g.CDZ[k + 1]);
+    const double f1 = f2h(m, g, d, I_XVW, k, 1, i, j), f2 = f2h(m, g, d, I_XVW, k, 2, i, j);
     const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + IA]) + f2 * 0.5 * (D[p] + D[p + IA]);
     const double vel = 0.5 * (MZ[p] + MZ[p + IA]) / den;
     const double fb = qZb[p];
-    aadd(NDb + oVZ + p, fb);
-    const double velb = fb * face_adj<S>(zkind(k, KS, KE), MY, MYb, p, pl, vel, fb * vel);
+    aadd(NDb + oVZ + p, fb * m.gs(I_XVW, p));
+    const double velb = J33G * fb * face_adj<S>(zkind(k, KS, KE), MY, MYb, p, pl, vel, J33G * fb * vel);
     aadd(MZb + p, 0.5 * velb / den); aadd(MZb + p + IA, 0.5 * velb / den);
     const double denb = -velb * vel / den;
     aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + IA, 0.5 * f1 * denb);
     aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + IA, 0.5 * f2 * denb);
   }, st));
+  if constexpr (TER) {
+    HIPCHK(fluxj_uvz(1, 0, MY, MYb, q13b));  // fluxJ13_XVZ
+    HIPCHK(fluxj_uvz(1, 1, MY, MYb, q23b));  // fluxJ23_XVZ
+  }
   // fluxX_XVZ (UD3.F90:1463-1512)
   HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double den = 0.25 * (D[p] + D[p + 1] + D[p + IA] + D[p + 1 + IA]);
     const double vel = 0.5 * (MX[p] + MX[p + IA]) / den;
     const double fb = qXb[p];
-    aadd(NDb + oVX + p, fb);
-    const double velb = fb * face_adj<S>(0, MY, MYb, p, 1, vel, fb * vel);
+    const double G = m.gs(I_UVZ, p), gm = G / m.mapf(i, j, 1, I_UV);
+    aadd(NDb + oVX + p, fb * G);
+    const double velb = fb * gm * face_adj<S>(0, MY, MYb, p, 1, vel, fb * gm * vel);
     aadd(MXb + p, 0.5 * velb / den); aadd(MXb + p + IA, 0.5 * velb / den);
     const double denb = 0.25 * (-velb * vel / den);
     aadd(Db + p, denb); aadd(Db + p + 1, denb); aadd(Db + p + IA, denb); aadd(Db + p + 1 + IA, denb);
@@ -360,49 +461,85 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const size_t p = IX(d, k, i, j), q = p - IA;
     const double vel = 0.5 * (MY[p] + MY[q]) / D[p];
     const double fb = qYb[q];
-    aadd(NDb + oVY + p, fb);
-    const double velb = fb * face_adj<S>(0, MY, MYb, q, IA, vel, fb * vel);
+    const double G = m.gs(I_XYZ, p), gm = G / m.mapf(i, j, 0, I_XY);
+    aadd(NDb + oVY + p, fb * G);
+    const double velb = fb * gm * face_adj<S>(0, MY, MYb, q, IA, vel, fb * gm * vel);
     aadd(MYb + p, 0.5 * velb / D[p]); aadd(MYb + q, 0.5 * velb / D[p]);
     aadd(Db + p, -velb * vel / D[p]);
   }, st));
 
   // ---- momentum x (HEVI.F90:816-927) ----
-  HIPCHK(hipMemsetAsync(w[W_QZ], 0, (W_QY - W_QZ + 1) * n3 * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(w[W_QZ], 0, (W_Q23 - W_QZ + 1) * n3 * sizeof(double), st));
   HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double b = obMX[p];
     aadd(MX0b + p, b);
     const double T = dt * b;
-    aadd(qZb + p, -T * g.RCDZ[k]); aadd(qZb + p - pl, T * g.RCDZ[k]);
-    aadd(qXb + p, -T * g.RFDX[i]); aadd(qXb + p - 1, T * g.RFDX[i]);
-    aadd(qYb + p, -T * g.RCDY[j]); aadd(qYb + p - IA, T * g.RCDY[j]);
-    aadd(DPRESb + p + 1, -T * g.RFDX[i]); aadd(DPRESb + p, T * g.RFDX[i]);
+    const double u = T / m.gs(I_UYZ, p);
+    aadd(qZb + p, -u * g.RCDZ[k]); aadd(qZb + p - pl, u * g.RCDZ[k]);
+    if constexpr (TER) {
+      aadd(q13b + p, -u * g.RCDZ[k]); aadd(q13b + p - pl, u * g.RCDZ[k]);
+      aadd(q23b + p, -u * g.RCDZ[k]); aadd(q23b + p - pl, u * g.RCDZ[k]);
+    }
+    const double um = u * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY);
+    aadd(qXb + p, -um * g.RFDX[i]); aadd(qXb + p - 1, um * g.RFDX[i]);
+    aadd(qYb + p, -um * g.RCDY[j]); aadd(qYb + p - IA, um * g.RCDY[j]);
+    // pg: the bracket times MAPF(i,j,1,I_UY)
+    const double v = -u * m.mapf(i, j, 0, I_UY);
+    aadd(DPRESb + p + 1, v * g.RFDX[i] * m.gs(I_XYZ, p + 1)); aadd(DPRESb + p, -v * g.RFDX[i] * m.gs(I_XYZ, p));
+    if constexpr (TER) {
+      // both J13 terms take F2H at level k, as HEVI.F90 writes them
+      const double wv = v * g.RCDZ[k] * 0.5;
+      const double a1 = wv * m.j13(I_UYW, p), a2 = wv * m.j13(I_UYW, p - pl);
+      const double f1k = f2h(m, g, d, I_UYZ, k, 1, i, j), f2k = f2h(m, g, d, I_UYZ, k, 2, i, j);
+      aadd(DPRESb + p + pl + 1, a1 * f1k); aadd(DPRESb + p + pl, a1 * f1k);
+      aadd(DPRESb + p + 1, a1 * f2k - a2 * f1k); aadd(DPRESb + p, a1 * f2k - a2 * f1k);
+      aadd(DPRESb + p - pl + 1, -a2 * f2k); aadd(DPRESb + p - pl, -a2 * f2k);
+    }
     if (cor) {
       const double cc = 0.125 * (COR[(size_t)(i + 1) + (size_t)IA * j] + COR[(size_t)i + (size_t)IA * j]) * T;
       aadd(MYb + p + 1, cc); aadd(MYb + p, cc); aadd(MYb + p + 1 - IA, cc); aadd(MYb + p - IA, cc);
+    }
+    if constexpr (TER) {
+      // the map-factor term of cf, c0 Ys (Ys a - MOMX b) 2/(D(i+1) + D(i))
+      const double Ys = MY[p] + MY[p - IA] + MY[p + 1] + MY[p + 1 - IA];
+      const double c0 = 0.25 * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY);
+      const double aa = 0.25 * (1.0 / m.mapf(i + 1, j, 1, I_XY) - 1.0 / m.mapf(i, j, 1, I_XY)) * g.RFDX[i];
+      const double bb = (1.0 / m.mapf(i, j, 0, I_UV) - 1.0 / m.mapf(i, j - 1, 0, I_UV)) * g.RCDY[j];
+      const double Dd = D[p + 1] + D[p], rD = 2.0 / Dd;
+      const double ysb = T * c0 * (2.0 * Ys * aa - MX[p] * bb) * rD;
+      aadd(MYb + p, ysb); aadd(MYb + p - IA, ysb); aadd(MYb + p + 1, ysb); aadd(MYb + p + 1 - IA, ysb);
+      aadd(MXb + p, -T * c0 * Ys * bb * rD);
+      const double cf2 = c0 * Ys * (Ys * aa - MX[p] * bb) * rD;
+      aadd(Db + p + 1, -T * cf2 / Dd); aadd(Db + p, -T * cf2 / Dd);
     }
   }, st));
   // fluxZ_UYZ (UD3.F90:753-845)
   HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double f1 = g.CDZ[k] / (g.CDZ[k] + g.CDZ[k + 1]), f2 = g.CDZ[k + 1] / (g.CDZ[k] + g.CDZ[k + 1]);
+    const double f1 = f2h(m, g, d, I_UYW, k, 1, i, j), f2 = f2h(m, g, d, I_UYW, k, 2, i, j);
     const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + 1]) + f2 * 0.5 * (D[p] + D[p + 1]);
     const double vel = 0.5 * (MZ[p] + MZ[p + 1]) / den;
     const double fb = qZb[p];
-    aadd(NDb + oUZ + p, fb);
-    const double velb = fb * face_adj<S>(zkind(k, KS, KE), MX, MXb, p, pl, vel, fb * vel);
+    aadd(NDb + oUZ + p, fb * m.gs(I_UYW, p));
+    const double velb = J33G * fb * face_adj<S>(zkind(k, KS, KE), MX, MXb, p, pl, vel, J33G * fb * vel);
     aadd(MZb + p, 0.5 * velb / den); aadd(MZb + p + 1, 0.5 * velb / den);
     const double denb = -velb * vel / den;
     aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + 1, 0.5 * f1 * denb);
     aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + 1, 0.5 * f2 * denb);
   }, st));
+  if constexpr (TER) {
+    HIPCHK(fluxj_uvz(0, 0, MX, MXb, q13b));  // fluxJ13_UYZ
+    HIPCHK(fluxj_uvz(0, 1, MX, MXb, q23b));  // fluxJ23_UYZ
+  }
   // fluxX_UYZ (UD3.F90:1043-1097): flux(k,i-1,j) at loop index i
   HIPCHK(for3(Box{KS, KE, IS, IE + 1, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j), q = p - 1;
     const double vel = 0.5 * (MX[p] + MX[q]) / D[p];
     const double fb = qXb[q];
-    aadd(NDb + oUX + p, fb);
-    const double velb = fb * face_adj<S>(0, MX, MXb, q, 1, vel, fb * vel);
+    const double G = m.gs(I_XYZ, p), gm = G / m.mapf(i, j, 1, I_XY);
+    aadd(NDb + oUX + p, fb * G);
+    const double velb = fb * gm * face_adj<S>(0, MX, MXb, q, 1, vel, fb * gm * vel);
     aadd(MXb + p, 0.5 * velb / D[p]); aadd(MXb + q, 0.5 * velb / D[p]);
     aadd(Db + p, -velb * vel / D[p]);
   }, st));
@@ -412,8 +549,9 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const double den = 0.25 * (D[p] + D[p + 1] + D[p + IA] + D[p + 1 + IA]);
     const double vel = 0.5 * (MY[p] + MY[p + 1]) / den;
     const double fb = qYb[p];
-    aadd(NDb + oUY + p, fb);
-    const double velb = fb * face_adj<S>(0, MX, MXb, p, IA, vel, fb * vel);
+    const double G = m.gs(I_UVZ, p), gm = G / m.mapf(i, j, 0, I_UV);
+    aadd(NDb + oUY + p, fb * G);
+    const double velb = fb * gm * face_adj<S>(0, MX, MXb, p, IA, vel, fb * gm * vel);
     aadd(MYb + p, 0.5 * velb / den); aadd(MYb + p + 1, 0.5 * velb / den);
     const double denb = 0.25 * (-velb * vel / den);
     aadd(Db + p, denb); aadd(Db + p + 1, denb); aadd(Db + p + IA, denb); aadd(Db + p + 1 + IA, denb);
@@ -432,7 +570,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     if (k <= KE - 1) {
       const double b = obMZ[p];
       MZ0b[p] += b; MZb[p] -= b;
-      const double r = dt * g.RCDZ[k], r1 = dt * g.RCDZ[k + 1];
+      const double r = dt * J33G * g.RCDZ[k] / m.gs(I_XYZ, p), r1 = dt * J33G * g.RCDZ[k + 1] / m.gs(I_XYZ, p + pl);
       const double bd1 = obD[p + pl], bt1 = obRT[p + pl];
       cpb = b - r * bd - r * PT[p] * bt + r1 * bd1 + r1 * PT[p] * bt1;
       ptb = -r * Cp[p] * bt + r1 * Cp[p] * bt1;
@@ -444,11 +582,11 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   HIPCHK(for3(Box{0, 0, IS, IE, JS, JE}, [=] __device__(int, int i, int j) {
     const size_t c0 = IX(d, 0, i, j);
     auto at = [&](int k) { return c0 + pl * (size_t)k; };
-    auto Af = [&](int k) { return dt * dt * g.RCDZ[k] * RT2P[at(k)]; };
-    auto Bf = [&](int k) { return GRAV * (dt * dt) / (g.CDZ[k + 1] + g.CDZ[k]); };
-    auto F1f = [&](int k) { return -(PT[at(k + 1)] * g.RFDZ[k] * Af(k + 1) + Bf(k)); };
-    auto F2f = [&](int k) { return 1.0 + PT[at(k)] * g.RFDZ[k] * (Af(k + 1) + Af(k)); };
-    auto F3f = [&](int k) { return -(PT[at(k - 1)] * g.RFDZ[k] * Af(k) - Bf(k)); };
+    auto Af = [&](int k) { return dt * dt * J33G * g.RCDZ[k] * RT2P[at(k)] * J33G / m.gs(I_XYZ, at(k)); };
+    auto Bf = [&](int k) { return GRAV * (dt * dt) * J33G / (g.CDZ[k + 1] + g.CDZ[k]); };
+    auto F1f = [&](int k) { return -(PT[at(k + 1)] * g.RFDZ[k] * Af(k + 1) + Bf(k)) / m.gs(I_XYW, at(k)); };
+    auto F2f = [&](int k) { return 1.0 + (PT[at(k)] * g.RFDZ[k] * (Af(k + 1) + Af(k))) / m.gs(I_XYW, at(k)); };
+    auto F3f = [&](int k) { return -(PT[at(k - 1)] * g.RFDZ[k] * Af(k) - Bf(k)) / m.gs(I_XYW, at(k)); };
     double den = F2f(KS);
     double cpk = (KS < KE - 1 ? F3f(KS + 1) : 0.0) / den;
     double dpk = Cpb[at(KS)] / den;
@@ -468,51 +606,52 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     }
   }, st));
   // (c) per level: adjoints of the band coefficients (through PT and
-  //     A = dt^2 RCDZ RT2P) and of the right-hand side C = MOMZ + dt (pg + Sw),
-  //     level L gathering rows L-1, L and L+1
+  //     A = dt^2 J33G RCDZ RT2P J33G / GSQRT) and of the right-hand side
+  //     C = MOMZ + dt (pg + Sw), level L gathering rows L-1, L and L+1
   HIPCHK(for3(inter, [=] __device__(int L, int i, int j) {
     const size_t p = IX(d, L, i, j);
-    auto A_ = [&](int k) { return dt * dt * g.RCDZ[k] * RT2P[IX(d, k, i, j)]; };
+    auto A_ = [&](int k) { const size_t q = IX(d, k, i, j); return dt * dt * J33G * g.RCDZ[k] * RT2P[q] * J33G / m.gs(I_XYZ, q); };
+    auto gw = [&](int k) { return m.gs(I_XYW, IX(d, k, i, j)); };
     auto row = [&](int k) { return k >= KS && k <= KE - 1; };
     double ptb = 0.0, ab = 0.0;
     if (row(L)) {
       const double cb = Cb[p];
-      const double F2b = -cb * Cp[p];
+      const double F2b = -cb * Cp[p] / gw(L);
       ptb += F2b * g.RFDZ[L] * (A_(L + 1) + A_(L));
       ab += F2b * PT[p] * g.RFDZ[L];
-      if (L >= KS + 1) ab -= (-cb * Cp[p - pl]) * PT[p - pl] * g.RFDZ[L];  // F3 of row L
+      if (L >= KS + 1) ab -= (-cb * Cp[p - pl] / gw(L)) * PT[p - pl] * g.RFDZ[L];  // F3 of row L
     }
     if (row(L - 1)) {
       const double cbm = Cb[p - pl];
-      ab += (-cbm * Cp[p - pl]) * PT[p - pl] * g.RFDZ[L - 1];  // F2 of row L-1 through A(L)
-      if (L - 1 <= KE - 2) {                                  // F1 of row L-1
-        const double F1b = -cbm * Cp[p];
+      ab += (-cbm * Cp[p - pl] / gw(L - 1)) * PT[p - pl] * g.RFDZ[L - 1];  // F2 of row L-1 through A(L)
+      if (L - 1 <= KE - 2) {                                               // F1 of row L-1
+        const double F1b = -cbm * Cp[p] / gw(L - 1);
         ptb -= F1b * g.RFDZ[L - 1] * A_(L);
         ab -= F1b * PT[p] * g.RFDZ[L - 1];
       }
     }
-    if (row(L + 1) && L + 1 >= KS + 1) ptb -= (-Cb[p + pl] * Cp[p]) * g.RFDZ[L + 1] * A_(L + 1);  // F3 of row L+1
+    if (row(L + 1) && L + 1 >= KS + 1) ptb -= (-Cb[p + pl] * Cp[p] / gw(L + 1)) * g.RFDZ[L + 1] * A_(L + 1);  // F3 of row L+1
     PTb[p] += ptb;
-    double rt2pb = ab * dt * dt * g.RCDZ[L];
-    // right-hand side: pg = -(DPRES(k+1) + RT2P(k+1) dt St(k+1) - DPRES(k) - RT2P(k) dt St(k)) RFDZ(k)
+    double rt2pb = ab * dt * dt * J33G * g.RCDZ[L] * J33G / m.gs(I_XYZ, p);
+    // right-hand side: pg = -(DPRES(k+1) + RT2P(k+1) dt St(k+1) - DPRES(k) - RT2P(k) dt St(k)) RFDZ(k) J33G / GSQRT
     //                       - GRAV (f1 (D(k+1) - REFd + dt Sr(k+1)) + f2 (D(k) - REFd + dt Sr(k)))
     double dpb = 0.0, stb = 0.0, db = 0.0, srb = 0.0;
     if (row(L)) {
       const double cb = Cb[p];
       MZb[p] += cb;
       Swb[p] += dt * cb;
-      const double a = -dt * cb * g.RFDZ[L];
+      const double a = -dt * cb * g.RFDZ[L] * J33G / gw(L);
       dpb -= a; rt2pb -= a * dt * St[p]; stb -= a * RT2P[p] * dt;
       const double bb = -GRAV * dt * cb;
-      const double f2 = g.CDZ[L + 1] / (g.CDZ[L] + g.CDZ[L + 1]);
+      const double f2 = f2h(m, g, d, I_XYZ, L, 2, i, j);
       db += bb * f2; srb += bb * f2 * dt;
     }
     if (row(L - 1)) {
       const double cbm = Cb[p - pl];
-      const double a = -dt * cbm * g.RFDZ[L - 1];
+      const double a = -dt * cbm * g.RFDZ[L - 1] * J33G / gw(L - 1);
       dpb += a; rt2pb += a * dt * St[p]; stb += a * RT2P[p] * dt;
       const double bb = -GRAV * dt * cbm;
-      const double f1 = g.CDZ[L - 1] / (g.CDZ[L - 1] + g.CDZ[L]);
+      const double f1 = f2h(m, g, d, I_XYZ, L - 1, 1, i, j);
       db += bb * f1; srb += bb * f1 * dt;
     }
     DPRESb[p] += dpb; RT2Pb[p] += rt2pb; Stb[p] += stb; Db[p] += db; Srb[p] += srb;
@@ -526,50 +665,87 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   // ---- St (HEVI.F90:634-657) and the theta fluxes (UD3.F90:157-354) ----
   HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double b = Stb[p];
+    const double b = Stb[p] * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p);
     aadd(tZb + p, -b * g.RCDZ[k]); aadd(tZb + p - pl, b * g.RCDZ[k]);
     aadd(tXb + p, -b * g.RCDX[i]); aadd(tXb + p - 1, b * g.RCDX[i]);
     aadd(tYb + p, -b * g.RCDY[j]); aadd(tYb + p - IA, b * g.RCDY[j]);
   }, st));
   HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double fb = tZb[p], vel = ND[oDZ + p];
-    aadd(NDb + oTZ + p, fb);
+    const double fb = tZb[p], vel = mflx_z0(m, d, MX, MY, ND + oDZ, k, i, j);
+    aadd(NDb + oTZ + p, fb * m.gs(I_XYW, p));
     aadd(mZb + p, fb * face_adj<S>(zkind(k, KS, KE), POTT, POTTb, p, pl, vel, fb * vel));
   }, st));
   HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double fb = tXb[p], vel = MX[p] + ND[oDX + p];
-    aadd(NDb + oTX + p, fb);
+    const double G = m.gs(I_UYZ, p);
+    const double fb = tXb[p], vel = G / m.mapf(i, j, 1, I_UY) * (MX[p] + ND[oDX + p]);
+    aadd(NDb + oTX + p, fb * G);
     aadd(mXb + p, fb * face_adj<S>(0, POTT, POTTb, p, 1, vel, fb * vel));
   }, st));
   HIPCHK(for3(Box{KS, KE, IS, IE, JS - 1, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double fb = tYb[p], vel = MY[p] + ND[oDY + p];
-    aadd(NDb + oTY + p, fb);
+    const double G = m.gs(I_XVZ, p);
+    const double fb = tYb[p], vel = G / m.mapf(i, j, 0, I_XV) * (MY[p] + ND[oDY + p]);
+    aadd(NDb + oTY + p, fb * G);
     aadd(mYb + p, fb * face_adj<S>(0, POTT, POTTb, p, IA, vel, fb * vel));
   }, st));
 
   // ---- Sw (HEVI.F90:538-577) and the MOMZ fluxes (HEVI.F90:486-525) ----
-  HIPCHK(hipMemsetAsync(w[W_QZ], 0, (W_QY - W_QZ + 1) * n3 * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(w[W_QZ], 0, (W_Q23 - W_QZ + 1) * n3 * sizeof(double), st));
   HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double b = Swb[p];
-    aadd(qZb + p, -b * g.RFDZ[k]); aadd(qZb + p - pl, b * g.RFDZ[k]);
-    aadd(qXb + p, -b * g.RCDX[i]); aadd(qXb + p - 1, b * g.RCDX[i]);
-    aadd(qYb + p, -b * g.RCDY[j]); aadd(qYb + p - IA, b * g.RCDY[j]);
+    const double u = b / m.gs(I_XYW, p);
+    aadd(qZb + p, -u * g.RFDZ[k]); aadd(qZb + p - pl, u * g.RFDZ[k]);
+    if constexpr (TER) {
+      aadd(q13b + p, -u * g.RFDZ[k]); aadd(q13b + p - pl, u * g.RFDZ[k]);
+      aadd(q23b + p, -u * g.RFDZ[k]); aadd(q23b + p - pl, u * g.RFDZ[k]);
+    }
+    const double um = u * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY);
+    aadd(qXb + p, -um * g.RCDX[i]); aadd(qXb + p - 1, um * g.RCDX[i]);
+    aadd(qYb + p, -um * g.RCDY[j]); aadd(qYb + p - IA, um * g.RCDY[j]);
     aadd(MZ0b + p, -wd[k] * b);
   }, st));
   // fluxZ_XYW (UD3.F90:358-452): face m at loop index m+1
-  HIPCHK(for3(Box{KS, KE - 2, IS, IE, JS, JE}, [=] __device__(int m, int i, int j) {
-    const size_t p = IX(d, m, i, j);
+  HIPCHK(for3(Box{KS, KE - 2, IS, IE, JS, JE}, [=] __device__(int f, int i, int j) {
+    const size_t p = IX(d, f, i, j);
     const double vel = 0.5 * (MZ[p] + MZ[p + pl]) / D[p + pl];
     const double fb = qZb[p];
-    aadd(NDb + oWZ + p + pl, fb);
-    const double velb = fb * face_adj<S>(m == KS ? 1 : 0, MZ, MZb, p, pl, vel, fb * vel);
+    aadd(NDb + oWZ + p + pl, fb * m.gs(I_XYZ, p + pl));
+    const double velb = J33G * fb * face_adj<S>(f == KS ? 1 : 0, MZ, MZb, p, pl, vel, J33G * fb * vel);
     aadd(MZb + p, 0.5 * velb / D[p + pl]); aadd(MZb + p + pl, 0.5 * velb / D[p + pl]);
     aadd(Db + p + pl, -velb * vel / D[p + pl]);
   }, st));
+  // fluxJ13_XYW / fluxJ23_XYW (UD3.F90:457-593), terrain only: face f at
+  // loop index f+1, the face KS with the mean velocity of levels KS, KS+1
+  if constexpr (TER) {
+    for (int jd = 0; jd < 2; ++jd) {
+      const double* mom = jd == 0 ? MX : MY;
+      double* momb = jd == 0 ? MXb : MYb;
+      const double* qb = jd == 0 ? q13b : q23b;
+      const size_t sd = jd == 0 ? 1 : (size_t)IA;  // mom averaged over (i, i-1) or (j, j-1)
+      const int mn = jd == 0 ? 1 : 0;
+      HIPCHK(for3(Box{KS, KE - 2, IS, IE, JS, JE}, [=] __device__(int f, int i, int j) {
+        const size_t p = IX(d, f, i, j), p1 = p + pl;
+        const double Jm = jd == 0 ? m.j13(I_XYZ, p1) : m.j23(I_XYZ, p1);
+        const double mp = m.mapf(i, j, mn, I_XY);
+        const double t1 = 0.5 * (mom[p1] + mom[p1 - sd]) / D[p1];
+        const double t0 = f == KS ? 0.5 * (mom[p] + mom[p - sd]) / D[p] : 0.0;
+        const double vr = f == KS ? (t1 + t0) * 0.5 : t1;
+        const double vel = vr * Jm;
+        const double fb = qb[p];
+        const double velb = fb * face_adj<S>(f == KS ? 1 : 0, MZ, MZb, p, pl, vel, fb * vel / mp) / mp;
+        const double vrb = velb * Jm;
+        const double t1b = f == KS ? 0.5 * vrb : vrb;
+        aadd(momb + p1, 0.5 * t1b / D[p1]); aadd(momb + p1 - sd, 0.5 * t1b / D[p1]); aadd(Db + p1, -t1b * t1 / D[p1]);
+        if (f == KS) {
+          const double t0b = 0.5 * vrb;
+          aadd(momb + p, 0.5 * t0b / D[p]); aadd(momb + p - sd, 0.5 * t0b / D[p]); aadd(Db + p, -t0b * t0 / D[p]);
+        }
+      }, st));
+    }
+  }
   // fluxX_XYW / fluxY_XYW (UD3.F90:598-748)
   for (int dir = 0; dir < 2; ++dir) {
     const size_t sd = dir == 0 ? 1 : (size_t)IA;
@@ -577,15 +753,17 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     double* momb = dir == 0 ? MXb : MYb;
     const double* qb = dir == 0 ? qXb : qYb;
     const size_t on = dir == 0 ? oWX : oWY;
+    const int cG = dir == 0 ? I_UYW : I_XVW, cM = dir == 0 ? I_UY : I_XV, mn = dir == 0 ? 1 : 0;
     HIPCHK(for3(Box{KS, KE - 1, dir == 0 ? IS - 1 : IS, IE, dir == 0 ? JS : JS - 1, JE}, [=] __device__(int k, int i, int j) {
       const size_t p = IX(d, k, i, j);
-      const double f1 = g.CDZ[k] / (g.CDZ[k] + g.CDZ[k + 1]), f2 = g.CDZ[k + 1] / (g.CDZ[k] + g.CDZ[k + 1]);
+      const double f1 = f2h(m, g, d, cG, k, 1, i, j), f2 = f2h(m, g, d, cG, k, 2, i, j);
       const double num = f1 * mom[p + pl] + f2 * mom[p];
       const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + sd]) + f2 * 0.5 * (D[p] + D[p + sd]);
       const double vel = num / den;
       const double fb = qb[p];
-      aadd(NDb + on + p, fb);
-      const double velb = fb * face_adj<S>(0, MZ, MZb, p, sd, vel, fb * vel);
+      const double G = m.gs(cG, p), gm = G / m.mapf(i, j, mn, cM);
+      aadd(NDb + on + p, fb * G);
+      const double velb = fb * gm * face_adj<S>(0, MZ, MZb, p, sd, vel, fb * gm * vel);
       const double numb = velb / den, denb = -velb * vel / den;
       aadd(momb + p + pl, f1 * numb); aadd(momb + p, f2 * numb);
       aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + sd, 0.5 * f1 * denb);
@@ -596,7 +774,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   // ---- Sr (HEVI.F90:453-475) ----
   HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double b = Srb[p];
+    const double b = Srb[p] * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p);
     aadd(mZb + p, -b * g.RCDZ[k]); aadd(mZb + p - pl, b * g.RCDZ[k]);
     aadd(mXb + p, -b * g.RCDX[i]); aadd(mXb + p - 1, b * g.RCDX[i]);
     aadd(mYb + p, -b * g.RCDY[j]); aadd(mYb + p - IA, b * g.RCDY[j]);
@@ -605,10 +783,23 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS - 1, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     if (j >= JS) {
-      if (k <= KE - 1) aadd(NDb + oDZ + p, mZb[p]);
-      aadd(MXb + p, mXb[p]); aadd(NDb + oDX + p, mXb[p]);
+      if (k <= KE - 1) {
+        const double b = mZb[p];
+        const double M0 = m.mapf(i, j, 0, I_XY), M1 = m.mapf(i, j, 1, I_XY);
+        aadd(NDb + oDZ + p, b * (m.gs(I_XYW, p) / (M0 * M1)));
+        if constexpr (TER) {
+          const double c13 = b * (m.j13(I_XYW, p) * 0.25 / M1), c23 = b * (m.j23(I_XYW, p) * 0.25 / M0);
+          aadd(MXb + p + pl, c13); aadd(MXb + p + pl - 1, c13); aadd(MXb + p, c13); aadd(MXb + p - 1, c13);
+          aadd(MYb + p + pl, c23); aadd(MYb + p + pl - IA, c23); aadd(MYb + p, c23); aadd(MYb + p - IA, c23);
+        }
+      }
+      const double gm = m.gs(I_UYZ, p) / m.mapf(i, j, 1, I_UY);
+      aadd(MXb + p, mXb[p] * gm); aadd(NDb + oDX + p, mXb[p] * gm);
     }
-    if (i >= IS) { aadd(MYb + p, mYb[p]); aadd(NDb + oDY + p, mYb[p]); }
+    if (i >= IS) {
+      const double gm = m.gs(I_XVZ, p) / m.mapf(i, j, 0, I_XV);
+      aadd(MYb + p, mYb[p] * gm); aadd(NDb + oDY + p, mYb[p] * gm);
+    }
   }, st));
   // ---- POTT (HEVI.F90:588-598) and DPRES (HEVI.F90:346-360) ----
   HIPCHK(for3(Box{KS, KE, IS - 2, IE + 2, JS - 2, JE + 2}, [=] __device__(int k, int i, int j) {
@@ -621,15 +812,29 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       aadd(RT2Pb + p, b * (RT[p] - REFrt[p]));
       aadd(RTb + p, b * RT2P[p]);
       aadd(REFrtb + p, -b * RT2P[p]);
+      if constexpr (TER) {
+        // the ghost levels, read by the terrain terms of the horizontal
+        // pressure gradient: DPRES0 - DENS(KS) (PHI(KS-1) - PHI(KS+1)) and its KE mirror
+        if (k == KS) {
+          const double bg = DPRESb[p - pl];
+          aadd(DPRES0b + p - pl, bg);
+          aadd(Db + p, -bg * (PHI[p - pl] - PHI[p + pl]));
+        }
+        if (k == KE) {
+          const double bg = DPRESb[p + pl];
+          aadd(DPRES0b + p + pl, bg);
+          aadd(Db + p, -bg * (PHI[p + pl] - PHI[p - pl]));
+        }
+      }
     }
   }, st));
   return hipSuccess;
 }
 
 hipError_t hevi_stage_adj(Ctx& c, const StageAdjIO& io, hipStream_t s) {
-  if (c.terrain) return hipErrorNotSupported;
   HIPCHK(adj_setup(c, 0, 0));
-  return c.scheme == UD3 ? stage_adj_t<UD3>(c, io, s) : stage_adj_t<CD4>(c, io, s);
+  if (c.terrain) return c.scheme == UD3 ? stage_adj_t<UD3, true>(c, io, s) : stage_adj_t<CD4, true>(c, io, s);
+  return c.scheme == UD3 ? stage_adj_t<UD3, false>(c, io, s) : stage_adj_t<CD4, false>(c, io, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1146,7 +1351,7 @@ static StepRec model_rec(Ctx& c) {
 }
 
 static hipError_t check_adjoint_model(const Ctx& c) {
-  if (!c.model || c.terrain || c.d.IS != 2) return hipErrorNotSupported;
+  if (!c.model || c.d.IS != 2) return hipErrorNotSupported;
   const Model& M = *c.model;
   if (M.mixed || M.heve || M.divdmp_coef != 0.0 || M.tinteg_short != 4) return hipErrorNotSupported;
   if ((int)ceil((M.dtl - EPS_RP) / M.dts) != 1) return hipErrorNotSupported;
@@ -1212,8 +1417,12 @@ hipError_t model_adjoint(Ctx& c, double* const xb[5]) {
     const bool have = recs && sg >= nseg - A.taped;
     const int base = have ? (sg - (nseg - A.tseg)) * A.every : 0;
     auto seg = [&](int n, double** x) { for (int v = 0; v < 5; ++v) x[v] = A.seg[5 * (base + n - n0) + v]; };
-    if (!have)
+    if (!have) {
       for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[v], A.ckpt[5 * sg + v], bytes, hipMemcpyDeviceToDevice, s));
+      // slot 0 no longer holds the tape of its own segment (the earliest
+      // taped one): a later reverse sweep of the same run records it again
+      if (recs) A.taped = std::min(A.taped, A.tseg - 1);
+    }
     if (recs) {
       // one recording forward over the segment unless the checkpointed
       // forward kept its tape, then the reverse steps

@@ -540,7 +540,6 @@ int scale_hip_tstep_short_hevi_adj(scale_hip_ctx* h,
   if (int r = ensure_dropin(h)) return r;
   if (int r = set_grid1d(c, CDZ, FDZ, RCDZ, RFDZ, nullptr, FDX, RCDX, RFDX, nullptr, FDY, RCDY, RFDY)) return r;
   if (int r = set_metric3d(c, PHI, GSQRT, J13G, J23G, *J33G, MAPF)) return r;
-  if (c.terrain) return fail("the adjoint supports flat terrain only (GSQRT = 1, J13G = J23G = 0, MAPF = 1)");
   CK(shp::adj_setup(c, 0, 0));
   auto& B = h->dbuf;
   const double* s0[5] = {DENS0, MOMZ0, MOMX0, MOMY0, RHOT0};
@@ -1110,7 +1109,7 @@ int scale_hip_model_forward_checkpointed(scale_hip_ctx* h, int32_t n, int32_t ev
   if (n < 1 || every < 1) return fail("n and every must be positive");
   hipError_t e = shp::model_forward_checkpointed(c, n, every);
   if (e == hipErrorNotSupported)
-    return fail("the adjoint supports flat terrain, FP64, RK4 short steps, divdmp_coef = 0 and one short step per large step");
+    return fail("the adjoint supports FP64, RK4 short steps, divdmp_coef = 0 and one short step per large step");
   CK(e);
   CK(hipStreamSynchronize(c.stream));
   return 0;
@@ -1130,7 +1129,7 @@ int scale_hip_model_adjoint(scale_hip_ctx* h, double* DENS_b, double* MOMZ_b, do
   hipError_t e = shp::model_adjoint(c, xb);
   if (e == hipErrorNotReady) return fail("scale_hip_model_forward_checkpointed must run first");
   if (e == hipErrorNotSupported)
-    return fail("the adjoint supports flat terrain, FP64, RK4 short steps, divdmp_coef = 0 and one short step per large step");
+    return fail("the adjoint supports FP64, RK4 short steps, divdmp_coef = 0 and one short step per large step");
   CK(e);
   for (int v = 0; v < 5; ++v)
     if (int r = download3(c, hb[v], xb[v], 1)) return r;

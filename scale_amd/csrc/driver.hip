@@ -653,7 +653,7 @@ hipError_t rk4_combine_state(Ctx& c, double* const out[5], const double* const s
 
 hipError_t model_short_step_rk4(Ctx& c, double dts, const StepRec* rec) {
   Model& M = *c.model;
-  if (rec && (M.mixed || M.heve || c.terrain || M.divdmp_coef != 0.0 || M.tinteg_short != 4)) return hipErrorNotSupported;
+  if (rec && (M.mixed || M.heve || M.divdmp_coef != 0.0 || M.tinteg_short != 4)) return hipErrorNotSupported;
   if (M.mixed) {
     ShortBufs<float> B{{M.fS[0], M.fS[1], M.fS[2], M.fS[3], M.fS[4]}, M.fR, M.ft, M.fnum_diff, M.fmfR, M.ftfR, M.fmflx,
                        M.ftflx, M.fDDIV};

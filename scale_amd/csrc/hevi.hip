@@ -1791,18 +1791,26 @@ hipError_t heve_stage(const StageIO& io, hipStream_t s) {
   return heve_fct_theta(io, s);
 }
 
-// flat-terrain FP64 stage through hevi_tile, recording PT, St and the
-// solved vertical momentum C for the adjoint
+// FP64 stage recording PT, St and the solved vertical momentum C for the
+// adjoint: hevi_tile on flat grids, hevi_march with terrain
 hipError_t hevi_stage_traced(const StageIO& io, double* PT, double* St, double* Cp, hipStream_t s) {
   HeviArgs a;
   fill_args(a, io);
   const Ctx& c = *io.ctx;
-  if (c.terrain || io.divdmp_coef != 0.0 || io.combine) return hipErrorNotSupported;
+  if (io.divdmp_coef != 0.0 || io.combine) return hipErrorNotSupported;
   a.PT = PT; a.trSt = St; a.trCp = Cp;
   const Dims& d = c.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
+  const unsigned ni = (imax + 63) / 64;
+  if (c.terrain) {
+    constexpr unsigned TJ = 4;
+    const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
+    if (c.scheme == UD3) hipLaunchKernelGGL((hevi_march<UD3, true, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+    else hipLaunchKernelGGL((hevi_march<CD4, true, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+    return hipGetLastError();
+  }
   constexpr int TJ = 8;
-  const unsigned ni = (imax + 63) / 64, nb = ni * ((jmax + TJ - 1) / TJ);
+  const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
   if (c.scheme == UD3) hipLaunchKernelGGL((hevi_tile<UD3, false, TJ, double, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   else hipLaunchKernelGGL((hevi_tile<CD4, false, TJ, double, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   return hipGetLastError();

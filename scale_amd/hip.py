@@ -271,7 +271,7 @@ class Context:
                                            DPRES0, RT2P, CORIOLI, num_diff, wdamp_coef,
                                            CDZ, FDZ, FDX, FDY, RCDZ, RCDX, RCDY, RFDZ, RFDX, RFDY,
                                            PHI, GSQRT, J13G, J23G, J33G, MAPF, REF_dens, REF_rhot, dtrk) -> dict:
-        """Adjoint of one HEVI stage (flat terrain, FP64).  out_b holds the adjoints
+        """Adjoint of one HEVI stage (flat or terrain-following metrics, FP64).  out_b holds the adjoints
         DENS_RK..RHOT_RK of the stage outputs; returns the adjoints of the inputs
         (DENS0..RHOT0, DENS..RHOT, num_diff, DPRES0, RT2P, REF_rhot)."""
         g = self.grid

@@ -79,10 +79,14 @@ def test_model_adjoint_matches_oracle(records, tape, monkeypatch):
         for k in STATE:  # the forward itself matches the oracle
             assert close(ctx.model_get(k), traj[N][k], 1e-12), k
         got = ctx.model_adjoint(xbN)
+        # a second reverse sweep of the same run (the first one re-recorded
+        # into the tape slots of the segments it did not have taped)
+        again = ctx.model_adjoint(xbN)
     finally:
         ctx.close()
     for k in STATE:
         assert close(got[k], xb[k]), k
+        assert close(again[k], xb[k]), k
 
 
 @pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
