@@ -156,6 +156,8 @@ def parse():
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP32 + FP64 solve) timing")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     p.add_argument("--no-ensemble", action="store_true", help="skip the 64-member ensemble leg")
+    p.add_argument("--batched-adjoint", default=None, metavar="MEMBERSxSIZExKMAXxSTEPSxEVERYxTRACERS",
+                   help="run the batched terrain adjoint leg, e.g. 32x128x64x100x10x5 (off by default)")
     p.add_argument("--tracer-scheme", type=int, default=None, choices=[1, 2, 3, 4, 5, 6, 31],
                    help="also time large steps whose 3 FCT tracers use this scheme "
                         "(UD1=1, CD2=2, UD3=3, CD4=4, UD5=5, CD6=6, UD3KOREN1993=31; 5 and 6 on a lateral halo of 3)")
@@ -331,6 +333,87 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
                 fd_eps_sweep=sweep if ndirs > 0 else None)
 
 
+def batched_adjoint_leg(nmem, size, kmax, nsteps, every, ntracer):
+    """The batched terrain adjoint: nmem members of size x size x kmax over a
+    bell mountain (h = 500 m, half-width 5 km; DX = DY = 500 m, DZ = 250 m,
+    N = 0.012 1/s, u = 10 m/s, theta noise N(0, 0.1 K) per member seed,
+    ntracer FCT tracers, FP64, dt = 1 s), nsteps large steps with checkpoints
+    every `every`; per member J = sum over the 16x16x16 box above the
+    mountain top of (theta - REF_pott)^2 and its gradient with respect to the
+    member's state at the start.  Times, by the host clock around calls that
+    end with a device synchronisation: the forward of every member
+    (interleaved step by step over the members' streams), the batched
+    checkpointed forward and the batched reverse sweep (the latter with the
+    host transfers of the 5 adjoint fields of each member each way)."""
+    import numpy as np
+    from scale_amd import hip, inputs
+    members = inputs.mountain_members(nmem, imax=size, jmax=size, kmax=kmax, ntracer=ntracer)
+    names = hip.STATE
+    ctxs = []
+    try:
+        for case, zs in members:
+            ctxs.append(hip.model_from_case(case, zsfc=zs))
+        g = members[0][0].grid
+        ref_pott = members[0][0].ref["REF_pott"]
+        I, J = g.IS + size // 2, g.JS + size // 2
+        box = (slice(g.KS + 2, g.KS + 18), slice(I - 8, I + 8), slice(J - 8, J + 8))
+        x0 = [{k: c.model_get(k) for k in names} for c in ctxs]
+        q0 = [[c.model_get(hip.QTRC_BASE + iq) for iq in range(ntracer)] for c in ctxs]
+
+        def reset():
+            for c, x, q in zip(ctxs, x0, q0):
+                for k in names:
+                    c.model_set(k, x[k])
+                for iq in range(ntracer):
+                    c.model_set(hip.QTRC_BASE + iq, q[iq])
+
+        def sync():
+            for c in ctxs:
+                c.synchronize()
+
+        for c in ctxs:  # warm-up
+            c.model_large_step(1)
+        sync()
+        reset()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            for c in ctxs:
+                c.model_large_step(1)
+        sync()
+        fwd_s = time.perf_counter() - t0
+        progress(f"batched adjoint: forward of {nmem} members x {nsteps} steps {fwd_s:.2f} s")
+        reset()
+        sync()
+        t0 = time.perf_counter()
+        hip.batch_forward_checkpointed(ctxs, nsteps, every)
+        fck_s = time.perf_counter() - t0
+        Js, xbs = [], []
+        for c in ctxs:
+            st = {k: c.model_get(k) for k in ("DENS", "RHOT")}
+            th = st["RHOT"] / st["DENS"] - ref_pott
+            xb = {k: np.zeros(g.shape, order="F") for k in names}
+            xb["RHOT"][box] = 2.0 * th[box] / st["DENS"][box]
+            xb["DENS"][box] = -2.0 * th[box] * st["RHOT"][box] / st["DENS"][box] ** 2
+            Js.append(float(np.sum(th[box] ** 2)))
+            xbs.append(xb)
+        progress(f"batched adjoint: checkpointed forward {fck_s:.2f} s")
+        t0 = time.perf_counter()
+        got = hip.batch_adjoint(ctxs, xbs)
+        rev_s = time.perf_counter() - t0
+        norms = [float(np.sqrt(sum(np.sum(inputs.interior_gradient(g, x)[k] ** 2) for k in names))) for x in got]
+        progress(f"batched adjoint: reverse sweep {rev_s:.2f} s, adj/fwd {(fck_s + rev_s) / fwd_s:.2f}")
+    finally:
+        for c in ctxs:
+            c.close()
+    return dict(workload=f"{nmem} members of {size}x{size}x{kmax} over a bell mountain h=500 m, a=5 km, "
+                         f"DX=500 m, DZ=250 m, N=0.012 1/s, u=10 m/s, {ntracer} FCT tracers, FP64, UD3, dt=1 s",
+                members=nmem, steps=nsteps, checkpoint_every=every, fwd_s=fwd_s, forward_checkpointed_s=fck_s,
+                reverse_s=rev_s, adj_over_fwd=(fck_s + rev_s) / fwd_s,
+                member_J=Js, member_gradient_norms=norms,
+                precision="FP64 throughout (the mixed-precision adjoint is not built)")
+
+
 def fd_summary(ads, fd2, fd4, eps_list):
     import numpy as np
     ad = np.array(ads)
@@ -499,6 +582,10 @@ def main():
         ens = ensemble_leg()
         progress(f"ensemble: {ens['ensemble_graph_ms_per_large_step']:.2f} ms/large step "
                  f"({ens['speedup_vs_sequential']:.2f}x the sequential members)")
+    badj = None
+    if a.batched_adjoint:
+        nm, sz, kz, ns, ev, nt = (int(x) for x in a.batched_adjoint.split("x"))
+        badj = batched_adjoint_leg(nm, sz, kz, ns, ev, nt)
     theta = None
     if a.theta_check > 0:
         theta = theta_check(case, a.theta_check)
@@ -544,6 +631,8 @@ def main():
     }
     if theta is not None:
         out["theta_check"] = theta
+    if badj is not None:
+        out["batched_terrain_adjoint"] = badj
     if not a.no_cpu_baseline:
         cn, cj, ck = (int(x) for x in a.cpu_sample.split("x"))
         out["cpu_baseline"] = cpu_baseline(cn, cj, ck)

@@ -326,8 +326,8 @@ int scale_hip_profile_query(scale_hip_ctx *ctx, const char *name, int64_t *count
 /* ---- adjoint --------------------------------------------------------------
  * No counterpart in the reference.  The transpose of the Jacobian of the
  * forward above (with the UD3 upwind switch sign(vel) held fixed), for flat
- * terrain, FP64, divdmp_coef = 0 and one short step per large step; other
- * configurations are rejected.  Adjoint arrays have the shapes of the fields
+ * or terrain-following metrics (held fixed), FP64, divdmp_coef = 0 and one
+ * short step per large step; other configurations are rejected.  Adjoint arrays have the shapes of the fields
  * they belong to (Fortran layout, host memory).
  *
  * Adjoint of one scale_hip_tstep_short_hevi call: given the stage inputs
@@ -358,6 +358,20 @@ int scale_hip_tstep_short_hevi_adj(scale_hip_ctx *ctx,
  * checkpoint.  The model state is left at the end of the run. */
 int scale_hip_model_forward_checkpointed(scale_hip_ctx *ctx, int32_t n, int32_t every);
 int scale_hip_model_adjoint(scale_hip_ctx *ctx, double *DENS_b, double *MOMZ_b, double *MOMX_b, double *MOMY_b, double *RHOT_b);
+
+/* The same pair for a batch of n member contexts (each with its model, of
+ * any grid size, flat or terrain, all on one device), advanced together:
+ * each member runs on its own stream, the forwards step by step and the
+ * reverse sweeps segment by segment across the members, so that their
+ * kernels overlap, and the members split the device memory for tapes.  Each
+ * member's results equal its own model_forward_checkpointed/model_adjoint
+ * pair (up to the order of the adjoint's atomic sums).  DENS_b .. RHOT_b are
+ * arrays of n host pointers, one field per member, overwritten as in
+ * scale_hip_model_adjoint.  No counterpart in the reference (which runs
+ * members as separate jobs, scale-rm/src/scale-rm.F90:180-184). */
+int scale_hip_batch_forward_checkpointed(scale_hip_ctx *const *members, int32_t n, int32_t nsteps, int32_t every);
+int scale_hip_batch_adjoint(scale_hip_ctx *const *members, int32_t n, double *const *DENS_b, double *const *MOMZ_b,
+                            double *const *MOMX_b, double *const *MOMY_b, double *const *RHOT_b);
 
 /* Launch-shape and memory knobs, with no counterpart in the reference (whose
  * loops take IBLOCK/JBLOCK from PARAM_ATMOS_DYN; here the tiles are chosen

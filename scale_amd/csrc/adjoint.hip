@@ -129,7 +129,7 @@ enum {
 };
 
 struct AdjWork {
-  int nsteps = 0, every = 0, recorded = 0, tape_limit = -1;
+  int nsteps = 0, every = 0, recorded = 0, tape_limit = -1, share = 1;
   // the checkpointed forward ran its last `taped` segments with records: the
   // reverse sweep takes them without a recording forward
   int taped = 0;
@@ -176,7 +176,7 @@ static double* aalloc(AdjWork& A, size_t n) {
 
 // allocate the stage/filter work (nsteps = 0) or, in addition, the
 // checkpoint storage of a run of nsteps large steps saved every 'every' steps
-hipError_t adj_setup(Ctx& c, int nsteps, int every) {
+hipError_t adj_setup(Ctx& c, int nsteps, int every, int share) {
   if (!c.adj) {
     c.adj = new AdjWork();
     AdjWork& A = *c.adj;
@@ -200,7 +200,7 @@ hipError_t adj_setup(Ctx& c, int nsteps, int every) {
     if (!ok) { adj_free(c); return hipErrorOutOfMemory; }
   }
   AdjWork& A = *c.adj;
-  if (nsteps > 0 && (A.nsteps != nsteps || A.every != every || A.tape_limit != c.adj_tape)) {
+  if (nsteps > 0 && (A.nsteps != nsteps || A.every != every || A.tape_limit != c.adj_tape || A.share != share)) {
     const size_t n3 = c.d.n3;
     const int nck = (nsteps + every - 1) / every;
     // drop the storage of a previous run (kept in allocs until adj_free) and size anew
@@ -210,16 +210,17 @@ hipError_t adj_setup(Ctx& c, int nsteps, int every) {
       if (!p) return hipErrorOutOfMemory;
       A.ckpt.push_back(p);
     }
-    // the tape budget: c.adj_tape segments, or (0) as many as half the free
-    // device memory holds; SCALE_HIP_ADJ_RECORDS=0 keeps no records at all
-    // (each reversed step recomputes its own)
+    // the tape budget: c.adj_tape segments, or (0) as many as this model's
+    // share of half the free device memory holds; SCALE_HIP_ADJ_RECORDS=0
+    // keeps no records at all (each reversed step recomputes its own)
     const char* env = getenv("SCALE_HIP_ADJ_RECORDS");
     const bool records = !env || atoi(env) != 0;
     const size_t per_seg = (size_t)every * 50 * n3 * sizeof(double);
     int want = c.adj_tape > 0 ? std::min(c.adj_tape, nck) : nck;
     if (c.adj_tape == 0) {
       size_t fr = 0, tot = 0;
-      if (hipMemGetInfo(&fr, &tot) == hipSuccess) want = (int)std::max<size_t>(1, std::min<size_t>(nck, fr / 2 / per_seg));
+      if (hipMemGetInfo(&fr, &tot) == hipSuccess)
+        want = (int)std::max<size_t>(1, std::min<size_t>(nck, fr / 2 / (size_t)std::max(share, 1) / per_seg));
     }
     if (!records) want = 0;
     for (int sg = 0; sg < want; ++sg) {
@@ -254,7 +255,7 @@ hipError_t adj_setup(Ctx& c, int nsteps, int every) {
         A.seg.push_back(p);
       }
     }
-    A.nsteps = nsteps; A.every = every; A.tape_limit = c.adj_tape;
+    A.nsteps = nsteps; A.every = every; A.tape_limit = c.adj_tape; A.share = share;
   }
   return hipSuccess;
 }
@@ -1358,36 +1359,126 @@ static hipError_t check_adjoint_model(const Ctx& c) {
   return hipSuccess;
 }
 
-// forward run of nsteps large steps that keeps the state every 'every' steps;
-// its last A.tseg segments also keep their step states and records (the
-// tapes), as the reverse sweep's recording forward would
-hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every) {
+// The checkpointed forward in pieces: fwd_begin sets up the run,
+// fwd_step(n) advances large step n (saving the checkpoint or the tape it
+// owes), fwd_end marks the run recorded.  A forward of nsteps large steps
+// keeps the state every 'every' steps; its last A.tseg segments also keep
+// their step states and records (the tapes), as the reverse sweep's
+// recording forward would.  share: the number of models that split the
+// device memory for tapes.
+static hipError_t fwd_begin(Ctx& c, int nsteps, int every, int share) {
   HIPCHK(check_adjoint_model(c));
-  HIPCHK(adj_setup(c, nsteps, every));
+  HIPCHK(adj_setup(c, nsteps, every, share));
+  c.adj->recorded = 0;
+  c.adj->taped = 0;
+  return hipSuccess;
+}
+
+static hipError_t fwd_step(Ctx& c, int n) {
   AdjWork& A = *c.adj;
   Model& M = *c.model;
   const size_t bytes = c.d.n3 * sizeof(double);
   double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
-  const int nseg = (nsteps + every - 1) / every;
-  const int nfirst = (nseg - A.tseg) * every;  // first taped step
-  A.recorded = 0;
-  A.taped = 0;
-  prof_start(c, "model_forward_checkpointed", c.stream);
-  for (int n = 0; n < nsteps; ++n) {
-    if (n % every == 0)
-      for (int v = 0; v < 5; ++v)
-        HIPCHK(hipMemcpyAsync(A.ckpt[5 * (n / every) + v], st[v], bytes, hipMemcpyDeviceToDevice, c.stream));
-    if (A.tseg > 0 && n >= nfirst) {
-      for (int v = 0; v < 5; ++v)
-        HIPCHK(hipMemcpyAsync(A.seg[5 * (n - nfirst) + v], st[v], bytes, hipMemcpyDeviceToDevice, c.stream));
-      HIPCHK(model_large_step(c, &A.recs[n - nfirst]));
-    } else {
-      HIPCHK(model_large_step(c));
-    }
+  const int nseg = (A.nsteps + A.every - 1) / A.every;
+  const int nfirst = (nseg - A.tseg) * A.every;  // first taped step
+  if (n % A.every == 0)
+    for (int v = 0; v < 5; ++v)
+      HIPCHK(hipMemcpyAsync(A.ckpt[5 * (n / A.every) + v], st[v], bytes, hipMemcpyDeviceToDevice, c.stream));
+  if (A.tseg > 0 && n >= nfirst) {
+    for (int v = 0; v < 5; ++v)
+      HIPCHK(hipMemcpyAsync(A.seg[5 * (n - nfirst) + v], st[v], bytes, hipMemcpyDeviceToDevice, c.stream));
+    return model_large_step(c, &A.recs[n - nfirst]);
   }
+  return model_large_step(c);
+}
+
+static void fwd_end(Ctx& c) {
+  c.adj->recorded = 1;
+  c.adj->taped = c.adj->tseg;
+}
+
+hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every) {
+  HIPCHK(fwd_begin(c, nsteps, every, 1));
+  prof_start(c, "model_forward_checkpointed", c.stream);
+  for (int n = 0; n < nsteps; ++n) HIPCHK(fwd_step(c, n));
   prof_stop(c, c.stream);
-  A.recorded = 1;
-  A.taped = A.tseg;
+  fwd_end(c);
+  return hipSuccess;
+}
+
+// The reverse sweep in pieces: rev_begin keeps the final state and prepares
+// the tendencies, rev_segment(sg) reverses segment sg (the segments run from
+// the last to the first), rev_end restores the final state.
+static hipError_t rev_begin(Ctx& c) {
+  HIPCHK(check_adjoint_model(c));
+  if (!c.adj || !c.adj->recorded) return hipErrorNotReady;
+  AdjWork& A = *c.adj;
+  Model& M = *c.model;
+  const size_t bytes = c.d.n3 * sizeof(double);
+  double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
+  for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.xb[v], st[v], bytes, hipMemcpyDeviceToDevice, c.stream));
+  return model_prepare(c);  // the tendencies M.t, which every step shares
+}
+
+static hipError_t rev_segment(Ctx& c, int sg, double* const xb[5]) {
+  AdjWork& A = *c.adj;
+  Model& M = *c.model;
+  hipStream_t s = c.stream;
+  const size_t bytes = c.d.n3 * sizeof(double);
+  double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
+  const int nseg = (A.nsteps + A.every - 1) / A.every;
+  const bool recs = A.tseg > 0;
+  const int n0 = sg * A.every, n1 = std::min(A.nsteps, n0 + A.every);
+  // a taped segment reverses from its own slot of the tapes; the others
+  // record into slot 0, free by then (its segment is reversed before them)
+  const bool have = recs && sg >= nseg - A.taped;
+  const int base = have ? (sg - (nseg - A.tseg)) * A.every : 0;
+  auto seg = [&](int n, double** x) { for (int v = 0; v < 5; ++v) x[v] = A.seg[5 * (base + n - n0) + v]; };
+  if (!have) {
+    for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[v], A.ckpt[5 * sg + v], bytes, hipMemcpyDeviceToDevice, s));
+    // slot 0 no longer holds the tape of its own segment (the earliest
+    // taped one): a later reverse sweep of the same run records it again
+    if (recs) A.taped = std::min(A.taped, A.tseg - 1);
+  }
+  if (recs) {
+    // one recording forward over the segment unless the checkpointed
+    // forward kept its tape, then the reverse steps
+    for (int n = n0; n < n1 && !have; ++n) {
+      double *x[5], *xn[5];
+      seg(n, x);
+      if (n + 1 < n1) seg(n + 1, xn);
+      HIPCHK(step_fwd(c, x, A.recs[base + n - n0], n + 1 < n1 ? xn : nullptr));
+    }
+    for (int n = n1 - 1; n >= n0; --n) {
+      double* x[5];
+      seg(n, x);
+      HIPCHK(step_rev(c, x, xb, A.recs[base + n - n0]));
+    }
+    return hipSuccess;
+  }
+  // recompute the states of the segment, then each reversed step its own record
+  const StepRec one = model_rec(c);
+  for (int n = n0 + 1; n < n1; ++n) {
+    for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(st[v], A.seg[5 * (n - 1 - n0) + v], bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(model_prepare(c));
+    HIPCHK(model_short_step_rk4(c, M.dts));
+    for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[5 * (n - n0) + v], st[v], bytes, hipMemcpyDeviceToDevice, s));
+  }
+  for (int n = n1 - 1; n >= n0; --n) {
+    double* x[5];
+    seg(n, x);
+    HIPCHK(step_fwd(c, x, one, nullptr));
+    HIPCHK(step_rev(c, x, xb, one));
+  }
+  return hipSuccess;
+}
+
+static hipError_t rev_end(Ctx& c) {
+  AdjWork& A = *c.adj;
+  Model& M = *c.model;
+  const size_t bytes = c.d.n3 * sizeof(double);
+  double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
+  for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(st[v], A.xb[v], bytes, hipMemcpyDeviceToDevice, c.stream));
   return hipSuccess;
 }
 
@@ -1395,66 +1486,39 @@ hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every) {
 // of the state at its start on return.  The model state is left as the run
 // left it.
 hipError_t model_adjoint(Ctx& c, double* const xb[5]) {
-  HIPCHK(check_adjoint_model(c));
-  if (!c.adj || !c.adj->recorded) return hipErrorNotReady;
-  AdjWork& A = *c.adj;
-  Model& M = *c.model;
-  hipStream_t s = c.stream;
-  const size_t bytes = c.d.n3 * sizeof(double);
-  double* st[5] = {M.DENS, M.MOMZ, M.MOMX, M.MOMY, M.RHOT};
-  double* keep[5];
-  for (int v = 0; v < 5; ++v) keep[v] = A.xb[v];  // the final state, restored at the end
-  for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(keep[v], st[v], bytes, hipMemcpyDeviceToDevice, s));
-  prof_start(c, "model_adjoint", s);
-  HIPCHK(model_prepare(c));  // the tendencies M.t, which every step shares
-  const int nseg = (A.nsteps + A.every - 1) / A.every;
-  const bool recs = A.tseg > 0;
-  const StepRec one = model_rec(c);
-  for (int sg = nseg - 1; sg >= 0; --sg) {
-    const int n0 = sg * A.every, n1 = std::min(A.nsteps, n0 + A.every);
-    // a taped segment reverses from its own slot of the tapes; the others
-    // record into slot 0, free by then (its segment is reversed before them)
-    const bool have = recs && sg >= nseg - A.taped;
-    const int base = have ? (sg - (nseg - A.tseg)) * A.every : 0;
-    auto seg = [&](int n, double** x) { for (int v = 0; v < 5; ++v) x[v] = A.seg[5 * (base + n - n0) + v]; };
-    if (!have) {
-      for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[v], A.ckpt[5 * sg + v], bytes, hipMemcpyDeviceToDevice, s));
-      // slot 0 no longer holds the tape of its own segment (the earliest
-      // taped one): a later reverse sweep of the same run records it again
-      if (recs) A.taped = std::min(A.taped, A.tseg - 1);
+  HIPCHK(rev_begin(c));
+  const int nseg = (c.adj->nsteps + c.adj->every - 1) / c.adj->every;
+  prof_start(c, "model_adjoint", c.stream);
+  for (int sg = nseg - 1; sg >= 0; --sg) HIPCHK(rev_segment(c, sg, xb));
+  prof_stop(c, c.stream);
+  return rev_end(c);
+}
+
+// Batches of models (members of any grid size, flat or terrain), each on
+// its own stream: the checkpointed forwards advance step by step, member
+// after member, and the reverse sweeps segment by segment, so that the
+// kernels of different members overlap on the device.  The members split
+// the device memory for tapes.  xb[5 m + v]: field v of member m.
+hipError_t batch_forward_checkpointed(Ctx* const* cs, int n, int nsteps, int every) {
+  for (int m = 0; m < n; ++m) HIPCHK(check_adjoint_model(*cs[m]));
+  for (int m = 0; m < n; ++m) HIPCHK(fwd_begin(*cs[m], nsteps, every, n));
+  for (int k = 0; k < nsteps; ++k)
+    for (int m = 0; m < n; ++m) HIPCHK(fwd_step(*cs[m], k));
+  for (int m = 0; m < n; ++m) fwd_end(*cs[m]);
+  return hipSuccess;
+}
+
+hipError_t batch_adjoint(Ctx* const* cs, int n, double* const* xb) {
+  for (int m = 0; m < n; ++m) HIPCHK(rev_begin(*cs[m]));
+  int nseg = 0;
+  for (int m = 0; m < n; ++m) nseg = std::max(nseg, (cs[m]->adj->nsteps + cs[m]->adj->every - 1) / cs[m]->adj->every);
+  for (int sg = nseg - 1; sg >= 0; --sg)
+    for (int m = 0; m < n; ++m) {
+      const AdjWork& A = *cs[m]->adj;
+      const int off = nseg - (A.nsteps + A.every - 1) / A.every;  // runs of fewer segments end together
+      if (sg - off >= 0) HIPCHK(rev_segment(*cs[m], sg - off, xb + 5 * (size_t)m));
     }
-    if (recs) {
-      // one recording forward over the segment unless the checkpointed
-      // forward kept its tape, then the reverse steps
-      for (int n = n0; n < n1 && !have; ++n) {
-        double *x[5], *xn[5];
-        seg(n, x);
-        if (n + 1 < n1) seg(n + 1, xn);
-        HIPCHK(step_fwd(c, x, A.recs[base + n - n0], n + 1 < n1 ? xn : nullptr));
-      }
-      for (int n = n1 - 1; n >= n0; --n) {
-        double* x[5];
-        seg(n, x);
-        HIPCHK(step_rev(c, x, xb, A.recs[base + n - n0]));
-      }
-    } else {
-      // recompute the states of the segment, then each reversed step its own record
-      for (int n = n0 + 1; n < n1; ++n) {
-        for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(st[v], A.seg[5 * (n - 1 - n0) + v], bytes, hipMemcpyDeviceToDevice, s));
-        HIPCHK(model_prepare(c));
-        HIPCHK(model_short_step_rk4(c, M.dts));
-        for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(A.seg[5 * (n - n0) + v], st[v], bytes, hipMemcpyDeviceToDevice, s));
-      }
-      for (int n = n1 - 1; n >= n0; --n) {
-        double* x[5];
-        seg(n, x);
-        HIPCHK(step_fwd(c, x, one, nullptr));
-        HIPCHK(step_rev(c, x, xb, one));
-      }
-    }
-  }
-  prof_stop(c, s);
-  for (int v = 0; v < 5; ++v) HIPCHK(hipMemcpyAsync(st[v], keep[v], bytes, hipMemcpyDeviceToDevice, s));
+  for (int m = 0; m < n; ++m) HIPCHK(rev_end(*cs[m]));
   return hipSuccess;
 }
 

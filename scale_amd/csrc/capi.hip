@@ -1136,6 +1136,61 @@ int scale_hip_model_adjoint(scale_hip_ctx* h, double* DENS_b, double* MOMZ_b, do
   return 0;
 }
 
+// the member contexts of a batch: each with its model, all on one device
+static int batch_members(scale_hip_ctx* const* members, int32_t n, std::vector<Ctx*>& cs) {
+  if (!members || n <= 0) return fail("batch: members and n > 0 are required");
+  for (int m = 0; m < n; ++m) {
+    if (!members[m]) return fail("batch: null member context");
+    Ctx& c = members[m]->c;
+    if (!c.model) return fail("batch: every member needs a created model");
+    if (c.device != members[0]->c.device) return fail("batch: the members must be on one device");
+    for (Ctx* o : cs)
+      if (o == &c) return fail("batch: a context appears twice");
+    cs.push_back(&c);
+  }
+  CK(hipSetDevice(cs[0]->device));
+  return 0;
+}
+
+int scale_hip_batch_forward_checkpointed(scale_hip_ctx* const* members, int32_t n, int32_t nsteps, int32_t every) {
+  std::vector<Ctx*> cs;
+  if (int r = batch_members(members, n, cs)) return r;
+  if (nsteps < 1 || every < 1) return fail("nsteps and every must be positive");
+  hipError_t e = shp::batch_forward_checkpointed(cs.data(), n, nsteps, every);
+  if (e == hipErrorNotSupported)
+    return fail("the adjoint supports FP64, RK4 short steps, divdmp_coef = 0 and one short step per large step");
+  CK(e);
+  for (Ctx* c : cs) CK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int scale_hip_batch_adjoint(scale_hip_ctx* const* members, int32_t n, double* const* DENS_b, double* const* MOMZ_b,
+                            double* const* MOMX_b, double* const* MOMY_b, double* const* RHOT_b) {
+  std::vector<Ctx*> cs;
+  if (int r = batch_members(members, n, cs)) return r;
+  double* const* hb[5] = {DENS_b, MOMZ_b, MOMX_b, MOMY_b, RHOT_b};
+  for (int v = 0; v < 5; ++v)
+    if (!hb[v]) return fail("batch_adjoint: null array of member fields");
+  std::vector<double*> xb(5 * (size_t)n);
+  for (int m = 0; m < n; ++m) {
+    Ctx& c = *cs[m];
+    CK(shp::adj_setup(c, 0, 0));
+    for (int v = 0; v < 5; ++v) {
+      xb[5 * (size_t)m + v] = shp::adj_io(c, v);
+      if (int r = upload3(c, xb[5 * (size_t)m + v], hb[v][m], 1)) return r;
+    }
+  }
+  hipError_t e = shp::batch_adjoint(cs.data(), n, xb.data());
+  if (e == hipErrorNotReady) return fail("scale_hip_batch_forward_checkpointed must run first");
+  if (e == hipErrorNotSupported)
+    return fail("the adjoint supports FP64, RK4 short steps, divdmp_coef = 0 and one short step per large step");
+  CK(e);
+  for (int m = 0; m < n; ++m)
+    for (int v = 0; v < 5; ++v)
+      if (int r = download3(*cs[m], hb[v][m], xb[5 * (size_t)m + v], 1)) return r;
+  return 0;
+}
+
 int scale_hip_profile_enable(scale_hip_ctx* h, int32_t on) {
   if (int r = check_ctx(h)) return r;
   Ctx& c = h->c;

@@ -125,5 +125,8 @@ hipError_t model_forward_checkpointed(Ctx& c, int nsteps, int every);
 hipError_t rk4_combine_state(Ctx& c, double* const out[5], const double* const s0[5], double* const Rk[4][5], double* mf,
                              double* tf, hipStream_t s);
 hipError_t model_adjoint(Ctx& c, double* const xb[5]);
+// the same for n models at once, each on its own stream (xb[5 m + v])
+hipError_t batch_forward_checkpointed(Ctx* const* cs, int n, int nsteps, int every);
+hipError_t batch_adjoint(Ctx* const* cs, int n, double* const* xb);
 
 }  // namespace shp

@@ -232,7 +232,7 @@ hipError_t hevi_stage_adj(Ctx& c, const StageAdjIO& io, hipStream_t s);
 hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const double* REF_dens, const double* REF_pott,
                          double* const stb[5], hipStream_t s);
 // adjoint work arrays (nsteps > 0: also checkpoints of a run of nsteps saved every 'every')
-hipError_t adj_setup(Ctx& c, int nsteps, int every);
+hipError_t adj_setup(Ctx& c, int nsteps, int every, int share = 1);
 void adj_free(Ctx& c);
 double* adj_io(Ctx& c, int v);  // five device fields for the adjoint arguments of the C ABI
 // output buffers of a stage adjoint called through the C ABI (after adj_setup)

@@ -450,6 +450,30 @@ def model_from_case(case, ctx: Context | None = None, fct=None, nd_coef=1.0e-4, 
         ctx.model_set(QTRC_BASE + iq, np.asfortranarray(case.qtrc[..., iq]))
     return ctx
 
+def batch_forward_checkpointed(members, nsteps, every):
+    """The checkpointed forward of several member models at once
+    (scale_hip_batch_forward_checkpointed): each member on its own stream,
+    advanced step by step across the members; the members split the device
+    memory for tapes."""
+    n = len(members)
+    arr = (C.c_void_p * n)(*[m._h.value for m in members])
+    _check(load_library().scale_hip_batch_forward_checkpointed(arr, C.c_int32(n), C.c_int32(int(nsteps)),
+                                                               C.c_int32(int(every))))
+
+
+def batch_adjoint(members, xbs):
+    """Adjoints of the last batch_forward_checkpointed run
+    (scale_hip_batch_adjoint): xbs[m] holds member m's adjoints of DENS..RHOT
+    after the run (whole arrays); returns their adjoints at its start, one
+    dict per member."""
+    n = len(members)
+    outs = [{k: np.array(xb[k], dtype=np.float64, order="F", copy=True) for k in STATE} for xb in xbs]
+    arr = (C.c_void_p * n)(*[m._h.value for m in members])
+    ptrs = [(C.POINTER(C.c_double) * n)(*[_p(o[k]) for o in outs]) for k in STATE]
+    _check(load_library().scale_hip_batch_adjoint(arr, C.c_int32(n), *ptrs))
+    return outs
+
+
 class Ensemble:
     """Member contexts (each with its model) stepped together
     (scale_hip_ensemble_*): the filter and the HEVI stages of all members run

@@ -340,6 +340,23 @@ def bell_topography(g: Grid, height=500.0, half_width=5000.0, cx=None, cy=None, 
     return z
 
 
+def mountain_members(n=32, imax=128, jmax=128, kmax=64, ntracer=5, height=500.0, half_width=5000.0, seed=20261202 + 4000):
+    """Members of a batched adjoint over a bell mountain: n members of
+    imax x jmax x kmax (DX = DY = 500 m, DZ = 250 m, dt = 1 s), stable
+    stratification N = 0.012 1/s, uniform u = 10 m/s, no bubble, theta noise
+    N(0, 0.1 K) from each member's own seed, ntracer FCT tracers; each with the
+    surface height of a bell mountain (height, half_width) centred in its
+    domain.  Returns [(case, zsfc)]."""
+    kinds = ["blob", "tophat", "uniform", "blob", "tophat"]
+    out = []
+    for m in range(n):
+        case = make_case(kmax=kmax, imax=imax, jmax=jmax, dx=500.0, dy=500.0, dz=250.0, seed=seed + m, bv=0.012,
+                         u0=10.0, bubble=False, noise=0.1, ntracer=ntracer,
+                         tracer_kind=[kinds[q % len(kinds)] for q in range(ntracer)], fct=True, dt=1.0)
+        out.append((case, bell_topography(case.grid, height=height, half_width=half_width)))
+    return out
+
+
 def large_step_prep(case: Case):
     """RT2P, DPRES0, REF_rhot of LARGE.F90:443-471 (dry form) over the whole array."""
     g = case.grid
