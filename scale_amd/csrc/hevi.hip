@@ -17,6 +17,7 @@
 //               x and y face once;
 //   hevi_march  general metrics: neighbours come through the caches and each
 //               point evaluates the faces it needs.
+#include <cstdlib>
 #include <vector>
 #include "common.h"
 #include "shp_internal.h"
@@ -52,6 +53,8 @@ struct HeviArgsT {
   // scratch (n3 each): in FP32 the Thomas coefficients are rounded to float
   // between the sweeps (PT is a float value already), like the prognostics
   R *PT, *e, *f;
+  // Sw between the two launches of the split terrain stage (FP64 only)
+  double* Sw = nullptr;
   // optional records for the adjoint: St and the solved C
   double *trSt = nullptr, *trCp = nullptr;
   // mixed precision: DENS and RHOT (and their stage values) are stored as
@@ -431,7 +434,12 @@ __device__ __forceinline__ double momy_core(const Ops<S, TERRAIN, DIVDMP>& o, co
   return t;
 }
 
-template <int S, bool TERRAIN, bool DIVDMP>
+// PART selects the sections evaluated (P_ALL for the whole point): the
+// continuity and theta fluxes with Sr and St (P_CT), the explicit part of
+// MOMZ with PT (P_W), MOMX (P_U) and MOMY (P_V); each section carries its
+// own z fluxes in zc.
+enum { P_CT = 1, P_W = 2, P_U = 4, P_V = 8, P_ALL = 15 };
+template <int S, bool TERRAIN, bool DIVDMP, int PART = P_ALL>
 __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP>& o, const HeviArgs& a, int k, int i, int j,
                                                     ZCarry& zc) {
   const Dims& d = a.d;
@@ -443,33 +451,35 @@ __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP
   const auto& m = o.m;
 
   // ---- continuity (HEVI.F90:369-475) ----
-  const double mX = o.mflxX(k, i, j), mXm = o.mflxX(k, i - 1, j);
-  const double mY = o.mflxY(k, i, j), mYm = o.mflxY(k, i, j - 1);
-  r.mX = mX; r.mXm = mXm; r.mY = mY; r.mYm = mYm;
-  const double mZ = o.mflxZf(k, i, j), mZm = zc.mZ;
-  zc.mZ = mZ;
-  r.mZ = mZ;
-  r.mZw = (i == d.IS && k <= d.KE - 1) ? o.mflxZ(k, i - 1, j) : 0.0;
-  {
-    double advcv = -(mZ - mZm) * RCDZ[k];
-    double advch = -((mX - mXm) * RCDX[i] + (mY - mYm) * RCDY[j]);
-    r.Sr = (advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p) + a.DENS_t[p];
-  }
+  if constexpr ((PART & P_CT) != 0) {
+    const double mX = o.mflxX(k, i, j), mXm = o.mflxX(k, i - 1, j);
+    const double mY = o.mflxY(k, i, j), mYm = o.mflxY(k, i, j - 1);
+    r.mX = mX; r.mXm = mXm; r.mY = mY; r.mYm = mYm;
+    const double mZ = o.mflxZf(k, i, j), mZm = zc.mZ;
+    zc.mZ = mZ;
+    r.mZ = mZ;
+    r.mZw = (i == d.IS && k <= d.KE - 1) ? o.mflxZ(k, i - 1, j) : 0.0;
+    {
+      double advcv = -(mZ - mZm) * RCDZ[k];
+      double advch = -((mX - mXm) * RCDX[i] + (mY - mYm) * RCDY[j]);
+      r.Sr = (advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p) + a.DENS_t[p];
+    }
 
-  // ---- thermodynamics (HEVI.F90:588-657) ----
-  {
-    const double tX = o.tflxX(k, i, j, mX), tXm = o.tflxX(k, i - 1, j, mXm);
-    const double tY = o.tflxY(k, i, j, mY), tYm = o.tflxY(k, i, j - 1, mYm);
-    const double tZ = o.tflxZw(k, i, j, mZ, zc.pw), tZm = zc.tZ;
-    zc.tZ = tZ;
-    r.tX = tX; r.tXm = tXm; r.tY = tY; r.tYm = tYm; r.tZ = tZ;
-    double advcv = -(tZ - tZm) * RCDZ[k];
-    double advch = -((tX - tXm) * RCDX[i] + (tY - tYm) * RCDY[j]);
-    r.St = (advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p) + a.RHOT_t[p];
+    // ---- thermodynamics (HEVI.F90:588-657) ----
+    {
+      const double tX = o.tflxX(k, i, j, mX), tXm = o.tflxX(k, i - 1, j, mXm);
+      const double tY = o.tflxY(k, i, j, mY), tYm = o.tflxY(k, i, j - 1, mYm);
+      const double tZ = o.tflxZw(k, i, j, mZ, zc.pw), tZm = zc.tZ;
+      zc.tZ = tZ;
+      r.tX = tX; r.tXm = tXm; r.tY = tY; r.tYm = tYm; r.tZ = tZ;
+      double advcv = -(tZ - tZm) * RCDZ[k];
+      double advch = -((tX - tXm) * RCDX[i] + (tY - tYm) * RCDY[j]);
+      r.St = (advcv + advch) * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p) + a.RHOT_t[p];
+    }
   }
 
   // ---- momentum z, explicit part (HEVI.F90:486-577) and PT (HEVI.F90:691) ----
-  if (k <= d.KE - 1) {
+  if ((PART & P_W) != 0 && k <= d.KE - 1) {
     const double qw = o.qZ_XYW(k, i, j);
     double sv = qw - zc.qW;
     zc.qW = qw;
@@ -490,7 +500,7 @@ __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP
   }
 
   // ---- momentum x (HEVI.F90:816-927), iee = min(IIE,IEH) = IE for periodic x ----
-  {
+  if constexpr ((PART & P_U) != 0) {
     double t = momx_core(o, a, k, i, j, zc);
     if constexpr (DIVDMP) {
       t = t + a.divdmp_coef / a.dtrk * (a.DDIV[IX(d, k, i + 1, j)] / m.mapf(i + 1, j, 1, I_XY) - a.DDIV[p] / m.mapf(i, j, 0, I_XY))
@@ -500,7 +510,7 @@ __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP
   }
 
   // ---- momentum y (HEVI.F90:936-1047) ----
-  {
+  if constexpr ((PART & P_V) != 0) {
     double t = momy_core(o, a, k, i, j, zc);
     if constexpr (DIVDMP) {
       t = t + a.divdmp_coef / a.dtrk * (a.DDIV[IX(d, k, i, j + 1)] / m.mapf(i, j + 1, 0, I_XY) - a.DDIV[p] / m.mapf(i, j, 0, I_XY))
@@ -514,26 +524,29 @@ __device__ __forceinline__ PointTend explicit_point(const Ops<S, TERRAIN, DIVDMP
 // The stores of one point, issued after all of its loads: the mass and
 // theta fluxes (with the extra faces i = IS-1, j = JS-1 and the zero z faces
 // KS-1 and KE, HEVI.F90:370-438, 596-622) and MOMX_RK/MOMY_RK.
+template <int PART = P_ALL>
 __device__ __forceinline__ void store_point(const HeviArgs& a, int k, int i, int j, const PointTend& r) {
   const Dims& d = a.d;
   const size_t p = IX(d, k, i, j);
-  double* mfZ = a.mflx_hi; double* mfX = a.mflx_hi + d.n3; double* mfY = a.mflx_hi + 2 * d.n3;
-  double* tfZ = a.tflx_hi; double* tfX = a.tflx_hi + d.n3; double* tfY = a.tflx_hi + 2 * d.n3;
-  mfX[p] = r.mX;
-  mfY[p] = r.mY;
-  tfX[p] = r.tX;
-  tfY[p] = r.tY;
-  if (i == d.IS) { mfX[p - 1] = r.mXm; tfX[p - 1] = r.tXm; }
-  if (j == d.JS) { mfY[p - d.IA] = r.mYm; tfY[p - d.IA] = r.tYm; }
-  if (k <= d.KE - 1) { mfZ[p] = r.mZ; tfZ[p] = r.tZ; }
-  else { mfZ[p] = 0.0; tfZ[p] = 0.0; }
-  if (k == d.KS) { mfZ[p - d.pl] = 0.0; tfZ[p - d.pl] = 0.0; }
-  if (i == d.IS) {  // mflx_hi(Z) is also defined on the i = IS-1 column (HEVI.F90:370)
-    mfZ[p - 1] = r.mZw;
-    if (k == d.KS) mfZ[p - 1 - d.pl] = 0.0;
+  if constexpr ((PART & P_CT) != 0) {
+    double* mfZ = a.mflx_hi; double* mfX = a.mflx_hi + d.n3; double* mfY = a.mflx_hi + 2 * d.n3;
+    double* tfZ = a.tflx_hi; double* tfX = a.tflx_hi + d.n3; double* tfY = a.tflx_hi + 2 * d.n3;
+    mfX[p] = r.mX;
+    mfY[p] = r.mY;
+    tfX[p] = r.tX;
+    tfY[p] = r.tY;
+    if (i == d.IS) { mfX[p - 1] = r.mXm; tfX[p - 1] = r.tXm; }
+    if (j == d.JS) { mfY[p - d.IA] = r.mYm; tfY[p - d.IA] = r.tYm; }
+    if (k <= d.KE - 1) { mfZ[p] = r.mZ; tfZ[p] = r.tZ; }
+    else { mfZ[p] = 0.0; tfZ[p] = 0.0; }
+    if (k == d.KS) { mfZ[p - d.pl] = 0.0; tfZ[p - d.pl] = 0.0; }
+    if (i == d.IS) {  // mflx_hi(Z) is also defined on the i = IS-1 column (HEVI.F90:370)
+      mfZ[p - 1] = r.mZw;
+      if (k == d.KS) mfZ[p - 1 - d.pl] = 0.0;
+    }
   }
-  a.MOMX_RK[p] = r.MX;
-  a.MOMY_RK[p] = r.MY;
+  if constexpr ((PART & P_U) != 0) a.MOMX_RK[p] = r.MX;
+  if constexpr ((PART & P_V) != 0) a.MOMY_RK[p] = r.MY;
 }
 
 // ---------------------------------------------------------------------------
@@ -657,6 +670,229 @@ __global__ void __launch_bounds__(256) hevi_march(HeviArgs a, unsigned ni_tiles,
     a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + St_[q]);
     a.MOMZ_RK[px(KS - 1)] = 0.0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// The same stage split in launches, the terrain stage's default
+// (hevi_march runs one thread per column: 16K columns at 128x128 are 256
+// waves, and the whole point needs 256 VGPRs).  hevi_expl: the explicit part
+// (the sections PART of it), one thread per segment of L levels of a column,
+// the carried z fluxes of the face below the segment computed afresh (the
+// carry of hevi_march at that face); it stores the mass and theta fluxes,
+// MOMX_RK/MOMY_RK, Sr/St (in DENS_RK/RHOT_RK), PT and Sw.  hevi_solve: one
+// thread per column, the tridiagonal system, its forward sweep and the back
+// substitution of hevi_march from those arrays.  The expressions and their
+// order are hevi_march's: the results are its bits.
+template <int S, bool TERRAIN, bool DIVDMP, int PART>
+__global__ void __launch_bounds__(256) hevi_expl(HeviArgs a, unsigned ni_tiles, unsigned ncols, unsigned nblocks, int L) {
+  const Dims& d = a.d;
+  Ops<S, TERRAIN, DIVDMP> o(a);
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const unsigned col = lb % ncols, seg = lb / ncols;
+  const int i = d.IS + (int)((col % ni_tiles) * 64 + threadIdx.x);
+  const int j = d.JS + (int)((col / ni_tiles) * blockDim.y + threadIdx.y);
+  const int k0 = d.KS + (int)seg * L;
+  if (i > d.IE || j > d.JE || k0 > d.KE) return;
+  const int k1 = min(d.KE, k0 + L - 1);
+  constexpr bool PW = (PART & (P_CT | P_W)) != 0;  // the sections that read POTT around k
+  ZCarry zc;
+  if (k0 > d.KS) {
+    const int kb = k0 - 1;  // the face below the segment, as level kb of hevi_march leaves it
+    if constexpr ((PART & P_CT) != 0) {
+      double pw[4] = {o.POTT(kb - 1, i, j), o.POTT(kb, i, j), o.POTT(kb + 1, i, j), o.POTT(kb + 2, i, j)};
+      zc.mZ = o.mflxZf(kb, i, j);
+      zc.tZ = o.tflxZw(kb, i, j, zc.mZ, pw);
+    }
+    if constexpr ((PART & P_W) != 0) {
+      zc.qW = o.qZ_XYW(kb, i, j);
+      if constexpr (TERRAIN) { zc.qWJ13 = o.qJ_XYW(0, kb, i, j); zc.qWJ23 = o.qJ_XYW(1, kb, i, j); }
+    }
+    if constexpr ((PART & P_U) != 0) {
+      zc.qU = o.qZ_UV(0, kb, i, j);
+      if constexpr (TERRAIN) { zc.qUJ13 = o.qJ_UV(0, 0, kb, i, j); zc.qUJ23 = o.qJ_UV(0, 1, kb, i, j); }
+    }
+    if constexpr ((PART & P_V) != 0) {
+      zc.qV = o.qZ_UV(1, kb, i, j);
+      if constexpr (TERRAIN) { zc.qVJ13 = o.qJ_UV(1, 0, kb, i, j); zc.qVJ23 = o.qJ_UV(1, 1, kb, i, j); }
+    }
+  }
+  if constexpr (PW) { zc.pw[1] = o.POTT(k0 - 1, i, j); zc.pw[2] = o.POTT(k0, i, j); zc.pw[3] = o.POTT(k0 + 1, i, j); }
+  for (int k = k0; k <= k1; ++k) {
+    if constexpr (PW) {
+      zc.pw[0] = zc.pw[1]; zc.pw[1] = zc.pw[2]; zc.pw[2] = zc.pw[3];
+      zc.pw[3] = o.POTT(k + 2, i, j);
+    }
+    const PointTend t = explicit_point<S, TERRAIN, DIVDMP, PART>(o, a, k, i, j, zc);
+    const size_t q = IX(d, k, i, j);
+    store_point<PART>(a, k, i, j, t);
+    if constexpr ((PART & P_CT) != 0) {
+      a.DENS_RK[q] = t.Sr;
+      a.RHOT_RK[q] = t.St;
+      if (a.trSt) a.trSt[q] = t.St;
+    }
+    if constexpr ((PART & P_W) != 0) {
+      if (k <= d.KE - 1) { a.PT[q] = t.PT; a.Sw[q] = t.Sw; }
+    }
+  }
+}
+
+template <int S, bool TERRAIN, bool DIVDMP>
+__global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles, unsigned nblocks) {
+  const Dims& d = a.d;
+  Ops<S, TERRAIN, DIVDMP> o(a);
+  const unsigned lb = xcd_block(blockIdx.x, nblocks);
+  const int i = d.IS + (int)((lb % ni_tiles) * 64 + threadIdx.x);
+  const int j = d.JS + (int)((lb / ni_tiles) * blockDim.y + threadIdx.y);
+  if (i > d.IE || j > d.JE) return;
+  const auto& m = o.m;
+  const double* CDZ = a.g.CDZ; const double* RCDZ = a.g.RCDZ; const double* RFDZ = a.g.RFDZ;
+  const double dtrk = a.dtrk, J33G = a.J33G;
+  const int KS = d.KS, KE = d.KE;
+  auto px = [&](int k) { return IX(d, k, i, j); };
+  auto dpres = [&](int k) { size_t q = px(k); return a.DPRES0[q] + a.RT2P[q] * (a.RHOT[q] - a.REF_rhot[q]); };
+  auto Af = [&](int k, double rt) { return dtrk * dtrk * J33G * RCDZ[k] * rt * J33G / m.gs(I_XYZ, px(k)); };
+  auto f2h = [&](int k, int pp) {
+    return CDZ[k + pp - 1] * m.gs(I_XYZ, px(k + pp - 1)) / (CDZ[k] * m.gs(I_XYZ, px(k)) + CDZ[k + 1] * m.gs(I_XYZ, px(k + 1)));
+  };
+  double* Sr_ = a.DENS_RK;
+  double* St_ = a.RHOT_RK;
+
+  // ---- upward: row k-1 of the system once level k is known ----
+  double rt0 = 0.0, dp0 = 0.0, st0 = 0.0, sr0 = 0.0, d0 = 0.0, rd0 = 0.0, A0 = 0.0, pt0 = 0.0, ptm = 0.0, sw0 = 0.0;
+  double e_prev = 0.0, f_prev = 0.0, C_last = 0.0;
+  for (int k = KS; k <= KE; ++k) {
+    const size_t q1 = px(k);
+    const double rt1 = a.RT2P[q1], dp1 = dpres(k), d1 = a.DENS[q1], rd1 = a.REF_dens[q1], A1 = Af(k, rt1);
+    const double pt1 = (k <= KE - 1) ? a.PT[q1] : 0.0;
+    const double st1 = St_[q1], sr1 = Sr_[q1];
+    const double sw1 = (k <= KE - 1) ? a.Sw[q1] : 0.0;
+    double ek = 0.0, fk = 0.0;
+    if (k > KS) {
+      const int r = k - 1;
+      const size_t q = px(r);
+      const double gw = m.gs(I_XYW, q);
+      const double B = GRAV * (dtrk * dtrk) * J33G / (CDZ[r + 1] + CDZ[r]);
+      const double F1v = -(pt1 * RFDZ[r] * A1 + B) / gw;
+      const double F2v = 1.0 + (pt0 * RFDZ[r] * (A1 + A0)) / gw;
+      const double F3v = -(ptm * RFDZ[r] * A0 - B) / gw;
+      const double pg = -(dp1 + rt1 * dtrk * st1 - dp0 - rt0 * dtrk * st0) * RFDZ[r] * J33G / gw
+                      - GRAV * (f2h(r, 1) * (d1 - rd1 + sr1 * dtrk) + f2h(r, 2) * (d0 - rd0 + sr0 * dtrk));
+      const double Ck = a.MOMZ[q] + dtrk * (pg + sw0);
+      if (r == KS) {
+        const double rdenom = 1.0 / F2v;
+        ek = -F1v * rdenom;
+        fk = Ck * rdenom;
+      } else if (r < KE - 1) {
+        const double rdenom = 1.0 / (F2v + F3v * e_prev);
+        ek = -F1v * rdenom;
+        fk = (Ck - F3v * f_prev) * rdenom;
+      } else {
+        C_last = (Ck - F3v * f_prev) / (F2v + F3v * e_prev);
+      }
+      e_prev = ek; f_prev = fk;
+      ptm = pt0;
+      if (r < KE - 1) { a.e[q] = ek; a.f[q] = fk; }
+    }
+    rt0 = rt1; dp0 = dp1; st0 = st1; sr0 = sr1; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = sw1;
+  }
+
+  // ---- downward: back substitution and updates (HEVI.F90:728-790) ----
+  double* mfZ = a.mflx_hi; double* tfZ = a.tflx_hi;
+  const double M12 = m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY);
+  double Cn = C_last;
+  double ptn = a.PT[px(KE - 1)];
+  {
+    const size_t q = px(KE);
+    double advcv = Cn * J33G * RCDZ[KE] / m.gs(I_XYZ, q);
+    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + Sr_[q]);
+    advcv = Cn * ptn * J33G * RCDZ[KE] / m.gs(I_XYZ, q);
+    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + St_[q]);
+    a.MOMZ_RK[q] = 0.0;
+  }
+  {
+    const size_t q = px(KE - 1);
+    mfZ[q] = mfZ[q] + J33G * Cn / (M12);
+    tfZ[q] = tfZ[q] + J33G * Cn * ptn / (M12);
+    a.MOMZ_RK[q] = a.MOMZ0[q] + (Cn - a.MOMZ[q]);
+    if (a.trCp) a.trCp[q] = Cn;
+  }
+  for (int k = KE - 2; k >= KS; --k) {
+    const size_t q = px(k), q1 = px(k + 1);
+    const double Ck = a.e[q] * Cn + a.f[q];
+    const double ptk = a.PT[q];
+    mfZ[q] = mfZ[q] + J33G * Ck / (M12);
+    tfZ[q] = tfZ[q] + J33G * Ck * ptk / (M12);
+    a.MOMZ_RK[q] = a.MOMZ0[q] + (Ck - a.MOMZ[q]);
+    if (a.trCp) a.trCp[q] = Ck;
+    double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
+    a.DENS_RK[q1] = a.DENS0[q1] + dtrk * (advcv + Sr_[q1]);
+    advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
+    a.RHOT_RK[q1] = a.RHOT0[q1] + dtrk * (advcv + St_[q1]);
+    Cn = Ck; ptn = ptk;
+  }
+  {
+    const size_t q = px(KS);
+    double advcv = -Cn * J33G * RCDZ[KS] / m.gs(I_XYZ, q);
+    a.DENS_RK[q] = a.DENS0[q] + dtrk * (advcv + Sr_[q]);
+    advcv = -Cn * ptn * J33G * RCDZ[KS] / m.gs(I_XYZ, q);
+    a.RHOT_RK[q] = a.RHOT0[q] + dtrk * (advcv + St_[q]);
+    a.MOMZ_RK[px(KS - 1)] = 0.0;
+  }
+}
+
+// levels per segment of hevi_expl; SCALE_HIP_TERRAIN_SEG overrides the
+// default 8 (0: the single-launch hevi_march)
+static int terrain_seg_levels() {
+  const char* e = getenv("SCALE_HIP_TERRAIN_SEG");
+  return e ? atoi(e) : 8;
+}
+
+// the sections of hevi_expl per launch: SCALE_HIP_TERRAIN_PARTS lists the
+// section masks of the launches ("15": one launch; "3,12"; "1,2,4,8").  One
+// launch per section is the default: with the whole point in one launch a
+// thread needs 256 VGPRs (one wave per SIMD); per section 150-256, with two
+// to three waves for the momenta.  Measured at 128x128x64 / 512x512x128
+// (bell mountain, UD3): 1.07 / 25.4 ms per stage in one launch, 0.95 / 22.4
+// in two, 0.76 / 15.9 in four; hevi_march 2.92 / 27.6.
+static std::vector<int> terrain_parts() {
+  const char* e = getenv("SCALE_HIP_TERRAIN_PARTS");
+  std::vector<int> v;
+  for (const char* c = e ? e : "1,2,4,8"; *c;) {
+    char* end = nullptr;
+    const long m = strtol(c, &end, 10);
+    if (end == c) break;
+    v.push_back((int)m);
+    c = *end == ',' ? end + 1 : end;
+  }
+  return v;
+}
+
+template <int S, bool DIVDMP>
+static void launch_terrain(const HeviArgs& a, hipStream_t s) {
+  const Dims& d = a.d;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
+  constexpr unsigned TJ = 4;
+  const unsigned ni = (imax + 63) / 64, ncols = ni * ((jmax + TJ - 1) / TJ);
+  const int L = terrain_seg_levels();
+  if (L <= 0) {
+    hipLaunchKernelGGL((hevi_march<S, true, DIVDMP>), dim3(ncols), dim3(64, TJ), 0, s, a, ni, ncols);
+    return;
+  }
+  const unsigned nseg = (unsigned)((kmax + L - 1) / L), nb = ncols * nseg;
+  const dim3 g(nb), b(64, TJ);
+  for (int part : terrain_parts()) {
+    switch (part) {
+      case P_ALL: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_ALL>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_CT | P_W: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_CT | P_W>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_U | P_V: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_U | P_V>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_CT: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_CT>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_W: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_W>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_U: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_U>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_V: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_V>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      default: break;
+    }
+  }
+  hipLaunchKernelGGL((hevi_solve<S, true, DIVDMP>), dim3(ncols), dim3(64, TJ), 0, s, a, ni, ncols);
 }
 
 // ---------------------------------------------------------------------------
@@ -1469,9 +1705,7 @@ static hipError_t launch_t(const HeviArgs& a, hipStream_t s, bool cmb) {
     else hipLaunchKernelGGL((hevi_tile<S, DIVDMP, TJ, double, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   } else {
     if (cmb) { prof_stop(c, s); return hipErrorNotSupported; }
-    constexpr unsigned TJ = 4;
-    const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
-    hipLaunchKernelGGL((hevi_march<S, TERRAIN, DIVDMP>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+    launch_terrain<S, DIVDMP>(a, s);
   }
   prof_stop(c, s);
   return hipGetLastError();
@@ -1492,7 +1726,7 @@ static void fill_args(HeviArgsT<R>& a, const StageIOT<R>& io) {
   a.divdmp_coef = io.divdmp_coef; a.dtrk = io.dtrk; a.has_coriolis = io.has_coriolis;
   a.DENS_RK = io.o[0]; a.MOMZ_RK = io.o[1]; a.MOMX_RK = io.o[2]; a.MOMY_RK = io.o[3]; a.RHOT_RK = io.o[4];
   a.mflx_hi = io.mflx_hi; a.tflx_hi = io.tflx_hi;
-  a.PT = (R*)c.scr[3]; a.e = (R*)c.scr[4]; a.f = (R*)c.scr[5];
+  a.PT = (R*)c.scr[3]; a.e = (R*)c.scr[4]; a.f = (R*)c.scr[5]; a.Sw = c.scr[6];
   for (int v = 0; v < 5; ++v) { a.c1[v] = io.c1[v]; a.c2[v] = io.c2[v]; a.c3[v] = io.c3[v]; a.st[v] = io.st[v]; }
   for (int n = 0; n < 3; ++n) { a.cm[n] = io.cm[n]; a.ct[n] = io.ct[n]; }
   a.mflx_av = io.mflx_av; a.tflx_av = io.tflx_av;
@@ -1803,10 +2037,8 @@ hipError_t hevi_stage_traced(const StageIO& io, double* PT, double* St, double* 
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
   const unsigned ni = (imax + 63) / 64;
   if (c.terrain) {
-    constexpr unsigned TJ = 4;
-    const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
-    if (c.scheme == UD3) hipLaunchKernelGGL((hevi_march<UD3, true, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
-    else hipLaunchKernelGGL((hevi_march<CD4, true, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
+    if (c.scheme == UD3) launch_terrain<UD3, false>(a, s);
+    else launch_terrain<CD4, false>(a, s);
     return hipGetLastError();
   }
   constexpr int TJ = 8;

@@ -72,6 +72,29 @@ def test_hevi_stage_bitwise(hip, scheme, terrain):
         H.compare(ref[k], got[k], k)
 
 
+@pytest.mark.parametrize("seg,parts", [(0, ""), (1, ""), (3, ""), (8, ""), (64, ""), (8, "15"), (3, "3,12")],
+                         ids=["march", "seg1", "seg3", "seg8", "seg64", "seg8-one_launch", "seg3-two_launches"])
+def test_hevi_stage_terrain_segments(hip, seg, parts, monkeypatch):
+    """The terrain stage split in launches (hevi_expl over column segments of
+    seg levels, each starting from the z fluxes of the face below it, one
+    launch per section of the point by default, then hevi_solve) and in one
+    (seg = 0: hevi_march): bit for bit the oracle's, with segment lengths
+    that do and do not divide the 20 levels and the sections in one, two or
+    four launches."""
+    monkeypatch.setenv("SCALE_HIP_TERRAIN_SEG", str(seg))
+    if parts:
+        monkeypatch.setenv("SCALE_HIP_TERRAIN_PARTS", parts)
+    print(f"[hevi stage] terrain, segments of {seg} levels, sections {parts or 'default'}", flush=True)
+    case = H.small_case(seed=1003)
+    inp = H.stage_inputs(case, seed=23, coriolis=True)
+    metric = H.terrain_metrics(case)
+    inp["wdamp_coef"] = np.linspace(0.0, 0.01, case.grid.KA)
+    ref = oracle_stage(case, inp, 0.5, metric, 3)
+    got = run_stage_gpu(hip, case, inp, 0.5, metric, 3)
+    for k in OUTS:
+        H.compare(ref[k], got[k], k)
+
+
 @pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
 def test_hevi_stage_divdmp(hip, terrain):
     print(f"[hevi stage] divergence damping terrain={terrain}", flush=True)
