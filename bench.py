@@ -312,7 +312,7 @@ def adjoint_leg(nsteps, every, ndirs, eps_list, seed=20261202 + 2000):
         # the error falls with eps until no face velocity crosses zero (the
         # upwind switch is a kink of J), then rounding takes over
         sweep = {}
-        for e in (1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9):
+        for e in ((1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9) if dirs else ()):
             errs = []
             for q in range(min(sweep_dirs, len(dirs))):
                 fd = (J_at(dirs[q], e) - J_at(dirs[q], -e)) / (2.0 * e)
