@@ -122,7 +122,7 @@ enum {
   W_O = 0,                 // 5 traced stage outputs
   W_PT = 5, W_ST, W_CP, W_POTT,
   W_POTTB, W_STB, W_SRB, W_SWB, W_DPRESB,
-  W_QZ, W_QX, W_QY, W_Q13, W_Q23, W_TZ, W_TX, W_TY, W_MZ, W_MX, W_MY,
+  W_MZ, W_MX, W_MY,
   W_CPB, W_PTB, W_AB, W_CB, W_CPV, W_DPV,
   W_FLD, W_FB, W_W0, W_W1, W_W2, W_WB0, W_WB1, W_WB2,
   NW
@@ -286,6 +286,41 @@ __device__ __forceinline__ double mflx_z0(const Met<TER>& m, const Dims& d, cons
          + m.j23(I_XYW, p) * 0.25 / M0 * (MY[p + pl] + MY[p + pl - IA] + MY[p] + MY[p - IA]) + gz;
   else return gz;
 }
+
+// The adjoint of a flux face from the tendency adjoints of the two cells it
+// separates, the transpose of each cell's difference of its faces
+// (tendency(k) = -(F(k) - F(k-1)) R(k) ... / GSQRT(k)): a face gathers
+// -u(lo) R(lo) + u(hi) R(hi) instead of each cell scattering to its faces.
+// u is the cell's tendency adjoint (scale ob) over GSQRT, zero outside the
+// cells [k0, k1] x [IS, IE] x [JS, JE]; the horizontal faces (and, with
+// mm_all, the vertical ones) carry MAPF(1) MAPF(2) of component cM.
+template <bool TER>
+struct FaceAdj {
+  Met<TER> m;
+  Dims d;
+  const double* ob;
+  double scale;
+  int cG, cM, k0, k1;
+  bool mm_all;
+  const double *RZ, *RX, *RY;
+  __device__ __forceinline__ double cell(int k, int i, int j, bool horiz) const {
+    if (k < k0 || k > k1 || i < d.IS || i > d.IE || j < d.JS || j > d.JE) return 0.0;
+    const size_t p = IX(d, k, i, j);
+    double u = scale * ob[p] / m.gs(cG, p);
+    if (horiz || mm_all) u = u * m.mapf(i, j, 0, cM) * m.mapf(i, j, 1, cM);
+    return u;
+  }
+  // the z face between levels k and k+1, the x face between i and i+1, the y face between j and j+1
+  __device__ __forceinline__ double z(int k, int i, int j) const {
+    return -cell(k, i, j, false) * RZ[k] + cell(k + 1, i, j, false) * RZ[k + 1];
+  }
+  __device__ __forceinline__ double x(int k, int i, int j) const {
+    return -cell(k, i, j, true) * RX[i] + cell(k, i + 1, j, true) * RX[i + 1];
+  }
+  __device__ __forceinline__ double y(int k, int i, int j) const {
+    return -cell(k, i, j, true) * RY[j] + cell(k, i, j + 1, true) * RY[j + 1];
+  }
+};
 }  // namespace
 
 template <int S, bool TER>
@@ -318,8 +353,6 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   const double *PT = io.trPT ? io.trPT : w[W_PT], *St = io.trSt ? io.trSt : w[W_ST], *Cp = io.trCp ? io.trCp : w[W_CP];
   double *POTT = w[W_POTT];
   double *POTTb = w[W_POTTB], *Stb = w[W_STB], *Srb = w[W_SRB], *Swb = w[W_SWB], *DPRESb = w[W_DPRESB];
-  double *qZb = w[W_QZ], *qXb = w[W_QX], *qYb = w[W_QY], *q13b = w[W_Q13], *q23b = w[W_Q23];
-  double *tZb = w[W_TZ], *tXb = w[W_TX], *tYb = w[W_TY];
   double *mZb = w[W_MZ], *mXb = w[W_MX], *mYb = w[W_MY];
   double *Cpb = w[W_CPB], *PTb = w[W_PTB], *Cb = w[W_CB], *cpv = w[W_CPV], *dpv = w[W_DPV];
   double *Db = io.sb[0], *MZb = io.sb[1], *MXb = io.sb[2], *MYb = io.sb[3], *RTb = io.sb[4];
@@ -334,6 +367,12 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   const size_t oVZ = nd(I_MOMY, ZDIR), oVX = nd(I_MOMY, XDIR), oVY = nd(I_MOMY, YDIR);
   const size_t oTZ = nd(I_RHOT, ZDIR), oTX = nd(I_RHOT, XDIR), oTY = nd(I_RHOT, YDIR);
   const Box inter{KS, KE, IS, IE, JS, JE};
+  // the face adjoints of MOMY, MOMX, MOMZ (Sw), RHOT (St) and DENS (Sr)
+  const FaceAdj<TER> fV{m, d, obMY, dt, I_XVZ, I_XV, KS, KE, false, g.RCDZ, g.RCDX, g.RFDY};
+  const FaceAdj<TER> fU{m, d, obMX, dt, I_UYZ, I_UY, KS, KE, false, g.RCDZ, g.RFDX, g.RCDY};
+  const FaceAdj<TER> fW{m, d, Swb, 1.0, I_XYW, I_XY, KS, KE - 1, false, g.RFDZ, g.RCDX, g.RCDY};
+  const FaceAdj<TER> fT{m, d, Stb, 1.0, I_XYZ, I_XY, KS, KE, true, g.RCDZ, g.RCDX, g.RCDY};
+  const FaceAdj<TER> fR{m, d, Srb, 1.0, I_XYZ, I_XY, KS, KE, true, g.RCDZ, g.RCDX, g.RCDY};
 
   // POTT = RHOT / DENS (HEVI.F90:588-598)
   HIPCHK(for3(Box{KS, KE, IS - 2, IE + 2, JS - 2, JE + 2}, [=] __device__(int k, int i, int j) {
@@ -346,7 +385,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   // vel = (f1 mom(k+1) + f2 mom(k)) / (f1 <D>(k+1) + f2 <D>(k)); mom is the
   // 4-point mean (quad) where it lives at the other horizontal staggering.
   // stag 0: the UYZ family (D averaged over i, i+1), 1: XVZ (over j, j+1).
-  auto fluxj_uvz = [&](int stag, int jd, const double* val, double* valb, const double* qb) -> hipError_t {
+  auto fluxj_uvz = [&](int stag, int jd, const double* val, double* valb, const FaceAdj<TER> fa) -> hipError_t {
     const bool quad = stag != jd;
     const double* mom = jd == 0 ? MX : MY;
     double* momb = jd == 0 ? MXb : MYb;
@@ -366,7 +405,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       const double vr = num / den;
       const double Jm = jd == 0 ? m.j13(cJ, p) : m.j23(cJ, p);
       const double vel = vr * Jm, mp = m.mapf(i, j, mn, cM);
-      const double fb = qb[p];
+      const double fb = fa.z(k, i, j);
       const double velb = fb * face_adj<S>(zkind(k, KS, KE), val, valb, p, pl, vel, fb * vel / mp) / mp;
       const double vrb = velb * Jm;
       const double numb = vrb / den, denb = -vrb * vr / den;
@@ -388,14 +427,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     aadd(MY0b + p, b);
     const double T = dt * b;
     const double u = T / m.gs(I_XVZ, p);
-    aadd(qZb + p, -u * g.RCDZ[k]); aadd(qZb + p - pl, u * g.RCDZ[k]);
-    if constexpr (TER) {
-      aadd(q13b + p, -u * g.RCDZ[k]); aadd(q13b + p - pl, u * g.RCDZ[k]);
-      aadd(q23b + p, -u * g.RCDZ[k]); aadd(q23b + p - pl, u * g.RCDZ[k]);
-    }
-    const double um = u * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV);
-    aadd(qXb + p, -um * g.RCDX[i]); aadd(qXb + p - 1, um * g.RCDX[i]);
-    aadd(qYb + p, -um * g.RFDY[j]); aadd(qYb + p - IA, um * g.RFDY[j]);
+    // (the flux faces gather their adjoints from obMY: fV)
     // pg: the bracket times MAPF(i,j,2,I_XV)
     const double v = -u * m.mapf(i, j, 1, I_XV);
     aadd(DPRESb + p + IA, v * g.RFDY[j] * m.gs(I_XYZ, p + IA)); aadd(DPRESb + p, -v * g.RFDY[j] * m.gs(I_XYZ, p));
@@ -432,7 +464,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const double f1 = f2h(m, g, d, I_XVW, k, 1, i, j), f2 = f2h(m, g, d, I_XVW, k, 2, i, j);
     const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + IA]) + f2 * 0.5 * (D[p] + D[p + IA]);
     const double vel = 0.5 * (MZ[p] + MZ[p + IA]) / den;
-    const double fb = qZb[p];
+    const double fb = fV.z(k, i, j);
     aadd(NDb + oVZ + p, fb * m.gs(I_XVW, p));
     const double velb = J33G * fb * face_adj<S>(zkind(k, KS, KE), MY, MYb, p, pl, vel, J33G * fb * vel);
     aadd(MZb + p, 0.5 * velb / den); aadd(MZb + p + IA, 0.5 * velb / den);
@@ -441,15 +473,15 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + IA, 0.5 * f2 * denb);
   }, st));
   if constexpr (TER) {
-    HIPCHK(fluxj_uvz(1, 0, MY, MYb, q13b));  // fluxJ13_XVZ
-    HIPCHK(fluxj_uvz(1, 1, MY, MYb, q23b));  // fluxJ23_XVZ
+    HIPCHK(fluxj_uvz(1, 0, MY, MYb, fV));  // fluxJ13_XVZ
+    HIPCHK(fluxj_uvz(1, 1, MY, MYb, fV));  // fluxJ23_XVZ
   }
   // fluxX_XVZ (UD3.F90:1463-1512)
   HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double den = 0.25 * (D[p] + D[p + 1] + D[p + IA] + D[p + 1 + IA]);
     const double vel = 0.5 * (MX[p] + MX[p + IA]) / den;
-    const double fb = qXb[p];
+    const double fb = fV.x(k, i, j);
     const double G = m.gs(I_UVZ, p), gm = G / m.mapf(i, j, 1, I_UV);
     aadd(NDb + oVX + p, fb * G);
     const double velb = fb * gm * face_adj<S>(0, MY, MYb, p, 1, vel, fb * gm * vel);
@@ -461,7 +493,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   HIPCHK(for3(Box{KS, KE, IS, IE, JS, JE + 1}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j), q = p - IA;
     const double vel = 0.5 * (MY[p] + MY[q]) / D[p];
-    const double fb = qYb[q];
+    const double fb = fV.y(k, i, j - 1);
     const double G = m.gs(I_XYZ, p), gm = G / m.mapf(i, j, 0, I_XY);
     aadd(NDb + oVY + p, fb * G);
     const double velb = fb * gm * face_adj<S>(0, MY, MYb, q, IA, vel, fb * gm * vel);
@@ -470,21 +502,13 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   }, st));
 
   // ---- momentum x (HEVI.F90:816-927) ----
-  HIPCHK(hipMemsetAsync(w[W_QZ], 0, (W_Q23 - W_QZ + 1) * n3 * sizeof(double), st));
   HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double b = obMX[p];
     aadd(MX0b + p, b);
     const double T = dt * b;
     const double u = T / m.gs(I_UYZ, p);
-    aadd(qZb + p, -u * g.RCDZ[k]); aadd(qZb + p - pl, u * g.RCDZ[k]);
-    if constexpr (TER) {
-      aadd(q13b + p, -u * g.RCDZ[k]); aadd(q13b + p - pl, u * g.RCDZ[k]);
-      aadd(q23b + p, -u * g.RCDZ[k]); aadd(q23b + p - pl, u * g.RCDZ[k]);
-    }
-    const double um = u * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY);
-    aadd(qXb + p, -um * g.RFDX[i]); aadd(qXb + p - 1, um * g.RFDX[i]);
-    aadd(qYb + p, -um * g.RCDY[j]); aadd(qYb + p - IA, um * g.RCDY[j]);
+    // (the flux faces gather their adjoints from obMX: fU)
     // pg: the bracket times MAPF(i,j,1,I_UY)
     const double v = -u * m.mapf(i, j, 0, I_UY);
     aadd(DPRESb + p + 1, v * g.RFDX[i] * m.gs(I_XYZ, p + 1)); aadd(DPRESb + p, -v * g.RFDX[i] * m.gs(I_XYZ, p));
@@ -521,7 +545,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const double f1 = f2h(m, g, d, I_UYW, k, 1, i, j), f2 = f2h(m, g, d, I_UYW, k, 2, i, j);
     const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + 1]) + f2 * 0.5 * (D[p] + D[p + 1]);
     const double vel = 0.5 * (MZ[p] + MZ[p + 1]) / den;
-    const double fb = qZb[p];
+    const double fb = fU.z(k, i, j);
     aadd(NDb + oUZ + p, fb * m.gs(I_UYW, p));
     const double velb = J33G * fb * face_adj<S>(zkind(k, KS, KE), MX, MXb, p, pl, vel, J33G * fb * vel);
     aadd(MZb + p, 0.5 * velb / den); aadd(MZb + p + 1, 0.5 * velb / den);
@@ -530,14 +554,14 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + 1, 0.5 * f2 * denb);
   }, st));
   if constexpr (TER) {
-    HIPCHK(fluxj_uvz(0, 0, MX, MXb, q13b));  // fluxJ13_UYZ
-    HIPCHK(fluxj_uvz(0, 1, MX, MXb, q23b));  // fluxJ23_UYZ
+    HIPCHK(fluxj_uvz(0, 0, MX, MXb, fU));  // fluxJ13_UYZ
+    HIPCHK(fluxj_uvz(0, 1, MX, MXb, fU));  // fluxJ23_UYZ
   }
   // fluxX_UYZ (UD3.F90:1043-1097): flux(k,i-1,j) at loop index i
   HIPCHK(for3(Box{KS, KE, IS, IE + 1, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j), q = p - 1;
     const double vel = 0.5 * (MX[p] + MX[q]) / D[p];
-    const double fb = qXb[q];
+    const double fb = fU.x(k, i - 1, j);
     const double G = m.gs(I_XYZ, p), gm = G / m.mapf(i, j, 1, I_XY);
     aadd(NDb + oUX + p, fb * G);
     const double velb = fb * gm * face_adj<S>(0, MX, MXb, q, 1, vel, fb * gm * vel);
@@ -549,7 +573,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const size_t p = IX(d, k, i, j);
     const double den = 0.25 * (D[p] + D[p + 1] + D[p + IA] + D[p + 1 + IA]);
     const double vel = 0.5 * (MY[p] + MY[p + 1]) / den;
-    const double fb = qYb[p];
+    const double fb = fU.y(k, i, j);
     const double G = m.gs(I_UVZ, p), gm = G / m.mapf(i, j, 0, I_UV);
     aadd(NDb + oUY + p, fb * G);
     const double velb = fb * gm * face_adj<S>(0, MX, MXb, p, IA, vel, fb * gm * vel);
@@ -663,56 +687,40 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     (void)face_adj<S>(zkind(k, KS, KE), POTT, POTTb, p, pl, MZ[p], PTb[p]);
   }, st));
 
-  // ---- St (HEVI.F90:634-657) and the theta fluxes (UD3.F90:157-354) ----
-  HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
-    const size_t p = IX(d, k, i, j);
-    const double b = Stb[p] * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p);
-    aadd(tZb + p, -b * g.RCDZ[k]); aadd(tZb + p - pl, b * g.RCDZ[k]);
-    aadd(tXb + p, -b * g.RCDX[i]); aadd(tXb + p - 1, b * g.RCDX[i]);
-    aadd(tYb + p, -b * g.RCDY[j]); aadd(tYb + p - IA, b * g.RCDY[j]);
-  }, st));
+  // ---- St (HEVI.F90:634-657) and the theta fluxes (UD3.F90:157-354): each
+  //      face gathers its adjoint from Stb (fT) and owns its mass-flux adjoint ----
   HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double fb = tZb[p], vel = mflx_z0(m, d, MX, MY, ND + oDZ, k, i, j);
+    const double fb = fT.z(k, i, j), vel = mflx_z0(m, d, MX, MY, ND + oDZ, k, i, j);
     aadd(NDb + oTZ + p, fb * m.gs(I_XYW, p));
-    aadd(mZb + p, fb * face_adj<S>(zkind(k, KS, KE), POTT, POTTb, p, pl, vel, fb * vel));
+    mZb[p] += fb * face_adj<S>(zkind(k, KS, KE), POTT, POTTb, p, pl, vel, fb * vel);
   }, st));
   HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double G = m.gs(I_UYZ, p);
-    const double fb = tXb[p], vel = G / m.mapf(i, j, 1, I_UY) * (MX[p] + ND[oDX + p]);
+    const double fb = fT.x(k, i, j), vel = G / m.mapf(i, j, 1, I_UY) * (MX[p] + ND[oDX + p]);
     aadd(NDb + oTX + p, fb * G);
-    aadd(mXb + p, fb * face_adj<S>(0, POTT, POTTb, p, 1, vel, fb * vel));
+    mXb[p] += fb * face_adj<S>(0, POTT, POTTb, p, 1, vel, fb * vel);
   }, st));
   HIPCHK(for3(Box{KS, KE, IS, IE, JS - 1, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double G = m.gs(I_XVZ, p);
-    const double fb = tYb[p], vel = G / m.mapf(i, j, 0, I_XV) * (MY[p] + ND[oDY + p]);
+    const double fb = fT.y(k, i, j), vel = G / m.mapf(i, j, 0, I_XV) * (MY[p] + ND[oDY + p]);
     aadd(NDb + oTY + p, fb * G);
-    aadd(mYb + p, fb * face_adj<S>(0, POTT, POTTb, p, IA, vel, fb * vel));
+    mYb[p] += fb * face_adj<S>(0, POTT, POTTb, p, IA, vel, fb * vel);
   }, st));
 
-  // ---- Sw (HEVI.F90:538-577) and the MOMZ fluxes (HEVI.F90:486-525) ----
-  HIPCHK(hipMemsetAsync(w[W_QZ], 0, (W_Q23 - W_QZ + 1) * n3 * sizeof(double), st));
+  // ---- Sw (HEVI.F90:538-577) and the MOMZ fluxes (HEVI.F90:486-525): the
+  //      faces gather their adjoints from Swb (fW); Sw's damping term ----
   HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double b = Swb[p];
-    const double u = b / m.gs(I_XYW, p);
-    aadd(qZb + p, -u * g.RFDZ[k]); aadd(qZb + p - pl, u * g.RFDZ[k]);
-    if constexpr (TER) {
-      aadd(q13b + p, -u * g.RFDZ[k]); aadd(q13b + p - pl, u * g.RFDZ[k]);
-      aadd(q23b + p, -u * g.RFDZ[k]); aadd(q23b + p - pl, u * g.RFDZ[k]);
-    }
-    const double um = u * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY);
-    aadd(qXb + p, -um * g.RCDX[i]); aadd(qXb + p - 1, um * g.RCDX[i]);
-    aadd(qYb + p, -um * g.RCDY[j]); aadd(qYb + p - IA, um * g.RCDY[j]);
-    aadd(MZ0b + p, -wd[k] * b);
+    MZ0b[p] += -wd[k] * Swb[p];
   }, st));
   // fluxZ_XYW (UD3.F90:358-452): face m at loop index m+1
   HIPCHK(for3(Box{KS, KE - 2, IS, IE, JS, JE}, [=] __device__(int f, int i, int j) {
     const size_t p = IX(d, f, i, j);
     const double vel = 0.5 * (MZ[p] + MZ[p + pl]) / D[p + pl];
-    const double fb = qZb[p];
+    const double fb = fW.z(f, i, j);
     aadd(NDb + oWZ + p + pl, fb * m.gs(I_XYZ, p + pl));
     const double velb = J33G * fb * face_adj<S>(f == KS ? 1 : 0, MZ, MZb, p, pl, vel, J33G * fb * vel);
     aadd(MZb + p, 0.5 * velb / D[p + pl]); aadd(MZb + p + pl, 0.5 * velb / D[p + pl]);
@@ -724,7 +732,6 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     for (int jd = 0; jd < 2; ++jd) {
       const double* mom = jd == 0 ? MX : MY;
       double* momb = jd == 0 ? MXb : MYb;
-      const double* qb = jd == 0 ? q13b : q23b;
       const size_t sd = jd == 0 ? 1 : (size_t)IA;  // mom averaged over (i, i-1) or (j, j-1)
       const int mn = jd == 0 ? 1 : 0;
       HIPCHK(for3(Box{KS, KE - 2, IS, IE, JS, JE}, [=] __device__(int f, int i, int j) {
@@ -735,7 +742,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
         const double t0 = f == KS ? 0.5 * (mom[p] + mom[p - sd]) / D[p] : 0.0;
         const double vr = f == KS ? (t1 + t0) * 0.5 : t1;
         const double vel = vr * Jm;
-        const double fb = qb[p];
+        const double fb = fW.z(f, i, j);
         const double velb = fb * face_adj<S>(f == KS ? 1 : 0, MZ, MZb, p, pl, vel, fb * vel / mp) / mp;
         const double vrb = velb * Jm;
         const double t1b = f == KS ? 0.5 * vrb : vrb;
@@ -752,7 +759,6 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const size_t sd = dir == 0 ? 1 : (size_t)IA;
     const double* mom = dir == 0 ? MX : MY;
     double* momb = dir == 0 ? MXb : MYb;
-    const double* qb = dir == 0 ? qXb : qYb;
     const size_t on = dir == 0 ? oWX : oWY;
     const int cG = dir == 0 ? I_UYW : I_XVW, cM = dir == 0 ? I_UY : I_XV, mn = dir == 0 ? 1 : 0;
     HIPCHK(for3(Box{KS, KE - 1, dir == 0 ? IS - 1 : IS, IE, dir == 0 ? JS : JS - 1, JE}, [=] __device__(int k, int i, int j) {
@@ -761,7 +767,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       const double num = f1 * mom[p + pl] + f2 * mom[p];
       const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + sd]) + f2 * 0.5 * (D[p] + D[p + sd]);
       const double vel = num / den;
-      const double fb = qb[p];
+      const double fb = dir == 0 ? fW.x(k, i, j) : fW.y(k, i, j);
       const double G = m.gs(cG, p), gm = G / m.mapf(i, j, mn, cM);
       aadd(NDb + on + p, fb * G);
       const double velb = fb * gm * face_adj<S>(0, MZ, MZb, p, sd, vel, fb * gm * vel);
@@ -772,20 +778,13 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     }, st));
   }
 
-  // ---- Sr (HEVI.F90:453-475) ----
-  HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
-    const size_t p = IX(d, k, i, j);
-    const double b = Srb[p] * m.mapf(i, j, 0, I_XY) * m.mapf(i, j, 1, I_XY) / m.gs(I_XYZ, p);
-    aadd(mZb + p, -b * g.RCDZ[k]); aadd(mZb + p - pl, b * g.RCDZ[k]);
-    aadd(mXb + p, -b * g.RCDX[i]); aadd(mXb + p - 1, b * g.RCDX[i]);
-    aadd(mYb + p, -b * g.RCDY[j]); aadd(mYb + p - IA, b * g.RCDY[j]);
-  }, st));
-  // ---- mass fluxes before the solve (HEVI.F90:369-438) ----
+  // ---- mass fluxes before the solve (HEVI.F90:369-438): the theta faces'
+  //      adjoints plus those gathered from Sr (HEVI.F90:453-475, fR) ----
   HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS - 1, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     if (j >= JS) {
       if (k <= KE - 1) {
-        const double b = mZb[p];
+        const double b = mZb[p] + fR.z(k, i, j);
         const double M0 = m.mapf(i, j, 0, I_XY), M1 = m.mapf(i, j, 1, I_XY);
         aadd(NDb + oDZ + p, b * (m.gs(I_XYW, p) / (M0 * M1)));
         if constexpr (TER) {
@@ -795,11 +794,13 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
         }
       }
       const double gm = m.gs(I_UYZ, p) / m.mapf(i, j, 1, I_UY);
-      aadd(MXb + p, mXb[p] * gm); aadd(NDb + oDX + p, mXb[p] * gm);
+      const double bx = mXb[p] + fR.x(k, i, j);
+      aadd(MXb + p, bx * gm); aadd(NDb + oDX + p, bx * gm);
     }
     if (i >= IS) {
       const double gm = m.gs(I_XVZ, p) / m.mapf(i, j, 0, I_XV);
-      aadd(MYb + p, mYb[p] * gm); aadd(NDb + oDY + p, mYb[p] * gm);
+      const double by = mYb[p] + fR.y(k, i, j);
+      aadd(MYb + p, by * gm); aadd(NDb + oDY + p, by * gm);
     }
   }, st));
   // ---- POTT (HEVI.F90:588-598) and DPRES (HEVI.F90:346-360) ----
