@@ -424,7 +424,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double b = obMY[p];
-    aadd(MY0b + p, b);
+    MY0b[p] += b;
     const double T = dt * b;
     const double u = T / m.gs(I_XVZ, p);
     // (the flux faces gather their adjoints from obMY: fV)
@@ -465,7 +465,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + IA]) + f2 * 0.5 * (D[p] + D[p + IA]);
     const double vel = 0.5 * (MZ[p] + MZ[p + IA]) / den;
     const double fb = fV.z(k, i, j);
-    aadd(NDb + oVZ + p, fb * m.gs(I_XVW, p));
+    NDb[oVZ + p] += fb * m.gs(I_XVW, p);
     const double velb = J33G * fb * face_adj<S>(zkind(k, KS, KE), MY, MYb, p, pl, vel, J33G * fb * vel);
     aadd(MZb + p, 0.5 * velb / den); aadd(MZb + p + IA, 0.5 * velb / den);
     const double denb = -velb * vel / den;
@@ -483,7 +483,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const double vel = 0.5 * (MX[p] + MX[p + IA]) / den;
     const double fb = fV.x(k, i, j);
     const double G = m.gs(I_UVZ, p), gm = G / m.mapf(i, j, 1, I_UV);
-    aadd(NDb + oVX + p, fb * G);
+    NDb[oVX + p] += fb * G;
     const double velb = fb * gm * face_adj<S>(0, MY, MYb, p, 1, vel, fb * gm * vel);
     aadd(MXb + p, 0.5 * velb / den); aadd(MXb + p + IA, 0.5 * velb / den);
     const double denb = 0.25 * (-velb * vel / den);
@@ -495,7 +495,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const double vel = 0.5 * (MY[p] + MY[q]) / D[p];
     const double fb = fV.y(k, i, j - 1);
     const double G = m.gs(I_XYZ, p), gm = G / m.mapf(i, j, 0, I_XY);
-    aadd(NDb + oVY + p, fb * G);
+    NDb[oVY + p] += fb * G;
     const double velb = fb * gm * face_adj<S>(0, MY, MYb, q, IA, vel, fb * gm * vel);
     aadd(MYb + p, 0.5 * velb / D[p]); aadd(MYb + q, 0.5 * velb / D[p]);
     aadd(Db + p, -velb * vel / D[p]);
@@ -505,7 +505,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double b = obMX[p];
-    aadd(MX0b + p, b);
+    MX0b[p] += b;
     const double T = dt * b;
     const double u = T / m.gs(I_UYZ, p);
     // (the flux faces gather their adjoints from obMX: fU)
@@ -546,7 +546,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + 1]) + f2 * 0.5 * (D[p] + D[p + 1]);
     const double vel = 0.5 * (MZ[p] + MZ[p + 1]) / den;
     const double fb = fU.z(k, i, j);
-    aadd(NDb + oUZ + p, fb * m.gs(I_UYW, p));
+    NDb[oUZ + p] += fb * m.gs(I_UYW, p);
     const double velb = J33G * fb * face_adj<S>(zkind(k, KS, KE), MX, MXb, p, pl, vel, J33G * fb * vel);
     aadd(MZb + p, 0.5 * velb / den); aadd(MZb + p + 1, 0.5 * velb / den);
     const double denb = -velb * vel / den;
@@ -563,7 +563,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const double vel = 0.5 * (MX[p] + MX[q]) / D[p];
     const double fb = fU.x(k, i - 1, j);
     const double G = m.gs(I_XYZ, p), gm = G / m.mapf(i, j, 1, I_XY);
-    aadd(NDb + oUX + p, fb * G);
+    NDb[oUX + p] += fb * G;
     const double velb = fb * gm * face_adj<S>(0, MX, MXb, q, 1, vel, fb * gm * vel);
     aadd(MXb + p, 0.5 * velb / D[p]); aadd(MXb + q, 0.5 * velb / D[p]);
     aadd(Db + p, -velb * vel / D[p]);
@@ -575,7 +575,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const double vel = 0.5 * (MY[p] + MY[p + 1]) / den;
     const double fb = fU.y(k, i, j);
     const double G = m.gs(I_UVZ, p), gm = G / m.mapf(i, j, 0, I_UV);
-    aadd(NDb + oUY + p, fb * G);
+    NDb[oUY + p] += fb * G;
     const double velb = fb * gm * face_adj<S>(0, MX, MXb, p, IA, vel, fb * gm * vel);
     aadd(MYb + p, 0.5 * velb / den); aadd(MYb + p + 1, 0.5 * velb / den);
     const double denb = 0.25 * (-velb * vel / den);
@@ -692,21 +692,21 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double fb = fT.z(k, i, j), vel = mflx_z0(m, d, MX, MY, ND + oDZ, k, i, j);
-    aadd(NDb + oTZ + p, fb * m.gs(I_XYW, p));
+    NDb[oTZ + p] += fb * m.gs(I_XYW, p);
     mZb[p] += fb * face_adj<S>(zkind(k, KS, KE), POTT, POTTb, p, pl, vel, fb * vel);
   }, st));
   HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double G = m.gs(I_UYZ, p);
     const double fb = fT.x(k, i, j), vel = G / m.mapf(i, j, 1, I_UY) * (MX[p] + ND[oDX + p]);
-    aadd(NDb + oTX + p, fb * G);
+    NDb[oTX + p] += fb * G;
     mXb[p] += fb * face_adj<S>(0, POTT, POTTb, p, 1, vel, fb * vel);
   }, st));
   HIPCHK(for3(Box{KS, KE, IS, IE, JS - 1, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
     const double G = m.gs(I_XVZ, p);
     const double fb = fT.y(k, i, j), vel = G / m.mapf(i, j, 0, I_XV) * (MY[p] + ND[oDY + p]);
-    aadd(NDb + oTY + p, fb * G);
+    NDb[oTY + p] += fb * G;
     mYb[p] += fb * face_adj<S>(0, POTT, POTTb, p, IA, vel, fb * vel);
   }, st));
 
@@ -721,7 +721,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const size_t p = IX(d, f, i, j);
     const double vel = 0.5 * (MZ[p] + MZ[p + pl]) / D[p + pl];
     const double fb = fW.z(f, i, j);
-    aadd(NDb + oWZ + p + pl, fb * m.gs(I_XYZ, p + pl));
+    NDb[oWZ + p + pl] += fb * m.gs(I_XYZ, p + pl);
     const double velb = J33G * fb * face_adj<S>(f == KS ? 1 : 0, MZ, MZb, p, pl, vel, J33G * fb * vel);
     aadd(MZb + p, 0.5 * velb / D[p + pl]); aadd(MZb + p + pl, 0.5 * velb / D[p + pl]);
     aadd(Db + p + pl, -velb * vel / D[p + pl]);
@@ -769,7 +769,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       const double vel = num / den;
       const double fb = dir == 0 ? fW.x(k, i, j) : fW.y(k, i, j);
       const double G = m.gs(cG, p), gm = G / m.mapf(i, j, mn, cM);
-      aadd(NDb + on + p, fb * G);
+      NDb[on + p] += fb * G;
       const double velb = fb * gm * face_adj<S>(0, MZ, MZb, p, sd, vel, fb * gm * vel);
       const double numb = velb / den, denb = -velb * vel / den;
       aadd(momb + p + pl, f1 * numb); aadd(momb + p, f2 * numb);
@@ -786,7 +786,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       if (k <= KE - 1) {
         const double b = mZb[p] + fR.z(k, i, j);
         const double M0 = m.mapf(i, j, 0, I_XY), M1 = m.mapf(i, j, 1, I_XY);
-        aadd(NDb + oDZ + p, b * (m.gs(I_XYW, p) / (M0 * M1)));
+        NDb[oDZ + p] += b * (m.gs(I_XYW, p) / (M0 * M1));
         if constexpr (TER) {
           const double c13 = b * (m.j13(I_XYW, p) * 0.25 / M1), c23 = b * (m.j23(I_XYW, p) * 0.25 / M0);
           aadd(MXb + p + pl, c13); aadd(MXb + p + pl - 1, c13); aadd(MXb + p, c13); aadd(MXb + p - 1, c13);
@@ -795,37 +795,37 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       }
       const double gm = m.gs(I_UYZ, p) / m.mapf(i, j, 1, I_UY);
       const double bx = mXb[p] + fR.x(k, i, j);
-      aadd(MXb + p, bx * gm); aadd(NDb + oDX + p, bx * gm);
+      aadd(MXb + p, bx * gm); NDb[oDX + p] += bx * gm;
     }
     if (i >= IS) {
       const double gm = m.gs(I_XVZ, p) / m.mapf(i, j, 0, I_XV);
       const double by = mYb[p] + fR.y(k, i, j);
-      aadd(MYb + p, by * gm); aadd(NDb + oDY + p, by * gm);
+      aadd(MYb + p, by * gm); NDb[oDY + p] += by * gm;
     }
   }, st));
   // ---- POTT (HEVI.F90:588-598) and DPRES (HEVI.F90:346-360) ----
   HIPCHK(for3(Box{KS, KE, IS - 2, IE + 2, JS - 2, JE + 2}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    aadd(RTb + p, POTTb[p] / D[p]);
-    aadd(Db + p, -POTTb[p] * POTT[p] / D[p]);
+    RTb[p] += POTTb[p] / D[p];
+    Db[p] += -POTTb[p] * POTT[p] / D[p];
     if (i >= IS && i <= IE + 1 && j >= JS && j <= JE + 1) {
       const double b = DPRESb[p];
-      aadd(DPRES0b + p, b);
-      aadd(RT2Pb + p, b * (RT[p] - REFrt[p]));
-      aadd(RTb + p, b * RT2P[p]);
-      aadd(REFrtb + p, -b * RT2P[p]);
+      DPRES0b[p] += b;
+      RT2Pb[p] += b * (RT[p] - REFrt[p]);
+      RTb[p] += b * RT2P[p];
+      REFrtb[p] += -b * RT2P[p];
       if constexpr (TER) {
         // the ghost levels, read by the terrain terms of the horizontal
         // pressure gradient: DPRES0 - DENS(KS) (PHI(KS-1) - PHI(KS+1)) and its KE mirror
         if (k == KS) {
           const double bg = DPRESb[p - pl];
-          aadd(DPRES0b + p - pl, bg);
-          aadd(Db + p, -bg * (PHI[p - pl] - PHI[p + pl]));
+          DPRES0b[p - pl] += bg;
+          Db[p] += -bg * (PHI[p - pl] - PHI[p + pl]);
         }
         if (k == KE) {
           const double bg = DPRESb[p + pl];
-          aadd(DPRES0b + p + pl, bg);
-          aadd(Db + p, -bg * (PHI[p + pl] - PHI[p - pl]));
+          DPRES0b[p + pl] += bg;
+          Db[p] += -bg * (PHI[p + pl] - PHI[p - pl]);
         }
       }
     }
@@ -987,23 +987,38 @@ static hipError_t diff3_adj(const Ctx& c, const NfTab t, double* dZb, const doub
       }
     }
   }, s));
-  return for3(Box{d.KS, d.KE - KO, d.IS, d.IE, d.JS, d.JE}, [=] __device__(int k, int i, int j) {
-    const size_t p = IX(d, k, i, j), IA = d.IA;
-    const double bx = dXb[p], by = dYb[p];
-    if (IO == 0) {
-      aadd(phib + p + 2, t.cx(1, i + 1, 1) * bx); aadd(phib + p + 1, -t.cx(2, i + 1, 1) * bx);
-      aadd(phib + p, t.cx(3, i + 1, 1) * bx); aadd(phib + p - 1, -t.cx(1, i, 1) * bx);
-    } else {
-      aadd(phib + p + 1, t.cx(1, i, 2) * bx); aadd(phib + p, -t.cx(2, i, 2) * bx);
-      aadd(phib + p - 1, t.cx(3, i, 2) * bx); aadd(phib + p - 2, -t.cx(1, i - 1, 2) * bx);
+  // the x and y differences, gathered: each point of phi sums the terms of
+  // the (up to four) differences along x and along y that read it; the
+  // differences exist on k = KS..KE-KO, i = IS..IE, j = JS..JE
+  return for3(Box{d.KS, d.KE - KO, d.IS - 2, d.IE + 2, d.JS - 2, d.JE + 2}, [=] __device__(int k, int i, int j) {
+    const size_t q = IX(d, k, i, j), IA = d.IA;
+    const bool rowx = j >= d.JS && j <= d.JE, rowy = i >= d.IS && i <= d.IE;
+    auto inx = [&](int ii) { return rowx && ii >= d.IS && ii <= d.IE; };
+    auto iny = [&](int jj) { return rowy && jj >= d.JS && jj <= d.JE; };
+    double acc = 0.0;
+    if (IO == 0) {  // dX(i) = cx(1,i+1,1) phi(i+2) - cx(2,i+1,1) phi(i+1) + cx(3,i+1,1) phi(i) - cx(1,i,1) phi(i-1)
+      if (inx(i - 2)) acc += t.cx(1, i - 1, 1) * dXb[q - 2];
+      if (inx(i - 1)) acc += -t.cx(2, i, 1) * dXb[q - 1];
+      if (inx(i)) acc += t.cx(3, i + 1, 1) * dXb[q];
+      if (inx(i + 1)) acc += -t.cx(1, i + 1, 1) * dXb[q + 1];
+    } else {        // dX(i) = cx(1,i,2) phi(i+1) - cx(2,i,2) phi(i) + cx(3,i,2) phi(i-1) - cx(1,i-1,2) phi(i-2)
+      if (inx(i - 1)) acc += t.cx(1, i - 1, 2) * dXb[q - 1];
+      if (inx(i)) acc += -t.cx(2, i, 2) * dXb[q];
+      if (inx(i + 1)) acc += t.cx(3, i + 1, 2) * dXb[q + 1];
+      if (inx(i + 2)) acc += -t.cx(1, i + 1, 2) * dXb[q + 2];
     }
     if (JO == 0) {
-      aadd(phib + p + 2 * IA, t.cy(1, j + 1, 1) * by); aadd(phib + p + IA, -t.cy(2, j + 1, 1) * by);
-      aadd(phib + p, t.cy(3, j + 1, 1) * by); aadd(phib + p - IA, -t.cy(1, j, 1) * by);
+      if (iny(j - 2)) acc += t.cy(1, j - 1, 1) * dYb[q - 2 * IA];
+      if (iny(j - 1)) acc += -t.cy(2, j, 1) * dYb[q - IA];
+      if (iny(j)) acc += t.cy(3, j + 1, 1) * dYb[q];
+      if (iny(j + 1)) acc += -t.cy(1, j + 1, 1) * dYb[q + IA];
     } else {
-      aadd(phib + p + IA, t.cy(1, j, 2) * by); aadd(phib + p, -t.cy(2, j, 2) * by);
-      aadd(phib + p - IA, t.cy(3, j, 2) * by); aadd(phib + p - 2 * IA, -t.cy(1, j - 1, 2) * by);
+      if (iny(j - 1)) acc += t.cy(1, j - 1, 2) * dYb[q - IA];
+      if (iny(j)) acc += -t.cy(2, j, 2) * dYb[q];
+      if (iny(j + 1)) acc += t.cy(3, j + 1, 2) * dYb[q + IA];
+      if (iny(j + 2)) acc += -t.cy(1, j + 1, 2) * dYb[q + 2 * IA];
     }
+    phib[q] += acc;
   }, s);
 }
 

@@ -346,6 +346,7 @@ int scale_hip_init(const scale_hip_grid* g, scale_hip_ctx** out) {
   c.scheme = g->flux_scheme;
   c.scheme_tracer = g->flux_scheme_tracer;
   if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("hipStreamCreate failed"); }
+  if (hipDeviceGetAttribute(&c.ncu, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || c.ncu <= 0) c.ncu = 256;
   const size_t n1 = 4 * (size_t)(c.d.KA + c.d.IA + c.d.JA);
   c.g1buf = dalloc(c, n1, nullptr);
   c.MAPF = dalloc(c, 8 * c.d.pl, nullptr);

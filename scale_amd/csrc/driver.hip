@@ -526,8 +526,18 @@ struct ShortBufs {
 };
 
 // the inputs and outputs of stage r of a short step of length dts
+// the fourth RK4 stage combines the stages into the state itself (flat
+// HEVI on the tile kernel: FP32 always, FP64 where the tiles fill the device
+// or the stage is one of an ensemble's batched launches)
 template <typename R>
-static StageIOT<R> short_stage_io(Ctx& c, double dts, const ShortBufs<R>& B, int r, const StepRec* rec = nullptr) {
+static bool fused_rk4(const Ctx& c, bool batched) {
+  const Model& M = *c.model;
+  return M.tinteg_short == 4 && !c.terrain && !M.heve && (sizeof(R) == sizeof(float) || batched || hevi_flat_tile(c));
+}
+
+template <typename R>
+static StageIOT<R> short_stage_io(Ctx& c, double dts, const ShortBufs<R>& B, int r, const StepRec* rec = nullptr,
+                                  bool batched = false) {
   Model& M = *c.model;
   const Dims& d = c.d;
   R* const num_diff = rec ? (R*)rec->nd : B.num_diff;
@@ -549,8 +559,7 @@ static StageIOT<R> short_stage_io(Ctx& c, double dts, const ShortBufs<R>& B, int
   io.DPRES0 = M.DPRES0; io.RT2P = M.RT2P; io.CORIOLI = M.CORIOLI; io.num_diff = num_diff;
   io.wdamp_coef = M.wdamp_coef; io.DDIV = B.DDIV; io.REF_dens = M.REF_dens; io.REF_rhot = M.REF_rhot;
   io.divdmp_coef = M.divdmp_coef; io.dtrk = dtrk[r]; io.has_coriolis = M.has_coriolis;
-  // flat-terrain HEVI RK4: the last stage combines the four stages into the state itself
-  io.combine = (rk4 && r == 3 && !c.terrain && !M.heve) ? 1 : 0;
+  io.combine = (r == 3 && fused_rk4<R>(c, batched)) ? 1 : 0;
   for (int v = 0; v < 5; ++v) { io.c1[v] = Rk(0, v); io.c2[v] = Rk(1, v); io.c3[v] = Rk(2, v); io.st[v] = B.st[v]; }
   for (int n = 0; n < 3; ++n) { io.cm[n] = B.mfR[n]; io.ct[n] = B.tfR[n]; }
   io.mflx_av = B.mflx; io.tflx_av = B.tflx;
@@ -616,7 +625,7 @@ static hipError_t short_step_t(Ctx& c, double dts, const ShortBufs<R>& B, const 
     HIPCHK(hipGetLastError());
     return halo_fill(c, B.st, 5, s);
   }
-  if (!c.terrain && !M.heve) return halo_fill(c, B.st, 5, s);
+  if (fused_rk4<R>(c, false)) return halo_fill(c, B.st, 5, s);
   CombArgs<R> a;
   a.d = d;
   for (int v = 0; v < 5; ++v) {
@@ -1208,7 +1217,7 @@ static hipError_t ensemble_build_t(Ensemble& E) {
     for (Ctx* cp : E.members) {
       Model& M = *cp->model;
       const ShortBufs<R> B = model_bufs<R>(M);
-      ios.push_back(short_stage_io(*cp, M.dtl / model_nstep(M), B, r));
+      ios.push_back(short_stage_io(*cp, M.dtl / model_nstep(M), B, r, nullptr, true));
     }
     HIPCHK(hevi_batch_build(ios.data(), n, E.stage[r]));
   }

@@ -867,32 +867,44 @@ static std::vector<int> terrain_parts() {
   return v;
 }
 
-template <int S, bool DIVDMP>
-static void launch_terrain(const HeviArgs& a, hipStream_t s) {
+template <int S, bool TERRAIN, bool DIVDMP>
+static void launch_split(const HeviArgs& a, hipStream_t s) {
   const Dims& d = a.d;
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1, kmax = d.KE - d.KS + 1;
   constexpr unsigned TJ = 4;
   const unsigned ni = (imax + 63) / 64, ncols = ni * ((jmax + TJ - 1) / TJ);
   const int L = terrain_seg_levels();
   if (L <= 0) {
-    hipLaunchKernelGGL((hevi_march<S, true, DIVDMP>), dim3(ncols), dim3(64, TJ), 0, s, a, ni, ncols);
+    hipLaunchKernelGGL((hevi_march<S, TERRAIN, DIVDMP>), dim3(ncols), dim3(64, TJ), 0, s, a, ni, ncols);
     return;
   }
   const unsigned nseg = (unsigned)((kmax + L - 1) / L), nb = ncols * nseg;
   const dim3 g(nb), b(64, TJ);
   for (int part : terrain_parts()) {
     switch (part) {
-      case P_ALL: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_ALL>), g, b, 0, s, a, ni, ncols, nb, L); break;
-      case P_CT | P_W: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_CT | P_W>), g, b, 0, s, a, ni, ncols, nb, L); break;
-      case P_U | P_V: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_U | P_V>), g, b, 0, s, a, ni, ncols, nb, L); break;
-      case P_CT: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_CT>), g, b, 0, s, a, ni, ncols, nb, L); break;
-      case P_W: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_W>), g, b, 0, s, a, ni, ncols, nb, L); break;
-      case P_U: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_U>), g, b, 0, s, a, ni, ncols, nb, L); break;
-      case P_V: hipLaunchKernelGGL((hevi_expl<S, true, DIVDMP, P_V>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_ALL: hipLaunchKernelGGL((hevi_expl<S, TERRAIN, DIVDMP, P_ALL>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_CT | P_W: hipLaunchKernelGGL((hevi_expl<S, TERRAIN, DIVDMP, P_CT | P_W>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_U | P_V: hipLaunchKernelGGL((hevi_expl<S, TERRAIN, DIVDMP, P_U | P_V>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_CT: hipLaunchKernelGGL((hevi_expl<S, TERRAIN, DIVDMP, P_CT>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_W: hipLaunchKernelGGL((hevi_expl<S, TERRAIN, DIVDMP, P_W>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_U: hipLaunchKernelGGL((hevi_expl<S, TERRAIN, DIVDMP, P_U>), g, b, 0, s, a, ni, ncols, nb, L); break;
+      case P_V: hipLaunchKernelGGL((hevi_expl<S, TERRAIN, DIVDMP, P_V>), g, b, 0, s, a, ni, ncols, nb, L); break;
       default: break;
     }
   }
-  hipLaunchKernelGGL((hevi_solve<S, true, DIVDMP>), dim3(ncols), dim3(64, TJ), 0, s, a, ni, ncols);
+  hipLaunchKernelGGL((hevi_solve<S, TERRAIN, DIVDMP>), dim3(ncols), dim3(64, TJ), 0, s, a, ni, ncols);
+}
+
+// Flat grids whose 64 x 8 tiles do not give every CU a block (hevi_tile at
+// 128x128 is 32 blocks on 256 CUs) take the split launches instead; the RK4
+// combination then runs as its own kernel.  SCALE_HIP_FLAT_SPLIT=0 keeps the
+// tile kernel, =1 the split launches, whatever the grid.
+bool hevi_flat_tile(const Ctx& c) {
+  const char* e = getenv("SCALE_HIP_FLAT_SPLIT");
+  if (e && *e) return atoi(e) == 0;
+  const Dims& d = c.d;
+  const unsigned nt = (unsigned)((d.IE - d.IS + 1 + 63) / 64) * (unsigned)((d.JE - d.JS + 1 + 7) / 8);
+  return nt >= (unsigned)c.ncu;
 }
 
 // ---------------------------------------------------------------------------
@@ -1698,6 +1710,11 @@ static hipError_t launch_t(const HeviArgs& a, hipStream_t s, bool cmb) {
   const unsigned ni = (imax + 63) / 64;
   prof_start(c, "hevi_stage", s);
   if constexpr (!TERRAIN) {
+    if (!cmb && !hevi_flat_tile(c)) {
+      launch_split<S, false, DIVDMP>(a, s);
+      prof_stop(c, s);
+      return hipGetLastError();
+    }
     // flat terrain: LDS tiles of 64 x 8 columns
     constexpr int TJ = 8;
     const unsigned nb = ni * ((jmax + TJ - 1) / TJ);
@@ -1705,7 +1722,7 @@ static hipError_t launch_t(const HeviArgs& a, hipStream_t s, bool cmb) {
     else hipLaunchKernelGGL((hevi_tile<S, DIVDMP, TJ, double, false>), dim3(nb), dim3(64, TJ), 0, s, a, ni, nb);
   } else {
     if (cmb) { prof_stop(c, s); return hipErrorNotSupported; }
-    launch_terrain<S, DIVDMP>(a, s);
+    launch_split<S, true, DIVDMP>(a, s);
   }
   prof_stop(c, s);
   return hipGetLastError();
@@ -2037,8 +2054,13 @@ hipError_t hevi_stage_traced(const StageIO& io, double* PT, double* St, double* 
   const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
   const unsigned ni = (imax + 63) / 64;
   if (c.terrain) {
-    if (c.scheme == UD3) launch_terrain<UD3, false>(a, s);
-    else launch_terrain<CD4, false>(a, s);
+    if (c.scheme == UD3) launch_split<UD3, true, false>(a, s);
+    else launch_split<CD4, true, false>(a, s);
+    return hipGetLastError();
+  }
+  if (!hevi_flat_tile(c)) {
+    if (c.scheme == UD3) launch_split<UD3, false, false>(a, s);
+    else launch_split<CD4, false, false>(a, s);
     return hipGetLastError();
   }
   constexpr int TJ = 8;

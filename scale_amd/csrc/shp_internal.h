@@ -17,6 +17,7 @@ struct Prof;   // named kernel timers (profile.hip)
 struct Ctx {
   Dims d;
   int device = 0;
+  int ncu = 256;  // compute units of the device
   int scheme = UD3, scheme_tracer = UD3;
   bool terrain = false;          // general metrics (GSQRT/J13G/J23G/MAPF) in use
   bool metrics_set = false;
@@ -122,6 +123,9 @@ hipError_t hevi_batch_build(const StageIOT<float>* ios, int n, BatchBuf& b);
 // blocks_per_cu = 0: one block per tile; otherwise a persistent grid of
 // blocks_per_cu blocks per CU walking the tiles
 hipError_t hevi_batch_launch(const BatchBuf& b, int blocks_per_cu, int ncu, hipStream_t s);
+// the flat FP64 stage on LDS tiles (false: the split launches, for grids
+// too small to give every CU a tile)
+bool hevi_flat_tile(const Ctx& c);
 // the numerical filter of every member (then each member's halo fill)
 template <typename R>
 struct NfIn {

@@ -95,6 +95,26 @@ def test_hevi_stage_terrain_segments(hip, seg, parts, monkeypatch):
         H.compare(ref[k], got[k], k)
 
 
+@pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
+@pytest.mark.parametrize("divdmp", [False, True], ids=["plain", "divdmp"])
+@pytest.mark.parametrize("seg", [3, 8], ids=["seg3", "seg8"])
+def test_hevi_stage_flat_split(hip, scheme, divdmp, seg, monkeypatch):
+    """The flat stage as split launches (the path of grids too small for the
+    tile kernel to fill the device): bit for bit the oracle's."""
+    monkeypatch.setenv("SCALE_HIP_FLAT_SPLIT", "1")
+    monkeypatch.setenv("SCALE_HIP_TERRAIN_SEG", str(seg))
+    print(f"[hevi stage] flat, split launches, scheme={scheme} divdmp={divdmp} seg={seg}", flush=True)
+    case = H.small_case(seed=1100 + scheme, stretched=(50.0, 500.0), kmax=20)
+    inp = H.stage_inputs(case, seed=25, coriolis=True)
+    if divdmp:
+        inp["divdmp_coef"] = 0.1
+        inp["DDIV"] = np.asfortranarray(np.random.Generator(np.random.PCG64(4)).standard_normal(case.grid.shape) * 1e-3)
+    ref = oracle_stage(case, inp, 0.5, None, scheme)
+    got = run_stage_gpu(hip, case, inp, 0.5, None, scheme)
+    for k in OUTS:
+        H.compare(ref[k], got[k], k)
+
+
 @pytest.mark.parametrize("terrain", [False, True], ids=["flat", "terrain"])
 def test_hevi_stage_divdmp(hip, terrain):
     print(f"[hevi stage] divergence damping terrain={terrain}", flush=True)
@@ -410,10 +430,14 @@ def test_model_parity_heve(hip, terrain):
         ctx.close()
 
 
-def test_model_parity_warm_bubble(hip):
+@pytest.mark.parametrize("split", ["0", "1"], ids=["tile", "split_launches"])
+def test_model_parity_warm_bubble(hip, split, monkeypatch):
     """Device-resident large steps (dynamics RK4 + RK3 tracers with FCT) against
-    the oracle integrator, 10 steps of a reduced warm bubble."""
-    print("[model] warm bubble 24x20x16, 10 steps", flush=True)
+    the oracle integrator, 10 steps of a reduced warm bubble: with the flat
+    stage on the tile kernel (the fourth stage fused with the RK4
+    combination) and as split launches (the combination its own kernel)."""
+    monkeypatch.setenv("SCALE_HIP_FLAT_SPLIT", split)
+    print(f"[model] warm bubble 24x20x16, 10 steps, split={split}", flush=True)
     case = inputs.make_case(kmax=24, imax=20, jmax=16, noise=0.01, ntracer=2, tracer_kind=["blob", "tophat"], fct=True,
                             bubble_r=2000.0, bubble_zc=2000.0)
     m = po.Model(case)

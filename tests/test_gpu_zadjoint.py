@@ -49,14 +49,17 @@ def test_stage_adjoint_matches_oracle(scheme):
         assert close(got[key], ref[key]), key
 
 
-@pytest.mark.parametrize("records,tape", [("1", 0), ("1", 1), ("0", 0)],
-                         ids=["all_segments_taped", "last_segment_taped", "per_step_recompute"])
-def test_model_adjoint_matches_oracle(records, tape, monkeypatch):
+@pytest.mark.parametrize("records,tape,split", [("1", 0, "0"), ("1", 1, "0"), ("0", 0, "0"), ("1", 1, "1")],
+                         ids=["all_segments_taped", "last_segment_taped", "per_step_recompute",
+                              "last_segment_taped-split_launches"])
+def test_model_adjoint_matches_oracle(records, tape, split, monkeypatch):
     """The three reverse sweeps: every segment's tape kept by the forward (the
     default where memory holds them), only the last one's (the others recorded
     again from their checkpoints) and, with SCALE_HIP_ADJ_RECORDS=0, the
-    per-step recomputation used when no records fit in device memory."""
+    per-step recomputation used when no records fit in device memory; and
+    the forward's flat stage as split launches (the path of small grids)."""
     monkeypatch.setenv("SCALE_HIP_ADJ_RECORDS", records)
+    monkeypatch.setenv("SCALE_HIP_FLAT_SPLIT", split)
     print(f"[adjoint] model 8x8x10, 3 steps, checkpoint every 2, records={records}, taped segments={tape}", flush=True)
     case = perturbed_case()
     N = 3
