@@ -96,20 +96,75 @@ __device__ __forceinline__ void fv_weights(int kind, double vel, double w[4]) {
 }
 
 // Face value of q at the face whose q0 is q[p] (stride st) and its adjoint:
-// qb[p + (m-1)*st] += scale * dfv/dq_m.  Returns the face value.
-template <int S>
-__device__ __forceinline__ double face_adj(int kind, const double* q, double* qb, size_t p, size_t st, double vel, double scale) {
+// sink(m, scale * dfv/dq_m) for the points p + (m-1)*st.  Returns the face value.
+template <int S, typename Sink>
+__device__ __forceinline__ double face_adj_s(int kind, const double* q, size_t p, size_t st, double vel, double scale, Sink&& sink) {
   double w[4];
   fv_weights<S>(kind, vel, w);
   if (scale != 0.0) {
-    if (kind != 1) aadd(qb + p - st, scale * w[0]);
-    aadd(qb + p, scale * w[1]);
-    aadd(qb + p + st, scale * w[2]);
-    if (kind != 2) aadd(qb + p + 2 * st, scale * w[3]);
+    if (kind != 1) sink(0, scale * w[0]);
+    sink(1, scale * w[1]);
+    sink(2, scale * w[2]);
+    if (kind != 2) sink(3, scale * w[3]);
   }
   if (kind == 1) return fv_lo<S, double>(q[p], q[p + st], q[p + 2 * st], vel);
   if (kind == 2) return fv_hi<S, double>(q[p - st], q[p], q[p + st], vel);
   return fv_int<S, double>(q[p - st], q[p], q[p + st], q[p + 2 * st], vel);
+}
+
+// the same, adding to qb[p + (m-1)*st] with atomics
+template <int S>
+__device__ __forceinline__ double face_adj(int kind, const double* q, double* qb, size_t p, size_t st, double vel, double scale) {
+  return face_adj_s<S>(kind, q, p, st, vel, scale, [&](int m, double v) { aadd(qb + p + (size_t)(m - 1) * st, v); });
+}
+
+// Vertical faces of a box, L consecutive faces of a column per thread,
+// upward: the adjoint of a stencil along z (contributions to the points
+// k-1..k+2 of the face's own column) collects in a register window and each
+// point of the column is added once, without an atomic where every face that
+// reaches it (faces q-2..q+1) is the thread's own.  f(k, i, j, win) adds
+// win.a[m] for the point k-1+m.
+struct ZWin {
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+};
+
+template <typename F>
+__global__ void __launch_bounds__(256) zseg_kernel(Dims d, Box b, int L, double* tgt, F f) {
+  const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+  const int nseg = (nk + L - 1) / L;
+  const size_t n = (size_t)ni * (size_t)nj * (size_t)nseg;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = b.i0 + (int)(t % (size_t)ni);
+  const size_t r = t / (size_t)ni;
+  const int j = b.j0 + (int)(r % (size_t)nj);
+  const int k0 = b.k0 + (int)(r / (size_t)nj) * L, k1 = min(b.k1, k0 + L - 1);
+  auto flush = [&](int k, double v, bool own) {
+    if (v == 0.0) return;
+    double* q = tgt + IX(d, k, i, j);
+    if (own) *q += v;
+    else aadd(q, v);
+  };
+  ZWin w;
+  for (int k = k0; k <= k1; ++k) {
+    f(k, i, j, w);
+    flush(k - 1, w.a[0], k - 1 >= k0 + 2);  // no later face reaches k-1
+    w.a[0] = w.a[1]; w.a[1] = w.a[2]; w.a[2] = w.a[3]; w.a[3] = 0.0;
+  }
+  flush(k1, w.a[0], false);
+  flush(k1 + 1, w.a[1], false);
+  flush(k1 + 2, w.a[2], false);
+}
+
+constexpr int ZSEG_L = 8;
+
+template <typename F>
+hipError_t zseg(const Dims& d, Box b, double* tgt, F f, hipStream_t s) {
+  if (b.k1 < b.k0 || b.i1 < b.i0 || b.j1 < b.j0) return hipSuccess;
+  const int nk = b.k1 - b.k0 + 1;
+  const size_t n = (size_t)(b.i1 - b.i0 + 1) * (size_t)(b.j1 - b.j0 + 1) * (size_t)((nk + ZSEG_L - 1) / ZSEG_L);
+  hipLaunchKernelGGL(zseg_kernel<F>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, ZSEG_L, tgt, f);
+  return hipGetLastError();
 }
 
 __device__ __forceinline__ int zkind(int k, int KS, int KE) { return k == KS ? 1 : (k == KE - 1 ? 2 : 0); }
@@ -393,7 +448,9 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const ptrdiff_t oa = stag == 0 ? 1 : -1, ob = stag == 0 ? -(ptrdiff_t)IA : (ptrdiff_t)IA;
     const int cG = stag == 0 ? I_UYZ : I_XVZ, cJ = stag == 0 ? I_UYW : I_XVW, cM = stag == 0 ? I_UY : I_XV;
     const int mn = jd == 0 ? 1 : 0;
-    return for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+    // z faces of a column: the val adjoint (and, with mom the same field, the
+    // mom adjoint of the own column) in the segment's window
+    return zseg(d, Box{KS, KE - 1, IS, IE, JS, JE}, valb, [=] __device__(int k, int i, int j, ZWin& zw) {
       const size_t p = IX(d, k, i, j);
       const double f1 = f2h(m, g, d, cG, k, 1, i, j), f2 = f2h(m, g, d, cG, k, 2, i, j);
       auto mm = [&](size_t q) {
@@ -406,15 +463,20 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       const double Jm = jd == 0 ? m.j13(cJ, p) : m.j23(cJ, p);
       const double vel = vr * Jm, mp = m.mapf(i, j, mn, cM);
       const double fb = fa.z(k, i, j);
-      const double velb = fb * face_adj<S>(zkind(k, KS, KE), val, valb, p, pl, vel, fb * vel / mp) / mp;
+      const double velb =
+          fb * face_adj_s<S>(zkind(k, KS, KE), val, p, pl, vel, fb * vel / mp, [&](int q, double v) { zw.a[q] += v; }) / mp;
       const double vrb = velb * Jm;
       const double numb = vrb / den, denb = -vrb * vr / den;
-      auto mmb = [&](size_t q, double x) {
-        double* r = momb + q;
-        if (quad) { const double y = 0.25 * x; aadd(r, y); aadd(r + oa, y); aadd(r + ob, y); aadd(r + oa + ob, y); }
-        else aadd(r, x);
-      };
-      mmb(p + pl, f1 * numb); mmb(p, f2 * numb);
+      if (quad) {
+        auto mmb = [&](size_t q, double x) {
+          double* r = momb + q;
+          const double y = 0.25 * x;
+          aadd(r, y); aadd(r + oa, y); aadd(r + ob, y); aadd(r + oa + ob, y);
+        };
+        mmb(p + pl, f1 * numb); mmb(p, f2 * numb);
+      } else {  // mom is val itself, on the own column
+        zw.a[2] += f1 * numb; zw.a[1] += f2 * numb;
+      }
       aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + sd, 0.5 * f1 * denb);
       aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + sd, 0.5 * f2 * denb);
     }, st);
@@ -458,15 +520,15 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       aadd(Db + p + IA, -T * cf2 / Dd); aadd(Db + p, -T * cf2 / Dd);
     }
   }, st));
-  // fluxZ_XVZ (UD3.F90:1173-1265)
-  HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+  // fluxZ_XVZ (UD3.F90:1173-1265), the MOMY adjoint of the column in a window
+  HIPCHK(zseg(d, Box{KS, KE - 1, IS, IE, JS, JE}, MYb, [=] __device__(int k, int i, int j, ZWin& zw) {
     const size_t p = IX(d, k, i, j);
     const double f1 = f2h(m, g, d, I_XVW, k, 1, i, j), f2 = f2h(m, g, d, I_XVW, k, 2, i, j);
     const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + IA]) + f2 * 0.5 * (D[p] + D[p + IA]);
     const double vel = 0.5 * (MZ[p] + MZ[p + IA]) / den;
     const double fb = fV.z(k, i, j);
     NDb[oVZ + p] += fb * m.gs(I_XVW, p);
-    const double velb = J33G * fb * face_adj<S>(zkind(k, KS, KE), MY, MYb, p, pl, vel, J33G * fb * vel);
+    const double velb = J33G * fb * face_adj_s<S>(zkind(k, KS, KE), MY, p, pl, vel, J33G * fb * vel, [&](int q, double v) { zw.a[q] += v; });
     aadd(MZb + p, 0.5 * velb / den); aadd(MZb + p + IA, 0.5 * velb / den);
     const double denb = -velb * vel / den;
     aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + IA, 0.5 * f1 * denb);
@@ -539,15 +601,15 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       aadd(Db + p + 1, -T * cf2 / Dd); aadd(Db + p, -T * cf2 / Dd);
     }
   }, st));
-  // fluxZ_UYZ (UD3.F90:753-845)
-  HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+  // fluxZ_UYZ (UD3.F90:753-845), the MOMX adjoint of the column in a window
+  HIPCHK(zseg(d, Box{KS, KE - 1, IS, IE, JS, JE}, MXb, [=] __device__(int k, int i, int j, ZWin& zw) {
     const size_t p = IX(d, k, i, j);
     const double f1 = f2h(m, g, d, I_UYW, k, 1, i, j), f2 = f2h(m, g, d, I_UYW, k, 2, i, j);
     const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + 1]) + f2 * 0.5 * (D[p] + D[p + 1]);
     const double vel = 0.5 * (MZ[p] + MZ[p + 1]) / den;
     const double fb = fU.z(k, i, j);
     NDb[oUZ + p] += fb * m.gs(I_UYW, p);
-    const double velb = J33G * fb * face_adj<S>(zkind(k, KS, KE), MX, MXb, p, pl, vel, J33G * fb * vel);
+    const double velb = J33G * fb * face_adj_s<S>(zkind(k, KS, KE), MX, p, pl, vel, J33G * fb * vel, [&](int q, double v) { zw.a[q] += v; });
     aadd(MZb + p, 0.5 * velb / den); aadd(MZb + p + 1, 0.5 * velb / den);
     const double denb = -velb * vel / den;
     aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + 1, 0.5 * f1 * denb);
@@ -681,19 +743,19 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     }
     DPRESb[p] += dpb; RT2Pb[p] += rt2pb; Stb[p] += stb; Db[p] += db; Srb[p] += srb;
   }, st));
-  // (d) PT = valueW_Z(MOMZ, POTT) (UD3.F90:98-153), one face per thread
-  HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+  // (d) PT = valueW_Z(MOMZ, POTT) (UD3.F90:98-153), the POTT adjoint of the column in a window
+  HIPCHK(zseg(d, Box{KS, KE - 1, IS, IE, JS, JE}, POTTb, [=] __device__(int k, int i, int j, ZWin& zw) {
     const size_t p = IX(d, k, i, j);
-    (void)face_adj<S>(zkind(k, KS, KE), POTT, POTTb, p, pl, MZ[p], PTb[p]);
+    (void)face_adj_s<S>(zkind(k, KS, KE), POTT, p, pl, MZ[p], PTb[p], [&](int q, double v) { zw.a[q] += v; });
   }, st));
 
   // ---- St (HEVI.F90:634-657) and the theta fluxes (UD3.F90:157-354): each
   //      face gathers its adjoint from Stb (fT) and owns its mass-flux adjoint ----
-  HIPCHK(for3(Box{KS, KE - 1, IS, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+  HIPCHK(zseg(d, Box{KS, KE - 1, IS, IE, JS, JE}, POTTb, [=] __device__(int k, int i, int j, ZWin& zw) {
     const size_t p = IX(d, k, i, j);
     const double fb = fT.z(k, i, j), vel = mflx_z0(m, d, MX, MY, ND + oDZ, k, i, j);
     NDb[oTZ + p] += fb * m.gs(I_XYW, p);
-    mZb[p] += fb * face_adj<S>(zkind(k, KS, KE), POTT, POTTb, p, pl, vel, fb * vel);
+    mZb[p] += fb * face_adj_s<S>(zkind(k, KS, KE), POTT, p, pl, vel, fb * vel, [&](int q, double v) { zw.a[q] += v; });
   }, st));
   HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS, JE}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
@@ -716,15 +778,16 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const size_t p = IX(d, k, i, j);
     MZ0b[p] += -wd[k] * Swb[p];
   }, st));
-  // fluxZ_XYW (UD3.F90:358-452): face m at loop index m+1
-  HIPCHK(for3(Box{KS, KE - 2, IS, IE, JS, JE}, [=] __device__(int f, int i, int j) {
+  // fluxZ_XYW (UD3.F90:358-452): face m at loop index m+1; the MOMZ adjoint
+  // of the column (value and velocity) in a window
+  HIPCHK(zseg(d, Box{KS, KE - 2, IS, IE, JS, JE}, MZb, [=] __device__(int f, int i, int j, ZWin& zw) {
     const size_t p = IX(d, f, i, j);
     const double vel = 0.5 * (MZ[p] + MZ[p + pl]) / D[p + pl];
     const double fb = fW.z(f, i, j);
     NDb[oWZ + p + pl] += fb * m.gs(I_XYZ, p + pl);
-    const double velb = J33G * fb * face_adj<S>(f == KS ? 1 : 0, MZ, MZb, p, pl, vel, J33G * fb * vel);
-    aadd(MZb + p, 0.5 * velb / D[p + pl]); aadd(MZb + p + pl, 0.5 * velb / D[p + pl]);
-    aadd(Db + p + pl, -velb * vel / D[p + pl]);
+    const double velb = J33G * fb * face_adj_s<S>(f == KS ? 1 : 0, MZ, p, pl, vel, J33G * fb * vel, [&](int q, double v) { zw.a[q] += v; });
+    zw.a[1] += 0.5 * velb / D[p + pl]; zw.a[2] += 0.5 * velb / D[p + pl];
+    Db[p + pl] += -velb * vel / D[p + pl];
   }, st));
   // fluxJ13_XYW / fluxJ23_XYW (UD3.F90:457-593), terrain only: face f at
   // loop index f+1, the face KS with the mean velocity of levels KS, KS+1
@@ -734,7 +797,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       double* momb = jd == 0 ? MXb : MYb;
       const size_t sd = jd == 0 ? 1 : (size_t)IA;  // mom averaged over (i, i-1) or (j, j-1)
       const int mn = jd == 0 ? 1 : 0;
-      HIPCHK(for3(Box{KS, KE - 2, IS, IE, JS, JE}, [=] __device__(int f, int i, int j) {
+      HIPCHK(zseg(d, Box{KS, KE - 2, IS, IE, JS, JE}, MZb, [=] __device__(int f, int i, int j, ZWin& zw) {
         const size_t p = IX(d, f, i, j), p1 = p + pl;
         const double Jm = jd == 0 ? m.j13(I_XYZ, p1) : m.j23(I_XYZ, p1);
         const double mp = m.mapf(i, j, mn, I_XY);
@@ -743,13 +806,14 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
         const double vr = f == KS ? (t1 + t0) * 0.5 : t1;
         const double vel = vr * Jm;
         const double fb = fW.z(f, i, j);
-        const double velb = fb * face_adj<S>(f == KS ? 1 : 0, MZ, MZb, p, pl, vel, fb * vel / mp) / mp;
+        const double velb =
+            fb * face_adj_s<S>(f == KS ? 1 : 0, MZ, p, pl, vel, fb * vel / mp, [&](int q, double v) { zw.a[q] += v; }) / mp;
         const double vrb = velb * Jm;
         const double t1b = f == KS ? 0.5 * vrb : vrb;
-        aadd(momb + p1, 0.5 * t1b / D[p1]); aadd(momb + p1 - sd, 0.5 * t1b / D[p1]); aadd(Db + p1, -t1b * t1 / D[p1]);
+        aadd(momb + p1, 0.5 * t1b / D[p1]); aadd(momb + p1 - sd, 0.5 * t1b / D[p1]); Db[p1] += -t1b * t1 / D[p1];
         if (f == KS) {
           const double t0b = 0.5 * vrb;
-          aadd(momb + p, 0.5 * t0b / D[p]); aadd(momb + p - sd, 0.5 * t0b / D[p]); aadd(Db + p, -t0b * t0 / D[p]);
+          aadd(momb + p, 0.5 * t0b / D[p]); aadd(momb + p - sd, 0.5 * t0b / D[p]); Db[p] += -t0b * t0 / D[p];
         }
       }, st));
     }
