@@ -666,29 +666,44 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   }, st));
   // (b) the transposed Thomas solve, one thread per column:
   //     M(k,k-1) = F3(k), M(k,k) = F2(k), M(k,k+1) = F1(k)
+  //     (the bands F1(k-1), F2(k), F3(k+1) of level k all read PT(k); A(k)
+  //     is carried from the level below; the loads of level k+1 are issued
+  //     before the work of level k)
   HIPCHK(for3(Box{0, 0, IS, IE, JS, JE}, [=] __device__(int, int i, int j) {
     const size_t c0 = IX(d, 0, i, j);
     auto at = [&](int k) { return c0 + pl * (size_t)k; };
-    auto Af = [&](int k) { return dt * dt * J33G * g.RCDZ[k] * RT2P[at(k)] * J33G / m.gs(I_XYZ, at(k)); };
+    auto Af = [&](int k, double rt) { return dt * dt * J33G * g.RCDZ[k] * rt * J33G / m.gs(I_XYZ, at(k)); };
     auto Bf = [&](int k) { return GRAV * (dt * dt) * J33G / (g.CDZ[k + 1] + g.CDZ[k]); };
-    auto F1f = [&](int k) { return -(PT[at(k + 1)] * g.RFDZ[k] * Af(k + 1) + Bf(k)) / m.gs(I_XYW, at(k)); };
-    auto F2f = [&](int k) { return 1.0 + (PT[at(k)] * g.RFDZ[k] * (Af(k + 1) + Af(k))) / m.gs(I_XYW, at(k)); };
-    auto F3f = [&](int k) { return -(PT[at(k - 1)] * g.RFDZ[k] * Af(k) - Bf(k)) / m.gs(I_XYW, at(k)); };
-    double den = F2f(KS);
-    double cpk = (KS < KE - 1 ? F3f(KS + 1) : 0.0) / den;
+    auto gw = [&](int k) { return m.gs(I_XYW, at(k)); };
+    double ptk = PT[at(KS)];
+    double Ak = Af(KS, RT2P[at(KS)]), Ak1 = Af(KS + 1, RT2P[at(KS + 1)]);
+    double den = 1.0 + (ptk * g.RFDZ[KS] * (Ak1 + Ak)) / gw(KS);
+    double cpk = (KS < KE - 1 ? -(ptk * g.RFDZ[KS + 1] * Ak1 - Bf(KS + 1)) / gw(KS + 1) : 0.0) / den;
     double dpk = Cpb[at(KS)] / den;
     cpv[at(KS)] = cpk; dpv[at(KS)] = dpk;
+    double pt_n = KS + 1 <= KE - 1 ? PT[at(KS + 1)] : 0.0;
+    double rt_n = KS + 2 <= KE ? RT2P[at(KS + 2)] : 0.0;
+    double cb_n = KS + 1 <= KE - 1 ? Cpb[at(KS + 1)] : 0.0;
     for (int k = KS + 1; k <= KE - 1; ++k) {
-      const double f1m = F1f(k - 1);
-      den = F2f(k) - f1m * cpk;
-      cpk = (k < KE - 1 ? F3f(k + 1) : 0.0) / den;
-      dpk = (Cpb[at(k)] - f1m * dpk) / den;
+      ptk = pt_n;
+      Ak = Ak1;
+      Ak1 = Af(k + 1, rt_n);
+      const double cbk = cb_n;
+      if (k + 1 <= KE - 1) { pt_n = PT[at(k + 1)]; cb_n = Cpb[at(k + 1)]; }
+      if (k + 2 <= KE) rt_n = RT2P[at(k + 2)];
+      const double f1m = -(ptk * g.RFDZ[k - 1] * Ak + Bf(k - 1)) / gw(k - 1);
+      den = (1.0 + (ptk * g.RFDZ[k] * (Ak1 + Ak)) / gw(k)) - f1m * cpk;
+      cpk = (k < KE - 1 ? -(ptk * g.RFDZ[k + 1] * Ak1 - Bf(k + 1)) / gw(k + 1) : 0.0) / den;
+      dpk = (cbk - f1m * dpk) / den;
       cpv[at(k)] = cpk; dpv[at(k)] = dpk;
     }
     double cn = dpk;
     Cb[at(KE - 1)] = cn;
+    double dv = KE - 2 >= KS ? dpv[at(KE - 2)] : 0.0, cv = KE - 2 >= KS ? cpv[at(KE - 2)] : 0.0;
     for (int k = KE - 2; k >= KS; --k) {
-      cn = dpv[at(k)] - cpv[at(k)] * cn;
+      const double dk = dv, ck = cv;
+      if (k - 1 >= KS) { dv = dpv[at(k - 1)]; cv = cpv[at(k - 1)]; }
+      cn = dk - ck * cn;
       Cb[at(k)] = cn;
     }
   }, st));

@@ -757,15 +757,29 @@ __global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles,
   double* Sr_ = a.DENS_RK;
   double* St_ = a.RHOT_RK;
 
-  // ---- upward: row k-1 of the system once level k is known ----
-  double rt0 = 0.0, dp0 = 0.0, st0 = 0.0, sr0 = 0.0, d0 = 0.0, rd0 = 0.0, A0 = 0.0, pt0 = 0.0, ptm = 0.0, sw0 = 0.0;
+  // ---- upward: row k-1 of the system once level k is known; the loads of
+  //      level k+1 are issued before the work of level k ----
+  struct Up {
+    double rt, dp0, rh, rr, dn, rd, pt, st, sr, sw, mz;
+  };
+  auto up_load = [&](int k, Up& L) {
+    const size_t q = px(k);
+    L.rt = a.RT2P[q]; L.dp0 = a.DPRES0[q]; L.rh = a.RHOT[q]; L.rr = a.REF_rhot[q];
+    L.dn = a.DENS[q]; L.rd = a.REF_dens[q];
+    L.st = St_[q]; L.sr = Sr_[q];
+    const bool w = k <= KE - 1;
+    L.pt = w ? a.PT[q] : 0.0;
+    L.sw = w ? a.Sw[q] : 0.0;
+    L.mz = w ? a.MOMZ[q] : 0.0;
+  };
+  double rt0 = 0.0, dp0 = 0.0, st0 = 0.0, sr0 = 0.0, d0 = 0.0, rd0 = 0.0, A0 = 0.0, pt0 = 0.0, ptm = 0.0, sw0 = 0.0, mz0 = 0.0;
   double e_prev = 0.0, f_prev = 0.0, C_last = 0.0;
+  Up cur{}, nxt{};
+  up_load(KS, cur);
   for (int k = KS; k <= KE; ++k) {
-    const size_t q1 = px(k);
-    const double rt1 = a.RT2P[q1], dp1 = dpres(k), d1 = a.DENS[q1], rd1 = a.REF_dens[q1], A1 = Af(k, rt1);
-    const double pt1 = (k <= KE - 1) ? a.PT[q1] : 0.0;
-    const double st1 = St_[q1], sr1 = Sr_[q1];
-    const double sw1 = (k <= KE - 1) ? a.Sw[q1] : 0.0;
+    if (k < KE) up_load(k + 1, nxt);
+    const double rt1 = cur.rt, dp1 = cur.dp0 + cur.rt * (cur.rh - cur.rr), d1 = cur.dn, rd1 = cur.rd, A1 = Af(k, rt1);
+    const double pt1 = cur.pt, st1 = cur.st, sr1 = cur.sr, sw1 = cur.sw;
     double ek = 0.0, fk = 0.0;
     if (k > KS) {
       const int r = k - 1;
@@ -777,7 +791,7 @@ __global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles,
       const double F3v = -(ptm * RFDZ[r] * A0 - B) / gw;
       const double pg = -(dp1 + rt1 * dtrk * st1 - dp0 - rt0 * dtrk * st0) * RFDZ[r] * J33G / gw
                       - GRAV * (f2h(r, 1) * (d1 - rd1 + sr1 * dtrk) + f2h(r, 2) * (d0 - rd0 + sr0 * dtrk));
-      const double Ck = a.MOMZ[q] + dtrk * (pg + sw0);
+      const double Ck = mz0 + dtrk * (pg + sw0);
       if (r == KS) {
         const double rdenom = 1.0 / F2v;
         ek = -F1v * rdenom;
@@ -793,7 +807,8 @@ __global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles,
       ptm = pt0;
       if (r < KE - 1) { a.e[q] = ek; a.f[q] = fk; }
     }
-    rt0 = rt1; dp0 = dp1; st0 = st1; sr0 = sr1; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = sw1;
+    rt0 = rt1; dp0 = dp1; st0 = st1; sr0 = sr1; d0 = d1; rd0 = rd1; A0 = A1; pt0 = pt1; sw0 = sw1; mz0 = cur.mz;
+    cur = nxt;
   }
 
   // ---- downward: back substitution and updates (HEVI.F90:728-790) ----
@@ -816,19 +831,34 @@ __global__ void __launch_bounds__(256) hevi_solve(HeviArgs a, unsigned ni_tiles,
     a.MOMZ_RK[q] = a.MOMZ0[q] + (Cn - a.MOMZ[q]);
     if (a.trCp) a.trCp[q] = Cn;
   }
-  for (int k = KE - 2; k >= KS; --k) {
+  // the loads of level k-1 (and of level k's Sr/St/DENS0/RHOT0) are issued
+  // before the work of level k
+  struct Dn {
+    double e, f, pt, mz0, mz, mf, tf, d0, r0, sr, st;
+  };
+  auto dn_load = [&](int k, Dn& L) {
     const size_t q = px(k), q1 = px(k + 1);
-    const double Ck = a.e[q] * Cn + a.f[q];
-    const double ptk = a.PT[q];
-    mfZ[q] = mfZ[q] + J33G * Ck / (M12);
-    tfZ[q] = tfZ[q] + J33G * Ck * ptk / (M12);
-    a.MOMZ_RK[q] = a.MOMZ0[q] + (Ck - a.MOMZ[q]);
+    L.e = a.e[q]; L.f = a.f[q]; L.pt = a.PT[q]; L.mz0 = a.MOMZ0[q]; L.mz = a.MOMZ[q];
+    L.mf = mfZ[q]; L.tf = tfZ[q];
+    L.d0 = a.DENS0[q1]; L.r0 = a.RHOT0[q1]; L.sr = Sr_[q1]; L.st = St_[q1];
+  };
+  Dn dc{}, dn{};
+  if (KE - 2 >= KS) dn_load(KE - 2, dc);
+  for (int k = KE - 2; k >= KS; --k) {
+    if (k > KS) dn_load(k - 1, dn);
+    const size_t q = px(k), q1 = px(k + 1);
+    const double Ck = dc.e * Cn + dc.f;
+    const double ptk = dc.pt;
+    mfZ[q] = dc.mf + J33G * Ck / (M12);
+    tfZ[q] = dc.tf + J33G * Ck * ptk / (M12);
+    a.MOMZ_RK[q] = dc.mz0 + (Ck - dc.mz);
     if (a.trCp) a.trCp[q] = Ck;
     double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
-    a.DENS_RK[q1] = a.DENS0[q1] + dtrk * (advcv + Sr_[q1]);
+    a.DENS_RK[q1] = dc.d0 + dtrk * (advcv + dc.sr);
     advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1] / m.gs(I_XYZ, q1);
-    a.RHOT_RK[q1] = a.RHOT0[q1] + dtrk * (advcv + St_[q1]);
+    a.RHOT_RK[q1] = dc.r0 + dtrk * (advcv + dc.st);
     Cn = Ck; ptn = ptk;
+    dc = dn;
   }
   {
     const size_t q = px(KS);
