@@ -118,54 +118,76 @@ __device__ __forceinline__ double face_adj(int kind, const double* q, double* qb
   return face_adj_s<S>(kind, q, p, st, vel, scale, [&](int m, double v) { aadd(qb + p + (size_t)(m - 1) * st, v); });
 }
 
-// Vertical faces of a box, L consecutive faces of a column per thread,
-// upward: the adjoint of a stencil along z (contributions to the points
-// k-1..k+2 of the face's own column) collects in a register window and each
-// point of the column is added once, without an atomic where every face that
-// reaches it (faces q-2..q+1) is the thread's own.  f(k, i, j, win) adds
-// win.a[m] for the point k-1+m.
+// Faces of a box along z, y or x (DIR 0, 1, 2), L consecutive faces of a
+// column (a y row, an x row) per thread, in order: the adjoint of a stencil
+// along that direction
+// (contributions to the points l-1..l+2 of the face's own line) collects in a
+// register window and each point of the line is added once, without an
+// atomic where every face that reaches it (faces q-2..q+1) is the thread's
+// own.  f(k, i, j, win) adds win.a[m] for the point l-1+m of tgt, and
+// win.b[m] for the point l-1+m of tgt2 (added with atomics: other lines may
+// reach it too).  The x stencils keep one face per thread: with a thread per
+// x row the lanes of a wave read 64 rows apart, and the warm bubble's
+// reverse sweep ran 32 % slower that way than with their atomics.
 struct ZWin {
   double a[4] = {0.0, 0.0, 0.0, 0.0};
+  double b[4] = {0.0, 0.0, 0.0, 0.0};
 };
 
-template <typename F>
-__global__ void __launch_bounds__(256) zseg_kernel(Dims d, Box b, int L, double* tgt, F f) {
+template <int DIR, typename F>
+__global__ void __launch_bounds__(256) lseg_kernel(Dims d, Box b, int L, double* tgt, double* tgt2, F f) {
   const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
-  const int nseg = (nk + L - 1) / L;
-  const size_t n = (size_t)ni * (size_t)nj * (size_t)nseg;
+  // the fastest thread index and the other one across the lines (i and j,
+  // i and k, j and k), and the line's own index (k, j, i)
+  const int n1 = DIR == 2 ? nj : ni, n2 = DIR == 0 ? nj : nk, nl = DIR == 0 ? nk : (DIR == 1 ? nj : ni);
+  const int nseg = (nl + L - 1) / L;
+  const size_t n = (size_t)n1 * (size_t)n2 * (size_t)nseg;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int i = b.i0 + (int)(t % (size_t)ni);
-  const size_t r = t / (size_t)ni;
-  const int j = b.j0 + (int)(r % (size_t)nj);
-  const int k0 = b.k0 + (int)(r / (size_t)nj) * L, k1 = min(b.k1, k0 + L - 1);
-  auto flush = [&](int k, double v, bool own) {
-    if (v == 0.0) return;
-    double* q = tgt + IX(d, k, i, j);
-    if (own) *q += v;
-    else aadd(q, v);
+  const int c1 = (int)(t % (size_t)n1);
+  const size_t r = t / (size_t)n1;
+  const int c2 = (int)(r % (size_t)n2);
+  const int lb = DIR == 0 ? b.k0 : (DIR == 1 ? b.j0 : b.i0), le = DIR == 0 ? b.k1 : (DIR == 1 ? b.j1 : b.i1);
+  const int l0 = lb + (int)(r / (size_t)n2) * L, l1 = min(le, l0 + L - 1);
+  const int i = DIR == 2 ? 0 : b.i0 + c1;
+  const int j = DIR == 2 ? b.j0 + c1 : (DIR == 0 ? b.j0 + c2 : 0);
+  const int k = DIR == 0 ? 0 : b.k0 + c2;
+  auto at = [&](int l) { return DIR == 0 ? IX(d, l, i, j) : (DIR == 1 ? IX(d, k, i, l) : IX(d, k, l, j)); };
+  auto flush = [&](int l, double va, double vb, bool own) {
+    if (va != 0.0) {
+      double* q = tgt + at(l);
+      if (own) *q += va;
+      else aadd(q, va);
+    }
+    if (vb != 0.0) aadd(tgt2 + at(l), vb);
   };
   ZWin w;
-  for (int k = k0; k <= k1; ++k) {
-    f(k, i, j, w);
-    flush(k - 1, w.a[0], k - 1 >= k0 + 2);  // no later face reaches k-1
+  for (int l = l0; l <= l1; ++l) {
+    if constexpr (DIR == 0) f(l, i, j, w);
+    else if constexpr (DIR == 1) f(k, i, l, w);
+    else f(k, l, j, w);
+    flush(l - 1, w.a[0], w.b[0], l - 1 >= l0 + 2);  // no later face reaches l-1
     w.a[0] = w.a[1]; w.a[1] = w.a[2]; w.a[2] = w.a[3]; w.a[3] = 0.0;
+    w.b[0] = w.b[1]; w.b[1] = w.b[2]; w.b[2] = w.b[3]; w.b[3] = 0.0;
   }
-  flush(k1, w.a[0], false);
-  flush(k1 + 1, w.a[1], false);
-  flush(k1 + 2, w.a[2], false);
+  flush(l1, w.a[0], w.b[0], false);
+  flush(l1 + 1, w.a[1], w.b[1], false);
+  flush(l1 + 2, w.a[2], w.b[2], false);
 }
 
 constexpr int ZSEG_L = 8;
 
-template <typename F>
-hipError_t zseg(const Dims& d, Box b, double* tgt, F f, hipStream_t s) {
+template <int DIR, typename F>
+hipError_t lseg(const Dims& d, Box b, double* tgt, double* tgt2, F f, hipStream_t s) {
   if (b.k1 < b.k0 || b.i1 < b.i0 || b.j1 < b.j0) return hipSuccess;
-  const int nk = b.k1 - b.k0 + 1;
-  const size_t n = (size_t)(b.i1 - b.i0 + 1) * (size_t)(b.j1 - b.j0 + 1) * (size_t)((nk + ZSEG_L - 1) / ZSEG_L);
-  hipLaunchKernelGGL(zseg_kernel<F>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, ZSEG_L, tgt, f);
+  const size_t ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+  const size_t nl = DIR == 0 ? nk : (DIR == 1 ? nj : ni);
+  const size_t n = ni * nj * nk / nl * ((nl + ZSEG_L - 1) / ZSEG_L);
+  hipLaunchKernelGGL((lseg_kernel<DIR, F>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, ZSEG_L, tgt, tgt2, f);
   return hipGetLastError();
 }
+template <typename F>
+hipError_t zseg(const Dims& d, Box b, double* tgt, F f, hipStream_t s) { return lseg<0>(d, b, tgt, nullptr, f, s); }
 
 __device__ __forceinline__ int zkind(int k, int KS, int KE) { return k == KS ? 1 : (k == KE - 1 ? 2 : 0); }
 
@@ -551,16 +573,18 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     const double denb = 0.25 * (-velb * vel / den);
     aadd(Db + p, denb); aadd(Db + p + 1, denb); aadd(Db + p + IA, denb); aadd(Db + p + 1 + IA, denb);
   }, st));
-  // fluxY_XVZ (UD3.F90:1536-1587): flux(k,i,j-1) at loop index j
-  HIPCHK(for3(Box{KS, KE, IS, IE, JS, JE + 1}, [=] __device__(int k, int i, int j) {
+  // fluxY_XVZ (UD3.F90:1536-1587): flux(k,i,j-1) at loop index j, here the
+  // face jf = j-1; the MOMY and DENS adjoints of the y row in windows
+  HIPCHK(lseg<1>(d, Box{KS, KE, IS, IE, JS - 1, JE}, MYb, Db, [=] __device__(int k, int i, int jf, ZWin& yw) {
+    const int j = jf + 1;
     const size_t p = IX(d, k, i, j), q = p - IA;
     const double vel = 0.5 * (MY[p] + MY[q]) / D[p];
-    const double fb = fV.y(k, i, j - 1);
+    const double fb = fV.y(k, i, jf);
     const double G = m.gs(I_XYZ, p), gm = G / m.mapf(i, j, 0, I_XY);
     NDb[oVY + p] += fb * G;
-    const double velb = fb * gm * face_adj<S>(0, MY, MYb, q, IA, vel, fb * gm * vel);
-    aadd(MYb + p, 0.5 * velb / D[p]); aadd(MYb + q, 0.5 * velb / D[p]);
-    aadd(Db + p, -velb * vel / D[p]);
+    const double velb = fb * gm * face_adj_s<S>(0, MY, q, IA, vel, fb * gm * vel, [&](int c, double v) { yw.a[c] += v; });
+    yw.a[2] += 0.5 * velb / D[p]; yw.a[1] += 0.5 * velb / D[p];
+    yw.b[2] += -velb * vel / D[p];
   }, st));
 
   // ---- momentum x (HEVI.F90:816-927) ----
@@ -630,18 +654,19 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     aadd(MXb + p, 0.5 * velb / D[p]); aadd(MXb + q, 0.5 * velb / D[p]);
     aadd(Db + p, -velb * vel / D[p]);
   }, st));
-  // fluxY_UYZ (UD3.F90:1101-1150)
-  HIPCHK(for3(Box{KS, KE, IS, IE, JS - 1, JE}, [=] __device__(int k, int i, int j) {
+  // fluxY_UYZ (UD3.F90:1101-1150): the MOMX adjoint of the y row in a window
+  HIPCHK(lseg<1>(d, Box{KS, KE, IS, IE, JS - 1, JE}, MXb, Db, [=] __device__(int k, int i, int j, ZWin& yw) {
     const size_t p = IX(d, k, i, j);
     const double den = 0.25 * (D[p] + D[p + 1] + D[p + IA] + D[p + 1 + IA]);
     const double vel = 0.5 * (MY[p] + MY[p + 1]) / den;
     const double fb = fU.y(k, i, j);
     const double G = m.gs(I_UVZ, p), gm = G / m.mapf(i, j, 0, I_UV);
     NDb[oUY + p] += fb * G;
-    const double velb = fb * gm * face_adj<S>(0, MX, MXb, p, IA, vel, fb * gm * vel);
+    const double velb = fb * gm * face_adj_s<S>(0, MX, p, IA, vel, fb * gm * vel, [&](int c, double v) { yw.a[c] += v; });
     aadd(MYb + p, 0.5 * velb / den); aadd(MYb + p + 1, 0.5 * velb / den);
     const double denb = 0.25 * (-velb * vel / den);
-    aadd(Db + p, denb); aadd(Db + p + 1, denb); aadd(Db + p + IA, denb); aadd(Db + p + 1 + IA, denb);
+    yw.b[1] += denb; yw.b[2] += denb;
+    aadd(Db + p + 1, denb); aadd(Db + p + 1 + IA, denb);
   }, st));
 
   // ---- vertical implicit block (HEVI.F90:688-804) ----
@@ -779,12 +804,12 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     NDb[oTX + p] += fb * G;
     mXb[p] += fb * face_adj<S>(0, POTT, POTTb, p, 1, vel, fb * vel);
   }, st));
-  HIPCHK(for3(Box{KS, KE, IS, IE, JS - 1, JE}, [=] __device__(int k, int i, int j) {
+  HIPCHK(lseg<1>(d, Box{KS, KE, IS, IE, JS - 1, JE}, POTTb, nullptr, [=] __device__(int k, int i, int j, ZWin& yw) {
     const size_t p = IX(d, k, i, j);
     const double G = m.gs(I_XVZ, p);
     const double fb = fT.y(k, i, j), vel = G / m.mapf(i, j, 0, I_XV) * (MY[p] + ND[oDY + p]);
     NDb[oTY + p] += fb * G;
-    mYb[p] += fb * face_adj<S>(0, POTT, POTTb, p, IA, vel, fb * vel);
+    mYb[p] += fb * face_adj_s<S>(0, POTT, p, IA, vel, fb * vel, [&](int c, double v) { yw.a[c] += v; });
   }, st));
 
   // ---- Sw (HEVI.F90:538-577) and the MOMZ fluxes (HEVI.F90:486-525): the
@@ -833,29 +858,38 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       }, st));
     }
   }
-  // fluxX_XYW / fluxY_XYW (UD3.F90:598-748)
-  for (int dir = 0; dir < 2; ++dir) {
-    const size_t sd = dir == 0 ? 1 : (size_t)IA;
-    const double* mom = dir == 0 ? MX : MY;
-    double* momb = dir == 0 ? MXb : MYb;
-    const size_t on = dir == 0 ? oWX : oWY;
-    const int cG = dir == 0 ? I_UYW : I_XVW, cM = dir == 0 ? I_UY : I_XV, mn = dir == 0 ? 1 : 0;
-    HIPCHK(for3(Box{KS, KE - 1, dir == 0 ? IS - 1 : IS, IE, dir == 0 ? JS : JS - 1, JE}, [=] __device__(int k, int i, int j) {
-      const size_t p = IX(d, k, i, j);
-      const double f1 = f2h(m, g, d, cG, k, 1, i, j), f2 = f2h(m, g, d, cG, k, 2, i, j);
-      const double num = f1 * mom[p + pl] + f2 * mom[p];
-      const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + sd]) + f2 * 0.5 * (D[p] + D[p + sd]);
-      const double vel = num / den;
-      const double fb = dir == 0 ? fW.x(k, i, j) : fW.y(k, i, j);
-      const double G = m.gs(cG, p), gm = G / m.mapf(i, j, mn, cM);
-      NDb[on + p] += fb * G;
-      const double velb = fb * gm * face_adj<S>(0, MZ, MZb, p, sd, vel, fb * gm * vel);
-      const double numb = velb / den, denb = -velb * vel / den;
-      aadd(momb + p + pl, f1 * numb); aadd(momb + p, f2 * numb);
-      aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + sd, 0.5 * f1 * denb);
-      aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + sd, 0.5 * f2 * denb);
-    }, st));
-  }
+  // fluxX_XYW (UD3.F90:598-671)
+  HIPCHK(for3(Box{KS, KE - 1, IS - 1, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+    const size_t p = IX(d, k, i, j);
+    const double f1 = f2h(m, g, d, I_UYW, k, 1, i, j), f2 = f2h(m, g, d, I_UYW, k, 2, i, j);
+    const double num = f1 * MX[p + pl] + f2 * MX[p];
+    const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + 1]) + f2 * 0.5 * (D[p] + D[p + 1]);
+    const double vel = num / den;
+    const double fb = fW.x(k, i, j);
+    const double G = m.gs(I_UYW, p), gm = G / m.mapf(i, j, 1, I_UY);
+    NDb[oWX + p] += fb * G;
+    const double velb = fb * gm * face_adj<S>(0, MZ, MZb, p, 1, vel, fb * gm * vel);
+    const double numb = velb / den, denb = -velb * vel / den;
+    aadd(MXb + p + pl, f1 * numb); aadd(MXb + p, f2 * numb);
+    aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + 1, 0.5 * f1 * denb);
+    aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + 1, 0.5 * f2 * denb);
+  }, st));
+  // fluxY_XYW (UD3.F90:675-748): the MOMZ adjoint of the y row in a window
+  HIPCHK(lseg<1>(d, Box{KS, KE - 1, IS, IE, JS - 1, JE}, MZb, Db, [=] __device__(int k, int i, int j, ZWin& yw) {
+    const size_t p = IX(d, k, i, j);
+    const double f1 = f2h(m, g, d, I_XVW, k, 1, i, j), f2 = f2h(m, g, d, I_XVW, k, 2, i, j);
+    const double num = f1 * MY[p + pl] + f2 * MY[p];
+    const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + IA]) + f2 * 0.5 * (D[p] + D[p + IA]);
+    const double vel = num / den;
+    const double fb = fW.y(k, i, j);
+    const double G = m.gs(I_XVW, p), gm = G / m.mapf(i, j, 0, I_XV);
+    NDb[oWY + p] += fb * G;
+    const double velb = fb * gm * face_adj_s<S>(0, MZ, p, IA, vel, fb * gm * vel, [&](int c, double v) { yw.a[c] += v; });
+    const double numb = velb / den, denb = -velb * vel / den;
+    aadd(MYb + p + pl, f1 * numb); aadd(MYb + p, f2 * numb);
+    aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + IA, 0.5 * f1 * denb);
+    yw.b[1] += 0.5 * f2 * denb; yw.b[2] += 0.5 * f2 * denb;
+  }, st));
 
   // ---- mass fluxes before the solve (HEVI.F90:369-438): the theta faces'
   //      adjoints plus those gathered from Sr (HEVI.F90:453-475, fR) ----
