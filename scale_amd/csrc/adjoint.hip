@@ -692,9 +692,11 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   // (b) the transposed Thomas solve, one thread per column:
   //     M(k,k-1) = F3(k), M(k,k) = F2(k), M(k,k+1) = F1(k)
   //     (the bands F1(k-1), F2(k), F3(k+1) of level k all read PT(k); A(k)
-  //     is carried from the level below; the loads of level k+1 are issued
-  //     before the work of level k)
+  //     is carried from the level below.  With few columns this runs about
+  //     one wave per CU, so the loads go in chunks of ZCH levels, the next
+  //     chunk's in flight during the current chunk's work)
   HIPCHK(for3(Box{0, 0, IS, IE, JS, JE}, [=] __device__(int, int i, int j) {
+    constexpr int ZCH = 8;
     const size_t c0 = IX(d, 0, i, j);
     auto at = [&](int k) { return c0 + pl * (size_t)k; };
     auto Af = [&](int k, double rt) { return dt * dt * J33G * g.RCDZ[k] * rt * J33G / m.gs(I_XYZ, at(k)); };
@@ -706,30 +708,62 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     double cpk = (KS < KE - 1 ? -(ptk * g.RFDZ[KS + 1] * Ak1 - Bf(KS + 1)) / gw(KS + 1) : 0.0) / den;
     double dpk = Cpb[at(KS)] / den;
     cpv[at(KS)] = cpk; dpv[at(KS)] = dpk;
-    double pt_n = KS + 1 <= KE - 1 ? PT[at(KS + 1)] : 0.0;
-    double rt_n = KS + 2 <= KE ? RT2P[at(KS + 2)] : 0.0;
-    double cb_n = KS + 1 <= KE - 1 ? Cpb[at(KS + 1)] : 0.0;
-    for (int k = KS + 1; k <= KE - 1; ++k) {
-      ptk = pt_n;
-      Ak = Ak1;
-      Ak1 = Af(k + 1, rt_n);
-      const double cbk = cb_n;
-      if (k + 1 <= KE - 1) { pt_n = PT[at(k + 1)]; cb_n = Cpb[at(k + 1)]; }
-      if (k + 2 <= KE) rt_n = RT2P[at(k + 2)];
-      const double f1m = -(ptk * g.RFDZ[k - 1] * Ak + Bf(k - 1)) / gw(k - 1);
-      den = (1.0 + (ptk * g.RFDZ[k] * (Ak1 + Ak)) / gw(k)) - f1m * cpk;
-      cpk = (k < KE - 1 ? -(ptk * g.RFDZ[k + 1] * Ak1 - Bf(k + 1)) / gw(k + 1) : 0.0) / den;
-      dpk = (cbk - f1m * dpk) / den;
-      cpv[at(k)] = cpk; dpv[at(k)] = dpk;
+    // chunk of levels kc..kc+ZCH-1 (up to KE-1): PT(k), RT2P(k+1), Cpb(k)
+    double pt[ZCH], rt[ZCH], cb[ZCH], npt[ZCH], nrt[ZCH], ncb[ZCH];
+    auto load = [&](int kc, double* P, double* R, double* Cc) {
+_Pragma("unroll")
+      for (int c = 0; c < ZCH; ++c) {
+        const int k = kc + c;
+        const bool in = k <= KE - 1;
+        P[c] = in ? PT[at(k)] : 0.0;
+        R[c] = in ? RT2P[at(k + 1)] : 0.0;
+        Cc[c] = in ? Cpb[at(k)] : 0.0;
+      }
+    };
+    load(KS + 1, pt, rt, cb);
+    for (int kc = KS + 1; kc <= KE - 1; kc += ZCH) {
+      if (kc + ZCH <= KE - 1) load(kc + ZCH, npt, nrt, ncb);
+_Pragma("unroll")
+      for (int c = 0; c < ZCH; ++c) {
+        const int k = kc + c;
+        if (k > KE - 1) break;
+        ptk = pt[c];
+        Ak = Ak1;
+        Ak1 = Af(k + 1, rt[c]);
+        const double f1m = -(ptk * g.RFDZ[k - 1] * Ak + Bf(k - 1)) / gw(k - 1);
+        den = (1.0 + (ptk * g.RFDZ[k] * (Ak1 + Ak)) / gw(k)) - f1m * cpk;
+        cpk = (k < KE - 1 ? -(ptk * g.RFDZ[k + 1] * Ak1 - Bf(k + 1)) / gw(k + 1) : 0.0) / den;
+        dpk = (cb[c] - f1m * dpk) / den;
+        cpv[at(k)] = cpk; dpv[at(k)] = dpk;
+      }
+_Pragma("unroll")
+      for (int c = 0; c < ZCH; ++c) { pt[c] = npt[c]; rt[c] = nrt[c]; cb[c] = ncb[c]; }
     }
     double cn = dpk;
     Cb[at(KE - 1)] = cn;
-    double dv = KE - 2 >= KS ? dpv[at(KE - 2)] : 0.0, cv = KE - 2 >= KS ? cpv[at(KE - 2)] : 0.0;
-    for (int k = KE - 2; k >= KS; --k) {
-      const double dk = dv, ck = cv;
-      if (k - 1 >= KS) { dv = dpv[at(k - 1)]; cv = cpv[at(k - 1)]; }
-      cn = dk - ck * cn;
-      Cb[at(k)] = cn;
+    // back substitution, chunks of levels kc..kc-ZCH+1 (down to KS)
+    double dv[ZCH], cv[ZCH], ndv[ZCH], ncv[ZCH];
+    auto loadb = [&](int kc, double* Dv, double* Cv) {
+_Pragma("unroll")
+      for (int c = 0; c < ZCH; ++c) {
+        const int k = kc - c;
+        const bool in = k >= KS;
+        Dv[c] = in ? dpv[at(k)] : 0.0;
+        Cv[c] = in ? cpv[at(k)] : 0.0;
+      }
+    };
+    loadb(KE - 2, dv, cv);
+    for (int kc = KE - 2; kc >= KS; kc -= ZCH) {
+      if (kc - ZCH >= KS) loadb(kc - ZCH, ndv, ncv);
+_Pragma("unroll")
+      for (int c = 0; c < ZCH; ++c) {
+        const int k = kc - c;
+        if (k < KS) break;
+        cn = dv[c] - cv[c] * cn;
+        Cb[at(k)] = cn;
+      }
+_Pragma("unroll")
+      for (int c = 0; c < ZCH; ++c) { dv[c] = ndv[c]; cv[c] = ncv[c]; }
     }
   }, st));
   // (c) per level: adjoints of the band coefficients (through PT and
