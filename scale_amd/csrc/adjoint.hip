@@ -1115,9 +1115,17 @@ static hipError_t diff3_adj(const Ctx& c, const NfTab t, double* dZb, const doub
       B(KS + 1) -= B(KS - 2); B(KS) -= B(KS - 1); B(KS - 2) = 0.0; B(KS - 1) = 0.0;
       { const double b = B(KS);
         Pb(KS + 2) += t.cz(1, KS + 1, 1) * b; Pb(KS + 1) -= t.cz(2, KS + 1, 1) * b; Pb(KS) += t.cz(3, KS + 1, 1) * b; Pb(KS + 1) -= t.cz(1, KS, 1) * b; }
-      for (int k = KS + 1; k <= KE - 2; ++k) {
-        const double b = B(k);
-        Pb(k + 2) += t.cz(1, k + 1, 1) * b; Pb(k + 1) -= t.cz(2, k + 1, 1) * b; Pb(k) += t.cz(3, k + 1, 1) * b; Pb(k - 1) -= t.cz(1, k, 1) * b;
+      // rows KS+1..KE-2 in a register window w0..w3 = Pb(k-1..k+2): each
+      // level is loaded and stored once, its terms added in the rows' order
+      if (KS + 1 <= KE - 2) {
+        double w0 = Pb(KS), w1 = Pb(KS + 1), w2 = Pb(KS + 2), w3 = Pb(KS + 3);
+        for (int k = KS + 1; k <= KE - 2; ++k) {
+          const double b = B(k);
+          w3 += t.cz(1, k + 1, 1) * b; w2 -= t.cz(2, k + 1, 1) * b; w1 += t.cz(3, k + 1, 1) * b; w0 -= t.cz(1, k, 1) * b;
+          Pb(k - 1) = w0;
+          w0 = w1; w1 = w2; w2 = w3; w3 = Pb(k + 3);
+        }
+        Pb(KE - 2) = w0; Pb(KE - 1) = w1; Pb(KE) = w2;
       }
     } else {
       B(KE - 1) -= B(KE + 2); B(KE) -= B(KE + 1); B(KE + 1) = 0.0; B(KE + 2) = 0.0;
@@ -1128,9 +1136,16 @@ static hipError_t diff3_adj(const Ctx& c, const NfTab t, double* dZb, const doub
       B(KS + 3) -= B(KS - 2); B(KS + 2) -= B(KS - 1); B(KS + 1) -= B(KS); B(KS - 2) = 0.0; B(KS - 1) = 0.0; B(KS) = 0.0;
       { const double b = B(KS + 1);
         Pb(KS + 2) += t.cz(1, KS + 1, 2) * b; Pb(KS + 1) -= t.cz(2, KS + 1, 2) * b; Pb(KS) += t.cz(3, KS + 1, 2) * b; Pb(KS + 1) -= t.cz(1, KS, 2) * b; }
-      for (int k = KS + 2; k <= KE - 2; ++k) {
-        const double b = B(k);
-        Pb(k + 1) += t.cz(1, k, 2) * b; Pb(k) -= t.cz(2, k, 2) * b; Pb(k - 1) += t.cz(3, k, 2) * b; Pb(k - 2) -= t.cz(1, k - 1, 2) * b;
+      // rows KS+2..KE-2 in a register window w0..w3 = Pb(k-2..k+1)
+      if (KS + 2 <= KE - 2) {
+        double w0 = Pb(KS), w1 = Pb(KS + 1), w2 = Pb(KS + 2), w3 = Pb(KS + 3);
+        for (int k = KS + 2; k <= KE - 2; ++k) {
+          const double b = B(k);
+          w3 += t.cz(1, k, 2) * b; w2 -= t.cz(2, k, 2) * b; w1 += t.cz(3, k, 2) * b; w0 -= t.cz(1, k - 1, 2) * b;
+          Pb(k - 2) = w0;
+          w0 = w1; w1 = w2; w2 = w3; w3 = Pb(k + 2);
+        }
+        Pb(KE - 3) = w0; Pb(KE - 2) = w1; Pb(KE - 1) = w2; Pb(KE) = w3;
       }
     }
   }, s));
