@@ -1229,7 +1229,7 @@ hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const d
   HIPCHK(diff3_adj(c, t, wZb, wXb, wYb, fb, 0, 0, 0, s));
   HIPCHK(for3(Box{KS, KE, IS - 1, IE + 2, JS - 1, JE + 2}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    aadd(Db + p, fb[p]);
+    Db[p] += fb[p];
   }, s));
 
   // z-momentum (COMMON.F90:629-707)
@@ -1259,12 +1259,18 @@ hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const d
     }, s));
   }
   HIPCHK(diff3_adj(c, t, wZb, wXb, wYb, fb, 1, 0, 0, s));
-  HIPCHK(for3(Box{KS, KE - 1, IS - 2, IE + 2, JS - 2, JE + 2}, [=] __device__(int k, int i, int j) {
+  // fld = 2 MOMZ / (D(k+1) + D(k)) on k = KS..KE-1: MOMZ's adjoint at the
+  // point, DENS's gathered from the two faces whose average reads it
+  HIPCHK(for3(Box{KS, KE, IS - 2, IE + 2, JS - 2, JE + 2}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double ds = D[p + pl] + D[p];
-    aadd(MZb + p, fb[p] * 2.0 / ds);
-    const double dsb = -fb[p] * fld[p] / ds;
-    aadd(Db + p + pl, dsb); aadd(Db + p, dsb);
+    auto dsb = [&](size_t r) { return -fb[r] * fld[r] / (D[r + pl] + D[r]); };
+    double acc = 0.0;
+    if (k <= KE - 1) {
+      MZb[p] += fb[p] * 2.0 / (D[p + pl] + D[p]);
+      acc += dsb(p);
+    }
+    if (k - 1 >= KS) acc += dsb(p - pl);
+    Db[p] += acc;
   }, s));
 
   // x-momentum (COMMON.F90:713-790)
@@ -1294,12 +1300,17 @@ hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const d
     }, s));
   }
   HIPCHK(diff3_adj(c, t, wZb, wXb, wYb, fb, 0, 1, 0, s));
-  HIPCHK(for3(Box{KS, KE, IS - 2, IE + 1, JS - 2, JE + 2}, [=] __device__(int k, int i, int j) {
+  // fld = 2 MOMX / (D(i+1) + D(i)) on i = IS-2..IE+1, gathered as for MOMZ
+  HIPCHK(for3(Box{KS, KE, IS - 2, IE + 2, JS - 2, JE + 2}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double ds = D[p + 1] + D[p];
-    aadd(MXb + p, fb[p] * 2.0 / ds);
-    const double dsb = -fb[p] * fld[p] / ds;
-    aadd(Db + p + 1, dsb); aadd(Db + p, dsb);
+    auto dsb = [&](size_t r) { return -fb[r] * fld[r] / (D[r + 1] + D[r]); };
+    double acc = 0.0;
+    if (i <= IE + 1) {
+      MXb[p] += fb[p] * 2.0 / (D[p + 1] + D[p]);
+      acc += dsb(p);
+    }
+    if (i - 1 >= IS - 2) acc += dsb(p - 1);
+    Db[p] += acc;
   }, s));
 
   // y-momentum (COMMON.F90:796-873)
@@ -1329,12 +1340,17 @@ hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const d
     }, s));
   }
   HIPCHK(diff3_adj(c, t, wZb, wXb, wYb, fb, 0, 0, 1, s));
-  HIPCHK(for3(Box{KS, KE, IS - 2, IE + 2, JS - 2, JE + 1}, [=] __device__(int k, int i, int j) {
+  // fld = 2 MOMY / (D(j+1) + D(j)) on j = JS-2..JE+1, gathered as for MOMZ
+  HIPCHK(for3(Box{KS, KE, IS - 2, IE + 2, JS - 2, JE + 2}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    const double ds = D[p + IA] + D[p];
-    aadd(MYb + p, fb[p] * 2.0 / ds);
-    const double dsb = -fb[p] * fld[p] / ds;
-    aadd(Db + p + IA, dsb); aadd(Db + p, dsb);
+    auto dsb = [&](size_t r) { return -fb[r] * fld[r] / (D[r + IA] + D[r]); };
+    double acc = 0.0;
+    if (j <= JE + 1) {
+      MYb[p] += fb[p] * 2.0 / (D[p + IA] + D[p]);
+      acc += dsb(p);
+    }
+    if (j - 1 >= JS - 2) acc += dsb(p - IA);
+    Db[p] += acc;
   }, s));
 
   // rho*theta (COMMON.F90:880-962)
@@ -1375,8 +1391,8 @@ hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const d
   HIPCHK(diff3_adj(c, t, wZb, wXb, wYb, fb, 0, 0, 0, s));
   HIPCHK(for3(Box{KS, KE, IS - 1, IE + 2, JS - 1, JE + 2}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
-    aadd(RTb + p, fb[p] / D[p]);
-    aadd(Db + p, -fb[p] * RT[p] / (D[p] * D[p]));
+    RTb[p] += fb[p] / D[p];
+    Db[p] += -fb[p] * RT[p] / (D[p] * D[p]);
   }, s));
   return hipSuccess;
 }
