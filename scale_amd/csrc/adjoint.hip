@@ -201,7 +201,7 @@ enum {
   W_POTTB, W_STB, W_SRB, W_SWB, W_DPRESB,
   W_MZ, W_MX, W_MY,
   W_CPB, W_PTB, W_AB, W_CB, W_CPV, W_DPV,
-  W_FLD, W_FB, W_W0, W_W1, W_W2, W_WB0, W_WB1, W_WB2,
+  W_FLD, W_W0, W_W1, W_W2, W_WB0, W_WB1, W_WB2, W_FB,  // W_WB0..W_FB adjacent: one fill clears them
   NW
 };
 
@@ -1211,9 +1211,8 @@ hipError_t numfilter_adj(Ctx& c, double* NDb, const double* const st[5], const d
     b[IX(d, KS, i, j)] *= sfc;
     b[IX(d, KS + 1, i, j)] *= (1.0 + sfc) * 0.5;
   }, s));
-  auto zero_w = [&]() -> hipError_t {
-    for (double* p : {wZb, wXb, wYb, fb}) HIPCHK(hipMemsetAsync(p, 0, n3 * sizeof(double), s));
-    return hipSuccess;
+  auto zero_w = [&]() -> hipError_t {  // wZb, wXb, wYb and fb: adjacent work fields
+    return hipMemsetAsync(wZb, 0, (W_FB - W_WB0 + 1) * n3 * sizeof(double), s);
   };
 
   // density (COMMON.F90:541-621)
