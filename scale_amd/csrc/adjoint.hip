@@ -126,9 +126,9 @@ __device__ __forceinline__ double face_adj(int kind, const double* q, double* qb
 // atomic where every face that reaches it (faces q-2..q+1) is the thread's
 // own.  f(k, i, j, win) adds win.a[m] for the point l-1+m of tgt, and
 // win.b[m] for the point l-1+m of tgt2 (added with atomics: other lines may
-// reach it too).  The x stencils keep one face per thread: with a thread per
-// x row the lanes of a wave read 64 rows apart, and the warm bubble's
-// reverse sweep ran 32 % slower that way than with their atomics.
+// reach it too).  The x stencils keep one face per thread (xrow below): with
+// a thread per x row the lanes of a wave read 64 rows apart, and the warm
+// bubble's reverse sweep ran 32 % slower that way than with atomics.
 struct ZWin {
   double a[4] = {0.0, 0.0, 0.0, 0.0};
   double b[4] = {0.0, 0.0, 0.0, 0.0};
@@ -173,6 +173,65 @@ __global__ void __launch_bounds__(256) lseg_kernel(Dims d, Box b, int L, double*
   flush(l1, w.a[0], w.b[0], false);
   flush(l1 + 1, w.a[1], w.b[1], false);
   flush(l1 + 2, w.a[2], w.b[2], false);
+}
+
+// Faces of a box along x, one face per thread as for3 maps them (i fastest):
+// f(k, i, j, win) returns the face's contributions to the points p-1..p+2 of
+// its x row (p = IX(k, i, j)) in win.v[t][0..3], for the target tg[t] (a field,
+// or a field shifted to another row: Db + IA).  Neighbouring lanes of a wave
+// hold neighbouring faces of a row, so each lane sums what its wave's faces
+// give its own point (shuffles) and adds that with one atomic; contributions
+// whose point no lane of the wave owns (row ends, wave ends) are added by
+// their own lane.
+template <int NT>
+struct XWin {
+  double v[NT][4];
+};
+template <int NT>
+struct XTgt {
+  double* p[NT];
+};
+
+template <int NT, typename F>
+__global__ void __launch_bounds__(256) xrow_kernel(Dims d, Box b, XTgt<NT> tg, F f) {
+  const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+  const size_t n = (size_t)ni * (size_t)nj * (size_t)nk;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  XWin<NT> w;
+#pragma unroll
+  for (int q = 0; q < NT; ++q) w.v[q][0] = w.v[q][1] = w.v[q][2] = w.v[q][3] = 0.0;
+  long long p = -8;  // no point: never p-1 or p-2 of another lane's
+  if (t < n) {
+    const int i = b.i0 + (int)(t % (size_t)ni);
+    const size_t r = t / (size_t)ni;
+    const int j = b.j0 + (int)(r % (size_t)nj);
+    const int k = b.k0 + (int)(r / (size_t)nj);
+    p = (long long)IX(d, k, i, j);
+    f(k, i, j, w);
+  }
+  // every lane of the wave takes part in the shuffles (a lane with no
+  // neighbour in that direction reads its own value, which fails the test)
+  const long long pu1 = __shfl_up(p, 1), pu2 = __shfl_up(p, 2), pd1 = __shfl_down(p, 1), pd2 = __shfl_down(p, 2);
+  const bool u1 = pu1 == p - 1, u2 = u1 && pu2 == p - 2, d1 = pd1 == p + 1, d2 = d1 && pd2 == p + 2;
+#pragma unroll
+  for (int q = 0; q < NT; ++q) {
+    const double from_d1 = __shfl_down(w.v[q][0], 1), from_u1 = __shfl_up(w.v[q][2], 1), from_u2 = __shfl_up(w.v[q][3], 2);
+    if (p < 0) continue;
+    double* base = tg.p[q] + p;
+    const double own = w.v[q][1] + (d1 ? from_d1 : 0.0) + (u1 ? from_u1 : 0.0) + (u2 ? from_u2 : 0.0);
+    if (own != 0.0) aadd(base, own);
+    if (!u1 && w.v[q][0] != 0.0) aadd(base - 1, w.v[q][0]);
+    if (!d1 && w.v[q][2] != 0.0) aadd(base + 1, w.v[q][2]);
+    if (!d2 && w.v[q][3] != 0.0) aadd(base + 2, w.v[q][3]);
+  }
+}
+
+template <int NT, typename F>
+hipError_t xrow(const Dims& d, Box b, XTgt<NT> tg, F f, hipStream_t s) {
+  if (b.k1 < b.k0 || b.i1 < b.i0 || b.j1 < b.j0) return hipSuccess;
+  const size_t n = (size_t)(b.i1 - b.i0 + 1) * (size_t)(b.j1 - b.j0 + 1) * (size_t)(b.k1 - b.k0 + 1);
+  hipLaunchKernelGGL((xrow_kernel<NT, F>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, tg, f);
+  return hipGetLastError();
 }
 
 constexpr int ZSEG_L = 8;
@@ -561,17 +620,17 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     HIPCHK(fluxj_uvz(1, 1, MY, MYb, fV));  // fluxJ23_XVZ
   }
   // fluxX_XVZ (UD3.F90:1463-1512)
-  HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+  HIPCHK(xrow<3>(d, Box{KS, KE, IS - 1, IE, JS, JE}, XTgt<3>{{MYb, Db, Db + IA}}, [=] __device__(int k, int i, int j, XWin<3>& xw) {
     const size_t p = IX(d, k, i, j);
     const double den = 0.25 * (D[p] + D[p + 1] + D[p + IA] + D[p + 1 + IA]);
     const double vel = 0.5 * (MX[p] + MX[p + IA]) / den;
     const double fb = fV.x(k, i, j);
     const double G = m.gs(I_UVZ, p), gm = G / m.mapf(i, j, 1, I_UV);
     NDb[oVX + p] += fb * G;
-    const double velb = fb * gm * face_adj<S>(0, MY, MYb, p, 1, vel, fb * gm * vel);
+    const double velb = fb * gm * face_adj_s<S>(0, MY, p, 1, vel, fb * gm * vel, [&](int c, double v) { xw.v[0][c] += v; });
     aadd(MXb + p, 0.5 * velb / den); aadd(MXb + p + IA, 0.5 * velb / den);
     const double denb = 0.25 * (-velb * vel / den);
-    aadd(Db + p, denb); aadd(Db + p + 1, denb); aadd(Db + p + IA, denb); aadd(Db + p + 1 + IA, denb);
+    xw.v[1][1] += denb; xw.v[1][2] += denb; xw.v[2][1] += denb; xw.v[2][2] += denb;
   }, st));
   // fluxY_XVZ (UD3.F90:1536-1587): flux(k,i,j-1) at loop index j, here the
   // face jf = j-1; the MOMY and DENS adjoints of the y row in windows
@@ -643,16 +702,18 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     HIPCHK(fluxj_uvz(0, 0, MX, MXb, fU));  // fluxJ13_UYZ
     HIPCHK(fluxj_uvz(0, 1, MX, MXb, fU));  // fluxJ23_UYZ
   }
-  // fluxX_UYZ (UD3.F90:1043-1097): flux(k,i-1,j) at loop index i
-  HIPCHK(for3(Box{KS, KE, IS, IE + 1, JS, JE}, [=] __device__(int k, int i, int j) {
-    const size_t p = IX(d, k, i, j), q = p - 1;
+  // fluxX_UYZ (UD3.F90:1043-1097): flux(k,i-1,j) at loop index i, here the
+  // face if = i-1 (q its point, p = q+1 the loop's)
+  HIPCHK(xrow<2>(d, Box{KS, KE, IS - 1, IE, JS, JE}, XTgt<2>{{MXb, Db}}, [=] __device__(int k, int fi, int j, XWin<2>& xw) {
+    const int i = fi + 1;
+    const size_t q = IX(d, k, fi, j), p = q + 1;
     const double vel = 0.5 * (MX[p] + MX[q]) / D[p];
-    const double fb = fU.x(k, i - 1, j);
+    const double fb = fU.x(k, fi, j);
     const double G = m.gs(I_XYZ, p), gm = G / m.mapf(i, j, 1, I_XY);
     NDb[oUX + p] += fb * G;
-    const double velb = fb * gm * face_adj<S>(0, MX, MXb, q, 1, vel, fb * gm * vel);
-    aadd(MXb + p, 0.5 * velb / D[p]); aadd(MXb + q, 0.5 * velb / D[p]);
-    aadd(Db + p, -velb * vel / D[p]);
+    const double velb = fb * gm * face_adj_s<S>(0, MX, q, 1, vel, fb * gm * vel, [&](int c, double v) { xw.v[0][c] += v; });
+    xw.v[0][2] += 0.5 * velb / D[p]; xw.v[0][1] += 0.5 * velb / D[p];
+    xw.v[1][2] += -velb * vel / D[p];
   }, st));
   // fluxY_UYZ (UD3.F90:1101-1150): the MOMX adjoint of the y row in a window
   HIPCHK(lseg<1>(d, Box{KS, KE, IS, IE, JS - 1, JE}, MXb, Db, [=] __device__(int k, int i, int j, ZWin& yw) {
@@ -831,12 +892,12 @@ _Pragma("unroll")
     NDb[oTZ + p] += fb * m.gs(I_XYW, p);
     mZb[p] += fb * face_adj_s<S>(zkind(k, KS, KE), POTT, p, pl, vel, fb * vel, [&](int q, double v) { zw.a[q] += v; });
   }, st));
-  HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+  HIPCHK(xrow<1>(d, Box{KS, KE, IS - 1, IE, JS, JE}, XTgt<1>{{POTTb}}, [=] __device__(int k, int i, int j, XWin<1>& xw) {
     const size_t p = IX(d, k, i, j);
     const double G = m.gs(I_UYZ, p);
     const double fb = fT.x(k, i, j), vel = G / m.mapf(i, j, 1, I_UY) * (MX[p] + ND[oDX + p]);
     NDb[oTX + p] += fb * G;
-    mXb[p] += fb * face_adj<S>(0, POTT, POTTb, p, 1, vel, fb * vel);
+    mXb[p] += fb * face_adj_s<S>(0, POTT, p, 1, vel, fb * vel, [&](int c, double v) { xw.v[0][c] += v; });
   }, st));
   HIPCHK(lseg<1>(d, Box{KS, KE, IS, IE, JS - 1, JE}, POTTb, nullptr, [=] __device__(int k, int i, int j, ZWin& yw) {
     const size_t p = IX(d, k, i, j);
@@ -893,7 +954,7 @@ _Pragma("unroll")
     }
   }
   // fluxX_XYW (UD3.F90:598-671)
-  HIPCHK(for3(Box{KS, KE - 1, IS - 1, IE, JS, JE}, [=] __device__(int k, int i, int j) {
+  HIPCHK(xrow<3>(d, Box{KS, KE - 1, IS - 1, IE, JS, JE}, XTgt<3>{{MZb, Db, Db + pl}}, [=] __device__(int k, int i, int j, XWin<3>& xw) {
     const size_t p = IX(d, k, i, j);
     const double f1 = f2h(m, g, d, I_UYW, k, 1, i, j), f2 = f2h(m, g, d, I_UYW, k, 2, i, j);
     const double num = f1 * MX[p + pl] + f2 * MX[p];
@@ -902,11 +963,11 @@ _Pragma("unroll")
     const double fb = fW.x(k, i, j);
     const double G = m.gs(I_UYW, p), gm = G / m.mapf(i, j, 1, I_UY);
     NDb[oWX + p] += fb * G;
-    const double velb = fb * gm * face_adj<S>(0, MZ, MZb, p, 1, vel, fb * gm * vel);
+    const double velb = fb * gm * face_adj_s<S>(0, MZ, p, 1, vel, fb * gm * vel, [&](int c, double v) { xw.v[0][c] += v; });
     const double numb = velb / den, denb = -velb * vel / den;
     aadd(MXb + p + pl, f1 * numb); aadd(MXb + p, f2 * numb);
-    aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + 1, 0.5 * f1 * denb);
-    aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + 1, 0.5 * f2 * denb);
+    xw.v[2][1] += 0.5 * f1 * denb; xw.v[2][2] += 0.5 * f1 * denb;
+    xw.v[1][1] += 0.5 * f2 * denb; xw.v[1][2] += 0.5 * f2 * denb;
   }, st));
   // fluxY_XYW (UD3.F90:675-748): the MOMZ adjoint of the y row in a window
   HIPCHK(lseg<1>(d, Box{KS, KE - 1, IS, IE, JS - 1, JE}, MZb, Db, [=] __device__(int k, int i, int j, ZWin& yw) {
