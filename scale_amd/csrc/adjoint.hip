@@ -234,6 +234,45 @@ hipError_t xrow(const Dims& d, Box b, XTgt<NT> tg, F f, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Points of a box along y (DIR 1), L consecutive points of a y row per
+// thread (i fastest across threads): f(k, i, j, win) adds its contributions
+// to the points j-1..j+2 of the row in win.v[t][0..3] for the target tg[t]
+// (a field, or a field shifted by whole levels or rows); each point of the
+// row is added once per thread, with an atomic (the shifted targets of other
+// threads reach the same points).
+template <int NT, typename F>
+__global__ void __launch_bounds__(256) ysegn_kernel(Dims d, Box b, int L, XTgt<NT> tg, F f) {
+  const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+  const int nseg = (nj + L - 1) / L;
+  const size_t n = (size_t)ni * (size_t)nk * (size_t)nseg;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = b.i0 + (int)(t % (size_t)ni);
+  const size_t r = t / (size_t)ni;
+  const int k = b.k0 + (int)(r % (size_t)nk);
+  const int j0 = b.j0 + (int)(r / (size_t)nk) * L, j1 = min(b.j1, j0 + L - 1);
+  XWin<NT> w;
+#pragma unroll
+  for (int q = 0; q < NT; ++q) w.v[q][0] = w.v[q][1] = w.v[q][2] = w.v[q][3] = 0.0;
+  auto flush = [&](int jj, int m) {
+    const size_t p = IX(d, k, i, jj);
+#pragma unroll
+    for (int q = 0; q < NT; ++q)
+      if (w.v[q][m] != 0.0) aadd(tg.p[q] + p, w.v[q][m]);
+  };
+  for (int j = j0; j <= j1; ++j) {
+    f(k, i, j, w);
+    flush(j - 1, 0);
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+      w.v[q][0] = w.v[q][1]; w.v[q][1] = w.v[q][2]; w.v[q][2] = w.v[q][3]; w.v[q][3] = 0.0;
+    }
+  }
+  flush(j1, 0);
+  flush(j1 + 1, 1);
+  flush(j1 + 2, 2);
+}
+
 constexpr int ZSEG_L = 8;
 
 template <int DIR, typename F>
@@ -243,6 +282,14 @@ hipError_t lseg(const Dims& d, Box b, double* tgt, double* tgt2, F f, hipStream_
   const size_t nl = DIR == 0 ? nk : (DIR == 1 ? nj : ni);
   const size_t n = ni * nj * nk / nl * ((nl + ZSEG_L - 1) / ZSEG_L);
   hipLaunchKernelGGL((lseg_kernel<DIR, F>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, ZSEG_L, tgt, tgt2, f);
+  return hipGetLastError();
+}
+template <int NT, typename F>
+hipError_t ysegn(const Dims& d, Box b, XTgt<NT> tg, F f, hipStream_t s) {
+  if (b.k1 < b.k0 || b.i1 < b.i0 || b.j1 < b.j0) return hipSuccess;
+  const size_t nj = b.j1 - b.j0 + 1;
+  const size_t n = (size_t)(b.i1 - b.i0 + 1) * (size_t)(b.k1 - b.k0 + 1) * ((nj + ZSEG_L - 1) / ZSEG_L);
+  hipLaunchKernelGGL((ysegn_kernel<NT, F>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, ZSEG_L, tg, f);
   return hipGetLastError();
 }
 template <typename F>
@@ -563,44 +610,57 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     }, st);
   };
 
-  // ---- momentum y (HEVI.F90:936-1047) ----
-  HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
-    const size_t p = IX(d, k, i, j);
-    const double b = obMY[p];
-    MY0b[p] += b;
-    const double T = dt * b;
-    const double u = T / m.gs(I_XVZ, p);
-    // (the flux faces gather their adjoints from obMY: fV)
-    // pg: the bracket times MAPF(i,j,2,I_XV)
-    const double v = -u * m.mapf(i, j, 1, I_XV);
-    aadd(DPRESb + p + IA, v * g.RFDY[j] * m.gs(I_XYZ, p + IA)); aadd(DPRESb + p, -v * g.RFDY[j] * m.gs(I_XYZ, p));
-    if constexpr (TER) {
-      const double wv = v * g.RCDZ[k] * 0.5;
-      const double a1 = wv * m.j23(I_XVW, p), a2 = wv * m.j23(I_XVW, p - pl);
-      const double f1k = f2h(m, g, d, I_XVZ, k, 1, i, j), f2k = f2h(m, g, d, I_XVZ, k, 2, i, j);
-      const double f1m = f2h(m, g, d, I_XVZ, k - 1, 1, i, j), f2m = f2h(m, g, d, I_XVZ, k - 1, 2, i, j);
-      aadd(DPRESb + p + pl + IA, a1 * f1k); aadd(DPRESb + p + pl, a1 * f1k);
-      aadd(DPRESb + p + IA, a1 * f2k - a2 * f1m); aadd(DPRESb + p, a1 * f2k - a2 * f1m);
-      aadd(DPRESb + p - pl + IA, -a2 * f2m); aadd(DPRESb + p - pl, -a2 * f2m);
+  // ---- momentum y (HEVI.F90:936-1047): the pressure-gradient, Coriolis and
+  //      map-factor adjoints of a y row in windows (DPRES at levels k, k+1
+  //      and k-1; MOMX at i and i-1) ----
+  {
+    constexpr int NY = TER ? 6 : 3;
+    XTgt<NY> ty;
+    {
+      double* all[6] = {DPRESb, MXb, MXb - 1, Db, DPRESb + pl, DPRESb - pl};
+      for (int q = 0; q < NY; ++q) ty.p[q] = all[q];
     }
-    if (cor) {
-      const double cc = -0.125 * (COR[(size_t)i + (size_t)IA * (j + 1)] + COR[(size_t)i + (size_t)IA * j]) * T;
-      aadd(MXb + p + IA, cc); aadd(MXb + p, cc); aadd(MXb + p - 1 + IA, cc); aadd(MXb + p - 1, cc);
-    }
-    if constexpr (TER) {
-      // the map-factor term of cf, -c0 Xs (MOMY a - Xs/4 b) 2/(D(j+1) + D(j))
-      const double Xs = MX[p] + MX[p - 1] + MX[p + IA] + MX[p - 1 + IA];
-      const double c0 = 0.25 * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV);
-      const double aa = (1.0 / m.mapf(i, j, 1, I_UV) - 1.0 / m.mapf(i - 1, j, 1, I_UV)) * g.RCDX[i];
-      const double bb = (1.0 / m.mapf(i, j + 1, 0, I_XY) - 1.0 / m.mapf(i, j, 0, I_XY)) * g.RFDY[j];
-      const double Dd = D[p + IA] + D[p], rD = 2.0 / Dd;
-      const double xsb = -T * c0 * (MY[p] * aa - 0.5 * Xs * bb) * rD;
-      aadd(MXb + p, xsb); aadd(MXb + p - 1, xsb); aadd(MXb + p + IA, xsb); aadd(MXb + p - 1 + IA, xsb);
-      aadd(MYb + p, -T * c0 * Xs * aa * rD);
-      const double cf2 = -c0 * Xs * (MY[p] * aa - 0.25 * Xs * bb) * rD;
-      aadd(Db + p + IA, -T * cf2 / Dd); aadd(Db + p, -T * cf2 / Dd);
-    }
-  }, st));
+    HIPCHK(ysegn<NY>(d, inter, ty, [=] __device__(int k, int i, int j, XWin<NY>& yw) {
+      const size_t p = IX(d, k, i, j);
+      const double b = obMY[p];
+      MY0b[p] += b;
+      const double T = dt * b;
+      const double u = T / m.gs(I_XVZ, p);
+      // (the flux faces gather their adjoints from obMY: fV)
+      // pg: the bracket times MAPF(i,j,2,I_XV)
+      const double v = -u * m.mapf(i, j, 1, I_XV);
+      yw.v[0][2] += v * g.RFDY[j] * m.gs(I_XYZ, p + IA); yw.v[0][1] += -v * g.RFDY[j] * m.gs(I_XYZ, p);
+      if constexpr (TER) {
+        const double wv = v * g.RCDZ[k] * 0.5;
+        const double a1 = wv * m.j23(I_XVW, p), a2 = wv * m.j23(I_XVW, p - pl);
+        const double f1k = f2h(m, g, d, I_XVZ, k, 1, i, j), f2k = f2h(m, g, d, I_XVZ, k, 2, i, j);
+        const double f1m = f2h(m, g, d, I_XVZ, k - 1, 1, i, j), f2m = f2h(m, g, d, I_XVZ, k - 1, 2, i, j);
+        yw.v[4][2] += a1 * f1k; yw.v[4][1] += a1 * f1k;
+        yw.v[0][2] += a1 * f2k - a2 * f1m; yw.v[0][1] += a1 * f2k - a2 * f1m;
+        yw.v[5][2] += -a2 * f2m; yw.v[5][1] += -a2 * f2m;
+      }
+      if (cor) {
+        const double cc = -0.125 * (COR[(size_t)i + (size_t)IA * (j + 1)] + COR[(size_t)i + (size_t)IA * j]) * T;
+        yw.v[1][2] += cc; yw.v[1][1] += cc; yw.v[2][2] += cc; yw.v[2][1] += cc;
+      }
+      if constexpr (TER) {
+        // the map-factor term of cf, -c0 Xs (MOMY a - Xs/4 b) 2/(D(j+1) + D(j));
+        // zero where the map factors are uniform
+        const double aa = (1.0 / m.mapf(i, j, 1, I_UV) - 1.0 / m.mapf(i - 1, j, 1, I_UV)) * g.RCDX[i];
+        const double bb = (1.0 / m.mapf(i, j + 1, 0, I_XY) - 1.0 / m.mapf(i, j, 0, I_XY)) * g.RFDY[j];
+        if (aa != 0.0 || bb != 0.0) {
+          const double Xs = MX[p] + MX[p - 1] + MX[p + IA] + MX[p - 1 + IA];
+          const double c0 = 0.25 * m.mapf(i, j, 0, I_XV) * m.mapf(i, j, 1, I_XV);
+          const double Dd = D[p + IA] + D[p], rD = 2.0 / Dd;
+          const double xsb = -T * c0 * (MY[p] * aa - 0.5 * Xs * bb) * rD;
+          yw.v[1][1] += xsb; yw.v[2][1] += xsb; yw.v[1][2] += xsb; yw.v[2][2] += xsb;
+          MYb[p] += -T * c0 * Xs * aa * rD;  // the point's own: no other thread adds to it here
+          const double cf2 = -c0 * Xs * (MY[p] * aa - 0.25 * Xs * bb) * rD;
+          yw.v[3][2] += -T * cf2 / Dd; yw.v[3][1] += -T * cf2 / Dd;
+        }
+      }
+    }, st));
+  }
   // fluxZ_XVZ (UD3.F90:1173-1265), the MOMY adjoint of the column in a window
   HIPCHK(zseg(d, Box{KS, KE - 1, IS, IE, JS, JE}, MYb, [=] __device__(int k, int i, int j, ZWin& zw) {
     const size_t p = IX(d, k, i, j);
@@ -646,44 +706,57 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     yw.b[2] += -velb * vel / D[p];
   }, st));
 
-  // ---- momentum x (HEVI.F90:816-927) ----
-  HIPCHK(for3(inter, [=] __device__(int k, int i, int j) {
-    const size_t p = IX(d, k, i, j);
-    const double b = obMX[p];
-    MX0b[p] += b;
-    const double T = dt * b;
-    const double u = T / m.gs(I_UYZ, p);
-    // (the flux faces gather their adjoints from obMX: fU)
-    // pg: the bracket times MAPF(i,j,1,I_UY)
-    const double v = -u * m.mapf(i, j, 0, I_UY);
-    aadd(DPRESb + p + 1, v * g.RFDX[i] * m.gs(I_XYZ, p + 1)); aadd(DPRESb + p, -v * g.RFDX[i] * m.gs(I_XYZ, p));
-    if constexpr (TER) {
-      // both J13 terms take F2H at level k, as HEVI.F90 writes them
-      const double wv = v * g.RCDZ[k] * 0.5;
-      const double a1 = wv * m.j13(I_UYW, p), a2 = wv * m.j13(I_UYW, p - pl);
-      const double f1k = f2h(m, g, d, I_UYZ, k, 1, i, j), f2k = f2h(m, g, d, I_UYZ, k, 2, i, j);
-      aadd(DPRESb + p + pl + 1, a1 * f1k); aadd(DPRESb + p + pl, a1 * f1k);
-      aadd(DPRESb + p + 1, a1 * f2k - a2 * f1k); aadd(DPRESb + p, a1 * f2k - a2 * f1k);
-      aadd(DPRESb + p - pl + 1, -a2 * f2k); aadd(DPRESb + p - pl, -a2 * f2k);
+  // ---- momentum x (HEVI.F90:816-927): the pressure-gradient, Coriolis and
+  //      map-factor adjoints of an x row summed across the wave (DPRES at
+  //      levels k, k+1 and k-1; MOMY at j and j-1) ----
+  {
+    constexpr int NX = TER ? 6 : 3;
+    XTgt<NX> tx;
+    {
+      double* all[6] = {DPRESb, MYb, MYb - IA, Db, DPRESb + pl, DPRESb - pl};
+      for (int q = 0; q < NX; ++q) tx.p[q] = all[q];
     }
-    if (cor) {
-      const double cc = 0.125 * (COR[(size_t)(i + 1) + (size_t)IA * j] + COR[(size_t)i + (size_t)IA * j]) * T;
-      aadd(MYb + p + 1, cc); aadd(MYb + p, cc); aadd(MYb + p + 1 - IA, cc); aadd(MYb + p - IA, cc);
-    }
-    if constexpr (TER) {
-      // the map-factor term of cf, c0 Ys (Ys a - MOMX b) 2/(D(i+1) + D(i))
-      const double Ys = MY[p] + MY[p - IA] + MY[p + 1] + MY[p + 1 - IA];
-      const double c0 = 0.25 * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY);
-      const double aa = 0.25 * (1.0 / m.mapf(i + 1, j, 1, I_XY) - 1.0 / m.mapf(i, j, 1, I_XY)) * g.RFDX[i];
-      const double bb = (1.0 / m.mapf(i, j, 0, I_UV) - 1.0 / m.mapf(i, j - 1, 0, I_UV)) * g.RCDY[j];
-      const double Dd = D[p + 1] + D[p], rD = 2.0 / Dd;
-      const double ysb = T * c0 * (2.0 * Ys * aa - MX[p] * bb) * rD;
-      aadd(MYb + p, ysb); aadd(MYb + p - IA, ysb); aadd(MYb + p + 1, ysb); aadd(MYb + p + 1 - IA, ysb);
-      aadd(MXb + p, -T * c0 * Ys * bb * rD);
-      const double cf2 = c0 * Ys * (Ys * aa - MX[p] * bb) * rD;
-      aadd(Db + p + 1, -T * cf2 / Dd); aadd(Db + p, -T * cf2 / Dd);
-    }
-  }, st));
+    HIPCHK(xrow<NX>(d, inter, tx, [=] __device__(int k, int i, int j, XWin<NX>& xw) {
+      const size_t p = IX(d, k, i, j);
+      const double b = obMX[p];
+      MX0b[p] += b;
+      const double T = dt * b;
+      const double u = T / m.gs(I_UYZ, p);
+      // (the flux faces gather their adjoints from obMX: fU)
+      // pg: the bracket times MAPF(i,j,1,I_UY)
+      const double v = -u * m.mapf(i, j, 0, I_UY);
+      xw.v[0][2] += v * g.RFDX[i] * m.gs(I_XYZ, p + 1); xw.v[0][1] += -v * g.RFDX[i] * m.gs(I_XYZ, p);
+      if constexpr (TER) {
+        // both J13 terms take F2H at level k, as HEVI.F90 writes them
+        const double wv = v * g.RCDZ[k] * 0.5;
+        const double a1 = wv * m.j13(I_UYW, p), a2 = wv * m.j13(I_UYW, p - pl);
+        const double f1k = f2h(m, g, d, I_UYZ, k, 1, i, j), f2k = f2h(m, g, d, I_UYZ, k, 2, i, j);
+        xw.v[4][2] += a1 * f1k; xw.v[4][1] += a1 * f1k;
+        xw.v[0][2] += a1 * f2k - a2 * f1k; xw.v[0][1] += a1 * f2k - a2 * f1k;
+        xw.v[5][2] += -a2 * f2k; xw.v[5][1] += -a2 * f2k;
+      }
+      if (cor) {
+        const double cc = 0.125 * (COR[(size_t)(i + 1) + (size_t)IA * j] + COR[(size_t)i + (size_t)IA * j]) * T;
+        xw.v[1][2] += cc; xw.v[1][1] += cc; xw.v[2][2] += cc; xw.v[2][1] += cc;
+      }
+      if constexpr (TER) {
+        // the map-factor term of cf, c0 Ys (Ys a - MOMX b) 2/(D(i+1) + D(i));
+        // zero where the map factors are uniform
+        const double aa = 0.25 * (1.0 / m.mapf(i + 1, j, 1, I_XY) - 1.0 / m.mapf(i, j, 1, I_XY)) * g.RFDX[i];
+        const double bb = (1.0 / m.mapf(i, j, 0, I_UV) - 1.0 / m.mapf(i, j - 1, 0, I_UV)) * g.RCDY[j];
+        if (aa != 0.0 || bb != 0.0) {
+          const double Ys = MY[p] + MY[p - IA] + MY[p + 1] + MY[p + 1 - IA];
+          const double c0 = 0.25 * m.mapf(i, j, 0, I_UY) * m.mapf(i, j, 1, I_UY);
+          const double Dd = D[p + 1] + D[p], rD = 2.0 / Dd;
+          const double ysb = T * c0 * (2.0 * Ys * aa - MX[p] * bb) * rD;
+          xw.v[1][1] += ysb; xw.v[2][1] += ysb; xw.v[1][2] += ysb; xw.v[2][2] += ysb;
+          MXb[p] += -T * c0 * Ys * bb * rD;  // the point's own: no other thread adds to it here
+          const double cf2 = c0 * Ys * (Ys * aa - MX[p] * bb) * rD;
+          xw.v[3][2] += -T * cf2 / Dd; xw.v[3][1] += -T * cf2 / Dd;
+        }
+      }
+    }, st));
+  }
   // fluxZ_UYZ (UD3.F90:753-845), the MOMX adjoint of the column in a window
   HIPCHK(zseg(d, Box{KS, KE - 1, IS, IE, JS, JE}, MXb, [=] __device__(int k, int i, int j, ZWin& zw) {
     const size_t p = IX(d, k, i, j);
