@@ -25,6 +25,7 @@
 // hevi_stage_traced when the step's stages are recomputed.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include <stdlib.h>
@@ -234,43 +235,53 @@ hipError_t xrow(const Dims& d, Box b, XTgt<NT> tg, F f, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Points of a box along y (DIR 1), L consecutive points of a y row per
-// thread (i fastest across threads): f(k, i, j, win) adds its contributions
-// to the points j-1..j+2 of the row in win.v[t][0..3] for the target tg[t]
-// (a field, or a field shifted by whole levels or rows); each point of the
-// row is added once per thread, with an atomic (the shifted targets of other
-// threads reach the same points).
-template <int NT, typename F>
-__global__ void __launch_bounds__(256) ysegn_kernel(Dims d, Box b, int L, XTgt<NT> tg, F f) {
+// Points of a box along z or y (DIR 0, 1), L consecutive points of a column
+// (a y row) per thread, i fastest across threads: f(k, i, j, win) adds its
+// contributions to the points l-1..l+2 of the line in win.v[t][0..3] for the
+// target tg[t] (a field, or a field shifted by whole levels, rows or
+// columns); each point of the line is added once per thread and target, with
+// an atomic (the shifted targets of other threads reach the same points),
+// except, with OWN0, target 0's points that no other thread reaches (every
+// face q-2..q+1 of point q the thread's own).
+template <int DIR, int NT, bool OWN0, typename F>
+__global__ void __launch_bounds__(256) lsegn_kernel(Dims d, Box b, int L, XTgt<NT> tg, F f) {
   const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
-  const int nseg = (nj + L - 1) / L;
-  const size_t n = (size_t)ni * (size_t)nk * (size_t)nseg;
+  const int n2 = DIR == 0 ? nj : nk, nl = DIR == 0 ? nk : nj;
+  const int nseg = (nl + L - 1) / L;
+  const size_t n = (size_t)ni * (size_t)n2 * (size_t)nseg;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = b.i0 + (int)(t % (size_t)ni);
   const size_t r = t / (size_t)ni;
-  const int k = b.k0 + (int)(r % (size_t)nk);
-  const int j0 = b.j0 + (int)(r / (size_t)nk) * L, j1 = min(b.j1, j0 + L - 1);
+  const int c2 = (int)(r % (size_t)n2);
+  const int lb = DIR == 0 ? b.k0 : b.j0, le = DIR == 0 ? b.k1 : b.j1;
+  const int l0 = lb + (int)(r / (size_t)n2) * L, l1 = min(le, l0 + L - 1);
+  const int j = DIR == 0 ? b.j0 + c2 : 0, k = DIR == 0 ? 0 : b.k0 + c2;
   XWin<NT> w;
 #pragma unroll
   for (int q = 0; q < NT; ++q) w.v[q][0] = w.v[q][1] = w.v[q][2] = w.v[q][3] = 0.0;
-  auto flush = [&](int jj, int m) {
-    const size_t p = IX(d, k, i, jj);
+  auto flush = [&](int l, int m, bool own) {
+    const size_t p = DIR == 0 ? IX(d, l, i, j) : IX(d, k, i, l);
 #pragma unroll
-    for (int q = 0; q < NT; ++q)
-      if (w.v[q][m] != 0.0) aadd(tg.p[q] + p, w.v[q][m]);
+    for (int q = 0; q < NT; ++q) {
+      const double v = w.v[q][m];
+      if (v == 0.0) continue;
+      if (OWN0 && q == 0 && own) tg.p[0][p] += v;
+      else aadd(tg.p[q] + p, v);
+    }
   };
-  for (int j = j0; j <= j1; ++j) {
-    f(k, i, j, w);
-    flush(j - 1, 0);
+  for (int l = l0; l <= l1; ++l) {
+    if constexpr (DIR == 0) f(l, i, j, w);
+    else f(k, i, l, w);
+    flush(l - 1, 0, l - 1 >= l0 + 2);  // no later face reaches l-1
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
       w.v[q][0] = w.v[q][1]; w.v[q][1] = w.v[q][2]; w.v[q][2] = w.v[q][3]; w.v[q][3] = 0.0;
     }
   }
-  flush(j1, 0);
-  flush(j1 + 1, 1);
-  flush(j1 + 2, 2);
+  flush(l1, 0, false);
+  flush(l1 + 1, 1, false);
+  flush(l1 + 2, 2, false);
 }
 
 constexpr int ZSEG_L = 8;
@@ -284,12 +295,13 @@ hipError_t lseg(const Dims& d, Box b, double* tgt, double* tgt2, F f, hipStream_
   hipLaunchKernelGGL((lseg_kernel<DIR, F>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, ZSEG_L, tgt, tgt2, f);
   return hipGetLastError();
 }
-template <int NT, typename F>
-hipError_t ysegn(const Dims& d, Box b, XTgt<NT> tg, F f, hipStream_t s) {
+template <int DIR, int NT, bool OWN0, typename F>
+hipError_t lsegn(const Dims& d, Box b, XTgt<NT> tg, F f, hipStream_t s) {
   if (b.k1 < b.k0 || b.i1 < b.i0 || b.j1 < b.j0) return hipSuccess;
-  const size_t nj = b.j1 - b.j0 + 1;
-  const size_t n = (size_t)(b.i1 - b.i0 + 1) * (size_t)(b.k1 - b.k0 + 1) * ((nj + ZSEG_L - 1) / ZSEG_L);
-  hipLaunchKernelGGL((ysegn_kernel<NT, F>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, ZSEG_L, tg, f);
+  const size_t nl = DIR == 0 ? b.k1 - b.k0 + 1 : b.j1 - b.j0 + 1, n2 = DIR == 0 ? b.j1 - b.j0 + 1 : b.k1 - b.k0 + 1;
+  const size_t n = (size_t)(b.i1 - b.i0 + 1) * n2 * ((nl + ZSEG_L - 1) / ZSEG_L);
+  hipLaunchKernelGGL((lsegn_kernel<DIR, NT, OWN0, F>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, ZSEG_L, tg,
+                     f);
   return hipGetLastError();
 }
 template <typename F>
@@ -568,22 +580,31 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
   // vel = (f1 mom(k+1) + f2 mom(k)) / (f1 <D>(k+1) + f2 <D>(k)); mom is the
   // 4-point mean (quad) where it lives at the other horizontal staggering.
   // stag 0: the UYZ family (D averaged over i, i+1), 1: XVZ (over j, j+1).
-  auto fluxj_uvz = [&](int stag, int jd, const double* val, double* valb, const FaceAdj<TER> fa) -> hipError_t {
-    const bool quad = stag != jd;
+  auto fluxj_uvz = [&](auto quadc, int stag, int jd, const double* val, double* valb, const FaceAdj<TER> fa) -> hipError_t {
+    constexpr bool quad = decltype(quadc)::value;  // stag != jd
     const double* mom = jd == 0 ? MX : MY;
     double* momb = jd == 0 ? MXb : MYb;
     const ptrdiff_t sd = stag == 0 ? 1 : IA;
     const ptrdiff_t oa = stag == 0 ? 1 : -1, ob = stag == 0 ? -(ptrdiff_t)IA : (ptrdiff_t)IA;
     const int cG = stag == 0 ? I_UYZ : I_XVZ, cJ = stag == 0 ? I_UYW : I_XVW, cM = stag == 0 ? I_UY : I_XV;
     const int mn = jd == 0 ? 1 : 0;
-    // z faces of a column: the val adjoint (and, with mom the same field, the
-    // mom adjoint of the own column) in the segment's window
-    return zseg(d, Box{KS, KE - 1, IS, IE, JS, JE}, valb, [=] __device__(int k, int i, int j, ZWin& zw) {
+    // z faces of a column in the segment's windows: the val adjoint (and,
+    // with mom the same field, the mom adjoint of the own column), the
+    // density adjoint of the column and its neighbour, and with quad the
+    // mom adjoints of the four columns of the mean
+    constexpr int NT = quad ? 7 : 3;
+    XTgt<NT> tg;
+    {
+      double* all[7] = {valb, Db, Db + sd, momb, momb + oa, momb + ob, momb + oa + ob};
+      for (int q = 0; q < NT; ++q) tg.p[q] = all[q];
+    }
+    return lsegn<0, NT, true>(d, Box{KS, KE - 1, IS, IE, JS, JE}, tg, [=] __device__(int k, int i, int j, XWin<NT>& zw) {
       const size_t p = IX(d, k, i, j);
       const double f1 = f2h(m, g, d, cG, k, 1, i, j), f2 = f2h(m, g, d, cG, k, 2, i, j);
       auto mm = [&](size_t q) {
         const double* r = mom + q;
-        return quad ? 0.25 * (r[0] + r[oa] + r[ob] + r[oa + ob]) : r[0];
+        if constexpr (quad) return 0.25 * (r[0] + r[oa] + r[ob] + r[oa + ob]);
+        else return r[0];
       };
       const double num = f1 * mm(p + pl) + f2 * mm(p);
       const double den = f1 * 0.5 * (D[p + pl] + D[p + pl + sd]) + f2 * 0.5 * (D[p] + D[p + sd]);
@@ -592,21 +613,17 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       const double vel = vr * Jm, mp = m.mapf(i, j, mn, cM);
       const double fb = fa.z(k, i, j);
       const double velb =
-          fb * face_adj_s<S>(zkind(k, KS, KE), val, p, pl, vel, fb * vel / mp, [&](int q, double v) { zw.a[q] += v; }) / mp;
+          fb * face_adj_s<S>(zkind(k, KS, KE), val, p, pl, vel, fb * vel / mp, [&](int q, double v) { zw.v[0][q] += v; }) / mp;
       const double vrb = velb * Jm;
       const double numb = vrb / den, denb = -vrb * vr / den;
-      if (quad) {
-        auto mmb = [&](size_t q, double x) {
-          double* r = momb + q;
-          const double y = 0.25 * x;
-          aadd(r, y); aadd(r + oa, y); aadd(r + ob, y); aadd(r + oa + ob, y);
-        };
-        mmb(p + pl, f1 * numb); mmb(p, f2 * numb);
+      if constexpr (quad) {
+#pragma unroll
+        for (int q = 3; q < 7; ++q) { zw.v[q][2] += 0.25 * (f1 * numb); zw.v[q][1] += 0.25 * (f2 * numb); }
       } else {  // mom is val itself, on the own column
-        zw.a[2] += f1 * numb; zw.a[1] += f2 * numb;
+        zw.v[0][2] += f1 * numb; zw.v[0][1] += f2 * numb;
       }
-      aadd(Db + p + pl, 0.5 * f1 * denb); aadd(Db + p + pl + sd, 0.5 * f1 * denb);
-      aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + sd, 0.5 * f2 * denb);
+      zw.v[1][2] += 0.5 * f1 * denb; zw.v[2][2] += 0.5 * f1 * denb;
+      zw.v[1][1] += 0.5 * f2 * denb; zw.v[2][1] += 0.5 * f2 * denb;
     }, st);
   };
 
@@ -620,7 +637,7 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
       double* all[6] = {DPRESb, MXb, MXb - 1, Db, DPRESb + pl, DPRESb - pl};
       for (int q = 0; q < NY; ++q) ty.p[q] = all[q];
     }
-    HIPCHK(ysegn<NY>(d, inter, ty, [=] __device__(int k, int i, int j, XWin<NY>& yw) {
+    HIPCHK((lsegn<1, NY, false>)(d, inter, ty, [=] __device__(int k, int i, int j, XWin<NY>& yw) {
       const size_t p = IX(d, k, i, j);
       const double b = obMY[p];
       MY0b[p] += b;
@@ -676,8 +693,8 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + IA, 0.5 * f2 * denb);
   }, st));
   if constexpr (TER) {
-    HIPCHK(fluxj_uvz(1, 0, MY, MYb, fV));  // fluxJ13_XVZ
-    HIPCHK(fluxj_uvz(1, 1, MY, MYb, fV));  // fluxJ23_XVZ
+    HIPCHK(fluxj_uvz(std::true_type{}, 1, 0, MY, MYb, fV));  // fluxJ13_XVZ
+    HIPCHK(fluxj_uvz(std::false_type{}, 1, 1, MY, MYb, fV));  // fluxJ23_XVZ
   }
   // fluxX_XVZ (UD3.F90:1463-1512)
   HIPCHK(xrow<3>(d, Box{KS, KE, IS - 1, IE, JS, JE}, XTgt<3>{{MYb, Db, Db + IA}}, [=] __device__(int k, int i, int j, XWin<3>& xw) {
@@ -772,8 +789,8 @@ static hipError_t stage_adj_t(Ctx& c, const StageAdjIO& io, hipStream_t st) {
     aadd(Db + p, 0.5 * f2 * denb); aadd(Db + p + 1, 0.5 * f2 * denb);
   }, st));
   if constexpr (TER) {
-    HIPCHK(fluxj_uvz(0, 0, MX, MXb, fU));  // fluxJ13_UYZ
-    HIPCHK(fluxj_uvz(0, 1, MX, MXb, fU));  // fluxJ23_UYZ
+    HIPCHK(fluxj_uvz(std::false_type{}, 0, 0, MX, MXb, fU));  // fluxJ13_UYZ
+    HIPCHK(fluxj_uvz(std::true_type{}, 0, 1, MX, MXb, fU));  // fluxJ23_UYZ
   }
   // fluxX_UYZ (UD3.F90:1043-1097): flux(k,i-1,j) at loop index i, here the
   // face if = i-1 (q its point, p = q+1 the loop's)
@@ -1061,29 +1078,52 @@ _Pragma("unroll")
 
   // ---- mass fluxes before the solve (HEVI.F90:369-438): the theta faces'
   //      adjoints plus those gathered from Sr (HEVI.F90:453-475, fR) ----
-  HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS - 1, JE}, [=] __device__(int k, int i, int j) {
-    const size_t p = IX(d, k, i, j);
-    if (j >= JS) {
-      if (k <= KE - 1) {
-        const double b = mZb[p] + fR.z(k, i, j);
-        const double M0 = m.mapf(i, j, 0, I_XY), M1 = m.mapf(i, j, 1, I_XY);
-        NDb[oDZ + p] += b * (m.gs(I_XYW, p) / (M0 * M1));
-        if constexpr (TER) {
+  //      Flat, each point adds to its own MOMX and MOMY adjoints only; over
+  //      terrain the J13/J23 terms add to the neighbours and the level above
+  //      too, summed in the column's windows.
+  if constexpr (TER) {
+    XTgt<4> tm{{MXb, MXb - 1, MYb, MYb - IA}};
+    HIPCHK((lsegn<0, 4, false>)(d, Box{KS, KE, IS - 1, IE, JS - 1, JE}, tm, [=] __device__(int k, int i, int j, XWin<4>& zw) {
+      const size_t p = IX(d, k, i, j);
+      if (j >= JS) {
+        if (k <= KE - 1) {
+          const double b = mZb[p] + fR.z(k, i, j);
+          const double M0 = m.mapf(i, j, 0, I_XY), M1 = m.mapf(i, j, 1, I_XY);
+          NDb[oDZ + p] += b * (m.gs(I_XYW, p) / (M0 * M1));
           const double c13 = b * (m.j13(I_XYW, p) * 0.25 / M1), c23 = b * (m.j23(I_XYW, p) * 0.25 / M0);
-          aadd(MXb + p + pl, c13); aadd(MXb + p + pl - 1, c13); aadd(MXb + p, c13); aadd(MXb + p - 1, c13);
-          aadd(MYb + p + pl, c23); aadd(MYb + p + pl - IA, c23); aadd(MYb + p, c23); aadd(MYb + p - IA, c23);
+          zw.v[0][2] += c13; zw.v[1][2] += c13; zw.v[0][1] += c13; zw.v[1][1] += c13;
+          zw.v[2][2] += c23; zw.v[3][2] += c23; zw.v[2][1] += c23; zw.v[3][1] += c23;
         }
+        const double gm = m.gs(I_UYZ, p) / m.mapf(i, j, 1, I_UY);
+        const double bx = mXb[p] + fR.x(k, i, j);
+        zw.v[0][1] += bx * gm; NDb[oDX + p] += bx * gm;
       }
-      const double gm = m.gs(I_UYZ, p) / m.mapf(i, j, 1, I_UY);
-      const double bx = mXb[p] + fR.x(k, i, j);
-      aadd(MXb + p, bx * gm); NDb[oDX + p] += bx * gm;
-    }
-    if (i >= IS) {
-      const double gm = m.gs(I_XVZ, p) / m.mapf(i, j, 0, I_XV);
-      const double by = mYb[p] + fR.y(k, i, j);
-      aadd(MYb + p, by * gm); NDb[oDY + p] += by * gm;
-    }
-  }, st));
+      if (i >= IS) {
+        const double gm = m.gs(I_XVZ, p) / m.mapf(i, j, 0, I_XV);
+        const double by = mYb[p] + fR.y(k, i, j);
+        zw.v[2][1] += by * gm; NDb[oDY + p] += by * gm;
+      }
+    }, st));
+  } else {
+    HIPCHK(for3(Box{KS, KE, IS - 1, IE, JS - 1, JE}, [=] __device__(int k, int i, int j) {
+      const size_t p = IX(d, k, i, j);
+      if (j >= JS) {
+        if (k <= KE - 1) {
+          const double b = mZb[p] + fR.z(k, i, j);
+          const double M0 = m.mapf(i, j, 0, I_XY), M1 = m.mapf(i, j, 1, I_XY);
+          NDb[oDZ + p] += b * (m.gs(I_XYW, p) / (M0 * M1));
+        }
+        const double gm = m.gs(I_UYZ, p) / m.mapf(i, j, 1, I_UY);
+        const double bx = mXb[p] + fR.x(k, i, j);
+        MXb[p] += bx * gm; NDb[oDX + p] += bx * gm;
+      }
+      if (i >= IS) {
+        const double gm = m.gs(I_XVZ, p) / m.mapf(i, j, 0, I_XV);
+        const double by = mYb[p] + fR.y(k, i, j);
+        MYb[p] += by * gm; NDb[oDY + p] += by * gm;
+      }
+    }, st));
+  }
   // ---- POTT (HEVI.F90:588-598) and DPRES (HEVI.F90:346-360) ----
   HIPCHK(for3(Box{KS, KE, IS - 2, IE + 2, JS - 2, JE + 2}, [=] __device__(int k, int i, int j) {
     const size_t p = IX(d, k, i, j);
