@@ -96,27 +96,29 @@ __device__ __forceinline__ void trt_ratio_body(const TtArgs& a, unsigned lb, uns
 
   // phi_lo of the ring cells at level k1 (COMMON.F90:2033-2045); the flux
   // through their bottom face is sLZ, the one through the top face replaces it
-  auto ring_philo = [&](int k1) {
+  // phi_lo of ring cell t (row-major over the H1 x W1 ring) at level k1
+  auto ring_cell = [&](int k1, int t) {
     const double(*Q1)[W2] = sQ0[k1 % 5];
     const double(*Q2)[W2] = sQ0[(k1 + 1) % 5];
     const size_t pk = d.pl * (size_t)k1;
-    for (int t = tid; t < H1 * W1; t += nthr) {
-      const int r = t / W1, c = t - r * W1;
-      const int ii = i0 - 1 + c, jj = j0 - 1 + r;
-      if (ii > d.IE + 1 || jj > d.JE + 1) continue;
-      const size_t p = pk + (size_t)ii + (size_t)d.IA * jj;
-      const int c2 = c + 1, r2 = r + 1;
-      const double lzu = (k1 == KE) ? 0.0 : tt_lflux(mZ[p], Q1[r2][c2], Q2[r2][c2]);
-      const double lzd = sLZ[r][c];
-      const double lx = tt_lflux(mX[p], Q1[r2][c2], Q1[r2][c2 + 1]);
-      const double lxm = tt_lflux(mX[p - 1], Q1[r2][c2 - 1], Q1[r2][c2]);
-      const double ly = tt_lflux(mY[p], Q1[r2][c2], Q1[r2 + 1][c2]);
-      const double lym = tt_lflux(mY[p - d.IA], Q1[r2 - 1][c2], Q1[r2][c2]);
-      sPL[k1 % 3][r][c] = (Q1[r2][c2] * a.DENS0[p]
-                           + dt * (-((lzu - lzd) * a.RCDZ[k1] + (lx - lxm) * a.RCDX[ii] + (ly - lym) * a.RCDY[jj])))
-                        / a.DENS[p];
-      sLZ[r][c] = lzu;
-    }
+    const int r = t / W1, c = t - r * W1;
+    const int ii = i0 - 1 + c, jj = j0 - 1 + r;
+    if (ii > d.IE + 1 || jj > d.JE + 1) return;
+    const size_t p = pk + (size_t)ii + (size_t)d.IA * jj;
+    const int c2 = c + 1, r2 = r + 1;
+    const double lzu = (k1 == KE) ? 0.0 : tt_lflux(mZ[p], Q1[r2][c2], Q2[r2][c2]);
+    const double lzd = sLZ[r][c];
+    const double lx = tt_lflux(mX[p], Q1[r2][c2], Q1[r2][c2 + 1]);
+    const double lxm = tt_lflux(mX[p - 1], Q1[r2][c2 - 1], Q1[r2][c2]);
+    const double ly = tt_lflux(mY[p], Q1[r2][c2], Q1[r2 + 1][c2]);
+    const double lym = tt_lflux(mY[p - d.IA], Q1[r2 - 1][c2], Q1[r2][c2]);
+    sPL[k1 % 3][r][c] = (Q1[r2][c2] * a.DENS0[p]
+                         + dt * (-((lzu - lzd) * a.RCDZ[k1] + (lx - lxm) * a.RCDX[ii] + (ly - lym) * a.RCDY[jj])))
+                      / a.DENS[p];
+    sLZ[r][c] = lzu;
+  };
+  auto ring_philo = [&](int k1) {
+    for (int t = tid; t < H1 * W1; t += nthr) ring_cell(k1, t);
   };
 
   tt_load<H2, W2>(sQ0[KS % 5], a.QTRC0, d, KS, i0 - 2, j0 - 2, tid, nthr);
@@ -133,30 +135,38 @@ __device__ __forceinline__ void trt_ratio_body(const TtArgs& a, unsigned lb, uns
     if (k + 2 <= KE) tt_load<H2, W2>(sQ0[(k + 2) % 5], a.QTRC0, d, k + 2, i0 - 2, j0 - 2, tid, nthr);
     tt_load<H2, W2>(sQ, a.QTRC, d, k, i0 - 2, j0 - 2, tid, nthr);
     __syncthreads();
-    if (k + 1 <= KE) ring_philo(k + 1);
     {
+      // one pass over the level's x faces, y faces and (below the top) the
+      // ring cells of phi_lo at k+1: the block's threads take them in turn,
+      // so no pass runs with a few threads while the rest wait
       const double(*Q0)[W2] = sQ0[k % 5];
       const size_t pk = d.pl * (size_t)k;
-      for (int t = tid; t < TJ * 65; t += nthr) {
-        const int r = t / 65, f = t - r * 65;
-        const int ii = i0 - 1 + f, jj = j0 + r;
-        if (ii <= d.IE && jj <= d.JE) {
-          const size_t p = pk + (size_t)ii + (size_t)d.IA * jj;
-          const int c = f + 1, rr = r + 2;
-          const double vel = mX[p];
-          sAX[r][f] = tt_hflux<S>(vel, sQ[rr][c - 1], sQ[rr][c], sQ[rr][c + 1], sQ[rr][c + 2], (a.nd ? a.nd[p + a.nds] : 0.0))
-                    - tt_lflux(vel, Q0[rr][c], Q0[rr][c + 1]);
-        }
-      }
-      for (int t = tid; t < (TJ + 1) * 64; t += nthr) {
-        const int r = t / 64, c0 = t - r * 64;
-        const int ii = i0 + c0, jj = j0 - 1 + r;
-        if (ii <= d.IE && jj <= d.JE) {
-          const size_t p = pk + (size_t)ii + (size_t)d.IA * jj;
-          const int c = c0 + 2, rr = r + 1;
-          const double vel = mY[p];
-          sAY[r][c0] = tt_hflux<S>(vel, sQ[rr - 1][c], sQ[rr][c], sQ[rr + 1][c], sQ[rr + 2][c], (a.nd ? a.nd[p + 2 * a.nds] : 0.0))
-                     - tt_lflux(vel, Q0[rr][c], Q0[rr + 1][c]);
+      constexpr int NXF = TJ * 65, NYF = (TJ + 1) * 64;
+      const int nwork = NXF + NYF + (k + 1 <= KE ? H1 * W1 : 0);
+      for (int t = tid; t < nwork; t += nthr) {
+        if (t < NXF) {
+          const int r = t / 65, f = t - r * 65;
+          const int ii = i0 - 1 + f, jj = j0 + r;
+          if (ii <= d.IE && jj <= d.JE) {
+            const size_t p = pk + (size_t)ii + (size_t)d.IA * jj;
+            const int c = f + 1, rr = r + 2;
+            const double vel = mX[p];
+            sAX[r][f] = tt_hflux<S>(vel, sQ[rr][c - 1], sQ[rr][c], sQ[rr][c + 1], sQ[rr][c + 2], (a.nd ? a.nd[p + a.nds] : 0.0))
+                      - tt_lflux(vel, Q0[rr][c], Q0[rr][c + 1]);
+          }
+        } else if (t < NXF + NYF) {
+          const int u = t - NXF;
+          const int r = u / 64, c0 = u - r * 64;
+          const int ii = i0 + c0, jj = j0 - 1 + r;
+          if (ii <= d.IE && jj <= d.JE) {
+            const size_t p = pk + (size_t)ii + (size_t)d.IA * jj;
+            const int c = c0 + 2, rr = r + 1;
+            const double vel = mY[p];
+            sAY[r][c0] = tt_hflux<S>(vel, sQ[rr - 1][c], sQ[rr][c], sQ[rr + 1][c], sQ[rr + 2][c], (a.nd ? a.nd[p + 2 * a.nds] : 0.0))
+                       - tt_lflux(vel, Q0[rr][c], Q0[rr + 1][c]);
+          }
+        } else {
+          ring_cell(k + 1, t - NXF - NYF);
         }
       }
     }
