@@ -1680,22 +1680,92 @@ __device__ __forceinline__ void hevi_tile_body(const HeviArgsT<R>& a, unsigned l
       if (a.trCp) a.trCp[q] = Cn;
     }
   }
-  for (int k = KE - 2; k >= KS; --k) {
-    const size_t q = px(k), q1 = px(k + 1);
-    const double Ck = (double)a.e[q] * Cn + (double)a.f[q];
-    const double ptk = (double)a.PT[q];
-    const R z0 = a.MOMZ0[q];
-    put_zflux(q, R((double)nd(I_DENS, ZDIR, q) + J33G * Ck), R((double)tfZ[q] + J33G * Ck * ptk));
-    put_state(1, q, R((double)z0 + (Ck - (double)a.MOMZ[q])), z0, k);
-    if constexpr (sizeof(R) == sizeof(double)) {
-      if (a.trCp) a.trCp[q] = Ck;
+  // Levels KE-2 .. KS: each level's loads (its own and the level above's
+  // DENS0/RHOT0/Sr/St, with the combination its stages 1-3) go in chunks of
+  // DCH levels, the next chunk's issued (clamped to KS, unconditionally)
+  // before the current chunk's work: one wait per chunk instead of one per
+  // level.  A prefetched level is written only after its loads, by the
+  // level that consumes them.
+  // (measured at 512x512x128: the mixed short step 13.67 -> 13.00 ms, the
+  // FP64 one unchanged; 8 / 4 levels in FP64 spill and run 27 % slower)
+  constexpr int DCH = CMB ? 2 : 4;
+  struct DnV {
+    R e, f, pt, z0, ndz, tf, mz, d0, r0, drk, rrk;
+    R cz[3], cd[3], cr[3], cm[3], ct[3];  // the combination: stages 1-3 of MOMZ(k), DENS/RHOT(k+1), the z fluxes
+  };
+  auto dload = [&](int k, DnV& L) {
+    const size_t q = px(max(k, KS)), q1 = q + d.pl;
+    L.e = a.e[q]; L.f = a.f[q]; L.pt = a.PT[q]; L.z0 = a.MOMZ0[q]; L.ndz = nd(I_DENS, ZDIR, q);
+    L.tf = tfZ[q]; L.mz = a.MOMZ[q];
+    L.d0 = a.DENS0[q1]; L.r0 = a.RHOT0[q1]; L.drk = a.DENS_RK[q1]; L.rrk = a.RHOT_RK[q1];
+    if constexpr (CMB) {
+      const R* const* cs[3] = {a.c1, a.c2, a.c3};
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        L.cz[n] = cs[n][1][q]; L.cd[n] = cs[n][0][q1]; L.cr[n] = cs[n][4][q1];
+        L.cm[n] = a.cm[n][q]; L.ct[n] = a.ct[n][q];
+      }
     }
-    const R d0 = a.DENS0[q1], r0 = a.RHOT0[q1];
-    double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1];
-    put_state(0, q1, devD(fullDd(d0, k + 1) + dtrk * (advcv + (double)a.DENS_RK[q1]), k + 1), d0, k + 1);
-    advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1];
-    put_state(4, q1, devT(fullTd(r0, k + 1) + dtrk * (advcv + (double)a.RHOT_RK[q1]), k + 1), r0, k + 1);
-    Cn = Ck; ptn = ptk;
+  };
+  // put_state / put_zflux on prefetched stage values
+  auto put_state_p = [&](int v, size_t q, R val, R s0v, int k, const R* cv) {
+    if constexpr (CMB) {
+      const R c = rk4c(cv[0], cv[1], cv[2], val, s0v);
+      a.st[v][q] = c;
+      if (k == KS) {
+        for (int kk = 0; kk <= KS - 1; ++kk) {
+          R gv = (v == 1) ? R(0) : c;
+          if constexpr (DEV) {
+            if (v == 0) gv = devD(fullDd(c, k), kk);
+            if (v == 4) gv = devT(fullTd(c, k), kk);
+          }
+          a.st[v][p0 + d.pl * (size_t)kk] = gv;
+        }
+      }
+    } else {
+      R* o = v == 0 ? a.DENS_RK : (v == 1 ? a.MOMZ_RK : a.RHOT_RK);
+      o[q] = val;
+    }
+  };
+  auto put_zflux_p = [&](size_t q, R mv, R tv, const DnV& L) {
+    if constexpr (CMB) {
+      a.mflx_av[q] = rk4f(L.cm[0], L.cm[1], L.cm[2], mv);
+      a.tflx_av[q] = rk4f(L.ct[0], L.ct[1], L.ct[2], tv);
+    } else {
+      mfZ[q] = mv; tfZ[q] = tv;
+    }
+  };
+  DnV cur[DCH], nxt[DCH];
+#pragma unroll
+  for (int c = 0; c < DCH; ++c) dload(KE - 2 - c, cur[c]);
+  for (int kc = KE - 2; kc >= KS; kc -= DCH) {
+    if (kc - DCH >= KS) {
+#pragma unroll
+      for (int c = 0; c < DCH; ++c) dload(kc - DCH - c, nxt[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < DCH; ++c) {
+      const int k = kc - c;
+      if (k < KS) break;
+      const DnV& L = cur[c];
+      const size_t q = px(k), q1 = px(k + 1);
+      const double Ck = (double)L.e * Cn + (double)L.f;
+      const double ptk = (double)L.pt;
+      const R z0 = L.z0;
+      put_zflux_p(q, R((double)L.ndz + J33G * Ck), R((double)L.tf + J33G * Ck * ptk), L);
+      put_state_p(1, q, R((double)z0 + (Ck - (double)L.mz)), z0, k, L.cz);
+      if constexpr (sizeof(R) == sizeof(double)) {
+        if (a.trCp) a.trCp[q] = Ck;
+      }
+      const R d0 = L.d0, r0 = L.r0;
+      double advcv = -(Cn - Ck) * J33G * RCDZ[k + 1];
+      put_state_p(0, q1, devD(fullDd(d0, k + 1) + dtrk * (advcv + (double)L.drk), k + 1), d0, k + 1, L.cd);
+      advcv = -(Cn * ptn - Ck * ptk) * J33G * RCDZ[k + 1];
+      put_state_p(4, q1, devT(fullTd(r0, k + 1) + dtrk * (advcv + (double)L.rrk), k + 1), r0, k + 1, L.cr);
+      Cn = Ck; ptn = ptk;
+    }
+#pragma unroll
+    for (int c = 0; c < DCH; ++c) cur[c] = nxt[c];
   }
   {
     const size_t q = px(KS);
