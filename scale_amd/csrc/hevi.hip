@@ -2143,7 +2143,8 @@ hipError_t heve_stage(const StageIO& io, hipStream_t s) {
 }
 
 // FP64 stage recording PT, St and the solved vertical momentum C for the
-// adjoint: hevi_tile on flat grids, hevi_march with terrain
+// adjoint: hevi_tile on flat grids whose tiles fill the device, the split
+// launches (hevi_expl sections + hevi_solve) otherwise and with terrain
 hipError_t hevi_stage_traced(const StageIO& io, double* PT, double* St, double* Cp, hipStream_t s) {
   HeviArgs a;
   fill_args(a, io);
