@@ -268,7 +268,9 @@ __global__ void __launch_bounds__(64 * TJ) trt_ratio(TtArgs a, unsigned ni_tiles
 // branch hides from the wait counters how many loads are outstanding.  This
 // holds 176 VGPRs (one block per CU) and beats the same pass without the
 // prefetch at two blocks per CU by 15 %; the ratio pass, heavier in
-// arithmetic, keeps two blocks per CU without it.
+// arithmetic, keeps two blocks per CU without it (with its two staged planes
+// loaded a level ahead it ran 18 % slower held to two blocks per CU, 6 VGPRs
+// spilled, and 26 % slower at one block per CU).
 template <int S, int TJ>
 __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, unsigned ni_tiles) {
   constexpr int W2 = 68, H2 = TJ + 4, W1 = 66, H1 = TJ + 2, NT = 64 * TJ;
