@@ -186,6 +186,8 @@ hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s);
 // with R+/R- in the given fields; tracer_stage uses it where it applies
 bool tracer_tile_supported(const Ctx& c);
 hipError_t tracer_stage_tile(const Ctx& c, const TracerIO& io, double* rjpls, double* rjmns, hipStream_t s);
+// the stage without FCT on the same tiles (flat, two-cell schemes)
+hipError_t tracer_nofct_tile(const Ctx& c, const TracerIO& io, hipStream_t s);
 // the same on caller-supplied device metrics (GSQRT(..,7), MAPF(IA,JA,2),
 // RCDZ/RCDX/RCDY) and work arrays (3 n3 doubles: phi_lo, R+, R-)
 hipError_t tracer_stage_ext(const Ctx& c, const TracerIO& io, const double* GSQRT, const double* MAPF_XY,

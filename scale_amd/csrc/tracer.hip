@@ -462,9 +462,10 @@ hipError_t tracer_stage_ext(const Ctx& c, const TracerIO& io, const double* GSQR
 }
 
 hipError_t tracer_stage(const Ctx& c, const TracerIO& io, hipStream_t s) {
-  // FCT on flat terrain with a two-cell scheme and a grid with blocks enough:
-  // the LDS-tiled kernels (tracer_tile.hip)
+  // flat terrain with a two-cell scheme and a grid with blocks enough: the
+  // LDS-tiled kernels (tracer_tile.hip), with FCT or without
   if (io.flag_fct && c.tracer_tile && tracer_tile_supported(c)) return tracer_stage_tile(c, io, c.scr[1], c.scr[2], s);
+  if (!io.flag_fct && c.tracer_tile && tracer_tile_supported(c)) return tracer_nofct_tile(c, io, s);
   TrArgs a;
   a.d = c.d;
   a.QTRC = io.QTRC; a.QTRC0 = io.QTRC0; a.RHOQ_t = io.RHOQ_t; a.DENS0 = io.DENS0; a.DENS = io.DENS;

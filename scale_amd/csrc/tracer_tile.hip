@@ -271,7 +271,9 @@ __global__ void __launch_bounds__(64 * TJ) trt_ratio(TtArgs a, unsigned ni_tiles
 // arithmetic, keeps two blocks per CU without it (with its two staged planes
 // loaded a level ahead it ran 18 % slower held to two blocks per CU, 6 VGPRs
 // spilled, and 26 % slower at one block per CU).
-template <int S, int TJ>
+// With LIM false the same march is the stage without FCT (tracer_fvm_heve.F90:
+// 226-231): the high-order fluxes alone, no limiter arrays staged or read.
+template <int S, int TJ, bool LIM = true>
 __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, unsigned ni_tiles) {
   constexpr int W2 = 68, H2 = TJ + 4, W1 = 66, H1 = TJ + 2, NT = 64 * TJ;
   constexpr int NQ = (H2 * W2 + NT - 1) / NT, NR = (H1 * W1 + NT - 1) / NT;
@@ -312,16 +314,18 @@ __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, un
       const double v = a.QTRC[g];
       pQ[h] = g ? v : 0.0;
     }
-    const bool r1 = k + 1 <= KE;
+    if constexpr (LIM) {
+      const bool r1 = k + 1 <= KE;
 #pragma unroll
-    for (int h = 0; h < NR; ++h) {
-      const int q = tid + h * NT;
-      const size_t g0 = cell(q, W1, 1, k, q < H1 * W1);
-      const size_t g1 = cell(q, W1, 1, k + 1, r1 && q < H1 * W1);
-      const double v0 = a.QTRC0[g0], vp = a.rjpls[g1], vm = a.rjmns[g1];
-      pQ0[h] = g0 ? v0 : 0.0;
-      pRP[h] = g1 ? vp : 0.0;
-      pRM[h] = g1 ? vm : 0.0;
+      for (int h = 0; h < NR; ++h) {
+        const int q = tid + h * NT;
+        const size_t g0 = cell(q, W1, 1, k, q < H1 * W1);
+        const size_t g1 = cell(q, W1, 1, k + 1, r1 && q < H1 * W1);
+        const double v0 = a.QTRC0[g0], vp = a.rjpls[g1], vm = a.rjmns[g1];
+        pQ0[h] = g0 ? v0 : 0.0;
+        pRP[h] = g1 ? vp : 0.0;
+        pRM[h] = g1 ? vm : 0.0;
+      }
     }
     const size_t pk = d.pl * (size_t)k;
 #pragma unroll
@@ -353,8 +357,10 @@ __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, un
     pD0 = v0; pRQ = vq; pD = vd;
   };
 
-  tt_load<H1, W1>(sRP[KS % 3], a.rjpls, d, KS, i0 - 1, j0 - 1, tid, NT);
-  tt_load<H1, W1>(sRM[KS % 3], a.rjmns, d, KS, i0 - 1, j0 - 1, tid, NT);
+  if constexpr (LIM) {
+    tt_load<H1, W1>(sRP[KS % 3], a.rjpls, d, KS, i0 - 1, j0 - 1, tid, NT);
+    tt_load<H1, W1>(sRM[KS % 3], a.rjmns, d, KS, i0 - 1, j0 - 1, tid, NT);
+  }
   double w[4];
   for (int m = 0; m < 4; ++m) w[m] = a.QTRC[pc + d.pl * (size_t)(KS - 1 + m)];
   double q0c = a.QTRC0[pc + d.pl * (size_t)KS], q0n = a.QTRC0[pc + d.pl * (size_t)(KS + 1)];
@@ -367,12 +373,14 @@ __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, un
       const int q = tid + h * NT;
       if (q < H2 * W2) sQ[q / W2][q % W2] = pQ[h];
     }
+    if constexpr (LIM) {
 #pragma unroll
-    for (int h = 0; h < NR; ++h) {
-      const int q = tid + h * NT;
-      if (q < H1 * W1) {
-        sQ0[q / W1][q % W1] = pQ0[h];
-        if (k + 1 <= KE) { sRP[(k + 1) % 3][q / W1][q % W1] = pRP[h]; sRM[(k + 1) % 3][q / W1][q % W1] = pRM[h]; }
+      for (int h = 0; h < NR; ++h) {
+        const int q = tid + h * NT;
+        if (q < H1 * W1) {
+          sQ0[q / W1][q % W1] = pQ0[h];
+          if (k + 1 <= KE) { sRP[(k + 1) % 3][q / W1][q % W1] = pRP[h]; sRM[(k + 1) % 3][q / W1][q % W1] = pRM[h]; }
+        }
       }
     }
     double cXm[NX], cXn[NX], cYm[NY], cYn[NY];
@@ -399,9 +407,11 @@ __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, un
           const int c1 = f, r1 = r + 1;
           const double vel = cXm[h];
           const double hf = tt_hflux<S>(vel, sQ[rr][c - 1], sQ[rr][c], sQ[rr][c + 1], sQ[rr][c + 2], cXn[h]);
-          const double an = hf - tt_lflux(vel, sQ0[r1][c1], sQ0[r1][c1 + 1]);
           sHX[r][f] = hf;
-          sLX[r][f] = lim(an, RP[r1][c1 + 1], RM[r1][c1], RP[r1][c1], RM[r1][c1 + 1]);
+          if constexpr (LIM) {
+            const double an = hf - tt_lflux(vel, sQ0[r1][c1], sQ0[r1][c1 + 1]);
+            sLX[r][f] = lim(an, RP[r1][c1 + 1], RM[r1][c1], RP[r1][c1], RM[r1][c1 + 1]);
+          }
         }
       }
 #pragma unroll
@@ -414,9 +424,11 @@ __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, un
           const int c1 = c0 + 1, r1 = r;
           const double vel = cYm[h];
           const double hf = tt_hflux<S>(vel, sQ[rr - 1][c], sQ[rr][c], sQ[rr + 1][c], sQ[rr + 2][c], cYn[h]);
-          const double an = hf - tt_lflux(vel, sQ0[r1][c1], sQ0[r1 + 1][c1]);
           sHY[r][c0] = hf;
-          sLY[r][c0] = lim(an, RP[r1 + 1][c1], RM[r1][c1], RP[r1][c1], RM[r1 + 1][c1]);
+          if constexpr (LIM) {
+            const double an = hf - tt_lflux(vel, sQ0[r1][c1], sQ0[r1 + 1][c1]);
+            sLY[r][c0] = lim(an, RP[r1 + 1][c1], RM[r1][c1], RP[r1][c1], RM[r1 + 1][c1]);
+          }
         }
       }
     }
@@ -428,15 +440,25 @@ __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, un
       double hz = 0.0, lz = 0.0;
       if (k != KE) {
         hz = tt_hz<S>(d, k, cZ, w, cN);
-        const double an = hz - tt_lflux(cZ, q0c, q0n);
-        lz = lim(an, sRP[(k + 1) % 3][r1][c1], sRM[k % 3][r1][c1], sRP[k % 3][r1][c1], sRM[(k + 1) % 3][r1][c1]);
+        if constexpr (LIM) {
+          const double an = hz - tt_lflux(cZ, q0c, q0n);
+          lz = lim(an, sRP[(k + 1) % 3][r1][c1], sRM[k % 3][r1][c1], sRP[k % 3][r1][c1], sRM[(k + 1) % 3][r1][c1]);
+        }
       }
-      a.QTRCo[p] = (q0c * cD0
-                    + a.dtl * (-((hz - lz - hzm + lzm) * a.RCDZ[k]
-                                 + (sHX[ty][tx + 1] - sLX[ty][tx + 1] - sHX[ty][tx] + sLX[ty][tx]) * a.RCDX[i]
-                                 + (sHY[ty + 1][tx] - sLY[ty + 1][tx] - sHY[ty][tx] + sLY[ty][tx]) * a.RCDY[j])
-                               + cRQ))
-                 / cD;
+      if constexpr (LIM) {
+        a.QTRCo[p] = (q0c * cD0
+                      + a.dtl * (-((hz - lz - hzm + lzm) * a.RCDZ[k]
+                                   + (sHX[ty][tx + 1] - sLX[ty][tx + 1] - sHX[ty][tx] + sLX[ty][tx]) * a.RCDX[i]
+                                   + (sHY[ty + 1][tx] - sLY[ty + 1][tx] - sHY[ty][tx] + sLY[ty][tx]) * a.RCDY[j])
+                                 + cRQ))
+                   / cD;
+      } else {  // tracer_nofct's expression; its metric factors are 1.0 here
+        a.QTRCo[p] = (q0c * cD0
+                      + a.dtl * (-((hz - hzm) * a.RCDZ[k] + (sHX[ty][tx + 1] - sHX[ty][tx]) * a.RCDX[i]
+                                   + (sHY[ty + 1][tx] - sHY[ty][tx]) * a.RCDY[j])
+                                 + cRQ))
+                   / cD;
+      }
       hzm = hz;
       lzm = lz;
     }
@@ -446,6 +468,11 @@ __device__ __forceinline__ void trt_update_body(const TtArgs& a, unsigned lb, un
     if (k + 2 < d.KA) q0n = a.QTRC0[pc + d.pl * (size_t)(k + 2)];
     __syncthreads();
   }
+}
+
+template <int S, int TJ>
+__global__ void __launch_bounds__(64 * TJ) trt_nofct(TtArgs a, unsigned ni_tiles, unsigned nblocks) {
+  trt_update_body<S, TJ, false>(a, xcd_block(blockIdx.x, nblocks), ni_tiles);
 }
 
 template <int S, int TJ>
@@ -497,10 +524,41 @@ bool tracer_tile_supported(const Ctx& c) {
   const int st = c.scheme_tracer;
   if (c.terrain || !(st == UD3 || st == CD4 || st == UD3KOREN || st == UD1 || st == CD2)) return false;
   int ncu = 256;
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess) ncu = 256;
   const Dims& d = c.d;
   const long nb = (long)((d.IE - d.IS + 1 + 63) / 64) * ((d.JE - d.JS + 1 + TT_TJ - 1) / TT_TJ);
   return c.tracer_tile == 2 || nb >= 2L * ncu;
+}
+
+// the stage without FCT on the same tiles (8 rows, as the update)
+template <int S>
+static hipError_t trn_launch(const Ctx& c, const TtArgs& a, hipStream_t s) {
+  const Dims& d = c.d;
+  const int imax = d.IE - d.IS + 1, jmax = d.JE - d.JS + 1;
+  const unsigned ni = (imax + 63) / 64, nb = ni * ((jmax + TT_TJ - 1) / TT_TJ);
+  prof_start(c, "tracer_nofct", s);
+  hipLaunchKernelGGL((trt_nofct<S, TT_TJ>), dim3(nb), dim3(64, TT_TJ), 0, s, a, ni, nb);
+  prof_stop(c, s);
+  return hipGetLastError();
+}
+
+hipError_t tracer_nofct_tile(const Ctx& c, const TracerIO& io, hipStream_t s) {
+  TtArgs a;
+  a.d = c.d;
+  a.QTRC = io.QTRC; a.QTRC0 = io.QTRC0; a.RHOQ_t = io.RHOQ_t; a.DENS0 = io.DENS0; a.DENS = io.DENS;
+  a.mflx = io.mflx_hi; a.nd = io.num_diff; a.nds = c.d.n3;
+  a.RCDZ = c.g1.RCDZ; a.RCDX = c.g1.RCDX; a.RCDY = c.g1.RCDY;
+  a.dtl = io.dtl; a.QTRCo = io.QTRCo;
+  a.rjpls = nullptr; a.rjmns = nullptr;
+  if (io.flag_fct) return hipErrorNotSupported;
+  switch (c.scheme_tracer) {
+    case UD3: return trn_launch<UD3>(c, a, s);
+    case CD4: return trn_launch<CD4>(c, a, s);
+    case UD3KOREN: return trn_launch<UD3KOREN>(c, a, s);
+    case UD1: return trn_launch<UD1>(c, a, s);
+    case CD2: return trn_launch<CD2>(c, a, s);
+    default: return hipErrorNotSupported;
+  }
 }
 
 hipError_t tracer_stage_tile(const Ctx& c, const TracerIO& io, double* rjpls, double* rjmns, hipStream_t s) {

@@ -1,5 +1,6 @@
-"""The tiled FCT tracer kernels (LDS tiles of 64 x 8 columns marching k: the
-ratio pass with phi_lo on a one-cell ring, the update) against the oracle, bit
+"""The tiled tracer kernels (LDS tiles of 64 x 8 columns marching k: the FCT
+ratio pass with phi_lo on a one-cell ring, the FCT update, and the stage
+without FCT) against the oracle, bit
 for bit, on grids with several tiles in i and j, partial tiles at the east
 and north edges, and a stretched vertical grid; and against the per-point
 kernels (tuning knob SCALE_HIP_TUNE_TRACER_TILE = 0), bit for bit.  The small
@@ -51,7 +52,7 @@ def run(hip, case, inp, fct, along, tile):
     return out
 
 
-@pytest.mark.parametrize("fct,along", [(True, True), (True, False)], ids=["fct_along", "fct_7pt"])
+@pytest.mark.parametrize("fct,along", [(True, True), (True, False), (False, False)], ids=["fct_along", "fct_7pt", "nofct"])
 @pytest.mark.parametrize("scheme", [3, 4, 31, 1, 2], ids=["UD3", "CD4", "UD3KOREN1993", "UD1", "CD2"])
 def test_tracer_tile_bitwise(hip, scheme, fct, along):
     """70 x 20 x 24 interior: two i-tiles (the second 6 columns wide) and
