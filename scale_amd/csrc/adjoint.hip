@@ -121,11 +121,10 @@ __device__ __forceinline__ double face_adj(int kind, const double* q, double* qb
 
 // Faces of a box along z, y or x (DIR 0, 1, 2), L consecutive faces of a
 // column (a y row, an x row) per thread, in order: the adjoint of a stencil
-// along that direction
-// (contributions to the points l-1..l+2 of the face's own line) collects in a
-// register window and each point of the line is added once, without an
-// atomic where every face that reaches it (faces q-2..q+1) is the thread's
-// own.  f(k, i, j, win) adds win.a[m] for the point l-1+m of tgt, and
+// along that direction (contributions to the points l-1..l+2 of the face's
+// own line) collects in a register window and each point of the line is
+// added once, without an atomic where every face that reaches it (faces
+// q-2..q+1) is the thread's own.  f(k, i, j, win) adds win.a[m] for the point l-1+m of tgt, and
 // win.b[m] for the point l-1+m of tgt2 (added with atomics: other lines may
 // reach it too).  The x stencils keep one face per thread (xrow below): with
 // a thread per x row the lanes of a wave read 64 rows apart, and the warm
