@@ -49,6 +49,33 @@ def test_stage_adjoint_matches_oracle(scheme):
         assert close(got[key], ref[key]), key
 
 
+@pytest.mark.parametrize("imax,jmax,kmax", [(65, 3, 10), (130, 2, 9)], ids=["65x3x10", "130x2x9"])
+def test_stage_adjoint_odd_widths(imax, jmax, kmax):
+    """x rows of 66 and 131 faces, which cross the 64-lane waves of the
+    shuffle-summed x scatters at varying offsets; y rows and columns shorter
+    than, or a partial multiple of, the 8-point line windows."""
+    print(f"[adjoint] stage {imax}x{jmax}x{kmax} UD3", flush=True)
+    case = inputs.make_case(kmax=kmax, imax=imax, jmax=jmax, v0=2.0, momz_noise=0.05, noise=0.1, ntracer=0, scheme=3)
+    g = case.grid
+    rng = np.random.default_rng(23)
+    for k, a in case.state.items():
+        a[g.KS:g.KE + 1] *= 1.0 + 1e-3 * rng.standard_normal(a[g.KS:g.KE + 1].shape)
+        fill(g, a, k)
+    inp = stage_inputs(case)
+    out_b = {k + "_RK": np.asfortranarray(rng.standard_normal(g.shape)) for k in STATE}
+    dt = 0.5
+    ref = O.adj_tstep_short_hevi(case, inp, dt, out_b)
+    m = case.metric
+    with hip.Context(g, 3, 3) as ctx:
+        got = ctx.ATMOS_DYN_Tstep_short_fvm_hevi_adj(
+            out_b, *[inp[k + "0"] for k in STATE], *[inp[k] for k in STATE], *[inp[k + "_t"] for k in STATE],
+            inp["DPRES0"], inp["RT2P"], inp["CORIOLI"], inp["num_diff"], inp["wdamp_coef"],
+            m["CDZ"], m["FDZ"], m["FDX"], m["FDY"], m["RCDZ"], m["RCDX"], m["RCDY"], m["RFDZ"], m["RFDX"], m["RFDY"],
+            m["PHI"], m["GSQRT"], m["J13G"], m["J23G"], m["J33G"], m["MAPF"], inp["REF_dens"], inp["REF_rhot"], dt)
+    for key in ref:
+        assert close(got[key], ref[key]), key
+
+
 @pytest.mark.parametrize("records,tape,split", [("1", 0, "0"), ("1", 1, "0"), ("0", 0, "0"), ("1", 1, "1")],
                          ids=["all_segments_taped", "last_segment_taped", "per_step_recompute",
                               "last_segment_taped-split_launches"])
