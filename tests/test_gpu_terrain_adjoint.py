@@ -55,6 +55,37 @@ def test_terrain_stage_adjoint_dot_product(scheme):
             assert abs(lhs - rhs) <= 1e-10 * abs(lhs), (lhs, rhs)
 
 
+def test_terrain_stage_adjoint_odd_width():
+    """The same on 65 x 4 x 10 (UD3): x rows of 66 faces across the 64-lane
+    waves of the shuffle-summed scatters, y rows and columns shorter than, or
+    a partial multiple of, the 8-point line windows of the J13/J23, mass-flux
+    and momentum scatters."""
+    print("[terrain adjoint] stage 65x4x10 on random metrics, UD3", flush=True)
+    case, inp = terrain_stage_case(3, kmax=10, imax=65, jmax=4)
+    g = case.grid
+    m = case.metric
+    dt = 0.5
+    sl = (slice(g.KS, g.KE + 1), slice(g.IS, g.IE + 1), slice(g.JS, g.JE + 1))
+    rng = np.random.default_rng(49)
+    with hip.Context(g, 3, 3) as ctx:
+        t = stage_directions(case, rng)
+        y = {}
+        for k in T.STAGE_OUT:
+            a = np.zeros(g.shape, order="F")
+            a[sl] = rng.standard_normal(a[sl].shape)
+            y[k] = a
+        _, tan = T.stage_tangent(case, inp, dt, tangents=t)
+        lhs = sum(float(np.sum(tan[k][sl] * y[k][sl])) for k in T.STAGE_OUT)
+        got = ctx.ATMOS_DYN_Tstep_short_fvm_hevi_adj(
+            y, *[inp[k + "0"] for k in STATE], *[inp[k] for k in STATE], *[inp[k + "_t"] for k in STATE],
+            inp["DPRES0"], inp["RT2P"], inp["CORIOLI"], inp["num_diff"], case.wdamp_coef,
+            m["CDZ"], m["FDZ"], m["FDX"], m["FDY"], m["RCDZ"], m["RCDX"], m["RCDY"], m["RFDZ"], m["RFDX"], m["RFDY"],
+            m["PHI"], m["GSQRT"], m["J13G"], m["J23G"], m["J33G"], m["MAPF"], inp["REF_dens"], inp["REF_rhot"], dt)
+        rhs = sum(float(np.sum(got[key] * t[key])) for key in t)
+    print(f"  <L dx, y> = {lhs:.15e}, <dx, L^T y> (GPU) = {rhs:.15e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs), (lhs, rhs)
+
+
 @pytest.mark.parametrize("scheme", [3, 4], ids=["UD3", "CD4"])
 def test_terrain_model_adjoint_dot_product(scheme):
     """6 large steps over a bell mountain (h = 500 m, a = 2.5 km; metrics

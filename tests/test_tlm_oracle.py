@@ -219,13 +219,13 @@ def test_frozen_limiter_tangent(along):
     assert err_free > 1e-4, err_free
 
 
-def terrain_stage_case(scheme):
+def terrain_stage_case(scheme, kmax=10, imax=8, jmax=8):
     """A HEVI stage on random smooth GSQRT, J13G, J23G and MAPF (every metric
     term of the stage non-trivial), with Coriolis."""
     import dataclasses
 
     import helpers as H
-    case = H.small_case(kmax=10, imax=8, jmax=8, seed=77, scheme=scheme, ntracer=0)
+    case = H.small_case(kmax=kmax, imax=imax, jmax=jmax, seed=77, scheme=scheme, ntracer=0)
     case = dataclasses.replace(case, metric=H.terrain_metrics(case, seed=8))
     inp = H.stage_inputs(case, seed=9, coriolis=True)
     return case, inp
